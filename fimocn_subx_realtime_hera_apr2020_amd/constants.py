@@ -1,0 +1,48 @@
+"""Physical constants and hot-path namelist defaults of FIM.
+
+Values follow FIMsrc/cntl/module_constants.F90:14-24 and the namelist defaults
+of FIMsrc/cntl/fimnamelist.F90:95-119; the fixed-size parameters follow
+FIMsrc/cntl/module_control.F90:95-104.  Arrays are stored with the level index
+fastest, so a Fortran ``(nvl, nip)`` array is a C-order ``(nip, nvl)`` array.
+"""
+
+PI = 3.14159265            # module_constants.F90:14 (truncated value, used by grndspd)
+RADDEG = 180.0 / PI
+EARTHRAD = 6371220.0       # m
+OMEGX2 = 1.4584e-4         # 2 x earth rotation rate (1/s)
+GRVITY = 9.80665
+P1000 = 100000.0           # Pa
+CP = 1004.6855
+RD = 287.0586
+QVMIN = 1.0e-10
+QWMIN = 1.0e-10
+
+NPP = 6                    # module_control.F90:101
+NABL = 3                   # Adams-Bashforth levels
+KBL = 2                    # fixed-depth surface layers
+
+# hot-path namelist defaults (fimnamelist.F90:95-119)
+INTFC_SMOOTH = 30.0
+PTOP = 30.0
+THKTOP = 30.0
+RLEIGH = 2.0
+RLTHRESH = 90.0
+RLTAPER = 0.4
+SLAK = 0.5
+MIGLIM = 1
+REMAP_OPTN = 2
+VELDFF_BKGND = 0.3
+VELDFF_BOOST = 3.0
+
+# the 64-layer tables FIMrun/theta_coor64.txt and FIMrun/dpsig64.txt (first 64 values)
+THETA_COOR64 = [
+    224, 232, 240, 245, 250, 255, 260, 265, 270, 273,
+    276, 279, 282, 285, 288, 291, 293, 295, 297, 299,
+    301, 303, 305, 307, 309, 311, 313, 315, 317, 319,
+    321, 323, 325, 328, 332, 337, 343, 350, 358, 367,
+    377, 388, 400, 413, 427, 443, 462, 485, 513, 547,
+    590, 638, 692, 746, 800, 850, 900, 960, 1030, 1100,
+    1200, 1350, 1700, 2200]
+DPSIG64 = [
+    330, 510, 675, 825, 960, 1080, 1185, 1275, 1350, 1410,
+    1455, 1485] + [1500] * 52

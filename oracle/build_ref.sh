@@ -1,0 +1,79 @@
+#!/usr/bin/env bash
+# Build the reference FIM code itself, from its sources where they lie under
+# /root/reference, into oracle/_ref/.  Test infrastructure only: nothing in the
+# product links or runs what this script builds.
+#
+#   oracle/_ref/bin/grid, ginfo, getlvl   reference grid prep chain (default real 4,
+#                                          as PREP_FFLAGS in macros.make.linuxpcgnu:25)
+#   oracle/_ref/bin/dynref_r8             reference dynamics hot path, -fdefault-real-8
+#   oracle/_ref/bin/dynref_r4             reference dynamics hot path, default real 4
+#
+# The two dynref programs link the reference's own modules (cnuity, trcadv,
+# edgvar1, edgvar3, momtum, hybgen, hystat, abstart, globsum, dffusn and the
+# control/constant modules they use) to the driver oracle/ref/dynref.F90.
+# No reference source is copied; no stand-in modules are written.
+set -euo pipefail
+REF=${FIM_REFERENCE:-/root/reference}
+S=$REF/FIMsrc
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=$HERE/_ref
+FC=${FC:-amdflang}
+if [ ! -d "$S" ]; then
+  echo "build_ref: $S not present; reference build skipped" >&2
+  exit 0
+fi
+if ! command -v "$FC" >/dev/null 2>&1; then
+  echo "build_ref: $FC not found; reference build skipped" >&2
+  exit 0
+fi
+mkdir -p "$OUT/bin"
+
+CNTL_SRC="$S/cntl/units.F90 $S/utils/module_initial_chem_namelists.F90 $S/cntl/fimnamelist.F90
+          $S/cntl/module_wrf_control.F90 $S/cntl/module_control.F90"
+
+compile() {  # compile <objdir> <flags> <files...>
+  local od=$1; shift; local fl=$1; shift
+  mkdir -p "$od"
+  for f in "$@"; do
+    local b; b=$(basename "${f%.*}")
+    if [ ! -f "$od/$b.o" ] || [ "$f" -nt "$od/$b.o" ]; then
+      (cd "$od" && $FC -c $fl -J"$od" -I"$od" "$f" -o "$od/$b.o")
+    fi
+  done
+}
+
+# ---- grid prep chain (default real 4) ------------------------------------
+G=$S/prep/grid
+GOBJ=$OUT/obj_grid
+GFL="-cpp -DSERIAL -O2 -ffp-contract=off -I$S/dummygptl"
+compile "$GOBJ" "$GFL" $CNTL_SRC $S/utils/headers.F90 $S/sys_share/sys_share.F90 \
+  $G/datastru.F90 $G/avg.F90 $G/middle.F90 $G/third.F90 $G/rotate.F90 \
+  $G/top_gridpoints.F90 $G/top_triangles.F90 $G/trisect.F90 $G/trisect_triangle.F90 \
+  $G/bisect_triangle.F90 $G/icos.F90 \
+  $G/GetRegions.F90 $G/hilbert.F90 $G/ijblock.F90 $G/IJblockLayout.F90 $G/mpi_stubs.F90 \
+  $G/perm.F90 $G/SquareDecomp.F90 $G/SquareLayout.F90 $G/GridGen.F90 \
+  $S/prep/lsfit/LSfitting.F90 $G/getlvl.F90
+compile "$GOBJ" "-O2 -ffp-contract=off" $S/prep/lsfit/svd.f $S/prep/lsfit/blas.f $S/prep/lsfit/la_aux.f
+C="$GOBJ/headers.o $GOBJ/module_initial_chem_namelists.o $GOBJ/fimnamelist.o $GOBJ/module_control.o
+   $GOBJ/module_wrf_control.o $GOBJ/units.o $GOBJ/sys_share.o"
+$FC -o "$OUT/bin/grid" $GOBJ/{avg,bisect_triangle,datastru,icos,middle,rotate,third,top_gridpoints,top_triangles,trisect,trisect_triangle}.o $C
+$FC -o "$OUT/bin/ginfo" $GOBJ/{datastru,GetRegions,GridGen,hilbert,ijblock,IJblockLayout,mpi_stubs,perm,SquareDecomp,SquareLayout}.o $C
+$FC -o "$OUT/bin/getlvl" $GOBJ/{getlvl,LSfitting,svd,blas,la_aux}.o $C
+
+# ---- dynamics hot path, two precisions -----------------------------------
+H=$S/fim/horizontal
+DYN_SRC="$CNTL_SRC $S/cntl/module_constants.F90 $S/cntl/module_variables.F90
+  $H/global_bounds.F90 $S/dummygptl/gptl.F90 $H/findmxmn.F90 $H/stencilprint.F90 $H/stenedgprint.F90
+  $H/dffusn.F90 $H/cnuity_compute.F90 $H/cnuity.F90 $H/trcadv_compute.F90 $H/trcadv.F90
+  $H/edgvar1.F90 $H/edgvar3.F90 $H/momtum_compute.F90 $H/momtum.F90 $H/hybgen.F90
+  $H/hystat.F90 $H/abstart.F90 $H/globsum.F90 $H/GetIpnGlobal.F90 $S/prep/ss2icos/lay2lay.F90 $S/prep/ss2icos/lin2stp.F90"
+for prec in r8 r4; do
+  if [ $prec = r8 ]; then PF="-fdefault-real-8 -fdefault-double-8"; else PF=""; fi
+  OD=$OUT/obj_$prec
+  FL="-cpp -DSERIAL -O2 -ffp-contract=off $PF -I$H -I$S/dummygptl"
+  compile "$OD" "$FL" $DYN_SRC "$HERE/ref/dynref.F90"
+  objs=""
+  for f in $DYN_SRC "$HERE/ref/dynref.F90"; do b=$(basename "${f%.*}"); objs="$objs $OD/$b.o"; done
+  $FC -o "$OUT/bin/dynref_$prec" $objs
+done
+echo "build_ref: reference programs in $OUT/bin"

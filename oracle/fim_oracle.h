@@ -1,0 +1,41 @@
+/* fim_oracle.h - the CPU checker's interface (test infrastructure only).
+ * See fim_oracle.c for the array layouts. */
+#pragma once
+#include <stddef.h>
+
+typedef struct {
+  int nip, nvl, ntra;
+  int pow_kind;              /* 0: C library pow (as the Fortran), 1: fim_pow (as the GPU) */
+  double dt;
+  const int *prox, *proxs, *nprox;
+  const double *cs, *sn, *sidevec_c, *sidevec_e, *sideln, *hfcoef;
+  const double *area, *rarea, *corio, *grndspd;
+  const double *dfflen, *smoo_coeff, *thetac, *dpsig;
+} fo_grid;
+
+typedef struct {
+  double *us, *vs, *dp, *pr, *ex, *ph, *mp, *tr, *trdp;
+  double *u_tdcy, *v_tdcy, *dp_tdcy, *dpl_tdcy, *trc_tdcy, *trl_tdcy, *massfx;
+  double *ws, *relvor, *sdot, *psrf, *ptdcy, *psurf;
+  double *u_edg, *v_edg, *dp_edg, *lp_edg, *mont_edg, *uvsq_edg, *trc_edg;
+  double *work_e1, *work_e2;                               /* edge-sized scratch */
+  double *work_c1, *work_c2, *work_c3, *work_c4, *work_c5; /* cell-sized scratch */
+  double *work_l1, *work_l2;                               /* level-sized scratch */
+  int nf, of, vof;
+  double ab1, ab2, ab3;
+} fo_state;
+
+void fo_vertical_params(int nvl, double dt, double *dfflen, double *smoo_coeff);
+void fo_cell_params(int nip, const double *lat, const double *area, double *corio, double *rarea, double *grndspd);
+void fo_abstart(fo_state *s, int its, double dt);
+void fo_edgvar1(const fo_grid *g, fo_state *s);
+void fo_edgvar3(const fo_grid *g, fo_state *s);
+void fo_cnuity(const fo_grid *g, fo_state *s);
+void fo_trcadv(const fo_grid *g, fo_state *s);
+void fo_momtum(const fo_grid *g, fo_state *s);
+void fo_hybgen(const fo_grid *g, fo_state *s, int its);
+void fo_hystat(const fo_grid *g, fo_state *s, int its);
+void fo_init(const fo_grid *g, fo_state *s, int init_hybgen);
+void fo_step(const fo_grid *g, fo_state *s, int its, int mode);
+void fo_globsum(const fo_grid *g, const fo_state *s, double *mass, double *trtot);
+double fo_pow(double x, double y);

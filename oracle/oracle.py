@@ -1,0 +1,174 @@
+"""ctypes driver for the C restatement in fim_oracle.c (the checker).
+
+Test infrastructure only.  ``OracleRun`` owns a full FP64 model state in the
+reference layout (level fastest) and advances it with the C restatement.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+CSRC = os.path.join(os.path.dirname(HERE), "fimocn_subx_realtime_hera_apr2020_amd", "csrc")
+
+
+def build(force: bool = False) -> str:
+    src = os.path.join(HERE, "fim_oracle.c")
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(src):
+        cmd = ["gcc", "-O2", "-mfma", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
+               "-shared", "-I" + HERE, "-I" + CSRC, src, "-o", LIB + ".tmp", "-lm"]
+        subprocess.check_call(cmd)
+        os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+_PD = ct.POINTER(ct.c_double)
+_PI = ct.POINTER(ct.c_int)
+
+
+class _Grid(ct.Structure):
+    _fields_ = [("nip", ct.c_int), ("nvl", ct.c_int), ("ntra", ct.c_int), ("pow_kind", ct.c_int),
+                ("dt", ct.c_double),
+                ("prox", _PI), ("proxs", _PI), ("nprox", _PI),
+                ("cs", _PD), ("sn", _PD), ("sidevec_c", _PD), ("sidevec_e", _PD), ("sideln", _PD),
+                ("hfcoef", _PD), ("area", _PD), ("rarea", _PD), ("corio", _PD), ("grndspd", _PD),
+                ("dfflen", _PD), ("smoo_coeff", _PD), ("thetac", _PD), ("dpsig", _PD)]
+
+
+_STATE_ARRAYS = ["us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp",
+                 "u_tdcy", "v_tdcy", "dp_tdcy", "dpl_tdcy", "trc_tdcy", "trl_tdcy", "massfx",
+                 "ws", "relvor", "sdot", "psrf", "ptdcy", "psurf",
+                 "u_edg", "v_edg", "dp_edg", "lp_edg", "mont_edg", "uvsq_edg", "trc_edg",
+                 "work_e1", "work_e2", "work_c1", "work_c2", "work_c3", "work_c4", "work_c5",
+                 "work_l1", "work_l2"]
+
+
+class _State(ct.Structure):
+    _fields_ = [(n, _PD) for n in _STATE_ARRAYS] + [
+        ("nf", ct.c_int), ("of", ct.c_int), ("vof", ct.c_int),
+        ("ab1", ct.c_double), ("ab2", ct.c_double), ("ab3", ct.c_double)]
+
+
+def _lib():
+    lib = ct.CDLL(build())
+    for fn in ("fo_edgvar1", "fo_edgvar3", "fo_cnuity", "fo_trcadv", "fo_momtum"):
+        getattr(lib, fn).argtypes = [ct.POINTER(_Grid), ct.POINTER(_State)]
+    lib.fo_hybgen.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), ct.c_int]
+    lib.fo_hystat.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), ct.c_int]
+    lib.fo_init.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), ct.c_int]
+    lib.fo_step.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), ct.c_int, ct.c_int]
+    lib.fo_abstart.argtypes = [ct.POINTER(_State), ct.c_int, ct.c_double]
+    lib.fo_globsum.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), _PD, _PD]
+    lib.fo_vertical_params.argtypes = [ct.c_int, ct.c_double, _PD, _PD]
+    lib.fo_cell_params.argtypes = [ct.c_int, _PD, _PD, _PD, _PD, _PD]
+    lib.fo_pow.argtypes = [ct.c_double, ct.c_double]
+    lib.fo_pow.restype = ct.c_double
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = _lib()
+    return _LIB
+
+
+def _pd(a):
+    return a.ctypes.data_as(_PD)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_PI)
+
+
+def grid_arrays(grid, nvl: int, dt: float, thetac, dpsig) -> dict:
+    """FP64 / 0-based grid arrays in the oracle layout from a glvl.Grid."""
+    nip = grid.nip
+    prox = grid.dyn_prox().astype(np.int32) - 1
+    prox[prox < 0] = 0
+    proxs = grid.proxs.astype(np.int32) - 1
+    proxs[proxs < 0] = 0
+    d = dict(
+        prox=np.ascontiguousarray(prox), proxs=np.ascontiguousarray(proxs),
+        nprox=np.ascontiguousarray(grid.nprox.astype(np.int32)),
+        cs=np.ascontiguousarray(grid.cs, np.float64), sn=np.ascontiguousarray(grid.sn, np.float64),
+        sidevec_c=np.ascontiguousarray(grid.sidevec_c, np.float64),
+        sidevec_e=np.ascontiguousarray(grid.sidevec_e, np.float64),
+        sideln=np.ascontiguousarray(grid.sideln, np.float64),
+        hfcoef=np.ascontiguousarray(grid.hfcoef, np.float64),
+        area=np.ascontiguousarray(grid.area, np.float64),
+        lat=np.ascontiguousarray(grid.lat, np.float64),
+        thetac=np.ascontiguousarray(thetac, np.float64), dpsig=np.ascontiguousarray(dpsig, np.float64),
+    )
+    L = lib()
+    d["rarea"] = np.empty(nip)
+    d["corio"] = np.empty(nip)
+    d["grndspd"] = np.empty(nip)
+    L.fo_cell_params(nip, _pd(d["lat"]), _pd(d["area"]), _pd(d["corio"]), _pd(d["rarea"]), _pd(d["grndspd"]))
+    d["dfflen"] = np.empty(nvl)
+    d["smoo_coeff"] = np.empty(nvl)
+    L.fo_vertical_params(nvl, dt, _pd(d["dfflen"]), _pd(d["smoo_coeff"]))
+    return d
+
+
+class OracleRun:
+    """A full FP64 state advanced by the C restatement."""
+
+    def __init__(self, garr: dict, nvl: int, ntra: int, dt: float, pow_kind: int = 1):
+        self.garr = garr
+        nip = garr["nprox"].shape[0]
+        self.nip, self.nvl, self.ntra, self.dt = nip, nvl, ntra, dt
+        g = _Grid()
+        g.nip, g.nvl, g.ntra, g.pow_kind, g.dt = nip, nvl, ntra, pow_kind, dt
+        for n in ("prox", "proxs", "nprox"):
+            setattr(g, n, _pi(garr[n]))
+        for n in ("cs", "sn", "sidevec_c", "sidevec_e", "sideln", "hfcoef", "area", "rarea", "corio",
+                  "grndspd", "dfflen", "smoo_coeff", "thetac", "dpsig"):
+            setattr(g, n, _pd(garr[n]))
+        self.g = g
+        c, l, e = (nip, nvl), (nip, nvl + 1), (nip, 6, nvl)
+        shapes = dict(us=c, vs=c, dp=c, pr=l, ex=l, ph=l, mp=c, tr=(ntra,) + c, trdp=(ntra,) + c,
+                      u_tdcy=(3,) + c, v_tdcy=(3,) + c, dp_tdcy=(3,) + c, dpl_tdcy=(3,) + c,
+                      trc_tdcy=(ntra, 3) + c, trl_tdcy=(ntra, 3) + c, massfx=(3,) + e,
+                      ws=c, relvor=c, sdot=l, psrf=(nip,), ptdcy=(2, nip), psurf=(nip,),
+                      u_edg=e, v_edg=e, dp_edg=e, lp_edg=e, mont_edg=e, uvsq_edg=e, trc_edg=(ntra,) + e,
+                      work_e1=e, work_e2=e, work_c1=c, work_c2=c, work_c3=c, work_c4=c, work_c5=c,
+                      work_l1=l, work_l2=l)
+        self.a = {n: np.zeros(shapes[n]) for n in _STATE_ARRAYS}
+        s = _State()
+        for n in _STATE_ARRAYS:
+            setattr(s, n, _pd(self.a[n]))
+        self.s = s
+
+    def load(self, st: dict):
+        for n in ("us", "vs", "dp", "pr", "ex", "ph", "tr"):
+            self.a[n][...] = st[n]
+
+    def init(self, init_hybgen: bool = True):
+        lib().fo_init(ct.byref(self.g), ct.byref(self.s), 1 if init_hybgen else 0)
+
+    def step(self, its: int, mode: int = 0):
+        lib().fo_step(ct.byref(self.g), ct.byref(self.s), its, mode)
+
+    def call(self, name: str, *args):
+        getattr(lib(), "fo_" + name)(ct.byref(self.g), ct.byref(self.s), *args)
+
+    def abstart(self, its: int):
+        lib().fo_abstart(ct.byref(self.s), its, self.dt)
+
+    def globsum(self):
+        m = np.zeros(1)
+        t = np.zeros(self.ntra)
+        lib().fo_globsum(ct.byref(self.g), ct.byref(self.s), _pd(m), _pd(t))
+        return float(m[0]), t
+
+    @property
+    def slots(self):
+        return self.s.nf, self.s.of, self.s.vof

@@ -1,0 +1,113 @@
+"""Drive the reference FIM programs built by oracle/build_ref.sh.
+
+Test infrastructure only (the checker): it runs the reference grid prep chain
+(grid -> ginfo -> getlvl) and the reference dynamics driver (dynref_r8 /
+dynref_r4) in scratch directories under oracle/_ref/, and reads their output.
+Available only where /root/reference was present at build time.
+"""
+from __future__ import annotations
+
+import os
+import resource
+import struct
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = os.path.join(HERE, "_ref")
+BIN = os.path.join(REF, "bin")
+TOP_GRID = os.environ.get("FIM_REFERENCE", "/root/reference") + "/FIMrun/top_grid"
+
+_NAMELISTS = ("chemwrf DIAGnamelist gfsphys ISOBARICnamelist LANDnamelist MODELnamelist "
+              "OCEANnamelist OUTPUTnamelist PHYSICSnamelist POSTnamelist QUEUEnamelist "
+              "SYSTEMnamelist TIMEnamelist TOPOnamelist wrfphysics WRITETASKnamelist").split()
+
+
+def available() -> bool:
+    return all(os.path.exists(os.path.join(BIN, b)) for b in ("grid", "ginfo", "getlvl", "dynref_r8"))
+
+
+def ref_glvl(glvl: int) -> str:
+    """Path of the reference glvl.dat for an all-bisection grid of level glvl
+    (gtype=2 modified recursive, curve=0 IJ order), generated on first use."""
+    d = os.path.join(REF, "grids", "G%d" % glvl)
+    out = os.path.join(d, "glvl.dat")
+    if os.path.exists(out):
+        return out
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "FIMnamelist"), "w") as f:
+        for g in _NAMELISTS:
+            f.write(" &%s\n /\n" % g)
+        f.write(" &CNTLnamelist\n   glvl = %d\n   SubdivNum = 20*2\n   nvl = 64\n /\n" % glvl)
+        f.write(" &PREPnamelist\n   curve = 0\n   gtype = 2\n /\n")
+    with open(TOP_GRID) as src, open(os.path.join(d, "top_grid"), "w") as dst:
+        dst.write(src.read())
+    for prog in ("grid", "ginfo", "getlvl"):
+        r = subprocess.run([os.path.join(BIN, prog)], cwd=d, capture_output=True, text=True)
+        if prog == "getlvl" and not os.path.exists(out):
+            raise RuntimeError("reference getlvl failed: %s" % r.stdout[-2000:])
+    return out
+
+
+def ref_file(glvl: int, name: str) -> str:
+    ref_glvl(glvl)
+    return os.path.join(REF, "grids", "G%d" % glvl, name)
+
+
+MAGIC = 1179208497
+
+
+def write_state(path, st, thetac, dpsig, *, nsteps, dt, mode=0, dumpstep=0, init_hybgen=True):
+    nip, nvl = st["dp"].shape
+    ntra = st["tr"].shape[0]
+    with open(path, "wb") as f:
+        f.write(struct.pack("<8i", MAGIC, nvl, nip, ntra, nsteps, mode, dumpstep, 1 if init_hybgen else 0))
+        f.write(struct.pack("<d", dt))
+        for a in (thetac, dpsig, st["us"], st["vs"], st["dp"], st["pr"], st["ex"], st["ph"], st["tr"]):
+            f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+
+
+def read_records(path) -> dict:
+    """Records of a dynref output file; repeated tags (tracers) become lists."""
+    out: dict = {}
+    with open(path, "rb") as f:
+        buf = f.read()
+    off = 0
+    while off < len(buf):
+        tag = buf[off:off + 16].decode().strip()
+        (n,) = struct.unpack_from("<q", buf, off + 16)
+        v = np.frombuffer(buf, dtype="<f8", count=n, offset=off + 24).copy()
+        off += 24 + 8 * n
+        if tag in out:
+            if not isinstance(out[tag], list):
+                out[tag] = [out[tag]]
+            out[tag].append(v)
+        else:
+            out[tag] = v
+    return out
+
+
+def _big_stack():
+    # the reference keeps (nvl,npp,nip) work arrays as automatic (stack) arrays
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
+def run_dynref(glvl_path, st, thetac, dpsig, *, nsteps, dt, mode=0, dumpstep=0,
+               init_hybgen=True, prec="r8", workdir=None) -> dict:
+    workdir = workdir or os.path.join(REF, "runs")
+    os.makedirs(workdir, exist_ok=True)
+    tag = "%d_%d_%d_%s" % (os.getpid(), nsteps, mode, prec)
+    fin = os.path.join(workdir, "state_%s.bin" % tag)
+    fout = os.path.join(workdir, "out_%s.bin" % tag)
+    write_state(fin, st, thetac, dpsig, nsteps=nsteps, dt=dt, mode=mode, dumpstep=dumpstep,
+                init_hybgen=init_hybgen)
+    r = subprocess.run([os.path.join(BIN, "dynref_" + prec), glvl_path, fin, fout],
+                       cwd=workdir, capture_output=True, text=True, preexec_fn=_big_stack)
+    if r.returncode != 0:
+        raise RuntimeError("dynref failed (%d): %s" % (r.returncode, (r.stdout + r.stderr)[-3000:]))
+    rec = read_records(fout)
+    rec["_stdout"] = r.stdout
+    os.remove(fin)
+    os.remove(fout)
+    return rec
