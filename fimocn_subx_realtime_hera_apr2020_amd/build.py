@@ -1,0 +1,62 @@
+"""Build the gfx950 shared library libfimhip.so in-tree with hipcc.
+
+The FP64 kernels are compiled with fp contraction off so that their results
+follow the reference's operation order; packed FP32 instructions are disabled
+for all device code.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "libfimhip.so")
+
+SOURCES = ["fimhip_api.hip", "kernels_plain.hip", "hybgen.hip"]
+HEADERS = ["fimhip_internal.h", "fim_pow.h", "fim_pow_tables.h"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("FIMHIP_ARCH", "gfx950")
+
+COMMON = [
+    "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + INCLUDE, "-I" + CSRC,
+]
+
+
+def _objs_dir():
+    d = os.path.join(PKG, "_build")
+    os.makedirs(d, exist_ok=True)
+    return d
+
+
+def build(force: bool = True, verbose: bool = False) -> str:
+    """Compile every source and link libfimhip.so (always recompiles when force)."""
+    od = _objs_dir()
+    objs = []
+    hdr_mtime = max(os.path.getmtime(os.path.join(CSRC, h)) for h in HEADERS)
+    hdr_mtime = max(hdr_mtime, os.path.getmtime(os.path.join(INCLUDE, "fimhip.h")))
+    for src in SOURCES:
+        sp = os.path.join(CSRC, src)
+        op = os.path.join(od, src.replace(".hip", ".o"))
+        objs.append(op)
+        if not force and os.path.exists(op) and os.path.getmtime(op) > max(os.path.getmtime(sp), hdr_mtime):
+            continue
+        cmd = [HIPCC, *COMMON, "-c", sp, "-o", op]
+        if verbose:
+            print(" ".join(cmd), file=sys.stderr)
+        subprocess.check_call(cmd)
+    tmp = LIB + ".tmp"
+    subprocess.check_call([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,502 @@
+// C ABI of the FIM dynamics library: instance creation, the per-routine
+// drop-in entries with reference argument order, and the device-resident
+// fast path.  See include/fimhip.h.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "fimhip.h"
+#include "fimhip_internal.h"
+#include "fim_pow.h"
+
+using namespace fimhip;
+
+struct fimhip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nip = 0, nvl = 0, ntra = 0;
+  double dt = 0.;
+  DevGrid g{};
+  DevState s{};
+  AB ab{0, 2, 1, 0., 0., 0.};
+  std::vector<void *> allocs;
+  double *gsum_partial = nullptr;   // globsum block partials
+  int gsum_blocks = 0;
+};
+
+#define HIPCHK(x)                                                    \
+  do {                                                               \
+    hipError_t e_ = (x);                                             \
+    if (e_ != hipSuccess) {                                          \
+      fprintf(stderr, "fimhip: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+      return FIMHIP_EHIP;                                            \
+    }                                                                \
+  } while (0)
+
+static int dalloc(fimhip_ctx *c, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (hipMalloc(p, bytes) != hipSuccess) return FIMHIP_ENOMEM;
+  if (hipMemsetAsync(*p, 0, bytes, c->stream) != hipSuccess) return FIMHIP_EHIP;
+  c->allocs.push_back(*p);
+  return 0;
+}
+
+template <class T>
+static int dalloc_t(fimhip_ctx *c, T **p, size_t n) {
+  return dalloc(c, (void **)p, n * sizeof(T));
+}
+
+template <class T>
+static int upload_t(fimhip_ctx *c, T **p, const std::vector<T> &h) {
+  int r = dalloc_t(c, p, h.size());
+  if (r) return r;
+  if (hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    return FIMHIP_EHIP;
+  return 0;
+}
+
+// dyn_init.F90:125 taper for smoo_coeff
+static double taper_smoo(int k1, int k2) {
+  double d = (double)(k2 - k1);
+  return d * d / (9. + d * d);
+}
+
+extern "C" int fimhip_create(const fimhip_grid_desc *gd, int device, fimhip_ctx **out) {
+  if (!gd || !out) return FIMHIP_EINVAL;
+  *out = nullptr;
+  const int nip = gd->nip, nvl = gd->nvl, ntra = gd->ntra;
+  if (nip < 12 || nvl < 4 || nvl > NVL_MAX || ntra < 1 || ntra > 8 || !(gd->dt > 0.)) return FIMHIP_EINVAL;
+  if (!gd->nprox || !gd->prox || !gd->proxs || !gd->cs || !gd->sn || !gd->sidevec_c || !gd->sidevec_e ||
+      !gd->sideln || !gd->hfcoef || !gd->area || !gd->lat || !gd->thetac || !gd->dpsig)
+    return FIMHIP_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return FIMHIP_ENODEV;
+  fimhip_ctx *c = new fimhip_ctx();
+  c->device = device;
+  c->nip = nip; c->nvl = nvl; c->ntra = ntra; c->dt = gd->dt;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return FIMHIP_EHIP;
+  }
+  int r = 0;
+  // ---- grid tables, 0-based, pentagon slot 6 = slot 5 (dyn_init.F90:546-548)
+  std::vector<int> prox(nip * NPP), proxs(nip * NPP), nprox(nip);
+  for (int i = 0; i < nip; i++) {
+    int n = gd->nprox[i];
+    if (n != 5 && n != 6) { fimhip_destroy(c); return FIMHIP_EINVAL; }
+    nprox[i] = n;
+    for (int e = 0; e < NPP; e++) {
+      int ee = (e < n) ? e : n - 1;
+      int p = gd->prox[i * NPP + ee] - 1, q = gd->proxs[i * NPP + ee] - 1;
+      if (p < 0 || p >= nip || q < 0 || q >= NPP) { fimhip_destroy(c); return FIMHIP_EINVAL; }
+      prox[i * NPP + e] = p;
+      proxs[i * NPP + e] = q;
+    }
+  }
+  auto vec = [](const double *p, size_t n) { return std::vector<double>(p, p + n); };
+  std::vector<double> rarea(nip), corio(nip), grndspd(nip), dfflen(nvl), smoo(nvl);
+  for (int i = 0; i < nip; i++) {          // dyn_init.F90:753-757
+    corio[i] = OMEGX2 * sin(gd->lat[i]);
+    rarea[i] = 1. / gd->area[i];
+    grndspd[i] = EARTHRAD * 2. * PI_T * cos(gd->lat[i]) / 86164.;
+  }
+  {                                        // dyn_init.F90:309-321
+    int ndamp = (int)(0.1 * nvl);
+    for (int k = 1; k <= nvl; k++) {
+      double wgt = (double)(k - nvl + ndamp) / (double)ndamp;
+      wgt = wgt > 0. ? wgt : 0.;
+      dfflen[k - 1] = gd->dt * (VELDFF_BKGND * (1. - wgt) + (VELDFF_BKGND > VELDFF_BOOST ? VELDFF_BKGND : VELDFF_BOOST) * wgt);
+      smoo[k - 1] = 0.;
+    }
+    for (int k = KBL + 2; k <= nvl; k++)
+      smoo[k - 1] = gd->dt * INTFC_SMOOTH * taper_smoo(k, KBL + 1) * taper_smoo(k, nvl + 1);
+  }
+  DevGrid &g = c->g;
+  g.nip = nip; g.nvl = nvl; g.ntra = ntra; g.dt = gd->dt;
+  int *dprox, *dproxs, *dnprox;
+  double *dcs, *dsn, *dsvc, *dsve, *dsideln, *dhf, *darea, *drarea, *dcorio, *dgrnd, *ddff, *dsmoo, *dth, *dds;
+  if ((r = upload_t(c, &dprox, prox)) || (r = upload_t(c, &dproxs, proxs)) || (r = upload_t(c, &dnprox, nprox)) ||
+      (r = upload_t(c, &dcs, vec(gd->cs, (size_t)nip * NPP * 4))) ||
+      (r = upload_t(c, &dsn, vec(gd->sn, (size_t)nip * NPP * 4))) ||
+      (r = upload_t(c, &dsvc, vec(gd->sidevec_c, (size_t)nip * NPP * 2))) ||
+      (r = upload_t(c, &dsve, vec(gd->sidevec_e, (size_t)nip * NPP * 2))) ||
+      (r = upload_t(c, &dsideln, vec(gd->sideln, (size_t)nip * NPP))) ||
+      (r = upload_t(c, &dhf, vec(gd->hfcoef, (size_t)nip * NPP * NPP))) ||
+      (r = upload_t(c, &darea, vec(gd->area, nip))) || (r = upload_t(c, &drarea, rarea)) ||
+      (r = upload_t(c, &dcorio, corio)) || (r = upload_t(c, &dgrnd, grndspd)) ||
+      (r = upload_t(c, &ddff, dfflen)) || (r = upload_t(c, &dsmoo, smoo)) ||
+      (r = upload_t(c, &dth, vec(gd->thetac, nvl))) || (r = upload_t(c, &dds, vec(gd->dpsig, nvl)))) {
+    fimhip_destroy(c);
+    return r;
+  }
+  g.prox = dprox; g.proxs = dproxs; g.nprox = dnprox;
+  g.cs = dcs; g.sn = dsn; g.svc = dsvc; g.sve = dsve; g.sideln = dsideln; g.hfcoef = dhf;
+  g.area = darea; g.rarea = drarea; g.corio = dcorio; g.grndspd = dgrnd;
+  g.dfflen = ddff; g.smoo = dsmoo; g.thetac = dth; g.dpsig = dds;
+  // ---- state and work arrays
+  const size_t cz = (size_t)nip * nvl, lz = (size_t)nip * (nvl + 1), ez = (size_t)nip * NPP * nvl;
+  DevState &s = c->s;
+  struct { double **p; size_t n; } arrs[] = {
+      {&s.us, cz}, {&s.vs, cz}, {&s.dp, cz}, {&s.pr, lz}, {&s.ex, lz}, {&s.ph, lz}, {&s.mp, cz},
+      {&s.tr, cz * ntra}, {&s.trdp, cz * ntra}, {&s.u_tdcy, 3 * cz}, {&s.v_tdcy, 3 * cz},
+      {&s.dp_tdcy, 3 * cz}, {&s.dpl_tdcy, 3 * cz}, {&s.trc_tdcy, 3 * cz * ntra}, {&s.trl_tdcy, 3 * cz * ntra},
+      {&s.massfx, 3 * ez}, {&s.ws, cz}, {&s.relvor, cz}, {&s.sdot, lz}, {&s.psrf, (size_t)nip},
+      {&s.ptdcy, 2 * (size_t)nip}, {&s.u_edg, ez}, {&s.v_edg, ez}, {&s.dp_edg, ez}, {&s.lp_edg, ez},
+      {&s.mont_edg, ez}, {&s.uvsq_edg, ez}, {&s.trc_edg, ez * ntra}, {&s.antifx, ez}, {&s.delp_lo, cz},
+      {&s.r_plus, cz}, {&s.r_mnus, cz}, {&s.antiflx, ez * ntra}, {&s.trcr_lo, cz * ntra},
+      {&s.trmax, cz * ntra}, {&s.trmin, cz * ntra}, {&s.tr_rp, cz * ntra}, {&s.tr_rm, cz * ntra},
+      {&s.u2, cz}, {&s.v2, cz}, {&s.exsmo, lz}};
+  for (auto &a : arrs)
+    if ((r = dalloc_t(c, a.p, a.n))) { fimhip_destroy(c); return r; }
+  c->gsum_blocks = 1024;
+  if ((r = dalloc_t(c, &c->gsum_partial, (size_t)c->gsum_blocks * (1 + ntra)))) { fimhip_destroy(c); return r; }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) { fimhip_destroy(c); return FIMHIP_EHIP; }
+  *out = c;
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_destroy(fimhip_ctx *c) {
+  if (!c) return FIMHIP_EINVAL;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return FIMHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+static void ab_rotate(AB &ab, int its, double dt) {   // abstart.F90:38-66
+  ab.nf = ab.nf + 1; if (ab.nf == 4) ab.nf = 1;
+  ab.of = ab.of + 1; if (ab.of == 4) ab.of = 1;
+  ab.vof = ab.vof + 1; if (ab.vof == 4) ab.vof = 1;
+  if (its == 1) { ab.ab1 = dt; ab.ab2 = 0.; ab.ab3 = 0.; }
+  if (its == 2) { ab.ab1 = dt * 1.5; ab.ab2 = -dt * .5; ab.ab3 = 0.; }
+  if (its == 3) { ab.ab1 = dt * 23. / 12.; ab.ab2 = -dt * 16. / 12.; ab.ab3 = dt * 5. / 12.; }
+}
+
+extern "C" int fimhip_abstart(fimhip_ctx *c, int its, int itsDFI, int *nf, int *of, int *vof,
+                              double *a1, double *a2, double *a3) {
+  if (!c || !nf || !of || !vof || !a1 || !a2 || !a3) return FIMHIP_EINVAL;
+  (void)itsDFI;   // digifilt is off on this path
+  AB ab{*nf, *of, *vof, *a1, *a2, *a3};
+  ab_rotate(ab, its, c->dt);
+  *nf = ab.nf; *of = ab.of; *vof = ab.vof; *a1 = ab.ab1; *a2 = ab.ab2; *a3 = ab.ab3;
+  c->ab = ab;
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_set_ab(fimhip_ctx *c, int nf, int of, int vof, double a1, double a2, double a3) {
+  if (!c || nf < 1 || nf > 3 || of < 1 || of > 3 || vof < 1 || vof > 3) return FIMHIP_EINVAL;
+  c->ab = AB{nf, of, vof, a1, a2, a3};
+  return FIMHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+static double **field_ptr(fimhip_ctx *c, int f) {
+  DevState &s = c->s;
+  switch (f) {
+    case FIMHIP_US: return &s.us;
+    case FIMHIP_VS: return &s.vs;
+    case FIMHIP_DP: return &s.dp;
+    case FIMHIP_PR: return &s.pr;
+    case FIMHIP_EX: return &s.ex;
+    case FIMHIP_PH: return &s.ph;
+    case FIMHIP_MP: return &s.mp;
+    case FIMHIP_TR: return &s.tr;
+    case FIMHIP_TRDP: return &s.trdp;
+    case FIMHIP_WS: return &s.ws;
+    case FIMHIP_RELVOR: return &s.relvor;
+    case FIMHIP_SDOT: return &s.sdot;
+    case FIMHIP_PSRF: return &s.psrf;
+    case FIMHIP_PTDCY: return &s.ptdcy;
+    case FIMHIP_U_TDCY: return &s.u_tdcy;
+    case FIMHIP_V_TDCY: return &s.v_tdcy;
+    case FIMHIP_DP_TDCY: return &s.dp_tdcy;
+    case FIMHIP_DPL_TDCY: return &s.dpl_tdcy;
+    case FIMHIP_TRC_TDCY: return &s.trc_tdcy;
+    case FIMHIP_TRL_TDCY: return &s.trl_tdcy;
+    case FIMHIP_MASSFX: return &s.massfx;
+    case FIMHIP_U_EDG: return &s.u_edg;
+    case FIMHIP_V_EDG: return &s.v_edg;
+    case FIMHIP_DP_EDG: return &s.dp_edg;
+    case FIMHIP_LP_EDG: return &s.lp_edg;
+    case FIMHIP_MONT_EDG: return &s.mont_edg;
+    case FIMHIP_UVSQ_EDG: return &s.uvsq_edg;
+    case FIMHIP_TRC_EDG: return &s.trc_edg;
+  }
+  return nullptr;
+}
+
+extern "C" long long fimhip_field_size(const fimhip_ctx *c, int f) {
+  if (!c) return FIMHIP_EINVAL;
+  const long long cz = (long long)c->nip * c->nvl, lz = (long long)c->nip * (c->nvl + 1);
+  const long long ez = (long long)c->nip * NPP * c->nvl;
+  switch (f) {
+    case FIMHIP_US: case FIMHIP_VS: case FIMHIP_DP: case FIMHIP_MP: case FIMHIP_WS: case FIMHIP_RELVOR:
+      return cz;
+    case FIMHIP_PR: case FIMHIP_EX: case FIMHIP_PH: case FIMHIP_SDOT: return lz;
+    case FIMHIP_TR: case FIMHIP_TRDP: return cz * c->ntra;
+    case FIMHIP_PSRF: return c->nip;
+    case FIMHIP_PTDCY: return 2LL * c->nip;
+    case FIMHIP_U_TDCY: case FIMHIP_V_TDCY: case FIMHIP_DP_TDCY: case FIMHIP_DPL_TDCY: return 3 * cz;
+    case FIMHIP_TRC_TDCY: case FIMHIP_TRL_TDCY: return 3 * cz * c->ntra;
+    case FIMHIP_MASSFX: return 3 * ez;
+    case FIMHIP_U_EDG: case FIMHIP_V_EDG: case FIMHIP_DP_EDG: case FIMHIP_LP_EDG: case FIMHIP_MONT_EDG:
+    case FIMHIP_UVSQ_EDG: return ez;
+    case FIMHIP_TRC_EDG: return ez * c->ntra;
+  }
+  return FIMHIP_EINVAL;
+}
+
+static int h2d(fimhip_ctx *c, int f, const double *h) {
+  if (!h) return FIMHIP_EINVAL;
+  double **p = field_ptr(c, f);
+  HIPCHK(hipMemcpyAsync(*p, h, fimhip_field_size(c, f) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+static int d2h(fimhip_ctx *c, int f, double *h) {
+  if (!h) return FIMHIP_EINVAL;
+  double **p = field_ptr(c, f);
+  HIPCHK(hipMemcpyAsync(h, *p, fimhip_field_size(c, f) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
+extern "C" int fimhip_upload(fimhip_ctx *c, int f, const double *h) {
+  if (!c || f < 0 || f >= FIMHIP_NFIELDS) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  int r = h2d(c, f, h);
+  if (r) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int fimhip_download(fimhip_ctx *c, int f, double *h) {
+  if (!c || f < 0 || f >= FIMHIP_NFIELDS) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  int r = d2h(c, f, h);
+  if (r) return r;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int finish(fimhip_ctx *c) {
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+#define TRY(x) do { int r_ = (x); if (r_) return r_; } while (0)
+
+// ---------------------------------------------------------------------------
+// per-routine drop-in entries (reference argument order, host arrays)
+extern "C" int fimhip_edgvar1_r8(fimhip_ctx *c, int its, const double *u_vel, const double *v_vel,
+                                 const double *delp, const double *pres, const double *exner, const double *geop,
+                                 const double *montg, const double *tracr, double *u_edg, double *v_edg,
+                                 double *dp_edg, double *lp_edg, double *geop_edg, double *mont_edg,
+                                 double *uvsq_edg, double *trc_edg) {
+  (void)its; (void)delp; (void)exner; (void)geop; (void)dp_edg; (void)geop_edg;
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(h2d(c, FIMHIP_US, u_vel)); TRY(h2d(c, FIMHIP_VS, v_vel)); TRY(h2d(c, FIMHIP_PR, pres));
+  TRY(h2d(c, FIMHIP_MP, montg)); TRY(h2d(c, FIMHIP_TR, tracr));
+  // trc_edg is intent(out) for tracer 1 only; the other tracers pass through
+  TRY(h2d(c, FIMHIP_TRC_EDG, trc_edg));
+  launch_edgvar1(c->g, c->s, c->stream);
+  TRY(d2h(c, FIMHIP_U_EDG, u_edg)); TRY(d2h(c, FIMHIP_V_EDG, v_edg)); TRY(d2h(c, FIMHIP_LP_EDG, lp_edg));
+  TRY(d2h(c, FIMHIP_MONT_EDG, mont_edg)); TRY(d2h(c, FIMHIP_UVSQ_EDG, uvsq_edg));
+  TRY(d2h(c, FIMHIP_TRC_EDG, trc_edg));
+  return finish(c);
+}
+
+extern "C" int fimhip_edgvar3_r8(fimhip_ctx *c, int its, const double *u_vel, const double *v_vel,
+                                 const double *delp, const double *pres, const double *exner, const double *geop,
+                                 const double *montg, const double *tracr, double *u_edg, double *v_edg,
+                                 double *dp_edg, double *lp_edg, double *geop_edg, double *mont_edg,
+                                 double *uvsq_edg, double *trc_edg) {
+  (void)its; (void)u_vel; (void)v_vel; (void)pres; (void)exner; (void)geop; (void)montg;
+  (void)u_edg; (void)v_edg; (void)lp_edg; (void)geop_edg; (void)mont_edg; (void)uvsq_edg;
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(h2d(c, FIMHIP_DP, delp)); TRY(h2d(c, FIMHIP_TR, tracr));
+  TRY(h2d(c, FIMHIP_TRC_EDG, trc_edg));   // tracer 1 edges pass through
+  launch_edgvar3(c->g, c->s, c->stream);
+  TRY(d2h(c, FIMHIP_DP_EDG, dp_edg)); TRY(d2h(c, FIMHIP_TRC_EDG, trc_edg));
+  return finish(c);
+}
+
+extern "C" int fimhip_cnuity_r8(fimhip_ctx *c, int its, const double *u_velo, const double *v_velo,
+                                const double *u_edg, const double *v_edg, const double *dp_edg,
+                                const double *lp_edg, double *delp, double *pres, double *exner,
+                                double *dp_tndcy, double *dplo_tndcy, double *massfx, double *omega,
+                                int timing) {
+  (void)its; (void)timing;
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(h2d(c, FIMHIP_US, u_velo)); TRY(h2d(c, FIMHIP_VS, v_velo)); TRY(h2d(c, FIMHIP_U_EDG, u_edg));
+  TRY(h2d(c, FIMHIP_V_EDG, v_edg)); TRY(h2d(c, FIMHIP_DP_EDG, dp_edg)); TRY(h2d(c, FIMHIP_LP_EDG, lp_edg));
+  TRY(h2d(c, FIMHIP_DP, delp)); TRY(h2d(c, FIMHIP_PR, pres)); TRY(h2d(c, FIMHIP_EX, exner));
+  TRY(h2d(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(h2d(c, FIMHIP_DPL_TDCY, dplo_tndcy));
+  TRY(h2d(c, FIMHIP_MASSFX, massfx));
+  launch_cnuity(c->g, c->s, c->ab, c->stream);
+  TRY(d2h(c, FIMHIP_DP, delp)); TRY(d2h(c, FIMHIP_PR, pres)); TRY(d2h(c, FIMHIP_EX, exner));
+  TRY(d2h(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(d2h(c, FIMHIP_DPL_TDCY, dplo_tndcy));
+  TRY(d2h(c, FIMHIP_MASSFX, massfx)); TRY(d2h(c, FIMHIP_WS, omega));
+  return finish(c);
+}
+
+extern "C" int fimhip_trcadv_r8(fimhip_ctx *c, int its, const double *trcr_edg, double *tracr, double *trcdp,
+                                double *trc_tdcy, double *trclo_tdcy, const double *massfx, const double *delp,
+                                int timing) {
+  (void)its; (void)timing;
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(h2d(c, FIMHIP_TRC_EDG, trcr_edg)); TRY(h2d(c, FIMHIP_TR, tracr)); TRY(h2d(c, FIMHIP_TRDP, trcdp));
+  TRY(h2d(c, FIMHIP_TRC_TDCY, trc_tdcy)); TRY(h2d(c, FIMHIP_TRL_TDCY, trclo_tdcy));
+  TRY(h2d(c, FIMHIP_MASSFX, massfx)); TRY(h2d(c, FIMHIP_DP, delp));
+  launch_trcadv(c->g, c->s, c->ab, c->stream);
+  TRY(d2h(c, FIMHIP_TR, tracr)); TRY(d2h(c, FIMHIP_TRDP, trcdp));
+  TRY(d2h(c, FIMHIP_TRC_TDCY, trc_tdcy)); TRY(d2h(c, FIMHIP_TRL_TDCY, trclo_tdcy));
+  return finish(c);
+}
+
+extern "C" int fimhip_momtum_r8(fimhip_ctx *c, int its, double *u_velo, double *v_velo, const double *exner,
+                                double *relvor, const double *u_edg, const double *v_edg, const double *trcr_edg,
+                                const double *geop_edg, const double *mont_edg, const double *uvsq_edg,
+                                double *u_tndcy, double *v_tndcy, const double *dp3d) {
+  (void)its; (void)geop_edg;
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(h2d(c, FIMHIP_US, u_velo)); TRY(h2d(c, FIMHIP_VS, v_velo)); TRY(h2d(c, FIMHIP_EX, exner));
+  TRY(h2d(c, FIMHIP_U_EDG, u_edg)); TRY(h2d(c, FIMHIP_V_EDG, v_edg)); TRY(h2d(c, FIMHIP_TRC_EDG, trcr_edg));
+  TRY(h2d(c, FIMHIP_MONT_EDG, mont_edg)); TRY(h2d(c, FIMHIP_UVSQ_EDG, uvsq_edg));
+  TRY(h2d(c, FIMHIP_U_TDCY, u_tndcy)); TRY(h2d(c, FIMHIP_V_TDCY, v_tndcy)); TRY(h2d(c, FIMHIP_DP, dp3d));
+  launch_momtum(c->g, c->s, c->ab, c->stream);
+  TRY(d2h(c, FIMHIP_US, u_velo)); TRY(d2h(c, FIMHIP_VS, v_velo)); TRY(d2h(c, FIMHIP_RELVOR, relvor));
+  TRY(d2h(c, FIMHIP_U_TDCY, u_tndcy)); TRY(d2h(c, FIMHIP_V_TDCY, v_tndcy));
+  return finish(c);
+}
+
+extern "C" int fimhip_hybgen_r8(fimhip_ctx *c, int its, const double *targt, double *us3d, double *vs3d,
+                                double *tr3d, double *sdot, double *ex3d, double *dp3d, double *pr3d,
+                                int timing) {
+  (void)timing;
+  if (!c || !targt) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync((void *)c->g.thetac, targt, c->nvl * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  TRY(h2d(c, FIMHIP_US, us3d)); TRY(h2d(c, FIMHIP_VS, vs3d)); TRY(h2d(c, FIMHIP_TR, tr3d));
+  TRY(h2d(c, FIMHIP_EX, ex3d)); TRY(h2d(c, FIMHIP_DP, dp3d)); TRY(h2d(c, FIMHIP_PR, pr3d));
+  launch_hybgen(c->g, c->s, its, c->stream);
+  TRY(d2h(c, FIMHIP_US, us3d)); TRY(d2h(c, FIMHIP_VS, vs3d)); TRY(d2h(c, FIMHIP_TR, tr3d));
+  TRY(d2h(c, FIMHIP_SDOT, sdot)); TRY(d2h(c, FIMHIP_EX, ex3d)); TRY(d2h(c, FIMHIP_DP, dp3d));
+  TRY(d2h(c, FIMHIP_PR, pr3d));
+  return finish(c);
+}
+
+extern "C" int fimhip_hystat_r8(fimhip_ctx *c, int its, double *ph3d, const double *ex3d, double *mp3d,
+                                const double *dp3d, double *tr3d, double *trdp, double *psrf, double *ptdcy) {
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(h2d(c, FIMHIP_PH, ph3d)); TRY(h2d(c, FIMHIP_EX, ex3d)); TRY(h2d(c, FIMHIP_DP, dp3d));
+  TRY(h2d(c, FIMHIP_TR, tr3d)); TRY(h2d(c, FIMHIP_PSRF, psrf)); TRY(h2d(c, FIMHIP_PTDCY, ptdcy));
+  launch_hystat(c->g, c->s, its, c->stream);
+  TRY(d2h(c, FIMHIP_PH, ph3d)); TRY(d2h(c, FIMHIP_MP, mp3d)); TRY(d2h(c, FIMHIP_TR, tr3d));
+  TRY(d2h(c, FIMHIP_TRDP, trdp)); TRY(d2h(c, FIMHIP_PSRF, psrf)); TRY(d2h(c, FIMHIP_PTDCY, ptdcy));
+  return finish(c);
+}
+
+// ---------------------------------------------------------------------------
+// device-resident fast path
+extern "C" int fimhip_init_state(fimhip_ctx *c, int init_hybgen) {
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  launch_trdp(c->g, c->s, c->stream);
+  if (init_hybgen) {
+    launch_hybgen(c->g, c->s, 0, c->stream);
+    launch_pres_from_exner(c->g, c->s, c->stream);
+  }
+  launch_hystat(c->g, c->s, 0, c->stream);
+  DevState &s = c->s;
+  const size_t cz = (size_t)c->nip * c->nvl, ez = (size_t)c->nip * NPP * c->nvl;
+  HIPCHK(hipMemsetAsync(s.u_tdcy, 0, 3 * cz * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(s.v_tdcy, 0, 3 * cz * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(s.dp_tdcy, 0, 3 * cz * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(s.dpl_tdcy, 0, 3 * cz * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(s.trc_tdcy, 0, 3 * cz * c->ntra * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(s.trl_tdcy, 0, 3 * cz * c->ntra * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(s.massfx, 0, 3 * ez * sizeof(double), c->stream));
+  c->ab = AB{0, 2, 1, 0., 0., 0.};
+  return finish(c);
+}
+
+extern "C" int fimhip_step(fimhip_ctx *c, int its, int mode) {
+  if (!c || its < 1 || (mode != 0 && mode != 1)) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  ab_rotate(c->ab, its, c->dt);
+  launch_edgvar1(c->g, c->s, c->stream);
+  launch_edgvar3(c->g, c->s, c->stream);
+  launch_cnuity(c->g, c->s, c->ab, c->stream);
+  launch_trcadv(c->g, c->s, c->ab, c->stream);
+  if (mode == 0) {
+    launch_momtum(c->g, c->s, c->ab, c->stream);
+    launch_hybgen(c->g, c->s, its, c->stream);
+    launch_hystat(c->g, c->s, its, c->stream);
+  }
+  HIPCHK(hipGetLastError());
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_sync(fimhip_ctx *c) {
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  return finish(c);
+}
+
+// globsum: fixed-order block partials (cells strided by the grid), summed on
+// the host in block order, so the result is reproducible run to run
+__global__ void __launch_bounds__(256) k_globsum(DevGrid g, DevState s, double *part) {
+  __shared__ double sh[256];
+  const int nip = g.nip, nvl = g.nvl, ntra = g.ntra;
+  for (int q = 0; q <= ntra; q++) {
+    double acc = 0.;
+    for (int ipn = blockIdx.x * 256 + threadIdx.x; ipn < nip; ipn += gridDim.x * 256) {
+      double col = 0.;
+      for (int k = 0; k < nvl; k++) {
+        double v = g.area[ipn] * s.dp[(size_t)ipn * nvl + k];
+        if (q > 0) v = v * s.tr[((size_t)(q - 1) * nip + ipn) * nvl + k];
+        col = col + v;
+      }
+      acc = acc + col;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = sh[0];
+    __syncthreads();
+  }
+}
+
+extern "C" int fimhip_globsum(fimhip_ctx *c, double *mass, double *trtot) {
+  if (!c || !mass) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  k_globsum<<<c->gsum_blocks, 256, 0, c->stream>>>(c->g, c->s, c->gsum_partial);
+  std::vector<double> h((size_t)c->gsum_blocks * (1 + c->ntra));
+  HIPCHK(hipMemcpyAsync(h.data(), c->gsum_partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TRY(finish(c));
+  for (int q = 0; q <= c->ntra; q++) {
+    double acc = 0.;
+    for (int b = 0; b < c->gsum_blocks; b++) acc += h[(size_t)q * c->gsum_blocks + b];
+    if (q == 0) *mass = acc / GRVITY;
+    else if (trtot) trtot[q - 1] = acc;
+  }
+  return FIMHIP_OK;
+}
+
+extern "C" double fimhip_pow(double x, double y) { return fim_pow(x, y); }
+extern "C" const char *fimhip_version(void) { return "fimhip 0.1 gfx950"; }

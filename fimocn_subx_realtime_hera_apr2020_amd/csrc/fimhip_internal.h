@@ -1,0 +1,76 @@
+// Internal types of the FIM dynamics library (device state and kernels).
+//
+// Data layout in HBM follows the reference's Fortran layout with the level
+// index fastest (FIMsrc/cntl/module_constants.F90:45-94 and the dummy
+// argument shapes of cnuity.F90:42-71):
+//   cell field  (nvl,nip)          a[ipn*nvl + k]
+//   level field (nvl+1,nip)        a[ipn*(nvl+1) + k]
+//   edge field  (nvl,6,nip)        a[(ipn*6 + e)*nvl + k]
+//   tracers     (nvl,nip,ntra)     a[(t*nip + ipn)*nvl + k]
+//   AB history  (nvl,nip,3)        a[(s*nip + ipn)*nvl + k]
+//   tracer AB   (nvl,nip,3,ntra)   a[((t*3 + s)*nip + ipn)*nvl + k]
+//   massfx      (nvl,6,nip,3)      a[((s*nip + ipn)*6 + e)*nvl + k]
+// Neighbour tables are 0-based on the device, with slot 6 of a pentagon
+// equal to slot 5 (dyn_init.F90:546-548).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fimhip {
+
+constexpr int NPP = 6;
+constexpr int KBL = 2;
+constexpr int NVL_MAX = 64;      // one wavefront lane per level
+
+// module_constants.F90:14-24 and the hot-path namelist defaults
+// (fimnamelist.F90:100-117)
+constexpr double OMEGX2 = 1.4584e-4;
+constexpr double EARTHRAD = 6371220.;
+constexpr double PI_T = 3.14159265;
+constexpr double GRVITY = 9.80665;
+constexpr double P1000 = 100000.;
+constexpr double CP = 1004.6855;
+constexpr double RD = 287.0586;
+constexpr double QVMIN = 1.e-10, QWMIN = 1.e-10;
+constexpr double INTFC_SMOOTH = 30., PTOP = 30., THKTOP = 30.;
+constexpr double RLEIGH = 2., RLTHRESH = 90., RLTAPER = 0.4, SLAK = 0.5;
+constexpr double VELDFF_BKGND = 0.3, VELDFF_BOOST = 3.;
+
+struct DevGrid {
+  int nip, nvl, ntra;
+  double dt;
+  const int *prox, *proxs, *nprox;
+  const double *cs, *sn, *svc, *sve, *sideln, *hfcoef;
+  const double *area, *rarea, *corio, *grndspd;
+  const double *dfflen, *smoo, *thetac, *dpsig;
+};
+
+struct DevState {
+  double *us, *vs, *dp, *pr, *ex, *ph, *mp, *tr, *trdp;
+  double *u_tdcy, *v_tdcy, *dp_tdcy, *dpl_tdcy, *trc_tdcy, *trl_tdcy, *massfx;
+  double *ws, *relvor, *sdot, *psrf, *ptdcy;
+  double *u_edg, *v_edg, *dp_edg, *lp_edg, *mont_edg, *uvsq_edg, *trc_edg;
+  // work
+  double *antifx, *delp_lo, *r_plus, *r_mnus;
+  double *antiflx, *trcr_lo, *trmax, *trmin, *tr_rp, *tr_rm;   // per tracer
+  double *u2, *v2, *exsmo;
+};
+
+struct AB {
+  int nf, of, vof;       // 1-based slots as in module_variables.F90
+  double ab1, ab2, ab3;
+};
+
+// launch helpers (kernels_plain.hip)
+void launch_edgvar1(const DevGrid &g, const DevState &s, hipStream_t st);
+void launch_edgvar3(const DevGrid &g, const DevState &s, hipStream_t st);
+void launch_cnuity(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st);
+void launch_trcadv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st);
+void launch_momtum(const DevGrid &g, DevState &s, const AB &ab, hipStream_t st);
+void launch_hybgen(const DevGrid &g, const DevState &s, int its, hipStream_t st);
+void launch_hystat(const DevGrid &g, const DevState &s, int its, hipStream_t st);
+void launch_pres_from_exner(const DevGrid &g, const DevState &s, hipStream_t st);
+void launch_trdp(const DevGrid &g, const DevState &s, hipStream_t st);
+
+}  // namespace fimhip

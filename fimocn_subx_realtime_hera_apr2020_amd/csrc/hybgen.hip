@@ -1,0 +1,344 @@
+// hybgen (FIMsrc/fim/horizontal/hybgen.F90) on the GPU, in FP64.
+//
+// regrid_1d (207-512), restp_1d (792-890), equilb (1282-1437) and inflate
+// (893-1007) run one thread per column with the column held in private
+// arrays (the scratch segment is lane-interleaved, so lane i's element k and
+// lane i+1's element k are adjacent and the accesses coalesce).  The lateral
+// interface smoothing dffusn_lev (dffusn.F90:37-58) is a cell-level kernel.
+// remap_1d (515-789) with pcwise (1010-1279) is again one thread per column.
+// Loop orders and operation orders are those of the reference.
+#include "fimhip_internal.h"
+#include "fim_pow.h"
+
+namespace fimhip {
+
+#define C2(a, ipn, k) (a)[(size_t)(ipn) * nvl + (k)]
+#define L2(a, ipn, k) (a)[(size_t)(ipn) * nvlp1 + (k)]
+#define TR(a, t, ipn, k) (a)[((size_t)(t) * nip + (ipn)) * nvl + (k)]
+#define PROX(ipn, e) g.prox[(ipn) * NPP + (e)]
+
+constexpr int LMAX = NVL_MAX + 1;
+constexpr int NTRA_MAX = 8;
+constexpr double THIN = RD / P1000;
+
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+
+__device__ void restp_1d(const double *thold, const double *pkold, int kold, double *thnew, double *pknew,
+                         const double *targt, int knew) {
+  double oldth[NVL_MAX];
+  oldth[kold - 1] = thold[kold - 1];
+  for (int k = kold; k >= 2; k--) oldth[k - 2] = dmin(oldth[k - 1], thold[k - 2]);
+  for (int k = 0; k < knew; k++) thnew[k] = targt[k];
+  thnew[0] = dmin(dmin(oldth[0], oldth[kold - 1]), targt[0]);
+  thnew[knew - 1] = dmax(dmax(oldth[0], oldth[kold - 1]), targt[knew - 1]);
+  pknew[0] = pkold[0];
+  pknew[knew] = pkold[kold];
+  int ko = 1;
+  for (int k = 1; k <= knew - 1; k++) {
+    double pinteg = 0., tha, thb = thnew[k - 1];
+    for (;;) {
+      tha = thb;
+      thb = dmin(thnew[k], dmax(thnew[k - 1], oldth[ko - 1]));
+      pinteg = pinteg + pkold[ko - 1] * (thb - tha);
+      if (oldth[ko - 1] < thnew[k]) {
+        if (ko < kold) { ko = ko + 1; continue; }
+        tha = thb;
+        thb = thnew[k];
+        pinteg = pinteg + pkold[kold] * (thb - tha);
+      }
+      break;
+    }
+    pknew[k] = pknew[k - 1];
+    if (thnew[k] > thnew[k - 1]) pknew[k] = pinteg / (thnew[k] - thnew[k - 1]);
+  }
+}
+
+__device__ __forceinline__ double cushn(double arg, double arg1) {
+  double m = dmin(arg, 2. * arg1);
+  return .25 * (m * m) / arg1 + dmax(arg, 2. * arg1) - arg1;
+}
+
+__device__ void inflate(const double *dpsig, int nvl, double psurf, double *exner) {
+  double dpsum = 0.;
+  const double extop = CP * fim_pow(PTOP / P1000, RD / CP);
+  const double top = .2;
+  double shrink = dmin(1., (psurf / P1000 - top) / (1. - top));
+  int last = 1;
+  for (int k = 1; k <= nvl - 1; k++) {
+    double dp0 = RD * dpsig[k - 1] / P1000 * shrink;
+    dpsum = dpsum + dp0;
+    double exold = exner[k];
+    if (k <= KBL) {
+      double dp1 = dp0;
+      last = k + 1;
+      exner[k] = exner[k - 1] - dp1;
+    } else {
+      if (k > last) {
+        double p2 = 1.;
+        for (int i = 0; i < k - last; i++) p2 = p2 * 2.;
+        dp0 = dp0 / dmin(10., 1.25 * p2);
+        dp0 = dmax(dp0, RD * 100. / P1000);
+      }
+      double dp1 = cushn(dmax(0., exner[k - 1] - dmin(exold, exner[0] - dpsum)), dp0);
+      if (exner[k - 1] - dp1 < exold - THIN) {
+        if (k == last) last = k + 1;
+        exner[k] = dmax(extop, exner[k - 1] - dp1);
+      }
+    }
+  }
+  for (int k = nvl; k >= 2; k--) {
+    double q = exner[k - 1] / CP;
+    q = q * q * sqrt(q);
+    double tq = .25 * (double)(nvl - k);
+    double tap = 1. / (1. + tq * tq);
+    double dp0 = RD * THKTOP / P1000 * tap / q;
+    double exold = exner[k - 1];
+    double dp1 = cushn(dmax(0., exold - exner[k]), dp0);
+    if (exner[k] + dp1 > exold + THIN) exner[k - 1] = exner[k] + dp1;
+    if (exner[k - 1] < exner[k - 2] - dp0) break;
+  }
+}
+
+__device__ void equilb(const double *thet, double *pnew, int kk, double slak) {
+  for (int k = 3; k <= kk - 2; k++) {
+    if (pnew[k - 3] > pnew[0] - THIN) continue;
+    double dp1 = pnew[k - 3] - pnew[k - 2];
+    double dp2 = pnew[k - 2] - pnew[k - 1];
+    double dp3 = pnew[k - 1] - pnew[k];
+    double dp4 = pnew[k] - pnew[k + 1];
+    double dp5 = pnew[k + 1] - pnew[k + 2];
+    double th2 = thet[k - 2], th3 = thet[k - 1], th4 = thet[k];
+    if (dp2 > dp1 && dp4 > dp5) {
+      double goal = .5 * (dp3 + dmin(dp2, dp4));
+      if (dp2 > goal && dp4 > goal) {
+        double dis3 = dmin(dp2 - goal, goal - dp3) * slak;
+        double dis4 = dmin(dp4 - goal, goal - dp3) * slak;
+        if (th3 > th2 && th4 > th3) {
+          double ratio = (th4 - th3) / (th3 - th2);
+          if (ratio > 1.) dis4 = dis3 / ratio;
+          else dis3 = dis4 * ratio;
+        }
+        pnew[k - 1] = pnew[k - 1] + dis3;
+        pnew[k] = pnew[k] - dis4;
+      }
+    }
+  }
+}
+
+// regrid_1d for one column: thcol (theta, modified), excol (Exner, in/out)
+__global__ void __launch_bounds__(64) k_regrid(DevGrid g, DevState s, int its) {
+  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
+  const int ipn = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ipn >= nip) return;
+  double thcol[NVL_MAX], excol[LMAX], exwrk[LMAX], exnew[LMAX], thnew[NVL_MAX], heatfx[NVL_MAX];
+  const double *targt = g.thetac;
+  const double dffudt = .1, tolrnce = 0.001;
+  for (int k = 0; k < nvl; k++) thcol[k] = TR(s.tr, 0, ipn, k);
+  for (int k = 0; k < nvlp1; k++) excol[k] = L2(s.ex, ipn, k);
+  const double psurf = L2(s.pr, ipn, 0);
+  double eqlb_slak = .1 * (double)its / (10. + (double)its);
+  for (int k = 1; k <= nvl; k++)
+    if (excol[k - 1] < excol[k] - THIN) excol[k] = excol[k - 1];
+  for (int k = nvl - 1; k >= 1; k--) thcol[k - 1] = dmin(thcol[k - 1], thcol[k]);
+  for (int k = 0; k < nvlp1; k++) exwrk[k] = excol[k];
+  const double ttop = targt[nvl - 1];
+  for (int k = nvl; k >= 2; k--) {
+    if (thcol[k - 1] <= ttop + tolrnce) break;
+    double wgt1 = dmax(0., exwrk[k - 2] - exwrk[k - 1]);
+    double wgt2 = dmax(THIN, exwrk[k - 1] - exwrk[k]);
+    double try_ = (thcol[k - 2] * wgt1 + thcol[k - 1] * wgt2) / (wgt1 + wgt2);
+    if (try_ > ttop - tolrnce) {
+      exwrk[k - 1] = exwrk[nvlp1 - 1];
+      for (int k1 = k - 1; k1 <= nvl; k1++) thcol[k1 - 1] = try_;
+    } else {
+      exwrk[k - 1] = dmin(exwrk[k - 2], dmax(exwrk[k], (exwrk[k - 1] * (thcol[k - 1] - thcol[k - 2]) +
+                                                         exwrk[k] * (ttop - thcol[k - 1])) /
+                                                            (ttop - thcol[k - 2])));
+      for (int k1 = k; k1 <= nvl; k1++) thcol[k1 - 1] = ttop;
+      break;
+    }
+  }
+  restp_1d(thcol, exwrk, nvl, thnew, exnew, targt, nvl);
+  heatfx[nvl - 1] = 0.;
+  heatfx[0] = 0.;
+  for (int iter = 1; iter <= 3; iter++) {
+    for (int k = 0; k < nvlp1; k++) exwrk[k] = exnew[k];
+    for (int k = 2; k <= nvl - 1; k++) {
+      heatfx[k - 1] = 0.;
+      if (exwrk[k - 1] < exwrk[0] - .01 && exwrk[k] > exwrk[nvlp1 - 1] + .01) {
+        double cnv = GRVITY / targt[k - 1];
+        heatfx[k - 1] = .1 * cnv * cnv * .5 * (targt[k] - targt[k - 2]) / dmax(.03 * cnv, exwrk[k - 1] - exwrk[k]);
+      }
+    }
+    for (int k = 1; k <= nvl - 1; k++) {
+      if (exwrk[k] < exwrk[0] - .01 && exwrk[k] > exwrk[nvlp1 - 1] + .01)
+        exnew[k] = dmax(exwrk[k + 1], dmin(exwrk[k - 1], exwrk[k] + (heatfx[k] - heatfx[k - 1]) / (targt[k] - targt[k - 1])));
+    }
+    bool event = false;
+    for (int k = 2; k <= nvlp1; k++)
+      if (exnew[k - 1] > exnew[k - 2] + THIN) event = true;
+    if (!event) break;
+  }
+  (void)dffudt;
+  equilb(thnew, exnew, nvl, eqlb_slak);
+  inflate(g.dpsig, nvl, psurf, exnew);
+  for (int k = 0; k < nvlp1; k++) L2(s.sdot, ipn, k) = exnew[k];   // pre-smoothing Exner, kept in sdot
+}
+
+// dffusn_lev(exsmo, smoo_coeff, nvlp1, kbl+2, nvl): reads the regridded Exner
+// function from sdot and writes the smoothed one to exsmo
+__global__ void __launch_bounds__(256) k_dffusn_lev(DevGrid g, DevState s) {
+  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
+  const int ipn = blockIdx.x * 4 + threadIdx.y;
+  if (ipn >= nip) return;
+  const int k1 = KBL + 2, k2 = nvl;
+  for (int kk = threadIdx.x; kk < nvlp1; kk += blockDim.x) {
+    const double f = L2(s.sdot, ipn, kk);
+    double out = f;
+    const int k = kk + 1;
+    if (k >= k1 && k <= k2) {
+      double flx = 0.;
+      const int np = g.nprox[ipn];
+      for (int edg = 0; edg < np; edg++)
+        flx = flx + (f - L2(s.sdot, PROX(ipn, edg), kk)) * g.sideln[ipn * NPP + edg];
+      double factor = -g.smoo[kk] * g.rarea[ipn];
+      out = f + flx * factor;
+    }
+    L2(s.exsmo, ipn, kk) = out;
+  }
+}
+
+__device__ void pcwise(int kk, const double *xold, const double *yold, const double *xnew, double *ynew) {
+  double zold[LMAX], znew[LMAX], fco[NVL_MAX], fcn[NVL_MAX], vertfx[LMAX], fld[NVL_MAX], b[NVL_MAX];
+  const double small = 1.e-22;
+  if (xold[0] < xold[kk]) {
+    for (int k = 0; k <= kk; k++) { zold[k] = xold[k]; znew[k] = xnew[k]; }
+  } else {
+    for (int k = 0; k <= kk; k++) { zold[k] = -xold[k]; znew[k] = -xnew[k]; }
+  }
+  for (int k = 0; k < kk; k++) {
+    fco[k] = dmax(0., zold[k + 1] - zold[k]);
+    fcn[k] = dmax(0., znew[k + 1] - znew[k]);
+  }
+  for (int k = 0; k < kk; k++) fld[k] = yold[k];
+  for (int k = kk - 1; k >= 1; k--)
+    fld[k - 1] = (fld[k - 1] * fco[k - 1] + fld[k] * small) / (fco[k - 1] + small);
+  for (int k = 2; k <= kk; k++)
+    fld[k - 1] = (fld[k - 1] * fco[k - 1] + fld[k - 2] * small) / (fco[k - 1] + small);
+  b[0] = 0.;
+  b[kk - 1] = 0.;
+  for (int k = 2; k <= kk - 1; k++) {   // PLM (remap_optn = 2)
+    double fm = fld[k - 2], f0 = fld[k - 1], fp = fld[k];
+    b[k - 1] = 0.;
+    if (f0 <= dmin(fm, fp) || f0 >= dmax(fm, fp)) b[k - 1] = 0.;
+    else if ((fp - fm) * (fm + fp - 2. * f0) > 0.) b[k - 1] = f0 - fm;
+    else b[k - 1] = fp - f0;
+  }
+  for (int k = 2; k <= kk; k++) {       // miglim = 1
+    double slab = 0., amount = 0.;
+    vertfx[k - 1] = 0.;
+    const double dz = znew[k - 1] - zold[k - 1];
+    if (dz > 0.) {
+      const int lyr = k;
+      if (fco[lyr - 1] > 0.) {
+        double dslab = dmin(slab + fco[lyr - 1], dz) - dmin(slab, dz);
+        double dx = dslab / fco[lyr - 1];
+        double fcdx = fld[lyr - 1] + b[lyr - 1] * .5 * (dx - 1.) + 0. * (.25 - dx * (.5 - dx * (1. / 3.)));
+        amount = amount + fcdx * dslab;
+        slab = slab + dslab;
+      }
+    } else if (dz < 0.) {
+      const int lyr = k - 1;
+      if (fco[lyr - 1] > 0.) {
+        double dslab = dmin(slab + fco[lyr - 1], -dz) - dmin(slab, -dz);
+        double dx = dslab / fco[lyr - 1];
+        double fcdx = fld[lyr - 1] + b[lyr - 1] * .5 * (1. - dx) + 0. * (.25 - dx * (.5 - dx * (1. / 3.)));
+        amount = amount + fcdx * dslab;
+        slab = slab + dslab;
+      }
+    }
+    if (slab != 0.) vertfx[k - 1] = -dz * amount / slab;
+  }
+  vertfx[0] = 0.;
+  vertfx[kk] = 0.;
+  for (int k = 0; k < kk; k++) {
+    double vertdv = vertfx[k + 1] - vertfx[k];
+    double amount = fld[k] * fco[k] - vertdv;
+    ynew[k] = (amount + fld[k] * small) / (fcn[k] + small);
+  }
+}
+
+// remap_1d for one column; also sdot = new Exner - old Exner and ex = new Exner
+__global__ void __launch_bounds__(64) k_remap(DevGrid g, DevState s, int its) {
+  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1, ntra = g.ntra;
+  const int ipn = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ipn >= nip) return;
+  double exold[LMAX], prcol[LMAX], pk1col[LMAX], exnew[LMAX], prnew[LMAX], pk1new[LMAX];
+  double col[NVL_MAX], cnew[NVL_MAX], th1[NVL_MAX];
+  const double *targt = g.thetac;
+  const double small = 1.e-6;
+  for (int k = 0; k < nvlp1; k++) {
+    exold[k] = L2(s.ex, ipn, k);
+    prcol[k] = L2(s.pr, ipn, k);
+    exnew[k] = L2(s.exsmo, ipn, k);
+  }
+  for (int k = 0; k < nvlp1; k++) pk1col[k] = exold[k] * prcol[k] / CP;
+  inflate(g.dpsig, nvl, prcol[0], exnew);
+  if (its > 0)
+    for (int k = 0; k < nvlp1; k++) exnew[k] = SLAK * exnew[k] + (1. - SLAK) * exold[k];
+  for (int k = 2; k <= nvlp1 - 1; k++)
+    exnew[k - 1] = dmax(.0001 * exold[k - 1] + .9999 * exold[k], dmin(.0001 * exold[k - 1] + .9999 * exold[k - 2], exnew[k - 1]));
+  for (int k = nvlp1; k >= 1; k--) prnew[k - 1] = P1000 * fim_pow(exnew[k - 1] / CP, CP / RD);
+  for (int k = 0; k < nvl; k++) C2(s.dp, ipn, k) = prnew[k] - prnew[k + 1];
+  for (int k = 0; k < nvlp1; k++) pk1new[k] = exnew[k] * prnew[k] / CP;
+  prnew[nvlp1 - 1] = prcol[nvlp1 - 1];
+  pk1new[nvlp1 - 1] = pk1col[nvlp1 - 1];
+  prnew[0] = prcol[0];
+  pk1new[0] = pk1col[0];
+  // u, v and tracers 2..ntra are remapped in pressure
+  for (int k = 0; k < nvl; k++) col[k] = C2(s.us, ipn, k);
+  pcwise(nvl, prcol, col, prnew, cnew);
+  for (int k = 0; k < nvl; k++) C2(s.us, ipn, k) = cnew[k];
+  for (int k = 0; k < nvl; k++) col[k] = C2(s.vs, ipn, k);
+  pcwise(nvl, prcol, col, prnew, cnew);
+  for (int k = 0; k < nvl; k++) C2(s.vs, ipn, k) = cnew[k];
+  for (int n = 1; n < ntra; n++) {
+    for (int k = 0; k < nvl; k++) col[k] = TR(s.tr, n, ipn, k);
+    pcwise(nvl, prcol, col, prnew, cnew);
+    for (int k = 0; k < nvl; k++) TR(s.tr, n, ipn, k) = cnew[k];
+  }
+  // theta is remapped in p^(kappa+1)
+  for (int k = 0; k < nvl; k++) col[k] = TR(s.tr, 0, ipn, k);
+  pcwise(nvl, pk1col, col, pk1new, th1);
+  for (int k = nvl; k >= 3; k--) {
+    double dplo = dmax(pk1new[k - 2] - pk1new[k - 1], pk1new[0] * small);
+    double dpup = dmax(pk1new[k - 1] - pk1new[k], pk1new[0] * small);
+    double tha = th1[k - 2], thb = th1[k - 1];
+    double devilo = (tha - targt[k - 2]) * dplo;
+    double deviup = (thb - targt[k - 1]) * dpup;
+    if (deviup > 0. && devilo < 0.) {
+      th1[k - 2] = tha + dmin(deviup, -devilo) / dplo;
+      th1[k - 1] = thb - dmin(deviup, -devilo) / dpup;
+    } else if (deviup < 0. && devilo > 0.) {
+      th1[k - 2] = tha - dmin(devilo, -deviup) / dplo;
+      th1[k - 1] = thb + dmin(devilo, -deviup) / dpup;
+    }
+  }
+  for (int k = 0; k < nvl; k++) TR(s.tr, 0, ipn, k) = th1[k];
+  for (int k = 1; k <= nvl; k++)
+    if (exnew[k - 1] < exnew[k]) exnew[k] = exnew[k - 1];
+  for (int k = 0; k < nvlp1; k++) {
+    L2(s.pr, ipn, k) = prnew[k];
+    L2(s.sdot, ipn, k) = exnew[k] - exold[k];
+    L2(s.ex, ipn, k) = exnew[k];
+  }
+}
+
+void launch_hybgen(const DevGrid &g, const DevState &s, int its, hipStream_t st) {
+  k_regrid<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its);
+  k_dffusn_lev<<<(g.nip + 3) / 4, dim3(64, 4), 0, st>>>(g, s);
+  k_remap<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its);
+}
+
+}  // namespace fimhip

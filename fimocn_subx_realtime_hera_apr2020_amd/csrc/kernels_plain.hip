@@ -1,0 +1,507 @@
+// FP64 kernels of the FIM dynamics hot path, one kernel family per reference
+// routine, following the reference's operation order exactly (build with
+// -ffp-contract=off; x**y through fim_pow).  Mapping: one wavefront lane per
+// level and one wavefront per cell column (block = 64 x 4 columns), so every
+// neighbour-column gather through prox is a coalesced 64-element load.  The
+// column-serial parts of hybgen and hystat run one thread per column.
+#include "fimhip_internal.h"
+#include "fim_pow.h"
+
+namespace fimhip {
+
+#define C2(a, ipn, k) (a)[(size_t)(ipn) * nvl + (k)]
+#define L2(a, ipn, k) (a)[(size_t)(ipn) * nvlp1 + (k)]
+#define E3(a, ipn, e, k) (a)[((size_t)(ipn) * NPP + (e)) * nvl + (k)]
+#define AB3(a, s, ipn, k) (a)[((size_t)(s) * nip + (ipn)) * nvl + (k)]
+#define TR(a, t, ipn, k) (a)[((size_t)(t) * nip + (ipn)) * nvl + (k)]
+#define TAB(a, t, s, ipn, k) (a)[(((size_t)(t) * 3 + (s)) * nip + (ipn)) * nvl + (k)]
+#define MF(a, s, ipn, e, k) (a)[(((size_t)(s) * nip + (ipn)) * NPP + (e)) * nvl + (k)]
+#define PROX(ipn, e) g.prox[(ipn) * NPP + (e)]
+#define PROXS(ipn, e) g.proxs[(ipn) * NPP + (e)]
+#define CS(ipn, e, i) g.cs[((ipn) * NPP + (e)) * 4 + (i)]
+#define SN(ipn, e, i) g.sn[((ipn) * NPP + (e)) * 4 + (i)]
+#define SVC(ipn, e, i) g.svc[((ipn) * NPP + (e)) * 2 + (i)]
+#define SVE(ipn, e, i) g.sve[((ipn) * NPP + (e)) * 2 + (i)]
+#define HF(ipn, i, j) g.hfcoef[((ipn) * NPP + (i)) * NPP + (j)]
+
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+
+constexpr int COLS = 4;   // columns per block
+
+static inline dim3 col_grid(int nip) { return dim3((nip + COLS - 1) / COLS); }
+static inline dim3 col_block() { return dim3(NVL_MAX, COLS); }
+
+// ---------------------------------------------------------------------------
+// edgvar1.F90:160-421 (eqwgt branch) + tracer-1 loop 425-511
+__global__ void __launch_bounds__(256) k_edgvar1(DevGrid g, DevState s) {
+  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
+  const int k = threadIdx.x;
+  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  if (ipn >= nip || k >= nvl) return;
+  const double divby6 = 1. / 6., eps = 0.001;
+  const int np = g.nprox[ipn];
+  for (int edg = 0; edg < np; edg++) {
+    const int i1 = ipn, i2 = PROX(ipn, edg);
+    const int i3t = PROX(ipn, (edg - 1 + np) % np);
+    const int i3l = PROX(ipn, (edg + 1) % np);
+    double u3trl = CS(ipn, edg, 2) * C2(s.us, i3t, k) + SN(ipn, edg, 2) * C2(s.vs, i3t, k);
+    double v3trl = -SN(ipn, edg, 2) * C2(s.us, i3t, k) + CS(ipn, edg, 2) * C2(s.vs, i3t, k);
+    double u1ldg = CS(ipn, edg, 0) * C2(s.us, i1, k) + SN(ipn, edg, 0) * C2(s.vs, i1, k);
+    double v1ldg = -SN(ipn, edg, 0) * C2(s.us, i1, k) + CS(ipn, edg, 0) * C2(s.vs, i1, k);
+    double u2ldg = CS(ipn, edg, 1) * C2(s.us, i2, k) + SN(ipn, edg, 1) * C2(s.vs, i2, k);
+    double v2ldg = -SN(ipn, edg, 1) * C2(s.us, i2, k) + CS(ipn, edg, 1) * C2(s.vs, i2, k);
+    double u3ldg = CS(ipn, edg, 3) * C2(s.us, i3l, k) + SN(ipn, edg, 3) * C2(s.vs, i3l, k);
+    double v3ldg = -SN(ipn, edg, 3) * C2(s.us, i3l, k) + CS(ipn, edg, 3) * C2(s.vs, i3l, k);
+    double ue = (2. * (u1ldg + u2ldg) + u3trl + u3ldg) * divby6;
+    double ve = (2. * (v1ldg + v2ldg) + v3trl + v3ldg) * divby6;
+    double m_1ldg = sqrt(u1ldg * u1ldg + v1ldg * v1ldg);
+    double m_2ldg = sqrt(u2ldg * u2ldg + v2ldg * v2ldg);
+    double m_3trl = sqrt(u3trl * u3trl + v3trl * v3trl);
+    double m_3ldg = sqrt(u3ldg * u3ldg + v3ldg * v3ldg);
+    double m_1 = sqrt(ue * ue + ve * ve);
+    double m_2 = (2. * (m_1ldg + m_2ldg) + m_3trl + m_3ldg) * divby6;
+    ue = ue * m_2 / dmax(eps, m_1);
+    ve = ve * m_2 / dmax(eps, m_1);
+    E3(s.u_edg, ipn, edg, k) = ue;
+    E3(s.v_edg, ipn, edg, k) = ve;
+    E3(s.lp_edg, ipn, edg, k) =
+        (L2(s.pr, i1, k) + L2(s.pr, i1, k + 1) + L2(s.pr, i2, k) + L2(s.pr, i2, k + 1) +
+         .5 * (L2(s.pr, i3t, k) + L2(s.pr, i3t, k + 1) + L2(s.pr, i3l, k) + L2(s.pr, i3l, k + 1))) * divby6;
+    E3(s.mont_edg, ipn, edg, k) =
+        (2. * (C2(s.mp, i1, k) + C2(s.mp, i2, k)) + C2(s.mp, i3t, k) + C2(s.mp, i3l, k)) * divby6;
+    E3(s.uvsq_edg, ipn, edg, k) = .5 * (ue * ue + ve * ve);
+    E3(s.trc_edg, ipn, edg, k) = (2. * (TR(s.tr, 0, i1, k) + TR(s.tr, 0, i2, k)) + TR(s.tr, 0, i3t, k) +
+                                  TR(s.tr, 0, i3l, k)) * divby6;
+  }
+  if (np < NPP) E3(s.trc_edg, ipn, NPP - 1, k) = 0.;
+}
+
+// ---------------------------------------------------------------------------
+// edgvar3.F90:65-163: LS reconstruction on both sides of each edge
+// (edgvalEval) and the two-sided average of loop 2, fused: each cell-level
+// recomputes its neighbour's reconstruction of the shared edge.
+__device__ __forceinline__ double edgval(const DevGrid &g, const double *fld, int c, int i, int k, int nvl) {
+  const double b0 = fld[(size_t)c * nvl + k];
+  double v = b0;
+  const int n = g.nprox[c];
+  for (int j = 0; j < n; j++) v = v + (fld[(size_t)PROX(c, j) * nvl + k] - b0) * HF(c, i, j);
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_edgvar3(DevGrid g, DevState s) {
+  const int nip = g.nip, nvl = g.nvl;
+  const int k = threadIdx.x;
+  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int f = blockIdx.y;   // 0: dp, f >= 1: tracer f
+  if (ipn >= nip || k >= nvl) return;
+  const double *fld = f == 0 ? s.dp : s.tr + (size_t)f * nip * nvl;
+  double *out = f == 0 ? s.dp_edg : s.trc_edg + (size_t)f * nip * NPP * nvl;
+  const int np = g.nprox[ipn];
+  for (int i = 0; i < np; i++) {
+    const int nb = PROX(ipn, i), eo = PROXS(ipn, i);
+    double own = edgval(g, fld, ipn, i, k, nvl);
+    double oth = edgval(g, fld, nb, eo, k, nvl);
+    E3(out, ipn, i, k) = 0.5 * (own + oth);
+  }
+  for (int i = np; i < NPP; i++) E3(out, ipn, i, k) = 0.;
+}
+
+// ---------------------------------------------------------------------------
+// cnuity1 + cnuity2 (cnuity_compute.F90:17-150)
+__global__ void __launch_bounds__(256) k_cnuity12(DevGrid g, DevState s, AB ab) {
+  const int nip = g.nip, nvl = g.nvl;
+  const int k = threadIdx.x;
+  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  if (ipn >= nip || k >= nvl) return;
+  const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
+  const int np = g.nprox[ipn];
+  const double dpi = C2(s.dp, ipn, k);
+  double acc = 0.;
+  for (int edg = 0; edg < np; edg++) {
+    const int ipx = PROX(ipn, edg), edx = PROXS(ipn, edg);
+    double vn = SVE(ipn, edg, 1) * E3(s.u_edg, ipn, edg, k) - SVE(ipn, edg, 0) * E3(s.v_edg, ipn, edg, k);
+    double vx = SVE(ipx, edx, 1) * E3(s.u_edg, ipx, edx, k) - SVE(ipx, edx, 0) * E3(s.v_edg, ipx, edx, k);
+    double flxhi = vn * E3(s.dp_edg, ipn, edg, k);
+    double m = 0.5 * ((vn + fabs(vn)) * dpi - (vx + fabs(vx)) * C2(s.dp, ipx, k));
+    MF(s.massfx, nf, ipn, edg, k) = m;
+    E3(s.antifx, ipn, edg, k) = flxhi - m;
+    acc = acc + m;
+  }
+  if (np < NPP) {
+    MF(s.massfx, nf, ipn, NPP - 1, k) = 0.;
+    E3(s.antifx, ipn, NPP - 1, k) = 0.;
+  }
+  double dplo = -acc * g.rarea[ipn];
+  AB3(s.dpl_tdcy, nf, ipn, k) = dplo;
+  C2(s.delp_lo, ipn, k) = dpi + ab.ab1 * dplo + ab.ab2 * AB3(s.dpl_tdcy, of, ipn, k) +
+                          ab.ab3 * AB3(s.dpl_tdcy, vof, ipn, k);
+}
+
+// cnuity3 (cnuity_compute.F90:152-259)
+__global__ void __launch_bounds__(256) k_cnuity3(DevGrid g, DevState s, AB ab) {
+  const int nip = g.nip, nvl = g.nvl;
+  const int k = threadIdx.x;
+  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  if (ipn >= nip || k >= nvl) return;
+  const int of = ab.of - 1, vof = ab.vof - 1;
+  const double thshld = 1.e-11;
+  const int np = g.nprox[ipn];
+  double pp = 0., pm = 0.;
+  for (int edg = 0; edg < np; edg++) {
+    double a = E3(s.antifx, ipn, edg, k);
+    if (a <= 0.) pp = pp - a; else pm = pm + a;
+  }
+  const double lo = C2(s.delp_lo, ipn, k);
+  double dpmax = dmax(lo, C2(s.dp, ipn, k));
+  double dpmin = dmin(lo, C2(s.dp, ipn, k));
+  for (int e = 0; e < NPP; e++) {
+    int x = PROX(ipn, e);
+    double a = C2(s.delp_lo, x, k), b = C2(s.dp, x, k);
+    dpmax = dmax(dpmax, dmax(a, b));
+    dpmin = dmin(dpmin, dmin(a, b));
+  }
+  dpmax = dmax(0., dpmax);
+  dpmin = dmax(0., dpmin);
+  const double to = AB3(s.dp_tdcy, of, ipn, k), tv = AB3(s.dp_tdcy, vof, ipn, k);
+  double q_plus = (dpmax - lo - (ab.ab2 * dmin(0., to) + ab.ab3 * dmax(0., tv))) / ab.ab1 * g.area[ipn];
+  double q_mnus = (lo - dpmin + (ab.ab2 * dmax(0., to) + ab.ab3 * dmin(0., tv))) / ab.ab1 * g.area[ipn];
+  C2(s.r_plus, ipn, k) = dmax(0., dmin(1., q_plus / dmax(pp, thshld)));
+  C2(s.r_mnus, ipn, k) = dmax(0., dmin(1., q_mnus / dmax(pm, thshld)));
+}
+
+// cnuity4 + cnuity5 (cnuity_compute.F90:261-451), one block row per column:
+// the top-down pressure and column-tendency sums are serial in lane 0 (the
+// reference summation order), the Exner function is per lane.
+__global__ void __launch_bounds__(256) k_cnuity45(DevGrid g, DevState s, AB ab) {
+  __shared__ double sh_dp[COLS][NVL_MAX + 1];
+  __shared__ double sh_lt[COLS][NVL_MAX + 1];
+  __shared__ double sh_pr[COLS][NVL_MAX + 1];
+  __shared__ double sh_om[COLS][NVL_MAX + 1];
+  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
+  const int k = threadIdx.x, c = threadIdx.y;
+  const int ipn = blockIdx.x * COLS + c;
+  const bool live = ipn < nip && k < nvl;
+  const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
+  double dtn = 0.;
+  if (live) {
+    const int np = g.nprox[ipn];
+    double at = 0.;
+    for (int edg = 0; edg < np; edg++) {
+      const int x = PROX(ipn, edg);
+      double a = E3(s.antifx, ipn, edg, k), clip;
+      if (a >= 0.) clip = dmin(C2(s.r_mnus, ipn, k), C2(s.r_plus, x, k));
+      else clip = dmin(C2(s.r_plus, ipn, k), C2(s.r_mnus, x, k));
+      a = a * clip;
+      MF(s.massfx, nf, ipn, edg, k) = MF(s.massfx, nf, ipn, edg, k) + a;
+      at = at + a;
+    }
+    at = -at * g.rarea[ipn];
+    dtn = AB3(s.dpl_tdcy, nf, ipn, k) + at;
+    AB3(s.dp_tdcy, nf, ipn, k) = dtn;
+    const double to = AB3(s.dp_tdcy, of, ipn, k), tv = AB3(s.dp_tdcy, vof, ipn, k);
+    double dpn = C2(s.dp, ipn, k) + ab.ab1 * dtn + ab.ab2 * to + ab.ab3 * tv;
+    C2(s.dp, ipn, k) = dpn;
+    sh_dp[c][k] = dpn;
+    sh_lt[c][k] = (ab.ab1 * dtn + ab.ab2 * to + ab.ab3 * tv) / g.dt;
+  }
+  __syncthreads();
+  if (k == 0 && ipn < nip) {
+    double p = L2(s.pr, ipn, nvl);
+    double coltend = 0.;
+    for (int kk = nvl - 1; kk >= 0; kk--) {
+      p = p + sh_dp[c][kk];
+      sh_pr[c][kk] = p;
+      double lt = sh_lt[c][kk];
+      sh_om[c][kk] = coltend + 0.5 * lt;
+      coltend = coltend + lt;
+    }
+  }
+  __syncthreads();
+  if (live) {
+    const double p = sh_pr[c][k];
+    L2(s.pr, ipn, k) = p;
+    L2(s.ex, ipn, k) = CP * fim_pow(p / P1000, RD / CP);
+    const int np = g.nprox[ipn];
+    double dpdx = 0., dpdy = 0.;
+    for (int edg = 0; edg < np; edg++) {
+      dpdx = dpdx + E3(s.lp_edg, ipn, edg, k) * SVC(ipn, edg, 1);
+      dpdy = dpdy - E3(s.lp_edg, ipn, edg, k) * SVC(ipn, edg, 0);
+    }
+    C2(s.ws, ipn, k) = sh_om[c][k] + (C2(s.us, ipn, k) * dpdx + C2(s.vs, ipn, k) * dpdy) * g.rarea[ipn];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// trcadv1-3 (trcadv_compute.F90:19-315); blockIdx.y = tracer
+__global__ void __launch_bounds__(256) k_trcadv1(DevGrid g, DevState s, AB ab) {
+  const int nip = g.nip, nvl = g.nvl;
+  const int k = threadIdx.x;
+  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int t = blockIdx.y;
+  if (ipn >= nip || k >= nvl) return;
+  const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
+  const double thshld = 1.e-11;
+  const size_t esz = (size_t)nip * NPP * nvl, csz = (size_t)nip * nvl;
+  const double *tre = s.trc_edg + t * esz;
+  const double *tracr = s.tr + t * csz;
+  double *antiflx = s.antiflx + t * esz;
+  const int np = g.nprox[ipn];
+  const double ti = C2(tracr, ipn, k);
+  double acc = 0.;
+  for (int edg = 0; edg < np; edg++) {
+    const int ipx = PROX(ipn, edg), edx = PROXS(ipn, edg);
+    double m = MF(s.massfx, nf, ipn, edg, k), mx = MF(s.massfx, nf, ipx, edx, k);
+    double flx_hi = m * E3(tre, ipn, edg, k);
+    double fl = 0.5 * ((m + fabs(m)) * ti - (mx + fabs(mx)) * C2(tracr, ipx, k));
+    E3(antiflx, ipn, edg, k) = flx_hi - fl;
+    acc = acc + fl;
+  }
+  if (np < NPP) E3(antiflx, ipn, NPP - 1, k) = 0.;
+  double tlo = -acc * g.rarea[ipn];
+  TAB(s.trl_tdcy, t, nf, ipn, k) = tlo;
+  double trcdp_lo = C2(s.trdp + t * csz, ipn, k) + ab.ab1 * tlo + ab.ab2 * TAB(s.trl_tdcy, t, of, ipn, k) +
+                    ab.ab3 * TAB(s.trl_tdcy, t, vof, ipn, k);
+  double mx = ti, mn = ti;
+  for (int e = 0; e < NPP; e++) {
+    double v = C2(tracr, PROX(ipn, e), k);
+    mx = dmax(mx, v);
+    mn = dmin(mn, v);
+  }
+  mx = dmax(0., mx);
+  mn = dmax(0., mn);
+  C2(s.trmax + t * csz, ipn, k) = mx;
+  C2(s.trmin + t * csz, ipn, k) = mn;
+  C2(s.trcr_lo + t * csz, ipn, k) = dmax(mn, dmin(mx, trcdp_lo / dmax(thshld, C2(s.dp, ipn, k))));
+}
+
+__global__ void __launch_bounds__(256) k_trcadv2(DevGrid g, DevState s, AB ab) {
+  const int nip = g.nip, nvl = g.nvl;
+  const int k = threadIdx.x;
+  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int t = blockIdx.y;
+  if (ipn >= nip || k >= nvl) return;
+  const int of = ab.of - 1, vof = ab.vof - 1;
+  const double thshld = 1.e-11;
+  const size_t esz = (size_t)nip * NPP * nvl, csz = (size_t)nip * nvl;
+  const double *antiflx = s.antiflx + t * esz;
+  const double *lo_ = s.trcr_lo + t * csz;
+  const int np = g.nprox[ipn];
+  double sp = 0., sm = 0.;
+  for (int edg = 0; edg < np; edg++) {
+    double a = E3(antiflx, ipn, edg, k);
+    sp = sp - dmin(0., a);
+    sm = sm + dmax(0., a);
+  }
+  const double lo = C2(lo_, ipn, k);
+  double mx = dmax(C2(s.trmax + t * csz, ipn, k), lo);
+  double mn = dmin(C2(s.trmin + t * csz, ipn, k), lo);
+  for (int e = 0; e < NPP; e++) {
+    double v = C2(lo_, PROX(ipn, e), k);
+    mx = dmax(mx, v);
+    mn = dmin(mn, v);
+  }
+  mx = dmax(0., mx);
+  mn = dmax(0., mn);
+  C2(s.trmax + t * csz, ipn, k) = mx;
+  C2(s.trmin + t * csz, ipn, k) = mn;
+  const double dp = C2(s.dp, ipn, k);
+  const double to = TAB(s.trc_tdcy, t, of, ipn, k), tv = TAB(s.trc_tdcy, t, vof, ipn, k);
+  double q_plus = ((mx - lo) * dp - (ab.ab2 * dmin(0., to) + ab.ab3 * dmax(0., tv))) / ab.ab1 * g.area[ipn];
+  double q_mnus = ((lo - mn) * dp + (ab.ab2 * dmax(0., to) + ab.ab3 * dmin(0., tv))) / ab.ab1 * g.area[ipn];
+  C2(s.tr_rp + t * csz, ipn, k) = dmax(0., dmin(1., q_plus / dmax(sp, thshld)));
+  C2(s.tr_rm + t * csz, ipn, k) = dmax(0., dmin(1., q_mnus / dmax(sm, thshld)));
+}
+
+__global__ void __launch_bounds__(256) k_trcadv3(DevGrid g, DevState s, AB ab) {
+  const int nip = g.nip, nvl = g.nvl;
+  const int k = threadIdx.x;
+  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int t = blockIdx.y;
+  if (ipn >= nip || k >= nvl) return;
+  const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
+  const double thshld = 1.e-11;
+  const size_t esz = (size_t)nip * NPP * nvl, csz = (size_t)nip * nvl;
+  const double *antiflx = s.antiflx + t * esz;
+  const double *rp = s.tr_rp + t * csz, *rm = s.tr_rm + t * csz;
+  const int np = g.nprox[ipn];
+  double at = 0.;
+  for (int edg = 0; edg < np; edg++) {
+    const int x = PROX(ipn, edg);
+    double a = E3(antiflx, ipn, edg, k);
+    if (a >= 0.) a = a * dmin(C2(rm, ipn, k), C2(rp, x, k));
+    else a = a * dmin(C2(rp, ipn, k), C2(rm, x, k));
+    at = at + a;
+  }
+  at = -at * g.rarea[ipn];
+  double tt = TAB(s.trl_tdcy, t, nf, ipn, k) + at;
+  TAB(s.trc_tdcy, t, nf, ipn, k) = tt;
+  double *trcdp = s.trdp + t * csz;
+  double tdp = C2(trcdp, ipn, k) + ab.ab1 * tt + ab.ab2 * TAB(s.trc_tdcy, t, of, ipn, k) +
+               ab.ab3 * TAB(s.trc_tdcy, t, vof, ipn, k);
+  C2(trcdp, ipn, k) = tdp;
+  C2(s.tr + t * csz, ipn, k) = dmax(C2(s.trmin + t * csz, ipn, k),
+                                    dmin(C2(s.trmax + t * csz, ipn, k), tdp / dmax(thshld, C2(s.dp, ipn, k))));
+}
+
+// ---------------------------------------------------------------------------
+// momtum (momtum.F90:101-158; dissip momtum_compute.F90:102-229; momtum1
+// 231-450).  Reads the winds from us/vs and writes the new winds to u2/v2,
+// because dissip reads the neighbours' winds before any of them is updated.
+__global__ void __launch_bounds__(256) k_momtum(DevGrid g, DevState s, AB ab) {
+  __shared__ double sh_q[COLS][3];
+  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
+  const int k = threadIdx.x, c = threadIdx.y;
+  const int ipn = blockIdx.x * COLS + c;
+  const bool live = ipn < nip && k < nvl;
+  const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
+  const double thshld = 1.e-11;
+  double un = 0., vn = 0.;
+  if (live) {
+    const int np = g.nprox[ipn];
+    const double wui = C2(s.us, ipn, k) + g.grndspd[ipn], wvi = C2(s.vs, ipn, k);
+    const double dpi = C2(s.dp, ipn, k);
+    const double factor = g.rarea[ipn] / dmax(dpi, thshld);
+    double wu = 0., wv = 0.;
+    for (int edg = 0; edg < np; edg++) {
+      const int ipx = PROX(ipn, edg);
+      const double wux = C2(s.us, ipx, k) + g.grndspd[ipx], wvx = C2(s.vs, ipx, k);
+      double uxy1 = CS(ipn, edg, 0) * wui + SN(ipn, edg, 0) * wvi;
+      double vxy1 = -SN(ipn, edg, 0) * wui + CS(ipn, edg, 0) * wvi;
+      double uxy2 = CS(ipn, edg, 1) * wux + SN(ipn, edg, 1) * wvx;
+      double vxy2 = -SN(ipn, edg, 1) * wux + CS(ipn, edg, 1) * wvx;
+      double a = dmax(dpi, thshld), b = dmax(C2(s.dp, ipx, k), thshld);
+      double hh = a * b / (a + b);
+      double ufx = (uxy2 - uxy1) * g.dfflen[k] * g.sideln[ipn * NPP + edg] * 2. * hh;
+      double vfx = (vxy2 - vxy1) * g.dfflen[k] * g.sideln[ipn * NPP + edg] * 2. * hh;
+      double uflux = CS(ipn, edg, 0) * ufx - SN(ipn, edg, 0) * vfx;
+      double vflux = SN(ipn, edg, 0) * ufx + CS(ipn, edg, 0) * vfx;
+      wu = wu + uflux * factor;
+      wv = wv + vflux * factor;
+    }
+    const double u1 = C2(s.us, ipn, k) + wu, v1 = C2(s.vs, ipn, k) + wv;
+    double px = 0., py = 0., rv = 0.;
+    const double exm = .5 * (L2(s.ex, ipn, k) + L2(s.ex, ipn, k + 1));
+    for (int edg = 0; edg < np; edg++) {
+      double b = E3(s.mont_edg, ipn, edg, k) + E3(s.uvsq_edg, ipn, edg, k);
+      double te = E3(s.trc_edg, ipn, edg, k);
+      px = px - b * SVC(ipn, edg, 1) + exm * te * SVC(ipn, edg, 1);
+      py = py + b * SVC(ipn, edg, 0) - exm * te * SVC(ipn, edg, 0);
+      rv = rv + (SVE(ipn, edg, 0) * E3(s.u_edg, ipn, edg, k) + SVE(ipn, edg, 1) * E3(s.v_edg, ipn, edg, k));
+    }
+    px = px * g.rarea[ipn];
+    py = py * g.rarea[ipn];
+    rv = rv * g.rarea[ipn];
+    C2(s.relvor, ipn, k) = rv;
+    double uzeta = (g.corio[ipn] + rv) * u1;
+    double vzeta = (g.corio[ipn] + rv) * v1;
+    double ut = px + vzeta, vt = py - uzeta;
+    AB3(s.u_tdcy, nf, ipn, k) = ut;
+    AB3(s.v_tdcy, nf, ipn, k) = vt;
+    un = u1 + ab.ab1 * ut + ab.ab2 * AB3(s.u_tdcy, of, ipn, k) + ab.ab3 * AB3(s.u_tdcy, vof, ipn, k);
+    vn = v1 + ab.ab1 * vt + ab.ab2 * AB3(s.v_tdcy, of, ipn, k) + ab.ab3 * AB3(s.v_tdcy, vof, ipn, k);
+    if (k >= nvl - 3) sh_q[c][nvl - 1 - k] = un * un + vn * vn;
+  }
+  __syncthreads();
+  if (live) {
+    // Rayleigh damping of the top layers (momtum_compute.F90:406-416)
+    double qq = dmax(dmax(sh_q[c][0], sh_q[c][1]), sh_q[c][2]) / (RLTHRESH * RLTHRESH);
+    const int ndamp = (int)(0.25 * nvl);
+    const int k1 = k + 1;
+    if (qq > 1. && k1 >= nvl - ndamp) {
+      qq = sqrt((qq - 1.) * qq);
+      double factor = 1. - (g.dt / 86400.) * RLEIGH * qq * fim_pow(2., RLTAPER * (double)(k1 - nvl));
+      un = un * factor;
+      vn = vn * factor;
+    }
+    C2(s.u2, ipn, k) = un;
+    C2(s.v2, ipn, k) = vn;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// hystat.F90:23-115, one thread per column
+__global__ void __launch_bounds__(64) k_hystat(DevGrid g, DevState s, int its) {
+  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1, ntra = g.ntra;
+  const int ipn = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ipn >= nip) return;
+  double work = s.psrf[ipn];
+  double ps = C2(s.dp, ipn, nvl - 1);
+  for (int k = nvl - 1; k >= 1; k--) ps = ps + C2(s.dp, ipn, k - 1);
+  s.psrf[ipn] = ps;
+  if (its > 0) s.ptdcy[(size_t)(its % 2) * nip + ipn] = ps - work;
+  double mp = L2(s.ex, ipn, 0) * TR(s.tr, 0, ipn, 0) + L2(s.ph, ipn, 0);
+  C2(s.mp, ipn, 0) = mp;
+  L2(s.ph, ipn, 1) = mp - TR(s.tr, 0, ipn, 0) * L2(s.ex, ipn, 1);
+  for (int k = 2; k <= nvl; k++) {
+    mp = mp + L2(s.ex, ipn, k - 1) * (TR(s.tr, 0, ipn, k - 1) - TR(s.tr, 0, ipn, k - 2));
+    C2(s.mp, ipn, k - 1) = mp;
+    L2(s.ph, ipn, k) = mp - TR(s.tr, 0, ipn, k - 1) * L2(s.ex, ipn, k);
+  }
+}
+
+// tracer floors and tracer*thickness (hystat.F90:82-95), per cell-level
+__global__ void __launch_bounds__(256) k_hystat_tr(DevGrid g, DevState s) {
+  const int nip = g.nip, nvl = g.nvl, ntra = g.ntra;
+  const int k = threadIdx.x;
+  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  if (ipn >= nip || k >= nvl) return;
+  if (ntra > 1) TR(s.tr, 1, ipn, k) = dmax(QVMIN, TR(s.tr, 1, ipn, k));
+  if (ntra > 2) TR(s.tr, 2, ipn, k) = dmax(QWMIN, TR(s.tr, 2, ipn, k));
+  const double dp = C2(s.dp, ipn, k);
+  for (int t = 0; t < ntra; t++) TR(s.trdp, t, ipn, k) = TR(s.tr, t, ipn, k) * dp;
+}
+
+// pressure from the Exner function after the start-up hybgen (dyn_init.F90:884-893)
+__global__ void __launch_bounds__(256) k_pres_from_exner(DevGrid g, DevState s) {
+  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
+  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  if (ipn >= nip) return;
+  for (int k = threadIdx.x; k < nvlp1; k += blockDim.x)
+    L2(s.pr, ipn, k) = P1000 * fim_pow(L2(s.ex, ipn, k) / CP, CP / RD);
+}
+
+__global__ void __launch_bounds__(256) k_trdp(DevGrid g, DevState s) {
+  const int nip = g.nip, nvl = g.nvl, ntra = g.ntra;
+  const int k = threadIdx.x;
+  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  if (ipn >= nip || k >= nvl) return;
+  for (int t = 0; t < ntra; t++) TR(s.trdp, t, ipn, k) = TR(s.tr, t, ipn, k) * C2(s.dp, ipn, k);
+}
+
+// ---------------------------------------------------------------------------
+void launch_edgvar1(const DevGrid &g, const DevState &s, hipStream_t st) {
+  k_edgvar1<<<col_grid(g.nip), col_block(), 0, st>>>(g, s);
+}
+void launch_edgvar3(const DevGrid &g, const DevState &s, hipStream_t st) {
+  dim3 grid((g.nip + COLS - 1) / COLS, g.ntra);
+  k_edgvar3<<<grid, col_block(), 0, st>>>(g, s);
+}
+void launch_cnuity(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st) {
+  k_cnuity12<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab);
+  k_cnuity3<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab);
+  k_cnuity45<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab);
+}
+void launch_trcadv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st) {
+  dim3 grid((g.nip + COLS - 1) / COLS, g.ntra);
+  k_trcadv1<<<grid, col_block(), 0, st>>>(g, s, ab);
+  k_trcadv2<<<grid, col_block(), 0, st>>>(g, s, ab);
+  k_trcadv3<<<grid, col_block(), 0, st>>>(g, s, ab);
+}
+void launch_momtum(const DevGrid &g, DevState &s, const AB &ab, hipStream_t st) {
+  k_momtum<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab);
+  double *t = s.us; s.us = s.u2; s.u2 = t;
+  t = s.vs; s.vs = s.v2; s.v2 = t;
+}
+void launch_hystat(const DevGrid &g, const DevState &s, int its, hipStream_t st) {
+  k_hystat<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its);
+  k_hystat_tr<<<col_grid(g.nip), col_block(), 0, st>>>(g, s);
+}
+void launch_pres_from_exner(const DevGrid &g, const DevState &s, hipStream_t st) {
+  k_pres_from_exner<<<col_grid(g.nip), col_block(), 0, st>>>(g, s);
+}
+void launch_trdp(const DevGrid &g, const DevState &s, hipStream_t st) {
+  k_trdp<<<col_grid(g.nip), col_block(), 0, st>>>(g, s);
+}
+
+}  // namespace fimhip

@@ -1,0 +1,232 @@
+"""Host-side mirror of the reference dynamics interface over libfimhip.so.
+
+The functions below keep the names, argument order and argument meaning of
+the reference subroutines (FIMsrc/fim/horizontal: edgvar1.F90:27,
+edgvar3.F90:25, cnuity.F90:23, trcadv.F90:19, momtum.F90:16, hybgen.F90:20,
+hystat.F90:23, abstart.F90:14) and forward to the C ABI declared in
+include/fimhip.h.  Arrays are numpy float64 arrays in the reference layout
+with the level index fastest (Fortran (nvl,nip) == C-order (nip,nvl)); inout
+arrays are updated in place, as Fortran does.  A non-zero status from the
+library raises FimhipError.
+
+``Instance`` owns one device instance; ``Model`` adds the device-resident
+fast path (upload, dyn_init start-up, asynchronous steps, download).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+from . import glvl as _glvl
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "libfimhip.so")
+
+FIELDS = ["us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp", "ws", "relvor", "sdot", "psrf", "ptdcy",
+          "u_tdcy", "v_tdcy", "dp_tdcy", "dpl_tdcy", "trc_tdcy", "trl_tdcy", "massfx", "u_edg", "v_edg",
+          "dp_edg", "lp_edg", "mont_edg", "uvsq_edg", "trc_edg"]
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+
+# symbols declared in include/fimhip.h
+EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab", "fimhip_edgvar1_r8",
+           "fimhip_edgvar3_r8", "fimhip_cnuity_r8", "fimhip_trcadv_r8", "fimhip_momtum_r8",
+           "fimhip_hybgen_r8", "fimhip_hystat_r8", "fimhip_field_size", "fimhip_upload", "fimhip_download",
+           "fimhip_init_state", "fimhip_step", "fimhip_sync", "fimhip_globsum", "fimhip_pow",
+           "fimhip_version"]
+
+
+class FimhipError(RuntimeError):
+    pass
+
+
+_PD = ct.POINTER(ct.c_double)
+_PI = ct.POINTER(ct.c_int)
+
+
+class GridDesc(ct.Structure):
+    _fields_ = [("nip", ct.c_int), ("nvl", ct.c_int), ("ntra", ct.c_int), ("dt", ct.c_double),
+                ("nprox", _PI), ("prox", _PI), ("proxs", _PI), ("cs", _PD), ("sn", _PD),
+                ("sidevec_c", _PD), ("sidevec_e", _PD), ("sideln", _PD), ("hfcoef", _PD),
+                ("area", _PD), ("lat", _PD), ("thetac", _PD), ("dpsig", _PD)]
+
+
+_LIB = None
+
+
+def load_library() -> ct.CDLL:
+    """Load libfimhip.so; raises if it has not been built (no CPU fallback)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise FimhipError("libfimhip.so not built: run __graft_entry__.build() (hipcc, gfx950)")
+    lib = ct.CDLL(LIBPATH)
+    vp = ct.c_void_p
+    lib.fimhip_create.argtypes = [ct.POINTER(GridDesc), ct.c_int, ct.POINTER(vp)]
+    lib.fimhip_destroy.argtypes = [vp]
+    lib.fimhip_abstart.argtypes = [vp, ct.c_int, ct.c_int, _PI, _PI, _PI, _PD, _PD, _PD]
+    lib.fimhip_set_ab.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_double]
+    lib.fimhip_edgvar1_r8.argtypes = [vp, ct.c_int] + [_PD] * 16
+    lib.fimhip_edgvar3_r8.argtypes = [vp, ct.c_int] + [_PD] * 16
+    lib.fimhip_cnuity_r8.argtypes = [vp, ct.c_int] + [_PD] * 13 + [ct.c_int]
+    lib.fimhip_trcadv_r8.argtypes = [vp, ct.c_int] + [_PD] * 7 + [ct.c_int]
+    lib.fimhip_momtum_r8.argtypes = [vp, ct.c_int] + [_PD] * 13
+    lib.fimhip_hybgen_r8.argtypes = [vp, ct.c_int] + [_PD] * 8 + [ct.c_int]
+    lib.fimhip_hystat_r8.argtypes = [vp, ct.c_int] + [_PD] * 8
+    lib.fimhip_field_size.argtypes = [vp, ct.c_int]
+    lib.fimhip_field_size.restype = ct.c_longlong
+    lib.fimhip_upload.argtypes = [vp, ct.c_int, _PD]
+    lib.fimhip_download.argtypes = [vp, ct.c_int, _PD]
+    lib.fimhip_init_state.argtypes = [vp, ct.c_int]
+    lib.fimhip_step.argtypes = [vp, ct.c_int, ct.c_int]
+    lib.fimhip_sync.argtypes = [vp]
+    lib.fimhip_globsum.argtypes = [vp, _PD, _PD]
+    lib.fimhip_pow.argtypes = [ct.c_double, ct.c_double]
+    lib.fimhip_pow.restype = ct.c_double
+    lib.fimhip_version.restype = ct.c_char_p
+    _LIB = lib
+    return lib
+
+
+def _chk(rc: int, what: str):
+    if rc != 0:
+        raise FimhipError("%s failed with status %d" % (what, rc))
+
+
+def _p(a: np.ndarray):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(_PD)
+
+
+class Instance:
+    """One device instance of the dynamics on one GPU (fimhip_create)."""
+
+    def __init__(self, grid: _glvl.Grid, nvl: int, ntra: int, dt: float, thetac, dpsig, device: int = 0):
+        self.lib = load_library()
+        self.grid, self.nvl, self.ntra, self.dt = grid, nvl, ntra, dt
+        nip = grid.nip
+        self.nip = nip
+        keep = dict(
+            nprox=np.ascontiguousarray(grid.nprox, np.int32),
+            prox=np.ascontiguousarray(grid.prox, np.int32),
+            proxs=np.ascontiguousarray(grid.proxs, np.int32),
+            cs=np.ascontiguousarray(grid.cs, np.float64), sn=np.ascontiguousarray(grid.sn, np.float64),
+            sidevec_c=np.ascontiguousarray(grid.sidevec_c, np.float64),
+            sidevec_e=np.ascontiguousarray(grid.sidevec_e, np.float64),
+            sideln=np.ascontiguousarray(grid.sideln, np.float64),
+            hfcoef=np.ascontiguousarray(grid.hfcoef, np.float64),
+            area=np.ascontiguousarray(grid.area, np.float64),
+            lat=np.ascontiguousarray(grid.lat, np.float64),
+            thetac=np.ascontiguousarray(thetac, np.float64), dpsig=np.ascontiguousarray(dpsig, np.float64))
+        self._keep = keep
+        gd = GridDesc()
+        gd.nip, gd.nvl, gd.ntra, gd.dt = nip, nvl, ntra, dt
+        for n in ("nprox", "prox", "proxs"):
+            setattr(gd, n, keep[n].ctypes.data_as(_PI))
+        for n in ("cs", "sn", "sidevec_c", "sidevec_e", "sideln", "hfcoef", "area", "lat", "thetac", "dpsig"):
+            setattr(gd, n, keep[n].ctypes.data_as(_PD))
+        h = ct.c_void_p()
+        _chk(self.lib.fimhip_create(ct.byref(gd), device, ct.byref(h)), "fimhip_create")
+        self.h = h
+        self.nf, self.of, self.vof = 0, 2, 1
+        self.adbash = [0.0, 0.0, 0.0]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fimhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- reference subroutines --------------------------------------------
+    def abstart(self, its: int, itsDFI: int = 1):
+        nf, of, vof = ct.c_int(self.nf), ct.c_int(self.of), ct.c_int(self.vof)
+        a = [ct.c_double(x) for x in self.adbash]
+        _chk(self.lib.fimhip_abstart(self.h, its, itsDFI, ct.byref(nf), ct.byref(of), ct.byref(vof),
+                                     ct.byref(a[0]), ct.byref(a[1]), ct.byref(a[2])), "abstart")
+        self.nf, self.of, self.vof = nf.value, of.value, vof.value
+        self.adbash = [x.value for x in a]
+
+    def edgvar1(self, its, u_vel, v_vel, delp, pres, exner, geop, montg, tracr, u_edg, v_edg, dp_edg,
+                lp_edg, geop_edg, mont_edg, uvsq_edg, trc_edg):
+        _chk(self.lib.fimhip_edgvar1_r8(self.h, its, *map(_p, (u_vel, v_vel, delp, pres, exner, geop, montg,
+             tracr, u_edg, v_edg, dp_edg, lp_edg, geop_edg, mont_edg, uvsq_edg, trc_edg))), "edgvar1")
+
+    def edgvar3(self, its, u_vel, v_vel, delp, pres, exner, geop, montg, tracr, u_edg, v_edg, dp_edg,
+                lp_edg, geop_edg, mont_edg, uvsq_edg, trc_edg):
+        _chk(self.lib.fimhip_edgvar3_r8(self.h, its, *map(_p, (u_vel, v_vel, delp, pres, exner, geop, montg,
+             tracr, u_edg, v_edg, dp_edg, lp_edg, geop_edg, mont_edg, uvsq_edg, trc_edg))), "edgvar3")
+
+    def cnuity(self, its, u_velo, v_velo, u_edg, v_edg, dp_edg, lp_edg, delp, pres, exner, dp_tndcy,
+               dplo_tndcy, massfx, omega, TimingBarriers=False):
+        _chk(self.lib.fimhip_cnuity_r8(self.h, its, *map(_p, (u_velo, v_velo, u_edg, v_edg, dp_edg, lp_edg,
+             delp, pres, exner, dp_tndcy, dplo_tndcy, massfx, omega)), int(TimingBarriers)), "cnuity")
+
+    def trcadv(self, its, trcr_edg, tracr, trcdp, trc_tdcy, trclo_tdcy, massfx, delp, TimingBarriers=False):
+        _chk(self.lib.fimhip_trcadv_r8(self.h, its, *map(_p, (trcr_edg, tracr, trcdp, trc_tdcy, trclo_tdcy,
+             massfx, delp)), int(TimingBarriers)), "trcadv")
+
+    def momtum(self, its, u_velo, v_velo, exner, relvor, u_edg, v_edg, trcr_edg, geop_edg, mont_edg,
+               uvsq_edg, u_tndcy, v_tndcy, dp3d):
+        _chk(self.lib.fimhip_momtum_r8(self.h, its, *map(_p, (u_velo, v_velo, exner, relvor, u_edg, v_edg,
+             trcr_edg, geop_edg, mont_edg, uvsq_edg, u_tndcy, v_tndcy, dp3d))), "momtum")
+
+    def hybgen(self, its, targt, us3d, vs3d, tr3d, sdot, ex3d, dp3d, pr3d, TimingBarriers=False):
+        _chk(self.lib.fimhip_hybgen_r8(self.h, its, *map(_p, (np.ascontiguousarray(targt, np.float64), us3d,
+             vs3d, tr3d, sdot, ex3d, dp3d, pr3d)), int(TimingBarriers)), "hybgen")
+
+    def hystat(self, its, ph3d, ex3d, mp3d, dp3d, tr3d, trdp, psrf, ptdcy):
+        _chk(self.lib.fimhip_hystat_r8(self.h, its, *map(_p, (ph3d, ex3d, mp3d, dp3d, tr3d, trdp, psrf,
+             ptdcy))), "hystat")
+
+
+class Model(Instance):
+    """Device-resident state advanced by fimhip_step."""
+
+    def shape(self, name: str):
+        nip, nvl, ntra = self.nip, self.nvl, self.ntra
+        c, l, e = (nip, nvl), (nip, nvl + 1), (nip, 6, nvl)
+        return dict(us=c, vs=c, dp=c, pr=l, ex=l, ph=l, mp=c, tr=(ntra,) + c, trdp=(ntra,) + c, ws=c,
+                    relvor=c, sdot=l, psrf=(nip,), ptdcy=(2, nip), u_tdcy=(3,) + c, v_tdcy=(3,) + c,
+                    dp_tdcy=(3,) + c, dpl_tdcy=(3,) + c, trc_tdcy=(ntra, 3) + c, trl_tdcy=(ntra, 3) + c,
+                    massfx=(3,) + e, u_edg=e, v_edg=e, dp_edg=e, lp_edg=e, mont_edg=e, uvsq_edg=e,
+                    trc_edg=(ntra,) + e)[name]
+
+    def upload(self, name: str, a):
+        a = np.ascontiguousarray(a, np.float64)
+        if a.shape != self.shape(name):
+            raise ValueError("%s: shape %s, expected %s" % (name, a.shape, self.shape(name)))
+        _chk(self.lib.fimhip_upload(self.h, FIELD_ID[name], _p(a)), "upload " + name)
+
+    def download(self, name: str) -> np.ndarray:
+        a = np.empty(self.shape(name))
+        _chk(self.lib.fimhip_download(self.h, FIELD_ID[name], _p(a)), "download " + name)
+        return a
+
+    def load(self, st: dict):
+        for n in ("us", "vs", "dp", "pr", "ex", "ph", "tr"):
+            self.upload(n, st[n])
+
+    def init_state(self, init_hybgen: bool = True):
+        _chk(self.lib.fimhip_init_state(self.h, 1 if init_hybgen else 0), "init_state")
+
+    def step(self, its: int, mode: int = 0):
+        _chk(self.lib.fimhip_step(self.h, its, mode), "step")
+
+    def sync(self):
+        _chk(self.lib.fimhip_sync(self.h), "sync")
+
+    def globsum(self):
+        m = ct.c_double()
+        t = np.zeros(self.ntra)
+        _chk(self.lib.fimhip_globsum(self.h, ct.byref(m), _p(t)), "globsum")
+        return m.value, t

@@ -1,0 +1,131 @@
+/* fimhip.h - C ABI of the MI355X (gfx950) FIM horizontal-dynamics library.
+ *
+ * A single-task serial FIM build calls these entry points in place of the
+ * reference subroutines of FIMsrc/fim/horizontal (through the bind(C)
+ * interfaces shown in INTEGRATION.md).  All arrays are host arrays in the
+ * reference's Fortran layout, level index fastest:
+ *   cell (nvl,nip)   level (nvl+1,nip)   edge (nvl,6,nip)   tracers (nvl,nip,ntra)
+ *   tendencies (nvl,nip,3)   tracer tendencies (nvl,nip,3,ntra)   massfx (nvl,6,nip,3)
+ * The _r8 entries take REAL*8 arrays (the -fdefault-real-8 build).  Per-routine
+ * entries are synchronous: outputs are valid on return.  Every function
+ * returns 0 on success and a negative FIMHIP_E* code on error; the library
+ * never aborts the caller.  SMS halo exchanges of the reference are no-ops
+ * here (one GPU holds the whole grid).
+ */
+#ifndef FIMHIP_H
+#define FIMHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FIMHIP_OK 0
+#define FIMHIP_EINVAL (-1)      /* bad argument or size */
+#define FIMHIP_EHIP (-2)        /* a HIP runtime call failed */
+#define FIMHIP_ENOMEM (-3)      /* device allocation failed */
+#define FIMHIP_ENODEV (-88)     /* no usable GPU (GPUinit.cu:27-34 uses -88) */
+
+typedef struct fimhip_ctx fimhip_ctx;
+
+/* Grid and vertical tables, as module_constants holds them after dyn_init
+ * (dyn_init.F90:373-548, 789-818).  Neighbour indices are 1-based as in
+ * glvl.dat; slot 6 of a pentagon is ignored. */
+typedef struct {
+  int nip, nvl, ntra;           /* nvl <= 64, ntra <= 8 */
+  double dt;                    /* model time step (s) */
+  const int *nprox;             /* (nip) */
+  const int *prox;              /* (6,nip) */
+  const int *proxs;             /* (6,nip) */
+  const double *cs, *sn;        /* (4,6,nip) */
+  const double *sidevec_c;      /* (2,6,nip) */
+  const double *sidevec_e;      /* (2,6,nip) */
+  const double *sideln;         /* (6,nip) */
+  const double *hfcoef;         /* (6,6,nip) */
+  const double *area;           /* (nip) m^2 */
+  const double *lat;            /* (nip) radians */
+  const double *thetac;         /* (nvl) hybgen target theta (theta_coor.txt) */
+  const double *dpsig;          /* (nvl) minimum layer thickness, Pa (dpsig.txt) */
+} fimhip_grid_desc;
+
+/* Create a device instance on GPU 'device' (replaces GPUinit + copytoGPUinit,
+ * dyn_init.F90:199, 1198).  Derives corio, rarea, grndspd (dyn_init.F90:753-757),
+ * dfflen and smoo_coeff (dyn_init.F90:309-321). */
+int fimhip_create(const fimhip_grid_desc *grid, int device, fimhip_ctx **out);
+int fimhip_destroy(fimhip_ctx *ctx);
+
+/* abstart.F90:14 - rotate the Adams-Bashforth slots and set the weights;
+ * the instance keeps its own copy, and the caller's module copy is updated. */
+int fimhip_abstart(fimhip_ctx *ctx, int its, int itsDFI, int *nf, int *of, int *vof,
+                   double *adbash1, double *adbash2, double *adbash3);
+/* set the AB slots and weights explicitly (restart, or a caller that keeps them) */
+int fimhip_set_ab(fimhip_ctx *ctx, int nf, int of, int vof, double adbash1, double adbash2, double adbash3);
+
+/* edgvar1.F90:27 (eqwgt, janjic=0, hiOrderFluxComp) - geop_edg and dp_edg
+ * are not written (the reference leaves them untouched on this path). */
+int fimhip_edgvar1_r8(fimhip_ctx *ctx, int its, const double *u_vel, const double *v_vel,
+                      const double *delp, const double *pres, const double *exner, const double *geop,
+                      const double *montg, const double *tracr, double *u_edg, double *v_edg,
+                      double *dp_edg, double *lp_edg, double *geop_edg, double *mont_edg,
+                      double *uvsq_edg, double *trc_edg);
+/* edgvar3.F90:25 - dp_edg and trc_edg(:,:,:,2:ntra) */
+int fimhip_edgvar3_r8(fimhip_ctx *ctx, int its, const double *u_vel, const double *v_vel,
+                      const double *delp, const double *pres, const double *exner, const double *geop,
+                      const double *montg, const double *tracr, double *u_edg, double *v_edg,
+                      double *dp_edg, double *lp_edg, double *geop_edg, double *mont_edg,
+                      double *uvsq_edg, double *trc_edg);
+/* cnuity.F90:23 */
+int fimhip_cnuity_r8(fimhip_ctx *ctx, int its, const double *u_velo, const double *v_velo,
+                     const double *u_edg, const double *v_edg, const double *dp_edg,
+                     const double *lp_edg, double *delp, double *pres, double *exner,
+                     double *dp_tndcy, double *dplo_tndcy, double *massfx, double *omega,
+                     int TimingBarriers);
+/* trcadv.F90:19 */
+int fimhip_trcadv_r8(fimhip_ctx *ctx, int its, const double *trcr_edg, double *tracr, double *trcdp,
+                     double *trc_tdcy, double *trclo_tdcy, const double *massfx, const double *delp,
+                     int TimingBarriers);
+/* momtum.F90:16 */
+int fimhip_momtum_r8(fimhip_ctx *ctx, int its, double *u_velo, double *v_velo, const double *exner,
+                     double *relvor, const double *u_edg, const double *v_edg, const double *trcr_edg,
+                     const double *geop_edg, const double *mont_edg, const double *uvsq_edg,
+                     double *u_tndcy, double *v_tndcy, const double *dp3d);
+/* hybgen.F90:20 */
+int fimhip_hybgen_r8(fimhip_ctx *ctx, int its, const double *targt, double *us3d, double *vs3d,
+                     double *tr3d, double *sdot, double *ex3d, double *dp3d, double *pr3d,
+                     int TimingBarriers);
+/* hystat.F90:23 */
+int fimhip_hystat_r8(fimhip_ctx *ctx, int its, double *ph3d, const double *ex3d, double *mp3d,
+                     const double *dp3d, double *tr3d, double *trdp, double *psrf, double *ptdcy);
+
+/* ---- device-resident fast path ------------------------------------------ */
+enum {
+  FIMHIP_US = 0, FIMHIP_VS, FIMHIP_DP, FIMHIP_PR, FIMHIP_EX, FIMHIP_PH, FIMHIP_MP, FIMHIP_TR,
+  FIMHIP_TRDP, FIMHIP_WS, FIMHIP_RELVOR, FIMHIP_SDOT, FIMHIP_PSRF, FIMHIP_PTDCY,
+  FIMHIP_U_TDCY, FIMHIP_V_TDCY, FIMHIP_DP_TDCY, FIMHIP_DPL_TDCY, FIMHIP_TRC_TDCY, FIMHIP_TRL_TDCY,
+  FIMHIP_MASSFX, FIMHIP_U_EDG, FIMHIP_V_EDG, FIMHIP_DP_EDG, FIMHIP_LP_EDG, FIMHIP_MONT_EDG,
+  FIMHIP_UVSQ_EDG, FIMHIP_TRC_EDG, FIMHIP_NFIELDS
+};
+/* number of doubles of a field */
+long long fimhip_field_size(const fimhip_ctx *ctx, int field);
+int fimhip_upload(fimhip_ctx *ctx, int field, const double *host);
+int fimhip_download(fimhip_ctx *ctx, int field, double *host);
+/* dyn_init start-up: optional hybgen(its=0) + pressure refresh, hystat,
+ * zero tendencies and mass fluxes, AB slots nf=0, of=2, vof=1
+ * (dyn_init.F90:870-1094) */
+int fimhip_init_state(fimhip_ctx *ctx, int init_hybgen);
+/* one dynamics step on the device, asynchronous: mode 0 = DYN_1(its) then
+ * DYN_2(its) (abstart, edgvar1, edgvar3, cnuity, trcadv, momtum, hybgen,
+ * hystat: dyn_run.F90:119-425); mode 1 = transport only (abstart, edgvar1,
+ * edgvar3, cnuity, trcadv) */
+int fimhip_step(fimhip_ctx *ctx, int its, int mode);
+int fimhip_sync(fimhip_ctx *ctx);
+/* globsum.F90:70-101: column mass (sum area*dp/g) and tracer totals, FP64 */
+int fimhip_globsum(fimhip_ctx *ctx, double *mass, double *trtot);
+
+/* x**y as computed on the device (the host build of the same routine) */
+double fimhip_pow(double x, double y);
+const char *fimhip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

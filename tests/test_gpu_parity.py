@@ -1,0 +1,119 @@
+"""Parity of the gfx950 FP64 path with the checker, through the C ABI.
+
+The checker is the C restatement with the project pow (pow_kind=1), which the
+reference-vs-checker tests pin to the reference Fortran bit for bit (with the
+C library pow) and to 1e-13 (with the project pow).  Here the device results
+must have exactly the checker's bytes."""
+import numpy as np
+import pytest
+
+from helpers import golden_grid, seeded_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+NVL, NTRA, DT = 64, 4, 240.0
+
+
+@pytest.fixture(scope="module")
+def case(oracle_lib):
+    O = oracle_lib
+    g, st, th, ds = seeded_case(3)
+    ga = O.grid_arrays(g, NVL, DT, th, ds)
+    return O, g, st, th, ds, ga
+
+
+def _oracle(O, ga, st):
+    o = O.OracleRun(ga, NVL, NTRA, DT, pow_kind=1)
+    o.load(st)
+    return o
+
+
+def _same(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def test_dropin_routines_match_checker_bitwise(case):
+    """Each per-routine entry, called with host arrays in the reference
+    layout as the Fortran caller would, returns the checker's bytes."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    O, g, st, th, ds, ga = case
+    print("case: G3 nvl=64 ntra=4, per-routine drop-ins", flush=True)
+    o = _oracle(O, ga, st)
+    o.init(True)
+    inst = fimhip.Instance(g, NVL, NTRA, DT, th, ds)
+    try:
+        a = {k: v.copy() for k, v in o.a.items()}
+        for its in (1, 2, 3, 4):
+            inst.abstart(its)
+            o.abstart(its)
+            assert (inst.nf, inst.of, inst.vof) == o.slots
+            geop_edg = np.zeros_like(a["u_edg"])
+            args = [a["us"], a["vs"], a["dp"], a["pr"], a["ex"], a["ph"], a["mp"], a["tr"], a["u_edg"],
+                    a["v_edg"], a["dp_edg"], a["lp_edg"], geop_edg, a["mont_edg"], a["uvsq_edg"], a["trc_edg"]]
+            inst.edgvar1(its, *args)
+            o.call("edgvar1")
+            for k in ("u_edg", "v_edg", "lp_edg", "mont_edg", "uvsq_edg"):
+                assert _same(a[k], o.a[k]), ("edgvar1", its, k)
+            assert _same(a["trc_edg"][0], o.a["trc_edg"][0])
+            inst.edgvar3(its, *args)
+            o.call("edgvar3")
+            assert _same(a["dp_edg"], o.a["dp_edg"]), ("edgvar3", its)
+            assert _same(a["trc_edg"], o.a["trc_edg"]), ("edgvar3", its)
+            inst.cnuity(its, a["us"], a["vs"], a["u_edg"], a["v_edg"], a["dp_edg"], a["lp_edg"], a["dp"], a["pr"],
+                        a["ex"], a["dp_tdcy"], a["dpl_tdcy"], a["massfx"], a["ws"])
+            o.call("cnuity")
+            for k in ("dp", "pr", "ex", "dp_tdcy", "dpl_tdcy", "massfx", "ws"):
+                assert _same(a[k], o.a[k]), ("cnuity", its, k)
+            inst.trcadv(its, a["trc_edg"], a["tr"], a["trdp"], a["trc_tdcy"], a["trl_tdcy"], a["massfx"], a["dp"])
+            o.call("trcadv")
+            for k in ("tr", "trdp", "trc_tdcy", "trl_tdcy"):
+                assert _same(a[k], o.a[k]), ("trcadv", its, k)
+            inst.momtum(its, a["us"], a["vs"], a["ex"], a["relvor"], a["u_edg"], a["v_edg"], a["trc_edg"], geop_edg,
+                        a["mont_edg"], a["uvsq_edg"], a["u_tdcy"], a["v_tdcy"], a["dp"])
+            o.call("momtum")
+            for k in ("us", "vs", "relvor", "u_tdcy", "v_tdcy"):
+                assert _same(a[k], o.a[k]), ("momtum", its, k)
+            inst.hybgen(its, th, a["us"], a["vs"], a["tr"], a["sdot"], a["ex"], a["dp"], a["pr"])
+            o.call("hybgen", its)
+            for k in ("us", "vs", "tr", "sdot", "ex", "dp", "pr"):
+                assert _same(a[k], o.a[k]), ("hybgen", its, k)
+            inst.hystat(its, a["ph"], a["ex"], a["mp"], a["dp"], a["tr"], a["trdp"], a["psrf"], a["ptdcy"])
+            o.call("hystat", its)
+            for k in ("ph", "mp", "tr", "trdp", "psrf", "ptdcy"):
+                assert _same(a[k], o.a[k]), ("hystat", its, k)
+    finally:
+        inst.close()
+
+
+@pytest.mark.parametrize("mode,nsteps", [(0, 12), (1, 12)])
+def test_device_resident_steps_match_checker_bitwise(case, mode, nsteps):
+    """Start-up plus 12 steps entirely on the device (fimhip_step) against
+    the checker: every prognostic and diagnostic field identical."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    O, g, st, th, ds, ga = case
+    print("case: G3 nvl=64 ntra=4 mode=%d steps=%d device-resident" % (mode, nsteps), flush=True)
+    o = _oracle(O, ga, st)
+    o.init(True)
+    m = fimhip.Model(g, NVL, NTRA, DT, th, ds)
+    try:
+        m.load(st)
+        m.init_state(True)
+        for f in ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp"):
+            assert _same(m.download(f), o.a[f]), ("init", f)
+        for its in range(1, nsteps + 1):
+            m.step(its, mode)
+            o.step(its, mode)
+        m.sync()
+        fields = ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp", "ws", "massfx", "dp_tdcy",
+                  "trc_tdcy", "trl_tdcy")
+        if mode == 0:
+            fields += ("relvor", "sdot", "psrf", "u_tdcy", "v_tdcy")
+        for f in fields:
+            got = m.download(f)
+            assert _same(got, o.a[f]), (f, rel_l2(got, o.a[f]))
+        gm, gt = m.globsum()
+        om, ot = o.globsum()
+        assert abs(gm - om) <= 1e-13 * abs(om)
+        assert np.all(np.abs(gt - ot) <= 1e-12 * np.abs(ot))
+    finally:
+        m.close()

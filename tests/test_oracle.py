@@ -1,0 +1,188 @@
+"""The checker itself: the C restatement (oracle/fim_oracle.c) against the
+reference FIM Fortran, and against what the mathematics fixes."""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import golden_input, golden_ref, golden_grid, sha, seeded_case, rel_l2
+
+
+def _oracle_run(O, g, st, th, ds, dt, pow_kind=0, ntra=4):
+    ga = O.grid_arrays(g, st["dp"].shape[1], dt, th, ds)
+    o = O.OracleRun(ga, st["dp"].shape[1], ntra, dt, pow_kind=pow_kind)
+    o.load(st)
+    return o
+
+
+def _check_state(o, rec, stage, fields=("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr")):
+    bad = []
+    for f in fields:
+        key = stage + "." + f
+        if key not in rec:
+            continue
+        want = rec[key]["sha256"]
+        got = [sha(o.a[f][t]) for t in range(len(want))] if f == "tr" else [sha(o.a[f])]
+        if got != want:
+            bad.append(key)
+    return bad
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_oracle_bitexact_vs_reference_golden(oracle_lib, mode):
+    """Every state field after every sub-call of step 1, and after 3 steps,
+    has the exact bytes of the reference FP64 build (dyn_G2_ref.json)."""
+    O = oracle_lib
+    st, th, ds, dt = golden_input()
+    ref = golden_ref()["runs"]["mode%d" % mode]
+    g = golden_grid(2)
+    o = _oracle_run(O, g, st, th, ds, dt)
+    for key in ("grid.corio", "grid.rarea", "grid.grndspd"):
+        assert sha(o.garr[key.split(".")[1]]) == ref[key]["sha256"][0], key
+    assert sha(o.garr["dfflen"]) == ref["vert.dfflen"]["sha256"][0]
+    assert sha(o.garr["smoo_coeff"]) == ref["vert.smoo"]["sha256"][0]
+    o.init(True)
+    assert _check_state(o, ref, "init") == []
+    o.abstart(1)
+    o.call("edgvar1")
+    o.call("edgvar3")
+    for f, k in (("u", "u_edg"), ("v", "v_edg"), ("lp", "lp_edg"), ("mont", "mont_edg"), ("uvsq", "uvsq_edg"),
+                 ("dp", "dp_edg")):
+        assert sha(o.a[k]) == ref["edgvar3." + f]["sha256"][0], f
+    assert [sha(o.a["trc_edg"][t]) for t in range(4)] == ref["edgvar3.trc"]["sha256"]
+    o.call("cnuity")
+    assert _check_state(o, ref, "cnuity") == []
+    assert sha(o.a["massfx"][o.slots[0] - 1]) == ref["cnuity.massfx"]["sha256"][0]
+    assert sha(o.a["ws"]) == ref["cnuity.omega"]["sha256"][0]
+    o.call("trcadv")
+    assert _check_state(o, ref, "trcadv") == []
+    if mode == 0:
+        o.call("momtum")
+        assert _check_state(o, ref, "momtum") == []
+        assert sha(o.a["relvor"]) == ref["momtum.relvor"]["sha256"][0]
+        o.call("hybgen", 1)
+        assert _check_state(o, ref, "hybgen") == []
+        assert sha(o.a["sdot"]) == ref["hybgen.sdot"]["sha256"][0]
+        o.call("hystat", 1)
+        assert _check_state(o, ref, "hystat") == []
+    for its in (2, 3):
+        o.step(its, mode)
+    assert _check_state(o, ref, "final") == []
+
+
+def test_oracle_bitexact_vs_live_reference(oracle_lib):
+    """Where the reference build exists, G3 x 64 layers for 8 full steps."""
+    from oracle import refrun
+    if not refrun.available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    O = oracle_lib
+    g, st, th, ds = seeded_case(3)
+    r = refrun.run_dynref(refrun.ref_glvl(3), st, th, ds, nsteps=8, dt=240.0, mode=0)
+    o = _oracle_run(O, g, st, th, ds, 240.0)
+    o.init(True)
+    for its in range(1, 9):
+        o.step(its, 0)
+    for f in ("us", "vs", "dp", "pr", "ex", "ph", "mp"):
+        assert np.array_equal(o.a[f].ravel(), r["final." + f]), f
+    for t in range(4):
+        assert np.array_equal(o.a["tr"][t].ravel(), r["final.tr"][t])
+
+
+def test_project_pow_drift_within_bound(oracle_lib):
+    """fim_pow (the device pow) instead of the C library pow moves the 3-step
+    G2 state by far less than 1e-12 relative L2."""
+    O = oracle_lib
+    st, th, ds, dt = golden_input()
+    g = golden_grid(2)
+    runs = []
+    for pk in (0, 1):
+        o = _oracle_run(O, g, st, th, ds, dt, pow_kind=pk)
+        o.init(True)
+        for its in (1, 2, 3):
+            o.step(its, 0)
+        runs.append(o)
+    for f in ("us", "vs", "dp", "ex", "tr"):
+        assert rel_l2(runs[1].a[f], runs[0].a[f]) < 1e-13, f
+
+
+# ---- fim_pow against the C library and closed forms -----------------------
+
+def test_fim_pow_vs_libm(oracle_lib):
+    O = oracle_lib
+    lib = O.lib()
+    rng = np.random.default_rng(7)
+    kap = 287.0586 / 1004.6855
+    xs = np.concatenate([rng.uniform(5e-4, 1.3, 20000), rng.uniform(0.1, 1.05, 20000)])
+    ys = np.concatenate([np.full(20000, kap), np.full(20000, 1.0 / kap)])
+    same = 0
+    for x, y in zip(xs, ys):
+        a = math.pow(x, y)
+        b = lib.fo_pow(float(x), float(y))
+        d = abs(np.frombuffer(np.float64(a).tobytes(), np.int64)[0] - np.frombuffer(np.float64(b).tobytes(), np.int64)[0])
+        assert d <= 1
+        same += d == 0
+    assert same / len(xs) > 0.998
+
+
+def test_fim_pow_closed_forms(oracle_lib):
+    lib = oracle_lib.lib()
+    for n in range(-20, 21):
+        assert lib.fo_pow(2.0, float(n)) == 2.0 ** n
+    for x in (1e-300, 3e-5, 0.37, 1.0, 2.5, 7.0e5):
+        assert lib.fo_pow(x, 1.0) == x
+        assert lib.fo_pow(x, 0.0) == 1.0
+        assert abs(lib.fo_pow(x, 0.5) - math.sqrt(x)) <= 2.3e-16 * math.sqrt(x)
+        assert abs(lib.fo_pow(x, 2.0) - x * x) <= 2.3e-16 * x * x
+    assert lib.fo_pow(0.0, 0.3) == 0.0
+    assert math.isnan(lib.fo_pow(-1.0, 0.3))
+
+
+# ---- properties the transport must have -----------------------------------
+
+def test_transport_conserves_mass_and_keeps_constant_tracer(oracle_lib):
+    """Transport only: global mass is conserved to rounding; a spatially
+    constant tracer stays constant; tracers stay within 0 and their
+    initial global maximum (FCT positivity/monotonicity)."""
+    O = oracle_lib
+    g, st, th, ds = seeded_case(3, wind_noise=False)
+    st["tr"][3] = 0.25
+    o = _oracle_run(O, g, st, th, ds, 480.0)
+    o.init(False)
+    m0, t0 = o.globsum()
+    tmax = [o.a["tr"][t].max() for t in range(4)]
+    for its in range(1, 11):
+        o.step(its, 1)
+    m1, t1 = o.globsum()
+    assert abs(m1 - m0) / m0 < 1e-13
+    assert abs(t1[3] - t0[3]) / t0[3] < 1e-12
+    assert np.abs(o.a["tr"][3] - 0.25).max() < 1e-13
+    for t in (1, 2):
+        assert o.a["tr"][t].min() >= 0.0
+        assert o.a["tr"][t].max() <= tmax[t] * (1 + 1e-12)
+
+
+def test_resting_atmosphere_has_no_tendency(oracle_lib):
+    """A resting, horizontally uniform atmosphere: edge winds, vorticity and
+    the momentum tendency are zero, layer thicknesses do not change."""
+    O = oracle_lib
+    g, st, th, ds = seeded_case(2, wind_noise=False)
+    nvl = 64
+    for f in ("us", "vs"):
+        st[f][:] = 0.0
+    st["dp"][:] = st["dp"][0]
+    st["pr"][:] = st["pr"][0]
+    st["ex"][:] = st["ex"][0]
+    st["tr"][:, :, :] = st["tr"][:, :1, :]
+    o = _oracle_run(O, g, st, th, ds, 240.0)
+    o.init(False)
+    dp0 = o.a["dp"].copy()
+    o.abstart(1)
+    o.call("edgvar1")
+    assert np.abs(o.a["u_edg"]).max() == 0 and np.abs(o.a["v_edg"]).max() == 0
+    o.call("edgvar3")
+    o.call("cnuity")
+    assert np.array_equal(o.a["dp"], dp0)
+    o.call("trcadv")
+    o.call("momtum")
+    assert np.abs(o.a["relvor"]).max() == 0.0
+    assert np.abs(o.a["u_tdcy"][0]).max() < 1e-9 * 1e5
