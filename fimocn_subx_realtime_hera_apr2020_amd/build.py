@@ -17,6 +17,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libfimhip.so")
 
 SOURCES = ["fimhip_api.hip", "kernels_plain.hip", "hybgen.hip"]
+HOST_SOURCES = ["grid.cpp"]      # host-only C++ (g++)
 HEADERS = ["fimhip_internal.h", "fim_pow.h", "fim_pow_tables.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -49,6 +50,17 @@ def build(force: bool = True, verbose: bool = False) -> str:
         if not force and os.path.exists(op) and os.path.getmtime(op) > max(os.path.getmtime(sp), hdr_mtime):
             continue
         cmd = [HIPCC, *COMMON, "-c", sp, "-o", op]
+        if verbose:
+            print(" ".join(cmd), file=sys.stderr)
+        subprocess.check_call(cmd)
+    for src in HOST_SOURCES:
+        sp = os.path.join(CSRC, src)
+        op = os.path.join(od, src.replace(".cpp", ".o"))
+        objs.append(op)
+        if not force and os.path.exists(op) and os.path.getmtime(op) > max(os.path.getmtime(sp), hdr_mtime):
+            continue
+        cmd = ["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I" + INCLUDE,
+               "-c", sp, "-o", op]
         if verbose:
             print(" ".join(cmd), file=sys.stderr)
         subprocess.check_call(cmd)

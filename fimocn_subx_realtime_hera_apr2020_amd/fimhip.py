@@ -33,8 +33,8 @@ FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab", "fimhip_edgvar1_r8",
            "fimhip_edgvar3_r8", "fimhip_cnuity_r8", "fimhip_trcadv_r8", "fimhip_momtum_r8",
            "fimhip_hybgen_r8", "fimhip_hystat_r8", "fimhip_field_size", "fimhip_upload", "fimhip_download",
-           "fimhip_init_state", "fimhip_step", "fimhip_sync", "fimhip_globsum", "fimhip_pow",
-           "fimhip_version"]
+           "fimhip_init_state", "fimhip_step", "fimhip_sync", "fimhip_globsum", "fimhip_icos_grid",
+           "fimhip_pow", "fimhip_version"]
 
 
 class FimhipError(RuntimeError):
@@ -83,6 +83,9 @@ def load_library() -> ct.CDLL:
     lib.fimhip_step.argtypes = [vp, ct.c_int, ct.c_int]
     lib.fimhip_sync.argtypes = [vp]
     lib.fimhip_globsum.argtypes = [vp, _PD, _PD]
+    _PF = ct.POINTER(ct.c_float)
+    lib.fimhip_icos_grid.argtypes = [ct.c_int, _PF, _PF, _PI, _PI, _PI, _PF, _PF, _PF, _PF, _PF, _PF, _PF,
+                                     _PF, _PF, _PI]
     lib.fimhip_pow.argtypes = [ct.c_double, ct.c_double]
     lib.fimhip_pow.restype = ct.c_double
     lib.fimhip_version.restype = ct.c_char_p
@@ -101,6 +104,23 @@ def _p(a: np.ndarray):
     if a.dtype != np.float64 or not a.flags.c_contiguous:
         raise TypeError("expected a C-contiguous float64 array")
     return a.ctypes.data_as(_PD)
+
+
+def icos_grid(level: int) -> _glvl.Grid:
+    """Generate the glvl.dat tables of an all-bisection grid (fimhip_icos_grid)."""
+    lib = load_library()
+    nip = _glvl.nip_of(level)
+    f = lambda *shape: np.zeros(shape, np.float32)
+    i = lambda *shape: np.zeros(shape, np.int32)
+    a = dict(lat=f(nip), lon=f(nip), nprox=i(nip), prox=i(nip, 6), proxs=i(nip, 6), area=f(nip),
+             cs=f(nip, 6, 4), sn=f(nip, 6, 4), sidevec_c=f(nip, 6, 2), sidevec_e=f(nip, 6, 2),
+             sideln=f(nip, 6), rprox_ln=f(nip, 6), corner_xy=f(nip, 2, 6), hfcoef=f(nip, 6, 6),
+             inv_perm=i(nip))
+    order = ["lat", "lon", "nprox", "prox", "proxs", "area", "cs", "sn", "sidevec_c", "sidevec_e", "sideln",
+             "rprox_ln", "corner_xy", "hfcoef", "inv_perm"]
+    ptrs = [a[n].ctypes.data_as(_PI if a[n].dtype == np.int32 else ct.POINTER(ct.c_float)) for n in order]
+    _chk(lib.fimhip_icos_grid(level, *ptrs), "fimhip_icos_grid")
+    return _glvl.Grid(level, 0, **a)
 
 
 class Instance:

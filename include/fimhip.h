@@ -121,6 +121,18 @@ int fimhip_sync(fimhip_ctx *ctx);
 /* globsum.F90:70-101: column mass (sum area*dp/g) and tracer totals, FP64 */
 int fimhip_globsum(fimhip_ctx *ctx, double *mass, double *trtot);
 
+/* ---- grid generation ----------------------------------------------------
+ * The reference prep chain grid -> ginfo -> getlvl (FIMsrc/prep/grid/icos.F90:27,
+ * GridGen.F90:1, getlvl.F90:2) for glvl = 0..10, all bisection, modified
+ * recursive construction (gtype=2), IJ order (curve=0).  Outputs are the
+ * records of glvl.dat in its layout (getlvl.F90:296-346); nip = 10*4^glvl+2:
+ *   lat, lon, area (nip)  nprox, inv_perm (nip)  prox, proxs (6,nip), 1-based
+ *   cs, sn (4,6,nip)  sidevec_c, sidevec_e (2,6,nip)  sideln, rprox_ln (6,nip)
+ *   corner_xy (6,2,nip)  hfcoef (6,6,nip).  Any output pointer may be NULL. */
+int fimhip_icos_grid(int glvl, float *lat, float *lon, int *nprox, int *prox, int *proxs, float *area,
+                     float *cs, float *sn, float *sidevec_c, float *sidevec_e, float *sideln,
+                     float *rprox_ln, float *corner_xy, float *hfcoef, int *inv_perm);
+
 /* x**y as computed on the device (the host build of the same routine) */
 double fimhip_pow(double x, double y);
 const char *fimhip_version(void);

@@ -1,0 +1,70 @@
+"""The product's icosahedral grid generator (fimhip_icos_grid) against the
+reference prep chain's glvl.dat (tests/golden, made by the reference
+grid -> ginfo -> getlvl programs) and against the geometry it must satisfy."""
+import math
+
+import numpy as np
+import pytest
+
+from fimocn_subx_realtime_hera_apr2020_amd import fimhip, glvl
+from helpers import golden_grid
+
+EXACT = ["lat", "lon", "nprox", "prox", "proxs", "area", "cs", "sn", "sidevec_c", "sidevec_e", "sideln",
+         "rprox_ln", "corner_xy", "inv_perm"]
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_grid_bitexact_vs_reference(level):
+    ref = golden_grid(level)
+    got = fimhip.icos_grid(level)
+    assert got.nip == ref.nip == glvl.nip_of(level)
+    for f in EXACT:
+        assert np.array_equal(getattr(got, f), getattr(ref, f)), f
+    # hfcoef: the reference's single-precision sgesvd pseudo-inverse versus a
+    # double-precision QR one, both rounded to single; weights are O(1)
+    d = np.abs(got.hfcoef.astype(np.float64) - ref.hfcoef.astype(np.float64))
+    assert d.max() < 5e-6
+
+
+def test_grid_live_reference_g5():
+    from oracle import refrun
+    if not refrun.available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    ref = glvl.read_glvl(refrun.ref_glvl(5))
+    got = fimhip.icos_grid(5)
+    for f in EXACT:
+        assert np.array_equal(getattr(got, f), getattr(ref, f)), f
+
+
+@pytest.mark.parametrize("level", [4, 5])
+def test_grid_geometry(level):
+    g = fimhip.icos_grid(level)
+    nip = g.nip
+    assert nip == 10 * 4 ** level + 2
+    assert (g.nprox == 5).sum() == 12 and (g.nprox == 6).sum() == nip - 12
+    # neighbour symmetry: prox(proxs(e,i), prox(e,i)) == i (dyn_init.F90:615-628)
+    for i in range(nip):
+        for e in range(g.nprox[i]):
+            j = g.prox[i, e] - 1
+            assert g.prox[j, g.proxs[i, e] - 1] == i + 1
+    # the cells tile the sphere: the planar (stereographic) polygon areas
+    # exceed the spherical ones by a second-order amount, 0.15 / 4**level
+    total = g.area.astype(np.float64).sum()
+    excess = total / (4 * math.pi * 6371220.0 ** 2) - 1
+    assert 0 < excess < 0.2 / 4 ** level
+    # each edge vector seen from the other side is its negative (corner swap of getlvl.F90:129-145)
+    for i in range(0, nip, 37):
+        for e in range(g.nprox[i]):
+            j, eo = g.prox[i, e] - 1, g.proxs[i, e] - 1
+            assert np.allclose(g.sidevec_e[i, e], -g.sidevec_e[j, eo], rtol=2e-5, atol=1e-2)
+    # the closed polygon: edge vectors from the centre sum to ~0
+    s = np.abs(g.sidevec_c.astype(np.float64).sum(axis=1)).max()
+    assert s < 1e-3 * g.sideln.max()
+    # LS reconstruction reproduces a linear field exactly (to FP32)
+    x = np.cos(g.lat.astype(np.float64)) * np.cos(g.lon.astype(np.float64))
+    for i in range(0, nip, 53):
+        n = g.nprox[i]
+        nb = x[g.prox[i, :n] - 1]
+        rec = x[i] + (g.hfcoef[i, :n, :n].astype(np.float64) * (nb - x[i])[None, :]).sum(axis=1)
+        edge_mid = 0.5 * (x[i] + nb)
+        assert np.abs(rec - edge_mid).max() < 0.05 * np.abs(nb - x[i]).max() + 1e-7
