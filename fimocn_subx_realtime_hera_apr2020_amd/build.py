@@ -18,7 +18,7 @@ LIB = os.path.join(PKG, "libfimhip.so")
 
 SOURCES = ["fimhip_api.hip", "kernels_plain.hip", "hybgen.hip"]
 HOST_SOURCES = ["grid.cpp"]      # host-only C++ (g++)
-HEADERS = ["fimhip_internal.h", "fim_pow.h", "fim_pow_tables.h"]
+HEADERS = ["fimhip_internal.h", "fim_pow.h", "fim_pow_tables.h", "synth.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FIMHIP_ARCH", "gfx950")

@@ -6,13 +6,63 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
+#include <string>
 #include <vector>
 
 #include "fimhip.h"
 #include "fimhip_internal.h"
 #include "fim_pow.h"
+#include "synth.h"
 
 using namespace fimhip;
+
+// ---------------------------------------------------------------------------
+// per-kernel event timing
+namespace fimhip {
+struct KTimer {
+  struct Rec { const char *name; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long>> acc;
+  hipEvent_t get() {
+    hipEvent_t e = nullptr;
+    if (!pool.empty()) { e = pool.back(); pool.pop_back(); }
+    else if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+    return e;
+  }
+  int flush(hipStream_t st) {
+    if (hipStreamSynchronize(st) != hipSuccess) return FIMHIP_EHIP;
+    for (auto &r : recs) {
+      float ms = 0.f;
+      if (r.a && r.b && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+        auto &v = acc[r.name];
+        v.first += ms;
+        v.second += 1;
+      }
+      if (r.a) pool.push_back(r.a);
+      if (r.b) pool.push_back(r.b);
+    }
+    recs.clear();
+    return 0;
+  }
+  ~KTimer() {
+    for (auto &r : recs) { if (r.a) (void)hipEventDestroy(r.a); if (r.b) (void)hipEventDestroy(r.b); }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+thread_local KTimer *g_ktimer = nullptr;
+void kt_begin(const char *name, hipStream_t st) {
+  hipEvent_t e = g_ktimer->get();
+  if (e) (void)hipEventRecord(e, st);
+  g_ktimer->recs.push_back({name, e, nullptr});
+}
+void kt_end(hipStream_t st) {
+  hipEvent_t e = g_ktimer->get();
+  if (e) (void)hipEventRecord(e, st);
+  g_ktimer->recs.back().b = e;
+}
+}  // namespace fimhip
 
 struct fimhip_ctx {
   int device = 0;
@@ -25,6 +75,8 @@ struct fimhip_ctx {
   std::vector<void *> allocs;
   double *gsum_partial = nullptr;   // globsum block partials
   int gsum_blocks = 0;
+  std::vector<double> lat, lon;      // host copies for the synthetic-state profiles
+  KTimer *kt = nullptr;              // non-null while profiling
 };
 
 #define HIPCHK(x)                                                    \
@@ -70,13 +122,15 @@ extern "C" int fimhip_create(const fimhip_grid_desc *gd, int device, fimhip_ctx 
   const int nip = gd->nip, nvl = gd->nvl, ntra = gd->ntra;
   if (nip < 12 || nvl < 4 || nvl > NVL_MAX || ntra < 1 || ntra > 8 || !(gd->dt > 0.)) return FIMHIP_EINVAL;
   if (!gd->nprox || !gd->prox || !gd->proxs || !gd->cs || !gd->sn || !gd->sidevec_c || !gd->sidevec_e ||
-      !gd->sideln || !gd->hfcoef || !gd->area || !gd->lat || !gd->thetac || !gd->dpsig)
+      !gd->sideln || !gd->hfcoef || !gd->area || !gd->lat || !gd->lon || !gd->thetac || !gd->dpsig)
     return FIMHIP_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return FIMHIP_ENODEV;
   fimhip_ctx *c = new fimhip_ctx();
   c->device = device;
   c->nip = nip; c->nvl = nvl; c->ntra = ntra; c->dt = gd->dt;
+  c->lat.assign(gd->lat, gd->lat + nip);
+  c->lon.assign(gd->lon, gd->lon + nip);
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
     return FIMHIP_EHIP;
@@ -162,7 +216,8 @@ extern "C" int fimhip_destroy(fimhip_ctx *c) {
   if (!c) return FIMHIP_EINVAL;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
-  for (void *p : c->allocs) hipFree(p);
+  delete c->kt;
+  for (void *p : c->allocs) (void)hipFree(p);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return FIMHIP_OK;
@@ -435,6 +490,10 @@ extern "C" int fimhip_init_state(fimhip_ctx *c, int init_hybgen) {
 extern "C" int fimhip_step(fimhip_ctx *c, int its, int mode) {
   if (!c || its < 1 || (mode != 0 && mode != 1)) return FIMHIP_EINVAL;
   HIPCHK(hipSetDevice(c->device));
+  struct TimerScope {
+    explicit TimerScope(KTimer *t) { g_ktimer = t; }
+    ~TimerScope() { g_ktimer = nullptr; }
+  } scope(c->kt);
   ab_rotate(c->ab, its, c->dt);
   launch_edgvar1(c->g, c->s, c->stream);
   launch_edgvar3(c->g, c->s, c->stream);
@@ -495,6 +554,106 @@ extern "C" int fimhip_globsum(fimhip_ctx *c, double *mass, double *trtot) {
     if (q == 0) *mass = acc / GRVITY;
     else if (trtot) trtot[q - 1] = acc;
   }
+  return FIMHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// synthetic states (synth.h)
+static void synth_profiles(int nip, int nvl, const double *lat, const double *lon, std::vector<double> &u0,
+                           std::vector<double> &v0, std::vector<double> &bell, std::vector<double> &pint) {
+  const double uu = 38.6, alpha = 0.25 * M_PI;   // Williamson et al. 1992 test 1
+  const double lam_c = 1.5 * M_PI, rb = 1.0 / 3.0;
+  u0.resize(nip); v0.resize(nip); bell.resize(nip); pint.resize(nvl + 1);
+  for (int i = 0; i < nip; i++) {
+    u0[i] = uu * (cos(lat[i]) * cos(alpha) + sin(lat[i]) * cos(lon[i]) * sin(alpha));
+    v0[i] = -uu * sin(lon[i]) * sin(alpha);
+    double cr = cos(lat[i]) * cos(lon[i] - lam_c);
+    cr = cr > 1. ? 1. : (cr < -1. ? -1. : cr);
+    double r = acos(cr);
+    bell[i] = r < rb ? 0.5 * (1.0 + cos(M_PI * r / rb)) : 0.0;
+  }
+  for (int k = 0; k <= nvl; k++) pint[k] = 1.0e5 * fim_pow(5.0e-4, (double)k / (double)nvl);
+}
+
+static int check_synth(int nip, int nvl, int ntra) {
+  return (nip < 1 || nvl < 2 || nvl > NVL_MAX || ntra < 1 || ntra > 8) ? FIMHIP_EINVAL : 0;
+}
+
+extern "C" int fimhip_synth_state(int nip, int nvl, int ntra, const double *lat, const double *lon,
+                                  unsigned long long seed, int member, int flags, double *us, double *vs,
+                                  double *dp, double *pr, double *ex, double *ph, double *tr) {
+  if (check_synth(nip, nvl, ntra) || !lat || !lon || !us || !vs || !dp || !pr || !ex || !ph || !tr)
+    return FIMHIP_EINVAL;
+  std::vector<double> u0, v0, bell, pint;
+  synth_profiles(nip, nvl, lat, lon, u0, v0, bell, pint);
+  fim_synth_params p{nip, nvl, ntra, (uint64_t)seed, member, flags & 1, (flags >> 1) & 1, (flags >> 2) & 1};
+  for (int i = 0; i < nip; i++)
+    fim_synth_column(p, i, u0[i], v0[i], bell[i], pint.data(), us + (size_t)i * nvl, vs + (size_t)i * nvl,
+                     dp + (size_t)i * nvl, pr + (size_t)i * (nvl + 1), ex + (size_t)i * (nvl + 1),
+                     ph + (size_t)i * (nvl + 1), tr + (size_t)i * nvl, (size_t)nip * nvl);
+  return FIMHIP_OK;
+}
+
+__global__ void __launch_bounds__(64) k_synth(fim_synth_params p, const double *u0, const double *v0,
+                                              const double *bell, const double *pint, DevState s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.nip) return;
+  const int nvl = p.nvl;
+  double pl[NVL_MAX + 1];
+  for (int k = 0; k <= nvl; k++) pl[k] = pint[k];
+  fim_synth_column(p, i, u0[i], v0[i], bell[i], pl, s.us + (size_t)i * nvl, s.vs + (size_t)i * nvl,
+                   s.dp + (size_t)i * nvl, s.pr + (size_t)i * (nvl + 1), s.ex + (size_t)i * (nvl + 1),
+                   s.ph + (size_t)i * (nvl + 1), s.tr + (size_t)i * nvl, (size_t)p.nip * nvl);
+}
+
+extern "C" int fimhip_synth_device(fimhip_ctx *c, unsigned long long seed, int member, int flags) {
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<double> u0, v0, bell, pint;
+  synth_profiles(c->nip, c->nvl, c->lat.data(), c->lon.data(), u0, v0, bell, pint);
+  std::vector<double> packed;
+  packed.reserve(3 * c->nip + c->nvl + 1);
+  packed.insert(packed.end(), u0.begin(), u0.end());
+  packed.insert(packed.end(), v0.begin(), v0.end());
+  packed.insert(packed.end(), bell.begin(), bell.end());
+  packed.insert(packed.end(), pint.begin(), pint.end());
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, packed.size() * sizeof(double)));
+  hipError_t e = hipMemcpy(d, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    fim_synth_params p{c->nip, c->nvl, c->ntra, (uint64_t)seed, member, flags & 1, (flags >> 1) & 1, (flags >> 2) & 1};
+    k_synth<<<(c->nip + 63) / 64, 64, 0, c->stream>>>(p, d, d + c->nip, d + 2 * c->nip, d + 3 * c->nip, c->s);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  (void)hipFree(d);
+  HIPCHK(e);
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_profile(fimhip_ctx *c, int on) {
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  delete c->kt;
+  c->kt = on ? new KTimer() : nullptr;
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_profile_report(fimhip_ctx *c, char *buf, int len) {
+  if (!c || !buf || len < 1) return FIMHIP_EINVAL;
+  buf[0] = 0;
+  if (!c->kt) return FIMHIP_OK;
+  HIPCHK(hipSetDevice(c->device));
+  int r = c->kt->flush(c->stream);
+  if (r) return r;
+  std::string out;
+  char line[256];
+  for (auto &kv : c->kt->acc) {
+    snprintf(line, sizeof line, "%s %ld %.6f\n", kv.first.c_str(), kv.second.second, kv.second.first);
+    out += line;
+  }
+  if ((int)out.size() >= len) return FIMHIP_EINVAL;
+  memcpy(buf, out.c_str(), out.size() + 1);
   return FIMHIP_OK;
 }
 

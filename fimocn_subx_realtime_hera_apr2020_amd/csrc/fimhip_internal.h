@@ -62,6 +62,19 @@ struct AB {
   double ab1, ab2, ab3;
 };
 
+// Per-kernel HIP-event timing: when g_ktimer is set (by the API for an
+// instance with profiling on), every launch is bracketed by events on its stream.
+struct KTimer;
+extern thread_local KTimer *g_ktimer;
+void kt_begin(const char *name, hipStream_t st);
+void kt_end(hipStream_t st);
+#define KT_LAUNCH(name, st, ...)                 \
+  do {                                           \
+    if (::fimhip::g_ktimer) ::fimhip::kt_begin(name, st); \
+    __VA_ARGS__;                                 \
+    if (::fimhip::g_ktimer) ::fimhip::kt_end(st); \
+  } while (0)
+
 // launch helpers (kernels_plain.hip)
 void launch_edgvar1(const DevGrid &g, const DevState &s, hipStream_t st);
 void launch_edgvar3(const DevGrid &g, const DevState &s, hipStream_t st);

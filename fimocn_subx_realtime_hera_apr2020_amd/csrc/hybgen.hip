@@ -336,9 +336,9 @@ __global__ void __launch_bounds__(64) k_remap(DevGrid g, DevState s, int its) {
 }
 
 void launch_hybgen(const DevGrid &g, const DevState &s, int its, hipStream_t st) {
-  k_regrid<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its);
-  k_dffusn_lev<<<(g.nip + 3) / 4, dim3(64, 4), 0, st>>>(g, s);
-  k_remap<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its);
+  KT_LAUNCH("k_regrid", st, k_regrid<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its));
+  KT_LAUNCH("k_dffusn_lev", st, k_dffusn_lev<<<(g.nip + 3) / 4, dim3(64, 4), 0, st>>>(g, s));
+  KT_LAUNCH("k_remap", st, k_remap<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its));
 }
 
 }  // namespace fimhip

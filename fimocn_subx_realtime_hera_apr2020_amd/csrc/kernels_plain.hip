@@ -471,37 +471,37 @@ __global__ void __launch_bounds__(256) k_trdp(DevGrid g, DevState s) {
 
 // ---------------------------------------------------------------------------
 void launch_edgvar1(const DevGrid &g, const DevState &s, hipStream_t st) {
-  k_edgvar1<<<col_grid(g.nip), col_block(), 0, st>>>(g, s);
+  KT_LAUNCH("k_edgvar1", st, k_edgvar1<<<col_grid(g.nip), col_block(), 0, st>>>(g, s));
 }
 void launch_edgvar3(const DevGrid &g, const DevState &s, hipStream_t st) {
   dim3 grid((g.nip + COLS - 1) / COLS, g.ntra);
-  k_edgvar3<<<grid, col_block(), 0, st>>>(g, s);
+  KT_LAUNCH("k_edgvar3", st, k_edgvar3<<<grid, col_block(), 0, st>>>(g, s));
 }
 void launch_cnuity(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st) {
-  k_cnuity12<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab);
-  k_cnuity3<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab);
-  k_cnuity45<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab);
+  KT_LAUNCH("k_cnuity12", st, k_cnuity12<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
+  KT_LAUNCH("k_cnuity3", st, k_cnuity3<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
+  KT_LAUNCH("k_cnuity45", st, k_cnuity45<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
 }
 void launch_trcadv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st) {
   dim3 grid((g.nip + COLS - 1) / COLS, g.ntra);
-  k_trcadv1<<<grid, col_block(), 0, st>>>(g, s, ab);
-  k_trcadv2<<<grid, col_block(), 0, st>>>(g, s, ab);
-  k_trcadv3<<<grid, col_block(), 0, st>>>(g, s, ab);
+  KT_LAUNCH("k_trcadv1", st, k_trcadv1<<<grid, col_block(), 0, st>>>(g, s, ab));
+  KT_LAUNCH("k_trcadv2", st, k_trcadv2<<<grid, col_block(), 0, st>>>(g, s, ab));
+  KT_LAUNCH("k_trcadv3", st, k_trcadv3<<<grid, col_block(), 0, st>>>(g, s, ab));
 }
 void launch_momtum(const DevGrid &g, DevState &s, const AB &ab, hipStream_t st) {
-  k_momtum<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab);
+  KT_LAUNCH("k_momtum", st, k_momtum<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
   double *t = s.us; s.us = s.u2; s.u2 = t;
   t = s.vs; s.vs = s.v2; s.v2 = t;
 }
 void launch_hystat(const DevGrid &g, const DevState &s, int its, hipStream_t st) {
-  k_hystat<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its);
-  k_hystat_tr<<<col_grid(g.nip), col_block(), 0, st>>>(g, s);
+  KT_LAUNCH("k_hystat", st, k_hystat<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its));
+  KT_LAUNCH("k_hystat_tr", st, k_hystat_tr<<<col_grid(g.nip), col_block(), 0, st>>>(g, s));
 }
 void launch_pres_from_exner(const DevGrid &g, const DevState &s, hipStream_t st) {
-  k_pres_from_exner<<<col_grid(g.nip), col_block(), 0, st>>>(g, s);
+  KT_LAUNCH("k_pres_from_exner", st, k_pres_from_exner<<<col_grid(g.nip), col_block(), 0, st>>>(g, s));
 }
 void launch_trdp(const DevGrid &g, const DevState &s, hipStream_t st) {
-  k_trdp<<<col_grid(g.nip), col_block(), 0, st>>>(g, s);
+  KT_LAUNCH("k_trdp", st, k_trdp<<<col_grid(g.nip), col_block(), 0, st>>>(g, s));
 }
 
 }  // namespace fimhip

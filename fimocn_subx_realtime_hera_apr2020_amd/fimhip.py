@@ -34,7 +34,10 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_edgvar3_r8", "fimhip_cnuity_r8", "fimhip_trcadv_r8", "fimhip_momtum_r8",
            "fimhip_hybgen_r8", "fimhip_hystat_r8", "fimhip_field_size", "fimhip_upload", "fimhip_download",
            "fimhip_init_state", "fimhip_step", "fimhip_sync", "fimhip_globsum", "fimhip_icos_grid",
+           "fimhip_synth_state", "fimhip_synth_device", "fimhip_profile", "fimhip_profile_report",
            "fimhip_pow", "fimhip_version"]
+
+SYNTH_WIND_NOISE, SYNTH_PERTURB, SYNTH_LOGNORMAL = 1, 2, 4
 
 
 class FimhipError(RuntimeError):
@@ -49,7 +52,7 @@ class GridDesc(ct.Structure):
     _fields_ = [("nip", ct.c_int), ("nvl", ct.c_int), ("ntra", ct.c_int), ("dt", ct.c_double),
                 ("nprox", _PI), ("prox", _PI), ("proxs", _PI), ("cs", _PD), ("sn", _PD),
                 ("sidevec_c", _PD), ("sidevec_e", _PD), ("sideln", _PD), ("hfcoef", _PD),
-                ("area", _PD), ("lat", _PD), ("thetac", _PD), ("dpsig", _PD)]
+                ("area", _PD), ("lat", _PD), ("lon", _PD), ("thetac", _PD), ("dpsig", _PD)]
 
 
 _LIB = None
@@ -86,6 +89,11 @@ def load_library() -> ct.CDLL:
     _PF = ct.POINTER(ct.c_float)
     lib.fimhip_icos_grid.argtypes = [ct.c_int, _PF, _PF, _PI, _PI, _PI, _PF, _PF, _PF, _PF, _PF, _PF, _PF,
                                      _PF, _PF, _PI]
+    lib.fimhip_synth_state.argtypes = [ct.c_int, ct.c_int, ct.c_int, _PD, _PD, ct.c_ulonglong, ct.c_int,
+                                       ct.c_int] + [_PD] * 7
+    lib.fimhip_synth_device.argtypes = [vp, ct.c_ulonglong, ct.c_int, ct.c_int]
+    lib.fimhip_profile.argtypes = [vp, ct.c_int]
+    lib.fimhip_profile_report.argtypes = [vp, ct.c_char_p, ct.c_int]
     lib.fimhip_pow.argtypes = [ct.c_double, ct.c_double]
     lib.fimhip_pow.restype = ct.c_double
     lib.fimhip_version.restype = ct.c_char_p
@@ -123,6 +131,22 @@ def icos_grid(level: int) -> _glvl.Grid:
     return _glvl.Grid(level, 0, **a)
 
 
+def synth_state(lat, lon, nvl: int, ntra: int, seed: int = 20240601, member: int = 0, flags: int = 0) -> dict:
+    """One member's seeded synthetic state on the host (fimhip_synth_state);
+    the device generator fimhip_synth_device writes the same bytes."""
+    lib = load_library()
+    lat = np.ascontiguousarray(lat, np.float64)
+    lon = np.ascontiguousarray(lon, np.float64)
+    nip = lat.shape[0]
+    c, l = (nip, nvl), (nip, nvl + 1)
+    st = dict(us=np.empty(c), vs=np.empty(c), dp=np.empty(c), pr=np.empty(l), ex=np.empty(l), ph=np.empty(l),
+              tr=np.empty((ntra,) + c))
+    _chk(lib.fimhip_synth_state(nip, nvl, ntra, _p(lat), _p(lon), seed, member, flags,
+                                *[_p(st[n]) for n in ("us", "vs", "dp", "pr", "ex", "ph", "tr")]),
+         "fimhip_synth_state")
+    return st
+
+
 class Instance:
     """One device instance of the dynamics on one GPU (fimhip_create)."""
 
@@ -142,13 +166,15 @@ class Instance:
             hfcoef=np.ascontiguousarray(grid.hfcoef, np.float64),
             area=np.ascontiguousarray(grid.area, np.float64),
             lat=np.ascontiguousarray(grid.lat, np.float64),
+            lon=np.ascontiguousarray(grid.lon, np.float64),
             thetac=np.ascontiguousarray(thetac, np.float64), dpsig=np.ascontiguousarray(dpsig, np.float64))
         self._keep = keep
         gd = GridDesc()
         gd.nip, gd.nvl, gd.ntra, gd.dt = nip, nvl, ntra, dt
         for n in ("nprox", "prox", "proxs"):
             setattr(gd, n, keep[n].ctypes.data_as(_PI))
-        for n in ("cs", "sn", "sidevec_c", "sidevec_e", "sideln", "hfcoef", "area", "lat", "thetac", "dpsig"):
+        for n in ("cs", "sn", "sidevec_c", "sidevec_e", "sideln", "hfcoef", "area", "lat", "lon", "thetac",
+                  "dpsig"):
             setattr(gd, n, keep[n].ctypes.data_as(_PD))
         h = ct.c_void_p()
         _chk(self.lib.fimhip_create(ct.byref(gd), device, ct.byref(h)), "fimhip_create")
@@ -236,6 +262,10 @@ class Model(Instance):
         for n in ("us", "vs", "dp", "pr", "ex", "ph", "tr"):
             self.upload(n, st[n])
 
+    def synth(self, seed: int = 20240601, member: int = 0, flags: int = 0):
+        """Fill the device state with a seeded synthetic member (fimhip_synth_device)."""
+        _chk(self.lib.fimhip_synth_device(self.h, seed, member, flags), "synth_device")
+
     def init_state(self, init_hybgen: bool = True):
         _chk(self.lib.fimhip_init_state(self.h, 1 if init_hybgen else 0), "init_state")
 
@@ -244,6 +274,19 @@ class Model(Instance):
 
     def sync(self):
         _chk(self.lib.fimhip_sync(self.h), "sync")
+
+    def profile(self, on: bool = True):
+        _chk(self.lib.fimhip_profile(self.h, 1 if on else 0), "profile")
+
+    def profile_report(self) -> dict:
+        """{kernel: (launches, total_ms)} for the steps run while profiling."""
+        buf = ct.create_string_buffer(1 << 16)
+        _chk(self.lib.fimhip_profile_report(self.h, buf, len(buf)), "profile_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split()
+            out[name] = (int(n), float(ms))
+        return out
 
     def globsum(self):
         m = ct.c_double()

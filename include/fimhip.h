@@ -42,7 +42,7 @@ typedef struct {
   const double *sideln;         /* (6,nip) */
   const double *hfcoef;         /* (6,6,nip) */
   const double *area;           /* (nip) m^2 */
-  const double *lat;            /* (nip) radians */
+  const double *lat, *lon;      /* (nip) radians */
   const double *thetac;         /* (nvl) hybgen target theta (theta_coor.txt) */
   const double *dpsig;          /* (nvl) minimum layer thickness, Pa (dpsig.txt) */
 } fimhip_grid_desc;
@@ -121,6 +121,11 @@ int fimhip_sync(fimhip_ctx *ctx);
 /* globsum.F90:70-101: column mass (sum area*dp/g) and tracer totals, FP64 */
 int fimhip_globsum(fimhip_ctx *ctx, double *mass, double *trtot);
 
+/* per-kernel timing with HIP events on the instance's stream, for steps run
+ * while profiling is on; the report lists "kernel launches total_ms" lines */
+int fimhip_profile(fimhip_ctx *ctx, int on);
+int fimhip_profile_report(fimhip_ctx *ctx, char *buf, int len);
+
 /* ---- grid generation ----------------------------------------------------
  * The reference prep chain grid -> ginfo -> getlvl (FIMsrc/prep/grid/icos.F90:27,
  * GridGen.F90:1, getlvl.F90:2) for glvl = 0..10, all bisection, modified
@@ -132,6 +137,18 @@ int fimhip_globsum(fimhip_ctx *ctx, double *mass, double *trtot);
 int fimhip_icos_grid(int glvl, float *lat, float *lon, int *nprox, int *prox, int *proxs, float *area,
                      float *cs, float *sn, float *sidevec_c, float *sidevec_e, float *sideln,
                      float *rprox_ln, float *corner_xy, float *hfcoef, int *inv_perm);
+
+/* ---- synthetic seeded states --------------------------------------------
+ * One member's state, bit-identical on the host and on the device: splitmix64
+ * hashes of (seed, member, field, cell, level).  flags: bit 0 adds N(0,1) m/s
+ * wind noise, bit 1 applies the ensemble-member perturbations (theta N(0,0.5 K),
+ * u,v N(0,1), dp x (1+0.02 U[-1,1])), bit 2 makes tracers 2..T log-normal
+ * (median 1e-9, sigma 1.5) times the cosine bell.  Arrays as fimhip_upload. */
+int fimhip_synth_state(int nip, int nvl, int ntra, const double *lat, const double *lon,
+                       unsigned long long seed, int member, int flags, double *us, double *vs, double *dp,
+                       double *pr, double *ex, double *ph, double *tr);
+/* the same, written straight into the instance's device state */
+int fimhip_synth_device(fimhip_ctx *ctx, unsigned long long seed, int member, int flags);
 
 /* x**y as computed on the device (the host build of the same routine) */
 double fimhip_pow(double x, double y);

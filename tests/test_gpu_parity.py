@@ -117,3 +117,67 @@ def test_device_resident_steps_match_checker_bitwise(case, mode, nsteps):
         assert np.all(np.abs(gt - ot) <= 1e-12 * np.abs(ot))
     finally:
         m.close()
+
+
+def test_device_synth_matches_host_generator():
+    """fimhip_synth_device writes exactly the bytes of the host generator."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+    g = golden_grid(3)
+    th, ds = state.vertical_tables(NVL)
+    m = fimhip.Model(g, NVL, NTRA, DT, th, ds)
+    try:
+        for flags in (0, fimhip.SYNTH_WIND_NOISE | fimhip.SYNTH_PERTURB, fimhip.SYNTH_LOGNORMAL):
+            m.synth(seed=99, member=5, flags=flags)
+            host = fimhip.synth_state(g.lat, g.lon, NVL, NTRA, seed=99, member=5, flags=flags)
+            for f in ("us", "vs", "dp", "pr", "ex", "ph", "tr"):
+                assert _same(m.download(f), host[f]), (flags, f)
+    finally:
+        m.close()
+
+
+def test_g5_96_steps_match_checker_bitwise(oracle_lib):
+    """G5 x 64 layers, 96 full steps at dt=240 s on the device: every field
+    identical to the checker."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+    O = oracle_lib
+    g = fimhip.icos_grid(5)
+    th, ds = state.vertical_tables(NVL)
+    print("case: G5 nvl=64 ntra=4 96 full steps", flush=True)
+    m = fimhip.Model(g, NVL, NTRA, DT, th, ds)
+    try:
+        m.synth(seed=20240601, member=0, flags=fimhip.SYNTH_WIND_NOISE)
+        st = {f: m.download(f) for f in ("us", "vs", "dp", "pr", "ex", "ph", "tr")}
+        m.init_state(True)
+        o = O.OracleRun(O.grid_arrays(g, NVL, DT, th, ds), NVL, NTRA, DT, pow_kind=1)
+        o.load(st)
+        o.init(True)
+        for its in range(1, 97):
+            m.step(its, 0)
+            o.step(its, 0)
+        m.sync()
+        for f in ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr"):
+            assert _same(m.download(f), o.a[f]), f
+        gm, _ = m.globsum()
+        om, _ = o.globsum()
+        assert abs(gm - om) <= 1e-13 * abs(om)
+    finally:
+        m.close()
+
+
+def test_profiler_reports_every_kernel():
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+    g = golden_grid(3)
+    th, ds = state.vertical_tables(NVL)
+    m = fimhip.Model(g, NVL, NTRA, DT, th, ds)
+    try:
+        m.synth(flags=1)
+        m.init_state(True)
+        m.profile(True)
+        for its in (1, 2):
+            m.step(its, 0)
+        rep = m.profile_report()
+        for k in ("k_edgvar1", "k_edgvar3", "k_cnuity12", "k_cnuity3", "k_cnuity45", "k_trcadv1", "k_trcadv2",
+                  "k_trcadv3", "k_momtum", "k_regrid", "k_dffusn_lev", "k_remap", "k_hystat", "k_hystat_tr"):
+            assert rep[k][0] == 2 and rep[k][1] > 0, k
+    finally:
+        m.close()
