@@ -209,64 +209,74 @@ __global__ void __launch_bounds__(256) k_dffusn_lev(DevGrid g, DevState s) {
   }
 }
 
-__device__ void pcwise(int kk, const double *xold, const double *yold, const double *xnew, double *ynew) {
-  double zold[LMAX], znew[LMAX], fco[NVL_MAX], fcn[NVL_MAX], vertfx[LMAX], fld[NVL_MAX], b[NVL_MAX];
+// pcwise (PLM, miglim = 1) for one column.  The old and new interface
+// coordinates x*, the cell widths fco/fcn and the displacements delz are
+// formed on the fly from xold/xnew (the reference's zold/znew are +-x exactly),
+// and the slopes and interface fluxes are formed in one forward sweep, so the
+// only column array is the filled field itself.
+__device__ void pcwise(int kk, const double *xold, const double *xnew, double *fld, double *ynew) {
   const double small = 1.e-22;
-  if (xold[0] < xold[kk]) {
-    for (int k = 0; k <= kk; k++) { zold[k] = xold[k]; znew[k] = xnew[k]; }
-  } else {
-    for (int k = 0; k <= kk; k++) { zold[k] = -xold[k]; znew[k] = -xnew[k]; }
+  const bool up = xold[0] < xold[kk];
+  auto zo = [&](int k) { return up ? xold[k] : -xold[k]; };
+  auto zn = [&](int k) { return up ? xnew[k] : -xnew[k]; };
+  auto fco = [&](int k) { return dmax(0., zo(k + 1) - zo(k)); };
+  // fill zero-width cells from their neighbours (two sweeps)
+  for (int k = kk - 1; k >= 1; k--) {
+    const double f = fco(k - 1);
+    fld[k - 1] = (fld[k - 1] * f + fld[k] * small) / (f + small);
   }
-  for (int k = 0; k < kk; k++) {
-    fco[k] = dmax(0., zold[k + 1] - zold[k]);
-    fcn[k] = dmax(0., znew[k + 1] - znew[k]);
+  for (int k = 2; k <= kk; k++) {
+    const double f = fco(k - 1);
+    fld[k - 1] = (fld[k - 1] * f + fld[k - 2] * small) / (f + small);
   }
-  for (int k = 0; k < kk; k++) fld[k] = yold[k];
-  for (int k = kk - 1; k >= 1; k--)
-    fld[k - 1] = (fld[k - 1] * fco[k - 1] + fld[k] * small) / (fco[k - 1] + small);
-  for (int k = 2; k <= kk; k++)
-    fld[k - 1] = (fld[k - 1] * fco[k - 1] + fld[k - 2] * small) / (fco[k - 1] + small);
-  b[0] = 0.;
-  b[kk - 1] = 0.;
-  for (int k = 2; k <= kk - 1; k++) {   // PLM (remap_optn = 2)
-    double fm = fld[k - 2], f0 = fld[k - 1], fp = fld[k];
-    b[k - 1] = 0.;
-    if (f0 <= dmin(fm, fp) || f0 >= dmax(fm, fp)) b[k - 1] = 0.;
-    else if ((fp - fm) * (fm + fp - 2. * f0) > 0.) b[k - 1] = f0 - fm;
-    else b[k - 1] = fp - f0;
-  }
-  for (int k = 2; k <= kk; k++) {       // miglim = 1
-    double slab = 0., amount = 0.;
-    vertfx[k - 1] = 0.;
-    const double dz = znew[k - 1] - zold[k - 1];
+  auto slope = [&](int k) {   // PLM slope b(k), 1-based
+    if (k == 1 || k == kk) return 0.;
+    const double fm = fld[k - 2], f0 = fld[k - 1], fp = fld[k];
+    if (f0 <= dmin(fm, fp) || f0 >= dmax(fm, fp)) return 0.;
+    if ((fp - fm) * (fm + fp - 2. * f0) > 0.) return f0 - fm;
+    return fp - f0;
+  };
+  auto vflux = [&](int k) {   // vertfx(k), 1-based, k = 2..kk
+    double slab = 0., amount = 0., vf = 0.;
+    const double dz = zn(k - 1) - zo(k - 1);
     if (dz > 0.) {
       const int lyr = k;
-      if (fco[lyr - 1] > 0.) {
-        double dslab = dmin(slab + fco[lyr - 1], dz) - dmin(slab, dz);
-        double dx = dslab / fco[lyr - 1];
-        double fcdx = fld[lyr - 1] + b[lyr - 1] * .5 * (dx - 1.) + 0. * (.25 - dx * (.5 - dx * (1. / 3.)));
+      const double fc = fco(lyr - 1);
+      if (fc > 0.) {
+        double dslab = dmin(slab + fc, dz) - dmin(slab, dz);
+        double dx = dslab / fc;
+        double fcdx = fld[lyr - 1] + slope(lyr) * .5 * (dx - 1.) + 0. * (.25 - dx * (.5 - dx * (1. / 3.)));
         amount = amount + fcdx * dslab;
         slab = slab + dslab;
       }
     } else if (dz < 0.) {
       const int lyr = k - 1;
-      if (fco[lyr - 1] > 0.) {
-        double dslab = dmin(slab + fco[lyr - 1], -dz) - dmin(slab, -dz);
-        double dx = dslab / fco[lyr - 1];
-        double fcdx = fld[lyr - 1] + b[lyr - 1] * .5 * (1. - dx) + 0. * (.25 - dx * (.5 - dx * (1. / 3.)));
+      const double fc = fco(lyr - 1);
+      if (fc > 0.) {
+        double dslab = dmin(slab + fc, -dz) - dmin(slab, -dz);
+        double dx = dslab / fc;
+        double fcdx = fld[lyr - 1] + slope(lyr) * .5 * (1. - dx) + 0. * (.25 - dx * (.5 - dx * (1. / 3.)));
         amount = amount + fcdx * dslab;
         slab = slab + dslab;
       }
     }
-    if (slab != 0.) vertfx[k - 1] = -dz * amount / slab;
+    if (slab != 0.) vf = -dz * amount / slab;
+    return vf;
+  };
+  // the fluxes use the filled field, which the output loop overwrites:
+  // write the results one level behind the flux sweep
+  double vlo = 0.;                       // vertfx(1)
+  double prev_new = 0.;
+  for (int k = 1; k <= kk; k++) {
+    const double vhi = k < kk ? vflux(k + 1) : 0.;   // vertfx(kk+1) = 0
+    const double fo = fco(k - 1), fn = dmax(0., zn(k) - zn(k - 1));
+    const double amount = fld[k - 1] * fo - (vhi - vlo);
+    const double y = (amount + fld[k - 1] * small) / (fn + small);
+    if (k > 1) ynew[k - 2] = prev_new;
+    prev_new = y;
+    vlo = vhi;
   }
-  vertfx[0] = 0.;
-  vertfx[kk] = 0.;
-  for (int k = 0; k < kk; k++) {
-    double vertdv = vertfx[k + 1] - vertfx[k];
-    double amount = fld[k] * fco[k] - vertdv;
-    ynew[k] = (amount + fld[k] * small) / (fcn[k] + small);
-  }
+  ynew[kk - 1] = prev_new;
 }
 
 // remap_1d for one column; also sdot = new Exner - old Exner and ex = new Exner
@@ -274,21 +284,28 @@ __global__ void __launch_bounds__(64) k_remap(DevGrid g, DevState s, int its) {
   const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1, ntra = g.ntra;
   const int ipn = blockIdx.x * blockDim.x + threadIdx.x;
   if (ipn >= nip) return;
-  double exold[LMAX], prcol[LMAX], pk1col[LMAX], exnew[LMAX], prnew[LMAX], pk1new[LMAX];
-  double col[NVL_MAX], cnew[NVL_MAX], th1[NVL_MAX];
+  double prcol[LMAX], pk1col[LMAX], exnew[LMAX], prnew[LMAX], pk1new[LMAX];
+  double fld[NVL_MAX], th1[NVL_MAX];
   const double *targt = g.thetac;
   const double small = 1.e-6;
   for (int k = 0; k < nvlp1; k++) {
-    exold[k] = L2(s.ex, ipn, k);
+    const double eo = L2(s.ex, ipn, k);
     prcol[k] = L2(s.pr, ipn, k);
     exnew[k] = L2(s.exsmo, ipn, k);
+    pk1col[k] = eo * prcol[k] / CP;
   }
-  for (int k = 0; k < nvlp1; k++) pk1col[k] = exold[k] * prcol[k] / CP;
   inflate(g.dpsig, nvl, prcol[0], exnew);
   if (its > 0)
-    for (int k = 0; k < nvlp1; k++) exnew[k] = SLAK * exnew[k] + (1. - SLAK) * exold[k];
-  for (int k = 2; k <= nvlp1 - 1; k++)
-    exnew[k - 1] = dmax(.0001 * exold[k - 1] + .9999 * exold[k], dmin(.0001 * exold[k - 1] + .9999 * exold[k - 2], exnew[k - 1]));
+    for (int k = 0; k < nvlp1; k++) exnew[k] = SLAK * exnew[k] + (1. - SLAK) * L2(s.ex, ipn, k);
+  {
+    double em = L2(s.ex, ipn, 0), e0 = L2(s.ex, ipn, 1);
+    for (int k = 2; k <= nvlp1 - 1; k++) {
+      const double ep = L2(s.ex, ipn, k);
+      exnew[k - 1] = dmax(.0001 * e0 + .9999 * ep, dmin(.0001 * e0 + .9999 * em, exnew[k - 1]));
+      em = e0;
+      e0 = ep;
+    }
+  }
   for (int k = nvlp1; k >= 1; k--) prnew[k - 1] = P1000 * fim_pow(exnew[k - 1] / CP, CP / RD);
   for (int k = 0; k < nvl; k++) C2(s.dp, ipn, k) = prnew[k] - prnew[k + 1];
   for (int k = 0; k < nvlp1; k++) pk1new[k] = exnew[k] * prnew[k] / CP;
@@ -297,20 +314,17 @@ __global__ void __launch_bounds__(64) k_remap(DevGrid g, DevState s, int its) {
   prnew[0] = prcol[0];
   pk1new[0] = pk1col[0];
   // u, v and tracers 2..ntra are remapped in pressure
-  for (int k = 0; k < nvl; k++) col[k] = C2(s.us, ipn, k);
-  pcwise(nvl, prcol, col, prnew, cnew);
-  for (int k = 0; k < nvl; k++) C2(s.us, ipn, k) = cnew[k];
-  for (int k = 0; k < nvl; k++) col[k] = C2(s.vs, ipn, k);
-  pcwise(nvl, prcol, col, prnew, cnew);
-  for (int k = 0; k < nvl; k++) C2(s.vs, ipn, k) = cnew[k];
+  for (int k = 0; k < nvl; k++) fld[k] = C2(s.us, ipn, k);
+  pcwise(nvl, prcol, prnew, fld, &C2(s.us, ipn, 0));
+  for (int k = 0; k < nvl; k++) fld[k] = C2(s.vs, ipn, k);
+  pcwise(nvl, prcol, prnew, fld, &C2(s.vs, ipn, 0));
   for (int n = 1; n < ntra; n++) {
-    for (int k = 0; k < nvl; k++) col[k] = TR(s.tr, n, ipn, k);
-    pcwise(nvl, prcol, col, prnew, cnew);
-    for (int k = 0; k < nvl; k++) TR(s.tr, n, ipn, k) = cnew[k];
+    for (int k = 0; k < nvl; k++) fld[k] = TR(s.tr, n, ipn, k);
+    pcwise(nvl, prcol, prnew, fld, &TR(s.tr, n, ipn, 0));
   }
   // theta is remapped in p^(kappa+1)
-  for (int k = 0; k < nvl; k++) col[k] = TR(s.tr, 0, ipn, k);
-  pcwise(nvl, pk1col, col, pk1new, th1);
+  for (int k = 0; k < nvl; k++) fld[k] = TR(s.tr, 0, ipn, k);
+  pcwise(nvl, pk1col, pk1new, fld, th1);
   for (int k = nvl; k >= 3; k--) {
     double dplo = dmax(pk1new[k - 2] - pk1new[k - 1], pk1new[0] * small);
     double dpup = dmax(pk1new[k - 1] - pk1new[k], pk1new[0] * small);
@@ -330,7 +344,7 @@ __global__ void __launch_bounds__(64) k_remap(DevGrid g, DevState s, int its) {
     if (exnew[k - 1] < exnew[k]) exnew[k] = exnew[k - 1];
   for (int k = 0; k < nvlp1; k++) {
     L2(s.pr, ipn, k) = prnew[k];
-    L2(s.sdot, ipn, k) = exnew[k] - exold[k];
+    L2(s.sdot, ipn, k) = exnew[k] - L2(s.ex, ipn, k);
     L2(s.ex, ipn, k) = exnew[k];
   }
 }

@@ -29,6 +29,14 @@ __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : 
 
 constexpr int COLS = 4;   // columns per block
 
+// The column index of a block row is the same for all 64 lanes of a wave
+// (blockDim.x == 64 == wavefront size); making it wave-uniform lets every
+// grid-table access (prox, cs, sn, sidevec, hfcoef, area) go through
+// scalar loads instead of vector gathers.
+__device__ __forceinline__ int wave_col() {
+  return __builtin_amdgcn_readfirstlane((int)(blockIdx.x * COLS + threadIdx.y));
+}
+
 static inline dim3 col_grid(int nip) { return dim3((nip + COLS - 1) / COLS); }
 static inline dim3 col_block() { return dim3(NVL_MAX, COLS); }
 
@@ -37,7 +45,7 @@ static inline dim3 col_block() { return dim3(NVL_MAX, COLS); }
 __global__ void __launch_bounds__(256) k_edgvar1(DevGrid g, DevState s) {
   const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
   const int k = threadIdx.x;
-  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int ipn = wave_col();
   if (ipn >= nip || k >= nvl) return;
   const double divby6 = 1. / 6., eps = 0.001;
   const int np = g.nprox[ipn];
@@ -81,27 +89,27 @@ __global__ void __launch_bounds__(256) k_edgvar1(DevGrid g, DevState s) {
 // edgvar3.F90:65-163: LS reconstruction on both sides of each edge
 // (edgvalEval) and the two-sided average of loop 2, fused: each cell-level
 // recomputes its neighbour's reconstruction of the shared edge.
-__device__ __forceinline__ double edgval(const DevGrid &g, const double *fld, int c, int i, int k, int nvl) {
-  const double b0 = fld[(size_t)c * nvl + k];
-  double v = b0;
-  const int n = g.nprox[c];
-  for (int j = 0; j < n; j++) v = v + (fld[(size_t)PROX(c, j) * nvl + k] - b0) * HF(c, i, j);
-  return v;
-}
-
 __global__ void __launch_bounds__(256) k_edgvar3(DevGrid g, DevState s) {
   const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x;
-  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int ipn = wave_col();
   const int f = blockIdx.y;   // 0: dp, f >= 1: tracer f
   if (ipn >= nip || k >= nvl) return;
   const double *fld = f == 0 ? s.dp : s.tr + (size_t)f * nip * nvl;
   double *out = f == 0 ? s.dp_edg : s.trc_edg + (size_t)f * nip * NPP * nvl;
   const int np = g.nprox[ipn];
+  const double b0 = fld[(size_t)ipn * nvl + k];
+  double nbv[NPP];
+  for (int j = 0; j < NPP; j++) nbv[j] = fld[(size_t)PROX(ipn, j) * nvl + k];
   for (int i = 0; i < np; i++) {
-    const int nb = PROX(ipn, i), eo = PROXS(ipn, i);
-    double own = edgval(g, fld, ipn, i, k, nvl);
-    double oth = edgval(g, fld, nb, eo, k, nvl);
+    // this cell's reconstruction of edge i (edgvalEval)
+    double own = b0;
+    for (int j = 0; j < np; j++) own = own + (nbv[j] - b0) * HF(ipn, i, j);
+    // the neighbour's reconstruction of the same edge
+    const int nb = PROX(ipn, i), eo = PROXS(ipn, i), nn = g.nprox[nb];
+    const double c0 = nbv[i];
+    double oth = c0;
+    for (int j = 0; j < nn; j++) oth = oth + (fld[(size_t)PROX(nb, j) * nvl + k] - c0) * HF(nb, eo, j);
     E3(out, ipn, i, k) = 0.5 * (own + oth);
   }
   for (int i = np; i < NPP; i++) E3(out, ipn, i, k) = 0.;
@@ -112,7 +120,7 @@ __global__ void __launch_bounds__(256) k_edgvar3(DevGrid g, DevState s) {
 __global__ void __launch_bounds__(256) k_cnuity12(DevGrid g, DevState s, AB ab) {
   const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x;
-  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int ipn = wave_col();
   if (ipn >= nip || k >= nvl) return;
   const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
   const int np = g.nprox[ipn];
@@ -142,7 +150,7 @@ __global__ void __launch_bounds__(256) k_cnuity12(DevGrid g, DevState s, AB ab) 
 __global__ void __launch_bounds__(256) k_cnuity3(DevGrid g, DevState s, AB ab) {
   const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x;
-  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int ipn = wave_col();
   if (ipn >= nip || k >= nvl) return;
   const int of = ab.of - 1, vof = ab.vof - 1;
   const double thshld = 1.e-11;
@@ -180,7 +188,7 @@ __global__ void __launch_bounds__(256) k_cnuity45(DevGrid g, DevState s, AB ab) 
   __shared__ double sh_om[COLS][NVL_MAX + 1];
   const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
   const int k = threadIdx.x, c = threadIdx.y;
-  const int ipn = blockIdx.x * COLS + c;
+  const int ipn = wave_col();
   const bool live = ipn < nip && k < nvl;
   const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
   double dtn = 0.;
@@ -237,7 +245,7 @@ __global__ void __launch_bounds__(256) k_cnuity45(DevGrid g, DevState s, AB ab) 
 __global__ void __launch_bounds__(256) k_trcadv1(DevGrid g, DevState s, AB ab) {
   const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x;
-  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int ipn = wave_col();
   const int t = blockIdx.y;
   if (ipn >= nip || k >= nvl) return;
   const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
@@ -278,7 +286,7 @@ __global__ void __launch_bounds__(256) k_trcadv1(DevGrid g, DevState s, AB ab) {
 __global__ void __launch_bounds__(256) k_trcadv2(DevGrid g, DevState s, AB ab) {
   const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x;
-  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int ipn = wave_col();
   const int t = blockIdx.y;
   if (ipn >= nip || k >= nvl) return;
   const int of = ab.of - 1, vof = ab.vof - 1;
@@ -316,7 +324,7 @@ __global__ void __launch_bounds__(256) k_trcadv2(DevGrid g, DevState s, AB ab) {
 __global__ void __launch_bounds__(256) k_trcadv3(DevGrid g, DevState s, AB ab) {
   const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x;
-  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int ipn = wave_col();
   const int t = blockIdx.y;
   if (ipn >= nip || k >= nvl) return;
   const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
@@ -352,7 +360,7 @@ __global__ void __launch_bounds__(256) k_momtum(DevGrid g, DevState s, AB ab) {
   __shared__ double sh_q[COLS][3];
   const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
   const int k = threadIdx.x, c = threadIdx.y;
-  const int ipn = blockIdx.x * COLS + c;
+  const int ipn = wave_col();
   const bool live = ipn < nip && k < nvl;
   const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
   const double thshld = 1.e-11;
@@ -420,23 +428,45 @@ __global__ void __launch_bounds__(256) k_momtum(DevGrid g, DevState s, AB ab) {
 }
 
 // ---------------------------------------------------------------------------
-// hystat.F90:23-115, one thread per column
-__global__ void __launch_bounds__(64) k_hystat(DevGrid g, DevState s, int its) {
-  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1, ntra = g.ntra;
-  const int ipn = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ipn >= nip) return;
-  double work = s.psrf[ipn];
-  double ps = C2(s.dp, ipn, nvl - 1);
-  for (int k = nvl - 1; k >= 1; k--) ps = ps + C2(s.dp, ipn, k - 1);
-  s.psrf[ipn] = ps;
-  if (its > 0) s.ptdcy[(size_t)(its % 2) * nip + ipn] = ps - work;
-  double mp = L2(s.ex, ipn, 0) * TR(s.tr, 0, ipn, 0) + L2(s.ph, ipn, 0);
-  C2(s.mp, ipn, 0) = mp;
-  L2(s.ph, ipn, 1) = mp - TR(s.tr, 0, ipn, 0) * L2(s.ex, ipn, 1);
-  for (int k = 2; k <= nvl; k++) {
-    mp = mp + L2(s.ex, ipn, k - 1) * (TR(s.tr, 0, ipn, k - 1) - TR(s.tr, 0, ipn, k - 2));
-    C2(s.mp, ipn, k - 1) = mp;
-    L2(s.ph, ipn, k) = mp - TR(s.tr, 0, ipn, k - 1) * L2(s.ex, ipn, k);
+// hystat.F90:23-115 (column part), one wavefront per column: the column is
+// staged in LDS with coalesced loads, lane 0 runs the serial surface-pressure
+// and Montgomery-potential sums in the reference order, and every lane then
+// forms its level's geopotential.
+__global__ void __launch_bounds__(256) k_hystat(DevGrid g, DevState s, int its) {
+  __shared__ double sh_ex[COLS][NVL_MAX + 1];
+  __shared__ double sh_th[COLS][NVL_MAX];
+  __shared__ double sh_dp[COLS][NVL_MAX];
+  __shared__ double sh_mp[COLS][NVL_MAX];
+  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
+  const int k = threadIdx.x, c = threadIdx.y;
+  const int ipn = wave_col();
+  const bool live = ipn < nip;
+  if (live) {
+    for (int kk = k; kk < nvlp1; kk += NVL_MAX) sh_ex[c][kk] = L2(s.ex, ipn, kk);
+    if (k < nvl) {
+      sh_th[c][k] = TR(s.tr, 0, ipn, k);
+      sh_dp[c][k] = C2(s.dp, ipn, k);
+    }
+  }
+  __syncthreads();
+  if (live && k == 0) {
+    double work = s.psrf[ipn];
+    double ps = sh_dp[c][nvl - 1];
+    for (int kk = nvl - 1; kk >= 1; kk--) ps = ps + sh_dp[c][kk - 1];
+    s.psrf[ipn] = ps;
+    if (its > 0) s.ptdcy[(size_t)(its % 2) * nip + ipn] = ps - work;
+    double mp = sh_ex[c][0] * sh_th[c][0] + L2(s.ph, ipn, 0);
+    sh_mp[c][0] = mp;
+    for (int kk = 2; kk <= nvl; kk++) {
+      mp = mp + sh_ex[c][kk - 1] * (sh_th[c][kk - 1] - sh_th[c][kk - 2]);
+      sh_mp[c][kk - 1] = mp;
+    }
+  }
+  __syncthreads();
+  if (live && k < nvl) {
+    const double mp = sh_mp[c][k];
+    C2(s.mp, ipn, k) = mp;
+    L2(s.ph, ipn, k + 1) = mp - sh_th[c][k] * sh_ex[c][k + 1];
   }
 }
 
@@ -444,7 +474,7 @@ __global__ void __launch_bounds__(64) k_hystat(DevGrid g, DevState s, int its) {
 __global__ void __launch_bounds__(256) k_hystat_tr(DevGrid g, DevState s) {
   const int nip = g.nip, nvl = g.nvl, ntra = g.ntra;
   const int k = threadIdx.x;
-  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int ipn = wave_col();
   if (ipn >= nip || k >= nvl) return;
   if (ntra > 1) TR(s.tr, 1, ipn, k) = dmax(QVMIN, TR(s.tr, 1, ipn, k));
   if (ntra > 2) TR(s.tr, 2, ipn, k) = dmax(QWMIN, TR(s.tr, 2, ipn, k));
@@ -455,7 +485,7 @@ __global__ void __launch_bounds__(256) k_hystat_tr(DevGrid g, DevState s) {
 // pressure from the Exner function after the start-up hybgen (dyn_init.F90:884-893)
 __global__ void __launch_bounds__(256) k_pres_from_exner(DevGrid g, DevState s) {
   const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
-  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int ipn = wave_col();
   if (ipn >= nip) return;
   for (int k = threadIdx.x; k < nvlp1; k += blockDim.x)
     L2(s.pr, ipn, k) = P1000 * fim_pow(L2(s.ex, ipn, k) / CP, CP / RD);
@@ -464,7 +494,7 @@ __global__ void __launch_bounds__(256) k_pres_from_exner(DevGrid g, DevState s) 
 __global__ void __launch_bounds__(256) k_trdp(DevGrid g, DevState s) {
   const int nip = g.nip, nvl = g.nvl, ntra = g.ntra;
   const int k = threadIdx.x;
-  const int ipn = blockIdx.x * COLS + threadIdx.y;
+  const int ipn = wave_col();
   if (ipn >= nip || k >= nvl) return;
   for (int t = 0; t < ntra; t++) TR(s.trdp, t, ipn, k) = TR(s.tr, t, ipn, k) * C2(s.dp, ipn, k);
 }
@@ -494,7 +524,7 @@ void launch_momtum(const DevGrid &g, DevState &s, const AB &ab, hipStream_t st) 
   t = s.vs; s.vs = s.v2; s.v2 = t;
 }
 void launch_hystat(const DevGrid &g, const DevState &s, int its, hipStream_t st) {
-  KT_LAUNCH("k_hystat", st, k_hystat<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its));
+  KT_LAUNCH("k_hystat", st, k_hystat<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, its));
   KT_LAUNCH("k_hystat_tr", st, k_hystat_tr<<<col_grid(g.nip), col_block(), 0, st>>>(g, s));
 }
 void launch_pres_from_exner(const DevGrid &g, const DevState &s, hipStream_t st) {
