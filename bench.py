@@ -70,7 +70,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (one MI355X per box); "
-              "multi-GPU sharding is described in DESIGN.md, not run", file=sys.stderr)
+              "ensembles would shard by member, one large member by IJ blocks with a 1-ring halo; not run here", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

@@ -1,8 +1,8 @@
 /* fimhip.h - C ABI of the MI355X (gfx950) FIM horizontal-dynamics library.
  *
  * A single-task serial FIM build calls these entry points in place of the
- * reference subroutines of FIMsrc/fim/horizontal (through the bind(C)
- * interfaces shown in INTEGRATION.md).  All arrays are host arrays in the
+ * reference subroutines of FIMsrc/fim/horizontal (through a Fortran module of
+ * bind(C) interfaces).  All arrays are host arrays in the
  * reference's Fortran layout, level index fastest:
  *   cell (nvl,nip)   level (nvl+1,nip)   edge (nvl,6,nip)   tracers (nvl,nip,ntra)
  *   tendencies (nvl,nip,3)   tracer tendencies (nvl,nip,3,ntra)   massfx (nvl,6,nip,3)
