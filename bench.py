@@ -1,20 +1,32 @@
 """Benchmark of the FIM horizontal dynamics step on one MI355X.
 
-Workload (default): one member of the G7 (163842 cells) x 64-layer state with
-4 tracers, dt = 240 s, full dynamics steps (abstart, edgvar1, edgvar3,
-cnuity, trcadv, momtum, hybgen, hystat) on the FP64 path, seeded synthetic
-state generated on the device.  One step = one pass of the hot path over the
-whole state; value = members * nip * nvl * steps / wall seconds
-(cell-level-steps/s), with the state resident in HBM before the timed region.
+Headline workload: a 16-member ensemble on the G7 grid (163842 cells) x 64
+layers with 4 tracers, dt = 90 s (the reference's step for this grid), full
+dynamics steps (abstart, edgvar1,
+edgvar3, cnuity, trcadv, momtum, hybgen, hystat), in the batched
+mixed-precision form (FP32 winds, tracers and edge values; FP64 thickness,
+pressure, Exner function, Montgomery potential and hybgen), the step replayed
+from a captured HIP graph with autotuned launch shapes.  Members are seeded
+synthetic states (wind noise and member perturbations) generated on the
+device.  value = members * nip * nvl * steps / wall seconds
+(cell-level-steps/s), state resident in HBM before the timed region.
+
+Beside it, in the same run: the FP64 plain path (one member of the same
+grid, bit-identical to the checker), the accuracy of the FP32 form against
+FP64 after 96 steps and the global mass drift, and the C restatement timed on
+the host's cores.
 
 Prints ONE JSON line.  Besides the contract fields it carries
-  roofline:     the dominant kernel (most device time in the timed region,
-                timed with HIP events on the library's stream): algorithmic
-                bytes per launch / mean launch duration against 8 TB/s
-  step_hbm:     the whole step against the compulsory-byte model
-                (18 + 8T values per cell-level-step, 8 B each in FP64)
-  cpu_baseline: the C restatement (oracle/fim_oracle.c, FP64, OpenMP) timed
-                on this host's cores on a bounded sample of the same workload
+  roofline:     the headline form's dominant kernel (most device time, timed
+                with HIP events on the library's stream): algorithmic bytes
+                per launch / mean launch duration against 8 TB/s
+  step_hbm:     the whole step against the compulsory-byte model (18 + 8T
+                values per cell-level-step; 220 B in the mixed form, 400 B
+                in FP64)
+  plain_fp64:   the FP64 path's throughput, roofline and step_hbm
+  accuracy_96:  relative L2 of member 0 against FP64 after 96 steps, mass drift
+  cpu_baseline: the C restatement (oracle/fim_oracle.c, FP64, OpenMP) on a
+                bounded sample of the same grid
 """
 import argparse
 import json
@@ -26,12 +38,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+SEED = 20240601
 
-# Algorithmic (compulsory) HBM values touched per cell-level by each FP64
-# kernel of one launch: distinct array elements it must read plus those it
-# writes, with T tracers.  Each value is 8 bytes.
-def kernel_values(T):
-    return {
+
+def plain_kernel_bytes(T):
+    """Compulsory HBM bytes per cell-level of each FP64 kernel launch: the
+    distinct array elements it must read plus those it writes (8 B each)."""
+    v = {
         "k_edgvar1": 5 + 36,                 # us vs pr mp tr1 -> u v lp mont uvsq trc1 on 6 edges
         "k_edgvar3": 7 * T,                  # dp, tracers 2..T -> 6 edge values each
         "k_cnuity12": 18 + 3 + 6 + 6 + 2,    # u/v/dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo
@@ -47,6 +60,27 @@ def kernel_values(T):
         "k_hystat": 5,
         "k_hystat_tr": 2 * T + 3,
     }
+    return {k: 8 * n for k, n in v.items()}
+
+
+def ens_kernel_bytes(T):
+    """The same for the mixed-precision kernels (4 B per FP32 value, 8 B per
+    FP64 value; inputs shared by the tracers of one launch counted once)."""
+    return {
+        "e_edgvar1": 28 + 168,               # us vs tr1 (f32) pr mp (f64) -> u v uvsq lp (f32) mont (f64) trc1 edges
+        "e_edgvar3": 32 + 28 * (T - 1),      # dp (f64) -> dp_edg; tracers 2..T -> trc_edg
+        "e_cnuity12": 144,                   # u/v/dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo
+        "e_cnuity3": 60,                     # antifx, delp_lo, dp, dp_tdcy(of,vof) -> r+, r-
+        "e_cnuity45": 180,                   # antifx r+- massfx dp_tdcy dp pr lp_edg u v -> massfx dp_tdcy dp pr ex omega
+        "e_trcadv1": 32 + 80 * T,
+        "e_trcadv2": 8 + 60 * T,
+        "e_trcadv3": 8 + 68 * T,
+        "e_momtum": 204,
+        "k_regrid": 20,
+        "k_dffusn_lev": 16,
+        "k_remap": 56 + 40 + 8 * T,
+        "e_hystat": 40 + 8 * T,
+    }
 
 
 def parse():
@@ -57,8 +91,13 @@ def parse():
     ap.add_argument("--glvl", type=int, default=7)
     ap.add_argument("--nvl", type=int, default=64)
     ap.add_argument("--ntra", type=int, default=4)
-    ap.add_argument("--dt", type=float, default=240.0)
+    ap.add_argument("--members", type=int, default=16)
+    ap.add_argument("--dt", type=float, default=None,
+                    help="time step (s); default the reference's 1.2*4800/(2**glvl/2) (90 s at G7)")
     ap.add_argument("--mode", type=int, default=0, help="0 full step, 1 transport only")
+    ap.add_argument("--form", choices=["both", "ens", "plain"], default="both")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=96, help="0 skips the accuracy leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -66,35 +105,38 @@ def parse():
     return ap.parse_args()
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only (one MI355X per box); "
-              "ensembles would shard by member, one large member by IJ blocks with a 1-ring halo; not run here", file=sys.stderr)
-        sys.exit(2)
-    import numpy as np
-    import torch
-    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+def roofline(prof, kbytes, cells):
+    """The kernel with the most device time: algorithmic bytes per launch over
+    its mean launch time."""
+    dname, (dn, dms) = max(prof.items(), key=lambda kv: kv[1][1])
+    dmean = dms / dn * 1e-3
+    dbytes = kbytes.get(dname, 0) * cells
+    return {"kernel": dname, "bound": "hbm", "achieved": dbytes / dmean / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": dbytes / dmean / HBM_PEAK, "traffic": None,
+            "bytes_per_launch": dbytes, "mean_launch_ms": dmean * 1e3}
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    torch.cuda.init()
-    grid = fimhip.icos_grid(a.glvl)
-    th, ds = state.vertical_tables(a.nvl)
+
+def kernel_table(prof, kbytes, cells):
+    tot = sum(v[1] for v in prof.values())
+    out = {}
+    for k, (n, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+        mean = ms / n
+        out[k] = {"launches": n, "mean_ms": mean, "share": ms / tot}
+        if k in kbytes:
+            out[k]["GBs"] = kbytes[k] * cells / (mean * 1e-3) / 1e9
+    return out
+
+
+def run_plain(a, grid, th, ds, fimhip):
+    import torch
     m = fimhip.Model(grid, a.nvl, a.ntra, a.dt, th, ds, device=0)
-    flags = fimhip.SYNTH_WIND_NOISE
-    m.synth(seed=20240601, member=0, flags=flags)
+    m.synth(seed=SEED, member=0, flags=fimhip.SYNTH_WIND_NOISE | fimhip.SYNTH_PERTURB)
     m.init_state(True)
-    if a.copy_calibration:
-        x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
-        y = torch.empty_like(x)
-        y.copy_(x)
-        torch.cuda.synchronize()
     its = 0
     for _ in range(a.warmup):
         its += 1
         m.step(its, a.mode)
     m.sync()
-    m.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -103,54 +145,112 @@ def main():
     m.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # per-kernel event timing over as many further steps
+    m.profile(True)
+    for _ in range(a.steps):
+        its += 1
+        m.step(its, a.mode)
+    m.sync()
     prof = m.profile_report()
     m.profile(False)
-
-    nip, nvl, T = grid.nip, a.nvl, a.ntra
-    cells = nip * nvl
+    cells = grid.nip * a.nvl
+    kb = plain_kernel_bytes(a.ntra)
+    v_step = (18 + 8 * a.ntra) if a.mode == 0 else (10 + 8 * a.ntra)
     value = cells * a.steps / wall
-    vals = kernel_values(T)
-    dom = max(prof.items(), key=lambda kv: kv[1][1])
-    dname, (dn, dms) = dom
-    dmean = dms / dn * 1e-3
-    dbytes = vals.get(dname, 0) * 8 * cells
-    roof = {"kernel": dname, "bound": "hbm", "achieved": dbytes / dmean / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": dbytes / dmean / HBM_PEAK, "traffic": None,
-            "bytes_per_launch": dbytes, "mean_launch_ms": dmean * 1e3}
-    v_step = (18 + 8 * T) if a.mode == 0 else (10 + 8 * T)
-    step_hbm = {"values_per_cell_level": v_step, "bytes_per_value": 8,
-                "achieved_GBs": value * v_step * 8 / 1e9, "frac": value * v_step * 8 / HBM_PEAK}
-    kernels = {k: {"launches": n, "mean_ms": ms / n, "share": ms / sum(v[1] for v in prof.values())}
-               for k, (n, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1])}
-
-    cpu = None
-    if not a.no_cpu_baseline:
-        cpu = cpu_baseline(m, grid, a, th, ds)
-    m.close()
-
-    out = {
-        "metric": "cell-level-steps/s at G7x64L, FP32+batched vs FP64 plain; % HBM peak",
-        "value": value, "unit": "cell-level-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded, generated on the device)",
-        "config": {"workload": "G%d x %dL, %d tracers, dt=%gs, %s step, FP64 plain path, 1 member"
-                   % (a.glvl, nvl, T, a.dt, "full" if a.mode == 0 else "transport"),
-                   "nip": nip, "nvl": nvl, "ntra": T, "members": 1, "form": "plain-fp64"},
-        "roofline": roof, "step_hbm": step_hbm, "kernels": kernels, "cpu_baseline": cpu,
-    }
-    print(json.dumps(out))
+    res = {"value": value, "ms_per_step": wall / a.steps * 1e3, "members": 1,
+           "roofline": roofline(prof, kb, cells),
+           "step_hbm": {"bytes_per_cell_level": 8 * v_step, "achieved_GBs": value * 8 * v_step / 1e9,
+                        "frac": value * 8 * v_step / HBM_PEAK},
+           "kernels": kernel_table(prof, kb, cells)}
+    return m, res
 
 
-def cpu_baseline(m, grid, a, th, ds):
-    """Time the C restatement on the host: one member of the same workload,
-    started from the device's post-start-up state, a few full steps."""
+def synth_ens(e, members, fimhip):
+    for mm in range(members):
+        e.synth(mm, seed=SEED, synth_member=mm, flags=fimhip.SYNTH_WIND_NOISE | fimhip.SYNTH_PERTURB)
+    e.init_state(True)
+
+
+def run_ens(a, grid, th, ds, fimhip):
+    import torch
+    M = a.members
+    e = fimhip.Ensemble(grid, M, a.nvl, a.ntra, a.dt, th, ds, device=0)
+    tune = None
+    if not a.no_autotune:
+        synth_ens(e, M, fimhip)
+        tt = e.autotune(a.mode, 2)
+        names = e.kernel_names()
+        tiles = e.tiles()
+        tune = {n: {"tile": int(tiles[n]), "ms": [round(float(x), 4) for x in tt[k]]}
+                for k, n in enumerate(names)}
+    synth_ens(e, M, fimhip)
+    e.step(a.warmup, a.mode)
+    e.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.step(a.steps, a.mode)
+    e.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    e.profile(True)
+    e.step(a.steps, a.mode)
+    e.sync()
+    prof = e.profile_report()
+    e.profile(False)
+    cells = M * grid.nip * a.nvl
+    kb = ens_kernel_bytes(a.ntra)
+    value = cells * a.steps / wall
+    b_step = 220 if a.mode == 0 else 156
+    res = {"value": value, "ms_per_step": wall / a.steps * 1e3, "members": M,
+           "roofline": roofline(prof, kb, cells),
+           "step_hbm": {"bytes_per_cell_level": b_step, "achieved_GBs": value * b_step / 1e9,
+                        "frac": value * b_step / HBM_PEAK},
+           "kernels": kernel_table(prof, kb, cells), "tiles": tune}
+    return e, res
+
+
+def accuracy(a, e, m, grid, fimhip):
+    """Member 0 of the ensemble against the FP64 path after n steps, and the
+    global mass drift of every member."""
     import numpy as np
+    n = a.accuracy_steps
+    M = a.members
+    synth_ens(e, M, fimhip)
+    m.synth(seed=SEED, member=0, flags=fimhip.SYNTH_WIND_NOISE | fimhip.SYNTH_PERTURB)
+    m.init_state(True)
+    m0, t0, _, _ = e.globsum()
+    p0, _ = m.globsum()
+    e.step(n, a.mode)
+    for its in range(1, n + 1):
+        m.step(its, a.mode)
+    m.sync()
+    e.sync()
+    m1, t1, k1, kt1 = e.globsum()
+    p1, _ = m.globsum()
+
+    def rl2(x, y):
+        return float(np.sqrt(np.sum((x - y) ** 2) / np.sum(y ** 2)))
+
+    err = {f: rl2(e.download(0, f), m.download(f)) for f in ("us", "vs", "dp", "pr", "ex", "ph", "mp")}
+    tre, trp = e.download(0, "tr"), m.download("tr")
+    for t in range(a.ntra):
+        err["tr%d" % (t + 1)] = rl2(tre[t], trp[t])
+    return {"steps": n, "rel_l2_vs_fp64": err,
+            "mass_drift_max": float(np.max(np.abs(m1 - m0) / m0)),
+            "mass_drift_fp64_path": float(abs(p1 - p0) / p0),
+            "tracer_drift_max": [float(x) for x in np.max(np.abs(t1 - t0) / np.abs(t0), axis=0)],
+            "kahan_fp32_vs_fp64_mass": float(np.max(np.abs(k1 - m1) / m1)),
+            "kahan_fp32_vs_fp64_tracers": float(np.max(np.abs(kt1 - t1) / np.abs(t1)))}
+
+
+def cpu_baseline(m, grid, a, th, ds, fimhip):
+    """Time the C restatement on the host: one member of the same grid,
+    started from the device's post-start-up state, a few full steps."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     o = O.OracleRun(O.grid_arrays(grid, a.nvl, a.dt, th, ds), a.nvl, a.ntra, a.dt, pow_kind=0)
-    # fresh member state at the step the device started from
-    m.synth(seed=20240601, member=0, flags=1)
+    m.synth(seed=SEED, member=0, flags=fimhip.SYNTH_WIND_NOISE | fimhip.SYNTH_PERTURB)
     m.init_state(True)
     for n in ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp"):
         o.a[n][...] = m.download(n)
@@ -162,8 +262,80 @@ def cpu_baseline(m, grid, a, th, ds):
     cells = grid.nip * a.nvl
     return {"value": cells * a.cpu_steps / wall, "unit": "cell-level-steps/s", "cores": threads,
             "kind": "port",
-            "sample": "1 member of G%d x %dL, %d tracers, %d full FP64 steps (C restatement, OpenMP %d threads)"
-                      % (a.glvl, a.nvl, a.ntra, a.cpu_steps, threads)}
+            "sample": "1 member of G%d x %dL, %d tracers, %d %s FP64 steps (C restatement, OpenMP %d threads)"
+                      % (a.glvl, a.nvl, a.ntra, a.cpu_steps, "full" if a.mode == 0 else "transport", threads)}
+
+
+def main():
+    a = parse()
+    if a.dt is None:        # module_control.F90:214-220 (SubdivNum = 2 at every level)
+        a.dt = 1.2 * 4800.0 / float(2 ** a.glvl // 2)
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only; ensembles would shard by member and one large "
+              "member by IJ blocks with a 1-ring halo, which is not run here", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.init()
+    grid = fimhip.icos_grid(a.glvl)
+    th, ds = state.vertical_tables(a.nvl)
+    if a.copy_calibration:
+        x = torch.empty(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+    step_name = "full" if a.mode == 0 else "transport"
+    plain = ens = None
+    m = e = None
+    if a.form in ("both", "plain"):
+        m, plain = run_plain(a, grid, th, ds, fimhip)
+        print("plain FP64: %.3f ms/step" % plain["ms_per_step"], file=sys.stderr, flush=True)
+    if a.form in ("both", "ens"):
+        e, ens = run_ens(a, grid, th, ds, fimhip)
+        print("ensemble: %.3f ms/step for %d members" % (ens["ms_per_step"], a.members), file=sys.stderr,
+              flush=True)
+    acc = None
+    if e is not None and m is not None and a.accuracy_steps > 0:
+        acc = accuracy(a, e, m, grid, fimhip)
+        print("accuracy leg done", file=sys.stderr, flush=True)
+    cpu = None
+    if not a.no_cpu_baseline:
+        if m is None:
+            m = fimhip.Model(grid, a.nvl, a.ntra, a.dt, th, ds, device=0)
+        cpu = cpu_baseline(m, grid, a, th, ds, fimhip)
+    if e is not None:
+        e.close()
+    if m is not None:
+        m.close()
+
+    head = ens if ens is not None else plain
+    if ens is not None:
+        workload = ("%d members x G%d (%d cells) x %d layers, %d tracers, dt=%gs, %s steps, FP32 form with FP64 "
+                    "thickness/pressure/PGF/hybgen, graph-replayed step, autotuned tiles"
+                    % (a.members, a.glvl, grid.nip, a.nvl, a.ntra, a.dt, step_name))
+        form, dtype = "fp32-batched", "f32+f64"
+    else:
+        workload = ("1 member x G%d (%d cells) x %d layers, %d tracers, dt=%gs, %s steps, FP64 plain path"
+                    % (a.glvl, grid.nip, a.nvl, a.ntra, a.dt, step_name))
+        form, dtype = "plain-fp64", "f64"
+    out = {
+        "metric": "cell-level-steps/s at G7x64L, FP32+batched vs FP64 plain; % HBM peak",
+        "value": head["value"], "unit": "cell-level-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype,
+        "data": "synthetic (seeded member states generated on the device)",
+        "config": {"workload": workload, "nip": grid.nip, "nvl": a.nvl, "ntra": a.ntra,
+                   "members": head["members"], "form": form, "dt": a.dt,
+                   "dt_note": "the reference's own time step for this grid (module_control.F90:214-220); at G7 "
+                              "a 240 s step overflows the FP64 state by step 6, so it is not timed"},
+        "roofline": head["roofline"], "step_hbm": head["step_hbm"],
+        "speedup_vs_plain_fp64": (ens["value"] / plain["value"]) if (ens and plain) else None,
+        "plain_fp64": plain, "ensemble": ens, "accuracy_96": acc, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

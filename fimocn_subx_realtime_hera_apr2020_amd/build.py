@@ -16,9 +16,9 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libfimhip.so")
 
-SOURCES = ["fimhip_api.hip", "kernels_plain.hip", "hybgen.hip"]
+SOURCES = ["fimhip_api.hip", "kernels_plain.hip", "hybgen.hip", "ens_api.hip", "kernels_ens.hip"]
 HOST_SOURCES = ["grid.cpp"]      # host-only C++ (g++)
-HEADERS = ["fimhip_internal.h", "fim_pow.h", "fim_pow_tables.h", "synth.h"]
+HEADERS = ["fimhip_internal.h", "fim_pow.h", "fim_pow_tables.h", "synth.h", "host_common.h", "ens_internal.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FIMHIP_ARCH", "gfx950")
@@ -40,30 +40,29 @@ def _objs_dir():
 def build(force: bool = True, verbose: bool = False) -> str:
     """Compile every source and link libfimhip.so (always recompiles when force)."""
     od = _objs_dir()
-    objs = []
+    objs, jobs = [], []
     hdr_mtime = max(os.path.getmtime(os.path.join(CSRC, h)) for h in HEADERS)
     hdr_mtime = max(hdr_mtime, os.path.getmtime(os.path.join(INCLUDE, "fimhip.h")))
-    for src in SOURCES:
+    for src in SOURCES + HOST_SOURCES:
         sp = os.path.join(CSRC, src)
-        op = os.path.join(od, src.replace(".hip", ".o"))
+        op = os.path.join(od, os.path.splitext(src)[0] + ".o")
         objs.append(op)
         if not force and os.path.exists(op) and os.path.getmtime(op) > max(os.path.getmtime(sp), hdr_mtime):
             continue
-        cmd = [HIPCC, *COMMON, "-c", sp, "-o", op]
+        if src.endswith(".hip"):
+            cmd = [HIPCC, *COMMON, "-c", sp, "-o", op]
+        else:
+            cmd = ["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I" + INCLUDE,
+                   "-c", sp, "-o", op]
+        jobs.append(cmd)
+    procs = []
+    for cmd in jobs:                      # the sources compile independently
         if verbose:
             print(" ".join(cmd), file=sys.stderr)
-        subprocess.check_call(cmd)
-    for src in HOST_SOURCES:
-        sp = os.path.join(CSRC, src)
-        op = os.path.join(od, src.replace(".cpp", ".o"))
-        objs.append(op)
-        if not force and os.path.exists(op) and os.path.getmtime(op) > max(os.path.getmtime(sp), hdr_mtime):
-            continue
-        cmd = ["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I" + INCLUDE,
-               "-c", sp, "-o", op]
-        if verbose:
-            print(" ".join(cmd), file=sys.stderr)
-        subprocess.check_call(cmd)
+        procs.append((cmd, subprocess.Popen(cmd)))
+    failed = [cmd for cmd, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = LIB + ".tmp"
     subprocess.check_call([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, LIB)

@@ -1,0 +1,440 @@
+// C ABI of the batched mixed-precision form (fimhip_ens_*): creation,
+// seeded member states, start-up, the graph-captured step, downloads, global
+// sums with FP32 Kahan partials, per-kernel launch shapes and their
+// autotuner, and per-kernel event timing.
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ens_internal.h"
+#include "fimhip.h"
+#include "host_common.h"
+
+using namespace fimhip;
+
+struct fimhip_ens {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nip = 0, nvl = 0, ntra = 0, members = 0;
+  double dt = 0.;
+  EnsGrid g{};
+  EnsState s{};
+  Arena arena;
+  DevState stage{};                  // one member in FP64 (synth / load staging)
+  std::vector<double> lat, lon;
+  int tile[EK_NKERNELS];
+  // globsum partials
+  int gs_blocks = 0;
+  double *gs64 = nullptr;
+  float *gs32 = nullptr;
+  // captured step
+  bool use_graph = true;
+  int graph_mode = -1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  KTimer *kt = nullptr;
+};
+
+static void drop_graph(fimhip_ens *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+  e->graph_mode = -1;
+}
+
+template <class T>
+static int to_float(Arena &a, const T **out, const double *src, size_t n) {
+  std::vector<float> h(n);
+  for (size_t i = 0; i < n; i++) h[i] = (float)src[i];
+  float *p = nullptr;
+  int r = a.upload(&p, h);
+  *out = p;
+  return r;
+}
+
+extern "C" int fimhip_ens_create(const fimhip_grid_desc *gd, int members, int device, fimhip_ens **out) {
+  if (!gd || !out) return FIMHIP_EINVAL;
+  *out = nullptr;
+  const int nip = gd->nip, nvl = gd->nvl, ntra = gd->ntra;
+  if (members < 1 || members > 4096 || nip < 12 || nvl < 4 || nvl > NVL_MAX || ntra < 1 || ntra > 8 ||
+      !(gd->dt > 0.))
+    return FIMHIP_EINVAL;
+  if (!gd->nprox || !gd->prox || !gd->proxs || !gd->cs || !gd->sn || !gd->sidevec_c || !gd->sidevec_e ||
+      !gd->sideln || !gd->hfcoef || !gd->area || !gd->lat || !gd->lon || !gd->thetac || !gd->dpsig)
+    return FIMHIP_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return FIMHIP_ENODEV;
+  HostGrid hg;
+  int r = host_grid_prepare(gd, hg);
+  if (r) return r;
+  fimhip_ens *e = new fimhip_ens();
+  e->device = device;
+  e->nip = nip; e->nvl = nvl; e->ntra = ntra; e->members = members; e->dt = gd->dt;
+  e->lat.assign(gd->lat, gd->lat + nip);
+  e->lon.assign(gd->lon, gd->lon + nip);
+  for (int k = 0; k < EK_NKERNELS; k++) e->tile[k] = 2;   // 4 columns per block, no swizzle
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess) {
+    delete e;
+    return FIMHIP_EHIP;
+  }
+  e->arena.stream = e->stream;
+  Arena &a = e->arena;
+  EnsGrid &g = e->g;
+  if ((r = dev_grid_upload(a, gd, hg, g.d))) { fimhip_ens_destroy(e); return r; }
+  if ((r = to_float(a, &g.cs, gd->cs, (size_t)nip * NPP * 4)) || (r = to_float(a, &g.sn, gd->sn, (size_t)nip * NPP * 4)) ||
+      (r = to_float(a, &g.svc, gd->sidevec_c, (size_t)nip * NPP * 2)) ||
+      (r = to_float(a, &g.sve, gd->sidevec_e, (size_t)nip * NPP * 2)) ||
+      (r = to_float(a, &g.sideln, gd->sideln, (size_t)nip * NPP)) ||
+      (r = to_float(a, &g.hfcoef, gd->hfcoef, (size_t)nip * NPP * NPP)) ||
+      (r = to_float(a, &g.area, gd->area, nip)) || (r = to_float(a, &g.rarea, hg.rarea.data(), nip)) ||
+      (r = to_float(a, &g.corio, hg.corio.data(), nip)) || (r = to_float(a, &g.grndspd, hg.grndspd.data(), nip)) ||
+      (r = to_float(a, &g.dfflen, hg.dfflen.data(), nvl))) {
+    fimhip_ens_destroy(e);
+    return r;
+  }
+  const size_t ncol = (size_t)members * nip;
+  const size_t cz = ncol * nvl, lz = ncol * (nvl + 1), ez = ncol * NPP * nvl;
+  EnsState &s = e->s;
+  s.members = members;
+  s.ncol = (int)ncol;
+  struct { float **p; size_t n; } f32[] = {
+      {&s.us, cz}, {&s.vs, cz}, {&s.u2, cz}, {&s.v2, cz}, {&s.tr, cz * ntra}, {&s.trdp, cz * ntra},
+      {&s.u_tdcy, 3 * cz}, {&s.v_tdcy, 3 * cz}, {&s.dpl_tdcy, 3 * cz}, {&s.trc_tdcy, 3 * cz * ntra},
+      {&s.trl_tdcy, 3 * cz * ntra}, {&s.massfx, ez}, {&s.ws, cz}, {&s.relvor, cz}, {&s.u_edg, ez},
+      {&s.v_edg, ez}, {&s.dp_edg, ez}, {&s.lp_edg, ez}, {&s.uvsq_edg, ez}, {&s.trc_edg, ez * ntra},
+      {&s.antifx, ez}, {&s.delp_lo, cz}, {&s.r_plus, cz}, {&s.r_mnus, cz}, {&s.antiflx, ez * ntra},
+      {&s.trcr_lo, cz * ntra}, {&s.trmax, cz * ntra}, {&s.trmin, cz * ntra}, {&s.tr_rp, cz * ntra},
+      {&s.tr_rm, cz * ntra}};
+  struct { double **p; size_t n; } f64[] = {
+      {&s.dp, cz}, {&s.pr, lz}, {&s.ex, lz}, {&s.ph, lz}, {&s.mp, cz}, {&s.sdot, lz}, {&s.exsmo, lz},
+      {&s.psrf, ncol}, {&s.ptdcy, 2 * ncol}, {&s.dp_tdcy, 3 * cz}, {&s.mont_edg, ez}};
+  for (auto &x : f32)
+    if ((r = a.alloc(x.p, x.n))) { fimhip_ens_destroy(e); return r; }
+  for (auto &x : f64)
+    if ((r = a.alloc(x.p, x.n))) { fimhip_ens_destroy(e); return r; }
+  if ((r = a.alloc(&s.ctl, 1))) { fimhip_ens_destroy(e); return r; }
+  // staging member (FP64, plain layout)
+  const size_t mcz = (size_t)nip * nvl, mlz = (size_t)nip * (nvl + 1);
+  DevState &t = e->stage;
+  struct { double **p; size_t n; } st[] = {{&t.us, mcz}, {&t.vs, mcz}, {&t.dp, mcz}, {&t.pr, mlz},
+                                           {&t.ex, mlz}, {&t.ph, mlz}, {&t.tr, mcz * ntra}};
+  for (auto &x : st)
+    if ((r = a.alloc(x.p, x.n))) { fimhip_ens_destroy(e); return r; }
+  e->gs_blocks = 256;
+  if ((r = a.alloc(&e->gs64, (size_t)members * (ntra + 1) * e->gs_blocks)) ||
+      (r = a.alloc(&e->gs32, 2 * (size_t)members * (ntra + 1) * e->gs_blocks))) {
+    fimhip_ens_destroy(e);
+    return r;
+  }
+  if (hipStreamSynchronize(e->stream) != hipSuccess) { fimhip_ens_destroy(e); return FIMHIP_EHIP; }
+  *out = e;
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_ens_destroy(fimhip_ens *e) {
+  if (!e) return FIMHIP_EINVAL;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_graph(e);
+  delete e->kt;
+  e->arena.release();
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return FIMHIP_OK;
+}
+
+static int check_member(const fimhip_ens *e, int m) { return (!e || m < 0 || m >= e->members) ? FIMHIP_EINVAL : 0; }
+
+extern "C" int fimhip_ens_synth(fimhip_ens *e, int member, unsigned long long seed, int synth_member, int flags) {
+  if (check_member(e, member)) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  int r = synth_into(e->nip, e->nvl, e->ntra, e->lat.data(), e->lon.data(), seed, synth_member, flags, e->stage,
+                     e->stream);
+  if (r) return r;
+  ens_launch_convert_member(e->g, e->s, member, e->stage, e->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_ens_load_member(fimhip_ens *e, int member, const double *us, const double *vs,
+                                      const double *dp, const double *pr, const double *ex, const double *ph,
+                                      const double *tr) {
+  if (check_member(e, member) || !us || !vs || !dp || !pr || !ex || !ph || !tr) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  const size_t cz = (size_t)e->nip * e->nvl, lz = (size_t)e->nip * (e->nvl + 1);
+  const DevState &t = e->stage;
+  HIPCHK(hipMemcpyAsync(t.us, us, cz * 8, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(t.vs, vs, cz * 8, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(t.dp, dp, cz * 8, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(t.pr, pr, lz * 8, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(t.ex, ex, lz * 8, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(t.ph, ph, lz * 8, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipMemcpyAsync(t.tr, tr, cz * e->ntra * 8, hipMemcpyHostToDevice, e->stream));
+  ens_launch_convert_member(e->g, e->s, member, t, e->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return FIMHIP_OK;
+}
+
+static HybCols<float> hyb_cols(fimhip_ens *e) {
+  const EnsState &s = e->s;
+  const DevGrid &d = e->g.d;
+  return HybCols<float>{d.nip, s.ncol, d.nvl, d.ntra, s.u2, s.v2, s.us, s.vs, s.tr, s.dp, s.pr, s.ex, s.sdot,
+                        s.exsmo, d.thetac, d.dpsig, d.smoo, d.sideln, d.rarea, d.prox, d.nprox};
+}
+
+// dyn_init start-up for every member (dyn_init.F90:870-1094): optional
+// hybgen(its=0) and pressure refresh, hystat(0), tracer mass, tendencies
+// cleared, step counter and AB slots nf=0, of=2, vof=1.
+extern "C" int fimhip_ens_init_state(fimhip_ens *e, int init_hybgen) {
+  if (!e) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  EnsState &s = e->s;
+  const size_t cz = (size_t)s.ncol * e->nvl;
+  if (init_hybgen) {
+    // the start-up remap reads the winds from u2/v2
+    HIPCHK(hipMemcpyAsync(s.u2, s.us, cz * 4, hipMemcpyDeviceToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(s.v2, s.vs, cz * 4, hipMemcpyDeviceToDevice, e->stream));
+    launch_hybgen_cols<float>(hyb_cols(e), nullptr, 0, e->stream);
+    ens_launch_pres_from_exner(e->g, s, e->stream);
+  }
+  ens_launch_hystat_init(e->g, s, e->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemsetAsync(s.u_tdcy, 0, 3 * cz * 4, e->stream));
+  HIPCHK(hipMemsetAsync(s.v_tdcy, 0, 3 * cz * 4, e->stream));
+  HIPCHK(hipMemsetAsync(s.dpl_tdcy, 0, 3 * cz * 4, e->stream));
+  HIPCHK(hipMemsetAsync(s.dp_tdcy, 0, 3 * cz * 8, e->stream));
+  HIPCHK(hipMemsetAsync(s.trc_tdcy, 0, 3 * cz * e->ntra * 4, e->stream));
+  HIPCHK(hipMemsetAsync(s.trl_tdcy, 0, 3 * cz * e->ntra * 4, e->stream));
+  HIPCHK(hipMemsetAsync(s.massfx, 0, cz * NPP * 4, e->stream));
+  StepCtl c{0, 0, 2, 1, 0., 0., 0.};
+  HIPCHK(hipMemcpyAsync(s.ctl, &c, sizeof c, hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return FIMHIP_OK;
+}
+
+// one step's launches (DYN_1 then DYN_2, dyn_run.F90:119-425)
+static void enqueue_step(fimhip_ens *e, int mode) {
+  const EnsGrid &g = e->g;
+  const EnsState &s = e->s;
+  hipStream_t st = e->stream;
+  ens_launch_abstart(s, e->dt, st);
+  for (int k = EK_EDGVAR1; k <= EK_TRCADV3; k++) ens_launch(k, e->tile[k], g, s, st);
+  if (mode == 0) {
+    ens_launch(EK_MOMTUM, e->tile[EK_MOMTUM], g, s, st);
+    launch_hybgen_cols<float>(hyb_cols(e), s.ctl, 0, st);
+    ens_launch(EK_HYSTAT, e->tile[EK_HYSTAT], g, s, st);
+  }
+}
+
+extern "C" int fimhip_ens_step(fimhip_ens *e, int nsteps, int mode) {
+  if (!e || nsteps < 0 || (mode != 0 && mode != 1)) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  if (!e->use_graph || e->kt) {
+    g_ktimer = e->kt;
+    for (int i = 0; i < nsteps; i++) enqueue_step(e, mode);
+    g_ktimer = nullptr;
+    HIPCHK(hipGetLastError());
+    if (e->kt && e->kt->flush(e->stream)) return FIMHIP_EHIP;
+    return FIMHIP_OK;
+  }
+  if (e->graph_mode != mode) {
+    drop_graph(e);
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(e, mode);
+    HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
+    HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    e->graph_mode = mode;
+  }
+  for (int i = 0; i < nsteps; i++) HIPCHK(hipGraphLaunch(e->exec, e->stream));
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_ens_use_graph(fimhip_ens *e, int on) {
+  if (!e) return FIMHIP_EINVAL;
+  e->use_graph = on != 0;
+  if (!e->use_graph) drop_graph(e);
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_ens_sync(fimhip_ens *e) {
+  if (!e) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return FIMHIP_OK;
+}
+
+// element count of one member's field
+extern "C" long long fimhip_ens_field_size(const fimhip_ens *e, int field) {
+  if (!e) return FIMHIP_EINVAL;
+  const long long cz = (long long)e->nip * e->nvl, lz = (long long)e->nip * (e->nvl + 1);
+  switch (field) {
+    case FIMHIP_US: case FIMHIP_VS: case FIMHIP_DP: case FIMHIP_MP: case FIMHIP_WS: case FIMHIP_RELVOR: return cz;
+    case FIMHIP_PR: case FIMHIP_EX: case FIMHIP_PH: case FIMHIP_SDOT: return lz;
+    case FIMHIP_TR: case FIMHIP_TRDP: return cz * e->ntra;
+    case FIMHIP_PSRF: return e->nip;
+    default: return FIMHIP_EINVAL;
+  }
+}
+
+template <class T>
+static int fetch(fimhip_ens *e, const T *src, size_t n, double *dst) {
+  std::vector<T> h(n);
+  HIPCHK(hipMemcpyAsync(h.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  for (size_t i = 0; i < n; i++) dst[i] = (double)h[i];
+  return 0;
+}
+
+// one member's field, widened to FP64, in the plain (reference) layout
+extern "C" int fimhip_ens_download(fimhip_ens *e, int member, int field, double *host) {
+  if (check_member(e, member) || !host) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  const size_t cz = (size_t)e->nip * e->nvl, lz = (size_t)e->nip * (e->nvl + 1);
+  const size_t oc = member * cz, ol = member * lz, ecz = (size_t)e->s.ncol * e->nvl;
+  const EnsState &s = e->s;
+  switch (field) {
+    case FIMHIP_US: return fetch(e, s.us + oc, cz, host);
+    case FIMHIP_VS: return fetch(e, s.vs + oc, cz, host);
+    case FIMHIP_DP: return fetch(e, s.dp + oc, cz, host);
+    case FIMHIP_MP: return fetch(e, s.mp + oc, cz, host);
+    case FIMHIP_WS: return fetch(e, s.ws + oc, cz, host);
+    case FIMHIP_RELVOR: return fetch(e, s.relvor + oc, cz, host);
+    case FIMHIP_PR: return fetch(e, s.pr + ol, lz, host);
+    case FIMHIP_EX: return fetch(e, s.ex + ol, lz, host);
+    case FIMHIP_PH: return fetch(e, s.ph + ol, lz, host);
+    case FIMHIP_SDOT: return fetch(e, s.sdot + ol, lz, host);
+    case FIMHIP_PSRF: return fetch(e, s.psrf + (size_t)member * e->nip, e->nip, host);
+    case FIMHIP_TR:
+    case FIMHIP_TRDP: {
+      const float *b = field == FIMHIP_TR ? s.tr : s.trdp;
+      for (int t = 0; t < e->ntra; t++) {
+        int r = fetch(e, b + t * ecz + oc, cz, host + t * cz);
+        if (r) return r;
+      }
+      return 0;
+    }
+    default: return FIMHIP_EINVAL;
+  }
+}
+
+// Global mass (sum area*dp/g) and tracer totals (sum area*dp*tr) of every
+// member (globsum.F90:70-101): mass[m], trtot[m*ntra + t] from FP64 sums;
+// mass32/trtot32 (optional) from FP32 Kahan-compensated sums of the same
+// products, the block partials combined in compensated FP32 pairs on the host.
+extern "C" int fimhip_ens_globsum(fimhip_ens *e, double *mass, double *trtot, double *mass32, double *trtot32) {
+  if (!e || !mass) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  ens_launch_globsum(e->g, e->s, e->gs64, e->gs32, e->gs_blocks, e->stream);
+  HIPCHK(hipGetLastError());
+  const size_t n = (size_t)e->members * (e->ntra + 1) * e->gs_blocks;
+  std::vector<double> h64(n);
+  std::vector<float> h32(2 * n);
+  HIPCHK(hipMemcpyAsync(h64.data(), e->gs64, n * 8, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipMemcpyAsync(h32.data(), e->gs32, 2 * n * 4, hipMemcpyDeviceToHost, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  const int nb = e->gs_blocks, nq = e->ntra + 1;
+  for (int m = 0; m < e->members; m++)
+    for (int q = 0; q < nq; q++) {
+      const size_t o = ((size_t)m * nq + q) * nb;
+      double acc = 0.;
+      volatile float ss = 0.f, cc = 0.f;   // compensated FP32 pair
+      for (int b = 0; b < nb; b++) {
+        acc += h64[o + b];
+        const float bs = h32[2 * (o + b)], bc = h32[2 * (o + b) + 1];
+        const float sum = ss + bs;
+        const float bb = sum - ss;
+        const float err = (ss - (sum - bb)) + (bs - bb);
+        cc = cc + bc - err;
+        ss = sum;
+      }
+      const double k32 = (double)ss - (double)cc;
+      if (q == 0) {
+        mass[m] = acc / GRVITY;
+        if (mass32) mass32[m] = k32 / GRVITY;
+      } else {
+        if (trtot) trtot[(size_t)m * e->ntra + q - 1] = acc;
+        if (trtot32) trtot32[(size_t)m * e->ntra + q - 1] = k32;
+      }
+    }
+  return FIMHIP_OK;
+}
+
+// ---- launch shapes -------------------------------------------------------
+extern "C" int fimhip_ens_kernel_count(void) { return EK_NKERNELS; }
+extern "C" const char *fimhip_ens_kernel_name(int k) {
+  return (k >= 0 && k < EK_NKERNELS) ? ens_kernel_names[k] : "";
+}
+extern "C" int fimhip_ens_set_tile(fimhip_ens *e, int kernel, int tile) {
+  if (!e || kernel < 0 || kernel >= EK_NKERNELS || tile < 0 || tile >= ENS_NTILES) return FIMHIP_EINVAL;
+  if (e->tile[kernel] != tile) drop_graph(e);
+  e->tile[kernel] = tile;
+  return FIMHIP_OK;
+}
+extern "C" int fimhip_ens_get_tile(const fimhip_ens *e, int kernel) {
+  if (!e || kernel < 0 || kernel >= EK_NKERNELS) return FIMHIP_EINVAL;
+  return e->tile[kernel];
+}
+
+// Autotuner: for each launch shape in turn, every kernel runs with that
+// shape for 'reps' steps of the given mode under event timing; each kernel
+// then keeps the shape with the least mean time.  The steps advance the
+// state, so callers re-create or reload it afterwards.  times (optional)
+// receives EK_NKERNELS x ENS_NTILES mean milliseconds.
+extern "C" int fimhip_ens_autotune(fimhip_ens *e, int mode, int reps, double *times) {
+  if (!e || reps < 1 || (mode != 0 && mode != 1)) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  std::vector<double> best(EK_NKERNELS, 1e30);
+  std::vector<int> pick(EK_NKERNELS, e->tile[0]);
+  for (int k = 0; k < EK_NKERNELS; k++) pick[k] = e->tile[k];
+  KTimer *saved = e->kt;
+  for (int t = 0; t < ENS_NTILES; t++) {
+    for (int k = 0; k < EK_NKERNELS; k++) e->tile[k] = t;
+    KTimer kt;
+    g_ktimer = &kt;
+    for (int i = 0; i < reps; i++) enqueue_step(e, mode);
+    g_ktimer = nullptr;
+    HIPCHK(hipGetLastError());
+    if (kt.flush(e->stream)) return FIMHIP_EHIP;
+    for (int k = 0; k < EK_NKERNELS; k++) {
+      auto it = kt.acc.find(ens_kernel_names[k]);
+      if (it == kt.acc.end() || it->second.second == 0) continue;
+      const double ms = it->second.first / it->second.second;
+      if (times) times[k * ENS_NTILES + t] = ms;
+      if (ms < best[k]) { best[k] = ms; pick[k] = t; }
+    }
+  }
+  e->kt = saved;
+  for (int k = 0; k < EK_NKERNELS; k++) e->tile[k] = pick[k];
+  drop_graph(e);
+  return FIMHIP_OK;
+}
+
+// ---- event timing ----------------------------------------------------------
+extern "C" int fimhip_ens_profile(fimhip_ens *e, int on) {
+  if (!e) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  delete e->kt;
+  e->kt = on ? new KTimer() : nullptr;
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_ens_profile_report(fimhip_ens *e, char *buf, int len) {
+  if (!e || !buf || len < 1) return FIMHIP_EINVAL;
+  std::string out;
+  if (e->kt)
+    for (auto &kv : e->kt->acc) {
+      char line[160];
+      snprintf(line, sizeof line, "%s %ld %.6f\n", kv.first.c_str(), kv.second.second, kv.second.first);
+      out += line;
+    }
+  if ((int)out.size() >= len) return FIMHIP_EINVAL;
+  memcpy(buf, out.c_str(), out.size() + 1);
+  return FIMHIP_OK;
+}

@@ -14,43 +14,13 @@
 #include "fimhip_internal.h"
 #include "fim_pow.h"
 #include "synth.h"
+#include "host_common.h"
 
 using namespace fimhip;
 
 // ---------------------------------------------------------------------------
 // per-kernel event timing
 namespace fimhip {
-struct KTimer {
-  struct Rec { const char *name; hipEvent_t a, b; };
-  std::vector<Rec> recs;
-  std::vector<hipEvent_t> pool;
-  std::map<std::string, std::pair<double, long>> acc;
-  hipEvent_t get() {
-    hipEvent_t e = nullptr;
-    if (!pool.empty()) { e = pool.back(); pool.pop_back(); }
-    else if (hipEventCreate(&e) != hipSuccess) e = nullptr;
-    return e;
-  }
-  int flush(hipStream_t st) {
-    if (hipStreamSynchronize(st) != hipSuccess) return FIMHIP_EHIP;
-    for (auto &r : recs) {
-      float ms = 0.f;
-      if (r.a && r.b && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
-        auto &v = acc[r.name];
-        v.first += ms;
-        v.second += 1;
-      }
-      if (r.a) pool.push_back(r.a);
-      if (r.b) pool.push_back(r.b);
-    }
-    recs.clear();
-    return 0;
-  }
-  ~KTimer() {
-    for (auto &r : recs) { if (r.a) (void)hipEventDestroy(r.a); if (r.b) (void)hipEventDestroy(r.b); }
-    for (auto e : pool) (void)hipEventDestroy(e);
-  }
-};
 thread_local KTimer *g_ktimer = nullptr;
 void kt_begin(const char *name, hipStream_t st) {
   hipEvent_t e = g_ktimer->get();
@@ -72,49 +42,83 @@ struct fimhip_ctx {
   DevGrid g{};
   DevState s{};
   AB ab{0, 2, 1, 0., 0., 0.};
-  std::vector<void *> allocs;
+  Arena arena;
   double *gsum_partial = nullptr;   // globsum block partials
   int gsum_blocks = 0;
   std::vector<double> lat, lon;      // host copies for the synthetic-state profiles
   KTimer *kt = nullptr;              // non-null while profiling
 };
 
-#define HIPCHK(x)                                                    \
-  do {                                                               \
-    hipError_t e_ = (x);                                             \
-    if (e_ != hipSuccess) {                                          \
-      fprintf(stderr, "fimhip: %s failed: %s\n", #x, hipGetErrorString(e_)); \
-      return FIMHIP_EHIP;                                            \
-    }                                                                \
-  } while (0)
-
-static int dalloc(fimhip_ctx *c, void **p, size_t bytes) {
-  if (bytes == 0) bytes = 8;
-  if (hipMalloc(p, bytes) != hipSuccess) return FIMHIP_ENOMEM;
-  if (hipMemsetAsync(*p, 0, bytes, c->stream) != hipSuccess) return FIMHIP_EHIP;
-  c->allocs.push_back(*p);
-  return 0;
-}
-
-template <class T>
-static int dalloc_t(fimhip_ctx *c, T **p, size_t n) {
-  return dalloc(c, (void **)p, n * sizeof(T));
-}
-
-template <class T>
-static int upload_t(fimhip_ctx *c, T **p, const std::vector<T> &h) {
-  int r = dalloc_t(c, p, h.size());
-  if (r) return r;
-  if (hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream) != hipSuccess)
-    return FIMHIP_EHIP;
-  return 0;
-}
-
 // dyn_init.F90:125 taper for smoo_coeff
 static double taper_smoo(int k1, int k2) {
   double d = (double)(k2 - k1);
   return d * d / (9. + d * d);
 }
+
+namespace fimhip {
+int host_grid_prepare(const fimhip_grid_desc *gd, HostGrid &h) {
+  const int nip = gd->nip, nvl = gd->nvl;
+  h.prox.resize((size_t)nip * NPP);
+  h.proxs.resize((size_t)nip * NPP);
+  h.nprox.resize(nip);
+  for (int i = 0; i < nip; i++) {
+    int n = gd->nprox[i];
+    if (n != 5 && n != 6) return FIMHIP_EINVAL;
+    h.nprox[i] = n;
+    for (int e = 0; e < NPP; e++) {
+      int ee = (e < n) ? e : n - 1;
+      int p = gd->prox[i * NPP + ee] - 1, q = gd->proxs[i * NPP + ee] - 1;
+      if (p < 0 || p >= nip || q < 0 || q >= NPP) return FIMHIP_EINVAL;
+      h.prox[i * NPP + e] = p;
+      h.proxs[i * NPP + e] = q;
+    }
+  }
+  h.rarea.resize(nip); h.corio.resize(nip); h.grndspd.resize(nip);
+  h.dfflen.resize(nvl); h.smoo.resize(nvl);
+  for (int i = 0; i < nip; i++) {          // dyn_init.F90:753-757
+    h.corio[i] = OMEGX2 * sin(gd->lat[i]);
+    h.rarea[i] = 1. / gd->area[i];
+    h.grndspd[i] = EARTHRAD * 2. * PI_T * cos(gd->lat[i]) / 86164.;
+  }
+  int ndamp = (int)(0.1 * nvl);            // dyn_init.F90:309-321
+  for (int k = 1; k <= nvl; k++) {
+    double wgt = (double)(k - nvl + ndamp) / (double)ndamp;
+    wgt = wgt > 0. ? wgt : 0.;
+    h.dfflen[k - 1] =
+        gd->dt * (VELDFF_BKGND * (1. - wgt) + (VELDFF_BKGND > VELDFF_BOOST ? VELDFF_BKGND : VELDFF_BOOST) * wgt);
+    h.smoo[k - 1] = 0.;
+  }
+  for (int k = KBL + 2; k <= nvl; k++)
+    h.smoo[k - 1] = gd->dt * INTFC_SMOOTH * taper_smoo(k, KBL + 1) * taper_smoo(k, nvl + 1);
+  return 0;
+}
+
+int dev_grid_upload(Arena &a, const fimhip_grid_desc *gd, const HostGrid &h, DevGrid &g) {
+  const int nip = gd->nip, nvl = gd->nvl;
+  auto vec = [](const double *p, size_t n) { return std::vector<double>(p, p + n); };
+  g.nip = nip; g.nvl = nvl; g.ntra = gd->ntra; g.dt = gd->dt;
+  int *dprox, *dproxs, *dnprox;
+  double *dcs, *dsn, *dsvc, *dsve, *dsideln, *dhf, *darea, *drarea, *dcorio, *dgrnd, *ddff, *dsmoo, *dth, *dds;
+  int r = 0;
+  if ((r = a.upload(&dprox, h.prox)) || (r = a.upload(&dproxs, h.proxs)) || (r = a.upload(&dnprox, h.nprox)) ||
+      (r = a.upload(&dcs, vec(gd->cs, (size_t)nip * NPP * 4))) ||
+      (r = a.upload(&dsn, vec(gd->sn, (size_t)nip * NPP * 4))) ||
+      (r = a.upload(&dsvc, vec(gd->sidevec_c, (size_t)nip * NPP * 2))) ||
+      (r = a.upload(&dsve, vec(gd->sidevec_e, (size_t)nip * NPP * 2))) ||
+      (r = a.upload(&dsideln, vec(gd->sideln, (size_t)nip * NPP))) ||
+      (r = a.upload(&dhf, vec(gd->hfcoef, (size_t)nip * NPP * NPP))) ||
+      (r = a.upload(&darea, vec(gd->area, nip))) || (r = a.upload(&drarea, h.rarea)) ||
+      (r = a.upload(&dcorio, h.corio)) || (r = a.upload(&dgrnd, h.grndspd)) ||
+      (r = a.upload(&ddff, h.dfflen)) || (r = a.upload(&dsmoo, h.smoo)) ||
+      (r = a.upload(&dth, vec(gd->thetac, nvl))) || (r = a.upload(&dds, vec(gd->dpsig, nvl))))
+    return r;
+  g.prox = dprox; g.proxs = dproxs; g.nprox = dnprox;
+  g.cs = dcs; g.sn = dsn; g.svc = dsvc; g.sve = dsve; g.sideln = dsideln; g.hfcoef = dhf;
+  g.area = darea; g.rarea = drarea; g.corio = dcorio; g.grndspd = dgrnd;
+  g.dfflen = ddff; g.smoo = dsmoo; g.thetac = dth; g.dpsig = dds;
+  return 0;
+}
+}  // namespace fimhip
 
 extern "C" int fimhip_create(const fimhip_grid_desc *gd, int device, fimhip_ctx **out) {
   if (!gd || !out) return FIMHIP_EINVAL;
@@ -126,6 +130,9 @@ extern "C" int fimhip_create(const fimhip_grid_desc *gd, int device, fimhip_ctx 
     return FIMHIP_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return FIMHIP_ENODEV;
+  HostGrid hg;
+  int r = host_grid_prepare(gd, hg);
+  if (r) return r;
   fimhip_ctx *c = new fimhip_ctx();
   c->device = device;
   c->nip = nip; c->nvl = nvl; c->ntra = ntra; c->dt = gd->dt;
@@ -135,61 +142,8 @@ extern "C" int fimhip_create(const fimhip_grid_desc *gd, int device, fimhip_ctx 
     delete c;
     return FIMHIP_EHIP;
   }
-  int r = 0;
-  // ---- grid tables, 0-based, pentagon slot 6 = slot 5 (dyn_init.F90:546-548)
-  std::vector<int> prox(nip * NPP), proxs(nip * NPP), nprox(nip);
-  for (int i = 0; i < nip; i++) {
-    int n = gd->nprox[i];
-    if (n != 5 && n != 6) { fimhip_destroy(c); return FIMHIP_EINVAL; }
-    nprox[i] = n;
-    for (int e = 0; e < NPP; e++) {
-      int ee = (e < n) ? e : n - 1;
-      int p = gd->prox[i * NPP + ee] - 1, q = gd->proxs[i * NPP + ee] - 1;
-      if (p < 0 || p >= nip || q < 0 || q >= NPP) { fimhip_destroy(c); return FIMHIP_EINVAL; }
-      prox[i * NPP + e] = p;
-      proxs[i * NPP + e] = q;
-    }
-  }
-  auto vec = [](const double *p, size_t n) { return std::vector<double>(p, p + n); };
-  std::vector<double> rarea(nip), corio(nip), grndspd(nip), dfflen(nvl), smoo(nvl);
-  for (int i = 0; i < nip; i++) {          // dyn_init.F90:753-757
-    corio[i] = OMEGX2 * sin(gd->lat[i]);
-    rarea[i] = 1. / gd->area[i];
-    grndspd[i] = EARTHRAD * 2. * PI_T * cos(gd->lat[i]) / 86164.;
-  }
-  {                                        // dyn_init.F90:309-321
-    int ndamp = (int)(0.1 * nvl);
-    for (int k = 1; k <= nvl; k++) {
-      double wgt = (double)(k - nvl + ndamp) / (double)ndamp;
-      wgt = wgt > 0. ? wgt : 0.;
-      dfflen[k - 1] = gd->dt * (VELDFF_BKGND * (1. - wgt) + (VELDFF_BKGND > VELDFF_BOOST ? VELDFF_BKGND : VELDFF_BOOST) * wgt);
-      smoo[k - 1] = 0.;
-    }
-    for (int k = KBL + 2; k <= nvl; k++)
-      smoo[k - 1] = gd->dt * INTFC_SMOOTH * taper_smoo(k, KBL + 1) * taper_smoo(k, nvl + 1);
-  }
-  DevGrid &g = c->g;
-  g.nip = nip; g.nvl = nvl; g.ntra = ntra; g.dt = gd->dt;
-  int *dprox, *dproxs, *dnprox;
-  double *dcs, *dsn, *dsvc, *dsve, *dsideln, *dhf, *darea, *drarea, *dcorio, *dgrnd, *ddff, *dsmoo, *dth, *dds;
-  if ((r = upload_t(c, &dprox, prox)) || (r = upload_t(c, &dproxs, proxs)) || (r = upload_t(c, &dnprox, nprox)) ||
-      (r = upload_t(c, &dcs, vec(gd->cs, (size_t)nip * NPP * 4))) ||
-      (r = upload_t(c, &dsn, vec(gd->sn, (size_t)nip * NPP * 4))) ||
-      (r = upload_t(c, &dsvc, vec(gd->sidevec_c, (size_t)nip * NPP * 2))) ||
-      (r = upload_t(c, &dsve, vec(gd->sidevec_e, (size_t)nip * NPP * 2))) ||
-      (r = upload_t(c, &dsideln, vec(gd->sideln, (size_t)nip * NPP))) ||
-      (r = upload_t(c, &dhf, vec(gd->hfcoef, (size_t)nip * NPP * NPP))) ||
-      (r = upload_t(c, &darea, vec(gd->area, nip))) || (r = upload_t(c, &drarea, rarea)) ||
-      (r = upload_t(c, &dcorio, corio)) || (r = upload_t(c, &dgrnd, grndspd)) ||
-      (r = upload_t(c, &ddff, dfflen)) || (r = upload_t(c, &dsmoo, smoo)) ||
-      (r = upload_t(c, &dth, vec(gd->thetac, nvl))) || (r = upload_t(c, &dds, vec(gd->dpsig, nvl)))) {
-    fimhip_destroy(c);
-    return r;
-  }
-  g.prox = dprox; g.proxs = dproxs; g.nprox = dnprox;
-  g.cs = dcs; g.sn = dsn; g.svc = dsvc; g.sve = dsve; g.sideln = dsideln; g.hfcoef = dhf;
-  g.area = darea; g.rarea = drarea; g.corio = dcorio; g.grndspd = dgrnd;
-  g.dfflen = ddff; g.smoo = dsmoo; g.thetac = dth; g.dpsig = dds;
+  c->arena.stream = c->stream;
+  if ((r = dev_grid_upload(c->arena, gd, hg, c->g))) { fimhip_destroy(c); return r; }
   // ---- state and work arrays
   const size_t cz = (size_t)nip * nvl, lz = (size_t)nip * (nvl + 1), ez = (size_t)nip * NPP * nvl;
   DevState &s = c->s;
@@ -204,9 +158,9 @@ extern "C" int fimhip_create(const fimhip_grid_desc *gd, int device, fimhip_ctx 
       {&s.trmax, cz * ntra}, {&s.trmin, cz * ntra}, {&s.tr_rp, cz * ntra}, {&s.tr_rm, cz * ntra},
       {&s.u2, cz}, {&s.v2, cz}, {&s.exsmo, lz}};
   for (auto &a : arrs)
-    if ((r = dalloc_t(c, a.p, a.n))) { fimhip_destroy(c); return r; }
+    if ((r = c->arena.alloc(a.p, a.n))) { fimhip_destroy(c); return r; }
   c->gsum_blocks = 1024;
-  if ((r = dalloc_t(c, &c->gsum_partial, (size_t)c->gsum_blocks * (1 + ntra)))) { fimhip_destroy(c); return r; }
+  if ((r = c->arena.alloc(&c->gsum_partial, (size_t)c->gsum_blocks * (1 + ntra)))) { fimhip_destroy(c); return r; }
   if (hipStreamSynchronize(c->stream) != hipSuccess) { fimhip_destroy(c); return FIMHIP_EHIP; }
   *out = c;
   return FIMHIP_OK;
@@ -214,11 +168,11 @@ extern "C" int fimhip_create(const fimhip_grid_desc *gd, int device, fimhip_ctx 
 
 extern "C" int fimhip_destroy(fimhip_ctx *c) {
   if (!c) return FIMHIP_EINVAL;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   delete c->kt;
-  for (void *p : c->allocs) (void)hipFree(p);
-  if (c->stream) hipStreamDestroy(c->stream);
+  c->arena.release();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return FIMHIP_OK;
 }
@@ -606,13 +560,14 @@ __global__ void __launch_bounds__(64) k_synth(fim_synth_params p, const double *
                    s.ph + (size_t)i * (nvl + 1), s.tr + (size_t)i * nvl, (size_t)p.nip * nvl);
 }
 
-extern "C" int fimhip_synth_device(fimhip_ctx *c, unsigned long long seed, int member, int flags) {
-  if (!c) return FIMHIP_EINVAL;
-  HIPCHK(hipSetDevice(c->device));
+namespace fimhip {
+// one member's seeded state written into dst (plain FP64 layout) on st
+int synth_into(int nip, int nvl, int ntra, const double *lat, const double *lon, unsigned long long seed,
+               int member, int flags, const DevState &dst, hipStream_t st) {
   std::vector<double> u0, v0, bell, pint;
-  synth_profiles(c->nip, c->nvl, c->lat.data(), c->lon.data(), u0, v0, bell, pint);
+  synth_profiles(nip, nvl, lat, lon, u0, v0, bell, pint);
   std::vector<double> packed;
-  packed.reserve(3 * c->nip + c->nvl + 1);
+  packed.reserve(3 * (size_t)nip + nvl + 1);
   packed.insert(packed.end(), u0.begin(), u0.end());
   packed.insert(packed.end(), v0.begin(), v0.end());
   packed.insert(packed.end(), bell.begin(), bell.end());
@@ -621,14 +576,21 @@ extern "C" int fimhip_synth_device(fimhip_ctx *c, unsigned long long seed, int m
   HIPCHK(hipMalloc(&d, packed.size() * sizeof(double)));
   hipError_t e = hipMemcpy(d, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    fim_synth_params p{c->nip, c->nvl, c->ntra, (uint64_t)seed, member, flags & 1, (flags >> 1) & 1, (flags >> 2) & 1};
-    k_synth<<<(c->nip + 63) / 64, 64, 0, c->stream>>>(p, d, d + c->nip, d + 2 * c->nip, d + 3 * c->nip, c->s);
+    fim_synth_params p{nip, nvl, ntra, (uint64_t)seed, member, flags & 1, (flags >> 1) & 1, (flags >> 2) & 1};
+    k_synth<<<(nip + 63) / 64, 64, 0, st>>>(p, d, d + nip, d + 2 * (size_t)nip, d + 3 * (size_t)nip, dst);
     e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
   }
   (void)hipFree(d);
   HIPCHK(e);
   return FIMHIP_OK;
+}
+}  // namespace fimhip
+
+extern "C" int fimhip_synth_device(fimhip_ctx *c, unsigned long long seed, int member, int flags) {
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  return synth_into(c->nip, c->nvl, c->ntra, c->lat.data(), c->lon.data(), seed, member, flags, c->s, c->stream);
 }
 
 extern "C" int fimhip_profile(fimhip_ctx *c, int on) {

@@ -37,6 +37,16 @@ constexpr double INTFC_SMOOTH = 30., PTOP = 30., THKTOP = 30.;
 constexpr double RLEIGH = 2., RLTHRESH = 90., RLTAPER = 0.4, SLAK = 0.5;
 constexpr double VELDFF_BKGND = 0.3, VELDFF_BOOST = 3.;
 
+// Read-only table access through the constant address space: with a
+// wave-uniform index the compiler issues a scalar load even when the kernel
+// stores to other arrays in between (a load through a generic pointer after a
+// store must otherwise be a vector load, since the store might alias it).
+// Only for arrays no kernel writes while it runs (grid tables, step control).
+template <typename T>
+__device__ __forceinline__ T ldc(const T *p, size_t i) {
+  return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
 struct DevGrid {
   int nip, nvl, ntra;
   double dt;
@@ -62,6 +72,37 @@ struct AB {
   double ab1, ab2, ab3;
 };
 
+// Step control kept in device memory for the batched form: a one-thread
+// kernel advances it at the start of each step (abstart.F90:38-66), and the
+// step's kernels read the slots and weights from it, so one captured graph
+// serves every step.
+struct StepCtl {
+  int its, nf, of, vof;   // 1-based slots
+  double ab1, ab2, ab3;
+};
+__device__ __forceinline__ StepCtl load_ctl(const StepCtl *p) {
+  const __attribute__((address_space(4))) StepCtl *q = (const __attribute__((address_space(4))) StepCtl *)p;
+  StepCtl c;
+  c.its = q->its; c.nf = q->nf; c.of = q->of; c.vof = q->vof;
+  c.ab1 = q->ab1; c.ab2 = q->ab2; c.ab3 = q->ab3;
+  return c;
+}
+
+// The columns hybgen works on: every column of every member (ncol = members *
+// nip, column = member * nip + cell), winds and tracers of value type V,
+// thickness, pressure and Exner function in FP64.
+template <typename V>
+struct HybCols {
+  int nip, ncol, nvl, ntra;
+  const V *us_in, *vs_in;   // winds remapped by remap_1d (may be the output arrays)
+  V *us, *vs, *tr;
+  double *dp, *pr, *ex, *sdot, *exsmo;
+  const double *thetac, *dpsig, *smoo, *sideln, *rarea;
+  const int *prox, *nprox;
+};
+template <typename V>
+void launch_hybgen_cols(const HybCols<V> &h, const StepCtl *ctl, int its, hipStream_t st);
+
 // Per-kernel HIP-event timing: when g_ktimer is set (by the API for an
 // instance with profiling on), every launch is bracketed by events on its stream.
 struct KTimer;
@@ -85,5 +126,8 @@ void launch_hybgen(const DevGrid &g, const DevState &s, int its, hipStream_t st)
 void launch_hystat(const DevGrid &g, const DevState &s, int its, hipStream_t st);
 void launch_pres_from_exner(const DevGrid &g, const DevState &s, hipStream_t st);
 void launch_trdp(const DevGrid &g, const DevState &s, hipStream_t st);
+// seeded synthetic state of one member into dst (fimhip_api.hip)
+int synth_into(int nip, int nvl, int ntra, const double *lat, const double *lon, unsigned long long seed,
+               int member, int flags, const DevState &dst, hipStream_t st);
 
 }  // namespace fimhip

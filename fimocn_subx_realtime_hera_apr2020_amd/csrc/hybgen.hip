@@ -12,10 +12,11 @@
 
 namespace fimhip {
 
-#define C2(a, ipn, k) (a)[(size_t)(ipn) * nvl + (k)]
-#define L2(a, ipn, k) (a)[(size_t)(ipn) * nvlp1 + (k)]
-#define TR(a, t, ipn, k) (a)[((size_t)(t) * nip + (ipn)) * nvl + (k)]
-#define PROX(ipn, e) g.prox[(ipn) * NPP + (e)]
+// Column indices run over every column of every member (ncol = members * nip,
+// column c = member * nip + cell); tracers are (nvl, ncol, ntra).
+#define C2(a, col, k) (a)[(size_t)(col) * nvl + (k)]
+#define L2(a, col, k) (a)[(size_t)(col) * nvlp1 + (k)]
+#define TR(a, t, col, k) (a)[((size_t)(t) * ncol + (col)) * nvl + (k)]
 
 constexpr int LMAX = NVL_MAX + 1;
 constexpr int NTRA_MAX = 8;
@@ -127,14 +128,16 @@ __device__ void equilb(const double *thet, double *pnew, int kk, double slak) {
 }
 
 // regrid_1d for one column: thcol (theta, modified), excol (Exner, in/out)
-__global__ void __launch_bounds__(64) k_regrid(DevGrid g, DevState s, int its) {
-  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
+template <typename V>
+__global__ void __launch_bounds__(64) k_regrid(HybCols<V> s, const StepCtl *ctl, int its) {
+  const int ncol = s.ncol, nvl = s.nvl, nvlp1 = nvl + 1;
   const int ipn = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ipn >= nip) return;
+  if (ipn >= ncol) return;
+  if (ctl) its = ctl->its;
   double thcol[NVL_MAX], excol[LMAX], exwrk[LMAX], exnew[LMAX], thnew[NVL_MAX], heatfx[NVL_MAX];
-  const double *targt = g.thetac;
+  const double *targt = s.thetac;
   const double dffudt = .1, tolrnce = 0.001;
-  for (int k = 0; k < nvl; k++) thcol[k] = TR(s.tr, 0, ipn, k);
+  for (int k = 0; k < nvl; k++) thcol[k] = (double)TR(s.tr, 0, ipn, k);
   for (int k = 0; k < nvlp1; k++) excol[k] = L2(s.ex, ipn, k);
   const double psurf = L2(s.pr, ipn, 0);
   double eqlb_slak = .1 * (double)its / (10. + (double)its);
@@ -182,30 +185,32 @@ __global__ void __launch_bounds__(64) k_regrid(DevGrid g, DevState s, int its) {
   }
   (void)dffudt;
   equilb(thnew, exnew, nvl, eqlb_slak);
-  inflate(g.dpsig, nvl, psurf, exnew);
+  inflate(s.dpsig, nvl, psurf, exnew);
   for (int k = 0; k < nvlp1; k++) L2(s.sdot, ipn, k) = exnew[k];   // pre-smoothing Exner, kept in sdot
 }
 
 // dffusn_lev(exsmo, smoo_coeff, nvlp1, kbl+2, nvl): reads the regridded Exner
 // function from sdot and writes the smoothed one to exsmo
-__global__ void __launch_bounds__(256) k_dffusn_lev(DevGrid g, DevState s) {
-  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
-  const int ipn = blockIdx.x * 4 + threadIdx.y;
-  if (ipn >= nip) return;
+template <typename V>
+__global__ void __launch_bounds__(256) k_dffusn_lev(HybCols<V> s) {
+  const int nvl = s.nvl, nvlp1 = nvl + 1, nip = s.nip;
+  const int col = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + threadIdx.y));
+  if (col >= s.ncol) return;
+  const int m = col / nip, ipn = col - m * nip;
   const int k1 = KBL + 2, k2 = nvl;
   for (int kk = threadIdx.x; kk < nvlp1; kk += blockDim.x) {
-    const double f = L2(s.sdot, ipn, kk);
+    const double f = L2(s.sdot, col, kk);
     double out = f;
     const int k = kk + 1;
     if (k >= k1 && k <= k2) {
       double flx = 0.;
-      const int np = g.nprox[ipn];
+      const int np = ldc(s.nprox, ipn);
       for (int edg = 0; edg < np; edg++)
-        flx = flx + (f - L2(s.sdot, PROX(ipn, edg), kk)) * g.sideln[ipn * NPP + edg];
-      double factor = -g.smoo[kk] * g.rarea[ipn];
+        flx = flx + (f - L2(s.sdot, m * nip + ldc(s.prox, ipn * NPP + edg), kk)) * ldc(s.sideln, ipn * NPP + edg);
+      double factor = -s.smoo[kk] * ldc(s.rarea, ipn);
       out = f + flx * factor;
     }
-    L2(s.exsmo, ipn, kk) = out;
+    L2(s.exsmo, col, kk) = out;
   }
 }
 
@@ -214,7 +219,8 @@ __global__ void __launch_bounds__(256) k_dffusn_lev(DevGrid g, DevState s) {
 // formed on the fly from xold/xnew (the reference's zold/znew are +-x exactly),
 // and the slopes and interface fluxes are formed in one forward sweep, so the
 // only column array is the filled field itself.
-__device__ void pcwise(int kk, const double *xold, const double *xnew, double *fld, double *ynew) {
+template <typename Y>
+__device__ void pcwise(int kk, const double *xold, const double *xnew, double *fld, Y *ynew) {
   const double small = 1.e-22;
   const bool up = xold[0] < xold[kk];
   auto zo = [&](int k) { return up ? xold[k] : -xold[k]; };
@@ -272,21 +278,23 @@ __device__ void pcwise(int kk, const double *xold, const double *xnew, double *f
     const double fo = fco(k - 1), fn = dmax(0., zn(k) - zn(k - 1));
     const double amount = fld[k - 1] * fo - (vhi - vlo);
     const double y = (amount + fld[k - 1] * small) / (fn + small);
-    if (k > 1) ynew[k - 2] = prev_new;
+    if (k > 1) ynew[k - 2] = (Y)prev_new;
     prev_new = y;
     vlo = vhi;
   }
-  ynew[kk - 1] = prev_new;
+  ynew[kk - 1] = (Y)prev_new;
 }
 
 // remap_1d for one column; also sdot = new Exner - old Exner and ex = new Exner
-__global__ void __launch_bounds__(64) k_remap(DevGrid g, DevState s, int its) {
-  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1, ntra = g.ntra;
+template <typename V>
+__global__ void __launch_bounds__(64) k_remap(HybCols<V> s, const StepCtl *ctl, int its) {
+  const int ncol = s.ncol, nvl = s.nvl, nvlp1 = nvl + 1, ntra = s.ntra;
   const int ipn = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ipn >= nip) return;
+  if (ipn >= ncol) return;
+  if (ctl) its = ctl->its;
   double prcol[LMAX], pk1col[LMAX], exnew[LMAX], prnew[LMAX], pk1new[LMAX];
   double fld[NVL_MAX], th1[NVL_MAX];
-  const double *targt = g.thetac;
+  const double *targt = s.thetac;
   const double small = 1.e-6;
   for (int k = 0; k < nvlp1; k++) {
     const double eo = L2(s.ex, ipn, k);
@@ -294,7 +302,7 @@ __global__ void __launch_bounds__(64) k_remap(DevGrid g, DevState s, int its) {
     exnew[k] = L2(s.exsmo, ipn, k);
     pk1col[k] = eo * prcol[k] / CP;
   }
-  inflate(g.dpsig, nvl, prcol[0], exnew);
+  inflate(s.dpsig, nvl, prcol[0], exnew);
   if (its > 0)
     for (int k = 0; k < nvlp1; k++) exnew[k] = SLAK * exnew[k] + (1. - SLAK) * L2(s.ex, ipn, k);
   {
@@ -314,16 +322,16 @@ __global__ void __launch_bounds__(64) k_remap(DevGrid g, DevState s, int its) {
   prnew[0] = prcol[0];
   pk1new[0] = pk1col[0];
   // u, v and tracers 2..ntra are remapped in pressure
-  for (int k = 0; k < nvl; k++) fld[k] = C2(s.us, ipn, k);
+  for (int k = 0; k < nvl; k++) fld[k] = (double)C2(s.us_in, ipn, k);
   pcwise(nvl, prcol, prnew, fld, &C2(s.us, ipn, 0));
-  for (int k = 0; k < nvl; k++) fld[k] = C2(s.vs, ipn, k);
+  for (int k = 0; k < nvl; k++) fld[k] = (double)C2(s.vs_in, ipn, k);
   pcwise(nvl, prcol, prnew, fld, &C2(s.vs, ipn, 0));
   for (int n = 1; n < ntra; n++) {
-    for (int k = 0; k < nvl; k++) fld[k] = TR(s.tr, n, ipn, k);
+    for (int k = 0; k < nvl; k++) fld[k] = (double)TR(s.tr, n, ipn, k);
     pcwise(nvl, prcol, prnew, fld, &TR(s.tr, n, ipn, 0));
   }
   // theta is remapped in p^(kappa+1)
-  for (int k = 0; k < nvl; k++) fld[k] = TR(s.tr, 0, ipn, k);
+  for (int k = 0; k < nvl; k++) fld[k] = (double)TR(s.tr, 0, ipn, k);
   pcwise(nvl, pk1col, pk1new, fld, th1);
   for (int k = nvl; k >= 3; k--) {
     double dplo = dmax(pk1new[k - 2] - pk1new[k - 1], pk1new[0] * small);
@@ -339,7 +347,7 @@ __global__ void __launch_bounds__(64) k_remap(DevGrid g, DevState s, int its) {
       th1[k - 1] = thb + dmin(devilo, -deviup) / dpup;
     }
   }
-  for (int k = 0; k < nvl; k++) TR(s.tr, 0, ipn, k) = th1[k];
+  for (int k = 0; k < nvl; k++) TR(s.tr, 0, ipn, k) = (V)th1[k];
   for (int k = 1; k <= nvl; k++)
     if (exnew[k - 1] < exnew[k]) exnew[k] = exnew[k - 1];
   for (int k = 0; k < nvlp1; k++) {
@@ -349,10 +357,18 @@ __global__ void __launch_bounds__(64) k_remap(DevGrid g, DevState s, int its) {
   }
 }
 
+template <typename V>
+void launch_hybgen_cols(const HybCols<V> &h, const StepCtl *ctl, int its, hipStream_t st) {
+  KT_LAUNCH("k_regrid", st, k_regrid<V><<<(h.ncol + 63) / 64, 64, 0, st>>>(h, ctl, its));
+  KT_LAUNCH("k_dffusn_lev", st, k_dffusn_lev<V><<<(h.ncol + 3) / 4, dim3(64, 4), 0, st>>>(h));
+  KT_LAUNCH("k_remap", st, k_remap<V><<<(h.ncol + 63) / 64, 64, 0, st>>>(h, ctl, its));
+}
+template void launch_hybgen_cols<float>(const HybCols<float> &, const StepCtl *, int, hipStream_t);
+
 void launch_hybgen(const DevGrid &g, const DevState &s, int its, hipStream_t st) {
-  KT_LAUNCH("k_regrid", st, k_regrid<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its));
-  KT_LAUNCH("k_dffusn_lev", st, k_dffusn_lev<<<(g.nip + 3) / 4, dim3(64, 4), 0, st>>>(g, s));
-  KT_LAUNCH("k_remap", st, k_remap<<<(g.nip + 63) / 64, 64, 0, st>>>(g, s, its));
+  HybCols<double> h{g.nip, g.nip, g.nvl, g.ntra, s.us, s.vs, s.us, s.vs, s.tr, s.dp, s.pr, s.ex, s.sdot, s.exsmo,
+                    g.thetac, g.dpsig, g.smoo, g.sideln, g.rarea, g.prox, g.nprox};
+  launch_hybgen_cols<double>(h, nullptr, its, st);
 }
 
 }  // namespace fimhip

@@ -1,0 +1,730 @@
+// Kernels of the batched mixed-precision form (precision split and layout in
+// ens_internal.h).  Each kernel follows the same reference routine as its
+// FP64 counterpart in kernels_plain.hip; the differences are the arithmetic
+// type, the member index (blockIdx.y), the one-sided mass and tracer fluxes
+// that make the FP32 fluxes cancel exactly, and parallel column scans in
+// place of the lane-0 serial sums (the sums stay FP64).
+//
+// Mapping: one wavefront per column, lane = level; COLS columns per block;
+// blockIdx.y = member, blockIdx.z = tracer where a kernel loops over tracers.
+#include "ens_internal.h"
+#include "fim_pow.h"
+
+namespace fimhip {
+
+const char *const ens_kernel_names[EK_NKERNELS] = {
+    "e_edgvar1", "e_edgvar3", "e_cnuity12", "e_cnuity3", "e_cnuity45",
+    "e_trcadv1", "e_trcadv2", "e_trcadv3", "e_momtum", "e_hystat"};
+
+#define EC(a, col, k) (a)[(size_t)(col) * nvl + (k)]
+#define EL(a, col, k) (a)[(size_t)(col) * nvlp1 + (k)]
+#define EE(a, col, e, k) (a)[((size_t)(col) * NPP + (e)) * nvl + (k)]
+#define EAB(a, sl, col, k) (a)[((size_t)(sl) * ncol + (col)) * nvl + (k)]
+#define ETR(a, t, col, k) (a)[((size_t)(t) * ncol + (col)) * nvl + (k)]
+#define ETAB(a, t, sl, col, k) (a)[(((size_t)(t) * 3 + (sl)) * ncol + (col)) * nvl + (k)]
+#define ETE(a, t, col, e, k) (a)[(((size_t)(t) * ncol + (col)) * NPP + (e)) * nvl + (k)]
+#define GPROX(i, e) ldc(g.d.prox, (i) * NPP + (e))
+#define GPROXS(i, e) ldc(g.d.proxs, (i) * NPP + (e))
+#define FCS(i, e, j) ldc(g.cs, ((i) * NPP + (e)) * 4 + (j))
+#define FSN(i, e, j) ldc(g.sn, ((i) * NPP + (e)) * 4 + (j))
+#define FSVC(i, e, j) ldc(g.svc, ((i) * NPP + (e)) * 2 + (j))
+#define FSVE(i, e, j) ldc(g.sve, ((i) * NPP + (e)) * 2 + (j))
+#define FHF(i, a, b) ldc(g.hfcoef, ((i) * NPP + (a)) * NPP + (b))
+#define GNP(i) ldc(g.d.nprox, (i))
+#define GF(a, i) ldc(g.a, (i))
+#define GD(a, i) ldc(g.d.a, (i))
+
+// The cell a block row works on, wave-uniform.  With SWZ the block index is
+// permuted so that the blocks the hardware sends to XCD x (block ids
+// congruent to x mod 8) cover one contiguous eighth of the cells; the host
+// pads gridDim.x to a multiple of 8 for it.
+template <int COLS, bool SWZ>
+__device__ __forceinline__ int tile_cell() {
+  int bx = blockIdx.x;
+  if (SWZ) {
+    const int per = gridDim.x >> 3;
+    bx = (bx & 7) * per + (bx >> 3);
+  }
+  return __builtin_amdgcn_readfirstlane(bx * COLS + (int)threadIdx.y);
+}
+
+__device__ __forceinline__ double wave_suffix_sum(double x, int lane) {
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_down(x, d, 64);
+    if (lane + d < 64) x += y;
+  }
+  return x;
+}
+__device__ __forceinline__ double wave_prefix_sum(double x, int lane) {
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// edgvar1 (edgvar1.F90:162-421, 425-511): the six neighbour columns are read
+// once into registers and every edge's stencil is formed from them.
+template <int NP>
+__device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &s, int ipn, int base, int k) {
+  const int nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
+  const int col = base + ipn;
+  int nb[NP];
+  float nu[NP], nv[NP], nth[NP];
+  double np0[NP], np1[NP], nmp[NP];
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    nb[e] = base + GPROX(ipn, e);
+    nu[e] = EC(s.us, nb[e], k);
+    nv[e] = EC(s.vs, nb[e], k);
+    nth[e] = ETR(s.tr, 0, nb[e], k);
+    np0[e] = EL(s.pr, nb[e], k);
+    np1[e] = EL(s.pr, nb[e], k + 1);
+    nmp[e] = EC(s.mp, nb[e], k);
+  }
+  const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k), thi = ETR(s.tr, 0, col, k);
+  const double pi0 = EL(s.pr, col, k), pi1 = EL(s.pr, col, k + 1), mpi = EC(s.mp, col, k);
+  const float d6 = 1.f / 6.f, eps = 0.001f;
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
+    const float c0 = FCS(ipn, e, 0), s0 = FSN(ipn, e, 0), c1 = FCS(ipn, e, 1), s1 = FSN(ipn, e, 1);
+    const float c2 = FCS(ipn, e, 2), s2 = FSN(ipn, e, 2), c3 = FCS(ipn, e, 3), s3 = FSN(ipn, e, 3);
+    const float u3trl = c2 * nu[et] + s2 * nv[et], v3trl = -s2 * nu[et] + c2 * nv[et];
+    const float u1 = c0 * ui + s0 * vi, v1 = -s0 * ui + c0 * vi;
+    const float u2 = c1 * nu[e] + s1 * nv[e], v2 = -s1 * nu[e] + c1 * nv[e];
+    const float u3l = c3 * nu[el] + s3 * nv[el], v3l = -s3 * nu[el] + c3 * nv[el];
+    float ue = (2.f * (u1 + u2) + u3trl + u3l) * d6;
+    float ve = (2.f * (v1 + v2) + v3trl + v3l) * d6;
+    const float m1l = sqrtf(u1 * u1 + v1 * v1), m2l = sqrtf(u2 * u2 + v2 * v2);
+    const float m3t = sqrtf(u3trl * u3trl + v3trl * v3trl), m3l = sqrtf(u3l * u3l + v3l * v3l);
+    const float m1 = sqrtf(ue * ue + ve * ve);
+    const float m2 = (2.f * (m1l + m2l) + m3t + m3l) * d6;
+    const float sc = m2 / fmaxf(eps, m1);
+    ue = ue * sc;
+    ve = ve * sc;
+    EE(s.u_edg, col, e, k) = ue;
+    EE(s.v_edg, col, e, k) = ve;
+    EE(s.uvsq_edg, col, e, k) = .5f * (ue * ue + ve * ve);
+    EE(s.lp_edg, col, e, k) =
+        (float)((pi0 + pi1 + np0[e] + np1[e] + .5 * (np0[et] + np1[et] + np0[el] + np1[el])) * (1. / 6.));
+    EE(s.mont_edg, col, e, k) = (2. * (mpi + nmp[e]) + nmp[et] + nmp[el]) * (1. / 6.);
+    // theta on the edge, the same bits from both cells of the edge: the two
+    // outer stencil cells are added in column order
+    float ta = nth[et], tb = nth[el];
+    if (nb[et] > nb[el]) { const float t = ta; ta = tb; tb = t; }
+    ETE(s.trc_edg, 0, col, e, k) = (2.f * (thi + nth[e]) + ta + tb) * d6;
+  }
+  if (NP < NPP) ETE(s.trc_edg, 0, col, NPP - 1, k) = 0.f;
+}
+
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_edgvar1(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip || k >= nvl) return;
+  const int base = blockIdx.y * nip;
+  if (GNP(ipn) == 6) e_edgvar1_col<6>(g, s, ipn, base, k);
+  else e_edgvar1_col<5>(g, s, ipn, base, k);
+}
+
+// edgvar3 (edgvar3.F90:65-110, edgvalEval): this cell's least-squares
+// reconstruction of each edge.  The two-sided average of edgvar3.F90:111-132
+// is formed by the readers (cnuity, trcadv) from both cells' values.  The
+// thickness is reconstructed as a deviation from the cell's own value, the
+// differences taken in FP64.
+template <int NP>
+__device__ __forceinline__ void e_edgvar3_col(const EnsGrid &g, const EnsState &s, int ipn, int base, int k,
+                                              int f) {
+  const int nvl = g.d.nvl, ncol = s.ncol;
+  const int col = base + ipn;
+  float hf[NP][NP];
+#pragma unroll
+  for (int i = 0; i < NP; i++)
+#pragma unroll
+    for (int j = 0; j < NP; j++) hf[i][j] = FHF(ipn, i, j);
+  float d[NP], out[NP];
+  if (f == 0) {
+    const double b0 = EC(s.dp, col, k);
+#pragma unroll
+    for (int j = 0; j < NP; j++) d[j] = (float)(EC(s.dp, base + GPROX(ipn, j), k) - b0);
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      float acc = 0.f;
+#pragma unroll
+      for (int j = 0; j < NP; j++) acc = acc + d[j] * hf[i][j];
+      out[i] = (float)(b0 + (double)acc);
+    }
+#pragma unroll
+    for (int i = 0; i < NP; i++) EE(s.dp_edg, col, i, k) = out[i];
+    if (NP < NPP) EE(s.dp_edg, col, NPP - 1, k) = 0.f;
+  } else {
+    const float b0 = ETR(s.tr, f, col, k);
+#pragma unroll
+    for (int j = 0; j < NP; j++) d[j] = ETR(s.tr, f, base + GPROX(ipn, j), k) - b0;
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      float acc = 0.f;
+#pragma unroll
+      for (int j = 0; j < NP; j++) acc = acc + d[j] * hf[i][j];
+      out[i] = b0 + acc;
+    }
+#pragma unroll
+    for (int i = 0; i < NP; i++) ETE(s.trc_edg, f, col, i, k) = out[i];
+    if (NP < NPP) ETE(s.trc_edg, f, col, NPP - 1, k) = 0.f;
+  }
+}
+
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_edgvar3(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip || k >= nvl) return;
+  const int base = blockIdx.y * nip;
+  if (GNP(ipn) == 6) e_edgvar3_col<6>(g, s, ipn, base, k, blockIdx.z);
+  else e_edgvar3_col<5>(g, s, ipn, base, k, blockIdx.z);
+}
+
+// ---------------------------------------------------------------------------
+// cnuity1 + cnuity2 (cnuity_compute.F90:17-150).  The high-order flux of an
+// edge is formed by the cell with the lower index and negated by the other;
+// the low-order (donor-cell) flux is antisymmetric as written.
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_cnuity12(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip || k >= nvl) return;
+  const StepCtl c = load_ctl(s.ctl);
+  const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
+  const int base = blockIdx.y * nip, col = base + ipn;
+  const int np = GNP(ipn);
+  const double dpi = EC(s.dp, col, k);
+  const float dpf = (float)dpi;
+  double acc = 0.;
+  for (int e = 0; e < np; e++) {
+    const int x = GPROX(ipn, e), ex = GPROXS(ipn, e), xc = base + x;
+    const float vn = FSVE(ipn, e, 1) * EE(s.u_edg, col, e, k) - FSVE(ipn, e, 0) * EE(s.v_edg, col, e, k);
+    const float vx = FSVE(x, ex, 1) * EE(s.u_edg, xc, ex, k) - FSVE(x, ex, 0) * EE(s.v_edg, xc, ex, k);
+    const float dpe = .5f * (EE(s.dp_edg, col, e, k) + EE(s.dp_edg, xc, ex, k));
+    const float hi = ipn < x ? vn * dpe : -(vx * dpe);
+    const float m = .5f * ((vn + fabsf(vn)) * dpf - (vx + fabsf(vx)) * (float)EC(s.dp, xc, k));
+    EE(s.massfx, col, e, k) = m;
+    EE(s.antifx, col, e, k) = hi - m;
+    acc = acc + (double)m;
+  }
+  if (np < NPP) {
+    EE(s.massfx, col, NPP - 1, k) = 0.f;
+    EE(s.antifx, col, NPP - 1, k) = 0.f;
+  }
+  const double dplo = -acc * GD(rarea, ipn);
+  EAB(s.dpl_tdcy, nf, col, k) = (float)dplo;
+  EC(s.delp_lo, col, k) = (float)(dpi + c.ab1 * dplo + c.ab2 * (double)EAB(s.dpl_tdcy, of, col, k) +
+                                  c.ab3 * (double)EAB(s.dpl_tdcy, vof, col, k));
+}
+
+// cnuity3 (cnuity_compute.F90:152-259)
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_cnuity3(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip || k >= nvl) return;
+  const StepCtl c = load_ctl(s.ctl);
+  const int of = c.of - 1, vof = c.vof - 1;
+  const int base = blockIdx.y * nip, col = base + ipn;
+  const float thshld = 1.e-11f;
+  const int np = GNP(ipn);
+  float pp = 0.f, pm = 0.f;
+  for (int e = 0; e < np; e++) {
+    const float a = EE(s.antifx, col, e, k);
+    if (a <= 0.f) pp = pp - a; else pm = pm + a;
+  }
+  const float lo = EC(s.delp_lo, col, k), dpi = (float)EC(s.dp, col, k);
+  float mx = fmaxf(lo, dpi), mn = fminf(lo, dpi);
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const int xc = base + GPROX(ipn, e);
+    const float a = EC(s.delp_lo, xc, k), b = (float)EC(s.dp, xc, k);
+    mx = fmaxf(mx, fmaxf(a, b));
+    mn = fminf(mn, fminf(a, b));
+  }
+  mx = fmaxf(0.f, mx);
+  mn = fmaxf(0.f, mn);
+  const double to = EAB(s.dp_tdcy, of, col, k), tv = EAB(s.dp_tdcy, vof, col, k);
+  const float qp = (float)(((double)mx - lo - (c.ab2 * fmin(0., to) + c.ab3 * fmax(0., tv))) / c.ab1) * GF(area, ipn);
+  const float qm = (float)(((double)lo - mn + (c.ab2 * fmax(0., to) + c.ab3 * fmin(0., tv))) / c.ab1) * GF(area, ipn);
+  EC(s.r_plus, col, k) = fmaxf(0.f, fminf(1.f, qp / fmaxf(pp, thshld)));
+  EC(s.r_mnus, col, k) = fmaxf(0.f, fminf(1.f, qm / fmaxf(pm, thshld)));
+}
+
+// cnuity4 + cnuity5 (cnuity_compute.F90:261-451).  The thickness tendency is
+// formed in FP64 from the final (antisymmetric) FP32 fluxes, the thickness is
+// advanced in FP64, and the top-down pressure sum and the column tendency for
+// omega are wavefront suffix sums.
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_cnuity45(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip) return;
+  const bool live = k < nvl;
+  const StepCtl c = load_ctl(s.ctl);
+  const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
+  const int base = blockIdx.y * nip, col = base + ipn;
+  const int np = GNP(ipn);
+  double dpn = 0., lt = 0.;
+  if (live) {
+    double at = 0.;
+    const float rp = EC(s.r_plus, col, k), rm = EC(s.r_mnus, col, k);
+    for (int e = 0; e < np; e++) {
+      const int xc = base + GPROX(ipn, e);
+      float a = EE(s.antifx, col, e, k);
+      const float clip = a >= 0.f ? fminf(rm, EC(s.r_plus, xc, k)) : fminf(rp, EC(s.r_mnus, xc, k));
+      const float mf = EE(s.massfx, col, e, k) + a * clip;
+      EE(s.massfx, col, e, k) = mf;
+      at = at + (double)mf;
+    }
+    const double dtn = -at * GD(rarea, ipn);
+    EAB(s.dp_tdcy, nf, col, k) = dtn;
+    const double inc = c.ab1 * dtn + c.ab2 * EAB(s.dp_tdcy, of, col, k) + c.ab3 * EAB(s.dp_tdcy, vof, col, k);
+    dpn = EC(s.dp, col, k) + inc;
+    EC(s.dp, col, k) = dpn;
+    lt = inc / g.d.dt;
+  }
+  const double sp = wave_suffix_sum(dpn, k);
+  const double sl = wave_suffix_sum(lt, k);
+  if (!live) return;
+  const double p = EL(s.pr, col, nvl) + sp;
+  EL(s.pr, col, k) = p;
+  EL(s.ex, col, k) = CP * fim_pow(p / P1000, RD / CP);
+  float dpdx = 0.f, dpdy = 0.f;
+  for (int e = 0; e < np; e++) {
+    const float l = EE(s.lp_edg, col, e, k);
+    dpdx = dpdx + l * FSVC(ipn, e, 1);
+    dpdy = dpdy - l * FSVC(ipn, e, 0);
+  }
+  const float om = (float)(sl - .5 * lt);
+  EC(s.ws, col, k) = om + (EC(s.us, col, k) * dpdx + EC(s.vs, col, k) * dpdy) * GF(rarea, ipn);
+}
+
+// ---------------------------------------------------------------------------
+// trcadv1-3 (trcadv_compute.F90:19-315), blockIdx.z = tracer.  The final mass
+// flux is antisymmetric, so the neighbour's flux through the shared edge is
+// the negated own one; tracer edge values are the same bits on both sides
+// (tracer 1 from the edge-symmetric stencil, the others as the average of
+// both cells' reconstructions).
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_trcadv1(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  const int t = blockIdx.z;
+  if (ipn >= nip || k >= nvl) return;
+  const StepCtl c = load_ctl(s.ctl);
+  const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
+  const int base = blockIdx.y * nip, col = base + ipn;
+  const float thshld = 1.e-11f;
+  const int np = GNP(ipn);
+  const float ti = ETR(s.tr, t, col, k);
+  float acc = 0.f;
+  for (int e = 0; e < np; e++) {
+    const int x = GPROX(ipn, e), xc = base + x;
+    const float m = EE(s.massfx, col, e, k);
+    const float te = t == 0 ? ETE(s.trc_edg, 0, col, e, k)
+                            : .5f * (ETE(s.trc_edg, t, col, e, k) + ETE(s.trc_edg, t, xc, GPROXS(ipn, e), k));
+    const float fl = .5f * ((m + fabsf(m)) * ti - (fabsf(m) - m) * ETR(s.tr, t, xc, k));
+    ETE(s.antiflx, t, col, e, k) = m * te - fl;
+    acc = acc + fl;
+  }
+  if (np < NPP) ETE(s.antiflx, t, col, NPP - 1, k) = 0.f;
+  const float tlo = -acc * GF(rarea, ipn);
+  ETAB(s.trl_tdcy, t, nf, col, k) = tlo;
+  const float lo = ETR(s.trdp, t, col, k) + (float)c.ab1 * tlo + (float)c.ab2 * ETAB(s.trl_tdcy, t, of, col, k) +
+                   (float)c.ab3 * ETAB(s.trl_tdcy, t, vof, col, k);
+  float mx = ti, mn = ti;
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const float v = ETR(s.tr, t, base + GPROX(ipn, e), k);
+    mx = fmaxf(mx, v);
+    mn = fminf(mn, v);
+  }
+  mx = fmaxf(0.f, mx);
+  mn = fmaxf(0.f, mn);
+  ETR(s.trmax, t, col, k) = mx;
+  ETR(s.trmin, t, col, k) = mn;
+  ETR(s.trcr_lo, t, col, k) = fmaxf(mn, fminf(mx, lo / fmaxf(thshld, (float)EC(s.dp, col, k))));
+}
+
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_trcadv2(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  const int t = blockIdx.z;
+  if (ipn >= nip || k >= nvl) return;
+  const StepCtl c = load_ctl(s.ctl);
+  const int of = c.of - 1, vof = c.vof - 1;
+  const int base = blockIdx.y * nip, col = base + ipn;
+  const float thshld = 1.e-11f;
+  const int np = GNP(ipn);
+  float sp = 0.f, sm = 0.f;
+  for (int e = 0; e < np; e++) {
+    const float a = ETE(s.antiflx, t, col, e, k);
+    sp = sp - fminf(0.f, a);
+    sm = sm + fmaxf(0.f, a);
+  }
+  const float lo = ETR(s.trcr_lo, t, col, k);
+  float mx = fmaxf(ETR(s.trmax, t, col, k), lo), mn = fminf(ETR(s.trmin, t, col, k), lo);
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const float v = ETR(s.trcr_lo, t, base + GPROX(ipn, e), k);
+    mx = fmaxf(mx, v);
+    mn = fminf(mn, v);
+  }
+  mx = fmaxf(0.f, mx);
+  mn = fmaxf(0.f, mn);
+  ETR(s.trmax, t, col, k) = mx;
+  ETR(s.trmin, t, col, k) = mn;
+  const float dp = (float)EC(s.dp, col, k);
+  const float to = ETAB(s.trc_tdcy, t, of, col, k), tv = ETAB(s.trc_tdcy, t, vof, col, k);
+  const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
+  const float qp = ((mx - lo) * dp - (a2 * fminf(0.f, to) + a3 * fmaxf(0.f, tv))) / a1 * GF(area, ipn);
+  const float qm = ((lo - mn) * dp + (a2 * fmaxf(0.f, to) + a3 * fminf(0.f, tv))) / a1 * GF(area, ipn);
+  ETR(s.tr_rp, t, col, k) = fmaxf(0.f, fminf(1.f, qp / fmaxf(sp, thshld)));
+  ETR(s.tr_rm, t, col, k) = fmaxf(0.f, fminf(1.f, qm / fmaxf(sm, thshld)));
+}
+
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_trcadv3(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  const int t = blockIdx.z;
+  if (ipn >= nip || k >= nvl) return;
+  const StepCtl c = load_ctl(s.ctl);
+  const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
+  const int base = blockIdx.y * nip, col = base + ipn;
+  const float thshld = 1.e-11f;
+  const int np = GNP(ipn);
+  const float rp = ETR(s.tr_rp, t, col, k), rm = ETR(s.tr_rm, t, col, k);
+  float at = 0.f;
+  for (int e = 0; e < np; e++) {
+    const int xc = base + GPROX(ipn, e);
+    float a = ETE(s.antiflx, t, col, e, k);
+    if (a >= 0.f) a = a * fminf(rm, ETR(s.tr_rp, t, xc, k));
+    else a = a * fminf(rp, ETR(s.tr_rm, t, xc, k));
+    at = at + a;
+  }
+  at = -at * GF(rarea, ipn);
+  const float tt = ETAB(s.trl_tdcy, t, nf, col, k) + at;
+  ETAB(s.trc_tdcy, t, nf, col, k) = tt;
+  const float tdp = ETR(s.trdp, t, col, k) + (float)c.ab1 * tt + (float)c.ab2 * ETAB(s.trc_tdcy, t, of, col, k) +
+                    (float)c.ab3 * ETAB(s.trc_tdcy, t, vof, col, k);
+  ETR(s.trdp, t, col, k) = tdp;
+  ETR(s.tr, t, col, k) = fmaxf(ETR(s.trmin, t, col, k),
+                               fminf(ETR(s.trmax, t, col, k), tdp / fmaxf(thshld, (float)EC(s.dp, col, k))));
+}
+
+// ---------------------------------------------------------------------------
+// momtum (momtum.F90:101-158; dissip momtum_compute.F90:102-229; momtum1
+// 231-450).  Diffusion, vorticity, Coriolis and the AB update in FP32; the
+// pressure-gradient line integral of Montgomery potential and Exner function
+// in FP64.  New winds go to u2/v2 (hybgen's remap reads them from there).
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_momtum(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip) return;
+  const bool live = k < nvl;
+  const StepCtl c = load_ctl(s.ctl);
+  const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
+  const int base = blockIdx.y * nip, col = base + ipn;
+  const float thshld = 1.e-11f;
+  float un = 0.f, vn = 0.f;
+  if (live) {
+    const int np = GNP(ipn);
+    const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k);
+    const float wui = ui + GF(grndspd, ipn), wvi = vi;
+    const float a = fmaxf((float)EC(s.dp, col, k), thshld);
+    const float factor = GF(rarea, ipn) / a;
+    const float dff = g.dfflen[k];
+    float wu = 0.f, wv = 0.f;
+    for (int e = 0; e < np; e++) {
+      const int x = GPROX(ipn, e), xc = base + x;
+      const float wux = EC(s.us, xc, k) + GF(grndspd, x), wvx = EC(s.vs, xc, k);
+      const float c0 = FCS(ipn, e, 0), s0 = FSN(ipn, e, 0), c1 = FCS(ipn, e, 1), s1 = FSN(ipn, e, 1);
+      const float uxy1 = c0 * wui + s0 * wvi, vxy1 = -s0 * wui + c0 * wvi;
+      const float uxy2 = c1 * wux + s1 * wvx, vxy2 = -s1 * wux + c1 * wvx;
+      const float b = fmaxf((float)EC(s.dp, xc, k), thshld);
+      const float hh = a * b / (a + b);
+      const float w = dff * GF(sideln, ipn * NPP + e) * 2.f * hh;
+      const float ufx = (uxy2 - uxy1) * w, vfx = (vxy2 - vxy1) * w;
+      wu = wu + (c0 * ufx - s0 * vfx) * factor;
+      wv = wv + (s0 * ufx + c0 * vfx) * factor;
+    }
+    const float u1 = ui + wu, v1 = vi + wv;
+    double px = 0., py = 0.;
+    float rv = 0.f;
+    const double exm = .5 * (EL(s.ex, col, k) + EL(s.ex, col, k + 1));
+    for (int e = 0; e < np; e++) {
+      const double b = EE(s.mont_edg, col, e, k) + (double)EE(s.uvsq_edg, col, e, k);
+      const double te = (double)ETE(s.trc_edg, 0, col, e, k);
+      const double sx = FSVC(ipn, e, 1), sy = FSVC(ipn, e, 0);
+      px = px - b * sx + exm * te * sx;
+      py = py + b * sy - exm * te * sy;
+      rv = rv + (FSVE(ipn, e, 0) * EE(s.u_edg, col, e, k) + FSVE(ipn, e, 1) * EE(s.v_edg, col, e, k));
+    }
+    const float pgx = (float)(px * GD(rarea, ipn)), pgy = (float)(py * GD(rarea, ipn));
+    rv = rv * GF(rarea, ipn);
+    EC(s.relvor, col, k) = rv;
+    const float f = GF(corio, ipn) + rv;
+    const float ut = pgx + f * v1, vt = pgy - f * u1;
+    EAB(s.u_tdcy, nf, col, k) = ut;
+    EAB(s.v_tdcy, nf, col, k) = vt;
+    un = u1 + (float)c.ab1 * ut + (float)c.ab2 * EAB(s.u_tdcy, of, col, k) + (float)c.ab3 * EAB(s.u_tdcy, vof, col, k);
+    vn = v1 + (float)c.ab1 * vt + (float)c.ab2 * EAB(s.v_tdcy, of, col, k) + (float)c.ab3 * EAB(s.v_tdcy, vof, col, k);
+  }
+  // Rayleigh damping of the top layers (momtum_compute.F90:406-416): the
+  // largest squared speed of the top three layers, from their lanes
+  const float q = (live && k >= nvl - 3) ? un * un + vn * vn : 0.f;
+  float qmax = q;
+  for (int d = 1; d < 64; d <<= 1) qmax = fmaxf(qmax, __shfl_xor(qmax, d, 64));
+  if (!live) return;
+  float qq = qmax / (float)(RLTHRESH * RLTHRESH);
+  const int ndamp = (int)(0.25 * nvl);
+  const int k1 = k + 1;
+  if (qq > 1.f && k1 >= nvl - ndamp) {
+    qq = sqrtf((qq - 1.f) * qq);
+    const float fac = (float)(1. - (g.d.dt / 86400.) * RLEIGH * (double)qq * fim_pow(2., RLTAPER * (double)(k1 - nvl)));
+    un = un * fac;
+    vn = vn * fac;
+  }
+  EC(s.u2, col, k) = un;
+  EC(s.v2, col, k) = vn;
+}
+
+// ---------------------------------------------------------------------------
+// hystat (hystat.F90:23-115) with the tracer floors and tracer mass: the
+// surface-pressure sum and the Montgomery-potential recurrence are FP64
+// wavefront scans.
+__device__ __forceinline__ void e_hystat_col(const EnsGrid &g, const EnsState &s, int col, int k, int its) {
+  const int nip = g.d.nip, nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol, ntra = g.d.ntra;
+  const bool live = k < nvl;
+  const double dp = live ? EC(s.dp, col, k) : 0.;
+  const double th = live ? (double)ETR(s.tr, 0, col, k) : 0.;
+  const double exk = live ? EL(s.ex, col, k) : 0.;
+  double ps = dp;
+  for (int d = 1; d < 64; d <<= 1) ps += __shfl_xor(ps, d, 64);
+  const double thm = __shfl_up(th, 1, 64);
+  const double inc = k == 0 ? exk * th + EL(s.ph, col, 0) : (live ? exk * (th - thm) : 0.);
+  const double mp = wave_prefix_sum(inc, k);
+  if (k == 0) {
+    const double old = s.psrf[col];
+    s.psrf[col] = ps;
+    if (its > 0) s.ptdcy[(size_t)(its % 2) * ncol + col] = ps - old;
+  }
+  if (!live) return;
+  EC(s.mp, col, k) = mp;
+  EL(s.ph, col, k + 1) = mp - th * EL(s.ex, col, k + 1);
+  if (ntra > 1) ETR(s.tr, 1, col, k) = fmaxf((float)QVMIN, ETR(s.tr, 1, col, k));
+  if (ntra > 2) ETR(s.tr, 2, col, k) = fmaxf((float)QWMIN, ETR(s.tr, 2, col, k));
+  const float dpf = (float)dp;
+  for (int t = 0; t < ntra; t++) ETR(s.trdp, t, col, k) = ETR(s.tr, t, col, k) * dpf;
+  (void)nip;
+}
+
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_hystat(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip;
+  const int ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip) return;
+  e_hystat_col(g, s, blockIdx.y * nip + ipn, threadIdx.x, load_ctl(s.ctl).its);
+}
+
+// start-up hystat(its = 0) (dyn_init.F90:1037)
+__global__ void __launch_bounds__(256) k_e_hystat0(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip;
+  const int ipn = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + threadIdx.y));
+  if (ipn >= nip) return;
+  e_hystat_col(g, s, blockIdx.y * nip + ipn, threadIdx.x, 0);
+}
+
+// abstart (abstart.F90:38-66) on the device
+__global__ void k_e_abstart(StepCtl *ctl, double dt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  StepCtl c = *ctl;
+  c.its = c.its + 1;
+  c.nf = c.nf + 1; if (c.nf == 4) c.nf = 1;
+  c.of = c.of + 1; if (c.of == 4) c.of = 1;
+  c.vof = c.vof + 1; if (c.vof == 4) c.vof = 1;
+  if (c.its == 1) { c.ab1 = dt; c.ab2 = 0.; c.ab3 = 0.; }
+  if (c.its == 2) { c.ab1 = dt * 1.5; c.ab2 = -dt * .5; c.ab3 = 0.; }
+  if (c.its == 3) { c.ab1 = dt * 23. / 12.; c.ab2 = -dt * 16. / 12.; c.ab3 = dt * 5. / 12.; }
+  *ctl = c;
+}
+
+// ---------------------------------------------------------------------------
+// start-up helpers
+__global__ void __launch_bounds__(256) k_e_pres_from_exner(EnsGrid g, EnsState s) {
+  const size_t n = (size_t)s.ncol * (g.d.nvl + 1);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    s.pr[i] = P1000 * fim_pow(s.ex[i] / CP, CP / RD);
+}
+
+__global__ void __launch_bounds__(256) k_e_trdp(EnsGrid g, EnsState s) {
+  const size_t cz = (size_t)s.ncol * g.d.nvl;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cz; i += (size_t)gridDim.x * blockDim.x) {
+    const float dp = (float)s.dp[i];
+    for (int t = 0; t < g.d.ntra; t++) s.trdp[(size_t)t * cz + i] = s.tr[(size_t)t * cz + i] * dp;
+  }
+}
+
+// one member's FP64 state (plain layout) into member slot m
+__global__ void __launch_bounds__(256) k_e_convert(EnsGrid g, EnsState s, int m, DevState src) {
+  const int nip = g.d.nip, nvl = g.d.nvl, ntra = g.d.ntra;
+  const size_t cz = (size_t)nip * nvl, lz = (size_t)nip * (nvl + 1);
+  const size_t ecz = (size_t)s.ncol * nvl;
+  const size_t oc = (size_t)m * cz, ol = (size_t)m * lz;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < lz; i += (size_t)gridDim.x * blockDim.x) {
+    s.pr[ol + i] = src.pr[i];
+    s.ex[ol + i] = src.ex[i];
+    s.ph[ol + i] = src.ph[i];
+    if (i < cz) {
+      s.us[oc + i] = (float)src.us[i];
+      s.vs[oc + i] = (float)src.vs[i];
+      s.dp[oc + i] = src.dp[i];
+      for (int t = 0; t < ntra; t++) s.tr[(size_t)t * ecz + oc + i] = (float)src.tr[(size_t)t * cz + i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// globsum (globsum.F90:70-101) per member: FP64 block partials, and FP32
+// Kahan-compensated partials (value + compensation) of the same sums.
+struct KPair { float s, c; };
+__device__ __forceinline__ KPair kp_add(KPair a, float x) {    // Kahan step
+  const float y = x - a.c;
+  const float t = a.s + y;
+  a.c = (t - a.s) - y;
+  a.s = t;
+  return a;
+}
+__device__ __forceinline__ KPair kp_merge(KPair a, KPair b) {  // a + b, compensations carried
+  const float s = a.s + b.s;
+  const float bb = s - a.s;
+  const float err = (a.s - (s - bb)) + (b.s - bb);             // two-sum error of a.s + b.s
+  KPair r;
+  r.s = s;
+  r.c = a.c + b.c - err;
+  return r;
+}
+
+__global__ void __launch_bounds__(256) k_e_globsum(EnsGrid g, EnsState s, double *p64, float *p32) {
+  __shared__ double sh[256];
+  __shared__ float shs[256], shc[256];
+  const int nip = g.d.nip, nvl = g.d.nvl, ntra = g.d.ntra, ncol = s.ncol;
+  const int m = blockIdx.y, nb = gridDim.x;
+  for (int q = 0; q <= ntra; q++) {
+    double acc = 0.;
+    KPair kp{0.f, 0.f};
+    for (int ipn = blockIdx.x * 256 + threadIdx.x; ipn < nip; ipn += nb * 256) {
+      const int col = m * nip + ipn;
+      double colsum = 0.;
+      for (int k = 0; k < nvl; k++) {
+        const double dp = EC(s.dp, col, k);
+        double v = g.d.area[ipn] * dp;
+        float vf = GF(area, ipn) * (float)dp;
+        if (q > 0) {
+          const float tr = ETR(s.tr, q - 1, col, k);
+          v = v * (double)tr;
+          vf = vf * tr;
+        }
+        colsum = colsum + v;
+        kp = kp_add(kp, vf);
+      }
+      acc = acc + colsum;
+    }
+    sh[threadIdx.x] = acc;
+    shs[threadIdx.x] = kp.s;
+    shc[threadIdx.x] = kp.c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (threadIdx.x < w) {
+        sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+        KPair r = kp_merge(KPair{shs[threadIdx.x], shc[threadIdx.x]}, KPair{shs[threadIdx.x + w], shc[threadIdx.x + w]});
+        shs[threadIdx.x] = r.s;
+        shc[threadIdx.x] = r.c;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const size_t o = ((size_t)m * (ntra + 1) + q) * nb + blockIdx.x;
+      p64[o] = sh[0];
+      p32[2 * o] = shs[0];
+      p32[2 * o + 1] = shc[0];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+static dim3 ens_grid(int nip, int members, int z, int cols, bool swz) {
+  int gx = (nip + cols - 1) / cols;
+  if (swz) gx = (gx + 7) / 8 * 8;
+  return dim3(gx, members, z);
+}
+
+#define ENS_DISPATCH(KNAME, z)                                                                      \
+  do {                                                                                              \
+    const dim3 gr = ens_grid(g.d.nip, s.members, (z), tile_cols(tile), tile_swz(tile));             \
+    const dim3 bl(64, tile_cols(tile));                                                            \
+    switch (tile) {                                                                                 \
+      case 0: KNAME<1, false><<<gr, bl, 0, st>>>(g, s); break;                                      \
+      case 1: KNAME<2, false><<<gr, bl, 0, st>>>(g, s); break;                                      \
+      case 2: KNAME<4, false><<<gr, bl, 0, st>>>(g, s); break;                                      \
+      case 3: KNAME<8, false><<<gr, bl, 0, st>>>(g, s); break;                                      \
+      case 4: KNAME<1, true><<<gr, bl, 0, st>>>(g, s); break;                                       \
+      case 5: KNAME<2, true><<<gr, bl, 0, st>>>(g, s); break;                                       \
+      case 6: KNAME<4, true><<<gr, bl, 0, st>>>(g, s); break;                                       \
+      default: KNAME<8, true><<<gr, bl, 0, st>>>(g, s); break;                                      \
+    }                                                                                               \
+  } while (0)
+
+void ens_launch(int kernel, int tile, const EnsGrid &g, const EnsState &s, hipStream_t st) {
+  const int ntra = g.d.ntra;
+  const char *name = ens_kernel_names[kernel];
+  if (::fimhip::g_ktimer) kt_begin(name, st);
+  switch (kernel) {
+    case EK_EDGVAR1: ENS_DISPATCH(k_e_edgvar1, 1); break;
+    case EK_EDGVAR3: ENS_DISPATCH(k_e_edgvar3, ntra); break;
+    case EK_CNUITY12: ENS_DISPATCH(k_e_cnuity12, 1); break;
+    case EK_CNUITY3: ENS_DISPATCH(k_e_cnuity3, 1); break;
+    case EK_CNUITY45: ENS_DISPATCH(k_e_cnuity45, 1); break;
+    case EK_TRCADV1: ENS_DISPATCH(k_e_trcadv1, ntra); break;
+    case EK_TRCADV2: ENS_DISPATCH(k_e_trcadv2, ntra); break;
+    case EK_TRCADV3: ENS_DISPATCH(k_e_trcadv3, ntra); break;
+    case EK_MOMTUM: ENS_DISPATCH(k_e_momtum, 1); break;
+    default: ENS_DISPATCH(k_e_hystat, 1); break;
+  }
+  if (::fimhip::g_ktimer) kt_end(st);
+}
+
+void ens_launch_abstart(const EnsState &s, double dt, hipStream_t st) {
+  k_e_abstart<<<1, 64, 0, st>>>(s.ctl, dt);
+}
+void ens_launch_pres_from_exner(const EnsGrid &g, const EnsState &s, hipStream_t st) {
+  k_e_pres_from_exner<<<2048, 256, 0, st>>>(g, s);
+}
+void ens_launch_trdp(const EnsGrid &g, const EnsState &s, hipStream_t st) {
+  k_e_trdp<<<2048, 256, 0, st>>>(g, s);
+}
+void ens_launch_hystat_init(const EnsGrid &g, const EnsState &s, hipStream_t st) {
+  k_e_hystat0<<<dim3((g.d.nip + 3) / 4, s.members), dim3(64, 4), 0, st>>>(g, s);
+}
+void ens_launch_convert_member(const EnsGrid &g, const EnsState &s, int member, const DevState &src,
+                               hipStream_t st) {
+  k_e_convert<<<2048, 256, 0, st>>>(g, s, member, src);
+}
+void ens_launch_globsum(const EnsGrid &g, const EnsState &s, double *part64, float *part32, int nblocks,
+                        hipStream_t st) {
+  k_e_globsum<<<dim3(nblocks, s.members), 256, 0, st>>>(g, s, part64, part32);
+}
+
+}  // namespace fimhip

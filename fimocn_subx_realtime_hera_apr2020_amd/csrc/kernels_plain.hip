@@ -16,13 +16,15 @@ namespace fimhip {
 #define TR(a, t, ipn, k) (a)[((size_t)(t) * nip + (ipn)) * nvl + (k)]
 #define TAB(a, t, s, ipn, k) (a)[(((size_t)(t) * 3 + (s)) * nip + (ipn)) * nvl + (k)]
 #define MF(a, s, ipn, e, k) (a)[(((size_t)(s) * nip + (ipn)) * NPP + (e)) * nvl + (k)]
-#define PROX(ipn, e) g.prox[(ipn) * NPP + (e)]
-#define PROXS(ipn, e) g.proxs[(ipn) * NPP + (e)]
-#define CS(ipn, e, i) g.cs[((ipn) * NPP + (e)) * 4 + (i)]
-#define SN(ipn, e, i) g.sn[((ipn) * NPP + (e)) * 4 + (i)]
-#define SVC(ipn, e, i) g.svc[((ipn) * NPP + (e)) * 2 + (i)]
-#define SVE(ipn, e, i) g.sve[((ipn) * NPP + (e)) * 2 + (i)]
-#define HF(ipn, i, j) g.hfcoef[((ipn) * NPP + (i)) * NPP + (j)]
+#define PROX(ipn, e) ldc(g.prox, (ipn) * NPP + (e))
+#define PROXS(ipn, e) ldc(g.proxs, (ipn) * NPP + (e))
+#define CS(ipn, e, i) ldc(g.cs, ((ipn) * NPP + (e)) * 4 + (i))
+#define SN(ipn, e, i) ldc(g.sn, ((ipn) * NPP + (e)) * 4 + (i))
+#define SVC(ipn, e, i) ldc(g.svc, ((ipn) * NPP + (e)) * 2 + (i))
+#define SVE(ipn, e, i) ldc(g.sve, ((ipn) * NPP + (e)) * 2 + (i))
+#define HF(ipn, i, j) ldc(g.hfcoef, ((ipn) * NPP + (i)) * NPP + (j))
+#define GNP(ipn) ldc(g.nprox, (ipn))
+#define GV(a, ipn) ldc(g.a, (ipn))
 
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
@@ -48,7 +50,7 @@ __global__ void __launch_bounds__(256) k_edgvar1(DevGrid g, DevState s) {
   const int ipn = wave_col();
   if (ipn >= nip || k >= nvl) return;
   const double divby6 = 1. / 6., eps = 0.001;
-  const int np = g.nprox[ipn];
+  const int np = GNP(ipn);
   for (int edg = 0; edg < np; edg++) {
     const int i1 = ipn, i2 = PROX(ipn, edg);
     const int i3t = PROX(ipn, (edg - 1 + np) % np);
@@ -97,7 +99,7 @@ __global__ void __launch_bounds__(256) k_edgvar3(DevGrid g, DevState s) {
   if (ipn >= nip || k >= nvl) return;
   const double *fld = f == 0 ? s.dp : s.tr + (size_t)f * nip * nvl;
   double *out = f == 0 ? s.dp_edg : s.trc_edg + (size_t)f * nip * NPP * nvl;
-  const int np = g.nprox[ipn];
+  const int np = GNP(ipn);
   const double b0 = fld[(size_t)ipn * nvl + k];
   double nbv[NPP];
   for (int j = 0; j < NPP; j++) nbv[j] = fld[(size_t)PROX(ipn, j) * nvl + k];
@@ -106,7 +108,7 @@ __global__ void __launch_bounds__(256) k_edgvar3(DevGrid g, DevState s) {
     double own = b0;
     for (int j = 0; j < np; j++) own = own + (nbv[j] - b0) * HF(ipn, i, j);
     // the neighbour's reconstruction of the same edge
-    const int nb = PROX(ipn, i), eo = PROXS(ipn, i), nn = g.nprox[nb];
+    const int nb = PROX(ipn, i), eo = PROXS(ipn, i), nn = GNP(nb);
     const double c0 = nbv[i];
     double oth = c0;
     for (int j = 0; j < nn; j++) oth = oth + (fld[(size_t)PROX(nb, j) * nvl + k] - c0) * HF(nb, eo, j);
@@ -123,7 +125,7 @@ __global__ void __launch_bounds__(256) k_cnuity12(DevGrid g, DevState s, AB ab) 
   const int ipn = wave_col();
   if (ipn >= nip || k >= nvl) return;
   const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
-  const int np = g.nprox[ipn];
+  const int np = GNP(ipn);
   const double dpi = C2(s.dp, ipn, k);
   double acc = 0.;
   for (int edg = 0; edg < np; edg++) {
@@ -140,7 +142,7 @@ __global__ void __launch_bounds__(256) k_cnuity12(DevGrid g, DevState s, AB ab) 
     MF(s.massfx, nf, ipn, NPP - 1, k) = 0.;
     E3(s.antifx, ipn, NPP - 1, k) = 0.;
   }
-  double dplo = -acc * g.rarea[ipn];
+  double dplo = -acc * GV(rarea, ipn);
   AB3(s.dpl_tdcy, nf, ipn, k) = dplo;
   C2(s.delp_lo, ipn, k) = dpi + ab.ab1 * dplo + ab.ab2 * AB3(s.dpl_tdcy, of, ipn, k) +
                           ab.ab3 * AB3(s.dpl_tdcy, vof, ipn, k);
@@ -154,7 +156,7 @@ __global__ void __launch_bounds__(256) k_cnuity3(DevGrid g, DevState s, AB ab) {
   if (ipn >= nip || k >= nvl) return;
   const int of = ab.of - 1, vof = ab.vof - 1;
   const double thshld = 1.e-11;
-  const int np = g.nprox[ipn];
+  const int np = GNP(ipn);
   double pp = 0., pm = 0.;
   for (int edg = 0; edg < np; edg++) {
     double a = E3(s.antifx, ipn, edg, k);
@@ -172,8 +174,8 @@ __global__ void __launch_bounds__(256) k_cnuity3(DevGrid g, DevState s, AB ab) {
   dpmax = dmax(0., dpmax);
   dpmin = dmax(0., dpmin);
   const double to = AB3(s.dp_tdcy, of, ipn, k), tv = AB3(s.dp_tdcy, vof, ipn, k);
-  double q_plus = (dpmax - lo - (ab.ab2 * dmin(0., to) + ab.ab3 * dmax(0., tv))) / ab.ab1 * g.area[ipn];
-  double q_mnus = (lo - dpmin + (ab.ab2 * dmax(0., to) + ab.ab3 * dmin(0., tv))) / ab.ab1 * g.area[ipn];
+  double q_plus = (dpmax - lo - (ab.ab2 * dmin(0., to) + ab.ab3 * dmax(0., tv))) / ab.ab1 * GV(area, ipn);
+  double q_mnus = (lo - dpmin + (ab.ab2 * dmax(0., to) + ab.ab3 * dmin(0., tv))) / ab.ab1 * GV(area, ipn);
   C2(s.r_plus, ipn, k) = dmax(0., dmin(1., q_plus / dmax(pp, thshld)));
   C2(s.r_mnus, ipn, k) = dmax(0., dmin(1., q_mnus / dmax(pm, thshld)));
 }
@@ -193,7 +195,7 @@ __global__ void __launch_bounds__(256) k_cnuity45(DevGrid g, DevState s, AB ab) 
   const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
   double dtn = 0.;
   if (live) {
-    const int np = g.nprox[ipn];
+    const int np = GNP(ipn);
     double at = 0.;
     for (int edg = 0; edg < np; edg++) {
       const int x = PROX(ipn, edg);
@@ -204,7 +206,7 @@ __global__ void __launch_bounds__(256) k_cnuity45(DevGrid g, DevState s, AB ab) 
       MF(s.massfx, nf, ipn, edg, k) = MF(s.massfx, nf, ipn, edg, k) + a;
       at = at + a;
     }
-    at = -at * g.rarea[ipn];
+    at = -at * GV(rarea, ipn);
     dtn = AB3(s.dpl_tdcy, nf, ipn, k) + at;
     AB3(s.dp_tdcy, nf, ipn, k) = dtn;
     const double to = AB3(s.dp_tdcy, of, ipn, k), tv = AB3(s.dp_tdcy, vof, ipn, k);
@@ -230,13 +232,13 @@ __global__ void __launch_bounds__(256) k_cnuity45(DevGrid g, DevState s, AB ab) 
     const double p = sh_pr[c][k];
     L2(s.pr, ipn, k) = p;
     L2(s.ex, ipn, k) = CP * fim_pow(p / P1000, RD / CP);
-    const int np = g.nprox[ipn];
+    const int np = GNP(ipn);
     double dpdx = 0., dpdy = 0.;
     for (int edg = 0; edg < np; edg++) {
       dpdx = dpdx + E3(s.lp_edg, ipn, edg, k) * SVC(ipn, edg, 1);
       dpdy = dpdy - E3(s.lp_edg, ipn, edg, k) * SVC(ipn, edg, 0);
     }
-    C2(s.ws, ipn, k) = sh_om[c][k] + (C2(s.us, ipn, k) * dpdx + C2(s.vs, ipn, k) * dpdy) * g.rarea[ipn];
+    C2(s.ws, ipn, k) = sh_om[c][k] + (C2(s.us, ipn, k) * dpdx + C2(s.vs, ipn, k) * dpdy) * GV(rarea, ipn);
   }
 }
 
@@ -254,7 +256,7 @@ __global__ void __launch_bounds__(256) k_trcadv1(DevGrid g, DevState s, AB ab) {
   const double *tre = s.trc_edg + t * esz;
   const double *tracr = s.tr + t * csz;
   double *antiflx = s.antiflx + t * esz;
-  const int np = g.nprox[ipn];
+  const int np = GNP(ipn);
   const double ti = C2(tracr, ipn, k);
   double acc = 0.;
   for (int edg = 0; edg < np; edg++) {
@@ -266,7 +268,7 @@ __global__ void __launch_bounds__(256) k_trcadv1(DevGrid g, DevState s, AB ab) {
     acc = acc + fl;
   }
   if (np < NPP) E3(antiflx, ipn, NPP - 1, k) = 0.;
-  double tlo = -acc * g.rarea[ipn];
+  double tlo = -acc * GV(rarea, ipn);
   TAB(s.trl_tdcy, t, nf, ipn, k) = tlo;
   double trcdp_lo = C2(s.trdp + t * csz, ipn, k) + ab.ab1 * tlo + ab.ab2 * TAB(s.trl_tdcy, t, of, ipn, k) +
                     ab.ab3 * TAB(s.trl_tdcy, t, vof, ipn, k);
@@ -294,7 +296,7 @@ __global__ void __launch_bounds__(256) k_trcadv2(DevGrid g, DevState s, AB ab) {
   const size_t esz = (size_t)nip * NPP * nvl, csz = (size_t)nip * nvl;
   const double *antiflx = s.antiflx + t * esz;
   const double *lo_ = s.trcr_lo + t * csz;
-  const int np = g.nprox[ipn];
+  const int np = GNP(ipn);
   double sp = 0., sm = 0.;
   for (int edg = 0; edg < np; edg++) {
     double a = E3(antiflx, ipn, edg, k);
@@ -315,8 +317,8 @@ __global__ void __launch_bounds__(256) k_trcadv2(DevGrid g, DevState s, AB ab) {
   C2(s.trmin + t * csz, ipn, k) = mn;
   const double dp = C2(s.dp, ipn, k);
   const double to = TAB(s.trc_tdcy, t, of, ipn, k), tv = TAB(s.trc_tdcy, t, vof, ipn, k);
-  double q_plus = ((mx - lo) * dp - (ab.ab2 * dmin(0., to) + ab.ab3 * dmax(0., tv))) / ab.ab1 * g.area[ipn];
-  double q_mnus = ((lo - mn) * dp + (ab.ab2 * dmax(0., to) + ab.ab3 * dmin(0., tv))) / ab.ab1 * g.area[ipn];
+  double q_plus = ((mx - lo) * dp - (ab.ab2 * dmin(0., to) + ab.ab3 * dmax(0., tv))) / ab.ab1 * GV(area, ipn);
+  double q_mnus = ((lo - mn) * dp + (ab.ab2 * dmax(0., to) + ab.ab3 * dmin(0., tv))) / ab.ab1 * GV(area, ipn);
   C2(s.tr_rp + t * csz, ipn, k) = dmax(0., dmin(1., q_plus / dmax(sp, thshld)));
   C2(s.tr_rm + t * csz, ipn, k) = dmax(0., dmin(1., q_mnus / dmax(sm, thshld)));
 }
@@ -332,7 +334,7 @@ __global__ void __launch_bounds__(256) k_trcadv3(DevGrid g, DevState s, AB ab) {
   const size_t esz = (size_t)nip * NPP * nvl, csz = (size_t)nip * nvl;
   const double *antiflx = s.antiflx + t * esz;
   const double *rp = s.tr_rp + t * csz, *rm = s.tr_rm + t * csz;
-  const int np = g.nprox[ipn];
+  const int np = GNP(ipn);
   double at = 0.;
   for (int edg = 0; edg < np; edg++) {
     const int x = PROX(ipn, edg);
@@ -341,7 +343,7 @@ __global__ void __launch_bounds__(256) k_trcadv3(DevGrid g, DevState s, AB ab) {
     else a = a * dmin(C2(rp, ipn, k), C2(rm, x, k));
     at = at + a;
   }
-  at = -at * g.rarea[ipn];
+  at = -at * GV(rarea, ipn);
   double tt = TAB(s.trl_tdcy, t, nf, ipn, k) + at;
   TAB(s.trc_tdcy, t, nf, ipn, k) = tt;
   double *trcdp = s.trdp + t * csz;
@@ -366,22 +368,22 @@ __global__ void __launch_bounds__(256) k_momtum(DevGrid g, DevState s, AB ab) {
   const double thshld = 1.e-11;
   double un = 0., vn = 0.;
   if (live) {
-    const int np = g.nprox[ipn];
-    const double wui = C2(s.us, ipn, k) + g.grndspd[ipn], wvi = C2(s.vs, ipn, k);
+    const int np = GNP(ipn);
+    const double wui = C2(s.us, ipn, k) + GV(grndspd, ipn), wvi = C2(s.vs, ipn, k);
     const double dpi = C2(s.dp, ipn, k);
-    const double factor = g.rarea[ipn] / dmax(dpi, thshld);
+    const double factor = GV(rarea, ipn) / dmax(dpi, thshld);
     double wu = 0., wv = 0.;
     for (int edg = 0; edg < np; edg++) {
       const int ipx = PROX(ipn, edg);
-      const double wux = C2(s.us, ipx, k) + g.grndspd[ipx], wvx = C2(s.vs, ipx, k);
+      const double wux = C2(s.us, ipx, k) + GV(grndspd, ipx), wvx = C2(s.vs, ipx, k);
       double uxy1 = CS(ipn, edg, 0) * wui + SN(ipn, edg, 0) * wvi;
       double vxy1 = -SN(ipn, edg, 0) * wui + CS(ipn, edg, 0) * wvi;
       double uxy2 = CS(ipn, edg, 1) * wux + SN(ipn, edg, 1) * wvx;
       double vxy2 = -SN(ipn, edg, 1) * wux + CS(ipn, edg, 1) * wvx;
       double a = dmax(dpi, thshld), b = dmax(C2(s.dp, ipx, k), thshld);
       double hh = a * b / (a + b);
-      double ufx = (uxy2 - uxy1) * g.dfflen[k] * g.sideln[ipn * NPP + edg] * 2. * hh;
-      double vfx = (vxy2 - vxy1) * g.dfflen[k] * g.sideln[ipn * NPP + edg] * 2. * hh;
+      double ufx = (uxy2 - uxy1) * g.dfflen[k] * GV(sideln, ipn * NPP + edg) * 2. * hh;
+      double vfx = (vxy2 - vxy1) * g.dfflen[k] * GV(sideln, ipn * NPP + edg) * 2. * hh;
       double uflux = CS(ipn, edg, 0) * ufx - SN(ipn, edg, 0) * vfx;
       double vflux = SN(ipn, edg, 0) * ufx + CS(ipn, edg, 0) * vfx;
       wu = wu + uflux * factor;
@@ -397,12 +399,12 @@ __global__ void __launch_bounds__(256) k_momtum(DevGrid g, DevState s, AB ab) {
       py = py + b * SVC(ipn, edg, 0) - exm * te * SVC(ipn, edg, 0);
       rv = rv + (SVE(ipn, edg, 0) * E3(s.u_edg, ipn, edg, k) + SVE(ipn, edg, 1) * E3(s.v_edg, ipn, edg, k));
     }
-    px = px * g.rarea[ipn];
-    py = py * g.rarea[ipn];
-    rv = rv * g.rarea[ipn];
+    px = px * GV(rarea, ipn);
+    py = py * GV(rarea, ipn);
+    rv = rv * GV(rarea, ipn);
     C2(s.relvor, ipn, k) = rv;
-    double uzeta = (g.corio[ipn] + rv) * u1;
-    double vzeta = (g.corio[ipn] + rv) * v1;
+    double uzeta = (GV(corio, ipn) + rv) * u1;
+    double vzeta = (GV(corio, ipn) + rv) * v1;
     double ut = px + vzeta, vt = py - uzeta;
     AB3(s.u_tdcy, nf, ipn, k) = ut;
     AB3(s.v_tdcy, nf, ipn, k) = vt;

@@ -35,7 +35,11 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_hybgen_r8", "fimhip_hystat_r8", "fimhip_field_size", "fimhip_upload", "fimhip_download",
            "fimhip_init_state", "fimhip_step", "fimhip_sync", "fimhip_globsum", "fimhip_icos_grid",
            "fimhip_synth_state", "fimhip_synth_device", "fimhip_profile", "fimhip_profile_report",
-           "fimhip_pow", "fimhip_version"]
+           "fimhip_pow", "fimhip_version", "fimhip_ens_create", "fimhip_ens_destroy", "fimhip_ens_synth",
+           "fimhip_ens_load_member", "fimhip_ens_init_state", "fimhip_ens_step", "fimhip_ens_use_graph",
+           "fimhip_ens_sync", "fimhip_ens_field_size", "fimhip_ens_download", "fimhip_ens_globsum",
+           "fimhip_ens_kernel_count", "fimhip_ens_kernel_name", "fimhip_ens_set_tile", "fimhip_ens_get_tile",
+           "fimhip_ens_autotune", "fimhip_ens_profile", "fimhip_ens_profile_report"]
 
 SYNTH_WIND_NOISE, SYNTH_PERTURB, SYNTH_LOGNORMAL = 1, 2, 4
 
@@ -97,6 +101,25 @@ def load_library() -> ct.CDLL:
     lib.fimhip_pow.argtypes = [ct.c_double, ct.c_double]
     lib.fimhip_pow.restype = ct.c_double
     lib.fimhip_version.restype = ct.c_char_p
+    lib.fimhip_ens_create.argtypes = [ct.POINTER(GridDesc), ct.c_int, ct.c_int, ct.POINTER(vp)]
+    lib.fimhip_ens_destroy.argtypes = [vp]
+    lib.fimhip_ens_synth.argtypes = [vp, ct.c_int, ct.c_ulonglong, ct.c_int, ct.c_int]
+    lib.fimhip_ens_load_member.argtypes = [vp, ct.c_int] + [_PD] * 7
+    lib.fimhip_ens_init_state.argtypes = [vp, ct.c_int]
+    lib.fimhip_ens_step.argtypes = [vp, ct.c_int, ct.c_int]
+    lib.fimhip_ens_use_graph.argtypes = [vp, ct.c_int]
+    lib.fimhip_ens_sync.argtypes = [vp]
+    lib.fimhip_ens_field_size.argtypes = [vp, ct.c_int]
+    lib.fimhip_ens_field_size.restype = ct.c_longlong
+    lib.fimhip_ens_download.argtypes = [vp, ct.c_int, ct.c_int, _PD]
+    lib.fimhip_ens_globsum.argtypes = [vp, _PD, _PD, _PD, _PD]
+    lib.fimhip_ens_kernel_name.argtypes = [ct.c_int]
+    lib.fimhip_ens_kernel_name.restype = ct.c_char_p
+    lib.fimhip_ens_set_tile.argtypes = [vp, ct.c_int, ct.c_int]
+    lib.fimhip_ens_get_tile.argtypes = [vp, ct.c_int]
+    lib.fimhip_ens_autotune.argtypes = [vp, ct.c_int, ct.c_int, _PD]
+    lib.fimhip_ens_profile.argtypes = [vp, ct.c_int]
+    lib.fimhip_ens_profile_report.argtypes = [vp, ct.c_char_p, ct.c_int]
     _LIB = lib
     return lib
 
@@ -147,35 +170,38 @@ def synth_state(lat, lon, nvl: int, ntra: int, seed: int = 20240601, member: int
     return st
 
 
+def _grid_desc(grid: _glvl.Grid, nvl: int, ntra: int, dt: float, thetac, dpsig):
+    """The fimhip_grid_desc of a grid, with the arrays it points to."""
+    keep = dict(
+        nprox=np.ascontiguousarray(grid.nprox, np.int32),
+        prox=np.ascontiguousarray(grid.prox, np.int32),
+        proxs=np.ascontiguousarray(grid.proxs, np.int32),
+        cs=np.ascontiguousarray(grid.cs, np.float64), sn=np.ascontiguousarray(grid.sn, np.float64),
+        sidevec_c=np.ascontiguousarray(grid.sidevec_c, np.float64),
+        sidevec_e=np.ascontiguousarray(grid.sidevec_e, np.float64),
+        sideln=np.ascontiguousarray(grid.sideln, np.float64),
+        hfcoef=np.ascontiguousarray(grid.hfcoef, np.float64),
+        area=np.ascontiguousarray(grid.area, np.float64),
+        lat=np.ascontiguousarray(grid.lat, np.float64),
+        lon=np.ascontiguousarray(grid.lon, np.float64),
+        thetac=np.ascontiguousarray(thetac, np.float64), dpsig=np.ascontiguousarray(dpsig, np.float64))
+    gd = GridDesc()
+    gd.nip, gd.nvl, gd.ntra, gd.dt = grid.nip, nvl, ntra, dt
+    for n in ("nprox", "prox", "proxs"):
+        setattr(gd, n, keep[n].ctypes.data_as(_PI))
+    for n in ("cs", "sn", "sidevec_c", "sidevec_e", "sideln", "hfcoef", "area", "lat", "lon", "thetac", "dpsig"):
+        setattr(gd, n, keep[n].ctypes.data_as(_PD))
+    return gd, keep
+
+
 class Instance:
     """One device instance of the dynamics on one GPU (fimhip_create)."""
 
     def __init__(self, grid: _glvl.Grid, nvl: int, ntra: int, dt: float, thetac, dpsig, device: int = 0):
         self.lib = load_library()
         self.grid, self.nvl, self.ntra, self.dt = grid, nvl, ntra, dt
-        nip = grid.nip
-        self.nip = nip
-        keep = dict(
-            nprox=np.ascontiguousarray(grid.nprox, np.int32),
-            prox=np.ascontiguousarray(grid.prox, np.int32),
-            proxs=np.ascontiguousarray(grid.proxs, np.int32),
-            cs=np.ascontiguousarray(grid.cs, np.float64), sn=np.ascontiguousarray(grid.sn, np.float64),
-            sidevec_c=np.ascontiguousarray(grid.sidevec_c, np.float64),
-            sidevec_e=np.ascontiguousarray(grid.sidevec_e, np.float64),
-            sideln=np.ascontiguousarray(grid.sideln, np.float64),
-            hfcoef=np.ascontiguousarray(grid.hfcoef, np.float64),
-            area=np.ascontiguousarray(grid.area, np.float64),
-            lat=np.ascontiguousarray(grid.lat, np.float64),
-            lon=np.ascontiguousarray(grid.lon, np.float64),
-            thetac=np.ascontiguousarray(thetac, np.float64), dpsig=np.ascontiguousarray(dpsig, np.float64))
-        self._keep = keep
-        gd = GridDesc()
-        gd.nip, gd.nvl, gd.ntra, gd.dt = nip, nvl, ntra, dt
-        for n in ("nprox", "prox", "proxs"):
-            setattr(gd, n, keep[n].ctypes.data_as(_PI))
-        for n in ("cs", "sn", "sidevec_c", "sidevec_e", "sideln", "hfcoef", "area", "lat", "lon", "thetac",
-                  "dpsig"):
-            setattr(gd, n, keep[n].ctypes.data_as(_PD))
+        self.nip = grid.nip
+        gd, self._keep = _grid_desc(grid, nvl, ntra, dt, thetac, dpsig)
         h = ct.c_void_p()
         _chk(self.lib.fimhip_create(ct.byref(gd), device, ct.byref(h)), "fimhip_create")
         self.h = h
@@ -293,3 +319,102 @@ class Model(Instance):
         t = np.zeros(self.ntra)
         _chk(self.lib.fimhip_globsum(self.h, ct.byref(m), _p(t)), "globsum")
         return m.value, t
+
+
+class Ensemble:
+    """Members advanced together in the batched mixed-precision form
+    (fimhip_ens_*): FP32 winds, tracers and edge values, FP64 thickness,
+    pressure, Exner function, Montgomery potential and hybgen."""
+
+    FIELDS_OUT = ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp", "ws", "relvor", "sdot", "psrf")
+
+    def __init__(self, grid: _glvl.Grid, members: int, nvl: int, ntra: int, dt: float, thetac, dpsig,
+                 device: int = 0):
+        self.lib = load_library()
+        self.grid, self.members, self.nvl, self.ntra, self.dt = grid, members, nvl, ntra, dt
+        self.nip = grid.nip
+        gd, self._keep = _grid_desc(grid, nvl, ntra, dt, thetac, dpsig)
+        h = ct.c_void_p()
+        _chk(self.lib.fimhip_ens_create(ct.byref(gd), members, device, ct.byref(h)), "fimhip_ens_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fimhip_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synth(self, member: int, seed: int = 20240601, synth_member: int | None = None, flags: int = 0):
+        sm = member if synth_member is None else synth_member
+        _chk(self.lib.fimhip_ens_synth(self.h, member, seed, sm, flags), "ens_synth")
+
+    def load_member(self, member: int, st: dict):
+        a = [np.ascontiguousarray(st[n], np.float64) for n in ("us", "vs", "dp", "pr", "ex", "ph", "tr")]
+        _chk(self.lib.fimhip_ens_load_member(self.h, member, *map(_p, a)), "ens_load_member")
+
+    def init_state(self, init_hybgen: bool = True):
+        _chk(self.lib.fimhip_ens_init_state(self.h, 1 if init_hybgen else 0), "ens_init_state")
+
+    def step(self, nsteps: int = 1, mode: int = 0):
+        _chk(self.lib.fimhip_ens_step(self.h, nsteps, mode), "ens_step")
+
+    def use_graph(self, on: bool):
+        _chk(self.lib.fimhip_ens_use_graph(self.h, 1 if on else 0), "ens_use_graph")
+
+    def sync(self):
+        _chk(self.lib.fimhip_ens_sync(self.h), "ens_sync")
+
+    def shape(self, name: str):
+        nip, nvl, ntra = self.nip, self.nvl, self.ntra
+        c, l = (nip, nvl), (nip, nvl + 1)
+        return dict(us=c, vs=c, dp=c, mp=c, ws=c, relvor=c, pr=l, ex=l, ph=l, sdot=l, tr=(ntra,) + c,
+                    trdp=(ntra,) + c, psrf=(nip,))[name]
+
+    def download(self, member: int, name: str) -> np.ndarray:
+        a = np.empty(self.shape(name))
+        _chk(self.lib.fimhip_ens_download(self.h, member, FIELD_ID[name], _p(a)), "ens_download " + name)
+        return a
+
+    def globsum(self):
+        """(mass[m], trtot[m, t], mass32[m], trtot32[m, t]): FP64 sums and FP32
+        Kahan-compensated sums of every member."""
+        m, t = np.zeros(self.members), np.zeros((self.members, self.ntra))
+        m32, t32 = np.zeros(self.members), np.zeros((self.members, self.ntra))
+        _chk(self.lib.fimhip_ens_globsum(self.h, _p(m), _p(t), _p(m32), _p(t32)), "ens_globsum")
+        return m, t, m32, t32
+
+    def kernel_names(self):
+        return [self.lib.fimhip_ens_kernel_name(k).decode() for k in range(self.lib.fimhip_ens_kernel_count())]
+
+    def tiles(self) -> dict:
+        return {n: self.lib.fimhip_ens_get_tile(self.h, k) for k, n in enumerate(self.kernel_names())}
+
+    def set_tiles(self, tiles: dict):
+        for k, n in enumerate(self.kernel_names()):
+            if n in tiles:
+                _chk(self.lib.fimhip_ens_set_tile(self.h, k, int(tiles[n])), "ens_set_tile")
+
+    def autotune(self, mode: int = 0, reps: int = 2) -> np.ndarray:
+        """Pick each kernel's fastest launch shape (advances the state);
+        returns the kernel x tile table of mean milliseconds."""
+        nk = self.lib.fimhip_ens_kernel_count()
+        t = np.full((nk, 8), np.nan)
+        _chk(self.lib.fimhip_ens_autotune(self.h, mode, reps, _p(t)), "ens_autotune")
+        return t
+
+    def profile(self, on: bool = True):
+        _chk(self.lib.fimhip_ens_profile(self.h, 1 if on else 0), "ens_profile")
+
+    def profile_report(self) -> dict:
+        buf = ct.create_string_buffer(1 << 16)
+        _chk(self.lib.fimhip_ens_profile_report(self.h, buf, len(buf)), "ens_profile_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split()
+            out[name] = (int(n), float(ms))
+        return out

@@ -150,6 +150,52 @@ int fimhip_synth_state(int nip, int nvl, int ntra, const double *lat, const doub
 /* the same, written straight into the instance's device state */
 int fimhip_synth_device(fimhip_ctx *ctx, unsigned long long seed, int member, int flags);
 
+/* ---- batched mixed-precision form ---------------------------------------
+ * Several members on the same grid (an ensemble, or instances that differ
+ * only in their state) advanced together by one set of kernel launches per
+ * step.  Winds, tracers, tracer mass, their AB histories, the edge values and
+ * the FCT work arrays are FP32; layer thickness and its AB history, pressure,
+ * Exner function, Montgomery potential, geopotential and all of hybgen are
+ * FP64.  Each edge's FP32 mass flux is formed by one of its two cells and
+ * negated by the other, so the mass fluxes cancel exactly in the global sum.
+ * No reference routine has this form; it runs the same step as fimhip_step
+ * (dyn_run.F90:119-425) for every member.  Fields are exchanged per member in
+ * the plain layout, widened to FP64. */
+typedef struct fimhip_ens fimhip_ens;
+int fimhip_ens_create(const fimhip_grid_desc *grid, int members, int device, fimhip_ens **out);
+int fimhip_ens_destroy(fimhip_ens *ens);
+/* seeded state (as fimhip_synth_state with synth_member and flags) into member 'member' */
+int fimhip_ens_synth(fimhip_ens *ens, int member, unsigned long long seed, int synth_member, int flags);
+/* host FP64 arrays of one member (layouts as fimhip_upload) */
+int fimhip_ens_load_member(fimhip_ens *ens, int member, const double *us, const double *vs, const double *dp,
+                           const double *pr, const double *ex, const double *ph, const double *tr);
+/* dyn_init start-up for every member (dyn_init.F90:870-1094); resets the step counter */
+int fimhip_ens_init_state(fimhip_ens *ens, int init_hybgen);
+/* nsteps dynamics steps of every member, asynchronous; the step counter and
+ * the AB slots advance on the device (abstart.F90:38-66), so the step is
+ * captured once as a HIP graph and replayed (mode 0 full, 1 transport only) */
+int fimhip_ens_step(fimhip_ens *ens, int nsteps, int mode);
+int fimhip_ens_use_graph(fimhip_ens *ens, int on);
+int fimhip_ens_sync(fimhip_ens *ens);
+/* per-member element count of FIMHIP_US, VS, DP, PR, EX, PH, MP, TR, TRDP, WS, RELVOR, SDOT, PSRF */
+long long fimhip_ens_field_size(const fimhip_ens *ens, int field);
+int fimhip_ens_download(fimhip_ens *ens, int member, int field, double *host);
+/* globsum.F90:70-101 per member: mass[m] and trtot[m*ntra+t] from FP64 sums;
+ * mass32/trtot32 (may be NULL) from FP32 Kahan-compensated sums */
+int fimhip_ens_globsum(fimhip_ens *ens, double *mass, double *trtot, double *mass32, double *trtot32);
+/* launch shapes: kernel k in 0..fimhip_ens_kernel_count()-1, tile 0..7 =
+ * (1,2,4,8 columns per block) x (XCD-contiguous block order off/on) */
+int fimhip_ens_kernel_count(void);
+const char *fimhip_ens_kernel_name(int kernel);
+int fimhip_ens_set_tile(fimhip_ens *ens, int kernel, int tile);
+int fimhip_ens_get_tile(const fimhip_ens *ens, int kernel);
+/* time every tile of every kernel over reps steps and keep each kernel's
+ * fastest; advances the state (reload it afterwards); times may be NULL or
+ * receive kernel_count x 8 mean milliseconds */
+int fimhip_ens_autotune(fimhip_ens *ens, int mode, int reps, double *times);
+int fimhip_ens_profile(fimhip_ens *ens, int on);
+int fimhip_ens_profile_report(fimhip_ens *ens, char *buf, int len);
+
 /* x**y as computed on the device (the host build of the same routine) */
 double fimhip_pow(double x, double y);
 const char *fimhip_version(void);
