@@ -10,6 +10,9 @@
 #include "fimhip_internal.h"
 #include "fim_pow.h"
 
+#include <stdlib.h>
+#include <string.h>
+
 namespace fimhip {
 
 // Column indices run over every column of every member (ncol = members * nip,
@@ -76,8 +79,7 @@ __device__ void inflate(const double *dpsig, int nvl, double psurf, double *exne
       exner[k] = exner[k - 1] - dp1;
     } else {
       if (k > last) {
-        double p2 = 1.;
-        for (int i = 0; i < k - last; i++) p2 = p2 * 2.;
+        const double p2 = ldexp(1., k - last);   // 2**(k-last), exact
         dp0 = dp0 / dmin(10., 1.25 * p2);
         dp0 = dmax(dp0, RD * 100. / P1000);
       }
@@ -101,8 +103,8 @@ __device__ void inflate(const double *dpsig, int nvl, double psurf, double *exne
   }
 }
 
-__device__ void equilb(const double *thet, double *pnew, int kk, double slak) {
-  for (int k = 3; k <= kk - 2; k++) {
+__device__ void equilb(const double *thet, double *pnew, int kk, double slak, int k0 = 3) {
+  for (int k = k0; k <= kk - 2; k++) {
     if (pnew[k - 3] > pnew[0] - THIN) continue;
     double dp1 = pnew[k - 3] - pnew[k - 2];
     double dp2 = pnew[k - 2] - pnew[k - 1];
@@ -357,11 +359,523 @@ __global__ void __launch_bounds__(64) k_remap(HybCols<V> s, const StepCtl *ctl, 
   }
 }
 
+
+// ===========================================================================
+// The same column algorithm with one wavefront per column (lane = level) and
+// the column in LDS.  Every element is computed by the same operations in the
+// same order as above; the loops of the reference are mapped as follows:
+//  * element-wise loops (heat diffusion, slak, miglim, pow, the PLM slopes,
+//    fluxes and outputs) run one level per lane;
+//  * running minima (thcol, oldth, the final Exner monotonicity) are wavefront
+//    scans, exact because min is;
+//  * restp_1d: each target layer's integration range starts where the
+//    previous one stopped, which for non-decreasing targets is found by a
+//    binary search, so each lane integrates its own layer in the reference
+//    order (a lane-0 sweep takes over for non-monotone targets);
+//  * the excol fix and equilb change nothing unless their condition holds on
+//    the unmodified column somewhere; lane 0 then runs the sweep from the
+//    first such level;
+//  * the two fill sweeps of pcwise: a cell's update ignores its neighbour
+//    whenever neighbour*1e-22 is provably below half an ulp of the cell's own
+//    term (bounded by the running maximum of the column); the remaining
+//    chains are resolved neighbour by neighbour in sweep order;
+//  * inflate, the top-of-column loop and the theta redistribution run on
+//    lane 0.
+constexpr int HW = 4;   // columns (wavefronts) per block
+
+__device__ __forceinline__ void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ bool wany(bool p) { return __ballot(p) != 0ull; }
+__device__ __forceinline__ double w_suffix_min(double x, int lane) {
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_down(x, d, 64);
+    if (lane + d < 64) x = dmin(x, y);
+  }
+  return x;
+}
+__device__ __forceinline__ double w_prefix_min(double x, int lane) {
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(x, d, 64);
+    if (lane >= d) x = dmin(x, y);
+  }
+  return x;
+}
+__device__ __forceinline__ double w_suffix_max(double x, int lane) {
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_down(x, d, 64);
+    if (lane + d < 64) x = dmax(x, y);
+  }
+  return x;
+}
+__device__ __forceinline__ double w_prefix_max(double x, int lane) {
+  for (int d = 1; d < 64; d <<= 1) {
+    const double y = __shfl_up(x, d, 64);
+    if (lane >= d) x = dmax(x, y);
+  }
+  return x;
+}
+__device__ __forceinline__ int w_min_int(int x) {
+  for (int d = 1; d < 64; d <<= 1) x = min(x, __shfl_xor(x, d, 64));
+  return x;
+}
+
+// inflate (hybgen.F90:893-1007) for one column on lane 0, the per-level
+// constants prepared beforehand by all lanes: dp0b[k-1] = rd*dpsig(k)/p1000*
+// shrink (k = 1..nvl-1) and tap[k] = 1/(1+tq**2), tq = (nvl-k)/4 (k = 2..nvl),
+// each formed by the same operations as in inflate().
+__device__ void inflate_w(const double *dp0b, const double *tap, int nvl, double extop, double *exner) {
+  double dpsum = 0.;
+  int last = 1;
+  for (int k = 1; k <= nvl - 1; k++) {
+    double dp0 = dp0b[k - 1];
+    dpsum = dpsum + dp0;
+    const double exold = exner[k];
+    if (k <= KBL) {
+      last = k + 1;
+      exner[k] = exner[k - 1] - dp0;
+    } else {
+      if (k > last) {
+        const double p2 = ldexp(1., k - last);
+        dp0 = dp0 / dmin(10., 1.25 * p2);
+        dp0 = dmax(dp0, RD * 100. / P1000);
+      }
+      const double dp1 = cushn(dmax(0., exner[k - 1] - dmin(exold, exner[0] - dpsum)), dp0);
+      if (exner[k - 1] - dp1 < exold - THIN) {
+        if (k == last) last = k + 1;
+        exner[k] = dmax(extop, exner[k - 1] - dp1);
+      }
+    }
+  }
+  for (int k = nvl; k >= 2; k--) {
+    double q = exner[k - 1] / CP;
+    q = q * q * sqrt(q);
+    const double dp0 = RD * THKTOP / P1000 * tap[k] / q;
+    const double exold = exner[k - 1];
+    const double dp1 = cushn(dmax(0., exold - exner[k]), dp0);
+    if (exner[k] + dp1 > exold + THIN) exner[k - 1] = exner[k] + dp1;
+    if (exner[k - 1] < exner[k - 2] - dp0) break;
+  }
+}
+
+// all lanes: the per-level constants of inflate_w for surface pressure psurf
+__device__ __forceinline__ void inflate_prep(const double *dpsig, int nvl, double psurf, double *dp0b, double *tap,
+                                             int lane) {
+  const double top = .2;
+  const double shrink = dmin(1., (psurf / P1000 - top) / (1. - top));
+  for (int k = 1 + lane; k <= nvl; k += 64) {
+    if (k <= nvl - 1) dp0b[k - 1] = RD * ldc(dpsig, k - 1) / P1000 * shrink;
+    if (k >= 2) {
+      const double tq = .25 * (double)(nvl - k);
+      tap[k] = 1. / (1. + tq * tq);
+    }
+  }
+}
+
+// true when y*1e-22 cannot move t, for every |y| <= bound (|y*1e-22| < |t|*2^-55,
+// with margin for the growth of the bound along a sweep)
+__device__ __forceinline__ bool absorbs(double t, double bound) {
+  const double at = fabs(t);
+  return at > 1.e-280 && bound * (1. + 1.e-12) * (1.e-22 * 36028797018963968.) < at;
+}
+
+// pcwise for one column: lane j < kk holds layer j's value a; returns the
+// remapped value of layer j.  xold/xnew are the interface coordinates in LDS.
+__device__ double pcwise_w(int kk, const double *xold, const double *xnew, double a, int lane) {
+  const double small = 1.e-22;
+  const bool up = xold[0] < xold[kk];
+  auto zo = [&](int k) { return up ? xold[k] : -xold[k]; };
+  auto zn = [&](int k) { return up ? xnew[k] : -xnew[k]; };
+  const bool live = lane < kk;
+  const double f = live ? dmax(0., zo(lane + 1) - zo(lane)) : 0.;    // fco(lane)
+  // sweep 1, from the top: layer j (j <= kk-2) takes layer j+1's new value
+  double v;
+  {
+    const double smx = w_suffix_max(live ? fabs(a) : 0., lane);
+    double bnd = __shfl_down(smx, 1, 64);
+    if (lane == 63) bnd = 0.;
+    bool fin;
+    if (lane >= kk - 1) {
+      v = a;
+      fin = true;
+    } else {
+      const double t = a * f;
+      fin = absorbs(t, bnd);
+      v = fin ? t / (f + small) : a;
+    }
+    while (wany(!fin)) {
+      const double vu = __shfl_down(v, 1, 64);
+      const int fu = __shfl_down((int)fin, 1, 64);
+      if (!fin && fu) {
+        v = (a * f + vu * small) / (f + small);
+        fin = true;
+      }
+    }
+  }
+  // sweep 2, from the bottom: layer j (j >= 1) takes layer j-1's new value
+  double w;
+  {
+    const double a2 = v;
+    const double pmx = w_prefix_max(live ? fabs(a2) : 0., lane);
+    double bnd = __shfl_up(pmx, 1, 64);
+    if (lane == 0) bnd = 0.;
+    bool fin;
+    if (lane == 0 || !live) {
+      w = a2;
+      fin = true;
+    } else {
+      const double t = a2 * f;
+      fin = absorbs(t, bnd);
+      w = fin ? t / (f + small) : a2;
+    }
+    while (wany(!fin)) {
+      const double wd = __shfl_up(w, 1, 64);
+      const int fd = __shfl_up((int)fin, 1, 64);
+      if (!fin && fd) {
+        w = (a2 * f + wd * small) / (f + small);
+        fin = true;
+      }
+    }
+  }
+  // PLM slope of layer lane (1-based k = lane + 1)
+  const double fm = __shfl_up(w, 1, 64), fp = __shfl_down(w, 1, 64);
+  double b = 0.;
+  if (live && lane != 0 && lane != kk - 1) {
+    if (!(w <= dmin(fm, fp) || w >= dmax(fm, fp))) {
+      if ((fp - fm) * (fm + fp - 2. * w) > 0.) b = w - fm;
+      else b = fp - w;
+    }
+  }
+  // flux through interface lane (1-based vertfx(lane + 1)), lanes 1..kk-1
+  double vf = 0.;
+  {
+    const double wl = __shfl_up(w, 1, 64), bl = __shfl_up(b, 1, 64), fl = __shfl_up(f, 1, 64);
+    if (live && lane >= 1) {
+      double slab = 0., amount = 0.;
+      const double dz = zn(lane) - zo(lane);
+      if (dz > 0.) {
+        const double fc = f;                       // fco(layer lane)
+        if (fc > 0.) {
+          double dslab = dmin(slab + fc, dz) - dmin(slab, dz);
+          double dx = dslab / fc;
+          double fcdx = w + b * .5 * (dx - 1.) + 0. * (.25 - dx * (.5 - dx * (1. / 3.)));
+          amount = amount + fcdx * dslab;
+          slab = slab + dslab;
+        }
+      } else if (dz < 0.) {
+        const double fc = fl;                      // fco(layer lane - 1)
+        if (fc > 0.) {
+          double dslab = dmin(slab + fc, -dz) - dmin(slab, -dz);
+          double dx = dslab / fc;
+          double fcdx = wl + bl * .5 * (1. - dx) + 0. * (.25 - dx * (.5 - dx * (1. / 3.)));
+          amount = amount + fcdx * dslab;
+          slab = slab + dslab;
+        }
+      }
+      if (slab != 0.) vf = -dz * amount / slab;
+    }
+  }
+  double vhi = __shfl_down(vf, 1, 64);
+  if (lane >= kk - 1) vhi = 0.;
+  const double vlo = vf;
+  double y = 0.;
+  if (live) {
+    const double fn = dmax(0., zn(lane + 1) - zn(lane));
+    const double amount = w * f - (vhi - vlo);
+    y = (amount + w * small) / (fn + small);
+  }
+  return y;
+}
+
+// regrid_1d (hybgen.F90:207-512), one wavefront per column
+template <typename V>
+__global__ void __launch_bounds__(64 * HW) k_regrid_w(HybCols<V> s, const StepCtl *ctl, int its) {
+  __shared__ double sh[HW][7][LMAX];
+  const int ncol = s.ncol, nvl = s.nvl, nvlp1 = nvl + 1;
+  const int lane = threadIdx.x, c = threadIdx.y;
+  const int col = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * HW + c));
+  if (col >= ncol) return;
+  if (ctl) its = load_ctl(ctl).its;
+  double *thcol = sh[c][0], *excol = sh[c][1], *exwrk = sh[c][2], *exnew = sh[c][3], *thnew = sh[c][4],
+         *heatfx = sh[c][5], *oldth = sh[c][6];
+  const double *targt = s.thetac;
+  const double tolrnce = 0.001;
+  for (int k = lane; k < nvl; k += 64) thcol[k] = (double)TR(s.tr, 0, col, k);
+  for (int k = lane; k < nvlp1; k += 64) excol[k] = L2(s.ex, col, k);
+  const double psurf = L2(s.pr, col, 0);
+  const double eqlb_slak = .1 * (double)its / (10. + (double)its);
+  wsync();
+  {
+    bool t = false;
+    for (int k = 1 + lane; k <= nvl; k += 64) t = t || (excol[k - 1] < excol[k] - THIN);
+    if (wany(t)) {
+      if (lane == 0)
+        for (int k = 1; k <= nvl; k++)
+          if (excol[k - 1] < excol[k] - THIN) excol[k] = excol[k - 1];
+      wsync();
+    }
+  }
+  {
+    const double m = w_suffix_min(lane < nvl ? thcol[lane] : 1.e300, lane);
+    if (lane < nvl) thcol[lane] = m;
+  }
+  for (int k = lane; k < nvlp1; k += 64) exwrk[k] = excol[k];
+  wsync();
+  const double ttop = ldc(targt, nvl - 1);
+  if (lane == 0) {
+    for (int k = nvl; k >= 2; k--) {
+      if (thcol[k - 1] <= ttop + tolrnce) break;
+      double wgt1 = dmax(0., exwrk[k - 2] - exwrk[k - 1]);
+      double wgt2 = dmax(THIN, exwrk[k - 1] - exwrk[k]);
+      double try_ = (thcol[k - 2] * wgt1 + thcol[k - 1] * wgt2) / (wgt1 + wgt2);
+      if (try_ > ttop - tolrnce) {
+        exwrk[k - 1] = exwrk[nvlp1 - 1];
+        for (int k1 = k - 1; k1 <= nvl; k1++) thcol[k1 - 1] = try_;
+      } else {
+        exwrk[k - 1] = dmin(exwrk[k - 2], dmax(exwrk[k], (exwrk[k - 1] * (thcol[k - 1] - thcol[k - 2]) +
+                                                         exwrk[k] * (ttop - thcol[k - 1])) /
+                                                            (ttop - thcol[k - 2])));
+        for (int k1 = k; k1 <= nvl; k1++) thcol[k1 - 1] = ttop;
+        break;
+      }
+    }
+  }
+  wsync();
+  // restp_1d(thcol, exwrk, nvl, thnew, exnew, targt, nvl)
+  {
+    const int kold = nvl, knew = nvl;
+    const double m = w_suffix_min(lane < kold ? thcol[lane] : 1.e300, lane);
+    if (lane < kold) oldth[lane] = m;
+    wsync();
+    const double o0 = oldth[0], on = oldth[kold - 1];
+    bool nonmono = false;
+    for (int k = lane; k < knew; k += 64) {
+      double t = ldc(targt, k);
+      if (k == 0) t = dmin(dmin(o0, on), ldc(targt, 0));
+      if (k == knew - 1) t = dmax(dmax(o0, on), ldc(targt, knew - 1));
+      thnew[k] = t;
+    }
+    wsync();
+    for (int k = 1 + lane; k < knew; k += 64) nonmono = nonmono || (thnew[k] < thnew[k - 1]);
+    if (wany(nonmono)) {
+      if (lane == 0) restp_1d(thcol, exwrk, kold, thnew, exnew, targt, knew);
+      wsync();
+    } else {
+      if (lane == 0) {
+        exnew[0] = exwrk[0];
+        exnew[knew] = exwrk[kold];
+      }
+      bool copy = false;
+      for (int k = 1 + lane; k <= knew - 1; k += 64) {
+        const double tk = thnew[k], tkm = thnew[k - 1];
+        int ko = 1;
+        if (k >= 2) {            // where layer k-1's integration stopped
+          int lo = 0, hi = kold;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (oldth[mid] < tkm) lo = mid + 1;
+            else hi = mid;
+          }
+          ko = lo + 1 < kold ? lo + 1 : kold;
+        }
+        double pinteg = 0., tha, thb = tkm;
+        for (;;) {
+          tha = thb;
+          thb = dmin(tk, dmax(tkm, oldth[ko - 1]));
+          pinteg = pinteg + exwrk[ko - 1] * (thb - tha);
+          if (oldth[ko - 1] < tk) {
+            if (ko < kold) {
+              ko = ko + 1;
+              continue;
+            }
+            tha = thb;
+            thb = tk;
+            pinteg = pinteg + exwrk[kold] * (thb - tha);
+          }
+          break;
+        }
+        if (tk > tkm) exnew[k] = pinteg / (tk - tkm);
+        else copy = true;
+      }
+      if (wany(copy)) {
+        wsync();
+        if (lane == 0)
+          for (int k = 1; k <= knew - 1; k++)
+            if (!(thnew[k] > thnew[k - 1])) exnew[k] = exnew[k - 1];
+      }
+      wsync();
+    }
+  }
+  // heat diffusion, up to 3 iterations
+  if (lane == 0) {
+    heatfx[nvl - 1] = 0.;
+    heatfx[0] = 0.;
+  }
+  for (int iter = 1; iter <= 3; iter++) {
+    for (int k = lane; k < nvlp1; k += 64) exwrk[k] = exnew[k];
+    wsync();
+    for (int k = 2 + lane; k <= nvl - 1; k += 64) {
+      double h = 0.;
+      if (exwrk[k - 1] < exwrk[0] - .01 && exwrk[k] > exwrk[nvlp1 - 1] + .01) {
+        double cnv = GRVITY / ldc(targt, k - 1);
+        h = .1 * cnv * cnv * .5 * (ldc(targt, k) - ldc(targt, k - 2)) / dmax(.03 * cnv, exwrk[k - 1] - exwrk[k]);
+      }
+      heatfx[k - 1] = h;
+    }
+    wsync();
+    for (int k = 1 + lane; k <= nvl - 1; k += 64) {
+      if (exwrk[k] < exwrk[0] - .01 && exwrk[k] > exwrk[nvlp1 - 1] + .01)
+        exnew[k] = dmax(exwrk[k + 1], dmin(exwrk[k - 1], exwrk[k] + (heatfx[k] - heatfx[k - 1]) /
+                                                                       (ldc(targt, k) - ldc(targt, k - 1))));
+    }
+    wsync();
+    bool ev = false;
+    for (int k = 2 + lane; k <= nvlp1; k += 64) ev = ev || (exnew[k - 1] > exnew[k - 2] + THIN);
+    if (!wany(ev)) break;
+  }
+  // equilb from the first level where it would move an interface
+  {
+    int first = 1 << 30;
+    for (int k = 3 + lane; k <= nvl - 2; k += 64) {
+      if (exnew[k - 3] > exnew[0] - THIN) continue;
+      const double dp1 = exnew[k - 3] - exnew[k - 2], dp2 = exnew[k - 2] - exnew[k - 1];
+      const double dp3 = exnew[k - 1] - exnew[k], dp4 = exnew[k] - exnew[k + 1], dp5 = exnew[k + 1] - exnew[k + 2];
+      if (dp2 > dp1 && dp4 > dp5) {
+        const double goal = .5 * (dp3 + dmin(dp2, dp4));
+        if (dp2 > goal && dp4 > goal) first = min(first, k);
+      }
+    }
+    first = w_min_int(first);
+    if (first <= nvl - 2) {
+      if (lane == 0) equilb(thnew, exnew, nvl, eqlb_slak, first);
+      wsync();
+    }
+  }
+  inflate_prep(s.dpsig, nvl, psurf, heatfx, oldth, lane);
+  wsync();
+  if (lane == 0) inflate_w(heatfx, oldth, nvl, CP * fim_pow(PTOP / P1000, RD / CP), exnew);
+  wsync();
+  for (int k = lane; k < nvlp1; k += 64) L2(s.sdot, col, k) = exnew[k];   // pre-smoothing Exner, kept in sdot
+}
+
+// remap_1d (hybgen.F90:515-789), one wavefront per column
+template <typename V>
+__global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl *ctl, int its) {
+  __shared__ double sh[HW][7][LMAX];
+  const int ncol = s.ncol, nvl = s.nvl, nvlp1 = nvl + 1, ntra = s.ntra;
+  const int lane = threadIdx.x, c = threadIdx.y;
+  const int col = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * HW + c));
+  if (col >= ncol) return;
+  if (ctl) its = load_ctl(ctl).its;
+  double *prcol = sh[c][0], *pk1col = sh[c][1], *exnew = sh[c][2], *prnew = sh[c][3], *pk1new = sh[c][4],
+         *th1 = sh[c][5], *exold = sh[c][6];
+  const double *targt = s.thetac;
+  const double small = 1.e-6;
+  for (int k = lane; k < nvlp1; k += 64) {
+    const double eo = L2(s.ex, col, k);
+    const double p = L2(s.pr, col, k);
+    exold[k] = eo;
+    prcol[k] = p;
+    exnew[k] = L2(s.exsmo, col, k);
+    pk1col[k] = eo * p / CP;
+  }
+  inflate_prep(s.dpsig, nvl, L2(s.pr, col, 0), prnew, pk1new, lane);
+  wsync();
+  if (lane == 0) inflate_w(prnew, pk1new, nvl, CP * fim_pow(PTOP / P1000, RD / CP), exnew);
+  wsync();
+  for (int k = lane; k < nvlp1; k += 64) {
+    double e = exnew[k];
+    if (its > 0) e = SLAK * e + (1. - SLAK) * exold[k];
+    if (k >= 1 && k <= nvlp1 - 2) e = dmax(.0001 * exold[k] + .9999 * exold[k + 1], dmin(.0001 * exold[k] + .9999 * exold[k - 1], e));
+    exnew[k] = e;
+    const double pn = P1000 * fim_pow(e / CP, CP / RD);
+    prnew[k] = pn;
+    pk1new[k] = e * pn / CP;
+  }
+  wsync();
+  for (int k = lane; k < nvl; k += 64) C2(s.dp, col, k) = prnew[k] - prnew[k + 1];
+  wsync();
+  if (lane == 0) {
+    prnew[nvlp1 - 1] = prcol[nvlp1 - 1];
+    pk1new[nvlp1 - 1] = pk1col[nvlp1 - 1];
+    prnew[0] = prcol[0];
+    pk1new[0] = pk1col[0];
+  }
+  wsync();
+  // u, v and tracers 2..ntra are remapped in pressure
+  {
+    const bool live = lane < nvl;
+    double y = pcwise_w(nvl, prcol, prnew, live ? (double)C2(s.us_in, col, lane) : 0., lane);
+    if (live) C2(s.us, col, lane) = (V)y;
+    y = pcwise_w(nvl, prcol, prnew, live ? (double)C2(s.vs_in, col, lane) : 0., lane);
+    if (live) C2(s.vs, col, lane) = (V)y;
+    for (int n = 1; n < ntra; n++) {
+      y = pcwise_w(nvl, prcol, prnew, live ? (double)TR(s.tr, n, col, lane) : 0., lane);
+      if (live) TR(s.tr, n, col, lane) = (V)y;
+    }
+    // theta is remapped in p^(kappa+1)
+    y = pcwise_w(nvl, pk1col, pk1new, live ? (double)TR(s.tr, 0, col, lane) : 0., lane);
+    if (live) th1[lane] = y;
+  }
+  wsync();
+  if (lane == 0) {
+    for (int k = nvl; k >= 3; k--) {
+      double dplo = dmax(pk1new[k - 2] - pk1new[k - 1], pk1new[0] * small);
+      double dpup = dmax(pk1new[k - 1] - pk1new[k], pk1new[0] * small);
+      double tha = th1[k - 2], thb = th1[k - 1];
+      double devilo = (tha - ldc(targt, k - 2)) * dplo;
+      double deviup = (thb - ldc(targt, k - 1)) * dpup;
+      if (deviup > 0. && devilo < 0.) {
+        th1[k - 2] = tha + dmin(deviup, -devilo) / dplo;
+        th1[k - 1] = thb - dmin(deviup, -devilo) / dpup;
+      } else if (deviup < 0. && devilo > 0.) {
+        th1[k - 2] = tha - dmin(devilo, -deviup) / dplo;
+        th1[k - 1] = thb + dmin(devilo, -deviup) / dpup;
+      }
+    }
+  }
+  wsync();
+  if (lane < nvl) TR(s.tr, 0, col, lane) = (V)th1[lane];
+  // exnew(k) = min over exnew(0..k): the reference's top-down monotonicity fix
+  {
+    const double m = w_prefix_min(lane < nvlp1 ? exnew[lane] : 1.e300, lane);
+    double mtop = 0.;
+    if (nvlp1 > 64) mtop = dmin(__shfl(m, 63, 64), exnew[64]);
+    wsync();
+    if (lane < nvlp1) exnew[lane] = m;
+    if (nvlp1 > 64 && lane == 0) exnew[64] = mtop;
+  }
+  wsync();
+  for (int k = lane; k < nvlp1; k += 64) {
+    L2(s.pr, col, k) = prnew[k];
+    L2(s.sdot, col, k) = exnew[k] - exold[k];
+    L2(s.ex, col, k) = exnew[k];
+  }
+}
+
+// FIMHIP_HYBGEN=thread selects the one-thread-per-column kernels
+static bool hybgen_thread_per_column() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("FIMHIP_HYBGEN");
+    v = (e && strcmp(e, "thread") == 0) ? 1 : 0;
+  }
+  return v == 1;
+}
+
 template <typename V>
 void launch_hybgen_cols(const HybCols<V> &h, const StepCtl *ctl, int its, hipStream_t st) {
-  KT_LAUNCH("k_regrid", st, k_regrid<V><<<(h.ncol + 63) / 64, 64, 0, st>>>(h, ctl, its));
+  if (hybgen_thread_per_column()) {
+    KT_LAUNCH("k_regrid", st, k_regrid<V><<<(h.ncol + 63) / 64, 64, 0, st>>>(h, ctl, its));
+    KT_LAUNCH("k_dffusn_lev", st, k_dffusn_lev<V><<<(h.ncol + 3) / 4, dim3(64, 4), 0, st>>>(h));
+    KT_LAUNCH("k_remap", st, k_remap<V><<<(h.ncol + 63) / 64, 64, 0, st>>>(h, ctl, its));
+    return;
+  }
+  KT_LAUNCH("k_regrid", st, k_regrid_w<V><<<(h.ncol + HW - 1) / HW, dim3(64, HW), 0, st>>>(h, ctl, its));
   KT_LAUNCH("k_dffusn_lev", st, k_dffusn_lev<V><<<(h.ncol + 3) / 4, dim3(64, 4), 0, st>>>(h));
-  KT_LAUNCH("k_remap", st, k_remap<V><<<(h.ncol + 63) / 64, 64, 0, st>>>(h, ctl, its));
+  KT_LAUNCH("k_remap", st, k_remap_w<V><<<(h.ncol + HW - 1) / HW, dim3(64, HW), 0, st>>>(h, ctl, its));
 }
 template void launch_hybgen_cols<float>(const HybCols<float> &, const StepCtl *, int, hipStream_t);
 
