@@ -111,7 +111,7 @@ extern "C" int fimhip_ens_create(const fimhip_grid_desc *gd, int members, int de
       {&s.tr_rm, cz * ntra}};
   struct { double **p; size_t n; } f64[] = {
       {&s.dp, cz}, {&s.pr, lz}, {&s.ex, lz}, {&s.ph, lz}, {&s.mp, cz}, {&s.sdot, lz}, {&s.exsmo, lz},
-      {&s.psrf, ncol}, {&s.ptdcy, 2 * ncol}, {&s.dp_tdcy, 3 * cz}, {&s.mont_edg, ez}};
+      {&s.psrf, ncol}, {&s.ptdcy, 2 * ncol}, {&s.dp_tdcy, 3 * cz}, {&s.mont_edg, ez}, {&s.thbuf, cz}};
   for (auto &x : f32)
     if ((r = a.alloc(x.p, x.n))) { fimhip_ens_destroy(e); return r; }
   for (auto &x : f64)
@@ -185,7 +185,7 @@ static HybCols<float> hyb_cols(fimhip_ens *e) {
   const EnsState &s = e->s;
   const DevGrid &d = e->g.d;
   return HybCols<float>{d.nip, s.ncol, d.nvl, d.ntra, s.u2, s.v2, s.us, s.vs, s.tr, s.dp, s.pr, s.ex, s.sdot,
-                        s.exsmo, d.thetac, d.dpsig, d.smoo, d.sideln, d.rarea, d.prox, d.nprox};
+                        s.exsmo, s.thbuf, d.thetac, d.dpsig, d.smoo, d.sideln, d.rarea, d.prox, d.nprox};
 }
 
 // dyn_init start-up for every member (dyn_init.F90:870-1094): optional

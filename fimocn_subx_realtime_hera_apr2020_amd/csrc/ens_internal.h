@@ -31,7 +31,7 @@ struct EnsGrid {
 struct EnsState {
   int members, ncol;
   float *us, *vs, *u2, *v2;
-  double *dp, *pr, *ex, *ph, *mp, *sdot, *exsmo, *psrf, *ptdcy;
+  double *dp, *pr, *ex, *ph, *mp, *sdot, *exsmo, *psrf, *ptdcy, *thbuf;
   float *tr, *trdp;
   float *u_tdcy, *v_tdcy, *dpl_tdcy;
   double *dp_tdcy;

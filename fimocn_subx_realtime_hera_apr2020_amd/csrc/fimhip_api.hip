@@ -156,7 +156,7 @@ extern "C" int fimhip_create(const fimhip_grid_desc *gd, int device, fimhip_ctx 
       {&s.mont_edg, ez}, {&s.uvsq_edg, ez}, {&s.trc_edg, ez * ntra}, {&s.antifx, ez}, {&s.delp_lo, cz},
       {&s.r_plus, cz}, {&s.r_mnus, cz}, {&s.antiflx, ez * ntra}, {&s.trcr_lo, cz * ntra},
       {&s.trmax, cz * ntra}, {&s.trmin, cz * ntra}, {&s.tr_rp, cz * ntra}, {&s.tr_rm, cz * ntra},
-      {&s.u2, cz}, {&s.v2, cz}, {&s.exsmo, lz}};
+      {&s.u2, cz}, {&s.v2, cz}, {&s.exsmo, lz}, {&s.thbuf, cz}};
   for (auto &a : arrs)
     if ((r = c->arena.alloc(a.p, a.n))) { fimhip_destroy(c); return r; }
   c->gsum_blocks = 1024;

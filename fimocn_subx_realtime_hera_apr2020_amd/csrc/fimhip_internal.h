@@ -64,7 +64,7 @@ struct DevState {
   // work
   double *antifx, *delp_lo, *r_plus, *r_mnus;
   double *antiflx, *trcr_lo, *trmax, *trmin, *tr_rp, *tr_rm;   // per tracer
-  double *u2, *v2, *exsmo;
+  double *u2, *v2, *exsmo, *thbuf;
 };
 
 struct AB {
@@ -97,6 +97,7 @@ struct HybCols {
   const V *us_in, *vs_in;   // winds remapped by remap_1d (may be the output arrays)
   V *us, *vs, *tr;
   double *dp, *pr, *ex, *sdot, *exsmo;
+  double *thbuf;            // (nvl, ncol) work column for the split form, or null
   const double *thetac, *dpsig, *smoo, *sideln, *rarea;
   const int *prox, *nprox;
 };

@@ -590,7 +590,10 @@ __device__ double pcwise_w(int kk, const double *xold, const double *xnew, doubl
 }
 
 // regrid_1d (hybgen.F90:207-512), one wavefront per column
-template <typename V>
+// With SPLIT the column-serial equilb and inflate are left to
+// k_regrid_tail, which runs them one thread per column with the column in
+// registers; the Exner function after the heat diffusion goes to sdot.
+template <typename V, bool SPLIT>
 __global__ void __launch_bounds__(64 * HW) k_regrid_w(HybCols<V> s, const StepCtl *ctl, int its) {
   __shared__ double sh[HW][7][LMAX];
   const int ncol = s.ncol, nvl = s.nvl, nvlp1 = nvl + 1;
@@ -735,6 +738,10 @@ __global__ void __launch_bounds__(64 * HW) k_regrid_w(HybCols<V> s, const StepCt
     for (int k = 2 + lane; k <= nvlp1; k += 64) ev = ev || (exnew[k - 1] > exnew[k - 2] + THIN);
     if (!wany(ev)) break;
   }
+  if (SPLIT) {
+    for (int k = lane; k < nvlp1; k += 64) L2(s.sdot, col, k) = exnew[k];
+    return;
+  }
   // equilb from the first level where it would move an interface
   {
     int first = 1 << 30;
@@ -761,7 +768,11 @@ __global__ void __launch_bounds__(64 * HW) k_regrid_w(HybCols<V> s, const StepCt
 }
 
 // remap_1d (hybgen.F90:515-789), one wavefront per column
-template <typename V>
+// With SPLIT, inflate has been applied to exsmo by k_remap_inflate, and the
+// theta redistribution is left to k_remap_theta: the remapped theta goes to
+// thbuf and the layer depths in p**(kappa+1), max(pk1new(k)-pk1new(k+1),
+// pk1new(1)*1e-6), to exsmo.
+template <typename V, bool SPLIT>
 __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl *ctl, int its) {
   __shared__ double sh[HW][7][LMAX];
   const int ncol = s.ncol, nvl = s.nvl, nvlp1 = nvl + 1, ntra = s.ntra;
@@ -781,9 +792,11 @@ __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl
     exnew[k] = L2(s.exsmo, col, k);
     pk1col[k] = eo * p / CP;
   }
-  inflate_prep(s.dpsig, nvl, L2(s.pr, col, 0), prnew, pk1new, lane);
-  wsync();
-  if (lane == 0) inflate_w(prnew, pk1new, nvl, CP * fim_pow(PTOP / P1000, RD / CP), exnew);
+  if (!SPLIT) {
+    inflate_prep(s.dpsig, nvl, L2(s.pr, col, 0), prnew, pk1new, lane);
+    wsync();
+    if (lane == 0) inflate_w(prnew, pk1new, nvl, CP * fim_pow(PTOP / P1000, RD / CP), exnew);
+  }
   wsync();
   for (int k = lane; k < nvlp1; k += 64) {
     double e = exnew[k];
@@ -820,7 +833,12 @@ __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl
     if (live) th1[lane] = y;
   }
   wsync();
-  if (lane == 0) {
+  if (SPLIT) {
+    for (int k = lane; k < nvl; k += 64) {
+      C2(s.thbuf, col, k) = th1[k];
+      L2(s.exsmo, col, k) = dmax(pk1new[k] - pk1new[k + 1], pk1new[0] * small);
+    }
+  } else if (lane == 0) {
     for (int k = nvl; k >= 3; k--) {
       double dplo = dmax(pk1new[k - 2] - pk1new[k - 1], pk1new[0] * small);
       double dpup = dmax(pk1new[k - 1] - pk1new[k], pk1new[0] * small);
@@ -837,7 +855,7 @@ __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl
     }
   }
   wsync();
-  if (lane < nvl) TR(s.tr, 0, col, lane) = (V)th1[lane];
+  if (!SPLIT && lane < nvl) TR(s.tr, 0, col, lane) = (V)th1[lane];
   // exnew(k) = min over exnew(0..k): the reference's top-down monotonicity fix
   {
     const double m = w_prefix_min(lane < nvlp1 ? exnew[lane] : 1.e300, lane);
@@ -853,6 +871,142 @@ __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl
     L2(s.sdot, col, k) = exnew[k] - exold[k];
     L2(s.ex, col, k) = exnew[k];
   }
+}
+
+
+// ===========================================================================
+// The column-serial sweeps of hybgen, one thread per column with the column
+// in registers (NVL levels, every loop unrolled so each array index is a
+// compile-time constant), for the split form of the wave kernels above.  The
+// operations are those of equilb, inflate and the theta redistribution of
+// remap_1d, in the same order.
+template <int NVL>
+__device__ __forceinline__ void equilb_r(const double *targt, double *pnew, double slak) {
+#pragma unroll
+  for (int k = 3; k <= NVL - 2; k++) {
+    if (pnew[k - 3] > pnew[0] - THIN) continue;
+    const double dp1 = pnew[k - 3] - pnew[k - 2];
+    const double dp2 = pnew[k - 2] - pnew[k - 1];
+    const double dp3 = pnew[k - 1] - pnew[k];
+    const double dp4 = pnew[k] - pnew[k + 1];
+    const double dp5 = pnew[k + 1] - pnew[k + 2];
+    const double th2 = ldc(targt, k - 2), th3 = ldc(targt, k - 1), th4 = ldc(targt, k);
+    if (dp2 > dp1 && dp4 > dp5) {
+      const double goal = .5 * (dp3 + dmin(dp2, dp4));
+      if (dp2 > goal && dp4 > goal) {
+        double dis3 = dmin(dp2 - goal, goal - dp3) * slak;
+        double dis4 = dmin(dp4 - goal, goal - dp3) * slak;
+        if (th3 > th2 && th4 > th3) {
+          const double ratio = (th4 - th3) / (th3 - th2);
+          if (ratio > 1.) dis4 = dis3 / ratio;
+          else dis3 = dis4 * ratio;
+        }
+        pnew[k - 1] = pnew[k - 1] + dis3;
+        pnew[k] = pnew[k] - dis4;
+      }
+    }
+  }
+}
+
+template <int NVL>
+__device__ __forceinline__ void inflate_r(const double *dpsig, double psurf, double extop, double *exner) {
+  double dpsum = 0.;
+  const double top = .2;
+  const double shrink = dmin(1., (psurf / P1000 - top) / (1. - top));
+  int last = 1;
+#pragma unroll
+  for (int k = 1; k <= NVL - 1; k++) {
+    double dp0 = RD * ldc(dpsig, k - 1) / P1000 * shrink;
+    dpsum = dpsum + dp0;
+    const double exold = exner[k];
+    if (k <= KBL) {
+      last = k + 1;
+      exner[k] = exner[k - 1] - dp0;
+    } else {
+      if (k > last) {
+        const double p2 = ldexp(1., k - last);
+        dp0 = dp0 / dmin(10., 1.25 * p2);
+        dp0 = dmax(dp0, RD * 100. / P1000);
+      }
+      const double dp1 = cushn(dmax(0., exner[k - 1] - dmin(exold, exner[0] - dpsum)), dp0);
+      if (exner[k - 1] - dp1 < exold - THIN) {
+        if (k == last) last = k + 1;
+        exner[k] = dmax(extop, exner[k - 1] - dp1);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = NVL; k >= 2; k--) {
+    double q = exner[k - 1] / CP;
+    q = q * q * sqrt(q);
+    const double tq = .25 * (double)(NVL - k);
+    const double tap = 1. / (1. + tq * tq);
+    const double dp0 = RD * THKTOP / P1000 * tap / q;
+    const double exold = exner[k - 1];
+    const double dp1 = cushn(dmax(0., exold - exner[k]), dp0);
+    if (exner[k] + dp1 > exold + THIN) exner[k - 1] = exner[k] + dp1;
+    if (exner[k - 1] < exner[k - 2] - dp0) break;
+  }
+}
+
+// equilb + inflate after the wave part of regrid_1d (Exner function in sdot)
+template <int NVL, typename V>
+__global__ void __launch_bounds__(64) k_regrid_tail(HybCols<V> s, const StepCtl *ctl, int its, double extop) {
+  const int nvl = NVL, nvlp1 = NVL + 1;
+  const int col = blockIdx.x * 64 + threadIdx.x;
+  if (col >= s.ncol) return;
+  if (ctl) its = load_ctl(ctl).its;
+  double ex[NVL + 1];
+#pragma unroll
+  for (int k = 0; k <= NVL; k++) ex[k] = L2(s.sdot, col, k);
+  const double psurf = L2(s.pr, col, 0);
+  equilb_r<NVL>(s.thetac, ex, .1 * (double)its / (10. + (double)its));
+  inflate_r<NVL>(s.dpsig, psurf, extop, ex);
+#pragma unroll
+  for (int k = 0; k <= NVL; k++) L2(s.sdot, col, k) = ex[k];
+  (void)nvl;
+}
+
+// inflate of the smoothed Exner function at the start of remap_1d (in place)
+template <int NVL, typename V>
+__global__ void __launch_bounds__(64) k_remap_inflate(HybCols<V> s, double extop) {
+  const int nvlp1 = NVL + 1;
+  const int col = blockIdx.x * 64 + threadIdx.x;
+  if (col >= s.ncol) return;
+  double ex[NVL + 1];
+#pragma unroll
+  for (int k = 0; k <= NVL; k++) ex[k] = L2(s.exsmo, col, k);
+  inflate_r<NVL>(s.dpsig, L2(s.pr, col, 0), extop, ex);
+#pragma unroll
+  for (int k = 0; k <= NVL; k++) L2(s.exsmo, col, k) = ex[k];
+}
+
+// theta redistribution of remap_1d (hybgen.F90:515-789, final loop over
+// k = nvl..3), from thbuf and the layer depths left in exsmo
+template <int NVL, typename V>
+__global__ void __launch_bounds__(64) k_remap_theta(HybCols<V> s) {
+  const int nvl = NVL, nvlp1 = NVL + 1, ncol = s.ncol;
+  const int col = blockIdx.x * 64 + threadIdx.x;
+  if (col >= ncol) return;
+  double th[NVL];
+#pragma unroll
+  for (int k = 0; k < NVL; k++) th[k] = C2(s.thbuf, col, k);
+#pragma unroll
+  for (int k = NVL; k >= 3; k--) {
+    const double dplo = L2(s.exsmo, col, k - 2), dpup = L2(s.exsmo, col, k - 1);
+    const double tha = th[k - 2], thb = th[k - 1];
+    const double devilo = (tha - ldc(s.thetac, k - 2)) * dplo;
+    const double deviup = (thb - ldc(s.thetac, k - 1)) * dpup;
+    if (deviup > 0. && devilo < 0.) {
+      th[k - 2] = tha + dmin(deviup, -devilo) / dplo;
+      th[k - 1] = thb - dmin(deviup, -devilo) / dpup;
+    } else if (deviup < 0. && devilo > 0.) {
+      th[k - 2] = tha - dmin(devilo, -deviup) / dplo;
+      th[k - 1] = thb + dmin(devilo, -deviup) / dpup;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NVL; k++) TR(s.tr, 0, col, k) = (V)th[k];
 }
 
 // FIMHIP_HYBGEN=thread selects the one-thread-per-column kernels
@@ -873,15 +1027,27 @@ void launch_hybgen_cols(const HybCols<V> &h, const StepCtl *ctl, int its, hipStr
     KT_LAUNCH("k_remap", st, k_remap<V><<<(h.ncol + 63) / 64, 64, 0, st>>>(h, ctl, its));
     return;
   }
-  KT_LAUNCH("k_regrid", st, k_regrid_w<V><<<(h.ncol + HW - 1) / HW, dim3(64, HW), 0, st>>>(h, ctl, its));
+  const dim3 wg((h.ncol + HW - 1) / HW), wb(64, HW);
+  if (h.nvl == NVL_MAX && h.thbuf) {
+    const dim3 tg((h.ncol + 63) / 64), tb(64);
+    const double extop = CP * fim_pow(PTOP / P1000, RD / CP);
+    KT_LAUNCH("k_regrid", st, (k_regrid_w<V, true><<<wg, wb, 0, st>>>(h, ctl, its)));
+    KT_LAUNCH("k_regrid_tail", st, (k_regrid_tail<NVL_MAX, V><<<tg, tb, 0, st>>>(h, ctl, its, extop)));
+    KT_LAUNCH("k_dffusn_lev", st, k_dffusn_lev<V><<<(h.ncol + 3) / 4, dim3(64, 4), 0, st>>>(h));
+    KT_LAUNCH("k_remap_inflate", st, (k_remap_inflate<NVL_MAX, V><<<tg, tb, 0, st>>>(h, extop)));
+    KT_LAUNCH("k_remap", st, (k_remap_w<V, true><<<wg, wb, 0, st>>>(h, ctl, its)));
+    KT_LAUNCH("k_remap_theta", st, (k_remap_theta<NVL_MAX, V><<<tg, tb, 0, st>>>(h)));
+    return;
+  }
+  KT_LAUNCH("k_regrid", st, (k_regrid_w<V, false><<<wg, wb, 0, st>>>(h, ctl, its)));
   KT_LAUNCH("k_dffusn_lev", st, k_dffusn_lev<V><<<(h.ncol + 3) / 4, dim3(64, 4), 0, st>>>(h));
-  KT_LAUNCH("k_remap", st, k_remap_w<V><<<(h.ncol + HW - 1) / HW, dim3(64, HW), 0, st>>>(h, ctl, its));
+  KT_LAUNCH("k_remap", st, (k_remap_w<V, false><<<wg, wb, 0, st>>>(h, ctl, its)));
 }
 template void launch_hybgen_cols<float>(const HybCols<float> &, const StepCtl *, int, hipStream_t);
 
 void launch_hybgen(const DevGrid &g, const DevState &s, int its, hipStream_t st) {
   HybCols<double> h{g.nip, g.nip, g.nvl, g.ntra, s.us, s.vs, s.us, s.vs, s.tr, s.dp, s.pr, s.ex, s.sdot, s.exsmo,
-                    g.thetac, g.dpsig, g.smoo, g.sideln, g.rarea, g.prox, g.nprox};
+                    s.thbuf, g.thetac, g.dpsig, g.smoo, g.sideln, g.rarea, g.prox, g.nprox};
   launch_hybgen_cols<double>(h, nullptr, its, st);
 }
 
