@@ -54,9 +54,12 @@ def plain_kernel_bytes(T):
         "k_trcadv2": 16 * T,
         "k_trcadv3": 18 * T,
         "k_momtum": 3 + 1 + 30 + 4 + 5,
-        "k_regrid": 4,
+        "k_regrid": 4,                       # theta, Exner -> Exner (sdot)
+        "k_regrid_tail": 2,                  # Exner in sdot, in place
         "k_dffusn_lev": 2,
-        "k_remap": 11 + 2 * T,
+        "k_remap_inflate": 2,                # smoothed Exner, in place
+        "k_remap": 13 + 2 * T,               # ex pr exsmo u v tracers -> dp u v tracers theta work dlay pr sdot ex
+        "k_remap_theta": 3,                  # theta work, layer depths -> theta
         "k_hystat": 5,
         "k_hystat_tr": 2 * T + 3,
     }
@@ -67,18 +70,21 @@ def ens_kernel_bytes(T):
     """The same for the mixed-precision kernels (4 B per FP32 value, 8 B per
     FP64 value; inputs shared by the tracers of one launch counted once)."""
     return {
-        "e_edgvar1": 28 + 168,               # us vs tr1 (f32) pr mp (f64) -> u v uvsq lp (f32) mont (f64) trc1 edges
+        "e_edgvar1": 20 + 96,                # us vs tr1 (f32) pr (f64) -> u v lp trc1 edges (f32)
         "e_edgvar3": 32 + 28 * (T - 1),      # dp (f64) -> dp_edg; tracers 2..T -> trc_edg
-        "e_cnuity12": 144,                   # u/v/dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo
-        "e_cnuity3": 60,                     # antifx, delp_lo, dp, dp_tdcy(of,vof) -> r+, r-
+        "e_cnuity12": 152,                   # u/v/dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo, in/outflow sums
+        "e_cnuity3": 44,                     # in/outflow sums, delp_lo, dp, dp_tdcy(of,vof) -> r+, r-
         "e_cnuity45": 180,                   # antifx r+- massfx dp_tdcy dp pr lp_edg u v -> massfx dp_tdcy dp pr ex omega
-        "e_trcadv1": 32 + 80 * T,
-        "e_trcadv2": 8 + 60 * T,
+        "e_trcadv1": 32 + 80 * T,            # massfx dp; per tracer trc_edg tr trdp trl -> antiflx trl lo sums
+        "e_trcadv2": 8 + 40 * T,
         "e_trcadv3": 8 + 68 * T,
-        "e_momtum": 204,
+        "e_momtum": 148,                     # u v dp ex mp, u/v/theta edges, tendencies -> relvor tendency u2 v2
         "k_regrid": 20,
+        "k_regrid_tail": 16,
         "k_dffusn_lev": 16,
-        "k_remap": 56 + 40 + 8 * T,
+        "k_remap_inflate": 16,
+        "k_remap": 100 + 8 * T,
+        "k_remap_theta": 20,
         "e_hystat": 40 + 8 * T,
     }
 

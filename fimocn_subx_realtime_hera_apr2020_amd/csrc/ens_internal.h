@@ -2,12 +2,12 @@
 // other instances on the same grid) advanced by the same kernels.
 //
 // Precision split (per field):
-//   FP32  winds, tracers, tracer mass, their Adams-Bashforth histories, every
-//         edge value except the Montgomery potential, the FCT work arrays and
-//         the mass fluxes;
+//   FP32  winds, tracers, tracer mass, their Adams-Bashforth histories, the
+//         edge values, the FCT work arrays and the mass fluxes;
 //   FP64  layer thickness and its AB history (mass conservation), interface
-//         pressure and Exner function, Montgomery potential and geopotential
-//         (the pressure-gradient force), and all of hybgen.
+//         pressure and Exner function, Montgomery potential (also on the
+//         edges, formed where momtum needs it) and geopotential (the
+//         pressure-gradient force), and all of hybgen.
 // Each edge's mass flux is taken from one side (the lower cell index) and
 // negated on the other, so the FP32 fluxes cancel exactly in the global sum.
 //
@@ -37,8 +37,7 @@ struct EnsState {
   double *dp_tdcy;
   float *trc_tdcy, *trl_tdcy;
   float *massfx, *ws, *relvor;
-  float *u_edg, *v_edg, *dp_edg, *lp_edg, *uvsq_edg, *trc_edg;
-  double *mont_edg;
+  float *u_edg, *v_edg, *dp_edg, *lp_edg, *trc_edg;
   float *antifx, *delp_lo, *r_plus, *r_mnus;
   float *antiflx, *trcr_lo, *trmax, *trmin, *tr_rp, *tr_rm;
   StepCtl *ctl;

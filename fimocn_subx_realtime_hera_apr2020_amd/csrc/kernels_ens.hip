@@ -72,7 +72,7 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
   const int col = base + ipn;
   int nb[NP];
   float nu[NP], nv[NP], nth[NP];
-  double np0[NP], np1[NP], nmp[NP];
+  double np0[NP], np1[NP];
 #pragma unroll
   for (int e = 0; e < NP; e++) {
     nb[e] = base + GPROX(ipn, e);
@@ -81,10 +81,9 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
     nth[e] = ETR(s.tr, 0, nb[e], k);
     np0[e] = EL(s.pr, nb[e], k);
     np1[e] = EL(s.pr, nb[e], k + 1);
-    nmp[e] = EC(s.mp, nb[e], k);
   }
   const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k), thi = ETR(s.tr, 0, col, k);
-  const double pi0 = EL(s.pr, col, k), pi1 = EL(s.pr, col, k + 1), mpi = EC(s.mp, col, k);
+  const double pi0 = EL(s.pr, col, k), pi1 = EL(s.pr, col, k + 1);
   const float d6 = 1.f / 6.f, eps = 0.001f;
 #pragma unroll
   for (int e = 0; e < NP; e++) {
@@ -106,10 +105,8 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
     ve = ve * sc;
     EE(s.u_edg, col, e, k) = ue;
     EE(s.v_edg, col, e, k) = ve;
-    EE(s.uvsq_edg, col, e, k) = .5f * (ue * ue + ve * ve);
     EE(s.lp_edg, col, e, k) =
         (float)((pi0 + pi1 + np0[e] + np1[e] + .5 * (np0[et] + np1[et] + np0[el] + np1[el])) * (1. / 6.));
-    EE(s.mont_edg, col, e, k) = (2. * (mpi + nmp[e]) + nmp[et] + nmp[el]) * (1. / 6.);
     // theta on the edge, the same bits from both cells of the edge: the two
     // outer stencil cells are added in column order
     float ta = nth[et], tb = nth[el];
@@ -202,6 +199,7 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity12(EnsGrid g, EnsState s)
   const double dpi = EC(s.dp, col, k);
   const float dpf = (float)dpi;
   double acc = 0.;
+  float pp = 0.f, pm = 0.f;     // cnuity3's sums of the antidiffusive in- and outflows
   for (int e = 0; e < np; e++) {
     const int x = GPROX(ipn, e), ex = GPROXS(ipn, e), xc = base + x;
     const float vn = FSVE(ipn, e, 1) * EE(s.u_edg, col, e, k) - FSVE(ipn, e, 0) * EE(s.v_edg, col, e, k);
@@ -209,10 +207,14 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity12(EnsGrid g, EnsState s)
     const float dpe = .5f * (EE(s.dp_edg, col, e, k) + EE(s.dp_edg, xc, ex, k));
     const float hi = ipn < x ? vn * dpe : -(vx * dpe);
     const float m = .5f * ((vn + fabsf(vn)) * dpf - (vx + fabsf(vx)) * (float)EC(s.dp, xc, k));
+    const float a = hi - m;
     EE(s.massfx, col, e, k) = m;
-    EE(s.antifx, col, e, k) = hi - m;
+    EE(s.antifx, col, e, k) = a;
+    if (a <= 0.f) pp = pp - a; else pm = pm + a;
     acc = acc + (double)m;
   }
+  EC(s.r_plus, col, k) = pp;
+  EC(s.r_mnus, col, k) = pm;
   if (np < NPP) {
     EE(s.massfx, col, NPP - 1, k) = 0.f;
     EE(s.antifx, col, NPP - 1, k) = 0.f;
@@ -223,7 +225,8 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity12(EnsGrid g, EnsState s)
                                   c.ab3 * (double)EAB(s.dpl_tdcy, vof, col, k));
 }
 
-// cnuity3 (cnuity_compute.F90:152-259)
+// cnuity3 (cnuity_compute.F90:152-259); the in- and outflow sums were left
+// in r_plus/r_mnus by cnuity12
 template <int COLS, bool SWZ>
 __global__ void __launch_bounds__(64 * COLS) k_e_cnuity3(EnsGrid g, EnsState s) {
   const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
@@ -233,12 +236,7 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity3(EnsGrid g, EnsState s) 
   const int of = c.of - 1, vof = c.vof - 1;
   const int base = blockIdx.y * nip, col = base + ipn;
   const float thshld = 1.e-11f;
-  const int np = GNP(ipn);
-  float pp = 0.f, pm = 0.f;
-  for (int e = 0; e < np; e++) {
-    const float a = EE(s.antifx, col, e, k);
-    if (a <= 0.f) pp = pp - a; else pm = pm + a;
-  }
+  const float pp = EC(s.r_plus, col, k), pm = EC(s.r_mnus, col, k);
   const float lo = EC(s.delp_lo, col, k), dpi = (float)EC(s.dp, col, k);
   float mx = fmaxf(lo, dpi), mn = fminf(lo, dpi);
 #pragma unroll
@@ -307,178 +305,232 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity45(EnsGrid g, EnsState s)
 }
 
 // ---------------------------------------------------------------------------
-// trcadv1-3 (trcadv_compute.F90:19-315), blockIdx.z = tracer.  The final mass
+// trcadv1-3 (trcadv_compute.F90:19-315), every tracer of a column in one
+// thread, so the mass fluxes and the thickness are read once.  The final mass
 // flux is antisymmetric, so the neighbour's flux through the shared edge is
 // the negated own one; tracer edge values are the same bits on both sides
 // (tracer 1 from the edge-symmetric stencil, the others as the average of
-// both cells' reconstructions).
+// both cells' reconstructions).  trcadv1 leaves trcadv2's sums of the
+// antidiffusive out- and inflows in tr_rp/tr_rm, and trcadv2 forms the local
+// bounds of trcadv1 again from the tracer values instead of reading them.
 template <int COLS, bool SWZ>
 __global__ void __launch_bounds__(64 * COLS) k_e_trcadv1(EnsGrid g, EnsState s) {
-  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
+  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol, ntra = g.d.ntra;
   const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
-  const int t = blockIdx.z;
   if (ipn >= nip || k >= nvl) return;
   const StepCtl c = load_ctl(s.ctl);
   const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
   const int base = blockIdx.y * nip, col = base + ipn;
   const float thshld = 1.e-11f;
   const int np = GNP(ipn);
-  const float ti = ETR(s.tr, t, col, k);
-  float acc = 0.f;
-  for (int e = 0; e < np; e++) {
-    const int x = GPROX(ipn, e), xc = base + x;
-    const float m = EE(s.massfx, col, e, k);
-    const float te = t == 0 ? ETE(s.trc_edg, 0, col, e, k)
-                            : .5f * (ETE(s.trc_edg, t, col, e, k) + ETE(s.trc_edg, t, xc, GPROXS(ipn, e), k));
-    const float fl = .5f * ((m + fabsf(m)) * ti - (fabsf(m) - m) * ETR(s.tr, t, xc, k));
-    ETE(s.antiflx, t, col, e, k) = m * te - fl;
-    acc = acc + fl;
-  }
-  if (np < NPP) ETE(s.antiflx, t, col, NPP - 1, k) = 0.f;
-  const float tlo = -acc * GF(rarea, ipn);
-  ETAB(s.trl_tdcy, t, nf, col, k) = tlo;
-  const float lo = ETR(s.trdp, t, col, k) + (float)c.ab1 * tlo + (float)c.ab2 * ETAB(s.trl_tdcy, t, of, col, k) +
-                   (float)c.ab3 * ETAB(s.trl_tdcy, t, vof, col, k);
-  float mx = ti, mn = ti;
+  float m[NPP];
+  int xs[NPP], exs[NPP];
 #pragma unroll
   for (int e = 0; e < NPP; e++) {
-    const float v = ETR(s.tr, t, base + GPROX(ipn, e), k);
-    mx = fmaxf(mx, v);
-    mn = fminf(mn, v);
+    xs[e] = base + GPROX(ipn, e);
+    exs[e] = GPROXS(ipn, e);
+    m[e] = e < np ? EE(s.massfx, col, e, k) : 0.f;
   }
-  mx = fmaxf(0.f, mx);
-  mn = fmaxf(0.f, mn);
-  ETR(s.trmax, t, col, k) = mx;
-  ETR(s.trmin, t, col, k) = mn;
-  ETR(s.trcr_lo, t, col, k) = fmaxf(mn, fminf(mx, lo / fmaxf(thshld, (float)EC(s.dp, col, k))));
+  const float dpf = (float)EC(s.dp, col, k), rar = GF(rarea, ipn);
+  const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
+  for (int t = 0; t < ntra; t++) {
+    const float ti = ETR(s.tr, t, col, k);
+    float acc = 0.f, sp = 0.f, sm = 0.f, mx = ti, mn = ti;
+#pragma unroll
+    for (int e = 0; e < NPP; e++) {
+      const float tx = ETR(s.tr, t, xs[e], k);
+      mx = fmaxf(mx, tx);
+      mn = fminf(mn, tx);
+      if (e < np) {
+        const float te = t == 0 ? ETE(s.trc_edg, 0, col, e, k)
+                                : .5f * (ETE(s.trc_edg, t, col, e, k) + ETE(s.trc_edg, t, xs[e], exs[e], k));
+        const float fl = .5f * ((m[e] + fabsf(m[e])) * ti - (fabsf(m[e]) - m[e]) * tx);
+        const float a = m[e] * te - fl;
+        ETE(s.antiflx, t, col, e, k) = a;
+        sp = sp - fminf(0.f, a);
+        sm = sm + fmaxf(0.f, a);
+        acc = acc + fl;
+      }
+    }
+    if (np < NPP) ETE(s.antiflx, t, col, NPP - 1, k) = 0.f;
+    const float tlo = -acc * rar;
+    ETAB(s.trl_tdcy, t, nf, col, k) = tlo;
+    const float lo = ETR(s.trdp, t, col, k) + a1 * tlo + a2 * ETAB(s.trl_tdcy, t, of, col, k) +
+                     a3 * ETAB(s.trl_tdcy, t, vof, col, k);
+    mx = fmaxf(0.f, mx);
+    mn = fmaxf(0.f, mn);
+    ETR(s.trcr_lo, t, col, k) = fmaxf(mn, fminf(mx, lo / fmaxf(thshld, dpf)));
+    ETR(s.tr_rp, t, col, k) = sp;
+    ETR(s.tr_rm, t, col, k) = sm;
+  }
 }
 
 template <int COLS, bool SWZ>
 __global__ void __launch_bounds__(64 * COLS) k_e_trcadv2(EnsGrid g, EnsState s) {
-  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
+  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol, ntra = g.d.ntra;
   const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
-  const int t = blockIdx.z;
   if (ipn >= nip || k >= nvl) return;
   const StepCtl c = load_ctl(s.ctl);
   const int of = c.of - 1, vof = c.vof - 1;
   const int base = blockIdx.y * nip, col = base + ipn;
   const float thshld = 1.e-11f;
-  const int np = GNP(ipn);
-  float sp = 0.f, sm = 0.f;
-  for (int e = 0; e < np; e++) {
-    const float a = ETE(s.antiflx, t, col, e, k);
-    sp = sp - fminf(0.f, a);
-    sm = sm + fmaxf(0.f, a);
-  }
-  const float lo = ETR(s.trcr_lo, t, col, k);
-  float mx = fmaxf(ETR(s.trmax, t, col, k), lo), mn = fminf(ETR(s.trmin, t, col, k), lo);
+  int xs[NPP];
 #pragma unroll
-  for (int e = 0; e < NPP; e++) {
-    const float v = ETR(s.trcr_lo, t, base + GPROX(ipn, e), k);
-    mx = fmaxf(mx, v);
-    mn = fminf(mn, v);
-  }
-  mx = fmaxf(0.f, mx);
-  mn = fmaxf(0.f, mn);
-  ETR(s.trmax, t, col, k) = mx;
-  ETR(s.trmin, t, col, k) = mn;
-  const float dp = (float)EC(s.dp, col, k);
-  const float to = ETAB(s.trc_tdcy, t, of, col, k), tv = ETAB(s.trc_tdcy, t, vof, col, k);
+  for (int e = 0; e < NPP; e++) xs[e] = base + GPROX(ipn, e);
+  const float dp = (float)EC(s.dp, col, k), area = GF(area, ipn);
   const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
-  const float qp = ((mx - lo) * dp - (a2 * fminf(0.f, to) + a3 * fmaxf(0.f, tv))) / a1 * GF(area, ipn);
-  const float qm = ((lo - mn) * dp + (a2 * fmaxf(0.f, to) + a3 * fminf(0.f, tv))) / a1 * GF(area, ipn);
-  ETR(s.tr_rp, t, col, k) = fmaxf(0.f, fminf(1.f, qp / fmaxf(sp, thshld)));
-  ETR(s.tr_rm, t, col, k) = fmaxf(0.f, fminf(1.f, qm / fmaxf(sm, thshld)));
+  for (int t = 0; t < ntra; t++) {
+    const float sp = ETR(s.tr_rp, t, col, k), sm = ETR(s.tr_rm, t, col, k);
+    const float ti = ETR(s.tr, t, col, k), lo = ETR(s.trcr_lo, t, col, k);
+    float mx = ti, mn = ti;
+#pragma unroll
+    for (int e = 0; e < NPP; e++) {
+      const float v = ETR(s.tr, t, xs[e], k);
+      mx = fmaxf(mx, v);
+      mn = fminf(mn, v);
+    }
+    mx = fmaxf(fmaxf(0.f, mx), lo);
+    mn = fminf(fmaxf(0.f, mn), lo);
+#pragma unroll
+    for (int e = 0; e < NPP; e++) {
+      const float v = ETR(s.trcr_lo, t, xs[e], k);
+      mx = fmaxf(mx, v);
+      mn = fminf(mn, v);
+    }
+    mx = fmaxf(0.f, mx);
+    mn = fmaxf(0.f, mn);
+    ETR(s.trmax, t, col, k) = mx;
+    ETR(s.trmin, t, col, k) = mn;
+    const float to = ETAB(s.trc_tdcy, t, of, col, k), tv = ETAB(s.trc_tdcy, t, vof, col, k);
+    const float qp = ((mx - lo) * dp - (a2 * fminf(0.f, to) + a3 * fmaxf(0.f, tv))) / a1 * area;
+    const float qm = ((lo - mn) * dp + (a2 * fmaxf(0.f, to) + a3 * fminf(0.f, tv))) / a1 * area;
+    ETR(s.tr_rp, t, col, k) = fmaxf(0.f, fminf(1.f, qp / fmaxf(sp, thshld)));
+    ETR(s.tr_rm, t, col, k) = fmaxf(0.f, fminf(1.f, qm / fmaxf(sm, thshld)));
+  }
 }
 
 template <int COLS, bool SWZ>
 __global__ void __launch_bounds__(64 * COLS) k_e_trcadv3(EnsGrid g, EnsState s) {
-  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
+  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol, ntra = g.d.ntra;
   const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
-  const int t = blockIdx.z;
   if (ipn >= nip || k >= nvl) return;
   const StepCtl c = load_ctl(s.ctl);
   const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
   const int base = blockIdx.y * nip, col = base + ipn;
   const float thshld = 1.e-11f;
   const int np = GNP(ipn);
-  const float rp = ETR(s.tr_rp, t, col, k), rm = ETR(s.tr_rm, t, col, k);
-  float at = 0.f;
-  for (int e = 0; e < np; e++) {
-    const int xc = base + GPROX(ipn, e);
-    float a = ETE(s.antiflx, t, col, e, k);
-    if (a >= 0.f) a = a * fminf(rm, ETR(s.tr_rp, t, xc, k));
-    else a = a * fminf(rp, ETR(s.tr_rm, t, xc, k));
-    at = at + a;
+  int xs[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) xs[e] = base + GPROX(ipn, e);
+  const float dpd = fmaxf(thshld, (float)EC(s.dp, col, k)), rar = GF(rarea, ipn);
+  const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
+  for (int t = 0; t < ntra; t++) {
+    const float rp = ETR(s.tr_rp, t, col, k), rm = ETR(s.tr_rm, t, col, k);
+    float at = 0.f;
+#pragma unroll
+    for (int e = 0; e < NPP; e++) {
+      if (e < np) {
+        float a = ETE(s.antiflx, t, col, e, k);
+        if (a >= 0.f) a = a * fminf(rm, ETR(s.tr_rp, t, xs[e], k));
+        else a = a * fminf(rp, ETR(s.tr_rm, t, xs[e], k));
+        at = at + a;
+      }
+    }
+    at = -at * rar;
+    const float tt = ETAB(s.trl_tdcy, t, nf, col, k) + at;
+    ETAB(s.trc_tdcy, t, nf, col, k) = tt;
+    const float tdp = ETR(s.trdp, t, col, k) + a1 * tt + a2 * ETAB(s.trc_tdcy, t, of, col, k) +
+                      a3 * ETAB(s.trc_tdcy, t, vof, col, k);
+    ETR(s.trdp, t, col, k) = tdp;
+    ETR(s.tr, t, col, k) = fmaxf(ETR(s.trmin, t, col, k), fminf(ETR(s.trmax, t, col, k), tdp / dpd));
   }
-  at = -at * GF(rarea, ipn);
-  const float tt = ETAB(s.trl_tdcy, t, nf, col, k) + at;
-  ETAB(s.trc_tdcy, t, nf, col, k) = tt;
-  const float tdp = ETR(s.trdp, t, col, k) + (float)c.ab1 * tt + (float)c.ab2 * ETAB(s.trc_tdcy, t, of, col, k) +
-                    (float)c.ab3 * ETAB(s.trc_tdcy, t, vof, col, k);
-  ETR(s.trdp, t, col, k) = tdp;
-  ETR(s.tr, t, col, k) = fmaxf(ETR(s.trmin, t, col, k),
-                               fminf(ETR(s.trmax, t, col, k), tdp / fmaxf(thshld, (float)EC(s.dp, col, k))));
 }
 
 // ---------------------------------------------------------------------------
 // momtum (momtum.F90:101-158; dissip momtum_compute.F90:102-229; momtum1
 // 231-450).  Diffusion, vorticity, Coriolis and the AB update in FP32; the
 // pressure-gradient line integral of Montgomery potential and Exner function
-// in FP64.  New winds go to u2/v2 (hybgen's remap reads them from there).
+// in FP64.  The edge Montgomery potential and kinetic energy are formed here
+// from the cell values and the edge winds, with the operations edgvar1 uses
+// for them (edgvar1.F90:162-421), instead of being stored by edgvar1 and read
+// back.  New winds go to u2/v2 (hybgen's remap reads them from there).
+template <int NP>
+__device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s, const StepCtl &c, int ipn,
+                                             int base, int k, float &un, float &vn) {
+  const int nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
+  const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
+  const int col = base + ipn;
+  const float thshld = 1.e-11f;
+  int nb[NP];
+  float nu[NP], nv[NP], ndp[NP];
+  double nmp[NP];
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    nb[e] = base + GPROX(ipn, e);
+    nu[e] = EC(s.us, nb[e], k);
+    nv[e] = EC(s.vs, nb[e], k);
+    ndp[e] = (float)EC(s.dp, nb[e], k);
+    nmp[e] = EC(s.mp, nb[e], k);
+  }
+  const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k);
+  const float wui = ui + GF(grndspd, ipn), wvi = vi;
+  const float a = fmaxf((float)EC(s.dp, col, k), thshld);
+  const float factor = GF(rarea, ipn) / a;
+  const float dff = g.dfflen[k];
+  float wu = 0.f, wv = 0.f;
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    const float wux = nu[e] + GF(grndspd, nb[e] - base), wvx = nv[e];
+    const float c0 = FCS(ipn, e, 0), s0 = FSN(ipn, e, 0), c1 = FCS(ipn, e, 1), s1 = FSN(ipn, e, 1);
+    const float uxy1 = c0 * wui + s0 * wvi, vxy1 = -s0 * wui + c0 * wvi;
+    const float uxy2 = c1 * wux + s1 * wvx, vxy2 = -s1 * wux + c1 * wvx;
+    const float b = fmaxf(ndp[e], thshld);
+    const float hh = a * b / (a + b);
+    const float w = dff * GF(sideln, ipn * NPP + e) * 2.f * hh;
+    const float ufx = (uxy2 - uxy1) * w, vfx = (vxy2 - vxy1) * w;
+    wu = wu + (c0 * ufx - s0 * vfx) * factor;
+    wv = wv + (s0 * ufx + c0 * vfx) * factor;
+  }
+  const float u1 = ui + wu, v1 = vi + wv;
+  double px = 0., py = 0.;
+  float rv = 0.f;
+  const double exm = .5 * (EL(s.ex, col, k) + EL(s.ex, col, k + 1));
+  const double mpi = EC(s.mp, col, k);
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
+    const float ue = EE(s.u_edg, col, e, k), ve = EE(s.v_edg, col, e, k);
+    const double mont = (2. * (mpi + nmp[e]) + nmp[et] + nmp[el]) * (1. / 6.);
+    const double b = mont + (double)(.5f * (ue * ue + ve * ve));
+    const double te = (double)ETE(s.trc_edg, 0, col, e, k);
+    const double sx = FSVC(ipn, e, 1), sy = FSVC(ipn, e, 0);
+    px = px - b * sx + exm * te * sx;
+    py = py + b * sy - exm * te * sy;
+    rv = rv + (FSVE(ipn, e, 0) * ue + FSVE(ipn, e, 1) * ve);
+  }
+  const float pgx = (float)(px * GD(rarea, ipn)), pgy = (float)(py * GD(rarea, ipn));
+  rv = rv * GF(rarea, ipn);
+  EC(s.relvor, col, k) = rv;
+  const float f = GF(corio, ipn) + rv;
+  const float ut = pgx + f * v1, vt = pgy - f * u1;
+  EAB(s.u_tdcy, nf, col, k) = ut;
+  EAB(s.v_tdcy, nf, col, k) = vt;
+  un = u1 + (float)c.ab1 * ut + (float)c.ab2 * EAB(s.u_tdcy, of, col, k) + (float)c.ab3 * EAB(s.u_tdcy, vof, col, k);
+  vn = v1 + (float)c.ab1 * vt + (float)c.ab2 * EAB(s.v_tdcy, of, col, k) + (float)c.ab3 * EAB(s.v_tdcy, vof, col, k);
+}
+
 template <int COLS, bool SWZ>
 __global__ void __launch_bounds__(64 * COLS) k_e_momtum(EnsGrid g, EnsState s) {
-  const int nip = g.d.nip, nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
+  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
   const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
   if (ipn >= nip) return;
   const bool live = k < nvl;
   const StepCtl c = load_ctl(s.ctl);
-  const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
   const int base = blockIdx.y * nip, col = base + ipn;
-  const float thshld = 1.e-11f;
   float un = 0.f, vn = 0.f;
   if (live) {
-    const int np = GNP(ipn);
-    const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k);
-    const float wui = ui + GF(grndspd, ipn), wvi = vi;
-    const float a = fmaxf((float)EC(s.dp, col, k), thshld);
-    const float factor = GF(rarea, ipn) / a;
-    const float dff = g.dfflen[k];
-    float wu = 0.f, wv = 0.f;
-    for (int e = 0; e < np; e++) {
-      const int x = GPROX(ipn, e), xc = base + x;
-      const float wux = EC(s.us, xc, k) + GF(grndspd, x), wvx = EC(s.vs, xc, k);
-      const float c0 = FCS(ipn, e, 0), s0 = FSN(ipn, e, 0), c1 = FCS(ipn, e, 1), s1 = FSN(ipn, e, 1);
-      const float uxy1 = c0 * wui + s0 * wvi, vxy1 = -s0 * wui + c0 * wvi;
-      const float uxy2 = c1 * wux + s1 * wvx, vxy2 = -s1 * wux + c1 * wvx;
-      const float b = fmaxf((float)EC(s.dp, xc, k), thshld);
-      const float hh = a * b / (a + b);
-      const float w = dff * GF(sideln, ipn * NPP + e) * 2.f * hh;
-      const float ufx = (uxy2 - uxy1) * w, vfx = (vxy2 - vxy1) * w;
-      wu = wu + (c0 * ufx - s0 * vfx) * factor;
-      wv = wv + (s0 * ufx + c0 * vfx) * factor;
-    }
-    const float u1 = ui + wu, v1 = vi + wv;
-    double px = 0., py = 0.;
-    float rv = 0.f;
-    const double exm = .5 * (EL(s.ex, col, k) + EL(s.ex, col, k + 1));
-    for (int e = 0; e < np; e++) {
-      const double b = EE(s.mont_edg, col, e, k) + (double)EE(s.uvsq_edg, col, e, k);
-      const double te = (double)ETE(s.trc_edg, 0, col, e, k);
-      const double sx = FSVC(ipn, e, 1), sy = FSVC(ipn, e, 0);
-      px = px - b * sx + exm * te * sx;
-      py = py + b * sy - exm * te * sy;
-      rv = rv + (FSVE(ipn, e, 0) * EE(s.u_edg, col, e, k) + FSVE(ipn, e, 1) * EE(s.v_edg, col, e, k));
-    }
-    const float pgx = (float)(px * GD(rarea, ipn)), pgy = (float)(py * GD(rarea, ipn));
-    rv = rv * GF(rarea, ipn);
-    EC(s.relvor, col, k) = rv;
-    const float f = GF(corio, ipn) + rv;
-    const float ut = pgx + f * v1, vt = pgy - f * u1;
-    EAB(s.u_tdcy, nf, col, k) = ut;
-    EAB(s.v_tdcy, nf, col, k) = vt;
-    un = u1 + (float)c.ab1 * ut + (float)c.ab2 * EAB(s.u_tdcy, of, col, k) + (float)c.ab3 * EAB(s.u_tdcy, vof, col, k);
-    vn = v1 + (float)c.ab1 * vt + (float)c.ab2 * EAB(s.v_tdcy, of, col, k) + (float)c.ab3 * EAB(s.v_tdcy, vof, col, k);
+    if (GNP(ipn) == 6) e_momtum_col<6>(g, s, c, ipn, base, k, un, vn);
+    else e_momtum_col<5>(g, s, c, ipn, base, k, un, vn);
   }
   // Rayleigh damping of the top layers (momtum_compute.F90:406-416): the
   // largest squared speed of the top three layers, from their lanes
@@ -697,9 +749,9 @@ void ens_launch(int kernel, int tile, const EnsGrid &g, const EnsState &s, hipSt
     case EK_CNUITY12: ENS_DISPATCH(k_e_cnuity12, 1); break;
     case EK_CNUITY3: ENS_DISPATCH(k_e_cnuity3, 1); break;
     case EK_CNUITY45: ENS_DISPATCH(k_e_cnuity45, 1); break;
-    case EK_TRCADV1: ENS_DISPATCH(k_e_trcadv1, ntra); break;
-    case EK_TRCADV2: ENS_DISPATCH(k_e_trcadv2, ntra); break;
-    case EK_TRCADV3: ENS_DISPATCH(k_e_trcadv3, ntra); break;
+    case EK_TRCADV1: ENS_DISPATCH(k_e_trcadv1, 1); break;
+    case EK_TRCADV2: ENS_DISPATCH(k_e_trcadv2, 1); break;
+    case EK_TRCADV3: ENS_DISPATCH(k_e_trcadv3, 1); break;
     case EK_MOMTUM: ENS_DISPATCH(k_e_momtum, 1); break;
     default: ENS_DISPATCH(k_e_hystat, 1); break;
   }
