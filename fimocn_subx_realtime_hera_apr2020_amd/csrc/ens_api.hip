@@ -60,7 +60,7 @@ extern "C" int fimhip_ens_create(const fimhip_grid_desc *gd, int members, int de
   if (!gd || !out) return FIMHIP_EINVAL;
   *out = nullptr;
   const int nip = gd->nip, nvl = gd->nvl, ntra = gd->ntra;
-  if (members < 1 || members > 4096 || nip < 12 || nvl < 4 || nvl > NVL_MAX || ntra < 1 || ntra > 8 ||
+  if (members < 1 || members > 4096 || nip < 12 || nvl < 4 || nvl > NVL_MAX || ntra < 1 || ntra > NTRA_MAX ||
       !(gd->dt > 0.))
     return FIMHIP_EINVAL;
   if (!gd->nprox || !gd->prox || !gd->proxs || !gd->cs || !gd->sn || !gd->sidevec_c || !gd->sidevec_e ||

@@ -124,7 +124,7 @@ extern "C" int fimhip_create(const fimhip_grid_desc *gd, int device, fimhip_ctx 
   if (!gd || !out) return FIMHIP_EINVAL;
   *out = nullptr;
   const int nip = gd->nip, nvl = gd->nvl, ntra = gd->ntra;
-  if (nip < 12 || nvl < 4 || nvl > NVL_MAX || ntra < 1 || ntra > 8 || !(gd->dt > 0.)) return FIMHIP_EINVAL;
+  if (nip < 12 || nvl < 4 || nvl > NVL_MAX || ntra < 1 || ntra > NTRA_MAX || !(gd->dt > 0.)) return FIMHIP_EINVAL;
   if (!gd->nprox || !gd->prox || !gd->proxs || !gd->cs || !gd->sn || !gd->sidevec_c || !gd->sidevec_e ||
       !gd->sideln || !gd->hfcoef || !gd->area || !gd->lat || !gd->lon || !gd->thetac || !gd->dpsig)
     return FIMHIP_EINVAL;
@@ -530,7 +530,7 @@ static void synth_profiles(int nip, int nvl, const double *lat, const double *lo
 }
 
 static int check_synth(int nip, int nvl, int ntra) {
-  return (nip < 1 || nvl < 2 || nvl > NVL_MAX || ntra < 1 || ntra > 8) ? FIMHIP_EINVAL : 0;
+  return (nip < 1 || nvl < 2 || nvl > NVL_MAX || ntra < 1 || ntra > NTRA_MAX) ? FIMHIP_EINVAL : 0;
 }
 
 extern "C" int fimhip_synth_state(int nip, int nvl, int ntra, const double *lat, const double *lon,

@@ -22,6 +22,7 @@ namespace fimhip {
 constexpr int NPP = 6;
 constexpr int KBL = 2;
 constexpr int NVL_MAX = 64;      // one wavefront lane per level
+constexpr int NTRA_MAX = 32;     // tracers per instance
 
 // module_constants.F90:14-24 and the hot-path namelist defaults
 // (fimnamelist.F90:100-117)

@@ -22,7 +22,6 @@ namespace fimhip {
 #define TR(a, t, col, k) (a)[((size_t)(t) * ncol + (col)) * nvl + (k)]
 
 constexpr int LMAX = NVL_MAX + 1;
-constexpr int NTRA_MAX = 8;
 constexpr double THIN = RD / P1000;
 
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }

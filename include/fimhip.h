@@ -31,7 +31,7 @@ typedef struct fimhip_ctx fimhip_ctx;
  * (dyn_init.F90:373-548, 789-818).  Neighbour indices are 1-based as in
  * glvl.dat; slot 6 of a pentagon is ignored. */
 typedef struct {
-  int nip, nvl, ntra;           /* nvl <= 64, ntra <= 8 */
+  int nip, nvl, ntra;           /* nvl <= 64, ntra <= 32 */
   double dt;                    /* model time step (s) */
   const int *nprox;             /* (nip) */
   const int *prox;              /* (6,nip) */

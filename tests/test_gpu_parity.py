@@ -181,3 +181,35 @@ def test_profiler_reports_every_kernel():
             assert rep[k][0] == 2 and rep[k][1] > 0, k
     finally:
         m.close()
+
+
+def test_g5_transport_dt480_matches_checker_bitwise(oracle_lib):
+    """The CPU-runnable configuration: G5 x 64 layers, 4 tracers, 10
+    transport-only steps (edgvar1, edgvar3, cnuity, trcadv) at dt = 480 s from
+    the seeded state without wind noise: every field identical to the
+    checker."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+    O = oracle_lib
+    g = fimhip.icos_grid(5)
+    th, ds = state.vertical_tables(NVL)
+    dt = 480.0
+    print("case: G5 nvl=64 ntra=4 dt=480 10 transport steps", flush=True)
+    m = fimhip.Model(g, NVL, NTRA, dt, th, ds)
+    try:
+        m.synth(seed=20240601, member=0, flags=0)
+        st = {f: m.download(f) for f in ("us", "vs", "dp", "pr", "ex", "ph", "tr")}
+        m.init_state(False)
+        o = O.OracleRun(O.grid_arrays(g, NVL, dt, th, ds), NVL, NTRA, dt, pow_kind=1)
+        o.load(st)
+        o.init(False)
+        for its in range(1, 11):
+            m.step(its, 1)
+            o.step(its, 1)
+        m.sync()
+        for f in ("us", "vs", "dp", "pr", "ex", "tr", "trdp", "massfx", "ws"):
+            assert _same(m.download(f), o.a[f]), f
+        gm, gt = m.globsum()
+        om, ot = o.globsum()
+        assert abs(gm - om) <= 1e-13 * abs(om)
+    finally:
+        m.close()

@@ -147,3 +147,45 @@ def test_fp32_form_against_fp64_after_96_steps():
     finally:
         e.close()
         p.close()
+
+
+def test_twenty_lognormal_tracers_transport_fp32_vs_fp64():
+    """Chemistry-load transport: 20 seeded log-normal tracers (median 1e-9,
+    sigma 1.5, times the cosine bell), transport-only steps at dt = 120 s, the
+    FP32 form against FP64 (G5 here; the same kernels run at G8).  Each
+    tracer's relative L2 error stays below 1e-4 after 48 steps, the global mass
+    is conserved to 1e-12, and the FP32 Kahan-compensated tracer totals agree
+    with the FP64 sums of the same state to 5e-8: the compensation leaves only
+    the rounding of the FP32 products area*dp*tr, of which a log-normal field
+    has few dominant ones (measured 4.4e-9 on MI355X)."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+    g = fimhip.icos_grid(5)
+    th, ds = state.vertical_tables(NVL)
+    nt, dt, flags = 20, 120.0, fimhip.SYNTH_LOGNORMAL
+    print("case: G5 nvl=64 ntra=20 log-normal, 48 transport steps at dt=120, FP32 vs FP64", flush=True)
+    e = fimhip.Ensemble(g, 1, NVL, nt, dt, th, ds)
+    p = fimhip.Model(g, NVL, nt, dt, th, ds)
+    try:
+        e.synth(0, seed=20240603, synth_member=0, flags=flags)
+        e.init_state(False)
+        p.synth(seed=20240603, member=0, flags=flags)
+        p.init_state(False)
+        m0, t0, _, _ = e.globsum()
+        e.step(48, 1)
+        for its in range(1, 49):
+            p.step(its, 1)
+        p.sync()
+        e.sync()
+        m1, t1, k1, kt1 = e.globsum()
+        tre, trp = e.download(0, "tr"), p.download("tr")
+        errs = [rel_l2(tre[t], trp[t]) for t in range(nt)]
+        print("max tracer rel L2 %.2e, tracer-total drift %.2e, Kahan vs FP64 %.2e" %
+              (max(errs), np.max(np.abs(t1 - t0) / t0), np.max(np.abs(kt1 - t1) / t1)), flush=True)
+        assert max(errs) < 1e-4, errs
+        assert np.all(np.abs(m1 - m0) <= 1e-12 * m0)
+        assert np.all(np.abs(kt1 - t1) <= 5e-8 * np.abs(t1))
+        assert np.all(np.abs(k1 - m1) <= 1e-9 * m1)
+        assert tre.min() >= 0.0
+    finally:
+        e.close()
+        p.close()

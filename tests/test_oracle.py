@@ -88,6 +88,30 @@ def test_oracle_bitexact_vs_live_reference(oracle_lib):
         assert np.array_equal(o.a["tr"][t].ravel(), r["final.tr"][t])
 
 
+def test_live_reference_g5_transport_dt480_bitwise(oracle_lib):
+    """The CPU-runnable configuration against the reference Fortran itself:
+    G5 x 64 layers, 4 tracers, 10 transport-only steps at dt = 480 s, from the
+    seeded state without wind noise, on the reference's own G5 grid."""
+    from oracle import refrun
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, glvl, state
+    if not refrun.available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    O = oracle_lib
+    path = refrun.ref_glvl(5)
+    g = glvl.read_glvl(path)
+    th, ds = state.vertical_tables(64)
+    st = fimhip.synth_state(g.lat, g.lon, 64, 4, seed=20240601, member=0, flags=0)
+    r = refrun.run_dynref(path, st, th, ds, nsteps=10, dt=480.0, mode=1, init_hybgen=False)
+    o = _oracle_run(O, g, st, th, ds, 480.0)
+    o.init(False)
+    for its in range(1, 11):
+        o.step(its, 1)
+    for f in ("us", "vs", "dp", "pr", "ex"):
+        assert np.array_equal(o.a[f].ravel(), r["final." + f]), f
+    for t in range(4):
+        assert np.array_equal(o.a["tr"][t].ravel(), r["final.tr"][t]), t
+
+
 def test_project_pow_drift_within_bound(oracle_lib):
     """fim_pow (the device pow) instead of the C library pow moves the 3-step
     G2 state by far less than 1e-12 relative L2."""
