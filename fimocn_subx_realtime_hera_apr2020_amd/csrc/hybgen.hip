@@ -402,20 +402,6 @@ __device__ __forceinline__ double w_prefix_min(double x, int lane) {
   }
   return x;
 }
-__device__ __forceinline__ double w_suffix_max(double x, int lane) {
-  for (int d = 1; d < 64; d <<= 1) {
-    const double y = __shfl_down(x, d, 64);
-    if (lane + d < 64) x = dmax(x, y);
-  }
-  return x;
-}
-__device__ __forceinline__ double w_prefix_max(double x, int lane) {
-  for (int d = 1; d < 64; d <<= 1) {
-    const double y = __shfl_up(x, d, 64);
-    if (lane >= d) x = dmax(x, y);
-  }
-  return x;
-}
 __device__ __forceinline__ int w_min_int(int x) {
   for (int d = 1; d < 64; d <<= 1) x = min(x, __shfl_xor(x, d, 64));
   return x;
@@ -480,21 +466,41 @@ __device__ __forceinline__ bool absorbs(double t, double bound) {
   return at > 1.e-280 && bound * (1. + 1.e-12) * (1.e-22 * 36028797018963968.) < at;
 }
 
-// pcwise for one column: lane j < kk holds layer j's value a; returns the
-// remapped value of layer j.  xold/xnew are the interface coordinates in LDS.
-__device__ double pcwise_w(int kk, const double *xold, const double *xnew, double a, int lane) {
-  const double small = 1.e-22;
+// The interface geometry of pcwise for one column, lane j holding layer j:
+// the old and new layer widths (fco, fcn), the width of the layer below
+// (fco(j-1)) and the displacement of interface j.  xold/xnew (LDS) are the
+// interface coordinates; the reference works in the direction in which they
+// increase (zold/znew = +-x).
+struct PcwGeom {
+  double f, fl, fn, dz;
+};
+__device__ __forceinline__ PcwGeom pcwise_geom(int kk, const double *xold, const double *xnew, int lane) {
   const bool up = xold[0] < xold[kk];
   auto zo = [&](int k) { return up ? xold[k] : -xold[k]; };
   auto zn = [&](int k) { return up ? xnew[k] : -xnew[k]; };
+  PcwGeom g{0., 0., 0., 0.};
+  if (lane < kk) {
+    g.f = dmax(0., zo(lane + 1) - zo(lane));
+    g.fn = dmax(0., zn(lane + 1) - zn(lane));
+    g.dz = zn(lane) - zo(lane);
+  }
+  g.fl = __shfl_up(g.f, 1, 64);
+  return g;
+}
+
+// pcwise for one column: lane j < kk holds layer j's value a; returns the
+// remapped value of layer j.
+__device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
+  const double small = 1.e-22;
   const bool live = lane < kk;
-  const double f = live ? dmax(0., zo(lane + 1) - zo(lane)) : 0.;    // fco(lane)
+  const double f = gm.f;
+  // a bound on every value either fill sweep can produce: the largest
+  // magnitude in the column (the sweeps form convex combinations)
+  double bnd = live ? fabs(a) : 0.;
+  for (int d = 1; d < 64; d <<= 1) bnd = dmax(bnd, __shfl_xor(bnd, d, 64));
   // sweep 1, from the top: layer j (j <= kk-2) takes layer j+1's new value
   double v;
   {
-    const double smx = w_suffix_max(live ? fabs(a) : 0., lane);
-    double bnd = __shfl_down(smx, 1, 64);
-    if (lane == 63) bnd = 0.;
     bool fin;
     if (lane >= kk - 1) {
       v = a;
@@ -517,9 +523,6 @@ __device__ double pcwise_w(int kk, const double *xold, const double *xnew, doubl
   double w;
   {
     const double a2 = v;
-    const double pmx = w_prefix_max(live ? fabs(a2) : 0., lane);
-    double bnd = __shfl_up(pmx, 1, 64);
-    if (lane == 0) bnd = 0.;
     bool fin;
     if (lane == 0 || !live) {
       w = a2;
@@ -550,10 +553,10 @@ __device__ double pcwise_w(int kk, const double *xold, const double *xnew, doubl
   // flux through interface lane (1-based vertfx(lane + 1)), lanes 1..kk-1
   double vf = 0.;
   {
-    const double wl = __shfl_up(w, 1, 64), bl = __shfl_up(b, 1, 64), fl = __shfl_up(f, 1, 64);
+    const double wl = __shfl_up(w, 1, 64), bl = __shfl_up(b, 1, 64);
     if (live && lane >= 1) {
       double slab = 0., amount = 0.;
-      const double dz = zn(lane) - zo(lane);
+      const double dz = gm.dz;
       if (dz > 0.) {
         const double fc = f;                       // fco(layer lane)
         if (fc > 0.) {
@@ -564,7 +567,7 @@ __device__ double pcwise_w(int kk, const double *xold, const double *xnew, doubl
           slab = slab + dslab;
         }
       } else if (dz < 0.) {
-        const double fc = fl;                      // fco(layer lane - 1)
+        const double fc = gm.fl;                   // fco(layer lane - 1)
         if (fc > 0.) {
           double dslab = dmin(slab + fc, -dz) - dmin(slab, -dz);
           double dx = dslab / fc;
@@ -581,9 +584,8 @@ __device__ double pcwise_w(int kk, const double *xold, const double *xnew, doubl
   const double vlo = vf;
   double y = 0.;
   if (live) {
-    const double fn = dmax(0., zn(lane + 1) - zn(lane));
     const double amount = w * f - (vhi - vlo);
-    y = (amount + w * small) / (fn + small);
+    y = (amount + w * small) / (gm.fn + small);
   }
   return y;
 }
@@ -819,16 +821,17 @@ __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl
   // u, v and tracers 2..ntra are remapped in pressure
   {
     const bool live = lane < nvl;
-    double y = pcwise_w(nvl, prcol, prnew, live ? (double)C2(s.us_in, col, lane) : 0., lane);
+    const PcwGeom gp = pcwise_geom(nvl, prcol, prnew, lane);
+    double y = pcwise_w(nvl, gp, live ? (double)C2(s.us_in, col, lane) : 0., lane);
     if (live) C2(s.us, col, lane) = (V)y;
-    y = pcwise_w(nvl, prcol, prnew, live ? (double)C2(s.vs_in, col, lane) : 0., lane);
+    y = pcwise_w(nvl, gp, live ? (double)C2(s.vs_in, col, lane) : 0., lane);
     if (live) C2(s.vs, col, lane) = (V)y;
     for (int n = 1; n < ntra; n++) {
-      y = pcwise_w(nvl, prcol, prnew, live ? (double)TR(s.tr, n, col, lane) : 0., lane);
+      y = pcwise_w(nvl, gp, live ? (double)TR(s.tr, n, col, lane) : 0., lane);
       if (live) TR(s.tr, n, col, lane) = (V)y;
     }
     // theta is remapped in p^(kappa+1)
-    y = pcwise_w(nvl, pk1col, pk1new, live ? (double)TR(s.tr, 0, col, lane) : 0., lane);
+    y = pcwise_w(nvl, pcwise_geom(nvl, pk1col, pk1new, lane), live ? (double)TR(s.tr, 0, col, lane) : 0., lane);
     if (live) th1[lane] = y;
   }
   wsync();
