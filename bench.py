@@ -54,11 +54,11 @@ def plain_kernel_bytes(T):
         "k_trcadv2": 16 * T,
         "k_trcadv3": 18 * T,
         "k_momtum": 3 + 1 + 30 + 4 + 5,
-        "k_regrid": 4,                       # theta, Exner -> Exner (sdot)
+        "k_regrid": 3,                       # theta, Exner -> Exner (sdot)
         "k_regrid_tail": 2,                  # Exner in sdot, in place
         "k_dffusn_lev": 2,
         "k_remap_inflate": 2,                # smoothed Exner, in place
-        "k_remap": 13 + 2 * T,               # ex pr exsmo u v tracers -> dp u v tracers theta work dlay pr sdot ex
+        "k_remap": 12 + 2 * T,               # ex pr exsmo u v tracers -> dp u v tracers 2..T theta-work depths pr sdot ex
         "k_remap_theta": 3,                  # theta work, layer depths -> theta
         "k_hystat": 5,
         "k_hystat_tr": 2 * T + 3,
@@ -83,7 +83,7 @@ def ens_kernel_bytes(T):
         "k_regrid_tail": 16,
         "k_dffusn_lev": 16,
         "k_remap_inflate": 16,
-        "k_remap": 100 + 8 * T,
+        "k_remap": 84 + 8 * T,
         "k_remap_theta": 20,
         "e_hystat": 40 + 8 * T,
     }
@@ -111,14 +111,33 @@ def parse():
     return ap.parse_args()
 
 
-def roofline(prof, kbytes, cells):
+PMC_TABLE = os.path.join(ROOT, "profiles", "pmc_traffic_G7x64.md")
+
+
+def pmc_traffic(kernel, form, a):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    (copy-calibrated) for this kernel on the G7 x 64 x 4-tracer workload, from
+    the committed summary; None for any other workload."""
+    if (a.glvl, a.nvl, a.ntra) != (7, 64, 4) or (form == "ens16" and a.members != 16):
+        return None
+    if not os.path.exists(PMC_TABLE):
+        return None
+    for line in open(PMC_TABLE):
+        f = [x.strip() for x in line.strip().strip("|").split("|")]
+        if len(f) == 7 and f[0] == "`%s`" % kernel and f[1] == form:
+            return float(f[4])
+    return None
+
+
+def roofline(prof, kbytes, cells, traffic_of=None):
     """The kernel with the most device time: algorithmic bytes per launch over
-    its mean launch time."""
+    its mean launch time; traffic is the measured HBM bytes per launch."""
     dname, (dn, dms) = max(prof.items(), key=lambda kv: kv[1][1])
     dmean = dms / dn * 1e-3
     dbytes = kbytes.get(dname, 0) * cells
     return {"kernel": dname, "bound": "hbm", "achieved": dbytes / dmean / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": dbytes / dmean / HBM_PEAK, "traffic": None,
+            "unit": "GB/s", "frac": dbytes / dmean / HBM_PEAK,
+            "traffic": traffic_of(dname) if traffic_of else None,
             "bytes_per_launch": dbytes, "mean_launch_ms": dmean * 1e3}
 
 
@@ -164,7 +183,7 @@ def run_plain(a, grid, th, ds, fimhip):
     v_step = (18 + 8 * a.ntra) if a.mode == 0 else (10 + 8 * a.ntra)
     value = cells * a.steps / wall
     res = {"value": value, "ms_per_step": wall / a.steps * 1e3, "members": 1,
-           "roofline": roofline(prof, kb, cells),
+           "roofline": roofline(prof, kb, cells, lambda k: pmc_traffic(k, "plain", a) if a.mode == 0 else None),
            "step_hbm": {"bytes_per_cell_level": 8 * v_step, "achieved_GBs": value * 8 * v_step / 1e9,
                         "frac": value * 8 * v_step / HBM_PEAK},
            "kernels": kernel_table(prof, kb, cells)}
@@ -208,7 +227,7 @@ def run_ens(a, grid, th, ds, fimhip):
     value = cells * a.steps / wall
     b_step = 220 if a.mode == 0 else 156
     res = {"value": value, "ms_per_step": wall / a.steps * 1e3, "members": M,
-           "roofline": roofline(prof, kb, cells),
+           "roofline": roofline(prof, kb, cells, lambda k: pmc_traffic(k, "ens16", a) if a.mode == 0 else None),
            "step_hbm": {"bytes_per_cell_level": b_step, "achieved_GBs": value * b_step / 1e9,
                         "frac": value * b_step / HBM_PEAK},
            "kernels": kernel_table(prof, kb, cells), "tiles": tune}
