@@ -473,23 +473,35 @@ __device__ __forceinline__ bool absorbs(double t, double bound) {
 // increase (zold/znew = +-x).
 struct PcwGeom {
   double f, fl, fn, dz;
+  double rf, rfn, dx;   // FAST only: 1/(f+1e-22), 1/(fn+1e-22), the flux slab fraction
 };
+template <bool FAST>
 __device__ __forceinline__ PcwGeom pcwise_geom(int kk, const double *xold, const double *xnew, int lane) {
   const bool up = xold[0] < xold[kk];
   auto zo = [&](int k) { return up ? xold[k] : -xold[k]; };
   auto zn = [&](int k) { return up ? xnew[k] : -xnew[k]; };
-  PcwGeom g{0., 0., 0., 0.};
+  PcwGeom g{0., 0., 0., 0., 0., 0., 0.};
   if (lane < kk) {
     g.f = dmax(0., zo(lane + 1) - zo(lane));
     g.fn = dmax(0., zn(lane + 1) - zn(lane));
     g.dz = zn(lane) - zo(lane);
   }
   g.fl = __shfl_up(g.f, 1, 64);
+  if (FAST) {
+    g.rf = 1. / (g.f + 1.e-22);
+    g.rfn = 1. / (g.fn + 1.e-22);
+    const double fc = g.dz > 0. ? g.f : g.fl;
+    const double adz = fabs(g.dz);
+    g.dx = fc > 0. ? dmin(fc, adz) / fc : 0.;
+  }
   return g;
 }
 
 // pcwise for one column: lane j < kk holds layer j's value a; returns the
-// remapped value of layer j.
+// remapped value of layer j.  FAST (FP32 state) replaces the divisions by
+// multiplications with reciprocals from the geometry, which changes results by
+// rounding only; otherwise every operation is the reference's.
+template <bool FAST>
 __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
   const double small = 1.e-22;
   const bool live = lane < kk;
@@ -508,13 +520,13 @@ __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
     } else {
       const double t = a * f;
       fin = absorbs(t, bnd);
-      v = fin ? t / (f + small) : a;
+      v = fin ? (FAST ? t * gm.rf : t / (f + small)) : a;
     }
     while (wany(!fin)) {
       const double vu = __shfl_down(v, 1, 64);
       const int fu = __shfl_down((int)fin, 1, 64);
       if (!fin && fu) {
-        v = (a * f + vu * small) / (f + small);
+        v = FAST ? (a * f + vu * small) * gm.rf : (a * f + vu * small) / (f + small);
         fin = true;
       }
     }
@@ -530,13 +542,13 @@ __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
     } else {
       const double t = a2 * f;
       fin = absorbs(t, bnd);
-      w = fin ? t / (f + small) : a2;
+      w = fin ? (FAST ? t * gm.rf : t / (f + small)) : a2;
     }
     while (wany(!fin)) {
       const double wd = __shfl_up(w, 1, 64);
       const int fd = __shfl_up((int)fin, 1, 64);
       if (!fin && fd) {
-        w = (a2 * f + wd * small) / (f + small);
+        w = FAST ? (a2 * f + wd * small) * gm.rf : (a2 * f + wd * small) / (f + small);
         fin = true;
       }
     }
@@ -554,7 +566,14 @@ __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
   double vf = 0.;
   {
     const double wl = __shfl_up(w, 1, 64), bl = __shfl_up(b, 1, 64);
-    if (live && lane >= 1) {
+    if (FAST) {
+      // the slab is one layer's part, so amount/slab is the mean over it
+      if (live && lane >= 1) {
+        const double dz = gm.dz, dx = gm.dx;
+        if (dz > 0. && gm.f > 0.) vf = -dz * (w + b * .5 * (dx - 1.));
+        else if (dz < 0. && gm.fl > 0.) vf = -dz * (wl + bl * .5 * (1. - dx));
+      }
+    } else if (live && lane >= 1) {
       double slab = 0., amount = 0.;
       const double dz = gm.dz;
       if (dz > 0.) {
@@ -585,7 +604,7 @@ __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
   double y = 0.;
   if (live) {
     const double amount = w * f - (vhi - vlo);
-    y = (amount + w * small) / (gm.fn + small);
+    y = FAST ? (amount + w * small) * gm.rfn : (amount + w * small) / (gm.fn + small);
   }
   return y;
 }
@@ -821,17 +840,19 @@ __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl
   // u, v and tracers 2..ntra are remapped in pressure
   {
     const bool live = lane < nvl;
-    const PcwGeom gp = pcwise_geom(nvl, prcol, prnew, lane);
-    double y = pcwise_w(nvl, gp, live ? (double)C2(s.us_in, col, lane) : 0., lane);
+    constexpr bool FAST = sizeof(V) < sizeof(double);
+    const PcwGeom gp = pcwise_geom<FAST>(nvl, prcol, prnew, lane);
+    double y = pcwise_w<FAST>(nvl, gp, live ? (double)C2(s.us_in, col, lane) : 0., lane);
     if (live) C2(s.us, col, lane) = (V)y;
-    y = pcwise_w(nvl, gp, live ? (double)C2(s.vs_in, col, lane) : 0., lane);
+    y = pcwise_w<FAST>(nvl, gp, live ? (double)C2(s.vs_in, col, lane) : 0., lane);
     if (live) C2(s.vs, col, lane) = (V)y;
     for (int n = 1; n < ntra; n++) {
-      y = pcwise_w(nvl, gp, live ? (double)TR(s.tr, n, col, lane) : 0., lane);
+      y = pcwise_w<FAST>(nvl, gp, live ? (double)TR(s.tr, n, col, lane) : 0., lane);
       if (live) TR(s.tr, n, col, lane) = (V)y;
     }
     // theta is remapped in p^(kappa+1)
-    y = pcwise_w(nvl, pcwise_geom(nvl, pk1col, pk1new, lane), live ? (double)TR(s.tr, 0, col, lane) : 0., lane);
+    y = pcwise_w<FAST>(nvl, pcwise_geom<FAST>(nvl, pk1col, pk1new, lane), live ? (double)TR(s.tr, 0, col, lane) : 0.,
+                       lane);
     if (live) th1[lane] = y;
   }
   wsync();
