@@ -64,6 +64,29 @@ class Grid:
         )
 
 
+def concat(grids) -> Grid:
+    """Several grids as one: the cells of grid i follow those of grids 0..i-1
+    and their neighbour indices are offset accordingly, so no cell has a
+    neighbour in another grid.  A state on the union advances every grid's
+    instance in the same launches (instances of different resolution sharing
+    the layers and the time step); each instance's cells compute exactly what
+    they compute alone."""
+    off = 0
+    parts = {k: [] for k in ("lat", "lon", "nprox", "proxs", "prox", "area", "cs", "sn", "sidevec_c", "sidevec_e",
+                             "sideln", "rprox_ln", "inv_perm", "corner_xy", "hfcoef")}
+    for g in grids:
+        for k in parts:
+            a = getattr(g, k)
+            if k == "prox":          # slots past nprox (pentagons) are left as they are
+                used = np.arange(6)[None, :] < g.nprox[:, None]
+                a = np.where(used, a + off, a).astype(np.int32)
+            elif k == "inv_perm":
+                a = (a + off).astype(np.int32)
+            parts[k].append(a)
+        off += g.nip
+    return Grid(-1, grids[0].curve, **{k: np.ascontiguousarray(np.concatenate(v)) for k, v in parts.items()})
+
+
 def nip_of(glvl: int) -> int:
     """Cell count of an all-bisection grid (module_control.F90:209-213)."""
     return 10 * 4 ** glvl + 2

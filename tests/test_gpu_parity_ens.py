@@ -189,3 +189,51 @@ def test_twenty_lognormal_tracers_transport_fp32_vs_fp64():
     finally:
         e.close()
         p.close()
+
+
+@pytest.mark.parametrize("form", ["mixed", "fp64"])
+def test_mixed_resolution_instances_in_one_launch(form):
+    """A G4 and a G5 instance concatenated into one grid (glvl.concat) and
+    advanced by the same launches: each instance's cells come out exactly as
+    when that instance runs alone."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, glvl, state
+    th, ds = state.vertical_tables(NVL)
+    g4, g5 = fimhip.icos_grid(4), fimhip.icos_grid(5)
+    gu = glvl.concat([g4, g5])
+    n4 = g4.nip
+    print("case: G4+G5 union, %s form, 8 full steps" % form, flush=True)
+    names = ("us", "vs", "dp", "pr", "ex", "ph", "tr")
+
+    def run(grid, st):
+        if form == "mixed":
+            e = fimhip.Ensemble(grid, 1, NVL, NTRA, DT, th, ds)
+            try:
+                e.load_member(0, st)
+                e.init_state(True)
+                e.step(8, 0)
+                e.sync()
+                return {n: e.download(0, n) for n in names + ("mp",)}
+            finally:
+                e.close()
+        m = fimhip.Model(grid, NVL, NTRA, DT, th, ds)
+        try:
+            m.load(st)
+            m.init_state(True)
+            for its in range(1, 9):
+                m.step(its, 0)
+            m.sync()
+            return {n: m.download(n) for n in names + ("mp",)}
+        finally:
+            m.close()
+
+    st = fimhip.synth_state(gu.lat, gu.lon, NVL, NTRA, seed=7, member=0, flags=FLAGS)
+    if form == "mixed":       # the mixed form holds winds and tracers in FP32
+        for n in ("us", "vs", "tr"):
+            st[n] = st[n].astype(np.float32).astype(np.float64)
+    both = run(gu, st)
+    for lo, hi, g in ((0, n4, g4), (n4, gu.nip, g5)):
+        part = {n: (st[n][:, lo:hi] if n == "tr" else st[n][lo:hi]) for n in names}
+        alone = run(g, {n: np.ascontiguousarray(v) for n, v in part.items()})
+        for n, v in alone.items():
+            got = both[n][:, lo:hi] if n == "tr" else both[n][lo:hi]
+            assert np.array_equal(got, v), (form, g.nip, n)
