@@ -70,15 +70,15 @@ def ens_kernel_bytes(T):
     """The same for the mixed-precision kernels (4 B per FP32 value, 8 B per
     FP64 value; inputs shared by the tracers of one launch counted once)."""
     return {
-        "e_edgvar1": 20 + 96,                # us vs tr1 (f32) pr (f64) -> u v lp trc1 edges (f32)
+        "e_edgvar1": 20 + 72,                # us vs tr1 (f32) pr (f64) -> normal wind, lp, trc1 edges (f32)
         "e_edgvar3": 32 + 28 * (T - 1),      # dp (f64) -> dp_edg; tracers 2..T -> trc_edg
-        "e_cnuity12": 152,                   # u/v/dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo, in/outflow sums
+        "e_cnuity12": 104,                   # normal wind and dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo, sums
         "e_cnuity3": 44,                     # in/outflow sums, delp_lo, dp, dp_tdcy(of,vof) -> r+, r-
         "e_cnuity45": 180,                   # antifx r+- massfx dp_tdcy dp pr lp_edg u v -> massfx dp_tdcy dp pr ex omega
         "e_trcadv1": 32 + 80 * T,            # massfx dp; per tracer trc_edg tr trdp trl -> antiflx trl lo sums
         "e_trcadv2": 8 + 40 * T,
         "e_trcadv3": 8 + 68 * T,
-        "e_momtum": 148,                     # u v dp ex mp, u/v/theta edges, tendencies -> relvor tendency u2 v2
+        "e_momtum": 100,                     # u v dp ex mp, theta edges, tendencies -> relvor tendency u2 v2
         "k_regrid": 20,
         "k_regrid_tail": 16,
         "k_dffusn_lev": 16,

@@ -3,7 +3,9 @@
 //
 // Precision split (per field):
 //   FP32  winds, tracers, tracer mass, their Adams-Bashforth histories, the
-//         edge values, the FCT work arrays and the mass fluxes;
+//         edge values (of the edge wind only its normal component is stored;
+//         momtum forms the full edge winds itself), the FCT work arrays and
+//         the mass fluxes;
 //   FP64  layer thickness and its AB history (mass conservation), interface
 //         pressure and Exner function, Montgomery potential (also on the
 //         edges, formed where momtum needs it) and geopotential (the
@@ -37,7 +39,7 @@ struct EnsState {
   double *dp_tdcy;
   float *trc_tdcy, *trl_tdcy;
   float *massfx, *ws, *relvor;
-  float *u_edg, *v_edg, *dp_edg, *lp_edg, *trc_edg;
+  float *vn_edg, *dp_edg, *lp_edg, *trc_edg;   // vn_edg: normal edge wind
   float *antifx, *delp_lo, *r_plus, *r_mnus;
   float *antiflx, *trcr_lo, *trmax, *trmin, *tr_rp, *tr_rm;
   StepCtl *ctl;

@@ -63,9 +63,36 @@ __device__ __forceinline__ double wave_prefix_sum(double x, int lane) {
   return x;
 }
 
+// The edge wind of cell c at its slot e (edgvar1.F90:162-421, eqwgt): the
+// winds of the cell (i), the cell across the edge (x) and the two cells at
+// the edge's ends (t: slot e-1, l: slot e+1) rotated into the edge frame with
+// cell c's coefficients, trapezoid-weighted, the magnitude rescaled to the
+// weighted mean of the four magnitudes.  edgvar1, cnuity12 and momtum all form
+// edge winds through this one function, so every use sees the same bits.
+__device__ __forceinline__ float2 e_edge_wind(const EnsGrid &g, int c, int e, float ui, float vi, float ux, float vx,
+                                              float ut, float vt, float ul, float vl) {
+  const float d6 = 1.f / 6.f, eps = 0.001f;
+  const float c0 = FCS(c, e, 0), s0 = FSN(c, e, 0), c1 = FCS(c, e, 1), s1 = FSN(c, e, 1);
+  const float c2 = FCS(c, e, 2), s2 = FSN(c, e, 2), c3 = FCS(c, e, 3), s3 = FSN(c, e, 3);
+  const float u3trl = c2 * ut + s2 * vt, v3trl = -s2 * ut + c2 * vt;
+  const float u1 = c0 * ui + s0 * vi, v1 = -s0 * ui + c0 * vi;
+  const float u2 = c1 * ux + s1 * vx, v2 = -s1 * ux + c1 * vx;
+  const float u3l = c3 * ul + s3 * vl, v3l = -s3 * ul + c3 * vl;
+  float ue = (2.f * (u1 + u2) + u3trl + u3l) * d6;
+  float ve = (2.f * (v1 + v2) + v3trl + v3l) * d6;
+  const float m1l = sqrtf(u1 * u1 + v1 * v1), m2l = sqrtf(u2 * u2 + v2 * v2);
+  const float m3t = sqrtf(u3trl * u3trl + v3trl * v3trl), m3l = sqrtf(u3l * u3l + v3l * v3l);
+  const float m1 = sqrtf(ue * ue + ve * ve);
+  const float m2 = (2.f * (m1l + m2l) + m3t + m3l) * d6;
+  const float sc = m2 / fmaxf(eps, m1);
+  return make_float2(ue * sc, ve * sc);
+}
+
 // ---------------------------------------------------------------------------
-// edgvar1 (edgvar1.F90:162-421, 425-511): the six neighbour columns are read
-// once into registers and every edge's stencil is formed from them.
+// edgvar1 (edgvar1.F90:162-421, 425-511): the edge pressure and theta, and
+// of the edge wind only its normal component sidevec_e x (u,v), the one
+// cnuity1 uses (cnuity_compute.F90:17-95); momtum forms the edge winds it
+// needs itself (e_edge_wind).
 template <int NP>
 __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &s, int ipn, int base, int k) {
   const int nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
@@ -84,27 +111,12 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
   }
   const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k), thi = ETR(s.tr, 0, col, k);
   const double pi0 = EL(s.pr, col, k), pi1 = EL(s.pr, col, k + 1);
-  const float d6 = 1.f / 6.f, eps = 0.001f;
+  const float d6 = 1.f / 6.f;
 #pragma unroll
   for (int e = 0; e < NP; e++) {
     const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
-    const float c0 = FCS(ipn, e, 0), s0 = FSN(ipn, e, 0), c1 = FCS(ipn, e, 1), s1 = FSN(ipn, e, 1);
-    const float c2 = FCS(ipn, e, 2), s2 = FSN(ipn, e, 2), c3 = FCS(ipn, e, 3), s3 = FSN(ipn, e, 3);
-    const float u3trl = c2 * nu[et] + s2 * nv[et], v3trl = -s2 * nu[et] + c2 * nv[et];
-    const float u1 = c0 * ui + s0 * vi, v1 = -s0 * ui + c0 * vi;
-    const float u2 = c1 * nu[e] + s1 * nv[e], v2 = -s1 * nu[e] + c1 * nv[e];
-    const float u3l = c3 * nu[el] + s3 * nv[el], v3l = -s3 * nu[el] + c3 * nv[el];
-    float ue = (2.f * (u1 + u2) + u3trl + u3l) * d6;
-    float ve = (2.f * (v1 + v2) + v3trl + v3l) * d6;
-    const float m1l = sqrtf(u1 * u1 + v1 * v1), m2l = sqrtf(u2 * u2 + v2 * v2);
-    const float m3t = sqrtf(u3trl * u3trl + v3trl * v3trl), m3l = sqrtf(u3l * u3l + v3l * v3l);
-    const float m1 = sqrtf(ue * ue + ve * ve);
-    const float m2 = (2.f * (m1l + m2l) + m3t + m3l) * d6;
-    const float sc = m2 / fmaxf(eps, m1);
-    ue = ue * sc;
-    ve = ve * sc;
-    EE(s.u_edg, col, e, k) = ue;
-    EE(s.v_edg, col, e, k) = ve;
+    const float2 w = e_edge_wind(g, ipn, e, ui, vi, nu[e], nv[e], nu[et], nv[et], nu[el], nv[el]);
+    EE(s.vn_edg, col, e, k) = FSVE(ipn, e, 1) * w.x - FSVE(ipn, e, 0) * w.y;
     EE(s.lp_edg, col, e, k) =
         (float)((pi0 + pi1 + np0[e] + np1[e] + .5 * (np0[et] + np1[et] + np0[el] + np1[el])) * (1. / 6.));
     // theta on the edge, the same bits from both cells of the edge: the two
@@ -202,8 +214,7 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity12(EnsGrid g, EnsState s)
   float pp = 0.f, pm = 0.f;     // cnuity3's sums of the antidiffusive in- and outflows
   for (int e = 0; e < np; e++) {
     const int x = GPROX(ipn, e), ex = GPROXS(ipn, e), xc = base + x;
-    const float vn = FSVE(ipn, e, 1) * EE(s.u_edg, col, e, k) - FSVE(ipn, e, 0) * EE(s.v_edg, col, e, k);
-    const float vx = FSVE(x, ex, 1) * EE(s.u_edg, xc, ex, k) - FSVE(x, ex, 0) * EE(s.v_edg, xc, ex, k);
+    const float vn = EE(s.vn_edg, col, e, k), vx = EE(s.vn_edg, xc, ex, k);
     const float dpe = .5f * (EE(s.dp_edg, col, e, k) + EE(s.dp_edg, xc, ex, k));
     const float hi = ipn < x ? vn * dpe : -(vx * dpe);
     const float m = .5f * ((vn + fabsf(vn)) * dpf - (vx + fabsf(vx)) * (float)EC(s.dp, xc, k));
@@ -450,10 +461,9 @@ __global__ void __launch_bounds__(64 * COLS) k_e_trcadv3(EnsGrid g, EnsState s) 
 // momtum (momtum.F90:101-158; dissip momtum_compute.F90:102-229; momtum1
 // 231-450).  Diffusion, vorticity, Coriolis and the AB update in FP32; the
 // pressure-gradient line integral of Montgomery potential and Exner function
-// in FP64.  The edge Montgomery potential and kinetic energy are formed here
-// from the cell values and the edge winds, with the operations edgvar1 uses
-// for them (edgvar1.F90:162-421), instead of being stored by edgvar1 and read
-// back.  New winds go to u2/v2 (hybgen's remap reads them from there).
+// in FP64.  The edge winds, Montgomery potential and kinetic energy are
+// formed here from the cell values, with the operations edgvar1 uses for them
+// (edgvar1.F90:162-421), instead of being stored by edgvar1 and read back.  New winds go to u2/v2 (hybgen's remap reads them from there).
 template <int NP>
 __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s, const StepCtl &c, int ipn,
                                              int base, int k, float &un, float &vn) {
@@ -499,7 +509,8 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
 #pragma unroll
   for (int e = 0; e < NP; e++) {
     const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
-    const float ue = EE(s.u_edg, col, e, k), ve = EE(s.v_edg, col, e, k);
+    const float2 w = e_edge_wind(g, ipn, e, ui, vi, nu[e], nv[e], nu[et], nv[et], nu[el], nv[el]);
+    const float ue = w.x, ve = w.y;
     const double mont = (2. * (mpi + nmp[e]) + nmp[et] + nmp[el]) * (1. / 6.);
     const double b = mont + (double)(.5f * (ue * ue + ve * ve));
     const double te = (double)ETE(s.trc_edg, 0, col, e, k);
