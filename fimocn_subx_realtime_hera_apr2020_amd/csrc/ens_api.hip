@@ -31,8 +31,13 @@ struct fimhip_ens {
   double *gs64 = nullptr;
   float *gs32 = nullptr;
   // captured step
-  bool use_graph = true;
+  bool use_graph = true, overlap = true;
   int graph_mode = -1;
+  // a second stream for the kernels of a step that do not depend on each
+  // other (edgvar1 / edgvar3; momtum / the tracer FCT and the first part of
+  // hybgen), forked and joined with events; used in the captured graph
+  hipStream_t side = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   KTimer *kt = nullptr;
@@ -82,6 +87,9 @@ extern "C" int fimhip_ens_create(const fimhip_grid_desc *gd, int members, int de
     return FIMHIP_EHIP;
   }
   e->arena.stream = e->stream;
+  if (hipStreamCreate(&e->side) != hipSuccess) { fimhip_ens_destroy(e); return FIMHIP_EHIP; }
+  for (auto &v : e->ev)
+    if (hipEventCreateWithFlags(&v, hipEventDisableTiming) != hipSuccess) { fimhip_ens_destroy(e); return FIMHIP_EHIP; }
   Arena &a = e->arena;
   EnsGrid &g = e->g;
   if ((r = dev_grid_upload(a, gd, hg, g.d))) { fimhip_ens_destroy(e); return r; }
@@ -142,6 +150,9 @@ extern "C" int fimhip_ens_destroy(fimhip_ens *e) {
   drop_graph(e);
   delete e->kt;
   e->arena.release();
+  for (auto v : e->ev)
+    if (v) (void)hipEventDestroy(v);
+  if (e->side) (void)hipStreamDestroy(e->side);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return FIMHIP_OK;
@@ -219,15 +230,39 @@ extern "C" int fimhip_ens_init_state(fimhip_ens *e, int init_hybgen) {
 }
 
 // one step's launches (DYN_1 then DYN_2, dyn_run.F90:119-425)
-static void enqueue_step(fimhip_ens *e, int mode) {
+// With 'side', edgvar3 runs beside edgvar1, and momtum beside the tracer FCT
+// and hybgen up to remap_1d (the first reader of its winds): momtum reads only
+// what cnuity and edgvar1 left and writes only what remap_1d and the next
+// step read.
+static void enqueue_step(fimhip_ens *e, int mode, bool side) {
   const EnsGrid &g = e->g;
   const EnsState &s = e->s;
-  hipStream_t st = e->stream;
+  hipStream_t st = e->stream, sd = e->side;
   ens_launch_abstart(s, e->dt, st);
-  for (int k = EK_EDGVAR1; k <= EK_TRCADV3; k++) ens_launch(k, e->tile[k], g, s, st);
+  if (side) {
+    (void)hipEventRecord(e->ev[0], st);
+    (void)hipStreamWaitEvent(sd, e->ev[0], 0);
+    ens_launch(EK_EDGVAR3, e->tile[EK_EDGVAR3], g, s, sd);
+    (void)hipEventRecord(e->ev[1], sd);
+    ens_launch(EK_EDGVAR1, e->tile[EK_EDGVAR1], g, s, st);
+    (void)hipStreamWaitEvent(st, e->ev[1], 0);
+  } else {
+    ens_launch(EK_EDGVAR1, e->tile[EK_EDGVAR1], g, s, st);
+    ens_launch(EK_EDGVAR3, e->tile[EK_EDGVAR3], g, s, st);
+  }
+  for (int k = EK_CNUITY12; k <= EK_CNUITY45; k++) ens_launch(k, e->tile[k], g, s, st);
+  hipEvent_t momtum_done = nullptr;
+  if (mode == 0 && side) {
+    (void)hipEventRecord(e->ev[2], st);
+    (void)hipStreamWaitEvent(sd, e->ev[2], 0);
+    ens_launch(EK_MOMTUM, e->tile[EK_MOMTUM], g, s, sd);
+    (void)hipEventRecord(e->ev[3], sd);
+    momtum_done = e->ev[3];
+  }
+  for (int k = EK_TRCADV1; k <= EK_TRCADV3; k++) ens_launch(k, e->tile[k], g, s, st);
   if (mode == 0) {
-    ens_launch(EK_MOMTUM, e->tile[EK_MOMTUM], g, s, st);
-    launch_hybgen_cols<float>(hyb_cols(e), s.ctl, 0, st);
+    if (!side) ens_launch(EK_MOMTUM, e->tile[EK_MOMTUM], g, s, st);
+    launch_hybgen_cols<float>(hyb_cols(e), s.ctl, 0, st, momtum_done);
     ens_launch(EK_HYSTAT, e->tile[EK_HYSTAT], g, s, st);
   }
 }
@@ -237,7 +272,7 @@ extern "C" int fimhip_ens_step(fimhip_ens *e, int nsteps, int mode) {
   HIPCHK(hipSetDevice(e->device));
   if (!e->use_graph || e->kt) {
     g_ktimer = e->kt;
-    for (int i = 0; i < nsteps; i++) enqueue_step(e, mode);
+    for (int i = 0; i < nsteps; i++) enqueue_step(e, mode, false);
     g_ktimer = nullptr;
     HIPCHK(hipGetLastError());
     if (e->kt && e->kt->flush(e->stream)) return FIMHIP_EHIP;
@@ -246,7 +281,7 @@ extern "C" int fimhip_ens_step(fimhip_ens *e, int nsteps, int mode) {
   if (e->graph_mode != mode) {
     drop_graph(e);
     HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(e, mode);
+    enqueue_step(e, mode, e->overlap);
     HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
     HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
     e->graph_mode = mode;
@@ -258,7 +293,8 @@ extern "C" int fimhip_ens_step(fimhip_ens *e, int nsteps, int mode) {
 extern "C" int fimhip_ens_use_graph(fimhip_ens *e, int on) {
   if (!e) return FIMHIP_EINVAL;
   e->use_graph = on != 0;
-  if (!e->use_graph) drop_graph(e);
+  e->overlap = on != 2;       // 2: a graph without the second stream
+  drop_graph(e);
   return FIMHIP_OK;
 }
 
@@ -397,7 +433,7 @@ extern "C" int fimhip_ens_autotune(fimhip_ens *e, int mode, int reps, double *ti
     for (int k = 0; k < EK_NKERNELS; k++) e->tile[k] = t;
     KTimer kt;
     g_ktimer = &kt;
-    for (int i = 0; i < reps; i++) enqueue_step(e, mode);
+    for (int i = 0; i < reps; i++) enqueue_step(e, mode, false);
     g_ktimer = nullptr;
     HIPCHK(hipGetLastError());
     if (kt.flush(e->stream)) return FIMHIP_EHIP;

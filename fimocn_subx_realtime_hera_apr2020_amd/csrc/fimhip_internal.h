@@ -102,8 +102,11 @@ struct HybCols {
   const double *thetac, *dpsig, *smoo, *sideln, *rarea;
   const int *prox, *nprox;
 };
+// winds_ready (may be null): an event remap_1d's launch waits for, since it is
+// the first of these kernels to read the winds
 template <typename V>
-void launch_hybgen_cols(const HybCols<V> &h, const StepCtl *ctl, int its, hipStream_t st);
+void launch_hybgen_cols(const HybCols<V> &h, const StepCtl *ctl, int its, hipStream_t st,
+                        hipEvent_t winds_ready = nullptr);
 
 // Per-kernel HIP-event timing: when g_ktimer is set (by the API for an
 // instance with profiling on), every launch is bracketed by events on its stream.

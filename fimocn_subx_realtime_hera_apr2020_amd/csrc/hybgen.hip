@@ -1043,10 +1043,11 @@ static bool hybgen_thread_per_column() {
 }
 
 template <typename V>
-void launch_hybgen_cols(const HybCols<V> &h, const StepCtl *ctl, int its, hipStream_t st) {
+void launch_hybgen_cols(const HybCols<V> &h, const StepCtl *ctl, int its, hipStream_t st, hipEvent_t winds_ready) {
   if (hybgen_thread_per_column()) {
     KT_LAUNCH("k_regrid", st, k_regrid<V><<<(h.ncol + 63) / 64, 64, 0, st>>>(h, ctl, its));
     KT_LAUNCH("k_dffusn_lev", st, k_dffusn_lev<V><<<(h.ncol + 3) / 4, dim3(64, 4), 0, st>>>(h));
+    if (winds_ready) (void)hipStreamWaitEvent(st, winds_ready, 0);
     KT_LAUNCH("k_remap", st, k_remap<V><<<(h.ncol + 63) / 64, 64, 0, st>>>(h, ctl, its));
     return;
   }
@@ -1058,15 +1059,17 @@ void launch_hybgen_cols(const HybCols<V> &h, const StepCtl *ctl, int its, hipStr
     KT_LAUNCH("k_regrid_tail", st, (k_regrid_tail<NVL_MAX, V><<<tg, tb, 0, st>>>(h, ctl, its, extop)));
     KT_LAUNCH("k_dffusn_lev", st, k_dffusn_lev<V><<<(h.ncol + 3) / 4, dim3(64, 4), 0, st>>>(h));
     KT_LAUNCH("k_remap_inflate", st, (k_remap_inflate<NVL_MAX, V><<<tg, tb, 0, st>>>(h, extop)));
+    if (winds_ready) (void)hipStreamWaitEvent(st, winds_ready, 0);
     KT_LAUNCH("k_remap", st, (k_remap_w<V, true><<<wg, wb, 0, st>>>(h, ctl, its)));
     KT_LAUNCH("k_remap_theta", st, (k_remap_theta<NVL_MAX, V><<<tg, tb, 0, st>>>(h)));
     return;
   }
   KT_LAUNCH("k_regrid", st, (k_regrid_w<V, false><<<wg, wb, 0, st>>>(h, ctl, its)));
   KT_LAUNCH("k_dffusn_lev", st, k_dffusn_lev<V><<<(h.ncol + 3) / 4, dim3(64, 4), 0, st>>>(h));
+  if (winds_ready) (void)hipStreamWaitEvent(st, winds_ready, 0);
   KT_LAUNCH("k_remap", st, (k_remap_w<V, false><<<wg, wb, 0, st>>>(h, ctl, its)));
 }
-template void launch_hybgen_cols<float>(const HybCols<float> &, const StepCtl *, int, hipStream_t);
+template void launch_hybgen_cols<float>(const HybCols<float> &, const StepCtl *, int, hipStream_t, hipEvent_t);
 
 void launch_hybgen(const DevGrid &g, const DevState &s, int its, hipStream_t st) {
   HybCols<double> h{g.nip, g.nip, g.nvl, g.ntra, s.us, s.vs, s.us, s.vs, s.tr, s.dp, s.pr, s.ex, s.sdot, s.exsmo,
