@@ -144,20 +144,22 @@ __global__ void __launch_bounds__(64 * COLS) k_e_edgvar1(EnsGrid g, EnsState s) 
 // thickness is reconstructed as a deviation from the cell's own value, the
 // differences taken in FP64.
 template <int NP>
-__device__ __forceinline__ void e_edgvar3_col(const EnsGrid &g, const EnsState &s, int ipn, int base, int k,
-                                              int f) {
-  const int nvl = g.d.nvl, ncol = s.ncol;
+__device__ __forceinline__ void e_edgvar3_col(const EnsGrid &g, const EnsState &s, int ipn, int base, int k) {
+  const int nvl = g.d.nvl, ncol = s.ncol, ntra = g.d.ntra;
   const int col = base + ipn;
   float hf[NP][NP];
+  int nb[NP];
 #pragma unroll
-  for (int i = 0; i < NP; i++)
+  for (int i = 0; i < NP; i++) {
+    nb[i] = base + GPROX(ipn, i);
 #pragma unroll
     for (int j = 0; j < NP; j++) hf[i][j] = FHF(ipn, i, j);
+  }
   float d[NP], out[NP];
-  if (f == 0) {
+  {
     const double b0 = EC(s.dp, col, k);
 #pragma unroll
-    for (int j = 0; j < NP; j++) d[j] = (float)(EC(s.dp, base + GPROX(ipn, j), k) - b0);
+    for (int j = 0; j < NP; j++) d[j] = (float)(EC(s.dp, nb[j], k) - b0);
 #pragma unroll
     for (int i = 0; i < NP; i++) {
       float acc = 0.f;
@@ -168,10 +170,11 @@ __device__ __forceinline__ void e_edgvar3_col(const EnsGrid &g, const EnsState &
 #pragma unroll
     for (int i = 0; i < NP; i++) EE(s.dp_edg, col, i, k) = out[i];
     if (NP < NPP) EE(s.dp_edg, col, NPP - 1, k) = 0.f;
-  } else {
+  }
+  for (int f = 1; f < ntra; f++) {
     const float b0 = ETR(s.tr, f, col, k);
 #pragma unroll
-    for (int j = 0; j < NP; j++) d[j] = ETR(s.tr, f, base + GPROX(ipn, j), k) - b0;
+    for (int j = 0; j < NP; j++) d[j] = ETR(s.tr, f, nb[j], k) - b0;
 #pragma unroll
     for (int i = 0; i < NP; i++) {
       float acc = 0.f;
@@ -191,8 +194,8 @@ __global__ void __launch_bounds__(64 * COLS) k_e_edgvar3(EnsGrid g, EnsState s) 
   const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
   if (ipn >= nip || k >= nvl) return;
   const int base = blockIdx.y * nip;
-  if (GNP(ipn) == 6) e_edgvar3_col<6>(g, s, ipn, base, k, blockIdx.z);
-  else e_edgvar3_col<5>(g, s, ipn, base, k, blockIdx.z);
+  if (GNP(ipn) == 6) e_edgvar3_col<6>(g, s, ipn, base, k);
+  else e_edgvar3_col<5>(g, s, ipn, base, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -756,7 +759,7 @@ void ens_launch(int kernel, int tile, const EnsGrid &g, const EnsState &s, hipSt
   if (::fimhip::g_ktimer) kt_begin(name, st);
   switch (kernel) {
     case EK_EDGVAR1: ENS_DISPATCH(k_e_edgvar1, 1); break;
-    case EK_EDGVAR3: ENS_DISPATCH(k_e_edgvar3, ntra); break;
+    case EK_EDGVAR3: ENS_DISPATCH(k_e_edgvar3, 1); break;
     case EK_CNUITY12: ENS_DISPATCH(k_e_cnuity12, 1); break;
     case EK_CNUITY3: ENS_DISPATCH(k_e_cnuity3, 1); break;
     case EK_CNUITY45: ENS_DISPATCH(k_e_cnuity45, 1); break;
