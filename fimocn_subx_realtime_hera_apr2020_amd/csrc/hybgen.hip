@@ -402,6 +402,47 @@ __device__ __forceinline__ double w_prefix_min(double x, int lane) {
   }
   return x;
 }
+// Cross-lane moves on the VALU (DPP) instead of the LDS crossbar:
+// w_up1(x) on lane j is lane j-1's x, w_dn1(x) lane j+1's (0 where that lane
+// does not exist), w_max_all the column-wide maximum of x >= 0 on every lane.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int old, int x) {
+  return __builtin_amdgcn_update_dpp(old, x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_z(double x) {   // lanes without a source get 0
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, true);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_d(double old, double x) {
+  const unsigned long long o = (unsigned long long)__double_as_longlong(old);
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)dpp_i<CTRL, ROWS>((int)(unsigned)o, (int)(unsigned)b);
+  const unsigned hi = (unsigned)dpp_i<CTRL, ROWS>((int)(unsigned)(o >> 32), (int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+constexpr int DPP_XOR1 = 0xb1, DPP_XOR2 = 0x4e, DPP_ROW_MIRROR = 0x140, DPP_HALF_MIRROR = 0x141;
+constexpr int DPP_BCAST15 = 0x142, DPP_BCAST31 = 0x143;
+__device__ __forceinline__ double w_up1(double x) { return dpp_z<DPP_WAVE_SHR1>(x); }
+__device__ __forceinline__ double w_dn1(double x) { return dpp_z<DPP_WAVE_SHL1>(x); }
+__device__ __forceinline__ int w_dn1(int x) { return __builtin_amdgcn_mov_dpp(x, DPP_WAVE_SHL1, 0xf, 0xf, true); }
+__device__ __forceinline__ int w_up1(int x) { return __builtin_amdgcn_mov_dpp(x, DPP_WAVE_SHR1, 0xf, 0xf, true); }
+__device__ __forceinline__ double w_max_all(double x) {
+  x = dmax(x, dpp_d<DPP_XOR1>(x, x));          // quads
+  x = dmax(x, dpp_d<DPP_XOR2>(x, x));
+  x = dmax(x, dpp_d<DPP_HALF_MIRROR>(x, x));   // half rows
+  x = dmax(x, dpp_d<DPP_ROW_MIRROR>(x, x));    // rows of 16
+  x = dmax(x, dpp_d<DPP_BCAST15, 0xa>(x, x));  // row 1 <- row 0, row 3 <- row 2
+  x = dmax(x, dpp_d<DPP_BCAST31, 0xc>(x, x));  // rows 2, 3 <- row 1
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 __device__ __forceinline__ int w_min_int(int x) {
   for (int d = 1; d < 64; d <<= 1) x = min(x, __shfl_xor(x, d, 64));
   return x;
@@ -486,7 +527,7 @@ __device__ __forceinline__ PcwGeom pcwise_geom(int kk, const double *xold, const
     g.fn = dmax(0., zn(lane + 1) - zn(lane));
     g.dz = zn(lane) - zo(lane);
   }
-  g.fl = __shfl_up(g.f, 1, 64);
+  g.fl = w_up1(g.f);
   if (FAST) {
     g.rf = 1. / (g.f + 1.e-22);
     g.rfn = 1. / (g.fn + 1.e-22);
@@ -509,7 +550,7 @@ __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
   // a bound on every value either fill sweep can produce: the largest
   // magnitude in the column (the sweeps form convex combinations)
   double bnd = live ? fabs(a) : 0.;
-  for (int d = 1; d < 64; d <<= 1) bnd = dmax(bnd, __shfl_xor(bnd, d, 64));
+  bnd = w_max_all(bnd);
   // sweep 1, from the top: layer j (j <= kk-2) takes layer j+1's new value
   double v;
   {
@@ -523,8 +564,8 @@ __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
       v = fin ? (FAST ? t * gm.rf : t / (f + small)) : a;
     }
     while (wany(!fin)) {
-      const double vu = __shfl_down(v, 1, 64);
-      const int fu = __shfl_down((int)fin, 1, 64);
+      const double vu = w_dn1(v);
+      const int fu = w_dn1((int)fin);
       if (!fin && fu) {
         v = FAST ? (a * f + vu * small) * gm.rf : (a * f + vu * small) / (f + small);
         fin = true;
@@ -545,8 +586,8 @@ __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
       w = fin ? (FAST ? t * gm.rf : t / (f + small)) : a2;
     }
     while (wany(!fin)) {
-      const double wd = __shfl_up(w, 1, 64);
-      const int fd = __shfl_up((int)fin, 1, 64);
+      const double wd = w_up1(w);
+      const int fd = w_up1((int)fin);
       if (!fin && fd) {
         w = FAST ? (a2 * f + wd * small) * gm.rf : (a2 * f + wd * small) / (f + small);
         fin = true;
@@ -554,7 +595,7 @@ __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
     }
   }
   // PLM slope of layer lane (1-based k = lane + 1)
-  const double fm = __shfl_up(w, 1, 64), fp = __shfl_down(w, 1, 64);
+  const double fm = w_up1(w), fp = w_dn1(w);
   double b = 0.;
   if (live && lane != 0 && lane != kk - 1) {
     if (!(w <= dmin(fm, fp) || w >= dmax(fm, fp))) {
@@ -565,7 +606,7 @@ __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
   // flux through interface lane (1-based vertfx(lane + 1)), lanes 1..kk-1
   double vf = 0.;
   {
-    const double wl = __shfl_up(w, 1, 64), bl = __shfl_up(b, 1, 64);
+    const double wl = w_up1(w), bl = w_up1(b);
     if (FAST) {
       // the slab is one layer's part, so amount/slab is the mean over it
       if (live && lane >= 1) {
@@ -598,7 +639,7 @@ __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
       if (slab != 0.) vf = -dz * amount / slab;
     }
   }
-  double vhi = __shfl_down(vf, 1, 64);
+  double vhi = w_dn1(vf);
   if (lane >= kk - 1) vhi = 0.;
   const double vlo = vf;
   double y = 0.;

@@ -63,6 +63,13 @@ __device__ __forceinline__ double wave_prefix_sum(double x, int lane) {
   return x;
 }
 
+// Hardware square root and reciprocal (v_sqrt_f32, v_rcp_f32: about 1 ulp)
+// in place of the correctly rounded sequences of sqrtf and '/', which cost
+// about ten instructions each; every reciprocal below is of a value bounded
+// away from zero (a threshold or eps under fmaxf).
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // The edge wind of cell c at its slot e (edgvar1.F90:162-421, eqwgt): the
 // winds of the cell (i), the cell across the edge (x) and the two cells at
 // the edge's ends (t: slot e-1, l: slot e+1) rotated into the edge frame with
@@ -80,11 +87,11 @@ __device__ __forceinline__ float2 e_edge_wind(const EnsGrid &g, int c, int e, fl
   const float u3l = c3 * ul + s3 * vl, v3l = -s3 * ul + c3 * vl;
   float ue = (2.f * (u1 + u2) + u3trl + u3l) * d6;
   float ve = (2.f * (v1 + v2) + v3trl + v3l) * d6;
-  const float m1l = sqrtf(u1 * u1 + v1 * v1), m2l = sqrtf(u2 * u2 + v2 * v2);
-  const float m3t = sqrtf(u3trl * u3trl + v3trl * v3trl), m3l = sqrtf(u3l * u3l + v3l * v3l);
-  const float m1 = sqrtf(ue * ue + ve * ve);
+  const float m1l = fsqrt(u1 * u1 + v1 * v1), m2l = fsqrt(u2 * u2 + v2 * v2);
+  const float m3t = fsqrt(u3trl * u3trl + v3trl * v3trl), m3l = fsqrt(u3l * u3l + v3l * v3l);
+  const float m1 = fsqrt(ue * ue + ve * ve);
   const float m2 = (2.f * (m1l + m2l) + m3t + m3l) * d6;
-  const float sc = m2 / fmaxf(eps, m1);
+  const float sc = m2 * frcp(fmaxf(eps, m1));
   return make_float2(ue * sc, ve * sc);
 }
 
@@ -263,10 +270,11 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity3(EnsGrid g, EnsState s) 
   mx = fmaxf(0.f, mx);
   mn = fmaxf(0.f, mn);
   const double to = EAB(s.dp_tdcy, of, col, k), tv = EAB(s.dp_tdcy, vof, col, k);
-  const float qp = (float)(((double)mx - lo - (c.ab2 * fmin(0., to) + c.ab3 * fmax(0., tv))) / c.ab1) * GF(area, ipn);
-  const float qm = (float)(((double)lo - mn + (c.ab2 * fmax(0., to) + c.ab3 * fmin(0., tv))) / c.ab1) * GF(area, ipn);
-  EC(s.r_plus, col, k) = fmaxf(0.f, fminf(1.f, qp / fmaxf(pp, thshld)));
-  EC(s.r_mnus, col, k) = fmaxf(0.f, fminf(1.f, qm / fmaxf(pm, thshld)));
+  const double rab1 = 1. / c.ab1;
+  const float qp = (float)(((double)mx - lo - (c.ab2 * fmin(0., to) + c.ab3 * fmax(0., tv))) * rab1) * GF(area, ipn);
+  const float qm = (float)(((double)lo - mn + (c.ab2 * fmax(0., to) + c.ab3 * fmin(0., tv))) * rab1) * GF(area, ipn);
+  EC(s.r_plus, col, k) = fmaxf(0.f, fminf(1.f, qp * frcp(fmaxf(pp, thshld))));
+  EC(s.r_mnus, col, k) = fmaxf(0.f, fminf(1.f, qm * frcp(fmaxf(pm, thshld))));
 }
 
 // cnuity4 + cnuity5 (cnuity_compute.F90:261-451).  The thickness tendency is
@@ -345,7 +353,7 @@ __global__ void __launch_bounds__(64 * COLS) k_e_trcadv1(EnsGrid g, EnsState s) 
     exs[e] = GPROXS(ipn, e);
     m[e] = e < np ? EE(s.massfx, col, e, k) : 0.f;
   }
-  const float dpf = (float)EC(s.dp, col, k), rar = GF(rarea, ipn);
+  const float rdp = frcp(fmaxf(thshld, (float)EC(s.dp, col, k))), rar = GF(rarea, ipn);
   const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
   for (int t = 0; t < ntra; t++) {
     const float ti = ETR(s.tr, t, col, k);
@@ -373,7 +381,7 @@ __global__ void __launch_bounds__(64 * COLS) k_e_trcadv1(EnsGrid g, EnsState s) 
                      a3 * ETAB(s.trl_tdcy, t, vof, col, k);
     mx = fmaxf(0.f, mx);
     mn = fmaxf(0.f, mn);
-    ETR(s.trcr_lo, t, col, k) = fmaxf(mn, fminf(mx, lo / fmaxf(thshld, dpf)));
+    ETR(s.trcr_lo, t, col, k) = fmaxf(mn, fminf(mx, lo * rdp));
     ETR(s.tr_rp, t, col, k) = sp;
     ETR(s.tr_rm, t, col, k) = sm;
   }
@@ -392,7 +400,7 @@ __global__ void __launch_bounds__(64 * COLS) k_e_trcadv2(EnsGrid g, EnsState s) 
 #pragma unroll
   for (int e = 0; e < NPP; e++) xs[e] = base + GPROX(ipn, e);
   const float dp = (float)EC(s.dp, col, k), area = GF(area, ipn);
-  const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
+  const float a2 = (float)c.ab2, a3 = (float)c.ab3, ra1 = (float)(1. / c.ab1) * area;
   for (int t = 0; t < ntra; t++) {
     const float sp = ETR(s.tr_rp, t, col, k), sm = ETR(s.tr_rm, t, col, k);
     const float ti = ETR(s.tr, t, col, k), lo = ETR(s.trcr_lo, t, col, k);
@@ -416,10 +424,10 @@ __global__ void __launch_bounds__(64 * COLS) k_e_trcadv2(EnsGrid g, EnsState s) 
     ETR(s.trmax, t, col, k) = mx;
     ETR(s.trmin, t, col, k) = mn;
     const float to = ETAB(s.trc_tdcy, t, of, col, k), tv = ETAB(s.trc_tdcy, t, vof, col, k);
-    const float qp = ((mx - lo) * dp - (a2 * fminf(0.f, to) + a3 * fmaxf(0.f, tv))) / a1 * area;
-    const float qm = ((lo - mn) * dp + (a2 * fmaxf(0.f, to) + a3 * fminf(0.f, tv))) / a1 * area;
-    ETR(s.tr_rp, t, col, k) = fmaxf(0.f, fminf(1.f, qp / fmaxf(sp, thshld)));
-    ETR(s.tr_rm, t, col, k) = fmaxf(0.f, fminf(1.f, qm / fmaxf(sm, thshld)));
+    const float qp = ((mx - lo) * dp - (a2 * fminf(0.f, to) + a3 * fmaxf(0.f, tv))) * ra1;
+    const float qm = ((lo - mn) * dp + (a2 * fmaxf(0.f, to) + a3 * fminf(0.f, tv))) * ra1;
+    ETR(s.tr_rp, t, col, k) = fmaxf(0.f, fminf(1.f, qp * frcp(fmaxf(sp, thshld))));
+    ETR(s.tr_rm, t, col, k) = fmaxf(0.f, fminf(1.f, qm * frcp(fmaxf(sm, thshld))));
   }
 }
 
@@ -436,7 +444,7 @@ __global__ void __launch_bounds__(64 * COLS) k_e_trcadv3(EnsGrid g, EnsState s) 
   int xs[NPP];
 #pragma unroll
   for (int e = 0; e < NPP; e++) xs[e] = base + GPROX(ipn, e);
-  const float dpd = fmaxf(thshld, (float)EC(s.dp, col, k)), rar = GF(rarea, ipn);
+  const float rdp = frcp(fmaxf(thshld, (float)EC(s.dp, col, k))), rar = GF(rarea, ipn);
   const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
   for (int t = 0; t < ntra; t++) {
     const float rp = ETR(s.tr_rp, t, col, k), rm = ETR(s.tr_rm, t, col, k);
@@ -456,7 +464,7 @@ __global__ void __launch_bounds__(64 * COLS) k_e_trcadv3(EnsGrid g, EnsState s) 
     const float tdp = ETR(s.trdp, t, col, k) + a1 * tt + a2 * ETAB(s.trc_tdcy, t, of, col, k) +
                       a3 * ETAB(s.trc_tdcy, t, vof, col, k);
     ETR(s.trdp, t, col, k) = tdp;
-    ETR(s.tr, t, col, k) = fmaxf(ETR(s.trmin, t, col, k), fminf(ETR(s.trmax, t, col, k), tdp / dpd));
+    ETR(s.tr, t, col, k) = fmaxf(ETR(s.trmin, t, col, k), fminf(ETR(s.trmax, t, col, k), tdp * rdp));
   }
 }
 
@@ -488,7 +496,7 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
   const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k);
   const float wui = ui + GF(grndspd, ipn), wvi = vi;
   const float a = fmaxf((float)EC(s.dp, col, k), thshld);
-  const float factor = GF(rarea, ipn) / a;
+  const float factor = GF(rarea, ipn) * frcp(a);
   const float dff = g.dfflen[k];
   float wu = 0.f, wv = 0.f;
 #pragma unroll
@@ -498,7 +506,7 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
     const float uxy1 = c0 * wui + s0 * wvi, vxy1 = -s0 * wui + c0 * wvi;
     const float uxy2 = c1 * wux + s1 * wvx, vxy2 = -s1 * wux + c1 * wvx;
     const float b = fmaxf(ndp[e], thshld);
-    const float hh = a * b / (a + b);
+    const float hh = a * b * frcp(a + b);
     const float w = dff * GF(sideln, ipn * NPP + e) * 2.f * hh;
     const float ufx = (uxy2 - uxy1) * w, vfx = (vxy2 - vxy1) * w;
     wu = wu + (c0 * ufx - s0 * vfx) * factor;
