@@ -864,7 +864,8 @@ __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl
     if (its > 0) e = SLAK * e + (1. - SLAK) * exold[k];
     if (k >= 1 && k <= nvlp1 - 2) e = dmax(.0001 * exold[k] + .9999 * exold[k + 1], dmin(.0001 * exold[k] + .9999 * exold[k - 1], e));
     exnew[k] = e;
-    const double pn = P1000 * fim_pow(e / CP, CP / RD);
+    // FP32 state: the plain-double pow (a few ulp) instead of the correctly rounded one
+    const double pn = P1000 * (sizeof(V) < sizeof(double) ? fim_pow_fast(e / CP, CP / RD) : fim_pow(e / CP, CP / RD));
     prnew[k] = pn;
     pk1new[k] = e * pn / CP;
   }

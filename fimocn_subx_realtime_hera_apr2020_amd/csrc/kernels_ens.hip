@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity45(EnsGrid g, EnsState s)
   if (!live) return;
   const double p = EL(s.pr, col, nvl) + sp;
   EL(s.pr, col, k) = p;
-  EL(s.ex, col, k) = CP * fim_pow(p / P1000, RD / CP);
+  EL(s.ex, col, k) = CP * fim_pow_fast(p / P1000, RD / CP);
   float dpdx = 0.f, dpdy = 0.f;
   for (int e = 0; e < np; e++) {
     const float l = EE(s.lp_edg, col, e, k);

@@ -948,3 +948,4 @@ void fo_globsum(const fo_grid *g, const fo_state *s, double *mass, double *trtot
 }
 
 double fo_pow(double x, double y) { return fim_pow(x, y); }
+double fo_pow_fast(double x, double y) { return fim_pow_fast(x, y); }

@@ -39,3 +39,4 @@ void fo_init(const fo_grid *g, fo_state *s, int init_hybgen);
 void fo_step(const fo_grid *g, fo_state *s, int its, int mode);
 void fo_globsum(const fo_grid *g, const fo_state *s, double *mass, double *trtot);
 double fo_pow(double x, double y);
+double fo_pow_fast(double x, double y);   /* the mixed-precision form's pow */

@@ -67,6 +67,8 @@ def _lib():
     lib.fo_cell_params.argtypes = [ct.c_int, _PD, _PD, _PD, _PD, _PD]
     lib.fo_pow.argtypes = [ct.c_double, ct.c_double]
     lib.fo_pow.restype = ct.c_double
+    lib.fo_pow_fast.argtypes = [ct.c_double, ct.c_double]
+    lib.fo_pow_fast.restype = ct.c_double
     return lib
 
 

@@ -161,6 +161,26 @@ def test_fim_pow_closed_forms(oracle_lib):
     assert math.isnan(lib.fo_pow(-1.0, 0.3))
 
 
+def test_fim_pow_fast_error_bound(oracle_lib):
+    """The plain-double pow of the mixed-precision form: within one ulp of
+    the C library's pow on the Exner-function arguments and over a wider
+    range, and of 2**n."""
+    lib = oracle_lib.lib()
+    rng = np.random.default_rng(11)
+    kap = 287.0586 / 1004.6855
+    cases = [(rng.uniform(5e-4, 1.3, 20000), np.full(20000, kap)),
+             (rng.uniform(0.1, 1.05, 20000), np.full(20000, 1.0 / kap)),
+             (np.exp(rng.uniform(-7, 7, 20000)), rng.uniform(-3, 3, 20000))]
+    for xs, ys in cases:
+        for x, y in zip(xs, ys):
+            a = math.pow(x, y)
+            err = abs(lib.fo_pow_fast(float(x), float(y)) - a) / a
+            assert err <= 2.3e-16, (x, y, err)
+    for n in range(-20, 21):
+        assert abs(lib.fo_pow_fast(2.0, float(n)) - 2.0 ** n) <= 2.3e-16 * 2.0 ** n
+    assert lib.fo_pow_fast(1.0, 0.3) == 1.0
+
+
 # ---- properties the transport must have -----------------------------------
 
 def test_transport_conserves_mass_and_keeps_constant_tracer(oracle_lib):
