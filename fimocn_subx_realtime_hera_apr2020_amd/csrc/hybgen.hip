@@ -199,15 +199,24 @@ __global__ void __launch_bounds__(256) k_dffusn_lev(HybCols<V> s) {
   if (col >= s.ncol) return;
   const int m = col / nip, ipn = col - m * nip;
   const int k1 = KBL + 2, k2 = nvl;
+  // every neighbour load is issued before the sum (a pentagon's sixth slot
+  // repeats its fifth and is not added)
+  const int np = ldc(s.nprox, ipn);
+  int nb[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) nb[e] = m * nip + ldc(s.prox, ipn * NPP + e);
   for (int kk = threadIdx.x; kk < nvlp1; kk += blockDim.x) {
     const double f = L2(s.sdot, col, kk);
     double out = f;
     const int k = kk + 1;
     if (k >= k1 && k <= k2) {
+      double v[NPP];
+#pragma unroll
+      for (int e = 0; e < NPP; e++) v[e] = L2(s.sdot, nb[e], kk);
       double flx = 0.;
-      const int np = ldc(s.nprox, ipn);
-      for (int edg = 0; edg < np; edg++)
-        flx = flx + (f - L2(s.sdot, m * nip + ldc(s.prox, ipn * NPP + edg), kk)) * ldc(s.sideln, ipn * NPP + edg);
+#pragma unroll
+      for (int e = 0; e < NPP; e++)
+        if (e < np) flx = flx + (f - v[e]) * ldc(s.sideln, ipn * NPP + e);
       double factor = -s.smoo[kk] * ldc(s.rarea, ipn);
       out = f + flx * factor;
     }
