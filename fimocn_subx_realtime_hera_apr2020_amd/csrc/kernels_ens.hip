@@ -95,6 +95,16 @@ __device__ __forceinline__ float2 e_edge_wind(const EnsGrid &g, int c, int e, fl
   return make_float2(ue * sc, ve * sc);
 }
 
+// Theta on the edge between cell i and x (edgvar1.F90:425-511, eqwgt): the
+// trapezoid (2(i + x) + t + l)/6 with the two outer stencil cells t, l (slots
+// e-1, e+1 of i) added in column order, so both cells of the edge form the
+// same bits.
+__device__ __forceinline__ float e_theta_edge(float thi, float thx, float tht, float thl, int nbt, int nbl) {
+  float ta = tht, tb = thl;
+  if (nbt > nbl) { const float t = ta; ta = tb; tb = t; }
+  return (2.f * (thi + thx) + ta + tb) * (1.f / 6.f);
+}
+
 // ---------------------------------------------------------------------------
 // edgvar1 (edgvar1.F90:162-421, 425-511): the edge pressure and theta, and
 // of the edge wind only its normal component sidevec_e x (u,v), the one
@@ -118,7 +128,6 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
   }
   const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k), thi = ETR(s.tr, 0, col, k);
   const double pi0 = EL(s.pr, col, k), pi1 = EL(s.pr, col, k + 1);
-  const float d6 = 1.f / 6.f;
 #pragma unroll
   for (int e = 0; e < NP; e++) {
     const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
@@ -126,11 +135,7 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
     EE(s.vn_edg, col, e, k) = FSVE(ipn, e, 1) * w.x - FSVE(ipn, e, 0) * w.y;
     EE(s.lp_edg, col, e, k) =
         (float)((pi0 + pi1 + np0[e] + np1[e] + .5 * (np0[et] + np1[et] + np0[el] + np1[el])) * (1. / 6.));
-    // theta on the edge, the same bits from both cells of the edge: the two
-    // outer stencil cells are added in column order
-    float ta = nth[et], tb = nth[el];
-    if (nb[et] > nb[el]) { const float t = ta; ta = tb; tb = t; }
-    ETE(s.trc_edg, 0, col, e, k) = (2.f * (thi + nth[e]) + ta + tb) * d6;
+    ETE(s.trc_edg, 0, col, e, k) = e_theta_edge(thi, nth[e], nth[et], nth[el], nb[et], nb[el]);
   }
   if (NP < NPP) ETE(s.trc_edg, 0, col, NPP - 1, k) = 0.f;
 }
@@ -208,7 +213,11 @@ __global__ void __launch_bounds__(64 * COLS) k_e_edgvar3(EnsGrid g, EnsState s) 
 // ---------------------------------------------------------------------------
 // cnuity1 + cnuity2 (cnuity_compute.F90:17-150).  The high-order flux of an
 // edge is formed by the cell with the lower index and negated by the other;
-// the low-order (donor-cell) flux is antisymmetric as written.
+// the low-order (donor-cell) flux is antisymmetric as written.  Both cells
+// store their (negated) copies of the fluxes: storing them on the owner only
+// and reading the owner's slot from the other cell halves the writes but
+// measured slower overall (the scattered single-use reads stream worse than
+// each cell's own slots).
 template <int COLS, bool SWZ>
 __global__ void __launch_bounds__(64 * COLS) k_e_cnuity12(EnsGrid g, EnsState s) {
   const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
@@ -330,51 +339,60 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity45(EnsGrid g, EnsState s)
 // trcadv1-3 (trcadv_compute.F90:19-315), every tracer of a column in one
 // thread, so the mass fluxes and the thickness are read once.  The final mass
 // flux is antisymmetric, so the neighbour's flux through the shared edge is
-// the negated own one; tracer edge values are the same bits on both sides
-// (tracer 1 from the edge-symmetric stencil, the others as the average of
-// both cells' reconstructions).  trcadv1 leaves trcadv2's sums of the
+// the negated own one.  Tracer edge values are the same bits on both sides
+// of an edge: tracer 1 from the edge-symmetric stencil (e_theta_edge, formed
+// here from the tracer values this kernel loads anyway), the others as the
+// average of both cells' reconstructions (stored by edgvar3; forming the
+// neighbour's here from its ring instead measured 7 ms slower at 16 x G7 x
+// 64 than the stored values).  trcadv1 leaves trcadv2's sums of the
 // antidiffusive out- and inflows in tr_rp/tr_rm, and trcadv2 forms the local
 // bounds of trcadv1 again from the tracer values instead of reading them.
-template <int COLS, bool SWZ>
-__global__ void __launch_bounds__(64 * COLS) k_e_trcadv1(EnsGrid g, EnsState s) {
-  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol, ntra = g.d.ntra;
-  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
-  if (ipn >= nip || k >= nvl) return;
-  const StepCtl c = load_ctl(s.ctl);
+
+template <int NP>
+__device__ __forceinline__ void e_trcadv1_col(const EnsGrid &g, const EnsState &s, const StepCtl &c, int ipn,
+                                              int base, int k) {
+  const int nvl = g.d.nvl, ncol = s.ncol, ntra = g.d.ntra;
   const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
-  const int base = blockIdx.y * nip, col = base + ipn;
+  const int col = base + ipn;
   const float thshld = 1.e-11f;
-  const int np = GNP(ipn);
-  float m[NPP];
-  int xs[NPP], exs[NPP];
+  float m[NP];
+  int xs[NP];
 #pragma unroll
-  for (int e = 0; e < NPP; e++) {
+  for (int e = 0; e < NP; e++) {
     xs[e] = base + GPROX(ipn, e);
-    exs[e] = GPROXS(ipn, e);
-    m[e] = e < np ? EE(s.massfx, col, e, k) : 0.f;
+    m[e] = EE(s.massfx, col, e, k);
   }
   const float rdp = frcp(fmaxf(thshld, (float)EC(s.dp, col, k))), rar = GF(rarea, ipn);
   const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
   for (int t = 0; t < ntra; t++) {
-    const float ti = ETR(s.tr, t, col, k);
+    const float *trf = s.tr + (size_t)t * ncol * nvl;
+    const float ti = trf[(size_t)col * nvl + k];
+    float v[NP];
+#pragma unroll
+    for (int e = 0; e < NP; e++) v[e] = trf[(size_t)xs[e] * nvl + k];
     float acc = 0.f, sp = 0.f, sm = 0.f, mx = ti, mn = ti;
 #pragma unroll
-    for (int e = 0; e < NPP; e++) {
-      const float tx = ETR(s.tr, t, xs[e], k);
-      mx = fmaxf(mx, tx);
-      mn = fminf(mn, tx);
-      if (e < np) {
-        const float te = t == 0 ? ETE(s.trc_edg, 0, col, e, k)
-                                : .5f * (ETE(s.trc_edg, t, col, e, k) + ETE(s.trc_edg, t, xs[e], exs[e], k));
-        const float fl = .5f * ((m[e] + fabsf(m[e])) * ti - (fabsf(m[e]) - m[e]) * tx);
-        const float a = m[e] * te - fl;
-        ETE(s.antiflx, t, col, e, k) = a;
-        sp = sp - fminf(0.f, a);
-        sm = sm + fmaxf(0.f, a);
-        acc = acc + fl;
-      }
+    for (int e = 0; e < NP; e++) {
+      mx = fmaxf(mx, v[e]);
+      mn = fminf(mn, v[e]);
     }
-    if (np < NPP) ETE(s.antiflx, t, col, NPP - 1, k) = 0.f;
+#pragma unroll
+    for (int e = 0; e < NP; e++) {
+      const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
+      float te;
+      if (t == 0) {
+        te = e_theta_edge(ti, v[e], v[et], v[el], xs[et], xs[el]);
+      } else {
+        te = .5f * (ETE(s.trc_edg, t, col, e, k) + ETE(s.trc_edg, t, xs[e], GPROXS(ipn, e), k));
+      }
+      const float fl = .5f * ((m[e] + fabsf(m[e])) * ti - (fabsf(m[e]) - m[e]) * v[e]);
+      const float a = m[e] * te - fl;
+      ETE(s.antiflx, t, col, e, k) = a;
+      sp = sp - fminf(0.f, a);
+      sm = sm + fmaxf(0.f, a);
+      acc = acc + fl;
+    }
+    if (NP < NPP) ETE(s.antiflx, t, col, NPP - 1, k) = 0.f;
     const float tlo = -acc * rar;
     ETAB(s.trl_tdcy, t, nf, col, k) = tlo;
     const float lo = ETR(s.trdp, t, col, k) + a1 * tlo + a2 * ETAB(s.trl_tdcy, t, of, col, k) +
@@ -385,6 +403,17 @@ __global__ void __launch_bounds__(64 * COLS) k_e_trcadv1(EnsGrid g, EnsState s) 
     ETR(s.tr_rp, t, col, k) = sp;
     ETR(s.tr_rm, t, col, k) = sm;
   }
+}
+
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_trcadv1(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip || k >= nvl) return;
+  const StepCtl c = load_ctl(s.ctl);
+  const int base = blockIdx.y * nip;
+  if (GNP(ipn) == 6) e_trcadv1_col<6>(g, s, c, ipn, base, k);
+  else e_trcadv1_col<5>(g, s, c, ipn, base, k);
 }
 
 template <int COLS, bool SWZ>
