@@ -452,6 +452,25 @@ __device__ __forceinline__ double w_max_all(double x) {
   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)b >> 32), 63);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
+__device__ __forceinline__ float w_up1(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), DPP_WAVE_SHR1, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float w_dn1(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), DPP_WAVE_SHL1, 0xf, 0xf, true));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_fmax(float v) {
+  return fmaxf(v, __int_as_float(dpp_i<CTRL, ROWS>(__float_as_int(v), __float_as_int(v))));
+}
+__device__ __forceinline__ float w_max_all(float x) {
+  x = dpp_fmax<DPP_XOR1, 0xf>(x);
+  x = dpp_fmax<DPP_XOR2, 0xf>(x);
+  x = dpp_fmax<DPP_HALF_MIRROR, 0xf>(x);
+  x = dpp_fmax<DPP_ROW_MIRROR, 0xf>(x);
+  x = dpp_fmax<DPP_BCAST15, 0xa>(x);
+  x = dpp_fmax<DPP_BCAST31, 0xc>(x);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
 __device__ __forceinline__ int w_min_int(int x) {
   for (int d = 1; d < 64; d <<= 1) x = min(x, __shfl_xor(x, d, 64));
   return x;
@@ -656,6 +675,92 @@ __device__ double pcwise_w(int kk, const PcwGeom &gm, double a, int lane) {
     const double amount = w * f - (vhi - vlo);
     y = FAST ? (amount + w * small) * gm.rfn : (amount + w * small) / (gm.fn + small);
   }
+  return y;
+}
+
+// pcwise in FP32 for a field held in FP32 (the mixed-precision form's winds
+// and tracers 2..ntra): the geometry is formed in FP64 and rounded once, and
+// the sweeps, slopes and fluxes are those of pcwise_w<true> in FP32.  A
+// lane's sweep value is final once the carried-in term y*1e-22 is below a
+// quarter ulp of a*f (|y| <= the column maximum), as in absorbs() with
+// FP32's precision.
+struct PcwGeomF {
+  float f, fl, fn, dz, rf, rfn, dx;
+};
+__device__ __forceinline__ PcwGeomF pcwise_geom_f(const PcwGeom &g) {
+  return PcwGeomF{(float)g.f, (float)g.fl, (float)g.fn, (float)g.dz, (float)g.rf, (float)g.rfn, (float)g.dx};
+}
+__device__ __forceinline__ bool absorbs_f(float t, float bound) {
+  const float at = fabsf(t);
+  return at > 1.e-30f && bound * (1.f + 1.e-5f) * (1.e-22f * 67108864.f) < at;   // 2^26
+}
+__device__ float pcwise_wf(int kk, const PcwGeomF &gm, float a, int lane) {
+  const float small = 1.e-22f;
+  const bool live = lane < kk;
+  const float f = gm.f;
+  const float bnd = w_max_all(live ? fabsf(a) : 0.f);
+  float v;
+  {
+    bool fin;
+    if (lane >= kk - 1) {
+      v = a;
+      fin = true;
+    } else {
+      const float t = a * f;
+      fin = absorbs_f(t, bnd);
+      v = fin ? t * gm.rf : a;
+    }
+    while (wany(!fin)) {
+      const float vu = w_dn1(v);
+      const int fu = w_dn1((int)fin);
+      if (!fin && fu) {
+        v = (a * f + vu * small) * gm.rf;
+        fin = true;
+      }
+    }
+  }
+  float w;
+  {
+    const float a2 = v;
+    bool fin;
+    if (lane == 0 || !live) {
+      w = a2;
+      fin = true;
+    } else {
+      const float t = a2 * f;
+      fin = absorbs_f(t, bnd);
+      w = fin ? t * gm.rf : a2;
+    }
+    while (wany(!fin)) {
+      const float wd = w_up1(w);
+      const int fd = w_up1((int)fin);
+      if (!fin && fd) {
+        w = (a2 * f + wd * small) * gm.rf;
+        fin = true;
+      }
+    }
+  }
+  const float fm = w_up1(w), fp = w_dn1(w);
+  float b = 0.f;
+  if (live && lane != 0 && lane != kk - 1) {
+    if (!(w <= fminf(fm, fp) || w >= fmaxf(fm, fp))) {
+      if ((fp - fm) * (fm + fp - 2.f * w) > 0.f) b = w - fm;
+      else b = fp - w;
+    }
+  }
+  float vf = 0.f;
+  {
+    const float wl = w_up1(w), bl = w_up1(b);
+    if (live && lane >= 1) {
+      const float dz = gm.dz, dx = gm.dx;
+      if (dz > 0.f && gm.f > 0.f) vf = -dz * (w + b * .5f * (dx - 1.f));
+      else if (dz < 0.f && gm.fl > 0.f) vf = -dz * (wl + bl * .5f * (1.f - dx));
+    }
+  }
+  float vhi = w_dn1(vf);
+  if (lane >= kk - 1) vhi = 0.f;
+  float y = 0.f;
+  if (live) y = (w * f - (vhi - vf) + w * small) * gm.rfn;
   return y;
 }
 
@@ -893,13 +998,26 @@ __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl
     const bool live = lane < nvl;
     constexpr bool FAST = sizeof(V) < sizeof(double);
     const PcwGeom gp = pcwise_geom<FAST>(nvl, prcol, prnew, lane);
-    double y = pcwise_w<FAST>(nvl, gp, live ? (double)C2(s.us_in, col, lane) : 0., lane);
-    if (live) C2(s.us, col, lane) = (V)y;
-    y = pcwise_w<FAST>(nvl, gp, live ? (double)C2(s.vs_in, col, lane) : 0., lane);
-    if (live) C2(s.vs, col, lane) = (V)y;
-    for (int n = 1; n < ntra; n++) {
-      y = pcwise_w<FAST>(nvl, gp, live ? (double)TR(s.tr, n, col, lane) : 0., lane);
-      if (live) TR(s.tr, n, col, lane) = (V)y;
+    double y;
+    if (FAST) {   // FP32 state: FP32 remap
+      const PcwGeomF gf = pcwise_geom_f(gp);
+      float yf = pcwise_wf(nvl, gf, live ? (float)C2(s.us_in, col, lane) : 0.f, lane);
+      if (live) C2(s.us, col, lane) = (V)yf;
+      yf = pcwise_wf(nvl, gf, live ? (float)C2(s.vs_in, col, lane) : 0.f, lane);
+      if (live) C2(s.vs, col, lane) = (V)yf;
+      for (int n = 1; n < ntra; n++) {
+        yf = pcwise_wf(nvl, gf, live ? (float)TR(s.tr, n, col, lane) : 0.f, lane);
+        if (live) TR(s.tr, n, col, lane) = (V)yf;
+      }
+    } else {
+      y = pcwise_w<FAST>(nvl, gp, live ? (double)C2(s.us_in, col, lane) : 0., lane);
+      if (live) C2(s.us, col, lane) = (V)y;
+      y = pcwise_w<FAST>(nvl, gp, live ? (double)C2(s.vs_in, col, lane) : 0., lane);
+      if (live) C2(s.vs, col, lane) = (V)y;
+      for (int n = 1; n < ntra; n++) {
+        y = pcwise_w<FAST>(nvl, gp, live ? (double)TR(s.tr, n, col, lane) : 0., lane);
+        if (live) TR(s.tr, n, col, lane) = (V)y;
+      }
     }
     // theta is remapped in p^(kappa+1)
     y = pcwise_w<FAST>(nvl, pcwise_geom<FAST>(nvl, pk1col, pk1new, lane), live ? (double)TR(s.tr, 0, col, lane) : 0.,
