@@ -75,7 +75,7 @@ def ens_kernel_bytes(T):
         "e_cnuity12": 104,                   # normal wind and dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo, sums
         "e_cnuity3": 44,                     # in/outflow sums, delp_lo, dp, dp_tdcy(of,vof) -> r+, r-
         "e_cnuity45": 180,                   # antifx r+- massfx dp_tdcy dp pr lp_edg u v -> massfx dp_tdcy dp pr ex omega
-        "e_trcadv1": 32 + 80 * T,            # massfx dp; per tracer trc_edg tr trdp trl -> antiflx trl lo sums
+        "e_trcadv1": 8 + 80 * T,             # massfx dp; per tracer trc_edg (not theta's) tr trdp trl -> antiflx trl lo sums
         "e_trcadv2": 8 + 40 * T,
         "e_trcadv3": 8 + 68 * T,
         "e_momtum": 100,                     # u v dp ex mp, theta edges, tendencies -> relvor tendency u2 v2
