@@ -251,6 +251,10 @@ static void enqueue_step(fimhip_ens *e, int mode, bool side) {
     ens_launch(EK_EDGVAR3, e->tile[EK_EDGVAR3], g, s, st);
   }
   for (int k = EK_CNUITY12; k <= EK_CNUITY45; k++) ens_launch(k, e->tile[k], g, s, st);
+  for (int k = EK_TRCADV1; k <= EK_TRCADV3; k++) ens_launch(k, e->tile[k], g, s, st);
+  // momtum needs nothing of the tracer transport; it runs beside hybgen's
+  // column kernels up to the remap (latency-bound, so the two share the
+  // chip), rather than beside the bandwidth-bound tracer kernels
   hipEvent_t momtum_done = nullptr;
   if (mode == 0 && side) {
     (void)hipEventRecord(e->ev[2], st);
@@ -259,7 +263,6 @@ static void enqueue_step(fimhip_ens *e, int mode, bool side) {
     (void)hipEventRecord(e->ev[3], sd);
     momtum_done = e->ev[3];
   }
-  for (int k = EK_TRCADV1; k <= EK_TRCADV3; k++) ens_launch(k, e->tile[k], g, s, st);
   if (mode == 0) {
     if (!side) ens_launch(EK_MOMTUM, e->tile[EK_MOMTUM], g, s, st);
     launch_hybgen_cols<float>(hyb_cols(e), s.ctl, 0, st, momtum_done);
