@@ -218,25 +218,34 @@ __global__ void __launch_bounds__(64 * COLS) k_e_edgvar3(EnsGrid g, EnsState s) 
 // and reading the owner's slot from the other cell halves the writes but
 // measured slower overall (the scattered single-use reads stream worse than
 // each cell's own slots).
-template <int COLS, bool SWZ>
-__global__ void __launch_bounds__(64 * COLS) k_e_cnuity12(EnsGrid g, EnsState s) {
-  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
-  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
-  if (ipn >= nip || k >= nvl) return;
-  const StepCtl c = load_ctl(s.ctl);
+template <int NP>
+__device__ __forceinline__ void e_cnuity12_col(const EnsGrid &g, const EnsState &s, const StepCtl &c, int ipn,
+                                               int base, int k) {
+  const int nvl = g.d.nvl, ncol = s.ncol;
   const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
-  const int base = blockIdx.y * nip, col = base + ipn;
-  const int np = GNP(ipn);
+  const int col = base + ipn;
   const double dpi = EC(s.dp, col, k);
   const float dpf = (float)dpi;
+  const double lo_of = (double)EAB(s.dpl_tdcy, of, col, k), lo_vof = (double)EAB(s.dpl_tdcy, vof, col, k);
+  int x[NP];
+  float vn[NP], vx[NP], de[NP], dx[NP], dpx[NP];
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    x[e] = GPROX(ipn, e);
+    const int xc = base + x[e], ex = GPROXS(ipn, e);
+    vn[e] = EE(s.vn_edg, col, e, k);
+    vx[e] = EE(s.vn_edg, xc, ex, k);
+    de[e] = EE(s.dp_edg, col, e, k);
+    dx[e] = EE(s.dp_edg, xc, ex, k);
+    dpx[e] = (float)EC(s.dp, xc, k);
+  }
   double acc = 0.;
   float pp = 0.f, pm = 0.f;     // cnuity3's sums of the antidiffusive in- and outflows
-  for (int e = 0; e < np; e++) {
-    const int x = GPROX(ipn, e), ex = GPROXS(ipn, e), xc = base + x;
-    const float vn = EE(s.vn_edg, col, e, k), vx = EE(s.vn_edg, xc, ex, k);
-    const float dpe = .5f * (EE(s.dp_edg, col, e, k) + EE(s.dp_edg, xc, ex, k));
-    const float hi = ipn < x ? vn * dpe : -(vx * dpe);
-    const float m = .5f * ((vn + fabsf(vn)) * dpf - (vx + fabsf(vx)) * (float)EC(s.dp, xc, k));
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    const float dpe = .5f * (de[e] + dx[e]);
+    const float hi = ipn < x[e] ? vn[e] * dpe : -(vx[e] * dpe);
+    const float m = .5f * ((vn[e] + fabsf(vn[e])) * dpf - (vx[e] + fabsf(vx[e])) * dpx[e]);
     const float a = hi - m;
     EE(s.massfx, col, e, k) = m;
     EE(s.antifx, col, e, k) = a;
@@ -245,14 +254,24 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity12(EnsGrid g, EnsState s)
   }
   EC(s.r_plus, col, k) = pp;
   EC(s.r_mnus, col, k) = pm;
-  if (np < NPP) {
+  if (NP < NPP) {
     EE(s.massfx, col, NPP - 1, k) = 0.f;
     EE(s.antifx, col, NPP - 1, k) = 0.f;
   }
   const double dplo = -acc * GD(rarea, ipn);
   EAB(s.dpl_tdcy, nf, col, k) = (float)dplo;
-  EC(s.delp_lo, col, k) = (float)(dpi + c.ab1 * dplo + c.ab2 * (double)EAB(s.dpl_tdcy, of, col, k) +
-                                  c.ab3 * (double)EAB(s.dpl_tdcy, vof, col, k));
+  EC(s.delp_lo, col, k) = (float)(dpi + c.ab1 * dplo + c.ab2 * lo_of + c.ab3 * lo_vof);
+}
+
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_cnuity12(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip || k >= nvl) return;
+  const StepCtl c = load_ctl(s.ctl);
+  const int base = blockIdx.y * nip;
+  if (GNP(ipn) == 6) e_cnuity12_col<6>(g, s, c, ipn, base, k);
+  else e_cnuity12_col<5>(g, s, c, ipn, base, k);
 }
 
 // cnuity3 (cnuity_compute.F90:152-259); the in- and outflow sums were left
@@ -290,49 +309,71 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity3(EnsGrid g, EnsState s) 
 // formed in FP64 from the final (antisymmetric) FP32 fluxes, the thickness is
 // advanced in FP64, and the top-down pressure sum and the column tendency for
 // omega are wavefront suffix sums.
-template <int COLS, bool SWZ>
-__global__ void __launch_bounds__(64 * COLS) k_e_cnuity45(EnsGrid g, EnsState s) {
-  const int nip = g.d.nip, nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
-  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
-  if (ipn >= nip) return;
-  const bool live = k < nvl;
-  const StepCtl c = load_ctl(s.ctl);
+template <int NP>
+__device__ __forceinline__ void e_cnuity45_col(const EnsGrid &g, const EnsState &s, const StepCtl &c, int ipn,
+                                               int base, int k) {
+  const int nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
   const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
-  const int base = blockIdx.y * nip, col = base + ipn;
-  const int np = GNP(ipn);
+  const int col = base + ipn;
+  const bool live = k < nvl;
   double dpn = 0., lt = 0.;
   if (live) {
-    double at = 0.;
     const float rp = EC(s.r_plus, col, k), rm = EC(s.r_mnus, col, k);
-    for (int e = 0; e < np; e++) {
-      const int xc = base + GPROX(ipn, e);
-      float a = EE(s.antifx, col, e, k);
-      const float clip = a >= 0.f ? fminf(rm, EC(s.r_plus, xc, k)) : fminf(rp, EC(s.r_mnus, xc, k));
-      const float mf = EE(s.massfx, col, e, k) + a * clip;
+    const double t_of = EAB(s.dp_tdcy, of, col, k), t_vof = EAB(s.dp_tdcy, vof, col, k), dp0 = EC(s.dp, col, k);
+    float a[NP], m[NP], r[NP];
+#pragma unroll
+    for (int e = 0; e < NP; e++) {
+      a[e] = EE(s.antifx, col, e, k);
+      m[e] = EE(s.massfx, col, e, k);
+    }
+#pragma unroll
+    for (int e = 0; e < NP; e++) {
+      const float *src = a[e] >= 0.f ? s.r_plus : s.r_mnus;
+      r[e] = src[(size_t)(base + GPROX(ipn, e)) * nvl + k];
+    }
+    double at = 0.;
+#pragma unroll
+    for (int e = 0; e < NP; e++) {
+      const float mf = m[e] + a[e] * fminf(a[e] >= 0.f ? rm : rp, r[e]);
       EE(s.massfx, col, e, k) = mf;
       at = at + (double)mf;
     }
     const double dtn = -at * GD(rarea, ipn);
     EAB(s.dp_tdcy, nf, col, k) = dtn;
-    const double inc = c.ab1 * dtn + c.ab2 * EAB(s.dp_tdcy, of, col, k) + c.ab3 * EAB(s.dp_tdcy, vof, col, k);
-    dpn = EC(s.dp, col, k) + inc;
+    const double inc = c.ab1 * dtn + c.ab2 * t_of + c.ab3 * t_vof;
+    dpn = dp0 + inc;
     EC(s.dp, col, k) = dpn;
     lt = inc / g.d.dt;
   }
   const double sp = wave_suffix_sum(dpn, k);
   const double sl = wave_suffix_sum(lt, k);
   if (!live) return;
+  float l[NP];
+#pragma unroll
+  for (int e = 0; e < NP; e++) l[e] = EE(s.lp_edg, col, e, k);
+  const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k);
   const double p = EL(s.pr, col, nvl) + sp;
   EL(s.pr, col, k) = p;
   EL(s.ex, col, k) = CP * fim_pow_fast(p / P1000, RD / CP);
   float dpdx = 0.f, dpdy = 0.f;
-  for (int e = 0; e < np; e++) {
-    const float l = EE(s.lp_edg, col, e, k);
-    dpdx = dpdx + l * FSVC(ipn, e, 1);
-    dpdy = dpdy - l * FSVC(ipn, e, 0);
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    dpdx = dpdx + l[e] * FSVC(ipn, e, 1);
+    dpdy = dpdy - l[e] * FSVC(ipn, e, 0);
   }
   const float om = (float)(sl - .5 * lt);
-  EC(s.ws, col, k) = om + (EC(s.us, col, k) * dpdx + EC(s.vs, col, k) * dpdy) * GF(rarea, ipn);
+  EC(s.ws, col, k) = om + (ui * dpdx + vi * dpdy) * GF(rarea, ipn);
+}
+
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_cnuity45(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip) return;
+  const StepCtl c = load_ctl(s.ctl);
+  const int base = blockIdx.y * nip;
+  if (GNP(ipn) == 6) e_cnuity45_col<6>(g, s, c, ipn, base, k);
+  else e_cnuity45_col<5>(g, s, c, ipn, base, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -460,41 +501,56 @@ __global__ void __launch_bounds__(64 * COLS) k_e_trcadv2(EnsGrid g, EnsState s) 
   }
 }
 
-template <int COLS, bool SWZ>
-__global__ void __launch_bounds__(64 * COLS) k_e_trcadv3(EnsGrid g, EnsState s) {
-  const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol, ntra = g.d.ntra;
-  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
-  if (ipn >= nip || k >= nvl) return;
-  const StepCtl c = load_ctl(s.ctl);
+// trcadv3: the limiter's neighbour value is gathered through a per-lane
+// address (R+ or R- of the neighbour by the sign of the flux) and every load
+// of a tracer is issued before the first is used.
+template <int NP>
+__device__ __forceinline__ void e_trcadv3_col(const EnsGrid &g, const EnsState &s, const StepCtl &c, int ipn,
+                                              int base, int k) {
+  const int nvl = g.d.nvl, ncol = s.ncol, ntra = g.d.ntra;
   const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
-  const int base = blockIdx.y * nip, col = base + ipn;
+  const int col = base + ipn;
   const float thshld = 1.e-11f;
-  const int np = GNP(ipn);
-  int xs[NPP];
+  int xs[NP];
 #pragma unroll
-  for (int e = 0; e < NPP; e++) xs[e] = base + GPROX(ipn, e);
+  for (int e = 0; e < NP; e++) xs[e] = base + GPROX(ipn, e);
   const float rdp = frcp(fmaxf(thshld, (float)EC(s.dp, col, k))), rar = GF(rarea, ipn);
   const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
   for (int t = 0; t < ntra; t++) {
+    const size_t tb = (size_t)t * ncol;
     const float rp = ETR(s.tr_rp, t, col, k), rm = ETR(s.tr_rm, t, col, k);
+    const float trl = ETAB(s.trl_tdcy, t, nf, col, k), tco = ETAB(s.trc_tdcy, t, of, col, k);
+    const float tcv = ETAB(s.trc_tdcy, t, vof, col, k), tdp0 = ETR(s.trdp, t, col, k);
+    const float tmn = ETR(s.trmin, t, col, k), tmx = ETR(s.trmax, t, col, k);
+    float a[NP], r[NP];
+#pragma unroll
+    for (int e = 0; e < NP; e++) a[e] = ETE(s.antiflx, t, col, e, k);
+#pragma unroll
+    for (int e = 0; e < NP; e++) {
+      const float *src = a[e] >= 0.f ? s.tr_rp : s.tr_rm;
+      r[e] = src[(tb + xs[e]) * nvl + k];
+    }
     float at = 0.f;
 #pragma unroll
-    for (int e = 0; e < NPP; e++) {
-      if (e < np) {
-        float a = ETE(s.antiflx, t, col, e, k);
-        if (a >= 0.f) a = a * fminf(rm, ETR(s.tr_rp, t, xs[e], k));
-        else a = a * fminf(rp, ETR(s.tr_rm, t, xs[e], k));
-        at = at + a;
-      }
-    }
+    for (int e = 0; e < NP; e++) at = at + a[e] * fminf(a[e] >= 0.f ? rm : rp, r[e]);
     at = -at * rar;
-    const float tt = ETAB(s.trl_tdcy, t, nf, col, k) + at;
+    const float tt = trl + at;
+    const float tdp = tdp0 + a1 * tt + a2 * tco + a3 * tcv;
     ETAB(s.trc_tdcy, t, nf, col, k) = tt;
-    const float tdp = ETR(s.trdp, t, col, k) + a1 * tt + a2 * ETAB(s.trc_tdcy, t, of, col, k) +
-                      a3 * ETAB(s.trc_tdcy, t, vof, col, k);
     ETR(s.trdp, t, col, k) = tdp;
-    ETR(s.tr, t, col, k) = fmaxf(ETR(s.trmin, t, col, k), fminf(ETR(s.trmax, t, col, k), tdp * rdp));
+    ETR(s.tr, t, col, k) = fmaxf(tmn, fminf(tmx, tdp * rdp));
   }
+}
+
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_trcadv3(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip || k >= nvl) return;
+  const StepCtl c = load_ctl(s.ctl);
+  const int base = blockIdx.y * nip;
+  if (GNP(ipn) == 6) e_trcadv3_col<6>(g, s, c, ipn, base, k);
+  else e_trcadv3_col<5>(g, s, c, ipn, base, k);
 }
 
 // ---------------------------------------------------------------------------
