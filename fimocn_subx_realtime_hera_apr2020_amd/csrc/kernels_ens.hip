@@ -406,11 +406,30 @@ __device__ __forceinline__ void e_trcadv1_col(const EnsGrid &g, const EnsState &
   const float rdp = frcp(fmaxf(thshld, (float)EC(s.dp, col, k))), rar = GF(rarea, ipn);
   const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
   for (int t = 0; t < ntra; t++) {
+    // every load of this tracer before its first store
     const float *trf = s.tr + (size_t)t * ncol * nvl;
     const float ti = trf[(size_t)col * nvl + k];
-    float v[NP];
+    float v[NP], te[NP];
 #pragma unroll
     for (int e = 0; e < NP; e++) v[e] = trf[(size_t)xs[e] * nvl + k];
+    const float tdp0 = ETR(s.trdp, t, col, k), tlo_of = ETAB(s.trl_tdcy, t, of, col, k);
+    const float tlo_vof = ETAB(s.trl_tdcy, t, vof, col, k);
+    if (t == 0) {
+#pragma unroll
+      for (int e = 0; e < NP; e++) {
+        const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
+        te[e] = e_theta_edge(ti, v[e], v[et], v[el], xs[et], xs[el]);
+      }
+    } else {
+      float eo[NP], en[NP];
+#pragma unroll
+      for (int e = 0; e < NP; e++) {
+        eo[e] = ETE(s.trc_edg, t, col, e, k);
+        en[e] = ETE(s.trc_edg, t, xs[e], GPROXS(ipn, e), k);
+      }
+#pragma unroll
+      for (int e = 0; e < NP; e++) te[e] = .5f * (eo[e] + en[e]);
+    }
     float acc = 0.f, sp = 0.f, sm = 0.f, mx = ti, mn = ti;
 #pragma unroll
     for (int e = 0; e < NP; e++) {
@@ -419,15 +438,8 @@ __device__ __forceinline__ void e_trcadv1_col(const EnsGrid &g, const EnsState &
     }
 #pragma unroll
     for (int e = 0; e < NP; e++) {
-      const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
-      float te;
-      if (t == 0) {
-        te = e_theta_edge(ti, v[e], v[et], v[el], xs[et], xs[el]);
-      } else {
-        te = .5f * (ETE(s.trc_edg, t, col, e, k) + ETE(s.trc_edg, t, xs[e], GPROXS(ipn, e), k));
-      }
       const float fl = .5f * ((m[e] + fabsf(m[e])) * ti - (fabsf(m[e]) - m[e]) * v[e]);
-      const float a = m[e] * te - fl;
+      const float a = m[e] * te[e] - fl;
       ETE(s.antiflx, t, col, e, k) = a;
       sp = sp - fminf(0.f, a);
       sm = sm + fmaxf(0.f, a);
@@ -436,8 +448,7 @@ __device__ __forceinline__ void e_trcadv1_col(const EnsGrid &g, const EnsState &
     if (NP < NPP) ETE(s.antiflx, t, col, NPP - 1, k) = 0.f;
     const float tlo = -acc * rar;
     ETAB(s.trl_tdcy, t, nf, col, k) = tlo;
-    const float lo = ETR(s.trdp, t, col, k) + a1 * tlo + a2 * ETAB(s.trl_tdcy, t, of, col, k) +
-                     a3 * ETAB(s.trl_tdcy, t, vof, col, k);
+    const float lo = tdp0 + a1 * tlo + a2 * tlo_of + a3 * tlo_vof;
     mx = fmaxf(0.f, mx);
     mn = fmaxf(0.f, mn);
     ETR(s.trcr_lo, t, col, k) = fmaxf(mn, fminf(mx, lo * rdp));
@@ -579,6 +590,8 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
     nmp[e] = EC(s.mp, nb[e], k);
   }
   const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k);
+  const float uo = EAB(s.u_tdcy, of, col, k), uvo = EAB(s.u_tdcy, vof, col, k);
+  const float vo = EAB(s.v_tdcy, of, col, k), vvo = EAB(s.v_tdcy, vof, col, k);
   const float wui = ui + GF(grndspd, ipn), wvi = vi;
   const float a = fmaxf((float)EC(s.dp, col, k), thshld);
   const float factor = GF(rarea, ipn) * frcp(a);
@@ -617,13 +630,13 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
   }
   const float pgx = (float)(px * GD(rarea, ipn)), pgy = (float)(py * GD(rarea, ipn));
   rv = rv * GF(rarea, ipn);
-  EC(s.relvor, col, k) = rv;
   const float f = GF(corio, ipn) + rv;
   const float ut = pgx + f * v1, vt = pgy - f * u1;
+  un = u1 + (float)c.ab1 * ut + (float)c.ab2 * uo + (float)c.ab3 * uvo;
+  vn = v1 + (float)c.ab1 * vt + (float)c.ab2 * vo + (float)c.ab3 * vvo;
+  EC(s.relvor, col, k) = rv;
   EAB(s.u_tdcy, nf, col, k) = ut;
   EAB(s.v_tdcy, nf, col, k) = vt;
-  un = u1 + (float)c.ab1 * ut + (float)c.ab2 * EAB(s.u_tdcy, of, col, k) + (float)c.ab3 * EAB(s.u_tdcy, vof, col, k);
-  vn = v1 + (float)c.ab1 * vt + (float)c.ab2 * EAB(s.v_tdcy, of, col, k) + (float)c.ab3 * EAB(s.v_tdcy, vof, col, k);
 }
 
 template <int COLS, bool SWZ>
@@ -668,6 +681,7 @@ __device__ __forceinline__ void e_hystat_col(const EnsGrid &g, const EnsState &s
   const double dp = live ? EC(s.dp, col, k) : 0.;
   const double th = live ? (double)ETR(s.tr, 0, col, k) : 0.;
   const double exk = live ? EL(s.ex, col, k) : 0.;
+  const double exk1 = live ? EL(s.ex, col, k + 1) : 0.;
   double ps = dp;
   for (int d = 1; d < 64; d <<= 1) ps += __shfl_xor(ps, d, 64);
   const double thm = __shfl_up(th, 1, 64);
@@ -680,11 +694,26 @@ __device__ __forceinline__ void e_hystat_col(const EnsGrid &g, const EnsState &s
   }
   if (!live) return;
   EC(s.mp, col, k) = mp;
-  EL(s.ph, col, k + 1) = mp - th * EL(s.ex, col, k + 1);
-  if (ntra > 1) ETR(s.tr, 1, col, k) = fmaxf((float)QVMIN, ETR(s.tr, 1, col, k));
-  if (ntra > 2) ETR(s.tr, 2, col, k) = fmaxf((float)QWMIN, ETR(s.tr, 2, col, k));
+  EL(s.ph, col, k + 1) = mp - th * exk1;
+  // tracers 2 and 3 clamped (qv, qw), then trdp = tr * dp; four tracers'
+  // loads are issued before their stores
   const float dpf = (float)dp;
-  for (int t = 0; t < ntra; t++) ETR(s.trdp, t, col, k) = ETR(s.tr, t, col, k) * dpf;
+  for (int t0 = 0; t0 < ntra; t0 += 4) {
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = ETR(s.tr, min(t0 + j, ntra - 1), col, k);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int t = t0 + j;
+      if (t >= ntra) break;
+      float x = v[j];
+      if (t == 1 || t == 2) {
+        x = fmaxf(t == 1 ? (float)QVMIN : (float)QWMIN, x);
+        ETR(s.tr, t, col, k) = x;
+      }
+      ETR(s.trdp, t, col, k) = x * dpf;
+    }
+  }
   (void)nip;
 }
 
