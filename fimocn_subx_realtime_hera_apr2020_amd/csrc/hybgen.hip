@@ -998,16 +998,29 @@ __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl
     const bool live = lane < nvl;
     constexpr bool FAST = sizeof(V) < sizeof(double);
     const PcwGeom gp = pcwise_geom<FAST>(nvl, prcol, prnew, lane);
+    const double th0 = live ? (double)TR(s.tr, 0, col, lane) : 0.;   // loaded before the stores below
     double y;
     if (FAST) {   // FP32 state: FP32 remap
       const PcwGeomF gf = pcwise_geom_f(gp);
-      float yf = pcwise_wf(nvl, gf, live ? (float)C2(s.us_in, col, lane) : 0.f, lane);
-      if (live) C2(s.us, col, lane) = (V)yf;
-      yf = pcwise_wf(nvl, gf, live ? (float)C2(s.vs_in, col, lane) : 0.f, lane);
-      if (live) C2(s.vs, col, lane) = (V)yf;
-      for (int n = 1; n < ntra; n++) {
-        yf = pcwise_wf(nvl, gf, live ? (float)TR(s.tr, n, col, lane) : 0.f, lane);
-        if (live) TR(s.tr, n, col, lane) = (V)yf;
+      // the inputs of four fields are loaded before any result is stored
+      const int kl = live ? lane : 0;
+      const float u0 = (float)C2(s.us_in, col, kl), v0 = (float)C2(s.vs_in, col, kl);
+      float yf = pcwise_wf(nvl, gf, live ? u0 : 0.f, lane);
+      const float yv = pcwise_wf(nvl, gf, live ? v0 : 0.f, lane);
+      if (live) {
+        C2(s.us, col, lane) = (V)yf;
+        C2(s.vs, col, lane) = (V)yv;
+      }
+      for (int n0 = 1; n0 < ntra; n0 += 4) {
+        float x[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) x[j] = (float)TR(s.tr, min(n0 + j, ntra - 1), col, kl);
+        float y4[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) y4[j] = n0 + j < ntra ? pcwise_wf(nvl, gf, live ? x[j] : 0.f, lane) : 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (live && n0 + j < ntra) TR(s.tr, n0 + j, col, lane) = (V)y4[j];
       }
     } else {
       y = pcwise_w<FAST>(nvl, gp, live ? (double)C2(s.us_in, col, lane) : 0., lane);
@@ -1020,8 +1033,7 @@ __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl
       }
     }
     // theta is remapped in p^(kappa+1)
-    y = pcwise_w<FAST>(nvl, pcwise_geom<FAST>(nvl, pk1col, pk1new, lane), live ? (double)TR(s.tr, 0, col, lane) : 0.,
-                       lane);
+    y = pcwise_w<FAST>(nvl, pcwise_geom<FAST>(nvl, pk1col, pk1new, lane), th0, lane);
     if (live) th1[lane] = y;
   }
   wsync();
