@@ -205,8 +205,12 @@ __global__ void __launch_bounds__(256) k_dffusn_lev(HybCols<V> s) {
   int nb[NPP];
 #pragma unroll
   for (int e = 0; e < NPP; e++) nb[e] = m * nip + ldc(s.prox, ipn * NPP + e);
+  // lane kk < 64 smooths interface kk; the levels outside k1..k2 (the
+  // bottom ones and the top interface, which lane 0 copies as well) are copied
   for (int kk = threadIdx.x; kk < nvlp1; kk += blockDim.x) {
     const double f = L2(s.sdot, col, kk);
+    const bool top = kk == 0 && nvlp1 - 1 >= (int)blockDim.x;
+    const double ftop = top ? L2(s.sdot, col, nvlp1 - 1) : 0.;
     double out = f;
     const int k = kk + 1;
     if (k >= k1 && k <= k2) {
@@ -221,6 +225,8 @@ __global__ void __launch_bounds__(256) k_dffusn_lev(HybCols<V> s) {
       out = f + flx * factor;
     }
     L2(s.exsmo, col, kk) = out;
+    if (top) L2(s.exsmo, col, nvlp1 - 1) = ftop;
+    if (kk + (int)blockDim.x == nvlp1 - 1) break;   // the top interface is done
   }
 }
 
