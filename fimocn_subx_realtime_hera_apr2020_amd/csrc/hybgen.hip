@@ -403,6 +403,16 @@ __device__ __forceinline__ void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 __device__ __forceinline__ bool wany(bool p) { return __ballot(p) != 0ull; }
+// a / b: exact division in the FP64 form; in the mixed-precision form the
+// hardware reciprocal refined by two Newton steps (within an ulp or two)
+template <bool FAST>
+__device__ __forceinline__ double ddiv(double a, double b) {
+  if (!FAST) return a / b;
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(r, fma(-b, r, 1.), r);
+  r = fma(r, fma(-b, r, 1.), r);
+  return a * r;
+}
 __device__ __forceinline__ double w_suffix_min(double x, int lane) {
   for (int d = 1; d < 64; d <<= 1) {
     const double y = __shfl_down(x, d, 64);
@@ -880,7 +890,7 @@ __global__ void __launch_bounds__(64 * HW) k_regrid_w(HybCols<V> s, const StepCt
           }
           break;
         }
-        if (tk > tkm) exnew[k] = pinteg / (tk - tkm);
+        if (tk > tkm) exnew[k] = ddiv<sizeof(V) < sizeof(double)>(pinteg, tk - tkm);
         else copy = true;
       }
       if (wany(copy)) {
@@ -903,16 +913,18 @@ __global__ void __launch_bounds__(64 * HW) k_regrid_w(HybCols<V> s, const StepCt
     for (int k = 2 + lane; k <= nvl - 1; k += 64) {
       double h = 0.;
       if (exwrk[k - 1] < exwrk[0] - .01 && exwrk[k] > exwrk[nvlp1 - 1] + .01) {
-        double cnv = GRVITY / ldc(targt, k - 1);
-        h = .1 * cnv * cnv * .5 * (ldc(targt, k) - ldc(targt, k - 2)) / dmax(.03 * cnv, exwrk[k - 1] - exwrk[k]);
+        constexpr bool F = sizeof(V) < sizeof(double);
+        double cnv = ddiv<F>(GRVITY, ldc(targt, k - 1));
+        h = ddiv<F>(.1 * cnv * cnv * .5 * (ldc(targt, k) - ldc(targt, k - 2)), dmax(.03 * cnv, exwrk[k - 1] - exwrk[k]));
       }
       heatfx[k - 1] = h;
     }
     wsync();
     for (int k = 1 + lane; k <= nvl - 1; k += 64) {
       if (exwrk[k] < exwrk[0] - .01 && exwrk[k] > exwrk[nvlp1 - 1] + .01)
-        exnew[k] = dmax(exwrk[k + 1], dmin(exwrk[k - 1], exwrk[k] + (heatfx[k] - heatfx[k - 1]) /
-                                                                       (ldc(targt, k) - ldc(targt, k - 1))));
+        exnew[k] = dmax(exwrk[k + 1], dmin(exwrk[k - 1], exwrk[k] + ddiv<sizeof(V) < sizeof(double)>(
+                                                                       heatfx[k] - heatfx[k - 1],
+                                                                       ldc(targt, k) - ldc(targt, k - 1))));
     }
     wsync();
     bool ev = false;

@@ -623,9 +623,9 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
     const double mont = (2. * (mpi + nmp[e]) + nmp[et] + nmp[el]) * (1. / 6.);
     const double b = mont + (double)(.5f * (ue * ue + ve * ve));
     const double te = (double)ETE(s.trc_edg, 0, col, e, k);
-    const double sx = FSVC(ipn, e, 1), sy = FSVC(ipn, e, 0);
-    px = px - b * sx + exm * te * sx;
-    py = py + b * sy - exm * te * sy;
+    const double q = exm * te - b;          // -(M + KE) + Exner * theta, times sidevec_c
+    px = px + q * (double)FSVC(ipn, e, 1);
+    py = py - q * (double)FSVC(ipn, e, 0);
     rv = rv + (FSVE(ipn, e, 0) * ue + FSVE(ipn, e, 1) * ve);
   }
   const float pgx = (float)(px * GD(rarea, ipn)), pgy = (float)(py * GD(rarea, ipn));
