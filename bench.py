@@ -70,7 +70,7 @@ def ens_kernel_bytes(T):
     """The same for the mixed-precision kernels (4 B per FP32 value, 8 B per
     FP64 value; inputs shared by the tracers of one launch counted once)."""
     return {
-        "e_edgvar1": 20 + 72,                # us vs tr1 (f32) pr (f64) -> normal wind, lp, trc1 edges (f32)
+        "e_edgvar1": 20 + 52,                # us vs tr1 (f32) pr (f64) -> normal wind and lp edges, step-start theta (f32)
         "e_edgvar3": 32 + 28 * (T - 1),      # dp (f64) -> dp_edg; tracers 2..T -> trc_edg
         "e_cnuity12": 104,                   # normal wind and dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo, sums
         "e_cnuity3": 44,                     # in/outflow sums, delp_lo, dp, dp_tdcy(of,vof) -> r+, r-
@@ -78,7 +78,7 @@ def ens_kernel_bytes(T):
         "e_trcadv1": 8 + 80 * T,             # massfx dp; per tracer trc_edg (not theta's) tr trdp trl -> antiflx trl lo sums
         "e_trcadv2": 8 + 40 * T,
         "e_trcadv3": 8 + 68 * T,
-        "e_momtum": 100,                     # u v dp ex mp, theta edges, tendencies -> relvor tendency u2 v2
+        "e_momtum": 80,                      # u v dp ex mp, step-start theta, tendencies -> relvor tendency u2 v2
         "k_regrid": 20,
         "k_regrid_tail": 16,
         "k_dffusn_lev": 16,

@@ -17,7 +17,9 @@
 // index col = member * nip + cell (ncol = members * nip):
 //   cell (nvl, ncol)   level (nvl+1, ncol)   edge (nvl, 6, ncol)
 //   tracer (nvl, ncol, ntra)   AB (nvl, ncol, 3)   tracer AB (nvl, ncol, 3, ntra)
-//   tracer edge (nvl, 6, ncol, ntra); the mass flux keeps only the current slot.
+//   tracer edge (nvl, 6, ncol, ntra - 1) for tracers 2..ntra (theta's edge
+//   values are formed where they are used); the mass flux keeps only the
+//   current slot.
 #pragma once
 #include "fimhip_internal.h"
 
@@ -39,7 +41,8 @@ struct EnsState {
   double *dp_tdcy;
   float *trc_tdcy, *trl_tdcy;
   float *massfx, *ws, *relvor;
-  float *vn_edg, *dp_edg, *lp_edg, *trc_edg;   // vn_edg: normal edge wind
+  float *vn_edg, *dp_edg, *lp_edg, *trc_edg;   // vn_edg: normal edge wind; trc_edg: tracers 2..ntra
+  float *th0;                                  // theta at the start of the step
   float *antifx, *delp_lo, *r_plus, *r_mnus;
   float *antiflx, *trcr_lo, *trmax, *trmin, *tr_rp, *tr_rm;
   StepCtl *ctl;

@@ -106,23 +106,24 @@ __device__ __forceinline__ float e_theta_edge(float thi, float thx, float tht, f
 }
 
 // ---------------------------------------------------------------------------
-// edgvar1 (edgvar1.F90:162-421, 425-511): the edge pressure and theta, and
-// of the edge wind only its normal component sidevec_e x (u,v), the one
-// cnuity1 uses (cnuity_compute.F90:17-95); momtum forms the edge winds it
-// needs itself (e_edge_wind).
+// edgvar1 (edgvar1.F90:162-421, 425-511): the edge pressure, and of the
+// edge wind only its normal component sidevec_e x (u,v), the one cnuity1
+// uses (cnuity_compute.F90:17-95); momtum forms the edge winds it needs
+// itself (e_edge_wind).  Theta's edge values are formed where they are used
+// (trcadv1, momtum: e_theta_edge) from the cells' theta; edgvar1 keeps the
+// step's starting theta in th0 for momtum.
 template <int NP>
 __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &s, int ipn, int base, int k) {
   const int nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
   const int col = base + ipn;
   int nb[NP];
-  float nu[NP], nv[NP], nth[NP];
+  float nu[NP], nv[NP];
   double np0[NP], np1[NP];
 #pragma unroll
   for (int e = 0; e < NP; e++) {
     nb[e] = base + GPROX(ipn, e);
     nu[e] = EC(s.us, nb[e], k);
     nv[e] = EC(s.vs, nb[e], k);
-    nth[e] = ETR(s.tr, 0, nb[e], k);
     np0[e] = EL(s.pr, nb[e], k);
     np1[e] = EL(s.pr, nb[e], k + 1);
   }
@@ -135,9 +136,10 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
     EE(s.vn_edg, col, e, k) = FSVE(ipn, e, 1) * w.x - FSVE(ipn, e, 0) * w.y;
     EE(s.lp_edg, col, e, k) =
         (float)((pi0 + pi1 + np0[e] + np1[e] + .5 * (np0[et] + np1[et] + np0[el] + np1[el])) * (1. / 6.));
-    ETE(s.trc_edg, 0, col, e, k) = e_theta_edge(thi, nth[e], nth[et], nth[el], nb[et], nb[el]);
   }
-  if (NP < NPP) ETE(s.trc_edg, 0, col, NPP - 1, k) = 0.f;
+  // theta at the start of the step, for momtum's edge values (the tracer
+  // itself is updated by trcadv before momtum runs)
+  EC(s.th0, col, k) = thi;
 }
 
 template <int COLS, bool SWZ>
@@ -195,8 +197,8 @@ __device__ __forceinline__ void e_edgvar3_col(const EnsGrid &g, const EnsState &
       out[i] = b0 + acc;
     }
 #pragma unroll
-    for (int i = 0; i < NP; i++) ETE(s.trc_edg, f, col, i, k) = out[i];
-    if (NP < NPP) ETE(s.trc_edg, f, col, NPP - 1, k) = 0.f;
+    for (int i = 0; i < NP; i++) ETE(s.trc_edg, f - 1, col, i, k) = out[i];
+    if (NP < NPP) ETE(s.trc_edg, f - 1, col, NPP - 1, k) = 0.f;
   }
 }
 
@@ -424,8 +426,8 @@ __device__ __forceinline__ void e_trcadv1_col(const EnsGrid &g, const EnsState &
       float eo[NP], en[NP];
 #pragma unroll
       for (int e = 0; e < NP; e++) {
-        eo[e] = ETE(s.trc_edg, t, col, e, k);
-        en[e] = ETE(s.trc_edg, t, xs[e], GPROXS(ipn, e), k);
+        eo[e] = ETE(s.trc_edg, t - 1, col, e, k);
+        en[e] = ETE(s.trc_edg, t - 1, xs[e], GPROXS(ipn, e), k);
       }
 #pragma unroll
       for (int e = 0; e < NP; e++) te[e] = .5f * (eo[e] + en[e]);
@@ -579,7 +581,7 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
   const int col = base + ipn;
   const float thshld = 1.e-11f;
   int nb[NP];
-  float nu[NP], nv[NP], ndp[NP];
+  float nu[NP], nv[NP], ndp[NP], nth[NP];
   double nmp[NP];
 #pragma unroll
   for (int e = 0; e < NP; e++) {
@@ -588,7 +590,9 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
     nv[e] = EC(s.vs, nb[e], k);
     ndp[e] = (float)EC(s.dp, nb[e], k);
     nmp[e] = EC(s.mp, nb[e], k);
+    nth[e] = EC(s.th0, nb[e], k);
   }
+  const float thi = EC(s.th0, col, k);
   const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k);
   const float uo = EAB(s.u_tdcy, of, col, k), uvo = EAB(s.u_tdcy, vof, col, k);
   const float vo = EAB(s.v_tdcy, of, col, k), vvo = EAB(s.v_tdcy, vof, col, k);
@@ -622,7 +626,7 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
     const float ue = w.x, ve = w.y;
     const double mont = (2. * (mpi + nmp[e]) + nmp[et] + nmp[el]) * (1. / 6.);
     const double b = mont + (double)(.5f * (ue * ue + ve * ve));
-    const double te = (double)ETE(s.trc_edg, 0, col, e, k);
+    const double te = (double)e_theta_edge(thi, nth[e], nth[et], nth[el], nb[et], nb[el]);
     const double q = exm * te - b;          // -(M + KE) + Exner * theta, times sidevec_c
     px = px + q * (double)FSVC(ipn, e, 1);
     py = py - q * (double)FSVC(ipn, e, 0);
