@@ -1203,32 +1203,83 @@ __global__ void __launch_bounds__(64) k_remap_inflate(HybCols<V> s, double extop
   for (int k = 0; k <= NVL; k++) L2(s.exsmo, col, k) = ex[k];
 }
 
-// theta redistribution of remap_1d (hybgen.F90:515-789, final loop over
-// k = nvl..3), from thbuf and the layer depths left in exsmo
+// one exchange of the theta redistribution of remap_1d (hybgen.F90:515-789,
+// final loop): layers p (tha, depth dplo) and p+1 (thb, dpup)
+__device__ __forceinline__ void theta_pair(double &tha, double &thb, double dplo, double dpup, double tlo,
+                                           double tup) {
+  const double devilo = (tha - tlo) * dplo;
+  const double deviup = (thb - tup) * dpup;
+  if (deviup > 0. && devilo < 0.) {
+    tha = tha + dmin(deviup, -devilo) / dplo;
+    thb = thb - dmin(deviup, -devilo) / dpup;
+  } else if (deviup < 0. && devilo > 0.) {
+    tha = tha - dmin(devilo, -deviup) / dplo;
+    thb = thb + dmin(devilo, -deviup) / dpup;
+  }
+}
+
+// theta redistribution of remap_1d (final loop over k = nvl..3, i.e. pairs
+// p = k-2 = nvl-2 .. 1 in that order), one thread per column for 64 columns,
+// from the remapped theta in thbuf and the layer depths left in exsmo.  The
+// inputs are staged through LDS in chunks of 16 levels (each load instruction
+// reads 16 levels of 4 columns), the sweep works on one chunk in registers and
+// carries the upper layer of the pair that straddles two chunks, and the
+// result goes back to column-major tracer 1 through an LDS tile, so every
+// global access is coalesced.
 template <int NVL, typename V>
 __global__ void __launch_bounds__(64) k_remap_theta(HybCols<V> s) {
+  constexpr int CH = 16, NCH = NVL / CH;
+  static_assert(NVL % CH == 0, "whole chunks");
+  __shared__ double sth[CH][65], sdl[CH][65];
+  __shared__ V sout[CH + 1][65];   // levels k0 .. k0+CH of the chunk being finished
   const int nvl = NVL, nvlp1 = NVL + 1, ncol = s.ncol;
-  const int col = blockIdx.x * 64 + threadIdx.x;
-  if (col >= ncol) return;
-  double th[NVL];
+  const int lane = threadIdx.x, c0 = blockIdx.x * 64;
+  const int sl = lane & 15, sj = lane >> 4;
+  double cth = 0., cdl = 0.;   // level k0+CH (the chunk above): theta still to be finalised, layer depth
+#pragma unroll 1
+  for (int ch = NCH - 1; ch >= 0; ch--) {
+    const int k0 = ch * CH;
 #pragma unroll
-  for (int k = 0; k < NVL; k++) th[k] = C2(s.thbuf, col, k);
-#pragma unroll
-  for (int k = NVL; k >= 3; k--) {
-    const double dplo = L2(s.exsmo, col, k - 2), dpup = L2(s.exsmo, col, k - 1);
-    const double tha = th[k - 2], thb = th[k - 1];
-    const double devilo = (tha - ldc(s.thetac, k - 2)) * dplo;
-    const double deviup = (thb - ldc(s.thetac, k - 1)) * dpup;
-    if (deviup > 0. && devilo < 0.) {
-      th[k - 2] = tha + dmin(deviup, -devilo) / dplo;
-      th[k - 1] = thb - dmin(deviup, -devilo) / dpup;
-    } else if (deviup < 0. && devilo > 0.) {
-      th[k - 2] = tha - dmin(devilo, -deviup) / dplo;
-      th[k - 1] = thb + dmin(devilo, -deviup) / dpup;
+    for (int i = 0; i < 16; i++) {
+      const int c = i * 4 + sj;
+      const int cc = min(c0 + c, ncol - 1);   // a partial last tile repeats its last column
+      sth[sl][c] = C2(s.thbuf, cc, k0 + sl);
+      sdl[sl][c] = L2(s.exsmo, cc, k0 + sl);
     }
-  }
+    wsync();
+    double th[CH], dl[CH];
 #pragma unroll
-  for (int k = 0; k < NVL; k++) TR(s.tr, 0, col, k) = (V)th[k];
+    for (int j = 0; j < CH; j++) {
+      th[j] = sth[j][lane];
+      dl[j] = sdl[j][lane];
+    }
+    if (ch < NCH - 1) {   // the pair (k0+CH-1, k0+CH) finishes level k0+CH
+      theta_pair(th[CH - 1], cth, dl[CH - 1], cdl, ldc(s.thetac, k0 + CH - 1), ldc(s.thetac, k0 + CH));
+      sout[CH][lane] = (V)cth;
+    }
+#pragma unroll
+    for (int j = CH - 2; j >= 0; j--) {
+      if (k0 + j < 1) break;
+      theta_pair(th[j], th[j + 1], dl[j], dl[j + 1], ldc(s.thetac, k0 + j), ldc(s.thetac, k0 + j + 1));
+    }
+    // levels k0+1 .. k0+CH-1 are final; k0 is carried (final in the bottom chunk)
+#pragma unroll
+    for (int j = 0; j < CH; j++) sout[j][lane] = (V)th[j];
+    cth = th[0];
+    cdl = dl[0];
+    wsync();
+    // store levels k0+1 .. k0+CH (16 levels of 4 columns per instruction), and level 0 at the bottom
+    const int lev = k0 + 1 + sl;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int c = i * 4 + sj;
+      if (c0 + c < ncol && lev < NVL) TR(s.tr, 0, c0 + c, lev) = sout[1 + sl][c];
+    }
+    if (ch == 0 && c0 + lane < ncol) TR(s.tr, 0, c0 + lane, 0) = sout[0][lane];
+    wsync();
+  }
+  (void)nvl;
+  (void)nvlp1;
 }
 
 // FIMHIP_HYBGEN=thread selects the one-thread-per-column kernels
