@@ -24,13 +24,20 @@
 namespace fimhip {
 
 // Device allocations of one instance, zeroed on its stream and freed together.
+// Each array starts at a different offset (a multiple of 4352 bytes below
+// 512 KiB) from its 2 MiB-aligned allocation, so that kernels streaming many
+// arrays at the same index do not send every stream to the same memory
+// channel at the same time.
 struct Arena {
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;
   int raw(void **p, size_t bytes) {
     if (bytes == 0) bytes = 8;
-    if (hipMalloc(p, bytes) != hipSuccess) return FIMHIP_ENOMEM;
-    allocs.push_back(*p);
+    const size_t pad = (allocs.size() * 4352) % (512 * 1024);
+    void *base = nullptr;
+    if (hipMalloc(&base, bytes + pad) != hipSuccess) return FIMHIP_ENOMEM;
+    allocs.push_back(base);
+    *p = (char *)base + pad;
     if (hipMemsetAsync(*p, 0, bytes, stream) != hipSuccess) return FIMHIP_EHIP;
     return 0;
   }

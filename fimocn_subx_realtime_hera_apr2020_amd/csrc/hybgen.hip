@@ -711,6 +711,7 @@ __device__ __forceinline__ bool absorbs_f(float t, float bound) {
   return at > 1.e-30f && bound * (1.f + 1.e-5f) * (1.e-22f * 67108864.f) < at;   // 2^26
 }
 __device__ float pcwise_wf(int kk, const PcwGeomF &gm, float a, int lane) {
+#pragma clang fp contract(fast)   // FP32 state: fused multiply-adds
   const float small = 1.e-22f;
   const bool live = lane < kk;
   const float f = gm.f;

@@ -63,6 +63,12 @@ __device__ __forceinline__ double wave_prefix_sum(double x, int lane) {
   return x;
 }
 
+// The edge-wind, edgvar1 and momtum bodies are compiled with a*b+c
+// contracted to fused multiply-adds (one rounding instead of two): none of
+// their results has to match another cell's bit for bit.  The flux code of
+// cnuity and trcadv, where the two cells of an edge must form exactly
+// opposite fluxes, and the shared theta edge value keep separate roundings.
+//
 // Hardware square root and reciprocal (v_sqrt_f32, v_rcp_f32: about 1 ulp)
 // in place of the correctly rounded sequences of sqrtf and '/', which cost
 // about ten instructions each; every reciprocal below is of a value bounded
@@ -74,10 +80,12 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 // winds of the cell (i), the cell across the edge (x) and the two cells at
 // the edge's ends (t: slot e-1, l: slot e+1) rotated into the edge frame with
 // cell c's coefficients, trapezoid-weighted, the magnitude rescaled to the
-// weighted mean of the four magnitudes.  edgvar1, cnuity12 and momtum all form
-// edge winds through this one function, so every use sees the same bits.
+// weighted mean of the four magnitudes.  edgvar1 (normal component) and
+// momtum (kinetic energy, vorticity) both form edge winds through this one
+// function.
 __device__ __forceinline__ float2 e_edge_wind(const EnsGrid &g, int c, int e, float ui, float vi, float ux, float vx,
                                               float ut, float vt, float ul, float vl) {
+#pragma clang fp contract(fast)
   const float d6 = 1.f / 6.f, eps = 0.001f;
   const float c0 = FCS(c, e, 0), s0 = FSN(c, e, 0), c1 = FCS(c, e, 1), s1 = FSN(c, e, 1);
   const float c2 = FCS(c, e, 2), s2 = FSN(c, e, 2), c3 = FCS(c, e, 3), s3 = FSN(c, e, 3);
@@ -114,6 +122,7 @@ __device__ __forceinline__ float e_theta_edge(float thi, float thx, float tht, f
 // step's starting theta in th0 for momtum.
 template <int NP>
 __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &s, int ipn, int base, int k) {
+#pragma clang fp contract(fast)
   const int nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
   const int col = base + ipn;
   int nb[NP];
@@ -576,6 +585,7 @@ __global__ void __launch_bounds__(64 * COLS) k_e_trcadv3(EnsGrid g, EnsState s) 
 template <int NP>
 __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s, const StepCtl &c, int ipn,
                                              int base, int k, float &un, float &vn) {
+#pragma clang fp contract(fast)
   const int nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
   const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
   const int col = base + ipn;
