@@ -215,6 +215,8 @@ void fo_edgvar3(const fo_grid *g, fo_state *s) {
   }
 }
 
+static void update_pres(const fo_grid *g, fo_state *s);
+
 /* ------------------------------------------------------------------------
  * cnuity.F90:23-249 with cnuity_compute.F90:17-451 */
 void fo_cnuity(const fo_grid *g, fo_state *s) {
@@ -312,7 +314,15 @@ void fo_cnuity(const fo_grid *g, fo_state *s) {
                           ab2 * AB(s->dp_tdcy, of, ipn, k) + ab3 * AB(s->dp_tdcy, vof, ipn, k);
     }
   }
-  /* cnuity5 */
+  update_pres(g, s);
+}
+
+/* cnuity5 (cnuity_compute.F90:376-451), the same as update_pres of mass_adv
+ * (massadv_compute.F90:73-171): top-down pressure, Exner function and omega */
+static void update_pres(const fo_grid *g, fo_state *s) {
+  const int nip = g->nip, nvl = g->nvl, nvlp1 = nvl + 1;
+  const int nf = s->nf - 1, of = s->of - 1, vof = s->vof - 1;
+  const double ab1 = s->ab1, ab2 = s->ab2, ab3 = s->ab3;
   const double rdcp = RD / CP;
   #pragma omp parallel for schedule(static)
   for (int ipn = 0; ipn < nip; ipn++) {
@@ -332,6 +342,110 @@ void fo_cnuity(const fo_grid *g, fo_state *s) {
       C2(s->ws, ipn, k) = om + (C2(s->us, ipn, k) * dpdx + C2(s->vs, ipn, k) * dpdy) * g->rarea[ipn];
     }
   }
+}
+
+/* ------------------------------------------------------------------------
+ * mass_adv (flux_conserv_schm = 2, dyn_run.F90:377-386): massadv.F90:21-160
+ * with flux_limiter.F90:35-262 (init_mass_limiter, edgeNormVel, limiter
+ * with M1 defined) and massadv_compute.F90:9-171 (massflux_newdepth,
+ * update_pres).  The limited edge depth replaces dp_edg, as edgeVal does. */
+void fo_mass_adv(const fo_grid *g, fo_state *s) {
+  const int nip = g->nip, nvl = g->nvl;
+  const int nf = s->nf - 1, of = s->of - 1, vof = s->vof - 1;
+  const double ab1 = s->ab1, ab2 = s->ab2, ab3 = s->ab3, dt = g->dt;
+  const double max_out_bound = 100000.0;
+  double *vnorm = s->work_e1, *c_num = s->work_e2;
+  double *q_min_out = s->work_c1, *q_max_out = s->work_c2;
+  double *massfx = s->massfx, *dp = s->dp, *dpe = s->dp_edg;
+  /* edgeNormVel (flux_limiter.F90:78-103) */
+  #pragma omp parallel for schedule(static)
+  for (int ipn = 0; ipn < nip; ipn++)
+    for (int edg = 0; edg < g->nprox[ipn]; edg++)
+      for (int k = 0; k < nvl; k++) {
+        const double vn = SVE(ipn, edg, 1) * E3(s->u_edg, ipn, edg, k) - SVE(ipn, edg, 0) * E3(s->v_edg, ipn, edg, k);
+        E3(vnorm, ipn, edg, k) = vn;
+        E3(c_num, ipn, edg, k) = fabs(vn) * dt * g->rarea[ipn];
+      }
+  /* limiter (flux_limiter.F90:132-260): the edge value clamped to the four
+   * cells around the edge (q_star, kept in dp_edg), then the in/out bounds */
+  #pragma omp parallel for schedule(static)
+  for (int ipn = 0; ipn < nip; ipn++) {
+    const int np = g->nprox[ipn];
+    for (int edg = 0; edg < np; edg++) {
+      const int nbm1 = PROX(ipn, (edg + np - 1) % np), nbp1 = PROX(ipn, (edg + 1) % np), ipx = PROX(ipn, edg);
+      for (int k = 0; k < nvl; k++) {
+        const double a = C2(dp, ipn, k), b = C2(dp, ipx, k), c = C2(dp, nbm1, k), d = C2(dp, nbp1, k);
+        const double q_min = dmin(dmin(a, b), dmin(c, d)), q_max = dmax(dmax(a, b), dmax(c, d));
+        E3(dpe, ipn, edg, k) = dmax(q_min, dmin(q_max, E3(dpe, ipn, edg, k)));
+      }
+    }
+  }
+  #pragma omp parallel for schedule(static)
+  for (int ipn = 0; ipn < nip; ipn++) {
+    const int np = g->nprox[ipn];
+    for (int k = 0; k < nvl; k++) {
+      double q_min_in_min = 1.7976931348623157e308, q_max_in_max = 0.;   /* huge(), 0 */
+      double cq_min_in_sum = 0., cq_max_in_sum = 0., c_num_in_sum = 0., c_num_out_sum = 0.;
+      const double gv = C2(dp, ipn, k);
+      for (int edg = 0; edg < np; edg++) {
+        const int ipx = PROX(ipn, edg);
+        const double qs = E3(dpe, ipn, edg, k), cn = E3(c_num, ipn, edg, k);
+        if (E3(vnorm, ipn, edg, k) > 0.) {
+          c_num_out_sum = c_num_out_sum + cn;
+        } else {
+          const double q_min = dmin(qs, C2(dp, ipx, k)), q_max = dmax(qs, C2(dp, ipx, k));
+          c_num_in_sum = c_num_in_sum + cn;
+          q_min_in_min = dmin(q_min_in_min, q_min);
+          q_max_in_max = dmax(q_max_in_max, q_max);
+          cq_min_in_sum = cq_min_in_sum + cn * q_min;
+          cq_max_in_sum = cq_max_in_sum + cn * q_max;
+        }
+      }
+      const double q_min_next = dmin(gv, q_min_in_min), q_max_next = dmax(gv, q_max_in_max);
+      if (c_num_out_sum != 0.) {
+        const double at = ab2 * AB(s->dp_tdcy, of, ipn, k), av = ab3 * AB(s->dp_tdcy, vof, ipn, k);
+        C2(q_min_out, ipn, k) = (gv + ab1 / dt * cq_max_in_sum - q_max_next * (1.0 + c_num_in_sum - c_num_out_sum) +
+                                 at + av) / c_num_out_sum / ab1 * dt;
+        C2(q_max_out, ipn, k) = (gv + ab1 / dt * cq_min_in_sum - q_min_next * (1.0 + c_num_in_sum - c_num_out_sum) +
+                                 at + av) / c_num_out_sum / ab1 * dt;
+      } else {
+        C2(q_min_out, ipn, k) = -max_out_bound;
+        C2(q_max_out, ipn, k) = max_out_bound;
+      }
+    }
+  }
+  #pragma omp parallel for schedule(static)
+  for (int ipn = 0; ipn < nip; ipn++)
+    for (int edg = 0; edg < g->nprox[ipn]; edg++) {
+      const int ipx = PROX(ipn, edg);
+      for (int k = 0; k < nvl; k++) {
+        const int up = E3(vnorm, ipn, edg, k) > 0. ? ipn : ipx;
+        E3(dpe, ipn, edg, k) = dmax(C2(q_min_out, up, k), dmin(C2(q_max_out, up, k), E3(dpe, ipn, edg, k)));
+      }
+    }
+  /* massflux_newdepth (massadv_compute.F90:9-70) */
+  #pragma omp parallel for schedule(static)
+  for (int ipn = 0; ipn < nip; ipn++) {
+    double ps = 0.;
+    for (int k = nvl - 1; k >= 0; k--) ps = ps + C2(dp, ipn, k);
+    s->psurf[ipn] = ps;
+    for (int k = 0; k < nvl; k++) MF(massfx, nf, ipn, NPP - 1, k) = 0.;
+    for (int edg = 0; edg < g->nprox[ipn]; edg++)
+      for (int k = 0; k < nvl; k++) MF(massfx, nf, ipn, edg, k) = E3(vnorm, ipn, edg, k) * E3(dpe, ipn, edg, k);
+  }
+  #pragma omp parallel for schedule(static)
+  for (int ipn = 0; ipn < nip; ipn++) {
+    for (int k = 0; k < nvl; k++) AB(s->dp_tdcy, nf, ipn, k) = 0.;
+    for (int edg = 0; edg < g->nprox[ipn]; edg++)
+      for (int k = 0; k < nvl; k++) AB(s->dp_tdcy, nf, ipn, k) = AB(s->dp_tdcy, nf, ipn, k) + MF(massfx, nf, ipn, edg, k);
+    for (int k = 0; k < nvl; k++) AB(s->dp_tdcy, nf, ipn, k) = -AB(s->dp_tdcy, nf, ipn, k) * g->rarea[ipn];
+    for (int k = 0; k < nvl; k++) {
+      double d = C2(dp, ipn, k) + ab1 * AB(s->dp_tdcy, nf, ipn, k) + ab2 * AB(s->dp_tdcy, of, ipn, k) +
+                 ab3 * AB(s->dp_tdcy, vof, ipn, k);
+      C2(dp, ipn, k) = dmax(d, 0.0);
+    }
+  }
+  update_pres(g, s);
 }
 
 /* ------------------------------------------------------------------------
@@ -923,7 +1037,8 @@ void fo_step(const fo_grid *g, fo_state *s, int its, int mode) {
   fo_abstart(s, its, g->dt);
   fo_edgvar1(g, s);
   fo_edgvar3(g, s);
-  fo_cnuity(g, s);
+  if (g->flux_scheme == 2) fo_mass_adv(g, s);
+  else fo_cnuity(g, s);
   fo_trcadv(g, s);
   if (mode == 0) {
     fo_momtum(g, s);

@@ -6,6 +6,7 @@
 typedef struct {
   int nip, nvl, ntra;
   int pow_kind;              /* 0: C library pow (as the Fortran), 1: fim_pow (as the GPU) */
+  int flux_scheme;           /* flux_conserv_schm: 1 cnuity (FCT), 2 mass_adv (flux limiter) */
   double dt;
   const int *prox, *proxs, *nprox;
   const double *cs, *sn, *sidevec_c, *sidevec_e, *sideln, *hfcoef;
@@ -31,6 +32,7 @@ void fo_abstart(fo_state *s, int its, double dt);
 void fo_edgvar1(const fo_grid *g, fo_state *s);
 void fo_edgvar3(const fo_grid *g, fo_state *s);
 void fo_cnuity(const fo_grid *g, fo_state *s);
+void fo_mass_adv(const fo_grid *g, fo_state *s);
 void fo_trcadv(const fo_grid *g, fo_state *s);
 void fo_momtum(const fo_grid *g, fo_state *s);
 void fo_hybgen(const fo_grid *g, fo_state *s, int its);

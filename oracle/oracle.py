@@ -32,7 +32,7 @@ _PI = ct.POINTER(ct.c_int)
 
 class _Grid(ct.Structure):
     _fields_ = [("nip", ct.c_int), ("nvl", ct.c_int), ("ntra", ct.c_int), ("pow_kind", ct.c_int),
-                ("dt", ct.c_double),
+                ("flux_scheme", ct.c_int), ("dt", ct.c_double),
                 ("prox", _PI), ("proxs", _PI), ("nprox", _PI),
                 ("cs", _PD), ("sn", _PD), ("sidevec_c", _PD), ("sidevec_e", _PD), ("sideln", _PD),
                 ("hfcoef", _PD), ("area", _PD), ("rarea", _PD), ("corio", _PD), ("grndspd", _PD),
@@ -123,12 +123,13 @@ def grid_arrays(grid, nvl: int, dt: float, thetac, dpsig) -> dict:
 class OracleRun:
     """A full FP64 state advanced by the C restatement."""
 
-    def __init__(self, garr: dict, nvl: int, ntra: int, dt: float, pow_kind: int = 1):
+    def __init__(self, garr: dict, nvl: int, ntra: int, dt: float, pow_kind: int = 1, flux_scheme: int = 1):
         self.garr = garr
         nip = garr["nprox"].shape[0]
         self.nip, self.nvl, self.ntra, self.dt = nip, nvl, ntra, dt
         g = _Grid()
         g.nip, g.nvl, g.ntra, g.pow_kind, g.dt = nip, nvl, ntra, pow_kind, dt
+        g.flux_scheme = flux_scheme
         for n in ("prox", "proxs", "nprox"):
             setattr(g, n, _pi(garr[n]))
         for n in ("cs", "sn", "sidevec_c", "sidevec_e", "sideln", "hfcoef", "area", "rarea", "corio",

@@ -14,6 +14,8 @@
 ! mode 1: transport = edgvar1, edgvar3, cnuity, trcadv with fixed winds
 ! flags bit 0: run the dyn_init start-up hybgen(its=0) + pressure refresh
 !              (dyn_init.F90:880-893) before the start-up hystat.
+! flags bit 1: continuity by mass_adv (flux_conserv_schm = 2,
+!              dyn_run.F90:377-386) instead of cnuity.
 !
 ! Output: a sequence of records  character(16) tag, int64 n, real64 vals(n).
 ! Stage records are written for step 'dumpstep' after every sub-call.
@@ -22,12 +24,13 @@ program dynref
   use module_control,   only: nip, nvlp1, dt, ntra, npp, nabl, kbl
   use fimnamelist,      only: nvl, intfc_smooth, veldff_bkgnd, veldff_boost
   use module_constants
-  use module_variables, only: nf, of, vof, adbash1, adbash2, adbash3, worka, workb
+  use module_variables, only: nf, of, vof, adbash1, adbash2, adbash3, worka, workb, q_min_out, q_max_out
   use global_bounds,    only: set_global_bounds, set_global_bounds_halo
   use module_abstart,   only: abstart
   use module_edgvar1,   only: edgvar1
   use module_edgvar3,   only: edgvar3
   use cnuity_driver,    only: cnuity
+  use massadv_driver,   only: mass_adv
   use trcadv_driver,    only: trcadv
   use momtum_driver,    only: momtum
   use module_hybgen,    only: hybgen
@@ -189,6 +192,9 @@ program dynref
   allocate (u_edg(nvl,npp,nip), v_edg(nvl,npp,nip), dp_edg(nvl,npp,nip), lp_edg(nvl,npp,nip))
   allocate (geop_edg(nvl,npp,nip), mont_edg(nvl,npp,nip), uvsq_edg(nvl,npp,nip), trc_edg(nvl,npp,nip,ntra))
   allocate (worka(nvl,nip), workb(nvl,nip))
+  allocate (q_min_out(nvl,nip), q_max_out(nvl,nip))   ! dyn_alloc.F90:284-302
+  q_min_out = 0.
+  q_max_out = 0.
   allocate (r8a(nvl,nip))
   read (21) r8a; us3d = r8a
   read (21) r8a; vs3d = r8a
@@ -245,8 +251,13 @@ program dynref
     call edgvar3 (its, us3d, vs3d, dp3d, pr3d, ex3d, ph3d, mp3d, tr3d, u_edg, v_edg, dp_edg, &
                   lp_edg, geop_edg, mont_edg, uvsq_edg, trc_edg)
     if (its == dumpstep) call dumpedges('edgvar3')
-    call cnuity (its, us3d, vs3d, u_edg, v_edg, dp_edg, lp_edg, dp3d, pr3d, ex3d, &
-                 dp_tdcy, dpl_tdcy, massfx, ws3d, .false.)
+    if (btest(flags,1)) then
+      call mass_adv (its, us3d, vs3d, u_edg, v_edg, dp_edg, lp_edg, dp3d, pr3d, ex3d, &
+                     dp_tdcy, massfx, ws3d, .false.)
+    else
+      call cnuity (its, us3d, vs3d, u_edg, v_edg, dp_edg, lp_edg, dp3d, pr3d, ex3d, &
+                   dp_tdcy, dpl_tdcy, massfx, ws3d, .false.)
+    end if
     if (its == dumpstep) then
       call dumpstate('cnuity')
       call out3('cnuity.massfx', real(massfx(:,:,:,nf),8))

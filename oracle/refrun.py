@@ -58,11 +58,12 @@ def ref_file(glvl: int, name: str) -> str:
 MAGIC = 1179208497
 
 
-def write_state(path, st, thetac, dpsig, *, nsteps, dt, mode=0, dumpstep=0, init_hybgen=True):
+def write_state(path, st, thetac, dpsig, *, nsteps, dt, mode=0, dumpstep=0, init_hybgen=True, flux_scheme=1):
     nip, nvl = st["dp"].shape
     ntra = st["tr"].shape[0]
+    flags = (1 if init_hybgen else 0) | (2 if flux_scheme == 2 else 0)
     with open(path, "wb") as f:
-        f.write(struct.pack("<8i", MAGIC, nvl, nip, ntra, nsteps, mode, dumpstep, 1 if init_hybgen else 0))
+        f.write(struct.pack("<8i", MAGIC, nvl, nip, ntra, nsteps, mode, dumpstep, flags))
         f.write(struct.pack("<d", dt))
         for a in (thetac, dpsig, st["us"], st["vs"], st["dp"], st["pr"], st["ex"], st["ph"], st["tr"]):
             f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
@@ -94,14 +95,14 @@ def _big_stack():
 
 
 def run_dynref(glvl_path, st, thetac, dpsig, *, nsteps, dt, mode=0, dumpstep=0,
-               init_hybgen=True, prec="r8", workdir=None) -> dict:
+               init_hybgen=True, prec="r8", workdir=None, flux_scheme=1) -> dict:
     workdir = workdir or os.path.join(REF, "runs")
     os.makedirs(workdir, exist_ok=True)
-    tag = "%d_%d_%d_%s" % (os.getpid(), nsteps, mode, prec)
+    tag = "%d_%d_%d_%s_%d" % (os.getpid(), nsteps, mode, prec, flux_scheme)
     fin = os.path.join(workdir, "state_%s.bin" % tag)
     fout = os.path.join(workdir, "out_%s.bin" % tag)
     write_state(fin, st, thetac, dpsig, nsteps=nsteps, dt=dt, mode=mode, dumpstep=dumpstep,
-                init_hybgen=init_hybgen)
+                init_hybgen=init_hybgen, flux_scheme=flux_scheme)
     r = subprocess.run([os.path.join(BIN, "dynref_" + prec), glvl_path, fin, fout],
                        cwd=workdir, capture_output=True, text=True, preexec_fn=_big_stack)
     if r.returncode != 0:
