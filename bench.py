@@ -23,7 +23,9 @@ Prints ONE JSON line.  Besides the contract fields it carries
   step_hbm:     the whole step against the compulsory-byte model (18 + 8T
                 values per cell-level-step; 220 B in the mixed form, 400 B
                 in FP64)
-  plain_fp64:   the FP64 path's throughput, roofline and step_hbm
+  plain_fp64:   the FP64 path's throughput, roofline and step_hbm; under
+                mass_adv the same step with the flux-limited mass transport
+                (flux_conserv_schm = 2) and its two kernels
   accuracy_96:  relative L2 of member 0 against FP64 after 96 steps, mass drift
   cpu_baseline: the C restatement (oracle/fim_oracle.c, FP64, OpenMP) on a
                 bounded sample of the same grid
@@ -50,6 +52,9 @@ def plain_kernel_bytes(T):
         "k_cnuity12": 18 + 3 + 6 + 6 + 2,    # u/v/dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo
         "k_cnuity3": 6 + 4 + 2,              # antifx, delp_lo, dp, dp_tdcy(of,vof) -> r+, r-
         "k_cnuity45": 27 + 11,               # antifx r+- massfx dplo dptdcy dp pr lp_edg u v -> massfx dptdcy dp pr ex omega
+        "k_massadv1": 12 + 6 + 1 + 2 + 6 + 2,   # u/v/dp edges, dp, dp_tdcy(of,vof) -> limited dp edges, q_min/max_out
+        "k_massadv2": 12 + 2 + 6 + 2 + 1 + 6 + 2 + 6 + 6 + 1 + 1 + 1 + 1 + 1,
+        # u/v/dp edges, q_min/max_out, dp_tdcy(of,vof), dp, lp_edg, u, v -> dp edges, massfx, dp_tdcy, dp, pr, ex, omega
         "k_trcadv1": 27 * T,
         "k_trcadv2": 16 * T,
         "k_trcadv3": 18 * T,
@@ -200,7 +205,42 @@ def run_plain(a, grid, th, ds, fimhip):
            "step_hbm": {"bytes_per_cell_level": 8 * v_step, "achieved_GBs": value * 8 * v_step / 1e9,
                         "frac": value * 8 * v_step / HBM_PEAK},
            "kernels": kernel_table(prof, kb, cells)}
+    res["mass_adv"] = run_plain_mass_adv(a, m, cells, kb, fimhip)
     return m, res
+
+
+def run_plain_mass_adv(a, m, cells, kb, fimhip):
+    """The same FP64 step with the flux-limited mass transport in place of
+    FCT cnuity (flux_conserv_schm = 2): step time and its two kernels."""
+    import torch
+    m.set_flux_scheme(2)
+    m.synth(seed=SEED, member=0, flags=fimhip.SYNTH_WIND_NOISE | fimhip.SYNTH_PERTURB)
+    m.init_state(True)
+    its = 0
+    for _ in range(a.warmup):
+        its += 1
+        m.step(its, a.mode)
+    m.sync()
+    torch.cuda.synchronize()
+    n_timed = 40
+    t0 = time.perf_counter()
+    for _ in range(n_timed):
+        its += 1
+        m.step(its, a.mode)
+    m.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    m.profile(True)
+    for _ in range(a.steps):
+        its += 1
+        m.step(its, a.mode)
+    m.sync()
+    prof = m.profile_report()
+    m.profile(False)
+    m.set_flux_scheme(1)
+    kt = kernel_table(prof, kb, cells)
+    return {"value": cells * n_timed / wall, "ms_per_step": wall / n_timed * 1e3, "timed_steps": n_timed,
+            "kernels": {k: kt[k] for k in ("k_massadv1", "k_massadv2") if k in kt}}
 
 
 def synth_ens(e, members, fimhip):

@@ -47,6 +47,7 @@ struct fimhip_ctx {
   int gsum_blocks = 0;
   std::vector<double> lat, lon;      // host copies for the synthetic-state profiles
   KTimer *kt = nullptr;              // non-null while profiling
+  int flux_conserv_schm = 1;         // 1 = cnuity (FCT), 2 = mass_adv
 };
 
 // dyn_init.F90:125 taper for smoo_coeff
@@ -204,6 +205,12 @@ extern "C" int fimhip_set_ab(fimhip_ctx *c, int nf, int of, int vof, double a1, 
   return FIMHIP_OK;
 }
 
+extern "C" int fimhip_set_flux_scheme(fimhip_ctx *c, int flux_conserv_schm) {
+  if (!c || (flux_conserv_schm != 1 && flux_conserv_schm != 2)) return FIMHIP_EINVAL;
+  c->flux_conserv_schm = flux_conserv_schm;
+  return FIMHIP_OK;
+}
+
 // ---------------------------------------------------------------------------
 static double **field_ptr(fimhip_ctx *c, int f) {
   DevState &s = c->s;
@@ -357,6 +364,24 @@ extern "C" int fimhip_cnuity_r8(fimhip_ctx *c, int its, const double *u_velo, co
   return finish(c);
 }
 
+extern "C" int fimhip_mass_adv_r8(fimhip_ctx *c, int its, const double *u_velo, const double *v_velo,
+                                  const double *u_edg, const double *v_edg, double *dp_edg, const double *lp_edg,
+                                  double *delp, double *pres, double *exner, double *dp_tndcy, double *massfx,
+                                  double *omega, int timing) {
+  (void)its; (void)timing;
+  if (!c) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(h2d(c, FIMHIP_US, u_velo)); TRY(h2d(c, FIMHIP_VS, v_velo)); TRY(h2d(c, FIMHIP_U_EDG, u_edg));
+  TRY(h2d(c, FIMHIP_V_EDG, v_edg)); TRY(h2d(c, FIMHIP_DP_EDG, dp_edg)); TRY(h2d(c, FIMHIP_LP_EDG, lp_edg));
+  TRY(h2d(c, FIMHIP_DP, delp)); TRY(h2d(c, FIMHIP_PR, pres)); TRY(h2d(c, FIMHIP_EX, exner));
+  TRY(h2d(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(h2d(c, FIMHIP_MASSFX, massfx));
+  launch_mass_adv(c->g, c->s, c->ab, c->stream);
+  TRY(d2h(c, FIMHIP_DP_EDG, dp_edg)); TRY(d2h(c, FIMHIP_DP, delp)); TRY(d2h(c, FIMHIP_PR, pres));
+  TRY(d2h(c, FIMHIP_EX, exner)); TRY(d2h(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(d2h(c, FIMHIP_MASSFX, massfx));
+  TRY(d2h(c, FIMHIP_WS, omega));
+  return finish(c);
+}
+
 extern "C" int fimhip_trcadv_r8(fimhip_ctx *c, int its, const double *trcr_edg, double *tracr, double *trcdp,
                                 double *trc_tdcy, double *trclo_tdcy, const double *massfx, const double *delp,
                                 int timing) {
@@ -451,7 +476,8 @@ extern "C" int fimhip_step(fimhip_ctx *c, int its, int mode) {
   ab_rotate(c->ab, its, c->dt);
   launch_edgvar1(c->g, c->s, c->stream);
   launch_edgvar3(c->g, c->s, c->stream);
-  launch_cnuity(c->g, c->s, c->ab, c->stream);
+  if (c->flux_conserv_schm == 2) launch_mass_adv(c->g, c->s, c->ab, c->stream);
+  else launch_cnuity(c->g, c->s, c->ab, c->stream);
   launch_trcadv(c->g, c->s, c->ab, c->stream);
   if (mode == 0) {
     launch_momtum(c->g, c->s, c->ab, c->stream);

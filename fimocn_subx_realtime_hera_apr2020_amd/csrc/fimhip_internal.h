@@ -125,6 +125,7 @@ void kt_end(hipStream_t st);
 void launch_edgvar1(const DevGrid &g, const DevState &s, hipStream_t st);
 void launch_edgvar3(const DevGrid &g, const DevState &s, hipStream_t st);
 void launch_cnuity(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st);
+void launch_mass_adv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st);
 void launch_trcadv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st);
 void launch_momtum(const DevGrid &g, DevState &s, const AB &ab, hipStream_t st);
 void launch_hybgen(const DevGrid &g, const DevState &s, int its, hipStream_t st);

@@ -180,6 +180,42 @@ __global__ void __launch_bounds__(256) k_cnuity3(DevGrid g, DevState s, AB ab) {
   C2(s.r_mnus, ipn, k) = dmax(0., dmin(1., q_mnus / dmax(pm, thshld)));
 }
 
+// cnuity5 (cnuity_compute.F90:376-451), which is also update_pres of mass_adv
+// (massadv_compute.F90:73-171): the top-down pressure and column-tendency
+// sums run serially in lane 0 in the reference summation order over the new
+// thickness (dpn) and layer tendency (lt) each lane left in LDS; then the
+// Exner function and omega per lane.  Called by every thread of the block.
+__device__ __forceinline__ void update_pres_col(const DevGrid &g, const DevState &s, int ipn, bool live,
+                                                const double *dpn, const double *lt, double *pr, double *om) {
+  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
+  const int k = threadIdx.x;
+  __syncthreads();
+  if (k == 0 && ipn < nip) {
+    double p = L2(s.pr, ipn, nvl);
+    double coltend = 0.;
+    for (int kk = nvl - 1; kk >= 0; kk--) {
+      p = p + dpn[kk];
+      pr[kk] = p;
+      const double t = lt[kk];
+      om[kk] = coltend + 0.5 * t;
+      coltend = coltend + t;
+    }
+  }
+  __syncthreads();
+  if (live) {
+    const double p = pr[k];
+    L2(s.pr, ipn, k) = p;
+    L2(s.ex, ipn, k) = CP * fim_pow(p / P1000, RD / CP);
+    const int np = GNP(ipn);
+    double dpdx = 0., dpdy = 0.;
+    for (int edg = 0; edg < np; edg++) {
+      dpdx = dpdx + E3(s.lp_edg, ipn, edg, k) * SVC(ipn, edg, 1);
+      dpdy = dpdy - E3(s.lp_edg, ipn, edg, k) * SVC(ipn, edg, 0);
+    }
+    C2(s.ws, ipn, k) = om[k] + (C2(s.us, ipn, k) * dpdx + C2(s.vs, ipn, k) * dpdy) * GV(rarea, ipn);
+  }
+}
+
 // cnuity4 + cnuity5 (cnuity_compute.F90:261-451), one block row per column:
 // the top-down pressure and column-tendency sums are serial in lane 0 (the
 // reference summation order), the Exner function is per lane.
@@ -188,7 +224,7 @@ __global__ void __launch_bounds__(256) k_cnuity45(DevGrid g, DevState s, AB ab) 
   __shared__ double sh_lt[COLS][NVL_MAX + 1];
   __shared__ double sh_pr[COLS][NVL_MAX + 1];
   __shared__ double sh_om[COLS][NVL_MAX + 1];
-  const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
+  const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x, c = threadIdx.y;
   const int ipn = wave_col();
   const bool live = ipn < nip && k < nvl;
@@ -215,31 +251,108 @@ __global__ void __launch_bounds__(256) k_cnuity45(DevGrid g, DevState s, AB ab) 
     sh_dp[c][k] = dpn;
     sh_lt[c][k] = (ab.ab1 * dtn + ab.ab2 * to + ab.ab3 * tv) / g.dt;
   }
-  __syncthreads();
-  if (k == 0 && ipn < nip) {
-    double p = L2(s.pr, ipn, nvl);
-    double coltend = 0.;
-    for (int kk = nvl - 1; kk >= 0; kk--) {
-      p = p + sh_dp[c][kk];
-      sh_pr[c][kk] = p;
-      double lt = sh_lt[c][kk];
-      sh_om[c][kk] = coltend + 0.5 * lt;
-      coltend = coltend + lt;
+  update_pres_col(g, s, ipn, live, sh_dp[c], sh_lt[c], sh_pr[c], sh_om[c]);
+}
+
+// ---------------------------------------------------------------------------
+// mass_adv, the flux-limited mass transport of flux_conserv_schm = 2
+// (massadv.F90:21-160, dyn_run.F90:377-386).
+//
+// k_massadv1: edgeNormVel (flux_limiter.F90:78-103) and limiter
+// (flux_limiter.F90:132-260, M1 form) for one column: each edge depth is
+// clamped to the four cells around the edge (q_star, stored back into
+// dp_edg), then the inflow sums give the bounds q_min_out / q_max_out of the
+// column's outflow edge values (kept in r_plus / r_mnus).  Normal winds and
+// Courant numbers stay in registers.
+__global__ void __launch_bounds__(256) k_massadv1(DevGrid g, DevState s, AB ab) {
+  const int nip = g.nip, nvl = g.nvl;
+  const int k = threadIdx.x;
+  const int ipn = wave_col();
+  if (ipn >= nip || k >= nvl) return;
+  const int of = ab.of - 1, vof = ab.vof - 1;
+  const double dt = g.dt, rar = GV(rarea, ipn);
+  const double max_out_bound = 100000.0;
+  const int np = GNP(ipn);
+  const double gv = C2(s.dp, ipn, k);
+  const double to = AB3(s.dp_tdcy, of, ipn, k), tv = AB3(s.dp_tdcy, vof, ipn, k);
+  double q_min_in_min = 1.7976931348623157e308, q_max_in_max = 0.;   // huge(), 0
+  double cq_min_in_sum = 0., cq_max_in_sum = 0., c_num_in_sum = 0., c_num_out_sum = 0.;
+  for (int edg = 0; edg < np; edg++) {
+    const int ipx = PROX(ipn, edg);
+    const int nbm1 = PROX(ipn, edg == 0 ? np - 1 : edg - 1), nbp1 = PROX(ipn, edg + 1 == np ? 0 : edg + 1);
+    const double vn = SVE(ipn, edg, 1) * E3(s.u_edg, ipn, edg, k) - SVE(ipn, edg, 0) * E3(s.v_edg, ipn, edg, k);
+    const double cn = fabs(vn) * dt * rar;
+    const double b = C2(s.dp, ipx, k), cc = C2(s.dp, nbm1, k), d = C2(s.dp, nbp1, k);
+    const double q_min = dmin(dmin(gv, b), dmin(cc, d)), q_max = dmax(dmax(gv, b), dmax(cc, d));
+    const double qs = dmax(q_min, dmin(q_max, E3(s.dp_edg, ipn, edg, k)));
+    E3(s.dp_edg, ipn, edg, k) = qs;
+    if (vn > 0.) {
+      c_num_out_sum = c_num_out_sum + cn;
+    } else {
+      const double qn = dmin(qs, b), qx = dmax(qs, b);
+      c_num_in_sum = c_num_in_sum + cn;
+      q_min_in_min = dmin(q_min_in_min, qn);
+      q_max_in_max = dmax(q_max_in_max, qx);
+      cq_min_in_sum = cq_min_in_sum + cn * qn;
+      cq_max_in_sum = cq_max_in_sum + cn * qx;
     }
   }
-  __syncthreads();
+  const double q_min_next = dmin(gv, q_min_in_min), q_max_next = dmax(gv, q_max_in_max);
+  double qmo, qxo;
+  if (c_num_out_sum != 0.) {
+    const double at = ab.ab2 * to, av = ab.ab3 * tv;
+    qmo = (gv + ab.ab1 / dt * cq_max_in_sum - q_max_next * (1.0 + c_num_in_sum - c_num_out_sum) + at + av) /
+          c_num_out_sum / ab.ab1 * dt;
+    qxo = (gv + ab.ab1 / dt * cq_min_in_sum - q_min_next * (1.0 + c_num_in_sum - c_num_out_sum) + at + av) /
+          c_num_out_sum / ab.ab1 * dt;
+  } else {
+    qmo = -max_out_bound;
+    qxo = max_out_bound;
+  }
+  C2(s.r_plus, ipn, k) = qmo;
+  C2(s.r_mnus, ipn, k) = qxo;
+}
+
+// k_massadv2: the last limiter clamp of each edge value by the upwind
+// column's outflow bounds (flux_limiter.F90:244-258), massflux_newdepth
+// (massadv_compute.F90:22-70: mass flux, dp tendency, Adams-Bashforth depth
+// update kept non-negative) and update_pres.  The surface pressure the
+// reference sums there is a local of mass_adv that nothing reads, so it is
+// not formed.
+__global__ void __launch_bounds__(256) k_massadv2(DevGrid g, DevState s, AB ab) {
+  __shared__ double sh_dp[COLS][NVL_MAX + 1];
+  __shared__ double sh_lt[COLS][NVL_MAX + 1];
+  __shared__ double sh_pr[COLS][NVL_MAX + 1];
+  __shared__ double sh_om[COLS][NVL_MAX + 1];
+  const int nip = g.nip, nvl = g.nvl;
+  const int k = threadIdx.x, c = threadIdx.y;
+  const int ipn = wave_col();
+  const bool live = ipn < nip && k < nvl;
+  const int nf = ab.nf - 1, of = ab.of - 1, vof = ab.vof - 1;
   if (live) {
-    const double p = sh_pr[c][k];
-    L2(s.pr, ipn, k) = p;
-    L2(s.ex, ipn, k) = CP * fim_pow(p / P1000, RD / CP);
     const int np = GNP(ipn);
-    double dpdx = 0., dpdy = 0.;
+    double acc = 0.;
     for (int edg = 0; edg < np; edg++) {
-      dpdx = dpdx + E3(s.lp_edg, ipn, edg, k) * SVC(ipn, edg, 1);
-      dpdy = dpdy - E3(s.lp_edg, ipn, edg, k) * SVC(ipn, edg, 0);
+      const int ipx = PROX(ipn, edg);
+      const double vn = SVE(ipn, edg, 1) * E3(s.u_edg, ipn, edg, k) - SVE(ipn, edg, 0) * E3(s.v_edg, ipn, edg, k);
+      const int up = vn > 0. ? ipn : ipx;
+      const double qs = dmax(C2(s.r_plus, up, k), dmin(C2(s.r_mnus, up, k), E3(s.dp_edg, ipn, edg, k)));
+      E3(s.dp_edg, ipn, edg, k) = qs;
+      const double m = vn * qs;
+      MF(s.massfx, nf, ipn, edg, k) = m;
+      acc = acc + m;
     }
-    C2(s.ws, ipn, k) = sh_om[c][k] + (C2(s.us, ipn, k) * dpdx + C2(s.vs, ipn, k) * dpdy) * GV(rarea, ipn);
+    if (np < NPP) MF(s.massfx, nf, ipn, NPP - 1, k) = 0.;
+    const double dtn = -acc * GV(rarea, ipn);
+    AB3(s.dp_tdcy, nf, ipn, k) = dtn;
+    const double to = AB3(s.dp_tdcy, of, ipn, k), tv = AB3(s.dp_tdcy, vof, ipn, k);
+    const double d = C2(s.dp, ipn, k) + ab.ab1 * dtn + ab.ab2 * to + ab.ab3 * tv;
+    const double dpn = dmax(d, 0.0);
+    C2(s.dp, ipn, k) = dpn;
+    sh_dp[c][k] = dpn;
+    sh_lt[c][k] = (ab.ab1 * dtn + ab.ab2 * to + ab.ab3 * tv) / g.dt;
   }
+  update_pres_col(g, s, ipn, live, sh_dp[c], sh_lt[c], sh_pr[c], sh_om[c]);
 }
 
 // ---------------------------------------------------------------------------
@@ -513,6 +626,10 @@ void launch_cnuity(const DevGrid &g, const DevState &s, const AB &ab, hipStream_
   KT_LAUNCH("k_cnuity12", st, k_cnuity12<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
   KT_LAUNCH("k_cnuity3", st, k_cnuity3<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
   KT_LAUNCH("k_cnuity45", st, k_cnuity45<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
+}
+void launch_mass_adv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st) {
+  KT_LAUNCH("k_massadv1", st, k_massadv1<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
+  KT_LAUNCH("k_massadv2", st, k_massadv2<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
 }
 void launch_trcadv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st) {
   dim3 grid((g.nip + COLS - 1) / COLS, g.ntra);

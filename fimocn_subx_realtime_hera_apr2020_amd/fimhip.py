@@ -31,7 +31,8 @@ FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 
 # symbols declared in include/fimhip.h
 EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab", "fimhip_edgvar1_r8",
-           "fimhip_edgvar3_r8", "fimhip_cnuity_r8", "fimhip_trcadv_r8", "fimhip_momtum_r8",
+           "fimhip_edgvar3_r8", "fimhip_cnuity_r8", "fimhip_mass_adv_r8", "fimhip_set_flux_scheme",
+           "fimhip_trcadv_r8", "fimhip_momtum_r8",
            "fimhip_hybgen_r8", "fimhip_hystat_r8", "fimhip_field_size", "fimhip_upload", "fimhip_download",
            "fimhip_init_state", "fimhip_step", "fimhip_sync", "fimhip_globsum", "fimhip_icos_grid",
            "fimhip_synth_state", "fimhip_synth_device", "fimhip_profile", "fimhip_profile_report",
@@ -78,6 +79,8 @@ def load_library() -> ct.CDLL:
     lib.fimhip_edgvar1_r8.argtypes = [vp, ct.c_int] + [_PD] * 16
     lib.fimhip_edgvar3_r8.argtypes = [vp, ct.c_int] + [_PD] * 16
     lib.fimhip_cnuity_r8.argtypes = [vp, ct.c_int] + [_PD] * 13 + [ct.c_int]
+    lib.fimhip_mass_adv_r8.argtypes = [vp, ct.c_int] + [_PD] * 12 + [ct.c_int]
+    lib.fimhip_set_flux_scheme.argtypes = [vp, ct.c_int]
     lib.fimhip_trcadv_r8.argtypes = [vp, ct.c_int] + [_PD] * 7 + [ct.c_int]
     lib.fimhip_momtum_r8.argtypes = [vp, ct.c_int] + [_PD] * 13
     lib.fimhip_hybgen_r8.argtypes = [vp, ct.c_int] + [_PD] * 8 + [ct.c_int]
@@ -242,6 +245,15 @@ class Instance:
                dplo_tndcy, massfx, omega, TimingBarriers=False):
         _chk(self.lib.fimhip_cnuity_r8(self.h, its, *map(_p, (u_velo, v_velo, u_edg, v_edg, dp_edg, lp_edg,
              delp, pres, exner, dp_tndcy, dplo_tndcy, massfx, omega)), int(TimingBarriers)), "cnuity")
+
+    def mass_adv(self, its, u_velo, v_velo, u_edg, v_edg, dp_edg, lp_edg, delp, pres, exner, dp_tndcy,
+                 massfx, omega, TimingBarriers=False):
+        _chk(self.lib.fimhip_mass_adv_r8(self.h, its, *map(_p, (u_velo, v_velo, u_edg, v_edg, dp_edg, lp_edg,
+             delp, pres, exner, dp_tndcy, massfx, omega)), int(TimingBarriers)), "mass_adv")
+
+    def set_flux_scheme(self, flux_conserv_schm: int):
+        """1 = cnuity (FCT), 2 = mass_adv, for the steps that follow (dyn_run.F90:377-386)."""
+        _chk(self.lib.fimhip_set_flux_scheme(self.h, int(flux_conserv_schm)), "set_flux_scheme")
 
     def trcadv(self, its, trcr_edg, tracr, trcdp, trc_tdcy, trclo_tdcy, massfx, delp, TimingBarriers=False):
         _chk(self.lib.fimhip_trcadv_r8(self.h, its, *map(_p, (trcr_edg, tracr, trcdp, trc_tdcy, trclo_tdcy,

@@ -79,6 +79,14 @@ int fimhip_cnuity_r8(fimhip_ctx *ctx, int its, const double *u_velo, const doubl
                      const double *lp_edg, double *delp, double *pres, double *exner,
                      double *dp_tndcy, double *dplo_tndcy, double *massfx, double *omega,
                      int TimingBarriers);
+/* massadv.F90:21 (mass_adv: the flux-limited mass transport run when
+ * flux_conserv_schm = 2, dyn_run.F90:377-386).  dp_edg is limited in place
+ * (flux_limiter.F90:132-260); dplo_tndcy is not an argument, as in the
+ * reference. */
+int fimhip_mass_adv_r8(fimhip_ctx *ctx, int its, const double *u_velo, const double *v_velo,
+                       const double *u_edg, const double *v_edg, double *dp_edg, const double *lp_edg,
+                       double *delp, double *pres, double *exner, double *dp_tndcy, double *massfx,
+                       double *omega, int TimingBarriers);
 /* trcadv.F90:19 */
 int fimhip_trcadv_r8(fimhip_ctx *ctx, int its, const double *trcr_edg, double *tracr, double *trcdp,
                      double *trc_tdcy, double *trclo_tdcy, const double *massfx, const double *delp,
@@ -117,6 +125,9 @@ int fimhip_init_state(fimhip_ctx *ctx, int init_hybgen);
  * hystat: dyn_run.F90:119-425); mode 1 = transport only (abstart, edgvar1,
  * edgvar3, cnuity, trcadv) */
 int fimhip_step(fimhip_ctx *ctx, int its, int mode);
+/* the namelist switch flux_conserv_schm (fimnamelist.F90; dyn_run.F90:377-386)
+ * for fimhip_step: 1 = cnuity (FCT, the default), 2 = mass_adv */
+int fimhip_set_flux_scheme(fimhip_ctx *ctx, int flux_conserv_schm);
 int fimhip_sync(fimhip_ctx *ctx);
 /* globsum.F90:70-101: column mass (sum area*dp/g) and tracer totals, FP64 */
 int fimhip_globsum(fimhip_ctx *ctx, double *mass, double *trtot);

@@ -55,7 +55,7 @@ class _State(ct.Structure):
 
 def _lib():
     lib = ct.CDLL(build())
-    for fn in ("fo_edgvar1", "fo_edgvar3", "fo_cnuity", "fo_trcadv", "fo_momtum"):
+    for fn in ("fo_edgvar1", "fo_edgvar3", "fo_cnuity", "fo_mass_adv", "fo_trcadv", "fo_momtum"):
         getattr(lib, fn).argtypes = [ct.POINTER(_Grid), ct.POINTER(_State)]
     lib.fo_hybgen.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), ct.c_int]
     lib.fo_hystat.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), ct.c_int]

@@ -213,3 +213,99 @@ def test_g5_transport_dt480_matches_checker_bitwise(oracle_lib):
         assert abs(gm - om) <= 1e-13 * abs(om)
     finally:
         m.close()
+
+
+def test_mass_adv_dropin_matches_checker_bitwise(case):
+    """The flux-limited mass transport (flux_conserv_schm = 2) through its
+    drop-in entry, between edgvar3 and trcadv as dyn_run calls it: the limited
+    edge depths, thickness, pressure, Exner function, tendency, mass fluxes
+    and omega all have the checker's bytes."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    O, g, st, th, ds, ga = case
+    print("case: G3 nvl=64 ntra=4, mass_adv drop-in", flush=True)
+    o = O.OracleRun(ga, NVL, NTRA, DT, pow_kind=1, flux_scheme=2)
+    o.load(st)
+    o.init(True)
+    inst = fimhip.Instance(g, NVL, NTRA, DT, th, ds)
+    try:
+        a = {k: v.copy() for k, v in o.a.items()}
+        for its in (1, 2, 3, 4):
+            inst.abstart(its)
+            o.abstart(its)
+            geop_edg = np.zeros_like(a["u_edg"])
+            args = [a["us"], a["vs"], a["dp"], a["pr"], a["ex"], a["ph"], a["mp"], a["tr"], a["u_edg"],
+                    a["v_edg"], a["dp_edg"], a["lp_edg"], geop_edg, a["mont_edg"], a["uvsq_edg"], a["trc_edg"]]
+            inst.edgvar1(its, *args)
+            o.call("edgvar1")
+            inst.edgvar3(its, *args)
+            o.call("edgvar3")
+            assert _same(a["dp_edg"], o.a["dp_edg"]), ("edgvar3", its)
+            inst.mass_adv(its, a["us"], a["vs"], a["u_edg"], a["v_edg"], a["dp_edg"], a["lp_edg"], a["dp"],
+                          a["pr"], a["ex"], a["dp_tdcy"], a["massfx"], a["ws"])
+            o.call("mass_adv")
+            for k in ("dp_edg", "dp", "pr", "ex", "dp_tdcy", "massfx", "ws"):
+                assert _same(a[k], o.a[k]), ("mass_adv", its, k)
+            inst.trcadv(its, a["trc_edg"], a["tr"], a["trdp"], a["trc_tdcy"], a["trl_tdcy"], a["massfx"], a["dp"])
+            o.call("trcadv")
+            for k in ("tr", "trdp", "trc_tdcy", "trl_tdcy"):
+                assert _same(a[k], o.a[k]), ("trcadv", its, k)
+    finally:
+        inst.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mass_adv_steps_match_checker_bitwise(case, mode):
+    """Start-up plus 12 device-resident steps with flux_conserv_schm = 2
+    against the checker run with the same scheme: every field identical."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    O, g, st, th, ds, ga = case
+    print("case: G3 nvl=64 ntra=4 mode=%d steps=12 mass_adv" % mode, flush=True)
+    o = O.OracleRun(ga, NVL, NTRA, DT, pow_kind=1, flux_scheme=2)
+    o.load(st)
+    o.init(True)
+    m = fimhip.Model(g, NVL, NTRA, DT, th, ds)
+    try:
+        m.set_flux_scheme(2)
+        m.load(st)
+        m.init_state(True)
+        for its in range(1, 13):
+            m.step(its, mode)
+            o.step(its, mode)
+        m.sync()
+        fields = ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp", "ws", "massfx", "dp_tdcy", "dp_edg",
+                  "trc_tdcy", "trl_tdcy")
+        for f in fields:
+            got = m.download(f)
+            assert _same(got, o.a[f]), (f, rel_l2(got, o.a[f]))
+    finally:
+        m.close()
+
+
+def test_g5_mass_adv_transport_dt480_matches_checker_bitwise(oracle_lib):
+    """G5 x 64 layers, 4 tracers, 10 transport-only steps at dt = 480 s with
+    the flux-limited mass transport: every field identical to the checker,
+    and no layer thickness negative."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+    O = oracle_lib
+    g = fimhip.icos_grid(5)
+    th, ds = state.vertical_tables(NVL)
+    dt = 480.0
+    print("case: G5 nvl=64 ntra=4 dt=480 10 transport steps mass_adv", flush=True)
+    m = fimhip.Model(g, NVL, NTRA, dt, th, ds)
+    try:
+        m.set_flux_scheme(2)
+        m.synth(seed=20240601, member=0, flags=0)
+        st = {f: m.download(f) for f in ("us", "vs", "dp", "pr", "ex", "ph", "tr")}
+        m.init_state(False)
+        o = O.OracleRun(O.grid_arrays(g, NVL, dt, th, ds), NVL, NTRA, dt, pow_kind=1, flux_scheme=2)
+        o.load(st)
+        o.init(False)
+        for its in range(1, 11):
+            m.step(its, 1)
+            o.step(its, 1)
+        m.sync()
+        for f in ("us", "vs", "dp", "pr", "ex", "tr", "trdp", "massfx", "ws", "dp_edg"):
+            assert _same(m.download(f), o.a[f]), f
+        assert m.download("dp").min() >= 0.
+    finally:
+        m.close()
