@@ -26,6 +26,7 @@ struct fimhip_ens {
   DevState stage{};                  // one member in FP64 (synth / load staging)
   std::vector<double> lat, lon;
   int tile[EK_NKERNELS];
+  int flux_conserv_schm = 1;         // 1 = cnuity (FCT), 2 = mass_adv
   // globsum partials
   int gs_blocks = 0;
   double *gs64 = nullptr;
@@ -250,7 +251,12 @@ static void enqueue_step(fimhip_ens *e, int mode, bool side) {
     ens_launch(EK_EDGVAR1, e->tile[EK_EDGVAR1], g, s, st);
     ens_launch(EK_EDGVAR3, e->tile[EK_EDGVAR3], g, s, st);
   }
-  for (int k = EK_CNUITY12; k <= EK_CNUITY45; k++) ens_launch(k, e->tile[k], g, s, st);
+  if (e->flux_conserv_schm == 2) {
+    ens_launch(EK_MASSADV1, e->tile[EK_MASSADV1], g, s, st);
+    ens_launch(EK_MASSADV2, e->tile[EK_MASSADV2], g, s, st);
+  } else {
+    for (int k = EK_CNUITY12; k <= EK_CNUITY45; k++) ens_launch(k, e->tile[k], g, s, st);
+  }
   for (int k = EK_TRCADV1; k <= EK_TRCADV3; k++) ens_launch(k, e->tile[k], g, s, st);
   // momtum needs nothing of the tracer transport; it runs beside hybgen's
   // column kernels up to the remap (latency-bound, so the two share the
@@ -290,6 +296,13 @@ extern "C" int fimhip_ens_step(fimhip_ens *e, int nsteps, int mode) {
     e->graph_mode = mode;
   }
   for (int i = 0; i < nsteps; i++) HIPCHK(hipGraphLaunch(e->exec, e->stream));
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_ens_set_flux_scheme(fimhip_ens *e, int flux_conserv_schm) {
+  if (!e || (flux_conserv_schm != 1 && flux_conserv_schm != 2)) return FIMHIP_EINVAL;
+  if (e->flux_conserv_schm != flux_conserv_schm) drop_graph(e);
+  e->flux_conserv_schm = flux_conserv_schm;
   return FIMHIP_OK;
 }
 

@@ -57,7 +57,7 @@ inline bool tile_swz(int t) { return (t >> 2) & 1; }
 
 enum EnsKernel {
   EK_EDGVAR1 = 0, EK_EDGVAR3, EK_CNUITY12, EK_CNUITY3, EK_CNUITY45, EK_TRCADV1, EK_TRCADV2,
-  EK_TRCADV3, EK_MOMTUM, EK_HYSTAT, EK_NKERNELS
+  EK_TRCADV3, EK_MOMTUM, EK_HYSTAT, EK_MASSADV1, EK_MASSADV2, EK_NKERNELS
 };
 extern const char *const ens_kernel_names[EK_NKERNELS];
 

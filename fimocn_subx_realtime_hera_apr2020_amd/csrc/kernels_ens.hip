@@ -14,7 +14,7 @@ namespace fimhip {
 
 const char *const ens_kernel_names[EK_NKERNELS] = {
     "e_edgvar1", "e_edgvar3", "e_cnuity12", "e_cnuity3", "e_cnuity45",
-    "e_trcadv1", "e_trcadv2", "e_trcadv3", "e_momtum", "e_hystat"};
+    "e_trcadv1", "e_trcadv2", "e_trcadv3", "e_momtum", "e_hystat", "e_massadv1", "e_massadv2"};
 
 #define EC(a, col, k) (a)[(size_t)(col) * nvl + (k)]
 #define EL(a, col, k) (a)[(size_t)(col) * nvlp1 + (k)]
@@ -316,6 +316,35 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity3(EnsGrid g, EnsState s) 
   EC(s.r_mnus, col, k) = fmaxf(0.f, fminf(1.f, qm * frcp(fmaxf(pm, thshld))));
 }
 
+// cnuity5 (cnuity_compute.F90:376-451), also update_pres of mass_adv
+// (massadv_compute.F90:73-171): from the new thickness dpn and the layer
+// tendency lt of each lane, the top-down pressure and the column tendency as
+// wavefront suffix sums (FP64), then the Exner function and omega.  Every
+// lane of the wavefront calls it (lanes at or above nvl with live = false).
+template <int NP>
+__device__ __forceinline__ void e_update_pres_col(const EnsGrid &g, const EnsState &s, int ipn, int col, int k,
+                                                  bool live, double dpn, double lt) {
+  const int nvl = g.d.nvl, nvlp1 = nvl + 1;
+  const double sp = wave_suffix_sum(dpn, k);
+  const double sl = wave_suffix_sum(lt, k);
+  if (!live) return;
+  float l[NP];
+#pragma unroll
+  for (int e = 0; e < NP; e++) l[e] = EE(s.lp_edg, col, e, k);
+  const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k);
+  const double p = EL(s.pr, col, nvl) + sp;
+  EL(s.pr, col, k) = p;
+  EL(s.ex, col, k) = CP * fim_pow_fast(p / P1000, RD / CP);
+  float dpdx = 0.f, dpdy = 0.f;
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    dpdx = dpdx + l[e] * FSVC(ipn, e, 1);
+    dpdy = dpdy - l[e] * FSVC(ipn, e, 0);
+  }
+  const float om = (float)(sl - .5 * lt);
+  EC(s.ws, col, k) = om + (ui * dpdx + vi * dpdy) * GF(rarea, ipn);
+}
+
 // cnuity4 + cnuity5 (cnuity_compute.F90:261-451).  The thickness tendency is
 // formed in FP64 from the final (antisymmetric) FP32 fluxes, the thickness is
 // advanced in FP64, and the top-down pressure sum and the column tendency for
@@ -323,7 +352,7 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity3(EnsGrid g, EnsState s) 
 template <int NP>
 __device__ __forceinline__ void e_cnuity45_col(const EnsGrid &g, const EnsState &s, const StepCtl &c, int ipn,
                                                int base, int k) {
-  const int nvl = g.d.nvl, nvlp1 = nvl + 1, ncol = s.ncol;
+  const int nvl = g.d.nvl, ncol = s.ncol;
   const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
   const int col = base + ipn;
   const bool live = k < nvl;
@@ -356,24 +385,7 @@ __device__ __forceinline__ void e_cnuity45_col(const EnsGrid &g, const EnsState 
     EC(s.dp, col, k) = dpn;
     lt = inc / g.d.dt;
   }
-  const double sp = wave_suffix_sum(dpn, k);
-  const double sl = wave_suffix_sum(lt, k);
-  if (!live) return;
-  float l[NP];
-#pragma unroll
-  for (int e = 0; e < NP; e++) l[e] = EE(s.lp_edg, col, e, k);
-  const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k);
-  const double p = EL(s.pr, col, nvl) + sp;
-  EL(s.pr, col, k) = p;
-  EL(s.ex, col, k) = CP * fim_pow_fast(p / P1000, RD / CP);
-  float dpdx = 0.f, dpdy = 0.f;
-#pragma unroll
-  for (int e = 0; e < NP; e++) {
-    dpdx = dpdx + l[e] * FSVC(ipn, e, 1);
-    dpdy = dpdy - l[e] * FSVC(ipn, e, 0);
-  }
-  const float om = (float)(sl - .5 * lt);
-  EC(s.ws, col, k) = om + (ui * dpdx + vi * dpdy) * GF(rarea, ipn);
+  e_update_pres_col<NP>(g, s, ipn, col, k, live, dpn, lt);
 }
 
 template <int COLS, bool SWZ>
@@ -385,6 +397,138 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity45(EnsGrid g, EnsState s)
   const int base = blockIdx.y * nip;
   if (GNP(ipn) == 6) e_cnuity45_col<6>(g, s, c, ipn, base, k);
   else e_cnuity45_col<5>(g, s, c, ipn, base, k);
+}
+
+// ---------------------------------------------------------------------------
+// mass_adv, the flux-limited mass transport of flux_conserv_schm = 2
+// (massadv.F90:21-160; flux_limiter.F90:78-260, M1 form;
+// massadv_compute.F90:22-171).  The normal wind of an edge is the lower
+// index cell's, negated on the other side, and the edge depth is the average
+// of both cells' reconstructions clamped to the four cells around the edge
+// (q_star: the same bits from both sides), so the final limited edge depth,
+// taken from the upwind cell's bounds, is the same on both sides and the
+// FP32 mass flux vnorm * depth cancels exactly across the edge.
+//
+// massadv1: q_star (kept in antifx) and the column's outflow bounds
+// q_min_out / q_max_out (in r_plus / r_mnus).  The inflow sums and the bound
+// formula are FP64.
+template <int NP>
+__device__ __forceinline__ void e_massadv1_col(const EnsGrid &g, const EnsState &s, const StepCtl &c, int ipn,
+                                               int base, int k) {
+  const int nvl = g.d.nvl, ncol = s.ncol;
+  const int of = c.of - 1, vof = c.vof - 1;
+  const int col = base + ipn;
+  const double gv = EC(s.dp, col, k);
+  const float gvf = (float)gv;
+  const double t_of = EAB(s.dp_tdcy, of, col, k), t_vof = EAB(s.dp_tdcy, vof, col, k);
+  int x[NP];
+  float vn[NP], vx[NP], de[NP], dx[NP], dpx[NP];
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    x[e] = GPROX(ipn, e);
+    const int xc = base + x[e], ex = GPROXS(ipn, e);
+    vn[e] = EE(s.vn_edg, col, e, k);
+    vx[e] = EE(s.vn_edg, xc, ex, k);
+    de[e] = EE(s.dp_edg, col, e, k);
+    dx[e] = EE(s.dp_edg, xc, ex, k);
+    dpx[e] = (float)EC(s.dp, xc, k);
+  }
+  const float cdt = (float)(g.d.dt * GD(rarea, ipn));
+  double c_in = 0., c_out = 0., cq_min = 0., cq_max = 0.;
+  float q_min_in = 3.402823466e38f, q_max_in = 0.f;    // huge(), 0
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    const float vnorm = ipn < x[e] ? vn[e] : -vx[e];
+    const float cn = fabsf(vnorm) * cdt;
+    const float nm = dpx[(e + NP - 1) % NP], np1 = dpx[(e + 1) % NP];
+    const float lo4 = fminf(fminf(gvf, dpx[e]), fminf(nm, np1)), hi4 = fmaxf(fmaxf(gvf, dpx[e]), fmaxf(nm, np1));
+    const float qs = fmaxf(lo4, fminf(hi4, .5f * (de[e] + dx[e])));
+    EE(s.antifx, col, e, k) = qs;
+    if (vnorm > 0.f) {
+      c_out = c_out + (double)cn;
+    } else {
+      const float qn = fminf(qs, dpx[e]), qx = fmaxf(qs, dpx[e]);
+      c_in = c_in + (double)cn;
+      q_min_in = fminf(q_min_in, qn);
+      q_max_in = fmaxf(q_max_in, qx);
+      cq_min = cq_min + (double)cn * (double)qn;
+      cq_max = cq_max + (double)cn * (double)qx;
+    }
+  }
+  float qmo = -100000.f, qxo = 100000.f;
+  if (c_out != 0.) {
+    const double q_min_next = fmin(gv, (double)q_min_in), q_max_next = fmax(gv, (double)q_max_in);
+    const double rest = gv + c.ab2 * t_of + c.ab3 * t_vof, keep = 1.0 + c_in - c_out;
+    const double w = g.d.dt / (c_out * c.ab1), a = c.ab1 / g.d.dt;
+    qmo = (float)((rest + a * cq_max - q_max_next * keep) * w);
+    qxo = (float)((rest + a * cq_min - q_min_next * keep) * w);
+  }
+  EC(s.r_plus, col, k) = qmo;
+  EC(s.r_mnus, col, k) = qxo;
+}
+
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_massadv1(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip, nvl = g.d.nvl;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip || k >= nvl) return;
+  const StepCtl c = load_ctl(s.ctl);
+  const int base = blockIdx.y * nip;
+  if (GNP(ipn) == 6) e_massadv1_col<6>(g, s, c, ipn, base, k);
+  else e_massadv1_col<5>(g, s, c, ipn, base, k);
+}
+
+// massadv2: the final clamp by the upwind column's bounds, the mass flux,
+// the FP64 thickness tendency and Adams-Bashforth update kept non-negative
+// (massflux_newdepth), and update_pres.
+template <int NP>
+__device__ __forceinline__ void e_massadv2_col(const EnsGrid &g, const EnsState &s, const StepCtl &c, int ipn,
+                                               int base, int k) {
+  const int nvl = g.d.nvl, ncol = s.ncol;
+  const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
+  const int col = base + ipn;
+  const bool live = k < nvl;
+  double dpn = 0., lt = 0.;
+  if (live) {
+    const double t_of = EAB(s.dp_tdcy, of, col, k), t_vof = EAB(s.dp_tdcy, vof, col, k), dp0 = EC(s.dp, col, k);
+    float vnorm[NP], qs[NP], lo[NP], hi[NP];
+#pragma unroll
+    for (int e = 0; e < NP; e++) {
+      const int x = GPROX(ipn, e), xc = base + x, ex = GPROXS(ipn, e);
+      const float vn = EE(s.vn_edg, col, e, k), vx = EE(s.vn_edg, xc, ex, k);
+      vnorm[e] = ipn < x ? vn : -vx;
+      qs[e] = EE(s.antifx, col, e, k);
+      const size_t up = (size_t)(vnorm[e] > 0.f ? col : xc) * nvl + k;
+      lo[e] = s.r_plus[up];
+      hi[e] = s.r_mnus[up];
+    }
+    double at = 0.;
+#pragma unroll
+    for (int e = 0; e < NP; e++) {
+      const float mf = vnorm[e] * fmaxf(lo[e], fminf(hi[e], qs[e]));
+      EE(s.massfx, col, e, k) = mf;
+      at = at + (double)mf;
+    }
+    if (NP < NPP) EE(s.massfx, col, NPP - 1, k) = 0.f;
+    const double dtn = -at * GD(rarea, ipn);
+    EAB(s.dp_tdcy, nf, col, k) = dtn;
+    const double inc = c.ab1 * dtn + c.ab2 * t_of + c.ab3 * t_vof;
+    dpn = fmax(dp0 + inc, 0.);
+    EC(s.dp, col, k) = dpn;
+    lt = inc / g.d.dt;
+  }
+  e_update_pres_col<NP>(g, s, ipn, col, k, live, dpn, lt);
+}
+
+template <int COLS, bool SWZ>
+__global__ void __launch_bounds__(64 * COLS) k_e_massadv2(EnsGrid g, EnsState s) {
+  const int nip = g.d.nip;
+  const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
+  if (ipn >= nip) return;
+  const StepCtl c = load_ctl(s.ctl);
+  const int base = blockIdx.y * nip;
+  if (GNP(ipn) == 6) e_massadv2_col<6>(g, s, c, ipn, base, k);
+  else e_massadv2_col<5>(g, s, c, ipn, base, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -903,6 +1047,8 @@ void ens_launch(int kernel, int tile, const EnsGrid &g, const EnsState &s, hipSt
     case EK_TRCADV2: ENS_DISPATCH(k_e_trcadv2, 1); break;
     case EK_TRCADV3: ENS_DISPATCH(k_e_trcadv3, 1); break;
     case EK_MOMTUM: ENS_DISPATCH(k_e_momtum, 1); break;
+    case EK_MASSADV1: ENS_DISPATCH(k_e_massadv1, 1); break;
+    case EK_MASSADV2: ENS_DISPATCH(k_e_massadv2, 1); break;
     default: ENS_DISPATCH(k_e_hystat, 1); break;
   }
   if (::fimhip::g_ktimer) kt_end(st);

@@ -38,7 +38,8 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_synth_state", "fimhip_synth_device", "fimhip_profile", "fimhip_profile_report",
            "fimhip_pow", "fimhip_version", "fimhip_ens_create", "fimhip_ens_destroy", "fimhip_ens_synth",
            "fimhip_ens_load_member", "fimhip_ens_init_state", "fimhip_ens_step", "fimhip_ens_use_graph",
-           "fimhip_ens_sync", "fimhip_ens_field_size", "fimhip_ens_download", "fimhip_ens_globsum",
+           "fimhip_ens_set_flux_scheme", "fimhip_ens_sync", "fimhip_ens_field_size", "fimhip_ens_download",
+           "fimhip_ens_globsum",
            "fimhip_ens_kernel_count", "fimhip_ens_kernel_name", "fimhip_ens_set_tile", "fimhip_ens_get_tile",
            "fimhip_ens_autotune", "fimhip_ens_profile", "fimhip_ens_profile_report"]
 
@@ -111,6 +112,7 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ens_init_state.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_step.argtypes = [vp, ct.c_int, ct.c_int]
     lib.fimhip_ens_use_graph.argtypes = [vp, ct.c_int]
+    lib.fimhip_ens_set_flux_scheme.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_sync.argtypes = [vp]
     lib.fimhip_ens_field_size.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_field_size.restype = ct.c_longlong
@@ -377,6 +379,10 @@ class Ensemble:
 
     def use_graph(self, on: bool):
         _chk(self.lib.fimhip_ens_use_graph(self.h, 1 if on else 0), "ens_use_graph")
+
+    def set_flux_scheme(self, flux_conserv_schm: int):
+        """1 = cnuity (FCT), 2 = mass_adv, for the steps that follow (dyn_run.F90:377-386)."""
+        _chk(self.lib.fimhip_ens_set_flux_scheme(self.h, int(flux_conserv_schm)), "ens_set_flux_scheme")
 
     def sync(self):
         _chk(self.lib.fimhip_ens_sync(self.h), "ens_sync")

@@ -187,6 +187,10 @@ int fimhip_ens_init_state(fimhip_ens *ens, int init_hybgen);
  * captured once as a HIP graph and replayed (mode 0 full, 1 transport only) */
 int fimhip_ens_step(fimhip_ens *ens, int nsteps, int mode);
 int fimhip_ens_use_graph(fimhip_ens *ens, int on);
+/* flux_conserv_schm for the steps that follow (dyn_run.F90:377-386): 1 = FCT
+ * cnuity (default), 2 = mass_adv (massadv.F90:21-160), with the edge's normal
+ * wind taken from its lower-index cell so the FP32 mass fluxes still cancel */
+int fimhip_ens_set_flux_scheme(fimhip_ens *ens, int flux_conserv_schm);
 int fimhip_ens_sync(fimhip_ens *ens);
 /* per-member element count of FIMHIP_US, VS, DP, PR, EX, PH, MP, TR, TRDP, WS, RELVOR, SDOT, PSRF */
 long long fimhip_ens_field_size(const fimhip_ens *ens, int field);
