@@ -48,6 +48,8 @@ struct fimhip_ctx {
   std::vector<double> lat, lon;      // host copies for the synthetic-state profiles
   KTimer *kt = nullptr;              // non-null while profiling
   int flux_conserv_schm = 1;         // 1 = cnuity (FCT), 2 = mass_adv
+  int dtratio = 6;                   // class-B / class-A time step ratio
+  DevClassB cb{};                    // class-B tracer transport (cb.ntrb = 0: off)
 };
 
 // dyn_init.F90:125 taper for smoo_coeff
@@ -205,6 +207,35 @@ extern "C" int fimhip_set_ab(fimhip_ctx *c, int nf, int of, int vof, double a1, 
   return FIMHIP_OK;
 }
 
+extern "C" int fimhip_set_class_b(fimhip_ctx *c, int ntrb, int dtratio) {
+  if (!c || ntrb < 0 || ntrb > NTRA_MAX || dtratio < 1) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  c->dtratio = dtratio;
+  if (ntrb == c->cb.ntrb) return FIMHIP_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  DevClassB b{};
+  b.ntrb = ntrb;
+  if (ntrb > 0) {
+    // arrays of an earlier setting stay in the arena until the instance is destroyed
+    const size_t cz = (size_t)c->nip * c->nvl, ez = cz * NPP, tz = cz * ntrb;
+    struct { double **p; size_t n; } arrs[] = {
+        {&b.trb, tz}, {&b.cumufx, ez}, {&b.dpinit, cz}, {&b.w, cz}, {&b.fld, tz}, {&b.vfx, tz},
+        {&b.fmx, tz}, {&b.fmn, tz}, {&b.fld1, tz}, {&b.flp, tz}, {&b.fln, tz}, {&b.uan, ez * ntrb}};
+    int r;
+    for (auto &a : arrs)
+      if ((r = c->arena.alloc(a.p, a.n))) return r;
+    double *part = nullptr;
+    if ((r = c->arena.alloc(&part, (size_t)c->gsum_blocks * (1 + c->ntra + ntrb)))) return r;
+    c->gsum_partial = part;
+    HIPCHK(hipMemsetAsync(b.trb, 0, tz * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(b.cumufx, 0, ez * sizeof(double), c->stream));
+    HIPCHK(hipMemcpyAsync(b.dpinit, c->s.dp, cz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  c->cb = b;
+  return FIMHIP_OK;
+}
+
 extern "C" int fimhip_set_flux_scheme(fimhip_ctx *c, int flux_conserv_schm) {
   if (!c || (flux_conserv_schm != 1 && flux_conserv_schm != 2)) return FIMHIP_EINVAL;
   c->flux_conserv_schm = flux_conserv_schm;
@@ -243,6 +274,9 @@ static double **field_ptr(fimhip_ctx *c, int f) {
     case FIMHIP_MONT_EDG: return &s.mont_edg;
     case FIMHIP_UVSQ_EDG: return &s.uvsq_edg;
     case FIMHIP_TRC_EDG: return &s.trc_edg;
+    case FIMHIP_TRB: return &c->cb.trb;
+    case FIMHIP_CUMUFX: return &c->cb.cumufx;
+    case FIMHIP_DPINIT: return &c->cb.dpinit;
   }
   return nullptr;
 }
@@ -264,6 +298,9 @@ extern "C" long long fimhip_field_size(const fimhip_ctx *c, int f) {
     case FIMHIP_U_EDG: case FIMHIP_V_EDG: case FIMHIP_DP_EDG: case FIMHIP_LP_EDG: case FIMHIP_MONT_EDG:
     case FIMHIP_UVSQ_EDG: return ez;
     case FIMHIP_TRC_EDG: return ez * c->ntra;
+    case FIMHIP_TRB: return cz * c->cb.ntrb;
+    case FIMHIP_CUMUFX: return c->cb.ntrb ? ez : 0;
+    case FIMHIP_DPINIT: return c->cb.ntrb ? cz : 0;
   }
   return FIMHIP_EINVAL;
 }
@@ -271,12 +308,14 @@ extern "C" long long fimhip_field_size(const fimhip_ctx *c, int f) {
 static int h2d(fimhip_ctx *c, int f, const double *h) {
   if (!h) return FIMHIP_EINVAL;
   double **p = field_ptr(c, f);
+  if (!*p) return FIMHIP_EINVAL;     // class-B fields before fimhip_set_class_b
   HIPCHK(hipMemcpyAsync(*p, h, fimhip_field_size(c, f) * sizeof(double), hipMemcpyHostToDevice, c->stream));
   return 0;
 }
 static int d2h(fimhip_ctx *c, int f, double *h) {
   if (!h) return FIMHIP_EINVAL;
   double **p = field_ptr(c, f);
+  if (!*p) return FIMHIP_EINVAL;
   HIPCHK(hipMemcpyAsync(h, *p, fimhip_field_size(c, f) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   return 0;
 }
@@ -382,6 +421,43 @@ extern "C" int fimhip_mass_adv_r8(fimhip_ctx *c, int its, const double *u_velo, 
   return finish(c);
 }
 
+// class-B transport drop-ins; the class-B tracers are the fields ntra+1..ntra+ntrb
+// of the caller's tracer array (nvl,nip,ntra+ntrb)
+extern "C" int fimhip_transp0_r8(fimhip_ctx *c, int its, double *cumufx, const double *dp3d, double *dpinit) {
+  (void)its;
+  if (!c || c->cb.ntrb < 1) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(h2d(c, FIMHIP_DP, dp3d));
+  launch_transp0(c->g, c->s, c->cb, c->stream);
+  TRY(d2h(c, FIMHIP_CUMUFX, cumufx)); TRY(d2h(c, FIMHIP_DPINIT, dpinit));
+  return finish(c);
+}
+
+extern "C" int fimhip_transp1_r8(fimhip_ctx *c, int its, int nf, int of, int vof, double adbash1, double adbash2,
+                                 double adbash3, double *cumufx, const double *massfx) {
+  (void)its;
+  if (!c || c->cb.ntrb < 1 || nf < 1 || nf > 3 || of < 1 || of > 3 || vof < 1 || vof > 3) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  TRY(h2d(c, FIMHIP_CUMUFX, cumufx)); TRY(h2d(c, FIMHIP_MASSFX, massfx));
+  launch_transp1(c->g, c->s, c->cb, AB{nf, of, vof, adbash1, adbash2, adbash3}, c->stream);
+  TRY(d2h(c, FIMHIP_CUMUFX, cumufx));
+  return finish(c);
+}
+
+extern "C" int fimhip_transp2_r8(fimhip_ctx *c, int its, double *tracr, const double *cumufx, const double *dpinit,
+                                 const double *dpfinl, int timing) {
+  (void)its; (void)timing;
+  if (!c || c->cb.ntrb < 1 || !tracr) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t tz = (size_t)c->nip * c->nvl * c->cb.ntrb;
+  double *trb_host = tracr + (size_t)c->nip * c->nvl * c->ntra;
+  HIPCHK(hipMemcpyAsync(c->cb.trb, trb_host, tz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  TRY(h2d(c, FIMHIP_CUMUFX, cumufx)); TRY(h2d(c, FIMHIP_DPINIT, dpinit)); TRY(h2d(c, FIMHIP_DP, dpfinl));
+  launch_transp2(c->g, c->s, c->cb, c->stream);
+  HIPCHK(hipMemcpyAsync(trb_host, c->cb.trb, tz * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return finish(c);
+}
+
 extern "C" int fimhip_trcadv_r8(fimhip_ctx *c, int its, const double *trcr_edg, double *tracr, double *trcdp,
                                 double *trc_tdcy, double *trclo_tdcy, const double *massfx, const double *delp,
                                 int timing) {
@@ -463,6 +539,7 @@ extern "C" int fimhip_init_state(fimhip_ctx *c, int init_hybgen) {
   HIPCHK(hipMemsetAsync(s.trl_tdcy, 0, 3 * cz * c->ntra * sizeof(double), c->stream));
   HIPCHK(hipMemsetAsync(s.massfx, 0, 3 * ez * sizeof(double), c->stream));
   c->ab = AB{0, 2, 1, 0., 0., 0.};
+  if (c->cb.ntrb > 0) launch_transp0(c->g, c->s, c->cb, c->stream);   // dyn_init.F90:1096-1098
   return finish(c);
 }
 
@@ -478,6 +555,13 @@ extern "C" int fimhip_step(fimhip_ctx *c, int its, int mode) {
   launch_edgvar3(c->g, c->s, c->stream);
   if (c->flux_conserv_schm == 2) launch_mass_adv(c->g, c->s, c->ab, c->stream);
   else launch_cnuity(c->g, c->s, c->ab, c->stream);
+  if (c->cb.ntrb > 0) {             // dyn_run.F90:388-405
+    launch_transp1(c->g, c->s, c->cb, c->ab, c->stream);
+    if (its % c->dtratio == 0) {
+      launch_transp2(c->g, c->s, c->cb, c->stream);
+      launch_transp0(c->g, c->s, c->cb, c->stream);
+    }
+  }
   launch_trcadv(c->g, c->s, c->ab, c->stream);
   if (mode == 0) {
     launch_momtum(c->g, c->s, c->ab, c->stream);
@@ -496,16 +580,18 @@ extern "C" int fimhip_sync(fimhip_ctx *c) {
 
 // globsum: fixed-order block partials (cells strided by the grid), summed on
 // the host in block order, so the result is reproducible run to run
-__global__ void __launch_bounds__(256) k_globsum(DevGrid g, DevState s, double *part) {
+__global__ void __launch_bounds__(256) k_globsum(DevGrid g, DevState s, const double *trb, int ntrb,
+                                                 double *part) {
   __shared__ double sh[256];
   const int nip = g.nip, nvl = g.nvl, ntra = g.ntra;
-  for (int q = 0; q <= ntra; q++) {
+  for (int q = 0; q <= ntra + ntrb; q++) {
+    const double *tq = q > ntra ? trb + (size_t)(q - 1 - ntra) * nip * nvl : s.tr + (size_t)(q > 0 ? q - 1 : 0) * nip * nvl;
     double acc = 0.;
     for (int ipn = blockIdx.x * 256 + threadIdx.x; ipn < nip; ipn += gridDim.x * 256) {
       double col = 0.;
       for (int k = 0; k < nvl; k++) {
         double v = g.area[ipn] * s.dp[(size_t)ipn * nvl + k];
-        if (q > 0) v = v * s.tr[((size_t)(q - 1) * nip + ipn) * nvl + k];
+        if (q > 0) v = v * tq[(size_t)ipn * nvl + k];
         col = col + v;
       }
       acc = acc + col;
@@ -524,11 +610,12 @@ __global__ void __launch_bounds__(256) k_globsum(DevGrid g, DevState s, double *
 extern "C" int fimhip_globsum(fimhip_ctx *c, double *mass, double *trtot) {
   if (!c || !mass) return FIMHIP_EINVAL;
   HIPCHK(hipSetDevice(c->device));
-  k_globsum<<<c->gsum_blocks, 256, 0, c->stream>>>(c->g, c->s, c->gsum_partial);
-  std::vector<double> h((size_t)c->gsum_blocks * (1 + c->ntra));
+  const int nq = c->ntra + c->cb.ntrb;
+  k_globsum<<<c->gsum_blocks, 256, 0, c->stream>>>(c->g, c->s, c->cb.trb, c->cb.ntrb, c->gsum_partial);
+  std::vector<double> h((size_t)c->gsum_blocks * (1 + nq));
   HIPCHK(hipMemcpyAsync(h.data(), c->gsum_partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   TRY(finish(c));
-  for (int q = 0; q <= c->ntra; q++) {
+  for (int q = 0; q <= nq; q++) {
     double acc = 0.;
     for (int b = 0; b < c->gsum_blocks; b++) acc += h[(size_t)q * c->gsum_blocks + b];
     if (q == 0) *mass = acc / GRVITY;

@@ -68,6 +68,18 @@ struct DevState {
   double *u2, *v2, *exsmo, *thbuf;
 };
 
+// Class-B tracer transport state (transp3d.F90, fct3d.F90): the tracers
+// (nvl,nip,ntrb), the time-integrated mass flux (nvl,6,nip), the thickness at
+// the start of the integration, the vertical mass flux (nvl,nip), and per
+// class-B tracer fct3d work: the filled field, vertical flux, bounds, the
+// low-order update, the limiter ratios (nvl,nip,ntrb each) and the
+// antidiffusive fluxes (nvl,6,nip,ntrb).
+struct DevClassB {
+  int ntrb;
+  double *trb, *cumufx, *dpinit, *w;
+  double *fld, *vfx, *fmx, *fmn, *fld1, *flp, *fln, *uan;
+};
+
 struct AB {
   int nf, of, vof;       // 1-based slots as in module_variables.F90
   double ab1, ab2, ab3;
@@ -126,6 +138,9 @@ void launch_edgvar1(const DevGrid &g, const DevState &s, hipStream_t st);
 void launch_edgvar3(const DevGrid &g, const DevState &s, hipStream_t st);
 void launch_cnuity(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st);
 void launch_mass_adv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st);
+void launch_transp0(const DevGrid &g, const DevState &s, const DevClassB &b, hipStream_t st);
+void launch_transp1(const DevGrid &g, const DevState &s, const DevClassB &b, const AB &ab, hipStream_t st);
+void launch_transp2(const DevGrid &g, const DevState &s, const DevClassB &b, hipStream_t st);
 void launch_trcadv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st);
 void launch_momtum(const DevGrid &g, DevState &s, const AB &ab, hipStream_t st);
 void launch_hybgen(const DevGrid &g, const DevState &s, int its, hipStream_t st);

@@ -26,12 +26,13 @@ LIBPATH = os.path.join(PKG, "libfimhip.so")
 
 FIELDS = ["us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp", "ws", "relvor", "sdot", "psrf", "ptdcy",
           "u_tdcy", "v_tdcy", "dp_tdcy", "dpl_tdcy", "trc_tdcy", "trl_tdcy", "massfx", "u_edg", "v_edg",
-          "dp_edg", "lp_edg", "mont_edg", "uvsq_edg", "trc_edg"]
+          "dp_edg", "lp_edg", "mont_edg", "uvsq_edg", "trc_edg", "trb", "cumufx", "dpinit"]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 
 # symbols declared in include/fimhip.h
 EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab", "fimhip_edgvar1_r8",
            "fimhip_edgvar3_r8", "fimhip_cnuity_r8", "fimhip_mass_adv_r8", "fimhip_set_flux_scheme",
+           "fimhip_set_class_b", "fimhip_transp0_r8", "fimhip_transp1_r8", "fimhip_transp2_r8",
            "fimhip_trcadv_r8", "fimhip_momtum_r8",
            "fimhip_hybgen_r8", "fimhip_hystat_r8", "fimhip_field_size", "fimhip_upload", "fimhip_download",
            "fimhip_init_state", "fimhip_step", "fimhip_sync", "fimhip_globsum", "fimhip_icos_grid",
@@ -82,6 +83,11 @@ def load_library() -> ct.CDLL:
     lib.fimhip_cnuity_r8.argtypes = [vp, ct.c_int] + [_PD] * 13 + [ct.c_int]
     lib.fimhip_mass_adv_r8.argtypes = [vp, ct.c_int] + [_PD] * 12 + [ct.c_int]
     lib.fimhip_set_flux_scheme.argtypes = [vp, ct.c_int]
+    lib.fimhip_set_class_b.argtypes = [vp, ct.c_int, ct.c_int]
+    lib.fimhip_transp0_r8.argtypes = [vp, ct.c_int, _PD, _PD, _PD]
+    lib.fimhip_transp1_r8.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_double,
+                                      ct.c_double, _PD, _PD]
+    lib.fimhip_transp2_r8.argtypes = [vp, ct.c_int, _PD, _PD, _PD, _PD, ct.c_int]
     lib.fimhip_trcadv_r8.argtypes = [vp, ct.c_int] + [_PD] * 7 + [ct.c_int]
     lib.fimhip_momtum_r8.argtypes = [vp, ct.c_int] + [_PD] * 13
     lib.fimhip_hybgen_r8.argtypes = [vp, ct.c_int] + [_PD] * 8 + [ct.c_int]
@@ -205,6 +211,7 @@ class Instance:
     def __init__(self, grid: _glvl.Grid, nvl: int, ntra: int, dt: float, thetac, dpsig, device: int = 0):
         self.lib = load_library()
         self.grid, self.nvl, self.ntra, self.dt = grid, nvl, ntra, dt
+        self.ntrb = 0
         self.nip = grid.nip
         gd, self._keep = _grid_desc(grid, nvl, ntra, dt, thetac, dpsig)
         h = ct.c_void_p()
@@ -257,6 +264,23 @@ class Instance:
         """1 = cnuity (FCT), 2 = mass_adv, for the steps that follow (dyn_run.F90:377-386)."""
         _chk(self.lib.fimhip_set_flux_scheme(self.h, int(flux_conserv_schm)), "set_flux_scheme")
 
+    def set_class_b(self, ntrb: int, dtratio: int = 6):
+        """Class-B tracers (transp3d.F90 / fct3d.F90) moved every dtratio steps."""
+        _chk(self.lib.fimhip_set_class_b(self.h, int(ntrb), int(dtratio)), "set_class_b")
+        self.ntrb = int(ntrb)
+
+    def transp0(self, its, cumufx, dp3d, dpinit):
+        _chk(self.lib.fimhip_transp0_r8(self.h, its, _p(cumufx), _p(dp3d), _p(dpinit)), "transp0")
+
+    def transp1(self, its, nf, of, vof, adbash1, adbash2, adbash3, cumufx, massfx):
+        _chk(self.lib.fimhip_transp1_r8(self.h, its, nf, of, vof, adbash1, adbash2, adbash3, _p(cumufx),
+                                        _p(massfx)), "transp1")
+
+    def transp2(self, its, tracr, cumufx, dpinit, dpfinl, TimingBarriers=False):
+        """tracr: (ntra + ntrb, nip, nvl); the class-B fields are transported in place."""
+        _chk(self.lib.fimhip_transp2_r8(self.h, its, _p(tracr), _p(cumufx), _p(dpinit), _p(dpfinl),
+                                        int(TimingBarriers)), "transp2")
+
     def trcadv(self, its, trcr_edg, tracr, trcdp, trc_tdcy, trclo_tdcy, massfx, delp, TimingBarriers=False):
         _chk(self.lib.fimhip_trcadv_r8(self.h, its, *map(_p, (trcr_edg, tracr, trcdp, trc_tdcy, trclo_tdcy,
              massfx, delp)), int(TimingBarriers)), "trcadv")
@@ -285,7 +309,7 @@ class Model(Instance):
                     relvor=c, sdot=l, psrf=(nip,), ptdcy=(2, nip), u_tdcy=(3,) + c, v_tdcy=(3,) + c,
                     dp_tdcy=(3,) + c, dpl_tdcy=(3,) + c, trc_tdcy=(ntra, 3) + c, trl_tdcy=(ntra, 3) + c,
                     massfx=(3,) + e, u_edg=e, v_edg=e, dp_edg=e, lp_edg=e, mont_edg=e, uvsq_edg=e,
-                    trc_edg=(ntra,) + e)[name]
+                    trc_edg=(ntra,) + e, trb=(self.ntrb,) + c, cumufx=e, dpinit=c)[name]
 
     def upload(self, name: str, a):
         a = np.ascontiguousarray(a, np.float64)
@@ -301,6 +325,8 @@ class Model(Instance):
     def load(self, st: dict):
         for n in ("us", "vs", "dp", "pr", "ex", "ph", "tr"):
             self.upload(n, st[n])
+        if self.ntrb and "trb" in st:
+            self.upload("trb", st["trb"])
 
     def synth(self, seed: int = 20240601, member: int = 0, flags: int = 0):
         """Fill the device state with a seeded synthetic member (fimhip_synth_device)."""
@@ -330,7 +356,7 @@ class Model(Instance):
 
     def globsum(self):
         m = ct.c_double()
-        t = np.zeros(self.ntra)
+        t = np.zeros(self.ntra + self.ntrb)
         _chk(self.lib.fimhip_globsum(self.h, ct.byref(m), _p(t)), "globsum")
         return m.value, t
 

@@ -87,6 +87,21 @@ int fimhip_mass_adv_r8(fimhip_ctx *ctx, int its, const double *u_velo, const dou
                        const double *u_edg, const double *v_edg, double *dp_edg, const double *lp_edg,
                        double *delp, double *pres, double *exner, double *dp_tndcy, double *massfx,
                        double *omega, int TimingBarriers);
+/* Class-B tracers: ntrb tracers advected every dtratio steps by the 3-D FCT
+ * scheme (ntrb and dtratio of module_control.F90:67 / fimnamelist.F90:87;
+ * dyn_run.F90:388-405).  Allocates the class-B state on the device; with
+ * ntrb > 0, fimhip_step and fimhip_init_state run transp0/1/2 as dyn_run and
+ * dyn_init do, and fimhip_globsum returns ntra + ntrb tracer totals. */
+int fimhip_set_class_b(fimhip_ctx *ctx, int ntrb, int dtratio);
+/* transp3d.F90:22 (transp0) */
+int fimhip_transp0_r8(fimhip_ctx *ctx, int its, double *cumufx, const double *dp3d, double *dpinit);
+/* transp3d.F90:44 (transp1) */
+int fimhip_transp1_r8(fimhip_ctx *ctx, int its, int nf, int of, int vof, double adbash1, double adbash2,
+                      double adbash3, double *cumufx, const double *massfx);
+/* transp3d.F90:81 (transp2, calling fct3d.F90:4); tracr (nvl,nip,ntra+ntrb),
+ * of which the class-B fields ntra+1..ntra+ntrb are transported */
+int fimhip_transp2_r8(fimhip_ctx *ctx, int its, double *tracr, const double *cumufx, const double *dpinit,
+                      const double *dpfinl, int TimingBarriers);
 /* trcadv.F90:19 */
 int fimhip_trcadv_r8(fimhip_ctx *ctx, int its, const double *trcr_edg, double *tracr, double *trcdp,
                      double *trc_tdcy, double *trclo_tdcy, const double *massfx, const double *delp,
@@ -110,7 +125,9 @@ enum {
   FIMHIP_TRDP, FIMHIP_WS, FIMHIP_RELVOR, FIMHIP_SDOT, FIMHIP_PSRF, FIMHIP_PTDCY,
   FIMHIP_U_TDCY, FIMHIP_V_TDCY, FIMHIP_DP_TDCY, FIMHIP_DPL_TDCY, FIMHIP_TRC_TDCY, FIMHIP_TRL_TDCY,
   FIMHIP_MASSFX, FIMHIP_U_EDG, FIMHIP_V_EDG, FIMHIP_DP_EDG, FIMHIP_LP_EDG, FIMHIP_MONT_EDG,
-  FIMHIP_UVSQ_EDG, FIMHIP_TRC_EDG, FIMHIP_NFIELDS
+  FIMHIP_UVSQ_EDG, FIMHIP_TRC_EDG,
+  FIMHIP_TRB, FIMHIP_CUMUFX, FIMHIP_DPINIT,   /* class-B transport (after fimhip_set_class_b) */
+  FIMHIP_NFIELDS
 };
 /* number of doubles of a field */
 long long fimhip_field_size(const fimhip_ctx *ctx, int field);
@@ -129,7 +146,8 @@ int fimhip_step(fimhip_ctx *ctx, int its, int mode);
  * for fimhip_step: 1 = cnuity (FCT, the default), 2 = mass_adv */
 int fimhip_set_flux_scheme(fimhip_ctx *ctx, int flux_conserv_schm);
 int fimhip_sync(fimhip_ctx *ctx);
-/* globsum.F90:70-101: column mass (sum area*dp/g) and tracer totals, FP64 */
+/* globsum.F90:70-101: column mass (sum area*dp/g) and the totals of the
+ * ntra (+ ntrb class-B) tracers, FP64 */
 int fimhip_globsum(fimhip_ctx *ctx, double *mass, double *trtot);
 
 /* per-kernel timing with HIP events on the instance's stream, for steps run

@@ -9,8 +9,9 @@
 #   oracle/_ref/bin/dynref_r4             reference dynamics hot path, default real 4
 #
 # The two dynref programs link the reference's own modules (cnuity, mass_adv
-# and its flux limiter, trcadv, edgvar1, edgvar3, momtum, hybgen, hystat,
-# abstart, globsum, dffusn and the control/constant modules they use) to the
+# and its flux limiter, trcadv, the class-B transport transp3d/fct3d, edgvar1,
+# edgvar3, momtum, hybgen, hystat, abstart, globsum, dffusn and the
+# control/constant modules they use) to the
 # driver oracle/ref/dynref.F90.
 # No reference source is copied; no stand-in modules are written.
 set -euo pipefail
@@ -66,7 +67,7 @@ H=$S/fim/horizontal
 DYN_SRC="$CNTL_SRC $S/cntl/module_constants.F90 $S/cntl/module_variables.F90
   $H/global_bounds.F90 $S/dummygptl/gptl.F90 $H/findmxmn.F90 $H/stencilprint.F90 $H/stenedgprint.F90
   $H/dffusn.F90 $H/cnuity_compute.F90 $H/cnuity.F90 $H/trcadv_compute.F90 $H/trcadv.F90
-  $H/flux_limiter.F90 $H/massadv_compute.F90 $H/massadv.F90
+  $H/flux_limiter.F90 $H/massadv_compute.F90 $H/massadv.F90 $H/fct3d.F90 $H/transp3d.F90
   $H/edgvar1.F90 $H/edgvar3.F90 $H/momtum_compute.F90 $H/momtum.F90 $H/hybgen.F90
   $H/hystat.F90 $H/abstart.F90 $H/globsum.F90 $H/GetIpnGlobal.F90 $S/prep/ss2icos/lay2lay.F90 $S/prep/ss2icos/lin2stp.F90"
 for prec in r8 r4; do

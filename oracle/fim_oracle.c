@@ -1007,6 +1007,213 @@ void fo_hystat(const fo_grid *g, fo_state *s, int its) {
 
 /* start-up: hybgen(its=0), pressure refresh (dyn_init.F90:884-893), hystat;
  * zero tendencies and set the AB slots (dyn_init.F90:1042-1094) */
+/* ------------------------------------------------------------------------
+ * Class-B tracer transport on the long time step (module_transp3d,
+ * transp3d.F90:22-241; dyn_run.F90:388-405; dyn_init.F90:1096-1098).
+ * transp0: reset the time-integrated mass flux and keep the thickness. */
+void fo_transp0(const fo_grid *g, fo_state *s) {
+  const int nip = g->nip, nvl = g->nvl;
+  memset(s->cumufx, 0, sizeof(double) * (size_t)nip * NPP * nvl);
+  memcpy(s->dpinit, s->dp, sizeof(double) * (size_t)nip * nvl);
+}
+
+/* transp1 (transp3d.F90:44-78): add this step's AB-weighted mass flux */
+void fo_transp1(const fo_grid *g, fo_state *s) {
+  const int nip = g->nip, nvl = g->nvl;
+  const int nf = s->nf - 1, of = s->of - 1, vof = s->vof - 1;
+  const double ab1 = s->ab1, ab2 = s->ab2, ab3 = s->ab3;
+  #pragma omp parallel for schedule(static)
+  for (int ipn = 0; ipn < nip; ipn++)
+    for (int edg = 0; edg < g->nprox[ipn]; edg++)
+      for (int k = 0; k < nvl; k++)
+        E3(s->cumufx, ipn, edg, k) = E3(s->cumufx, ipn, edg, k) + ab1 * MF(s->massfx, nf, ipn, edg, k) +
+                                     ab2 * MF(s->massfx, of, ipn, edg, k) + ab3 * MF(s->massfx, vof, ipn, edg, k);
+}
+
+/* fct3d (fct3d.F90:4-483, diagno = .false., recovr = .false.): 3-D FCT of
+ * nfld fields (nvl,nip) with time-integrated horizontal mass fluxes u
+ * (nvl,6,nip), vertical mass fluxes w (nvl,nip), thickness thko -> thkn */
+void fo_fct3d(const fo_grid *g, double *fields, int nfld, const double *u, const double *w,
+              const double *thko, const double *thkn) {
+  const int nip = g->nip, nvl = g->nvl;
+  const double athird = 1. / 3., epsil = 1.e-11, onemu = 1.e-6;
+  const size_t cz = (size_t)nip * nvl, ez = cz * NPP;
+  double *fld = malloc(sizeof(double) * cz), *vfx = malloc(sizeof(double) * cz);
+  double *vdv = malloc(sizeof(double) * cz), *fmx = malloc(sizeof(double) * cz);
+  double *fmn = malloc(sizeof(double) * cz), *flp = malloc(sizeof(double) * cz);
+  double *fln = malloc(sizeof(double) * cz);
+  double *flx = malloc(sizeof(double) * ez), *uan = malloc(sizeof(double) * ez);
+  for (int n = 0; n < nfld; n++) {
+    double *f = fields + (size_t)n * cz;
+    /* massless cells take their values from the layers above and below */
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < nip; i++) {
+      for (int k = 0; k < nvl; k++) C2(fld, i, k) = C2(f, i, k);
+      for (int k = nvl - 2; k >= 0; k--)
+        C2(fld, i, k) = (C2(fld, i, k) * C2(thko, i, k) + C2(fld, i, k + 1) * onemu) / (C2(thko, i, k) + onemu);
+      for (int k = 1; k < nvl; k++)
+        C2(fld, i, k) = (C2(fld, i, k) * C2(thko, i, k) + C2(fld, i, k - 1) * onemu) / (C2(thko, i, k) + onemu);
+    }
+    /* piecewise parabolic fit per layer, vertical flux over the upstream slab */
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < nip; i++) {
+      double a[64 + 2], b[64 + 2], c[64 + 2];
+      const double *fc = fld + (size_t)i * nvl, *th = thko + (size_t)i * nvl, *wc = w + (size_t)i * nvl;
+      a[0] = fc[0]; b[0] = 0.; c[0] = 0.;
+      a[nvl - 1] = fc[nvl - 1]; b[nvl - 1] = 0.; c[nvl - 1] = 0.;
+      for (int k = 1; k < nvl - 1; k++) {
+        const double dxlft = dmax(epsil, th[k - 1]), dxmid = dmax(epsil, th[k]), dxrgt = dmax(epsil, th[k + 1]);
+        const double yl = (dxlft * fc[k] + dxmid * fc[k - 1]) / (dxlft + dxmid);
+        const double yr = (dxrgt * fc[k] + dxmid * fc[k + 1]) / (dxrgt + dxmid);
+        a[k] = 1.5 * fc[k] - .25 * (yl + yr);
+        b[k] = yr - yl;
+        c[k] = 6. * (.5 * (yl + yr) - fc[k]);
+        if (fabs(yr - yl) < 6. * fabs(.5 * (yl + yr) - fc[k])) {
+          if (fabs(yr - yl) > 2. * fabs(.5 * (yl + yr) - fc[k])) {
+            if ((yr - yl) * (.5 * (yl + yr) - fc[k]) > 0.) {
+              a[k] = .25 * (3. * fc[k] + yl);
+              c[k] = 3. * (fc[k] - yl);
+              b[k] = c[k];
+            } else {
+              a[k] = .25 * (3. * fc[k] + yr);
+              c[k] = 3. * (fc[k] - yr);
+              b[k] = -c[k];
+            }
+          } else {
+            a[k] = fc[k]; b[k] = 0.; c[k] = 0.;
+          }
+        }
+      }
+      double amount = 0.;   /* a zero w keeps the previous layer's amount; its flux is 0 */
+      for (int k = 0; k < nvl - 1; k++) {
+        double slab = onemu;
+        if (wc[k] < 0.) {                 /* interface moves up: slab from above */
+          amount = slab * fc[k + 1];
+          int kp = k;
+          for (;;) {
+            kp = kp + 1;
+            if (slab >= -wc[k]) break;
+            if (th[kp] > 0.) {
+              const double dslab = dmin(slab + th[kp], -wc[k]) - dmin(slab, -wc[k]);
+              const double dx = dslab / th[kp];
+              const double fcdx = a[kp] + b[kp] * .5 * (dx - 1.) + c[kp] * (.25 - dx * (.5 - dx * athird));
+              amount = amount + fcdx * dslab;
+              slab = slab + dslab;
+            }
+            if (!(kp < nvl - 1)) break;
+          }
+        } else if (wc[k] > 0.) {          /* interface moves down: slab from below */
+          amount = slab * fc[k];
+          int kp = k + 1;
+          for (;;) {
+            kp = kp - 1;
+            if (slab >= wc[k]) break;
+            if (th[kp] > 0.) {
+              const double dslab = dmin(slab + th[kp], wc[k]) - dmin(slab, wc[k]);
+              const double dx = dslab / th[kp];
+              const double fcdx = a[kp] + b[kp] * .5 * (1. - dx) + c[kp] * (.25 - dx * (.5 - dx * athird));
+              amount = amount + fcdx * dslab;
+              slab = slab + dslab;
+            }
+            if (!(kp > 1)) break;     /* the Fortran continues while kp > 2 (1-based) */
+          }
+        }
+        C2(vfx, i, k) = wc[k] * amount / slab;
+      }
+      C2(vfx, i, nvl - 1) = 0.;
+      C2(vdv, i, 0) = C2(vfx, i, 0);
+      for (int k = 1; k < nvl; k++) C2(vdv, i, k) = C2(vfx, i, k) - C2(vfx, i, k - 1);
+    }
+    /* low-order and antidiffusive fluxes, local bounds */
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < nip; i++)
+      for (int k = 0; k < nvl; k++) {
+        double mx = C2(fld, i, k), mn = C2(fld, i, k);
+        for (int edg = 0; edg < g->nprox[i]; edg++) {
+          const int ix = PROX(i, edg), edx = PROXS(i, edg);
+          const double ui = E3(u, i, edg, k), ux = E3(u, ix, edx, k);
+          E3(flx, i, edg, k) = 0.5 * ((ui + fabs(ui)) * C2(fld, i, k) - (ux + fabs(ux)) * C2(fld, ix, k));
+          const double q = .5 * (C2(fld, i, k) + C2(fld, ix, k));
+          E3(uan, i, edg, k) = q * ui - E3(flx, i, edg, k);
+          mx = dmax(mx, C2(fld, ix, k));
+          mn = dmin(mn, C2(fld, ix, k));
+        }
+        if (k < nvl - 1 && C2(w, i, k) < 0.) {
+          mx = dmax(mx, C2(vfx, i, k) / C2(w, i, k));
+          mn = dmin(mn, C2(vfx, i, k) / C2(w, i, k));
+        }
+        if (k > 0 && C2(w, i, k - 1) > 0.) {
+          mx = dmax(mx, C2(vfx, i, k - 1) / C2(w, i, k - 1));
+          mn = dmin(mn, C2(vfx, i, k - 1) / C2(w, i, k - 1));
+        }
+        C2(fmx, i, k) = mx;
+        C2(fmn, i, k) = mn;
+      }
+    /* low-order update, then the ratios of permissible to incoming and
+     * outgoing antidiffusive fluxes */
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < nip; i++)
+      for (int k = 0; k < nvl; k++) {
+        double hd = 0.;
+        for (int edg = 0; edg < g->nprox[i]; edg++) hd = hd + E3(flx, i, edg, k);
+        hd = hd * g->rarea[i];
+        const double q = C2(fld, i, k) * C2(thko, i, k) - hd - C2(vdv, i, k);
+        const double amount = dmax(dmax(0., C2(fmn, i, k) * C2(thkn, i, k)), dmin(q, C2(fmx, i, k) * C2(thkn, i, k)));
+        C2(fld, i, k) = (C2(fld, i, k) * onemu + amount) / (onemu + C2(thkn, i, k));
+      }
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < nip; i++)
+      for (int k = 0; k < nvl; k++) {
+        double totin = 0., totou = 0.;
+        for (int edg = 0; edg < g->nprox[i]; edg++) {
+          totin = totin - dmin(0., E3(uan, i, edg, k));
+          totou = totou + dmax(0., E3(uan, i, edg, k));
+        }
+        C2(flp, i, k) = (C2(fmx, i, k) - C2(fld, i, k)) * C2(thkn, i, k) / (totin + epsil) * g->rarea[i];
+        C2(fln, i, k) = (C2(fld, i, k) - C2(fmn, i, k)) * C2(thkn, i, k) / (totou + epsil) * g->rarea[i];
+      }
+    /* limited antidiffusive fluxes and the final update */
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < nip; i++)
+      for (int k = 0; k < nvl; k++) {
+        double hd = 0.;
+        for (int edg = 0; edg < g->nprox[i]; edg++) {
+          const int ix = PROX(i, edg);
+          const double an = E3(uan, i, edg, k);
+          const double clip = an >= 0. ? dmin(dmin(1., C2(fln, i, k)), C2(flp, ix, k))
+                                       : dmin(dmin(1., C2(flp, i, k)), C2(fln, ix, k));
+          hd = hd + an * clip;
+        }
+        hd = hd * g->rarea[i];
+        const double q = C2(fld, i, k) * C2(thkn, i, k) - hd;
+        const double amount = dmax(dmax(0., C2(fmn, i, k) * C2(thkn, i, k)), dmin(q, C2(fmx, i, k) * C2(thkn, i, k)));
+        C2(f, i, k) = (C2(fld, i, k) * onemu + amount) / (onemu + C2(thkn, i, k));
+      }
+  }
+  free(fld); free(vfx); free(vdv); free(fmx); free(fmn); free(flp); free(fln); free(flx); free(uan);
+}
+
+/* transp2 (transp3d.F90:81-241): the vertical mass flux that closes the
+ * time-integrated continuity equation, then fct3d of the class-B tracers
+ * from dpinit to the current thickness */
+void fo_transp2(const fo_grid *g, fo_state *s) {
+  const int nip = g->nip, nvl = g->nvl;
+  double *w = malloc(sizeof(double) * (size_t)nip * nvl);
+  #pragma omp parallel for schedule(static)
+  for (int i = 0; i < nip; i++) {
+    double hordiv[64];
+    for (int k = 0; k < nvl; k++) hordiv[k] = 0.;
+    for (int edg = 0; edg < g->nprox[i]; edg++)
+      for (int k = 0; k < nvl; k++) hordiv[k] = hordiv[k] + E3(s->cumufx, i, edg, k);
+    for (int k = 0; k < nvl; k++) {
+      const double vertdv = (C2(s->dpinit, i, k) - C2(s->dp, i, k)) - hordiv[k] * g->rarea[i];
+      C2(w, i, k) = k == 0 ? vertdv : C2(w, i, k - 1) + vertdv;
+    }
+  }
+  if (g->ntrb > 0) fo_fct3d(g, s->trb, g->ntrb, s->cumufx, w, s->dpinit, s->dp);
+  free(w);
+}
+
 void fo_init(const fo_grid *g, fo_state *s, int init_hybgen) {
   const int nip = g->nip, nvl = g->nvl, nvlp1 = nvl + 1, ntra = g->ntra;
   for (int t = 0; t < ntra; t++)
@@ -1029,6 +1236,7 @@ void fo_init(const fo_grid *g, fo_state *s, int init_hybgen) {
   memset(s->trl_tdcy, 0, sizeof(double) * 3 * ntra * nip * nvl);
   memset(s->massfx, 0, sizeof(double) * 3 * nip * NPP * nvl);
   s->nf = 0; s->of = 2; s->vof = 1;
+  if (g->ntrb > 0) fo_transp0(g, s);
 }
 
 /* one step in dyn_run order: DYN_1(its) then DYN_2(its) (dyn_run.F90:119-425);
@@ -1039,6 +1247,13 @@ void fo_step(const fo_grid *g, fo_state *s, int its, int mode) {
   fo_edgvar3(g, s);
   if (g->flux_scheme == 2) fo_mass_adv(g, s);
   else fo_cnuity(g, s);
+  if (g->ntrb > 0) {                 /* dyn_run.F90:388-405 */
+    fo_transp1(g, s);
+    if (its % g->dtratio == 0) {
+      fo_transp2(g, s);
+      fo_transp0(g, s);
+    }
+  }
   fo_trcadv(g, s);
   if (mode == 0) {
     fo_momtum(g, s);
@@ -1047,7 +1262,8 @@ void fo_step(const fo_grid *g, fo_state *s, int its, int mode) {
   }
 }
 
-/* globsum.F90:70-101: area-weighted column mass and tracer totals */
+/* globsum.F90:70-101: area-weighted column mass and tracer totals
+ * (ntra + ntrb of them) */
 void fo_globsum(const fo_grid *g, const fo_state *s, double *mass, double *trtot) {
   const int nip = g->nip, nvl = g->nvl, ntra = g->ntra;
   double summ = 0.;
@@ -1059,6 +1275,12 @@ void fo_globsum(const fo_grid *g, const fo_state *s, double *mass, double *trtot
     for (int ipn = 0; ipn < nip; ipn++)
       for (int k = 0; k < nvl; k++) st = st + g->area[ipn] * C2(s->dp, ipn, k) * TR(s->tr, t, ipn, k);
     trtot[t] = st;
+  }
+  for (int t = 0; t < g->ntrb; t++) {     /* class-B tracers follow (tr3d(:,:,ntra+1:)) */
+    double st = 0.;
+    for (int ipn = 0; ipn < nip; ipn++)
+      for (int k = 0; k < nvl; k++) st = st + g->area[ipn] * C2(s->dp, ipn, k) * TR(s->trb, t, ipn, k);
+    trtot[ntra + t] = st;
   }
 }
 

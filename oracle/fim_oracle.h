@@ -7,6 +7,8 @@ typedef struct {
   int nip, nvl, ntra;
   int pow_kind;              /* 0: C library pow (as the Fortran), 1: fim_pow (as the GPU) */
   int flux_scheme;           /* flux_conserv_schm: 1 cnuity (FCT), 2 mass_adv (flux limiter) */
+  int ntrb;                  /* class-B tracers (transported every dtratio steps by fct3d) */
+  int dtratio;               /* class-B / class-A time step ratio */
   double dt;
   const int *prox, *proxs, *nprox;
   const double *cs, *sn, *sidevec_c, *sidevec_e, *sideln, *hfcoef;
@@ -24,6 +26,9 @@ typedef struct {
   double *work_l1, *work_l2;                               /* level-sized scratch */
   int nf, of, vof;
   double ab1, ab2, ab3;
+  double *trb;      /* class-B tracers (nvl,nip,ntrb): tr3d(:,:,ntra+1:ntra+ntrb) */
+  double *cumufx;   /* time-integrated mass flux (nvl,6,nip) */
+  double *dpinit;   /* thickness at the start of the integration (nvl,nip) */
 } fo_state;
 
 void fo_vertical_params(int nvl, double dt, double *dfflen, double *smoo_coeff);
@@ -37,6 +42,11 @@ void fo_trcadv(const fo_grid *g, fo_state *s);
 void fo_momtum(const fo_grid *g, fo_state *s);
 void fo_hybgen(const fo_grid *g, fo_state *s, int its);
 void fo_hystat(const fo_grid *g, fo_state *s, int its);
+void fo_transp0(const fo_grid *g, fo_state *s);
+void fo_transp1(const fo_grid *g, fo_state *s);
+void fo_transp2(const fo_grid *g, fo_state *s);
+void fo_fct3d(const fo_grid *g, double *fields, int nfld, const double *u, const double *w,
+              const double *thko, const double *thkn);
 void fo_init(const fo_grid *g, fo_state *s, int init_hybgen);
 void fo_step(const fo_grid *g, fo_state *s, int its, int mode);
 void fo_globsum(const fo_grid *g, const fo_state *s, double *mass, double *trtot);

@@ -32,7 +32,7 @@ _PI = ct.POINTER(ct.c_int)
 
 class _Grid(ct.Structure):
     _fields_ = [("nip", ct.c_int), ("nvl", ct.c_int), ("ntra", ct.c_int), ("pow_kind", ct.c_int),
-                ("flux_scheme", ct.c_int), ("dt", ct.c_double),
+                ("flux_scheme", ct.c_int), ("ntrb", ct.c_int), ("dtratio", ct.c_int), ("dt", ct.c_double),
                 ("prox", _PI), ("proxs", _PI), ("nprox", _PI),
                 ("cs", _PD), ("sn", _PD), ("sidevec_c", _PD), ("sidevec_e", _PD), ("sideln", _PD),
                 ("hfcoef", _PD), ("area", _PD), ("rarea", _PD), ("corio", _PD), ("grndspd", _PD),
@@ -50,7 +50,8 @@ _STATE_ARRAYS = ["us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp",
 class _State(ct.Structure):
     _fields_ = [(n, _PD) for n in _STATE_ARRAYS] + [
         ("nf", ct.c_int), ("of", ct.c_int), ("vof", ct.c_int),
-        ("ab1", ct.c_double), ("ab2", ct.c_double), ("ab3", ct.c_double)]
+        ("ab1", ct.c_double), ("ab2", ct.c_double), ("ab3", ct.c_double),
+        ("trb", _PD), ("cumufx", _PD), ("dpinit", _PD)]
 
 
 def _lib():
@@ -60,6 +61,8 @@ def _lib():
     lib.fo_hybgen.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), ct.c_int]
     lib.fo_hystat.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), ct.c_int]
     lib.fo_init.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), ct.c_int]
+    for fn in ("fo_transp0", "fo_transp1", "fo_transp2"):
+        getattr(lib, fn).argtypes = [ct.POINTER(_Grid), ct.POINTER(_State)]
     lib.fo_step.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), ct.c_int, ct.c_int]
     lib.fo_abstart.argtypes = [ct.POINTER(_State), ct.c_int, ct.c_double]
     lib.fo_globsum.argtypes = [ct.POINTER(_Grid), ct.POINTER(_State), _PD, _PD]
@@ -123,13 +126,16 @@ def grid_arrays(grid, nvl: int, dt: float, thetac, dpsig) -> dict:
 class OracleRun:
     """A full FP64 state advanced by the C restatement."""
 
-    def __init__(self, garr: dict, nvl: int, ntra: int, dt: float, pow_kind: int = 1, flux_scheme: int = 1):
+    def __init__(self, garr: dict, nvl: int, ntra: int, dt: float, pow_kind: int = 1, flux_scheme: int = 1,
+                 ntrb: int = 0, dtratio: int = 6):
         self.garr = garr
         nip = garr["nprox"].shape[0]
         self.nip, self.nvl, self.ntra, self.dt = nip, nvl, ntra, dt
         g = _Grid()
         g.nip, g.nvl, g.ntra, g.pow_kind, g.dt = nip, nvl, ntra, pow_kind, dt
         g.flux_scheme = flux_scheme
+        g.ntrb, g.dtratio = ntrb, dtratio
+        self.ntrb = ntrb
         for n in ("prox", "proxs", "nprox"):
             setattr(g, n, _pi(garr[n]))
         for n in ("cs", "sn", "sidevec_c", "sidevec_e", "sideln", "hfcoef", "area", "rarea", "corio",
@@ -144,15 +150,18 @@ class OracleRun:
                       u_edg=e, v_edg=e, dp_edg=e, lp_edg=e, mont_edg=e, uvsq_edg=e, trc_edg=(ntra,) + e,
                       work_e1=e, work_e2=e, work_c1=c, work_c2=c, work_c3=c, work_c4=c, work_c5=c,
                       work_l1=l, work_l2=l)
-        self.a = {n: np.zeros(shapes[n]) for n in _STATE_ARRAYS}
+        shapes.update(trb=(max(ntrb, 1),) + c, cumufx=e, dpinit=c)
+        self.a = {n: np.zeros(shapes[n]) for n in _STATE_ARRAYS + ["trb", "cumufx", "dpinit"]}
         s = _State()
-        for n in _STATE_ARRAYS:
+        for n in _STATE_ARRAYS + ["trb", "cumufx", "dpinit"]:
             setattr(s, n, _pd(self.a[n]))
         self.s = s
 
     def load(self, st: dict):
         for n in ("us", "vs", "dp", "pr", "ex", "ph", "tr"):
             self.a[n][...] = st[n]
+        if self.ntrb and "trb" in st:
+            self.a["trb"][...] = st["trb"]
 
     def init(self, init_hybgen: bool = True):
         lib().fo_init(ct.byref(self.g), ct.byref(self.s), 1 if init_hybgen else 0)
@@ -168,7 +177,7 @@ class OracleRun:
 
     def globsum(self):
         m = np.zeros(1)
-        t = np.zeros(self.ntra)
+        t = np.zeros(self.ntra + self.ntrb)
         lib().fo_globsum(ct.byref(self.g), ct.byref(self.s), _pd(m), _pd(t))
         return float(m[0]), t
 

@@ -16,13 +16,17 @@
 !              (dyn_init.F90:880-893) before the start-up hystat.
 ! flags bit 1: continuity by mass_adv (flux_conserv_schm = 2,
 !              dyn_run.F90:377-386) instead of cnuity.
+! flags bits 8-15: ntrb, class-B tracers (tr in the state file then holds
+!              ntra+ntrb fields), transported by transp1/transp2/transp0
+!              (dyn_run.F90:388-405, dyn_init.F90:1096-1098);
+!              bits 16-23: dtratio.
 !
 ! Output: a sequence of records  character(16) tag, int64 n, real64 vals(n).
 ! Stage records are written for step 'dumpstep' after every sub-call.
 
 program dynref
-  use module_control,   only: nip, nvlp1, dt, ntra, npp, nabl, kbl
-  use fimnamelist,      only: nvl, intfc_smooth, veldff_bkgnd, veldff_boost
+  use module_control,   only: nip, nvlp1, dt, ntra, ntrb, npp, nabl, kbl
+  use fimnamelist,      only: nvl, intfc_smooth, veldff_bkgnd, veldff_boost, dtratio
   use module_constants
   use module_variables, only: nf, of, vof, adbash1, adbash2, adbash3, worka, workb, q_min_out, q_max_out
   use global_bounds,    only: set_global_bounds, set_global_bounds_halo
@@ -35,6 +39,7 @@ program dynref
   use momtum_driver,    only: momtum
   use module_hybgen,    only: hybgen
   use module_hystat,    only: hystat
+  use module_transp3d,  only: transp0, transp1, transp2
   implicit none
 
   character(len=512) :: fgrid, fstate, fout
@@ -55,6 +60,7 @@ program dynref
   real, allocatable :: trc_tdcy(:,:,:,:), trl_tdcy(:,:,:,:), massfx(:,:,:,:)
   real, allocatable :: u_edg(:,:,:), v_edg(:,:,:), dp_edg(:,:,:), lp_edg(:,:,:)
   real, allocatable :: geop_edg(:,:,:), mont_edg(:,:,:), uvsq_edg(:,:,:), trc_edg(:,:,:,:)
+  real, allocatable :: cumufx(:,:,:), dpinit(:,:)
 
   call get_command_argument(1, fgrid)
   call get_command_argument(2, fstate)
@@ -65,6 +71,8 @@ program dynref
   if (hdr(1) /= 1179208497) stop 'dynref: bad state magic'
   nvl = hdr(2); nip = hdr(3); ntra = hdr(4); nsteps = hdr(5)
   mode = hdr(6); dumpstep = hdr(7); flags = hdr(8)
+  ntrb = iand(ishft(flags, -8), 255)
+  dtratio = iand(ishft(flags, -16), 255)
   read (21) dt8
   dt = dt8
   nvlp1 = nvl + 1
@@ -185,7 +193,8 @@ program dynref
   read (21) r81; thetac = r81
   read (21) r81; dpsig = r81
   allocate (us3d(nvl,nip), vs3d(nvl,nip), dp3d(nvl,nip), pr3d(nvlp1,nip), ex3d(nvlp1,nip))
-  allocate (ph3d(nvlp1,nip), mp3d(nvl,nip), tr3d(nvl,nip,ntra), trdp(nvl,nip,ntra))
+  allocate (ph3d(nvlp1,nip), mp3d(nvl,nip), tr3d(nvl,nip,ntra+ntrb), trdp(nvl,nip,ntra))
+  allocate (cumufx(nvl,npp,nip), dpinit(nvl,nip))
   allocate (sdot(nvlp1,nip), ws3d(nvl,nip), relvor(nvl,nip), psrf(nip), ptdcy(nip,2))
   allocate (u_tdcy(nvl,nip,nabl), v_tdcy(nvl,nip,nabl), dp_tdcy(nvl,nip,nabl), dpl_tdcy(nvl,nip,nabl))
   allocate (trc_tdcy(nvl,nip,nabl,ntra), trl_tdcy(nvl,nip,nabl,ntra), massfx(nvl,npp,nip,nabl))
@@ -203,7 +212,7 @@ program dynref
   read (21) r8a; pr3d = r8a
   read (21) r8a; ex3d = r8a
   read (21) r8a; ph3d = r8a
-  allocate (r8b(nvl,nip,ntra))
+  allocate (r8b(nvl,nip,ntra+ntrb))
   read (21) r8b; tr3d = r8b
   close (21)
 
@@ -242,6 +251,7 @@ program dynref
   call hystat (its, ph3d, ex3d, mp3d, dp3d, tr3d, trdp, psrf, ptdcy)
   call dumpstate('init')
   nf = 0; of = 2; vof = 1
+  if (ntrb > 0) call transp0 (its, cumufx, dp3d, dpinit)
 
   do its = 1, nsteps
     call abstart (its, 1, nf, of, vof, adbash1, adbash2, adbash3)
@@ -257,6 +267,19 @@ program dynref
     else
       call cnuity (its, us3d, vs3d, u_edg, v_edg, dp_edg, lp_edg, dp3d, pr3d, ex3d, &
                    dp_tdcy, dpl_tdcy, massfx, ws3d, .false.)
+    end if
+    if (ntrb > 0) then
+      call transp1 (its, nf, of, vof, adbash1, adbash2, adbash3, cumufx, massfx)
+      if (its == dumpstep) call out3('transp1.cumufx', real(cumufx,8))
+      if (mod(its,dtratio) == 0) then
+        call transp2 (its, tr3d, cumufx, dpinit, dp3d, .false.)
+        if (its == dumpstep) then
+          do t = ntra+1, ntra+ntrb
+            call out2('transp2.trb', real(tr3d(:,:,t),8))
+          end do
+        end if
+        call transp0 (its, cumufx, dp3d, dpinit)
+      end if
     end if
     if (its == dumpstep) then
       call dumpstate('cnuity')
@@ -301,6 +324,13 @@ program dynref
   end do
   call out2('final.dptdcy', real(dp_tdcy(:,:,nf),8))
   call out2('final.utdcy', real(u_tdcy(:,:,nf),8))
+  if (ntrb > 0) then
+    do t = ntra+1, ntra+ntrb
+      call out2('final.trb', real(tr3d(:,:,t),8))
+    end do
+    call out3('final.cumufx', real(cumufx,8))
+    call out2('final.dpinit', real(dpinit,8))
+  end if
   close (23)
 
 contains

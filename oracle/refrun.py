@@ -58,15 +58,23 @@ def ref_file(glvl: int, name: str) -> str:
 MAGIC = 1179208497
 
 
-def write_state(path, st, thetac, dpsig, *, nsteps, dt, mode=0, dumpstep=0, init_hybgen=True, flux_scheme=1):
+def write_state(path, st, thetac, dpsig, *, nsteps, dt, mode=0, dumpstep=0, init_hybgen=True, flux_scheme=1,
+                dtratio=6):
+    """The state file dynref reads; class-B tracers (st["trb"], if present)
+    follow the class-A tracers as in tr3d(:,:,ntra+1:ntra+ntrb)."""
     nip, nvl = st["dp"].shape
     ntra = st["tr"].shape[0]
-    flags = (1 if init_hybgen else 0) | (2 if flux_scheme == 2 else 0)
+    trb = st.get("trb")
+    ntrb = 0 if trb is None else trb.shape[0]
+    assert ntrb < 256 and 0 < dtratio < 256
+    flags = (1 if init_hybgen else 0) | (2 if flux_scheme == 2 else 0) | (ntrb << 8) | (dtratio << 16)
     with open(path, "wb") as f:
         f.write(struct.pack("<8i", MAGIC, nvl, nip, ntra, nsteps, mode, dumpstep, flags))
         f.write(struct.pack("<d", dt))
         for a in (thetac, dpsig, st["us"], st["vs"], st["dp"], st["pr"], st["ex"], st["ph"], st["tr"]):
             f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+        if ntrb:
+            f.write(np.ascontiguousarray(trb, dtype="<f8").tobytes())
 
 
 def read_records(path) -> dict:
@@ -95,14 +103,15 @@ def _big_stack():
 
 
 def run_dynref(glvl_path, st, thetac, dpsig, *, nsteps, dt, mode=0, dumpstep=0,
-               init_hybgen=True, prec="r8", workdir=None, flux_scheme=1) -> dict:
+               init_hybgen=True, prec="r8", workdir=None, flux_scheme=1, dtratio=6) -> dict:
     workdir = workdir or os.path.join(REF, "runs")
     os.makedirs(workdir, exist_ok=True)
-    tag = "%d_%d_%d_%s_%d" % (os.getpid(), nsteps, mode, prec, flux_scheme)
+    tag = "%d_%d_%d_%s_%d_%d" % (os.getpid(), nsteps, mode, prec, flux_scheme,
+                                 0 if st.get("trb") is None else st["trb"].shape[0])
     fin = os.path.join(workdir, "state_%s.bin" % tag)
     fout = os.path.join(workdir, "out_%s.bin" % tag)
     write_state(fin, st, thetac, dpsig, nsteps=nsteps, dt=dt, mode=mode, dumpstep=dumpstep,
-                init_hybgen=init_hybgen, flux_scheme=flux_scheme)
+                init_hybgen=init_hybgen, flux_scheme=flux_scheme, dtratio=dtratio)
     r = subprocess.run([os.path.join(BIN, "dynref_" + prec), glvl_path, fin, fout],
                        cwd=workdir, capture_output=True, text=True, preexec_fn=_big_stack)
     if r.returncode != 0:

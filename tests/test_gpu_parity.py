@@ -309,3 +309,132 @@ def test_g5_mass_adv_transport_dt480_matches_checker_bitwise(oracle_lib):
         assert m.download("dp").min() >= 0.
     finally:
         m.close()
+
+
+def _class_b_case(g, nvl=NVL, ntrb=2, wind_noise=True):
+    """Seeded state plus ntrb log-normal class-B tracers (median 1e-9) times
+    the cosine bell."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    st = fimhip.synth_state(g.lat, g.lon, nvl, NTRA, seed=20240601, member=0,
+                            flags=fimhip.SYNTH_WIND_NOISE if wind_noise else 0)
+    lg = fimhip.synth_state(g.lat, g.lon, nvl, 1 + ntrb, seed=7, member=3, flags=fimhip.SYNTH_LOGNORMAL)
+    st["trb"] = np.ascontiguousarray(lg["tr"][1:1 + ntrb])
+    return st
+
+
+def test_class_b_dropins_match_checker_bitwise(case):
+    """transp1 every step and transp2 (fct3d) + transp0 every second step,
+    through the drop-in entries with host arrays, between cnuity and trcadv
+    as dyn_run calls them: the integrated mass flux, dpinit and the class-B
+    tracers have the checker's bytes."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    O, g, st0, th, ds, ga = case
+    print("case: G3 nvl=64 ntra=4 ntrb=2 dtratio=2, class-B drop-ins", flush=True)
+    st = _class_b_case(g)
+    o = O.OracleRun(ga, NVL, NTRA, DT, pow_kind=1, ntrb=2, dtratio=2)
+    o.load(st)
+    o.init(True)
+    inst = fimhip.Instance(g, NVL, NTRA, DT, th, ds)
+    try:
+        inst.set_class_b(2, 2)
+        a = {k: v.copy() for k, v in o.a.items()}
+        for its in (1, 2, 3, 4):
+            inst.abstart(its)
+            o.abstart(its)
+            geop_edg = np.zeros_like(a["u_edg"])
+            args = [a["us"], a["vs"], a["dp"], a["pr"], a["ex"], a["ph"], a["mp"], a["tr"], a["u_edg"],
+                    a["v_edg"], a["dp_edg"], a["lp_edg"], geop_edg, a["mont_edg"], a["uvsq_edg"], a["trc_edg"]]
+            inst.edgvar1(its, *args)
+            o.call("edgvar1")
+            inst.edgvar3(its, *args)
+            o.call("edgvar3")
+            inst.cnuity(its, a["us"], a["vs"], a["u_edg"], a["v_edg"], a["dp_edg"], a["lp_edg"], a["dp"], a["pr"],
+                        a["ex"], a["dp_tdcy"], a["dpl_tdcy"], a["massfx"], a["ws"])
+            o.call("cnuity")
+            assert _same(a["massfx"], o.a["massfx"]) and _same(a["dp"], o.a["dp"])
+            inst.transp1(its, inst.nf, inst.of, inst.vof, *inst.adbash, a["cumufx"], a["massfx"])
+            o.call("transp1")
+            assert _same(a["cumufx"], o.a["cumufx"]), ("transp1", its)
+            if its % 2 == 0:
+                tracr = np.concatenate([a["tr"], a["trb"]])
+                inst.transp2(its, tracr, a["cumufx"], a["dpinit"], a["dp"])
+                a["trb"][...] = tracr[NTRA:]
+                assert _same(tracr[:NTRA], a["tr"])
+                o.call("transp2")
+                assert _same(a["trb"], o.a["trb"]), ("transp2", its, rel_l2(a["trb"], o.a["trb"]))
+                inst.transp0(its, a["cumufx"], a["dp"], a["dpinit"])
+                o.call("transp0")
+                assert _same(a["cumufx"], o.a["cumufx"]) and _same(a["dpinit"], o.a["dpinit"]), ("transp0", its)
+            inst.trcadv(its, a["trc_edg"], a["tr"], a["trdp"], a["trc_tdcy"], a["trl_tdcy"], a["massfx"], a["dp"])
+            o.call("trcadv")
+        assert not np.array_equal(a["trb"], st["trb"])
+    finally:
+        inst.close()
+
+
+@pytest.mark.parametrize("mode,flux_scheme", [(0, 1), (1, 1), (0, 2)])
+def test_class_b_steps_match_checker_bitwise(case, mode, flux_scheme):
+    """Start-up plus 13 device-resident steps with 2 class-B tracers moved
+    every third step: class-B tracers, the integrated mass flux, dpinit and
+    every class-A field identical to the checker; global tracer totals
+    (ntra + ntrb) within 1e-12."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    O, g, st0, th, ds, ga = case
+    print("case: G3 nvl=64 ntra=4 ntrb=2 dtratio=3 mode=%d scheme=%d steps=13" % (mode, flux_scheme), flush=True)
+    st = _class_b_case(g)
+    o = O.OracleRun(ga, NVL, NTRA, DT, pow_kind=1, flux_scheme=flux_scheme, ntrb=2, dtratio=3)
+    o.load(st)
+    o.init(True)
+    m = fimhip.Model(g, NVL, NTRA, DT, th, ds)
+    try:
+        m.set_flux_scheme(flux_scheme)
+        m.set_class_b(2, 3)
+        m.load(st)
+        m.init_state(True)
+        for its in range(1, 14):
+            m.step(its, mode)
+            o.step(its, mode)
+        m.sync()
+        for f in ("trb", "cumufx", "dpinit", "us", "vs", "dp", "pr", "ex", "tr", "massfx"):
+            got = m.download(f)
+            assert _same(got, o.a[f]), (f, rel_l2(got, o.a[f]))
+        assert np.abs(o.a["cumufx"]).max() > 0.
+        gm, gt = m.globsum()
+        om, ot = o.globsum()
+        assert len(gt) == NTRA + 2
+        assert np.all(np.abs(gt - ot) <= 1e-12 * np.abs(ot))
+    finally:
+        m.close()
+
+
+def test_g5_class_b_transport_dt480_matches_checker_bitwise(oracle_lib):
+    """G5 x 64 layers, 4 class-A and 4 class-B tracers, 11 transport steps at
+    dt = 480 s with the class-B tracers moved every second step: identical to
+    the checker, non-negative, and within the initial global bounds."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+    O = oracle_lib
+    g = fimhip.icos_grid(5)
+    th, ds = state.vertical_tables(NVL)
+    dt = 480.0
+    print("case: G5 nvl=64 ntra=4 ntrb=4 dtratio=2 dt=480 11 transport steps", flush=True)
+    st = _class_b_case(g, ntrb=4, wind_noise=False)
+    m = fimhip.Model(g, NVL, NTRA, dt, th, ds)
+    try:
+        m.set_class_b(4, 2)
+        m.load(st)
+        m.init_state(False)
+        o = O.OracleRun(O.grid_arrays(g, NVL, dt, th, ds), NVL, NTRA, dt, pow_kind=1, ntrb=4, dtratio=2)
+        o.load(st)
+        o.init(False)
+        for its in range(1, 12):
+            m.step(its, 1)
+            o.step(its, 1)
+        m.sync()
+        trb = m.download("trb")
+        for f in ("trb", "cumufx", "dpinit", "dp", "tr"):
+            assert _same(m.download(f), o.a[f]), f
+        assert trb.min() >= 0.
+        for t in range(4):
+            assert trb[t].max() <= st["trb"][t].max() * (1 + 1e-12), t
+    finally:
+        m.close()

@@ -8,9 +8,11 @@ import pytest
 from helpers import golden_input, golden_ref, golden_grid, sha, seeded_case, rel_l2
 
 
-def _oracle_run(O, g, st, th, ds, dt, pow_kind=0, ntra=4, flux_scheme=1):
+def _oracle_run(O, g, st, th, ds, dt, pow_kind=0, ntra=4, flux_scheme=1, dtratio=6):
     ga = O.grid_arrays(g, st["dp"].shape[1], dt, th, ds)
-    o = O.OracleRun(ga, st["dp"].shape[1], ntra, dt, pow_kind=pow_kind, flux_scheme=flux_scheme)
+    ntrb = 0 if st.get("trb") is None else st["trb"].shape[0]
+    o = O.OracleRun(ga, st["dp"].shape[1], ntra, dt, pow_kind=pow_kind, flux_scheme=flux_scheme,
+                    ntrb=ntrb, dtratio=dtratio)
     o.load(st)
     return o
 
@@ -144,6 +146,51 @@ def test_mass_adv_vs_live_reference_bitwise(oracle_lib, glvl_, mode, nsteps, dt)
     for its in range(1, nsteps + 1):
         oc.step(its, mode)
     assert not np.array_equal(oc.a["dp"], o.a["dp"])
+
+
+def _class_b_state(g, mode, ntrb=2):
+    """Seeded state plus ntrb class-B tracers: log-normal fields (median 1e-9)
+    times a cosine bell, from the synthetic generator's tracers 2.."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    flags = fimhip.SYNTH_WIND_NOISE if mode == 0 else 0
+    st = fimhip.synth_state(g.lat, g.lon, 64, 4, seed=20240601, member=0, flags=flags)
+    lg = fimhip.synth_state(g.lat, g.lon, 64, 1 + ntrb, seed=7, member=3, flags=fimhip.SYNTH_LOGNORMAL)
+    st["trb"] = np.ascontiguousarray(lg["tr"][1:1 + ntrb])
+    return st
+
+
+@pytest.mark.parametrize("glvl_, mode, nsteps, dt, dtratio, flux_scheme",
+                         [(3, 0, 10, 240.0, 3, 1), (5, 1, 11, 480.0, 2, 1), (3, 0, 7, 240.0, 2, 2)])
+def test_class_b_transport_vs_live_reference_bitwise(oracle_lib, glvl_, mode, nsteps, dt, dtratio, flux_scheme):
+    """Class-B tracers (ntrb = 2) moved every dtratio steps by transp2/fct3d
+    with the mass fluxes integrated by transp1 (transp3d.F90, fct3d.F90),
+    against the reference Fortran itself: the class-B tracers, the integrated
+    mass flux and dpinit after the last step, and the class-A state, bit for
+    bit."""
+    from oracle import refrun
+    from fimocn_subx_realtime_hera_apr2020_amd import glvl, state
+    if not refrun.available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    O = oracle_lib
+    path = refrun.ref_glvl(glvl_)
+    g = glvl.read_glvl(path)
+    th, ds = state.vertical_tables(64)
+    st = _class_b_state(g, mode)
+    r = refrun.run_dynref(path, st, th, ds, nsteps=nsteps, dt=dt, mode=mode, init_hybgen=(mode == 0),
+                          flux_scheme=flux_scheme, dtratio=dtratio)
+    o = _oracle_run(O, g, st, th, ds, dt, flux_scheme=flux_scheme, dtratio=dtratio)
+    o.init(mode == 0)
+    for its in range(1, nsteps + 1):
+        o.step(its, mode)
+    for f in ("us", "vs", "dp", "pr", "ex"):
+        assert np.array_equal(o.a[f].ravel(), r["final." + f]), f
+    for t in range(2):
+        assert np.array_equal(o.a["trb"][t].ravel(), r["final.trb"][t]), ("trb", t)
+    assert np.array_equal(o.a["cumufx"].ravel(), r["final.cumufx"])
+    assert np.array_equal(o.a["dpinit"].ravel(), r["final.dpinit"])
+    assert np.abs(o.a["cumufx"]).max() > 0.     # nsteps is not a multiple of dtratio
+    # the transport moved the tracers
+    assert not np.array_equal(o.a["trb"], st["trb"])
 
 
 def test_project_pow_drift_within_bound(oracle_lib):
