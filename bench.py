@@ -25,7 +25,8 @@ Prints ONE JSON line.  Besides the contract fields it carries
                 in FP64)
   plain_fp64:   the FP64 path's throughput, roofline and step_hbm; under
                 mass_adv the same step with the flux-limited mass transport
-                (flux_conserv_schm = 2) and its two kernels
+                (flux_conserv_schm = 2) and its two kernels; under class_b the
+                step with 8 class-B tracers moved every 6 steps by fct3d
   accuracy_96:  relative L2 of member 0 against FP64 after 96 steps, mass drift
   cpu_baseline: the C restatement (oracle/fim_oracle.c, FP64, OpenMP) on a
                 bounded sample of the same grid
@@ -71,6 +72,20 @@ def plain_kernel_bytes(T):
     return {k: 8 * n for k, n in v.items()}
 
 
+def class_b_kernel_bytes(B):
+    """Compulsory bytes per cell-level of the class-B transport launches (FP64)
+    for B class-B tracers; arrays shared by the tracers of a launch counted once."""
+    v = {
+        "k_transp1": 6 + 18 + 6,             # cumufx, massfx (3 slots) -> cumufx
+        "k_transp2_w": 6 + 2 + 1,            # cumufx, dpinit, dp -> w
+        "k_transp0": 1 + 1 + 6,              # dp -> dpinit, cumufx
+        "k_fct3d_vert": 2 + 3 * B,           # dpinit, w; per tracer trb -> filled field, vertical flux
+        "k_fct3d_lo": 9 + 13 * B,            # cumufx w dpinit dp; per tracer field, vfx -> uan(6) bounds update ratios
+        "k_fct3d_clip": 1 + 12 * B,          # dp; per tracer ratios, uan(6), update, bounds -> trb
+    }
+    return {k: 8 * n for k, n in v.items()}
+
+
 def ens_kernel_bytes(T):
     """The same for the mixed-precision kernels (4 B per FP32 value, 8 B per
     FP64 value; inputs shared by the tracers of one launch counted once)."""
@@ -112,6 +127,8 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=96, help="0 skips the accuracy leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the mass_adv and class-B legs")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (for PMC byte calibration)")
@@ -207,8 +224,48 @@ def run_plain(a, grid, th, ds, fimhip):
            "step_hbm": {"bytes_per_cell_level": 8 * v_step, "achieved_GBs": value * 8 * v_step / 1e9,
                         "frac": value * 8 * v_step / HBM_PEAK},
            "kernels": kernel_table(prof, kb, cells)}
-    res["mass_adv"] = run_plain_mass_adv(a, m, cells, kb, fimhip)
+    if not a.headline_only:
+        res["mass_adv"] = run_plain_mass_adv(a, m, cells, kb, fimhip)
+        res["class_b"] = run_plain_class_b(a, m, grid, cells, fimhip)
     return m, res
+
+
+def run_plain_class_b(a, m, grid, cells, fimhip, ntrb=8, dtratio=6):
+    """The same FP64 step with ntrb class-B tracers (log-normal fields, the
+    chemistry-load case) moved every dtratio steps by transp2/fct3d: step time
+    over 2 * dtratio steps and the class-B kernels."""
+    import torch
+    m.set_class_b(ntrb, dtratio)
+    m.synth(seed=SEED, member=0, flags=fimhip.SYNTH_WIND_NOISE | fimhip.SYNTH_PERTURB)
+    lg = fimhip.synth_state(grid.lat, grid.lon, a.nvl, 1 + ntrb, seed=7, member=3, flags=fimhip.SYNTH_LOGNORMAL)
+    m.upload("trb", lg["tr"][1:])
+    del lg
+    m.init_state(True)
+    its = 0
+    for _ in range(dtratio):
+        its += 1
+        m.step(its, a.mode)
+    m.sync()
+    torch.cuda.synchronize()
+    n_timed = 2 * dtratio
+    t0 = time.perf_counter()
+    for _ in range(n_timed):
+        its += 1
+        m.step(its, a.mode)
+    m.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    m.profile(True)
+    for _ in range(n_timed):
+        its += 1
+        m.step(its, a.mode)
+    m.sync()
+    prof = m.profile_report()
+    m.profile(False)
+    m.set_class_b(0, dtratio)
+    kt = kernel_table(prof, class_b_kernel_bytes(ntrb), cells)
+    return {"ntrb": ntrb, "dtratio": dtratio, "ms_per_step": wall / n_timed * 1e3, "timed_steps": n_timed,
+            "kernels": {k: v for k, v in kt.items() if k.startswith(("k_transp", "k_fct3d"))}}
 
 
 def run_plain_mass_adv(a, m, cells, kb, fimhip):
@@ -286,7 +343,8 @@ def run_ens(a, grid, th, ds, fimhip):
            "step_hbm": {"bytes_per_cell_level": b_step, "achieved_GBs": value * b_step / 1e9,
                         "frac": value * b_step / HBM_PEAK},
            "kernels": kernel_table(prof, kb, cells), "tiles": tune}
-    res["mass_adv"] = run_ens_mass_adv(a, e, cells, kb, fimhip)
+    if not a.headline_only:
+        res["mass_adv"] = run_ens_mass_adv(a, e, cells, kb, fimhip)
     return e, res
 
 

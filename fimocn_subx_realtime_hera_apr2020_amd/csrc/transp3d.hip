@@ -11,8 +11,8 @@
 // -ffp-contract=off), so the results have the bytes of the reference
 // Fortran built with -fdefault-real-8.  Mapping: one wavefront per cell
 // column, lane = level, blockIdx.y = class-B tracer; the column-serial parts
-// (the vertical running sum of transp2 and the two massless-cell sweeps of
-// fct3d) run in lane 0 over the column held in LDS.
+// (the vertical running sum of transp2) run in lane 0 over the column held
+// in LDS; the two massless-cell sweeps of fct3d run one thread per column.
 #include "fimhip_internal.h"
 
 namespace fimhip {
@@ -87,10 +87,76 @@ __global__ void __launch_bounds__(256) k_transp2_w(DevGrid g, DevState s, DevCla
   if (live) C2(b.w, ipn, k) = sh[c][k];
 }
 
+// fct3d (fct3d.F90:127-140): massless cells take their values from the
+// layers above and below, in two dependent sweeps (downward, then upward).
+// One thread per column and class-B tracer (64 columns per wavefront) with
+// the column in registers; the column data pass through LDS in chunks of 16
+// levels so that every global load and store instruction covers whole cache
+// lines (16 levels of 4 columns).  Needs nvl == NVL.
+template <int NVL>
+__global__ void __launch_bounds__(64) k_fct3d_fill(DevGrid g, DevClassB b) {
+  constexpr int CH = 16, NCH = NVL / CH;
+  static_assert(NVL % CH == 0, "whole chunks");
+  __shared__ double sf[CH][65], sth[CH][65];
+  const int nip = g.nip, nvl = NVL;
+  const int lane = threadIdx.x, c0 = blockIdx.x * 64, n = blockIdx.y;
+  const double *trb = b.trb + (size_t)n * nip * nvl;
+  double *fld = b.fld + (size_t)n * nip * nvl;
+  double x[NVL];
+#pragma unroll
+  for (int ch = NCH - 1; ch >= 0; ch--) {       // downward sweep, top chunk first
+    const int k0 = ch * CH;
+    __syncthreads();
+    for (int i = 0; i < 16; i++) {
+      const int cc = i * 4 + (lane >> 4), kk = lane & 15, gc = c0 + cc;
+      const bool in = gc < nip;
+      sf[kk][cc] = in ? trb[(size_t)gc * nvl + k0 + kk] : 0.;
+      sth[kk][cc] = in ? b.dpinit[(size_t)gc * nvl + k0 + kk] : 1.;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = CH - 1; kk >= 0; kk--) {
+      const int k = k0 + kk;
+      if (k == NVL - 1) {
+        x[k] = sf[kk][lane];
+      } else {
+        const double t = sth[kk][lane];
+        x[k] = (sf[kk][lane] * t + x[k + 1] * ONEMU) / (t + ONEMU);
+      }
+    }
+  }
+#pragma unroll
+  for (int ch = 0; ch < NCH; ch++) {            // upward sweep, bottom chunk first
+    const int k0 = ch * CH;
+    __syncthreads();
+    for (int i = 0; i < 16; i++) {
+      const int cc = i * 4 + (lane >> 4), kk = lane & 15, gc = c0 + cc;
+      sth[kk][cc] = gc < nip ? b.dpinit[(size_t)gc * nvl + k0 + kk] : 1.;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < CH; kk++) {
+      const int k = k0 + kk;
+      if (k > 0) {
+        const double t = sth[kk][lane];
+        x[k] = (x[k] * t + x[k - 1] * ONEMU) / (t + ONEMU);
+      }
+      sf[kk][lane] = x[k];
+    }
+    __syncthreads();
+    for (int i = 0; i < 16; i++) {
+      const int cc = i * 4 + (lane >> 4), kk = lane & 15, gc = c0 + cc;
+      if (gc < nip) fld[(size_t)gc * nvl + k0 + kk] = sf[kk][cc];
+    }
+  }
+}
+
 // fct3d, first part (fct3d.F90:127-260): massless cells filled from the
-// layers above and below (two serial sweeps), the piecewise parabolic fit of
-// each layer, and the vertical tracer flux over the upstream slab of
+// layers above and below (two serial sweeps, in lane 0, unless FILLED: then
+// k_fct3d_fill has left the filled field in fld), the piecewise parabolic fit
+// of each layer, and the vertical tracer flux over the upstream slab of
 // thickness |w|.  Writes the filled field and the vertical flux.
+template <bool FILLED>
 __global__ void __launch_bounds__(256) k_fct3d_vert(DevGrid g, DevClassB b) {
   __shared__ double sf[COLS][NVL_MAX], sth[COLS][NVL_MAX];
   __shared__ double sa[COLS][NVL_MAX], sb[COLS][NVL_MAX], sc[COLS][NVL_MAX];
@@ -98,13 +164,13 @@ __global__ void __launch_bounds__(256) k_fct3d_vert(DevGrid g, DevClassB b) {
   const int k = threadIdx.x, c = threadIdx.y, ipn = wave_col(), n = blockIdx.y;
   const bool live = ipn < nip && k < nvl;
   if (live) {
-    sf[c][k] = TB(b.trb, n, ipn, k);
+    sf[c][k] = FILLED ? TB(b.fld, n, ipn, k) : TB(b.trb, n, ipn, k);
     sth[c][k] = C2(b.dpinit, ipn, k);
   }
   __syncthreads();
   double *f = sf[c];
   const double *th = sth[c];
-  if (k == 0 && ipn < nip) {
+  if (!FILLED && k == 0 && ipn < nip) {
     for (int kk = nvl - 2; kk >= 0; kk--) f[kk] = (f[kk] * th[kk] + f[kk + 1] * ONEMU) / (th[kk] + ONEMU);
     for (int kk = 1; kk < nvl; kk++) f[kk] = (f[kk] * th[kk] + f[kk - 1] * ONEMU) / (th[kk] + ONEMU);
   }
@@ -178,7 +244,7 @@ __global__ void __launch_bounds__(256) k_fct3d_vert(DevGrid g, DevClassB b) {
     vfx = wk * amount / slab;
   }
   TB(b.vfx, n, ipn, k) = vfx;
-  TB(b.fld, n, ipn, k) = f[k];
+  if (!FILLED) TB(b.fld, n, ipn, k) = f[k];
 }
 
 // fct3d, second part (fct3d.F90:276-370): low-order and antidiffusive
@@ -273,7 +339,12 @@ void launch_transp1(const DevGrid &g, const DevState &s, const DevClassB &b, con
 void launch_transp2(const DevGrid &g, const DevState &s, const DevClassB &b, hipStream_t st) {
   KT_LAUNCH("k_transp2_w", st, k_transp2_w<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, b));
   if (b.ntrb < 1) return;
-  KT_LAUNCH("k_fct3d_vert", st, k_fct3d_vert<<<col_grid(g.nip, b.ntrb), col_block(), 0, st>>>(g, b));
+  if (g.nvl == NVL_MAX) {
+    KT_LAUNCH("k_fct3d_fill", st, k_fct3d_fill<NVL_MAX><<<dim3((g.nip + 63) / 64, b.ntrb), 64, 0, st>>>(g, b));
+    KT_LAUNCH("k_fct3d_vert", st, k_fct3d_vert<true><<<col_grid(g.nip, b.ntrb), col_block(), 0, st>>>(g, b));
+  } else {
+    KT_LAUNCH("k_fct3d_vert", st, k_fct3d_vert<false><<<col_grid(g.nip, b.ntrb), col_block(), 0, st>>>(g, b));
+  }
   KT_LAUNCH("k_fct3d_lo", st, k_fct3d_lo<<<col_grid(g.nip, b.ntrb), col_block(), 0, st>>>(g, s, b));
   KT_LAUNCH("k_fct3d_clip", st, k_fct3d_clip<<<col_grid(g.nip, b.ntrb), col_block(), 0, st>>>(g, s, b));
 }
