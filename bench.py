@@ -114,8 +114,8 @@ def ens_kernel_bytes(T):
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--glvl", type=int, default=7)
     ap.add_argument("--nvl", type=int, default=64)
     ap.add_argument("--ntra", type=int, default=4)
@@ -207,7 +207,10 @@ def run_plain(a, grid, th, ds, fimhip):
     m.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    # per-kernel event timing over as many further steps
+    # per-kernel event timing over as many further steps, from a fresh state
+    m.synth(seed=SEED, member=0, flags=fimhip.SYNTH_WIND_NOISE | fimhip.SYNTH_PERTURB)
+    m.init_state(True)
+    its = 0
     m.profile(True)
     for _ in range(a.steps):
         its += 1

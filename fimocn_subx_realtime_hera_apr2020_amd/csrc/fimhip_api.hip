@@ -207,6 +207,14 @@ extern "C" int fimhip_set_ab(fimhip_ctx *c, int nf, int of, int vof, double a1, 
   return FIMHIP_OK;
 }
 
+extern "C" int fimhip_get_ab(const fimhip_ctx *c, int *nf, int *of, int *vof, double *a1, double *a2,
+                             double *a3) {
+  if (!c || !nf || !of || !vof || !a1 || !a2 || !a3) return FIMHIP_EINVAL;
+  *nf = c->ab.nf; *of = c->ab.of; *vof = c->ab.vof;
+  *a1 = c->ab.ab1; *a2 = c->ab.ab2; *a3 = c->ab.ab3;
+  return FIMHIP_OK;
+}
+
 extern "C" int fimhip_set_class_b(fimhip_ctx *c, int ntrb, int dtratio) {
   if (!c || ntrb < 0 || ntrb > NTRA_MAX || dtratio < 1) return FIMHIP_EINVAL;
   HIPCHK(hipSetDevice(c->device));

@@ -30,7 +30,8 @@ FIELDS = ["us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp", "ws", "relvor"
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 
 # symbols declared in include/fimhip.h
-EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab", "fimhip_edgvar1_r8",
+EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab", "fimhip_get_ab",
+           "fimhip_edgvar1_r8",
            "fimhip_edgvar3_r8", "fimhip_cnuity_r8", "fimhip_mass_adv_r8", "fimhip_set_flux_scheme",
            "fimhip_set_class_b", "fimhip_transp0_r8", "fimhip_transp1_r8", "fimhip_transp2_r8",
            "fimhip_trcadv_r8", "fimhip_momtum_r8",
@@ -78,6 +79,7 @@ def load_library() -> ct.CDLL:
     lib.fimhip_destroy.argtypes = [vp]
     lib.fimhip_abstart.argtypes = [vp, ct.c_int, ct.c_int, _PI, _PI, _PI, _PD, _PD, _PD]
     lib.fimhip_set_ab.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_double]
+    lib.fimhip_get_ab.argtypes = [vp, _PI, _PI, _PI, _PD, _PD, _PD]
     lib.fimhip_edgvar1_r8.argtypes = [vp, ct.c_int] + [_PD] * 16
     lib.fimhip_edgvar3_r8.argtypes = [vp, ct.c_int] + [_PD] * 16
     lib.fimhip_cnuity_r8.argtypes = [vp, ct.c_int] + [_PD] * 13 + [ct.c_int]
@@ -239,6 +241,19 @@ class Instance:
                                      ct.byref(a[0]), ct.byref(a[1]), ct.byref(a[2])), "abstart")
         self.nf, self.of, self.vof = nf.value, of.value, vof.value
         self.adbash = [x.value for x in a]
+
+    def set_ab(self, nf: int, of: int, vof: int, adbash1: float, adbash2: float, adbash3: float):
+        """Set the instance's AB slots (1-based) and weights (a restart)."""
+        _chk(self.lib.fimhip_set_ab(self.h, nf, of, vof, adbash1, adbash2, adbash3), "set_ab")
+        self.nf, self.of, self.vof = nf, of, vof
+        self.adbash = [adbash1, adbash2, adbash3]
+
+    def get_ab(self):
+        """(nf, of, vof, adbash1, adbash2, adbash3) the instance holds now."""
+        i = [ct.c_int() for _ in range(3)]
+        d = [ct.c_double() for _ in range(3)]
+        _chk(self.lib.fimhip_get_ab(self.h, *[ct.byref(x) for x in i + d]), "get_ab")
+        return tuple(x.value for x in i + d)
 
     def edgvar1(self, its, u_vel, v_vel, delp, pres, exner, geop, montg, tracr, u_edg, v_edg, dp_edg,
                 lp_edg, geop_edg, mont_edg, uvsq_edg, trc_edg):

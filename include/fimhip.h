@@ -59,6 +59,10 @@ int fimhip_abstart(fimhip_ctx *ctx, int its, int itsDFI, int *nf, int *of, int *
                    double *adbash1, double *adbash2, double *adbash3);
 /* set the AB slots and weights explicitly (restart, or a caller that keeps them) */
 int fimhip_set_ab(fimhip_ctx *ctx, int nf, int of, int vof, double adbash1, double adbash2, double adbash3);
+/* the instance's current AB slots and weights (module_variables.F90 nf, of,
+ * vof, adbash1..3), e.g. for a restart file (restart_dyn.F90:117) */
+int fimhip_get_ab(const fimhip_ctx *ctx, int *nf, int *of, int *vof, double *adbash1, double *adbash2,
+                  double *adbash3);
 
 /* edgvar1.F90:27 (eqwgt, janjic=0, hiOrderFluxComp) - geop_edg and dp_edg
  * are not written (the reference leaves them untouched on this path). */

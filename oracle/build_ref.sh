@@ -7,6 +7,8 @@
 #                                          as PREP_FFLAGS in macros.make.linuxpcgnu:25)
 #   oracle/_ref/bin/dynref_r8             reference dynamics hot path, -fdefault-real-8
 #   oracle/_ref/bin/dynref_r4             reference dynamics hot path, default real 4
+#   oracle/_ref/bin/restref_r4            reference restart writer/reader of the
+#                                          dynamics section, default real 4
 #
 # The two dynref programs link the reference's own modules (cnuity, mass_adv
 # and its flux limiter, trcadv, the class-B transport transp3d/fct3d, edgvar1,
@@ -79,4 +81,13 @@ for prec in r8 r4; do
   for f in $DYN_SRC "$HERE/ref/dynref.F90"; do b=$(basename "${f%.*}"); objs="$objs $OD/$b.o"; done
   $FC -o "$OUT/bin/dynref_$prec" $objs
 done
+
+# ---- restart writer/reader of the dynamics section (default real 4) -------
+RST_SRC="$S/cntl/module_sfc_variables.F90 $H/restart_dyn.F90 $H/writearr32_r.F90 $H/readarr32_r.F90"
+OD=$OUT/obj_r4
+FL="-cpp -DSERIAL -O2 -ffp-contract=off -I$H -I$S/dummygptl"
+compile "$OD" "$FL" $RST_SRC "$HERE/ref/restref.F90"
+objs=""
+for f in $DYN_SRC $RST_SRC "$HERE/ref/restref.F90"; do b=$(basename "${f%.*}"); objs="$objs $OD/$b.o"; done
+$FC -o "$OUT/bin/restref_r4" $objs
 echo "build_ref: reference programs in $OUT/bin"

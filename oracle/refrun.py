@@ -121,3 +121,45 @@ def run_dynref(glvl_path, st, thetac, dpsig, *, nsteps, dt, mode=0, dumpstep=0,
     os.remove(fin)
     os.remove(fout)
     return rec
+
+
+def run_restref_write(values: bytes, workdir=None) -> bytes:
+    """The restart file the reference write_restart_dyn writes for a value
+    stream (oracle/ref/restref.F90)."""
+    workdir = workdir or os.path.join(REF, "runs")
+    os.makedirs(workdir, exist_ok=True)
+    fin = os.path.join(workdir, "rvals_%d.bin" % os.getpid())
+    fout = os.path.join(workdir, "rfile_%d.bin" % os.getpid())
+    with open(fin, "wb") as f:
+        f.write(values)
+    r = subprocess.run([os.path.join(BIN, "restref_r4"), "write", fin, fout], cwd=workdir, capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError("restref write failed (%d): %s" % (r.returncode, (r.stdout + r.stderr)[-2000:]))
+    with open(fout, "rb") as f:
+        data = f.read()
+    os.remove(fin)
+    os.remove(fout)
+    return data
+
+
+def run_restref_read(restart_bytes: bytes, dims, workdir=None) -> bytes:
+    """The value stream the reference read_restart_dyn reads from a restart
+    file with sizes dims = (nvl, nip, ntra, ntrb)."""
+    workdir = workdir or os.path.join(REF, "runs")
+    os.makedirs(workdir, exist_ok=True)
+    fin = os.path.join(workdir, "rfile_in_%d.bin" % os.getpid())
+    fout = os.path.join(workdir, "rvals_out_%d.bin" % os.getpid())
+    with open(fin, "wb") as f:
+        f.write(restart_bytes)
+    with open(fin + ".dims", "wb") as f:
+        f.write(struct.pack("<4i", *dims))
+    r = subprocess.run([os.path.join(BIN, "restref_r4"), "read", fin, fout], cwd=workdir, capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError("restref read failed (%d): %s" % (r.returncode, (r.stdout + r.stderr)[-2000:]))
+    with open(fout, "rb") as f:
+        data = f.read()
+    for p in (fin, fin + ".dims", fout):
+        os.remove(p)
+    return data

@@ -438,3 +438,55 @@ def test_g5_class_b_transport_dt480_matches_checker_bitwise(oracle_lib):
             assert trb[t].max() <= st["trb"][t].max() * (1 + 1e-12), t
     finally:
         m.close()
+
+
+def test_restart_round_trip_then_steps_match_checker_bitwise(case, tmp_path):
+    """After 4 device steps (2 class-B tracers), the state goes to a restart
+    file in the reference format (rounded to real*4, as the reference stores
+    it), is read back into a fresh instance, and 4 more steps match the
+    checker run from the same restart contents, bit for bit; the restart
+    holds the AB slots and weights of step 4."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, restart
+    O, g, st0, th, ds, ga = case
+    print("case: G3 nvl=64 ntra=4 ntrb=2, restart after 4 steps then 4 more", flush=True)
+    st = _class_b_case(g)
+    m = fimhip.Model(g, NVL, NTRA, DT, th, ds)
+    m2 = fimhip.Model(g, NVL, NTRA, DT, th, ds)
+    try:
+        m.set_class_b(2, 3)
+        m.load(st)
+        m.init_state(True)
+        for its in range(1, 5):
+            m.step(its, 0)
+        m.sync()
+        rd = restart.update_from(m, restart.RestartDyn.empty(NVL, g.nip, NTRA + 2))
+        assert rd.ab[:3] == m.get_ab()[:3] and rd.ab[0] == 1   # slots after 4 rotations from (0, 2, 1)
+        path = str(tmp_path / "restart_its4")
+        restart.write_restart_dyn(path, rd)
+        rd2 = restart.read_restart_dyn(path, NTRA + 2)
+        m2.set_class_b(2, 3)
+        restart.load_into(m2, rd2)
+        assert m2.get_ab() == tuple(float(x) if i >= 3 else x for i, x in enumerate(rd2.ab))
+        o = O.OracleRun(ga, NVL, NTRA, DT, pow_kind=1, ntrb=2, dtratio=3)
+        f = {k: v.astype(np.float64) for k, v in rd2.fields.items()}
+        for k in ("us", "vs", "dp", "pr", "ex", "ph", "mp", "ws", "relvor", "sdot", "u_tdcy", "v_tdcy",
+                  "dp_tdcy", "dpl_tdcy", "massfx", "cumufx", "dpinit"):
+            o.a[k][...] = f[k]
+        o.a["tr"][...] = f["tr"][:NTRA]
+        o.a["trdp"][...] = f["trdp"][:NTRA]
+        o.a["trc_tdcy"][...] = f["trc_tdcy"][:NTRA]
+        o.a["trl_tdcy"][...] = f["trl_tdcy"][:NTRA]
+        o.a["trb"][...] = f["tr"][NTRA:]
+        o.a["psrf"][...] = rd2.header["psrf"].astype(np.float64)
+        o.a["ptdcy"][...] = rd2.header["ptdcy"].astype(np.float64).reshape(2, g.nip)
+        o.s.nf, o.s.of, o.s.vof, o.s.ab1, o.s.ab2, o.s.ab3 = rd2.ab
+        for its in range(5, 9):
+            m2.step(its, 0)
+            o.step(its, 0)
+        m2.sync()
+        for k in ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp", "trb", "cumufx", "dpinit", "psrf"):
+            got = m2.download(k)
+            assert _same(got, o.a[k]), (k, rel_l2(got, o.a[k]))
+    finally:
+        m.close()
+        m2.close()
