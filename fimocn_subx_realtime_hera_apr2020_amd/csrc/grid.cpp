@@ -609,10 +609,10 @@ static void pseudo_inverse(const float *xy, int n, float *pinv) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-extern "C" int fimhip_icos_grid(int glvl, float *lat_out, float *lon_out, int *nprox_out, int *prox_out,
-                                int *proxs_out, float *area_out, float *cs_out, float *sn_out,
-                                float *svc_out, float *sve_out, float *sideln_out, float *rprox_out,
-                                float *corner_out, float *hfcoef_out, int *inv_perm_out) {
+static int icos_grid_impl(int glvl, float *lat_out, float *lon_out, int *nprox_out, int *prox_out,
+                          int *proxs_out, float *area_out, float *cs_out, float *sn_out, float *svc_out,
+                          float *sve_out, float *sideln_out, float *rprox_out, float *corner_out,
+                          float *hfcoef_out, int *inv_perm_out, float *edge_out) {
   if (glvl < 0 || glvl > 10) return FIMHIP_EINVAL;
   // ---- icos.F90: grid points -------------------------------------------
   GPWnb top[12];
@@ -907,5 +907,25 @@ extern "C" int fimhip_icos_grid(int glvl, float *lat_out, float *lon_out, int *n
   if (sve_out) memcpy(sve_out, sve.data(), sve.size() * sizeof(float));
   if (corner_out) memcpy(corner_out, corner.data(), corner.size() * sizeof(float));
   if (hfcoef_out) memcpy(hfcoef_out, hf.data(), hf.size() * sizeof(float));
+  if (edge_out)
+    for (size_t i = 0; i < edge.size(); i++) edge_out[i] = (float)edge[i];
   return FIMHIP_OK;
+}
+
+extern "C" int fimhip_icos_grid(int glvl, float *lat_out, float *lon_out, int *nprox_out, int *prox_out,
+                                int *proxs_out, float *area_out, float *cs_out, float *sn_out,
+                                float *svc_out, float *sve_out, float *sideln_out, float *rprox_out,
+                                float *corner_out, float *hfcoef_out, int *inv_perm_out) {
+  return icos_grid_impl(glvl, lat_out, lon_out, nprox_out, prox_out, proxs_out, area_out, cs_out, sn_out, svc_out,
+                        sve_out, sideln_out, rprox_out, corner_out, hfcoef_out, inv_perm_out, nullptr);
+}
+
+// The ginfo output (GridGen.F90:437-497, single precision): cell centres in
+// radians, the pattern neighbours (prox(6,nip), slot 6 of a pentagon as the
+// pattern leaves it), nprox, the Voronoi corner of each edge end
+// (icos_edge(6,2,2,nip): edge, end, lat/lon) and inv_perm.
+extern "C" int fimhip_icos_grid_info(int glvl, float *lat_out, float *lon_out, int *prox_out, int *nprox_out,
+                                     float *edge_out, int *inv_perm_out) {
+  return icos_grid_impl(glvl, lat_out, lon_out, nprox_out, prox_out, nullptr, nullptr, nullptr, nullptr, nullptr,
+                        nullptr, nullptr, nullptr, nullptr, nullptr, inv_perm_out, edge_out);
 }

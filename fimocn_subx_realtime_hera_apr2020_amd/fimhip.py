@@ -37,6 +37,7 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_trcadv_r8", "fimhip_momtum_r8",
            "fimhip_hybgen_r8", "fimhip_hystat_r8", "fimhip_field_size", "fimhip_upload", "fimhip_download",
            "fimhip_init_state", "fimhip_step", "fimhip_sync", "fimhip_globsum", "fimhip_icos_grid",
+           "fimhip_icos_grid_info",
            "fimhip_synth_state", "fimhip_synth_device", "fimhip_profile", "fimhip_profile_report",
            "fimhip_pow", "fimhip_version", "fimhip_ens_create", "fimhip_ens_destroy", "fimhip_ens_synth",
            "fimhip_ens_load_member", "fimhip_ens_init_state", "fimhip_ens_step", "fimhip_ens_use_graph",
@@ -105,6 +106,7 @@ def load_library() -> ct.CDLL:
     _PF = ct.POINTER(ct.c_float)
     lib.fimhip_icos_grid.argtypes = [ct.c_int, _PF, _PF, _PI, _PI, _PI, _PF, _PF, _PF, _PF, _PF, _PF, _PF,
                                      _PF, _PF, _PI]
+    lib.fimhip_icos_grid_info.argtypes = [ct.c_int, _PF, _PF, _PI, _PI, _PF, _PI]
     lib.fimhip_synth_state.argtypes = [ct.c_int, ct.c_int, ct.c_int, _PD, _PD, ct.c_ulonglong, ct.c_int,
                                        ct.c_int] + [_PD] * 7
     lib.fimhip_synth_device.argtypes = [vp, ct.c_ulonglong, ct.c_int, ct.c_int]
@@ -165,6 +167,24 @@ def icos_grid(level: int) -> _glvl.Grid:
     ptrs = [a[n].ctypes.data_as(_PI if a[n].dtype == np.int32 else ct.POINTER(ct.c_float)) for n in order]
     _chk(lib.fimhip_icos_grid(level, *ptrs), "fimhip_icos_grid")
     return _glvl.Grid(level, 0, **a)
+
+
+def icos_grid_info(level: int) -> _glvl.GridInfo:
+    """Generate the contents of the ginfo file icos_grid_info_level.dat
+    (fimhip_icos_grid_info)."""
+    lib = load_library()
+    nip = _glvl.nip_of(level)
+    lat, lon = np.zeros(nip, np.float32), np.zeros(nip, np.float32)
+    prox, nprox = np.zeros((nip, 6), np.int32), np.zeros(nip, np.int32)
+    edge, inv = np.zeros((nip, 2, 2, 6), np.float32), np.zeros(nip, np.int32)
+    fp = lambda a: a.ctypes.data_as(ct.POINTER(ct.c_float))
+    _chk(lib.fimhip_icos_grid_info(level, fp(lat), fp(lon), _p32(prox), _p32(nprox), fp(edge), _p32(inv)),
+         "fimhip_icos_grid_info")
+    return _glvl.GridInfo(level, 0, lat, lon, prox, nprox, edge, inv)
+
+
+def _p32(a):
+    return a.ctypes.data_as(_PI)
 
 
 def synth_state(lat, lon, nvl: int, ntra: int, seed: int = 20240601, member: int = 0, flags: int = 0) -> dict:

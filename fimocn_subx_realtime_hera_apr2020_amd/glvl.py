@@ -202,3 +202,69 @@ def write_glvl(grid: Grid, path: str) -> None:
             f4(grid.hfcoef[:, isn, idx])
     with open(path, "wb") as f:
         f.write(bytes(out))
+
+
+# ---------------------------------------------------------------------------
+# icos_grid_info_level.dat: the ginfo output that getlvl reads
+# ---------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class GridInfo:
+    """The contents of icos_grid_info_level.dat (GridGen.F90:437-497, single
+    precision), the intermediate file between the prep programs ginfo and
+    getlvl."""
+    glvl: int
+    curve: int
+    lat: np.ndarray          # float32 [nip] (radians)
+    lon: np.ndarray          # float32 [nip]
+    prox: np.ndarray         # int32 [nip, 6]  pattern neighbours (1-based)
+    nprox: np.ndarray        # int32 [nip]
+    icos_edge: np.ndarray    # float32 [nip, 2, 2, 6]  Fortran icos_edge(edge, end, lat/lon, nip)
+    inv_perm: np.ndarray     # int32 [nip]
+
+    @property
+    def nip(self) -> int:
+        return int(self.lat.shape[0])
+
+
+def read_grid_info(path: str) -> GridInfo:
+    with open(path, "rb") as f:
+        buf = f.read()
+    recs = _records(buf)
+    hg = next(recs).decode()
+    hc = next(recs).decode()
+    ll = np.frombuffer(next(recs), dtype="<f4")
+    nip = ll.shape[0] // 2
+    i4 = lambda: np.frombuffer(next(recs), dtype="<i4").copy()
+    prox = np.stack([i4() for _ in range(6)], axis=1)
+    nprox = i4()
+    edge = np.empty((nip, 2, 2, 6), np.float32)
+    for i3 in range(2):
+        for i2 in range(2):
+            for isn in range(6):
+                edge[:, i3, i2, isn] = np.frombuffer(next(recs), dtype="<f4")
+    inv_perm = i4()
+    return GridInfo(int(hg[6:8]), int(hc[7:9]), ll[:nip].copy(), ll[nip:].copy(), prox, nprox, edge, inv_perm)
+
+
+def write_grid_info(info: GridInfo, path: str) -> None:
+    out = bytearray()
+
+    def rec(b: bytes):
+        out.extend(struct.pack("<i", len(b)))
+        out.extend(b)
+        out.extend(struct.pack("<i", len(b)))
+
+    rec(("glvl =%2d" % info.glvl).ljust(16).encode())
+    rec(("curve =%2d" % info.curve).ljust(16).encode())
+    rec(np.concatenate([np.asarray(info.lat, "<f4"), np.asarray(info.lon, "<f4")]).tobytes())
+    for isn in range(6):
+        rec(np.ascontiguousarray(info.prox[:, isn], dtype="<i4").tobytes())
+    rec(np.ascontiguousarray(info.nprox, dtype="<i4").tobytes())
+    for i3 in range(2):
+        for i2 in range(2):
+            for isn in range(6):
+                rec(np.ascontiguousarray(info.icos_edge[:, i3, i2, isn], dtype="<f4").tobytes())
+    rec(np.ascontiguousarray(info.inv_perm, dtype="<i4").tobytes())
+    with open(path, "wb") as f:
+        f.write(bytes(out))

@@ -170,6 +170,12 @@ int fimhip_profile_report(fimhip_ctx *ctx, char *buf, int len);
 int fimhip_icos_grid(int glvl, float *lat, float *lon, int *nprox, int *prox, int *proxs, float *area,
                      float *cs, float *sn, float *sidevec_c, float *sidevec_e, float *sideln,
                      float *rprox_ln, float *corner_xy, float *hfcoef, int *inv_perm);
+/* the ginfo intermediate file's contents (icos_grid_info_level.dat,
+ * GridGen.F90:437-497, single precision): lat, lon (nip, radians),
+ * prox (6,nip), nprox (nip), icos_edge (6,2,2,nip): the Voronoi corner at
+ * each end of each edge as (lat, lon), inv_perm (nip) */
+int fimhip_icos_grid_info(int glvl, float *lat, float *lon, int *prox, int *nprox, float *icos_edge,
+                          int *inv_perm);
 
 /* ---- synthetic seeded states --------------------------------------------
  * One member's state, bit-identical on the host and on the device: splitmix64

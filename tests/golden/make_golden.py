@@ -7,6 +7,8 @@ built oracle/_ref:
 Fixtures (data only):
   glvl_G{1,2,3}.dat     reference grid prep chain output (grid -> ginfo -> getlvl,
                         gtype=2, curve=0), the file dyn_init.F90 reads
+  icos_grid_info_G2.dat the reference ginfo output for G2 (icos_grid_info_level.dat,
+                        GridGen.F90:437-497), the file getlvl reads
   dyn_G2_input.npz      seeded FP64 input state on the G2 grid (nvl=64, ntra=4)
   dyn_G2_ref.json       SHA-256 of every output record of the reference FP64
                         build (dynref_r8) for 3 full steps (mode 0) and 3
@@ -40,6 +42,7 @@ def main():
         sys.exit("oracle/_ref not built")
     for lv in (1, 2, 3):
         shutil.copyfile(refrun.ref_glvl(lv), os.path.join(HERE, "glvl_G%d.dat" % lv))
+    shutil.copyfile(refrun.ref_file(2, "icos_grid_info_level.dat"), os.path.join(HERE, "icos_grid_info_G2.dat"))
     g = glvl.read_glvl(os.path.join(HERE, "glvl_G2.dat"))
     th, ds = state.vertical_tables(64)
     st = state.make_state(g.lat, g.lon, 64, 4, wind_noise=True)

@@ -60,3 +60,32 @@ def test_grid_geometry(level):
     # the closed polygon: edge vectors from the centre sum to ~0
     s = np.abs(g.sidevec_c.astype(np.float64).sum(axis=1)).max()
     assert s < 1e-3 * g.sideln.max()
+
+
+def test_grid_info_file_bitexact_vs_reference_fixture(tmp_path):
+    """icos_grid_info_level.dat (the ginfo output getlvl reads) written from
+    the product's generator is the reference ginfo's file byte for byte, and
+    reads back to the same contents."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "icos_grid_info_G2.dat")
+    info = fimhip.icos_grid_info(2)
+    out = tmp_path / "info.dat"
+    glvl.write_grid_info(info, str(out))
+    assert out.read_bytes() == open(path, "rb").read()
+    back = glvl.read_grid_info(path)
+    assert (back.glvl, back.curve, back.nip) == (2, 0, glvl.nip_of(2))
+    for f in ("lat", "lon", "prox", "nprox", "icos_edge", "inv_perm"):
+        assert np.array_equal(getattr(back, f), getattr(info, f)), f
+    # the glvl.dat tables of the same grid agree with the intermediate file
+    g = fimhip.icos_grid(2)
+    assert np.array_equal(g.lat, info.lat) and np.array_equal(g.nprox, info.nprox)
+
+
+@pytest.mark.parametrize("level", [3, 4, 5])
+def test_grid_info_file_live_reference(tmp_path, level):
+    from oracle import refrun
+    if not refrun.available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    out = tmp_path / "info.dat"
+    glvl.write_grid_info(fimhip.icos_grid_info(level), str(out))
+    assert out.read_bytes() == open(refrun.ref_file(level, "icos_grid_info_level.dat"), "rb").read()
