@@ -82,9 +82,14 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 // cell c's coefficients, trapezoid-weighted, the magnitude rescaled to the
 // weighted mean of the four magnitudes.  edgvar1 (normal component) and
 // momtum (kinetic energy, vorticity) both form edge winds through this one
-// function.
+// function.  The reference rescales by the magnitudes of the four rotated
+// winds; the rotation coefficients of a stencil cell satisfy cs^2 + sn^2 = 1
+// (to float rounding, within 3.4e-7 on the grid), so those are the
+// magnitudes of the cells' own winds (mi, mx, mt, ml), which the callers form
+// once per cell rather than once per edge and stencil position.
 __device__ __forceinline__ float2 e_edge_wind(const EnsGrid &g, int c, int e, float ui, float vi, float ux, float vx,
-                                              float ut, float vt, float ul, float vl) {
+                                              float ut, float vt, float ul, float vl, float mi, float mx, float mt,
+                                              float ml) {
 #pragma clang fp contract(fast)
   const float d6 = 1.f / 6.f, eps = 0.001f;
   const float c0 = FCS(c, e, 0), s0 = FSN(c, e, 0), c1 = FCS(c, e, 1), s1 = FSN(c, e, 1);
@@ -95,10 +100,8 @@ __device__ __forceinline__ float2 e_edge_wind(const EnsGrid &g, int c, int e, fl
   const float u3l = c3 * ul + s3 * vl, v3l = -s3 * ul + c3 * vl;
   float ue = (2.f * (u1 + u2) + u3trl + u3l) * d6;
   float ve = (2.f * (v1 + v2) + v3trl + v3l) * d6;
-  const float m1l = fsqrt(u1 * u1 + v1 * v1), m2l = fsqrt(u2 * u2 + v2 * v2);
-  const float m3t = fsqrt(u3trl * u3trl + v3trl * v3trl), m3l = fsqrt(u3l * u3l + v3l * v3l);
   const float m1 = fsqrt(ue * ue + ve * ve);
-  const float m2 = (2.f * (m1l + m2l) + m3t + m3l) * d6;
+  const float m2 = (2.f * (mi + mx) + mt + ml) * d6;
   const float sc = m2 * frcp(fmaxf(eps, m1));
   return make_float2(ue * sc, ve * sc);
 }
@@ -138,10 +141,15 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
   }
   const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k), thi = ETR(s.tr, 0, col, k);
   const double pi0 = EL(s.pr, col, k), pi1 = EL(s.pr, col, k + 1);
+  const float mi = fsqrt(ui * ui + vi * vi);
+  float mn[NP];
+#pragma unroll
+  for (int e = 0; e < NP; e++) mn[e] = fsqrt(nu[e] * nu[e] + nv[e] * nv[e]);
 #pragma unroll
   for (int e = 0; e < NP; e++) {
     const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
-    const float2 w = e_edge_wind(g, ipn, e, ui, vi, nu[e], nv[e], nu[et], nv[et], nu[el], nv[el]);
+    const float2 w = e_edge_wind(g, ipn, e, ui, vi, nu[e], nv[e], nu[et], nv[et], nu[el], nv[el], mi, mn[e], mn[et],
+                                 mn[el]);
     EE(s.vn_edg, col, e, k) = FSVE(ipn, e, 1) * w.x - FSVE(ipn, e, 0) * w.y;
     EE(s.lp_edg, col, e, k) =
         (float)((pi0 + pi1 + np0[e] + np1[e] + .5 * (np0[et] + np1[et] + np0[el] + np1[el])) * (1. / 6.));
@@ -773,10 +781,15 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
   float rv = 0.f;
   const double exm = .5 * (EL(s.ex, col, k) + EL(s.ex, col, k + 1));
   const double mpi = EC(s.mp, col, k);
+  const float mi = fsqrt(ui * ui + vi * vi);
+  float mn[NP];
+#pragma unroll
+  for (int e = 0; e < NP; e++) mn[e] = fsqrt(nu[e] * nu[e] + nv[e] * nv[e]);
 #pragma unroll
   for (int e = 0; e < NP; e++) {
     const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
-    const float2 w = e_edge_wind(g, ipn, e, ui, vi, nu[e], nv[e], nu[et], nv[et], nu[el], nv[el]);
+    const float2 w = e_edge_wind(g, ipn, e, ui, vi, nu[e], nv[e], nu[et], nv[et], nu[el], nv[el], mi, mn[e], mn[et],
+                                 mn[el]);
     const float ue = w.x, ve = w.y;
     const double mont = (2. * (mpi + nmp[e]) + nmp[et] + nmp[el]) * (1. / 6.);
     const double b = mont + (double)(.5f * (ue * ue + ve * ve));
