@@ -490,3 +490,41 @@ def test_restart_round_trip_then_steps_match_checker_bitwise(case, tmp_path):
     finally:
         m.close()
         m2.close()
+
+
+@pytest.mark.parametrize("flux_scheme,ntrb", [(1, 0), (2, 2)])
+def test_forty_layers_steps_match_checker_bitwise(oracle_lib, flux_scheme, ntrb):
+    """40 layers (the kernels' paths for nvl below the wavefront width: hybgen
+    with its serial sweeps in the column's wavefront, the fct3d sweeps in
+    lane 0): start-up plus 9 full steps identical to the checker."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    from helpers import seeded_case
+    O = oracle_lib
+    nvl = 40
+    g, st, th, ds = seeded_case(3, nvl=nvl)
+    print("case: G3 nvl=40 ntra=4 scheme=%d ntrb=%d 9 full steps" % (flux_scheme, ntrb), flush=True)
+    if ntrb:
+        st["trb"] = np.ascontiguousarray(_class_b_case(g, nvl=nvl, ntrb=ntrb)["trb"])
+    o = O.OracleRun(O.grid_arrays(g, nvl, DT, th, ds), nvl, NTRA, DT, pow_kind=1, flux_scheme=flux_scheme,
+                    ntrb=ntrb, dtratio=3)
+    o.load(st)
+    o.init(True)
+    m = fimhip.Model(g, nvl, NTRA, DT, th, ds)
+    try:
+        m.set_flux_scheme(flux_scheme)
+        if ntrb:
+            m.set_class_b(ntrb, 3)
+        m.load(st)
+        m.init_state(True)
+        for its in range(1, 10):
+            m.step(its, 0)
+            o.step(its, 0)
+        m.sync()
+        fields = ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp", "ws", "massfx", "sdot", "relvor")
+        if ntrb:
+            fields += ("trb", "cumufx", "dpinit")
+        for f in fields:
+            got = m.download(f)
+            assert _same(got, o.a[f]), (f, rel_l2(got, o.a[f]))
+    finally:
+        m.close()

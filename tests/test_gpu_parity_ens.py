@@ -351,3 +351,40 @@ def test_mass_adv_fp32_form_against_fp64_after_96_steps():
     finally:
         e.close()
         p.close()
+
+
+def test_forty_layers_fp32_form_against_fp64():
+    """40 layers in the batched form (hybgen's paths for nvl below the
+    wavefront width): member 0 after 24 full steps within BOUNDS_96 of the
+    FP64 path, mass conserved to 1e-12."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+    g = fimhip.icos_grid(4)
+    nvl = 40
+    th, ds = state.vertical_tables(nvl)
+    print("case: G4 nvl=40 FP32 form vs FP64, 24 full steps", flush=True)
+    e = fimhip.Ensemble(g, 2, nvl, NTRA, DT, th, ds)
+    p = fimhip.Model(g, nvl, NTRA, DT, th, ds)
+    try:
+        for m, sid in enumerate([0, 1]):
+            e.synth(m, seed=20240601, synth_member=sid, flags=FLAGS)
+        e.init_state(True)
+        p.synth(seed=20240601, member=0, flags=FLAGS)
+        p.init_state(True)
+        m0, _, _, _ = e.globsum()
+        e.step(24, 0)
+        for its in range(1, 25):
+            p.step(its, 0)
+        p.sync()
+        e.sync()
+        m1, _, _, _ = e.globsum()
+        errs = {n: rel_l2(e.download(0, n), p.download(n)) for n in ("us", "vs", "dp", "pr", "ex")}
+        tr_e, tr_p = e.download(0, "tr"), p.download("tr")
+        errs["tr0"] = rel_l2(tr_e[0], tr_p[0])
+        errs["tr"] = max(rel_l2(tr_e[t], tr_p[t]) for t in range(1, NTRA))
+        print("nvl=40 rel L2 after 24 steps:", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+        for n, b in BOUNDS_96.items():
+            assert errs[n] < b, (n, errs[n], b)
+        assert np.all(np.abs(m1 - m0) <= 1e-12 * m0)
+    finally:
+        e.close()
+        p.close()
