@@ -527,6 +527,178 @@ extern "C" int fimhip_hystat_r8(fimhip_ctx *c, int its, double *ph3d, const doub
 }
 
 // ---------------------------------------------------------------------------
+// real*4 drop-ins: the production FIM build has default real 4, so its
+// routines hand over float arrays.  Each _r4 entry widens the caller's arrays
+// into FP64 staging copies, runs the _r8 entry (FP64 arithmetic throughout),
+// and rounds the arrays the routine may write back to float.  Arrays the _r8
+// entry leaves untouched come back bit for bit (float -> double -> float is
+// exact).
+namespace {
+struct R4 {
+  float *p;
+  size_t n;
+  bool out;   // written by the routine: rounded back to the caller's array
+};
+template <size_t N, class F>
+int r4_call(const R4 (&a)[N], F &&f) {
+  std::vector<std::vector<double>> st(N);
+  double *d[N];
+  for (size_t i = 0; i < N; i++) {
+    if (!a[i].p) return FIMHIP_EINVAL;
+    st[i].assign(a[i].p, a[i].p + a[i].n);
+    d[i] = st[i].data();
+  }
+  const int r = f(d);
+  if (r) return r;
+  for (size_t i = 0; i < N; i++)
+    if (a[i].out)
+      for (size_t j = 0; j < a[i].n; j++) a[i].p[j] = (float)st[i][j];
+  return FIMHIP_OK;
+}
+struct Sz {
+  size_t c, l, e, t, te, nip, nvl;
+  explicit Sz(const fimhip_ctx *x)
+      : c((size_t)x->nip * x->nvl), l((size_t)x->nip * (x->nvl + 1)), e(c * NPP), t(c * x->ntra), te(e * x->ntra),
+        nip((size_t)x->nip), nvl((size_t)x->nvl) {}
+};
+}  // namespace
+
+extern "C" int fimhip_abstart_r4(fimhip_ctx *c, int its, int itsDFI, int *nf, int *of, int *vof, float *a1,
+                                 float *a2, float *a3) {
+  if (!a1 || !a2 || !a3) return FIMHIP_EINVAL;
+  double d1 = *a1, d2 = *a2, d3 = *a3;
+  const int r = fimhip_abstart(c, its, itsDFI, nf, of, vof, &d1, &d2, &d3);
+  if (r) return r;
+  *a1 = (float)d1; *a2 = (float)d2; *a3 = (float)d3;
+  return FIMHIP_OK;
+}
+
+#define FIMHIP_EDGVAR_R4(NAME, R8)                                                                         \
+  extern "C" int NAME(fimhip_ctx *c, int its, float *u_vel, float *v_vel, float *delp, float *pres,        \
+                      float *exner, float *geop, float *montg, float *tracr, float *u_edg, float *v_edg,  \
+                      float *dp_edg, float *lp_edg, float *geop_edg, float *mont_edg, float *uvsq_edg,    \
+                      float *trc_edg) {                                                                    \
+    if (!c) return FIMHIP_EINVAL;                                                                          \
+    const Sz z(c);                                                                                         \
+    const R4 a[] = {{u_vel, z.c, false},   {v_vel, z.c, false},    {delp, z.c, false},                     \
+                    {pres, z.l, false},    {exner, z.l, false},    {geop, z.l, false},                     \
+                    {montg, z.c, false},   {tracr, z.t, false},    {u_edg, z.e, true},                     \
+                    {v_edg, z.e, true},    {dp_edg, z.e, true},    {lp_edg, z.e, true},                    \
+                    {geop_edg, z.e, true}, {mont_edg, z.e, true},  {uvsq_edg, z.e, true},                  \
+                    {trc_edg, z.te, true}};                                                                \
+    return r4_call(a, [&](double **d) {                                                                    \
+      return R8(c, its, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11], d[12],  \
+                d[13], d[14], d[15]);                                                                      \
+    });                                                                                                    \
+  }
+FIMHIP_EDGVAR_R4(fimhip_edgvar1_r4, fimhip_edgvar1_r8)
+FIMHIP_EDGVAR_R4(fimhip_edgvar3_r4, fimhip_edgvar3_r8)
+#undef FIMHIP_EDGVAR_R4
+
+extern "C" int fimhip_cnuity_r4(fimhip_ctx *c, int its, float *u_velo, float *v_velo, float *u_edg, float *v_edg,
+                                float *dp_edg, float *lp_edg, float *delp, float *pres, float *exner,
+                                float *dp_tndcy, float *dplo_tndcy, float *massfx, float *omega, int timing) {
+  if (!c) return FIMHIP_EINVAL;
+  const Sz z(c);
+  const R4 a[] = {{u_velo, z.c, false}, {v_velo, z.c, false},  {u_edg, z.e, false},          {v_edg, z.e, false},
+                  {dp_edg, z.e, false}, {lp_edg, z.e, false},  {delp, z.c, true},            {pres, z.l, true},
+                  {exner, z.l, true},   {dp_tndcy, 3 * z.c, true}, {dplo_tndcy, 3 * z.c, true},
+                  {massfx, 3 * z.e, true}, {omega, z.c, true}};
+  return r4_call(a, [&](double **d) {
+    return fimhip_cnuity_r8(c, its, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11],
+                            d[12], timing);
+  });
+}
+
+extern "C" int fimhip_mass_adv_r4(fimhip_ctx *c, int its, float *u_velo, float *v_velo, float *u_edg,
+                                  float *v_edg, float *dp_edg, float *lp_edg, float *delp, float *pres,
+                                  float *exner, float *dp_tndcy, float *massfx, float *omega, int timing) {
+  if (!c) return FIMHIP_EINVAL;
+  const Sz z(c);
+  const R4 a[] = {{u_velo, z.c, false}, {v_velo, z.c, false}, {u_edg, z.e, false}, {v_edg, z.e, false},
+                  {dp_edg, z.e, true},  {lp_edg, z.e, false}, {delp, z.c, true},   {pres, z.l, true},
+                  {exner, z.l, true},   {dp_tndcy, 3 * z.c, true}, {massfx, 3 * z.e, true}, {omega, z.c, true}};
+  return r4_call(a, [&](double **d) {
+    return fimhip_mass_adv_r8(c, its, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11],
+                              timing);
+  });
+}
+
+extern "C" int fimhip_trcadv_r4(fimhip_ctx *c, int its, float *trcr_edg, float *tracr, float *trcdp,
+                                float *trc_tdcy, float *trclo_tdcy, float *massfx, float *delp, int timing) {
+  if (!c) return FIMHIP_EINVAL;
+  const Sz z(c);
+  const R4 a[] = {{trcr_edg, z.te, false}, {tracr, z.t, true}, {trcdp, z.t, true}, {trc_tdcy, 3 * z.t, true},
+                  {trclo_tdcy, 3 * z.t, true}, {massfx, 3 * z.e, false}, {delp, z.c, false}};
+  return r4_call(a, [&](double **d) {
+    return fimhip_trcadv_r8(c, its, d[0], d[1], d[2], d[3], d[4], d[5], d[6], timing);
+  });
+}
+
+extern "C" int fimhip_momtum_r4(fimhip_ctx *c, int its, float *u_velo, float *v_velo, float *exner, float *relvor,
+                                float *u_edg, float *v_edg, float *trcr_edg, float *geop_edg, float *mont_edg,
+                                float *uvsq_edg, float *u_tndcy, float *v_tndcy, float *dp3d) {
+  if (!c) return FIMHIP_EINVAL;
+  const Sz z(c);
+  const R4 a[] = {{u_velo, z.c, true},    {v_velo, z.c, true},   {exner, z.l, false},    {relvor, z.c, true},
+                  {u_edg, z.e, false},    {v_edg, z.e, false},   {trcr_edg, z.te, false}, {geop_edg, z.e, false},
+                  {mont_edg, z.e, false}, {uvsq_edg, z.e, false}, {u_tndcy, 3 * z.c, true},
+                  {v_tndcy, 3 * z.c, true}, {dp3d, z.c, false}};
+  return r4_call(a, [&](double **d) {
+    return fimhip_momtum_r8(c, its, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10], d[11],
+                            d[12]);
+  });
+}
+
+extern "C" int fimhip_hybgen_r4(fimhip_ctx *c, int its, float *targt, float *us3d, float *vs3d, float *tr3d,
+                                float *sdot, float *ex3d, float *dp3d, float *pr3d, int timing) {
+  if (!c) return FIMHIP_EINVAL;
+  const Sz z(c);
+  const R4 a[] = {{targt, z.nvl, false}, {us3d, z.c, true}, {vs3d, z.c, true}, {tr3d, z.t, true},
+                  {sdot, z.l, true},     {ex3d, z.l, true}, {dp3d, z.c, true}, {pr3d, z.l, true}};
+  return r4_call(a, [&](double **d) {
+    return fimhip_hybgen_r8(c, its, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], timing);
+  });
+}
+
+extern "C" int fimhip_hystat_r4(fimhip_ctx *c, int its, float *ph3d, float *ex3d, float *mp3d, float *dp3d,
+                                float *tr3d, float *trdp, float *psrf, float *ptdcy) {
+  if (!c) return FIMHIP_EINVAL;
+  const Sz z(c);
+  const R4 a[] = {{ph3d, z.l, true}, {ex3d, z.l, false}, {mp3d, z.c, true}, {dp3d, z.c, false},
+                  {tr3d, z.t, true}, {trdp, z.t, true},  {psrf, z.nip, true}, {ptdcy, 2 * z.nip, true}};
+  return r4_call(a, [&](double **d) {
+    return fimhip_hystat_r8(c, its, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7]);
+  });
+}
+
+extern "C" int fimhip_transp0_r4(fimhip_ctx *c, int its, float *cumufx, float *dp3d, float *dpinit) {
+  if (!c) return FIMHIP_EINVAL;
+  const Sz z(c);
+  const R4 a[] = {{cumufx, z.e, true}, {dp3d, z.c, false}, {dpinit, z.c, true}};
+  return r4_call(a, [&](double **d) { return fimhip_transp0_r8(c, its, d[0], d[1], d[2]); });
+}
+
+extern "C" int fimhip_transp1_r4(fimhip_ctx *c, int its, int nf, int of, int vof, float adbash1, float adbash2,
+                                 float adbash3, float *cumufx, float *massfx) {
+  if (!c) return FIMHIP_EINVAL;
+  const Sz z(c);
+  const R4 a[] = {{cumufx, z.e, true}, {massfx, 3 * z.e, false}};
+  return r4_call(a, [&](double **d) {
+    return fimhip_transp1_r8(c, its, nf, of, vof, adbash1, adbash2, adbash3, d[0], d[1]);
+  });
+}
+
+extern "C" int fimhip_transp2_r4(fimhip_ctx *c, int its, float *tracr, float *cumufx, float *dpinit, float *dpfinl,
+                                 int timing) {
+  if (!c) return FIMHIP_EINVAL;
+  const Sz z(c);
+  const R4 a[] = {{tracr, z.c * (size_t)(c->ntra + c->cb.ntrb), true}, {cumufx, z.e, false}, {dpinit, z.c, false},
+                  {dpfinl, z.c, false}};
+  return r4_call(a, [&](double **d) { return fimhip_transp2_r8(c, its, d[0], d[1], d[2], d[3], timing); });
+}
+
+// ---------------------------------------------------------------------------
 // device-resident fast path
 extern "C" int fimhip_init_state(fimhip_ctx *c, int init_hybgen) {
   if (!c) return FIMHIP_EINVAL;

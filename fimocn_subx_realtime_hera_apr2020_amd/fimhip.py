@@ -46,6 +46,12 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ens_kernel_count", "fimhip_ens_kernel_name", "fimhip_ens_set_tile", "fimhip_ens_get_tile",
            "fimhip_ens_autotune", "fimhip_ens_profile", "fimhip_ens_profile_report"]
 
+# real*4 drop-ins: name -> (number of float arrays, trailing int argument)
+R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, True), "mass_adv": (12, True),
+              "trcadv": (7, True), "momtum": (13, False), "hybgen": (8, True), "hystat": (8, False),
+              "transp0": (3, False), "transp2": (4, True)}
+EXPORTS += ["fimhip_%s_r4" % n for n in R4_ENTRIES] + ["fimhip_abstart_r4", "fimhip_transp1_r4"]
+
 SYNTH_WIND_NOISE, SYNTH_PERTURB, SYNTH_LOGNORMAL = 1, 2, 4
 
 
@@ -87,6 +93,12 @@ def load_library() -> ct.CDLL:
     lib.fimhip_mass_adv_r8.argtypes = [vp, ct.c_int] + [_PD] * 12 + [ct.c_int]
     lib.fimhip_set_flux_scheme.argtypes = [vp, ct.c_int]
     lib.fimhip_set_class_b.argtypes = [vp, ct.c_int, ct.c_int]
+    _PF4 = ct.POINTER(ct.c_float)
+    for n, (na, tail) in R4_ENTRIES.items():
+        getattr(lib, "fimhip_%s_r4" % n).argtypes = [vp, ct.c_int] + [_PF4] * na + ([ct.c_int] if tail else [])
+    lib.fimhip_abstart_r4.argtypes = [vp, ct.c_int, ct.c_int, _PI, _PI, _PI, _PF4, _PF4, _PF4]
+    lib.fimhip_transp1_r4.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_float, ct.c_float,
+                                      ct.c_float, _PF4, _PF4]
     lib.fimhip_transp0_r8.argtypes = [vp, ct.c_int, _PD, _PD, _PD]
     lib.fimhip_transp1_r8.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_double, ct.c_double,
                                       ct.c_double, _PD, _PD]
@@ -315,6 +327,20 @@ class Instance:
         """tracr: (ntra + ntrb, nip, nvl); the class-B fields are transported in place."""
         _chk(self.lib.fimhip_transp2_r8(self.h, its, _p(tracr), _p(cumufx), _p(dpinit), _p(dpfinl),
                                         int(TimingBarriers)), "transp2")
+
+    def call_r4(self, name: str, its: int, *arrays, tail=None):
+        """The real*4 drop-in fimhip_<name>_r4 on float32 arrays (updated in
+        place), with the trailing integer argument where the routine has one."""
+        na, has_tail = R4_ENTRIES[name]
+        if len(arrays) != na:
+            raise ValueError("%s_r4 takes %d arrays" % (name, na))
+        ptrs = []
+        for a in arrays:
+            if a.dtype != np.float32 or not a.flags.c_contiguous:
+                raise ValueError("%s_r4: arrays must be contiguous float32" % name)
+            ptrs.append(a.ctypes.data_as(ct.POINTER(ct.c_float)))
+        args = [self.h, its] + ptrs + ([int(tail or 0)] if has_tail else [])
+        _chk(getattr(self.lib, "fimhip_%s_r4" % name)(*args), name + "_r4")
 
     def trcadv(self, its, trcr_edg, tracr, trcdp, trc_tdcy, trclo_tdcy, massfx, delp, TimingBarriers=False):
         _chk(self.lib.fimhip_trcadv_r8(self.h, its, *map(_p, (trcr_edg, tracr, trcdp, trc_tdcy, trclo_tdcy,

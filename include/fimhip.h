@@ -123,6 +123,42 @@ int fimhip_hybgen_r8(fimhip_ctx *ctx, int its, const double *targt, double *us3d
 int fimhip_hystat_r8(fimhip_ctx *ctx, int its, double *ph3d, const double *ex3d, double *mp3d,
                      const double *dp3d, double *tr3d, double *trdp, double *psrf, double *ptdcy);
 
+/* ---- real*4 drop-ins ------------------------------------------------------
+ * The same routines for the production build (default real 4): the argument
+ * lists of the _r8 entries above with float arrays (and float AB weights).
+ * The arrays are widened to FP64, the _r8 entry runs, and the arrays the
+ * routine writes are rounded back to float; input arrays are not written. */
+int fimhip_abstart_r4(fimhip_ctx *ctx, int its, int itsDFI, int *nf, int *of, int *vof,
+                      float *adbash1, float *adbash2, float *adbash3);
+int fimhip_edgvar1_r4(fimhip_ctx *ctx, int its, float *u_vel, float *v_vel, float *delp, float *pres,
+                      float *exner, float *geop, float *montg, float *tracr, float *u_edg, float *v_edg,
+                      float *dp_edg, float *lp_edg, float *geop_edg, float *mont_edg, float *uvsq_edg,
+                      float *trc_edg);
+int fimhip_edgvar3_r4(fimhip_ctx *ctx, int its, float *u_vel, float *v_vel, float *delp, float *pres,
+                      float *exner, float *geop, float *montg, float *tracr, float *u_edg, float *v_edg,
+                      float *dp_edg, float *lp_edg, float *geop_edg, float *mont_edg, float *uvsq_edg,
+                      float *trc_edg);
+int fimhip_cnuity_r4(fimhip_ctx *ctx, int its, float *u_velo, float *v_velo, float *u_edg, float *v_edg,
+                     float *dp_edg, float *lp_edg, float *delp, float *pres, float *exner, float *dp_tndcy,
+                     float *dplo_tndcy, float *massfx, float *omega, int TimingBarriers);
+int fimhip_mass_adv_r4(fimhip_ctx *ctx, int its, float *u_velo, float *v_velo, float *u_edg, float *v_edg,
+                       float *dp_edg, float *lp_edg, float *delp, float *pres, float *exner, float *dp_tndcy,
+                       float *massfx, float *omega, int TimingBarriers);
+int fimhip_trcadv_r4(fimhip_ctx *ctx, int its, float *trcr_edg, float *tracr, float *trcdp, float *trc_tdcy,
+                     float *trclo_tdcy, float *massfx, float *delp, int TimingBarriers);
+int fimhip_momtum_r4(fimhip_ctx *ctx, int its, float *u_velo, float *v_velo, float *exner, float *relvor,
+                     float *u_edg, float *v_edg, float *trcr_edg, float *geop_edg, float *mont_edg,
+                     float *uvsq_edg, float *u_tndcy, float *v_tndcy, float *dp3d);
+int fimhip_hybgen_r4(fimhip_ctx *ctx, int its, float *targt, float *us3d, float *vs3d, float *tr3d, float *sdot,
+                     float *ex3d, float *dp3d, float *pr3d, int TimingBarriers);
+int fimhip_hystat_r4(fimhip_ctx *ctx, int its, float *ph3d, float *ex3d, float *mp3d, float *dp3d, float *tr3d,
+                     float *trdp, float *psrf, float *ptdcy);
+int fimhip_transp0_r4(fimhip_ctx *ctx, int its, float *cumufx, float *dp3d, float *dpinit);
+int fimhip_transp1_r4(fimhip_ctx *ctx, int its, int nf, int of, int vof, float adbash1, float adbash2,
+                      float adbash3, float *cumufx, float *massfx);
+int fimhip_transp2_r4(fimhip_ctx *ctx, int its, float *tracr, float *cumufx, float *dpinit, float *dpfinl,
+                      int TimingBarriers);
+
 /* ---- device-resident fast path ------------------------------------------ */
 enum {
   FIMHIP_US = 0, FIMHIP_VS, FIMHIP_DP, FIMHIP_PR, FIMHIP_EX, FIMHIP_PH, FIMHIP_MP, FIMHIP_TR,

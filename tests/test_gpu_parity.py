@@ -528,3 +528,67 @@ def test_forty_layers_steps_match_checker_bitwise(oracle_lib, flux_scheme, ntrb)
             assert _same(got, o.a[f]), (f, rel_l2(got, o.a[f]))
     finally:
         m.close()
+
+
+def test_r4_dropins_are_the_r8_path_rounded(case):
+    """The real*4 drop-ins (the production build's float arrays): every
+    routine's outputs are the _r8 entry's outputs on the widened inputs,
+    rounded to float, exactly, and the arrays it does not write come back
+    unchanged; two steps of the whole sequence, class-B transport included."""
+    import ctypes as ct
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    O, g, st0, th, ds, ga = case
+    print("case: G3 nvl=64 ntra=4 ntrb=2, real*4 drop-ins against the real*8 ones", flush=True)
+    st = _class_b_case(g)
+    o = O.OracleRun(ga, NVL, NTRA, DT, pow_kind=1, ntrb=2, dtratio=2)
+    o.load(st)
+    o.init(True)
+    f = {k: v.astype(np.float32) for k, v in o.a.items()}
+    f["geop_edg"] = np.zeros_like(f["u_edg"])
+    f["tr_all"] = np.ascontiguousarray(np.concatenate([f["tr"], f["trb"]]))
+    th32 = np.ascontiguousarray(th, np.float32)
+    inst = fimhip.Instance(g, NVL, NTRA, DT, th, ds)
+    try:
+        inst.set_class_b(2, 2)
+
+        def both(name, keys, tail=None, r8=None):
+            """r8 on widened copies, then r4 in place; compare every array"""
+            wide = [f[k].astype(np.float64) if isinstance(k, str) else k.astype(np.float64) for k in keys]
+            r8(*wide)
+            arrs = [f[k] if isinstance(k, str) else k for k in keys]
+            inst.call_r4(name, inst._its, *arrs, tail=tail)
+            for k, a, w in zip(keys, arrs, wide):
+                assert np.array_equal(a, w.astype(np.float32)), (name, k if isinstance(k, str) else "arg")
+
+        for its in (1, 2):
+            inst.abstart(its)
+            inst._its = its
+            ek = ["us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "u_edg", "v_edg", "dp_edg", "lp_edg", "geop_edg",
+                  "mont_edg", "uvsq_edg", "trc_edg"]
+            both("edgvar1", ek, r8=lambda *a: inst.edgvar1(its, *a))
+            both("edgvar3", ek, r8=lambda *a: inst.edgvar3(its, *a))
+            both("cnuity", ["us", "vs", "u_edg", "v_edg", "dp_edg", "lp_edg", "dp", "pr", "ex", "dp_tdcy", "dpl_tdcy",
+                            "massfx", "ws"], tail=0, r8=lambda *a: inst.cnuity(its, *a))
+            both("transp0", ["cumufx", "dp", "dpinit"], r8=lambda *a: inst.transp0(its, *a))
+            if its == 2:
+                f["tr_all"][NTRA:] = f["trb"]
+                both("transp2", ["tr_all", "cumufx", "dpinit", "dp"], tail=0, r8=lambda *a: inst.transp2(its, *a))
+            both("trcadv", ["trc_edg", "tr", "trdp", "trc_tdcy", "trl_tdcy", "massfx", "dp"], tail=0,
+                 r8=lambda *a: inst.trcadv(its, *a))
+            both("momtum", ["us", "vs", "ex", "relvor", "u_edg", "v_edg", "trc_edg", "geop_edg", "mont_edg",
+                            "uvsq_edg", "u_tdcy", "v_tdcy", "dp"], r8=lambda *a: inst.momtum(its, *a))
+            both("hybgen", [th32, "us", "vs", "tr", "sdot", "ex", "dp", "pr"], tail=0,
+                 r8=lambda *a: inst.hybgen(its, *a))
+            both("hystat", ["ph", "ex", "mp", "dp", "tr", "trdp", "psrf", "ptdcy"], r8=lambda *a: inst.hystat(its, *a))
+            both("mass_adv", ["us", "vs", "u_edg", "v_edg", "dp_edg", "lp_edg", "dp", "pr", "ex", "dp_tdcy",
+                              "massfx", "ws"], tail=0, r8=lambda *a: inst.mass_adv(its, *a))
+        assert np.isfinite(f["us"]).all() and f["dp"].min() > 0.
+        # abstart with float AB weights
+        nf, of, vof = ct.c_int(inst.nf), ct.c_int(inst.of), ct.c_int(inst.vof)
+        a = [ct.c_float(x) for x in inst.adbash]
+        assert inst.lib.fimhip_abstart_r4(inst.h, 3, 1, ct.byref(nf), ct.byref(of), ct.byref(vof),
+                                          *[ct.byref(x) for x in a]) == 0
+        assert (nf.value, of.value, vof.value) == ((inst.nf % 3) + 1, (inst.of % 3) + 1, (inst.vof % 3) + 1)
+        assert a[0].value == np.float32(DT * 23. / 12.)
+    finally:
+        inst.close()
