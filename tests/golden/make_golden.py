@@ -10,6 +10,11 @@ Fixtures (data only):
   icos_grid_info_G2.dat the reference ginfo output for G2 (icos_grid_info_level.dat,
                         GridGen.F90:437-497), the file getlvl reads
   dyn_G2_input.npz      seeded FP64 input state on the G2 grid (nvl=64, ntra=4)
+  dyn_G2_schemes_ref.json  SHA-256 of the final records of the reference FP64
+                        build for the alternative schemes on the same input:
+                        mass_adv (flux_conserv_schm=2), and class-B tracers
+                        (ntrb=2, dtratio=2, fields class_b_fields() of the
+                        input) with cnuity and with mass_adv
   dyn_G2_ref.json       SHA-256 of every output record of the reference FP64
                         build (dynref_r8) for 3 full steps (mode 0) and 3
                         transport steps (mode 1), stage dumps of step 1, plus
@@ -63,5 +68,35 @@ def main():
     print("golden fixtures written to", HERE)
 
 
+def class_b_fields(st):
+    """Two class-B tracer fields derived from the input state's tracers."""
+    return np.ascontiguousarray(np.stack([st["tr"][1], 2.5 * st["tr"][2] + 1.e-3]))
+
+
+SCHEME_RUNS = {   # name: (mode, flux_scheme, class-B tracers, steps)
+    "massadv_mode0": (0, 2, False, 3),
+    "classb_mode1": (1, 1, True, 5),
+    "classb_massadv_mode0": (0, 2, True, 5),
+}
+
+
+def schemes():
+    z = np.load(os.path.join(HERE, "dyn_G2_input.npz"))
+    st = {k: z[k] for k in ("us", "vs", "dp", "pr", "ex", "ph", "tr")}
+    th, ds, dt = z["thetac"], z["dpsig"], float(z["dt"])
+    out = {"dt": dt, "dtratio": 2, "runs": {}}
+    for name, (mode, scheme, cb, nsteps) in SCHEME_RUNS.items():
+        s2 = dict(st)
+        if cb:
+            s2["trb"] = class_b_fields(st)
+        r = refrun.run_dynref(refrun.ref_glvl(2), s2, th, ds, nsteps=nsteps, dt=dt, mode=mode,
+                              init_hybgen=(mode == 0), flux_scheme=scheme, dtratio=2)
+        out["runs"][name] = {k: [sha(x) for x in (v if isinstance(v, list) else [v])]
+                             for k, v in r.items() if k.startswith("final.")}
+    with open(os.path.join(HERE, "dyn_G2_schemes_ref.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     main()
+    schemes()

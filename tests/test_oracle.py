@@ -193,6 +193,45 @@ def test_class_b_transport_vs_live_reference_bitwise(oracle_lib, glvl_, mode, ns
     assert not np.array_equal(o.a["trb"], st["trb"])
 
 
+@pytest.mark.parametrize("run", ["massadv_mode0", "classb_mode1", "classb_massadv_mode0"])
+def test_schemes_bitexact_vs_reference_golden(oracle_lib, run):
+    """mass_adv and class-B transport: the final state has the exact bytes of
+    the reference FP64 build on the committed G2 input
+    (dyn_G2_schemes_ref.json, made by tests/golden/make_golden.py)."""
+    import json
+    import os
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    with open(os.path.join(gdir, "dyn_G2_schemes_ref.json")) as f:
+        gold = json.load(f)
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(gdir, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    mode, scheme, cb, nsteps = mg.SCHEME_RUNS[run]
+    ref = gold["runs"][run]
+    O = oracle_lib
+    st, th, ds, dt = golden_input()
+    if cb:
+        st["trb"] = mg.class_b_fields(st)
+    o = _oracle_run(O, golden_grid(2), st, th, ds, dt, flux_scheme=scheme, dtratio=gold["dtratio"])
+    o.init(mode == 0)
+    for its in range(1, nsteps + 1):
+        o.step(its, mode)
+    nf = o.slots[0] - 1
+    got = {"final.us": [sha(o.a["us"])], "final.vs": [sha(o.a["vs"])], "final.dp": [sha(o.a["dp"])],
+           "final.pr": [sha(o.a["pr"])], "final.ex": [sha(o.a["ex"])], "final.ph": [sha(o.a["ph"])],
+           "final.mp": [sha(o.a["mp"])], "final.psrf": [sha(o.a["psrf"])],
+           "final.tr": [sha(o.a["tr"][t]) for t in range(4)], "final.trdp": [sha(o.a["trdp"][t]) for t in range(4)],
+           "final.dptdcy": [sha(o.a["dp_tdcy"][nf])], "final.utdcy": [sha(o.a["u_tdcy"][nf])]}
+    if cb:
+        got["final.trb"] = [sha(o.a["trb"][t]) for t in range(2)]
+        got["final.cumufx"] = [sha(o.a["cumufx"])]
+        got["final.dpinit"] = [sha(o.a["dpinit"])]
+    assert sorted(got) == sorted(ref)
+    bad = [k for k in ref if got[k] != ref[k]]
+    assert bad == [], bad
+
+
 def test_project_pow_drift_within_bound(oracle_lib):
     """fim_pow (the device pow) instead of the C library pow moves the 3-step
     G2 state by far less than 1e-12 relative L2."""
