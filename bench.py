@@ -267,8 +267,11 @@ def run_plain_class_b(a, m, grid, cells, fimhip, ntrb=8, dtratio=6):
     m.profile(False)
     m.set_class_b(0, dtratio)
     kt = kernel_table(prof, class_b_kernel_bytes(ntrb), cells)
+    kt = {k: v for k, v in kt.items() if k.startswith(("k_transp", "k_fct3d"))}
+    per_call = sum(v["mean_ms"] for k, v in kt.items() if k in ("k_transp2_w", "k_fct3d_fill", "k_fct3d_vert",
+                                                                  "k_fct3d_lo", "k_fct3d_clip"))
     return {"ntrb": ntrb, "dtratio": dtratio, "ms_per_step": wall / n_timed * 1e3, "timed_steps": n_timed,
-            "kernels": {k: v for k, v in kt.items() if k.startswith(("k_transp", "k_fct3d"))}}
+            "transp2_ms_per_call": per_call, "kernels": kt}
 
 
 def run_plain_mass_adv(a, m, cells, kb, fimhip):
@@ -431,10 +434,50 @@ def cpu_baseline(m, grid, a, th, ds, fimhip):
         o.step(its, a.mode)
     wall = time.perf_counter() - t0
     cells = grid.nip * a.nvl
-    return {"value": cells * a.cpu_steps / wall, "unit": "cell-level-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": "1 member of G%d x %dL, %d tracers, %d %s FP64 steps (C restatement, OpenMP %d threads)"
-                      % (a.glvl, a.nvl, a.ntra, a.cpu_steps, "full" if a.mode == 0 else "transport", threads)}
+    out = {"value": cells * a.cpu_steps / wall, "unit": "cell-level-steps/s", "cores": threads,
+           "kind": "port",
+           "sample": "1 member of G%d x %dL, %d tracers, %d %s FP64 steps (C restatement, OpenMP %d threads)"
+                     % (a.glvl, a.nvl, a.ntra, a.cpu_steps, "full" if a.mode == 0 else "transport", threads)}
+    if not a.headline_only:
+        out["schemes"] = cpu_schemes(o, m, grid, a, th, ds, fimhip, O, threads)
+    return out
+
+
+def cpu_schemes(o, m, grid, a, th, ds, fimhip, O, threads, ntrb=8):
+    """The C restatement on the alternative schemes of the bench legs: the
+    same steps with mass_adv, and one class-B transport call (transp2 with
+    fct3d of ntrb tracers) after one step's integrated mass flux."""
+    cells = grid.nip * a.nvl
+    o.g.flux_scheme = 2
+    for n in ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp"):
+        o.a[n][...] = m.download(n)
+    for n in ("u_tdcy", "v_tdcy", "dp_tdcy", "dpl_tdcy", "trc_tdcy", "trl_tdcy", "massfx"):
+        o.a[n][...] = 0.
+    o.s.nf, o.s.of, o.s.vof = 0, 2, 1
+    t0 = time.perf_counter()
+    for its in range(1, a.cpu_steps + 1):
+        o.step(its, a.mode)
+    mwall = time.perf_counter() - t0
+    o.g.flux_scheme = 1
+    ob = O.OracleRun(o.garr, a.nvl, a.ntra, a.dt, pow_kind=0, ntrb=ntrb, dtratio=1)
+    for n in ("dp", "massfx"):
+        ob.a[n][...] = o.a[n]
+    lg = fimhip.synth_state(grid.lat, grid.lon, a.nvl, 1 + ntrb, seed=7, member=3, flags=fimhip.SYNTH_LOGNORMAL)
+    ob.a["trb"][...] = lg["tr"][1:]
+    del lg
+    ob.s.nf, ob.s.of, ob.s.vof = o.s.nf, o.s.of, o.s.vof
+    ob.s.ab1, ob.s.ab2, ob.s.ab3 = o.s.ab1, o.s.ab2, o.s.ab3
+    ob.a["dpinit"][...] = o.a["dp"] * 1.0001      # a thickness change, so the vertical fluxes are not zero
+    ob.call("transp1")
+    t0 = time.perf_counter()
+    ob.call("transp2")
+    bwall = time.perf_counter() - t0
+    return {"mass_adv": {"value": cells * a.cpu_steps / mwall, "unit": "cell-level-steps/s",
+                         "sample": "%d %s FP64 steps with flux_conserv_schm = 2" % (a.cpu_steps,
+                                                                                   "full" if a.mode == 0 else "transport")},
+            "class_b_transp2": {"ms_per_call": bwall * 1e3, "ntrb": ntrb,
+                                "sample": "one transp2 (fct3d of %d class-B tracers) on G%d x %dL" % (ntrb, a.glvl, a.nvl)},
+            "cores": threads, "kind": "port"}
 
 
 def main():
