@@ -160,7 +160,9 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
 }
 
 template <int COLS, bool SWZ>
-__global__ void __launch_bounds__(64 * COLS) k_e_edgvar1(EnsGrid g, EnsState s) {
+// waves_per_eu: the register allocation otherwise lands just above an
+// occupancy step (84 VGPRs, 5 waves per SIMD); 80 VGPRs give 6, no spills
+__global__ void __launch_bounds__(64 * COLS) __attribute__((amdgpu_waves_per_eu(6))) k_e_edgvar1(EnsGrid g, EnsState s) {
   const int nip = g.d.nip, nvl = g.d.nvl;
   const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
   if (ipn >= nip || k >= nvl) return;
@@ -397,7 +399,9 @@ __device__ __forceinline__ void e_cnuity45_col(const EnsGrid &g, const EnsState 
 }
 
 template <int COLS, bool SWZ>
-__global__ void __launch_bounds__(64 * COLS) k_e_cnuity45(EnsGrid g, EnsState s) {
+// waves_per_eu: 106 SGPRs would hold occupancy at 7 waves per SIMD; the
+// request fits 76 SGPRs and 42 VGPRs (8 waves), no spills
+__global__ void __launch_bounds__(64 * COLS) __attribute__((amdgpu_waves_per_eu(8))) k_e_cnuity45(EnsGrid g, EnsState s) {
   const int nip = g.d.nip;
   const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
   if (ipn >= nip) return;
@@ -717,7 +721,8 @@ __device__ __forceinline__ void e_trcadv3_col(const EnsGrid &g, const EnsState &
 }
 
 template <int COLS, bool SWZ>
-__global__ void __launch_bounds__(64 * COLS) k_e_trcadv3(EnsGrid g, EnsState s) {
+// waves_per_eu: 66 VGPRs -> 56 (8 waves per SIMD instead of 7), no spills
+__global__ void __launch_bounds__(64 * COLS) __attribute__((amdgpu_waves_per_eu(8))) k_e_trcadv3(EnsGrid g, EnsState s) {
   const int nip = g.d.nip, nvl = g.d.nvl;
   const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
   if (ipn >= nip || k >= nvl) return;
@@ -811,7 +816,9 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
 }
 
 template <int COLS, bool SWZ>
-__global__ void __launch_bounds__(64 * COLS) k_e_momtum(EnsGrid g, EnsState s) {
+// waves_per_eu: 101 VGPRs -> 91 (5 waves per SIMD instead of 4), scalar
+// registers spilled to vector lanes only
+__global__ void __launch_bounds__(64 * COLS) __attribute__((amdgpu_waves_per_eu(5))) k_e_momtum(EnsGrid g, EnsState s) {
   const int nip = g.d.nip, nvl = g.d.nvl, ncol = s.ncol;
   const int k = threadIdx.x, ipn = tile_cell<COLS, SWZ>();
   if (ipn >= nip) return;
