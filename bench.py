@@ -27,6 +27,8 @@ Prints ONE JSON line.  Besides the contract fields it carries
                 mass_adv the same step with the flux-limited mass transport
                 (flux_conserv_schm = 2) and its two kernels; under class_b the
                 step with 8 class-B tracers moved every 6 steps by fct3d
+  uneven_batch: G6 and G7 concatenated into one instance (two resolutions in
+                one launch), mixed form and FP64
   accuracy_96:  relative L2 of member 0 against FP64 after 96 steps, mass drift
   cpu_baseline: the C restatement (oracle/fim_oracle.c, FP64, OpenMP) on a
                 bounded sample of the same grid
@@ -383,6 +385,50 @@ def run_ens_mass_adv(a, e, cells, kb, fimhip):
             "kernels": {k: kt[k] for k in ("e_massadv1", "e_massadv2") if k in kt}}
 
 
+def run_uneven(a, th, ds, fimhip):
+    """Two resolutions in one launch: the G6 and G7 grids concatenated into one
+    instance (204804 cells x nvl layers, the atmosphere part of the uneven
+    batch), at the finer grid's time step, full steps, in the mixed form and in
+    FP64."""
+    import torch
+    from fimocn_subx_realtime_hera_apr2020_amd import glvl
+    gu = glvl.concat([fimhip.icos_grid(6), fimhip.icos_grid(7)])
+    dt = 1.2 * 4800.0 / float(2 ** 7 // 2)
+    cells = gu.nip * a.nvl
+    out = {"grids": "G6 + G7 in one instance", "nip": gu.nip, "dt": dt}
+    for form in ("mixed", "fp64"):
+        inst = (fimhip.Ensemble(gu, 1, a.nvl, a.ntra, dt, th, ds, device=0) if form == "mixed"
+                else fimhip.Model(gu, a.nvl, a.ntra, dt, th, ds, device=0))
+        try:
+            if form == "mixed":
+                inst.synth(0, seed=SEED, synth_member=0, flags=fimhip.SYNTH_WIND_NOISE)
+                inst.init_state(True)
+                inst.step(a.warmup, a.mode)
+                inst.sync()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                inst.step(a.steps, a.mode)
+                inst.sync()
+            else:
+                inst.synth(seed=SEED, member=0, flags=fimhip.SYNTH_WIND_NOISE)
+                inst.init_state(True)
+                for its in range(1, a.warmup + 1):
+                    inst.step(its, a.mode)
+                inst.sync()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for its in range(a.warmup + 1, a.warmup + a.steps + 1):
+                    inst.step(its, a.mode)
+                inst.sync()
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+        finally:
+            inst.close()
+        out[form] = {"value": cells * a.steps / wall, "ms_per_step": wall / a.steps * 1e3}
+    out["ratio"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def accuracy(a, e, m, grid, fimhip):
     """Member 0 of the ensemble against the FP64 path after n steps, and the
     global mass drift of every member."""
@@ -511,6 +557,10 @@ def main():
         e, ens = run_ens(a, grid, th, ds, fimhip)
         print("ensemble: %.3f ms/step for %d members" % (ens["ms_per_step"], a.members), file=sys.stderr,
               flush=True)
+    uneven = None
+    if a.form == "both" and not a.headline_only:
+        uneven = run_uneven(a, th, ds, fimhip)
+        print("uneven batch done", file=sys.stderr, flush=True)
     acc = None
     if e is not None and m is not None and a.accuracy_steps > 0:
         acc = accuracy(a, e, m, grid, fimhip)
@@ -547,7 +597,7 @@ def main():
                               "a 240 s step overflows the FP64 state by step 6, so it is not timed"},
         "roofline": head["roofline"], "step_hbm": head["step_hbm"],
         "speedup_vs_plain_fp64": (ens["value"] / plain["value"]) if (ens and plain) else None,
-        "plain_fp64": plain, "ensemble": ens, "accuracy_96": acc, "cpu_baseline": cpu,
+        "plain_fp64": plain, "ensemble": ens, "uneven_batch": uneven, "accuracy_96": acc, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
