@@ -16,7 +16,8 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libfimhip.so")
 
-SOURCES = ["fimhip_api.hip", "kernels_plain.hip", "transp3d.hip", "hybgen.hip", "ens_api.hip", "kernels_ens.hip"]
+SOURCES = ["fimhip_api.hip", "kernels_plain.hip", "transp3d.hip", "hybgen.hip", "ens_api.hip", "kernels_ens.hip",
+           "transp3d_ens.hip"]
 HOST_SOURCES = ["grid.cpp"]      # host-only C++ (g++)
 HEADERS = ["fimhip_internal.h", "fim_pow.h", "fim_pow_tables.h", "synth.h", "host_common.h", "ens_internal.h"]
 

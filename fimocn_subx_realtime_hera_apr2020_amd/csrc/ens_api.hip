@@ -27,6 +27,7 @@ struct fimhip_ens {
   std::vector<double> lat, lon;
   int tile[EK_NKERNELS];
   int flux_conserv_schm = 1;         // 1 = cnuity (FCT), 2 = mass_adv
+  EnsClassB cb{};                    // class-B transport (cb.ntrb = 0: off)
   // globsum partials
   int gs_blocks = 0;
   double *gs64 = nullptr;
@@ -226,6 +227,11 @@ extern "C" int fimhip_ens_init_state(fimhip_ens *e, int init_hybgen) {
   HIPCHK(hipMemsetAsync(s.massfx, 0, cz * NPP * 4, e->stream));
   StepCtl c{0, 0, 2, 1, 0., 0., 0.};
   HIPCHK(hipMemcpyAsync(s.ctl, &c, sizeof c, hipMemcpyHostToDevice, e->stream));
+  if (e->cb.ntrb > 0) {            // dyn_init.F90:1096-1098
+    HIPCHK(hipMemsetAsync(e->cb.ring, 0, 3 * cz * NPP * sizeof(float), e->stream));
+    ens_launch_class_b_init(e->g, s, e->cb, e->stream);
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipStreamSynchronize(e->stream));
   return FIMHIP_OK;
 }
@@ -257,6 +263,7 @@ static void enqueue_step(fimhip_ens *e, int mode, bool side) {
   } else {
     for (int k = EK_CNUITY12; k <= EK_CNUITY45; k++) ens_launch(k, e->tile[k], g, s, st);
   }
+  if (e->cb.ntrb > 0) ens_launch_class_b_step(g, s, e->cb, st);   // dyn_run.F90:388-405
   for (int k = EK_TRCADV1; k <= EK_TRCADV3; k++) ens_launch(k, e->tile[k], g, s, st);
   // momtum needs nothing of the tracer transport; it runs beside hybgen's
   // column kernels up to the remap (latency-bound, so the two share the
@@ -299,6 +306,55 @@ extern "C" int fimhip_ens_step(fimhip_ens *e, int nsteps, int mode) {
   return FIMHIP_OK;
 }
 
+extern "C" int fimhip_ens_set_class_b(fimhip_ens *e, int ntrb, int dtratio) {
+  if (!e || ntrb < 0 || ntrb > NTRA_MAX || dtratio < 1) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  drop_graph(e);
+  if (ntrb == e->cb.ntrb) {
+    e->cb.dtratio = dtratio;
+    return FIMHIP_OK;
+  }
+  HIPCHK(hipStreamSynchronize(e->stream));
+  EnsClassB b{};
+  b.ntrb = ntrb;
+  b.dtratio = dtratio;
+  if (ntrb > 0) {
+    // arrays of an earlier setting stay in the arena until the instance is destroyed
+    const size_t cz = (size_t)e->s.ncol * e->nvl, ez = cz * NPP, tz = cz * ntrb;
+    struct { float **p; size_t n; } f32[] = {{&b.trb, tz}, {&b.ring, 3 * ez}, {&b.cumufx, ez}, {&b.fld, tz},
+                                             {&b.vfx, tz}, {&b.fmx, tz}, {&b.fmn, tz}, {&b.fld1, tz},
+                                             {&b.flp, tz}, {&b.fln, tz}, {&b.uan, ez * ntrb}};
+    struct { double **p; size_t n; } f64[] = {{&b.dpinit, cz}, {&b.w, cz}};
+    int r;
+    for (auto &x : f32)
+      if ((r = e->arena.alloc(x.p, x.n))) return r;
+    for (auto &x : f64)
+      if ((r = e->arena.alloc(x.p, x.n))) return r;
+    HIPCHK(hipMemsetAsync(b.trb, 0, tz * sizeof(float), e->stream));
+    HIPCHK(hipMemsetAsync(b.ring, 0, 3 * ez * sizeof(float), e->stream));
+    HIPCHK(hipMemsetAsync(b.cumufx, 0, ez * sizeof(float), e->stream));
+    HIPCHK(hipMemcpyAsync(b.dpinit, e->s.dp, cz * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+  }
+  e->cb = b;
+  return FIMHIP_OK;
+}
+
+// class-B tracers of one member from host FP64 arrays (nvl, nip, ntrb)
+extern "C" int fimhip_ens_load_class_b(fimhip_ens *e, int member, const double *trb) {
+  if (check_member(e, member) || !trb || e->cb.ntrb < 1) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  const size_t cz = (size_t)e->nip * e->nvl, ecz = (size_t)e->s.ncol * e->nvl;
+  std::vector<float> h(cz);
+  for (int t = 0; t < e->cb.ntrb; t++) {
+    for (size_t i = 0; i < cz; i++) h[i] = (float)trb[(size_t)t * cz + i];
+    HIPCHK(hipMemcpyAsync(e->cb.trb + t * ecz + (size_t)member * cz, h.data(), cz * sizeof(float),
+                          hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));   // h is reused for the next tracer
+  }
+  return FIMHIP_OK;
+}
+
 extern "C" int fimhip_ens_set_flux_scheme(fimhip_ens *e, int flux_conserv_schm) {
   if (!e || (flux_conserv_schm != 1 && flux_conserv_schm != 2)) return FIMHIP_EINVAL;
   if (e->flux_conserv_schm != flux_conserv_schm) drop_graph(e);
@@ -329,6 +385,7 @@ extern "C" long long fimhip_ens_field_size(const fimhip_ens *e, int field) {
     case FIMHIP_US: case FIMHIP_VS: case FIMHIP_DP: case FIMHIP_MP: case FIMHIP_WS: case FIMHIP_RELVOR: return cz;
     case FIMHIP_PR: case FIMHIP_EX: case FIMHIP_PH: case FIMHIP_SDOT: return lz;
     case FIMHIP_TR: case FIMHIP_TRDP: return cz * e->ntra;
+    case FIMHIP_TRB: return cz * e->cb.ntrb;
     case FIMHIP_PSRF: return e->nip;
     default: return FIMHIP_EINVAL;
   }
@@ -367,6 +424,14 @@ extern "C" int fimhip_ens_download(fimhip_ens *e, int member, int field, double 
       const float *b = field == FIMHIP_TR ? s.tr : s.trdp;
       for (int t = 0; t < e->ntra; t++) {
         int r = fetch(e, b + t * ecz + oc, cz, host + t * cz);
+        if (r) return r;
+      }
+      return 0;
+    }
+    case FIMHIP_TRB: {
+      if (e->cb.ntrb < 1) return FIMHIP_EINVAL;
+      for (int t = 0; t < e->cb.ntrb; t++) {
+        int r = fetch(e, e->cb.trb + t * ecz + oc, cz, host + t * cz);
         if (r) return r;
       }
       return 0;

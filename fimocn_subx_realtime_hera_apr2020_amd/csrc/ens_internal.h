@@ -61,6 +61,20 @@ enum EnsKernel {
 };
 extern const char *const ens_kernel_names[EK_NKERNELS];
 
+// Class-B transport of the batched form: the tracers (nvl, ncol, ntrb) FP32,
+// the ring of the last three steps' mass fluxes (nvl, 6, ncol, 3) and their
+// AB-weighted time integral (nvl, 6, ncol) FP32, the thickness at the start of
+// the integration and the vertical mass flux (nvl, ncol) FP64, and per class-B
+// tracer the fct3d work fields (FP32; uan on the edges).
+struct EnsClassB {
+  int ntrb, dtratio;
+  float *trb, *ring, *cumufx;
+  double *dpinit, *w;
+  float *fld, *vfx, *fmx, *fmn, *fld1, *flp, *fln, *uan;
+};
+void ens_launch_class_b_init(const EnsGrid &g, const EnsState &s, const EnsClassB &b, hipStream_t st);
+void ens_launch_class_b_step(const EnsGrid &g, const EnsState &s, const EnsClassB &b, hipStream_t st);
+
 void ens_launch_abstart(const EnsState &s, double dt, hipStream_t st);
 void ens_launch(int kernel, int tile, const EnsGrid &g, const EnsState &s, hipStream_t st);
 void ens_launch_pres_from_exner(const EnsGrid &g, const EnsState &s, hipStream_t st);

@@ -41,7 +41,7 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_synth_state", "fimhip_synth_device", "fimhip_profile", "fimhip_profile_report",
            "fimhip_pow", "fimhip_version", "fimhip_ens_create", "fimhip_ens_destroy", "fimhip_ens_synth",
            "fimhip_ens_load_member", "fimhip_ens_init_state", "fimhip_ens_step", "fimhip_ens_use_graph",
-           "fimhip_ens_set_flux_scheme", "fimhip_ens_sync", "fimhip_ens_field_size", "fimhip_ens_download",
+           "fimhip_ens_set_flux_scheme", "fimhip_ens_set_class_b", "fimhip_ens_load_class_b", "fimhip_ens_sync", "fimhip_ens_field_size", "fimhip_ens_download",
            "fimhip_ens_globsum",
            "fimhip_ens_kernel_count", "fimhip_ens_kernel_name", "fimhip_ens_set_tile", "fimhip_ens_get_tile",
            "fimhip_ens_autotune", "fimhip_ens_profile", "fimhip_ens_profile_report"]
@@ -135,6 +135,8 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ens_step.argtypes = [vp, ct.c_int, ct.c_int]
     lib.fimhip_ens_use_graph.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_set_flux_scheme.argtypes = [vp, ct.c_int]
+    lib.fimhip_ens_set_class_b.argtypes = [vp, ct.c_int, ct.c_int]
+    lib.fimhip_ens_load_class_b.argtypes = [vp, ct.c_int, _PD]
     lib.fimhip_ens_sync.argtypes = [vp]
     lib.fimhip_ens_field_size.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_field_size.restype = ct.c_longlong
@@ -434,6 +436,7 @@ class Ensemble:
         self.lib = load_library()
         self.grid, self.members, self.nvl, self.ntra, self.dt = grid, members, nvl, ntra, dt
         self.nip = grid.nip
+        self.ntrb = 0
         gd, self._keep = _grid_desc(grid, nvl, ntra, dt, thetac, dpsig)
         h = ct.c_void_p()
         _chk(self.lib.fimhip_ens_create(ct.byref(gd), members, device, ct.byref(h)), "fimhip_ens_create")
@@ -471,6 +474,17 @@ class Ensemble:
         """1 = cnuity (FCT), 2 = mass_adv, for the steps that follow (dyn_run.F90:377-386)."""
         _chk(self.lib.fimhip_ens_set_flux_scheme(self.h, int(flux_conserv_schm)), "ens_set_flux_scheme")
 
+    def set_class_b(self, ntrb: int, dtratio: int = 6):
+        """Class-B tracers (transp3d.F90 / fct3d.F90) moved every dtratio steps."""
+        _chk(self.lib.fimhip_ens_set_class_b(self.h, int(ntrb), int(dtratio)), "ens_set_class_b")
+        self.ntrb = int(ntrb)
+
+    def load_class_b(self, member: int, trb):
+        a = np.ascontiguousarray(trb, np.float64)
+        if a.shape != (self.ntrb, self.nip, self.nvl):
+            raise ValueError("trb: shape %s, expected %s" % (a.shape, (self.ntrb, self.nip, self.nvl)))
+        _chk(self.lib.fimhip_ens_load_class_b(self.h, member, _p(a)), "ens_load_class_b")
+
     def sync(self):
         _chk(self.lib.fimhip_ens_sync(self.h), "ens_sync")
 
@@ -478,7 +492,7 @@ class Ensemble:
         nip, nvl, ntra = self.nip, self.nvl, self.ntra
         c, l = (nip, nvl), (nip, nvl + 1)
         return dict(us=c, vs=c, dp=c, mp=c, ws=c, relvor=c, pr=l, ex=l, ph=l, sdot=l, tr=(ntra,) + c,
-                    trdp=(ntra,) + c, psrf=(nip,))[name]
+                    trdp=(ntra,) + c, psrf=(nip,), trb=(self.ntrb,) + c)[name]
 
     def download(self, member: int, name: str) -> np.ndarray:
         a = np.empty(self.shape(name))

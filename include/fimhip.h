@@ -255,8 +255,15 @@ int fimhip_ens_use_graph(fimhip_ens *ens, int on);
  * cnuity (default), 2 = mass_adv (massadv.F90:21-160), with the edge's normal
  * wind taken from its lower-index cell so the FP32 mass fluxes still cancel */
 int fimhip_ens_set_flux_scheme(fimhip_ens *ens, int flux_conserv_schm);
+/* class-B tracers in the batched form (as fimhip_set_class_b): FP32 tracers,
+ * FP64 thickness and vertical mass flux, moved every dtratio steps on the
+ * device's step counter; with ntrb > 0, fimhip_ens_init_state restarts the
+ * integration.  load_class_b takes one member's (nvl,nip,ntrb) FP64 array;
+ * fimhip_ens_download returns them as FIMHIP_TRB. */
+int fimhip_ens_set_class_b(fimhip_ens *ens, int ntrb, int dtratio);
+int fimhip_ens_load_class_b(fimhip_ens *ens, int member, const double *trb);
 int fimhip_ens_sync(fimhip_ens *ens);
-/* per-member element count of FIMHIP_US, VS, DP, PR, EX, PH, MP, TR, TRDP, WS, RELVOR, SDOT, PSRF */
+/* per-member element count of FIMHIP_US, VS, DP, PR, EX, PH, MP, TR, TRDP, WS, RELVOR, SDOT, PSRF, TRB */
 long long fimhip_ens_field_size(const fimhip_ens *ens, int field);
 int fimhip_ens_download(fimhip_ens *ens, int member, int field, double *host);
 /* globsum.F90:70-101 per member: mass[m] and trtot[m*ntra+t] from FP64 sums;

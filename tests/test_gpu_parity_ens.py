@@ -388,3 +388,95 @@ def test_forty_layers_fp32_form_against_fp64():
     finally:
         e.close()
         p.close()
+
+
+def _class_b_fields(g, ntrb, nvl=NVL):
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    lg = fimhip.synth_state(g.lat, g.lon, nvl, 1 + ntrb, seed=7, member=3, flags=fimhip.SYNTH_LOGNORMAL)
+    return np.ascontiguousarray(lg["tr"][1:])
+
+
+def test_class_b_members_independent_graph_equals_eager():
+    """Class-B tracers in the batched form (moved every second step on the
+    device's step counter): each member of a graph-replayed batch has exactly
+    the bytes of the same member run alone, one kernel at a time, with other
+    launch shapes."""
+    g, th, ds = _grid(3)
+    print("case: G3 ensemble of 2 vs singles, 7 steps, 2 class-B tracers every 2nd step", flush=True)
+    trb = [_class_b_fields(g, 2) * (1 + m) for m in range(2)]
+    e2 = _ens(g, th, ds, 2, [0, 1])
+    try:
+        e2.set_class_b(2, 2)
+        for m in range(2):
+            e2.load_class_b(m, trb[m])
+        e2.init_state(True)
+        e2.step(7, 0)
+        e2.sync()
+        for m in range(2):
+            e1 = _ens(g, th, ds, 1, [m])
+            e1.set_class_b(2, 2)
+            e1.load_class_b(0, trb[m])
+            e1.init_state(True)
+            e1.use_graph(False)
+            e1.set_tiles({n: 6 for n in e1.kernel_names()})
+            for _ in range(7):
+                e1.step(1, 0)
+            e1.sync()
+            a, b = _fields(e2, m), _fields(e1, 0)
+            a["trb"], b["trb"] = e2.download(m, "trb"), e1.download(0, "trb")
+            for n in a:
+                assert np.array_equal(a[n], b[n]), (m, n)
+            assert not np.array_equal(a["trb"], trb[m].astype(np.float32))
+            e1.close()
+    finally:
+        e2.close()
+
+
+def test_class_b_fp32_form_against_fp64():
+    """Class-B tracers (2, moved every third step) in the FP32 batched form
+    against the FP64 path from the same state: relative L2 of member 0's
+    class-B tracers after 13 transport steps, and their global amounts drift
+    no more than the FP64 path's (fct3d clips without recovering)."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    g, th, ds = _grid(4)
+    print("case: G4 FP32 form vs FP64, 13 transport steps, 2 class-B tracers every 3rd step", flush=True)
+    trb = _class_b_fields(g, 2)
+    e = fimhip.Ensemble(g, 2, NVL, NTRA, DT, th, ds)
+    p = fimhip.Model(g, NVL, NTRA, DT, th, ds)
+    try:
+        st = fimhip.synth_state(g.lat, g.lon, NVL, NTRA, seed=20240601, member=0, flags=FLAGS)
+        for n in ("us", "vs", "tr"):          # the values the FP32 form holds
+            st[n] = st[n].astype(np.float32).astype(np.float64)
+        trb32 = trb.astype(np.float32).astype(np.float64)
+        for m in range(2):
+            e.load_member(m, st)
+        e.set_class_b(2, 3)
+        for m in range(2):
+            e.load_class_b(m, trb32)
+        e.init_state(False)
+        p.set_class_b(2, 3)
+        p.load(st)
+        p.upload("trb", trb32)
+        p.init_state(False)
+        area = g.area.astype(np.float64)[:, None]
+
+        def amount(dp, t):
+            return np.array([np.sum(area * dp * t[i]) for i in range(t.shape[0])])
+
+        a0 = amount(p.download("dp"), trb32)
+        e.step(13, 1)
+        for its in range(1, 14):
+            p.step(its, 1)
+        p.sync()
+        e.sync()
+        te, tp = e.download(0, "trb"), p.download("trb")
+        errs = [rel_l2(te[i], tp[i]) for i in range(2)]
+        de = amount(e.download(0, "dp"), te) / a0 - 1
+        dpl = amount(p.download("dp"), tp) / a0 - 1
+        print("class-B rel L2 %s; amount drift FP32 %s, FP64 %s" % (errs, de, dpl), flush=True)
+        assert max(errs) < 1e-4, errs
+        assert np.all(np.abs(de) <= np.abs(dpl) + 1e-6), (de, dpl)
+        assert te.min() >= 0.
+    finally:
+        e.close()
+        p.close()
