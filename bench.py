@@ -27,6 +27,8 @@ Prints ONE JSON line.  Besides the contract fields it carries
                 mass_adv the same step with the flux-limited mass transport
                 (flux_conserv_schm = 2) and its two kernels; under class_b the
                 step with 8 class-B tracers moved every 6 steps by fct3d
+                (ensemble.mass_adv, ensemble.class_b: the same for the
+                batched form, 4 class-B tracers per member)
   uneven_batch: G6 and G7 concatenated into one instance (two resolutions in
                 one launch), mixed form and FP64
   accuracy_96:  relative L2 of member 0 against FP64 after 96 steps, mass drift
@@ -353,7 +355,42 @@ def run_ens(a, grid, th, ds, fimhip):
            "kernels": kernel_table(prof, kb, cells), "tiles": tune}
     if not a.headline_only:
         res["mass_adv"] = run_ens_mass_adv(a, e, cells, kb, fimhip)
+        res["class_b"] = run_ens_class_b(a, e, grid, cells, fimhip)
     return e, res
+
+
+def run_ens_class_b(a, e, grid, cells, fimhip, ntrb=4, dtratio=6):
+    """The batched step with ntrb class-B tracers per member (log-normal
+    fields) moved every dtratio steps: step time over 2 * dtratio steps and
+    the class-B kernels (the transport kernels of the other steps return at
+    once, so their mean includes those launches)."""
+    import torch
+    e.set_class_b(ntrb, dtratio)
+    for mm in range(a.members):
+        e.synth(mm, seed=SEED, synth_member=mm, flags=fimhip.SYNTH_WIND_NOISE | fimhip.SYNTH_PERTURB)
+    lg = fimhip.synth_state(grid.lat, grid.lon, a.nvl, 1 + ntrb, seed=7, member=3, flags=fimhip.SYNTH_LOGNORMAL)
+    for mm in range(a.members):
+        e.load_class_b(mm, lg["tr"][1:])
+    del lg
+    e.init_state(True)
+    e.step(dtratio, a.mode)
+    e.sync()
+    torch.cuda.synchronize()
+    n_timed = 2 * dtratio
+    t0 = time.perf_counter()
+    e.step(n_timed, a.mode)
+    e.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    e.profile(True)
+    e.step(n_timed, a.mode)
+    e.sync()
+    prof = e.profile_report()
+    e.profile(False)
+    e.set_class_b(0, dtratio)
+    kt = kernel_table(prof, {}, cells)
+    return {"ntrb": ntrb, "dtratio": dtratio, "ms_per_step": wall / n_timed * 1e3, "timed_steps": n_timed,
+            "kernels": {k: v for k, v in kt.items() if k.startswith("eb_")}}
 
 
 def run_ens_mass_adv(a, e, cells, kb, fimhip):
