@@ -552,7 +552,9 @@ __global__ void __launch_bounds__(64 * COLS) k_e_massadv2(EnsGrid g, EnsState s)
 // here from the tracer values this kernel loads anyway), the others as the
 // average of both cells' reconstructions (stored by edgvar3; forming the
 // neighbour's here from its ring instead measured 7 ms slower at 16 x G7 x
-// 64 than the stored values).  trcadv1 leaves trcadv2's sums of the
+// 64 than the stored values; forming the cell's own one here from the
+// neighbour values already loaded saved no time either: 81 VGPRs instead of
+// 63, occupancy 5 instead of 7).  trcadv1 leaves trcadv2's sums of the
 // antidiffusive out- and inflows in tr_rp/tr_rm, and trcadv2 forms the local
 // bounds of trcadv1 again from the tracer values instead of reading them.
 
