@@ -530,6 +530,46 @@ def test_forty_layers_steps_match_checker_bitwise(oracle_lib, flux_scheme, ntrb)
         m.close()
 
 
+@pytest.mark.parametrize("nvl,flux_scheme,ntrb", [(64, 1, 0), (16, 2, 2)])
+def test_smallest_grid_steps_match_checker_bitwise(oracle_lib, nvl, flux_scheme, ntrb):
+    """G1, the smallest grid of the reference generator (42 cells, 12 of them
+    pentagons), with 64 and 16 layers, both continuity schemes and class-B
+    transport every 3 steps: start-up plus 10 full steps of the
+    device-resident path identical to the checker (which matches the
+    reference Fortran on these cases)."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    from helpers import seeded_case
+    O = oracle_lib
+    g, st, th, ds = seeded_case(1, nvl=nvl)
+    print("case: G1 nvl=%d ntra=4 scheme=%d ntrb=%d 10 full steps" % (nvl, flux_scheme, ntrb), flush=True)
+    if ntrb:
+        lg = fimhip.synth_state(g.lat, g.lon, nvl, 1 + ntrb, seed=7, member=3, flags=fimhip.SYNTH_LOGNORMAL)
+        st["trb"] = np.ascontiguousarray(lg["tr"][1:1 + ntrb])
+    o = O.OracleRun(O.grid_arrays(g, nvl, DT, th, ds), nvl, NTRA, DT, pow_kind=1, flux_scheme=flux_scheme,
+                    ntrb=ntrb, dtratio=3)
+    o.load(st)
+    o.init(True)
+    m = fimhip.Model(g, nvl, NTRA, DT, th, ds)
+    try:
+        m.set_flux_scheme(flux_scheme)
+        if ntrb:
+            m.set_class_b(ntrb, 3)
+        m.load(st)
+        m.init_state(True)
+        for its in range(1, 11):
+            m.step(its, 0)
+            o.step(its, 0)
+        m.sync()
+        fields = ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp", "ws", "massfx", "sdot", "relvor")
+        if ntrb:
+            fields += ("trb", "cumufx", "dpinit")
+        for f in fields:
+            got = m.download(f)
+            assert _same(got, o.a[f]), (f, rel_l2(got, o.a[f]))
+    finally:
+        m.close()
+
+
 def test_r4_dropins_are_the_r8_path_rounded(case):
     """The real*4 drop-ins (the production build's float arrays): every
     routine's outputs are the _r8 entry's outputs on the widened inputs,

@@ -390,6 +390,59 @@ def test_forty_layers_fp32_form_against_fp64():
         p.close()
 
 
+def test_smallest_grid_batched_form():
+    """G1 (42 cells, 12 pentagons; the launches are mostly padding) with 16
+    layers: each member of a batch of 3 is bit-identical to itself run alone
+    with other launch shapes, and member 0 after 24 full steps stays within
+    BOUNDS_96 of the FP64 path with mass conserved to 1e-12."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+    g = fimhip.icos_grid(1)
+    nvl = 16
+    th, ds = state.vertical_tables(nvl)
+    print("case: G1 nvl=16 batch of 3 vs singles, FP32 form vs FP64, 24 full steps", flush=True)
+    e = fimhip.Ensemble(g, 3, nvl, NTRA, DT, th, ds)
+    p = fimhip.Model(g, nvl, NTRA, DT, th, ds)
+    try:
+        for m in range(3):
+            e.synth(m, seed=20240601, synth_member=m, flags=FLAGS)
+        e.init_state(True)
+        p.synth(seed=20240601, member=0, flags=FLAGS)
+        p.init_state(True)
+        m0, _, _, _ = e.globsum()
+        e.step(24, 0)
+        for its in range(1, 25):
+            p.step(its, 0)
+        p.sync()
+        e.sync()
+        m1, _, _, _ = e.globsum()
+        fields = ("us", "vs", "dp", "pr", "ex", "tr")
+        for m in range(3):
+            e1 = fimhip.Ensemble(g, 1, nvl, NTRA, DT, th, ds)
+            try:
+                e1.synth(0, seed=20240601, synth_member=m, flags=FLAGS)
+                e1.init_state(True)
+                e1.use_graph(False)
+                e1.set_tiles({n: 5 for n in e1.kernel_names()})
+                for _ in range(24):
+                    e1.step(1, 0)
+                e1.sync()
+                for n in fields:
+                    assert np.array_equal(e.download(m, n), e1.download(0, n)), (m, n)
+            finally:
+                e1.close()
+        errs = {n: rel_l2(e.download(0, n), p.download(n)) for n in ("us", "vs", "dp", "pr", "ex")}
+        tr_e, tr_p = e.download(0, "tr"), p.download("tr")
+        errs["tr0"] = rel_l2(tr_e[0], tr_p[0])
+        errs["tr"] = max(rel_l2(tr_e[t], tr_p[t]) for t in range(1, NTRA))
+        print("G1 nvl=16 rel L2 after 24 steps:", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+        for n, b in BOUNDS_96.items():
+            assert errs[n] < b, (n, errs[n], b)
+        assert np.all(np.abs(m1 - m0) <= 1e-12 * m0)
+    finally:
+        e.close()
+        p.close()
+
+
 def _class_b_fields(g, ntrb, nvl=NVL):
     from fimocn_subx_realtime_hera_apr2020_amd import fimhip
     lg = fimhip.synth_state(g.lat, g.lon, nvl, 1 + ntrb, seed=7, member=3, flags=fimhip.SYNTH_LOGNORMAL)

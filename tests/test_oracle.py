@@ -193,6 +193,43 @@ def test_class_b_transport_vs_live_reference_bitwise(oracle_lib, glvl_, mode, ns
     assert not np.array_equal(o.a["trb"], st["trb"])
 
 
+@pytest.mark.parametrize("nvl, flux_scheme, ntrb, nsteps", [(64, 1, 0, 12), (16, 2, 2, 10), (16, 1, 0, 12)])
+def test_smallest_grid_vs_live_reference_bitwise(oracle_lib, nvl, flux_scheme, ntrb, nsteps):
+    """G1, the smallest grid the reference generator builds (42 cells, 12 of
+    them pentagons: the 5-neighbour paths carry a quarter of the cells), with
+    64 and 16 layers, both continuity schemes and class-B transport every 3
+    steps: start-up plus the full steps against the reference Fortran, bit
+    for bit."""
+    from oracle import refrun
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, glvl
+    if not refrun.available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    O = oracle_lib
+    path = refrun.ref_glvl(1)
+    g = glvl.read_glvl(path)
+    _, st, th, ds = seeded_case(g, nvl=nvl)
+    assert g.nip == 42 and int(np.sum(np.asarray(g.nprox) == 5)) == 12
+    if ntrb:
+        lg = fimhip.synth_state(g.lat, g.lon, nvl, 1 + ntrb, seed=7, member=3, flags=fimhip.SYNTH_LOGNORMAL)
+        st["trb"] = np.ascontiguousarray(lg["tr"][1:1 + ntrb])
+    r = refrun.run_dynref(path, st, th, ds, nsteps=nsteps, dt=240.0, mode=0, init_hybgen=True,
+                          flux_scheme=flux_scheme, dtratio=3)
+    o = _oracle_run(O, g, st, th, ds, 240.0, flux_scheme=flux_scheme, dtratio=3)
+    o.init(True)
+    for its in range(1, nsteps + 1):
+        o.step(its, 0)
+    for f in ("us", "vs", "dp", "pr", "ex", "ph", "mp"):
+        assert np.array_equal(o.a[f].ravel(), r["final." + f]), f
+    for t in range(4):
+        assert np.array_equal(o.a["tr"][t].ravel(), r["final.tr"][t]), ("tr", t)
+    for t in range(ntrb):
+        assert np.array_equal(o.a["trb"][t].ravel(), r["final.trb"][t]), ("trb", t)
+    if ntrb:
+        assert np.array_equal(o.a["cumufx"].ravel(), r["final.cumufx"])
+        assert not np.array_equal(o.a["trb"], st["trb"])
+    assert not np.array_equal(o.a["us"], st["us"])
+
+
 @pytest.mark.parametrize("run", ["massadv_mode0", "classb_mode1", "classb_massadv_mode0"])
 def test_schemes_bitexact_vs_reference_golden(oracle_lib, run):
     """mass_adv and class-B transport: the final state has the exact bytes of
