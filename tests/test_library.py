@@ -45,6 +45,27 @@ def test_create_rejects_bad_sizes_without_gpu_calls():
     assert lib.fimhip_create(None, 0, ct.byref(h)) == -1
 
 
+def test_every_entry_rejects_a_null_handle_without_gpu_calls():
+    """Every entry point that takes an instance handle returns
+    FIMHIP_EINVAL (-1) for a null one, with null arrays and zero sizes, and
+    touches no device (this runs without a GPU)."""
+    lib = fimhip.load_library()
+    no_handle = {"fimhip_create", "fimhip_ens_create", "fimhip_icos_grid", "fimhip_icos_grid_info",
+                 "fimhip_pow", "fimhip_ens_kernel_count", "fimhip_ens_kernel_name", "fimhip_synth_state",
+                 "fimhip_version"}
+    checked = 0
+    for name in fimhip.EXPORTS:
+        if name in no_handle:
+            continue
+        f = getattr(lib, name)
+        assert f.argtypes, name
+        args = [0.0 if t in (ct.c_double, ct.c_float) else 0 if t in (ct.c_int, ct.c_int64, ct.c_size_t) else None
+                for t in f.argtypes]
+        assert f(*args) == -1, name
+        checked += 1
+    assert checked >= 50
+
+
 def test_glvl_roundtrip(tmp_path):
     g = golden_grid(2)
     p = tmp_path / "glvl.dat"
