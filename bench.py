@@ -94,13 +94,13 @@ def ens_kernel_bytes(T):
     """The same for the mixed-precision kernels (4 B per FP32 value, 8 B per
     FP64 value; inputs shared by the tracers of one launch counted once)."""
     return {
-        "e_edgvar1": 20 + 52,                # us vs tr1 (f32) pr (f64) -> normal wind and lp edges, step-start theta (f32)
+        "e_edgvar1": 20 + 36,                # us vs tr1 (f32) pr (f64) -> normal wind edges, pressure gradient, step-start theta (f32)
         "e_edgvar3": 32 + 28 * (T - 1),      # dp (f64) -> dp_edg; tracers 2..T -> trc_edg
         "e_cnuity12": 104,                   # normal wind and dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo, sums
         "e_cnuity3": 44,                     # in/outflow sums, delp_lo, dp, dp_tdcy(of,vof) -> r+, r-
-        "e_cnuity45": 180,                   # antifx r+- massfx dp_tdcy dp pr lp_edg u v -> massfx dp_tdcy dp pr ex omega
+        "e_cnuity45": 164,                   # antifx r+- massfx dp_tdcy dp pr pressure gradient u v -> massfx dp_tdcy dp pr ex omega
         "e_massadv1": 104,                   # normal wind and dp edges, dp, dp_tdcy(of,vof) -> q_star edges, q_min/max_out
-        "e_massadv2": 172,                   # normal wind, q_star, bounds, dp_tdcy, dp, lp_edg, u v -> massfx dp_tdcy dp pr ex omega
+        "e_massadv2": 156,                   # normal wind, q_star, bounds, dp_tdcy, dp, pressure gradient, u v -> massfx dp_tdcy dp pr ex omega
         "e_trcadv1": 8 + 80 * T,             # massfx dp; per tracer trc_edg (not theta's) tr trdp trl -> antiflx trl lo sums
         "e_trcadv2": 8 + 40 * T,
         "e_trcadv3": 8 + 68 * T,

@@ -115,7 +115,7 @@ extern "C" int fimhip_ens_create(const fimhip_grid_desc *gd, int members, int de
       {&s.us, cz}, {&s.vs, cz}, {&s.u2, cz}, {&s.v2, cz}, {&s.tr, cz * ntra}, {&s.trdp, cz * ntra},
       {&s.u_tdcy, 3 * cz}, {&s.v_tdcy, 3 * cz}, {&s.dpl_tdcy, 3 * cz}, {&s.trc_tdcy, 3 * cz * ntra},
       {&s.trl_tdcy, 3 * cz * ntra}, {&s.massfx, ez}, {&s.ws, cz}, {&s.relvor, cz},
-      {&s.vn_edg, ez}, {&s.dp_edg, ez}, {&s.lp_edg, ez}, {&s.trc_edg, ez * (ntra > 1 ? ntra - 1 : 1)}, {&s.th0, cz},
+      {&s.vn_edg, ez}, {&s.dp_edg, ez}, {&s.pgx, cz}, {&s.pgy, cz}, {&s.trc_edg, ez * (ntra > 1 ? ntra - 1 : 1)}, {&s.th0, cz},
       {&s.antifx, ez}, {&s.delp_lo, cz}, {&s.r_plus, cz}, {&s.r_mnus, cz}, {&s.antiflx, ez * ntra},
       {&s.trcr_lo, cz * ntra}, {&s.trmax, cz * ntra}, {&s.trmin, cz * ntra}, {&s.tr_rp, cz * ntra},
       {&s.tr_rm, cz * ntra}};

@@ -41,7 +41,8 @@ struct EnsState {
   double *dp_tdcy;
   float *trc_tdcy, *trl_tdcy;
   float *massfx, *ws, *relvor;
-  float *vn_edg, *dp_edg, *lp_edg, *trc_edg;   // vn_edg: normal edge wind; trc_edg: tracers 2..ntra
+  float *vn_edg, *dp_edg, *trc_edg;   // vn_edg: normal edge wind; trc_edg: tracers 2..ntra
+  float *pgx, *pgy;                   // the edge-pressure line integrals of cnuity5 (per cell-level)
   float *th0;                                  // theta at the start of the step
   float *antifx, *delp_lo, *r_plus, *r_mnus;
   float *antiflx, *trcr_lo, *trmax, *trmin, *tr_rp, *tr_rm;

@@ -123,6 +123,24 @@ __device__ __forceinline__ float e_theta_edge(float thi, float thx, float tht, f
 // itself (e_edge_wind).  Theta's edge values are formed where they are used
 // (trcadv1, momtum: e_theta_edge) from the cells' theta; edgvar1 keeps the
 // step's starting theta in th0 for momtum.
+// cnuity5's pressure gradient (cnuity_compute.F90:421-428): the line integral
+// of the mid-layer edge pressures lp_edg along the cell's edge vectors.  It
+// depends only on the pressure at the start of the step, so edgvar1 forms it
+// once per cell-level (two values) where it forms the edge pressures, instead
+// of storing the six edge pressures for cnuity5; separate roundings, in the
+// reference's edge order.
+template <int NP>
+__device__ __forceinline__ float2 e_pres_grad(const EnsGrid &g, int ipn, const float *lp) {
+#pragma clang fp contract(off)
+  float dpdx = 0.f, dpdy = 0.f;
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    dpdx = dpdx + lp[e] * FSVC(ipn, e, 1);
+    dpdy = dpdy - lp[e] * FSVC(ipn, e, 0);
+  }
+  return make_float2(dpdx, dpdy);
+}
+
 template <int NP>
 __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &s, int ipn, int base, int k) {
 #pragma clang fp contract(fast)
@@ -145,15 +163,18 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
   float mn[NP];
 #pragma unroll
   for (int e = 0; e < NP; e++) mn[e] = fsqrt(nu[e] * nu[e] + nv[e] * nv[e]);
+  float lp[NP];
 #pragma unroll
   for (int e = 0; e < NP; e++) {
     const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
     const float2 w = e_edge_wind(g, ipn, e, ui, vi, nu[e], nv[e], nu[et], nv[et], nu[el], nv[el], mi, mn[e], mn[et],
                                  mn[el]);
     EE(s.vn_edg, col, e, k) = FSVE(ipn, e, 1) * w.x - FSVE(ipn, e, 0) * w.y;
-    EE(s.lp_edg, col, e, k) =
-        (float)((pi0 + pi1 + np0[e] + np1[e] + .5 * (np0[et] + np1[et] + np0[el] + np1[el])) * (1. / 6.));
+    lp[e] = (float)((pi0 + pi1 + np0[e] + np1[e] + .5 * (np0[et] + np1[et] + np0[el] + np1[el])) * (1. / 6.));
   }
+  const float2 pg = e_pres_grad<NP>(g, ipn, lp);
+  EC(s.pgx, col, k) = pg.x;
+  EC(s.pgy, col, k) = pg.y;
   // theta at the start of the step, for momtum's edge values (the tracer
   // itself is updated by trcadv before momtum runs)
   EC(s.th0, col, k) = thi;
@@ -338,19 +359,11 @@ __device__ __forceinline__ void e_update_pres_col(const EnsGrid &g, const EnsSta
   const double sp = wave_suffix_sum(dpn, k);
   const double sl = wave_suffix_sum(lt, k);
   if (!live) return;
-  float l[NP];
-#pragma unroll
-  for (int e = 0; e < NP; e++) l[e] = EE(s.lp_edg, col, e, k);
+  const float dpdx = EC(s.pgx, col, k), dpdy = EC(s.pgy, col, k);   // formed by edgvar1
   const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k);
   const double p = EL(s.pr, col, nvl) + sp;
   EL(s.pr, col, k) = p;
   EL(s.ex, col, k) = CP * fim_pow_fast(p / P1000, RD / CP);
-  float dpdx = 0.f, dpdy = 0.f;
-#pragma unroll
-  for (int e = 0; e < NP; e++) {
-    dpdx = dpdx + l[e] * FSVC(ipn, e, 1);
-    dpdy = dpdy - l[e] * FSVC(ipn, e, 0);
-  }
   const float om = (float)(sl - .5 * lt);
   EC(s.ws, col, k) = om + (ui * dpdx + vi * dpdy) * GF(rarea, ipn);
 }
