@@ -386,7 +386,7 @@ extern "C" int fimhip_edgvar3_r8(fimhip_ctx *c, int its, const double *u_vel, co
   HIPCHK(hipSetDevice(c->device));
   TRY(h2d(c, FIMHIP_DP, delp)); TRY(h2d(c, FIMHIP_TR, tracr));
   TRY(h2d(c, FIMHIP_TRC_EDG, trc_edg));   // tracer 1 edges pass through
-  launch_edgvar3(c->g, c->s, c->stream);
+  launch_edgvar3(c->g, c->s, c->stream, true);
   TRY(d2h(c, FIMHIP_DP_EDG, dp_edg)); TRY(d2h(c, FIMHIP_TRC_EDG, trc_edg));
   return finish(c);
 }
@@ -404,7 +404,7 @@ extern "C" int fimhip_cnuity_r8(fimhip_ctx *c, int its, const double *u_velo, co
   TRY(h2d(c, FIMHIP_DP, delp)); TRY(h2d(c, FIMHIP_PR, pres)); TRY(h2d(c, FIMHIP_EX, exner));
   TRY(h2d(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(h2d(c, FIMHIP_DPL_TDCY, dplo_tndcy));
   TRY(h2d(c, FIMHIP_MASSFX, massfx));
-  launch_cnuity(c->g, c->s, c->ab, c->stream);
+  launch_cnuity(c->g, c->s, c->ab, c->stream, true);
   TRY(d2h(c, FIMHIP_DP, delp)); TRY(d2h(c, FIMHIP_PR, pres)); TRY(d2h(c, FIMHIP_EX, exner));
   TRY(d2h(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(d2h(c, FIMHIP_DPL_TDCY, dplo_tndcy));
   TRY(d2h(c, FIMHIP_MASSFX, massfx)); TRY(d2h(c, FIMHIP_WS, omega));
@@ -422,7 +422,7 @@ extern "C" int fimhip_mass_adv_r8(fimhip_ctx *c, int its, const double *u_velo, 
   TRY(h2d(c, FIMHIP_V_EDG, v_edg)); TRY(h2d(c, FIMHIP_DP_EDG, dp_edg)); TRY(h2d(c, FIMHIP_LP_EDG, lp_edg));
   TRY(h2d(c, FIMHIP_DP, delp)); TRY(h2d(c, FIMHIP_PR, pres)); TRY(h2d(c, FIMHIP_EX, exner));
   TRY(h2d(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(h2d(c, FIMHIP_MASSFX, massfx));
-  launch_mass_adv(c->g, c->s, c->ab, c->stream);
+  launch_mass_adv(c->g, c->s, c->ab, c->stream, true);
   TRY(d2h(c, FIMHIP_DP_EDG, dp_edg)); TRY(d2h(c, FIMHIP_DP, delp)); TRY(d2h(c, FIMHIP_PR, pres));
   TRY(d2h(c, FIMHIP_EX, exner)); TRY(d2h(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(d2h(c, FIMHIP_MASSFX, massfx));
   TRY(d2h(c, FIMHIP_WS, omega));
@@ -475,7 +475,7 @@ extern "C" int fimhip_trcadv_r8(fimhip_ctx *c, int its, const double *trcr_edg, 
   TRY(h2d(c, FIMHIP_TRC_EDG, trcr_edg)); TRY(h2d(c, FIMHIP_TR, tracr)); TRY(h2d(c, FIMHIP_TRDP, trcdp));
   TRY(h2d(c, FIMHIP_TRC_TDCY, trc_tdcy)); TRY(h2d(c, FIMHIP_TRL_TDCY, trclo_tdcy));
   TRY(h2d(c, FIMHIP_MASSFX, massfx)); TRY(h2d(c, FIMHIP_DP, delp));
-  launch_trcadv(c->g, c->s, c->ab, c->stream);
+  launch_trcadv(c->g, c->s, c->ab, c->stream, true);
   TRY(d2h(c, FIMHIP_TR, tracr)); TRY(d2h(c, FIMHIP_TRDP, trcdp));
   TRY(d2h(c, FIMHIP_TRC_TDCY, trc_tdcy)); TRY(d2h(c, FIMHIP_TRL_TDCY, trclo_tdcy));
   return finish(c);
@@ -732,9 +732,9 @@ extern "C" int fimhip_step(fimhip_ctx *c, int its, int mode) {
   } scope(c->kt);
   ab_rotate(c->ab, its, c->dt);
   launch_edgvar1(c->g, c->s, c->stream);
-  launch_edgvar3(c->g, c->s, c->stream);
-  if (c->flux_conserv_schm == 2) launch_mass_adv(c->g, c->s, c->ab, c->stream);
-  else launch_cnuity(c->g, c->s, c->ab, c->stream);
+  launch_edgvar3(c->g, c->s, c->stream, false);
+  if (c->flux_conserv_schm == 2) launch_mass_adv(c->g, c->s, c->ab, c->stream, false);
+  else launch_cnuity(c->g, c->s, c->ab, c->stream, false);
   if (c->cb.ntrb > 0) {             // dyn_run.F90:388-405
     launch_transp1(c->g, c->s, c->cb, c->ab, c->stream);
     if (its % c->dtratio == 0) {
@@ -742,7 +742,7 @@ extern "C" int fimhip_step(fimhip_ctx *c, int its, int mode) {
       launch_transp0(c->g, c->s, c->cb, c->stream);
     }
   }
-  launch_trcadv(c->g, c->s, c->ab, c->stream);
+  launch_trcadv(c->g, c->s, c->ab, c->stream, false);
   if (mode == 0) {
     launch_momtum(c->g, c->s, c->ab, c->stream);
     launch_hybgen(c->g, c->s, its, c->stream);

@@ -135,13 +135,16 @@ void kt_end(hipStream_t st);
 
 // launch helpers (kernels_plain.hip)
 void launch_edgvar1(const DevGrid &g, const DevState &s, hipStream_t st);
-void launch_edgvar3(const DevGrid &g, const DevState &s, hipStream_t st);
-void launch_cnuity(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st);
-void launch_mass_adv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st);
+// two_sided: edgvar3 stores the two-sided edge averages (the drop-in entries'
+// arrays); false: each cell's own reconstructions, averaged by the readers
+// (the device-resident step).  Results are the same bits either way.
+void launch_edgvar3(const DevGrid &g, const DevState &s, hipStream_t st, bool two_sided);
+void launch_cnuity(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st, bool two_sided);
+void launch_mass_adv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st, bool two_sided);
 void launch_transp0(const DevGrid &g, const DevState &s, const DevClassB &b, hipStream_t st);
 void launch_transp1(const DevGrid &g, const DevState &s, const DevClassB &b, const AB &ab, hipStream_t st);
 void launch_transp2(const DevGrid &g, const DevState &s, const DevClassB &b, hipStream_t st);
-void launch_trcadv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st);
+void launch_trcadv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st, bool two_sided);
 void launch_momtum(const DevGrid &g, DevState &s, const AB &ab, hipStream_t st);
 void launch_hybgen(const DevGrid &g, const DevState &s, int its, hipStream_t st);
 void launch_hystat(const DevGrid &g, const DevState &s, int its, hipStream_t st);

@@ -88,9 +88,14 @@ __global__ void __launch_bounds__(256) k_edgvar1(DevGrid g, DevState s) {
 }
 
 // ---------------------------------------------------------------------------
-// edgvar3.F90:65-163: LS reconstruction on both sides of each edge
-// (edgvalEval) and the two-sided average of loop 2, fused: each cell-level
-// recomputes its neighbour's reconstruction of the shared edge.
+// edgvar3.F90:65-163: LS reconstruction of each edge (edgvalEval) and the
+// two-sided average of loop 2.  TWO_SIDED (the drop-in entry): fused, each
+// cell-level recomputes its neighbour's reconstruction of the shared edge
+// (2-ring gathers).  Otherwise (the device-resident step) each cell stores
+// only its own reconstructions and the readers (cnuity12, massadv1,
+// trcadv1) form 0.5*(own + neighbour's) themselves, the same operands in the
+// same order, so every result keeps its bits.
+template <bool TWO_SIDED>
 __global__ void __launch_bounds__(256) k_edgvar3(DevGrid g, DevState s) {
   const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x;
@@ -107,6 +112,10 @@ __global__ void __launch_bounds__(256) k_edgvar3(DevGrid g, DevState s) {
     // this cell's reconstruction of edge i (edgvalEval)
     double own = b0;
     for (int j = 0; j < np; j++) own = own + (nbv[j] - b0) * HF(ipn, i, j);
+    if (!TWO_SIDED) {
+      E3(out, ipn, i, k) = own;
+      continue;
+    }
     // the neighbour's reconstruction of the same edge
     const int nb = PROX(ipn, i), eo = PROXS(ipn, i), nn = GNP(nb);
     const double c0 = nbv[i];
@@ -117,8 +126,17 @@ __global__ void __launch_bounds__(256) k_edgvar3(DevGrid g, DevState s) {
   for (int i = np; i < NPP; i++) E3(out, ipn, i, k) = 0.;
 }
 
+// an edge value of edgvar3's fields as the readers see it: stored averaged
+// (TWO_SIDED), or formed here from both cells' own reconstructions
+template <bool TWO_SIDED>
+__device__ __forceinline__ double edge_val(const double *a, int ipn, int edg, int ipx, int edx, int k, int nvl) {
+  if (TWO_SIDED) return E3(a, ipn, edg, k);
+  return 0.5 * (E3(a, ipn, edg, k) + E3(a, ipx, edx, k));
+}
+
 // ---------------------------------------------------------------------------
 // cnuity1 + cnuity2 (cnuity_compute.F90:17-150)
+template <bool TWO_SIDED>
 __global__ void __launch_bounds__(256) k_cnuity12(DevGrid g, DevState s, AB ab) {
   const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x;
@@ -132,7 +150,7 @@ __global__ void __launch_bounds__(256) k_cnuity12(DevGrid g, DevState s, AB ab) 
     const int ipx = PROX(ipn, edg), edx = PROXS(ipn, edg);
     double vn = SVE(ipn, edg, 1) * E3(s.u_edg, ipn, edg, k) - SVE(ipn, edg, 0) * E3(s.v_edg, ipn, edg, k);
     double vx = SVE(ipx, edx, 1) * E3(s.u_edg, ipx, edx, k) - SVE(ipx, edx, 0) * E3(s.v_edg, ipx, edx, k);
-    double flxhi = vn * E3(s.dp_edg, ipn, edg, k);
+    double flxhi = vn * edge_val<TWO_SIDED>(s.dp_edg, ipn, edg, ipx, edx, k, nvl);
     double m = 0.5 * ((vn + fabs(vn)) * dpi - (vx + fabs(vx)) * C2(s.dp, ipx, k));
     MF(s.massfx, nf, ipn, edg, k) = m;
     E3(s.antifx, ipn, edg, k) = flxhi - m;
@@ -263,7 +281,10 @@ __global__ void __launch_bounds__(256) k_cnuity45(DevGrid g, DevState s, AB ab) 
 // clamped to the four cells around the edge (q_star, stored back into
 // dp_edg), then the inflow sums give the bounds q_min_out / q_max_out of the
 // column's outflow edge values (kept in r_plus / r_mnus).  Normal winds and
-// Courant numbers stay in registers.
+// Courant numbers stay in registers.  Without TWO_SIDED, dp_edg holds the
+// cells' own reconstructions, which the neighbours still read, so q_star goes
+// to antifx (unused by mass_adv) and massadv2 takes it from there.
+template <bool TWO_SIDED>
 __global__ void __launch_bounds__(256) k_massadv1(DevGrid g, DevState s, AB ab) {
   const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x;
@@ -284,8 +305,8 @@ __global__ void __launch_bounds__(256) k_massadv1(DevGrid g, DevState s, AB ab) 
     const double cn = fabs(vn) * dt * rar;
     const double b = C2(s.dp, ipx, k), cc = C2(s.dp, nbm1, k), d = C2(s.dp, nbp1, k);
     const double q_min = dmin(dmin(gv, b), dmin(cc, d)), q_max = dmax(dmax(gv, b), dmax(cc, d));
-    const double qs = dmax(q_min, dmin(q_max, E3(s.dp_edg, ipn, edg, k)));
-    E3(s.dp_edg, ipn, edg, k) = qs;
+    const double qs = dmax(q_min, dmin(q_max, edge_val<TWO_SIDED>(s.dp_edg, ipn, edg, ipx, PROXS(ipn, edg), k, nvl)));
+    E3(TWO_SIDED ? s.dp_edg : s.antifx, ipn, edg, k) = qs;
     if (vn > 0.) {
       c_num_out_sum = c_num_out_sum + cn;
     } else {
@@ -319,6 +340,7 @@ __global__ void __launch_bounds__(256) k_massadv1(DevGrid g, DevState s, AB ab) 
 // update kept non-negative) and update_pres.  The surface pressure the
 // reference sums there is a local of mass_adv that nothing reads, so it is
 // not formed.
+template <bool TWO_SIDED>
 __global__ void __launch_bounds__(256) k_massadv2(DevGrid g, DevState s, AB ab) {
   __shared__ double sh_dp[COLS][NVL_MAX + 1];
   __shared__ double sh_lt[COLS][NVL_MAX + 1];
@@ -336,7 +358,8 @@ __global__ void __launch_bounds__(256) k_massadv2(DevGrid g, DevState s, AB ab) 
       const int ipx = PROX(ipn, edg);
       const double vn = SVE(ipn, edg, 1) * E3(s.u_edg, ipn, edg, k) - SVE(ipn, edg, 0) * E3(s.v_edg, ipn, edg, k);
       const int up = vn > 0. ? ipn : ipx;
-      const double qs = dmax(C2(s.r_plus, up, k), dmin(C2(s.r_mnus, up, k), E3(s.dp_edg, ipn, edg, k)));
+      const double qs =
+          dmax(C2(s.r_plus, up, k), dmin(C2(s.r_mnus, up, k), E3(TWO_SIDED ? s.dp_edg : s.antifx, ipn, edg, k)));
       E3(s.dp_edg, ipn, edg, k) = qs;
       const double m = vn * qs;
       MF(s.massfx, nf, ipn, edg, k) = m;
@@ -356,7 +379,10 @@ __global__ void __launch_bounds__(256) k_massadv2(DevGrid g, DevState s, AB ab) 
 }
 
 // ---------------------------------------------------------------------------
-// trcadv1-3 (trcadv_compute.F90:19-315); blockIdx.y = tracer
+// trcadv1-3 (trcadv_compute.F90:19-315); blockIdx.y = tracer.  Tracer 1's
+// edge values come from edgvar1 (symmetric as stored); the others from
+// edgvar3, stored averaged or per cell (TWO_SIDED as in k_edgvar3).
+template <bool TWO_SIDED>
 __global__ void __launch_bounds__(256) k_trcadv1(DevGrid g, DevState s, AB ab) {
   const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x;
@@ -375,7 +401,7 @@ __global__ void __launch_bounds__(256) k_trcadv1(DevGrid g, DevState s, AB ab) {
   for (int edg = 0; edg < np; edg++) {
     const int ipx = PROX(ipn, edg), edx = PROXS(ipn, edg);
     double m = MF(s.massfx, nf, ipn, edg, k), mx = MF(s.massfx, nf, ipx, edx, k);
-    double flx_hi = m * E3(tre, ipn, edg, k);
+    double flx_hi = m * (t == 0 ? E3(tre, ipn, edg, k) : edge_val<TWO_SIDED>(tre, ipn, edg, ipx, edx, k, nvl));
     double fl = 0.5 * ((m + fabs(m)) * ti - (mx + fabs(mx)) * C2(tracr, ipx, k));
     E3(antiflx, ipn, edg, k) = flx_hi - fl;
     acc = acc + fl;
@@ -618,22 +644,30 @@ __global__ void __launch_bounds__(256) k_trdp(DevGrid g, DevState s) {
 void launch_edgvar1(const DevGrid &g, const DevState &s, hipStream_t st) {
   KT_LAUNCH("k_edgvar1", st, k_edgvar1<<<col_grid(g.nip), col_block(), 0, st>>>(g, s));
 }
-void launch_edgvar3(const DevGrid &g, const DevState &s, hipStream_t st) {
+void launch_edgvar3(const DevGrid &g, const DevState &s, hipStream_t st, bool two_sided) {
   dim3 grid((g.nip + COLS - 1) / COLS, g.ntra);
-  KT_LAUNCH("k_edgvar3", st, k_edgvar3<<<grid, col_block(), 0, st>>>(g, s));
+  if (two_sided) KT_LAUNCH("k_edgvar3", st, k_edgvar3<true><<<grid, col_block(), 0, st>>>(g, s));
+  else KT_LAUNCH("k_edgvar3", st, k_edgvar3<false><<<grid, col_block(), 0, st>>>(g, s));
 }
-void launch_cnuity(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st) {
-  KT_LAUNCH("k_cnuity12", st, k_cnuity12<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
+void launch_cnuity(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st, bool two_sided) {
+  if (two_sided) KT_LAUNCH("k_cnuity12", st, k_cnuity12<true><<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
+  else KT_LAUNCH("k_cnuity12", st, k_cnuity12<false><<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
   KT_LAUNCH("k_cnuity3", st, k_cnuity3<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
   KT_LAUNCH("k_cnuity45", st, k_cnuity45<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
 }
-void launch_mass_adv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st) {
-  KT_LAUNCH("k_massadv1", st, k_massadv1<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
-  KT_LAUNCH("k_massadv2", st, k_massadv2<<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
+void launch_mass_adv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st, bool two_sided) {
+  if (two_sided) {
+    KT_LAUNCH("k_massadv1", st, k_massadv1<true><<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
+    KT_LAUNCH("k_massadv2", st, k_massadv2<true><<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
+  } else {
+    KT_LAUNCH("k_massadv1", st, k_massadv1<false><<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
+    KT_LAUNCH("k_massadv2", st, k_massadv2<false><<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
+  }
 }
-void launch_trcadv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st) {
+void launch_trcadv(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st, bool two_sided) {
   dim3 grid((g.nip + COLS - 1) / COLS, g.ntra);
-  KT_LAUNCH("k_trcadv1", st, k_trcadv1<<<grid, col_block(), 0, st>>>(g, s, ab));
+  if (two_sided) KT_LAUNCH("k_trcadv1", st, k_trcadv1<true><<<grid, col_block(), 0, st>>>(g, s, ab));
+  else KT_LAUNCH("k_trcadv1", st, k_trcadv1<false><<<grid, col_block(), 0, st>>>(g, s, ab));
   KT_LAUNCH("k_trcadv2", st, k_trcadv2<<<grid, col_block(), 0, st>>>(g, s, ab));
   KT_LAUNCH("k_trcadv3", st, k_trcadv3<<<grid, col_block(), 0, st>>>(g, s, ab));
 }
