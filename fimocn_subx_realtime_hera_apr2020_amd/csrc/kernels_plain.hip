@@ -43,26 +43,37 @@ static inline dim3 col_grid(int nip) { return dim3((nip + COLS - 1) / COLS); }
 static inline dim3 col_block() { return dim3(NVL_MAX, COLS); }
 
 // ---------------------------------------------------------------------------
-// edgvar1.F90:160-421 (eqwgt branch) + tracer-1 loop 425-511
-__global__ void __launch_bounds__(256) k_edgvar1(DevGrid g, DevState s) {
+// edgvar1.F90:160-421 (eqwgt branch) + tracer-1 loop 425-511.  The values of
+// the cell and its NP neighbours are loaded once into registers (slot j of
+// the stencil at index j + 1, the cell itself at 0) and the edge loop is
+// unrolled over them; every edge value is formed by the reference's
+// operations in its order.
+template <int NP>
+__device__ __forceinline__ void edgvar1_col(const DevGrid &g, const DevState &s, int ipn, int k) {
   const int nip = g.nip, nvl = g.nvl, nvlp1 = nvl + 1;
-  const int k = threadIdx.x;
-  const int ipn = wave_col();
-  if (ipn >= nip || k >= nvl) return;
   const double divby6 = 1. / 6., eps = 0.001;
-  const int np = GNP(ipn);
-  for (int edg = 0; edg < np; edg++) {
-    const int i1 = ipn, i2 = PROX(ipn, edg);
-    const int i3t = PROX(ipn, (edg - 1 + np) % np);
-    const int i3l = PROX(ipn, (edg + 1) % np);
-    double u3trl = CS(ipn, edg, 2) * C2(s.us, i3t, k) + SN(ipn, edg, 2) * C2(s.vs, i3t, k);
-    double v3trl = -SN(ipn, edg, 2) * C2(s.us, i3t, k) + CS(ipn, edg, 2) * C2(s.vs, i3t, k);
-    double u1ldg = CS(ipn, edg, 0) * C2(s.us, i1, k) + SN(ipn, edg, 0) * C2(s.vs, i1, k);
-    double v1ldg = -SN(ipn, edg, 0) * C2(s.us, i1, k) + CS(ipn, edg, 0) * C2(s.vs, i1, k);
-    double u2ldg = CS(ipn, edg, 1) * C2(s.us, i2, k) + SN(ipn, edg, 1) * C2(s.vs, i2, k);
-    double v2ldg = -SN(ipn, edg, 1) * C2(s.us, i2, k) + CS(ipn, edg, 1) * C2(s.vs, i2, k);
-    double u3ldg = CS(ipn, edg, 3) * C2(s.us, i3l, k) + SN(ipn, edg, 3) * C2(s.vs, i3l, k);
-    double v3ldg = -SN(ipn, edg, 3) * C2(s.us, i3l, k) + CS(ipn, edg, 3) * C2(s.vs, i3l, k);
+  double cu[NP + 1], cv[NP + 1], cm[NP + 1], ct[NP + 1], cp0[NP + 1], cp1[NP + 1];
+#pragma unroll
+  for (int j = 0; j <= NP; j++) {
+    const int c = j == 0 ? ipn : PROX(ipn, j - 1);
+    cu[j] = C2(s.us, c, k);
+    cv[j] = C2(s.vs, c, k);
+    cm[j] = C2(s.mp, c, k);
+    ct[j] = TR(s.tr, 0, c, k);
+    cp0[j] = L2(s.pr, c, k);
+    cp1[j] = L2(s.pr, c, k + 1);
+  }
+#pragma unroll
+  for (int edg = 0; edg < NP; edg++) {
+    const int i2 = edg + 1, i3t = (edg - 1 + NP) % NP + 1, i3l = (edg + 1) % NP + 1;
+    double u3trl = CS(ipn, edg, 2) * cu[i3t] + SN(ipn, edg, 2) * cv[i3t];
+    double v3trl = -SN(ipn, edg, 2) * cu[i3t] + CS(ipn, edg, 2) * cv[i3t];
+    double u1ldg = CS(ipn, edg, 0) * cu[0] + SN(ipn, edg, 0) * cv[0];
+    double v1ldg = -SN(ipn, edg, 0) * cu[0] + CS(ipn, edg, 0) * cv[0];
+    double u2ldg = CS(ipn, edg, 1) * cu[i2] + SN(ipn, edg, 1) * cv[i2];
+    double v2ldg = -SN(ipn, edg, 1) * cu[i2] + CS(ipn, edg, 1) * cv[i2];
+    double u3ldg = CS(ipn, edg, 3) * cu[i3l] + SN(ipn, edg, 3) * cv[i3l];
+    double v3ldg = -SN(ipn, edg, 3) * cu[i3l] + CS(ipn, edg, 3) * cv[i3l];
     double ue = (2. * (u1ldg + u2ldg) + u3trl + u3ldg) * divby6;
     double ve = (2. * (v1ldg + v2ldg) + v3trl + v3ldg) * divby6;
     double m_1ldg = sqrt(u1ldg * u1ldg + v1ldg * v1ldg);
@@ -76,15 +87,21 @@ __global__ void __launch_bounds__(256) k_edgvar1(DevGrid g, DevState s) {
     E3(s.u_edg, ipn, edg, k) = ue;
     E3(s.v_edg, ipn, edg, k) = ve;
     E3(s.lp_edg, ipn, edg, k) =
-        (L2(s.pr, i1, k) + L2(s.pr, i1, k + 1) + L2(s.pr, i2, k) + L2(s.pr, i2, k + 1) +
-         .5 * (L2(s.pr, i3t, k) + L2(s.pr, i3t, k + 1) + L2(s.pr, i3l, k) + L2(s.pr, i3l, k + 1))) * divby6;
-    E3(s.mont_edg, ipn, edg, k) =
-        (2. * (C2(s.mp, i1, k) + C2(s.mp, i2, k)) + C2(s.mp, i3t, k) + C2(s.mp, i3l, k)) * divby6;
+        (cp0[0] + cp1[0] + cp0[i2] + cp1[i2] + .5 * (cp0[i3t] + cp1[i3t] + cp0[i3l] + cp1[i3l])) * divby6;
+    E3(s.mont_edg, ipn, edg, k) = (2. * (cm[0] + cm[i2]) + cm[i3t] + cm[i3l]) * divby6;
     E3(s.uvsq_edg, ipn, edg, k) = .5 * (ue * ue + ve * ve);
-    E3(s.trc_edg, ipn, edg, k) = (2. * (TR(s.tr, 0, i1, k) + TR(s.tr, 0, i2, k)) + TR(s.tr, 0, i3t, k) +
-                                  TR(s.tr, 0, i3l, k)) * divby6;
+    E3(s.trc_edg, ipn, edg, k) = (2. * (ct[0] + ct[i2]) + ct[i3t] + ct[i3l]) * divby6;
   }
-  if (np < NPP) E3(s.trc_edg, ipn, NPP - 1, k) = 0.;
+  if (NP < NPP) E3(s.trc_edg, ipn, NPP - 1, k) = 0.;
+}
+
+__global__ void __launch_bounds__(256) k_edgvar1(DevGrid g, DevState s) {
+  const int nip = g.nip, nvl = g.nvl;
+  const int k = threadIdx.x;
+  const int ipn = wave_col();
+  if (ipn >= nip || k >= nvl) return;
+  if (GNP(ipn) == 6) edgvar1_col<6>(g, s, ipn, k);
+  else edgvar1_col<5>(g, s, ipn, k);
 }
 
 // ---------------------------------------------------------------------------
