@@ -106,13 +106,13 @@ __global__ void __launch_bounds__(256) k_edgvar1(DevGrid g, DevState s) {
 
 // ---------------------------------------------------------------------------
 // edgvar3.F90:65-163: LS reconstruction of each edge (edgvalEval) and the
-// two-sided average of loop 2.  TWO_SIDED (the drop-in entry): fused, each
-// cell-level recomputes its neighbour's reconstruction of the shared edge
-// (2-ring gathers).  Otherwise (the device-resident step) each cell stores
-// only its own reconstructions and the readers (cnuity12, massadv1,
-// trcadv1) form 0.5*(own + neighbour's) themselves, the same operands in the
-// same order, so every result keeps its bits.
-template <bool TWO_SIDED>
+// two-sided average of loop 2.  k_edgvar3 (the drop-in entry) fuses them:
+// each cell-level recomputes its neighbour's reconstruction of the shared
+// edge (2-ring gathers).  k_edgvar3_own (the device-resident step) stores
+// only each cell's own reconstructions, every field of the cell in one
+// thread, and the readers (cnuity12, massadv1, trcadv1) form
+// 0.5*(own + neighbour's) themselves, the same operands in the same order,
+// so every result keeps its bits.
 __global__ void __launch_bounds__(256) k_edgvar3(DevGrid g, DevState s) {
   const int nip = g.nip, nvl = g.nvl;
   const int k = threadIdx.x;
@@ -129,10 +129,6 @@ __global__ void __launch_bounds__(256) k_edgvar3(DevGrid g, DevState s) {
     // this cell's reconstruction of edge i (edgvalEval)
     double own = b0;
     for (int j = 0; j < np; j++) own = own + (nbv[j] - b0) * HF(ipn, i, j);
-    if (!TWO_SIDED) {
-      E3(out, ipn, i, k) = own;
-      continue;
-    }
     // the neighbour's reconstruction of the same edge
     const int nb = PROX(ipn, i), eo = PROXS(ipn, i), nn = GNP(nb);
     const double c0 = nbv[i];
@@ -141,6 +137,39 @@ __global__ void __launch_bounds__(256) k_edgvar3(DevGrid g, DevState s) {
     E3(out, ipn, i, k) = 0.5 * (own + oth);
   }
   for (int i = np; i < NPP; i++) E3(out, ipn, i, k) = 0.;
+}
+
+template <int NP>
+__device__ __forceinline__ void edgvar3_own_col(const DevGrid &g, const DevState &s, int ipn, int k) {
+  const int nip = g.nip, nvl = g.nvl;
+  int nb[NP];
+#pragma unroll
+  for (int j = 0; j < NP; j++) nb[j] = PROX(ipn, j);
+  for (int f = 0; f < g.ntra; f++) {   // 0: dp, f >= 1: tracer f
+    const double *fld = f == 0 ? s.dp : s.tr + (size_t)f * nip * nvl;
+    double *out = f == 0 ? s.dp_edg : s.trc_edg + (size_t)f * nip * NPP * nvl;
+    const double b0 = C2(fld, ipn, k);
+    double nbv[NP];
+#pragma unroll
+    for (int j = 0; j < NP; j++) nbv[j] = C2(fld, nb[j], k);
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      double own = b0;
+#pragma unroll
+      for (int j = 0; j < NP; j++) own = own + (nbv[j] - b0) * HF(ipn, i, j);
+      E3(out, ipn, i, k) = own;
+    }
+    if (NP < NPP) E3(out, ipn, NPP - 1, k) = 0.;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_edgvar3_own(DevGrid g, DevState s) {
+  const int nip = g.nip, nvl = g.nvl;
+  const int k = threadIdx.x;
+  const int ipn = wave_col();
+  if (ipn >= nip || k >= nvl) return;
+  if (GNP(ipn) == 6) edgvar3_own_col<6>(g, s, ipn, k);
+  else edgvar3_own_col<5>(g, s, ipn, k);
 }
 
 // an edge value of edgvar3's fields as the readers see it: stored averaged
@@ -663,8 +692,8 @@ void launch_edgvar1(const DevGrid &g, const DevState &s, hipStream_t st) {
 }
 void launch_edgvar3(const DevGrid &g, const DevState &s, hipStream_t st, bool two_sided) {
   dim3 grid((g.nip + COLS - 1) / COLS, g.ntra);
-  if (two_sided) KT_LAUNCH("k_edgvar3", st, k_edgvar3<true><<<grid, col_block(), 0, st>>>(g, s));
-  else KT_LAUNCH("k_edgvar3", st, k_edgvar3<false><<<grid, col_block(), 0, st>>>(g, s));
+  if (two_sided) KT_LAUNCH("k_edgvar3", st, k_edgvar3<<<grid, col_block(), 0, st>>>(g, s));
+  else KT_LAUNCH("k_edgvar3", st, k_edgvar3_own<<<col_grid(g.nip), col_block(), 0, st>>>(g, s));
 }
 void launch_cnuity(const DevGrid &g, const DevState &s, const AB &ab, hipStream_t st, bool two_sided) {
   if (two_sided) KT_LAUNCH("k_cnuity12", st, k_cnuity12<true><<<col_grid(g.nip), col_block(), 0, st>>>(g, s, ab));
