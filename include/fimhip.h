@@ -169,6 +169,11 @@ enum {
   FIMHIP_TRB, FIMHIP_CUMUFX, FIMHIP_DPINIT,   /* class-B transport (after fimhip_set_class_b) */
   FIMHIP_NFIELDS
 };
+/* The edge fields are intermediates.  After the per-routine entries they hold
+   what the reference's arrays hold.  After fimhip_step, DP_EDG and TRC_EDG
+   (tracers 2..ntra) hold each cell's own reconstructions (the step's readers
+   form the two-sided averages), except that DP_EDG holds mass_adv's final
+   edge depths under flux_conserv_schm = 2. */
 /* number of doubles of a field */
 long long fimhip_field_size(const fimhip_ctx *ctx, int field);
 int fimhip_upload(fimhip_ctx *ctx, int field, const double *host);
