@@ -111,6 +111,20 @@ def test_device_resident_steps_match_checker_bitwise(case, mode, nsteps):
         for f in fields:
             got = m.download(f)
             assert _same(got, o.a[f]), (f, rel_l2(got, o.a[f]))
+        # after a step the thickness and tracer 2..ntra edge fields hold each
+        # cell's own reconstructions; their two-sided averages, formed here
+        # with the readers' operation, are the reference's edge arrays
+        nip = g.nprox.shape[0]
+        live = np.arange(6)[None, :] < g.nprox[:, None]
+        P = np.where(live, g.prox - 1, 0)
+        S = np.where(live, g.proxs - 1, 0)
+        own = m.download("dp_edg").reshape(nip, 6, NVL)
+        assert _same((own + own[P, S]) * 0.5 * live[:, :, None], o.a["dp_edg"].reshape(nip, 6, NVL) * live[:, :, None])
+        tre = m.download("trc_edg").reshape(NTRA, nip, 6, NVL)
+        ote = o.a["trc_edg"].reshape(NTRA, nip, 6, NVL)
+        assert _same(tre[0], ote[0])
+        for t in range(1, NTRA):
+            assert _same((tre[t] + tre[t][P, S]) * 0.5 * live[:, :, None], ote[t] * live[:, :, None]), t
         gm, gt = m.globsum()
         om, ot = o.globsum()
         assert abs(gm - om) <= 1e-13 * abs(om)
