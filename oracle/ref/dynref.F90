@@ -40,6 +40,7 @@ program dynref
   use module_hybgen,    only: hybgen
   use module_hystat,    only: hystat
   use module_transp3d,  only: transp0, transp1, transp2
+  use module_globsum,   only: globsum, qmass
   implicit none
 
   character(len=512) :: fgrid, fstate, fout
@@ -61,6 +62,7 @@ program dynref
   real, allocatable :: u_edg(:,:,:), v_edg(:,:,:), dp_edg(:,:,:), lp_edg(:,:,:)
   real, allocatable :: geop_edg(:,:,:), mont_edg(:,:,:), uvsq_edg(:,:,:), trc_edg(:,:,:,:)
   real, allocatable :: cumufx(:,:,:), dpinit(:,:)
+  real, allocatable :: rn2d(:), rc2d(:), qf2d(:), qtrcr(:)
 
   call get_command_argument(1, fgrid)
   call get_command_argument(2, fstate)
@@ -202,6 +204,8 @@ program dynref
   allocate (geop_edg(nvl,npp,nip), mont_edg(nvl,npp,nip), uvsq_edg(nvl,npp,nip), trc_edg(nvl,npp,nip,ntra))
   allocate (worka(nvl,nip), workb(nvl,nip))
   allocate (q_min_out(nvl,nip), q_max_out(nvl,nip))   ! dyn_alloc.F90:284-302
+  allocate (rn2d(nip), rc2d(nip), qf2d(nip), qtrcr(ntra+ntrb))   ! physics inputs of globsum: none here
+  rn2d = 0.; rc2d = 0.; qf2d = 0.; qtrcr = 0.
   q_min_out = 0.
   q_max_out = 0.
   allocate (r8a(nvl,nip))
@@ -250,6 +254,7 @@ program dynref
   end if
   call hystat (its, ph3d, ex3d, mp3d, dp3d, tr3d, trdp, psrf, ptdcy)
   call dumpstate('init')
+  call dumpsums('init')
   nf = 0; of = 2; vof = 1
   if (ntrb > 0) call transp0 (its, cumufx, dp3d, dpinit)
 
@@ -318,6 +323,7 @@ program dynref
     end if
   end do
   call dumpstate('final')
+  call dumpsums('final')
   call out1('final.psrf', real(psrf,8))
   do t = 1, ntra
     call out2('final.trdp', real(trdp(:,:,t),8))
@@ -381,6 +387,13 @@ contains
       call out2(stage//'.tr', real(tr3d(:,:,tt),8))
     end do
   end subroutine dumpstate
+
+  subroutine dumpsums(stage)     ! global mass and tracer totals (globsum.F90:39-130)
+    character(*), intent(in) :: stage
+    call globsum (its, dp3d, tr3d, rn2d, rc2d, pr3d, ex3d, qf2d, qtrcr)
+    call out1(stage//'.qmass', [real(qmass,8)])
+    call out1(stage//'.qtrcr', real(qtrcr,8))
+  end subroutine dumpsums
 
   subroutine dumpedges(stage)
     character(*), intent(in) :: stage

@@ -15,10 +15,18 @@ Fixtures (data only):
                         mass_adv (flux_conserv_schm=2), and class-B tracers
                         (ntrb=2, dtratio=2, fields class_b_fields() of the
                         input) with cnuity and with mass_adv
+  dyn_96_ref.json       the north star's FP64 bar: SHA-256 of the reference FP64
+                        build's final state after 96 full steps at dt = 240 s
+                        on the G3 grid (start-up hybgen, seeded state), and the
+                        relative L2 of the checker with the device pow
+                        (fim_pow) against the reference after 96 steps at G3
+                        and G4, as measured when the fixture was made
   dyn_G2_ref.json       SHA-256 of every output record of the reference FP64
                         build (dynref_r8) for 3 full steps (mode 0) and 3
-                        transport steps (mode 1), stage dumps of step 1, plus
-                        a few sample values for diagnostics
+                        transport steps (mode 1), stage dumps of step 1, the
+                        reference globsum totals (qmass, qtrcr) at start-up
+                        and after the last step, plus a few sample values for
+                        diagnostics
 """
 import hashlib
 import json
@@ -50,8 +58,13 @@ def main():
     shutil.copyfile(refrun.ref_file(2, "icos_grid_info_level.dat"), os.path.join(HERE, "icos_grid_info_G2.dat"))
     g = glvl.read_glvl(os.path.join(HERE, "glvl_G2.dat"))
     th, ds = state.vertical_tables(64)
-    st = state.make_state(g.lat, g.lon, 64, 4, wind_noise=True)
-    np.savez_compressed(os.path.join(HERE, "dyn_G2_input.npz"), thetac=th, dpsig=ds, dt=DT, **st)
+    inp = os.path.join(HERE, "dyn_G2_input.npz")
+    if os.path.exists(inp):      # the committed input is the fixture: digests are always made from it
+        z = np.load(inp)
+        st = {k: z[k] for k in ("us", "vs", "dp", "pr", "ex", "ph", "tr")}
+    else:
+        st = state.make_state(g.lat, g.lon, 64, 4, wind_noise=True)
+        np.savez_compressed(inp, thetac=th, dpsig=ds, dt=DT, **st)
     out = {"dt": DT, "nvl": 64, "ntra": 4, "nsteps": 3, "runs": {}}
     for mode in (0, 1):
         r = refrun.run_dynref(refrun.ref_glvl(2), st, th, ds, nsteps=3, dt=DT, mode=mode, dumpstep=1)
@@ -97,6 +110,46 @@ def schemes():
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def fp64_bar_state(g):
+    """The seeded state of the 96-step runs: the synthetic generator's
+    member 0 with wind noise (seed 20240601)."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    return fimhip.synth_state(g.lat, g.lon, 64, 4, seed=20240601, member=0, flags=fimhip.SYNTH_WIND_NOISE)
+
+
+FP64_FIELDS = ("us", "vs", "dp", "pr", "ex", "ph", "mp")
+
+
+def fp64_bar():
+    from oracle import oracle as O
+    th, ds = state.vertical_tables(64)
+    out = {"dt": DT, "nsteps": 96, "nvl": 64, "ntra": 4, "rel_l2_fim_pow_vs_reference": {}}
+    for lv in (3, 4):
+        path = refrun.ref_glvl(lv)
+        g = glvl.read_glvl(path)
+        st = fp64_bar_state(g)
+        r = refrun.run_dynref(path, st, th, ds, nsteps=96, dt=DT, mode=0)
+        o = O.OracleRun(O.grid_arrays(g, 64, DT, th, ds), 64, 4, DT, pow_kind=1)
+        o.load(st)
+        o.init(True)
+        for its in range(1, 97):
+            o.step(its, 0)
+        err = {}
+        for f in FP64_FIELDS:
+            d = o.a[f].ravel() - r["final." + f]
+            err[f] = float(np.linalg.norm(d) / np.linalg.norm(r["final." + f]))
+        for t in range(4):
+            d = o.a["tr"][t].ravel() - r["final.tr"][t]
+            err["tr%d" % (t + 1)] = float(np.linalg.norm(d) / np.linalg.norm(r["final.tr"][t]))
+        out["rel_l2_fim_pow_vs_reference"]["G%d" % lv] = err
+        if lv == 3:
+            out["G3_final_sha256"] = {k: [sha(x) for x in (v if isinstance(v, list) else [v])]
+                                      for k, v in r.items() if k.startswith("final.")}
+    with open(os.path.join(HERE, "dyn_96_ref.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     main()
     schemes()
+    fp64_bar()

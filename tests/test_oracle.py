@@ -30,6 +30,18 @@ def _check_state(o, rec, stage, fields=("us", "vs", "dp", "pr", "ex", "ph", "mp"
     return bad
 
 
+def _check_sums(o, rec, stage):
+    """The checker's globsum (mass and tracer totals) against the reference
+    globsum.F90 records of the same stage, byte for byte."""
+    m, t = o.globsum()
+    bad = []
+    if sha(np.array([m])) != rec[stage + ".qmass"]["sha256"][0]:
+        bad.append(stage + ".qmass")
+    if sha(t) != rec[stage + ".qtrcr"]["sha256"][0]:
+        bad.append(stage + ".qtrcr")
+    return bad
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_oracle_bitexact_vs_reference_golden(oracle_lib, mode):
     """Every state field after every sub-call of step 1, and after 3 steps,
@@ -45,6 +57,7 @@ def test_oracle_bitexact_vs_reference_golden(oracle_lib, mode):
     assert sha(o.garr["smoo_coeff"]) == ref["vert.smoo"]["sha256"][0]
     o.init(True)
     assert _check_state(o, ref, "init") == []
+    assert _check_sums(o, ref, "init") == []
     o.abstart(1)
     o.call("edgvar1")
     o.call("edgvar3")
@@ -70,6 +83,7 @@ def test_oracle_bitexact_vs_reference_golden(oracle_lib, mode):
     for its in (2, 3):
         o.step(its, mode)
     assert _check_state(o, ref, "final") == []
+    assert _check_sums(o, ref, "final") == []
 
 
 def test_oracle_bitexact_vs_live_reference(oracle_lib):
@@ -88,6 +102,8 @@ def test_oracle_bitexact_vs_live_reference(oracle_lib):
         assert np.array_equal(o.a[f].ravel(), r["final." + f]), f
     for t in range(4):
         assert np.array_equal(o.a["tr"][t].ravel(), r["final.tr"][t])
+    m, tt = o.globsum()
+    assert np.array_equal(np.array([m]), r["final.qmass"]) and np.array_equal(tt, r["final.qtrcr"])
 
 
 def test_live_reference_g5_transport_dt480_bitwise(oracle_lib):
@@ -254,12 +270,7 @@ def test_schemes_bitexact_vs_reference_golden(oracle_lib, run):
     o.init(mode == 0)
     for its in range(1, nsteps + 1):
         o.step(its, mode)
-    nf = o.slots[0] - 1
-    got = {"final.us": [sha(o.a["us"])], "final.vs": [sha(o.a["vs"])], "final.dp": [sha(o.a["dp"])],
-           "final.pr": [sha(o.a["pr"])], "final.ex": [sha(o.a["ex"])], "final.ph": [sha(o.a["ph"])],
-           "final.mp": [sha(o.a["mp"])], "final.psrf": [sha(o.a["psrf"])],
-           "final.tr": [sha(o.a["tr"][t]) for t in range(4)], "final.trdp": [sha(o.a["trdp"][t]) for t in range(4)],
-           "final.dptdcy": [sha(o.a["dp_tdcy"][nf])], "final.utdcy": [sha(o.a["u_tdcy"][nf])]}
+    got = _final_digests(o)
     if cb:
         got["final.trb"] = [sha(o.a["trb"][t]) for t in range(2)]
         got["final.cumufx"] = [sha(o.a["cumufx"])]
@@ -267,6 +278,100 @@ def test_schemes_bitexact_vs_reference_golden(oracle_lib, run):
     assert sorted(got) == sorted(ref)
     bad = [k for k in ref if got[k] != ref[k]]
     assert bad == [], bad
+
+
+def _final_digests(o):
+    """SHA-256 of the records dynref writes after its last step."""
+    nf = o.slots[0] - 1
+    got = {"final.us": [sha(o.a["us"])], "final.vs": [sha(o.a["vs"])], "final.dp": [sha(o.a["dp"])],
+           "final.pr": [sha(o.a["pr"])], "final.ex": [sha(o.a["ex"])], "final.ph": [sha(o.a["ph"])],
+           "final.mp": [sha(o.a["mp"])], "final.psrf": [sha(o.a["psrf"])],
+           "final.tr": [sha(o.a["tr"][t]) for t in range(4)], "final.trdp": [sha(o.a["trdp"][t]) for t in range(4)],
+           "final.dptdcy": [sha(o.a["dp_tdcy"][nf])], "final.utdcy": [sha(o.a["u_tdcy"][nf])]}
+    m, tt = o.globsum()
+    got["final.qmass"] = [sha(np.array([m]))]
+    got["final.qtrcr"] = [sha(tt)]
+    return got
+
+
+FP64_BAR = 1e-12      # relative L2 after 96 full steps, every prognostic field
+
+
+def _bar_errors(a, b):
+    """Relative L2 of every prognostic field of run a against run b (arrays
+    or dynref records)."""
+    def get(r, f, t=None):
+        if isinstance(r, dict) and ("final." + f) in r:
+            v = r["final." + f]
+            return (v[t] if t is not None else v).ravel()
+        v = r.a[f]
+        return (v[t] if t is not None else v).ravel()
+    err = {f: rel_l2(get(a, f), get(b, f)) for f in ("us", "vs", "dp", "pr", "ex", "ph", "mp")}
+    for t in range(4):
+        err["tr%d" % (t + 1)] = rel_l2(get(a, "tr", t), get(b, "tr", t))
+    return err
+
+
+def test_fp64_bar_96_steps_golden(oracle_lib):
+    """The north star's FP64 bar without the reference present: on the G3
+    grid, 96 full steps at dt = 240 s from the seeded state, the checker with
+    the C library pow has the exact bytes of the reference FP64 build's final
+    state (dyn_96_ref.json), and the checker with the device pow (fim_pow,
+    which the GPU equals bit for bit) is within 1e-12 relative L2 of it in
+    every prognostic field."""
+    import json
+    import os
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    with open(os.path.join(gdir, "dyn_96_ref.json")) as f:
+        gold = json.load(f)
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+    O = oracle_lib
+    g = golden_grid(3)
+    th, ds = state.vertical_tables(64)
+    st = fimhip.synth_state(g.lat, g.lon, 64, 4, seed=20240601, member=0, flags=fimhip.SYNTH_WIND_NOISE)
+    runs = []
+    for pk in (0, 1):
+        o = _oracle_run(O, g, st, th, ds, gold["dt"], pow_kind=pk)
+        o.init(True)
+        for its in range(1, gold["nsteps"] + 1):
+            o.step(its, 0)
+        runs.append(o)
+    got = _final_digests(runs[0])
+    ref = gold["G3_final_sha256"]
+    assert sorted(got) == sorted(ref)
+    assert [k for k in ref if got[k] != ref[k]] == []
+    err = _bar_errors(runs[1], runs[0])
+    assert max(err.values()) < FP64_BAR, err
+    # the drift is real (not a no-op pow switch) and matches the fixture's record
+    assert max(err.values()) > 1e-15
+    for k, v in gold["rel_l2_fim_pow_vs_reference"]["G3"].items():
+        assert err[k] == pytest.approx(v, rel=1e-6, abs=1e-18), k
+
+
+def test_fp64_bar_96_steps_vs_live_reference_g4(oracle_lib):
+    """Where the reference build exists: G4 x 64 layers, 96 full steps at
+    dt = 240 s, the checker with the device pow against the reference FP64
+    build itself: every prognostic field within 1e-12 relative L2, global
+    mass and tracer totals within 1e-13."""
+    from oracle import refrun
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, glvl, state
+    if not refrun.available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    O = oracle_lib
+    path = refrun.ref_glvl(4)
+    g = glvl.read_glvl(path)
+    th, ds = state.vertical_tables(64)
+    st = fimhip.synth_state(g.lat, g.lon, 64, 4, seed=20240601, member=0, flags=fimhip.SYNTH_WIND_NOISE)
+    r = refrun.run_dynref(path, st, th, ds, nsteps=96, dt=240.0, mode=0)
+    o = _oracle_run(O, g, st, th, ds, 240.0, pow_kind=1)
+    o.init(True)
+    for its in range(1, 97):
+        o.step(its, 0)
+    err = _bar_errors(o, r)
+    assert max(err.values()) < FP64_BAR, err
+    m, tt = o.globsum()
+    assert abs(m - r["final.qmass"][0]) / m < 1e-13
+    assert np.all(np.abs(tt - r["final.qtrcr"]) / tt < 1e-13)
 
 
 def test_project_pow_drift_within_bound(oracle_lib):
