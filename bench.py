@@ -17,13 +17,14 @@ FP64 after 96 steps and the global mass drift, and the C restatement timed on
 the host's cores.
 
 Prints ONE JSON line.  Besides the contract fields it carries
-  roofline:     the headline form's dominant kernel (most device time, timed
-                with HIP events on the library's stream): algorithmic bytes
-                per launch / mean launch duration against 8 TB/s
-  step_hbm:     the whole step against the compulsory-byte model (18 + 8T
-                values per cell-level-step; 220 B in the mixed form, 400 B
-                in FP64)
-  plain_fp64:   the FP64 path's throughput, roofline and step_hbm; under
+  roofline:     the headline form's whole step against the compulsory-byte
+                model (18 + 8T values per cell-level-step: 220 B in the
+                mixed form, 400 B in FP64) at 8 TB/s; traffic = the step's
+                PMC-measured HBM bytes (committed profile) where available
+  dominant_kernel: the kernel with the most device time (HIP events on the
+                library's stream): its own input/output bytes per launch over
+                its mean launch duration
+  plain_fp64:   the FP64 path's throughput, roofline and dominant kernel; under
                 mass_adv the same step with the flux-limited mass transport
                 (flux_conserv_schm = 2) and its two kernels; under class_b the
                 step with 8 class-B tracers moved every 6 steps by fct3d
@@ -115,6 +116,15 @@ def ens_kernel_bytes(T):
     }
 
 
+def ens_step_bytes(T, mode):
+    """Compulsory bytes per cell-level-step of the mixed form: the
+    18 + 8T values (10 + 8T transport-only) at 4 B, except the arrays kept in
+    FP64: the thickness (read and written) and its AB history (read twice,
+    written once), 5 values, +4 B each."""
+    v = (18 + 8 * T) if mode == 0 else (10 + 8 * T)
+    return 4 * v + 4 * 5
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,7 +143,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="skip the mass_adv and class-B legs")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (for PMC byte calibration)")
     return ap.parse_args()
@@ -157,16 +167,40 @@ def pmc_traffic(kernel, form, a):
     return None
 
 
-def roofline(prof, kbytes, cells, traffic_of=None):
-    """The kernel with the most device time: algorithmic bytes per launch over
-    its mean launch time; traffic is the measured HBM bytes per launch."""
+def dominant_kernel(prof, kbytes, cells, traffic_of=None):
+    """The kernel with the most device time: its own input/output bytes per
+    launch (bench's per-kernel model, intermediates included) over its mean
+    launch time; traffic is its measured HBM bytes per launch."""
     dname, (dn, dms) = max(prof.items(), key=lambda kv: kv[1][1])
     dmean = dms / dn * 1e-3
     dbytes = kbytes.get(dname, 0) * cells
-    return {"kernel": dname, "bound": "hbm", "achieved": dbytes / dmean / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": dbytes / dmean / HBM_PEAK,
+    return {"kernel": dname, "io_GBs": dbytes / dmean / 1e9, "io_frac": dbytes / dmean / HBM_PEAK,
             "traffic": traffic_of(dname) if traffic_of else None,
             "bytes_per_launch": dbytes, "mean_launch_ms": dmean * 1e3}
+
+
+def step_roofline(value, b_cell, cells, ms_per_step, prof, steps_profiled, traffic_of=None):
+    """The whole step against the compulsory-byte model: achieved =
+    cell-level-steps/s x compulsory bytes per cell-level-step (18 + 8T values:
+    the state read and written once, the Adams-Bashforth histories read twice
+    and written once), frac = achieved / 8 TB/s.  traffic = the measured HBM
+    bytes of one step, the sum over the step's kernels of their PMC-measured
+    bytes per launch times launches per step (None unless every kernel of the
+    step has a committed measurement for this workload)."""
+    traffic = None
+    if traffic_of is not None and prof:
+        tot = 0.
+        for k, (n, _) in prof.items():
+            t = traffic_of(k)
+            if t is None:
+                tot = None
+                break
+            tot += t * n / steps_profiled
+        traffic = tot
+    achieved = value * b_cell
+    return {"scope": "step", "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": traffic, "compulsory_bytes_per_cell_level": b_cell,
+            "compulsory_bytes_per_step": b_cell * cells, "ms_per_step": ms_per_step}
 
 
 def kernel_table(prof, kbytes, cells):
@@ -226,10 +260,10 @@ def run_plain(a, grid, th, ds, fimhip):
     kb = plain_kernel_bytes(a.ntra)
     v_step = (18 + 8 * a.ntra) if a.mode == 0 else (10 + 8 * a.ntra)
     value = cells * n_timed / wall
+    tr_of = (lambda k: pmc_traffic(k, "plain", a)) if a.mode == 0 else None
     res = {"value": value, "ms_per_step": wall / n_timed * 1e3, "members": 1, "timed_steps": n_timed,
-           "roofline": roofline(prof, kb, cells, lambda k: pmc_traffic(k, "plain", a) if a.mode == 0 else None),
-           "step_hbm": {"bytes_per_cell_level": 8 * v_step, "achieved_GBs": value * 8 * v_step / 1e9,
-                        "frac": value * 8 * v_step / HBM_PEAK},
+           "roofline": step_roofline(value, 8 * v_step, cells, wall / n_timed * 1e3, prof, a.steps, tr_of),
+           "dominant_kernel": dominant_kernel(prof, kb, cells, tr_of),
            "kernels": kernel_table(prof, kb, cells)}
     if not a.headline_only:
         res["mass_adv"] = run_plain_mass_adv(a, m, cells, kb, fimhip)
@@ -347,11 +381,11 @@ def run_ens(a, grid, th, ds, fimhip):
     cells = M * grid.nip * a.nvl
     kb = ens_kernel_bytes(a.ntra)
     value = cells * a.steps / wall
-    b_step = 220 if a.mode == 0 else 156
+    b_step = ens_step_bytes(a.ntra, a.mode)
+    tr_of = (lambda k: pmc_traffic(k, "ens16", a)) if a.mode == 0 else None
     res = {"value": value, "ms_per_step": wall / a.steps * 1e3, "members": M,
-           "roofline": roofline(prof, kb, cells, lambda k: pmc_traffic(k, "ens16", a) if a.mode == 0 else None),
-           "step_hbm": {"bytes_per_cell_level": b_step, "achieved_GBs": value * b_step / 1e9,
-                        "frac": value * b_step / HBM_PEAK},
+           "roofline": step_roofline(value, b_step, cells, wall / a.steps * 1e3, prof, a.steps, tr_of),
+           "dominant_kernel": dominant_kernel(prof, kb, cells, tr_of),
            "kernels": kernel_table(prof, kb, cells), "tiles": tune}
     if not a.headline_only:
         res["mass_adv"] = run_ens_mass_adv(a, e, cells, kb, fimhip)
@@ -504,7 +538,9 @@ def cpu_baseline(m, grid, a, th, ds, fimhip):
     """Time the C restatement on the host: one member of the same grid,
     started from the device's post-start-up state, a few full steps."""
     from oracle import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    # every host thread this process may use: the job's share where the
+    # environment sets one (OMP_NUM_THREADS), else every logical CPU
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
     o = O.OracleRun(O.grid_arrays(grid, a.nvl, a.dt, th, ds), a.nvl, a.ntra, a.dt, pow_kind=0)
     m.synth(seed=SEED, member=0, flags=fimhip.SYNTH_WIND_NOISE | fimhip.SYNTH_PERTURB)
@@ -517,13 +553,46 @@ def cpu_baseline(m, grid, a, th, ds, fimhip):
         o.step(its, a.mode)
     wall = time.perf_counter() - t0
     cells = grid.nip * a.nvl
-    out = {"value": cells * a.cpu_steps / wall, "unit": "cell-level-steps/s", "cores": threads,
+    value = cells * a.cpu_steps / wall
+    ratio = fortran_ratio()
+    out = {"value": value, "unit": "cell-level-steps/s", "cores": threads,
            "kind": "port",
-           "sample": "1 member of G%d x %dL, %d tracers, %d %s FP64 steps (C restatement, OpenMP %d threads)"
-                     % (a.glvl, a.nvl, a.ntra, a.cpu_steps, "full" if a.mode == 0 else "transport", threads)}
+           "sample": "1 member of G%d x %dL, %d tracers, %d %s FP64 steps (C restatement, OpenMP %d threads), "
+                     "%.1f s" % (a.glvl, a.nvl, a.ntra, a.cpu_steps, "full" if a.mode == 0 else "transport",
+                                 threads, wall),
+           "host": {"cpu_model": cpu_model(), "logical_cpus": os.cpu_count(), "threads_used": threads},
+           "fortran_ratio": ratio,
+           "fortran_equivalent": (value / ratio) if ratio else None,
+           "fortran_note": "the reference Fortran (-O2 -fopenmp -fdefault-real-8) runs the same steps at "
+                           "1/fortran_ratio times the restatement's rate (profiles/cpu_calibration_G5x64.md, "
+                           "8 threads, G5 x 64)"}
     if not a.headline_only:
         out["schemes"] = cpu_schemes(o, m, grid, a, th, ds, fimhip, O, threads)
     return out
+
+
+def cpu_model():
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:      # lscpu absent: the model is then unknown
+        pass
+    return "unknown"
+
+
+def fortran_ratio():
+    """restatement / reference-Fortran throughput at equal threads, from the
+    committed calibration (profiles/cpu_calibration_G5x64.md)."""
+    p = os.path.join(ROOT, "profiles", "cpu_calibration_G5x64.md")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("restatement / reference (FP64,") and "thread)" not in line:
+            return float(line.split("**")[1])
+    return None
 
 
 def cpu_schemes(o, m, grid, a, th, ds, fimhip, O, threads, ntrb=8):
@@ -632,7 +701,7 @@ def main():
                    "members": head["members"], "form": form, "dt": a.dt,
                    "dt_note": "the reference's own time step for this grid (module_control.F90:214-220); at G7 "
                               "a 240 s step overflows the FP64 state by step 6, so it is not timed"},
-        "roofline": head["roofline"], "step_hbm": head["step_hbm"],
+        "roofline": head["roofline"], "dominant_kernel": head["dominant_kernel"],
         "speedup_vs_plain_fp64": (ens["value"] / plain["value"]) if (ens and plain) else None,
         "plain_fp64": plain, "ensemble": ens, "uneven_batch": uneven, "accuracy_96": acc, "cpu_baseline": cpu,
     }

@@ -7,6 +7,7 @@
 #                                          as PREP_FFLAGS in macros.make.linuxpcgnu:25)
 #   oracle/_ref/bin/dynref_r8             reference dynamics hot path, -fdefault-real-8
 #   oracle/_ref/bin/dynref_r4             reference dynamics hot path, default real 4
+#   oracle/_ref/bin/dynref_r8omp          the r8 build with -fopenmp (timing calibration)
 #   oracle/_ref/bin/restref_r4            reference restart writer/reader of the
 #                                          dynamics section, default real 4
 #
@@ -72,14 +73,22 @@ DYN_SRC="$CNTL_SRC $S/cntl/module_constants.F90 $S/cntl/module_variables.F90
   $H/flux_limiter.F90 $H/massadv_compute.F90 $H/massadv.F90 $H/fct3d.F90 $H/transp3d.F90
   $H/edgvar1.F90 $H/edgvar3.F90 $H/momtum_compute.F90 $H/momtum.F90 $H/hybgen.F90
   $H/hystat.F90 $H/abstart.F90 $H/globsum.F90 $H/GetIpnGlobal.F90 $S/prep/ss2icos/lay2lay.F90 $S/prep/ss2icos/lin2stp.F90"
-for prec in r8 r4; do
-  if [ $prec = r8 ]; then PF="-fdefault-real-8 -fdefault-double-8"; else PF=""; fi
+# r8omp: the FP64 build with the reference's own OpenMP directives on
+# (-fopenmp), the CPU path the restatement's timing is calibrated against
+for prec in r8 r4 r8omp; do
+  case $prec in
+    r8) PF="-fdefault-real-8 -fdefault-double-8" ;;
+    r4) PF="" ;;
+    r8omp) PF="-fdefault-real-8 -fdefault-double-8 -fopenmp" ;;
+  esac
   OD=$OUT/obj_$prec
   FL="-cpp -DSERIAL -O2 -ffp-contract=off $PF -I$H -I$S/dummygptl"
   compile "$OD" "$FL" $DYN_SRC "$HERE/ref/dynref.F90"
   objs=""
   for f in $DYN_SRC "$HERE/ref/dynref.F90"; do b=$(basename "${f%.*}"); objs="$objs $OD/$b.o"; done
-  $FC -o "$OUT/bin/dynref_$prec" $objs
+  LF=""
+  [ $prec = r8omp ] && LF="-fopenmp"
+  $FC $LF -o "$OUT/bin/dynref_$prec" $objs
 done
 
 # ---- restart writer/reader of the dynamics section (default real 4) -------
