@@ -96,7 +96,7 @@ def ens_kernel_bytes(T):
     FP64 value; inputs shared by the tracers of one launch counted once)."""
     return {
         "e_edgvar1": 20 + 36,                # us vs tr1 (f32) pr (f64) -> normal wind edges, pressure gradient, step-start theta (f32)
-        "e_edgvar3": 32 + 28 * (T - 1),      # dp (f64) -> dp_edg; tracers 2..T -> trc_edg
+        "e_edgvar3": 32,                     # dp (f64) -> dp_edg (the fused tracer transport forms the tracer edge values)
         "e_cnuity12": 104,                   # normal wind and dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo, sums
         "e_cnuity3": 44,                     # in/outflow sums, delp_lo, dp, dp_tdcy(of,vof) -> r+, r-
         "e_cnuity45": 164,                   # antifx r+- massfx dp_tdcy dp pr pressure gradient u v -> massfx dp_tdcy dp pr ex omega
@@ -105,6 +105,7 @@ def ens_kernel_bytes(T):
         "e_trcadv1": 8 + 80 * T,             # massfx dp; per tracer trc_edg (not theta's) tr trdp trl -> antiflx trl lo sums
         "e_trcadv2": 8 + 40 * T,
         "e_trcadv3": 8 + 68 * T,
+        "e_trc_fct": 32 + 40 * T,            # massfx dp; per tracer tr trdp trl(of,vof) trc(of,vof) -> tr trdp trl trc
         "e_momtum": 80,                      # u v dp ex mp, step-start theta, tendencies -> relvor tendency u2 v2
         "k_regrid": 20,
         "k_regrid_tail": 16,
@@ -457,16 +458,19 @@ def run_ens_mass_adv(a, e, cells, kb, fimhip):
 
 
 def run_uneven(a, th, ds, fimhip):
-    """Two resolutions in one launch: the G6 and G7 grids concatenated into one
-    instance (204804 cells x nvl layers, the atmosphere part of the uneven
-    batch), at the finer grid's time step, full steps, in the mixed form and in
-    FP64."""
+    """The uneven batch of one launch: the G6 and G7 atmosphere grids
+    concatenated into one instance (204804 cells x nvl layers, 4 tracers, at
+    the finer grid's time step) and the ocean instance on G8 (655362 cells x
+    32 layers, temperature and salinity, ~30 % land, dt = 45 s), both in the
+    mixed-precision form, stepped together by one graph replay per step;
+    beside it each instance alone, and the atmosphere part in FP64."""
     import torch
-    from fimocn_subx_realtime_hera_apr2020_amd import glvl
+    from fimocn_subx_realtime_hera_apr2020_amd import glvl, ocean
     gu = glvl.concat([fimhip.icos_grid(6), fimhip.icos_grid(7)])
     dt = 1.2 * 4800.0 / float(2 ** 7 // 2)
     cells = gu.nip * a.nvl
-    out = {"grids": "G6 + G7 in one instance", "nip": gu.nip, "dt": dt}
+    out = {"grids": "G6 + G7 atmosphere in one instance; G8 x 32 ocean instance", "nip": gu.nip, "dt": dt}
+    # the atmosphere alone, mixed form and FP64
     for form in ("mixed", "fp64"):
         inst = (fimhip.Ensemble(gu, 1, a.nvl, a.ntra, dt, th, ds, device=0) if form == "mixed"
                 else fimhip.Model(gu, a.nvl, a.ntra, dt, th, ds, device=0))
@@ -497,6 +501,44 @@ def run_uneven(a, th, ds, fimhip):
             inst.close()
         out[form] = {"value": cells * a.steps / wall, "ms_per_step": wall / a.steps * 1e3}
     out["ratio"] = out["mixed"]["value"] / out["fp64"]["value"]
+    # the ocean instance, and both together
+    go = fimhip.icos_grid(8)
+    tho, dso = ocean.vertical_tables()
+    dto = 1.2 * 4800.0 / float(2 ** 8 // 2)
+    sto = ocean.make_ocean_state(go.lat, go.lon)
+    ocells = go.nip * ocean.NVL
+    atm = fimhip.Ensemble(gu, 1, a.nvl, a.ntra, dt, th, ds, device=0)
+    oce = fimhip.Ensemble(go, 1, ocean.NVL, ocean.NTRA, dto, tho, dso, device=0)
+    try:
+        atm.synth(0, seed=SEED, synth_member=0, flags=fimhip.SYNTH_WIND_NOISE)
+        atm.init_state(True)
+        oce.load_member(0, sto)
+        oce.set_wet(0, sto["wet"])
+        oce.init_state(True)
+        del sto
+        oce.step(a.warmup, a.mode)
+        oce.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        oce.step(a.steps, a.mode)
+        oce.sync()
+        torch.cuda.synchronize()
+        wo = time.perf_counter() - t0
+        fimhip.step_together([atm, oce], a.warmup, a.mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fimhip.step_together([atm, oce], a.steps, a.mode)
+        torch.cuda.synchronize()
+        wt = time.perf_counter() - t0
+        m, _, _, _ = oce.globsum()
+    finally:
+        atm.close()
+        oce.close()
+    out["ocean"] = {"grid": "G8 x 32 layers, T and S, 30 % land", "nip": go.nip, "dt": dto,
+                    "value": ocells * a.steps / wo, "ms_per_step": wo / a.steps * 1e3}
+    out["together"] = {"value": (cells + ocells) * a.steps / wt, "ms_per_step": wt / a.steps * 1e3,
+                       "cell_levels_per_step": cells + ocells,
+                       "note": "atmosphere and ocean instances, one graph replay per step"}
     return out
 
 

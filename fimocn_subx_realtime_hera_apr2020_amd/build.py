@@ -17,7 +17,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libfimhip.so")
 
 SOURCES = ["fimhip_api.hip", "kernels_plain.hip", "transp3d.hip", "hybgen.hip", "ens_api.hip", "kernels_ens.hip",
-           "transp3d_ens.hip"]
+           "transp3d_ens.hip", "fct_tiles.hip"]
 HOST_SOURCES = ["grid.cpp"]      # host-only C++ (g++)
 HEADERS = ["fimhip_internal.h", "fim_pow.h", "fim_pow_tables.h", "synth.h", "host_common.h", "ens_internal.h"]
 
@@ -70,5 +70,32 @@ def build(force: bool = True, verbose: bool = False) -> str:
     return LIB
 
 
+FC = os.environ.get("FC_AMD", "/opt/rocm/lib/llvm/bin/amdflang")
+FORTRAN_BIND = os.path.join(PKG, "fortran", "fimhip_bind.F90")
+CALLER_SRC = os.path.join(ROOT, "tests", "fortran", "fimhip_caller.F90")
+CALLER = os.path.join(ROOT, "tests", "fortran", "fimhip_caller")
+
+
+def build_fortran(verbose: bool = False):
+    """The Fortran binding module (fortran/fimhip_bind.F90) and the test
+    driver that calls the library through it (tests/fortran/fimhip_caller),
+    with amdflang; None when no Fortran compiler is installed."""
+    if not os.path.exists(FC):
+        return None
+    od = os.path.join(_objs_dir(), "fortran")
+    os.makedirs(od, exist_ok=True)
+    cmds = [[FC, "-c", "-O2", "-J" + od, FORTRAN_BIND, "-o", os.path.join(od, "fimhip_bind.o")],
+            [FC, "-c", "-O2", "-I" + od, "-J" + od, CALLER_SRC, "-o", os.path.join(od, "fimhip_caller.o")],
+            [FC, os.path.join(od, "fimhip_caller.o"), os.path.join(od, "fimhip_bind.o"), "-L" + PKG, "-lfimhip",
+             "-Wl,-rpath,$ORIGIN/../../" + os.path.basename(PKG), "-o", CALLER + ".tmp"]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), file=sys.stderr)
+        subprocess.check_call(cmd, cwd=od)
+    os.replace(CALLER + ".tmp", CALLER)
+    return CALLER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

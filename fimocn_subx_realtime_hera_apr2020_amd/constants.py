@@ -46,3 +46,9 @@ THETA_COOR64 = [
 DPSIG64 = [
     330, 510, 675, 825, 960, 1080, 1185, 1275, 1350, 1410,
     1455, 1485] + [1500] * 52
+
+# minimum ocean layer thickness (m), surface layer first: the first 32 values
+# of FIMrun/ocn_dpmin.txt, the 32-layer table hycom_init.F90:169-170 reads
+OCN_DPMIN32 = [2.0, 3.0, 4.0, 5.0, 6.0, 8.0, 10.0, 12.0, 15.0, 18.0, 21.0, 25.0, 29.0, 33.0, 38.0, 43.0,
+               48.0, 54.0, 60.0, 66.0, 73.0, 81.0, 90.0, 101.0, 114.0, 129.0, 146.0, 166.0, 189.0, 215.0,
+               244.0, 276.0]

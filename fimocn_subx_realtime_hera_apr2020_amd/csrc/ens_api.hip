@@ -7,6 +7,7 @@
 
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ens_internal.h"
@@ -40,17 +41,34 @@ struct fimhip_ens {
   // hybgen), forked and joined with events; used in the captured graph
   hipStream_t side = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // one captured step per buffer parity (the fused tracer FCT reads tr/trdp
+  // and writes tr_alt/trdp_alt, and the instance swaps them after the step)
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  int graph_modes[2] = {-1, -1};
   KTimer *kt = nullptr;
+  // the tracer FCT as one kernel over patches with recomputed halo rings
+  // (fct_tiles.hip), or as the three-kernel chain with edgvar3's tracer
+  // reconstructions in HBM; both give the same bits
+  bool fused = true;
+  EnsTiles tiles{};
 };
 
 static void drop_graph(fimhip_ens *e) {
-  if (e->exec) (void)hipGraphExecDestroy(e->exec);
-  if (e->graph) (void)hipGraphDestroy(e->graph);
-  e->exec = nullptr;
-  e->graph = nullptr;
+  for (int p = 0; p < 2; p++) {
+    if (e->exec[p]) (void)hipGraphExecDestroy(e->exec[p]);
+    if (e->graph[p]) (void)hipGraphDestroy(e->graph[p]);
+    e->exec[p] = nullptr;
+    e->graph[p] = nullptr;
+    e->graph_modes[p] = -1;
+  }
   e->graph_mode = -1;
+}
+
+// after a step with the fused tracer FCT the new tracers are in the other buffer
+static void swap_tracer_buffers(fimhip_ens *e) {
+  std::swap(e->s.tr, e->s.tr_alt);
+  std::swap(e->s.trdp, e->s.trdp_alt);
 }
 
 template <class T>
@@ -113,6 +131,7 @@ extern "C" int fimhip_ens_create(const fimhip_grid_desc *gd, int members, int de
   s.ncol = (int)ncol;
   struct { float **p; size_t n; } f32[] = {
       {&s.us, cz}, {&s.vs, cz}, {&s.u2, cz}, {&s.v2, cz}, {&s.tr, cz * ntra}, {&s.trdp, cz * ntra},
+      {&s.tr_alt, cz * ntra}, {&s.trdp_alt, cz * ntra},
       {&s.u_tdcy, 3 * cz}, {&s.v_tdcy, 3 * cz}, {&s.dpl_tdcy, 3 * cz}, {&s.trc_tdcy, 3 * cz * ntra},
       {&s.trl_tdcy, 3 * cz * ntra}, {&s.massfx, ez}, {&s.ws, cz}, {&s.relvor, cz},
       {&s.vn_edg, ez}, {&s.dp_edg, ez}, {&s.pgx, cz}, {&s.pgy, cz}, {&s.trc_edg, ez * (ntra > 1 ? ntra - 1 : 1)}, {&s.th0, cz},
@@ -127,6 +146,13 @@ extern "C" int fimhip_ens_create(const fimhip_grid_desc *gd, int members, int de
   for (auto &x : f64)
     if ((r = a.alloc(x.p, x.n))) { fimhip_ens_destroy(e); return r; }
   if ((r = a.alloc(&s.ctl, 1))) { fimhip_ens_destroy(e); return r; }
+  {
+    TileHost th;
+    if ((r = ens_build_tiles(hg, nip, th)) || (r = ens_upload_tiles(a, th, e->tiles))) {
+      fimhip_ens_destroy(e);
+      return r;
+    }
+  }
   // staging member (FP64, plain layout)
   const size_t mcz = (size_t)nip * nvl, mlz = (size_t)nip * (nvl + 1);
   DevState &t = e->stage;
@@ -194,11 +220,12 @@ extern "C" int fimhip_ens_load_member(fimhip_ens *e, int member, const double *u
   return FIMHIP_OK;
 }
 
-static HybCols<float> hyb_cols(fimhip_ens *e) {
-  const EnsState &s = e->s;
+// hybgen's view of a state: the step's own copy, whose tracer buffers are
+// the new ones after the fused tracer transport
+static HybCols<float> hyb_cols(const fimhip_ens *e, const EnsState &s) {
   const DevGrid &d = e->g.d;
   return HybCols<float>{d.nip, s.ncol, d.nvl, d.ntra, s.u2, s.v2, s.us, s.vs, s.tr, s.dp, s.pr, s.ex, s.sdot,
-                        s.exsmo, s.thbuf, d.thetac, d.dpsig, d.smoo, d.sideln, d.rarea, d.prox, d.nprox};
+                        s.exsmo, s.thbuf, d.thetac, d.dpsig, d.smoo, d.sideln, d.rarea, d.prox, d.nprox, s.wet};
 }
 
 // dyn_init start-up for every member (dyn_init.F90:870-1094): optional
@@ -213,7 +240,7 @@ extern "C" int fimhip_ens_init_state(fimhip_ens *e, int init_hybgen) {
     // the start-up remap reads the winds from u2/v2
     HIPCHK(hipMemcpyAsync(s.u2, s.us, cz * 4, hipMemcpyDeviceToDevice, e->stream));
     HIPCHK(hipMemcpyAsync(s.v2, s.vs, cz * 4, hipMemcpyDeviceToDevice, e->stream));
-    launch_hybgen_cols<float>(hyb_cols(e), nullptr, 0, e->stream);
+    launch_hybgen_cols<float>(hyb_cols(e, s), nullptr, 0, e->stream);
     ens_launch_pres_from_exner(e->g, s, e->stream);
   }
   ens_launch_hystat_init(e->g, s, e->stream);
@@ -243,7 +270,8 @@ extern "C" int fimhip_ens_init_state(fimhip_ens *e, int init_hybgen) {
 // step read.
 static void enqueue_step(fimhip_ens *e, int mode, bool side) {
   const EnsGrid &g = e->g;
-  const EnsState &s = e->s;
+  EnsState s = e->s;
+  s.edg3_ntr = e->fused ? 0 : e->ntra - 1;
   hipStream_t st = e->stream, sd = e->side;
   ens_launch_abstart(s, e->dt, st);
   if (side) {
@@ -264,7 +292,13 @@ static void enqueue_step(fimhip_ens *e, int mode, bool side) {
     for (int k = EK_CNUITY12; k <= EK_CNUITY45; k++) ens_launch(k, e->tile[k], g, s, st);
   }
   if (e->cb.ntrb > 0) ens_launch_class_b_step(g, s, e->cb, st);   // dyn_run.F90:388-405
-  for (int k = EK_TRCADV1; k <= EK_TRCADV3; k++) ens_launch(k, e->tile[k], g, s, st);
+  if (e->fused) {
+    ens_launch_trc_fct(g, s, e->tiles, s.tr, s.trdp, s.tr_alt, s.trdp_alt, st);
+    std::swap(s.tr, s.tr_alt);         // the rest of the step works on the new tracers
+    std::swap(s.trdp, s.trdp_alt);
+  } else {
+    for (int k = EK_TRCADV1; k <= EK_TRCADV3; k++) ens_launch(k, e->tile[k], g, s, st);
+  }
   // momtum needs nothing of the tracer transport; it runs beside hybgen's
   // column kernels up to the remap (latency-bound, so the two share the
   // chip), rather than beside the bandwidth-bound tracer kernels
@@ -278,9 +312,15 @@ static void enqueue_step(fimhip_ens *e, int mode, bool side) {
   }
   if (mode == 0) {
     if (!side) ens_launch(EK_MOMTUM, e->tile[EK_MOMTUM], g, s, st);
-    launch_hybgen_cols<float>(hyb_cols(e), s.ctl, 0, st, momtum_done);
+    launch_hybgen_cols<float>(hyb_cols(e, s), s.ctl, 0, st, momtum_done);
     ens_launch(EK_HYSTAT, e->tile[EK_HYSTAT], g, s, st);
   }
+}
+
+// one step enqueued directly (no graph), the tracer buffers swapped after it
+static void step_once(fimhip_ens *e, int mode, bool side) {
+  enqueue_step(e, mode, side);
+  if (e->fused) swap_tracer_buffers(e);
 }
 
 extern "C" int fimhip_ens_step(fimhip_ens *e, int nsteps, int mode) {
@@ -288,7 +328,7 @@ extern "C" int fimhip_ens_step(fimhip_ens *e, int nsteps, int mode) {
   HIPCHK(hipSetDevice(e->device));
   if (!e->use_graph || e->kt) {
     g_ktimer = e->kt;
-    for (int i = 0; i < nsteps; i++) enqueue_step(e, mode, false);
+    for (int i = 0; i < nsteps; i++) step_once(e, mode, false);
     g_ktimer = nullptr;
     HIPCHK(hipGetLastError());
     if (e->kt && e->kt->flush(e->stream)) return FIMHIP_EHIP;
@@ -296,13 +336,85 @@ extern "C" int fimhip_ens_step(fimhip_ens *e, int nsteps, int mode) {
   }
   if (e->graph_mode != mode) {
     drop_graph(e);
-    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(e, mode, e->overlap);
-    HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
-    HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
     e->graph_mode = mode;
   }
-  for (int i = 0; i < nsteps; i++) HIPCHK(hipGraphLaunch(e->exec, e->stream));
+  for (int i = 0; i < nsteps; i++) {
+    // the buffer parity of this step: which of the pair holds the tracers
+    const int p = (e->fused && e->s.tr > e->s.tr_alt) ? 1 : 0;
+    if (e->graph_modes[p] != mode) {
+      HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_step(e, mode, e->overlap);
+      HIPCHK(hipStreamEndCapture(e->stream, &e->graph[p]));
+      HIPCHK(hipGraphInstantiate(&e->exec[p], e->graph[p], nullptr, nullptr, 0));
+      e->graph_modes[p] = mode;
+    }
+    HIPCHK(hipGraphLaunch(e->exec[p], e->stream));
+    if (e->fused) swap_tracer_buffers(e);
+  }
+  return FIMHIP_OK;
+}
+
+// Several instances on one GPU stepped by one graph replay per step (the
+// uneven batch: resolutions and layer counts that cannot share a launch run
+// as branches of one graph, each instance's kernels on its own stream).  The
+// graph is captured for the call and replayed nsteps times; the call returns
+// when the steps are done.
+extern "C" int fimhip_ens_step_together(fimhip_ens *const *list, int n, int nsteps, int mode) {
+  if (!list || n < 1 || n > 64 || nsteps < 0 || (mode != 0 && mode != 1)) return FIMHIP_EINVAL;
+  for (int i = 0; i < n; i++) {
+    if (!list[i] || list[i]->device != list[0]->device || list[i]->kt) return FIMHIP_EINVAL;
+    for (int j = 0; j < i; j++)
+      if (list[j] == list[i]) return FIMHIP_EINVAL;
+  }
+  HIPCHK(hipSetDevice(list[0]->device));
+  hipStream_t a = list[0]->stream;
+  std::vector<hipEvent_t> ev(2 * (size_t)n, nullptr);
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipError_t err = hipSuccess;
+  for (auto &v : ev)
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&v, hipEventDisableTiming);
+  // every instance with the fused tracer transport swaps its tracer buffers
+  // after each step, so the captured steps alternate between two graphs
+  for (int it = 0; it < nsteps && err == hipSuccess; it++) {
+    const int p = it & 1;
+    if (!exec[p]) {
+      err = hipStreamBeginCapture(a, hipStreamCaptureModeThreadLocal);
+      for (int i = 0; i < n && err == hipSuccess; i++) {
+        fimhip_ens *e = list[i];
+        if (i > 0) {
+          (void)hipEventRecord(ev[2 * i], a);
+          (void)hipStreamWaitEvent(e->stream, ev[2 * i], 0);
+        }
+        enqueue_step(e, mode, e->overlap);
+        if (i > 0) {
+          (void)hipEventRecord(ev[2 * i + 1], e->stream);
+          (void)hipStreamWaitEvent(a, ev[2 * i + 1], 0);
+        }
+      }
+      hipError_t ec = hipStreamEndCapture(a, &graph[p]);
+      if (err == hipSuccess) err = ec;
+      if (err == hipSuccess) err = hipGraphInstantiate(&exec[p], graph[p], nullptr, nullptr, 0);
+    }
+    if (err == hipSuccess) err = hipGraphLaunch(exec[p], a);
+    for (int i = 0; i < n; i++)
+      if (list[i]->fused) swap_tracer_buffers(list[i]);
+  }
+  if (err == hipSuccess) err = hipStreamSynchronize(a);
+  for (int p = 0; p < 2; p++) {
+    if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
+    if (graph[p]) (void)hipGraphDestroy(graph[p]);
+  }
+  for (auto v : ev)
+    if (v) (void)hipEventDestroy(v);
+  HIPCHK(err);
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_ens_set_fused(fimhip_ens *e, int on) {
+  if (!e) return FIMHIP_EINVAL;
+  if (e->fused != (on != 0)) drop_graph(e);
+  e->fused = on != 0;
   return FIMHIP_OK;
 }
 
@@ -352,6 +464,27 @@ extern "C" int fimhip_ens_load_class_b(fimhip_ens *e, int member, const double *
                           hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));   // h is reused for the next tracer
   }
+  return FIMHIP_OK;
+}
+
+// land/sea mask of one member (nip ints, 0 = land); the first call gives the
+// instance a mask with every column wet
+extern "C" int fimhip_ens_set_wet(fimhip_ens *e, int member, const int *wet) {
+  if (check_member(e, member) || !wet) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(e->device));
+  int r;
+  if (!e->s.wet) {
+    std::vector<unsigned char> ones((size_t)e->s.ncol, 1);
+    unsigned char *w = nullptr;
+    if ((r = e->arena.upload(&w, ones))) return r;
+    e->s.wet = w;
+    drop_graph(e);
+  }
+  std::vector<unsigned char> h((size_t)e->nip);
+  for (int i = 0; i < e->nip; i++) h[i] = wet[i] != 0;
+  HIPCHK(hipMemcpyAsync((unsigned char *)e->s.wet + (size_t)member * e->nip, h.data(), h.size(),
+                        hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
   return FIMHIP_OK;
 }
 
@@ -514,7 +647,7 @@ extern "C" int fimhip_ens_autotune(fimhip_ens *e, int mode, int reps, double *ti
     for (int k = 0; k < EK_NKERNELS; k++) e->tile[k] = t;
     KTimer kt;
     g_ktimer = &kt;
-    for (int i = 0; i < reps; i++) enqueue_step(e, mode, false);
+    for (int i = 0; i < reps; i++) step_once(e, mode, false);
     g_ktimer = nullptr;
     HIPCHK(hipGetLastError());
     if (kt.flush(e->stream)) return FIMHIP_EHIP;

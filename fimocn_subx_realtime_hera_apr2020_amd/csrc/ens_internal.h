@@ -21,9 +21,15 @@
 //   values are formed where they are used); the mass flux keeps only the
 //   current slot.
 #pragma once
+#include <array>
+#include <vector>
+
 #include "fimhip_internal.h"
 
 namespace fimhip {
+
+struct HostGrid;
+struct Arena;
 
 // FP32 copies of the grid metrics (the reference stores them in FP32).
 struct EnsGrid {
@@ -35,6 +41,7 @@ struct EnsGrid {
 struct EnsState {
   int members, ncol;
   float *us, *vs, *u2, *v2;
+  float *tr_alt, *trdp_alt;          // the other buffer of tr / trdp (the fused FCT writes it)
   double *dp, *pr, *ex, *ph, *mp, *sdot, *exsmo, *psrf, *ptdcy, *thbuf;
   float *tr, *trdp;
   float *u_tdcy, *v_tdcy, *dpl_tdcy;
@@ -47,7 +54,16 @@ struct EnsState {
   float *antifx, *delp_lo, *r_plus, *r_mnus;
   float *antiflx, *trcr_lo, *trmax, *trmin, *tr_rp, *tr_rm;
   StepCtl *ctl;
+  int edg3_ntr;                       // tracers 2..edg3_ntr+1 reconstructed by edgvar3 (0: thickness only)
+  // Land/sea mask per column (0 = land), or null (every column wet).  An edge
+  // carries mass and tracer flux only if both of its cells are wet (as the
+  // ocean's continuity gates its fluxes, hycom_cnuity.F90:100-125); a land
+  // column keeps its thickness, winds and tracers, and the Laplacian
+  // momentum diffusion through a land edge is zero.
+  const unsigned char *wet;
 };
+// wet(column) of a mask that may be null
+__device__ __forceinline__ bool col_wet(const unsigned char *wet, int col) { return !wet || ldc(wet, col) != 0; }
 
 // Launch shapes ("tiles"): columns per 256-lane-or-smaller block and whether
 // the block index is remapped so that each XCD works on a contiguous range of
@@ -75,6 +91,31 @@ struct EnsClassB {
 };
 void ens_launch_class_b_init(const EnsGrid &g, const EnsState &s, const EnsClassB &b, hipStream_t st);
 void ens_launch_class_b_step(const EnsGrid &g, const EnsState &s, const EnsClassB &b, hipStream_t st);
+
+// Patches of cells for the fused tracer FCT (fct_tiles.hip): per patch the
+// cumulative ends n0..n3 of its core and three neighbour rings in its list
+// of tile cells, and per tile cell its cell index and, for rings 0-2, the
+// local indices (16 bits each) of its six neighbours in the tile.
+constexpr int FCT_NW = 16;                     // wavefronts per block
+constexpr int FCT_CORE_MAX = 64;               // core cells of a patch
+constexpr int FCT_J = FCT_CORE_MAX / FCT_NW;   // core cells per wavefront
+constexpr int FCT_R_MAX = 112;                 // cells in rings 0-1
+constexpr int FCT_LO_MAX = 160;                // cells in rings 0-2
+constexpr int FCT_TILE_MAX = 208;              // cells in rings 0-3
+struct EnsTiles {
+  int npatch;
+  const int *off;      // (npatch + 1) first tile cell of each patch
+  const int4 *cnt;     // (npatch) n0, n1, n2, n3
+  const int4 *ent;     // cell, neighbours 0|1, 2|3, 4|5
+};
+struct TileHost {
+  std::vector<int> off;
+  std::vector<std::array<int, 4>> cnt, ent;
+};
+int ens_build_tiles(const HostGrid &h, int nip, TileHost &out);
+int ens_upload_tiles(Arena &a, const TileHost &t, EnsTiles &d);
+void ens_launch_trc_fct(const EnsGrid &g, const EnsState &s, const EnsTiles &ts, const float *trs,
+                        const float *tds, float *trd, float *tdd, hipStream_t st);
 
 void ens_launch_abstart(const EnsState &s, double dt, hipStream_t st);
 void ens_launch(int kernel, int tile, const EnsGrid &g, const EnsState &s, hipStream_t st);

@@ -113,6 +113,7 @@ struct HybCols {
   double *thbuf;            // (nvl, ncol) work column for the split form, or null
   const double *thetac, *dpsig, *smoo, *sideln, *rarea;
   const int *prox, *nprox;
+  const unsigned char *wet;  // land/sea mask per column (0 = land: left as it is), or null
 };
 // winds_ready (may be null): an event remap_1d's launch waits for, since it is
 // the first of these kernels to read the winds

@@ -1,12 +1,13 @@
 // hybgen (FIMsrc/fim/horizontal/hybgen.F90) on the GPU, in FP64.
 //
-// regrid_1d (207-512), restp_1d (792-890), equilb (1282-1437) and inflate
-// (893-1007) run one thread per column with the column held in private
-// arrays (the scratch segment is lane-interleaved, so lane i's element k and
-// lane i+1's element k are adjacent and the accesses coalesce).  The lateral
-// interface smoothing dffusn_lev (dffusn.F90:37-58) is a cell-level kernel.
-// remap_1d (515-789) with pcwise (1010-1279) is again one thread per column.
-// Loop orders and operation orders are those of the reference.
+// regrid_1d (207-512) with restp_1d (792-890), equilb (1282-1437) and
+// inflate (893-1007), and remap_1d (515-789) with pcwise (1010-1279), run one
+// wavefront per column (lane = level) with the column in LDS; at 64 levels
+// the inherently serial sweeps (equilb, inflate, the theta redistribution)
+// run in separate kernels one thread per column with the column in
+// registers.  The lateral interface smoothing dffusn_lev (dffusn.F90:37-58)
+// is a cell-level kernel.  Loop orders and operation orders are those of the
+// reference.  Columns a land/sea mask marks as land are left as they are.
 #include "fimhip_internal.h"
 #include "fim_pow.h"
 
@@ -62,46 +63,6 @@ __device__ __forceinline__ double cushn(double arg, double arg1) {
   return .25 * (m * m) / arg1 + dmax(arg, 2. * arg1) - arg1;
 }
 
-__device__ void inflate(const double *dpsig, int nvl, double psurf, double *exner) {
-  double dpsum = 0.;
-  const double extop = CP * fim_pow(PTOP / P1000, RD / CP);
-  const double top = .2;
-  double shrink = dmin(1., (psurf / P1000 - top) / (1. - top));
-  int last = 1;
-  for (int k = 1; k <= nvl - 1; k++) {
-    double dp0 = RD * dpsig[k - 1] / P1000 * shrink;
-    dpsum = dpsum + dp0;
-    double exold = exner[k];
-    if (k <= KBL) {
-      double dp1 = dp0;
-      last = k + 1;
-      exner[k] = exner[k - 1] - dp1;
-    } else {
-      if (k > last) {
-        const double p2 = ldexp(1., k - last);   // 2**(k-last), exact
-        dp0 = dp0 / dmin(10., 1.25 * p2);
-        dp0 = dmax(dp0, RD * 100. / P1000);
-      }
-      double dp1 = cushn(dmax(0., exner[k - 1] - dmin(exold, exner[0] - dpsum)), dp0);
-      if (exner[k - 1] - dp1 < exold - THIN) {
-        if (k == last) last = k + 1;
-        exner[k] = dmax(extop, exner[k - 1] - dp1);
-      }
-    }
-  }
-  for (int k = nvl; k >= 2; k--) {
-    double q = exner[k - 1] / CP;
-    q = q * q * sqrt(q);
-    double tq = .25 * (double)(nvl - k);
-    double tap = 1. / (1. + tq * tq);
-    double dp0 = RD * THKTOP / P1000 * tap / q;
-    double exold = exner[k - 1];
-    double dp1 = cushn(dmax(0., exold - exner[k]), dp0);
-    if (exner[k] + dp1 > exold + THIN) exner[k - 1] = exner[k] + dp1;
-    if (exner[k - 1] < exner[k - 2] - dp0) break;
-  }
-}
-
 __device__ void equilb(const double *thet, double *pnew, int kk, double slak, int k0 = 3) {
   for (int k = k0; k <= kk - 2; k++) {
     if (pnew[k - 3] > pnew[0] - THIN) continue;
@@ -126,68 +87,6 @@ __device__ void equilb(const double *thet, double *pnew, int kk, double slak, in
       }
     }
   }
-}
-
-// regrid_1d for one column: thcol (theta, modified), excol (Exner, in/out)
-template <typename V>
-__global__ void __launch_bounds__(64) k_regrid(HybCols<V> s, const StepCtl *ctl, int its) {
-  const int ncol = s.ncol, nvl = s.nvl, nvlp1 = nvl + 1;
-  const int ipn = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ipn >= ncol) return;
-  if (ctl) its = ctl->its;
-  double thcol[NVL_MAX], excol[LMAX], exwrk[LMAX], exnew[LMAX], thnew[NVL_MAX], heatfx[NVL_MAX];
-  const double *targt = s.thetac;
-  const double dffudt = .1, tolrnce = 0.001;
-  for (int k = 0; k < nvl; k++) thcol[k] = (double)TR(s.tr, 0, ipn, k);
-  for (int k = 0; k < nvlp1; k++) excol[k] = L2(s.ex, ipn, k);
-  const double psurf = L2(s.pr, ipn, 0);
-  double eqlb_slak = .1 * (double)its / (10. + (double)its);
-  for (int k = 1; k <= nvl; k++)
-    if (excol[k - 1] < excol[k] - THIN) excol[k] = excol[k - 1];
-  for (int k = nvl - 1; k >= 1; k--) thcol[k - 1] = dmin(thcol[k - 1], thcol[k]);
-  for (int k = 0; k < nvlp1; k++) exwrk[k] = excol[k];
-  const double ttop = targt[nvl - 1];
-  for (int k = nvl; k >= 2; k--) {
-    if (thcol[k - 1] <= ttop + tolrnce) break;
-    double wgt1 = dmax(0., exwrk[k - 2] - exwrk[k - 1]);
-    double wgt2 = dmax(THIN, exwrk[k - 1] - exwrk[k]);
-    double try_ = (thcol[k - 2] * wgt1 + thcol[k - 1] * wgt2) / (wgt1 + wgt2);
-    if (try_ > ttop - tolrnce) {
-      exwrk[k - 1] = exwrk[nvlp1 - 1];
-      for (int k1 = k - 1; k1 <= nvl; k1++) thcol[k1 - 1] = try_;
-    } else {
-      exwrk[k - 1] = dmin(exwrk[k - 2], dmax(exwrk[k], (exwrk[k - 1] * (thcol[k - 1] - thcol[k - 2]) +
-                                                         exwrk[k] * (ttop - thcol[k - 1])) /
-                                                            (ttop - thcol[k - 2])));
-      for (int k1 = k; k1 <= nvl; k1++) thcol[k1 - 1] = ttop;
-      break;
-    }
-  }
-  restp_1d(thcol, exwrk, nvl, thnew, exnew, targt, nvl);
-  heatfx[nvl - 1] = 0.;
-  heatfx[0] = 0.;
-  for (int iter = 1; iter <= 3; iter++) {
-    for (int k = 0; k < nvlp1; k++) exwrk[k] = exnew[k];
-    for (int k = 2; k <= nvl - 1; k++) {
-      heatfx[k - 1] = 0.;
-      if (exwrk[k - 1] < exwrk[0] - .01 && exwrk[k] > exwrk[nvlp1 - 1] + .01) {
-        double cnv = GRVITY / targt[k - 1];
-        heatfx[k - 1] = .1 * cnv * cnv * .5 * (targt[k] - targt[k - 2]) / dmax(.03 * cnv, exwrk[k - 1] - exwrk[k]);
-      }
-    }
-    for (int k = 1; k <= nvl - 1; k++) {
-      if (exwrk[k] < exwrk[0] - .01 && exwrk[k] > exwrk[nvlp1 - 1] + .01)
-        exnew[k] = dmax(exwrk[k + 1], dmin(exwrk[k - 1], exwrk[k] + (heatfx[k] - heatfx[k - 1]) / (targt[k] - targt[k - 1])));
-    }
-    bool event = false;
-    for (int k = 2; k <= nvlp1; k++)
-      if (exnew[k - 1] > exnew[k - 2] + THIN) event = true;
-    if (!event) break;
-  }
-  (void)dffudt;
-  equilb(thnew, exnew, nvl, eqlb_slak);
-  inflate(s.dpsig, nvl, psurf, exnew);
-  for (int k = 0; k < nvlp1; k++) L2(s.sdot, ipn, k) = exnew[k];   // pre-smoothing Exner, kept in sdot
 }
 
 // dffusn_lev(exsmo, smoo_coeff, nvlp1, kbl+2, nvl): reads the regridded Exner
@@ -229,150 +128,6 @@ __global__ void __launch_bounds__(256) k_dffusn_lev(HybCols<V> s) {
     if (kk + (int)blockDim.x == nvlp1 - 1) break;   // the top interface is done
   }
 }
-
-// pcwise (PLM, miglim = 1) for one column.  The old and new interface
-// coordinates x*, the cell widths fco/fcn and the displacements delz are
-// formed on the fly from xold/xnew (the reference's zold/znew are +-x exactly),
-// and the slopes and interface fluxes are formed in one forward sweep, so the
-// only column array is the filled field itself.
-template <typename Y>
-__device__ void pcwise(int kk, const double *xold, const double *xnew, double *fld, Y *ynew) {
-  const double small = 1.e-22;
-  const bool up = xold[0] < xold[kk];
-  auto zo = [&](int k) { return up ? xold[k] : -xold[k]; };
-  auto zn = [&](int k) { return up ? xnew[k] : -xnew[k]; };
-  auto fco = [&](int k) { return dmax(0., zo(k + 1) - zo(k)); };
-  // fill zero-width cells from their neighbours (two sweeps)
-  for (int k = kk - 1; k >= 1; k--) {
-    const double f = fco(k - 1);
-    fld[k - 1] = (fld[k - 1] * f + fld[k] * small) / (f + small);
-  }
-  for (int k = 2; k <= kk; k++) {
-    const double f = fco(k - 1);
-    fld[k - 1] = (fld[k - 1] * f + fld[k - 2] * small) / (f + small);
-  }
-  auto slope = [&](int k) {   // PLM slope b(k), 1-based
-    if (k == 1 || k == kk) return 0.;
-    const double fm = fld[k - 2], f0 = fld[k - 1], fp = fld[k];
-    if (f0 <= dmin(fm, fp) || f0 >= dmax(fm, fp)) return 0.;
-    if ((fp - fm) * (fm + fp - 2. * f0) > 0.) return f0 - fm;
-    return fp - f0;
-  };
-  auto vflux = [&](int k) {   // vertfx(k), 1-based, k = 2..kk
-    double slab = 0., amount = 0., vf = 0.;
-    const double dz = zn(k - 1) - zo(k - 1);
-    if (dz > 0.) {
-      const int lyr = k;
-      const double fc = fco(lyr - 1);
-      if (fc > 0.) {
-        double dslab = dmin(slab + fc, dz) - dmin(slab, dz);
-        double dx = dslab / fc;
-        double fcdx = fld[lyr - 1] + slope(lyr) * .5 * (dx - 1.) + 0. * (.25 - dx * (.5 - dx * (1. / 3.)));
-        amount = amount + fcdx * dslab;
-        slab = slab + dslab;
-      }
-    } else if (dz < 0.) {
-      const int lyr = k - 1;
-      const double fc = fco(lyr - 1);
-      if (fc > 0.) {
-        double dslab = dmin(slab + fc, -dz) - dmin(slab, -dz);
-        double dx = dslab / fc;
-        double fcdx = fld[lyr - 1] + slope(lyr) * .5 * (1. - dx) + 0. * (.25 - dx * (.5 - dx * (1. / 3.)));
-        amount = amount + fcdx * dslab;
-        slab = slab + dslab;
-      }
-    }
-    if (slab != 0.) vf = -dz * amount / slab;
-    return vf;
-  };
-  // the fluxes use the filled field, which the output loop overwrites:
-  // write the results one level behind the flux sweep
-  double vlo = 0.;                       // vertfx(1)
-  double prev_new = 0.;
-  for (int k = 1; k <= kk; k++) {
-    const double vhi = k < kk ? vflux(k + 1) : 0.;   // vertfx(kk+1) = 0
-    const double fo = fco(k - 1), fn = dmax(0., zn(k) - zn(k - 1));
-    const double amount = fld[k - 1] * fo - (vhi - vlo);
-    const double y = (amount + fld[k - 1] * small) / (fn + small);
-    if (k > 1) ynew[k - 2] = (Y)prev_new;
-    prev_new = y;
-    vlo = vhi;
-  }
-  ynew[kk - 1] = (Y)prev_new;
-}
-
-// remap_1d for one column; also sdot = new Exner - old Exner and ex = new Exner
-template <typename V>
-__global__ void __launch_bounds__(64) k_remap(HybCols<V> s, const StepCtl *ctl, int its) {
-  const int ncol = s.ncol, nvl = s.nvl, nvlp1 = nvl + 1, ntra = s.ntra;
-  const int ipn = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ipn >= ncol) return;
-  if (ctl) its = ctl->its;
-  double prcol[LMAX], pk1col[LMAX], exnew[LMAX], prnew[LMAX], pk1new[LMAX];
-  double fld[NVL_MAX], th1[NVL_MAX];
-  const double *targt = s.thetac;
-  const double small = 1.e-6;
-  for (int k = 0; k < nvlp1; k++) {
-    const double eo = L2(s.ex, ipn, k);
-    prcol[k] = L2(s.pr, ipn, k);
-    exnew[k] = L2(s.exsmo, ipn, k);
-    pk1col[k] = eo * prcol[k] / CP;
-  }
-  inflate(s.dpsig, nvl, prcol[0], exnew);
-  if (its > 0)
-    for (int k = 0; k < nvlp1; k++) exnew[k] = SLAK * exnew[k] + (1. - SLAK) * L2(s.ex, ipn, k);
-  {
-    double em = L2(s.ex, ipn, 0), e0 = L2(s.ex, ipn, 1);
-    for (int k = 2; k <= nvlp1 - 1; k++) {
-      const double ep = L2(s.ex, ipn, k);
-      exnew[k - 1] = dmax(.0001 * e0 + .9999 * ep, dmin(.0001 * e0 + .9999 * em, exnew[k - 1]));
-      em = e0;
-      e0 = ep;
-    }
-  }
-  for (int k = nvlp1; k >= 1; k--) prnew[k - 1] = P1000 * fim_pow(exnew[k - 1] / CP, CP / RD);
-  for (int k = 0; k < nvl; k++) C2(s.dp, ipn, k) = prnew[k] - prnew[k + 1];
-  for (int k = 0; k < nvlp1; k++) pk1new[k] = exnew[k] * prnew[k] / CP;
-  prnew[nvlp1 - 1] = prcol[nvlp1 - 1];
-  pk1new[nvlp1 - 1] = pk1col[nvlp1 - 1];
-  prnew[0] = prcol[0];
-  pk1new[0] = pk1col[0];
-  // u, v and tracers 2..ntra are remapped in pressure
-  for (int k = 0; k < nvl; k++) fld[k] = (double)C2(s.us_in, ipn, k);
-  pcwise(nvl, prcol, prnew, fld, &C2(s.us, ipn, 0));
-  for (int k = 0; k < nvl; k++) fld[k] = (double)C2(s.vs_in, ipn, k);
-  pcwise(nvl, prcol, prnew, fld, &C2(s.vs, ipn, 0));
-  for (int n = 1; n < ntra; n++) {
-    for (int k = 0; k < nvl; k++) fld[k] = (double)TR(s.tr, n, ipn, k);
-    pcwise(nvl, prcol, prnew, fld, &TR(s.tr, n, ipn, 0));
-  }
-  // theta is remapped in p^(kappa+1)
-  for (int k = 0; k < nvl; k++) fld[k] = (double)TR(s.tr, 0, ipn, k);
-  pcwise(nvl, pk1col, pk1new, fld, th1);
-  for (int k = nvl; k >= 3; k--) {
-    double dplo = dmax(pk1new[k - 2] - pk1new[k - 1], pk1new[0] * small);
-    double dpup = dmax(pk1new[k - 1] - pk1new[k], pk1new[0] * small);
-    double tha = th1[k - 2], thb = th1[k - 1];
-    double devilo = (tha - targt[k - 2]) * dplo;
-    double deviup = (thb - targt[k - 1]) * dpup;
-    if (deviup > 0. && devilo < 0.) {
-      th1[k - 2] = tha + dmin(deviup, -devilo) / dplo;
-      th1[k - 1] = thb - dmin(deviup, -devilo) / dpup;
-    } else if (deviup < 0. && devilo > 0.) {
-      th1[k - 2] = tha - dmin(devilo, -deviup) / dplo;
-      th1[k - 1] = thb + dmin(devilo, -deviup) / dpup;
-    }
-  }
-  for (int k = 0; k < nvl; k++) TR(s.tr, 0, ipn, k) = (V)th1[k];
-  for (int k = 1; k <= nvl; k++)
-    if (exnew[k - 1] < exnew[k]) exnew[k] = exnew[k - 1];
-  for (int k = 0; k < nvlp1; k++) {
-    L2(s.pr, ipn, k) = prnew[k];
-    L2(s.sdot, ipn, k) = exnew[k] - L2(s.ex, ipn, k);
-    L2(s.ex, ipn, k) = exnew[k];
-  }
-}
-
 
 // ===========================================================================
 // The same column algorithm with one wavefront per column (lane = level) and
@@ -792,6 +547,10 @@ __global__ void __launch_bounds__(64 * HW) k_regrid_w(HybCols<V> s, const StepCt
   const int lane = threadIdx.x, c = threadIdx.y;
   const int col = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * HW + c));
   if (col >= ncol) return;
+  if (s.wet && !ldc(s.wet, col)) {     // land: no regrid; its Exner function for the neighbours' smoothing
+    for (int k = lane; k < nvlp1; k += 64) L2(s.sdot, col, k) = L2(s.ex, col, k);
+    return;
+  }
   if (ctl) its = load_ctl(ctl).its;
   double *thcol = sh[c][0], *excol = sh[c][1], *exwrk = sh[c][2], *exnew = sh[c][3], *thnew = sh[c][4],
          *heatfx = sh[c][5], *oldth = sh[c][6];
@@ -973,6 +732,10 @@ __global__ void __launch_bounds__(64 * HW) k_remap_w(HybCols<V> s, const StepCtl
   const int lane = threadIdx.x, c = threadIdx.y;
   const int col = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * HW + c));
   if (col >= ncol) return;
+  if (s.wet && !ldc(s.wet, col)) {    // land: no remap
+    for (int k = lane; k < nvlp1; k += 64) L2(s.sdot, col, k) = 0.;
+    return;
+  }
   if (ctl) its = load_ctl(ctl).its;
   double *prcol = sh[c][0], *pk1col = sh[c][1], *exnew = sh[c][2], *prnew = sh[c][3], *pk1new = sh[c][4],
          *th1 = sh[c][5], *exold = sh[c][6];
@@ -1177,7 +940,7 @@ template <int NVL, typename V>
 __global__ void __launch_bounds__(64) k_regrid_tail(HybCols<V> s, const StepCtl *ctl, int its, double extop) {
   const int nvl = NVL, nvlp1 = NVL + 1;
   const int col = blockIdx.x * 64 + threadIdx.x;
-  if (col >= s.ncol) return;
+  if (col >= s.ncol || (s.wet && !s.wet[col])) return;
   if (ctl) its = load_ctl(ctl).its;
   double ex[NVL + 1];
 #pragma unroll
@@ -1195,7 +958,7 @@ template <int NVL, typename V>
 __global__ void __launch_bounds__(64) k_remap_inflate(HybCols<V> s, double extop) {
   const int nvlp1 = NVL + 1;
   const int col = blockIdx.x * 64 + threadIdx.x;
-  if (col >= s.ncol) return;
+  if (col >= s.ncol || (s.wet && !s.wet[col])) return;
   double ex[NVL + 1];
 #pragma unroll
   for (int k = 0; k <= NVL; k++) ex[k] = L2(s.exsmo, col, k);
@@ -1274,34 +1037,17 @@ __global__ void __launch_bounds__(64) k_remap_theta(HybCols<V> s) {
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       const int c = i * 4 + sj;
-      if (c0 + c < ncol && lev < NVL) TR(s.tr, 0, c0 + c, lev) = sout[1 + sl][c];
+      if (c0 + c < ncol && lev < NVL && (!s.wet || s.wet[c0 + c])) TR(s.tr, 0, c0 + c, lev) = sout[1 + sl][c];
     }
-    if (ch == 0 && c0 + lane < ncol) TR(s.tr, 0, c0 + lane, 0) = sout[0][lane];
+    if (ch == 0 && c0 + lane < ncol && (!s.wet || s.wet[c0 + lane])) TR(s.tr, 0, c0 + lane, 0) = sout[0][lane];
     wsync();
   }
   (void)nvl;
   (void)nvlp1;
 }
 
-// FIMHIP_HYBGEN=thread selects the one-thread-per-column kernels
-static bool hybgen_thread_per_column() {
-  static int v = -1;
-  if (v < 0) {
-    const char *e = getenv("FIMHIP_HYBGEN");
-    v = (e && strcmp(e, "thread") == 0) ? 1 : 0;
-  }
-  return v == 1;
-}
-
 template <typename V>
 void launch_hybgen_cols(const HybCols<V> &h, const StepCtl *ctl, int its, hipStream_t st, hipEvent_t winds_ready) {
-  if (hybgen_thread_per_column()) {
-    KT_LAUNCH("k_regrid", st, k_regrid<V><<<(h.ncol + 63) / 64, 64, 0, st>>>(h, ctl, its));
-    KT_LAUNCH("k_dffusn_lev", st, k_dffusn_lev<V><<<(h.ncol + 3) / 4, dim3(64, 4), 0, st>>>(h));
-    if (winds_ready) (void)hipStreamWaitEvent(st, winds_ready, 0);
-    KT_LAUNCH("k_remap", st, k_remap<V><<<(h.ncol + 63) / 64, 64, 0, st>>>(h, ctl, its));
-    return;
-  }
   const dim3 wg((h.ncol + HW - 1) / HW), wb(64, HW);
   if (h.nvl == NVL_MAX && h.thbuf) {
     const dim3 tg((h.ncol + 63) / 64), tb(64);

@@ -164,12 +164,14 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
 #pragma unroll
   for (int e = 0; e < NP; e++) mn[e] = fsqrt(nu[e] * nu[e] + nv[e] * nv[e]);
   float lp[NP];
+  const bool wc = col_wet(s.wet, col);
 #pragma unroll
   for (int e = 0; e < NP; e++) {
     const int et = (e + NP - 1) % NP, el = (e + 1) % NP;
     const float2 w = e_edge_wind(g, ipn, e, ui, vi, nu[e], nv[e], nu[et], nv[et], nu[el], nv[el], mi, mn[e], mn[et],
                                  mn[el]);
-    EE(s.vn_edg, col, e, k) = FSVE(ipn, e, 1) * w.x - FSVE(ipn, e, 0) * w.y;
+    const float vnrm = FSVE(ipn, e, 1) * w.x - FSVE(ipn, e, 0) * w.y;
+    EE(s.vn_edg, col, e, k) = (wc && col_wet(s.wet, nb[e])) ? vnrm : 0.f;   // no flux through a land edge
     lp[e] = (float)((pi0 + pi1 + np0[e] + np1[e] + .5 * (np0[et] + np1[et] + np0[el] + np1[el])) * (1. / 6.));
   }
   const float2 pg = e_pres_grad<NP>(g, ipn, lp);
@@ -199,7 +201,7 @@ __global__ void __launch_bounds__(64 * COLS) __attribute__((amdgpu_waves_per_eu(
 // differences taken in FP64.
 template <int NP>
 __device__ __forceinline__ void e_edgvar3_col(const EnsGrid &g, const EnsState &s, int ipn, int base, int k) {
-  const int nvl = g.d.nvl, ncol = s.ncol, ntra = g.d.ntra;
+  const int nvl = g.d.nvl, ncol = s.ncol;
   const int col = base + ipn;
   float hf[NP][NP];
   int nb[NP];
@@ -225,7 +227,7 @@ __device__ __forceinline__ void e_edgvar3_col(const EnsGrid &g, const EnsState &
     for (int i = 0; i < NP; i++) EE(s.dp_edg, col, i, k) = out[i];
     if (NP < NPP) EE(s.dp_edg, col, NPP - 1, k) = 0.f;
   }
-  for (int f = 1; f < ntra; f++) {
+  for (int f = 1; f <= s.edg3_ntr; f++) {
     const float b0 = ETR(s.tr, f, col, k);
 #pragma unroll
     for (int j = 0; j < NP; j++) d[j] = ETR(s.tr, f, nb[j], k) - b0;
@@ -731,7 +733,7 @@ __device__ __forceinline__ void e_trcadv3_col(const EnsGrid &g, const EnsState &
     const float tdp = tdp0 + a1 * tt + a2 * tco + a3 * tcv;
     ETAB(s.trc_tdcy, t, nf, col, k) = tt;
     ETR(s.trdp, t, col, k) = tdp;
-    ETR(s.tr, t, col, k) = fmaxf(tmn, fminf(tmx, tdp * rdp));
+    if (col_wet(s.wet, col)) ETR(s.tr, t, col, k) = fmaxf(tmn, fminf(tmx, tdp * rdp));   // land keeps its tracers
   }
 }
 
@@ -785,6 +787,7 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
   float wu = 0.f, wv = 0.f;
 #pragma unroll
   for (int e = 0; e < NP; e++) {
+    if (!col_wet(s.wet, nb[e])) continue;          // no diffusion through a land edge
     const float wux = nu[e] + GF(grndspd, nb[e] - base), wvx = nv[e];
     const float c0 = FCS(ipn, e, 0), s0 = FSN(ipn, e, 0), c1 = FCS(ipn, e, 1), s1 = FSN(ipn, e, 1);
     const float uxy1 = c0 * wui + s0 * wvi, vxy1 = -s0 * wui + c0 * wvi;
@@ -840,6 +843,17 @@ __global__ void __launch_bounds__(64 * COLS) __attribute__((amdgpu_waves_per_eu(
   const bool live = k < nvl;
   const StepCtl c = load_ctl(s.ctl);
   const int base = blockIdx.y * nip, col = base + ipn;
+  if (!col_wet(s.wet, col)) {         // a land column keeps its winds
+    if (live) {
+      const int nf = c.nf - 1;
+      EC(s.u2, col, k) = EC(s.us, col, k);
+      EC(s.v2, col, k) = EC(s.vs, col, k);
+      EAB(s.u_tdcy, nf, col, k) = 0.f;
+      EAB(s.v_tdcy, nf, col, k) = 0.f;
+      EC(s.relvor, col, k) = 0.f;
+    }
+    return;
+  }
   float un = 0.f, vn = 0.f;
   if (live) {
     if (GNP(ipn) == 6) e_momtum_col<6>(g, s, c, ipn, base, k, un, vn);

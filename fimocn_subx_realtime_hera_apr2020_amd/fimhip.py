@@ -44,7 +44,9 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ens_set_flux_scheme", "fimhip_ens_set_class_b", "fimhip_ens_load_class_b", "fimhip_ens_sync", "fimhip_ens_field_size", "fimhip_ens_download",
            "fimhip_ens_globsum",
            "fimhip_ens_kernel_count", "fimhip_ens_kernel_name", "fimhip_ens_set_tile", "fimhip_ens_get_tile",
-           "fimhip_ens_autotune", "fimhip_ens_profile", "fimhip_ens_profile_report"]
+           "fimhip_ens_autotune", "fimhip_ens_profile", "fimhip_ens_profile_report",
+           "fimhip_ens_set_fused", "fimhip_ens_patches", "fimhip_ens_set_wet",
+           "fimhip_ens_step_together"]
 
 # real*4 drop-ins: name -> (number of float arrays, trailing int argument)
 R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, True), "mass_adv": (12, True),
@@ -134,6 +136,10 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ens_init_state.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_step.argtypes = [vp, ct.c_int, ct.c_int]
     lib.fimhip_ens_use_graph.argtypes = [vp, ct.c_int]
+    lib.fimhip_ens_set_fused.argtypes = [vp, ct.c_int]
+    lib.fimhip_ens_set_wet.argtypes = [vp, ct.c_int, _PI]
+    lib.fimhip_ens_step_together.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ct.c_int]
+    lib.fimhip_ens_patches.argtypes = [ct.POINTER(GridDesc), _PI, _PI, ct.c_longlong, _PI, ct.c_longlong]
     lib.fimhip_ens_set_flux_scheme.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_set_class_b.argtypes = [vp, ct.c_int, ct.c_int]
     lib.fimhip_ens_load_class_b.argtypes = [vp, ct.c_int, _PD]
@@ -239,6 +245,23 @@ def _grid_desc(grid: _glvl.Grid, nvl: int, ntra: int, dt: float, thetac, dpsig):
     for n in ("cs", "sn", "sidevec_c", "sidevec_e", "sideln", "hfcoef", "area", "lat", "lon", "thetac", "dpsig"):
         setattr(gd, n, keep[n].ctypes.data_as(_PD))
     return gd, keep
+
+
+def ens_patches(grid: _glvl.Grid, nvl: int = 64):
+    """The patch partition of the batched form's fused tracer transport:
+    (cnt, ent) with cnt (npatch, 4) the cumulative ends of each patch's core
+    and rings 1-3 in its entries, ent (nentries, 4) each entry's 0-based cell
+    and packed local neighbour indices."""
+    import numpy as _np
+    lib = load_library()
+    gd, keep = _grid_desc(grid, nvl, 1, 60.0, _np.zeros(nvl), _np.zeros(nvl))
+    sz = (ct.c_int * 2)()
+    _chk(lib.fimhip_ens_patches(ct.byref(gd), sz, None, 0, None, 0), "ens_patches")
+    cnt = _np.zeros((sz[0], 4), _np.int32)
+    ent = _np.zeros((sz[1], 4), _np.int32)
+    _chk(lib.fimhip_ens_patches(ct.byref(gd), sz, cnt.ctypes.data_as(_PI), cnt.size, ent.ctypes.data_as(_PI),
+                                ent.size), "ens_patches")
+    return cnt, ent
 
 
 class Instance:
@@ -424,6 +447,14 @@ class Model(Instance):
         return m.value, t
 
 
+def step_together(instances, nsteps: int = 1, mode: int = 0):
+    """nsteps steps of several Ensemble instances on one GPU, each step one
+    graph replay holding every instance's launches (synchronous)."""
+    lib = load_library()
+    arr = (ct.c_void_p * len(instances))(*[e.h.value for e in instances])
+    _chk(lib.fimhip_ens_step_together(arr, len(instances), nsteps, mode), "ens_step_together")
+
+
 class Ensemble:
     """Members advanced together in the batched mixed-precision form
     (fimhip_ens_*): FP32 winds, tracers and edge values, FP64 thickness,
@@ -466,6 +497,17 @@ class Ensemble:
 
     def step(self, nsteps: int = 1, mode: int = 0):
         _chk(self.lib.fimhip_ens_step(self.h, nsteps, mode), "ens_step")
+
+    def set_wet(self, member: int, wet):
+        """Land/sea mask of one member (nip values, 0 = land)."""
+        w = np.ascontiguousarray(wet, np.int32)
+        assert w.shape == (self.nip,)
+        _chk(self.lib.fimhip_ens_set_wet(self.h, member, w.ctypes.data_as(_PI)), "ens_set_wet")
+
+    def set_fused(self, on: bool):
+        """The tracer transport as one kernel over patches (True, the default)
+        or as one kernel per stage (False); both give the same bits."""
+        _chk(self.lib.fimhip_ens_set_fused(self.h, int(on)), "ens_set_fused")
 
     def use_graph(self, on: bool):
         _chk(self.lib.fimhip_ens_use_graph(self.h, 1 if on else 0), "ens_use_graph")

@@ -1,0 +1,162 @@
+! fimhip_bind: the Fortran side of the C ABI in include/fimhip.h, for a serial
+! FIM build (-DSERIAL, one GPU).  The bind(C) interfaces keep the argument
+! lists of the reference subroutines they replace:
+!   fimhip_edgvar1_r8 / fimhip_edgvar3_r8   edgvar1.F90:27, edgvar3.F90:25
+!   fimhip_cnuity_r8                        cnuity.F90:23
+!   fimhip_mass_adv_r8                      massadv.F90:21
+!   fimhip_trcadv_r8                        trcadv.F90:19
+!   fimhip_momtum_r8                        momtum.F90:16
+!   fimhip_hybgen_r8                        hybgen.F90:20
+!   fimhip_hystat_r8                        hystat.F90:23
+!   fimhip_transp0/1/2_r8                   transp3d.F90:22, 44, 81
+!   fimhip_abstart                          abstart.F90:14
+! plus instance creation (in place of GPUinit + copytoGPUinit,
+! dyn_init.F90:199, 1198) and the device-resident fast path.  Arrays are
+! passed as the reference passes them (explicit-shape, by reference); real*8
+! arrays bind to real(c_double).  Every function returns 0 or a negative
+! status; fimhip_check stops the program on an error, as dyn_init does on a
+! failed GPUinit (dyn_init.F90:199-203).
+module fimhip_bind
+  use iso_c_binding
+  implicit none
+  type(c_ptr), save :: fimhip_ctx = c_null_ptr
+
+  type, bind(C) :: fimhip_grid_desc
+    integer(c_int) :: nip, nvl, ntra
+    real(c_double) :: dt
+    type(c_ptr) :: nprox, prox, proxs, cs, sn, sidevec_c, sidevec_e, sideln, &
+                   hfcoef, area, lat, lon, thetac, dpsig
+  end type
+
+  interface
+    integer(c_int) function fimhip_create(grid, device, ctx) bind(C, name="fimhip_create")
+      import; type(fimhip_grid_desc), intent(in) :: grid
+      integer(c_int), value :: device; type(c_ptr), intent(out) :: ctx
+    end function
+    integer(c_int) function fimhip_destroy(ctx) bind(C, name="fimhip_destroy")
+      import; type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function fimhip_abstart(ctx, its, itsDFI, nf, of, vof, ab1, ab2, ab3) &
+        bind(C, name="fimhip_abstart")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its, itsDFI
+      integer(c_int) :: nf, of, vof; real(c_double) :: ab1, ab2, ab3
+    end function
+    integer(c_int) function fimhip_edgvar1_r8(ctx, its, u, v, delp, pres, exner, geop, montg, tracr, &
+        u_edg, v_edg, dp_edg, lp_edg, geop_edg, mont_edg, uvsq_edg, trc_edg) &
+        bind(C, name="fimhip_edgvar1_r8")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its
+      real(c_double), intent(in) :: u(*), v(*), delp(*), pres(*), exner(*), geop(*), montg(*), tracr(*)
+      real(c_double) :: u_edg(*), v_edg(*), dp_edg(*), lp_edg(*), geop_edg(*), mont_edg(*), &
+                        uvsq_edg(*), trc_edg(*)
+    end function
+    integer(c_int) function fimhip_edgvar3_r8(ctx, its, u, v, delp, pres, exner, geop, montg, tracr, &
+        u_edg, v_edg, dp_edg, lp_edg, geop_edg, mont_edg, uvsq_edg, trc_edg) &
+        bind(C, name="fimhip_edgvar3_r8")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its
+      real(c_double), intent(in) :: u(*), v(*), delp(*), pres(*), exner(*), geop(*), montg(*), tracr(*)
+      real(c_double) :: u_edg(*), v_edg(*), dp_edg(*), lp_edg(*), geop_edg(*), mont_edg(*), &
+                        uvsq_edg(*), trc_edg(*)
+    end function
+    integer(c_int) function fimhip_cnuity_r8(ctx, its, u_velo, v_velo, u_edg, v_edg, dp_edg, lp_edg, &
+        delp, pres, exner, dp_tndcy, dplo_tndcy, massfx, omega, TimingBarriers) &
+        bind(C, name="fimhip_cnuity_r8")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its, TimingBarriers
+      real(c_double), intent(in) :: u_velo(*), v_velo(*), u_edg(*), v_edg(*), dp_edg(*), lp_edg(*)
+      real(c_double) :: delp(*), pres(*), exner(*), dp_tndcy(*), dplo_tndcy(*), massfx(*), omega(*)
+    end function
+    integer(c_int) function fimhip_mass_adv_r8(ctx, its, u_velo, v_velo, u_edg, v_edg, dp_edg, lp_edg, &
+        delp, pres, exner, dp_tndcy, massfx, omega, TimingBarriers) bind(C, name="fimhip_mass_adv_r8")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its, TimingBarriers
+      real(c_double), intent(in) :: u_velo(*), v_velo(*), u_edg(*), v_edg(*), lp_edg(*)
+      real(c_double) :: dp_edg(*), delp(*), pres(*), exner(*), dp_tndcy(*), massfx(*), omega(*)
+    end function
+    integer(c_int) function fimhip_set_flux_scheme(ctx, flux_conserv_schm) &
+        bind(C, name="fimhip_set_flux_scheme")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: flux_conserv_schm
+    end function
+    integer(c_int) function fimhip_set_class_b(ctx, ntrb, dtratio) bind(C, name="fimhip_set_class_b")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: ntrb, dtratio
+    end function
+    integer(c_int) function fimhip_transp0_r8(ctx, its, cumufx, dp3d, dpinit) bind(C, name="fimhip_transp0_r8")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its
+      real(c_double), intent(in) :: dp3d(*); real(c_double) :: cumufx(*), dpinit(*)
+    end function
+    integer(c_int) function fimhip_transp1_r8(ctx, its, nf, of, vof, adbash1, adbash2, adbash3, &
+        cumufx, massfx) bind(C, name="fimhip_transp1_r8")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its, nf, of, vof
+      real(c_double), value :: adbash1, adbash2, adbash3
+      real(c_double) :: cumufx(*); real(c_double), intent(in) :: massfx(*)
+    end function
+    integer(c_int) function fimhip_transp2_r8(ctx, its, tracr, cumufx, dpinit, dpfinl, TimingBarriers) &
+        bind(C, name="fimhip_transp2_r8")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its, TimingBarriers
+      real(c_double) :: tracr(*); real(c_double), intent(in) :: cumufx(*), dpinit(*), dpfinl(*)
+    end function
+    integer(c_int) function fimhip_trcadv_r8(ctx, its, trcr_edg, tracr, trcdp, trc_tdcy, trclo_tdcy, &
+        massfx, delp, TimingBarriers) bind(C, name="fimhip_trcadv_r8")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its, TimingBarriers
+      real(c_double), intent(in) :: trcr_edg(*), massfx(*), delp(*)
+      real(c_double) :: tracr(*), trcdp(*), trc_tdcy(*), trclo_tdcy(*)
+    end function
+    integer(c_int) function fimhip_momtum_r8(ctx, its, u_velo, v_velo, exner, relvor, u_edg, v_edg, &
+        trcr_edg, geop_edg, mont_edg, uvsq_edg, u_tndcy, v_tndcy, dp3d) bind(C, name="fimhip_momtum_r8")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its
+      real(c_double), intent(in) :: exner(*), u_edg(*), v_edg(*), trcr_edg(*), geop_edg(*), &
+                                    mont_edg(*), uvsq_edg(*), dp3d(*)
+      real(c_double) :: u_velo(*), v_velo(*), relvor(*), u_tndcy(*), v_tndcy(*)
+    end function
+    integer(c_int) function fimhip_hybgen_r8(ctx, its, targt, us3d, vs3d, tr3d, sdot, ex3d, dp3d, pr3d, &
+        TimingBarriers) bind(C, name="fimhip_hybgen_r8")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its, TimingBarriers
+      real(c_double), intent(in) :: targt(*)
+      real(c_double) :: us3d(*), vs3d(*), tr3d(*), sdot(*), ex3d(*), dp3d(*), pr3d(*)
+    end function
+    integer(c_int) function fimhip_hystat_r8(ctx, its, ph3d, ex3d, mp3d, dp3d, tr3d, trdp, psrf, ptdcy) &
+        bind(C, name="fimhip_hystat_r8")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its
+      real(c_double), intent(in) :: ex3d(*), dp3d(*)
+      real(c_double) :: ph3d(*), mp3d(*), tr3d(*), trdp(*), psrf(*), ptdcy(*)
+    end function
+    ! device-resident fast path
+    integer(c_int) function fimhip_upload(ctx, field, host) bind(C, name="fimhip_upload")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: field
+      real(c_double), intent(in) :: host(*)
+    end function
+    integer(c_int) function fimhip_download(ctx, field, host) bind(C, name="fimhip_download")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: field
+      real(c_double) :: host(*)
+    end function
+    integer(c_int) function fimhip_init_state(ctx, init_hybgen) bind(C, name="fimhip_init_state")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: init_hybgen
+    end function
+    integer(c_int) function fimhip_step(ctx, its, mode) bind(C, name="fimhip_step")
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: its, mode
+    end function
+    integer(c_int) function fimhip_sync(ctx) bind(C, name="fimhip_sync")
+      import; type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function fimhip_globsum(ctx, mass, trtot) bind(C, name="fimhip_globsum")
+      import; type(c_ptr), value :: ctx; real(c_double) :: mass, trtot(*)
+    end function
+    ! x**y as the device computes it
+    real(c_double) function fimhip_pow(x, y) bind(C, name="fimhip_pow")
+      import; real(c_double), value :: x, y
+    end function
+  end interface
+
+  ! field ids of fimhip_upload / fimhip_download (the FIMHIP_* enum)
+  integer(c_int), parameter :: FIMHIP_US = 0, FIMHIP_VS = 1, FIMHIP_DP = 2, FIMHIP_PR = 3, &
+    FIMHIP_EX = 4, FIMHIP_PH = 5, FIMHIP_MP = 6, FIMHIP_TR = 7, FIMHIP_TRDP = 8
+
+contains
+
+  subroutine fimhip_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    if (rc /= 0) then
+      write(*,'(a,a,a,i0)') 'fimhip: ', where, ' failed with status ', rc
+      stop 1
+    end if
+  end subroutine
+
+end module fimhip_bind

@@ -1,0 +1,114 @@
+"""The ocean instance of the uneven batch (G8 x 32 layers, temperature and
+salinity, a land mask), as a seeded state for the batched dynamics step.
+
+It is a stand-in that carries an ocean-shaped workload through FIM's own
+dynamics kernels with a land/sea mask, not the reference's HYCOM ocean
+(hycom_*.F90): the edge fluxes are gated by the mask as the ocean's
+continuity gates them (hycom_cnuity.F90:100-125), everything else is FIM's
+horizontal step.  Because FIM's column code (hybgen's inflate, ptop, thktop)
+is written for pressures near 1e5 Pa, the ocean column is mapped into that
+range: 4000 m of water is 1e5 - 50 Pa of the model's pressure coordinate
+(PA_PER_M Pa per metre), the top interface at 50 Pa as in the atmosphere.
+
+* layers (32, level 1 at the bottom as in FIM): each at least its minimum
+  thickness of FIMrun/ocn_dpmin.txt (2 m at the surface, 276 m at the
+  bottom), all of them stretched by one factor to fill the 4000 m column,
+  times (1 + 0.05 U[-1,1]);
+* temperature (tracer 1, which plays theta): U[270, 303] K over the column,
+  from a smooth surface field spread uniformly over [297, 303] K decaying
+  towards 270 K with depth (e-folding depth 800 m), so it rises
+  monotonically from the bottom up and varies little along the layers (a
+  horizontal contrast of tens of kelvin would drive winds of 100-200 m/s in
+  FIM's equations);
+* salinity (tracer 2): U[34, 36] psu per cell and layer;
+* currents: solid body 1 m/s about an axis tilted 45 degrees plus N(0, 0.1)
+  m/s noise;
+* land: a smooth random field on the sphere (a sum of 24 seeded Gaussian
+  bumps) below its 30th percentile.
+hybgen's target temperatures rise linearly from 270.5 K (bottom) to 302.5 K
+(top); its minimum thicknesses are the ocn_dpmin layers in the model's
+pressure units.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import constants as C
+
+DEPTH_M = 4000.0
+P_TOP = 50.0
+PA_PER_M = (1.0e5 - P_TOP) / DEPTH_M
+NVL = 32
+NTRA = 2
+
+
+def vertical_tables(nvl: int = NVL):
+    """hybgen's target temperature and minimum thickness (Pa of the model's
+    pressure coordinate) per layer, bottom layer first."""
+    assert nvl == len(C.OCN_DPMIN32)
+    thetac = np.linspace(270.5, 302.5, nvl)
+    dpsig = np.array(C.OCN_DPMIN32[::-1], np.float64) * PA_PER_M
+    return thetac, dpsig
+
+
+def smooth_field(lat, lon, seed: int) -> np.ndarray:
+    """A seeded smooth random field on the sphere: a sum of 24 Gaussian bumps
+    of random centre, sign and width."""
+    rng = np.random.default_rng(seed)
+    lat = np.asarray(lat, np.float64)
+    lon = np.asarray(lon, np.float64)
+    x = np.stack([np.cos(lat) * np.cos(lon), np.cos(lat) * np.sin(lon), np.sin(lat)], axis=1)
+    c = rng.normal(size=(24, 3))
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    w = rng.uniform(-1.0, 1.0, 24)
+    kappa = rng.uniform(2.0, 8.0, 24)
+    return (w[None, :] * np.exp(kappa[None, :] * (x @ c.T - 1.0))).sum(axis=1)
+
+
+def land_mask(lat, lon, seed: int = 20240605, frac: float = 0.30) -> np.ndarray:
+    """1 for ocean, 0 for land: a seeded smooth field below its frac quantile
+    is land."""
+    f = smooth_field(lat, lon, seed)
+    return (f >= np.quantile(f, frac)).astype(np.int32)
+
+
+def make_ocean_state(lat, lon, nvl: int = NVL, seed: int = 20240605) -> dict:
+    """Seeded ocean state (arrays as state.make_state: (nip, nvl),
+    (nip, nvl+1), tracers (2, nip, nvl)) and its land mask ('wet')."""
+    from . import fimhip
+    rng = np.random.default_rng(seed + 1)
+    lat = np.asarray(lat, np.float64)
+    lon = np.asarray(lon, np.float64)
+    nip = lat.shape[0]
+    dpmin_m = np.array(C.OCN_DPMIN32[::-1], np.float64)          # bottom layer first
+    stretch = DEPTH_M / dpmin_m.sum()
+    dz = dpmin_m[None, :] * stretch * (1.0 + 0.05 * rng.uniform(-1.0, 1.0, (nip, nvl)))
+    dz = np.maximum(dz, dpmin_m[None, :])
+    dp = dz * PA_PER_M
+    pr = np.empty((nip, nvl + 1))
+    pr[:, nvl] = P_TOP
+    for k in range(nvl - 1, -1, -1):
+        pr[:, k] = pr[:, k + 1] + dp[:, k]
+    ex = C.CP * (pr / C.P1000) ** (C.RD / C.CP)
+    # depth of each layer's centre below the surface (m)
+    ztop = np.cumsum(dz[:, ::-1], axis=1)[:, ::-1]                 # depth of each layer's bottom
+    zc = ztop - 0.5 * dz
+    # surface temperature: a smooth field, its values mapped by rank onto
+    # [297, 303] K (uniformly distributed, spatially smooth); with the decay
+    # with depth the column spans 270-303 K
+    f = smooth_field(lat, lon, seed + 2)
+    tsurf = 297.0 + 6.0 * (np.argsort(np.argsort(f)) + 0.5) / nip
+    temp = 270.0 + (tsurf[:, None] - 270.0) * np.exp(-zc / 800.0)
+    sal = rng.uniform(34.0, 36.0, (nip, nvl))
+    # currents: tilted solid body (as state.make_state's wind) scaled to 1 m/s, plus noise
+    alpha = np.pi / 4
+    u0 = 1.0 * (np.cos(lat) * np.cos(alpha) + np.sin(lat) * np.cos(lon) * np.sin(alpha))
+    v0 = -1.0 * np.sin(lon) * np.sin(alpha)
+    us = u0[:, None] + 0.1 * rng.normal(size=(nip, nvl))
+    vs = v0[:, None] + 0.1 * rng.normal(size=(nip, nvl))
+    ph = np.zeros((nip, nvl + 1))
+    tr = np.stack([temp, sal])
+    wet = land_mask(lat, lon, seed)
+    del fimhip
+    return dict(us=np.ascontiguousarray(us), vs=np.ascontiguousarray(vs), dp=np.ascontiguousarray(dp),
+                pr=pr, ex=ex, ph=ph, tr=np.ascontiguousarray(tr), wet=wet)

@@ -256,6 +256,33 @@ int fimhip_ens_init_state(fimhip_ens *ens, int init_hybgen);
  * captured once as a HIP graph and replayed (mode 0 full, 1 transport only) */
 int fimhip_ens_step(fimhip_ens *ens, int nsteps, int mode);
 int fimhip_ens_use_graph(fimhip_ens *ens, int on);
+/* the tracer transport of the step (trcadv with edgvar3's tracer
+ * reconstructions, trcadv_compute.F90:19-315, edgvar3.F90:65-163): on = one
+ * kernel over patches of cells with their halo rings staged and recomputed
+ * on chip (the default); off = one kernel per stage with the intermediates in
+ * device memory.  Both give the same bits. */
+int fimhip_ens_set_fused(fimhip_ens *ens, int on);
+/* land/sea mask of member 'member' (wet[nip], 0 = land), as the ocean's
+ * continuity gates its fluxes (hycom_cnuity.F90:100-125): an edge carries mass
+ * and tracer flux only if both of its cells are wet, the Laplacian momentum
+ * diffusion through a land edge is zero, and land columns keep their
+ * thickness, winds and tracers (no momentum update, no tracer update, no
+ * hybgen).  Without a call every column is wet. */
+int fimhip_ens_set_wet(fimhip_ens *ens, int member, const int *wet);
+/* nsteps steps of n instances on one GPU (e.g. an uneven batch: grids of
+ * different resolution or layer count, each its own instance), every step
+ * one replay of a graph that holds all instances' launches as parallel
+ * branches; synchronous.  All instances take the same mode (0 full, 1
+ * transport only) and each its own time step. */
+int fimhip_ens_step_together(fimhip_ens *const *list, int n, int nsteps, int mode);
+/* the patch partition behind the fused tracer transport, for inspection:
+ * sizes[0] = patches, sizes[1] = tile entries; cnt (4 ints per patch: the
+ * cumulative ends of the core and of neighbour rings 1, 2, 3 in the patch's
+ * entries) and ent (4 ints per entry: 0-based cell, then the local indices of
+ * the cell's six neighbours, two 16-bit fields per int, for rings 0-2) are
+ * written when given with room (len in ints).  Host only. */
+int fimhip_ens_patches(const fimhip_grid_desc *grid, int *sizes, int *cnt, long long cnt_len, int *ent,
+                       long long ent_len);
 /* flux_conserv_schm for the steps that follow (dyn_run.F90:377-386): 1 = FCT
  * cnuity (default), 2 = mass_adv (massadv.F90:21-160), with the edge's normal
  * wind taken from its lower-index cell so the FP32 mass fluxes still cancel */

@@ -104,6 +104,10 @@ void fo_abstart(fo_state *s, int its, double dt) {
   if (its == 3) { s->ab1 = dt * 23. / 12.; s->ab2 = -dt * 16. / 12.; s->ab3 = dt * 5. / 12.; }
 }
 
+/* the land/sea mask (NULL: every cell wet) */
+static inline int fo_wet(const fo_grid *g, int i) { return !g->wet || g->wet[i]; }
+static inline int fo_open(const fo_grid *g, int i, int x) { return fo_wet(g, i) && fo_wet(g, x); }
+
 /* ------------------------------------------------------------------------
  * edgvar1.F90:160-421 (eqwgt branch; janjic = 0; hiOrderFluxComp: no dp_edg)
  * and the tracer loop 425-511 for tracer 1 only */
@@ -232,7 +236,8 @@ void fo_cnuity(const fo_grid *g, fo_state *s) {
   for (int ipn = 0; ipn < nip; ipn++)
     for (int edg = 0; edg < g->nprox[ipn]; edg++)
       for (int k = 0; k < nvl; k++)
-        E3(vnorm, ipn, edg, k) = SVE(ipn, edg, 1) * E3(s->u_edg, ipn, edg, k) - SVE(ipn, edg, 0) * E3(s->v_edg, ipn, edg, k);
+        E3(vnorm, ipn, edg, k) = !fo_open(g, ipn, PROX(ipn, edg)) ? 0. :
+            SVE(ipn, edg, 1) * E3(s->u_edg, ipn, edg, k) - SVE(ipn, edg, 0) * E3(s->v_edg, ipn, edg, k);
   #pragma omp parallel for schedule(static)
   for (int ipn = 0; ipn < nip; ipn++) {
     for (int k = 0; k < nvl; k++) { E3(antifx, ipn, NPP - 1, k) = 0.; MF(massfx, nf, ipn, NPP - 1, k) = 0.; }
@@ -362,7 +367,8 @@ void fo_mass_adv(const fo_grid *g, fo_state *s) {
   for (int ipn = 0; ipn < nip; ipn++)
     for (int edg = 0; edg < g->nprox[ipn]; edg++)
       for (int k = 0; k < nvl; k++) {
-        const double vn = SVE(ipn, edg, 1) * E3(s->u_edg, ipn, edg, k) - SVE(ipn, edg, 0) * E3(s->v_edg, ipn, edg, k);
+        const double vn = !fo_open(g, ipn, PROX(ipn, edg)) ? 0. :
+            SVE(ipn, edg, 1) * E3(s->u_edg, ipn, edg, k) - SVE(ipn, edg, 0) * E3(s->v_edg, ipn, edg, k);
         E3(vnorm, ipn, edg, k) = vn;
         E3(c_num, ipn, edg, k) = fabs(vn) * dt * g->rarea[ipn];
       }
@@ -546,8 +552,9 @@ void fo_trcadv(const fo_grid *g, fo_state *s) {
         TAB(s->trc_tdcy, t, nf, ipn, k) = TAB(s->trl_tdcy, t, nf, ipn, k) + at;
         C2(trcdp, ipn, k) = C2(trcdp, ipn, k) + ab1 * TAB(s->trc_tdcy, t, nf, ipn, k) +
                             ab2 * TAB(s->trc_tdcy, t, of, ipn, k) + ab3 * TAB(s->trc_tdcy, t, vof, ipn, k);
-        C2(tracr, ipn, k) = dmax(C2(trmin, ipn, k), dmin(C2(trmax, ipn, k),
-                                 C2(trcdp, ipn, k) / dmax(thshld, C2(s->dp, ipn, k))));
+        if (fo_wet(g, ipn))      /* a land column keeps its tracers */
+          C2(tracr, ipn, k) = dmax(C2(trmin, ipn, k), dmin(C2(trmax, ipn, k),
+                                   C2(trcdp, ipn, k) / dmax(thshld, C2(s->dp, ipn, k))));
       }
     }
   }
@@ -580,8 +587,9 @@ void fo_momtum(const fo_grid *g, fo_state *s) {
         double vxy2 = -SN(ipn, edg, 1) * C2(worku, ipx, k) + CS(ipn, edg, 1) * C2(workv, ipx, k);
         double a = dmax(C2(s->dp, ipn, k), thshld), b = dmax(C2(s->dp, ipx, k), thshld);
         double hh = a * b / (a + b);
-        E3(ufx, ipn, edg, k) = (uxy2 - uxy1) * g->dfflen[k] * g->sideln[ipn * NPP + edg] * 2. * hh;
-        E3(vfx, ipn, edg, k) = (vxy2 - vxy1) * g->dfflen[k] * g->sideln[ipn * NPP + edg] * 2. * hh;
+        const int open = fo_open(g, ipn, ipx);     /* no diffusion through a land edge */
+        E3(ufx, ipn, edg, k) = open ? (uxy2 - uxy1) * g->dfflen[k] * g->sideln[ipn * NPP + edg] * 2. * hh : 0.;
+        E3(vfx, ipn, edg, k) = open ? (vxy2 - vxy1) * g->dfflen[k] * g->sideln[ipn * NPP + edg] * 2. * hh : 0.;
       }
     }
   /* dissip loop 2, then the increment */
@@ -596,6 +604,7 @@ void fo_momtum(const fo_grid *g, fo_state *s) {
         wu = wu + uflux * factor;
         wv = wv + vflux * factor;
       }
+      if (!fo_wet(g, ipn)) continue;      /* a land column keeps its winds */
       C2(s->us, ipn, k) = C2(s->us, ipn, k) + wu;
       C2(s->vs, ipn, k) = C2(s->vs, ipn, k) + wv;
     }
@@ -605,6 +614,14 @@ void fo_momtum(const fo_grid *g, fo_state *s) {
   const double *trc1 = s->trc_edg;
   #pragma omp parallel for schedule(static)
   for (int ipn = 0; ipn < nip; ipn++) {
+    if (!fo_wet(g, ipn)) {               /* a land column keeps its winds */
+      for (int k = 0; k < nvl; k++) {
+        C2(s->relvor, ipn, k) = 0.;
+        AB(s->u_tdcy, nf, ipn, k) = 0.;
+        AB(s->v_tdcy, nf, ipn, k) = 0.;
+      }
+      continue;
+    }
     for (int k = 0; k < nvl; k++) {
       double px = 0., py = 0., rv = 0.;
       double exm = .5 * (L2(s->ex, ipn, k) + L2(s->ex, ipn, k + 1));
@@ -938,7 +955,7 @@ void fo_hybgen(const fo_grid *g, fo_state *s, int its) {
     for (int ipn = 0; ipn < nip; ipn++) {
       for (int k = 0; k < nvl; k++) thcol[k] = TR(s->tr, 0, ipn, k);   /* worka = theta */
       for (int k = 0; k < nvlp1; k++) L2(exsmo, ipn, k) = L2(s->ex, ipn, k);
-      regrid_1d(g, its, g->thetac, thcol, &L2(exsmo, ipn, 0), L2(s->pr, ipn, 0), work);
+      if (fo_wet(g, ipn)) regrid_1d(g, its, g->thetac, thcol, &L2(exsmo, ipn, 0), L2(s->pr, ipn, 0), work);
     }
     free(work);
     free(thcol);
@@ -967,6 +984,10 @@ void fo_hybgen(const fo_grid *g, fo_state *s, int its) {
     double *work = (double *)malloc(sizeof(double) * (14 + 2 * ntra) * nvlp1);
     #pragma omp for schedule(static)
     for (int ipn = 0; ipn < nip; ipn++) {
+      if (!fo_wet(g, ipn)) {             /* a land column: no remap */
+        for (int k = 0; k < nvlp1; k++) L2(s->sdot, ipn, k) = 0.;
+        continue;
+      }
       remap_1d(g, its, g->thetac, ntra, &TR(s->tr, 0, ipn, 0), (size_t)nip * nvl, &C2(s->us, ipn, 0),
                &C2(s->vs, ipn, 0), &L2(s->ex, ipn, 0), &L2(exsmo, ipn, 0), &C2(s->dp, ipn, 0),
                &L2(s->pr, ipn, 0), work);

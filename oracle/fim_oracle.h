@@ -14,6 +14,12 @@ typedef struct {
   const double *cs, *sn, *sidevec_c, *sidevec_e, *sideln, *hfcoef;
   const double *area, *rarea, *corio, *grndspd;
   const double *dfflen, *smoo_coeff, *thetac, *dpsig;
+  /* land/sea mask (nip, 0 = land) or NULL: no counterpart in the reference's
+   * atmosphere.  An edge carries mass and tracer flux only if both cells are
+   * wet (the gating of hycom_cnuity.F90:100-125); momentum diffusion through a
+   * land edge is zero; land columns keep thickness, winds and tracers and
+   * get no hybgen (sdot = 0). */
+  const int *wet;
 } fo_grid;
 
 typedef struct {

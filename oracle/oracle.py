@@ -36,7 +36,7 @@ class _Grid(ct.Structure):
                 ("prox", _PI), ("proxs", _PI), ("nprox", _PI),
                 ("cs", _PD), ("sn", _PD), ("sidevec_c", _PD), ("sidevec_e", _PD), ("sideln", _PD),
                 ("hfcoef", _PD), ("area", _PD), ("rarea", _PD), ("corio", _PD), ("grndspd", _PD),
-                ("dfflen", _PD), ("smoo_coeff", _PD), ("thetac", _PD), ("dpsig", _PD)]
+                ("dfflen", _PD), ("smoo_coeff", _PD), ("thetac", _PD), ("dpsig", _PD), ("wet", _PI)]
 
 
 _STATE_ARRAYS = ["us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp",
@@ -156,6 +156,17 @@ class OracleRun:
         for n in _STATE_ARRAYS + ["trb", "cumufx", "dpinit"]:
             setattr(s, n, _pd(self.a[n]))
         self.s = s
+
+    def set_wet(self, wet):
+        """Land/sea mask (nip values, 0 = land), or None for every cell wet
+        (the reference's behaviour)."""
+        if wet is None:
+            self._wet = None
+            self.g.wet = None
+            return
+        self._wet = np.ascontiguousarray(np.asarray(wet) != 0, np.int32)
+        assert self._wet.shape == (self.nip,)
+        self.g.wet = _pi(self._wet)
 
     def load(self, st: dict):
         for n in ("us", "vs", "dp", "pr", "ex", "ph", "tr"):

@@ -1,0 +1,77 @@
+"""The Fortran side of the boundary: the bind(C) module a FIM build adds
+(fimocn_subx_realtime_hera_apr2020_amd/fortran/fimhip_bind.F90) and a
+Fortran program that calls the library through it as dyn_init and dyn_run
+would (tests/fortran/fimhip_caller.F90), both built by amdflang."""
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from helpers import golden_grid, seeded_case
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CALLER = os.path.join(ROOT, "tests", "fortran", "fimhip_caller")
+BIND = os.path.join(ROOT, "fimocn_subx_realtime_hera_apr2020_amd", "fortran", "fimhip_bind.F90")
+
+needs_caller = pytest.mark.skipif(not os.path.exists(CALLER), reason="Fortran driver not built (no amdflang)")
+
+
+def test_fortran_interfaces_bind_declared_symbols():
+    from test_library import declared_symbols
+    names = re.findall(r'bind\(C,\s*name="([a-z0-9_]+)"\)', open(BIND).read())
+    assert len(names) >= 23
+    assert set(names) <= set(declared_symbols())
+
+
+@needs_caller
+def test_fortran_caller_rejects_bad_arguments():
+    r = subprocess.run([CALLER, "--reject"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "reject:  -1  -1  -1  -1  -1" in r.stdout
+
+
+def write_case(path, g, st, th, ds, dt, nsteps, init_hybgen=True):
+    """The case file fimhip_caller reads (layout in its header comment)."""
+    nip, nvl = st["dp"].shape
+    ntra = st["tr"].shape[0]
+    with open(path, "wb") as f:
+        f.write(struct.pack("<6i", 1179208498, nip, nvl, ntra, nsteps, 1 if init_hybgen else 0))
+        f.write(struct.pack("<d", dt))
+        for a in (g.nprox, g.prox, g.proxs):
+            f.write(np.ascontiguousarray(a, "<i4").tobytes())
+        for a in (g.cs, g.sn, g.sidevec_c, g.sidevec_e, g.sideln, g.hfcoef, g.area, g.lat, g.lon, th, ds,
+                  st["us"], st["vs"], st["dp"], st["pr"], st["ex"], st["ph"], st["tr"]):
+            f.write(np.ascontiguousarray(a, "<f8").tobytes())
+
+
+@pytest.mark.gpu
+@needs_caller
+@pytest.mark.skipif(not gpu_available(), reason="needs a GPU")
+def test_fortran_caller_steps_equal_checker(tmp_path, oracle_lib):
+    """Start-up and 4 full steps through the _r8 wrappers from Fortran: every
+    field has the checker's bytes (dyn_run.F90:329-423 order)."""
+    O = oracle_lib
+    g, st, th, ds = seeded_case(3)
+    dt, nsteps = 240.0, 4
+    case, out = str(tmp_path / "case.bin"), str(tmp_path / "out.bin")
+    write_case(case, g, st, th, ds, dt, nsteps)
+    print("case: Fortran caller, G3 x 64, %d full steps" % nsteps, flush=True)
+    r = subprocess.run([CALLER, case, out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = O.OracleRun(O.grid_arrays(g, 64, dt, th, ds), 64, 4, dt, pow_kind=1)
+    o.load(st)
+    o.init(True)
+    for its in range(1, nsteps + 1):
+        o.step(its, 0)
+    buf = np.fromfile(out, "<f8")
+    off = 0
+    for n in ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "trdp"):
+        a = o.a[n]
+        got = buf[off:off + a.size].reshape(a.shape)
+        off += a.size
+        assert np.array_equal(got, a), (n, float(np.abs(got - a).max()))
+    assert off == buf.size

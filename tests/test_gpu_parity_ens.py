@@ -92,8 +92,11 @@ def test_autotune_keeps_results():
     e = _ens(g, th, ds, 2, [0, 1])
     try:
         t = e.autotune(0, 1)
-        ran = ~np.isnan(t).all(axis=1)        # the mass_adv kernels do not run in an FCT step
-        assert ran.sum() == len(e.kernel_names()) - 2
+        ran = ~np.isnan(t).all(axis=1)
+        # not in an FCT step with the fused tracer transport: the mass_adv
+        # kernels and the three stage kernels of the tracer transport
+        idle = {"e_massadv1", "e_massadv2", "e_trcadv1", "e_trcadv2", "e_trcadv3"}
+        assert [n for n, r in zip(e.kernel_names(), ran) if not r] == [n for n in e.kernel_names() if n in idle]
         assert np.isfinite(t[ran]).all() and (t[ran] > 0).all()
         tiles = e.tiles()
         assert all(0 <= v < 8 for v in tiles.values())
@@ -533,3 +536,46 @@ def test_class_b_fp32_form_against_fp64():
     finally:
         e.close()
         p.close()
+
+
+@pytest.mark.parametrize("level, nvl, members, steps, mode, scheme, graph", [
+    (5, 64, 3, 7, 0, 1, True),      # odd step count: the tracer buffers end swapped
+    (5, 64, 2, 6, 1, 1, False),
+    (4, 64, 2, 5, 0, 2, True),      # mass_adv
+    (1, 64, 2, 9, 0, 1, True),      # 42 cells, 12 of them pentagons
+    (3, 32, 2, 6, 0, 1, False),     # half the lanes of every column idle
+])
+def test_fused_tracer_fct_equals_stage_chain(level, nvl, members, steps, mode, scheme, graph):
+    """The tracer transport as one kernel over patches with recomputed halo
+    rings gives the bytes of the three-kernel chain with edgvar3's tracer
+    reconstructions in device memory, for every field of every member."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, state
+    g = fimhip.icos_grid(level)
+    th, ds = state.vertical_tables(nvl)
+    out = []
+    for fused in (False, True):
+        print("case: G%d x %d, %d members, %d steps, mode %d, scheme %d, graph %s, fused %s"
+              % (level, nvl, members, steps, mode, scheme, graph, fused), flush=True)
+        e = fimhip.Ensemble(g, members, nvl, NTRA, DT, th, ds)
+        try:
+            e.set_fused(fused)
+            e.set_flux_scheme(scheme)
+            e.use_graph(graph)
+            for m in range(members):
+                e.synth(m, seed=20240601, synth_member=m, flags=FLAGS)
+            e.init_state(True)
+            if graph:
+                e.step(steps, mode)
+            else:
+                for _ in range(steps):
+                    e.step(1, mode)
+            e.sync()
+            out.append([_fields(e, m) for m in range(members)])
+            out[-1].append(e.globsum())
+        finally:
+            e.close()
+    for m in range(members):
+        for n in out[0][m]:
+            assert np.array_equal(out[0][m][n], out[1][m][n]), (m, n)
+    for a, b in zip(out[0][members], out[1][members]):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
