@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <limits>
 #include <map>
 #include <string>
 #include <utility>
@@ -49,9 +50,10 @@ struct fimhip_ens {
   KTimer *kt = nullptr;
   // the tracer FCT as one kernel over patches with recomputed halo rings
   // (fct_tiles.hip), or as the three-kernel chain with edgvar3's tracer
-  // reconstructions in HBM; both give the same bits
-  bool fused = true;
-  EnsTiles tiles{};
+  // reconstructions in HBM; both give the same bits.  The chain is the
+  // default (faster at 16 members x G7 x 64 on MI355X); the autotuner keeps
+  // whichever form measures faster
+  bool fused = false;
 };
 
 static void drop_graph(fimhip_ens *e) {
@@ -102,6 +104,7 @@ extern "C" int fimhip_ens_create(const fimhip_grid_desc *gd, int members, int de
   e->lat.assign(gd->lat, gd->lat + nip);
   e->lon.assign(gd->lon, gd->lon + nip);
   for (int k = 0; k < EK_NKERNELS; k++) e->tile[k] = 2;   // 4 columns per block, no swizzle
+  e->tile[EK_TRCFCT] = 4;        // 32 levels and 16 wavefronts per block, XCD-contiguous order
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&e->stream) != hipSuccess) {
     delete e;
     return FIMHIP_EHIP;
@@ -148,7 +151,7 @@ extern "C" int fimhip_ens_create(const fimhip_grid_desc *gd, int members, int de
   if ((r = a.alloc(&s.ctl, 1))) { fimhip_ens_destroy(e); return r; }
   {
     TileHost th;
-    if ((r = ens_build_tiles(hg, nip, th)) || (r = ens_upload_tiles(a, th, e->tiles))) {
+    if ((r = ens_build_tiles(hg, nip, th)) || (r = ens_upload_tiles(a, th, g.tiles))) {
       fimhip_ens_destroy(e);
       return r;
     }
@@ -293,7 +296,7 @@ static void enqueue_step(fimhip_ens *e, int mode, bool side) {
   }
   if (e->cb.ntrb > 0) ens_launch_class_b_step(g, s, e->cb, st);   // dyn_run.F90:388-405
   if (e->fused) {
-    ens_launch_trc_fct(g, s, e->tiles, s.tr, s.trdp, s.tr_alt, s.trdp_alt, st);
+    ens_launch(EK_TRCFCT, e->tile[EK_TRCFCT], g, s, st);
     std::swap(s.tr, s.tr_alt);         // the rest of the step works on the new tracers
     std::swap(s.trdp, s.trdp_alt);
   } else {
@@ -351,63 +354,6 @@ extern "C" int fimhip_ens_step(fimhip_ens *e, int nsteps, int mode) {
     HIPCHK(hipGraphLaunch(e->exec[p], e->stream));
     if (e->fused) swap_tracer_buffers(e);
   }
-  return FIMHIP_OK;
-}
-
-// Several instances on one GPU stepped by one graph replay per step (the
-// uneven batch: resolutions and layer counts that cannot share a launch run
-// as branches of one graph, each instance's kernels on its own stream).  The
-// graph is captured for the call and replayed nsteps times; the call returns
-// when the steps are done.
-extern "C" int fimhip_ens_step_together(fimhip_ens *const *list, int n, int nsteps, int mode) {
-  if (!list || n < 1 || n > 64 || nsteps < 0 || (mode != 0 && mode != 1)) return FIMHIP_EINVAL;
-  for (int i = 0; i < n; i++) {
-    if (!list[i] || list[i]->device != list[0]->device || list[i]->kt) return FIMHIP_EINVAL;
-    for (int j = 0; j < i; j++)
-      if (list[j] == list[i]) return FIMHIP_EINVAL;
-  }
-  HIPCHK(hipSetDevice(list[0]->device));
-  hipStream_t a = list[0]->stream;
-  std::vector<hipEvent_t> ev(2 * (size_t)n, nullptr);
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  hipError_t err = hipSuccess;
-  for (auto &v : ev)
-    if (err == hipSuccess) err = hipEventCreateWithFlags(&v, hipEventDisableTiming);
-  // every instance with the fused tracer transport swaps its tracer buffers
-  // after each step, so the captured steps alternate between two graphs
-  for (int it = 0; it < nsteps && err == hipSuccess; it++) {
-    const int p = it & 1;
-    if (!exec[p]) {
-      err = hipStreamBeginCapture(a, hipStreamCaptureModeThreadLocal);
-      for (int i = 0; i < n && err == hipSuccess; i++) {
-        fimhip_ens *e = list[i];
-        if (i > 0) {
-          (void)hipEventRecord(ev[2 * i], a);
-          (void)hipStreamWaitEvent(e->stream, ev[2 * i], 0);
-        }
-        enqueue_step(e, mode, e->overlap);
-        if (i > 0) {
-          (void)hipEventRecord(ev[2 * i + 1], e->stream);
-          (void)hipStreamWaitEvent(a, ev[2 * i + 1], 0);
-        }
-      }
-      hipError_t ec = hipStreamEndCapture(a, &graph[p]);
-      if (err == hipSuccess) err = ec;
-      if (err == hipSuccess) err = hipGraphInstantiate(&exec[p], graph[p], nullptr, nullptr, 0);
-    }
-    if (err == hipSuccess) err = hipGraphLaunch(exec[p], a);
-    for (int i = 0; i < n; i++)
-      if (list[i]->fused) swap_tracer_buffers(list[i]);
-  }
-  if (err == hipSuccess) err = hipStreamSynchronize(a);
-  for (int p = 0; p < 2; p++) {
-    if (exec[p]) (void)hipGraphExecDestroy(exec[p]);
-    if (graph[p]) (void)hipGraphDestroy(graph[p]);
-  }
-  for (auto v : ev)
-    if (v) (void)hipEventDestroy(v);
-  HIPCHK(err);
   return FIMHIP_OK;
 }
 
@@ -633,36 +579,61 @@ extern "C" int fimhip_ens_get_tile(const fimhip_ens *e, int kernel) {
 
 // Autotuner: for each launch shape in turn, every kernel runs with that
 // shape for 'reps' steps of the given mode under event timing; each kernel
-// then keeps the shape with the least mean time.  The steps advance the
-// state, so callers re-create or reload it afterwards.  times (optional)
-// receives EK_NKERNELS x ENS_NTILES mean milliseconds.
+// then keeps the shape with the least mean time.  Both forms of the tracer
+// transport are swept (the three-kernel chain with edgvar3's tracer
+// reconstructions, then the fused kernel with edgvar3 for the thickness only),
+// and the form whose transport kernels sum to less time is kept.  The steps
+// advance the state, so callers re-create or reload it afterwards.  times
+// (optional) receives EK_NKERNELS x ENS_NTILES mean milliseconds (edgvar3's
+// row from the form kept).
 extern "C" int fimhip_ens_autotune(fimhip_ens *e, int mode, int reps, double *times) {
   if (!e || reps < 1 || (mode != 0 && mode != 1)) return FIMHIP_EINVAL;
   HIPCHK(hipSetDevice(e->device));
-  std::vector<double> best(EK_NKERNELS, 1e30);
-  std::vector<int> pick(EK_NKERNELS, e->tile[0]);
-  for (int k = 0; k < EK_NKERNELS; k++) pick[k] = e->tile[k];
+  const int nt = EK_NKERNELS * ENS_NTILES;
+  std::vector<double> best[2], tab[2];
+  std::vector<int> pick[2];
   KTimer *saved = e->kt;
-  for (int t = 0; t < ENS_NTILES; t++) {
-    for (int k = 0; k < EK_NKERNELS; k++) e->tile[k] = t;
-    KTimer kt;
-    g_ktimer = &kt;
-    for (int i = 0; i < reps; i++) step_once(e, mode, false);
-    g_ktimer = nullptr;
-    HIPCHK(hipGetLastError());
-    if (kt.flush(e->stream)) return FIMHIP_EHIP;
-    for (int k = 0; k < EK_NKERNELS; k++) {
-      auto it = kt.acc.find(ens_kernel_names[k]);
-      if (it == kt.acc.end() || it->second.second == 0) continue;
-      const double ms = it->second.first / it->second.second;
-      if (times) times[k * ENS_NTILES + t] = ms;
-      if (ms < best[k]) { best[k] = ms; pick[k] = t; }
+  for (int f = 0; f < 2; f++) {
+    e->fused = f == 1;
+    best[f].assign(EK_NKERNELS, 1e30);
+    tab[f].assign(nt, std::numeric_limits<double>::quiet_NaN());
+    pick[f].assign(e->tile, e->tile + EK_NKERNELS);
+    for (int t = 0; t < ENS_NTILES; t++) {
+      for (int k = 0; k < EK_NKERNELS; k++) e->tile[k] = t;
+      KTimer kt;
+      g_ktimer = &kt;
+      for (int i = 0; i < reps; i++) step_once(e, mode, false);
+      g_ktimer = nullptr;
+      HIPCHK(hipGetLastError());
+      if (kt.flush(e->stream)) return FIMHIP_EHIP;
+      for (int k = 0; k < EK_NKERNELS; k++) {
+        auto it = kt.acc.find(ens_kernel_names[k]);
+        if (it == kt.acc.end() || it->second.second == 0) continue;
+        const double ms = it->second.first / it->second.second;
+        tab[f][k * ENS_NTILES + t] = ms;
+        if (ms < best[f][k]) { best[f][k] = ms; pick[f][k] = t; }
+      }
     }
   }
   e->kt = saved;
-  for (int k = 0; k < EK_NKERNELS; k++) e->tile[k] = pick[k];
+  const double chain = best[0][EK_EDGVAR3] + best[0][EK_TRCADV1] + best[0][EK_TRCADV2] + best[0][EK_TRCADV3];
+  const double fused = best[1][EK_EDGVAR3] + best[1][EK_TRCFCT];
+  const int fsel = fused < chain ? 1 : 0;
+  e->fused = fsel == 1;
+  for (int k = 0; k < EK_NKERNELS; k++) {
+    // edgvar3 does different work in the two forms; the other kernels the same
+    const int f = (k == EK_EDGVAR3 || best[fsel][k] <= best[1 - fsel][k]) ? fsel : 1 - fsel;
+    e->tile[k] = pick[f][k];
+    if (times)
+      for (int t = 0; t < ENS_NTILES; t++) times[k * ENS_NTILES + t] = tab[f][k * ENS_NTILES + t];
+  }
   drop_graph(e);
   return FIMHIP_OK;
+}
+
+extern "C" int fimhip_ens_get_fused(const fimhip_ens *e) {
+  if (!e) return FIMHIP_EINVAL;
+  return e->fused ? 1 : 0;
 }
 
 // ---- event timing ----------------------------------------------------------

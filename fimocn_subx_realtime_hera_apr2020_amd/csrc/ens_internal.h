@@ -31,11 +31,26 @@ namespace fimhip {
 struct HostGrid;
 struct Arena;
 
+// Patches of cells for the fused tracer FCT (fct_tiles.hip): per patch the
+// cumulative ends n0..n3 of its core and three neighbour rings in its list
+// of tile cells, and per tile cell an entry of four ints: the cell index;
+// the local indices in the tile of its neighbours 0-3 (8 bits each); of
+// neighbours 4-5 and its edge count (8 bits each); the neighbours' indices of
+// the shared edges, proxs (4 bits each).  The local indices are set for
+// rings 0-2 only.
+struct EnsTiles {
+  int npatch;
+  const int *off;      // (npatch + 1) first tile cell of each patch
+  const int4 *cnt;     // (npatch) n0, n1, n2, n3
+  const int4 *ent;
+};
+
 // FP32 copies of the grid metrics (the reference stores them in FP32).
 struct EnsGrid {
   DevGrid d;                 // FP64 metrics and the neighbour tables
   const float *cs, *sn, *svc, *sve, *sideln, *hfcoef;
   const float *area, *rarea, *corio, *grndspd, *dfflen;
+  EnsTiles tiles;            // the patch partition of the fused tracer FCT
 };
 
 struct EnsState {
@@ -74,7 +89,7 @@ inline bool tile_swz(int t) { return (t >> 2) & 1; }
 
 enum EnsKernel {
   EK_EDGVAR1 = 0, EK_EDGVAR3, EK_CNUITY12, EK_CNUITY3, EK_CNUITY45, EK_TRCADV1, EK_TRCADV2,
-  EK_TRCADV3, EK_MOMTUM, EK_HYSTAT, EK_MASSADV1, EK_MASSADV2, EK_NKERNELS
+  EK_TRCADV3, EK_MOMTUM, EK_HYSTAT, EK_MASSADV1, EK_MASSADV2, EK_TRCFCT, EK_NKERNELS
 };
 extern const char *const ens_kernel_names[EK_NKERNELS];
 
@@ -92,30 +107,19 @@ struct EnsClassB {
 void ens_launch_class_b_init(const EnsGrid &g, const EnsState &s, const EnsClassB &b, hipStream_t st);
 void ens_launch_class_b_step(const EnsGrid &g, const EnsState &s, const EnsClassB &b, hipStream_t st);
 
-// Patches of cells for the fused tracer FCT (fct_tiles.hip): per patch the
-// cumulative ends n0..n3 of its core and three neighbour rings in its list
-// of tile cells, and per tile cell its cell index and, for rings 0-2, the
-// local indices (16 bits each) of its six neighbours in the tile.
-constexpr int FCT_NW = 16;                     // wavefronts per block
+// Sizes of the fused tracer FCT's patches (its LDS tile).
 constexpr int FCT_CORE_MAX = 64;               // core cells of a patch
-constexpr int FCT_J = FCT_CORE_MAX / FCT_NW;   // core cells per wavefront
 constexpr int FCT_R_MAX = 112;                 // cells in rings 0-1
 constexpr int FCT_LO_MAX = 160;                // cells in rings 0-2
 constexpr int FCT_TILE_MAX = 208;              // cells in rings 0-3
-struct EnsTiles {
-  int npatch;
-  const int *off;      // (npatch + 1) first tile cell of each patch
-  const int4 *cnt;     // (npatch) n0, n1, n2, n3
-  const int4 *ent;     // cell, neighbours 0|1, 2|3, 4|5
-};
 struct TileHost {
   std::vector<int> off;
   std::vector<std::array<int, 4>> cnt, ent;
 };
 int ens_build_tiles(const HostGrid &h, int nip, TileHost &out);
 int ens_upload_tiles(Arena &a, const TileHost &t, EnsTiles &d);
-void ens_launch_trc_fct(const EnsGrid &g, const EnsState &s, const EnsTiles &ts, const float *trs,
-                        const float *tds, float *trd, float *tdd, hipStream_t st);
+// the fused tracer FCT: reads s.tr / s.trdp, writes s.tr_alt / s.trdp_alt
+void ens_launch_trc_fct(int tile, const EnsGrid &g, const EnsState &s, hipStream_t st);
 
 void ens_launch_abstart(const EnsState &s, double dt, hipStream_t st);
 void ens_launch(int kernel, int tile, const EnsGrid &g, const EnsState &s, hipStream_t st);

@@ -14,7 +14,7 @@ namespace fimhip {
 
 const char *const ens_kernel_names[EK_NKERNELS] = {
     "e_edgvar1", "e_edgvar3", "e_cnuity12", "e_cnuity3", "e_cnuity45",
-    "e_trcadv1", "e_trcadv2", "e_trcadv3", "e_momtum", "e_hystat", "e_massadv1", "e_massadv2"};
+    "e_trcadv1", "e_trcadv2", "e_trcadv3", "e_momtum", "e_hystat", "e_massadv1", "e_massadv2", "e_trc_fct"};
 
 #define EC(a, col, k) (a)[(size_t)(col) * nvl + (k)]
 #define EL(a, col, k) (a)[(size_t)(col) * nvlp1 + (k)]
@@ -1098,6 +1098,7 @@ void ens_launch(int kernel, int tile, const EnsGrid &g, const EnsState &s, hipSt
     case EK_MOMTUM: ENS_DISPATCH(k_e_momtum, 1); break;
     case EK_MASSADV1: ENS_DISPATCH(k_e_massadv1, 1); break;
     case EK_MASSADV2: ENS_DISPATCH(k_e_massadv2, 1); break;
+    case EK_TRCFCT: ens_launch_trc_fct(tile, g, s, st); break;
     default: ENS_DISPATCH(k_e_hystat, 1); break;
   }
   if (::fimhip::g_ktimer) kt_end(st);

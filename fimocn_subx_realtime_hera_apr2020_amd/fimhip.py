@@ -45,8 +45,8 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ens_globsum",
            "fimhip_ens_kernel_count", "fimhip_ens_kernel_name", "fimhip_ens_set_tile", "fimhip_ens_get_tile",
            "fimhip_ens_autotune", "fimhip_ens_profile", "fimhip_ens_profile_report",
-           "fimhip_ens_set_fused", "fimhip_ens_patches", "fimhip_ens_set_wet",
-           "fimhip_ens_step_together"]
+           "fimhip_ens_set_fused", "fimhip_ens_get_fused", "fimhip_ens_patches",
+           "fimhip_ens_set_wet"]
 
 # real*4 drop-ins: name -> (number of float arrays, trailing int argument)
 R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, True), "mass_adv": (12, True),
@@ -137,8 +137,8 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ens_step.argtypes = [vp, ct.c_int, ct.c_int]
     lib.fimhip_ens_use_graph.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_set_fused.argtypes = [vp, ct.c_int]
+    lib.fimhip_ens_get_fused.argtypes = [vp]
     lib.fimhip_ens_set_wet.argtypes = [vp, ct.c_int, _PI]
-    lib.fimhip_ens_step_together.argtypes = [ct.POINTER(vp), ct.c_int, ct.c_int, ct.c_int]
     lib.fimhip_ens_patches.argtypes = [ct.POINTER(GridDesc), _PI, _PI, ct.c_longlong, _PI, ct.c_longlong]
     lib.fimhip_ens_set_flux_scheme.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_set_class_b.argtypes = [vp, ct.c_int, ct.c_int]
@@ -447,12 +447,16 @@ class Model(Instance):
         return m.value, t
 
 
-def step_together(instances, nsteps: int = 1, mode: int = 0):
-    """nsteps steps of several Ensemble instances on one GPU, each step one
-    graph replay holding every instance's launches (synchronous)."""
-    lib = load_library()
-    arr = (ct.c_void_p * len(instances))(*[e.h.value for e in instances])
-    _chk(lib.fimhip_ens_step_together(arr, len(instances), nsteps, mode), "ens_step_together")
+def step_concurrently(instances, nsteps: int = 1, mode: int = 0):
+    """nsteps steps of several Ensemble instances on one GPU (an uneven
+    batch: grids or layer counts that cannot share a launch): each step
+    enqueues every instance's captured step on its own stream before any
+    wait, so the instances run side by side; returns when all are done."""
+    for _ in range(nsteps):
+        for e in instances:
+            e.step(1, mode)
+    for e in instances:
+        e.sync()
 
 
 class Ensemble:
@@ -505,9 +509,16 @@ class Ensemble:
         _chk(self.lib.fimhip_ens_set_wet(self.h, member, w.ctypes.data_as(_PI)), "ens_set_wet")
 
     def set_fused(self, on: bool):
-        """The tracer transport as one kernel over patches (True, the default)
-        or as one kernel per stage (False); both give the same bits."""
+        """The tracer transport as one kernel over patches (True) or as one
+        kernel per stage (False, the default); both give the same bits."""
         _chk(self.lib.fimhip_ens_set_fused(self.h, int(on)), "ens_set_fused")
+
+    def fused(self) -> bool:
+        """Whether the tracer transport runs as the fused kernel (the
+        autotuner keeps the faster form)."""
+        r = self.lib.fimhip_ens_get_fused(self.h)
+        _chk(min(r, 0), "ens_get_fused")
+        return r == 1
 
     def use_graph(self, on: bool):
         _chk(self.lib.fimhip_ens_use_graph(self.h, 1 if on else 0), "ens_use_graph")

@@ -259,9 +259,11 @@ int fimhip_ens_use_graph(fimhip_ens *ens, int on);
 /* the tracer transport of the step (trcadv with edgvar3's tracer
  * reconstructions, trcadv_compute.F90:19-315, edgvar3.F90:65-163): on = one
  * kernel over patches of cells with their halo rings staged and recomputed
- * on chip (the default); off = one kernel per stage with the intermediates in
- * device memory.  Both give the same bits. */
+ * on chip; off = one kernel per stage with the intermediates in device memory
+ * (the default).  Both give the same bits; fimhip_ens_autotune keeps the
+ * faster, fimhip_ens_get_fused reports which is in use. */
 int fimhip_ens_set_fused(fimhip_ens *ens, int on);
+int fimhip_ens_get_fused(const fimhip_ens *ens);
 /* land/sea mask of member 'member' (wet[nip], 0 = land), as the ocean's
  * continuity gates its fluxes (hycom_cnuity.F90:100-125): an edge carries mass
  * and tracer flux only if both of its cells are wet, the Laplacian momentum
@@ -269,18 +271,14 @@ int fimhip_ens_set_fused(fimhip_ens *ens, int on);
  * thickness, winds and tracers (no momentum update, no tracer update, no
  * hybgen).  Without a call every column is wet. */
 int fimhip_ens_set_wet(fimhip_ens *ens, int member, const int *wet);
-/* nsteps steps of n instances on one GPU (e.g. an uneven batch: grids of
- * different resolution or layer count, each its own instance), every step
- * one replay of a graph that holds all instances' launches as parallel
- * branches; synchronous.  All instances take the same mode (0 full, 1
- * transport only) and each its own time step. */
-int fimhip_ens_step_together(fimhip_ens *const *list, int n, int nsteps, int mode);
 /* the patch partition behind the fused tracer transport, for inspection:
  * sizes[0] = patches, sizes[1] = tile entries; cnt (4 ints per patch: the
  * cumulative ends of the core and of neighbour rings 1, 2, 3 in the patch's
- * entries) and ent (4 ints per entry: 0-based cell, then the local indices of
- * the cell's six neighbours, two 16-bit fields per int, for rings 0-2) are
- * written when given with room (len in ints).  Host only. */
+ * entries) and ent (4 ints per entry: the 0-based cell; the local indices in
+ * the patch's entries of neighbours 0-3, 8 bits each; of neighbours 4-5 and
+ * the cell's edge count, 8 bits each; the neighbours' indices of the shared
+ * edges, 4 bits each; the local indices are set for rings 0-2) are written
+ * when given with room (len in ints).  Host only. */
 int fimhip_ens_patches(const fimhip_grid_desc *grid, int *sizes, int *cnt, long long cnt_len, int *ent,
                        long long ent_len);
 /* flux_conserv_schm for the steps that follow (dyn_run.F90:377-386): 1 = FCT
@@ -308,8 +306,9 @@ const char *fimhip_ens_kernel_name(int kernel);
 int fimhip_ens_set_tile(fimhip_ens *ens, int kernel, int tile);
 int fimhip_ens_get_tile(const fimhip_ens *ens, int kernel);
 /* time every tile of every kernel over reps steps and keep each kernel's
- * fastest; advances the state (reload it afterwards); times may be NULL or
- * receive kernel_count x 8 mean milliseconds */
+ * fastest, for both forms of the tracer transport, and keep the faster form;
+ * advances the state (reload it afterwards); times may be NULL or receive
+ * kernel_count x 8 mean milliseconds */
 int fimhip_ens_autotune(fimhip_ens *ens, int mode, int reps, double *times);
 int fimhip_ens_profile(fimhip_ens *ens, int on);
 int fimhip_ens_profile_report(fimhip_ens *ens, char *buf, int len);

@@ -106,6 +106,9 @@ def test_masked_fp32_form_against_masked_checker():
         assert errs[n] < b, (n, errs[n], b)
 
 
+@pytest.mark.skip(reason="the entry that captured several instances into one graph "
+                         "(fimhip_ens_step_together) ended its first GPU run with a host segmentation fault "
+                         "whose cause was not found; the entry is taken out (see the design notes)")
 def test_atmosphere_and_ocean_in_one_graph_equal_each_alone():
     from fimocn_subx_realtime_hera_apr2020_amd import fimhip, glvl, state
     nsteps = 5
@@ -127,6 +130,41 @@ def test_atmosphere_and_ocean_in_one_graph_equal_each_alone():
                     atm.step(nsteps, 0)
                     oce.step(nsteps, 0)
                     atm.sync()
+                    oce.sync()
+                res.append((_fields(atm, 0), _fields(oce, 0)))
+            finally:
+                oce.close()
+        finally:
+            atm.close()
+    for i in range(2):
+        for n in res[0][i]:
+            assert np.array_equal(res[0][i][n], res[1][i][n]), (i, n)
+
+
+def test_atmosphere_and_ocean_stepped_side_by_side_equal_each_alone():
+    """The uneven batch stepped side by side (every instance's captured step
+    enqueued on its own stream before any wait) gives the bytes of each
+    instance stepped alone."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, glvl, state
+    nsteps = 5
+    res = []
+    for together in (False, True):
+        print("case: G2+G3 atmosphere x 64 and G3 ocean x 32 masked, %d steps, side by side %s" % (nsteps, together),
+              flush=True)
+        ga = glvl.concat([fimhip.icos_grid(2), fimhip.icos_grid(3)])
+        th, ds = state.vertical_tables(64)
+        atm = fimhip.Ensemble(ga, 1, 64, 4, 240.0, th, ds)
+        try:
+            atm.synth(0, seed=20240601, synth_member=0, flags=1)
+            atm.init_state(True)
+            _, oce, _ = _ocean(3)
+            try:
+                if together:
+                    fimhip.step_concurrently([atm, oce], nsteps, 0)
+                else:
+                    atm.step(nsteps, 0)
+                    atm.sync()
+                    oce.step(nsteps, 0)
                     oce.sync()
                 res.append((_fields(atm, 0), _fields(oce, 0)))
             finally:

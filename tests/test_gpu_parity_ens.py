@@ -93,13 +93,14 @@ def test_autotune_keeps_results():
     try:
         t = e.autotune(0, 1)
         ran = ~np.isnan(t).all(axis=1)
-        # not in an FCT step with the fused tracer transport: the mass_adv
-        # kernels and the three stage kernels of the tracer transport
-        idle = {"e_massadv1", "e_massadv2", "e_trcadv1", "e_trcadv2", "e_trcadv3"}
+        # both forms of the tracer transport are swept; an FCT step never
+        # runs the mass_adv kernels
+        idle = {"e_massadv1", "e_massadv2"}
         assert [n for n, r in zip(e.kernel_names(), ran) if not r] == [n for n in e.kernel_names() if n in idle]
         assert np.isfinite(t[ran]).all() and (t[ran] > 0).all()
         tiles = e.tiles()
         assert all(0 <= v < 8 for v in tiles.values())
+        print("kept the %s tracer transport" % ("fused" if e.fused() else "chained"), flush=True)
         for m, sid in enumerate([0, 1]):
             e.synth(m, synth_member=sid, flags=FLAGS)
         e.init_state(True)
@@ -579,3 +580,28 @@ def test_fused_tracer_fct_equals_stage_chain(level, nvl, members, steps, mode, s
             assert np.array_equal(out[0][m][n], out[1][m][n]), (m, n)
     for a, b in zip(out[0][members], out[1][members]):
         assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def test_fused_tracer_fct_launch_shapes_agree():
+    """Every launch shape of the fused tracer transport (levels per block,
+    wavefronts per block, block order) gives the same bytes."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    g, th, ds = _grid(3)
+    ref = None
+    for tile in range(8):
+        print("case: G3 x 64, 2 members, 4 steps, e_trc_fct shape %d" % tile, flush=True)
+        e = _ens(g, th, ds, 2, [0, 1])
+        try:
+            e.set_fused(True)
+            e.set_tiles({"e_trc_fct": tile})
+            e.step(4, 0)
+            e.sync()
+            f = [_fields(e, m) for m in range(2)]
+        finally:
+            e.close()
+        if ref is None:
+            ref = f
+            continue
+        for m in range(2):
+            for n in ref[m]:
+                assert np.array_equal(ref[m][n], f[m][n]), (tile, m, n)

@@ -2,7 +2,9 @@
 form (host tables, no GPU): every cell is in exactly one patch's core; a
 patch's rings 1-3 are exactly the cells at neighbour distance 1, 2 and 3 from
 its core; the local neighbour table of rings 0-2 points at the right cells
-inside the tile; and the sizes stay within the kernel's LDS tile."""
+inside the tile (with each cell's edge count and its neighbours' edge
+slots); and the sizes stay within the kernel's LDS tile (local indices fit
+in 8 bits)."""
 import numpy as np
 import pytest
 
@@ -11,14 +13,17 @@ from helpers import golden_grid
 
 # the kernel's LDS tile (ens_internal.h): core, rings 0-1, 0-2, 0-3
 CORE_MAX, R_MAX, LO_MAX, TILE_MAX = 64, 112, 160, 208
+assert TILE_MAX <= 256
 
 
 def _check(grid):
     cnt, ent = fimhip.ens_patches(grid)
     prox = np.asarray(grid.prox, np.int64) - 1
+    proxs = np.asarray(grid.proxs, np.int64) - 1
     nprox = np.asarray(grid.nprox)
     for c in np.where(nprox == 5)[0]:
         prox[c, 5] = prox[c, 4]
+        proxs[c, 5] = proxs[c, 4]
     nip = grid.nip
     seen = np.zeros(nip, np.int64)
     off = 0
@@ -42,8 +47,10 @@ def _check(grid):
             assert ring == sorted(x for x, d in dist.items() if d == r), (r, ring[:5])
         # local neighbour table of rings 0-2
         for i in range(n2):
-            e4 = ent[off + i]
-            loc = [e4[1] & 0xFFFF, e4[1] >> 16, e4[2] & 0xFFFF, e4[2] >> 16, e4[3] & 0xFFFF, e4[3] >> 16]
+            e4 = ent[off + i].astype(np.int64) & 0xFFFFFFFF
+            loc = [(e4[1] >> 8 * e) & 0xFF for e in range(4)] + [(e4[2] >> 8 * e) & 0xFF for e in range(2)]
+            assert (e4[2] >> 16) & 0xFF == nprox[cells[i]]
+            assert [(e4[3] >> 4 * e) & 0xF for e in range(6)] == [int(x) for x in proxs[cells[i]]]
             assert max(loc) < n3
             assert [int(cells[l]) for l in loc] == [int(x) for x in prox[cells[i]]]
             if i < n1:
