@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,8 @@ void kt_end(hipStream_t st) {
   g_ktimer->recs.back().b = e;
 }
 }  // namespace fimhip
+
+constexpr int GSUM_BLOCKS = 1024;   // blocks of the globsum kernels
 
 struct fimhip_ctx {
   int device = 0;
@@ -162,7 +165,7 @@ extern "C" int fimhip_create(const fimhip_grid_desc *gd, int device, fimhip_ctx 
       {&s.u2, cz}, {&s.v2, cz}, {&s.exsmo, lz}, {&s.thbuf, cz}};
   for (auto &a : arrs)
     if ((r = c->arena.alloc(a.p, a.n))) { fimhip_destroy(c); return r; }
-  c->gsum_blocks = 1024;
+  c->gsum_blocks = GSUM_BLOCKS;
   if ((r = c->arena.alloc(&c->gsum_partial, (size_t)c->gsum_blocks * (1 + ntra)))) { fimhip_destroy(c); return r; }
   if (hipStreamSynchronize(c->stream) != hipSuccess) { fimhip_destroy(c); return FIMHIP_EHIP; }
   *out = c;
@@ -758,32 +761,41 @@ extern "C" int fimhip_sync(fimhip_ctx *c) {
   return finish(c);
 }
 
-// globsum: fixed-order block partials (cells strided by the grid), summed on
-// the host in block order, so the result is reproducible run to run
+// globsum: fixed-order block partials (cells strided by the grid), summed in
+// block order, so the result is reproducible run to run.  One block's partial
+// of sum q: area*dp (q = 0) or area*dp*tr_q over its cells.
+
+__device__ __forceinline__ double gsum_block(const double *area, const double *dp, const double *tq, bool with_tr,
+                                             int nip, int nvl, double *sh) {
+  double acc = 0.;
+  for (int ipn = blockIdx.x * 256 + threadIdx.x; ipn < nip; ipn += gridDim.x * 256) {
+    double col = 0.;
+    for (int k = 0; k < nvl; k++) {
+      double v = area[ipn] * dp[(size_t)ipn * nvl + k];
+      if (with_tr) v = v * tq[(size_t)ipn * nvl + k];
+      col = col + v;
+    }
+    acc = acc + col;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
 __global__ void __launch_bounds__(256) k_globsum(DevGrid g, DevState s, const double *trb, int ntrb,
                                                  double *part) {
   __shared__ double sh[256];
   const int nip = g.nip, nvl = g.nvl, ntra = g.ntra;
   for (int q = 0; q <= ntra + ntrb; q++) {
     const double *tq = q > ntra ? trb + (size_t)(q - 1 - ntra) * nip * nvl : s.tr + (size_t)(q > 0 ? q - 1 : 0) * nip * nvl;
-    double acc = 0.;
-    for (int ipn = blockIdx.x * 256 + threadIdx.x; ipn < nip; ipn += gridDim.x * 256) {
-      double col = 0.;
-      for (int k = 0; k < nvl; k++) {
-        double v = g.area[ipn] * s.dp[(size_t)ipn * nvl + k];
-        if (q > 0) v = v * tq[(size_t)ipn * nvl + k];
-        col = col + v;
-      }
-      acc = acc + col;
-    }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = sh[0];
-    __syncthreads();
+    const double r = gsum_block(g.area, s.dp, tq, q > 0, nip, nvl, sh);
+    if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = r;
   }
 }
 
@@ -800,6 +812,73 @@ extern "C" int fimhip_globsum(fimhip_ctx *c, double *mass, double *trtot) {
     for (int b = 0; b < c->gsum_blocks; b++) acc += h[(size_t)q * c->gsum_blocks + b];
     if (q == 0) *mass = acc / GRVITY;
     else if (trtot) trtot[q - 1] = acc;
+  }
+  return FIMHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// globsum over caller-owned device arrays without any allocation: the block
+// partials go to a file-scope device array, `scratch` (each GPU holds its own
+// copy at its own address), and one thread per sum adds them in block order,
+// as fimhip_globsum does on the host, so both give the same bits.  Every call
+// on a GPU writes every partial it then reads (nothing needs clearing), and
+// calls that overlap on one GPU race on the array.
+static __device__ __align__(256) double scratch[GSUM_BLOCKS * FIMHIP_GLOBSUM_NOALLOC_MAX_Q];
+
+constexpr int GSUM_MAX_DEVICES = 64;
+static double *g_scratch_dev[GSUM_MAX_DEVICES];   // scratch's device address, per GPU
+static std::mutex g_scratch_mu;
+
+__global__ void __launch_bounds__(256) k_globsum_dev(const double *area, const double *dp, const double *tr, int nip,
+                                                     int nvl, int nq1, double *part) {
+  __shared__ double sh[256];
+  for (int q = 0; q < nq1; q++) {
+    const double *tq = tr + (size_t)(q > 0 ? q - 1 : 0) * nip * nvl;
+    const double r = gsum_block(area, dp, tq, q > 0, nip, nvl, sh);
+    if (threadIdx.x == 0) part[(size_t)q * GSUM_BLOCKS + blockIdx.x] = r;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_globsum_final(const double *part, int nq1, double *out) {
+  const int q = blockIdx.x * 64 + threadIdx.x;
+  if (q >= nq1) return;
+  double acc = 0.;
+  for (int b = 0; b < GSUM_BLOCKS; b++) acc += part[(size_t)q * GSUM_BLOCKS + b];
+  out[q] = q == 0 ? acc / GRVITY : acc;
+}
+
+extern "C" int fimhip_globsum_noalloc(int device, int nip, int nvl, int nq, const double *area, const double *dp,
+                                      const double *tr, double *out, void *stream) {
+  if (device < 0 || device >= GSUM_MAX_DEVICES || nip < 1 || nvl < 1 || nvl > NVL_MAX || nq < 0 ||
+      nq + 1 > FIMHIP_GLOBSUM_NOALLOC_MAX_Q || !area || !dp || !out || (nq > 0 && !tr))
+    return FIMHIP_EINVAL;
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device >= ndev) return FIMHIP_EINVAL;
+  int prev = 0;
+  HIPCHK(hipGetDevice(&prev));
+  if (prev != device) HIPCHK(hipSetDevice(device));
+  hipError_t e = hipSuccess;
+  double *part = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {       // set-up, once per GPU, with that GPU current
+      double *p = nullptr;
+      e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+      if (e == hipSuccess) g_scratch_dev[device] = p;
+    }
+    part = g_scratch_dev[device];
+  }
+  if (e == hipSuccess) {
+    hipStream_t st = (hipStream_t)stream;
+    k_globsum_dev<<<GSUM_BLOCKS, 256, 0, st>>>(area, dp, tr, nip, nvl, nq + 1, part);
+    k_globsum_final<<<(nq + 1 + 63) / 64, 64, 0, st>>>(part, nq + 1, out);
+    e = hipGetLastError();
+  }
+  if (prev != device) (void)hipSetDevice(prev);
+  if (e != hipSuccess) {
+    fprintf(stderr, "fimhip: globsum_noalloc: %s\n", hipGetErrorString(e));
+    return FIMHIP_EHIP;
   }
   return FIMHIP_OK;
 }

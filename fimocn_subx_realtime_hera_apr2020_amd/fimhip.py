@@ -45,7 +45,7 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ens_globsum",
            "fimhip_ens_kernel_count", "fimhip_ens_kernel_name", "fimhip_ens_set_tile", "fimhip_ens_get_tile",
            "fimhip_ens_autotune", "fimhip_ens_profile", "fimhip_ens_profile_report",
-           "fimhip_ens_set_fused", "fimhip_ens_get_fused", "fimhip_ens_patches",
+           "fimhip_globsum_noalloc", "fimhip_ens_set_fused", "fimhip_ens_get_fused", "fimhip_ens_patches",
            "fimhip_ens_set_wet"]
 
 # real*4 drop-ins: name -> (number of float arrays, trailing int argument)
@@ -55,6 +55,7 @@ R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, Tru
 EXPORTS += ["fimhip_%s_r4" % n for n in R4_ENTRIES] + ["fimhip_abstart_r4", "fimhip_transp1_r4"]
 
 SYNTH_WIND_NOISE, SYNTH_PERTURB, SYNTH_LOGNORMAL = 1, 2, 4
+GLOBSUM_NOALLOC_MAX_Q = 64        # FIMHIP_GLOBSUM_NOALLOC_MAX_Q: mass + tracer totals per call
 
 
 class FimhipError(RuntimeError):
@@ -117,6 +118,7 @@ def load_library() -> ct.CDLL:
     lib.fimhip_step.argtypes = [vp, ct.c_int, ct.c_int]
     lib.fimhip_sync.argtypes = [vp]
     lib.fimhip_globsum.argtypes = [vp, _PD, _PD]
+    lib.fimhip_globsum_noalloc.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, vp, vp, vp, vp, vp]
     _PF = ct.POINTER(ct.c_float)
     lib.fimhip_icos_grid.argtypes = [ct.c_int, _PF, _PF, _PI, _PI, _PI, _PF, _PF, _PF, _PF, _PF, _PF, _PF,
                                      _PF, _PF, _PI]

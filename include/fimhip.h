@@ -195,6 +195,22 @@ int fimhip_sync(fimhip_ctx *ctx);
  * ntra (+ ntrb class-B) tracers, FP64 */
 int fimhip_globsum(fimhip_ctx *ctx, double *mass, double *trtot);
 
+/* No counterpart in the reference: fimhip_globsum over caller-owned device
+ * arrays, for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  It allocates
+ * nothing: its block partials live in a file-scope device array of the
+ * library (1024 x FIMHIP_GLOBSUM_NOALLOC_MAX_Q doubles = 512 KiB, held on each
+ * GPU it runs on until the process ends).  area (nip), dp (nvl,nip) and tr
+ * (nvl,nip,nq) are FP64 device arrays on GPU 'device'; out is a device array
+ * of 1 + nq doubles receiving the mass (sum area*dp/g) and the nq tracer
+ * totals, the same bits fimhip_globsum returns for the same arrays.  Runs
+ * asynchronously on 'stream' (a hipStream_t; NULL = the default stream).
+ * nq + 1 > FIMHIP_GLOBSUM_NOALLOC_MAX_Q, nvl > 64 or a null array is
+ * FIMHIP_EINVAL.  Calls that overlap on one GPU race on the partials. */
+#define FIMHIP_GLOBSUM_NOALLOC_MAX_Q 64
+int fimhip_globsum_noalloc(int device, int nip, int nvl, int nq, const double *area, const double *dp,
+                           const double *tr, double *out, void *stream);
+
 /* per-kernel timing with HIP events on the instance's stream, for steps run
  * while profiling is on; the report lists "kernel launches total_ms" lines */
 int fimhip_profile(fimhip_ctx *ctx, int on);
