@@ -4,10 +4,10 @@
 // forms for them (edgvar3.F90:65-163), without any intermediate array in HBM.
 //
 // The cells are partitioned into patches of up to FCT_CORE_MAX cells.  One
-// block handles one patch of one tracer of one member over a chunk of levels:
-// it stages the tracer column chunk of every cell within three neighbour
-// rings of the patch in LDS (lanes along the level, the reference's
-// (nvl,nip) layout), then
+// block handles one patch of one member for a group of tracers over a chunk
+// of levels: it stages the tracers of every cell within three neighbour
+// rings of the patch in LDS (one wavefront per cell, its lanes along the
+// tracers and levels), then
 //   stage A  rings 0-2: the low-order fluxes, tendency and bounded
 //            low-order tracer (trcadv1); rings 0-1 also the two-sided edge
 //            values and antidiffusive fluxes and their in/outflow sums;
@@ -166,18 +166,22 @@ namespace fimhip {
 // ---------------------------------------------------------------------------
 // device
 //
-// A block covers LC levels of its patch's columns (LC = 64: one level chunk,
-// one cell per wavefront; LC = 32: two chunks, two cells per wavefront, half
-// the LDS per block so two blocks share a CU, and a 32-layer instance fills
-// every lane).  Lane l of wavefront w works on level chunk * LC + l % LC of
-// the tile cells (w * CPW + l / LC) + j * NW * CPW, j = 0, 1, ... (NW
-// wavefronts per block).
+// A block covers one patch of one member for TG tracers over LC = 64 / TG
+// levels: lane l of a wavefront works on tracer (group * TG + l / LC) at
+// level (chunk * LC + l % LC) of one tile cell, and wavefront w takes the
+// tile cells w, w + NW, w + 2 NW, ...  With one cell per wavefront the
+// cell's table entry, metric terms and least-squares coefficients are
+// wave-uniform (scalar loads) and shared by the TG tracers, and so are its
+// mass fluxes and thickness (one load per level for all TG tracers).  LDS
+// holds one value per lane and tile cell.
 
 #define EC(a, col, k) (a)[(size_t)(col) * nvl + (k)]
 #define EE(a, col, e, k) (a)[((size_t)(col) * NPP + (e)) * nvl + (k)]
-#define ETR(a, t, col, k) (a)[((size_t)(t) * ncol + (col)) * nvl + (k)]
-#define ETAB(a, t, sl, col, k) (a)[(((size_t)(t) * 3 + (sl)) * ncol + (col)) * nvl + (k)]
 #define FHF(i, a, b) ldc(g.hfcoef, ((size_t)(i) * NPP + (a)) * NPP + (b))
+// a lane's tracer field at column col: toff = t * ncol * nvl + k
+#define TRV(a, col) (a)[toff + (size_t)(col) * nvl]
+// a lane's AB slot sl of a tracer tendency: aoff = t * 3 * ncol * nvl + k
+#define TAB(a, sl, col) (a)[aoff + ((size_t)(sl) * ncol + (col)) * nvl]
 
 // a table entry of four ints (one load; a scalar load when the index is
 // wave-uniform)
@@ -193,10 +197,7 @@ __device__ __forceinline__ int lnb(const int4 &en, int e) {
 }
 __device__ __forceinline__ int enp(const int4 &en) { return (en.z >> 16) & 0xff; }
 __device__ __forceinline__ int eslot(const int4 &en, int e) { return (en.w >> (4 * e)) & 0xf; }
-
-// a value that is wave-uniform when a wavefront holds one cell
-template <int CPW>
-__device__ __forceinline__ int fct_uni(int x) { return CPW == 1 ? __builtin_amdgcn_readfirstlane(x) : x; }
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
 // Theta on the edge between cell i and x (edgvar1.F90:425-511, eqwgt), the
 // outer stencil cells t, l added in column order (as e_theta_edge).
@@ -208,41 +209,77 @@ __device__ __forceinline__ float fct_theta_edge(float thi, float thx, float tht,
 
 __device__ __forceinline__ float fct_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// The HBM inputs of stage A for one tile cell (a pentagon's sixth mass flux
+// is 0), loaded a pass ahead of their use.
+struct FctInA {
+  float m[NPP], tdp0, tlo_of, tlo_vof;
+  double dp;
+};
+__device__ __forceinline__ FctInA fct_load_a(const EnsState &s, const StepCtl &c, int nvl, int col, int k,
+                                             size_t toff, size_t aoff, bool live, const float *trdp_src) {
+  const int ncol = s.ncol, of = c.of - 1, vof = c.vof - 1;
+  FctInA r;
+  if (live) {
+#pragma unroll
+    for (int e = 0; e < NPP; e++) r.m[e] = EE(s.massfx, col, e, k);
+    r.tdp0 = TRV(trdp_src, col);
+    r.tlo_of = TAB(s.trl_tdcy, of, col);
+    r.tlo_vof = TAB(s.trl_tdcy, vof, col);
+    r.dp = EC(s.dp, col, k);
+  } else {
+#pragma unroll
+    for (int e = 0; e < NPP; e++) r.m[e] = 0.f;
+    r.tdp0 = r.tlo_of = r.tlo_vof = 0.f;
+    r.dp = 1.;
+  }
+  return r;
+}
+// The HBM inputs of stage B for one tile cell.
+struct FctInB {
+  double dp;
+  float to, tv;
+};
+__device__ __forceinline__ FctInB fct_load_b(const EnsState &s, const StepCtl &c, int nvl, int col, int k,
+                                             size_t aoff, bool live) {
+  const int ncol = s.ncol, of = c.of - 1, vof = c.vof - 1;
+  FctInB r{0., 0.f, 0.f};
+  if (live) {
+    r.dp = EC(s.dp, col, k);
+    r.to = TAB(s.trc_tdcy, of, col);
+    r.tv = TAB(s.trc_tdcy, vof, col);
+  }
+  return r;
+}
+
 // Stage A of tile cell i with NP edges (trcadv1): the low-order fluxes and
 // tendency, the bounded low-order tracer (to Slo); for rings 0-1 the two-
 // sided edge values and antidiffusive fluxes a_out and their out/inflow
-// sums.  Also returns the low-order tendency (written to HBM for the core),
-// the tracer mass at the start and 1/dp.
-template <int NP, int LC, int CPW>
+// sums.  Also returns the low-order tendency (written to HBM for the core)
+// and 1/dp.  Lanes of tracer 1 (theta) form its edge values from the edge
+// stencil, the others from the least-squares reconstructions of both cells
+// of the edge (edgvar3.F90:65-132).
+template <int NP>
 __device__ __forceinline__ void fct_stage_a(const EnsGrid &g, const EnsState &s, const StepCtl &c, const int4 &en,
-                                            int i, int n0, int n1, int t, int base, int k, int kk, bool live,
-                                            const float *__restrict__ trdp_src, const float *Str, float *Slo,
-                                            const int4 *Sent, float *a_out, float &sp_out,
-                                            float &sm_out, float &tlo_out, float &tdp0_out, float &rdp_out) {
+                                            int i, int n0, int n1, int t, size_t aoff, int base, int lane,
+                                            bool live, const FctInA &in, const float *Str, float *Slo,
+                                            const int4 *Sent, float *a_out, float &sp_out, float &sm_out,
+                                            float &tlo_out, float &rdp_out) {
   const int nvl = g.d.nvl, ncol = s.ncol;
-  const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
-  const int cell = en.x, col = base + cell;
+  const int nf = c.nf - 1;
+  const int cell = uni(en.x), col = base + cell;
   const float thshld = 1.e-11f;
   int l[NP];
 #pragma unroll
-  for (int e = 0; e < NP; e++) l[e] = lnb(en, e);
+  for (int e = 0; e < NP; e++) l[e] = uni(lnb(en, e));
   float m[NP];
-  float tdp0 = 0.f, tlo_of = 0.f, tlo_vof = 0.f, rdp = 0.f;
-  if (live) {
 #pragma unroll
-    for (int e = 0; e < NP; e++) m[e] = EE(s.massfx, col, e, k);
-    tdp0 = ETR(trdp_src, t, col, k);
-    tlo_of = ETAB(s.trl_tdcy, t, of, col, k);
-    tlo_vof = ETAB(s.trl_tdcy, t, vof, col, k);
-    rdp = fct_rcp(fmaxf(thshld, (float)EC(s.dp, col, k)));
-  } else {
-#pragma unroll
-    for (int e = 0; e < NP; e++) m[e] = 0.f;
-  }
-  const float ti = Str[i * LC + kk];
+  for (int e = 0; e < NP; e++) m[e] = in.m[e];
+  const float tdp0 = in.tdp0, tlo_of = in.tlo_of, tlo_vof = in.tlo_vof;
+  const float rdp = live ? fct_rcp(fmaxf(thshld, (float)in.dp)) : 0.f;
+  const float ti = Str[i * 64 + lane];
   float v[NP];
 #pragma unroll
-  for (int e = 0; e < NP; e++) v[e] = Str[l[e] * LC + kk];
+  for (int e = 0; e < NP; e++) v[e] = Str[l[e] * 64 + lane];
   const float rar = ldc(g.rarea, cell);
   const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
   float acc = 0.f, mx = ti, mn = ti, fl[NP];
@@ -260,11 +297,10 @@ __device__ __forceinline__ void fct_stage_a(const EnsGrid &g, const EnsState &s,
   const float lo = tdp0 + a1 * tlo + a2 * tlo_of + a3 * tlo_vof;
   mx = fmaxf(0.f, mx);
   mn = fmaxf(0.f, mn);
-  Slo[i * LC + kk] = fmaxf(mn, fminf(mx, lo * rdp));
+  Slo[i * 64 + lane] = fmaxf(mn, fminf(mx, lo * rdp));
   tlo_out = tlo;
-  tdp0_out = tdp0;
   rdp_out = rdp;
-  if (i < n0 && live) ETAB(s.trl_tdcy, t, nf, col, k) = tlo;
+  if (i < n0 && live) TAB(s.trl_tdcy, nf, col) = tlo;
   if (i >= n1) return;
   float te[NP];
   if (t == 0) {
@@ -279,21 +315,19 @@ __device__ __forceinline__ void fct_stage_a(const EnsGrid &g, const EnsState &s,
     float d[NP];
 #pragma unroll
     for (int j = 0; j < NP; j++) d[j] = v[j] - ti;
-    int4 ex[NP];
-#pragma unroll
-    for (int e = 0; e < NP; e++) ex[e] = Sent[l[e]];
 #pragma unroll
     for (int e = 0; e < NP; e++) {
       float ac = 0.f;
 #pragma unroll
-      for (int j = 0; j < NP; j++) ac = ac + d[j] * FHF(cell, e, j);
+      for (int j = 0; j < NP; j++) ac = __builtin_fmaf(d[j], FHF(cell, e, j), ac);
       const float eo = ti + ac;
-      const int xcell = ex[e].x, xs = eslot(en, e), xnp = enp(ex[e]);
+      const int4 ex = Sent[l[e]];
+      const int xcell = uni(ex.x), xs = uni(eslot(en, e)), xnp = uni(enp(ex));
       const float tx = v[e];
       float ax = 0.f;
 #pragma unroll
       for (int j = 0; j < NPP; j++) {
-        if (j < xnp) ax = ax + (Str[lnb(ex[e], j) * LC + kk] - tx) * FHF(xcell, xs, j);
+        if (j < xnp) ax = __builtin_fmaf(Str[uni(lnb(ex, j)) * 64 + lane] - tx, FHF(xcell, xs, j), ax);
       }
       const float en_ = tx + ax;
       te[e] = .5f * (eo + en_);
@@ -314,28 +348,19 @@ __device__ __forceinline__ void fct_stage_a(const EnsGrid &g, const EnsState &s,
 
 // Stage B (trcadv2): local bounds from the tracer and the low-order tracer
 // of the cell and its neighbours, and the limiter ratios.
-template <int LC>
-__device__ __forceinline__ void fct_stage_b(const EnsGrid &g, const EnsState &s, const StepCtl &c, const int4 &en,
-                                            int i, int t, int base, int k, int kk, bool live, const float *Str,
-                                            const float *Slo, float sp, float sm, float &rp, float &rm, float &mx_o,
-                                            float &mn_o, float &to_o, float &tv_o) {
-  const int nvl = g.d.nvl, ncol = s.ncol;
-  const int of = c.of - 1, vof = c.vof - 1;
-  const int cell = en.x, col = base + cell;
+__device__ __forceinline__ void fct_stage_b(const EnsGrid &g, const StepCtl &c, const int4 &en, int i, int lane,
+                                            const FctInB &in, const float *Str, const float *Slo, float sp,
+                                            float sm, float &rp, float &rm, float &mx_o, float &mn_o) {
+  const int cell = uni(en.x);
   const float thshld = 1.e-11f;
-  float dp = 0.f, to = 0.f, tv = 0.f;
-  if (live) {
-    dp = (float)EC(s.dp, col, k);
-    to = ETAB(s.trc_tdcy, t, of, col, k);
-    tv = ETAB(s.trc_tdcy, t, vof, col, k);
-  }
+  const float dp = (float)in.dp, to = in.to, tv = in.tv;
   const float area = ldc(g.area, cell);
   const float a2 = (float)c.ab2, a3 = (float)c.ab3, ra1 = (float)(1. / c.ab1) * area;
-  const float ti = Str[i * LC + kk], lo = Slo[i * LC + kk];
+  const float ti = Str[i * 64 + lane], lo = Slo[i * 64 + lane];
   float mx = ti, mn = ti;
 #pragma unroll
   for (int e = 0; e < NPP; e++) {
-    const float v = Str[lnb(en, e) * LC + kk];
+    const float v = Str[uni(lnb(en, e)) * 64 + lane];
     mx = fmaxf(mx, v);
     mn = fminf(mn, v);
   }
@@ -343,7 +368,7 @@ __device__ __forceinline__ void fct_stage_b(const EnsGrid &g, const EnsState &s,
   mn = fminf(fmaxf(0.f, mn), lo);
 #pragma unroll
   for (int e = 0; e < NPP; e++) {
-    const float v = Slo[lnb(en, e) * LC + kk];
+    const float v = Slo[uni(lnb(en, e)) * 64 + lane];
     mx = fmaxf(mx, v);
     mn = fminf(mn, v);
   }
@@ -355,193 +380,204 @@ __device__ __forceinline__ void fct_stage_b(const EnsGrid &g, const EnsState &s,
   rm = fmaxf(0.f, fminf(1.f, qm * fct_rcp(fmaxf(sm, thshld))));
   mx_o = mx;
   mn_o = mn;
-  to_o = to;
-  tv_o = tv;
 }
 
-// blockIdx.x: (patch, level chunk, tracer) in an order that gives each XCD
-// a contiguous range of patches (the host pads gridDim.x to a multiple of
-// 8); blockIdx.y: member.  trs/tds: the tracer and tracer-mass buffers read,
-// trd/tdd: those written.
+// blockIdx.x: (level chunk, tracer group, patch), the chunk fastest, in an
+// order that gives each XCD a contiguous range (the host pads gridDim.x to a
+// multiple of 8); blockIdx.y: member.  trs/tds: the tracer and tracer-mass
+// buffers read, trd/tdd: those written.
 // LDS: the tile's tracer (rings 0-3), its bounded low-order tracer (rings
 // 0-2), and per ring 0-1 cell a pair that holds the antidiffusive out/inflow
 // sums after stage A and the limiter ratios R+/R- after stage B.
-// With 8 wavefronts per block the register allocation is capped so that
-// two blocks (and their LDS tiles) share a CU.
-template <int LC, int NW, bool SWZ>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 1))) k_e_trc_fct(EnsGrid g, EnsState s, int nchunk, const float *trs,
-                                                       const float *tds, float *trd, float *tdd) {
-  constexpr int CPW = 64 / LC;                  // cells per wavefront
-  constexpr int STEP = NW * CPW;                // tile cells per pass of the block
-  constexpr int J0 = (FCT_TILE_MAX + STEP - 1) / STEP, JA = (FCT_LO_MAX + STEP - 1) / STEP;
-  constexpr int JB = (FCT_R_MAX + STEP - 1) / STEP, JC = (FCT_CORE_MAX + STEP - 1) / STEP;
-  __shared__ float Str[FCT_TILE_MAX * LC];
-  __shared__ float Slo[FCT_LO_MAX * LC];
-  __shared__ float Sr[FCT_R_MAX * 2 * LC];
+template <int TG, int NW, bool SWZ>
+__global__ void __launch_bounds__(64 * NW) k_e_trc_fct(EnsGrid g, EnsState s, int nchunk, int ngrp,
+                                                       const float *trs, const float *tds, float *trd, float *tdd) {
+  constexpr int LC = 64 / TG;                   // levels per block
+  constexpr int J0 = (FCT_TILE_MAX + NW - 1) / NW, JA = (FCT_LO_MAX + NW - 1) / NW;
+  constexpr int JB = (FCT_R_MAX + NW - 1) / NW, JC = (FCT_CORE_MAX + NW - 1) / NW;
+  __shared__ float Str[FCT_TILE_MAX * 64];
+  __shared__ float Slo[FCT_LO_MAX * 64];
+  __shared__ float Sr[FCT_R_MAX * 2 * 64];
   __shared__ int4 Sent[FCT_TILE_MAX];          // the tile's entries
   const int nvl = g.d.nvl, nip = g.d.nip, ncol = s.ncol, ntra = g.d.ntra;
-  const int w = threadIdx.y, kk = threadIdx.x % LC, h = threadIdx.x / LC;
+  const int w = threadIdx.y, lane = threadIdx.x;
   const EnsTiles &ts = g.tiles;
   int bx = blockIdx.x;
   if (SWZ) {
     const int per = gridDim.x >> 3;
     bx = (bx & 7) * per + (bx >> 3);
   }
-  const int t = bx % ntra, pc = bx / ntra, chunk = pc % nchunk, p = pc / nchunk;
+  const int chunk = bx % nchunk, pg = bx / nchunk, grp = pg % ngrp, p = pg / ngrp;
   if (p >= ts.npatch) return;          // the whole block: before any barrier
-  const int k = chunk * LC + kk;
+  const int t = grp * TG + lane / LC, k = chunk * LC + lane % LC;
+  const bool live = k < nvl && t < ntra;
+  const int tc = t < ntra ? t : 0, kc = k < nvl ? k : 0;   // in-range for the offsets of dead lanes
+  const size_t toff = (size_t)tc * ncol * nvl + kc, aoff = (size_t)tc * 3 * ncol * nvl + kc;
   const int base = blockIdx.y * nip;
   const int4 cn = ldc4(ts.cnt, p);
   const int off = ldc(ts.off, p);
   const int n0 = cn.x, n1 = cn.y, n2 = cn.z, n3 = cn.w;
-  const bool live = k < nvl;
   const StepCtl c = load_ctl(s.ctl);
-  const float *trf = trs + (size_t)t * ncol * nvl;
-  const int slot = w * CPW + h;
   // stage 0: the tracer columns of the tile, every load issued before the
   // first is stored
   {
     int4 ev[J0];
 #pragma unroll
-    for (int j = 0; j < J0; j++) ev[j] = ldc4(ts.ent, (size_t)off + fct_uni<CPW>(min(slot + j * STEP, n3 - 1)));
+    for (int j = 0; j < J0; j++) ev[j] = ldc4(ts.ent, (size_t)off + uni(min(w + j * NW, n3 - 1)));
     float v[J0];
 #pragma unroll
-    for (int j = 0; j < J0; j++) v[j] = live ? trf[(size_t)(base + ev[j].x) * nvl + k] : 0.f;
+    for (int j = 0; j < J0; j++) v[j] = live ? TRV(trs, base + ev[j].x) : 0.f;
 #pragma unroll
     for (int j = 0; j < J0; j++) {
-      const int i = slot + j * STEP;
+      const int i = w + j * NW;
       if (i < n3) {
-        Str[i * LC + kk] = v[j];
-        if (kk == 0) Sent[i] = ev[j];
+        Str[i * 64 + lane] = v[j];
+        if (lane == 0) Sent[i] = ev[j];
       }
     }
   }
   __syncthreads();
-  // stage A; the core cells keep their fluxes for stage C, and when a lane
-  // holds at most two core cells also their low-order tendency, tracer mass,
-  // 1/dp and old tendencies (otherwise stage C loads them again)
-  constexpr bool KEEP = JC <= 2;
-  constexpr int JK = KEEP ? JC : 1;
-  float a[JC][NPP], tlo[JK], tdp0[JK], rdp[JK];
+  // stage A, each pass's HBM inputs loaded during the pass before; the core
+  // cells (the first passes) keep their fluxes, low-order tendency, tracer
+  // mass and 1/dp for stage C
+  float a[JC][NPP], tlo[JC], tdp0[JC], rdp[JC];
+  {
+    FctInA cur, nxt;
+    {
+      const int i = uni(min(w, n2 - 1));
+      cur = fct_load_a(s, c, nvl, base + uni(Sent[i].x), k, toff, aoff, live, tds);
+    }
 #pragma unroll
-  for (int j = 0; j < JA; j++) {
-    const int i = fct_uni<CPW>(slot + j * STEP);
-    if (i < n2) {
-      const int4 en = Sent[i];
-      float aa[NPP], ssp = 0.f, ssm = 0.f, tl = 0.f, td = 0.f, rd = 0.f;
-      if (enp(en) == 6)
-        fct_stage_a<6, LC, CPW>(g, s, c, en, i, n0, n1, t, base, k, kk, live, tds, Str, Slo, Sent, aa, ssp,
-                                ssm, tl, td, rd);
-      else
-        fct_stage_a<5, LC, CPW>(g, s, c, en, i, n0, n1, t, base, k, kk, live, tds, Str, Slo, Sent, aa, ssp,
-                                ssm, tl, td, rd);
+    for (int j = 0; j < JA; j++) {
+      const int i = uni(w + j * NW);
+      if (j + 1 < JA) {
+        const int in = uni(min(w + (j + 1) * NW, n2 - 1));
+        nxt = fct_load_a(s, c, nvl, base + uni(Sent[in].x), k, toff, aoff, live, tds);
+      }
+      if (i < n2) {
+        const int4 en = Sent[i];
+        float aa[NPP], ssp = 0.f, ssm = 0.f, tl = 0.f, rd = 0.f;
+        if (uni(enp(en)) == 6)
+          fct_stage_a<6>(g, s, c, en, i, n0, n1, t, aoff, base, lane, live, cur, Str, Slo, Sent, aa, ssp, ssm,
+                         tl, rd);
+        else
+          fct_stage_a<5>(g, s, c, en, i, n0, n1, t, aoff, base, lane, live, cur, Str, Slo, Sent, aa, ssp, ssm,
+                         tl, rd);
+        if (i < n1) {
+          Sr[(2 * i) * 64 + lane] = ssp;
+          Sr[(2 * i + 1) * 64 + lane] = ssm;
+        }
+        if (j < JC) {
+          const int jj = j < JC ? j : 0;
+#pragma unroll
+          for (int e = 0; e < NPP; e++) a[jj][e] = aa[e];
+          tlo[jj] = tl;
+          tdp0[jj] = cur.tdp0;
+          rdp[jj] = rd;
+        }
+      }
+      if (j + 1 < JA) cur = nxt;
+    }
+  }
+  __syncthreads();
+  // stage B: R+/R- replace each cell's own sums in Sr; inputs a pass ahead
+  float tmx[JC], tmn[JC], tco[JC], tcv[JC];
+  {
+    FctInB cur, nxt;
+    {
+      const int i = uni(min(w, n1 - 1));
+      cur = fct_load_b(s, c, nvl, base + uni(Sent[i].x), k, aoff, live);
+    }
+#pragma unroll
+    for (int j = 0; j < JB; j++) {
+      const int i = uni(w + j * NW);
+      if (j + 1 < JB) {
+        const int in = uni(min(w + (j + 1) * NW, n1 - 1));
+        nxt = fct_load_b(s, c, nvl, base + uni(Sent[in].x), k, aoff, live);
+      }
       if (i < n1) {
-        Sr[(2 * i) * LC + kk] = ssp;
-        Sr[(2 * i + 1) * LC + kk] = ssm;
-      }
-      if (j < JC) {
-        const int jj = j < JC ? j : 0;
-#pragma unroll
-        for (int e = 0; e < NPP; e++) a[jj][e] = aa[e];
-        if (KEEP) {
-          const int jk = j < JK ? j : 0;
-          tlo[jk] = tl;
-          tdp0[jk] = td;
-          rdp[jk] = rd;
+        const int4 en = Sent[i];
+        float mx, mn, rp, rm;
+        fct_stage_b(g, c, en, i, lane, cur, Str, Slo, Sr[(2 * i) * 64 + lane], Sr[(2 * i + 1) * 64 + lane], rp,
+                    rm, mx, mn);
+        Sr[(2 * i) * 64 + lane] = rp;
+        Sr[(2 * i + 1) * 64 + lane] = rm;
+        if (j < JC) {
+          const int jj = j < JC ? j : 0;
+          tmx[jj] = mx;
+          tmn[jj] = mn;
+          tco[jj] = cur.to;
+          tcv[jj] = cur.tv;
         }
       }
-    }
-  }
-  __syncthreads();
-  // stage B: R+/R- replace each cell's own sums in Sr
-  float tmx[JC], tmn[JC], tco[JK], tcv[JK];
-#pragma unroll
-  for (int j = 0; j < JB; j++) {
-    const int i = fct_uni<CPW>(slot + j * STEP);
-    if (i < n1) {
-      const int4 en = Sent[i];
-      float mx, mn, to, tv, rp, rm;
-      fct_stage_b<LC>(g, s, c, en, i, t, base, k, kk, live, Str, Slo, Sr[(2 * i) * LC + kk],
-                      Sr[(2 * i + 1) * LC + kk], rp, rm, mx, mn, to, tv);
-      Sr[(2 * i) * LC + kk] = rp;
-      Sr[(2 * i + 1) * LC + kk] = rm;
-      if (j < JC) {
-        const int jj = j < JC ? j : 0;
-        tmx[jj] = mx;
-        tmn[jj] = mn;
-        if (KEEP) {
-          const int jk = j < JK ? j : 0;
-          tco[jk] = to;
-          tcv[jk] = tv;
-        }
-      }
+      if (j + 1 < JB) cur = nxt;
     }
   }
   __syncthreads();
   // stage C: the core
-  const int nf = c.nf - 1, of = c.of - 1, vof = c.vof - 1;
+  const int nf = c.nf - 1;
   const float a1 = (float)c.ab1, a2 = (float)c.ab2, a3 = (float)c.ab3;
 #pragma unroll
   for (int j = 0; j < JC; j++) {
-    const int i = fct_uni<CPW>(slot + j * STEP);
+    const int i = uni(w + j * NW);
     if (i < n0 && live) {
       const int4 en = Sent[i];
-      const int cell = en.x, col = base + cell;
-      const int np = enp(en);
+      const int cell = uni(en.x), col = base + cell;
+      const int np = uni(enp(en));
       const float rar = ldc(g.rarea, cell);
-      const int jk = KEEP && j < JK ? j : 0;
-      // the same values stage A and B used (stage A wrote the low-order tendency)
-      const float tlo_j = KEEP ? tlo[jk] : ETAB(s.trl_tdcy, t, nf, col, k);
-      const float tdp0_j = KEEP ? tdp0[jk] : ETR(tds, t, col, k);
-      const float rdp_j = KEEP ? rdp[jk] : fct_rcp(fmaxf(1.e-11f, (float)EC(s.dp, col, k)));
-      const float tco_j = KEEP ? tco[jk] : ETAB(s.trc_tdcy, t, of, col, k);
-      const float tcv_j = KEEP ? tcv[jk] : ETAB(s.trc_tdcy, t, vof, col, k);
-      const float rpi = Sr[(2 * i) * LC + kk], rmi = Sr[(2 * i + 1) * LC + kk];
+      const float rpi = Sr[(2 * i) * 64 + lane], rmi = Sr[(2 * i + 1) * 64 + lane];
       float r[NPP];
 #pragma unroll
       for (int e = 0; e < NPP; e++) {
-        const int x = lnb(en, e);
-        r[e] = Sr[(2 * x + (a[j][e] >= 0.f ? 0 : 1)) * LC + kk];
+        const int x = uni(lnb(en, e));
+        r[e] = Sr[(2 * x + (a[j][e] >= 0.f ? 0 : 1)) * 64 + lane];
       }
       float at = 0.f;
 #pragma unroll
       for (int e = 0; e < NPP; e++)
         if (e < np) at = at + a[j][e] * fminf(a[j][e] >= 0.f ? rmi : rpi, r[e]);
       at = -at * rar;
-      const float tt = tlo_j + at;
-      const float tdp = tdp0_j + a1 * tt + a2 * tco_j + a3 * tcv_j;
-      ETAB(s.trc_tdcy, t, nf, col, k) = tt;
-      ETR(tdd, t, col, k) = tdp;
+      const float tt = tlo[j] + at;
+      const float tdp = tdp0[j] + a1 * tt + a2 * tco[j] + a3 * tcv[j];
+      TAB(s.trc_tdcy, nf, col) = tt;
+      TRV(tdd, col) = tdp;
       // a land column keeps its tracers
-      ETR(trd, t, col, k) = col_wet(s.wet, col) ? fmaxf(tmn[j], fminf(tmx[j], tdp * rdp_j)) : Str[i * LC + kk];
+      TRV(trd, col) = col_wet(s.wet, col) ? fmaxf(tmn[j], fminf(tmx[j], tdp * rdp[j])) : Str[i * 64 + lane];
     }
   }
 }
 
-// Launch shapes (the autotuner's tiles of this kernel): tile & 3 selects
-// (levels per block, wavefronts per block) = (32, 16), (32, 8), (64, 16),
-// (64, 8); tile & 4 the XCD-contiguous block order.  Every shape performs the
-// same operations, so the results do not depend on it.
-template <int LC, int NW, bool SWZ>
+// Launch shapes (the autotuner's tiles of this kernel): tile & 1 selects 16
+// or 8 wavefronts per block, tile & 2 half the tracers per block (twice the
+// levels), tile & 4 the XCD-contiguous block order.  The tracers per block
+// are 4 for ntra >= 3, else ntra.  Every shape performs the same operations,
+// so the results do not depend on it.
+template <int TG, int NW, bool SWZ>
 static void launch_fct(const EnsGrid &g, const EnsState &s, hipStream_t st) {
-  const int nchunk = (g.d.nvl + LC - 1) / LC;
-  int gx = g.tiles.npatch * nchunk * g.d.ntra;
+  constexpr int LC = 64 / TG;
+  const int nchunk = (g.d.nvl + LC - 1) / LC, ngrp = (g.d.ntra + TG - 1) / TG;
+  int gx = g.tiles.npatch * nchunk * ngrp;
   if (SWZ) gx = (gx + 7) / 8 * 8;
-  k_e_trc_fct<LC, NW, SWZ><<<dim3(gx, s.members), dim3(64, NW), 0, st>>>(g, s, nchunk, s.tr, s.trdp, s.tr_alt,
-                                                                         s.trdp_alt);
+  k_e_trc_fct<TG, NW, SWZ><<<dim3(gx, s.members), dim3(64, NW), 0, st>>>(g, s, nchunk, ngrp, s.tr, s.trdp,
+                                                                         s.tr_alt, s.trdp_alt);
+}
+
+template <int TG>
+static void launch_fct_tg(int tile, const EnsGrid &g, const EnsState &s, hipStream_t st) {
+  switch (tile & 5) {
+    case 0: launch_fct<TG, 16, false>(g, s, st); break;
+    case 1: launch_fct<TG, 8, false>(g, s, st); break;
+    case 4: launch_fct<TG, 16, true>(g, s, st); break;
+    default: launch_fct<TG, 8, true>(g, s, st); break;
+  }
 }
 
 void ens_launch_trc_fct(int tile, const EnsGrid &g, const EnsState &s, hipStream_t st) {
-  switch (tile & 7) {
-    case 0: launch_fct<32, 16, false>(g, s, st); break;
-    case 1: launch_fct<32, 8, false>(g, s, st); break;
-    case 2: launch_fct<64, 16, false>(g, s, st); break;
-    case 3: launch_fct<64, 8, false>(g, s, st); break;
-    case 4: launch_fct<32, 16, true>(g, s, st); break;
-    case 5: launch_fct<32, 8, true>(g, s, st); break;
-    case 6: launch_fct<64, 16, true>(g, s, st); break;
-    default: launch_fct<64, 8, true>(g, s, st); break;
-  }
+  int tg = g.d.ntra >= 3 ? 4 : g.d.ntra;
+  if ((tile & 2) && tg > 1) tg >>= 1;
+  if (tg == 4) launch_fct_tg<4>(tile, g, s, st);
+  else if (tg == 2) launch_fct_tg<2>(tile, g, s, st);
+  else launch_fct_tg<1>(tile, g, s, st);
 }
 
 }  // namespace fimhip

@@ -196,7 +196,9 @@ __global__ void __launch_bounds__(64 * COLS) __attribute__((amdgpu_waves_per_eu(
 
 // edgvar3 (edgvar3.F90:65-110, edgvalEval): this cell's least-squares
 // reconstruction of each edge.  The two-sided average of edgvar3.F90:111-132
-// is formed by the readers (cnuity, trcadv) from both cells' values.  The
+// is formed by the readers (cnuity, trcadv) from both cells' values; the
+// tracers' sums are fused multiply-adds in stencil order, as the fused tracer
+// transport forms them for both cells of an edge.  The
 // thickness is reconstructed as a deviation from the cell's own value, the
 // differences taken in FP64.
 template <int NP>
@@ -235,7 +237,7 @@ __device__ __forceinline__ void e_edgvar3_col(const EnsGrid &g, const EnsState &
     for (int i = 0; i < NP; i++) {
       float acc = 0.f;
 #pragma unroll
-      for (int j = 0; j < NP; j++) acc = acc + d[j] * hf[i][j];
+      for (int j = 0; j < NP; j++) acc = __builtin_fmaf(d[j], hf[i][j], acc);
       out[i] = b0 + acc;
     }
 #pragma unroll

@@ -5,6 +5,8 @@
 #
 #   oracle/_ref/bin/grid, ginfo, getlvl   reference grid prep chain (default real 4,
 #                                          as PREP_FFLAGS in macros.make.linuxpcgnu:25)
+#   oracle/_ref/bin/lsfitref              getlvl's least-squares pseudo-inverse
+#                                          (LSfitting + sgesvd) on given point sets
 #   oracle/_ref/bin/dynref_r8             reference dynamics hot path, -fdefault-real-8
 #   oracle/_ref/bin/dynref_r4             reference dynamics hot path, default real 4
 #   oracle/_ref/bin/dynref_r8omp          the r8 build with -fopenmp (timing calibration)
@@ -64,6 +66,9 @@ C="$GOBJ/headers.o $GOBJ/module_initial_chem_namelists.o $GOBJ/fimnamelist.o $GO
 $FC -o "$OUT/bin/grid" $GOBJ/{avg,bisect_triangle,datastru,icos,middle,rotate,third,top_gridpoints,top_triangles,trisect,trisect_triangle}.o $C
 $FC -o "$OUT/bin/ginfo" $GOBJ/{datastru,GetRegions,GridGen,hilbert,ijblock,IJblockLayout,mpi_stubs,perm,SquareDecomp,SquareLayout}.o $C
 $FC -o "$OUT/bin/getlvl" $GOBJ/{getlvl,LSfitting,svd,blas,la_aux}.o $C
+# the least-squares pseudo-inverse alone, on given point sets
+compile "$GOBJ" "$GFL" "$HERE/ref/lsfitref.F90"
+$FC -o "$OUT/bin/lsfitref" $GOBJ/{lsfitref,LSfitting,svd,blas,la_aux}.o $C
 
 # ---- dynamics hot path, two precisions -----------------------------------
 H=$S/fim/horizontal

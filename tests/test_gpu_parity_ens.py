@@ -155,6 +155,65 @@ def test_fp32_form_against_fp64_after_96_steps():
         p.close()
 
 
+def test_sixteen_members_g7_96_steps_full_size():
+    """The bench's configuration at full size: 16 members x G7 x 64 levels, 4
+    tracers, 96 full steps at dt = 90 s (the G7 grid's own step, 1.2*4800/64 s,
+    module_control.F90:214-220; at 240 s the G7 state overflows in FP64 within
+    a few steps).  Every member conserves mass to 1e-12; member 0 stays within
+    BOUNDS_96 of the FP64 path (measured on MI355X: us 1.4e-4, vs 1.6e-4, dp
+    1.7e-4, pr 1.6e-5, ex 5.2e-6, theta 2.5e-6, tracers 8.6e-6); members 5
+    and 15, each run alone, give the bytes they give in the batch."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    g, th, ds = _grid(7)
+    dt, steps, keep = 90.0, 96, (0, 5, 15)
+    print("case: 16 members x G7 x 64, 96 full steps, dt 90 s", flush=True)
+    e = fimhip.Ensemble(g, 16, NVL, NTRA, dt, th, ds)
+    try:
+        for m in range(16):
+            e.synth(m, seed=20240601, synth_member=m, flags=FLAGS)
+        e.init_state(True)
+        m0, _, _, _ = e.globsum()
+        e.step(steps, 0)
+        e.sync()
+        m1, _, _, _ = e.globsum()
+        drift = np.abs(m1 - m0) / m0
+        print("mass drift per member: max %.2e" % drift.max(), flush=True)
+        assert drift.shape == (16,) and np.all(drift <= 1e-12), drift
+        batch = {m: _fields(e, m) for m in keep}
+    finally:
+        e.close()
+    p = fimhip.Model(g, NVL, NTRA, dt, th, ds)
+    try:
+        p.synth(seed=20240601, member=0, flags=FLAGS)
+        p.init_state(True)
+        for its in range(1, steps + 1):
+            p.step(its, 0)
+        p.sync()
+        b = batch[0]
+        errs = {n: rel_l2(b[n], p.download(n)) for n in ("us", "vs", "dp", "pr", "ex")}
+        trp = p.download("tr")
+        errs["tr0"] = rel_l2(b["tr"][0], trp[0])
+        errs["tr"] = max(rel_l2(b["tr"][t], trp[t]) for t in range(1, NTRA))
+        print("member 0 rel L2 vs FP64:", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+        for n, bound in BOUNDS_96.items():
+            assert errs[n] < bound, (n, errs[n], bound)
+    finally:
+        p.close()
+    for m in keep[1:]:
+        print("case: member %d alone" % m, flush=True)
+        e1 = fimhip.Ensemble(g, 1, NVL, NTRA, dt, th, ds)
+        try:
+            e1.synth(0, seed=20240601, synth_member=m, flags=FLAGS)
+            e1.init_state(True)
+            e1.step(steps, 0)
+            e1.sync()
+            f = _fields(e1, 0)
+        finally:
+            e1.close()
+        for n in f:
+            assert np.array_equal(f[n], batch[m][n]), (m, n)
+
+
 def test_twenty_lognormal_tracers_transport_fp32_vs_fp64():
     """Chemistry-load transport: 20 seeded log-normal tracers (median 1e-9,
     sigma 1.5, times the cosine bell), transport-only steps at dt = 120 s, the
