@@ -15,10 +15,9 @@
 // single-precision sin/cos.  Build without fp contraction or fast math.
 //
 // hfcoef (getlvl.F90:219-269) is the least-squares edge reconstruction
-// weight.  The reference obtains the pseudo-inverse from the single-precision
-// LAPACK sgesvd; here it is computed with Householder QR in double precision
-// and rounded to single, so hfcoef agrees with the reference to the FP32
-// accuracy of sgesvd, not bit for bit.
+// weight; its pseudo-inverse is the single-precision SVD of lsq_svd.cpp,
+// which follows the LAPACK routines the reference vendors, so hfcoef is the
+// reference's bit for bit as well.
 #include <math.h>
 #include <string.h>
 
@@ -552,60 +551,6 @@ static void middle_r(const float *p1, const float *p2, float *p) {
   if (p[1] < 0.0f) p[1] = p[1] + 2 * pi;
 }
 
-// Moore-Penrose pseudo-inverse of the n x 5 observation matrix A (rows =
-// neighbours, columns = x, y, x^2, xy, y^2; LSfitting.F90:190-265) by
-// Householder QR in double: pinv = R^-1 Q^T, written as pinv[b*6 + o].
-static void pseudo_inverse(const float *xy, int n, float *pinv) {
-  const int m = 5;
-  double A[6][5], Q[6][6];
-  for (int i = 0; i < n; i++) {
-    double x = xy[2 * i], y = xy[2 * i + 1];
-    // the reference forms the entries in single precision
-    A[i][0] = (double)(float)x;
-    A[i][1] = (double)(float)y;
-    A[i][2] = (double)((float)x * (float)x);
-    A[i][3] = (double)((float)x * (float)y);
-    A[i][4] = (double)((float)y * (float)y);
-  }
-  for (int i = 0; i < n; i++)
-    for (int j = 0; j < n; j++) Q[i][j] = i == j ? 1. : 0.;
-  // A = Q R with Householder reflections H_j = I - 2 v v^T / (v^T v)
-  for (int j = 0; j < m; j++) {
-    double norm = 0.;
-    for (int i = j; i < n; i++) norm += A[i][j] * A[i][j];
-    norm = sqrt(norm);
-    double alpha = A[j][j] > 0 ? -norm : norm;
-    double v[6] = {0, 0, 0, 0, 0, 0};
-    for (int i = j; i < n; i++) v[i] = A[i][j];
-    v[j] -= alpha;
-    double vv = 0.;
-    for (int i = j; i < n; i++) vv += v[i] * v[i];
-    if (vv == 0.) continue;
-    for (int c = j; c < m; c++) {
-      double s = 0.;
-      for (int i = j; i < n; i++) s += v[i] * A[i][c];
-      s = 2. * s / vv;
-      for (int i = j; i < n; i++) A[i][c] -= s * v[i];
-    }
-    for (int r = 0; r < n; r++) {          // Q <- Q H_j
-      double s = 0.;
-      for (int i = j; i < n; i++) s += Q[r][i] * v[i];
-      s = 2. * s / vv;
-      for (int i = j; i < n; i++) Q[r][i] -= s * v[i];
-    }
-  }
-  // pinv = R^{-1} Q_1^T: solve R X = Q_1^T column by column
-  for (int o = 0; o < n; o++) {
-    double x[5];
-    for (int b = m - 1; b >= 0; b--) {
-      double s = Q[o][b];
-      for (int c = b + 1; c < m; c++) s -= A[b][c] * x[c];
-      x[b] = s / A[b][b];
-    }
-    for (int b = 0; b < m; b++) pinv[b * 6 + o] = (float)x[b];
-  }
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -866,12 +811,12 @@ static int icos_grid_impl(int glvl, float *lat_out, float *lon_out, int *nprox_o
       int q = prox[(size_t)ipn * 6 + isn] - 1;
       ll2xy(lon[ipn], lat[ipn], lon[q], lat[q], &loc[2 * isn], &loc[2 * isn + 1]);
     }
-    pseudo_inverse(loc, np, pinv);
+    if (fimhip_ls_pseudo_inverse(loc, np, pinv) != FIMHIP_OK) return FIMHIP_EINVAL;
     for (int isn = 0; isn < np; isn++) {
       float *h = &hf[((size_t)ipn * 6 + isn) * 6];
       for (int ico = 0; ico < 6; ico++) h[ico] = 0.f;
       for (int ico = 0; ico < np; ico++)
-        for (int ib = 0; ib < 5; ib++) h[ico] = h[ico] + pxy[((size_t)ipn * 6 + isn) * 5 + ib] * pinv[ib * 6 + ico];
+        for (int ib = 0; ib < 5; ib++) h[ico] = h[ico] + pxy[((size_t)ipn * 6 + isn) * 5 + ib] * pinv[ib + ico * 5];
     }
     // cs, sn (getlvl.F90:273-294)
     for (int isn = 0; isn < np; isn++) {

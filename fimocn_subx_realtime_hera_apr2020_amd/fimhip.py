@@ -46,7 +46,7 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ens_kernel_count", "fimhip_ens_kernel_name", "fimhip_ens_set_tile", "fimhip_ens_get_tile",
            "fimhip_ens_autotune", "fimhip_ens_profile", "fimhip_ens_profile_report",
            "fimhip_globsum_noalloc", "fimhip_ens_set_fused", "fimhip_ens_get_fused", "fimhip_ens_patches",
-           "fimhip_ens_set_wet"]
+           "fimhip_ens_set_wet", "fimhip_ls_pseudo_inverse"]
 
 # real*4 drop-ins: name -> (number of float arrays, trailing int argument)
 R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, True), "mass_adv": (12, True),
@@ -141,6 +141,7 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ens_set_fused.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_get_fused.argtypes = [vp]
     lib.fimhip_ens_set_wet.argtypes = [vp, ct.c_int, _PI]
+    lib.fimhip_ls_pseudo_inverse.argtypes = [ct.POINTER(ct.c_float), ct.c_int, ct.POINTER(ct.c_float)]
     lib.fimhip_ens_patches.argtypes = [ct.POINTER(GridDesc), _PI, _PI, ct.c_longlong, _PI, ct.c_longlong]
     lib.fimhip_ens_set_flux_scheme.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_set_class_b.argtypes = [vp, ct.c_int, ct.c_int]
@@ -189,6 +190,18 @@ def icos_grid(level: int) -> _glvl.Grid:
     ptrs = [a[n].ctypes.data_as(_PI if a[n].dtype == np.int32 else ct.POINTER(ct.c_float)) for n in order]
     _chk(lib.fimhip_icos_grid(level, *ptrs), "fimhip_icos_grid")
     return _glvl.Grid(level, 0, **a)
+
+
+def ls_pseudo_inverse(xy) -> np.ndarray:
+    """getlvl's least-squares pseudo-inverse (fimhip_ls_pseudo_inverse) of
+    nobs = 5 or 6 points xy (nobs, 2) float32: the (5, nobs) array inv."""
+    lib = load_library()
+    p = np.ascontiguousarray(xy, np.float32)
+    assert p.ndim == 2 and p.shape[1] == 2
+    inv = np.zeros((p.shape[0], 5), np.float32)       # row per observation: inv(ibf, iobs), ibf fastest
+    _chk(lib.fimhip_ls_pseudo_inverse(p.ctypes.data_as(ct.POINTER(ct.c_float)), p.shape[0],
+                                      inv.ctypes.data_as(ct.POINTER(ct.c_float))), "ls_pseudo_inverse")
+    return np.ascontiguousarray(inv.T)
 
 
 def icos_grid_info(level: int) -> _glvl.GridInfo:

@@ -233,6 +233,15 @@ int fimhip_icos_grid(int glvl, float *lat, float *lon, int *nprox, int *prox, in
  * each end of each edge as (lat, lon), inv_perm (nip) */
 int fimhip_icos_grid_info(int glvl, float *lat, float *lon, int *prox, int *nprox, float *icos_edge,
                           int *inv_perm);
+/* getlvl's least-squares pseudo-inverse (LSfitting.F90:73-130,
+ * pseudoInverse, with num_basisfuns = 5): xy holds nobs = 5 or 6 (x, y)
+ * pairs; inv (5 x nobs, basis function fastest) receives V S^-1 U^T of the
+ * observation matrix [x, y, x*x, x*y, y*y], by the single-precision
+ * sgesvd('A','A') the reference vendors (prep/lsfit/svd.f), bit for bit.
+ * FIMHIP_EINVAL for other sizes, null pointers, a matrix sgesvd would have to
+ * rescale (largest entry outside [2^-40, 2^40]) or an iteration that does
+ * not converge.  Host only. */
+int fimhip_ls_pseudo_inverse(const float *xy, int nobs, float *inv);
 
 /* ---- synthetic seeded states --------------------------------------------
  * One member's state, bit-identical on the host and on the device: splitmix64
