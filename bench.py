@@ -91,12 +91,14 @@ def class_b_kernel_bytes(B):
     return {k: 8 * n for k, n in v.items()}
 
 
-def ens_kernel_bytes(T):
+def ens_kernel_bytes(T, fused=False):
     """The same for the mixed-precision kernels (4 B per FP32 value, 8 B per
-    FP64 value; inputs shared by the tracers of one launch counted once)."""
+    FP64 value; inputs shared by the tracers of one launch counted once).
+    With the fused tracer transport edgvar3 reconstructs the thickness only."""
     return {
         "e_edgvar1": 20 + 36,                # us vs tr1 (f32) pr (f64) -> normal wind edges, pressure gradient, step-start theta (f32)
-        "e_edgvar3": 32,                     # dp (f64) -> dp_edg (the fused tracer transport forms the tracer edge values)
+        # dp (f64) -> dp_edg; tracers 2..T -> trc_edg (the fused transport forms those itself)
+        "e_edgvar3": 32 if fused else 32 + 28 * (T - 1),
         "e_cnuity12": 104,                   # normal wind and dp edges, dp, dplo(of,vof) -> massfx, antifx, dplo, delp_lo, sums
         "e_cnuity3": 44,                     # in/outflow sums, delp_lo, dp, dp_tdcy(of,vof) -> r+, r-
         "e_cnuity45": 164,                   # antifx r+- massfx dp_tdcy dp pr pressure gradient u v -> massfx dp_tdcy dp pr ex omega

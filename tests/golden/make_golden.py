@@ -21,6 +21,11 @@ Fixtures (data only):
                         relative L2 of the checker with the device pow
                         (fim_pow) against the reference after 96 steps at G3
                         and G4, as measured when the fixture was made
+  dyn_G2_r4_ref.npz     the reference built with default real 4 (dynref_r4)
+                        after R4_STEPS full steps from dyn_G2_input (start-up
+                        hybgen, dt = 240 s): final us, vs, dp, pr, ex, ph, mp,
+                        tr as float32, and the relative L2 of that build
+                        against the FP64 build (dynref_r8) per field
   dyn_G2_ref.json       SHA-256 of every output record of the reference FP64
                         build (dynref_r8) for 3 full steps (mode 0) and 3
                         transport steps (mode 1), stage dumps of step 1, the
@@ -149,7 +154,32 @@ def fp64_bar():
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+R4_STEPS = 12
+R4_FIELDS = ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr")
+
+
+def r4_reference():
+    z = np.load(os.path.join(HERE, "dyn_G2_input.npz"))
+    st = {k: z[k] for k in ("us", "vs", "dp", "pr", "ex", "ph", "tr")}
+    th, ds, dt = z["thetac"], z["dpsig"], float(z["dt"])
+    path = refrun.ref_glvl(2)
+    r4 = refrun.run_dynref(path, st, th, ds, nsteps=R4_STEPS, dt=dt, mode=0, prec="r4")
+    r8 = refrun.run_dynref(path, st, th, ds, nsteps=R4_STEPS, dt=dt, mode=0, prec="r8")
+    out, err = {}, {}
+    for f in R4_FIELDS:
+        a, b = r4["final." + f], r8["final." + f]
+        if isinstance(a, list):
+            a, b = np.stack(a), np.stack(b)
+        shape = st[f].shape if f != "mp" else st["us"].shape
+        out[f] = a.reshape(shape).astype(np.float32)
+        err[f] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    np.savez_compressed(os.path.join(HERE, "dyn_G2_r4_ref.npz"), steps=R4_STEPS, dt=dt,
+                        rel_l2_fields=np.array(R4_FIELDS), rel_l2_r4_vs_r8=np.array([err[f] for f in R4_FIELDS]),
+                        **out)
+
+
 if __name__ == "__main__":
     main()
     schemes()
     fp64_bar()
+    r4_reference()

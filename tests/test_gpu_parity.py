@@ -584,6 +584,82 @@ def test_smallest_grid_steps_match_checker_bitwise(oracle_lib, nvl, flux_scheme,
         m.close()
 
 
+def test_r4_dropins_against_the_reference_real4_build(oracle_lib):
+    """The real*4 drop-ins driven as a default-real-4 FIM build would drive
+    them (dyn_init start-up hybgen and hystat, then dyn_run's order for each
+    step) from the seeded G2 input for 12 full steps at dt = 240 s, against
+    the reference built with default real 4 (dynref_r4; final state in
+    tests/golden/dyn_G2_r4_ref.npz) and against the FP64 path (the checker
+    with the C library's pow, which equals the reference FP64 build bit for
+    bit).  The drop-ins compute in FP64 and round what each routine writes to
+    real*4, where the reference's real*4 build rounds every operation, so the
+    two are different single-precision approximations of the FP64 result:
+    the drop-ins stay within the real*4 build's own distance from FP64 (per
+    field, relative L2), and the two real*4 results within twice that
+    distance of each other.  Measured on MI355X (drop-ins vs FP64 / real*4
+    build vs FP64 / drop-ins vs real*4 build): u 7.1e-7 / 1.6e-6 / 1.8e-6,
+    dp 1.7e-5 / 1.8e-5 / 2.4e-5, p 1.0e-6 / 1.2e-6 / 1.5e-6, tracers
+    2.4e-7 / 2.6e-7 / 3.6e-7."""
+    import os
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    from fimocn_subx_realtime_hera_apr2020_amd.constants import CP, P1000, RD
+    from helpers import GOLDEN, golden_input
+    O = oracle_lib
+    st, th, ds, dt = golden_input()
+    g = golden_grid(2)
+    z = np.load(os.path.join(GOLDEN, "dyn_G2_r4_ref.npz"))
+    steps = int(z["steps"])
+    ref4_vs_8 = dict(zip([str(x) for x in z["rel_l2_fields"]], [float(x) for x in z["rel_l2_r4_vs_r8"]]))
+    print("case: G2 nvl=64 ntra=4, real*4 drop-ins for %d steps vs dynref_r4 and FP64" % steps, flush=True)
+    nip = g.nip
+    c, lv, e = (nip, NVL), (nip, NVL + 1), (nip, 6, NVL)
+    f = {n: np.ascontiguousarray(st[n], np.float32) for n in ("us", "vs", "dp", "pr", "ex", "ph", "tr")}
+    f["trdp"] = np.ascontiguousarray(f["tr"] * f["dp"][None])
+    for n, sh in dict(mp=c, sdot=lv, ws=c, relvor=c, psrf=(nip,), ptdcy=(2, nip), u_tdcy=(3,) + c, v_tdcy=(3,) + c,
+                      dp_tdcy=(3,) + c, dpl_tdcy=(3,) + c, trc_tdcy=(NTRA, 3) + c, trl_tdcy=(NTRA, 3) + c,
+                      massfx=(3,) + e, u_edg=e, v_edg=e, dp_edg=e, lp_edg=e, geop_edg=e, mont_edg=e, uvsq_edg=e,
+                      trc_edg=(NTRA,) + e).items():
+        f[n] = np.zeros(sh, np.float32)
+    th32 = np.ascontiguousarray(th, np.float32)
+    inst = fimhip.Instance(g, NVL, NTRA, dt, th, ds)
+    try:
+        r4 = inst.call_r4
+        r4("hybgen", 0, th32, f["us"], f["vs"], f["tr"], f["sdot"], f["ex"], f["dp"], f["pr"], tail=0)
+        f["pr"][...] = np.float32(P1000) * (f["ex"] / np.float32(CP)) ** (np.float32(CP) / np.float32(RD))
+        r4("hystat", 0, f["ph"], f["ex"], f["mp"], f["dp"], f["tr"], f["trdp"], f["psrf"], f["ptdcy"])
+        ek = ["us", "vs", "dp", "pr", "ex", "ph", "mp", "tr", "u_edg", "v_edg", "dp_edg", "lp_edg", "geop_edg",
+              "mont_edg", "uvsq_edg", "trc_edg"]
+        for its in range(1, steps + 1):
+            inst.abstart(its)
+            r4("edgvar1", its, *[f[k] for k in ek])
+            r4("edgvar3", its, *[f[k] for k in ek])
+            r4("cnuity", its, *[f[k] for k in ("us", "vs", "u_edg", "v_edg", "dp_edg", "lp_edg", "dp", "pr", "ex",
+                                               "dp_tdcy", "dpl_tdcy", "massfx", "ws")], tail=0)
+            r4("trcadv", its, *[f[k] for k in ("trc_edg", "tr", "trdp", "trc_tdcy", "trl_tdcy", "massfx", "dp")],
+               tail=0)
+            r4("momtum", its, *[f[k] for k in ("us", "vs", "ex", "relvor", "u_edg", "v_edg", "trc_edg", "geop_edg",
+                                               "mont_edg", "uvsq_edg", "u_tdcy", "v_tdcy", "dp")])
+            r4("hybgen", its, th32, f["us"], f["vs"], f["tr"], f["sdot"], f["ex"], f["dp"], f["pr"], tail=0)
+            r4("hystat", its, f["ph"], f["ex"], f["mp"], f["dp"], f["tr"], f["trdp"], f["psrf"], f["ptdcy"])
+    finally:
+        inst.close()
+    o = O.OracleRun(O.grid_arrays(g, NVL, dt, th, ds), NVL, NTRA, dt, pow_kind=0)
+    o.load(st)
+    o.init(True)
+    for its in range(1, steps + 1):
+        o.step(its, 0)
+    d4, d8 = {}, {}
+    for n in ("us", "vs", "dp", "pr", "ex", "ph", "mp", "tr"):
+        d4[n] = rel_l2(f[n], z[n])
+        d8[n] = rel_l2(f[n], o.a[n])
+    print("drop-ins vs reference real*4 build:", {k: "%.2e" % v for k, v in d4.items()}, flush=True)
+    print("drop-ins vs FP64:                 ", {k: "%.2e" % v for k, v in d8.items()}, flush=True)
+    print("reference real*4 build vs FP64:   ", {k: "%.2e" % v for k, v in ref4_vs_8.items()}, flush=True)
+    for n in d4:
+        assert d8[n] <= ref4_vs_8[n], (n, d8[n], ref4_vs_8[n])
+        assert d4[n] <= 2. * ref4_vs_8[n], (n, d4[n], ref4_vs_8[n])
+
+
 def test_r4_dropins_are_the_r8_path_rounded(case):
     """The real*4 drop-ins (the production build's float arrays): every
     routine's outputs are the _r8 entry's outputs on the widened inputs,
