@@ -383,7 +383,7 @@ def run_ens(a, grid, th, ds, fimhip):
     prof = e.profile_report()
     e.profile(False)
     cells = M * grid.nip * a.nvl
-    kb = ens_kernel_bytes(a.ntra)
+    kb = ens_kernel_bytes(a.ntra, e.fused())
     value = cells * a.steps / wall
     b_step = ens_step_bytes(a.ntra, a.mode)
     tr_of = (lambda k: pmc_traffic(k, "ens16", a)) if a.mode == 0 else None

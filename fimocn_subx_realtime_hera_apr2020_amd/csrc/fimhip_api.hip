@@ -331,6 +331,34 @@ static int d2h(fimhip_ctx *c, int f, double *h) {
   return 0;
 }
 
+// One Adams-Bashforth slot (1-based, as nf/of/vof) of a tendency or mass-flux
+// field, whose outermost index is the slot ((nvl,nip,3), (nvl,npp,nip,3)) or
+// the slot within each tracer ((nvl,nip,3,ntra)): the drop-ins move only the
+// slots a routine reads (of, vof) or writes (nf) across PCIe, and the caller's
+// other slots stay as they were, as in the reference.
+static int ab_slot(fimhip_ctx *c, int f, double *h, int slot, bool to_device) {
+  if (!h || slot < 1 || slot > 3) return FIMHIP_EINVAL;
+  double **p = field_ptr(c, f);
+  if (!*p) return FIMHIP_EINVAL;
+  const int ntr = (f == FIMHIP_TRC_TDCY || f == FIMHIP_TRL_TDCY) ? c->ntra : 1;
+  const size_t chunk = (size_t)fimhip_field_size(c, f) / (3 * (size_t)ntr);
+  for (int t = 0; t < ntr; t++) {
+    const size_t off = ((size_t)t * 3 + (slot - 1)) * chunk;
+    if (to_device)
+      HIPCHK(hipMemcpyAsync(*p + off, h + off, chunk * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    else
+      HIPCHK(hipMemcpyAsync(h + off, *p + off, chunk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  return 0;
+}
+// the slots an AB update reads (of, vof) to the device; the one it writes (nf) back
+static int ab_in(fimhip_ctx *c, int f, const double *h) {
+  int r = ab_slot(c, f, const_cast<double *>(h), c->ab.of, true);
+  return r ? r : ab_slot(c, f, const_cast<double *>(h), c->ab.vof, true);
+}
+static int ab_nf_in(fimhip_ctx *c, int f, const double *h) { return ab_slot(c, f, const_cast<double *>(h), c->ab.nf, true); }
+static int ab_out(fimhip_ctx *c, int f, double *h) { return ab_slot(c, f, h, c->ab.nf, false); }
+
 extern "C" int fimhip_upload(fimhip_ctx *c, int f, const double *h) {
   if (!c || f < 0 || f >= FIMHIP_NFIELDS) return FIMHIP_EINVAL;
   HIPCHK(hipSetDevice(c->device));
@@ -405,12 +433,12 @@ extern "C" int fimhip_cnuity_r8(fimhip_ctx *c, int its, const double *u_velo, co
   TRY(h2d(c, FIMHIP_US, u_velo)); TRY(h2d(c, FIMHIP_VS, v_velo)); TRY(h2d(c, FIMHIP_U_EDG, u_edg));
   TRY(h2d(c, FIMHIP_V_EDG, v_edg)); TRY(h2d(c, FIMHIP_DP_EDG, dp_edg)); TRY(h2d(c, FIMHIP_LP_EDG, lp_edg));
   TRY(h2d(c, FIMHIP_DP, delp)); TRY(h2d(c, FIMHIP_PR, pres)); TRY(h2d(c, FIMHIP_EX, exner));
-  TRY(h2d(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(h2d(c, FIMHIP_DPL_TDCY, dplo_tndcy));
-  TRY(h2d(c, FIMHIP_MASSFX, massfx));
+  TRY(ab_in(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(ab_in(c, FIMHIP_DPL_TDCY, dplo_tndcy));
+  TRY(ab_nf_in(c, FIMHIP_MASSFX, massfx));     // a pentagon's sixth edge passes through
   launch_cnuity(c->g, c->s, c->ab, c->stream, true);
   TRY(d2h(c, FIMHIP_DP, delp)); TRY(d2h(c, FIMHIP_PR, pres)); TRY(d2h(c, FIMHIP_EX, exner));
-  TRY(d2h(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(d2h(c, FIMHIP_DPL_TDCY, dplo_tndcy));
-  TRY(d2h(c, FIMHIP_MASSFX, massfx)); TRY(d2h(c, FIMHIP_WS, omega));
+  TRY(ab_out(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(ab_out(c, FIMHIP_DPL_TDCY, dplo_tndcy));
+  TRY(ab_out(c, FIMHIP_MASSFX, massfx)); TRY(d2h(c, FIMHIP_WS, omega));
   return finish(c);
 }
 
@@ -424,10 +452,10 @@ extern "C" int fimhip_mass_adv_r8(fimhip_ctx *c, int its, const double *u_velo, 
   TRY(h2d(c, FIMHIP_US, u_velo)); TRY(h2d(c, FIMHIP_VS, v_velo)); TRY(h2d(c, FIMHIP_U_EDG, u_edg));
   TRY(h2d(c, FIMHIP_V_EDG, v_edg)); TRY(h2d(c, FIMHIP_DP_EDG, dp_edg)); TRY(h2d(c, FIMHIP_LP_EDG, lp_edg));
   TRY(h2d(c, FIMHIP_DP, delp)); TRY(h2d(c, FIMHIP_PR, pres)); TRY(h2d(c, FIMHIP_EX, exner));
-  TRY(h2d(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(h2d(c, FIMHIP_MASSFX, massfx));
+  TRY(ab_in(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(ab_nf_in(c, FIMHIP_MASSFX, massfx));
   launch_mass_adv(c->g, c->s, c->ab, c->stream, true);
   TRY(d2h(c, FIMHIP_DP_EDG, dp_edg)); TRY(d2h(c, FIMHIP_DP, delp)); TRY(d2h(c, FIMHIP_PR, pres));
-  TRY(d2h(c, FIMHIP_EX, exner)); TRY(d2h(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(d2h(c, FIMHIP_MASSFX, massfx));
+  TRY(d2h(c, FIMHIP_EX, exner)); TRY(ab_out(c, FIMHIP_DP_TDCY, dp_tndcy)); TRY(ab_out(c, FIMHIP_MASSFX, massfx));
   TRY(d2h(c, FIMHIP_WS, omega));
   return finish(c);
 }
@@ -476,11 +504,11 @@ extern "C" int fimhip_trcadv_r8(fimhip_ctx *c, int its, const double *trcr_edg, 
   if (!c) return FIMHIP_EINVAL;
   HIPCHK(hipSetDevice(c->device));
   TRY(h2d(c, FIMHIP_TRC_EDG, trcr_edg)); TRY(h2d(c, FIMHIP_TR, tracr)); TRY(h2d(c, FIMHIP_TRDP, trcdp));
-  TRY(h2d(c, FIMHIP_TRC_TDCY, trc_tdcy)); TRY(h2d(c, FIMHIP_TRL_TDCY, trclo_tdcy));
-  TRY(h2d(c, FIMHIP_MASSFX, massfx)); TRY(h2d(c, FIMHIP_DP, delp));
+  TRY(ab_in(c, FIMHIP_TRC_TDCY, trc_tdcy)); TRY(ab_in(c, FIMHIP_TRL_TDCY, trclo_tdcy));
+  TRY(ab_nf_in(c, FIMHIP_MASSFX, massfx)); TRY(h2d(c, FIMHIP_DP, delp));
   launch_trcadv(c->g, c->s, c->ab, c->stream, true);
   TRY(d2h(c, FIMHIP_TR, tracr)); TRY(d2h(c, FIMHIP_TRDP, trcdp));
-  TRY(d2h(c, FIMHIP_TRC_TDCY, trc_tdcy)); TRY(d2h(c, FIMHIP_TRL_TDCY, trclo_tdcy));
+  TRY(ab_out(c, FIMHIP_TRC_TDCY, trc_tdcy)); TRY(ab_out(c, FIMHIP_TRL_TDCY, trclo_tdcy));
   return finish(c);
 }
 
@@ -494,10 +522,10 @@ extern "C" int fimhip_momtum_r8(fimhip_ctx *c, int its, double *u_velo, double *
   TRY(h2d(c, FIMHIP_US, u_velo)); TRY(h2d(c, FIMHIP_VS, v_velo)); TRY(h2d(c, FIMHIP_EX, exner));
   TRY(h2d(c, FIMHIP_U_EDG, u_edg)); TRY(h2d(c, FIMHIP_V_EDG, v_edg)); TRY(h2d(c, FIMHIP_TRC_EDG, trcr_edg));
   TRY(h2d(c, FIMHIP_MONT_EDG, mont_edg)); TRY(h2d(c, FIMHIP_UVSQ_EDG, uvsq_edg));
-  TRY(h2d(c, FIMHIP_U_TDCY, u_tndcy)); TRY(h2d(c, FIMHIP_V_TDCY, v_tndcy)); TRY(h2d(c, FIMHIP_DP, dp3d));
+  TRY(ab_in(c, FIMHIP_U_TDCY, u_tndcy)); TRY(ab_in(c, FIMHIP_V_TDCY, v_tndcy)); TRY(h2d(c, FIMHIP_DP, dp3d));
   launch_momtum(c->g, c->s, c->ab, c->stream);
   TRY(d2h(c, FIMHIP_US, u_velo)); TRY(d2h(c, FIMHIP_VS, v_velo)); TRY(d2h(c, FIMHIP_RELVOR, relvor));
-  TRY(d2h(c, FIMHIP_U_TDCY, u_tndcy)); TRY(d2h(c, FIMHIP_V_TDCY, v_tndcy));
+  TRY(ab_out(c, FIMHIP_U_TDCY, u_tndcy)); TRY(ab_out(c, FIMHIP_V_TDCY, v_tndcy));
   return finish(c);
 }
 
