@@ -112,3 +112,63 @@ def make_ocean_state(lat, lon, nvl: int = NVL, seed: int = 20240605) -> dict:
     del fimhip
     return dict(us=np.ascontiguousarray(us), vs=np.ascontiguousarray(vs), dp=np.ascontiguousarray(dp),
                 pr=pr, ex=ex, ph=ph, tr=np.ascontiguousarray(tr), wet=wet)
+
+
+# ---------------------------------------------------------------------------
+# Inputs of the HYCOM ocean continuity equation (hycom_cnuity.F90, cnuity):
+# HYCOM's own units and orientation (pressure in Pa, 1 m of water = rho*g Pa,
+# layer 1 at the surface, pres(1) the surface pressure).
+RHO_OCN = 1000.0                  # hycom_constants.F90: rho
+ONEM = RHO_OCN * C.GRVITY         # rho * grvity: 1 m of water in Pa
+
+
+def hycom_cnuity_case(grid, kdm: int = 32, seed: int = 20240607, dtleap: float = 1800.0) -> dict:
+    """A seeded input set for cnuity on a glvl.Grid: the land mask 'wet'
+    (nip,) int32 (a smooth field below its 30th percentile is land, and so
+    is an ocean cell with no ocean neighbour); the
+    thickness dp (2, nip, kdm) at the two leapfrog levels and the interface
+    pressure pres (nip, kdm+1) (a smooth bathymetry of 2400-4000 m filled by
+    the ocn_dpmin layers (the first kdm, surface first) stretched per column,
+    times 1 + 0.3 U[-1,1] so the limiter has work, the second level 1 %
+    apart); edge velocities um/vm (mid time) and un/vn (old), (nip, 6, kdm),
+    each edge's value the mean of its two cells' solid-body-plus-noise
+    currents (0.3 m/s, N(0, 0.1)), so both cells of an edge see the same
+    one; the mid-time edge thickness dpm_edg likewise; the mass fluxes
+    massflx and mssfx_avg (nip, 6, kdm) zero.  Land cells and edges carry
+    the same construction; cnuity never reads them."""
+    rng = np.random.default_rng(seed)
+    lat = np.asarray(grid.lat, np.float64)
+    lon = np.asarray(grid.lon, np.float64)
+    nip = lat.shape[0]
+    prox = grid.dyn_prox().astype(np.int64) - 1
+    wet = land_mask(lat, lon, seed)
+    # an ocean cell with no ocean neighbour would make the biharmonic
+    # smoothing's neighbour average 0/0: such cells become land
+    nwet = (wet[prox] > 0).sum(axis=1)
+    wet = np.where(nwet > 0, wet, 0).astype(np.int32)
+    f = smooth_field(lat, lon, seed + 3)
+    depth = 2400.0 + 1600.0 * (np.argsort(np.argsort(f)) + 0.5) / nip
+    dpmin_m = np.array(C.OCN_DPMIN32[:kdm], np.float64)
+    dz = dpmin_m[None, :] * (depth / dpmin_m.sum())[:, None] * (1.0 + 0.3 * rng.uniform(-1.0, 1.0, (nip, kdm)))
+    dp0 = dz * ONEM
+    dp1 = dp0 * (1.0 + 0.01 * rng.uniform(-1.0, 1.0, (nip, kdm)))
+    dp = np.stack([dp0, dp1])
+    pres = np.zeros((nip, kdm + 1))
+    for k in range(kdm):
+        pres[:, k + 1] = pres[:, k] + dp0[:, k]
+
+    def cell_currents():
+        alpha = np.pi / 4
+        u0 = 0.3 * (np.cos(lat) * np.cos(alpha) + np.sin(lat) * np.cos(lon) * np.sin(alpha))
+        v0 = -0.3 * np.sin(lon) * np.sin(alpha)
+        return (u0[:, None] + 0.1 * rng.normal(size=(nip, kdm)), v0[:, None] + 0.1 * rng.normal(size=(nip, kdm)))
+
+    def on_edges(c):
+        return np.ascontiguousarray(0.5 * (c[:, None, :] + c[prox]))
+
+    um, vm = cell_currents()
+    un, vn = cell_currents()
+    z = np.zeros((nip, 6, kdm))
+    return dict(kdm=kdm, dtleap=dtleap, wet=wet, dp=np.ascontiguousarray(dp), pres=pres,
+                um_edg=on_edges(um), vm_edg=on_edges(vm), un_edg=on_edges(un), vn_edg=on_edges(vn),
+                dpm_edg=on_edges(dp[0]), massflx=z.copy(), mssfx_avg=z.copy())

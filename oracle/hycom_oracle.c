@@ -1,0 +1,350 @@
+/* hycom_oracle.c - CPU restatement, in plain C and FP64, of the HYCOM ocean
+ * continuity equation (FIMsrc/fim/horizontal/hycom_cnuity.F90, subroutine
+ * cnuity).  TEST INFRASTRUCTURE ONLY: the checker the HIP kernels are
+ * compared with; the product never links it.
+ *
+ * Every loop follows the reference operation for operation, in the
+ * reference loop order, so that with fp contraction off it reproduces the
+ * reference built with -fdefault-real-8 (oracle/_ref/bin/hycomref) bit for
+ * bit.  Diagnostics (prints, stencil dumps) are left out; the reference's
+ * duplicate-flux check (chkdup, hycom_cnuity.F90:1030-1075), which stops the
+ * program, returns 1 here instead.
+ *
+ * Layout (C index, level k fastest, cells i, 0-based):
+ *   dp      (kdm,nip,2)     dp[(s*nip + i)*kdm + k]      s = leapfrog slot-1
+ *   pres    (kdm+1,nip)     pres[i*(kdm+1) + k]
+ *   edge    (kdm,npp,nip)   a[(i*6 + e)*kdm + k]
+ * Grid: prox/proxs 0-based [i*6+e] (a pentagon's slot 6 repeats slot 5),
+ * sidevec_e [(i*6+e)*2 + c], sideln/rsideln/rprox_ln [i*6+e].
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "hycom_oracle.h"
+
+#define NPP 6
+/* hycom_constants.F90:30-77, module_constants.F90:19 */
+static const double GRVITY = 9.80665;
+static const double ONEM = 1000. * 9.80665;
+#define ONECM (0.01 * ONEM)
+static const double ASSLDP = .25;
+static const double LAND_SPVAL = -1.e33;
+static const double EPSIL = 1.e-11;
+
+static double dmax2(double a, double b) { return a > b ? a : b; }
+static double dmin2(double a, double b) { return a < b ? a : b; }
+
+int fo_hycom_cnuity(const fo_hyc_grid *g, fo_hyc_mass *ms, int nstep, int leapm, int leapn,
+                    const double *um_edg, const double *vm_edg, const double *un_edg, const double *vn_edg,
+                    const double *dpm_edg, double *dp, double *pres, double *massflx, double *mssfx_avg) {
+  const int nip = g->nip, kdm = g->kdm, kp = kdm + 1;
+  const double dtleap = g->dtleap, dtinv = 1. / dtleap;
+  const size_t cz = (size_t)nip * kdm, ez = cz * NPP;
+  double *antiflx = malloc(ez * sizeof(double)), *bolusfx = malloc(ez * sizeof(double));
+  double *flxdv = malloc(cz * sizeof(double)), *dp_lo = malloc(cz * sizeof(double));
+  double *dpold = malloc(cz * sizeof(double)), *flxplus = malloc(cz * sizeof(double));
+  double *flxmnus = malloc(cz * sizeof(double)), *rplus = malloc(cz * sizeof(double));
+  double *rmnus = malloc(cz * sizeof(double)), *del2p = malloc(cz * sizeof(double));
+  double *dptmp = malloc(cz * sizeof(double)), *clippd = malloc((size_t)nip * NPP * sizeof(double));
+  double *rpbot = malloc((size_t)nip * sizeof(double)), *aux1 = malloc((size_t)nip * sizeof(double));
+  double *sum = malloc((size_t)(kdm + 1) * sizeof(double));
+  memset(dpold, 0, cz * sizeof(double));
+  memset(rpbot, 0, (size_t)nip * sizeof(double));
+  double *dpm = dp + (size_t)(leapm - 1) * cz, *dpn = dp + (size_t)(leapn - 1) * cz;
+#define C(a, i, k) (a)[(size_t)(i) * kdm + (k)]
+#define P(i, k) pres[(size_t)(i) * kp + (k)]
+#define E(a, i, e, k) (a)[((size_t)(i) * NPP + (e)) * kdm + (k)]
+#define SVE(i, e, c) g->sidevec_e[((size_t)(i) * NPP + (e)) * 2 + (c)]
+#define WET(i) (g->wet[i] > 0)
+  /* low- and high-order fluxes (hycom_cnuity.F90:98-159) */
+  for (int i = 0; i < nip; i++) {
+    for (int e = 0; e < NPP; e++)
+      for (int k = 0; k < kdm; k++) {
+        E(antiflx, i, e, k) = LAND_SPVAL;
+        E(massflx, i, e, k) = LAND_SPVAL;
+      }
+    if (!WET(i)) continue;
+    for (int e = 0; e < g->nprox[i]; e++) {
+      const int ix = g->prox[i * NPP + e];
+      if (!WET(ix)) {
+        for (int k = 0; k < kdm; k++) {
+          E(massflx, i, e, k) = 0.;
+          E(antiflx, i, e, k) = 0.;
+        }
+        continue;
+      }
+      const int ex = g->proxs[i * NPP + e];
+      for (int k = 0; k < kdm; k++) {
+        double valu1 = SVE(i, e, 1) * E(um_edg, i, e, k) - SVE(i, e, 0) * E(vm_edg, i, e, k);
+        double valu2 = SVE(ix, ex, 1) * E(um_edg, ix, ex, k) - SVE(ix, ex, 0) * E(vm_edg, ix, ex, k);
+        const double outgom = .5 * (valu1 - valu2);
+        valu1 = g->rsideln[i * NPP + e] * dtleap * g->rprox_ln[i * NPP + e];
+        valu2 = g->rsideln[ix * NPP + ex] * dtleap * g->rprox_ln[ix * NPP + ex];
+        const double q = dmin2(fabs(outgom) * .5 * (valu1 + valu2), 1.);
+        const double dpedg = .5 * (E(dpm_edg, i, e, k) + E(dpm_edg, ix, ex, k));
+        double hi;
+        if (outgom >= 0.) hi = dpedg * (1. - q) + C(dpm, i, k) * q;
+        else hi = dpedg * (1. - q) + C(dpm, ix, k) * q;
+        hi = hi * outgom;
+        valu1 = SVE(i, e, 1) * E(un_edg, i, e, k) - SVE(i, e, 0) * E(vn_edg, i, e, k);
+        valu2 = SVE(ix, ex, 1) * E(un_edg, ix, ex, k) - SVE(ix, ex, 0) * E(vn_edg, ix, ex, k);
+        const double outgon = .5 * (valu1 - valu2);
+        const double lo = outgon >= 0. ? outgon * C(dpn, i, k) : outgon * C(dpn, ix, k);
+        E(antiflx, i, e, k) = hi - lo;
+        E(massflx, i, e, k) = lo;
+      }
+    }
+  }
+  /* low-order thickness (hycom_cnuity.F90:169-205) */
+  for (int i = 0; i < nip; i++) {
+    for (int k = 0; k < kdm; k++) C(dp_lo, i, k) = C(flxdv, i, k) = 0.;
+    if (!WET(i)) continue;
+    for (int e = 0; e < g->nprox[i]; e++)
+      for (int k = 0; k < kdm; k++) C(flxdv, i, k) = C(flxdv, i, k) + E(massflx, i, e, k);
+    for (int k = 0; k < kdm; k++) {
+      C(dpold, i, k) = C(dpn, i, k);
+      P(i, k + 1) = P(i, k) + C(dpn, i, k);
+      C(dp_lo, i, k) = C(dpn, i, k) - C(flxdv, i, k) * g->rarea[i] * dtleap;
+      if (C(dp_lo, i, k) < 0. && C(dp_lo, i, k) + ONECM > 0.) C(dp_lo, i, k) = 0.;
+    }
+  }
+  /* in- and outgoing antidiffusive fluxes (246-265) */
+  for (int i = 0; i < nip; i++) {
+    for (int k = 0; k < kdm; k++) {
+      C(flxplus, i, k) = EPSIL;
+      C(flxmnus, i, k) = -EPSIL;
+    }
+    if (!WET(i)) continue;
+    for (int e = 0; e < g->nprox[i]; e++)
+      for (int k = 0; k < kdm; k++) {
+        if (E(antiflx, i, e, k) >= 0.) C(flxplus, i, k) = C(flxplus, i, k) + E(antiflx, i, e, k);
+        else C(flxmnus, i, k) = C(flxmnus, i, k) + E(antiflx, i, e, k);
+      }
+  }
+  /* limiter ratios from the local bounds (269-300) */
+  for (int i = 0; i < nip; i++) {
+    for (int k = 0; k < kdm; k++) C(rplus, i, k) = C(rmnus, i, k) = 0.;
+    if (!WET(i)) continue;
+    double dpmax[kdm], dpmin[kdm];
+    for (int k = 0; k < kdm; k++) {
+      dpmax[k] = dmax2(C(dp_lo, i, k), C(dpn, i, k));
+      dpmin[k] = dmin2(C(dp_lo, i, k), C(dpn, i, k));
+    }
+    for (int e = 0; e < g->nprox[i]; e++) {
+      const int ix = g->prox[i * NPP + e];
+      if (!WET(ix)) continue;
+      for (int k = 0; k < kdm; k++) {
+        dpmax[k] = dmax2(dmax2(dpmax[k], C(dp_lo, ix, k)), C(dpn, ix, k));
+        dpmin[k] = dmin2(dmin2(dpmin[k], C(dp_lo, ix, k)), C(dpn, ix, k));
+      }
+    }
+    for (int k = 0; k < kdm; k++) {
+      dpmin[k] = dmax2(0., dpmin[k]);
+      C(rmnus, i, k) = (C(dp_lo, i, k) - dpmin[k]) * g->area[i] / (dtleap * C(flxplus, i, k));
+      C(rplus, i, k) = (C(dp_lo, i, k) - dpmax[k]) * g->area[i] / (dtleap * C(flxmnus, i, k));
+    }
+  }
+  /* limit the antidiffusive fluxes, keeping the clipped part (315-352) */
+  for (int i = 0; i < nip; i++) {
+    for (int e = 0; e < NPP; e++) clippd[i * NPP + e] = 0.;
+    if (!WET(i)) continue;
+    for (int e = 0; e < g->nprox[i]; e++) {
+      const int ix = g->prox[i * NPP + e];
+      if (!WET(ix)) continue;
+      for (int k = 0; k < kdm; k++) {
+        const double a = E(antiflx, i, e, k);
+        const double clip = a >= 0. ? dmin2(dmin2(1., C(rmnus, i, k)), C(rplus, ix, k))
+                                    : dmin2(dmin2(1., C(rplus, i, k)), C(rmnus, ix, k));
+        clippd[i * NPP + e] = clippd[i * NPP + e] + a * (1. - clip);
+        E(antiflx, i, e, k) = a * clip;
+        E(massflx, i, e, k) = E(massflx, i, e, k) + E(antiflx, i, e, k);
+      }
+    }
+  }
+  /* the antidiffusive update (357-393) */
+  for (int i = 0; i < nip; i++) {
+    for (int k = 0; k < kdm; k++) C(flxdv, i, k) = 0.;
+    if (!WET(i)) continue;
+    for (int e = 0; e < g->nprox[i]; e++)
+      for (int k = 0; k < kdm; k++) C(flxdv, i, k) = C(flxdv, i, k) + E(antiflx, i, e, k);
+    for (int k = 0; k < kdm; k++) {
+      C(dpn, i, k) = C(dp_lo, i, k) - C(flxdv, i, k) * g->rarea[i] * dtleap;
+      if (C(dpn, i, k) < 0. && C(dpn, i, k) + ONECM > 0.) C(dpn, i, k) = 0.;
+      P(i, k + 1) = P(i, k) + C(dpn, i, k);
+    }
+    rpbot[i] = 1. / P(i, kdm);
+  }
+  /* the clipped fluxes back as an upstream barotropic correction (434-491) */
+  for (int i = 0; i < nip; i++) {
+    for (int k = 0; k < kdm; k++) C(flxdv, i, k) = 0.;
+    if (!WET(i)) continue;
+    for (int e = 0; e < g->nprox[i]; e++) {
+      const int ix = g->prox[i * NPP + e];
+      if (!WET(ix)) continue;
+      const double cl = clippd[i * NPP + e];
+      for (int k = 0; k < kdm; k++) {
+        double q = 0.;
+        if (cl > 0.) q = cl * (C(dpn, i, k) * rpbot[i]);
+        else if (cl < 0.) q = cl * (C(dpn, ix, k) * rpbot[ix]);
+        C(flxdv, i, k) = C(flxdv, i, k) + q;
+        E(massflx, i, e, k) = E(massflx, i, e, k) + q;
+      }
+    }
+  }
+  for (int i = 0; i < nip; i++) {
+    if (!WET(i)) continue;
+    for (int k = 0; k < kdm; k++) {
+      C(dpn, i, k) = C(dpn, i, k) - C(flxdv, i, k) * g->rarea[i] * dtleap;
+      if (C(dpn, i, k) < 0. && C(dpn, i, k) + ONECM > 0.) C(dpn, i, k) = 0.;
+      P(i, k + 1) = P(i, k) + C(dpn, i, k);
+    }
+  }
+  if (g->thkdff != 0.) {
+    /* interface smoothing (bolus fluxes, 508-835) */
+    for (int i = 0; i < nip; i++) {
+      if (!WET(i)) continue;
+      for (int k = 0; k < kdm; k++) C(dpn, i, k) = P(i, k + 1) - P(i, k);
+    }
+    for (int i = 0; i < nip; i++) {
+      for (int k = 0; k < kdm; k++) C(del2p, i, k) = LAND_SPVAL;
+      if (g->biharm == 0.)
+        for (int k = 1; k < kdm; k++) C(del2p, i, k) = P(i, k);
+    }
+    if (g->biharm > 0.) {
+      for (int i = 0; i < nip; i++) {
+        if (!WET(i)) continue;
+        for (int k = 0; k < kdm; k++) sum[k] = 0.;
+        int ncount = 0;
+        for (int e = 0; e < g->nprox[i]; e++) {
+          const int ix = g->prox[i * NPP + e];
+          if (!WET(ix)) continue;
+          ncount = ncount + 1;
+          for (int k = 1; k < kdm; k++) {
+            double deltap;
+            if (P(ix, kdm) < P(i, k)) deltap = P(ix, kdm) - P(ix, k);
+            else if (P(i, kdm) < P(ix, k)) deltap = P(i, k) - P(i, kdm);
+            else deltap = P(i, k) - P(ix, k);
+            sum[k] = sum[k] + deltap;
+          }
+        }
+        for (int k = 1; k < kdm; k++)
+          C(del2p, i, k) = g->biharm * sum[k] / (double)ncount + (1. - g->biharm) * P(i, k);
+      }
+    }
+    for (int i = 0; i < nip; i++) {
+      for (int e = 0; e < NPP; e++)
+        for (int k = 0; k < kdm; k++) E(bolusfx, i, e, k) = WET(i) ? 0. : LAND_SPVAL;
+      if (!WET(i)) continue;
+      for (int e = 0; e < g->nprox[i]; e++) {
+        const int ix = g->prox[i * NPP + e];
+        if (!WET(ix)) continue;
+        for (int k = 1; k < kdm; k++) {
+          const double q = dtleap * g->thkdff * (C(del2p, i, k) - C(del2p, ix, k)) * g->sideln[i * NPP + e];
+          const double flxmx = (P(ix, kdm) - P(ix, k)) * g->area[ix] / (double)g->nprox[ix];
+          const double flxmn = -((P(i, kdm) - P(i, k)) * g->area[i] / (double)g->nprox[i]);
+          E(bolusfx, i, e, k) = dmin2(flxmx * .999, dmax2(flxmn * .999, q));
+          E(bolusfx, i, e, k - 1) = E(bolusfx, i, e, k) - E(bolusfx, i, e, k - 1);
+        }
+        E(bolusfx, i, e, kdm - 1) = -E(bolusfx, i, e, kdm - 1);
+      }
+    }
+    if (g->biharm > 0.) {
+      for (int i = 0; i < nip; i++) {
+        for (int k = 0; k < kdm; k++) C(flxplus, i, k) = EPSIL;
+        if (!WET(i)) continue;
+        for (int e = 0; e < g->nprox[i]; e++)
+          for (int k = 0; k < kdm; k++) C(flxplus, i, k) = C(flxplus, i, k) + dmax2(0., E(bolusfx, i, e, k));
+        for (int k = 0; k < kdm; k++) C(rmnus, i, k) = C(dpn, i, k) * g->area[i] / C(flxplus, i, k);
+      }
+      for (int i = 0; i < nip; i++) {
+        for (int e = 0; e < NPP; e++) clippd[i * NPP + e] = 0.;
+        if (!WET(i)) continue;
+        for (int e = 0; e < g->nprox[i]; e++) {
+          const int ix = g->prox[i * NPP + e];
+          if (!WET(ix)) continue;
+          for (int k = 0; k < kdm; k++) {
+            const double b = E(bolusfx, i, e, k);
+            if (b != 0.) {
+              const double clip = b > 0. ? dmin2(1., C(rmnus, i, k)) : dmin2(1., C(rmnus, ix, k));
+              clippd[i * NPP + e] = clippd[i * NPP + e] + b * (1. - clip);
+              E(bolusfx, i, e, k) = b * clip;
+            }
+          }
+        }
+      }
+    }
+    for (int i = 0; i < nip; i++) {
+      for (int k = 0; k < kdm; k++) C(flxdv, i, k) = 0.;
+      if (!WET(i)) continue;
+      for (int e = 0; e < g->nprox[i]; e++)
+        for (int k = 0; k < kdm; k++) C(flxdv, i, k) = C(flxdv, i, k) + E(bolusfx, i, e, k);
+      for (int k = 0; k < kdm; k++) {
+        C(dpn, i, k) = C(dpn, i, k) - C(flxdv, i, k) * g->rarea[i];
+        if (C(dpn, i, k) < 0. && C(dpn, i, k) + ONECM > 0.) C(dpn, i, k) = 0.;
+        C(dpn, i, k) = C(dpn, i, k) * (1. + ms->masscor);
+        P(i, k + 1) = P(i, k) + C(dpn, i, k);
+        C(dptmp, i, k) = C(dpn, i, k);
+      }
+      rpbot[i] = 1. / P(i, kdm);
+    }
+    if (g->biharm > 0.) {
+      for (int i = 0; i < nip; i++) {
+        for (int k = 0; k < kdm; k++) C(flxdv, i, k) = 0.;
+        if (!WET(i)) continue;
+        for (int e = 0; e < g->nprox[i]; e++) {
+          const int ix = g->prox[i * NPP + e];
+          const double cl = clippd[i * NPP + e];
+          double q = 0.;
+          for (int k = 0; k < kdm; k++) {
+            if (cl > 0.) q = cl * C(dptmp, i, k) * rpbot[i];
+            else if (cl < 0.) q = cl * C(dptmp, ix, k) * rpbot[ix];
+            C(flxdv, i, k) = C(flxdv, i, k) + q;
+            E(massflx, i, e, k) = E(massflx, i, e, k) + q * dtinv;
+          }
+        }
+        for (int k = 0; k < kdm; k++) {
+          C(dpn, i, k) = C(dpn, i, k) - C(flxdv, i, k) * g->rarea[i];
+          P(i, k + 1) = P(i, k) + C(dpn, i, k);
+        }
+      }
+    }
+    /* the instantaneous mass flux into its time average (896-905) */
+    for (int i = 0; i < nip; i++) {
+      if (!WET(i)) continue;
+      for (int e = 0; e < g->nprox[i]; e++)
+        for (int k = 0; k < kdm; k++) E(mssfx_avg, i, e, k) = E(mssfx_avg, i, e, k) + E(massflx, i, e, k);
+    }
+  }
+  /* time smoothing of the mid-time thickness (935-954) */
+  for (int i = 0; i < nip; i++) {
+    if (!WET(i)) continue;
+    for (int k = 0; k < kdm; k++)
+      C(dpm, i, k) = ASSLDP * (C(dpold, i, k) + C(dpn, i, k)) + (1. - ASSLDP - ASSLDP) * C(dpm, i, k);
+  }
+  /* duplicate edge fluxes must be bit-identical up to sign (chkdup) */
+  int bad = 0;
+  for (int i = 0; i < nip && !bad; i++) {
+    if (!WET(i)) continue;
+    for (int e = 0; e < g->nprox[i]; e++) {
+      const int ix = g->prox[i * NPP + e], ex = g->proxs[i * NPP + e];
+      if (!WET(ix)) continue;
+      for (int k = 0; k < kdm; k++)
+        if (E(massflx, i, e, k) + E(massflx, ix, ex, k) != 0. && E(massflx, i, e, k) - E(massflx, ix, ex, k) != 0.)
+          bad = 1;
+    }
+  }
+  /* drift correction factor for the total mass (977-998), glob2d in cell order */
+  const double day = (double)nstep * .5 * dtleap / 86400.;
+  if (nstep == 1 || fmod(day + .0001, 10.) < .0002) {
+    double tot = 0.;
+    for (int i = 0; i < nip; i++)
+      if (WET(i)) tot = tot + P(i, kdm) * g->area[i];
+    ms->totmass = tot / GRVITY;
+    if (nstep == 1) ms->massglb0 = ms->totmass;
+    ms->masscor = dtleap * (ms->massglb0 - ms->totmass) / (ms->massglb0 * 10. * 86400.);
+  }
+  free(antiflx); free(bolusfx); free(flxdv); free(dp_lo); free(dpold); free(flxplus); free(flxmnus);
+  free(rplus); free(rmnus); free(del2p); free(dptmp); free(clippd); free(rpbot); free(aux1); free(sum);
+  return bad;
+}

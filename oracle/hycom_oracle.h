@@ -1,0 +1,22 @@
+/* hycom_oracle.h - the CPU checker of the HYCOM ocean continuity equation
+ * (test infrastructure only).  See hycom_oracle.c for the array layouts. */
+#pragma once
+
+typedef struct {
+  int nip, kdm;
+  const int *prox, *proxs, *nprox, *wet;     /* 0-based neighbour tables; wet > 0: ocean */
+  const double *area, *rarea, *sidevec_e, *sideln, *rsideln, *rprox_ln;
+  double dtleap, thkdff, biharm;
+} fo_hyc_grid;
+
+/* hycom_variables.F90:128-135: the total mass, its value at the first step
+ * and the thickness factor that corrects its drift */
+typedef struct {
+  double totmass, massglb0, masscor;
+} fo_hyc_mass;
+
+/* hycom_cnuity.F90 cnuity(nstep, leapm, leapn, ...); returns 1 where the
+ * reference's duplicate-flux check would stop the program */
+int fo_hycom_cnuity(const fo_hyc_grid *g, fo_hyc_mass *ms, int nstep, int leapm, int leapn,
+                    const double *um_edg, const double *vm_edg, const double *un_edg, const double *vn_edg,
+                    const double *dpm_edg, double *dp, double *pres, double *massflx, double *mssfx_avg);

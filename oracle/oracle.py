@@ -17,10 +17,11 @@ CSRC = os.path.join(os.path.dirname(HERE), "fimocn_subx_realtime_hera_apr2020_am
 
 
 def build(force: bool = False) -> str:
-    src = os.path.join(HERE, "fim_oracle.c")
-    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(src):
+    srcs = [os.path.join(HERE, "fim_oracle.c"), os.path.join(HERE, "hycom_oracle.c")]
+    newest = max(os.path.getmtime(s) for s in srcs)
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < newest:
         cmd = ["gcc", "-O2", "-mfma", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
-               "-shared", "-I" + HERE, "-I" + CSRC, src, "-o", LIB + ".tmp", "-lm"]
+               "-shared", "-I" + HERE, "-I" + CSRC, *srcs, "-o", LIB + ".tmp", "-lm"]
         subprocess.check_call(cmd)
         os.replace(LIB + ".tmp", LIB)
     return LIB
@@ -195,3 +196,60 @@ class OracleRun:
     @property
     def slots(self):
         return self.s.nf, self.s.of, self.s.vof
+
+
+# ---- HYCOM ocean continuity equation (hycom_oracle.c) -----------------------
+class _HycGrid(ct.Structure):
+    _fields_ = [("nip", ct.c_int), ("kdm", ct.c_int), ("prox", _PI), ("proxs", _PI), ("nprox", _PI), ("wet", _PI),
+                ("area", _PD), ("rarea", _PD), ("sidevec_e", _PD), ("sideln", _PD), ("rsideln", _PD),
+                ("rprox_ln", _PD), ("dtleap", ct.c_double), ("thkdff", ct.c_double), ("biharm", ct.c_double)]
+
+
+class _HycMass(ct.Structure):
+    _fields_ = [("totmass", ct.c_double), ("massglb0", ct.c_double), ("masscor", ct.c_double)]
+
+
+class HycomCnuity:
+    """The checker's HYCOM cnuity on a glvl.Grid: arrays as
+    ocean.hycom_cnuity_case makes them (C order of the Fortran layouts),
+    updated in place by step(); mass holds totmass, massglb0, masscor."""
+
+    def __init__(self, grid, case: dict, thkdff: float = 0.05, biharm: float = 0.75, mass=(0.0, 0.0, 0.0)):
+        build()
+        self._keep = {}
+        nip = grid.nip
+        prox = grid.dyn_prox().astype(np.int32) - 1
+        prox[prox < 0] = 0
+        proxs = grid.proxs.astype(np.int32) - 1
+        proxs[proxs < 0] = 0
+        area = np.asarray(grid.area, np.float64)
+        sideln = np.ascontiguousarray(grid.sideln, np.float64)
+        arr = dict(prox=np.ascontiguousarray(prox), proxs=np.ascontiguousarray(proxs),
+                   nprox=np.ascontiguousarray(grid.nprox, np.int32), wet=np.ascontiguousarray(case["wet"], np.int32),
+                   area=np.ascontiguousarray(area), rarea=np.ascontiguousarray(1.0 / area),
+                   sidevec_e=np.ascontiguousarray(grid.sidevec_e, np.float64), sideln=sideln,
+                   rsideln=np.ascontiguousarray(np.where(sideln != 0, 1.0 / np.where(sideln != 0, sideln, 1), 0.0)),
+                   rprox_ln=np.ascontiguousarray(grid.rprox_ln, np.float64))
+        self._keep.update(arr)
+        g = _HycGrid()
+        g.nip, g.kdm = nip, case["kdm"]
+        for n in ("prox", "proxs", "nprox", "wet"):
+            setattr(g, n, arr[n].ctypes.data_as(_PI))
+        for n in ("area", "rarea", "sidevec_e", "sideln", "rsideln", "rprox_ln"):
+            setattr(g, n, arr[n].ctypes.data_as(_PD))
+        g.dtleap, g.thkdff, g.biharm = case["dtleap"], thkdff, biharm
+        self.g = g
+        self.m = _HycMass(*mass)
+        self.a = {n: np.ascontiguousarray(case[n], np.float64).copy()
+                  for n in ("dp", "pres", "um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "massflx", "mssfx_avg")}
+
+    def step(self, nstep: int, leapm: int, leapn: int) -> int:
+        a = self.a
+        f = lib().fo_hycom_cnuity
+        return f(ct.byref(self.g), ct.byref(self.m), nstep, leapm, leapn,
+                 *[a[n].ctypes.data_as(_PD) for n in ("um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "dp", "pres",
+                                                     "massflx", "mssfx_avg")])
+
+    @property
+    def mass(self):
+        return self.m.totmass, self.m.massglb0, self.m.masscor

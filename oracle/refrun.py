@@ -163,3 +163,37 @@ def run_restref_read(restart_bytes: bytes, dims, workdir=None) -> bytes:
     for p in (fin, fin + ".dims", fout):
         os.remove(p)
     return data
+
+
+HYC_MAGIC = 1213809491
+
+
+def run_hycomref(glvl_path, case: dict, *, nsteps: int, thkdff: float = 0.05, biharm: float = 0.75,
+                 nstep0: int = 1, mass=(0.0, 0.0, 0.0), workdir=None) -> dict:
+    """Run the reference HYCOM continuity equation (oracle/ref/hycomref.F90)
+    for nsteps leapfrog steps from step nstep0 on the inputs of
+    ocean.hycom_cnuity_case, the mass state (totmass, massglb0, masscor)
+    starting from 'mass'; returns the records, each a list with one array per
+    step."""
+    workdir = workdir or os.path.join(REF, "runs")
+    os.makedirs(workdir, exist_ok=True)
+    kdm = case["kdm"]
+    nip = case["wet"].shape[0]
+    fin = os.path.join(workdir, "hyc_state_%d.bin" % os.getpid())
+    fout = os.path.join(workdir, "hyc_out_%d.bin" % os.getpid())
+    with open(fin, "wb") as f:
+        f.write(struct.pack("<8i", HYC_MAGIC, kdm, nip, nsteps, nstep0, 0, 0, 0))
+        f.write(struct.pack("<6d", case["dtleap"], thkdff, biharm, *mass))
+        f.write(np.ascontiguousarray(case["wet"], dtype="<i4").tobytes())
+        for n in ("dp", "pres", "um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "massflx", "mssfx_avg"):
+            f.write(np.ascontiguousarray(case[n], dtype="<f8").tobytes())
+    r = subprocess.run([os.path.join(BIN, "hycomref"), glvl_path, fin, fout], cwd=workdir, capture_output=True,
+                       text=True, preexec_fn=_big_stack)
+    if r.returncode != 0:
+        raise RuntimeError("hycomref failed (%d): %s" % (r.returncode, (r.stdout + r.stderr)[-3000:]))
+    rec = read_records(fout)
+    rec = {k: (v if isinstance(v, list) else [v]) for k, v in rec.items()}
+    rec["_stdout"] = r.stdout
+    os.remove(fin)
+    os.remove(fout)
+    return rec

@@ -26,6 +26,11 @@ Fixtures (data only):
                         hybgen, dt = 240 s): final us, vs, dp, pr, ex, ph, mp,
                         tr as float32, and the relative L2 of that build
                         against the FP64 build (dynref_r8) per field
+  hycom_cnuity_ref.json SHA-256 of every output record of the reference HYCOM
+                        continuity equation (hycom_cnuity.F90, FP64, hycomref)
+                        on the seeded inputs of ocean.hycom_cnuity_case at G2
+                        and G3 x 32 layers, 3 leapfrog steps each, for the
+                        smoothing branches of HYCOM_CASES
   dyn_G2_ref.json       SHA-256 of every output record of the reference FP64
                         build (dynref_r8) for 3 full steps (mode 0) and 3
                         transport steps (mode 1), stage dumps of step 1, the
@@ -178,8 +183,30 @@ def r4_reference():
                         **out)
 
 
+# (name, thkdff, biharm, first step, starting (totmass, massglb0, masscor)):
+# biharmonic and Laplacian interface smoothing, none, and the biharmonic one
+# across the 10-day mass-correction update (step 960 at dtleap = 1800 s)
+HYCOM_CASES = (("biharm", 0.05, 0.75, 1, (0.0, 0.0, 0.0)), ("laplace", 0.05, 0.0, 1, (0.0, 0.0, 0.0)),
+               ("nosmooth", 0.0, 0.75, 1, (0.0, 0.0, 1e-9)), ("masscor", 0.05, 0.75, 959, (1.119e21, 1.12e21, 2.5e-9)))
+
+
+def hycom_reference():
+    from fimocn_subx_realtime_hera_apr2020_amd import ocean
+    out = {"kdm": 32, "nsteps": 3, "runs": {}}
+    for lv in (2, 3):
+        path = os.path.join(HERE, "glvl_G%d.dat" % lv)
+        g = glvl.read_glvl(path)
+        case = ocean.hycom_cnuity_case(g, kdm=32)
+        for name, thkdff, biharm, nstep0, mass in HYCOM_CASES:
+            r = refrun.run_hycomref(path, case, nsteps=3, thkdff=thkdff, biharm=biharm, nstep0=nstep0, mass=mass)
+            out["runs"]["G%d_%s" % (lv, name)] = {k: [sha(x) for x in v] for k, v in r.items() if not k.startswith("_")}
+    with open(os.path.join(HERE, "hycom_cnuity_ref.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     main()
     schemes()
     fp64_bar()
     r4_reference()
+    hycom_reference()
