@@ -46,7 +46,10 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ens_kernel_count", "fimhip_ens_kernel_name", "fimhip_ens_set_tile", "fimhip_ens_get_tile",
            "fimhip_ens_autotune", "fimhip_ens_profile", "fimhip_ens_profile_report",
            "fimhip_globsum_noalloc", "fimhip_ens_set_fused", "fimhip_ens_get_fused", "fimhip_ens_patches",
-           "fimhip_ens_set_wet", "fimhip_ls_pseudo_inverse"]
+           "fimhip_ens_set_wet", "fimhip_ls_pseudo_inverse",
+           "fimhip_ocn_create", "fimhip_ocn_destroy", "fimhip_ocn_cnuity_r8", "fimhip_ocn_cnuity",
+           "fimhip_ocn_field_size", "fimhip_ocn_upload", "fimhip_ocn_download", "fimhip_ocn_mass",
+           "fimhip_ocn_set_mass", "fimhip_ocn_profile", "fimhip_ocn_profile_report"]
 
 # real*4 drop-ins: name -> (number of float arrays, trailing int argument)
 R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, True), "mass_adv": (12, True),
@@ -158,6 +161,19 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ens_autotune.argtypes = [vp, ct.c_int, ct.c_int, _PD]
     lib.fimhip_ens_profile.argtypes = [vp, ct.c_int]
     lib.fimhip_ens_profile_report.argtypes = [vp, ct.c_char_p, ct.c_int]
+    lib.fimhip_ocn_create.argtypes = [ct.POINTER(GridDesc), _PD, ct.c_int, _PI, ct.c_double, ct.c_double,
+                                      ct.c_double, ct.c_int, ct.POINTER(vp)]
+    lib.fimhip_ocn_destroy.argtypes = [vp]
+    lib.fimhip_ocn_cnuity_r8.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int] + [_PD] * 9
+    lib.fimhip_ocn_cnuity.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int]
+    lib.fimhip_ocn_field_size.argtypes = [vp, ct.c_int]
+    lib.fimhip_ocn_field_size.restype = ct.c_longlong
+    lib.fimhip_ocn_upload.argtypes = [vp, ct.c_int, _PD]
+    lib.fimhip_ocn_download.argtypes = [vp, ct.c_int, _PD]
+    lib.fimhip_ocn_mass.argtypes = [vp, _PD, _PD, _PD]
+    lib.fimhip_ocn_set_mass.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double]
+    lib.fimhip_ocn_profile.argtypes = [vp, ct.c_int]
+    lib.fimhip_ocn_profile_report.argtypes = [vp, ct.c_char_p, ct.c_int]
     _LIB = lib
     return lib
 
@@ -604,4 +620,97 @@ class Ensemble:
         for line in buf.value.decode().splitlines():
             name, n, ms = line.split()
             out[name] = (int(n), float(ms))
+        return out
+
+
+OCN_FIELDS = ["dp", "pres", "um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "massflx", "mssfx_avg"]
+FIMHIP_EDUP = -4
+
+
+class OceanCnuity:
+    """The HYCOM ocean's continuity equation on one GPU (fimhip_ocn_*):
+    ``cnuity`` keeps the argument list of hycom_cnuity.F90:21 (nstep, leapm,
+    leapn, um_edg, vm_edg, un_edg, vn_edg, dpm_edg, dp, pres, massflx,
+    mssfx_avg) with numpy float64 arrays in the reference layout (C order
+    (2,nip,kdm), (nip,kdm+1), (nip,6,kdm)), dp, pres, massflx and mssfx_avg
+    updated in place.  ``mass`` is hycom_variables' (totmass, massglb0,
+    masscor).  As the reference stops on differing duplicate edge fluxes
+    (chkdup), so this raises FimhipError."""
+
+    def __init__(self, grid: _glvl.Grid, kdm: int, wet, dtleap: float, thkdff: float = 0.05,
+                 biharm: float = 0.75, device: int = 0):
+        self.lib = load_library()
+        self.grid, self.kdm, self.nip = grid, kdm, grid.nip
+        gd, self._keep = _grid_desc(grid, 1, 1, dtleap, np.zeros(1), np.zeros(1))
+        self._keep["rprox_ln"] = np.ascontiguousarray(grid.rprox_ln, np.float64)
+        self._keep["wet"] = np.ascontiguousarray(wet, np.int32)
+        assert self._keep["wet"].shape == (grid.nip,)
+        h = ct.c_void_p()
+        _chk(self.lib.fimhip_ocn_create(ct.byref(gd), _p(self._keep["rprox_ln"]), kdm,
+                                        self._keep["wet"].ctypes.data_as(_PI), dtleap, thkdff, biharm, device,
+                                        ct.byref(h)), "fimhip_ocn_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fimhip_ocn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shape(self, name: str):
+        return {"dp": (2, self.nip, self.kdm), "pres": (self.nip, self.kdm + 1)}.get(name, (self.nip, 6, self.kdm))
+
+    @staticmethod
+    def _step_rc(rc: int, what: str):
+        if rc == FIMHIP_EDUP:
+            raise FimhipError("%s: duplicate edge fluxes differ (chkdup)" % what)
+        _chk(rc, what)
+
+    def cnuity(self, nstep, leapm, leapn, um_edg, vm_edg, un_edg, vn_edg, dpm_edg, dp, pres, massflx, mssfx_avg):
+        arrs = (um_edg, vm_edg, un_edg, vn_edg, dpm_edg, dp, pres, massflx, mssfx_avg)
+        for n, a in zip(OCN_FIELDS[2:7] + OCN_FIELDS[:2] + OCN_FIELDS[7:], arrs):
+            if a.shape != self._shape(n):
+                raise ValueError("%s: shape %s, expected %s" % (n, a.shape, self._shape(n)))
+        self._step_rc(self.lib.fimhip_ocn_cnuity_r8(self.h, nstep, leapm, leapn, *map(_p, arrs)), "ocn_cnuity_r8")
+
+    def upload(self, name: str, a):
+        a = np.ascontiguousarray(a, np.float64)
+        if a.shape != self._shape(name):
+            raise ValueError("%s: shape %s, expected %s" % (name, a.shape, self._shape(name)))
+        _chk(self.lib.fimhip_ocn_upload(self.h, OCN_FIELDS.index(name), _p(a)), "ocn_upload")
+
+    def download(self, name: str) -> np.ndarray:
+        a = np.empty(self._shape(name), np.float64)
+        _chk(self.lib.fimhip_ocn_download(self.h, OCN_FIELDS.index(name), _p(a)), "ocn_download")
+        return a
+
+    def step(self, nstep: int, leapm: int, leapn: int):
+        """cnuity on the device-resident arrays (upload them first)."""
+        self._step_rc(self.lib.fimhip_ocn_cnuity(self.h, nstep, leapm, leapn), "ocn_cnuity")
+
+    @property
+    def mass(self):
+        v = np.zeros(3)
+        _chk(self.lib.fimhip_ocn_mass(self.h, *(v[j:].ctypes.data_as(_PD) for j in range(3))), "ocn_mass")
+        return tuple(float(x) for x in v)
+
+    @mass.setter
+    def mass(self, m):
+        _chk(self.lib.fimhip_ocn_set_mass(self.h, *map(float, m)), "ocn_set_mass")
+
+    def profile(self, on: bool = True):
+        _chk(self.lib.fimhip_ocn_profile(self.h, int(on)), "ocn_profile")
+
+    def profile_report(self) -> dict:
+        buf = ct.create_string_buffer(1 << 14)
+        _chk(self.lib.fimhip_ocn_profile_report(self.h, buf, len(buf)), "ocn_profile_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            n, c, t = line.split()
+            out[n] = (int(c), float(t))
         return out

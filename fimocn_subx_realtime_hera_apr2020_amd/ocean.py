@@ -128,7 +128,8 @@ def hycom_cnuity_case(grid, kdm: int = 32, seed: int = 20240607, dtleap: float =
     is an ocean cell with no ocean neighbour); the
     thickness dp (2, nip, kdm) at the two leapfrog levels and the interface
     pressure pres (nip, kdm+1) (a smooth bathymetry of 2400-4000 m filled by
-    the ocn_dpmin layers (the first kdm, surface first) stretched per column,
+    the ocn_dpmin layers (the first kdm, surface first; for kdm > 32 the 32
+    interpolated to kdm) stretched per column,
     times 1 + 0.3 U[-1,1] so the limiter has work, the second level 1 %
     apart); edge velocities um/vm (mid time) and un/vn (old), (nip, 6, kdm),
     each edge's value the mean of its two cells' solid-body-plus-noise
@@ -149,6 +150,9 @@ def hycom_cnuity_case(grid, kdm: int = 32, seed: int = 20240607, dtleap: float =
     f = smooth_field(lat, lon, seed + 3)
     depth = 2400.0 + 1600.0 * (np.argsort(np.argsort(f)) + 0.5) / nip
     dpmin_m = np.array(C.OCN_DPMIN32[:kdm], np.float64)
+    if kdm > len(C.OCN_DPMIN32):      # more layers: the same profile, interpolated
+        n = len(C.OCN_DPMIN32)
+        dpmin_m = np.interp(np.linspace(0.0, n - 1.0, kdm), np.arange(n), np.asarray(C.OCN_DPMIN32, np.float64))
     dz = dpmin_m[None, :] * (depth / dpmin_m.sum())[:, None] * (1.0 + 0.3 * rng.uniform(-1.0, 1.0, (nip, kdm)))
     dp0 = dz * ONEM
     dp1 = dp0 * (1.0 + 0.01 * rng.uniform(-1.0, 1.0, (nip, kdm)))

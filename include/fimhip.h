@@ -23,6 +23,8 @@ extern "C" {
 #define FIMHIP_EINVAL (-1)      /* bad argument or size */
 #define FIMHIP_EHIP (-2)        /* a HIP runtime call failed */
 #define FIMHIP_ENOMEM (-3)      /* device allocation failed */
+#define FIMHIP_EDUP (-4)        /* duplicate edge fluxes differ (hycom_cnuity.F90:1030-1075
+                                   stops the program there) */
 #define FIMHIP_ENODEV (-88)     /* no usable GPU (GPUinit.cu:27-34 uses -88) */
 
 typedef struct fimhip_ctx fimhip_ctx;
@@ -337,6 +339,55 @@ int fimhip_ens_get_tile(const fimhip_ens *ens, int kernel);
 int fimhip_ens_autotune(fimhip_ens *ens, int mode, int reps, double *times);
 int fimhip_ens_profile(fimhip_ens *ens, int on);
 int fimhip_ens_profile_report(fimhip_ens *ens, char *buf, int len);
+
+/* ---------------------------------------------------------------------------
+ * The HYCOM ocean's continuity equation (hycom_cnuity.F90:17, subroutine
+ * cnuity): FCT layer thickness transport on ocean cells (wet > 0), the clipped
+ * antidiffusive flux returned as a barotropic correction, interface smoothing
+ * by bolus fluxes (thkdff != 0; Laplacian for biharm = 0, else blended with
+ * the biharmonic form) and the leapfrog time smoothing, in FP64 with the
+ * reference's operations in its order.  The global mass drift correction
+ * (masscor, 977-998) is kept by the instance (totmass, massglb0, masscor of
+ * hycom_variables) and summed over the ocean cells in cell order.
+ * Arrays (Fortran layouts of hycom_cnuity.F90:21-37):
+ *   dp (kdm,nip,2)   pres (kdm+1,nip)   edge (kdm,6,nip): um/vm/un/vn/dpm_edg,
+ *   massflx, mssfx_avg.  A pentagon's sixth edge slot is ignored on input and
+ *   massflx holds -1e33 there and on every edge of a land cell (LAND_SPVAL).
+ * Returns FIMHIP_EDUP where the reference's chkdup stops the run; the arrays
+ * then hold the step's results. */
+typedef struct fimhip_ocn fimhip_ocn;
+#define FIMHIP_OCN_DP 0
+#define FIMHIP_OCN_PRES 1
+#define FIMHIP_OCN_UM_EDG 2
+#define FIMHIP_OCN_VM_EDG 3
+#define FIMHIP_OCN_UN_EDG 4
+#define FIMHIP_OCN_VN_EDG 5
+#define FIMHIP_OCN_DPM_EDG 6
+#define FIMHIP_OCN_MASSFLX 7
+#define FIMHIP_OCN_MSSFX_AVG 8
+/* hycom_init's constants (hycom_constants.F90 wet, dtleap; fimnamelist kdm,
+ * thkdff, biharm) on the grid of the descriptor (nprox, prox, proxs,
+ * sidevec_e, sideln, area; nvl, ntra, dt, cs, sn, sidevec_c, hfcoef, lat,
+ * lon, thetac and dpsig are not read) and rprox_ln (6,nip) of glvl.dat;
+ * kdm <= 64 */
+int fimhip_ocn_create(const fimhip_grid_desc *grid, const double *rprox_ln, int kdm, const int *wet,
+                      double dtleap, double thkdff, double biharm, int device, fimhip_ocn **out);
+int fimhip_ocn_destroy(fimhip_ocn *ocn);
+/* hycom_cnuity.F90:21 with host arrays: dp, pres, massflx and mssfx_avg are
+ * updated in place */
+int fimhip_ocn_cnuity_r8(fimhip_ocn *ocn, int nstep, int leapm, int leapn, const double *um_edg,
+                         const double *vm_edg, const double *un_edg, const double *vn_edg, const double *dpm_edg,
+                         double *dp, double *pres, double *massflx, double *mssfx_avg);
+/* the same step on the device-resident arrays (fimhip_ocn_upload first) */
+int fimhip_ocn_cnuity(fimhip_ocn *ocn, int nstep, int leapm, int leapn);
+long long fimhip_ocn_field_size(const fimhip_ocn *ocn, int field);
+int fimhip_ocn_upload(fimhip_ocn *ocn, int field, const double *host);
+int fimhip_ocn_download(fimhip_ocn *ocn, int field, double *host);
+/* hycom_variables totmass, massglb0, masscor (a restart sets them) */
+int fimhip_ocn_mass(const fimhip_ocn *ocn, double *totmass, double *massglb0, double *masscor);
+int fimhip_ocn_set_mass(fimhip_ocn *ocn, double totmass, double massglb0, double masscor);
+int fimhip_ocn_profile(fimhip_ocn *ocn, int on);
+int fimhip_ocn_profile_report(fimhip_ocn *ocn, char *buf, int len);
 
 /* x**y as computed on the device (the host build of the same routine) */
 double fimhip_pow(double x, double y);

@@ -53,6 +53,9 @@ sys.path.insert(0, ROOT)
 from fimocn_subx_realtime_hera_apr2020_amd import glvl, state  # noqa: E402
 from oracle import refrun  # noqa: E402
 
+sys.path.insert(0, os.path.dirname(HERE))
+from helpers import HYCOM_CASES  # noqa: E402
+
 DT = 240.0
 
 
@@ -186,8 +189,6 @@ def r4_reference():
 # (name, thkdff, biharm, first step, starting (totmass, massglb0, masscor)):
 # biharmonic and Laplacian interface smoothing, none, and the biharmonic one
 # across the 10-day mass-correction update (step 960 at dtleap = 1800 s)
-HYCOM_CASES = (("biharm", 0.05, 0.75, 1, (0.0, 0.0, 0.0)), ("laplace", 0.05, 0.0, 1, (0.0, 0.0, 0.0)),
-               ("nosmooth", 0.0, 0.75, 1, (0.0, 0.0, 1e-9)), ("masscor", 0.05, 0.75, 959, (1.119e21, 1.12e21, 2.5e-9)))
 
 
 def hycom_reference():

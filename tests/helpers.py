@@ -29,6 +29,14 @@ def golden_ref():
         return json.load(f)
 
 
+# the HYCOM continuity cases of the golden digests and the GPU tests: name,
+# thkdff, biharm, first step, (totmass, massglb0, masscor) at the start; the
+# last starts at step 959 so that its third step (day 10 at dtleap 1800 s)
+# takes the 10-day mass correction
+HYCOM_CASES = (("biharm", 0.05, 0.75, 1, (0.0, 0.0, 0.0)), ("laplace", 0.05, 0.0, 1, (0.0, 0.0, 0.0)),
+               ("nosmooth", 0.0, 0.75, 1, (0.0, 0.0, 1e-9)), ("masscor", 0.05, 0.75, 959, (1.119e21, 1.12e21, 2.5e-9)))
+
+
 # reference record suffix -> oracle / device field name
 STATE_FIELDS = {"us": "us", "vs": "vs", "dp": "dp", "pr": "pr", "ex": "ex", "ph": "ph", "mp": "mp", "tr": "tr"}
 

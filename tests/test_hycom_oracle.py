@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from fimocn_subx_realtime_hera_apr2020_amd import glvl, ocean
-from helpers import GOLDEN
+from helpers import GOLDEN, HYCOM_CASES
 
 RECORDS = ("dp", "pres", "massflx", "mssfx_avg")
 
@@ -24,11 +24,6 @@ def _sha(v):
     return hashlib.sha256(np.ascontiguousarray(v, "<f8").tobytes()).hexdigest()
 
 
-def _cases():
-    import sys
-    sys.path.insert(0, GOLDEN)
-    import make_golden
-    return make_golden.HYCOM_CASES
 
 
 def _run(O, g, case, thkdff, biharm, nstep0, mass, nsteps=3):
@@ -49,7 +44,7 @@ def test_hycom_cnuity_checker_equals_reference_digests(oracle_lib, level):
         ref = json.load(f)
     g = glvl.read_glvl(os.path.join(GOLDEN, "glvl_G%d.dat" % level))
     case = ocean.hycom_cnuity_case(g, kdm=ref["kdm"])
-    for name, thkdff, biharm, nstep0, mass in _cases():
+    for name, thkdff, biharm, nstep0, mass in HYCOM_CASES:
         _, out = _run(oracle_lib, g, case, thkdff, biharm, nstep0, mass, ref["nsteps"])
         want = ref["runs"]["G%d_%s" % (level, name)]
         for n, vals in out.items():
