@@ -32,6 +32,10 @@ Prints ONE JSON line.  Besides the contract fields it carries
                 batched form, 4 class-B tracers per member)
   uneven_batch: G6 and G7 concatenated into one instance (two resolutions in
                 one launch), mixed form and FP64
+  hycom_cnuity: the HYCOM ocean's continuity equation (FP64, bitwise the
+                checker's) on G8 x 32 layers: throughput, per-kernel times,
+                its dominant kernel against its algorithmic bytes, and the
+                checker timed on one host core
   accuracy_96:  relative L2 of member 0 against FP64 after 96 steps, mass drift
   cpu_baseline: the C restatement (oracle/fim_oracle.c, FP64, OpenMP) on a
                 bounded sample of the same grid
@@ -140,7 +144,8 @@ def parse():
     ap.add_argument("--dt", type=float, default=None,
                     help="time step (s); default the reference's 1.2*4800/(2**glvl/2) (90 s at G7)")
     ap.add_argument("--mode", type=int, default=0, help="0 full step, 1 transport only")
-    ap.add_argument("--form", choices=["both", "ens", "plain"], default="both")
+    ap.add_argument("--form", choices=["both", "ens", "plain", "hycom"], default="both",
+                    help="hycom: only the ocean continuity leg")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--accuracy-steps", type=int, default=96, help="0 skips the accuracy leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -547,6 +552,85 @@ def run_uneven(a, th, ds, fimhip):
     return out
 
 
+# HBM bytes each kernel of the ocean continuity equation must move per ocean
+# cell-level (FP64: edge arrays 6 x 8 B, cell arrays 8 B; a neighbour's value
+# counted once; per-cell grid tables and the clipped-flux sums left out)
+HYC_KERNEL_BYTES = {"hyc_flux": 352, "hyc_lowsum": 144, "hyc_limits": 48, "hyc_clip": 184, "hyc_recover": 128,
+                    "hyc_del2p": 16, "hyc_bolus": 80, "hyc_bclip": 88, "hyc_final": 248, "hyc_check": 48}
+# what a step must read and write at least per ocean cell-level: um, vm, un,
+# vn, dpm_edg and mssfx_avg in, massflx and mssfx_avg out (8 edge arrays), dp
+# at both leapfrog levels and pres in and out
+HYC_STEP_BYTES = 8 * 48 + 2 * 24
+
+
+def run_hycom(a, fimhip, cpu=True):
+    """The HYCOM ocean's continuity equation (hycom_cnuity.F90 cnuity, FP64)
+    on G8 (655362 cells) x 32 layers, ~30 % land, biharmonic interface
+    smoothing, dtleap 1800 s, on device-resident arrays; the checker's C
+    restatement timed on one host core over 2 steps of the same input."""
+    import torch
+    from fimocn_subx_realtime_hera_apr2020_amd import ocean
+    g = fimhip.icos_grid(8)
+    kdm, thkdff, biharm = 32, 0.05, 0.75
+    case = ocean.hycom_cnuity_case(g, kdm=kdm)
+    nwet = int((case["wet"] > 0).sum())
+    wcl = nwet * kdm
+    o = fimhip.OceanCnuity(g, kdm, case["wet"], case["dtleap"], thkdff=thkdff, biharm=biharm)
+    try:
+        for n in fimhip.OCN_FIELDS:
+            o.upload(n, case[n])
+        nstep = 1
+        for _ in range(a.warmup):
+            o.step(nstep, 1 + nstep % 2, 2 - nstep % 2)
+            nstep += 1
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            o.step(nstep, 1 + nstep % 2, 2 - nstep % 2)
+            nstep += 1
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        o.profile(True)
+        for _ in range(a.steps):
+            o.step(nstep, 1 + nstep % 2, 2 - nstep % 2)
+            nstep += 1
+        prof = o.profile_report()
+        o.profile(False)
+    finally:
+        o.close()
+    kern = {}
+    for n, (cnt, tot) in prof.items():
+        ms = tot / cnt
+        b = HYC_KERNEL_BYTES.get(n, 0) * wcl
+        kern[n] = {"ms": ms, "bytes": b, "GB_s": b / (ms * 1e-3) / 1e9}
+    dom = max(kern, key=lambda n: kern[n]["ms"])
+    dk = kern[dom]
+    ms_step = wall / a.steps * 1e3
+    out = {"workload": "G8 (%d cells, %d ocean) x %d layers, thkdff %g, biharm %g, dtleap %g s, FP64, resident"
+                       % (g.nip, nwet, kdm, thkdff, biharm, case["dtleap"]),
+           "value": wcl * a.steps / wall, "unit": "ocean cell-level-steps/s", "ms_per_step": ms_step,
+           "kernel_ms_per_step": sum(v["ms"] for v in kern.values()),
+           "roofline": {"bound": "hbm", "achieved": dk["GB_s"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": dk["GB_s"] * 1e9 / HBM_PEAK, "kernel": dom,
+                        "bytes_per_cell_level": HYC_KERNEL_BYTES.get(dom, 0)},
+           "step_roofline": {"compulsory_bytes_per_cell_level": HYC_STEP_BYTES,
+                             "achieved": HYC_STEP_BYTES * wcl / (ms_step * 1e-3) / 1e9, "unit": "GB/s",
+                             "frac": HYC_STEP_BYTES * wcl / (ms_step * 1e-3) / HBM_PEAK},
+           "kernels": kern}
+    if cpu:
+        from oracle import oracle as O
+        h = O.HycomCnuity(g, case, thkdff=thkdff, biharm=biharm)
+        t0 = time.perf_counter()
+        for st in (1, 2):
+            h.step(st, 1 + st % 2, 2 - st % 2)
+        tc = time.perf_counter() - t0
+        out["cpu_checker"] = {"value": wcl * 2 / tc, "unit": "ocean cell-level-steps/s", "cores": 1,
+                              "kind": "port", "sample": "2 steps of the same G8 x 32 input",
+                              "ms_per_step": tc / 2 * 1e3}
+        out["speedup_vs_cpu_checker"] = out["value"] / out["cpu_checker"]["value"]
+    return out
+
+
 def accuracy(a, e, m, grid, fimhip):
     """Member 0 of the ensemble against the FP64 path after n steps, and the
     global mass drift of every member."""
@@ -701,6 +785,9 @@ def main():
         torch.cuda.synchronize()
         del x, y
     step_name = "full" if a.mode == 0 else "transport"
+    if a.form == "hycom":
+        print(json.dumps({"hycom_cnuity": run_hycom(a, fimhip, cpu=not a.no_cpu_baseline)}))
+        return
     plain = ens = None
     m = e = None
     if a.form in ("both", "plain"):
@@ -714,6 +801,10 @@ def main():
     if a.form == "both" and not a.headline_only:
         uneven = run_uneven(a, th, ds, fimhip)
         print("uneven batch done", file=sys.stderr, flush=True)
+    hyc = None
+    if a.form == "both" and not a.headline_only:
+        hyc = run_hycom(a, fimhip, cpu=not a.no_cpu_baseline)
+        print("ocean continuity leg done", file=sys.stderr, flush=True)
     acc = None
     if e is not None and m is not None and a.accuracy_steps > 0:
         acc = accuracy(a, e, m, grid, fimhip)
@@ -750,7 +841,8 @@ def main():
                               "a 240 s step overflows the FP64 state by step 6, so it is not timed"},
         "roofline": head["roofline"], "dominant_kernel": head["dominant_kernel"],
         "speedup_vs_plain_fp64": (ens["value"] / plain["value"]) if (ens and plain) else None,
-        "plain_fp64": plain, "ensemble": ens, "uneven_batch": uneven, "accuracy_96": acc, "cpu_baseline": cpu,
+        "plain_fp64": plain, "ensemble": ens, "uneven_batch": uneven, "hycom_cnuity": hyc,
+        "accuracy_96": acc, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

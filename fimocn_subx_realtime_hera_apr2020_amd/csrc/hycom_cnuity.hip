@@ -7,24 +7,33 @@
 // Every operation is the reference's, in its order: the sums over a cell's
 // edges run in edge order per lane, and the two sums the reference forms
 // along the column (interface pressure pres(k+1) = pres(k) + dp(k), and the
-// clipped part of each edge's flux) are exact sequential chains that every
-// lane of the column evaluates in layer order from broadcast values.  A
-// kernel boundary stands wherever a stage reads a neighbour's value that the
-// previous stage writes:
-//   hyc_flux     low/high-order edge fluxes, low-order thickness, pressure,
-//                in/outflow sums of the antidiffusive fluxes    (98-265)
+// clipped part of each edge's flux) are exact sequential chains over values
+// the column's lanes exchange through LDS (a column never leaves its
+// wavefront, so the exchange needs no workgroup barrier).  The neighbour
+// values of all six edges are loaded unconditionally (every neighbour index
+// is a valid cell) and the land/sea mask selects afterwards, so each lane has
+// all its gathers in flight at once; a per-cell word packs the edge count,
+// the cell's own mask and the mask of its six neighbours.  Blocks are ordered
+// so that each XCD works on a contiguous eighth of the cells and neighbour
+// rows are found in its L2.  A kernel boundary stands wherever a stage reads
+// a neighbour's value that the previous stage writes:
+//   hyc_flux     low/high-order edge fluxes, each edge's pair once (98-159)
+//   hyc_lowsum   low-order thickness, pressure, in/outflow sums of the
+//                antidiffusive fluxes                           (169-265)
 //   hyc_limits   local bounds and limiter ratios              (269-300)
 //   hyc_clip     limited antidiffusive fluxes and update      (315-393)
-//   hyc_recover  the clipped flux as a barotropic correction  (434-462)
-//   hyc_update   its thickness update; dp from pres, del2p    (465-548)
+//   hyc_recover  the clipped flux as a barotropic correction and its
+//                thickness update; dp from pres, del2p's own part (434-548)
 //   hyc_del2p    the biharmonic del2p from the neighbours     (553-603)
 //   hyc_bolus    interface-slope bolus fluxes, outflow ratios (605-683)
 //   hyc_bclip    limited bolus fluxes and their update        (685-776)
 //   hyc_final    the clipped bolus correction, the mass-flux average, the
 //                leapfrog time smoothing, the bottom pressure (778-954)
-// The duplicate-flux check (chkdup) counts differing pairs into a flag; the
-// global mass and its drift correction (glob2d) are summed on the host in
-// cell order, as the reference does serially.
+// The thickness after the antidiffusive update goes to its own array (dpc),
+// which hyc_recover reads at the neighbours while it writes the new level.
+// The duplicate-flux check (chkdup) compares each pair of edge fluxes once
+// and sets a flag; the global mass and its drift correction (glob2d) are
+// summed on the host in cell order, as the reference does serially.
 #include <string.h>
 
 #include <algorithm>
@@ -43,58 +52,80 @@ constexpr double H_ASSLDP = .25;
 constexpr double H_LAND = -1.e33;
 constexpr double H_EPSIL = 1.e-11;
 
+// cinfo bits: 0-2 the number of edges, 3 the cell is ocean, 4+e edge e (< the
+// number of edges) leads to an ocean cell
 struct HycGrid {
   int nip, kdm;
-  const int *prox, *proxs, *nprox, *wet;
+  const int *prox, *proxs, *cinfo;
   const double *area, *rarea, *sve, *sideln, *rsideln, *rprox_ln;
 };
 struct HycState {
   const double *um, *vm, *un, *vn, *dpm_edg;
   double *dp, *pres, *massflx, *mssfx_avg;
   // work arrays
-  double *antiflx, *dp_lo, *dpold, *flxplus, *flxmnus, *rplus, *rmnus, *del2p, *dptmp, *flxdv;
+  double *antiflx, *dp_lo, *dpold, *dpc, *flxplus, *flxmnus, *rplus, *rmnus, *del2p, *dptmp;
   double *clippd, *rpbot, *aux;
   int *flag;
 };
 
 __device__ __forceinline__ double hmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double hmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ int ci_np(int ci) { return ci & 7; }
+__device__ __forceinline__ bool ci_wet(int ci) { return (ci >> 3) & 1; }
+__device__ __forceinline__ bool ci_open(int ci, int e) { return (ci >> (4 + e)) & 1; }
 
-// column c of the block and the lane's layer
+// the column of the lane (i, its slot cc in the block) and the lane's layer k;
+// the host pads the grid to a multiple of 8 blocks, and block b, which the
+// hardware sends to XCD b mod 8, takes the (b mod 8)-th eighth of the cells
 template <int LPC>
 struct Col {
-  int i, k;
+  int i, k, cc;
   bool ok;
 };
 template <int LPC>
 __device__ __forceinline__ Col<LPC> hcol(int ncol) {
   constexpr int CPB = 256 / LPC;
+  const int per = gridDim.x >> 3;
+  const int bx = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   Col<LPC> c;
-  c.i = blockIdx.x * CPB + threadIdx.x / LPC;
+  c.cc = threadIdx.x / LPC;
+  c.i = bx * CPB + c.cc;
   c.k = threadIdx.x % LPC;
   c.ok = c.i < ncol;
   return c;
 }
-// value of lane j of this column's LPC lanes
-template <int LPC>
-__device__ __forceinline__ double bcast(double v, int j) { return __shfl(v, j, LPC); }
+
+// orders the wavefront's LDS accesses around an exchange between its lanes
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // pres(k+1) = pres(k) + d(k), k = 0..kdm-1, from pres(0) = p0: every lane forms
 // the whole chain in layer order and keeps the interface below its layer
-template <int LPC>
-__device__ __forceinline__ double col_pres(double d, double p0, int k, int kdm) {
+__device__ __forceinline__ double col_pres(double *buf, double d, double p0, int k, int kdm) {
+  buf[k] = d;
+  wave_sync();
   double prev = p0, mine = 0.;
   for (int j = 0; j < kdm; j++) {
-    prev = prev + bcast<LPC>(d, j);
-    if (k == j) mine = prev;
+    prev = prev + buf[j];
+    if (j == k) mine = prev;
   }
+  wave_sync();
   return mine;
 }
-// the sum over the column's layers of v, in layer order from 0.
+// the six sums over the column's layers, in layer order from 0, of part[e]:
+// lane e < NPP of the column returns edge e's
 template <int LPC>
-__device__ __forceinline__ double col_sum(double v, int kdm) {
+__device__ __forceinline__ double col_sums6(double (*buf)[LPC + 1], const double *part, int k, int kdm) {
+#pragma unroll
+  for (int e = 0; e < NPP; e++) buf[e][k] = part[e];
+  wave_sync();
   double acc = 0.;
-  for (int j = 0; j < kdm; j++) acc = acc + bcast<LPC>(v, j);
+  if (k < NPP)
+    for (int j = 0; j < kdm; j++) acc = acc + buf[k][j];
+  wave_sync();
   return acc;
 }
 
@@ -103,17 +134,82 @@ __device__ __forceinline__ double col_sum(double v, int kdm) {
 #define HE(a, i, e, k) (a)[((size_t)(i) * NPP + (e)) * kdm + (k)]
 #define HSVE(i, e, c) g.sve[((size_t)(i) * NPP + (e)) * 2 + (c)]
 
+// the high- and low-order fluxes of one edge seen from the cell whose values
+// are 'a' (its edge ea) towards the cell 'b' (its edge eb)  (98-159)
+struct EdgeFlux {
+  double anti, low;
+};
+__device__ __forceinline__ EdgeFlux edge_flux(const HycGrid &g, const HycState &s, const double *dpm,
+                                              const double *dpn, int kdm, int a, int ea, int b, int eb, int k,
+                                              double dtleap) {
+  double valu1 = HSVE(a, ea, 1) * HE(s.um, a, ea, k) - HSVE(a, ea, 0) * HE(s.vm, a, ea, k);
+  double valu2 = HSVE(b, eb, 1) * HE(s.um, b, eb, k) - HSVE(b, eb, 0) * HE(s.vm, b, eb, k);
+  const double outgom = .5 * (valu1 - valu2);
+  valu1 = g.rsideln[a * NPP + ea] * dtleap * g.rprox_ln[a * NPP + ea];
+  valu2 = g.rsideln[b * NPP + eb] * dtleap * g.rprox_ln[b * NPP + eb];
+  const double q = hmin(fabs(outgom) * .5 * (valu1 + valu2), 1.);
+  const double dpedg = .5 * (HE(s.dpm_edg, a, ea, k) + HE(s.dpm_edg, b, eb, k));
+  double hi = outgom >= 0. ? dpedg * (1. - q) + HC(dpm, a, k) * q : dpedg * (1. - q) + HC(dpm, b, k) * q;
+  hi = hi * outgom;
+  valu1 = HSVE(a, ea, 1) * HE(s.un, a, ea, k) - HSVE(a, ea, 0) * HE(s.vn, a, ea, k);
+  valu2 = HSVE(b, eb, 1) * HE(s.un, b, eb, k) - HSVE(b, eb, 0) * HE(s.vn, b, eb, k);
+  const double outgon = .5 * (valu1 - valu2);
+  const double lo = outgon >= 0. ? outgon * HC(dpn, a, k) : outgon * HC(dpn, b, k);
+  return EdgeFlux{hi - lo, lo};
+}
+
+// the edge fluxes, each pair of an edge's two sides once: the cell of lower
+// index evaluates the reference's expression from both sides (the same
+// operations with the two cells' roles exchanged) and writes both
 template <int LPC>
 __global__ void __launch_bounds__(256) hyc_flux(HycGrid g, HycState s, int leapm, int leapn, double dtleap) {
+  const int kdm = g.kdm, nip = g.nip;
+  const Col<LPC> c = hcol<LPC>(nip);
+  if (!c.ok || c.k >= kdm) return;
+  const int i = c.i, k = c.k;
+  const int ci = g.cinfo[i];
+  int ix[NPP], ex[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    ix[e] = g.prox[i * NPP + e];
+    ex[e] = g.proxs[i * NPP + e];
+  }
+  if (!ci_wet(ci)) {      // (the work array of antidiffusive fluxes is never read on land)
+#pragma unroll
+    for (int e = 0; e < NPP; e++) HE(s.massflx, i, e, k) = H_LAND;
+    return;
+  }
+  const double *dpm = s.dp + (size_t)(leapm - 1) * nip * kdm, *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
+  const int np = ci_np(ci);
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const bool open = ci_open(ci, e);
+    if (open && i < ix[e]) {
+      const EdgeFlux fa = edge_flux(g, s, dpm, dpn, kdm, i, e, ix[e], ex[e], k, dtleap);
+      const EdgeFlux fb = edge_flux(g, s, dpm, dpn, kdm, ix[e], ex[e], i, e, k, dtleap);
+      HE(s.antiflx, i, e, k) = fa.anti;
+      HE(s.massflx, i, e, k) = fa.low;
+      HE(s.antiflx, ix[e], ex[e], k) = fb.anti;
+      HE(s.massflx, ix[e], ex[e], k) = fb.low;
+    } else if (!open) {      // an edge to land carries no flux; a pentagon's sixth slot is land-valued
+      HE(s.antiflx, i, e, k) = HE(s.massflx, i, e, k) = e < np ? 0. : H_LAND;
+    }
+  }
+}
+
+// low-order thickness, pressure and the in/outflow sums of the antidiffusive
+// fluxes (169-265)
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_lowsum(HycGrid g, HycState s, int leapn, double dtleap) {
+  __shared__ double sh[256 / LPC][LPC + 1];
   const int kdm = g.kdm, nip = g.nip;
   const Col<LPC> c = hcol<LPC>(nip);
   if (!c.ok) return;
   const int i = c.i, k = c.k;
   const bool live = k < kdm;
-  const double *dpm = s.dp + (size_t)(leapm - 1) * nip * kdm, *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
-  if (live)
-    for (int e = 0; e < NPP; e++) HE(s.antiflx, i, e, k) = HE(s.massflx, i, e, k) = H_LAND;
-  if (!g.wet[i]) {
+  const int kk = live ? k : 0;
+  const int ci = g.cinfo[i];
+  if (!ci_wet(ci)) {
     if (live) {
       HC(s.dp_lo, i, k) = 0.;
       HC(s.flxplus, i, k) = H_EPSIL;
@@ -121,46 +217,21 @@ __global__ void __launch_bounds__(256) hyc_flux(HycGrid g, HycState s, int leapm
     }
     return;
   }
-  const int np = g.nprox[i];
+  const double *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
+  const int np = ci_np(ci);
   double mf[NPP], af[NPP];
-  for (int e = 0; e < NPP; e++) mf[e] = af[e] = 0.;
 #pragma unroll
   for (int e = 0; e < NPP; e++) {
-    if (e >= np) break;
-    const int ix = g.prox[i * NPP + e];
-    if (!live) continue;
-    if (!g.wet[ix]) {
-      HE(s.massflx, i, e, k) = HE(s.antiflx, i, e, k) = 0.;
-      continue;
-    }
-    const int ex = g.proxs[i * NPP + e];
-    double valu1 = HSVE(i, e, 1) * HE(s.um, i, e, k) - HSVE(i, e, 0) * HE(s.vm, i, e, k);
-    double valu2 = HSVE(ix, ex, 1) * HE(s.um, ix, ex, k) - HSVE(ix, ex, 0) * HE(s.vm, ix, ex, k);
-    const double outgom = .5 * (valu1 - valu2);
-    valu1 = g.rsideln[i * NPP + e] * dtleap * g.rprox_ln[i * NPP + e];
-    valu2 = g.rsideln[ix * NPP + ex] * dtleap * g.rprox_ln[ix * NPP + ex];
-    const double q = hmin(fabs(outgom) * .5 * (valu1 + valu2), 1.);
-    const double dpedg = .5 * (HE(s.dpm_edg, i, e, k) + HE(s.dpm_edg, ix, ex, k));
-    double hi = outgom >= 0. ? dpedg * (1. - q) + HC(dpm, i, k) * q : dpedg * (1. - q) + HC(dpm, ix, k) * q;
-    hi = hi * outgom;
-    valu1 = HSVE(i, e, 1) * HE(s.un, i, e, k) - HSVE(i, e, 0) * HE(s.vn, i, e, k);
-    valu2 = HSVE(ix, ex, 1) * HE(s.un, ix, ex, k) - HSVE(ix, ex, 0) * HE(s.vn, ix, ex, k);
-    const double outgon = .5 * (valu1 - valu2);
-    const double lo = outgon >= 0. ? outgon * HC(dpn, i, k) : outgon * HC(dpn, ix, k);
-    af[e] = hi - lo;
-    mf[e] = lo;
-    HE(s.antiflx, i, e, k) = af[e];
-    HE(s.massflx, i, e, k) = mf[e];
+    mf[e] = HE(s.massflx, i, e, kk);
+    af[e] = HE(s.antiflx, i, e, kk);
   }
-  double flx = 0., d = 0.;
-  if (live) {
+  const double d = HC(dpn, i, kk);
+  double flx = 0.;
 #pragma unroll
-    for (int e = 0; e < NPP; e++)
-      if (e < np) flx = flx + mf[e];
-    d = HC(dpn, i, k);
-    HC(s.dpold, i, k) = d;
-  }
-  const double p = col_pres<LPC>(d, HP(i, 0), k, kdm);
+  for (int e = 0; e < NPP; e++)
+    if (e < np) flx = flx + mf[e];
+  if (live) HC(s.dpold, i, k) = d;
+  const double p = col_pres(sh[c.cc], live ? d : 0., HP(i, 0), k, kdm);
   if (!live) return;
   HP(i, k + 1) = p;
   double lo = d - flx * g.rarea[i] * dtleap;
@@ -183,19 +254,25 @@ __global__ void __launch_bounds__(256) hyc_limits(HycGrid g, HycState s, int lea
   const Col<LPC> c = hcol<LPC>(nip);
   if (!c.ok || c.k >= kdm) return;
   const int i = c.i, k = c.k;
-  if (!g.wet[i]) {
+  const int ci = g.cinfo[i];
+  int nb[NPP];      // neighbours, loaded before the mask is tested
+#pragma unroll
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) {
     HC(s.rplus, i, k) = HC(s.rmnus, i, k) = 0.;
     return;
   }
   const double *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
-  const double lo = HC(s.dp_lo, i, k);
-  double dpmax = hmax(lo, HC(dpn, i, k)), dpmin = hmin(lo, HC(dpn, i, k));
-  const int np = g.nprox[i];
-  for (int e = 0; e < np; e++) {
-    const int ix = g.prox[i * NPP + e];
-    if (!g.wet[ix]) continue;
-    dpmax = hmax(hmax(dpmax, HC(s.dp_lo, ix, k)), HC(dpn, ix, k));
-    dpmin = hmin(hmin(dpmin, HC(s.dp_lo, ix, k)), HC(dpn, ix, k));
+  const double lo = HC(s.dp_lo, i, k), dn = HC(dpn, i, k);
+  double dpmax = hmax(lo, dn), dpmin = hmin(lo, dn);
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const int ix = nb[e];
+    const double a = HC(s.dp_lo, ix, k), b = HC(dpn, ix, k);
+    const double nmax = hmax(hmax(dpmax, a), b), nmin = hmin(hmin(dpmin, a), b);
+    const bool open = ci_open(ci, e);
+    dpmax = open ? nmax : dpmax;
+    dpmin = open ? nmin : dpmin;
   }
   dpmin = hmax(0., dpmin);
   HC(s.rmnus, i, k) = (lo - dpmin) * g.area[i] / (dtleap * HC(s.flxplus, i, k));
@@ -203,104 +280,103 @@ __global__ void __launch_bounds__(256) hyc_limits(HycGrid g, HycState s, int lea
 }
 
 template <int LPC>
-__global__ void __launch_bounds__(256) hyc_clip(HycGrid g, HycState s, int leapn, double dtleap) {
+__global__ void __launch_bounds__(256) hyc_clip(HycGrid g, HycState s, double dtleap) {
+  __shared__ double sh[256 / LPC][NPP][LPC + 1];
   const int kdm = g.kdm, nip = g.nip;
   const Col<LPC> c = hcol<LPC>(nip);
   if (!c.ok) return;
   const int i = c.i, k = c.k;
   const bool live = k < kdm;
-  double cl[NPP];
-  for (int e = 0; e < NPP; e++) cl[e] = 0.;
-  if (g.wet[i]) {
-    const int np = g.nprox[i];
-    double flx = 0.;
+  const int kk = live ? k : 0;
+  const int ci = g.cinfo[i];
+  int nb[NPP];      // neighbours, loaded before the mask is tested
 #pragma unroll
-    for (int e = 0; e < NPP; e++) {
-      if (e >= np) break;
-      const int ix = g.prox[i * NPP + e];
-      double part = 0., a = 0.;         // an edge to land carries no flux
-      if (g.wet[ix]) {
-        if (live) {
-          a = HE(s.antiflx, i, e, k);
-          const double clip = a >= 0. ? hmin(hmin(1., HC(s.rmnus, i, k)), HC(s.rplus, ix, k))
-                                      : hmin(hmin(1., HC(s.rplus, i, k)), HC(s.rmnus, ix, k));
-          part = a * (1. - clip);
-          a = a * clip;
-          HE(s.massflx, i, e, k) = HE(s.massflx, i, e, k) + a;
-        }
-        cl[e] = col_sum<LPC>(part, kdm);
-      }
-      flx = flx + a;
-    }
-    double d = 0.;
-    double *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
-    if (live) {
-      d = HC(s.dp_lo, i, k) - flx * g.rarea[i] * dtleap;
-      if (d < 0. && d + H_ONECM > 0.) d = 0.;
-      HC(dpn, i, k) = d;
-    }
-    const double p = col_pres<LPC>(d, HP(i, 0), k, kdm);
-    if (live) HP(i, k + 1) = p;
-    if (k == kdm - 1) s.rpbot[i] = 1. / p;
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) {
+    if (k < NPP) s.clippd[i * NPP + k] = 0.;
+    return;
   }
-  if (k == 0)
-    for (int e = 0; e < NPP; e++) s.clippd[i * NPP + e] = cl[e];
+  const int np = ci_np(ci);
+  const double rmi = HC(s.rmnus, i, kk), rpi = HC(s.rplus, i, kk);
+  double part[NPP], flx = 0.;
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const int ix = nb[e];
+    const double a = HE(s.antiflx, i, e, kk), mfv = HE(s.massflx, i, e, kk);
+    const double rpx = HC(s.rplus, ix, kk), rmx = HC(s.rmnus, ix, kk);
+    const double clip = a >= 0. ? hmin(hmin(1., rmi), rpx) : hmin(hmin(1., rpi), rmx);
+    const bool open = ci_open(ci, e);
+    const double ac = a * clip;
+    part[e] = open && live ? a * (1. - clip) : 0.;
+    if (open && live) HE(s.massflx, i, e, k) = mfv + ac;
+    if (e < np) flx = flx + (open ? ac : a);    // a closed edge's antidiffusive flux is 0
+  }
+  const double cl = col_sums6<LPC>(sh[c.cc], part, k, kdm);
+  if (k < NPP) s.clippd[i * NPP + k] = cl;
+  double d = 0.;
+  if (live) {
+    d = HC(s.dp_lo, i, k) - flx * g.rarea[i] * dtleap;
+    if (d < 0. && d + H_ONECM > 0.) d = 0.;
+    HC(s.dpc, i, k) = d;
+  }
+  const double p = col_pres(sh[c.cc][0], d, HP(i, 0), k, kdm);
+  if (live) HP(i, k + 1) = p;
+  if (k == kdm - 1) s.rpbot[i] = 1. / p;
 }
 
+// the clipped fluxes back as an upstream barotropic correction and its
+// thickness update (434-491); with interface smoothing, dp again from the
+// interface pressures and del2p's own part (508-548)
 template <int LPC>
-__global__ void __launch_bounds__(256) hyc_recover(HycGrid g, HycState s, int leapn) {
-  const int kdm = g.kdm, nip = g.nip;
-  const Col<LPC> c = hcol<LPC>(nip);
-  if (!c.ok || c.k >= kdm) return;
-  const int i = c.i, k = c.k;
-  double flx = 0.;
-  if (g.wet[i]) {
-    const double *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
-    const int np = g.nprox[i];
-    for (int e = 0; e < np; e++) {
-      const int ix = g.prox[i * NPP + e];
-      if (!g.wet[ix]) continue;
-      const double cl = s.clippd[i * NPP + e];
-      double q = 0.;
-      if (cl > 0.) q = cl * (HC(dpn, i, k) * s.rpbot[i]);
-      else if (cl < 0.) q = cl * (HC(dpn, ix, k) * s.rpbot[ix]);
-      flx = flx + q;
-      HE(s.massflx, i, e, k) = HE(s.massflx, i, e, k) + q;
-    }
-  }
-  HC(s.flxdv, i, k) = flx;
-}
-
-// the recovered flux's update (465-491); with interface smoothing, dp again
-// from the interface pressures and del2p's own part (522-548)
-template <int LPC>
-__global__ void __launch_bounds__(256) hyc_update(HycGrid g, HycState s, int leapn, double dtleap, int smooth,
-                                                  double biharm) {
+__global__ void __launch_bounds__(256) hyc_recover(HycGrid g, HycState s, int leapn, double dtleap, int smooth,
+                                                   double biharm) {
+  __shared__ double sh[256 / LPC][LPC + 1];
   const int kdm = g.kdm, nip = g.nip;
   const Col<LPC> c = hcol<LPC>(nip);
   if (!c.ok) return;
   const int i = c.i, k = c.k;
   const bool live = k < kdm;
-  double *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
-  // pk: the pressure of interface k (the top of layer k) after the update
-  double pk = 0.;
-  if (g.wet[i]) {
-    double d = 0.;
-    if (live) {
-      d = HC(dpn, i, k) - HC(s.flxdv, i, k) * g.rarea[i] * dtleap;
-      if (d < 0. && d + H_ONECM > 0.) d = 0.;
-      HC(dpn, i, k) = d;
-    }
-    const double p0 = HP(i, 0);
-    const double p = col_pres<LPC>(d, p0, k, kdm);
-    if (live) HP(i, k + 1) = p;
-    const double pa = __shfl_up(p, 1, LPC);
-    pk = k > 0 ? pa : p0;
-    if (smooth && live) HC(dpn, i, k) = p - pk;
-  } else if (live) {
-    pk = HP(i, k);
+  const int kk = live ? k : 0;
+  const int ci = g.cinfo[i];
+  int nb[NPP];      // neighbours, loaded before the mask is tested
+#pragma unroll
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) {
+    if (smooth && live) HC(s.del2p, i, k) = biharm == 0. && k >= 1 ? HP(i, k) : H_LAND;
+    return;
   }
-  if (smooth && live) HC(s.del2p, i, k) = biharm == 0. && k >= 1 ? pk : H_LAND;
+  double *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
+  const double dci = HC(s.dpc, i, kk), rpi = s.rpbot[i];
+  double flx = 0.;
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const int ix = nb[e];
+    const double cl = s.clippd[i * NPP + e], dcx = HC(s.dpc, ix, kk), rpx = s.rpbot[ix];
+    const double mfv = HE(s.massflx, i, e, kk);
+    const double q = cl > 0. ? cl * (dci * rpi) : (cl < 0. ? cl * (dcx * rpx) : 0.);
+    if (ci_open(ci, e)) {
+      flx = flx + q;
+      if (live) HE(s.massflx, i, e, k) = mfv + q;
+    }
+  }
+  double d = 0.;
+  if (live) {
+    d = dci - flx * g.rarea[i] * dtleap;
+    if (d < 0. && d + H_ONECM > 0.) d = 0.;
+  }
+  const double p0 = HP(i, 0);
+  const double p = col_pres(sh[c.cc], d, p0, k, kdm);
+  if (live) HP(i, k + 1) = p;
+  if (smooth) {
+    const double pa = __shfl_up(p, 1, LPC);
+    const double pk = k > 0 ? pa : p0;         // the interface above layer k
+    if (live) {
+      HC(dpn, i, k) = p - pk;
+      HC(s.del2p, i, k) = biharm == 0. && k >= 1 ? pk : H_LAND;
+    }
+  } else if (live) {
+    HC(dpn, i, k) = d;
+  }
 }
 
 template <int LPC>
@@ -309,22 +385,24 @@ __global__ void __launch_bounds__(256) hyc_del2p(HycGrid g, HycState s, double b
   const Col<LPC> c = hcol<LPC>(nip);
   if (!c.ok || c.k >= kdm || c.k < 1) return;
   const int i = c.i, k = c.k;
-  if (!g.wet[i]) return;
-  const int np = g.nprox[i];
+  const int ci = g.cinfo[i];
+  int nb[NPP];      // neighbours, loaded before the mask is tested
+#pragma unroll
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) return;
   double sum = 0.;
-  int ncount = 0;
   const double pi = HP(i, k), pbi = HP(i, kdm);
-  for (int e = 0; e < np; e++) {
-    const int ix = g.prox[i * NPP + e];
-    if (!g.wet[ix]) continue;
-    ncount = ncount + 1;
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const int ix = nb[e];
     const double px = HP(ix, k), pbx = HP(ix, kdm);
     double deltap;
     if (pbx < pi) deltap = pbx - px;
     else if (pbi < px) deltap = pi - pbi;
     else deltap = pi - px;
-    sum = sum + deltap;
+    sum = ci_open(ci, e) ? sum + deltap : sum;
   }
+  const int ncount = __builtin_popcount((ci >> 4) & 63);
   HC(s.del2p, i, k) = biharm * sum / (double)ncount + (1. - biharm) * pi;
 }
 
@@ -338,136 +416,171 @@ __global__ void __launch_bounds__(256) hyc_bolus(HycGrid g, HycState s, int leap
   if (!c.ok) return;
   const int i = c.i, k = c.k;
   const bool live = k < kdm;
-  const bool wi = g.wet[i] != 0;
-  if (live)
-    for (int e = 0; e < NPP; e++) HE(s.antiflx, i, e, k) = wi ? 0. : H_LAND;
-  if (!wi) return;
-  const int np = g.nprox[i];
+  const int kk = live ? k : 0;
+  const int ci = g.cinfo[i];
+  int nb[NPP];      // neighbours, loaded before the mask is tested
+#pragma unroll
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) return;      // the bolus fluxes of land are never read
+  const int np = ci_np(ci);
+  const double pik = HP(i, kk), pbi = HP(i, kdm), di = HC(s.del2p, i, kk), ai = g.area[i];
   double fp = H_EPSIL;
-  for (int e = 0; e < np; e++) {
-    const int ix = g.prox[i * NPP + e];
-    double f = 0.;
-    if (g.wet[ix]) {
-      double b = 0.;      // the limited interface flux of interface k (layers k-1|k), lanes 1..kdm-1
-      if (live && k >= 1) {
-        const double q = dtleap * thkdff * (HC(s.del2p, i, k) - HC(s.del2p, ix, k)) * g.sideln[i * NPP + e];
-        const double flxmx = (HP(ix, kdm) - HP(ix, k)) * g.area[ix] / (double)g.nprox[ix];
-        const double flxmn = -((HP(i, kdm) - HP(i, k)) * g.area[i] / (double)np);
-        b = hmin(flxmx * .999, hmax(flxmn * .999, q));
-      }
-      // layer k's flux: interface k+1's minus interface k's (0 above the
-      // top), the bottom layer minus interface kdm-1's
-      const double bn = __shfl_down(b, 1, LPC);
-      if (live) f = k < kdm - 1 ? bn - b : -b;
-      if (live) HE(s.antiflx, i, e, k) = f;
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const int ix = nb[e];
+    const double dx = HC(s.del2p, ix, kk), pxk = HP(ix, kk), pbx = HP(ix, kdm), ax = g.area[ix];
+    const int npx = ci_np(g.cinfo[ix]);
+    const double sl = g.sideln[i * NPP + e];
+    // the limited flux of interface k (between layers k-1 and k), lanes 1..kdm-1
+    double b = 0.;
+    if (live && k >= 1) {
+      const double q = dtleap * thkdff * (di - dx) * sl;
+      const double flxmx = (pbx - pxk) * ax / (double)npx;
+      const double flxmn = -((pbi - pik) * ai / (double)np);
+      b = hmin(flxmx * .999, hmax(flxmn * .999, q));
     }
-    if (biharm > 0.) fp = fp + hmax(0., f);
+    // layer k's flux: interface k+1's minus interface k's (0 above the top),
+    // the bottom layer minus interface kdm-1's
+    const double bn = __shfl_down(b, 1, LPC);
+    double f = k < kdm - 1 ? bn - b : -b;
+    f = ci_open(ci, e) ? f : 0.;
+    if (live) HE(s.antiflx, i, e, k) = f;
+    if (biharm > 0. && e < np) fp = fp + hmax(0., f);
   }
   if (biharm > 0. && live) {
     const double *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
-    HC(s.rmnus, i, k) = HC(dpn, i, k) * g.area[i] / fp;
+    HC(s.rmnus, i, k) = HC(dpn, i, k) * ai / fp;
   }
 }
 
 template <int LPC>
 __global__ void __launch_bounds__(256) hyc_bclip(HycGrid g, HycState s, int leapn, double biharm, double masscor) {
+  __shared__ double sh[256 / LPC][NPP][LPC + 1];
   const int kdm = g.kdm, nip = g.nip;
   const Col<LPC> c = hcol<LPC>(nip);
   if (!c.ok) return;
   const int i = c.i, k = c.k;
   const bool live = k < kdm;
-  double cl[NPP];
-  for (int e = 0; e < NPP; e++) cl[e] = 0.;
-  if (g.wet[i]) {
-    const int np = g.nprox[i];
-    double flx = 0.;
+  const int kk = live ? k : 0;
+  const int ci = g.cinfo[i];
+  int nb[NPP];      // neighbours, loaded before the mask is tested
 #pragma unroll
-    for (int e = 0; e < NPP; e++) {
-      if (e >= np) break;
-      const int ix = g.prox[i * NPP + e];
-      double b = live ? HE(s.antiflx, i, e, k) : 0.;
-      if (biharm > 0. && g.wet[ix]) {
-        double part = 0.;
-        if (live && b != 0.) {
-          const double clip = b > 0. ? hmin(1., HC(s.rmnus, i, k)) : hmin(1., HC(s.rmnus, ix, k));
-          part = b * (1. - clip);
-          b = b * clip;
-        }
-        cl[e] = col_sum<LPC>(part, kdm);
-      }
-      flx = flx + b;
-    }
-    double d = 0.;
-    double *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
-    if (live) {
-      d = HC(dpn, i, k) - flx * g.rarea[i];
-      if (d < 0. && d + H_ONECM > 0.) d = 0.;
-      d = d * (1. + masscor);
-      HC(dpn, i, k) = d;
-      HC(s.dptmp, i, k) = d;
-    }
-    const double p = col_pres<LPC>(d, HP(i, 0), k, kdm);
-    if (live) HP(i, k + 1) = p;
-    if (k == kdm - 1) s.rpbot[i] = 1. / p;
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) {
+    if (biharm > 0. && k < NPP) s.clippd[i * NPP + k] = 0.;
+    return;
   }
-  if (biharm > 0. && k == 0)
-    for (int e = 0; e < NPP; e++) s.clippd[i * NPP + e] = cl[e];
+  const int np = ci_np(ci);
+  const double rmi = HC(s.rmnus, i, kk);
+  double part[NPP], flx = 0.;
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const int ix = nb[e];
+    double b = HE(s.antiflx, i, e, kk);
+    const double rmx = HC(s.rmnus, ix, kk);
+    double pt = 0.;
+    if (biharm > 0.) {
+      const double clip = b > 0. ? hmin(1., rmi) : hmin(1., rmx);
+      const bool doit = ci_open(ci, e) && b != 0.;
+      pt = doit ? b * (1. - clip) : 0.;
+      b = doit ? b * clip : b;
+    }
+    part[e] = live ? pt : 0.;
+    if (e < np) flx = flx + b;
+  }
+  if (biharm > 0.) {
+    const double cl = col_sums6<LPC>(sh[c.cc], part, k, kdm);
+    if (k < NPP) s.clippd[i * NPP + k] = cl;
+  }
+  double d = 0.;
+  double *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
+  if (live) {
+    d = HC(dpn, i, k) - flx * g.rarea[i];
+    if (d < 0. && d + H_ONECM > 0.) d = 0.;
+    d = d * (1. + masscor);
+    HC(dpn, i, k) = d;
+    HC(s.dptmp, i, k) = d;
+  }
+  const double p = col_pres(sh[c.cc][0], d, HP(i, 0), k, kdm);
+  if (live) HP(i, k + 1) = p;
+  if (k == kdm - 1) s.rpbot[i] = 1. / p;
 }
 
 template <int LPC>
 __global__ void __launch_bounds__(256) hyc_final(HycGrid g, HycState s, int leapm, int leapn, double dtleap,
                                                  int smooth, double biharm) {
+  __shared__ double sh[256 / LPC][LPC + 1];
   const int kdm = g.kdm, nip = g.nip;
   const Col<LPC> c = hcol<LPC>(nip);
   if (!c.ok) return;
   const int i = c.i, k = c.k;
   const bool live = k < kdm;
-  if (!g.wet[i]) {
+  const int kk = live ? k : 0;
+  const int ci = g.cinfo[i];
+  int nb[NPP];      // neighbours, loaded before the mask is tested
+#pragma unroll
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) {
     if (k == 0) s.aux[i] = 0.;
     return;
   }
   double *dpm = s.dp + (size_t)(leapm - 1) * nip * kdm, *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
-  const int np = g.nprox[i];
+  const int np = ci_np(ci);
+  double mf[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) mf[e] = HE(s.massflx, i, e, kk);
+  double dn = HC(dpn, i, kk);
   if (smooth && biharm > 0.) {
     const double dtinv = 1. / dtleap;
-    double flx = 0., d = 0.;
-    if (live) {
-      for (int e = 0; e < np; e++) {
-        const int ix = g.prox[i * NPP + e];
-        const double cl = s.clippd[i * NPP + e];
-        double q = 0.;
-        if (cl > 0.) q = cl * HC(s.dptmp, i, k) * s.rpbot[i];
-        else if (cl < 0.) q = cl * HC(s.dptmp, ix, k) * s.rpbot[ix];
+    const double dti = HC(s.dptmp, i, kk), rpi = s.rpbot[i];
+    double flx = 0.;
+#pragma unroll
+    for (int e = 0; e < NPP; e++) {
+      const int ix = nb[e];
+      const double cl = s.clippd[i * NPP + e], dtx = HC(s.dptmp, ix, kk), rpx = s.rpbot[ix];
+      const double q = cl > 0. ? cl * dti * rpi : (cl < 0. ? cl * dtx * rpx : 0.);
+      if (e < np) {
         flx = flx + q;
-        HE(s.massflx, i, e, k) = HE(s.massflx, i, e, k) + q * dtinv;
+        mf[e] = mf[e] + q * dtinv;
+        if (live) HE(s.massflx, i, e, k) = mf[e];
       }
-      d = HC(dpn, i, k) - flx * g.rarea[i];
-      HC(dpn, i, k) = d;
     }
-    const double p = col_pres<LPC>(d, HP(i, 0), k, kdm);
+    dn = dn - flx * g.rarea[i];
+    if (live) HC(dpn, i, k) = dn;
+    const double p = col_pres(sh[c.cc], live ? dn : 0., HP(i, 0), k, kdm);
     if (live) HP(i, k + 1) = p;
   }
   if (!live) return;
   if (smooth)
-    for (int e = 0; e < np; e++) HE(s.mssfx_avg, i, e, k) = HE(s.mssfx_avg, i, e, k) + HE(s.massflx, i, e, k);
-  HC(dpm, i, k) = H_ASSLDP * (HC(s.dpold, i, k) + HC(dpn, i, k)) + (1. - H_ASSLDP - H_ASSLDP) * HC(dpm, i, k);
+#pragma unroll
+    for (int e = 0; e < NPP; e++)
+      if (e < np) HE(s.mssfx_avg, i, e, k) = HE(s.mssfx_avg, i, e, k) + mf[e];
+  HC(dpm, i, k) = H_ASSLDP * (HC(s.dpold, i, k) + dn) + (1. - H_ASSLDP - H_ASSLDP) * HC(dpm, i, k);
   if (k == kdm - 1) s.aux[i] = HP(i, kdm);
 }
 
-// duplicate edge fluxes must be equal and opposite bit for bit (chkdup)
+// duplicate edge fluxes must be equal and opposite bit for bit (chkdup); the
+// cell of lower index checks each pair
 template <int LPC>
 __global__ void __launch_bounds__(256) hyc_check(HycGrid g, HycState s) {
   const int kdm = g.kdm, nip = g.nip;
   const Col<LPC> c = hcol<LPC>(nip);
-  if (!c.ok || c.k >= kdm || !g.wet[c.i]) return;
+  if (!c.ok || c.k >= kdm) return;
   const int i = c.i, k = c.k;
-  const int np = g.nprox[i];
-  for (int e = 0; e < np; e++) {
-    const int ix = g.prox[i * NPP + e], ex = g.proxs[i * NPP + e];
-    if (!g.wet[ix]) continue;
+  const int ci = g.cinfo[i];
+  int nb[NPP];      // neighbours, loaded before the mask is tested
+#pragma unroll
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) return;
+  bool bad = false;
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const int ix = nb[e], ex = g.proxs[i * NPP + e];
+    if (!ci_open(ci, e) || ix < i) continue;
     const double a = HE(s.massflx, i, e, k), b = HE(s.massflx, ix, ex, k);
-    if (a + b != 0. && a - b != 0.) atomicOr(s.flag, 1);
+    bad = bad || (a + b != 0. && a - b != 0.);
   }
+  if (bad) atomicOr(s.flag, 1);
 }
 
 }  // namespace
@@ -533,20 +646,34 @@ extern "C" int fimhip_ocn_create(const fimhip_grid_desc *gd, const double *rprox
       sve[(i * NPP + e) * 2 + 1] = gd->sidevec_e[(i * NPP + e) * 2 + 1];
     }
   }
+  // every edge's twin must point back (each pair of edge fluxes is formed once)
+  for (int i = 0; i < nip; i++)
+    for (int e = 0; e < nprox[i]; e++) {
+      const int x = prox[i * NPP + e], xs = proxs[i * NPP + e];
+      if (xs >= nprox[x] || prox[x * NPP + xs] != i) { fimhip_ocn_destroy(o); return FIMHIP_EINVAL; }
+    }
+  // the packed per-cell word: edge count, own mask, mask of each edge's neighbour
+  std::vector<int> cinfo(nip);
+  for (int i = 0; i < nip; i++) {
+    int w = nprox[i] | (o->wet[i] << 3);
+    for (int e = 0; e < nprox[i]; e++)
+      if (o->wet[prox[i * NPP + e]]) w |= 1 << (4 + e);
+    cinfo[i] = w;
+  }
   int r = 0;
   HycGrid &g = o->g;
   g.nip = nip;
   g.kdm = kdm;
-  int *pp, *ps, *pn, *pw;
+  int *pp, *ps, *pc;
   double *pa, *pr, *pv, *psl, *prs, *prl;
-  if ((r = o->arena.upload(&pp, prox)) || (r = o->arena.upload(&ps, proxs)) || (r = o->arena.upload(&pn, nprox)) ||
-      (r = o->arena.upload(&pw, o->wet)) || (r = o->arena.upload(&pa, o->area)) || (r = o->arena.upload(&pr, rarea)) ||
-      (r = o->arena.upload(&pv, sve)) || (r = o->arena.upload(&psl, sideln)) ||
-      (r = o->arena.upload(&prs, rsideln)) || (r = o->arena.upload(&prl, rpl))) {
+  if ((r = o->arena.upload(&pp, prox)) || (r = o->arena.upload(&ps, proxs)) || (r = o->arena.upload(&pc, cinfo)) ||
+      (r = o->arena.upload(&pa, o->area)) || (r = o->arena.upload(&pr, rarea)) || (r = o->arena.upload(&pv, sve)) ||
+      (r = o->arena.upload(&psl, sideln)) || (r = o->arena.upload(&prs, rsideln)) ||
+      (r = o->arena.upload(&prl, rpl))) {
     fimhip_ocn_destroy(o);
     return r;
   }
-  g.prox = pp; g.proxs = ps; g.nprox = pn; g.wet = pw;
+  g.prox = pp; g.proxs = ps; g.cinfo = pc;
   g.area = pa; g.rarea = pr; g.sve = pv; g.sideln = psl; g.rsideln = prs; g.rprox_ln = prl;
   const size_t cz = (size_t)nip * kdm, ez = cz * NPP;
   HycState &s = o->s;
@@ -554,7 +681,7 @@ extern "C" int fimhip_ocn_create(const fimhip_grid_desc *gd, const double *rprox
   struct { double **p; size_t n; } arrs[] = {
       {&um, ez}, {&vm, ez}, {&un, ez}, {&vn, ez}, {&dpe, ez}, {&s.dp, 2 * cz}, {&s.pres, (size_t)nip * (kdm + 1)},
       {&s.massflx, ez}, {&s.mssfx_avg, ez}, {&s.antiflx, ez}, {&s.dp_lo, cz}, {&s.dpold, cz}, {&s.flxplus, cz},
-      {&s.flxmnus, cz}, {&s.rplus, cz}, {&s.rmnus, cz}, {&s.del2p, cz}, {&s.dptmp, cz}, {&s.flxdv, cz},
+      {&s.flxmnus, cz}, {&s.rplus, cz}, {&s.rmnus, cz}, {&s.del2p, cz}, {&s.dptmp, cz}, {&s.dpc, cz},
       {&s.clippd, (size_t)nip * NPP}, {&s.rpbot, (size_t)nip}, {&s.aux, (size_t)nip}};
   for (auto &a : arrs)
     if ((r = o->arena.alloc(a.p, a.n))) { fimhip_ocn_destroy(o); return r; }
@@ -627,21 +754,23 @@ static void ocn_kernels(fimhip_ocn *o, int leapm, int leapn) {
   const HycGrid &g = o->g;
   const HycState &s = o->s;
   hipStream_t st = o->stream;
-  const dim3 grid((o->nip + 256 / LPC - 1) / (256 / LPC)), blk(256);
+  constexpr int CPB = 256 / LPC;
+  const int nblk = (o->nip + CPB - 1) / CPB;
+  const dim3 grid((nblk + 7) / 8 * 8), blk(256);     // a multiple of 8 blocks: see hcol
   const double dt = o->dtleap;
   const int smooth = o->thkdff != 0.;
   KT_LAUNCH("hyc_flux", st, hyc_flux<LPC><<<grid, blk, 0, st>>>(g, s, leapm, leapn, dt));
+  KT_LAUNCH("hyc_lowsum", st, hyc_lowsum<LPC><<<grid, blk, 0, st>>>(g, s, leapn, dt));
   KT_LAUNCH("hyc_limits", st, hyc_limits<LPC><<<grid, blk, 0, st>>>(g, s, leapn, dt));
-  KT_LAUNCH("hyc_clip", st, hyc_clip<LPC><<<grid, blk, 0, st>>>(g, s, leapn, dt));
-  KT_LAUNCH("hyc_recover", st, hyc_recover<LPC><<<grid, blk, 0, st>>>(g, s, leapn));
-  KT_LAUNCH("hyc_update", st, hyc_update<LPC><<<grid, blk, 0, st>>>(g, s, leapn, dt, smooth, o->biharm));
+  KT_LAUNCH("hyc_clip", st, hyc_clip<LPC><<<grid, blk, 0, st>>>(g, s, dt));
+  KT_LAUNCH("hyc_recover", st, hyc_recover<LPC><<<grid, blk, 0, st>>>(g, s, leapn, dt, smooth, o->biharm));
   if (smooth) {
     if (o->biharm > 0.) KT_LAUNCH("hyc_del2p", st, hyc_del2p<LPC><<<grid, blk, 0, st>>>(g, s, o->biharm));
     KT_LAUNCH("hyc_bolus", st, hyc_bolus<LPC><<<grid, blk, 0, st>>>(g, s, leapn, dt, o->thkdff, o->biharm));
     KT_LAUNCH("hyc_bclip", st, hyc_bclip<LPC><<<grid, blk, 0, st>>>(g, s, leapn, o->biharm, o->masscor));
   }
   KT_LAUNCH("hyc_final", st, hyc_final<LPC><<<grid, blk, 0, st>>>(g, s, leapm, leapn, dt, smooth, o->biharm));
-  hyc_check<LPC><<<grid, blk, 0, st>>>(g, s);
+  KT_LAUNCH("hyc_check", st, hyc_check<LPC><<<grid, blk, 0, st>>>(g, s));
 }
 
 // one call of cnuity on the device-resident arrays (hycom_cnuity.F90:21-361)
@@ -708,6 +837,12 @@ extern "C" int fimhip_ocn_cnuity_r8(fimhip_ocn *o, int nstep, int leapm, int lea
   }
   HIPCHK(hipStreamSynchronize(o->stream));
   return r;
+}
+
+extern "C" int fimhip_ocn_set_dtleap(fimhip_ocn *o, double dtleap) {
+  if (!o || !(dtleap > 0.)) return FIMHIP_EINVAL;
+  o->dtleap = dtleap;
+  return FIMHIP_OK;
 }
 
 extern "C" int fimhip_ocn_mass(const fimhip_ocn *o, double *totmass, double *massglb0, double *masscor) {

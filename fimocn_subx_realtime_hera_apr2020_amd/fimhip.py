@@ -49,7 +49,7 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ens_set_wet", "fimhip_ls_pseudo_inverse",
            "fimhip_ocn_create", "fimhip_ocn_destroy", "fimhip_ocn_cnuity_r8", "fimhip_ocn_cnuity",
            "fimhip_ocn_field_size", "fimhip_ocn_upload", "fimhip_ocn_download", "fimhip_ocn_mass",
-           "fimhip_ocn_set_mass", "fimhip_ocn_profile", "fimhip_ocn_profile_report"]
+           "fimhip_ocn_set_mass", "fimhip_ocn_set_dtleap", "fimhip_ocn_profile", "fimhip_ocn_profile_report"]
 
 # real*4 drop-ins: name -> (number of float arrays, trailing int argument)
 R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, True), "mass_adv": (12, True),
@@ -172,6 +172,7 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ocn_download.argtypes = [vp, ct.c_int, _PD]
     lib.fimhip_ocn_mass.argtypes = [vp, _PD, _PD, _PD]
     lib.fimhip_ocn_set_mass.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double]
+    lib.fimhip_ocn_set_dtleap.argtypes = [vp, ct.c_double]
     lib.fimhip_ocn_profile.argtypes = [vp, ct.c_int]
     lib.fimhip_ocn_profile_report.argtypes = [vp, ct.c_char_p, ct.c_int]
     _LIB = lib
@@ -702,6 +703,10 @@ class OceanCnuity:
     @mass.setter
     def mass(self, m):
         _chk(self.lib.fimhip_ocn_set_mass(self.h, *map(float, m)), "ocn_set_mass")
+
+    def set_dtleap(self, dtleap: float):
+        """hycom_constants' dtleap for the calls that follow (hycom_run.F90:136)."""
+        _chk(self.lib.fimhip_ocn_set_dtleap(self.h, float(dtleap)), "ocn_set_dtleap")
 
     def profile(self, on: bool = True):
         _chk(self.lib.fimhip_ocn_profile(self.h, int(on)), "ocn_profile")

@@ -142,6 +142,34 @@ module fimhip_bind
     real(c_double) function fimhip_pow(x, y) bind(C, name="fimhip_pow")
       import; real(c_double), value :: x, y
     end function
+    ! the HYCOM ocean's continuity equation (hycom_cnuity.F90:17): an instance
+    ! made in hycom_init from the grid descriptor, glvl.dat's rprox_ln, kdm, the
+    ! land/sea mask wet, dtleap, thkdff and biharm
+    integer(c_int) function fimhip_ocn_create(grid, rprox_ln, kdm, wet, dtleap, thkdff, biharm, device, ocn) &
+        bind(C, name="fimhip_ocn_create")
+      import; type(fimhip_grid_desc), intent(in) :: grid
+      real(c_double), intent(in) :: rprox_ln(*); integer(c_int), value :: kdm, device
+      integer(c_int), intent(in) :: wet(*); real(c_double), value :: dtleap, thkdff, biharm
+      type(c_ptr), intent(out) :: ocn
+    end function
+    integer(c_int) function fimhip_ocn_destroy(ocn) bind(C, name="fimhip_ocn_destroy")
+      import; type(c_ptr), value :: ocn
+    end function
+    integer(c_int) function fimhip_ocn_cnuity_r8(ocn, nstep, leapm, leapn, um_edg, vm_edg, un_edg, vn_edg, &
+        dpm_edg, dp, pres, massflx, mssfx_avg) bind(C, name="fimhip_ocn_cnuity_r8")
+      import; type(c_ptr), value :: ocn; integer(c_int), value :: nstep, leapm, leapn
+      real(c_double), intent(in) :: um_edg(*), vm_edg(*), un_edg(*), vn_edg(*), dpm_edg(*)
+      real(c_double) :: dp(*), pres(*), massflx(*), mssfx_avg(*)
+    end function
+    integer(c_int) function fimhip_ocn_set_dtleap(ocn, dtleap) bind(C, name="fimhip_ocn_set_dtleap")
+      import; type(c_ptr), value :: ocn; real(c_double), value :: dtleap
+    end function
+    integer(c_int) function fimhip_ocn_mass(ocn, totmass, massglb0, masscor) bind(C, name="fimhip_ocn_mass")
+      import; type(c_ptr), value :: ocn; real(c_double) :: totmass, massglb0, masscor
+    end function
+    integer(c_int) function fimhip_ocn_set_mass(ocn, totmass, massglb0, masscor) bind(C, name="fimhip_ocn_set_mass")
+      import; type(c_ptr), value :: ocn; real(c_double), value :: totmass, massglb0, masscor
+    end function
   end interface
 
   ! field ids of fimhip_upload / fimhip_download (the FIMHIP_* enum)

@@ -383,6 +383,9 @@ int fimhip_ocn_cnuity(fimhip_ocn *ocn, int nstep, int leapm, int leapn);
 long long fimhip_ocn_field_size(const fimhip_ocn *ocn, int field);
 int fimhip_ocn_upload(fimhip_ocn *ocn, int field, const double *host);
 int fimhip_ocn_download(fimhip_ocn *ocn, int field, double *host);
+/* hycom_constants' dtleap for the calls that follow (hycom_run.F90:136 sets
+ * it every step: batrop at the first, 2*batrop after) */
+int fimhip_ocn_set_dtleap(fimhip_ocn *ocn, double dtleap);
 /* hycom_variables totmass, massglb0, masscor (a restart sets them) */
 int fimhip_ocn_mass(const fimhip_ocn *ocn, double *totmass, double *massglb0, double *masscor);
 int fimhip_ocn_set_mass(fimhip_ocn *ocn, double totmass, double massglb0, double masscor);

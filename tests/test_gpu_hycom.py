@@ -75,6 +75,33 @@ def test_resident_matches_checker_bitwise(oracle_lib, level, kdm, name):
            resident=True)
 
 
+def test_leapfrog_start_with_changing_dtleap(oracle_lib):
+    """hycom_run sets dtleap = batrop at the first step and 2 * batrop after
+    (hycom_run.F90:136): the instance takes the new value between calls."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, ocean
+    print("case: G4 x 32, dtleap 900 s then 1800 s, drop-in", flush=True)
+    g = _grid(4)
+    case = ocean.hycom_cnuity_case(g, kdm=32, dtleap=900.0)
+    chk = oracle_lib.HycomCnuity(g, case)
+    ocn = fimhip.OceanCnuity(g, 32, case["wet"], 900.0)
+    try:
+        a = {n: np.ascontiguousarray(case[n], np.float64).copy() for n in fimhip.OCN_FIELDS}
+        for st in (1, 2, 3):
+            dtleap = 900.0 * min(st, 2)
+            chk.g.dtleap = dtleap
+            ocn.set_dtleap(dtleap)
+            leapm = 1 + st % 2
+            assert chk.step(st, leapm, 3 - leapm) == 0
+            ocn.cnuity(st, leapm, 3 - leapm, *(a[n] for n in fimhip.OCN_FIELDS[2:7]), *(a[n] for n in ARRAYS))
+            for n in ARRAYS:
+                assert _same(a[n], chk.a[n].reshape(a[n].shape)), (st, n)
+            assert ocn.mass == chk.mass
+        with pytest.raises(fimhip.FimhipError):
+            ocn.set_dtleap(0.0)
+    finally:
+        ocn.close()
+
+
 def test_g7_full_size_matches_checker_bitwise(oracle_lib):
     from fimocn_subx_realtime_hera_apr2020_amd import ocean
     print("case: G7 x 32, resident, biharm, 3 steps", flush=True)
