@@ -176,3 +176,45 @@ def hycom_cnuity_case(grid, kdm: int = 32, seed: int = 20240607, dtleap: float =
     return dict(kdm=kdm, dtleap=dtleap, wet=wet, dp=np.ascontiguousarray(dp), pres=pres,
                 um_edg=on_edges(um), vm_edg=on_edges(vm), un_edg=on_edges(un), vn_edg=on_edges(vn),
                 dpm_edg=on_edges(dp[0]), massflx=z.copy(), mssfx_avg=z.copy())
+
+
+def hycom_edgvar_case(grid, kdm: int = 32, seed: int = 20240611, dtleap: float = 1800.0) -> dict:
+    """The inputs of bclin_edgvar (hycom_edgvar.F90:128) on a glvl.Grid, on top
+    of hycom_cnuity_case's (whose dp, pres, wet, massflx and mssfx_avg it
+    keeps): the baroclinic velocities uclin/vclin (2, nip, kdm) at the two
+    leapfrog levels (0.2 m/s tilted solid body, N(0, 0.1) noise), the
+    barotropic ones utrop/vtrop (nip, 3) (N(0, 0.05)), the geopotential
+    anomaly geop (nip, kdm+1) at the interfaces (decreasing with depth from
+    a smooth surface anomaly), the Montgomery potential montg and the
+    specific-volume anomaly spcifv (nip, kdm) (layer means of smooth
+    profiles with noise), and the edge arrays bclin_edgvar writes, all 7.0
+    at the start (um_edg, vm_edg, un_edg, vn_edg, dpm_edg - the dp_edg that
+    cnuity reads as its dpm_edg -, geop_edg, mont_edg, uvsq_edg, spcv_edg
+    (nip, 6, kdm) and pr_edg (nip, 6, kdm+1)).  Land cells carry the same
+    construction; bclin_edgvar reads some of their values with zero weight."""
+    case = hycom_cnuity_case(grid, kdm=kdm, seed=seed, dtleap=dtleap)
+    rng = np.random.default_rng(seed + 1)
+    lat = np.asarray(grid.lat, np.float64)
+    lon = np.asarray(grid.lon, np.float64)
+    nip = lat.shape[0]
+    alpha = np.pi / 4
+    u0 = 0.2 * (np.cos(lat) * np.cos(alpha) + np.sin(lat) * np.cos(lon) * np.sin(alpha))
+    v0 = -0.2 * np.sin(lon) * np.sin(alpha)
+    uc = np.stack([u0[:, None] + 0.1 * rng.normal(size=(nip, kdm)) for _ in range(2)])
+    vc = np.stack([v0[:, None] + 0.1 * rng.normal(size=(nip, kdm)) for _ in range(2)])
+    ut = 0.05 * rng.normal(size=(nip, 3))
+    vt = 0.05 * rng.normal(size=(nip, 3))
+    pres = case["pres"]
+    depth = pres / ONEM
+    surf = 2.0 * smooth_field(lat, lon, seed + 5)
+    geop = (surf[:, None] - 0.01 * depth) * C.GRVITY * (1.0 + 0.01 * rng.uniform(-1.0, 1.0, pres.shape))
+    zmid = 0.5 * (depth[:, :-1] + depth[:, 1:])
+    spcifv = (1.0e-3 / 1.025) * (1.0 + 2.0e-3 * np.exp(-zmid / 800.0) + 1.0e-4 * rng.normal(size=(nip, kdm)))
+    montg = 0.5 * (geop[:, :-1] + geop[:, 1:]) + 0.02 * rng.normal(size=(nip, kdm))
+    case.update(uclin=np.ascontiguousarray(uc), vclin=np.ascontiguousarray(vc), utrop=ut, vtrop=vt,
+                geop=np.ascontiguousarray(geop), montg=np.ascontiguousarray(montg),
+                spcifv=np.ascontiguousarray(spcifv))
+    for n in ("um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "geop_edg", "mont_edg", "uvsq_edg", "spcv_edg"):
+        case[n] = np.full((nip, 6, kdm), 7.0)
+    case["pr_edg"] = np.full((nip, 6, kdm + 1), 7.0)
+    return case

@@ -1,7 +1,8 @@
 /* hycom_oracle.c - CPU restatement, in plain C and FP64, of the HYCOM ocean
  * continuity equation (FIMsrc/fim/horizontal/hycom_cnuity.F90, subroutine
- * cnuity).  TEST INFRASTRUCTURE ONLY: the checker the HIP kernels are
- * compared with; the product never links it.
+ * cnuity) and of its edge interpolation (hycom_edgvar.F90, bclin_edgvar with
+ * uvedge and blokf).  TEST INFRASTRUCTURE ONLY: the checker the HIP kernels
+ * are compared with; the product never links it.
  *
  * Every loop follows the reference operation for operation, in the
  * reference loop order, so that with fp contraction off it reproduces the
@@ -347,4 +348,201 @@ int fo_hycom_cnuity(const fo_hyc_grid *g, fo_hyc_mass *ms, int nstep, int leapm,
   free(antiflx); free(bolusfx); free(flxdv); free(dp_lo); free(dpold); free(flxplus); free(flxmnus);
   free(rplus); free(rmnus); free(del2p); free(dptmp); free(clippd); free(rpbot); free(aux1); free(sum);
   return bad;
+}
+
+/* ------------------------------------------------------------------------
+ * hycom_edgvar.F90.  Layouts as above, plus pr_edg (kdm+1,npp,nip) as
+ * [(i*6+e)*(kdm+1) + k], utrop/vtrop (3,nip) as [i*3 + slot-1] and
+ * cs/sn (4,npp,nip) as [(i*6+e)*4 + j].  The reference's nedge/permedge are
+ * nprox and the identity in a serial run. */
+static const double SLIP = -1.;               /* hycom_constants.F90:79 */
+#define TENM (10. * ONEM)                     /* hycom_constants.F90:36 */
+static const double DIVBY18 = 1. / 18.;       /* hycom_edgvar.F90:26 */
+
+/* hycom_edgvar.F90:6-124: rotate the velocities of the edge's two cells (double
+ * weight) and of the two joint neighbours into the edge's local frame and
+ * interpolate, walls by the slip condition, joint land neighbours as 0 */
+static void hyc_uvedge(const fo_hyc_grid *g, const double *cs, const double *sn, const double *uvel,
+                       const double *vvel, double *u_edg, double *v_edg) {
+  const int nip = g->nip, kdm = g->kdm;
+  for (int i = 0; i < nip; i++) {
+    for (int e = 0; e < NPP; e++)
+      for (int k = 0; k < kdm; k++) E(u_edg, i, e, k) = E(v_edg, i, e, k) = 0.;
+    const int np = g->nprox[i];
+    for (int edg = 0; edg < np; edg++) {
+      const int ipx = g->prox[i * NPP + edg];
+      const int im1 = g->prox[i * NPP + (edg - 1 + np) % np];
+      const int ip1 = g->prox[i * NPP + (edg + 1) % np];
+      const double *c = cs + (size_t)(i * NPP + edg) * 4, *s = sn + (size_t)(i * NPP + edg) * 4;
+      for (int k = 0; k < kdm; k++) {
+        double u1 = c[0] * C(uvel, i, k) + s[0] * C(vvel, i, k);
+        double v1 = -s[0] * C(uvel, i, k) + c[0] * C(vvel, i, k);
+        double u2, v2, u3, v3, u4, v4;
+        if (WET(ipx)) {
+          u2 = c[1] * C(uvel, ipx, k) + s[1] * C(vvel, ipx, k);
+          v2 = -s[1] * C(uvel, ipx, k) + c[1] * C(vvel, ipx, k);
+          if (g->wet[i] == 0) {
+            u1 = u2 * SLIP;
+            v1 = v2 * SLIP;
+          }
+        } else {
+          u2 = u1 * SLIP;
+          v2 = v1 * SLIP;
+        }
+        if (WET(im1)) {
+          u3 = c[2] * C(uvel, im1, k) + s[2] * C(vvel, im1, k);
+          v3 = -s[2] * C(uvel, im1, k) + c[2] * C(vvel, im1, k);
+        } else {
+          u3 = v3 = 0.;
+        }
+        if (WET(ip1)) {
+          u4 = c[3] * C(uvel, ip1, k) + s[3] * C(vvel, ip1, k);
+          v4 = -s[3] * C(uvel, ip1, k) + c[3] * C(vvel, ip1, k);
+        } else {
+          u4 = v4 = 0.;
+        }
+        E(u_edg, i, edg, k) = (8. * (u1 + u2) + u3 + u4) * DIVBY18;
+        E(v_edg, i, edg, k) = (8. * (v1 + v2) + v3 + v4) * DIVBY18;
+      }
+    }
+  }
+}
+
+/* hycom_edgvar.F90:479-493: the extent to which a sidewall is present */
+static double hyc_blokf(double dp, double plo, double pbx, double pupx) {
+  (void)pupx;
+  return dmax2(0., dmin2(1., (plo - pbx) / dmax2(EPSIL, dp)));
+}
+
+void fo_hycom_edgvar(const fo_hyc_grid *g, const double *cs, const double *sn, int janjic, int leapm, int leapn,
+                     const double *uclin, const double *vclin, const double *utrop, const double *vtrop,
+                     const double *dp, double *pres, const double *geop, const double *montg,
+                     const double *spcifv, double *um_edg, double *vm_edg, double *un_edg, double *vn_edg,
+                     double *dp_edg, double *pr_edg, double *geop_edg, double *mont_edg, double *uvsq_edg,
+                     double *spcv_edg) {
+  const int nip = g->nip, kdm = g->kdm, kp = kdm + 1;
+  const size_t cz = (size_t)nip * kdm;
+  const double *dpm = dp + (size_t)(leapm - 1) * cz;
+  double *utotm = malloc(cz * sizeof(double)), *utotn = malloc(cz * sizeof(double));
+  double *vtotm = malloc(cz * sizeof(double)), *vtotn = malloc(cz * sizeof(double));
+  double *geop_lyr = malloc(cz * sizeof(double)), *geopx = malloc(cz * NPP * sizeof(double));
+  double *p_lyr = malloc((size_t)kdm * sizeof(double));
+#define GEO(i, k) geop[(size_t)(i) * kp + (k)]
+#define PR(i, e, k) pr_edg[((size_t)(i) * NPP + (e)) * kp + (k)]
+  if (janjic) {
+    /* hydrostatic integration to the layer's mid pressure in the neighbouring
+     * columns, down to the shallowest neighbour at most (200-265) */
+    for (int i = 0; i < nip; i++) {
+      for (int k = 0; k < kdm; k++) {
+        C(geop_lyr, i, k) = LAND_SPVAL;
+        for (int e = 0; e < NPP; e++) E(geopx, i, e, k) = LAND_SPVAL;
+      }
+      if (!WET(i)) continue;
+      const int np = g->nprox[i];
+      for (int k = 0; k < kdm; k++) {
+        p_lyr[k] = .5 * (P(i, k) + P(i, k + 1));
+        C(geop_lyr, i, k) = .5 * (GEO(i, k) + GEO(i, k + 1));
+      }
+      double silhou = 1.e33;
+      for (int e = 0; e < np; e++) {
+        for (int k = 0; k < kdm; k++) E(geopx, i, e, k) = C(geop_lyr, i, k);
+        const int ipx = g->prox[i * NPP + e];
+        if (WET(ipx)) silhou = dmin2(silhou, P(ipx, kdm));
+      }
+      for (int e = 0; e < np; e++) {
+        const int ipx = g->prox[i * NPP + e];
+        if (!WET(ipx)) continue;
+        for (int k = 0; k < kdm; k++) {
+          const double plyrk = dmin2(p_lyr[k], silhou);
+          int kx = 0;
+          while (kx < kdm - 1 && P(ipx, kx + 1) < plyrk) kx++;
+          if (P(ipx, kx + 1) >= plyrk) {
+            const double thav = C(spcifv, ipx, kx);
+            E(geopx, i, e, k) = GEO(ipx, kx) - thav * (plyrk - P(ipx, kx));
+          }
+        }
+      }
+      /* nudge the pressure force to zero below the silhouette depth */
+      for (int k = 0; k < kdm; k++) {
+        const double tapr = dmax2(0., dmin2(1., (p_lyr[k] - silhou) / (100. * ONEM)));
+        for (int e = 0; e < np; e++) E(geopx, i, e, k) = (1. - tapr) * E(geopx, i, e, k) + tapr * C(geop_lyr, i, k);
+      }
+    }
+  }
+  /* total velocities and the interface pressure at mid-time (273-291) */
+  for (int i = 0; i < nip; i++) {
+    for (int k = 0; k < kdm; k++) C(utotm, i, k) = C(utotn, i, k) = C(vtotm, i, k) = C(vtotn, i, k) = LAND_SPVAL;
+    if (!WET(i)) continue;
+    for (int k = 0; k < kdm; k++) {
+      C(utotm, i, k) = C(uclin + (size_t)(leapm - 1) * cz, i, k) + utrop[i * 3 + leapm - 1];
+      C(utotn, i, k) = C(uclin + (size_t)(leapn - 1) * cz, i, k) + utrop[i * 3 + leapn - 1];
+      C(vtotm, i, k) = C(vclin + (size_t)(leapm - 1) * cz, i, k) + vtrop[i * 3 + leapm - 1];
+      C(vtotn, i, k) = C(vclin + (size_t)(leapn - 1) * cz, i, k) + vtrop[i * 3 + leapn - 1];
+      P(i, k + 1) = P(i, k) + C(dpm, i, k);
+    }
+  }
+  hyc_uvedge(g, cs, sn, utotm, vtotm, um_edg, vm_edg);
+  hyc_uvedge(g, cs, sn, utotn, vtotn, un_edg, vn_edg);
+  /* layer thickness, interface pressure, Montgomery potential, kinetic energy
+   * and specific volume on the edges (300-418) */
+  for (int i = 0; i < nip; i++) {
+    for (int e = 0; e < NPP; e++) {
+      for (int k = 0; k < kdm; k++)
+        E(dp_edg, i, e, k) = E(mont_edg, i, e, k) = E(uvsq_edg, i, e, k) = E(spcv_edg, i, e, k) = 0.;
+      for (int k = 0; k < kp; k++) PR(i, e, k) = 0.;
+    }
+    if (!WET(i)) continue;
+    const int np = g->nprox[i];
+    for (int edg = 0; edg < np; edg++) {
+      int ipx = g->prox[i * NPP + edg];
+      if (WET(ipx)) PR(i, edg, kdm) = dmin2(P(i, kdm), P(ipx, kdm));
+      else ipx = i;                               /* avoid stepping on land */
+      int im1 = g->prox[i * NPP + (edg - 1 + np) % np];
+      if (g->wet[im1] == 0) im1 = i;
+      int ip1 = g->prox[i * NPP + (edg + 1) % np];
+      if (g->wet[ip1] == 0) ip1 = i;
+      for (int k = kdm - 1; k >= 1; k--)
+        PR(i, edg, k) = PR(i, edg, k + 1) - .5 * dmin2(dmin2(TENM, C(dpm, i, k)), C(dpm, ipx, k));
+      for (int k = 0; k < kdm; k++) {
+        if (k < kdm - 1)
+          PR(i, edg, k + 1) = dmin2(PR(i, edg, k + 1), PR(i, edg, k) + (8. * (C(dpm, i, k) + C(dpm, ipx, k)) +
+                                                                      C(dpm, im1, k) + C(dpm, ip1, k)) * DIVBY18);
+        E(dp_edg, i, edg, k) = dmax2(0., PR(i, edg, k + 1) - PR(i, edg, k));
+      }
+      ipx = g->prox[i * NPP + edg];
+      im1 = g->prox[i * NPP + (edg - 1 + np) % np];
+      ip1 = g->prox[i * NPP + (edg + 1) % np];
+      for (int k = 0; k < kdm; k++) {
+        double blok = WET(ipx) ? hyc_blokf(C(dpm, i, k), P(i, k + 1), P(ipx, kdm), P(ipx, k)) : 1.;
+        const double ngborm1 = C(montg, ipx, k) * (1. - blok) + C(montg, i, k) * blok;
+        const double ngborv1 = C(spcifv, ipx, k) * (1. - blok) + C(spcifv, i, k) * blok;
+        blok = WET(im1) ? hyc_blokf(C(dpm, i, k), P(i, k + 1), P(im1, kdm), P(im1, k)) : 1.;
+        const double ngborm2 = C(montg, im1, k) * (1. - blok) + C(montg, i, k) * blok;
+        const double ngborv2 = C(spcifv, im1, k) * (1. - blok) + C(spcifv, i, k) * blok;
+        blok = WET(ip1) ? hyc_blokf(C(dpm, i, k), P(i, k + 1), P(ip1, kdm), P(ip1, k)) : 1.;
+        const double ngborm3 = C(montg, ip1, k) * (1. - blok) + C(montg, i, k) * blok;
+        const double ngborv3 = C(spcifv, ip1, k) * (1. - blok) + C(spcifv, i, k) * blok;
+        E(mont_edg, i, edg, k) = (8. * (C(montg, i, k) + ngborm1) + ngborm2 + ngborm3) * DIVBY18;
+        E(uvsq_edg, i, edg, k) = .5 * (E(um_edg, i, edg, k) * E(um_edg, i, edg, k) +
+                                       E(vm_edg, i, edg, k) * E(vm_edg, i, edg, k));
+        E(spcv_edg, i, edg, k) = (8. * (C(spcifv, i, k) + ngborv1) + ngborv2 + ngborv3) * DIVBY18;
+      }
+    }
+  }
+  if (janjic) {
+    /* geopotential on the edges (437-462) */
+    for (int i = 0; i < nip; i++) {
+      if (!WET(i)) continue;
+      const int np = g->nprox[i];
+      for (int edg = 0; edg < np; edg++) {
+        const int em1 = (edg - 1 + np) % np, ep1 = (edg + 1) % np;
+        for (int k = 0; k < kdm; k++)
+          E(geop_edg, i, edg, k) =
+              (8. * (C(geop_lyr, i, k) + E(geopx, i, edg, k)) + E(geopx, i, em1, k) + E(geopx, i, ep1, k)) * DIVBY18;
+      }
+    }
+  }
+#undef GEO
+#undef PR
+  free(utotm); free(utotn); free(vtotm); free(vtotn); free(geop_lyr); free(geopx); free(p_lyr);
 }

@@ -1,5 +1,6 @@
 /* hycom_oracle.h - the CPU checker of the HYCOM ocean continuity equation
- * (test infrastructure only).  See hycom_oracle.c for the array layouts. */
+ * and edge interpolation (test infrastructure only).  See hycom_oracle.c for
+ * the array layouts. */
 #pragma once
 
 typedef struct {
@@ -20,3 +21,15 @@ typedef struct {
 int fo_hycom_cnuity(const fo_hyc_grid *g, fo_hyc_mass *ms, int nstep, int leapm, int leapn,
                     const double *um_edg, const double *vm_edg, const double *un_edg, const double *vn_edg,
                     const double *dpm_edg, double *dp, double *pres, double *massflx, double *mssfx_avg);
+
+/* hycom_edgvar.F90 bclin_edgvar(nstep, leapm, leapn, uclin, vclin, utrop,
+ * vtrop, dp, pres, geop, montg, spcifv, um_edg, vm_edg, un_edg, vn_edg,
+ * dp_edg, pr_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg) with uvedge and
+ * blokf; cs/sn (4,6,nip) as [(i*6+e)*4 + j].  Only the grid fields nip, kdm,
+ * prox, nprox and wet are read. */
+void fo_hycom_edgvar(const fo_hyc_grid *g, const double *cs, const double *sn, int janjic, int leapm, int leapn,
+                     const double *uclin, const double *vclin, const double *utrop, const double *vtrop,
+                     const double *dp, double *pres, const double *geop, const double *montg,
+                     const double *spcifv, double *um_edg, double *vm_edg, double *un_edg, double *vn_edg,
+                     double *dp_edg, double *pr_edg, double *geop_edg, double *mont_edg, double *uvsq_edg,
+                     double *spcv_edg);

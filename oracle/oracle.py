@@ -240,6 +240,7 @@ class HycomCnuity:
         g.dtleap, g.thkdff, g.biharm = case["dtleap"], thkdff, biharm
         self.g = g
         self.m = _HycMass(*mass)
+        self.case = case
         self.a = {n: np.ascontiguousarray(case[n], np.float64).copy()
                   for n in ("dp", "pres", "um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "massflx", "mssfx_avg")}
 
@@ -253,3 +254,25 @@ class HycomCnuity:
     @property
     def mass(self):
         return self.m.totmass, self.m.massglb0, self.m.masscor
+
+    EDG_IN = ("uclin", "vclin", "utrop", "vtrop", "geop", "montg", "spcifv")
+    EDG_OUT = ("pr_edg", "geop_edg", "mont_edg", "uvsq_edg", "spcv_edg")
+
+    def edgvar(self, grid, leapm: int, leapn: int, janjic: bool = False):
+        """bclin_edgvar (hycom_edgvar.F90:128) on this instance's arrays: the
+        case must hold ocean.hycom_edgvar_case's inputs; um/vm/un/vn_edg and
+        dp_edg (as dpm_edg, which cnuity then reads) land in self.a, pr_edg,
+        geop_edg, mont_edg, uvsq_edg and spcv_edg in self.e."""
+        a = self.a
+        if not hasattr(self, "e"):
+            self.e = {n: np.ascontiguousarray(self.case[n], np.float64).copy() for n in self.EDG_IN + self.EDG_OUT}
+            self._keep["cs"] = np.ascontiguousarray(grid.cs, np.float64)
+            self._keep["sn"] = np.ascontiguousarray(grid.sn, np.float64)
+        e = self.e
+        lib().fo_hycom_edgvar(ct.byref(self.g), self._keep["cs"].ctypes.data_as(_PD),
+                              self._keep["sn"].ctypes.data_as(_PD), int(janjic), leapm, leapn,
+                              *[e[n].ctypes.data_as(_PD) for n in ("uclin", "vclin", "utrop", "vtrop")],
+                              a["dp"].ctypes.data_as(_PD), a["pres"].ctypes.data_as(_PD),
+                              *[e[n].ctypes.data_as(_PD) for n in ("geop", "montg", "spcifv")],
+                              *[a[n].ctypes.data_as(_PD) for n in ("um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg")],
+                              *[e[n].ctypes.data_as(_PD) for n in self.EDG_OUT])

@@ -168,6 +168,41 @@ def run_restref_read(restart_bytes: bytes, dims, workdir=None) -> bytes:
 HYC_MAGIC = 1213809491
 
 
+HYCE_MAGIC = 1213809477
+HYCE_RECORDS = ("pres", "um_edg", "vm_edg", "un_edg", "vn_edg", "dp_edg", "pr_edg", "geop_edg", "mont_edg",
+                "uvsq_edg", "spcv_edg")
+
+
+def run_hycedgref(glvl_path, case: dict, *, nstep: int = 1, leapm: int = 2, janjic: bool = False,
+                  chain: bool = False, thkdff: float = 0.05, biharm: float = 0.75, mass=(0.0, 0.0, 0.0),
+                  workdir=None) -> dict:
+    """Run the reference bclin_edgvar (oracle/ref/hycedgref.F90) once on the
+    inputs of ocean.hycom_edgvar_case, and with chain the reference cnuity
+    after it; returns the records by name (each one array)."""
+    workdir = workdir or os.path.join(REF, "runs")
+    os.makedirs(workdir, exist_ok=True)
+    kdm = case["kdm"]
+    nip = case["wet"].shape[0]
+    fin = os.path.join(workdir, "hyce_state_%d.bin" % os.getpid())
+    fout = os.path.join(workdir, "hyce_out_%d.bin" % os.getpid())
+    with open(fin, "wb") as f:
+        f.write(struct.pack("<8i", HYCE_MAGIC, kdm, nip, nstep, leapm, int(janjic), int(chain), 0))
+        f.write(struct.pack("<6d", case["dtleap"], thkdff, biharm, *mass))
+        f.write(np.ascontiguousarray(case["wet"], dtype="<i4").tobytes())
+        for n in ("utrop", "vtrop", "uclin", "vclin", "dp", "pres", "geop", "montg", "spcifv", "massflx",
+                  "mssfx_avg"):
+            f.write(np.ascontiguousarray(case[n], dtype="<f8").tobytes())
+    r = subprocess.run([os.path.join(BIN, "hycedgref"), glvl_path, fin, fout], cwd=workdir, capture_output=True,
+                       text=True, preexec_fn=_big_stack)
+    if r.returncode != 0:
+        raise RuntimeError("hycedgref failed (%d): %s" % (r.returncode, (r.stdout + r.stderr)[-3000:]))
+    rec = read_records(fout)
+    rec["_stdout"] = r.stdout
+    os.remove(fin)
+    os.remove(fout)
+    return rec
+
+
 def run_hycomref(glvl_path, case: dict, *, nsteps: int, thkdff: float = 0.05, biharm: float = 0.75,
                  nstep0: int = 1, mass=(0.0, 0.0, 0.0), workdir=None) -> dict:
     """Run the reference HYCOM continuity equation (oracle/ref/hycomref.F90)

@@ -31,6 +31,12 @@ Fixtures (data only):
                         on the seeded inputs of ocean.hycom_cnuity_case at G2
                         and G3 x 32 layers, 3 leapfrog steps each, for the
                         smoothing branches of HYCOM_CASES
+  hycom_edgvar_ref.json SHA-256 of every output record of the reference HYCOM
+                        edge interpolation (hycom_edgvar.F90 bclin_edgvar,
+                        FP64, hycedgref) and of the continuity equation run
+                        on its edge values, step 1, on the seeded inputs of
+                        ocean.hycom_edgvar_case at G2 and G3 x 32 layers,
+                        with and without janjic_ocn
   dyn_G2_ref.json       SHA-256 of every output record of the reference FP64
                         build (dynref_r8) for 3 full steps (mode 0) and 3
                         transport steps (mode 1), stage dumps of step 1, the
@@ -205,9 +211,28 @@ def hycom_reference():
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def hycom_edgvar_reference():
+    """Digests of the reference bclin_edgvar followed by cnuity at step 1
+    (hycedgref with chain) at G2 and G3 x 32 layers, with and without
+    janjic_ocn, leapm = 2."""
+    from fimocn_subx_realtime_hera_apr2020_amd import ocean
+    out = {"kdm": 32, "nstep": 1, "leapm": 2, "runs": {}}
+    for lv in (2, 3):
+        path = os.path.join(HERE, "glvl_G%d.dat" % lv)
+        g = glvl.read_glvl(path)
+        case = ocean.hycom_edgvar_case(g, kdm=32)
+        for janjic in (False, True):
+            r = refrun.run_hycedgref(path, case, nstep=1, leapm=2, janjic=janjic, chain=True)
+            out["runs"]["G%d_janjic%d" % (lv, int(janjic))] = {k: sha(v) for k, v in r.items()
+                                                               if not k.startswith("_")}
+    with open(os.path.join(HERE, "hycom_edgvar_ref.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     main()
     schemes()
     fp64_bar()
     r4_reference()
     hycom_reference()
+    hycom_edgvar_reference()

@@ -92,3 +92,80 @@ def test_hycom_cnuity_conserves_mass_and_keeps_land(oracle_lib):
                 ix, ex = prox[i, e], proxs[i, e]
                 if wet[ix]:
                     assert np.array_equal(fx[i, e], -fx[ix, ex])
+
+
+EDG_RECORDS = ("pres", "um_edg", "vm_edg", "un_edg", "vn_edg", "dp_edg", "pr_edg", "geop_edg", "mont_edg",
+               "uvsq_edg", "spcv_edg")
+
+
+def _edgvar_chain(O, g, case, janjic):
+    """bclin_edgvar then cnuity at step 1 (leapm 2), as hycom_run calls them;
+    the records by the reference driver's names."""
+    h = O.HycomCnuity(g, case)
+    h.edgvar(g, 2, 1, janjic)
+    out = {"pres": h.a["pres"].copy(), "dp_edg": h.a["dpm_edg"].copy()}
+    for n in ("um_edg", "vm_edg", "un_edg", "vn_edg"):
+        out[n] = h.a[n].copy()
+    for n in h.EDG_OUT:
+        out[n] = h.e[n].copy()
+    assert h.step(1, 2, 1) == 0
+    out.update(dp=h.a["dp"], pres2=h.a["pres"], massflx=h.a["massflx"], mssfx_avg=h.a["mssfx_avg"],
+               mass=np.array(h.mass))
+    return h, out
+
+
+@pytest.mark.parametrize("level", [2, 3])
+@pytest.mark.parametrize("janjic", [False, True])
+def test_hycom_edgvar_checker_equals_reference_digests(oracle_lib, level, janjic):
+    with open(os.path.join(GOLDEN, "hycom_edgvar_ref.json")) as f:
+        ref = json.load(f)
+    g = glvl.read_glvl(os.path.join(GOLDEN, "glvl_G%d.dat" % level))
+    case = ocean.hycom_edgvar_case(g, kdm=ref["kdm"])
+    _, out = _edgvar_chain(oracle_lib, g, case, janjic)
+    want = ref["runs"]["G%d_janjic%d" % (level, int(janjic))]
+    assert set(out) == set(want)
+    for n, v in out.items():
+        assert _sha(v) == want[n], n
+
+
+def test_hycom_edgvar_checker_equals_live_reference_g4(oracle_lib):
+    from oracle import refrun
+    if not os.path.exists(os.path.join(refrun.BIN, "hycedgref")):
+        pytest.skip("reference build (oracle/_ref) not present")
+    path = refrun.ref_glvl(4)
+    g = glvl.read_glvl(path)
+    case = ocean.hycom_edgvar_case(g, kdm=32, seed=9)
+    for janjic in (False, True):
+        r = refrun.run_hycedgref(path, case, nstep=1, leapm=2, janjic=janjic, chain=True)
+        _, out = _edgvar_chain(oracle_lib, g, case, janjic)
+        for n, v in out.items():
+            assert np.array_equal(np.ravel(v).view(np.uint64), np.ravel(r[n]).view(np.uint64)), (janjic, n)
+
+
+def test_hycom_edgvar_properties(oracle_lib):
+    """What the interpolation fixes: a horizontally uniform Montgomery
+    potential and specific volume stay uniform on the edges (the weights
+    8+8+1+1 over 18 and the sidewall blend sum to one), edge thicknesses and
+    kinetic energies are non-negative, land cells and a pentagon's sixth
+    slot get zero edge values, and without janjic_ocn geop_edg is not
+    touched."""
+    g = glvl.read_glvl(os.path.join(GOLDEN, "glvl_G3.dat"))
+    case = ocean.hycom_edgvar_case(g, kdm=32)
+    case["montg"] = np.tile(np.linspace(-3.0, 5.0, 32), (g.nip, 1))
+    case["spcifv"] = np.tile(np.linspace(9.7e-4, 9.8e-4, 32), (g.nip, 1))
+    h = oracle_lib.HycomCnuity(g, case)
+    h.edgvar(g, 2, 1, False)
+    wet = case["wet"] > 0
+    m, sv = h.e["mont_edg"], h.e["spcv_edg"]
+    for i in np.flatnonzero(wet):
+        np_ = g.nprox[i]
+        assert np.allclose(m[i, :np_], case["montg"][i], rtol=0, atol=4e-15), i
+        assert np.allclose(sv[i, :np_], case["spcifv"][i], rtol=3e-16, atol=0), i
+    assert (h.a["dpm_edg"] >= 0).all() and (h.e["uvsq_edg"] >= 0).all()
+    for n in ("mont_edg", "uvsq_edg", "spcv_edg"):
+        assert (h.e[n][~wet] == 0).all(), n
+    assert (h.a["dpm_edg"][~wet] == 0).all() and (h.e["pr_edg"][~wet] == 0).all()
+    pent = np.flatnonzero((g.nprox == 5) & wet)
+    assert len(pent) > 0
+    assert (h.a["um_edg"][pent, 5] == 0).all() and (h.a["dpm_edg"][pent, 5] == 0).all()
+    assert (h.e["geop_edg"] == 7.0).all()
