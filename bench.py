@@ -32,10 +32,11 @@ Prints ONE JSON line.  Besides the contract fields it carries
                 batched form, 4 class-B tracers per member)
   uneven_batch: G6 and G7 concatenated into one instance (two resolutions in
                 one launch), mixed form and FP64
-  hycom_cnuity: the HYCOM ocean's continuity equation (FP64, bitwise the
-                checker's) on G8 x 32 layers: throughput, per-kernel times,
-                its dominant kernel against its algorithmic bytes, and the
-                checker timed on one host core
+  hycom:        the HYCOM ocean (FP64, bitwise the checker's) on G8 x 32
+                layers: its continuity equation alone and the edge
+                interpolation followed by the continuity equation, each with
+                throughput, per-kernel times, its dominant kernel against its
+                algorithmic bytes, and the checker timed on one host core
   accuracy_96:  relative L2 of member 0 against FP64 after 96 steps, mass drift
   cpu_baseline: the C restatement (oracle/fim_oracle.c, FP64, OpenMP) on a
                 bounded sample of the same grid
@@ -561,73 +562,116 @@ HYC_KERNEL_BYTES = {"hyc_flux": 352, "hyc_lowsum": 144, "hyc_limits": 48, "hyc_c
 # vn, dpm_edg and mssfx_avg in, massflx and mssfx_avg out (8 edge arrays), dp
 # at both leapfrog levels and pres in and out
 HYC_STEP_BYTES = 8 * 48 + 2 * 24
+# the same for the edge interpolation's kernels (neighbour values counted once)
+HYC_EDG_KERNEL_BYTES = {
+    "hyc_uvedge": 8 * 2 + 2 * 48 + 8,          # one launch per level: uclin vclin -> u, v edges; dp -> pres (first, half counted)
+    "hyc_edgint": 8 + 8 + 16 + 96 + 50 + 4 * 48,   # dp pres montg spcifv um vm -> pr_edg dp_edg mont uvsq spcv
+    "hyc_janjic": 8 + 8 + 8 + 48,              # pres geop spcifv -> geop_edg
+}
+# bclin_edgvar then cnuity: in uclin, vclin and dp (2 levels), pres, montg,
+# spcifv, mssfx_avg; out the edge values the ocean step goes on to read (um,
+# vm, un, vn, dp_edg, mont_edg, uvsq_edg, spcv_edg on 6 edges, pr_edg on 6
+# edges x kdm+1 interfaces), massflx, mssfx_avg, dp (2 levels) and pres
+HYC_CHAIN_BYTES = (32 + 16 + 8 + 16 + 48) + (8 * 48 + 50) + (2 * 48 + 16 + 8)
 
 
-def run_hycom(a, fimhip, cpu=True):
-    """The HYCOM ocean's continuity equation (hycom_cnuity.F90 cnuity, FP64)
-    on G8 (655362 cells) x 32 layers, ~30 % land, biharmonic interface
-    smoothing, dtleap 1800 s, on device-resident arrays; the checker's C
-    restatement timed on one host core over 2 steps of the same input."""
+def _ocean_leg(a, o, wcl, step, byte_model):
+    """Time 'step(nstep)' over a.steps steps after a.warmup, then the same
+    number under per-kernel HIP-event timing; throughput, kernel table and
+    the dominant kernel against its algorithmic bytes."""
     import torch
-    from fimocn_subx_realtime_hera_apr2020_amd import ocean
-    g = fimhip.icos_grid(8)
-    kdm, thkdff, biharm = 32, 0.05, 0.75
-    case = ocean.hycom_cnuity_case(g, kdm=kdm)
-    nwet = int((case["wet"] > 0).sum())
-    wcl = nwet * kdm
-    o = fimhip.OceanCnuity(g, kdm, case["wet"], case["dtleap"], thkdff=thkdff, biharm=biharm)
-    try:
-        for n in fimhip.OCN_FIELDS:
-            o.upload(n, case[n])
-        nstep = 1
-        for _ in range(a.warmup):
-            o.step(nstep, 1 + nstep % 2, 2 - nstep % 2)
-            nstep += 1
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(a.steps):
-            o.step(nstep, 1 + nstep % 2, 2 - nstep % 2)
-            nstep += 1
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        o.profile(True)
-        for _ in range(a.steps):
-            o.step(nstep, 1 + nstep % 2, 2 - nstep % 2)
-            nstep += 1
-        prof = o.profile_report()
-        o.profile(False)
-    finally:
-        o.close()
+    nstep = 1
+    for _ in range(a.warmup):
+        step(nstep)
+        nstep += 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step(nstep)
+        nstep += 1
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    o.profile(True)
+    for _ in range(a.steps):
+        step(nstep)
+        nstep += 1
+    prof = o.profile_report()
+    o.profile(False)
     kern = {}
     for n, (cnt, tot) in prof.items():
         ms = tot / cnt
-        b = HYC_KERNEL_BYTES.get(n, 0) * wcl
-        kern[n] = {"ms": ms, "bytes": b, "GB_s": b / (ms * 1e-3) / 1e9}
-    dom = max(kern, key=lambda n: kern[n]["ms"])
+        b = byte_model.get(n, 0) * wcl
+        kern[n] = {"ms": ms, "launches_per_step": cnt / a.steps, "bytes": b, "GB_s": b / (ms * 1e-3) / 1e9}
+    dom = max(kern, key=lambda n: kern[n]["ms"] * kern[n]["launches_per_step"])
     dk = kern[dom]
-    ms_step = wall / a.steps * 1e3
-    out = {"workload": "G8 (%d cells, %d ocean) x %d layers, thkdff %g, biharm %g, dtleap %g s, FP64, resident"
-                       % (g.nip, nwet, kdm, thkdff, biharm, case["dtleap"]),
-           "value": wcl * a.steps / wall, "unit": "ocean cell-level-steps/s", "ms_per_step": ms_step,
-           "kernel_ms_per_step": sum(v["ms"] for v in kern.values()),
-           "roofline": {"bound": "hbm", "achieved": dk["GB_s"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                        "frac": dk["GB_s"] * 1e9 / HBM_PEAK, "kernel": dom,
-                        "bytes_per_cell_level": HYC_KERNEL_BYTES.get(dom, 0)},
-           "step_roofline": {"compulsory_bytes_per_cell_level": HYC_STEP_BYTES,
-                             "achieved": HYC_STEP_BYTES * wcl / (ms_step * 1e-3) / 1e9, "unit": "GB/s",
-                             "frac": HYC_STEP_BYTES * wcl / (ms_step * 1e-3) / HBM_PEAK},
-           "kernels": kern}
+    return {"value": wcl * a.steps / wall, "unit": "ocean cell-level-steps/s", "ms_per_step": wall / a.steps * 1e3,
+            "kernel_ms_per_step": sum(v["ms"] * v["launches_per_step"] for v in kern.values()),
+            "roofline": {"bound": "hbm", "achieved": dk["GB_s"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": dk["GB_s"] * 1e9 / HBM_PEAK, "kernel": dom,
+                         "bytes_per_cell_level": byte_model.get(dom, 0)},
+            "kernels": kern}
+
+
+def run_hycom(a, fimhip, cpu=True):
+    """The HYCOM ocean on G8 (655362 cells) x 32 layers, ~30 % land,
+    biharmonic interface smoothing, dtleap 1800 s, FP64 on device-resident
+    arrays: its continuity equation alone (hycom_cnuity.F90 cnuity), and the
+    edge interpolation followed by the continuity equation, as hycom_run
+    calls them (hycom_edgvar.F90 bclin_edgvar, then cnuity on its edge
+    values); the checker's C restatement of each timed on one host core over
+    2 steps of the same input."""
+    from fimocn_subx_realtime_hera_apr2020_amd import ocean
+    g = fimhip.icos_grid(8)
+    kdm, thkdff, biharm = 32, 0.05, 0.75
+    case = ocean.hycom_edgvar_case(g, kdm=kdm)
+    # the continuity equation alone reads hycom_cnuity_case's edge values (the
+    # same grid, mask and thickness; each edge's two sides carry one velocity)
+    cnu = ocean.hycom_cnuity_case(g, kdm=kdm, seed=ocean.EDGVAR_CASE_SEED)
+    nwet = int((case["wet"] > 0).sum())
+    wcl = nwet * kdm
+    workload = ("G8 (%d cells, %d ocean) x %d layers, thkdff %g, biharm %g, dtleap %g s, FP64, resident"
+                % (g.nip, nwet, kdm, thkdff, biharm, case["dtleap"]))
+    bytes_all = dict(HYC_KERNEL_BYTES, **HYC_EDG_KERNEL_BYTES)
+    out = {"workload": workload}
+    o = fimhip.Ocean(g, kdm, case["wet"], case["dtleap"], thkdff=thkdff, biharm=biharm)
+    try:
+        for n in fimhip.OCN_CNUITY_FIELDS:
+            o.upload(n, cnu[n])
+        leg = _ocean_leg(a, o, wcl, lambda st: o.step(st, 1 + st % 2, 2 - st % 2), bytes_all)
+        ms = leg["ms_per_step"]
+        leg["step_roofline"] = {"compulsory_bytes_per_cell_level": HYC_STEP_BYTES,
+                                "achieved": HYC_STEP_BYTES * wcl / (ms * 1e-3) / 1e9, "unit": "GB/s",
+                                "frac": HYC_STEP_BYTES * wcl / (ms * 1e-3) / HBM_PEAK}
+        out["cnuity"] = leg
+
+        def chain(st):
+            o.step_edgvar(1 + st % 2, 2 - st % 2)
+            o.step(st, 1 + st % 2, 2 - st % 2)
+        for n in fimhip.OCN_FIELDS:
+            o.upload(n, case[n])
+        leg = _ocean_leg(a, o, wcl, chain, bytes_all)
+        ms = leg["ms_per_step"]
+        leg["step_roofline"] = {"compulsory_bytes_per_cell_level": HYC_CHAIN_BYTES,
+                                "achieved": HYC_CHAIN_BYTES * wcl / (ms * 1e-3) / 1e9, "unit": "GB/s",
+                                "frac": HYC_CHAIN_BYTES * wcl / (ms * 1e-3) / HBM_PEAK}
+        out["edgvar_cnuity"] = leg
+    finally:
+        o.close()
     if cpu:
         from oracle import oracle as O
-        h = O.HycomCnuity(g, case, thkdff=thkdff, biharm=biharm)
-        t0 = time.perf_counter()
-        for st in (1, 2):
-            h.step(st, 1 + st % 2, 2 - st % 2)
-        tc = time.perf_counter() - t0
-        out["cpu_checker"] = {"value": wcl * 2 / tc, "unit": "ocean cell-level-steps/s", "cores": 1,
-                              "kind": "port", "sample": "2 steps of the same G8 x 32 input",
-                              "ms_per_step": tc / 2 * 1e3}
-        out["speedup_vs_cpu_checker"] = out["value"] / out["cpu_checker"]["value"]
+        for name, with_edgvar in (("cnuity", False), ("edgvar_cnuity", True)):
+            h = O.HycomCnuity(g, case if with_edgvar else cnu, thkdff=thkdff, biharm=biharm)
+            t0 = time.perf_counter()
+            for st in (1, 2):
+                if with_edgvar:
+                    h.edgvar(g, 1 + st % 2, 2 - st % 2)
+                h.step(st, 1 + st % 2, 2 - st % 2)
+            tc = time.perf_counter() - t0
+            leg = out[name]
+            leg["cpu_checker"] = {"value": wcl * 2 / tc, "unit": "ocean cell-level-steps/s", "cores": 1,
+                                  "kind": "port", "sample": "2 steps of the same G8 x 32 input",
+                                  "ms_per_step": tc / 2 * 1e3}
+            leg["speedup_vs_cpu_checker"] = leg["value"] / leg["cpu_checker"]["value"]
     return out
 
 
@@ -786,7 +830,7 @@ def main():
         del x, y
     step_name = "full" if a.mode == 0 else "transport"
     if a.form == "hycom":
-        print(json.dumps({"hycom_cnuity": run_hycom(a, fimhip, cpu=not a.no_cpu_baseline)}))
+        print(json.dumps({"hycom": run_hycom(a, fimhip, cpu=not a.no_cpu_baseline)}))
         return
     plain = ens = None
     m = e = None
@@ -804,7 +848,7 @@ def main():
     hyc = None
     if a.form == "both" and not a.headline_only:
         hyc = run_hycom(a, fimhip, cpu=not a.no_cpu_baseline)
-        print("ocean continuity leg done", file=sys.stderr, flush=True)
+        print("HYCOM leg done", file=sys.stderr, flush=True)
     acc = None
     if e is not None and m is not None and a.accuracy_steps > 0:
         acc = accuracy(a, e, m, grid, fimhip)
@@ -841,7 +885,7 @@ def main():
                               "a 240 s step overflows the FP64 state by step 6, so it is not timed"},
         "roofline": head["roofline"], "dominant_kernel": head["dominant_kernel"],
         "speedup_vs_plain_fp64": (ens["value"] / plain["value"]) if (ens and plain) else None,
-        "plain_fp64": plain, "ensemble": ens, "uneven_batch": uneven, "hycom_cnuity": hyc,
+        "plain_fp64": plain, "ensemble": ens, "uneven_batch": uneven, "hycom": hyc,
         "accuracy_96": acc, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -17,9 +17,11 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libfimhip.so")
 
 SOURCES = ["fimhip_api.hip", "kernels_plain.hip", "transp3d.hip", "hybgen.hip", "ens_api.hip", "kernels_ens.hip",
-           "transp3d_ens.hip", "fct_tiles.hip", "hycom_cnuity.hip"]
+           "transp3d_ens.hip", "fct_tiles.hip", "hycom_cnuity.hip",
+           "hycom_edgvar.hip"]
 HOST_SOURCES = ["grid.cpp", "lsq_svd.cpp"]      # host-only C++ (g++)
-HEADERS = ["fimhip_internal.h", "fim_pow.h", "fim_pow_tables.h", "synth.h", "host_common.h", "ens_internal.h"]
+HEADERS = ["fimhip_internal.h", "fim_pow.h", "fim_pow_tables.h", "synth.h", "host_common.h", "ens_internal.h",
+           "hycom_internal.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FIMHIP_ARCH", "gfx950")

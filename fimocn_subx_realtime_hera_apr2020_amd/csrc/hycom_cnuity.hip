@@ -40,99 +40,12 @@
 #include <string>
 #include <vector>
 
-#include "fimhip.h"
-#include "host_common.h"
+#include "hycom_internal.h"
 
 namespace fimhip {
+namespace hyc {
 namespace {
 
-constexpr double H_GRVITY = 9.80665;
-constexpr double H_ONECM = 0.01 * (1000. * 9.80665);
-constexpr double H_ASSLDP = .25;
-constexpr double H_LAND = -1.e33;
-constexpr double H_EPSIL = 1.e-11;
-
-// cinfo bits: 0-2 the number of edges, 3 the cell is ocean, 4+e edge e (< the
-// number of edges) leads to an ocean cell
-struct HycGrid {
-  int nip, kdm;
-  const int *prox, *proxs, *cinfo;
-  const double *area, *rarea, *sve, *sideln, *rsideln, *rprox_ln;
-};
-struct HycState {
-  const double *um, *vm, *un, *vn, *dpm_edg;
-  double *dp, *pres, *massflx, *mssfx_avg;
-  // work arrays
-  double *antiflx, *dp_lo, *dpold, *dpc, *flxplus, *flxmnus, *rplus, *rmnus, *del2p, *dptmp;
-  double *clippd, *rpbot, *aux;
-  int *flag;
-};
-
-__device__ __forceinline__ double hmin(double a, double b) { return a < b ? a : b; }
-__device__ __forceinline__ double hmax(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ int ci_np(int ci) { return ci & 7; }
-__device__ __forceinline__ bool ci_wet(int ci) { return (ci >> 3) & 1; }
-__device__ __forceinline__ bool ci_open(int ci, int e) { return (ci >> (4 + e)) & 1; }
-
-// the column of the lane (i, its slot cc in the block) and the lane's layer k;
-// the host pads the grid to a multiple of 8 blocks, and block b, which the
-// hardware sends to XCD b mod 8, takes the (b mod 8)-th eighth of the cells
-template <int LPC>
-struct Col {
-  int i, k, cc;
-  bool ok;
-};
-template <int LPC>
-__device__ __forceinline__ Col<LPC> hcol(int ncol) {
-  constexpr int CPB = 256 / LPC;
-  const int per = gridDim.x >> 3;
-  const int bx = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  Col<LPC> c;
-  c.cc = threadIdx.x / LPC;
-  c.i = bx * CPB + c.cc;
-  c.k = threadIdx.x % LPC;
-  c.ok = c.i < ncol;
-  return c;
-}
-
-// orders the wavefront's LDS accesses around an exchange between its lanes
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// pres(k+1) = pres(k) + d(k), k = 0..kdm-1, from pres(0) = p0: every lane forms
-// the whole chain in layer order and keeps the interface below its layer
-__device__ __forceinline__ double col_pres(double *buf, double d, double p0, int k, int kdm) {
-  buf[k] = d;
-  wave_sync();
-  double prev = p0, mine = 0.;
-  for (int j = 0; j < kdm; j++) {
-    prev = prev + buf[j];
-    if (j == k) mine = prev;
-  }
-  wave_sync();
-  return mine;
-}
-// the six sums over the column's layers, in layer order from 0, of part[e]:
-// lane e < NPP of the column returns edge e's
-template <int LPC>
-__device__ __forceinline__ double col_sums6(double (*buf)[LPC + 1], const double *part, int k, int kdm) {
-#pragma unroll
-  for (int e = 0; e < NPP; e++) buf[e][k] = part[e];
-  wave_sync();
-  double acc = 0.;
-  if (k < NPP)
-    for (int j = 0; j < kdm; j++) acc = acc + buf[k][j];
-  wave_sync();
-  return acc;
-}
-
-#define HC(a, i, k) (a)[(size_t)(i) * kdm + (k)]
-#define HP(i, k) s.pres[(size_t)(i) * (kdm + 1) + (k)]
-#define HE(a, i, e, k) (a)[((size_t)(i) * NPP + (e)) * kdm + (k)]
-#define HSVE(i, e, c) g.sve[((size_t)(i) * NPP + (e)) * 2 + (c)]
 
 // the high- and low-order fluxes of one edge seen from the cell whose values
 // are 'a' (its edge ea) towards the cell 'b' (its edge eb)  (98-159)
@@ -584,29 +497,18 @@ __global__ void __launch_bounds__(256) hyc_check(HycGrid g, HycState s) {
 }
 
 }  // namespace
+}  // namespace hyc
 }  // namespace fimhip
 
 using namespace fimhip;
-
-struct fimhip_ocn {
-  int device = 0, nip = 0, kdm = 0;
-  double dtleap = 0., thkdff = 0., biharm = 0.;
-  double totmass = 0., massglb0 = 0., masscor = 0.;
-  hipStream_t stream = nullptr;
-  Arena arena;
-  HycGrid g{};
-  HycState s{};
-  std::vector<int> wet;
-  std::vector<double> area;
-  KTimer *kt = nullptr;              // non-null while profiling
-};
+using namespace fimhip::hyc;
 
 static int ocn_launch(fimhip_ocn *o, int nstep, int leapm, int leapn);
 
 extern "C" int fimhip_ocn_create(const fimhip_grid_desc *gd, const double *rprox_ln, int kdm, const int *wet,
                                  double dtleap, double thkdff, double biharm, int device, fimhip_ocn **out) {
   if (!gd || !rprox_ln || !wet || !out || kdm < 2 || kdm > 64 || gd->nip < 12 || !gd->nprox || !gd->prox ||
-      !gd->proxs || !gd->sidevec_e || !gd->sideln || !gd->area || !(dtleap > 0.))
+      !gd->proxs || !gd->sidevec_e || !gd->sideln || !gd->area || !gd->cs || !gd->sn || !(dtleap > 0.))
     return FIMHIP_EINVAL;
   *out = nullptr;
   int ndev = 0;
@@ -631,6 +533,8 @@ extern "C" int fimhip_ocn_create(const fimhip_grid_desc *gd, const double *rprox
     const int np = gd->nprox[i];
     if (np < 5 || np > 6) { fimhip_ocn_destroy(o); return FIMHIP_EINVAL; }
     nprox[i] = np;
+    // the reference tests wet > 0 in some places and wet == 0 in others
+    if (wet[i] < 0) { fimhip_ocn_destroy(o); return FIMHIP_EINVAL; }
     o->wet[i] = wet[i] > 0 ? 1 : 0;
     rarea[i] = 1. / gd->area[i];
     for (int e = 0; e < NPP; e++) {
@@ -665,16 +569,18 @@ extern "C" int fimhip_ocn_create(const fimhip_grid_desc *gd, const double *rprox
   g.nip = nip;
   g.kdm = kdm;
   int *pp, *ps, *pc;
-  double *pa, *pr, *pv, *psl, *prs, *prl;
+  double *pa, *pr, *pv, *psl, *prs, *prl, *pcs, *psn;
+  const std::vector<double> csv(gd->cs, gd->cs + (size_t)nip * NPP * 4), snv(gd->sn, gd->sn + (size_t)nip * NPP * 4);
   if ((r = o->arena.upload(&pp, prox)) || (r = o->arena.upload(&ps, proxs)) || (r = o->arena.upload(&pc, cinfo)) ||
       (r = o->arena.upload(&pa, o->area)) || (r = o->arena.upload(&pr, rarea)) || (r = o->arena.upload(&pv, sve)) ||
       (r = o->arena.upload(&psl, sideln)) || (r = o->arena.upload(&prs, rsideln)) ||
-      (r = o->arena.upload(&prl, rpl))) {
+      (r = o->arena.upload(&prl, rpl)) || (r = o->arena.upload(&pcs, csv)) || (r = o->arena.upload(&psn, snv))) {
     fimhip_ocn_destroy(o);
     return r;
   }
   g.prox = pp; g.proxs = ps; g.cinfo = pc;
   g.area = pa; g.rarea = pr; g.sve = pv; g.sideln = psl; g.rsideln = prs; g.rprox_ln = prl;
+  g.cs = pcs; g.sn = psn;
   const size_t cz = (size_t)nip * kdm, ez = cz * NPP;
   HycState &s = o->s;
   double *um, *vm, *un, *vn, *dpe;
@@ -682,7 +588,12 @@ extern "C" int fimhip_ocn_create(const fimhip_grid_desc *gd, const double *rprox
       {&um, ez}, {&vm, ez}, {&un, ez}, {&vn, ez}, {&dpe, ez}, {&s.dp, 2 * cz}, {&s.pres, (size_t)nip * (kdm + 1)},
       {&s.massflx, ez}, {&s.mssfx_avg, ez}, {&s.antiflx, ez}, {&s.dp_lo, cz}, {&s.dpold, cz}, {&s.flxplus, cz},
       {&s.flxmnus, cz}, {&s.rplus, cz}, {&s.rmnus, cz}, {&s.del2p, cz}, {&s.dptmp, cz}, {&s.dpc, cz},
-      {&s.clippd, (size_t)nip * NPP}, {&s.rpbot, (size_t)nip}, {&s.aux, (size_t)nip}};
+      {&s.clippd, (size_t)nip * NPP}, {&s.rpbot, (size_t)nip}, {&s.aux, (size_t)nip},
+      // the edge interpolation's inputs and outputs
+      {&o->ed.uclin, 2 * cz}, {&o->ed.vclin, 2 * cz}, {&o->ed.utrop, (size_t)nip * 3}, {&o->ed.vtrop, (size_t)nip * 3},
+      {&o->ed.geop, (size_t)nip * (kdm + 1)}, {&o->ed.montg, cz}, {&o->ed.spcifv, cz},
+      {&o->ed.pr_edg, (size_t)nip * NPP * (kdm + 1)}, {&o->ed.geop_edg, ez}, {&o->ed.mont_edg, ez},
+      {&o->ed.uvsq_edg, ez}, {&o->ed.spcv_edg, ez}};
   for (auto &a : arrs)
     if ((r = o->arena.alloc(a.p, a.n))) { fimhip_ocn_destroy(o); return r; }
   s.um = um; s.vm = vm; s.un = un; s.vn = vn; s.dpm_edg = dpe;
@@ -703,19 +614,31 @@ extern "C" int fimhip_ocn_destroy(fimhip_ocn *o) {
   return FIMHIP_OK;
 }
 
-static double *ocn_field(fimhip_ocn *o, int f, size_t *n) {
+double *ocn_field(fimhip_ocn *o, int f, size_t *n) {
   const size_t cz = (size_t)o->nip * o->kdm, ez = cz * NPP;
   HycState &s = o->s;
   switch (f) {
     case FIMHIP_OCN_DP: *n = 2 * cz; return s.dp;
     case FIMHIP_OCN_PRES: *n = (size_t)o->nip * (o->kdm + 1); return s.pres;
-    case FIMHIP_OCN_UM_EDG: *n = ez; return const_cast<double *>(s.um);
-    case FIMHIP_OCN_VM_EDG: *n = ez; return const_cast<double *>(s.vm);
-    case FIMHIP_OCN_UN_EDG: *n = ez; return const_cast<double *>(s.un);
-    case FIMHIP_OCN_VN_EDG: *n = ez; return const_cast<double *>(s.vn);
-    case FIMHIP_OCN_DPM_EDG: *n = ez; return const_cast<double *>(s.dpm_edg);
+    case FIMHIP_OCN_UM_EDG: *n = ez; return s.um;
+    case FIMHIP_OCN_VM_EDG: *n = ez; return s.vm;
+    case FIMHIP_OCN_UN_EDG: *n = ez; return s.un;
+    case FIMHIP_OCN_VN_EDG: *n = ez; return s.vn;
+    case FIMHIP_OCN_DPM_EDG: *n = ez; return s.dpm_edg;
     case FIMHIP_OCN_MASSFLX: *n = ez; return s.massflx;
     case FIMHIP_OCN_MSSFX_AVG: *n = ez; return s.mssfx_avg;
+    case FIMHIP_OCN_UCLIN: *n = 2 * cz; return o->ed.uclin;
+    case FIMHIP_OCN_VCLIN: *n = 2 * cz; return o->ed.vclin;
+    case FIMHIP_OCN_UTROP: *n = (size_t)o->nip * 3; return o->ed.utrop;
+    case FIMHIP_OCN_VTROP: *n = (size_t)o->nip * 3; return o->ed.vtrop;
+    case FIMHIP_OCN_GEOP: *n = (size_t)o->nip * (o->kdm + 1); return o->ed.geop;
+    case FIMHIP_OCN_MONTG: *n = cz; return o->ed.montg;
+    case FIMHIP_OCN_SPCIFV: *n = cz; return o->ed.spcifv;
+    case FIMHIP_OCN_PR_EDG: *n = (size_t)o->nip * NPP * (o->kdm + 1); return o->ed.pr_edg;
+    case FIMHIP_OCN_GEOP_EDG: *n = ez; return o->ed.geop_edg;
+    case FIMHIP_OCN_MONT_EDG: *n = ez; return o->ed.mont_edg;
+    case FIMHIP_OCN_UVSQ_EDG: *n = ez; return o->ed.uvsq_edg;
+    case FIMHIP_OCN_SPCV_EDG: *n = ez; return o->ed.spcv_edg;
   }
   return nullptr;
 }
@@ -754,9 +677,7 @@ static void ocn_kernels(fimhip_ocn *o, int leapm, int leapn) {
   const HycGrid &g = o->g;
   const HycState &s = o->s;
   hipStream_t st = o->stream;
-  constexpr int CPB = 256 / LPC;
-  const int nblk = (o->nip + CPB - 1) / CPB;
-  const dim3 grid((nblk + 7) / 8 * 8), blk(256);     // a multiple of 8 blocks: see hcol
+  const dim3 grid(ocn_blocks(o, LPC)), blk(256);     // a multiple of 8 blocks: see hcol
   const double dt = o->dtleap;
   const int smooth = o->thkdff != 0.;
   KT_LAUNCH("hyc_flux", st, hyc_flux<LPC><<<grid, blk, 0, st>>>(g, s, leapm, leapn, dt));

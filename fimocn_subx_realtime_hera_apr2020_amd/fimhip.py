@@ -49,7 +49,8 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ens_set_wet", "fimhip_ls_pseudo_inverse",
            "fimhip_ocn_create", "fimhip_ocn_destroy", "fimhip_ocn_cnuity_r8", "fimhip_ocn_cnuity",
            "fimhip_ocn_field_size", "fimhip_ocn_upload", "fimhip_ocn_download", "fimhip_ocn_mass",
-           "fimhip_ocn_set_mass", "fimhip_ocn_set_dtleap", "fimhip_ocn_profile", "fimhip_ocn_profile_report"]
+           "fimhip_ocn_set_mass", "fimhip_ocn_set_dtleap", "fimhip_ocn_profile", "fimhip_ocn_profile_report",
+           "fimhip_ocn_edgvar_r8", "fimhip_ocn_edgvar", "fimhip_ocn_set_janjic"]
 
 # real*4 drop-ins: name -> (number of float arrays, trailing int argument)
 R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, True), "mass_adv": (12, True),
@@ -173,6 +174,9 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ocn_mass.argtypes = [vp, _PD, _PD, _PD]
     lib.fimhip_ocn_set_mass.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double]
     lib.fimhip_ocn_set_dtleap.argtypes = [vp, ct.c_double]
+    lib.fimhip_ocn_edgvar_r8.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int] + [_PD] * 19
+    lib.fimhip_ocn_edgvar.argtypes = [vp, ct.c_int, ct.c_int]
+    lib.fimhip_ocn_set_janjic.argtypes = [vp, ct.c_int]
     lib.fimhip_ocn_profile.argtypes = [vp, ct.c_int]
     lib.fimhip_ocn_profile_report.argtypes = [vp, ct.c_char_p, ct.c_int]
     _LIB = lib
@@ -429,6 +433,26 @@ class Model(Instance):
                     massfx=(3,) + e, u_edg=e, v_edg=e, dp_edg=e, lp_edg=e, mont_edg=e, uvsq_edg=e,
                     trc_edg=(ntra,) + e, trb=(self.ntrb,) + c, cumufx=e, dpinit=c)[name]
 
+    def edgvar(self, nstep, leapm, leapn, uclin, vclin, utrop, vtrop, dp, pres, geop, montg, spcifv, um_edg,
+               vm_edg, un_edg, vn_edg, dp_edg, pr_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg):
+        arrs = (uclin, vclin, utrop, vtrop, dp, pres, geop, montg, spcifv, um_edg, vm_edg, un_edg, vn_edg, dp_edg,
+                pr_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg)
+        names = OCN_EDG_IN[:4] + ["dp", "pres"] + OCN_EDG_IN[4:] + ["um_edg", "vm_edg", "un_edg", "vn_edg",
+                                                                     "dpm_edg"] + OCN_EDG_OUT
+        for n, a in zip(names, arrs):
+            if a.shape != self._shape(n):
+                raise ValueError("%s: shape %s, expected %s" % (n, a.shape, self._shape(n)))
+        _chk(self.lib.fimhip_ocn_edgvar_r8(self.h, nstep, leapm, leapn, *map(_p, arrs)), "ocn_edgvar_r8")
+
+    def step_edgvar(self, leapm: int, leapn: int):
+        """bclin_edgvar on the device-resident arrays: its edge values are the
+        ones ``step`` (cnuity) then reads."""
+        _chk(self.lib.fimhip_ocn_edgvar(self.h, leapm, leapn), "ocn_edgvar")
+
+    def set_janjic(self, on: bool):
+        """janjic_ocn (fimnamelist.F90:246, default off)."""
+        _chk(self.lib.fimhip_ocn_set_janjic(self.h, int(on)), "ocn_set_janjic")
+
     def upload(self, name: str, a):
         a = np.ascontiguousarray(a, np.float64)
         if a.shape != self.shape(name):
@@ -624,19 +648,27 @@ class Ensemble:
         return out
 
 
-OCN_FIELDS = ["dp", "pres", "um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "massflx", "mssfx_avg"]
+OCN_FIELDS = ["dp", "pres", "um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "massflx", "mssfx_avg",
+              "uclin", "vclin", "utrop", "vtrop", "geop", "montg", "spcifv", "pr_edg", "geop_edg", "mont_edg",
+              "uvsq_edg", "spcv_edg"]
+OCN_CNUITY_FIELDS = OCN_FIELDS[:9]       # the arrays cnuity reads or writes
+# bclin_edgvar's inputs and its outputs besides um/vm/un/vn_edg and dp_edg (dpm_edg here)
+OCN_EDG_IN = ["uclin", "vclin", "utrop", "vtrop", "geop", "montg", "spcifv"]
+OCN_EDG_OUT = ["pr_edg", "geop_edg", "mont_edg", "uvsq_edg", "spcv_edg"]
 FIMHIP_EDUP = -4
 
 
-class OceanCnuity:
-    """The HYCOM ocean's continuity equation on one GPU (fimhip_ocn_*):
-    ``cnuity`` keeps the argument list of hycom_cnuity.F90:21 (nstep, leapm,
-    leapn, um_edg, vm_edg, un_edg, vn_edg, dpm_edg, dp, pres, massflx,
-    mssfx_avg) with numpy float64 arrays in the reference layout (C order
-    (2,nip,kdm), (nip,kdm+1), (nip,6,kdm)), dp, pres, massflx and mssfx_avg
-    updated in place.  ``mass`` is hycom_variables' (totmass, massglb0,
-    masscor).  As the reference stops on differing duplicate edge fluxes
-    (chkdup), so this raises FimhipError."""
+class Ocean:
+    """The HYCOM ocean's edge interpolation and continuity equation on one GPU
+    (fimhip_ocn_*).  ``cnuity`` keeps the argument list of hycom_cnuity.F90:21
+    (nstep, leapm, leapn, um_edg, vm_edg, un_edg, vn_edg, dpm_edg, dp, pres,
+    massflx, mssfx_avg) and ``edgvar`` that of bclin_edgvar
+    (hycom_edgvar.F90:128), with numpy float64 arrays in the reference layout
+    (C order: dp, uclin, vclin (2,nip,kdm); pres, geop (nip,kdm+1); utrop,
+    vtrop (nip,3); montg, spcifv (nip,kdm); edges (nip,6,kdm), pr_edg
+    (nip,6,kdm+1)), the written arrays updated in place.  ``mass`` is
+    hycom_variables' (totmass, massglb0, masscor).  As the reference stops on
+    differing duplicate edge fluxes (chkdup), so this raises FimhipError."""
 
     def __init__(self, grid: _glvl.Grid, kdm: int, wet, dtleap: float, thkdff: float = 0.05,
                  biharm: float = 0.75, device: int = 0):
@@ -664,7 +696,10 @@ class OceanCnuity:
             pass
 
     def _shape(self, name: str):
-        return {"dp": (2, self.nip, self.kdm), "pres": (self.nip, self.kdm + 1)}.get(name, (self.nip, 6, self.kdm))
+        n, k = self.nip, self.kdm
+        return {"dp": (2, n, k), "pres": (n, k + 1), "uclin": (2, n, k), "vclin": (2, n, k), "utrop": (n, 3),
+                "vtrop": (n, 3), "geop": (n, k + 1), "montg": (n, k), "spcifv": (n, k),
+                "pr_edg": (n, 6, k + 1)}.get(name, (n, 6, k))
 
     @staticmethod
     def _step_rc(rc: int, what: str):
@@ -678,6 +713,26 @@ class OceanCnuity:
             if a.shape != self._shape(n):
                 raise ValueError("%s: shape %s, expected %s" % (n, a.shape, self._shape(n)))
         self._step_rc(self.lib.fimhip_ocn_cnuity_r8(self.h, nstep, leapm, leapn, *map(_p, arrs)), "ocn_cnuity_r8")
+
+    def edgvar(self, nstep, leapm, leapn, uclin, vclin, utrop, vtrop, dp, pres, geop, montg, spcifv, um_edg,
+               vm_edg, un_edg, vn_edg, dp_edg, pr_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg):
+        arrs = (uclin, vclin, utrop, vtrop, dp, pres, geop, montg, spcifv, um_edg, vm_edg, un_edg, vn_edg, dp_edg,
+                pr_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg)
+        names = OCN_EDG_IN[:4] + ["dp", "pres"] + OCN_EDG_IN[4:] + ["um_edg", "vm_edg", "un_edg", "vn_edg",
+                                                                     "dpm_edg"] + OCN_EDG_OUT
+        for n, a in zip(names, arrs):
+            if a.shape != self._shape(n):
+                raise ValueError("%s: shape %s, expected %s" % (n, a.shape, self._shape(n)))
+        _chk(self.lib.fimhip_ocn_edgvar_r8(self.h, nstep, leapm, leapn, *map(_p, arrs)), "ocn_edgvar_r8")
+
+    def step_edgvar(self, leapm: int, leapn: int):
+        """bclin_edgvar on the device-resident arrays: its edge values are the
+        ones ``step`` (cnuity) then reads."""
+        _chk(self.lib.fimhip_ocn_edgvar(self.h, leapm, leapn), "ocn_edgvar")
+
+    def set_janjic(self, on: bool):
+        """janjic_ocn (fimnamelist.F90:246, default off)."""
+        _chk(self.lib.fimhip_ocn_set_janjic(self.h, int(on)), "ocn_set_janjic")
 
     def upload(self, name: str, a):
         a = np.ascontiguousarray(a, np.float64)
@@ -719,3 +774,6 @@ class OceanCnuity:
             n, c, t = line.split()
             out[n] = (int(c), float(t))
         return out
+
+
+OceanCnuity = Ocean        # the continuity equation was its first routine

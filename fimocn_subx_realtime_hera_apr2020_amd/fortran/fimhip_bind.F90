@@ -161,6 +161,17 @@ module fimhip_bind
       real(c_double), intent(in) :: um_edg(*), vm_edg(*), un_edg(*), vn_edg(*), dpm_edg(*)
       real(c_double) :: dp(*), pres(*), massflx(*), mssfx_avg(*)
     end function
+    integer(c_int) function fimhip_ocn_edgvar_r8(ocn, nstep, leapm, leapn, uclin, vclin, utrop, vtrop, dp, &
+        pres, geop, montg, spcifv, um_edg, vm_edg, un_edg, vn_edg, dp_edg, pr_edg, geop_edg, mont_edg, &
+        uvsq_edg, spcv_edg) bind(C, name="fimhip_ocn_edgvar_r8")
+      import; type(c_ptr), value :: ocn; integer(c_int), value :: nstep, leapm, leapn
+      real(c_double), intent(in) :: uclin(*), vclin(*), utrop(*), vtrop(*), dp(*), geop(*), montg(*), spcifv(*)
+      real(c_double) :: pres(*), um_edg(*), vm_edg(*), un_edg(*), vn_edg(*), dp_edg(*), pr_edg(*), &
+                        geop_edg(*), mont_edg(*), uvsq_edg(*), spcv_edg(*)
+    end function
+    integer(c_int) function fimhip_ocn_set_janjic(ocn, on) bind(C, name="fimhip_ocn_set_janjic")
+      import; type(c_ptr), value :: ocn; integer(c_int), value :: on
+    end function
     integer(c_int) function fimhip_ocn_set_dtleap(ocn, dtleap) bind(C, name="fimhip_ocn_set_dtleap")
       import; type(c_ptr), value :: ocn; real(c_double), value :: dtleap
     end function

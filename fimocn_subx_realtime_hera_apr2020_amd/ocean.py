@@ -178,7 +178,10 @@ def hycom_cnuity_case(grid, kdm: int = 32, seed: int = 20240607, dtleap: float =
                 dpm_edg=on_edges(dp[0]), massflx=z.copy(), mssfx_avg=z.copy())
 
 
-def hycom_edgvar_case(grid, kdm: int = 32, seed: int = 20240611, dtleap: float = 1800.0) -> dict:
+EDGVAR_CASE_SEED = 20240611
+
+
+def hycom_edgvar_case(grid, kdm: int = 32, seed: int = EDGVAR_CASE_SEED, dtleap: float = 1800.0) -> dict:
     """The inputs of bclin_edgvar (hycom_edgvar.F90:128) on a glvl.Grid, on top
     of hycom_cnuity_case's (whose dp, pres, wet, massflx and mssfx_avg it
     keeps): the baroclinic velocities uclin/vclin (2, nip, kdm) at the two

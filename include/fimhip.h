@@ -365,11 +365,28 @@ typedef struct fimhip_ocn fimhip_ocn;
 #define FIMHIP_OCN_DPM_EDG 6
 #define FIMHIP_OCN_MASSFLX 7
 #define FIMHIP_OCN_MSSFX_AVG 8
+/* the edge interpolation's inputs (hycom_edgvar.F90:128-160): uclin, vclin
+ * (kdm,nip,2), utrop, vtrop (3,nip), geop (kdm+1,nip), montg, spcifv
+ * (kdm,nip); and outputs beside um/vm/un/vn_edg and dp_edg (= DPM_EDG, the
+ * array cnuity reads): pr_edg (kdm+1,6,nip), geop_edg, mont_edg, uvsq_edg,
+ * spcv_edg (kdm,6,nip) */
+#define FIMHIP_OCN_UCLIN 9
+#define FIMHIP_OCN_VCLIN 10
+#define FIMHIP_OCN_UTROP 11
+#define FIMHIP_OCN_VTROP 12
+#define FIMHIP_OCN_GEOP 13
+#define FIMHIP_OCN_MONTG 14
+#define FIMHIP_OCN_SPCIFV 15
+#define FIMHIP_OCN_PR_EDG 16
+#define FIMHIP_OCN_GEOP_EDG 17
+#define FIMHIP_OCN_MONT_EDG 18
+#define FIMHIP_OCN_UVSQ_EDG 19
+#define FIMHIP_OCN_SPCV_EDG 20
 /* hycom_init's constants (hycom_constants.F90 wet, dtleap; fimnamelist kdm,
- * thkdff, biharm) on the grid of the descriptor (nprox, prox, proxs,
- * sidevec_e, sideln, area; nvl, ntra, dt, cs, sn, sidevec_c, hfcoef, lat,
- * lon, thetac and dpsig are not read) and rprox_ln (6,nip) of glvl.dat;
- * kdm <= 64 */
+ * thkdff, biharm) on the grid of the descriptor (nprox, prox, proxs, cs, sn,
+ * sidevec_e, sideln, area; nvl, ntra, dt, sidevec_c, hfcoef, lat, lon,
+ * thetac and dpsig are not read) and rprox_ln (6,nip) of glvl.dat;
+ * kdm <= 64, wet >= 0 */
 int fimhip_ocn_create(const fimhip_grid_desc *grid, const double *rprox_ln, int kdm, const int *wet,
                       double dtleap, double thkdff, double biharm, int device, fimhip_ocn **out);
 int fimhip_ocn_destroy(fimhip_ocn *ocn);
@@ -383,6 +400,23 @@ int fimhip_ocn_cnuity(fimhip_ocn *ocn, int nstep, int leapm, int leapn);
 long long fimhip_ocn_field_size(const fimhip_ocn *ocn, int field);
 int fimhip_ocn_upload(fimhip_ocn *ocn, int field, const double *host);
 int fimhip_ocn_download(fimhip_ocn *ocn, int field, double *host);
+/* hycom_edgvar.F90:128 bclin_edgvar(nstep, leapm, leapn, uclin, vclin, utrop,
+ * vtrop, dp, pres, geop, montg, spcifv, um_edg, vm_edg, un_edg, vn_edg,
+ * dp_edg, pr_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg) with host arrays:
+ * pres and the edge arrays are updated (geop_edg only with janjic_ocn, on
+ * the ocean's edges) */
+int fimhip_ocn_edgvar_r8(fimhip_ocn *ocn, int nstep, int leapm, int leapn, const double *uclin,
+                         const double *vclin, const double *utrop, const double *vtrop, const double *dp,
+                         double *pres, const double *geop, const double *montg, const double *spcifv,
+                         double *um_edg, double *vm_edg, double *un_edg, double *vn_edg, double *dp_edg,
+                         double *pr_edg, double *geop_edg, double *mont_edg, double *uvsq_edg, double *spcv_edg);
+/* the same on the device-resident arrays, asynchronous (fimhip_ocn_cnuity
+ * and the downloads wait for it); its edge values are those
+ * fimhip_ocn_cnuity reads (dp_edg is FIMHIP_OCN_DPM_EDG) */
+int fimhip_ocn_edgvar(fimhip_ocn *ocn, int leapm, int leapn);
+/* janjic_ocn (fimnamelist.F90:246, default off): the geopotential on the
+ * edges from hydrostatic integration in the neighbouring columns */
+int fimhip_ocn_set_janjic(fimhip_ocn *ocn, int on);
 /* hycom_constants' dtleap for the calls that follow (hycom_run.F90:136 sets
  * it every step: batrop at the first, 2*batrop after) */
 int fimhip_ocn_set_dtleap(fimhip_ocn *ocn, double dtleap);

@@ -1,11 +1,13 @@
-"""The HYCOM ocean's continuity equation on the GPU (fimhip_ocn_*, the FP64
-kernels of hycom_cnuity.hip) against the checker's restatement
-(oracle/hycom_oracle.c, pinned to the reference's hycom_cnuity.F90 bit for
-bit): every output array and the mass state must hold the checker's bytes,
-through the host-array drop-in and the device-resident path, for every
+"""The HYCOM ocean's edge interpolation and continuity equation on the GPU
+(fimhip_ocn_*, the FP64 kernels of hycom_edgvar.hip and hycom_cnuity.hip)
+against the checker's restatement (oracle/hycom_oracle.c, pinned to the
+reference's hycom_edgvar.F90 and hycom_cnuity.F90 bit for bit): every output
+array and the mass state must hold the checker's bytes, through the
+host-array drop-ins and the device-resident path, for every
 interface-smoothing branch (biharmonic, Laplacian, none), across the 10-day
-mass correction, with 32 layers (two columns per wavefront) and with 48 (one
-column per wavefront), at G3 to G7 (163842 cells)."""
+mass correction, with and without janjic_ocn, with 32 layers (two columns
+per wavefront) and with 48 (one column per wavefront), at G3 to G7 (163842
+cells)."""
 import numpy as np
 import pytest
 
@@ -34,9 +36,9 @@ def _check(O, g, case, thkdff, biharm, nstep0, mass, nsteps, resident):
     ocn = fimhip.OceanCnuity(g, case["kdm"], case["wet"], case["dtleap"], thkdff=thkdff, biharm=biharm)
     try:
         ocn.mass = mass
-        a = {n: np.ascontiguousarray(case[n], np.float64).copy() for n in fimhip.OCN_FIELDS}
+        a = {n: np.ascontiguousarray(case[n], np.float64).copy() for n in fimhip.OCN_CNUITY_FIELDS}
         if resident:
-            for n in fimhip.OCN_FIELDS:
+            for n in fimhip.OCN_CNUITY_FIELDS:
                 ocn.upload(n, a[n])
         for st in range(nstep0, nstep0 + nsteps):
             leapm = 1 + st % 2
@@ -85,7 +87,7 @@ def test_leapfrog_start_with_changing_dtleap(oracle_lib):
     chk = oracle_lib.HycomCnuity(g, case)
     ocn = fimhip.OceanCnuity(g, 32, case["wet"], 900.0)
     try:
-        a = {n: np.ascontiguousarray(case[n], np.float64).copy() for n in fimhip.OCN_FIELDS}
+        a = {n: np.ascontiguousarray(case[n], np.float64).copy() for n in fimhip.OCN_CNUITY_FIELDS}
         for st in (1, 2, 3):
             dtleap = 900.0 * min(st, 2)
             chk.g.dtleap = dtleap
@@ -117,7 +119,7 @@ def test_bad_arguments_are_rejected():
         fimhip.OceanCnuity(g, 65, case["wet"], 1800.0)
     ocn = fimhip.OceanCnuity(g, 16, case["wet"], 1800.0)
     try:
-        for n in fimhip.OCN_FIELDS:
+        for n in fimhip.OCN_CNUITY_FIELDS:
             ocn.upload(n, case[n])
         for bad in ((1, 1, 1), (1, 0, 2), (0, 1, 2), (1, 2, 3)):
             with pytest.raises(fimhip.FimhipError):
@@ -126,3 +128,92 @@ def test_bad_arguments_are_rejected():
             ocn.upload("dp", case["dp"][0])
     finally:
         ocn.close()
+
+
+EDG_CHECK = ("um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "pres")
+
+
+def _edg_same(h, got, tag):
+    for n in EDG_CHECK:
+        assert _same(got[n], h.a[n].reshape(got[n].shape)), (tag, n)
+    for n in h.EDG_OUT:
+        assert _same(got[n], h.e[n].reshape(got[n].shape)), (tag, n)
+
+
+@pytest.mark.parametrize("janjic", [False, True])
+def test_edgvar_dropin_matches_checker_bitwise_g3(oracle_lib, janjic):
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, ocean
+    print("case: G3 x 32, bclin_edgvar drop-in, janjic %s" % janjic, flush=True)
+    g = _grid(3)
+    case = ocean.hycom_edgvar_case(g, kdm=32)
+    h = oracle_lib.HycomCnuity(g, case)
+    o = fimhip.Ocean(g, 32, case["wet"], case["dtleap"])
+    try:
+        o.set_janjic(janjic)
+        a = {n: np.ascontiguousarray(case[n], np.float64).copy() for n in fimhip.OCN_FIELDS}
+        for st in (1, 2):
+            leapm = 1 + st % 2
+            h.edgvar(g, leapm, 3 - leapm, janjic)
+            o.edgvar(st, leapm, 3 - leapm, a["uclin"], a["vclin"], a["utrop"], a["vtrop"], a["dp"], a["pres"],
+                     a["geop"], a["montg"], a["spcifv"], a["um_edg"], a["vm_edg"], a["un_edg"], a["vn_edg"],
+                     a["dpm_edg"], a["pr_edg"], a["geop_edg"], a["mont_edg"], a["uvsq_edg"], a["spcv_edg"])
+            _edg_same(h, a, st)
+    finally:
+        o.close()
+
+
+@pytest.mark.parametrize("level,kdm,janjic", [(4, 48, True), (5, 32, False), (5, 32, True)])
+def test_edgvar_then_cnuity_resident_matches_checker(oracle_lib, level, kdm, janjic):
+    """bclin_edgvar then cnuity each step, as hycom_run calls them, on the
+    device-resident arrays: 3 leapfrog steps from step 1 (the mass correction
+    included)."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, ocean
+    print("case: G%d x %d, edgvar + cnuity resident, janjic %s" % (level, kdm, janjic), flush=True)
+    g = _grid(level)
+    case = ocean.hycom_edgvar_case(g, kdm=kdm, seed=31 + level)
+    h = oracle_lib.HycomCnuity(g, case)
+    o = fimhip.Ocean(g, kdm, case["wet"], case["dtleap"])
+    try:
+        o.set_janjic(janjic)
+        for n in fimhip.OCN_FIELDS:
+            o.upload(n, case[n])
+        for st in (1, 2, 3):
+            leapm = 1 + st % 2
+            h.edgvar(g, leapm, 3 - leapm, janjic)
+            assert h.step(st, leapm, 3 - leapm) == 0
+            o.step_edgvar(leapm, 3 - leapm)
+            o.step(st, leapm, 3 - leapm)
+            got = {n: o.download(n) for n in EDG_CHECK[:5] + tuple(h.EDG_OUT)}
+            got["pres"] = h.a["pres"]     # pres is checked after cnuity below
+            _edg_same(h, got, st)
+            for n in ARRAYS:
+                assert _same(o.download(n), h.a[n]), (st, n)
+            assert o.mass == h.mass, st
+    finally:
+        o.close()
+
+
+def test_edgvar_then_cnuity_g7_full_size(oracle_lib):
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, ocean
+    print("case: G7 x 32, edgvar + cnuity resident, 2 steps", flush=True)
+    g = _grid(7)
+    case = ocean.hycom_edgvar_case(g, kdm=32)
+    h = oracle_lib.HycomCnuity(g, case)
+    o = fimhip.Ocean(g, 32, case["wet"], case["dtleap"])
+    try:
+        for n in fimhip.OCN_FIELDS:
+            o.upload(n, case[n])
+        for st in (1, 2):
+            leapm = 1 + st % 2
+            h.edgvar(g, leapm, 3 - leapm, False)
+            assert h.step(st, leapm, 3 - leapm) == 0
+            o.step_edgvar(leapm, 3 - leapm)
+            o.step(st, leapm, 3 - leapm)
+        for n in ARRAYS:
+            assert _same(o.download(n), h.a[n]), n
+        for n in ("um_edg", "dpm_edg"):
+            assert _same(o.download(n), h.a[n]), n
+        for n in h.EDG_OUT:
+            assert _same(o.download(n), h.e[n]), n
+    finally:
+        o.close()
