@@ -221,3 +221,34 @@ def hycom_edgvar_case(grid, kdm: int = 32, seed: int = EDGVAR_CASE_SEED, dtleap:
         case[n] = np.full((nip, 6, kdm), 7.0)
     case["pr_edg"] = np.full((nip, 6, kdm + 1), 7.0)
     return case
+
+
+def hycom_momtum_case(grid, kdm: int = 32, seed: int = EDGVAR_CASE_SEED, dtleap: float = 1800.0) -> dict:
+    """The inputs of momtum (hycom_momtum.F90:16) on top of
+    hycom_edgvar_case's: wind stress taux/tauy (nip,) N(0, 0.1) Pa, the
+    bottom friction velocity ustarb (nip,) zero, the barotropic time step
+    batrop = dtleap / 2 (hycom_run.F90:136 at the first step), and the
+    namelist defaults veldff = 0.1 m/s and smagcf = 0.3
+    (fimnamelist.F90:253-254).  In a quarter of the columns the three
+    bottom layers are thinned to 0.2-5 m (their thickness moved to the layer
+    above, the bottom kept) so that the downward momentum mixing into thin
+    bottom layers (hycom_momtum.F90:327-335) has work; pres follows the first
+    level.  The edge values momtum reads are bclin_edgvar's outputs, to be
+    formed from this case first."""
+    case = hycom_edgvar_case(grid, kdm=kdm, seed=seed, dtleap=dtleap)
+    rng = np.random.default_rng(seed + 2)
+    nip = case["wet"].shape[0]
+    dp = case["dp"].copy()
+    thin = np.flatnonzero(rng.uniform(size=nip) < 0.25)
+    for lv in range(2):
+        new = rng.uniform(0.2, 5.0, (thin.size, 3)) * ONEM
+        moved = dp[lv][thin][:, kdm - 3:].sum(axis=1) - new.sum(axis=1)
+        dp[lv][thin, kdm - 3:] = new
+        dp[lv][thin, kdm - 4] += moved
+    pres = np.zeros_like(case["pres"])
+    for k in range(kdm):
+        pres[:, k + 1] = pres[:, k] + dp[0][:, k]
+    case.update(dp=np.ascontiguousarray(dp), pres=pres)
+    case.update(taux=0.1 * rng.normal(size=nip), tauy=0.1 * rng.normal(size=nip), ustarb=np.zeros(nip),
+                batrop=0.5 * dtleap, veldff=0.1, smagcf=0.3)
+    return case

@@ -1,7 +1,8 @@
 /* hycom_oracle.c - CPU restatement, in plain C and FP64, of the HYCOM ocean
  * continuity equation (FIMsrc/fim/horizontal/hycom_cnuity.F90, subroutine
- * cnuity) and of its edge interpolation (hycom_edgvar.F90, bclin_edgvar with
- * uvedge and blokf).  TEST INFRASTRUCTURE ONLY: the checker the HIP kernels
+ * cnuity), of its edge interpolation (hycom_edgvar.F90, bclin_edgvar with
+ * uvedge and blokf) and of its momentum equation (hycom_momtum.F90 momtum
+ * with hycom_dissip.F90's del4prep and dissip).  TEST INFRASTRUCTURE ONLY: the checker the HIP kernels
  * are compared with; the product never links it.
  *
  * Every loop follows the reference operation for operation, in the
@@ -545,4 +546,215 @@ void fo_hycom_edgvar(const fo_hyc_grid *g, const double *cs, const double *sn, i
 #undef GEO
 #undef PR
   free(utotm); free(utotn); free(vtotm); free(vtotn); free(geop_lyr); free(geopx); free(p_lyr);
+}
+
+/* ------------------------------------------------------------------------
+ * hycom_momtum.F90 with hycom_dissip.F90.  Layouts as above; uclin/vclin
+ * (kdm,nip,2) as [(s*nip + i)*kdm + k], sidevec_c like sidevec_e. */
+#define MGRVITY 9.80665                        /* module_constants grvity */
+#define ONEMM (0.001 * ONEM)                   /* hycom_constants.F90:39 */
+#define TENCM (0.1 * ONEM)                     /* hycom_constants.F90:37 */
+static const double DRCOEF = 3.e-3;           /* hycom_constants.F90:44 */
+static const double ASSLUV = .25;             /* hycom_constants.F90:48 */
+static const double THSHLD = 1.e-11;          /* hycom_dissip.F90:145 */
+static const double OMEGX2 = 1.4584e-4;       /* module_constants.F90:18 */
+
+void fo_hycom_momtum(const fo_hyc_grid *g, const fo_hyc_mom *m, int leapm, int leapn, const double *utrop,
+                     const double *vtrop, double *uclin, double *vclin, const double *um_edg, const double *vm_edg,
+                     const double *geop_edg, const double *mont_edg, const double *uvsq_edg,
+                     const double *spcv_edg, const double *taux, const double *tauy, const double *dp,
+                     double *pres, double *ustarb) {
+  const int nip = g->nip, kdm = g->kdm, kp = kdm + 1;
+  const size_t cz = (size_t)nip * kdm;
+  const double dtleap = g->dtleap;
+  const double thktop = 50. * ONEM, thkbot = 10. * ONEM, cbar = .5;       /* hycom_momtum.F90:87-91 */
+  const double *dpm = dp + (size_t)(leapm - 1) * cz, *dpn = dp + (size_t)(leapn - 1) * cz;
+  double *ucm = uclin + (size_t)(leapm - 1) * cz, *ucn = uclin + (size_t)(leapn - 1) * cz;
+  double *vcm = vclin + (size_t)(leapm - 1) * cz, *vcn = vclin + (size_t)(leapn - 1) * cz;
+  double *gradx = calloc(cz, sizeof(double)), *grady = calloc(cz, sizeof(double));
+  double *utotn = calloc(cz, sizeof(double)), *vtotn = calloc(cz, sizeof(double));
+  double *worku = malloc(cz * sizeof(double)), *workv = malloc(cz * sizeof(double));
+  double *d4u = malloc(cz * sizeof(double)), *d4v = malloc(cz * sizeof(double));
+  double *ufx = malloc(cz * NPP * sizeof(double)), *vfx = malloc(cz * NPP * sizeof(double));
+  double *dpres = malloc((size_t)nip * kp * sizeof(double)), *drag = calloc((size_t)nip, sizeof(double));
+  double *pgfx = malloc((size_t)kdm * sizeof(double)), *pgfy = malloc((size_t)kdm * sizeof(double));
+  double *avgu = malloc((size_t)kdm * sizeof(double)), *avgv = malloc((size_t)kdm * sizeof(double));
+  double *dudx = malloc((size_t)kdm * sizeof(double)), *dudy = malloc((size_t)kdm * sizeof(double));
+  double *dvdx = malloc((size_t)kdm * sizeof(double)), *dvdy = malloc((size_t)kdm * sizeof(double));
+  double *uold = malloc((size_t)kdm * sizeof(double)), *vold = malloc((size_t)kdm * sizeof(double));
+#define CSN(a, i, e, j) (a)[((size_t)(i) * NPP + (e)) * 4 + (j)]
+#define SVC(i, e, c) m->sidevec_c[((size_t)(i) * NPP + (e)) * 2 + (c)]
+#define DPR(i, k) dpres[(size_t)(i) * kp + (k)]
+  /* bottom drag, pressure force (hycom_momtum.F90:104-183) */
+  for (int i = 0; i < nip; i++) {
+    for (int k = 0; k < kdm; k++) C(worku, i, k) = C(workv, i, k) = LAND_SPVAL;
+    if (!WET(i)) continue;
+    for (int k = 0; k < kdm; k++) P(i, k + 1) = P(i, k) + C(dpn, i, k);
+    double ubot = 0., vbot = 0.;
+    for (int k = 0; k < kdm; k++) {
+      C(utotn, i, k) = C(ucn, i, k) + utrop[i * 3 + leapn - 1];
+      C(vtotn, i, k) = C(vcn, i, k) + vtrop[i * 3 + leapn - 1];
+      const double q = dmin2(thkbot, P(i, kdm));
+      ubot = ubot + C(utotn, i, k) * (dmin2(P(i, k + 1), P(i, kdm) - q) - dmin2(P(i, k), P(i, kdm) - q));
+      vbot = vbot + C(vtotn, i, k) * (dmin2(P(i, k + 1), P(i, kdm) - q) - dmin2(P(i, k), P(i, kdm) - q));
+    }
+    const double botvel = sqrt(ubot * ubot + vbot * vbot) / thkbot;
+    drag[i] = dmin2(DRCOEF * (botvel + cbar) * ONEM / thkbot, 1. / m->batrop);
+    ustarb[i] = sqrt(DRCOEF) * botvel;
+    for (int k = 0; k < kdm; k++) pgfx[k] = pgfy[k] = 0.;
+    for (int e = 0; e < g->nprox[i]; e++)
+      for (int k = 0; k < kdm; k++) {
+        if (m->janjic) {
+          pgfx[k] = pgfx[k] - (E(geop_edg, i, e, k) + E(uvsq_edg, i, e, k)) * SVC(i, e, 1);
+          pgfy[k] = pgfy[k] + (E(geop_edg, i, e, k) + E(uvsq_edg, i, e, k)) * SVC(i, e, 0);
+        } else {
+          P(i, k + 1) = P(i, k) + C(dpm, i, k);
+          pgfx[k] = pgfx[k] - (E(mont_edg, i, e, k) + E(uvsq_edg, i, e, k)) * SVC(i, e, 1);
+          pgfy[k] = pgfy[k] + (E(mont_edg, i, e, k) + E(uvsq_edg, i, e, k)) * SVC(i, e, 0);
+          pgfx[k] = pgfx[k] + .5 * (P(i, k) + P(i, k + 1)) * E(spcv_edg, i, e, k) * SVC(i, e, 1);
+          pgfy[k] = pgfy[k] - .5 * (P(i, k) + P(i, k + 1)) * E(spcv_edg, i, e, k) * SVC(i, e, 0);
+        }
+      }
+    for (int k = 0; k < kdm; k++) {
+      C(gradx, i, k) = pgfx[k] * g->rarea[i];
+      C(grady, i, k) = pgfy[k] * g->rarea[i];
+      C(worku, i, k) = C(ucn, i, k);
+      C(workv, i, k) = C(vcn, i, k);
+    }
+  }
+  /* del4prep (hycom_dissip.F90:9-110): minus the mean of the neighbours,
+   * rotated into the cell's frame through the edge's, walls by the slip
+   * condition */
+  memcpy(d4u, worku, cz * sizeof(double));
+  memcpy(d4v, workv, cz * sizeof(double));
+  for (int i = 0; i < nip; i++) {
+    if (!WET(i)) continue;
+    const int np = g->nprox[i];
+    for (int k = 0; k < kdm; k++) avgu[k] = avgv[k] = 0.;
+    for (int e = 0; e < np; e++) {
+      const int ix = g->prox[i * NPP + e];
+      for (int k = 0; k < kdm; k++) {
+        double ungbor, vngbor;
+        if (WET(ix)) {
+          const double uedge = CSN(m->cs, i, e, 1) * C(worku, ix, k) + CSN(m->sn, i, e, 1) * C(workv, ix, k);
+          const double vedge = -CSN(m->sn, i, e, 1) * C(worku, ix, k) + CSN(m->cs, i, e, 1) * C(workv, ix, k);
+          ungbor = CSN(m->cs, i, e, 0) * uedge - CSN(m->sn, i, e, 0) * vedge;
+          vngbor = CSN(m->sn, i, e, 0) * uedge + CSN(m->cs, i, e, 0) * vedge;
+        } else {
+          ungbor = SLIP * C(worku, i, k);
+          vngbor = SLIP * C(workv, i, k);
+        }
+        avgu[k] = avgu[k] + ungbor;
+        avgv[k] = avgv[k] + vngbor;
+      }
+    }
+    for (int k = 0; k < kdm; k++) {
+      const double factor = 1. / np;
+      C(d4u, i, k) = C(worku, i, k) - avgu[k] * factor;
+      C(d4v, i, k) = C(workv, i, k) - avgv[k] * factor;
+    }
+  }
+  /* dissip (hycom_dissip.F90:113-272): thickness-weighted momentum fluxes
+   * through the edges, walls blended in by blokf, summed into a tendency */
+  for (int i = 0; i < nip; i++) {
+    for (int k = 0; k < kp; k++) DPR(i, k) = LAND_SPVAL;
+    if (!WET(i)) continue;
+    DPR(i, 0) = 0.;
+    for (int k = 0; k < kdm; k++) DPR(i, k + 1) = DPR(i, k) + C(dpn, i, k);
+  }
+  for (int i = 0; i < nip; i++) {
+    if (!WET(i)) continue;
+    for (int e = 0; e < g->nprox[i]; e++) {
+      const int ix = g->prox[i * NPP + e];
+      for (int k = 0; k < kdm; k++) {
+        const double urot = CSN(m->cs, i, e, 0) * C(d4u, i, k) + CSN(m->sn, i, e, 0) * C(d4v, i, k);
+        const double vrot = -CSN(m->sn, i, e, 0) * C(d4u, i, k) + CSN(m->cs, i, e, 0) * C(d4v, i, k);
+        double ungbor, vngbor;
+        if (WET(ix)) {
+          ungbor = CSN(m->cs, i, e, 1) * C(d4u, ix, k) + CSN(m->sn, i, e, 1) * C(d4v, ix, k);
+          vngbor = -CSN(m->sn, i, e, 1) * C(d4u, ix, k) + CSN(m->cs, i, e, 1) * C(d4v, ix, k);
+          const double blokd = hyc_blokf(C(dpn, i, k), DPR(i, k + 1), DPR(ix, kdm), DPR(ix, k));
+          ungbor = (1. - blokd) * ungbor + blokd * urot * SLIP;
+          vngbor = (1. - blokd) * vngbor + blokd * vrot * SLIP;
+        } else {
+          ungbor = urot * SLIP;
+          vngbor = vrot * SLIP;
+        }
+        const double a = dmax2(C(dpn, i, k), THSHLD), b = dmax2(C(dpn, ix, k), THSHLD);
+        const double hh = a * b / (a + b);
+        E(ufx, i, e, k) = (ungbor - urot) * g->sideln[i * NPP + e] * 2. * hh;
+        E(vfx, i, e, k) = (vngbor - vrot) * g->sideln[i * NPP + e] * 2. * hh;
+      }
+    }
+  }
+  for (int i = 0; i < nip; i++) {
+    if (!WET(i)) continue;
+    for (int k = 0; k < kdm; k++) C(worku, i, k) = C(workv, i, k) = 0.;
+    for (int e = 0; e < g->nprox[i]; e++)
+      for (int k = 0; k < kdm; k++) {
+        const double factor = g->rarea[i] / dmax2(C(dpn, i, k), THSHLD);
+        const double uflux = CSN(m->cs, i, e, 0) * E(ufx, i, e, k) - CSN(m->sn, i, e, 0) * E(vfx, i, e, k);
+        const double vflux = CSN(m->sn, i, e, 0) * E(ufx, i, e, k) + CSN(m->cs, i, e, 0) * E(vfx, i, e, k);
+        C(worku, i, k) = C(worku, i, k) + uflux * factor;
+        C(workv, i, k) = C(workv, i, k) + vflux * factor;
+      }
+  }
+  /* vorticity, deformation, dissipation, Coriolis force, stresses, the
+   * leapfrog update, downward mixing and time smoothing (211-389) */
+  for (int i = 0; i < nip; i++) {
+    if (!WET(i)) continue;
+    for (int k = 0; k < kdm; k++) dudx[k] = dudy[k] = dvdx[k] = dvdy[k] = 0.;
+    for (int e = 0; e < g->nprox[i]; e++)
+      for (int k = 0; k < kdm; k++) {
+        dudx[k] = dudx[k] + SVE(i, e, 1) * E(um_edg, i, e, k);
+        dudy[k] = dudy[k] - SVE(i, e, 0) * E(um_edg, i, e, k);
+        dvdx[k] = dvdx[k] + SVE(i, e, 1) * E(vm_edg, i, e, k);
+        dvdy[k] = dvdy[k] - SVE(i, e, 0) * E(vm_edg, i, e, k);
+      }
+    for (int k = 0; k < kdm; k++) {
+      dudx[k] = dudx[k] * g->rarea[i];
+      dudy[k] = dudy[k] * g->rarea[i];
+      dvdx[k] = dvdx[k] * g->rarea[i];
+      dvdy[k] = dvdy[k] * g->rarea[i];
+    }
+    for (int k = 0; k < kdm; k++) {
+      const double relvor = dvdx[k] - dudy[k];
+      const double a = dudx[k] - dvdy[k], b = dudy[k] + dvdx[k];
+      const double deform = sqrt(a * a + b * b);
+      const double smag = deform * sqrt(g->area[i]) * m->smagcf;
+      const double udissp = C(worku, i, k) * dmax2(m->veldff, smag);
+      const double vdissp = C(workv, i, k) * dmax2(m->veldff, smag);
+      const double corio = OMEGX2 * sin(m->lat[i]);
+      const double uzeta = C(ucm, i, k) * (relvor + corio) + utrop[i * 3 + leapm - 1] * relvor;
+      const double vzeta = C(vcm, i, k) * (relvor + corio) + vtrop[i * 3 + leapm - 1] * relvor;
+      double ustres = taux[i] * MGRVITY / thktop * (dmin2(P(i, k + 1) + ONEMM, thktop) - dmin2(P(i, k), thktop)) /
+                      dmax2(C(dpn, i, k), ONEMM);
+      double vstres = tauy[i] * MGRVITY / thktop * (dmin2(P(i, k + 1) + ONEMM, thktop) - dmin2(P(i, k), thktop)) /
+                      dmax2(C(dpn, i, k), ONEMM);
+      const double q = (dmax2(P(i, kdm) - thkbot, P(i, k + 1)) -
+                        dmax2(P(i, kdm) - thkbot, dmin2(P(i, k + 1) - ONEMM, P(i, k)))) / dmax2(C(dpn, i, k), ONEMM);
+      ustres = ustres - drag[i] * C(utotn, i, k) * q;
+      vstres = vstres - drag[i] * C(vtotn, i, k) * q;
+      uold[k] = C(ucn, i, k);
+      vold[k] = C(vcn, i, k);
+      C(ucn, i, k) = C(ucn, i, k) + dtleap * (C(gradx, i, k) + vzeta + udissp + ustres);
+      C(vcn, i, k) = C(vcn, i, k) + dtleap * (C(grady, i, k) - uzeta + vdissp + vstres);
+      if (k > 0) {            /* downward momentum mixing into thin bottom layers */
+        const double slab = dmax2(TENCM, P(i, k + 1) - P(i, kdm) + TENM);
+        const double qq = dmin2(C(dpn, i, k), slab);
+        C(ucn, i, k) = (C(ucn, i, k) * qq + C(ucn, i, k - 1) * (slab - qq)) / slab;
+        C(vcn, i, k) = (C(vcn, i, k) * qq + C(vcn, i, k - 1) * (slab - qq)) / slab;
+      }
+    }
+    for (int k = 0; k < kdm; k++) {
+      C(ucm, i, k) = ASSLUV * (uold[k] + C(ucn, i, k)) + (1. - ASSLUV - ASSLUV) * C(ucm, i, k);
+      C(vcm, i, k) = ASSLUV * (vold[k] + C(vcn, i, k)) + (1. - ASSLUV - ASSLUV) * C(vcm, i, k);
+    }
+  }
+#undef CSN
+#undef SVC
+#undef DPR
+  free(gradx); free(grady); free(utotn); free(vtotn); free(worku); free(workv); free(d4u); free(d4v);
+  free(ufx); free(vfx); free(dpres); free(drag); free(pgfx); free(pgfy); free(avgu); free(avgv);
+  free(dudx); free(dudy); free(dvdx); free(dvdy); free(uold); free(vold);
 }

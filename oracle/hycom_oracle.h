@@ -33,3 +33,27 @@ void fo_hycom_edgvar(const fo_hyc_grid *g, const double *cs, const double *sn, i
                      const double *spcifv, double *um_edg, double *vm_edg, double *un_edg, double *vn_edg,
                      double *dp_edg, double *pr_edg, double *geop_edg, double *mont_edg, double *uvsq_edg,
                      double *spcv_edg);
+
+/* the grid and namelist values hycom_momtum reads beyond fo_hyc_grid:
+ * cs/sn (4,npp,nip) as [(i*6+e)*4 + j], sidevec_c (2,npp,nip) as
+ * [(i*6+e)*2 + c], the latitude (radians; corio = omegx2*sin(lat) as
+ * dyn_init.F90:753 forms it, and meshsz = sqrt(area) as hycom_init.F90:327
+ * does, with the C library's sin and sqrt), the barotropic time step batrop,
+ * veldff and smagcf (fimnamelist.F90:253-254) and janjic_ocn */
+typedef struct {
+  const double *cs, *sn, *sidevec_c, *lat;
+  double batrop, veldff, smagcf;
+  int janjic;
+} fo_hyc_mom;
+
+/* hycom_momtum.F90 momtum(nstep, leapm, leapn, utrop, vtrop, uclin, vclin,
+ * um_edg, vm_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg, taux, tauy,
+ * ubforc, vbforc, dp, pres, ustarb) with hycom_dissip's del4prep and dissip;
+ * modesplit is .false. (hycom_control.F90:5), so ubforc and vbforc are not
+ * set and are left out.  Reads nip, kdm, prox, nprox, wet, rarea, sideln,
+ * sidevec_e and dtleap of g. */
+void fo_hycom_momtum(const fo_hyc_grid *g, const fo_hyc_mom *m, int leapm, int leapn, const double *utrop,
+                     const double *vtrop, double *uclin, double *vclin, const double *um_edg, const double *vm_edg,
+                     const double *geop_edg, const double *mont_edg, const double *uvsq_edg,
+                     const double *spcv_edg, const double *taux, const double *tauy, const double *dp,
+                     double *pres, double *ustarb);

@@ -209,6 +209,11 @@ class _HycMass(ct.Structure):
     _fields_ = [("totmass", ct.c_double), ("massglb0", ct.c_double), ("masscor", ct.c_double)]
 
 
+class _HycMom(ct.Structure):
+    _fields_ = [("cs", _PD), ("sn", _PD), ("sidevec_c", _PD), ("lat", _PD), ("batrop", ct.c_double),
+                ("veldff", ct.c_double), ("smagcf", ct.c_double), ("janjic", ct.c_int)]
+
+
 class HycomCnuity:
     """The checker's HYCOM cnuity on a glvl.Grid: arrays as
     ocean.hycom_cnuity_case makes them (C order of the Fortran layouts),
@@ -254,6 +259,30 @@ class HycomCnuity:
     @property
     def mass(self):
         return self.m.totmass, self.m.massglb0, self.m.masscor
+
+    def momtum(self, grid, leapm: int, leapn: int, janjic: bool = False):
+        """momtum (hycom_momtum.F90:16) on this instance's arrays (after
+        edgvar, which leaves its edge values in self.a / self.e): uclin,
+        vclin (self.e), pres (self.a) and ustarb (self.e) are updated; the
+        case supplies taux, tauy, ustarb, batrop, veldff and smagcf."""
+        a, e, case = self.a, self.e, self.case
+        for n in ("taux", "tauy", "ustarb"):
+            if n not in e:
+                e[n] = np.ascontiguousarray(case[n], np.float64).copy()
+        if "sidevec_c" not in self._keep:
+            self._keep["sidevec_c"] = np.ascontiguousarray(grid.sidevec_c, np.float64)
+            self._keep["lat"] = np.ascontiguousarray(grid.lat, np.float64)
+        mm = _HycMom()
+        for n in ("cs", "sn", "sidevec_c", "lat"):
+            setattr(mm, n, self._keep[n].ctypes.data_as(_PD))
+        mm.batrop, mm.veldff, mm.smagcf, mm.janjic = case["batrop"], case["veldff"], case["smagcf"], int(janjic)
+        lib().fo_hycom_momtum(ct.byref(self.g), ct.byref(mm), leapm, leapn,
+                              *[e[n].ctypes.data_as(_PD) for n in ("utrop", "vtrop", "uclin", "vclin")],
+                              a["um_edg"].ctypes.data_as(_PD), a["vm_edg"].ctypes.data_as(_PD),
+                              *[e[n].ctypes.data_as(_PD) for n in ("geop_edg", "mont_edg", "uvsq_edg", "spcv_edg",
+                                                                   "taux", "tauy")],
+                              a["dp"].ctypes.data_as(_PD), a["pres"].ctypes.data_as(_PD),
+                              e["ustarb"].ctypes.data_as(_PD))
 
     EDG_IN = ("uclin", "vclin", "utrop", "vtrop", "geop", "montg", "spcifv")
     EDG_OUT = ("pr_edg", "geop_edg", "mont_edg", "uvsq_edg", "spcv_edg")

@@ -203,6 +203,41 @@ def run_hycedgref(glvl_path, case: dict, *, nstep: int = 1, leapm: int = 2, janj
     return rec
 
 
+HYCM_MAGIC = 1213809485
+
+
+def run_hycmomref(glvl_path, case: dict, edges: dict, *, nstep: int = 1, leapm: int = 2, janjic: bool = False,
+                  workdir=None) -> dict:
+    """Run the reference momtum (oracle/ref/hycmomref.F90) once on the inputs
+    of ocean.hycom_momtum_case, with the edge values in 'edges' (um_edg,
+    vm_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg); returns the records."""
+    workdir = workdir or os.path.join(REF, "runs")
+    os.makedirs(workdir, exist_ok=True)
+    kdm = case["kdm"]
+    nip = case["wet"].shape[0]
+    fin = os.path.join(workdir, "hycm_state_%d.bin" % os.getpid())
+    fout = os.path.join(workdir, "hycm_out_%d.bin" % os.getpid())
+    with open(fin, "wb") as f:
+        f.write(struct.pack("<8i", HYCM_MAGIC, kdm, nip, nstep, leapm, int(janjic), 0, 0))
+        f.write(struct.pack("<4d", case["dtleap"], case["batrop"], case["veldff"], case["smagcf"]))
+        f.write(np.ascontiguousarray(case["wet"], dtype="<i4").tobytes())
+        for n in ("utrop", "vtrop", "uclin", "vclin"):
+            f.write(np.ascontiguousarray(case[n], dtype="<f8").tobytes())
+        for n in ("um_edg", "vm_edg", "geop_edg", "mont_edg", "uvsq_edg", "spcv_edg"):
+            f.write(np.ascontiguousarray(edges[n], dtype="<f8").tobytes())
+        for n in ("taux", "tauy", "dp", "pres", "ustarb"):
+            f.write(np.ascontiguousarray(case[n], dtype="<f8").tobytes())
+    r = subprocess.run([os.path.join(BIN, "hycmomref"), glvl_path, fin, fout], cwd=workdir, capture_output=True,
+                       text=True, preexec_fn=_big_stack)
+    if r.returncode != 0:
+        raise RuntimeError("hycmomref failed (%d): %s" % (r.returncode, (r.stdout + r.stderr)[-3000:]))
+    rec = read_records(fout)
+    rec["_stdout"] = r.stdout
+    os.remove(fin)
+    os.remove(fout)
+    return rec
+
+
 def run_hycomref(glvl_path, case: dict, *, nsteps: int, thkdff: float = 0.05, biharm: float = 0.75,
                  nstep0: int = 1, mass=(0.0, 0.0, 0.0), workdir=None) -> dict:
     """Run the reference HYCOM continuity equation (oracle/ref/hycomref.F90)

@@ -37,6 +37,12 @@ Fixtures (data only):
                         on its edge values, step 1, on the seeded inputs of
                         ocean.hycom_edgvar_case at G2 and G3 x 32 layers,
                         with and without janjic_ocn
+  hycom_momtum_ref.json SHA-256 of the outputs of the reference HYCOM momentum
+                        equation (hycom_momtum.F90 momtum with hycom_dissip.F90,
+                        FP64, hycmomref): uclin, vclin, pres, ustarb at step 1
+                        on ocean.hycom_momtum_case at G2 and G3 x 32 layers,
+                        with and without janjic_ocn, the edge values formed by
+                        the checker's bclin_edgvar (momtum_inputs)
   dyn_G2_ref.json       SHA-256 of every output record of the reference FP64
                         build (dynref_r8) for 3 full steps (mode 0) and 3
                         transport steps (mode 1), stage dumps of step 1, the
@@ -229,6 +235,38 @@ def hycom_edgvar_reference():
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def momtum_inputs(O, g, case):
+    """The edge values momtum reads, formed from the case by the checker's
+    bclin_edgvar (byte-equal to the reference's) with janjic_ocn on, so that
+    geop_edg is set too; and a checker instance whose pres is the case's."""
+    h = O.HycomCnuity(g, case)
+    h.edgvar(g, 2, 1, True)
+    edges = {n: h.a[n].copy() for n in ("um_edg", "vm_edg")}
+    edges.update({n: h.e[n].copy() for n in ("geop_edg", "mont_edg", "uvsq_edg", "spcv_edg")})
+    h.a["pres"][:] = case["pres"]
+    return h, edges
+
+
+def hycom_momtum_reference():
+    """Digests of the reference momtum at step 1 (leapm = 2) at G2 and G3 x
+    32 layers, with and without janjic_ocn, on ocean.hycom_momtum_case and
+    the edge values momtum_inputs forms from it."""
+    from fimocn_subx_realtime_hera_apr2020_amd import ocean
+    from oracle import oracle as O
+    out = {"kdm": 32, "nstep": 1, "leapm": 2, "runs": {}}
+    for lv in (2, 3):
+        path = os.path.join(HERE, "glvl_G%d.dat" % lv)
+        g = glvl.read_glvl(path)
+        case = ocean.hycom_momtum_case(g, kdm=32)
+        _, edges = momtum_inputs(O, g, case)
+        for janjic in (False, True):
+            r = refrun.run_hycmomref(path, case, edges, nstep=1, leapm=2, janjic=janjic)
+            out["runs"]["G%d_janjic%d" % (lv, int(janjic))] = {k: sha(v) for k, v in r.items()
+                                                               if not k.startswith("_")}
+    with open(os.path.join(HERE, "hycom_momtum_ref.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     main()
     schemes()
@@ -236,3 +274,4 @@ if __name__ == "__main__":
     r4_reference()
     hycom_reference()
     hycom_edgvar_reference()
+    hycom_momtum_reference()

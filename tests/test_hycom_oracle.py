@@ -169,3 +169,64 @@ def test_hycom_edgvar_properties(oracle_lib):
     assert len(pent) > 0
     assert (h.a["um_edg"][pent, 5] == 0).all() and (h.a["dpm_edg"][pent, 5] == 0).all()
     assert (h.e["geop_edg"] == 7.0).all()
+
+
+MOM_RECORDS = ("uclin", "vclin", "pres", "ustarb")
+
+
+def _momtum(O, g, janjic, seed=None):
+    import sys
+    sys.path.insert(0, GOLDEN)
+    import make_golden
+    case = ocean.hycom_momtum_case(g, kdm=32) if seed is None else ocean.hycom_momtum_case(g, kdm=32, seed=seed)
+    h, edges = make_golden.momtum_inputs(O, g, case)
+    h.momtum(g, 2, 1, janjic)
+    return case, edges, h, {"uclin": h.e["uclin"], "vclin": h.e["vclin"], "pres": h.a["pres"],
+                            "ustarb": h.e["ustarb"]}
+
+
+@pytest.mark.parametrize("level", [2, 3])
+@pytest.mark.parametrize("janjic", [False, True])
+def test_hycom_momtum_checker_equals_reference_digests(oracle_lib, level, janjic):
+    with open(os.path.join(GOLDEN, "hycom_momtum_ref.json")) as f:
+        ref = json.load(f)
+    g = glvl.read_glvl(os.path.join(GOLDEN, "glvl_G%d.dat" % level))
+    _, _, _, out = _momtum(oracle_lib, g, janjic)
+    want = ref["runs"]["G%d_janjic%d" % (level, int(janjic))]
+    assert set(out) == set(want)
+    for n, v in out.items():
+        assert _sha(v) == want[n], n
+
+
+def test_hycom_momtum_checker_equals_live_reference_g4(oracle_lib):
+    from oracle import refrun
+    if not os.path.exists(os.path.join(refrun.BIN, "hycmomref")):
+        pytest.skip("reference build (oracle/_ref) not present")
+    path = refrun.ref_glvl(4)
+    g = glvl.read_glvl(path)
+    for janjic in (False, True):
+        case, edges, _, out = _momtum(oracle_lib, g, janjic, seed=13)
+        r = refrun.run_hycmomref(path, case, edges, nstep=1, leapm=2, janjic=janjic)
+        for n, v in out.items():
+            assert np.array_equal(np.ravel(v).view(np.uint64), np.ravel(r[n]).view(np.uint64)), (janjic, n)
+
+
+def test_hycom_momtum_properties(oracle_lib):
+    """Land columns keep their velocities, pressure and friction velocity; the
+    friction velocity of the ocean is non-negative; the bottom layers that
+    the downward mixing reaches (thinner than their mixing slab) exist in the
+    input, so that the recurrence is exercised."""
+    g = glvl.read_glvl(os.path.join(GOLDEN, "glvl_G3.dat"))
+    case, _, h, out = _momtum(oracle_lib, g, False)
+    wet = case["wet"] > 0
+    for n in ("uclin", "vclin"):
+        assert np.array_equal(out[n][:, ~wet], case[n][:, ~wet]), n
+        assert not np.array_equal(out[n][:, wet], case[n][:, wet]), n
+    assert np.array_equal(out["pres"][~wet], case["pres"][~wet])
+    assert np.array_equal(out["ustarb"][~wet], case["ustarb"][~wet]) and (out["ustarb"][wet] >= 0).all()
+    pres, dpn, kdm = out["pres"], case["dp"][0], case["kdm"]
+    mixed = 0
+    for k in range(1, kdm):
+        slab = np.maximum(0.1 * ocean.ONEM, pres[:, k + 1] - pres[:, kdm] + 10 * ocean.ONEM)
+        mixed += int(np.sum((slab > np.minimum(dpn[:, k], slab)) & wet))
+    assert mixed > 100
