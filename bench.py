@@ -568,6 +568,16 @@ HYC_EDG_KERNEL_BYTES = {
     "hyc_edgint": 8 + 8 + 16 + 96 + 50 + 4 * 48,   # dp pres montg spcifv um vm -> pr_edg dp_edg mont uvsq spcv
     "hyc_janjic": 8 + 8 + 8 + 48,              # pres geop spcifv -> geop_edg
 }
+HYC_MOM_KERNEL_BYTES = {
+    "hyc_mom_pgf": 16 + 8 + 3 * 48 + 8 + 8 + 16,     # dp (2 levels) u v (new) mont uvsq spcv edges pres -> pres gradx grady
+    "hyc_del4": 16 + 16,                            # u v (new) -> their del4prep
+    "hyc_dissip": 16 + 8 + 16,                      # del4prep u v, dp -> dissipation tendencies
+    "hyc_mom_upd": 96 + 16 + 32 + 16 + 8 + 8 + 32,  # um vm edges, tendencies, u v (2 levels), grad, pres, dp -> u v
+}
+# bclin_edgvar, cnuity and momtum: in u, v and dp (2 levels), pres, montg,
+# spcifv, mssfx_avg; out u, v (2 levels), dp, pres, massflx, mssfx_avg (the
+# edge values stay inside)
+HYC_OCEAN_STEP_BYTES = (32 + 16 + 8 + 16 + 48) + (32 + 16 + 8 + 48 + 48)
 # bclin_edgvar then cnuity: in uclin, vclin and dp (2 levels), pres, montg,
 # spcifv, mssfx_avg; out the edge values the ocean step goes on to read (um,
 # vm, un, vn, dp_edg, mont_edg, uvsq_edg, spcv_edg on 6 edges, pr_edg on 6
@@ -615,15 +625,17 @@ def _ocean_leg(a, o, wcl, step, byte_model):
 def run_hycom(a, fimhip, cpu=True):
     """The HYCOM ocean on G8 (655362 cells) x 32 layers, ~30 % land,
     biharmonic interface smoothing, dtleap 1800 s, FP64 on device-resident
-    arrays: its continuity equation alone (hycom_cnuity.F90 cnuity), and the
+    arrays: its continuity equation alone (hycom_cnuity.F90 cnuity); the
     edge interpolation followed by the continuity equation, as hycom_run
     calls them (hycom_edgvar.F90 bclin_edgvar, then cnuity on its edge
-    values); the checker's C restatement of each timed on one host core over
-    2 steps of the same input."""
+    values); and those two followed by the momentum equation
+    (hycom_momtum.F90 momtum with hycom_dissip), the velocities it writes
+    feeding the next step; the checker's C restatement of each timed on one
+    host core over 2 steps of the same input."""
     from fimocn_subx_realtime_hera_apr2020_amd import ocean
     g = fimhip.icos_grid(8)
     kdm, thkdff, biharm = 32, 0.05, 0.75
-    case = ocean.hycom_edgvar_case(g, kdm=kdm)
+    case = ocean.hycom_momtum_case(g, kdm=kdm)
     # the continuity equation alone reads hycom_cnuity_case's edge values (the
     # same grid, mask and thickness; each edge's two sides carry one velocity)
     cnu = ocean.hycom_cnuity_case(g, kdm=kdm, seed=ocean.EDGVAR_CASE_SEED)
@@ -631,7 +643,7 @@ def run_hycom(a, fimhip, cpu=True):
     wcl = nwet * kdm
     workload = ("G8 (%d cells, %d ocean) x %d layers, thkdff %g, biharm %g, dtleap %g s, FP64, resident"
                 % (g.nip, nwet, kdm, thkdff, biharm, case["dtleap"]))
-    bytes_all = dict(HYC_KERNEL_BYTES, **HYC_EDG_KERNEL_BYTES)
+    bytes_all = dict(HYC_KERNEL_BYTES, **HYC_EDG_KERNEL_BYTES, **HYC_MOM_KERNEL_BYTES)
     out = {"workload": workload}
     o = fimhip.Ocean(g, kdm, case["wet"], case["dtleap"], thkdff=thkdff, biharm=biharm)
     try:
@@ -647,7 +659,7 @@ def run_hycom(a, fimhip, cpu=True):
         def chain(st):
             o.step_edgvar(1 + st % 2, 2 - st % 2)
             o.step(st, 1 + st % 2, 2 - st % 2)
-        for n in fimhip.OCN_FIELDS:
+        for n in fimhip.OCN_EDGVAR_FIELDS:
             o.upload(n, case[n])
         leg = _ocean_leg(a, o, wcl, chain, bytes_all)
         ms = leg["ms_per_step"]
@@ -655,17 +667,34 @@ def run_hycom(a, fimhip, cpu=True):
                                 "achieved": HYC_CHAIN_BYTES * wcl / (ms * 1e-3) / 1e9, "unit": "GB/s",
                                 "frac": HYC_CHAIN_BYTES * wcl / (ms * 1e-3) / HBM_PEAK}
         out["edgvar_cnuity"] = leg
+
+        def ocean_step(st):
+            o.step_edgvar(1 + st % 2, 2 - st % 2)
+            o.step(st, 1 + st % 2, 2 - st % 2)
+            o.step_momtum(1 + st % 2, 2 - st % 2)
+        o.set_momtum(case["batrop"], case["veldff"], case["smagcf"])
+        for n in fimhip.OCN_FIELDS:
+            o.upload(n, case[n])
+        leg = _ocean_leg(a, o, wcl, ocean_step, bytes_all)
+        ms = leg["ms_per_step"]
+        leg["step_roofline"] = {"compulsory_bytes_per_cell_level": HYC_OCEAN_STEP_BYTES,
+                                "achieved": HYC_OCEAN_STEP_BYTES * wcl / (ms * 1e-3) / 1e9, "unit": "GB/s",
+                                "frac": HYC_OCEAN_STEP_BYTES * wcl / (ms * 1e-3) / HBM_PEAK}
+        out["ocean_step"] = leg
     finally:
         o.close()
     if cpu:
         from oracle import oracle as O
-        for name, with_edgvar in (("cnuity", False), ("edgvar_cnuity", True)):
+        for name, with_edgvar, with_momtum in (("cnuity", False, False), ("edgvar_cnuity", True, False),
+                                               ("ocean_step", True, True)):
             h = O.HycomCnuity(g, case if with_edgvar else cnu, thkdff=thkdff, biharm=biharm)
             t0 = time.perf_counter()
             for st in (1, 2):
                 if with_edgvar:
                     h.edgvar(g, 1 + st % 2, 2 - st % 2)
                 h.step(st, 1 + st % 2, 2 - st % 2)
+                if with_momtum:
+                    h.momtum(g, 1 + st % 2, 2 - st % 2)
             tc = time.perf_counter() - t0
             leg = out[name]
             leg["cpu_checker"] = {"value": wcl * 2 / tc, "unit": "ocean cell-level-steps/s", "cores": 1,

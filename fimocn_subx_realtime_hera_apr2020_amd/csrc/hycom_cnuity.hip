@@ -508,7 +508,8 @@ static int ocn_launch(fimhip_ocn *o, int nstep, int leapm, int leapn);
 extern "C" int fimhip_ocn_create(const fimhip_grid_desc *gd, const double *rprox_ln, int kdm, const int *wet,
                                  double dtleap, double thkdff, double biharm, int device, fimhip_ocn **out) {
   if (!gd || !rprox_ln || !wet || !out || kdm < 2 || kdm > 64 || gd->nip < 12 || !gd->nprox || !gd->prox ||
-      !gd->proxs || !gd->sidevec_e || !gd->sideln || !gd->area || !gd->cs || !gd->sn || !(dtleap > 0.))
+      !gd->proxs || !gd->sidevec_e || !gd->sideln || !gd->area || !gd->cs || !gd->sn || !gd->sidevec_c ||
+      !gd->lat || !(dtleap > 0.))
     return FIMHIP_EINVAL;
   *out = nullptr;
   int ndev = 0;
@@ -571,16 +572,29 @@ extern "C" int fimhip_ocn_create(const fimhip_grid_desc *gd, const double *rprox
   int *pp, *ps, *pc;
   double *pa, *pr, *pv, *psl, *prs, *prl, *pcs, *psn;
   const std::vector<double> csv(gd->cs, gd->cs + (size_t)nip * NPP * 4), snv(gd->sn, gd->sn + (size_t)nip * NPP * 4);
+  const std::vector<double> svcv(gd->sidevec_c, gd->sidevec_c + (size_t)nip * NPP * 2);
+  // the momentum equation's cell values, with the C library's sin and sqrt as
+  // the reference forms them (dyn_init.F90:753, hycom_init.F90:327)
+  std::vector<double> corv(nip), meshv(nip);
+  for (int i = 0; i < nip; i++) {
+    corv[i] = 1.4584e-4 * sin(gd->lat[i]);
+    meshv[i] = sqrt(gd->area[i]);
+  }
+  double *pvc, *pco, *pms;
   if ((r = o->arena.upload(&pp, prox)) || (r = o->arena.upload(&ps, proxs)) || (r = o->arena.upload(&pc, cinfo)) ||
       (r = o->arena.upload(&pa, o->area)) || (r = o->arena.upload(&pr, rarea)) || (r = o->arena.upload(&pv, sve)) ||
       (r = o->arena.upload(&psl, sideln)) || (r = o->arena.upload(&prs, rsideln)) ||
-      (r = o->arena.upload(&prl, rpl)) || (r = o->arena.upload(&pcs, csv)) || (r = o->arena.upload(&psn, snv))) {
+      (r = o->arena.upload(&prl, rpl)) || (r = o->arena.upload(&pcs, csv)) || (r = o->arena.upload(&psn, snv)) ||
+      (r = o->arena.upload(&pvc, svcv)) || (r = o->arena.upload(&pco, corv)) || (r = o->arena.upload(&pms, meshv))) {
     fimhip_ocn_destroy(o);
     return r;
   }
   g.prox = pp; g.proxs = ps; g.cinfo = pc;
   g.area = pa; g.rarea = pr; g.sve = pv; g.sideln = psl; g.rsideln = prs; g.rprox_ln = prl;
   g.cs = pcs; g.sn = psn;
+  o->mo.svc = pvc; o->mo.corio = pco; o->mo.meshsz = pms;
+  o->mo.veldff = 0.1;              // fimnamelist.F90:254
+  o->mo.smagcf = 0.3;              // fimnamelist.F90:253
   const size_t cz = (size_t)nip * kdm, ez = cz * NPP;
   HycState &s = o->s;
   double *um, *vm, *un, *vn, *dpe;
@@ -593,7 +607,11 @@ extern "C" int fimhip_ocn_create(const fimhip_grid_desc *gd, const double *rprox
       {&o->ed.uclin, 2 * cz}, {&o->ed.vclin, 2 * cz}, {&o->ed.utrop, (size_t)nip * 3}, {&o->ed.vtrop, (size_t)nip * 3},
       {&o->ed.geop, (size_t)nip * (kdm + 1)}, {&o->ed.montg, cz}, {&o->ed.spcifv, cz},
       {&o->ed.pr_edg, (size_t)nip * NPP * (kdm + 1)}, {&o->ed.geop_edg, ez}, {&o->ed.mont_edg, ez},
-      {&o->ed.uvsq_edg, ez}, {&o->ed.spcv_edg, ez}};
+      {&o->ed.uvsq_edg, ez}, {&o->ed.spcv_edg, ez},
+      // the momentum equation's forcing and work arrays
+      {&o->mo.taux, (size_t)nip}, {&o->mo.tauy, (size_t)nip}, {&o->mo.ustarb, (size_t)nip},
+      {&o->mo.drag, (size_t)nip}, {&o->mo.dpbot, (size_t)nip}, {&o->mo.gradx, cz}, {&o->mo.grady, cz},
+      {&o->mo.d4u, cz}, {&o->mo.d4v, cz}, {&o->mo.du, cz}, {&o->mo.dv, cz}};
   for (auto &a : arrs)
     if ((r = o->arena.alloc(a.p, a.n))) { fimhip_ocn_destroy(o); return r; }
   s.um = um; s.vm = vm; s.un = un; s.vn = vn; s.dpm_edg = dpe;
@@ -639,6 +657,9 @@ double *ocn_field(fimhip_ocn *o, int f, size_t *n) {
     case FIMHIP_OCN_MONT_EDG: *n = ez; return o->ed.mont_edg;
     case FIMHIP_OCN_UVSQ_EDG: *n = ez; return o->ed.uvsq_edg;
     case FIMHIP_OCN_SPCV_EDG: *n = ez; return o->ed.spcv_edg;
+    case FIMHIP_OCN_TAUX: *n = (size_t)o->nip; return o->mo.taux;
+    case FIMHIP_OCN_TAUY: *n = (size_t)o->nip; return o->mo.tauy;
+    case FIMHIP_OCN_USTARB: *n = (size_t)o->nip; return o->mo.ustarb;
   }
   return nullptr;
 }

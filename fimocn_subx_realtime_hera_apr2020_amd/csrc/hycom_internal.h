@@ -105,6 +105,17 @@ struct HycEdg {
   double *pr_edg, *geop_edg, *mont_edg, *uvsq_edg, *spcv_edg;
 };
 
+// the momentum equation's grid values, forcing and work arrays
+// (hycom_momtum.F90, hycom_dissip.F90)
+struct HycMom {
+  const double *svc;              // sidevec_c (2,6,nip)
+  const double *corio, *meshsz;   // omegx2*sin(lat) (dyn_init.F90:753), sqrt(area) (hycom_init.F90:327)
+  double *taux, *tauy, *ustarb;   // (nip)
+  double *drag, *dpbot;           // (nip): bottom drag; bottom of dissip's pressure column
+  double *gradx, *grady, *d4u, *d4v, *du, *dv;   // (kdm,nip)
+  double batrop, veldff, smagcf;
+};
+
 }  // namespace hyc
 }  // namespace fimhip
 
@@ -122,6 +133,7 @@ struct fimhip_ocn {
   fimhip::hyc::HycGrid g{};
   fimhip::hyc::HycState s{};
   fimhip::hyc::HycEdg ed{};
+  fimhip::hyc::HycMom mo{};
   bool janjic = false;               // janjic_ocn (fimnamelist.F90:246)
   std::vector<int> wet;
   std::vector<double> area;

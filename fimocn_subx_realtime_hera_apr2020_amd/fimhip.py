@@ -50,7 +50,8 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ocn_create", "fimhip_ocn_destroy", "fimhip_ocn_cnuity_r8", "fimhip_ocn_cnuity",
            "fimhip_ocn_field_size", "fimhip_ocn_upload", "fimhip_ocn_download", "fimhip_ocn_mass",
            "fimhip_ocn_set_mass", "fimhip_ocn_set_dtleap", "fimhip_ocn_profile", "fimhip_ocn_profile_report",
-           "fimhip_ocn_edgvar_r8", "fimhip_ocn_edgvar", "fimhip_ocn_set_janjic"]
+           "fimhip_ocn_edgvar_r8", "fimhip_ocn_edgvar", "fimhip_ocn_set_janjic", "fimhip_ocn_momtum_r8",
+           "fimhip_ocn_momtum", "fimhip_ocn_set_momtum"]
 
 # real*4 drop-ins: name -> (number of float arrays, trailing int argument)
 R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, True), "mass_adv": (12, True),
@@ -177,6 +178,9 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ocn_edgvar_r8.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int] + [_PD] * 19
     lib.fimhip_ocn_edgvar.argtypes = [vp, ct.c_int, ct.c_int]
     lib.fimhip_ocn_set_janjic.argtypes = [vp, ct.c_int]
+    lib.fimhip_ocn_momtum_r8.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int] + [_PD] * 17
+    lib.fimhip_ocn_momtum.argtypes = [vp, ct.c_int, ct.c_int]
+    lib.fimhip_ocn_set_momtum.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double]
     lib.fimhip_ocn_profile.argtypes = [vp, ct.c_int]
     lib.fimhip_ocn_profile_report.argtypes = [vp, ct.c_char_p, ct.c_int]
     _LIB = lib
@@ -449,6 +453,27 @@ class Model(Instance):
         ones ``step`` (cnuity) then reads."""
         _chk(self.lib.fimhip_ocn_edgvar(self.h, leapm, leapn), "ocn_edgvar")
 
+    def momtum(self, nstep, leapm, leapn, utrop, vtrop, uclin, vclin, um_edg, vm_edg, geop_edg, mont_edg,
+               uvsq_edg, spcv_edg, taux, tauy, ubforc, vbforc, dp, pres, ustarb):
+        """momtum (hycom_momtum.F90:16); ubforc and vbforc are not written
+        (modesplit is off in the reference)."""
+        arrs = (utrop, vtrop, uclin, vclin, um_edg, vm_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg, taux, tauy,
+                ubforc, vbforc, dp, pres, ustarb)
+        names = ["utrop", "vtrop", "uclin", "vclin", "um_edg", "vm_edg", "geop_edg", "mont_edg", "uvsq_edg",
+                 "spcv_edg", "taux", "tauy", "taux", "tauy", "dp", "pres", "ustarb"]
+        for n, a in zip(names, arrs):
+            if a.shape != self._shape(n):
+                raise ValueError("%s: shape %s, expected %s" % (n, a.shape, self._shape(n)))
+        _chk(self.lib.fimhip_ocn_momtum_r8(self.h, nstep, leapm, leapn, *map(_p, arrs)), "ocn_momtum_r8")
+
+    def step_momtum(self, leapm: int, leapn: int):
+        """momtum on the device-resident arrays."""
+        _chk(self.lib.fimhip_ocn_momtum(self.h, leapm, leapn), "ocn_momtum")
+
+    def set_momtum(self, batrop: float, veldff: float = 0.1, smagcf: float = 0.3):
+        """batrop (hycom_init.F90:46) and the namelist's veldff, smagcf."""
+        _chk(self.lib.fimhip_ocn_set_momtum(self.h, float(batrop), float(veldff), float(smagcf)), "ocn_set_momtum")
+
     def set_janjic(self, on: bool):
         """janjic_ocn (fimnamelist.F90:246, default off)."""
         _chk(self.lib.fimhip_ocn_set_janjic(self.h, int(on)), "ocn_set_janjic")
@@ -650,8 +675,9 @@ class Ensemble:
 
 OCN_FIELDS = ["dp", "pres", "um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "massflx", "mssfx_avg",
               "uclin", "vclin", "utrop", "vtrop", "geop", "montg", "spcifv", "pr_edg", "geop_edg", "mont_edg",
-              "uvsq_edg", "spcv_edg"]
+              "uvsq_edg", "spcv_edg", "taux", "tauy", "ustarb"]
 OCN_CNUITY_FIELDS = OCN_FIELDS[:9]       # the arrays cnuity reads or writes
+OCN_EDGVAR_FIELDS = OCN_FIELDS[:21]      # with those of bclin_edgvar
 # bclin_edgvar's inputs and its outputs besides um/vm/un/vn_edg and dp_edg (dpm_edg here)
 OCN_EDG_IN = ["uclin", "vclin", "utrop", "vtrop", "geop", "montg", "spcifv"]
 OCN_EDG_OUT = ["pr_edg", "geop_edg", "mont_edg", "uvsq_edg", "spcv_edg"]
@@ -699,7 +725,7 @@ class Ocean:
         n, k = self.nip, self.kdm
         return {"dp": (2, n, k), "pres": (n, k + 1), "uclin": (2, n, k), "vclin": (2, n, k), "utrop": (n, 3),
                 "vtrop": (n, 3), "geop": (n, k + 1), "montg": (n, k), "spcifv": (n, k),
-                "pr_edg": (n, 6, k + 1)}.get(name, (n, 6, k))
+                "pr_edg": (n, 6, k + 1), "taux": (n,), "tauy": (n,), "ustarb": (n,)}.get(name, (n, 6, k))
 
     @staticmethod
     def _step_rc(rc: int, what: str):
@@ -729,6 +755,27 @@ class Ocean:
         """bclin_edgvar on the device-resident arrays: its edge values are the
         ones ``step`` (cnuity) then reads."""
         _chk(self.lib.fimhip_ocn_edgvar(self.h, leapm, leapn), "ocn_edgvar")
+
+    def momtum(self, nstep, leapm, leapn, utrop, vtrop, uclin, vclin, um_edg, vm_edg, geop_edg, mont_edg,
+               uvsq_edg, spcv_edg, taux, tauy, ubforc, vbforc, dp, pres, ustarb):
+        """momtum (hycom_momtum.F90:16); ubforc and vbforc are not written
+        (modesplit is off in the reference)."""
+        arrs = (utrop, vtrop, uclin, vclin, um_edg, vm_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg, taux, tauy,
+                ubforc, vbforc, dp, pres, ustarb)
+        names = ["utrop", "vtrop", "uclin", "vclin", "um_edg", "vm_edg", "geop_edg", "mont_edg", "uvsq_edg",
+                 "spcv_edg", "taux", "tauy", "taux", "tauy", "dp", "pres", "ustarb"]
+        for n, a in zip(names, arrs):
+            if a.shape != self._shape(n):
+                raise ValueError("%s: shape %s, expected %s" % (n, a.shape, self._shape(n)))
+        _chk(self.lib.fimhip_ocn_momtum_r8(self.h, nstep, leapm, leapn, *map(_p, arrs)), "ocn_momtum_r8")
+
+    def step_momtum(self, leapm: int, leapn: int):
+        """momtum on the device-resident arrays."""
+        _chk(self.lib.fimhip_ocn_momtum(self.h, leapm, leapn), "ocn_momtum")
+
+    def set_momtum(self, batrop: float, veldff: float = 0.1, smagcf: float = 0.3):
+        """batrop (hycom_init.F90:46) and the namelist's veldff, smagcf."""
+        _chk(self.lib.fimhip_ocn_set_momtum(self.h, float(batrop), float(veldff), float(smagcf)), "ocn_set_momtum")
 
     def set_janjic(self, on: bool):
         """janjic_ocn (fimnamelist.F90:246, default off)."""

@@ -169,6 +169,17 @@ module fimhip_bind
       real(c_double) :: pres(*), um_edg(*), vm_edg(*), un_edg(*), vn_edg(*), dp_edg(*), pr_edg(*), &
                         geop_edg(*), mont_edg(*), uvsq_edg(*), spcv_edg(*)
     end function
+    integer(c_int) function fimhip_ocn_momtum_r8(ocn, nstep, leapm, leapn, utrop, vtrop, uclin, vclin, um_edg, &
+        vm_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg, taux, tauy, ubforc, vbforc, dp, pres, ustarb) &
+        bind(C, name="fimhip_ocn_momtum_r8")
+      import; type(c_ptr), value :: ocn; integer(c_int), value :: nstep, leapm, leapn
+      real(c_double), intent(in) :: utrop(*), vtrop(*), um_edg(*), vm_edg(*), geop_edg(*), mont_edg(*), &
+                                    uvsq_edg(*), spcv_edg(*), taux(*), tauy(*), dp(*)
+      real(c_double) :: uclin(*), vclin(*), ubforc(*), vbforc(*), pres(*), ustarb(*)
+    end function
+    integer(c_int) function fimhip_ocn_set_momtum(ocn, batrop, veldff, smagcf) bind(C, name="fimhip_ocn_set_momtum")
+      import; type(c_ptr), value :: ocn; real(c_double), value :: batrop, veldff, smagcf
+    end function
     integer(c_int) function fimhip_ocn_set_janjic(ocn, on) bind(C, name="fimhip_ocn_set_janjic")
       import; type(c_ptr), value :: ocn; integer(c_int), value :: on
     end function

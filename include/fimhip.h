@@ -382,9 +382,13 @@ typedef struct fimhip_ocn fimhip_ocn;
 #define FIMHIP_OCN_MONT_EDG 18
 #define FIMHIP_OCN_UVSQ_EDG 19
 #define FIMHIP_OCN_SPCV_EDG 20
+/* the momentum equation's wind stress and bottom friction velocity (nip) */
+#define FIMHIP_OCN_TAUX 21
+#define FIMHIP_OCN_TAUY 22
+#define FIMHIP_OCN_USTARB 23
 /* hycom_init's constants (hycom_constants.F90 wet, dtleap; fimnamelist kdm,
  * thkdff, biharm) on the grid of the descriptor (nprox, prox, proxs, cs, sn,
- * sidevec_e, sideln, area; nvl, ntra, dt, sidevec_c, hfcoef, lat, lon,
+ * sidevec_e, sidevec_c, sideln, area, lat; nvl, ntra, dt, hfcoef, lon,
  * thetac and dpsig are not read) and rprox_ln (6,nip) of glvl.dat;
  * kdm <= 64, wet >= 0 */
 int fimhip_ocn_create(const fimhip_grid_desc *grid, const double *rprox_ln, int kdm, const int *wet,
@@ -414,6 +418,23 @@ int fimhip_ocn_edgvar_r8(fimhip_ocn *ocn, int nstep, int leapm, int leapn, const
  * and the downloads wait for it); its edge values are those
  * fimhip_ocn_cnuity reads (dp_edg is FIMHIP_OCN_DPM_EDG) */
 int fimhip_ocn_edgvar(fimhip_ocn *ocn, int leapm, int leapn);
+/* hycom_momtum.F90:16 momtum(nstep, leapm, leapn, utrop, vtrop, uclin, vclin,
+ * um_edg, vm_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg, taux, tauy, ubforc,
+ * vbforc, dp, pres, ustarb) with hycom_dissip's del4prep and dissip, on host
+ * arrays: uclin, vclin (both levels), pres and ustarb are updated; ubforc and
+ * vbforc are not written (modesplit is off, hycom_control.F90:5).  Needs
+ * fimhip_ocn_set_momtum first. */
+int fimhip_ocn_momtum_r8(fimhip_ocn *ocn, int nstep, int leapm, int leapn, const double *utrop,
+                         const double *vtrop, double *uclin, double *vclin, const double *um_edg,
+                         const double *vm_edg, const double *geop_edg, const double *mont_edg,
+                         const double *uvsq_edg, const double *spcv_edg, const double *taux, const double *tauy,
+                         double *ubforc, double *vbforc, const double *dp, double *pres, double *ustarb);
+/* the same on the device-resident arrays, asynchronous */
+int fimhip_ocn_momtum(fimhip_ocn *ocn, int leapm, int leapn);
+/* the barotropic time step batrop (hycom_init.F90:46, dt/subcycle_ocn) and
+ * the namelist's veldff and smagcf (fimnamelist.F90:253-254; 0.1 and 0.3
+ * until set) */
+int fimhip_ocn_set_momtum(fimhip_ocn *ocn, double batrop, double veldff, double smagcf);
 /* janjic_ocn (fimnamelist.F90:246, default off): the geopotential on the
  * edges from hydrostatic integration in the neighbouring columns */
 int fimhip_ocn_set_janjic(fimhip_ocn *ocn, int on);

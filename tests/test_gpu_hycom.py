@@ -150,7 +150,7 @@ def test_edgvar_dropin_matches_checker_bitwise_g3(oracle_lib, janjic):
     o = fimhip.Ocean(g, 32, case["wet"], case["dtleap"])
     try:
         o.set_janjic(janjic)
-        a = {n: np.ascontiguousarray(case[n], np.float64).copy() for n in fimhip.OCN_FIELDS}
+        a = {n: np.ascontiguousarray(case[n], np.float64).copy() for n in fimhip.OCN_EDGVAR_FIELDS}
         for st in (1, 2):
             leapm = 1 + st % 2
             h.edgvar(g, leapm, 3 - leapm, janjic)
@@ -175,7 +175,7 @@ def test_edgvar_then_cnuity_resident_matches_checker(oracle_lib, level, kdm, jan
     o = fimhip.Ocean(g, kdm, case["wet"], case["dtleap"])
     try:
         o.set_janjic(janjic)
-        for n in fimhip.OCN_FIELDS:
+        for n in fimhip.OCN_EDGVAR_FIELDS:
             o.upload(n, case[n])
         for st in (1, 2, 3):
             leapm = 1 + st % 2
@@ -201,7 +201,7 @@ def test_edgvar_then_cnuity_g7_full_size(oracle_lib):
     h = oracle_lib.HycomCnuity(g, case)
     o = fimhip.Ocean(g, 32, case["wet"], case["dtleap"])
     try:
-        for n in fimhip.OCN_FIELDS:
+        for n in fimhip.OCN_EDGVAR_FIELDS:
             o.upload(n, case[n])
         for st in (1, 2):
             leapm = 1 + st % 2
@@ -214,6 +214,106 @@ def test_edgvar_then_cnuity_g7_full_size(oracle_lib):
         for n in ("um_edg", "dpm_edg"):
             assert _same(o.download(n), h.a[n]), n
         for n in h.EDG_OUT:
+            assert _same(o.download(n), h.e[n]), n
+    finally:
+        o.close()
+
+
+MOM_OUT = ("uclin", "vclin", "pres", "ustarb")
+
+
+def _mom_setup(O, g, case, janjic_edges=True):
+    """The checker instance with the edge values momtum reads formed by its
+    bclin_edgvar (janjic_ocn on, so geop_edg is set) and the case's pres."""
+    h = O.HycomCnuity(g, case)
+    h.edgvar(g, 2, 1, janjic_edges)
+    h.a["pres"][:] = case["pres"]
+    return h
+
+
+@pytest.mark.parametrize("janjic", [False, True])
+def test_momtum_dropin_matches_checker_bitwise_g3(oracle_lib, janjic):
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, ocean
+    print("case: G3 x 32, momtum drop-in, janjic %s" % janjic, flush=True)
+    g = _grid(3)
+    case = ocean.hycom_momtum_case(g, kdm=32)
+    h = _mom_setup(oracle_lib, g, case)
+    a = {n: h.a[n].copy() for n in ("um_edg", "vm_edg", "dp", "pres")}
+    a.update({n: h.e[n].copy() for n in ("utrop", "vtrop", "uclin", "vclin", "geop_edg", "mont_edg", "uvsq_edg",
+                                          "spcv_edg")})
+    a.update({n: np.ascontiguousarray(case[n], np.float64).copy() for n in ("taux", "tauy", "ustarb")})
+    ub, vb = np.full(g.nip, 5.0), np.full(g.nip, 5.0)
+    o = fimhip.Ocean(g, 32, case["wet"], case["dtleap"])
+    try:
+        o.set_momtum(case["batrop"], case["veldff"], case["smagcf"])
+        o.set_janjic(janjic)
+        h.momtum(g, 2, 1, janjic)
+        o.momtum(1, 2, 1, a["utrop"], a["vtrop"], a["uclin"], a["vclin"], a["um_edg"], a["vm_edg"], a["geop_edg"],
+                 a["mont_edg"], a["uvsq_edg"], a["spcv_edg"], a["taux"], a["tauy"], ub, vb, a["dp"], a["pres"],
+                 a["ustarb"])
+        for n in MOM_OUT:
+            want = h.a[n] if n == "pres" else h.e[n]
+            assert _same(a[n], want), n
+        assert (ub == 5.0).all() and (vb == 5.0).all()
+    finally:
+        o.close()
+
+
+@pytest.mark.parametrize("level,kdm,janjic", [(4, 48, True), (5, 32, False)])
+def test_ocean_step_resident_matches_checker(oracle_lib, level, kdm, janjic):
+    """bclin_edgvar, cnuity and momtum each step, in hycom_run's order, on the
+    device-resident arrays: 3 leapfrog steps from step 1, the velocities that
+    momtum writes feeding the next step's edge values."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, ocean
+    print("case: G%d x %d, edgvar + cnuity + momtum resident, janjic %s" % (level, kdm, janjic), flush=True)
+    g = _grid(level)
+    case = ocean.hycom_momtum_case(g, kdm=kdm, seed=41 + level)
+    h = oracle_lib.HycomCnuity(g, case)
+    o = fimhip.Ocean(g, kdm, case["wet"], case["dtleap"])
+    try:
+        o.set_janjic(janjic)
+        o.set_momtum(case["batrop"], case["veldff"], case["smagcf"])
+        for n in fimhip.OCN_FIELDS:
+            o.upload(n, case[n])
+        for st in (1, 2, 3):
+            leapm = 1 + st % 2
+            h.edgvar(g, leapm, 3 - leapm, janjic)
+            assert h.step(st, leapm, 3 - leapm) == 0
+            h.momtum(g, leapm, 3 - leapm, janjic)
+            o.step_edgvar(leapm, 3 - leapm)
+            o.step(st, leapm, 3 - leapm)
+            o.step_momtum(leapm, 3 - leapm)
+            for n in ARRAYS:
+                assert _same(o.download(n), h.a[n]), (st, n)
+            for n in ("uclin", "vclin", "ustarb"):
+                assert _same(o.download(n), h.e[n]), (st, n)
+            assert o.mass == h.mass, st
+    finally:
+        o.close()
+
+
+def test_ocean_step_g7_full_size(oracle_lib):
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, ocean
+    print("case: G7 x 32, edgvar + cnuity + momtum resident, 2 steps", flush=True)
+    g = _grid(7)
+    case = ocean.hycom_momtum_case(g, kdm=32)
+    h = oracle_lib.HycomCnuity(g, case)
+    o = fimhip.Ocean(g, 32, case["wet"], case["dtleap"])
+    try:
+        o.set_momtum(case["batrop"], case["veldff"], case["smagcf"])
+        for n in fimhip.OCN_FIELDS:
+            o.upload(n, case[n])
+        for st in (1, 2):
+            leapm = 1 + st % 2
+            h.edgvar(g, leapm, 3 - leapm, False)
+            assert h.step(st, leapm, 3 - leapm) == 0
+            h.momtum(g, leapm, 3 - leapm, False)
+            o.step_edgvar(leapm, 3 - leapm)
+            o.step(st, leapm, 3 - leapm)
+            o.step_momtum(leapm, 3 - leapm)
+        for n in ARRAYS:
+            assert _same(o.download(n), h.a[n]), n
+        for n in ("uclin", "vclin", "ustarb"):
             assert _same(o.download(n), h.e[n]), n
     finally:
         o.close()
