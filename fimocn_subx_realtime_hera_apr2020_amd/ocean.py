@@ -252,3 +252,57 @@ def hycom_momtum_case(grid, kdm: int = 32, seed: int = EDGVAR_CASE_SEED, dtleap:
     case.update(taux=0.1 * rng.normal(size=nip), tauy=0.1 * rng.normal(size=nip), ustarb=np.zeros(nip),
                 batrop=0.5 * dtleap, veldff=0.1, smagcf=0.3)
     return case
+
+
+def hycom_transport_case(grid, kdm: int = 32, numtr: int = 1, seed: int = EDGVAR_CASE_SEED,
+                         batrop: float = 900.0) -> dict:
+    """The inputs of HYCOM's tracer transport (hycom_transp3d.F90 transp0/1/2)
+    on top of hycom_cnuity_case's grid, mask and thickness: the time-integrated
+    mass flux cumuflx (nip, 6, kdm), each edge's two sides equal and opposite
+    (U(-1,1) x 0.2 x the edge-mean thickness x the mean cell area, so layers
+    gain or lose up to tens of percent and the vertical displacements reach
+    several layers), 0 on edges to land, -1e33 in land cells; a mass flux
+    massflx for transp1 built the same way at 1e-3 of that size; the
+    thickness at the start (dpinit, the first leapfrog level) and end
+    (dpfinl, x (1 + 0.05 U[-1,1])) of the interval; temperature (deg C, a
+    smooth surface field of 10-28 decaying towards 2 with depth, e-folding
+    800 m), salinity U[34, 36] and numtr passive tracers U[0, 1] (kdm per
+    column); batrop and temdff = 0.02 (fimnamelist.F90:255)."""
+    case = hycom_cnuity_case(grid, kdm=kdm, seed=seed)
+    rng = np.random.default_rng(seed + 3)
+    lat = np.asarray(grid.lat, np.float64)
+    lon = np.asarray(grid.lon, np.float64)
+    nip = lat.shape[0]
+    wet = case["wet"] > 0
+    prox = grid.dyn_prox().astype(np.int64) - 1
+    proxs = grid.proxs.astype(np.int64) - 1
+    area = np.asarray(grid.area, np.float64)
+    dp0 = case["dp"][0]
+
+    def pair_flux(scale):
+        f = np.zeros((nip, 6, kdm))
+        for i in np.flatnonzero(wet):
+            for e in range(int(grid.nprox[i])):
+                ix = prox[i, e]
+                if wet[ix] and i < ix:
+                    v = rng.uniform(-1.0, 1.0, kdm) * scale * 0.5 * (dp0[i] + dp0[ix]) * 0.5 * (area[i] + area[ix])
+                    f[i, e] = v
+                    f[ix, proxs[i, e]] = -v
+        f[~wet] = -1.0e33
+        return f
+
+    cumuflx = pair_flux(0.2)
+    massflx = pair_flux(2.0e-4)
+    dpinit = dp0.copy()
+    dpfinl = dp0 * (1.0 + 0.05 * rng.uniform(-1.0, 1.0, dp0.shape))
+    depth = case["pres"] / ONEM
+    zmid = 0.5 * (depth[:, :-1] + depth[:, 1:])
+    sst = 10.0 + 18.0 * (smooth_field(lat, lon, seed + 7) - 0.0)
+    sst = np.clip(sst, 10.0, 28.0)
+    temp = 2.0 + (sst[:, None] - 2.0) * np.exp(-zmid / 800.0) + 0.1 * rng.normal(size=(nip, kdm))
+    saln = rng.uniform(34.0, 36.0, (nip, kdm))
+    passv = rng.uniform(0.0, 1.0, (numtr, nip, kdm))
+    case.update(cumuflx=cumuflx, massflx_tr=massflx, dpinit=dpinit, dpfinl=dpfinl,
+                temp=np.ascontiguousarray(temp), saln=saln, passv=np.ascontiguousarray(passv), numtr=numtr,
+                batrop=batrop, temdff=0.02)
+    return case

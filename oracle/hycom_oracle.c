@@ -1,8 +1,10 @@
 /* hycom_oracle.c - CPU restatement, in plain C and FP64, of the HYCOM ocean
  * continuity equation (FIMsrc/fim/horizontal/hycom_cnuity.F90, subroutine
  * cnuity), of its edge interpolation (hycom_edgvar.F90, bclin_edgvar with
- * uvedge and blokf) and of its momentum equation (hycom_momtum.F90 momtum
- * with hycom_dissip.F90's del4prep and dissip).  TEST INFRASTRUCTURE ONLY: the checker the HIP kernels
+ * uvedge and blokf), of its momentum equation (hycom_momtum.F90 momtum
+ * with hycom_dissip.F90's del4prep and dissip) and of its tracer transport
+ * (hycom_transp3d.F90 transp0/1/2 with hycom_fct3d.F90's fct3d and
+ * hycom_dffusn.F90's dffusn_lyr).  TEST INFRASTRUCTURE ONLY: the checker the HIP kernels
  * are compared with; the product never links it.
  *
  * Every loop follows the reference operation for operation, in the
@@ -757,4 +759,321 @@ void fo_hycom_momtum(const fo_hyc_grid *g, const fo_hyc_mom *m, int leapm, int l
   free(gradx); free(grady); free(utotn); free(vtotn); free(worku); free(workv); free(d4u); free(d4v);
   free(ufx); free(vfx); free(dpres); free(drag); free(pgfx); free(pgfy); free(avgu); free(avgv);
   free(dudx); free(dudy); free(dvdx); free(dvdy); free(uold); free(vold);
+}
+
+/* ------------------------------------------------------------------------
+ * hycom_transp3d.F90, hycom_fct3d.F90, hycom_dffusn.F90 */
+static const int BCLIN_FRQ = 30;              /* hycom_control.F90:9 */
+static const double TOFFSET = 10.;            /* hycom_transp3d.F90: keeps temperature positive */
+
+/* hycom_transp3d.F90 transp0 */
+void fo_hycom_transp0(const fo_hyc_grid *g, int leapn, const double *dp, double *dpinit, double *cumuflx) {
+  const int nip = g->nip, kdm = g->kdm;
+  const double *dpn = dp + (size_t)(leapn - 1) * nip * kdm;
+  for (int i = 0; i < nip; i++) {
+    for (int e = 0; e < NPP; e++)
+      for (int k = 0; k < kdm; k++) E(cumuflx, i, e, k) = LAND_SPVAL;
+    if (!WET(i)) continue;
+    for (int k = 0; k < kdm; k++) {
+      for (int e = 0; e < NPP; e++) E(cumuflx, i, e, k) = 0.;
+      C(dpinit, i, k) = C(dpn, i, k);
+    }
+  }
+}
+
+/* hycom_transp3d.F90 transp1 */
+void fo_hycom_transp1(const fo_hyc_grid *g, double batrop, const double *massflx, double *cumuflx) {
+  const int nip = g->nip, kdm = g->kdm;
+  for (int i = 0; i < nip; i++) {
+    if (!WET(i)) continue;
+    for (int e = 0; e < g->nprox[i]; e++)
+      for (int k = 0; k < kdm; k++) E(cumuflx, i, e, k) = E(cumuflx, i, e, k) + E(massflx, i, e, k) * 2. * batrop;
+  }
+}
+
+/* hycom_fct3d.F90 fct3d(nstep, fld3d, numfld, u, w, area, rarea, thko, thkn,
+ * diagno) with recovr = .true.: each field advected by the time-integrated
+ * mass fluxes u (horizontal) and w (vertical, positive up), low order plus
+ * limited antidiffusive fluxes, then shifted so that its mass-weighted mean
+ * over the ocean is what it was (the sums serial over cells, as written) */
+static void hyc_fct3d(const fo_hyc_grid *g, double *fld3d, int numfld, const double *u, const double *w,
+                      const double *thko, const double *thkn) {
+  const int nip = g->nip, kdm = g->kdm;
+  const size_t cz = (size_t)nip * kdm;
+  const double athird = 1. / 3., epsil = 1.e-11, onemu = 1.e-6;
+  double *fld = malloc(cz * sizeof(double)), *vertfx = malloc(cz * sizeof(double));
+  double *vertdv = malloc(cz * sizeof(double)), *fmx = malloc(cz * sizeof(double));
+  double *fmn = malloc(cz * sizeof(double)), *flp = malloc(cz * sizeof(double));
+  double *fln = malloc(cz * sizeof(double)), *flx = malloc(cz * NPP * sizeof(double));
+  double *uan = malloc(cz * NPP * sizeof(double));
+  double *a = malloc((size_t)kdm * sizeof(double)), *b = malloc((size_t)kdm * sizeof(double));
+  double *c = malloc((size_t)kdm * sizeof(double));
+  for (int type = 0; type < numfld; type++) {
+    double *f3 = fld3d + (size_t)type * cz;
+    for (int i = 0; i < nip; i++)
+      for (int k = 0; k < kdm; k++) C(fld, i, k) = WET(i) ? C(f3, i, k) : LAND_SPVAL;
+    /* piecewise-parabolic fits in each layer and the vertical fluxes they give
+     * (hycom_fct3d.F90:131-228) */
+    for (int i = 0; i < nip; i++) {
+      if (!WET(i)) continue;
+      a[0] = C(fld, i, 0);
+      b[0] = c[0] = 0.;
+      a[kdm - 1] = C(fld, i, kdm - 1);
+      b[kdm - 1] = c[kdm - 1] = 0.;
+      for (int k = 1; k < kdm - 1; k++) {
+        const double dxlft = dmax2(epsil, C(thko, i, k - 1)), dxmid = dmax2(epsil, C(thko, i, k));
+        const double dxrgt = dmax2(epsil, C(thko, i, k + 1));
+        const double fk = C(fld, i, k);
+        const double yl = (dxlft * fk + dxmid * C(fld, i, k - 1)) / (dxlft + dxmid);
+        const double yr = (dxrgt * fk + dxmid * C(fld, i, k + 1)) / (dxrgt + dxmid);
+        a[k] = 1.5 * fk - .25 * (yl + yr);
+        b[k] = yr - yl;
+        c[k] = 6. * (.5 * (yl + yr) - fk);
+        if (fabs(yr - yl) < 6. * fabs(.5 * (yl + yr) - fk)) {
+          if (fabs(yr - yl) > 2. * fabs(.5 * (yl + yr) - fk)) {
+            if ((yr - yl) * (.5 * (yl + yr) - fk) > 0.) {
+              a[k] = .25 * (3. * fk + yl);
+              c[k] = 3. * (fk - yl);
+              b[k] = c[k];
+            } else {
+              a[k] = .25 * (3. * fk + yr);
+              c[k] = 3. * (fk - yr);
+              b[k] = -c[k];
+            }
+          } else {                 /* -1/6 < x < +1/6 */
+            a[k] = fk;
+            b[k] = c[k] = 0.;
+          }
+        }
+      }
+      double amount = 0.;          /* (kept from the interface above where w = 0, as written) */
+      for (int k = 0; k < kdm - 1; k++) {
+        double slab = onemu;
+        const double wk = C(w, i, k);
+        if (wk < 0.) {             /* the interface moves down: take from the layers below */
+          amount = slab * C(fld, i, k + 1);
+          for (int kp = k + 1; kp < kdm; kp++) {
+            if (slab >= -wk) break;
+            if (C(thko, i, kp) > 0.) {
+              const double dslab = dmin2(slab + C(thko, i, kp), -wk) - dmin2(slab, -wk);
+              const double dx = dslab / C(thko, i, kp);
+              const double fcdx = a[kp] + b[kp] * .5 * (dx - 1.) + c[kp] * (.25 - dx * (.5 - dx * athird));
+              amount = amount + fcdx * dslab;
+              slab = slab + dslab;
+            }
+          }
+        } else if (wk > 0.) {      /* the interface moves up: take from the layers above */
+          amount = slab * C(fld, i, k);
+          /* the reference's loop tests its end after each layer (goto 25 while
+           * kp > 2, 1-based): it takes layer k first, whatever k is, and goes on
+           * up to the second layer */
+          int kp = k;
+          for (;;) {
+            if (slab >= wk) break;
+            if (C(thko, i, kp) > 0.) {
+              const double dslab = dmin2(slab + C(thko, i, kp), wk) - dmin2(slab, wk);
+              const double dx = dslab / C(thko, i, kp);
+              const double fcdx = a[kp] + b[kp] * .5 * (1. - dx) + c[kp] * (.25 - dx * (.5 - dx * athird));
+              amount = amount + fcdx * dslab;
+              slab = slab + dslab;
+            }
+            if (kp > 1) kp--;
+            else break;
+          }
+        }
+        C(vertfx, i, k) = wk * amount / slab;
+      }
+      C(vertfx, i, kdm - 1) = 0.;  /* no flux through the bottom */
+      C(vertdv, i, 0) = C(vertfx, i, 0);
+      for (int k = 1; k < kdm; k++) C(vertdv, i, k) = C(vertfx, i, k) - C(vertfx, i, k - 1);
+    }
+    /* the mass-weighted mean before (recovr) */
+    double bfore = 0., wtold = 0.;
+    for (int i = 0; i < nip; i++) {
+      if (!WET(i)) continue;
+      double col1 = 0., col2 = 0.;
+      for (int k = 0; k < kdm; k++) {
+        col1 = col1 + C(thko, i, k) * g->area[i];
+        col2 = col2 + C(fld, i, k) * C(thko, i, k) * g->area[i];
+      }
+      wtold = wtold + col1;
+      bfore = bfore + col2;
+    }
+    /* low-order and antidiffusive fluxes, local bounds (hycom_fct3d.F90:249-290) */
+    for (int i = 0; i < nip; i++) {
+      for (int e = 0; e < NPP; e++)
+        for (int k = 0; k < kdm; k++) E(flx, i, e, k) = E(uan, i, e, k) = LAND_SPVAL;
+      if (!WET(i)) continue;
+      for (int k = 0; k < kdm; k++) {
+        C(fmx, i, k) = C(fmn, i, k) = C(fld, i, k);
+        for (int e = 0; e < g->nprox[i]; e++) {
+          E(flx, i, e, k) = E(uan, i, e, k) = 0.;
+          const int ix = g->prox[i * NPP + e];
+          if (!WET(ix)) continue;
+          const int edx = g->proxs[i * NPP + e];
+          E(flx, i, e, k) = 0.5 * ((E(u, i, e, k) + fabs(E(u, i, e, k))) * C(fld, i, k) -
+                                   (E(u, ix, edx, k) + fabs(E(u, ix, edx, k))) * C(fld, ix, k));
+          const double q = .5 * (C(fld, i, k) + C(fld, ix, k));
+          E(uan, i, e, k) = q * E(u, i, e, k) - E(flx, i, e, k);
+          C(fmx, i, k) = dmax2(C(fmx, i, k), C(fld, ix, k));
+          C(fmn, i, k) = dmin2(C(fmn, i, k), C(fld, ix, k));
+        }
+        if (k < kdm - 1 && C(w, i, k) < 0.) {
+          C(fmx, i, k) = dmax2(C(fmx, i, k), C(vertfx, i, k) / C(w, i, k));
+          C(fmn, i, k) = dmin2(C(fmn, i, k), C(vertfx, i, k) / C(w, i, k));
+        }
+        if (k > 0 && C(w, i, k - 1) > 0.) {
+          C(fmx, i, k) = dmax2(C(fmx, i, k), C(vertfx, i, k - 1) / C(w, i, k - 1));
+          C(fmn, i, k) = dmin2(C(fmn, i, k), C(vertfx, i, k - 1) / C(w, i, k - 1));
+        }
+      }
+    }
+    /* the low-order update (292-311) */
+    for (int i = 0; i < nip; i++) {
+      if (!WET(i)) continue;
+      for (int k = 0; k < kdm; k++) {
+        double hordiv = 0.;
+        for (int e = 0; e < g->nprox[i]; e++) hordiv = hordiv + E(flx, i, e, k);
+        hordiv = hordiv * g->rarea[i];
+        const double unbound = C(fld, i, k) * C(thko, i, k) - hordiv - C(vertdv, i, k);
+        const double bounded =
+            dmax2(dmax2(0., C(fmn, i, k) * C(thkn, i, k)), dmin2(unbound, C(fmx, i, k) * C(thkn, i, k)));
+        C(fld, i, k) = (C(fld, i, k) * onemu + bounded) / (onemu + C(thkn, i, k));
+      }
+    }
+    /* limiter ratios with the bounds widened by the low-order neighbours (313-334) */
+    for (int i = 0; i < nip; i++) {
+      if (!WET(i)) continue;
+      for (int k = 0; k < kdm; k++) {
+        double totin = 0., totou = 0.;
+        for (int e = 0; e < g->nprox[i]; e++) {
+          totin = totin - dmin2(0., E(uan, i, e, k));
+          totou = totou + dmax2(0., E(uan, i, e, k));
+          const int ix = g->prox[i * NPP + e];
+          if (WET(ix)) {
+            C(fmx, i, k) = dmax2(C(fmx, i, k), C(fld, ix, k));
+            C(fmn, i, k) = dmin2(C(fmn, i, k), C(fld, ix, k));
+          }
+        }
+        C(flp, i, k) = (C(fmx, i, k) - C(fld, i, k)) * C(thkn, i, k) / (totin + epsil) * g->rarea[i];
+        C(fln, i, k) = (C(fld, i, k) - C(fmn, i, k)) * C(thkn, i, k) / (totou + epsil) * g->rarea[i];
+      }
+    }
+    /* the limited antidiffusive update (336-371) */
+    for (int i = 0; i < nip; i++) {
+      if (!WET(i)) continue;
+      for (int k = 0; k < kdm; k++) {
+        double hordiv = 0.;
+        for (int e = 0; e < g->nprox[i]; e++) {
+          const int ix = g->prox[i * NPP + e];
+          if (!WET(ix)) continue;
+          const double clip = E(uan, i, e, k) >= 0. ? dmin2(dmin2(1., C(fln, i, k)), C(flp, ix, k))
+                                                    : dmin2(dmin2(1., C(flp, i, k)), C(fln, ix, k));
+          E(flx, i, e, k) = E(uan, i, e, k) * clip;
+          hordiv = hordiv + E(flx, i, e, k);
+        }
+        hordiv = hordiv * g->rarea[i];
+        const double unbound = C(fld, i, k) * C(thkn, i, k) - hordiv;
+        const double bounded =
+            dmax2(dmax2(0., C(fmn, i, k) * C(thkn, i, k)), dmin2(unbound, C(fmx, i, k) * C(thkn, i, k)));
+        C(fld, i, k) = (C(fld, i, k) * onemu + bounded) / (onemu + C(thkn, i, k));
+      }
+    }
+    /* global conservation (recovr, 373-405): the shift of the mean */
+    double after = 0., wtnew = 0.;
+    for (int i = 0; i < nip; i++) {
+      if (!WET(i)) continue;
+      double col1 = 0., col2 = 0.;
+      for (int k = 0; k < kdm; k++) {
+        col1 = col1 + C(thkn, i, k) * g->area[i];
+        col2 = col2 + C(fld, i, k) * C(thkn, i, k) * g->area[i];
+      }
+      wtnew = wtnew + col1;
+      after = after + col2;
+    }
+    bfore = bfore / wtold;
+    after = after / wtnew;
+    const double q = bfore - after;
+    for (int i = 0; i < nip; i++) {
+      if (!WET(i)) continue;
+      for (int k = 0; k < kdm; k++) C(fld, i, k) = C(fld, i, k) + q;
+    }
+    for (int i = 0; i < nip; i++)
+      if (WET(i))
+        for (int k = 0; k < kdm; k++) C(f3, i, k) = C(fld, i, k);
+  }
+  free(fld); free(vertfx); free(vertdv); free(fmx); free(fmn); free(flp); free(fln); free(flx); free(uan);
+  free(a); free(b); free(c);
+}
+
+/* hycom_dffusn.F90 dffusn_lyr(fld, delp, kdm, 1, kdm, dffveldt): thickness-
+ * weighted Laplacian smoothing through the ocean edges */
+static void hyc_dffusn_lyr(const fo_hyc_grid *g, double *fld, const double *delp, double dffveldt) {
+  const int nip = g->nip, kdm = g->kdm;
+  const double thshld = 1.e-11;
+  double *flxdv = calloc((size_t)nip * kdm, sizeof(double));
+  for (int i = 0; i < nip; i++) {
+    if (!WET(i)) continue;
+    for (int e = 0; e < g->nprox[i]; e++) {
+      const int ix = g->prox[i * NPP + e];
+      if (!WET(ix)) continue;
+      for (int k = 0; k < kdm; k++) {
+        const double a = dmax2(C(delp, i, k), thshld), b = dmax2(C(delp, ix, k), thshld);
+        C(flxdv, i, k) = C(flxdv, i, k) + (C(fld, i, k) - C(fld, ix, k)) * g->sideln[i * NPP + e] * 2. * (a * b / (a + b));
+      }
+    }
+  }
+  for (int i = 0; i < nip; i++) {
+    if (!WET(i)) continue;
+    for (int k = 0; k < kdm; k++) {
+      const double factor = -dffveldt * g->rarea[i] / dmax2(C(delp, i, k), thshld);
+      C(fld, i, k) = C(fld, i, k) + C(flxdv, i, k) * factor;
+    }
+  }
+  free(flxdv);
+}
+
+/* hycom_transp3d.F90 transp2 */
+void fo_hycom_transp2(const fo_hyc_grid *g, int numtr, double batrop, double temdff, double *temp, double *saln,
+                      double *passv, const double *cumuflx, const double *dpinit, const double *dpfinl) {
+  const int nip = g->nip, kdm = g->kdm;
+  const size_t cz = (size_t)nip * kdm;
+  const int numfld = 2 + numtr;
+  double *vertfx = malloc(cz * sizeof(double)), *all = malloc(cz * numfld * sizeof(double));
+  /* the vertical mass flux the time-integrated horizontal fluxes imply */
+  for (int i = 0; i < nip; i++) {
+    for (int k = 0; k < kdm; k++) C(vertfx, i, k) = LAND_SPVAL;
+    if (!WET(i)) continue;
+    for (int k = 0; k < kdm; k++) {
+      double hordiv = 0.;
+      for (int e = 0; e < g->nprox[i]; e++) hordiv = hordiv + E(cumuflx, i, e, k);
+      const double vertdv = (C(dpinit, i, k) - C(dpfinl, i, k)) - hordiv * g->rarea[i];
+      C(vertfx, i, k) = k == 0 ? vertdv : C(vertfx, i, k - 1) + vertdv;
+    }
+  }
+  for (int t = 0; t < numfld; t++) {
+    const double *src = t == 0 ? temp : t == 1 ? saln : passv + (size_t)(t - 2) * cz;
+    double *dst = all + (size_t)t * cz;
+    for (int i = 0; i < nip; i++)
+      for (int k = 0; k < kdm; k++) C(dst, i, k) = WET(i) ? C(src, i, k) : LAND_SPVAL;
+  }
+  for (int i = 0; i < nip; i++)
+    if (WET(i))
+      for (int k = 0; k < kdm; k++) C(all, i, k) = C(all, i, k) + TOFFSET;
+  hyc_fct3d(g, all, numfld, cumuflx, vertfx, dpinit, dpfinl);
+  for (int i = 0; i < nip; i++)
+    if (WET(i))
+      for (int k = 0; k < kdm; k++) C(all, i, k) = C(all, i, k) - TOFFSET;
+  for (int t = 0; t < numfld; t++) {
+    double *dst = t == 0 ? temp : t == 1 ? saln : passv + (size_t)(t - 2) * cz;
+    const double *src = all + (size_t)t * cz;
+    for (int i = 0; i < nip; i++)
+      if (WET(i))
+        for (int k = 0; k < kdm; k++) C(dst, i, k) = C(src, i, k);
+  }
+  if (temdff > 0.) {
+    hyc_dffusn_lyr(g, temp, dpfinl, temdff * batrop * BCLIN_FRQ);
+    hyc_dffusn_lyr(g, saln, dpfinl, temdff * batrop * BCLIN_FRQ);
+  }
+  free(vertfx); free(all);
 }

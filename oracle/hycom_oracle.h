@@ -57,3 +57,16 @@ void fo_hycom_momtum(const fo_hyc_grid *g, const fo_hyc_mom *m, int leapm, int l
                      const double *geop_edg, const double *mont_edg, const double *uvsq_edg,
                      const double *spcv_edg, const double *taux, const double *tauy, const double *dp,
                      double *pres, double *ustarb);
+
+/* hycom_transp3d.F90: transp0 (start of a transport interval), transp1 (the
+ * mass flux added to its time integral, cumuflx += massflx*2*batrop) and
+ * transp2 (the tracers moved by hycom_fct3d.F90's fct3d with the integral,
+ * then temperature and salinity diffused by hycom_dffusn.F90's dffusn_lyr
+ * when temdff > 0).  temp, saln (kdm,nip); passv (kdm,nip,numtr) as
+ * [(t*nip + i)*kdm + k]; cumuflx, massflx edge layout; dpinit, dpfinl
+ * (kdm,nip).  Reads nip, kdm, prox, proxs, nprox, wet, area, rarea and
+ * sideln of g. */
+void fo_hycom_transp0(const fo_hyc_grid *g, int leapn, const double *dp, double *dpinit, double *cumuflx);
+void fo_hycom_transp1(const fo_hyc_grid *g, double batrop, const double *massflx, double *cumuflx);
+void fo_hycom_transp2(const fo_hyc_grid *g, int numtr, double batrop, double temdff, double *temp, double *saln,
+                      double *passv, const double *cumuflx, const double *dpinit, const double *dpfinl);

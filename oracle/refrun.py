@@ -203,6 +203,37 @@ def run_hycedgref(glvl_path, case: dict, *, nstep: int = 1, leapm: int = 2, janj
     return rec
 
 
+HYCT_MAGIC = 1213809490
+
+
+def run_hyctrref(glvl_path, case: dict, *, leapn: int = 1, workdir=None) -> dict:
+    """Run the reference tracer transport (oracle/ref/hyctrref.F90) on the
+    inputs of ocean.hycom_transport_case: transp0 on copies (records
+    cumuflx0, dpinit0), then transp1 and transp2 (cumuflx1, temp, saln,
+    passv)."""
+    workdir = workdir or os.path.join(REF, "runs")
+    os.makedirs(workdir, exist_ok=True)
+    kdm = case["kdm"]
+    nip = case["wet"].shape[0]
+    fin = os.path.join(workdir, "hyct_state_%d.bin" % os.getpid())
+    fout = os.path.join(workdir, "hyct_out_%d.bin" % os.getpid())
+    with open(fin, "wb") as f:
+        f.write(struct.pack("<8i", HYCT_MAGIC, kdm, nip, leapn, case["numtr"], 0, 0, 0))
+        f.write(struct.pack("<2d", case["batrop"], case["temdff"]))
+        f.write(np.ascontiguousarray(case["wet"], dtype="<i4").tobytes())
+        for n in ("dp", "massflx_tr", "cumuflx", "dpinit", "dpfinl", "temp", "saln", "passv"):
+            f.write(np.ascontiguousarray(case[n], dtype="<f8").tobytes())
+    r = subprocess.run([os.path.join(BIN, "hyctrref"), glvl_path, fin, fout], cwd=workdir, capture_output=True,
+                       text=True, preexec_fn=_big_stack)
+    if r.returncode != 0:
+        raise RuntimeError("hyctrref failed (%d): %s" % (r.returncode, (r.stdout + r.stderr)[-3000:]))
+    rec = read_records(fout)
+    rec["_stdout"] = r.stdout
+    os.remove(fin)
+    os.remove(fout)
+    return rec
+
+
 HYCM_MAGIC = 1213809485
 
 

@@ -37,6 +37,10 @@ Fixtures (data only):
                         on its edge values, step 1, on the seeded inputs of
                         ocean.hycom_edgvar_case at G2 and G3 x 32 layers,
                         with and without janjic_ocn
+  hycom_transport_ref.json  SHA-256 of the outputs of the reference HYCOM tracer
+                        transport (hycom_transp3d.F90 transp0/1/2 with
+                        hycom_fct3d.F90 and hycom_dffusn.F90, FP64, hyctrref) on
+                        ocean.hycom_transport_case at G2 and G3 x 32 layers
   hycom_momtum_ref.json SHA-256 of the outputs of the reference HYCOM momentum
                         equation (hycom_momtum.F90 momtum with hycom_dissip.F90,
                         FP64, hycmomref): uclin, vclin, pres, ustarb at step 1
@@ -267,6 +271,22 @@ def hycom_momtum_reference():
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def hycom_transport_reference():
+    """Digests of the reference tracer transport (transp0 on copies, then
+    transp1 and transp2) at G2 and G3 x 32 layers, one passive tracer, on
+    ocean.hycom_transport_case."""
+    from fimocn_subx_realtime_hera_apr2020_amd import ocean
+    out = {"kdm": 32, "numtr": 1, "leapn": 1, "runs": {}}
+    for lv in (2, 3):
+        path = os.path.join(HERE, "glvl_G%d.dat" % lv)
+        g = glvl.read_glvl(path)
+        case = ocean.hycom_transport_case(g, kdm=32)
+        r = refrun.run_hyctrref(path, case, leapn=1)
+        out["runs"]["G%d" % lv] = {k: sha(v) for k, v in r.items() if not k.startswith("_")}
+    with open(os.path.join(HERE, "hycom_transport_ref.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     main()
     schemes()
@@ -275,3 +295,4 @@ if __name__ == "__main__":
     hycom_reference()
     hycom_edgvar_reference()
     hycom_momtum_reference()
+    hycom_transport_reference()

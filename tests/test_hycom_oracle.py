@@ -230,3 +230,63 @@ def test_hycom_momtum_properties(oracle_lib):
         slab = np.maximum(0.1 * ocean.ONEM, pres[:, k + 1] - pres[:, kdm] + 10 * ocean.ONEM)
         mixed += int(np.sum((slab > np.minimum(dpn[:, k], slab)) & wet))
     assert mixed > 100
+
+
+TR_RECORDS = ("cumuflx0", "dpinit0", "cumuflx1", "temp", "saln", "passv")
+
+
+def _transport(O, g, seed=None):
+    case = ocean.hycom_transport_case(g, kdm=32) if seed is None else ocean.hycom_transport_case(g, kdm=32, seed=seed)
+    h0 = O.HycomCnuity(g, case)
+    h0.transport(leapn=1, transp0=True)
+    h = O.HycomCnuity(g, case)
+    h.transport()
+    return case, h, {"cumuflx0": h0.t["cumuflx"], "dpinit0": h0.t["dpinit"], "cumuflx1": h.t["cumuflx"],
+                     "temp": h.t["temp"], "saln": h.t["saln"], "passv": h.t["passv"]}
+
+
+@pytest.mark.parametrize("level", [2, 3])
+def test_hycom_transport_checker_equals_reference_digests(oracle_lib, level):
+    with open(os.path.join(GOLDEN, "hycom_transport_ref.json")) as f:
+        ref = json.load(f)
+    g = glvl.read_glvl(os.path.join(GOLDEN, "glvl_G%d.dat" % level))
+    _, _, out = _transport(oracle_lib, g)
+    want = ref["runs"]["G%d" % level]
+    assert set(out) == set(want)
+    for n, v in out.items():
+        assert _sha(v) == want[n], n
+
+
+def test_hycom_transport_checker_equals_live_reference_g4(oracle_lib):
+    from oracle import refrun
+    if not os.path.exists(os.path.join(refrun.BIN, "hyctrref")):
+        pytest.skip("reference build (oracle/_ref) not present")
+    path = refrun.ref_glvl(4)
+    g = glvl.read_glvl(path)
+    case, _, out = _transport(oracle_lib, g, seed=17)
+    r = refrun.run_hyctrref(path, case, leapn=1)
+    for n, v in out.items():
+        assert np.array_equal(np.ravel(v).view(np.uint64), np.ravel(r[n]).view(np.uint64)), n
+
+
+def test_hycom_transport_properties(oracle_lib):
+    """What the transport fixes: each tracer's mass-weighted mean over the
+    ocean is restored to rounding by the conservation step (the diffusion of
+    temperature and salinity that follows conserves it too, since each edge's
+    two fluxes cancel), a tracer that is constant stays constant, land
+    columns keep their values, and the vertical displacements reach past a
+    whole layer in both directions (the multi-layer walks are exercised)."""
+    g = glvl.read_glvl(os.path.join(GOLDEN, "glvl_G3.dat"))
+    case, h, out = _transport(oracle_lib, g)
+    wet = case["wet"] > 0
+    area = np.asarray(g.area, np.float64)
+    for n in ("temp", "saln"):
+        m0 = (case[n] * case["dpinit"] * area[:, None])[wet].sum() / (case["dpinit"] * area[:, None])[wet].sum()
+        m1 = (out[n] * case["dpfinl"] * area[:, None])[wet].sum() / (case["dpfinl"] * area[:, None])[wet].sum()
+        assert abs(m1 - m0) <= 1e-12 * abs(m0), (n, m1 - m0)
+        assert np.array_equal(out[n][~wet], case[n][~wet]), n
+    c = dict(case)
+    c["passv"] = np.full_like(case["passv"], 0.25)
+    h2 = oracle_lib.HycomCnuity(g, c)
+    h2.transport()
+    assert np.allclose(h2.t["passv"][0][wet], 0.25, rtol=0, atol=1e-13)
