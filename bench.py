@@ -673,7 +673,7 @@ def run_hycom(a, fimhip, cpu=True):
             o.step(st, 1 + st % 2, 2 - st % 2)
             o.step_momtum(1 + st % 2, 2 - st % 2)
         o.set_momtum(case["batrop"], case["veldff"], case["smagcf"])
-        for n in fimhip.OCN_FIELDS:
+        for n in fimhip.OCN_MOMTUM_FIELDS:
             o.upload(n, case[n])
         leg = _ocean_leg(a, o, wcl, ocean_step, bytes_all)
         ms = leg["ms_per_step"]

@@ -660,6 +660,12 @@ double *ocn_field(fimhip_ocn *o, int f, size_t *n) {
     case FIMHIP_OCN_TAUX: *n = (size_t)o->nip; return o->mo.taux;
     case FIMHIP_OCN_TAUY: *n = (size_t)o->nip; return o->mo.tauy;
     case FIMHIP_OCN_USTARB: *n = (size_t)o->nip; return o->mo.ustarb;
+    case FIMHIP_OCN_CUMUFLX: *n = ez; return o->tr.cumuflx;
+    case FIMHIP_OCN_DPINIT: *n = cz; return o->tr.dpinit;
+    case FIMHIP_OCN_DPFINL: *n = cz; return o->tr.dpfinl;
+    case FIMHIP_OCN_TEMP: *n = cz; return o->tr.temp;
+    case FIMHIP_OCN_SALN: *n = cz; return o->tr.saln;
+    case FIMHIP_OCN_PASSV: *n = o->tr.numtr * cz; return o->tr.numtr > 0 ? o->tr.passv : nullptr;
   }
   return nullptr;
 }

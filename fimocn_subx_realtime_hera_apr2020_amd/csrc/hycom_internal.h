@@ -1,7 +1,8 @@
 // Definitions shared by the HYCOM ocean kernels (hycom_cnuity.hip,
-// hycom_edgvar.hip): the constants of hycom_constants.F90, the grid and state
-// the kernels read, the lane-to-column mapping and the wavefront-local column
-// sums, and the ocean instance behind the fimhip_ocn_* entries.
+// hycom_edgvar.hip, hycom_momtum.hip, hycom_transp.hip): the constants of
+// hycom_constants.F90, the grid and state the kernels read, the lane-to-column
+// mapping and the wavefront-local column sums, and the ocean instance behind
+// the fimhip_ocn_* entries.
 #pragma once
 #include <vector>
 
@@ -116,6 +117,18 @@ struct HycMom {
   double batrop, veldff, smagcf;
 };
 
+// the tracer transport's arrays (hycom_transp3d.F90, hycom_fct3d.F90): the
+// time-integrated mass fluxes, the thicknesses at both ends of the interval,
+// the tracers, and fct3d's work arrays over the nfld = 2 + numtr fields
+// ((nfld,nip,kdm); uan (nfld,nip,6,kdm); colsum (nfld,4,nip) the column parts
+// of the conservation sums)
+struct HycTr {
+  double *cumuflx, *dpinit, *dpfinl, *temp, *saln, *passv;
+  double *w, *fld, *flo, *vfx, *vdv, *fmx, *fmn, *flp, *fln, *uan, *colsum, *flxdv, *q;
+  int nfld, numtr;
+  double temdff;
+};
+
 }  // namespace hyc
 }  // namespace fimhip
 
@@ -134,6 +147,7 @@ struct fimhip_ocn {
   fimhip::hyc::HycState s{};
   fimhip::hyc::HycEdg ed{};
   fimhip::hyc::HycMom mo{};
+  fimhip::hyc::HycTr tr{};
   bool janjic = false;               // janjic_ocn (fimnamelist.F90:246)
   std::vector<int> wet;
   std::vector<double> area;

@@ -1,0 +1,598 @@
+// HYCOM's tracer transport (FIMsrc/fim/horizontal/hycom_transp3d.F90 transp0,
+// transp1, transp2, with hycom_fct3d.F90's fct3d and hycom_dffusn.F90's
+// dffusn_lyr) on the GPU, in FP64, with the reference's operations in its
+// order.  The tracers of transp2 (temperature + 10, salinity, the passive
+// tracers) go through fct3d together, one per grid row (blockIdx.y):
+//   hyc_tr_w      the vertical mass flux the time-integrated horizontal
+//                 fluxes imply (a column sum) and the tracers' working copies
+//   hyc_fct_vert  parabolic fits in each layer and the vertical fluxes: each
+//                 interface's lane walks the layers its displacement spans,
+//                 over fits the column's lanes put in LDS; the
+//                 conservation sums' column parts before the update
+//   hyc_fct_lo    low-order and antidiffusive edge fluxes, local bounds, the
+//                 low-order update (into its own array: the neighbours still
+//                 read the old values)
+//   hyc_fct_ratio the bounds widened by the low-order neighbours, limiter
+//                 ratios
+//   hyc_fct_clip  the limited antidiffusive update; the conservation sums'
+//                 column parts after it
+//   hyc_fct_fix   the shift that restores each tracer's ocean mean (summed on
+//                 the host over the columns in cell order, as the reference
+//                 sums serially), the tracers written back
+//   hyc_dff_flux, hyc_dff_apply  dffusn_lyr for temperature and salinity
+#include <string.h>
+
+#include <vector>
+
+#include "hycom_internal.h"
+
+namespace fimhip {
+namespace hyc {
+namespace {
+
+constexpr double H_TOFFSET = 10.;        // keeps temperature positive (hycom_transp3d.F90)
+constexpr double H_ONEMU_F = 1.e-6;      // fct3d's onemu
+constexpr double H_ATHIRD = 1. / 3.;
+constexpr int H_BCLIN_FRQ = 30;          // hycom_control.F90:9
+
+// the column scan that starts at its first element: v(0), v(0)+v(1), ...
+__device__ __forceinline__ double col_scan0(double *buf, double v, int k, int kdm) {
+  buf[k] = v;
+  wave_sync();
+  double prev = buf[0], mine = prev;
+  for (int j = 1; j < kdm; j++) {
+    prev = prev + buf[j];
+    if (j == k) mine = prev;
+  }
+  wave_sync();
+  return mine;
+}
+// the sum over the column's layers of v, in layer order from 0
+__device__ __forceinline__ double col_sum0(double *buf, double v, int k, int kdm) {
+  buf[k] = v;
+  wave_sync();
+  double acc = 0.;
+  for (int j = 0; j < kdm; j++) acc = acc + buf[j];
+  wave_sync();
+  return acc;
+}
+
+#define TF(a, t, i, k) (a)[((size_t)(t) * nip + (i)) * kdm + (k)]
+
+// transp0: the integral restarts, dpinit from the new level
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_tr_init(HycGrid g, HycState s, HycTr tr, int leapn) {
+  const int kdm = g.kdm, nip = g.nip;
+  const Col<LPC> c = hcol<LPC>(nip);
+  if (!c.ok || c.k >= kdm) return;
+  const int i = c.i, k = c.k;
+  const bool wi = ci_wet(g.cinfo[i]);
+#pragma unroll
+  for (int e = 0; e < NPP; e++) HE(tr.cumuflx, i, e, k) = wi ? 0. : H_LAND;
+  if (wi) HC(tr.dpinit, i, k) = HC(s.dp + (size_t)(leapn - 1) * nip * kdm, i, k);
+}
+
+// transp1: cumuflx += massflx * 2 * batrop
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_tr_accum(HycGrid g, HycTr tr, const double *massflx, double batrop) {
+  const int kdm = g.kdm, nip = g.nip;
+  const Col<LPC> c = hcol<LPC>(nip);
+  if (!c.ok || c.k >= kdm) return;
+  const int i = c.i, k = c.k;
+  const int ci = g.cinfo[i];
+  if (!ci_wet(ci)) return;
+  const int np = ci_np(ci);
+#pragma unroll
+  for (int e = 0; e < NPP; e++)
+    if (e < np) HE(tr.cumuflx, i, e, k) = HE(tr.cumuflx, i, e, k) + HE(massflx, i, e, k) * 2. * batrop;
+}
+
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_tr_w(HycGrid g, HycTr tr, const double *dpfinl) {
+  __shared__ double sh[256 / LPC][LPC + 1];
+  const int kdm = g.kdm, nip = g.nip;
+  const Col<LPC> c = hcol<LPC>(nip);
+  if (!c.ok) return;
+  const int i = c.i, k = c.k;
+  const bool live = k < kdm;
+  const int kk = live ? k : 0;
+  const int ci = g.cinfo[i];
+  const bool wi = ci_wet(ci);
+  // the tracers' working copies, temperature offset
+  if (live)
+    for (int t = 0; t < tr.nfld; t++) {
+      const double *src = t == 0 ? tr.temp : t == 1 ? tr.saln : tr.passv + (size_t)(t - 2) * nip * kdm;
+      TF(tr.fld, t, i, k) = wi ? (t == 0 ? HC(src, i, k) + H_TOFFSET : HC(src, i, k)) : H_LAND;
+    }
+  if (!wi) {
+    if (live) HC(tr.w, i, k) = H_LAND;
+    return;
+  }
+  const int np = ci_np(ci);
+  double hordiv = 0.;
+#pragma unroll
+  for (int e = 0; e < NPP; e++)
+    if (e < np) hordiv = hordiv + HE(tr.cumuflx, i, e, kk);
+  const double vertdv = (HC(tr.dpinit, i, kk) - HC(dpfinl, i, kk)) - hordiv * g.rarea[i];
+  const double w = col_scan0(sh[c.cc], live ? vertdv : 0., k, kdm);
+  if (live) HC(tr.w, i, k) = w;
+}
+
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_fct_vert(HycGrid g, HycTr tr) {
+  constexpr int CPB = 256 / LPC;
+  __shared__ double sa[CPB][LPC + 1], sb[CPB][LPC + 1], sc[CPB][LPC + 1], st[CPB][LPC + 1];
+  __shared__ double sm[CPB][LPC + 1];
+  __shared__ int sz[CPB][LPC + 1];
+  const int kdm = g.kdm, nip = g.nip;
+  const Col<LPC> c = hcol<LPC>(nip);
+  if (!c.ok) return;
+  const int i = c.i, k = c.k, cc = c.cc, t = blockIdx.y;
+  const bool live = k < kdm;
+  const int kk = live ? k : 0;
+  if (!ci_wet(g.cinfo[i])) return;
+  const double eps = 1.e-11;
+  const double f = TF(tr.fld, t, i, kk), th = HC(tr.dpinit, i, kk);
+  // neighbours in the column (taken by every lane)
+  const double fu = __shfl_up(f, 1, LPC), fd = __shfl_down(f, 1, LPC);
+  const double tu = __shfl_up(th, 1, LPC), td = __shfl_down(th, 1, LPC);
+  double a = f, b = 0., cq = 0.;
+  if (k > 0 && k < kdm - 1) {
+    const double dxlft = hmax(eps, tu), dxmid = hmax(eps, th), dxrgt = hmax(eps, td);
+    const double yl = (dxlft * f + dxmid * fu) / (dxlft + dxmid);
+    const double yr = (dxrgt * f + dxmid * fd) / (dxrgt + dxmid);
+    a = 1.5 * f - .25 * (yl + yr);
+    b = yr - yl;
+    cq = 6. * (.5 * (yl + yr) - f);
+    if (fabs(yr - yl) < 6. * fabs(.5 * (yl + yr) - f)) {
+      if (fabs(yr - yl) > 2. * fabs(.5 * (yl + yr) - f)) {
+        if ((yr - yl) * (.5 * (yl + yr) - f) > 0.) {
+          a = .25 * (3. * f + yl);
+          cq = 3. * (f - yl);
+          b = cq;
+        } else {
+          a = .25 * (3. * f + yr);
+          cq = 3. * (f - yr);
+          b = -cq;
+        }
+      } else {
+        a = f;
+        b = cq = 0.;
+      }
+    }
+  }
+  sa[cc][k] = a;
+  sb[cc][k] = b;
+  sc[cc][k] = cq;
+  st[cc][k] = th;
+  wave_sync();
+  // the vertical flux through the interface below layer k (lanes k < kdm-1)
+  const double wk = live ? HC(tr.w, i, k) : 0.;
+  double amount = 0., slab = H_ONEMU_F;
+  const bool moved = wk < 0. || wk > 0.;
+  if (k < kdm - 1) {
+    if (wk < 0.) {
+      amount = slab * fd;
+      for (int kp = k + 1; kp < kdm; kp++) {
+        if (slab >= -wk) break;
+        const double tk = st[cc][kp];
+        if (tk > 0.) {
+          const double dslab = hmin(slab + tk, -wk) - hmin(slab, -wk);
+          const double dx = dslab / tk;
+          const double fcdx = sa[cc][kp] + sb[cc][kp] * .5 * (dx - 1.) + sc[cc][kp] * (.25 - dx * (.5 - dx * H_ATHIRD));
+          amount = amount + fcdx * dslab;
+          slab = slab + dslab;
+        }
+      }
+    } else if (wk > 0.) {
+      amount = slab * f;
+      int kp = k;
+      for (;;) {          // layer k first, whatever k, then up to the second layer
+        if (slab >= wk) break;
+        const double tk = st[cc][kp];
+        if (tk > 0.) {
+          const double dslab = hmin(slab + tk, wk) - hmin(slab, wk);
+          const double dx = dslab / tk;
+          const double fcdx = sa[cc][kp] + sb[cc][kp] * .5 * (1. - dx) + sc[cc][kp] * (.25 - dx * (.5 - dx * H_ATHIRD));
+          amount = amount + fcdx * dslab;
+          slab = slab + dslab;
+        }
+        if (kp > 1) kp--;
+        else break;
+      }
+    }
+  }
+  // where w is neither < 0 nor > 0 the reference keeps the amount of the
+  // nearest interface above where it was (its loop variable), 0 if none
+  sm[cc][k] = amount;
+  sz[cc][k] = moved ? 0 : 1;
+  wave_sync();
+  if (k < kdm - 1 && !moved) {
+    amount = 0.;
+    for (int j = k - 1; j >= 0; j--)
+      if (!sz[cc][j]) {
+        amount = sm[cc][j];
+        break;
+      }
+  }
+  wave_sync();
+  const double vfx = k < kdm - 1 ? wk * amount / slab : 0.;
+  const double vfu = __shfl_up(vfx, 1, LPC);
+  if (live) {
+    TF(tr.vfx, t, i, k) = vfx;
+    TF(tr.vdv, t, i, k) = k == 0 ? vfx : vfx - vfu;
+  }
+  // the conservation sums before the update: this column's parts
+  const double area = g.area[i];
+  const double col1 = col_sum0(sm[cc], live ? th * area : 0., k, kdm);
+  const double col2 = col_sum0(sm[cc], live ? f * th * area : 0., k, kdm);
+  if (k == 0) {
+    tr.colsum[((size_t)t * 4 + 0) * nip + i] = col1;
+    tr.colsum[((size_t)t * 4 + 1) * nip + i] = col2;
+  }
+}
+
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_fct_lo(HycGrid g, HycTr tr, const double *thkn) {
+  const int kdm = g.kdm, nip = g.nip;
+  const Col<LPC> c = hcol<LPC>(nip);
+  if (!c.ok || c.k >= kdm) return;
+  const int i = c.i, k = c.k, t = blockIdx.y;
+  const int ci = g.cinfo[i];
+  int nb[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) return;
+  const int np = ci_np(ci);
+  const double f = TF(tr.fld, t, i, k);
+  double fmx = f, fmn = f, hordiv = 0.;
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    if (e >= np) break;
+    const int ix = nb[e], edx = g.proxs[i * NPP + e];
+    const double ue = HE(tr.cumuflx, i, e, k), ux = HE(tr.cumuflx, ix, edx, k), fx = TF(tr.fld, t, ix, k);
+    double flx = 0., uan = 0.;
+    if (ci_open(ci, e)) {
+      flx = 0.5 * ((ue + fabs(ue)) * f - (ux + fabs(ux)) * fx);
+      const double q = .5 * (f + fx);
+      uan = q * ue - flx;
+      fmx = hmax(fmx, fx);
+      fmn = hmin(fmn, fx);
+    }
+    tr.uan[((size_t)t * nip * NPP + (size_t)i * NPP + e) * kdm + k] = uan;
+    hordiv = hordiv + flx;
+  }
+  const double wk = HC(tr.w, i, k);
+  if (k < kdm - 1 && wk < 0.) {
+    const double v = TF(tr.vfx, t, i, k) / wk;
+    fmx = hmax(fmx, v);
+    fmn = hmin(fmn, v);
+  }
+  if (k > 0) {
+    const double wu = HC(tr.w, i, k - 1);
+    if (wu > 0.) {
+      const double v = TF(tr.vfx, t, i, k - 1) / wu;
+      fmx = hmax(fmx, v);
+      fmn = hmin(fmn, v);
+    }
+  }
+  hordiv = hordiv * g.rarea[i];
+  const double tn = HC(thkn, i, k);
+  const double unbound = f * HC(tr.dpinit, i, k) - hordiv - TF(tr.vdv, t, i, k);
+  const double bounded = hmax(hmax(0., fmn * tn), hmin(unbound, fmx * tn));
+  TF(tr.flo, t, i, k) = (f * H_ONEMU_F + bounded) / (H_ONEMU_F + tn);
+  TF(tr.fmx, t, i, k) = fmx;
+  TF(tr.fmn, t, i, k) = fmn;
+}
+
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_fct_ratio(HycGrid g, HycTr tr, const double *thkn) {
+  const int kdm = g.kdm, nip = g.nip;
+  const Col<LPC> c = hcol<LPC>(nip);
+  if (!c.ok || c.k >= kdm) return;
+  const int i = c.i, k = c.k, t = blockIdx.y;
+  const int ci = g.cinfo[i];
+  int nb[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) return;
+  const int np = ci_np(ci);
+  double fmx = TF(tr.fmx, t, i, k), fmn = TF(tr.fmn, t, i, k);
+  const double f = TF(tr.flo, t, i, k);
+  double totin = 0., totou = 0.;
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    if (e >= np) break;
+    const double u = tr.uan[((size_t)t * nip * NPP + (size_t)i * NPP + e) * kdm + k];
+    totin = totin - hmin(0., u);
+    totou = totou + hmax(0., u);
+    const double fx = TF(tr.flo, t, nb[e], k);
+    if (ci_open(ci, e)) {
+      fmx = hmax(fmx, fx);
+      fmn = hmin(fmn, fx);
+    }
+  }
+  const double tn = HC(thkn, i, k), ra = g.rarea[i];
+  TF(tr.flp, t, i, k) = (fmx - f) * tn / (totin + 1.e-11) * ra;
+  TF(tr.fln, t, i, k) = (f - fmn) * tn / (totou + 1.e-11) * ra;
+  TF(tr.fmx, t, i, k) = fmx;
+  TF(tr.fmn, t, i, k) = fmn;
+}
+
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_fct_clip(HycGrid g, HycTr tr, const double *thkn) {
+  __shared__ double sh[256 / LPC][LPC + 1];
+  const int kdm = g.kdm, nip = g.nip;
+  const Col<LPC> c = hcol<LPC>(nip);
+  if (!c.ok) return;
+  const int i = c.i, k = c.k, t = blockIdx.y;
+  const bool live = k < kdm;
+  const int kk = live ? k : 0;
+  const int ci = g.cinfo[i];
+  int nb[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) return;
+  const int np = ci_np(ci);
+  const double fln = TF(tr.fln, t, i, kk), flp = TF(tr.flp, t, i, kk);
+  double hordiv = 0.;
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    if (e >= np) break;
+    const int ix = nb[e];
+    const double u = tr.uan[((size_t)t * nip * NPP + (size_t)i * NPP + e) * kdm + kk];
+    const double flpx = TF(tr.flp, t, ix, kk), flnx = TF(tr.fln, t, ix, kk);
+    if (ci_open(ci, e)) {
+      const double clip = u >= 0. ? hmin(hmin(1., fln), flpx) : hmin(hmin(1., flp), flnx);
+      hordiv = hordiv + u * clip;
+    }
+  }
+  hordiv = hordiv * g.rarea[i];
+  const double f = TF(tr.flo, t, i, kk), tn = HC(thkn, i, kk);
+  const double unbound = f * tn - hordiv;
+  const double bounded = hmax(hmax(0., TF(tr.fmn, t, i, kk) * tn), hmin(unbound, TF(tr.fmx, t, i, kk) * tn));
+  const double fnew = (f * H_ONEMU_F + bounded) / (H_ONEMU_F + tn);
+  if (live) TF(tr.fld, t, i, k) = fnew;
+  // the conservation sums after the update: this column's parts
+  const double area = g.area[i];
+  const double col1 = col_sum0(sh[c.cc], live ? tn * area : 0., k, kdm);
+  const double col2 = col_sum0(sh[c.cc], live ? fnew * tn * area : 0., k, kdm);
+  if (k == 0) {
+    tr.colsum[((size_t)t * 4 + 2) * nip + i] = col1;
+    tr.colsum[((size_t)t * 4 + 3) * nip + i] = col2;
+  }
+}
+
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_fct_fix(HycGrid g, HycTr tr, const double *q) {
+  const int kdm = g.kdm, nip = g.nip;
+  const Col<LPC> c = hcol<LPC>(nip);
+  if (!c.ok || c.k >= kdm) return;
+  const int i = c.i, k = c.k, t = blockIdx.y;
+  if (!ci_wet(g.cinfo[i])) return;
+  const double f = TF(tr.fld, t, i, k) + q[t];
+  double *dst = t == 0 ? tr.temp : t == 1 ? tr.saln : tr.passv + (size_t)(t - 2) * nip * kdm;
+  HC(dst, i, k) = t == 0 ? f - H_TOFFSET : f;
+}
+
+// dffusn_lyr: the fluxes through the ocean edges, then the update (two
+// kernels: the update changes what the neighbours read)
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_dff_flux(HycGrid g, HycTr tr, const double *delp) {
+  const int kdm = g.kdm, nip = g.nip;
+  const Col<LPC> c = hcol<LPC>(nip);
+  if (!c.ok || c.k >= kdm) return;
+  const int i = c.i, k = c.k, t = blockIdx.y;
+  const int ci = g.cinfo[i];
+  int nb[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
+  if (!ci_wet(ci)) return;
+  const int np = ci_np(ci);
+  const double *fld = t == 0 ? tr.temp : tr.saln;
+  const double fi = HC(fld, i, k), a = hmax(HC(delp, i, k), 1.e-11);
+  double flxdv = 0.;
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    if (e >= np) break;
+    const int ix = nb[e];
+    const double fx = HC(fld, ix, k), b = hmax(HC(delp, ix, k), 1.e-11);
+    if (ci_open(ci, e)) flxdv = flxdv + (fi - fx) * g.sideln[i * NPP + e] * 2. * (a * b / (a + b));
+  }
+  TF(tr.flxdv, t, i, k) = flxdv;
+}
+
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_dff_apply(HycGrid g, HycTr tr, const double *delp, double dffveldt) {
+  const int kdm = g.kdm, nip = g.nip;
+  const Col<LPC> c = hcol<LPC>(nip);
+  if (!c.ok || c.k >= kdm) return;
+  const int i = c.i, k = c.k, t = blockIdx.y;
+  if (!ci_wet(g.cinfo[i])) return;
+  double *fld = t == 0 ? tr.temp : tr.saln;
+  const double factor = -dffveldt * g.rarea[i] / hmax(HC(delp, i, k), 1.e-11);
+  HC(fld, i, k) = HC(fld, i, k) + TF(tr.flxdv, t, i, k) * factor;
+}
+
+}  // namespace
+}  // namespace hyc
+}  // namespace fimhip
+
+using namespace fimhip;
+using namespace fimhip::hyc;
+
+static int tr_ready(const fimhip_ocn *o) { return o->tr.nfld > 0 && o->mo.batrop > 0.; }
+
+template <int LPC>
+static int tr2_kernels(fimhip_ocn *o, const double *dpfinl) {
+  const HycGrid &g = o->g;
+  HycTr &tr = o->tr;
+  hipStream_t st = o->stream;
+  const unsigned nb = ocn_blocks(o, LPC);
+  const dim3 g1(nb), gf(nb, tr.nfld), blk(256);
+  KT_LAUNCH("hyc_tr_w", st, hyc_tr_w<LPC><<<g1, blk, 0, st>>>(g, tr, dpfinl));
+  KT_LAUNCH("hyc_fct_vert", st, hyc_fct_vert<LPC><<<gf, blk, 0, st>>>(g, tr));
+  KT_LAUNCH("hyc_fct_lo", st, hyc_fct_lo<LPC><<<gf, blk, 0, st>>>(g, tr, dpfinl));
+  KT_LAUNCH("hyc_fct_ratio", st, hyc_fct_ratio<LPC><<<gf, blk, 0, st>>>(g, tr, dpfinl));
+  KT_LAUNCH("hyc_fct_clip", st, hyc_fct_clip<LPC><<<gf, blk, 0, st>>>(g, tr, dpfinl));
+  HIPCHK(hipGetLastError());
+  // the conservation shift of each tracer: the sums over the ocean cells in
+  // cell order (hycom_fct3d.F90:290-309, 482-509; serial, as written)
+  const int nip = o->nip, nf = tr.nfld;
+  std::vector<double> cs((size_t)nf * 4 * nip), q(nf);
+  HIPCHK(hipMemcpyAsync(cs.data(), tr.colsum, cs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int t = 0; t < nf; t++) {
+    double wtold = 0., bfore = 0., wtnew = 0., after = 0.;
+    const double *c0 = &cs[((size_t)t * 4 + 0) * nip], *c1 = &cs[((size_t)t * 4 + 1) * nip];
+    const double *c2 = &cs[((size_t)t * 4 + 2) * nip], *c3 = &cs[((size_t)t * 4 + 3) * nip];
+    for (int i = 0; i < nip; i++) {
+      if (!o->wet[i]) continue;
+      wtold = wtold + c0[i];
+      bfore = bfore + c1[i];
+      wtnew = wtnew + c2[i];
+      after = after + c3[i];
+    }
+    bfore = bfore / wtold;
+    after = after / wtnew;
+    q[t] = bfore - after;
+  }
+  HIPCHK(hipMemcpyAsync(tr.q, q.data(), nf * sizeof(double), hipMemcpyHostToDevice, st));
+  KT_LAUNCH("hyc_fct_fix", st, hyc_fct_fix<LPC><<<gf, blk, 0, st>>>(g, tr, tr.q));
+  if (tr.temdff > 0.) {
+    const dim3 g2(nb, 2);
+    const double dffveldt = tr.temdff * o->mo.batrop * H_BCLIN_FRQ;
+    KT_LAUNCH("hyc_dff_flux", st, hyc_dff_flux<LPC><<<g2, blk, 0, st>>>(g, tr, dpfinl));
+    KT_LAUNCH("hyc_dff_apply", st, hyc_dff_apply<LPC><<<g2, blk, 0, st>>>(g, tr, dpfinl, dffveldt));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));      // q is a host vector
+  return FIMHIP_OK;
+}
+
+static int tr2_launch(fimhip_ocn *o, const double *dpfinl) {
+  if (!tr_ready(o)) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(o->device));
+  g_ktimer = o->kt;
+  const int r = o->kdm <= 32 ? tr2_kernels<32>(o, dpfinl) : tr2_kernels<64>(o, dpfinl);
+  g_ktimer = nullptr;
+  return r;
+}
+
+extern "C" int fimhip_ocn_set_transport(fimhip_ocn *o, int numtr, double temdff) {
+  if (!o || numtr < 0 || numtr > 8) return FIMHIP_EINVAL;
+  if (o->tr.nfld > 0) {             // the arrays exist: only temdff may change
+    if (o->tr.nfld != 2 + numtr) return FIMHIP_EINVAL;
+    o->tr.temdff = temdff;
+    return FIMHIP_OK;
+  }
+  HIPCHK(hipSetDevice(o->device));
+  const int nf = 2 + numtr, nip = o->nip, kdm = o->kdm;
+  const size_t cz = (size_t)nip * kdm, ez = cz * NPP;
+  HycTr &tr = o->tr;
+  int r = 0;
+  struct { double **p; size_t n; } arrs[] = {
+      {&tr.cumuflx, ez}, {&tr.dpinit, cz}, {&tr.dpfinl, cz}, {&tr.temp, cz}, {&tr.saln, cz}, {&tr.passv, numtr > 0 ? numtr * cz : 1},
+      {&tr.w, cz}, {&tr.fld, nf * cz}, {&tr.flo, nf * cz}, {&tr.vfx, nf * cz}, {&tr.vdv, nf * cz},
+      {&tr.fmx, nf * cz}, {&tr.fmn, nf * cz}, {&tr.flp, nf * cz}, {&tr.fln, nf * cz}, {&tr.uan, nf * ez},
+      {&tr.colsum, (size_t)nf * 4 * nip}, {&tr.flxdv, 2 * cz}, {&tr.q, (size_t)nf}};
+  for (auto &a : arrs)
+    if ((r = o->arena.alloc(a.p, a.n))) return r;
+  HIPCHK(hipStreamSynchronize(o->stream));
+  tr.nfld = nf;
+  tr.numtr = numtr;
+  tr.temdff = temdff;
+  return FIMHIP_OK;
+}
+
+// transp0 on the device-resident arrays (hycom_transp3d.F90:18-51)
+extern "C" int fimhip_ocn_transp0(fimhip_ocn *o, int leapn) {
+  if (!o || !tr_ready(o) || leapn < 1 || leapn > 2) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(o->device));
+  g_ktimer = o->kt;
+  const dim3 blk(256);
+  if (o->kdm <= 32) KT_LAUNCH("hyc_tr_init", o->stream, hyc_tr_init<32><<<ocn_blocks(o, 32), blk, 0, o->stream>>>(o->g, o->s, o->tr, leapn));
+  else KT_LAUNCH("hyc_tr_init", o->stream, hyc_tr_init<64><<<ocn_blocks(o, 64), blk, 0, o->stream>>>(o->g, o->s, o->tr, leapn));
+  g_ktimer = nullptr;
+  HIPCHK(hipGetLastError());
+  return FIMHIP_OK;
+}
+
+// transp1 with the instance's mass flux (hycom_transp3d.F90:54-85)
+extern "C" int fimhip_ocn_transp1(fimhip_ocn *o) {
+  if (!o || !tr_ready(o)) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(o->device));
+  g_ktimer = o->kt;
+  const dim3 blk(256);
+  if (o->kdm <= 32) KT_LAUNCH("hyc_tr_accum", o->stream, hyc_tr_accum<32><<<ocn_blocks(o, 32), blk, 0, o->stream>>>(o->g, o->tr, o->s.massflx, o->mo.batrop));
+  else KT_LAUNCH("hyc_tr_accum", o->stream, hyc_tr_accum<64><<<ocn_blocks(o, 64), blk, 0, o->stream>>>(o->g, o->tr, o->s.massflx, o->mo.batrop));
+  g_ktimer = nullptr;
+  HIPCHK(hipGetLastError());
+  return FIMHIP_OK;
+}
+
+// transp2 (hycom_transp3d.F90:88-321) with dpfinl = dp(:,:,leapn), as
+// hycom_run.F90:398-400 calls it
+extern "C" int fimhip_ocn_transp2(fimhip_ocn *o, int leapn) {
+  if (!o || leapn < 1 || leapn > 2) return FIMHIP_EINVAL;
+  return tr2_launch(o, o->s.dp + (size_t)(leapn - 1) * o->nip * o->kdm);
+}
+
+static int ocn_put(fimhip_ocn *o, int f, const double *h) {
+  size_t n = 0;
+  double *d = ocn_field(o, f, &n);
+  if (!d) return FIMHIP_EINVAL;
+  HIPCHK(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, o->stream));
+  return FIMHIP_OK;
+}
+static int ocn_get(fimhip_ocn *o, int f, double *h) {
+  size_t n = 0;
+  double *d = ocn_field(o, f, &n);
+  if (!d) return FIMHIP_EINVAL;
+  HIPCHK(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, o->stream));
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_ocn_transp0_r8(fimhip_ocn *o, int nstep, int leapn, double *cumuflx, const double *dp,
+                                    double *dpinit) {
+  (void)nstep;
+  if (!o || !cumuflx || !dp || !dpinit || !tr_ready(o)) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(o->device));
+  int r;
+  if ((r = ocn_put(o, FIMHIP_OCN_DP, dp)) || (r = ocn_put(o, FIMHIP_OCN_DPINIT, dpinit))) return r;
+  if ((r = fimhip_ocn_transp0(o, leapn))) return r;
+  if ((r = ocn_get(o, FIMHIP_OCN_CUMUFLX, cumuflx)) || (r = ocn_get(o, FIMHIP_OCN_DPINIT, dpinit))) return r;
+  HIPCHK(hipStreamSynchronize(o->stream));
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_ocn_transp1_r8(fimhip_ocn *o, int nstep, double *cumuflx, const double *massflx) {
+  (void)nstep;
+  if (!o || !cumuflx || !massflx || !tr_ready(o)) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(o->device));
+  int r;
+  if ((r = ocn_put(o, FIMHIP_OCN_CUMUFLX, cumuflx)) || (r = ocn_put(o, FIMHIP_OCN_MASSFLX, massflx))) return r;
+  if ((r = fimhip_ocn_transp1(o))) return r;
+  if ((r = ocn_get(o, FIMHIP_OCN_CUMUFLX, cumuflx))) return r;
+  HIPCHK(hipStreamSynchronize(o->stream));
+  return FIMHIP_OK;
+}
+
+extern "C" int fimhip_ocn_transp2_r8(fimhip_ocn *o, int nstep, double *temp, double *saln, double *passv_tr,
+                                    const double *cumuflx, const double *dpinit, const double *dpfinl) {
+  (void)nstep;
+  if (!o || !temp || !saln || !cumuflx || !dpinit || !dpfinl || !tr_ready(o) || (o->tr.numtr > 0 && !passv_tr))
+    return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(o->device));
+  int r;
+  if ((r = ocn_put(o, FIMHIP_OCN_TEMP, temp)) || (r = ocn_put(o, FIMHIP_OCN_SALN, saln)) ||
+      (r = ocn_put(o, FIMHIP_OCN_CUMUFLX, cumuflx)) || (r = ocn_put(o, FIMHIP_OCN_DPINIT, dpinit)) ||
+      (r = ocn_put(o, FIMHIP_OCN_DPFINL, dpfinl)))
+    return r;
+  if (o->tr.numtr > 0 && (r = ocn_put(o, FIMHIP_OCN_PASSV, passv_tr))) return r;
+  if ((r = tr2_launch(o, o->tr.dpfinl))) return r;
+  if ((r = ocn_get(o, FIMHIP_OCN_TEMP, temp)) || (r = ocn_get(o, FIMHIP_OCN_SALN, saln))) return r;
+  if (o->tr.numtr > 0 && (r = ocn_get(o, FIMHIP_OCN_PASSV, passv_tr))) return r;
+  HIPCHK(hipStreamSynchronize(o->stream));
+  return FIMHIP_OK;
+}

@@ -51,7 +51,9 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ocn_field_size", "fimhip_ocn_upload", "fimhip_ocn_download", "fimhip_ocn_mass",
            "fimhip_ocn_set_mass", "fimhip_ocn_set_dtleap", "fimhip_ocn_profile", "fimhip_ocn_profile_report",
            "fimhip_ocn_edgvar_r8", "fimhip_ocn_edgvar", "fimhip_ocn_set_janjic", "fimhip_ocn_momtum_r8",
-           "fimhip_ocn_momtum", "fimhip_ocn_set_momtum"]
+           "fimhip_ocn_momtum", "fimhip_ocn_set_momtum", "fimhip_ocn_set_transport", "fimhip_ocn_transp0_r8",
+           "fimhip_ocn_transp1_r8", "fimhip_ocn_transp2_r8", "fimhip_ocn_transp0", "fimhip_ocn_transp1",
+           "fimhip_ocn_transp2"]
 
 # real*4 drop-ins: name -> (number of float arrays, trailing int argument)
 R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, True), "mass_adv": (12, True),
@@ -181,6 +183,13 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ocn_momtum_r8.argtypes = [vp, ct.c_int, ct.c_int, ct.c_int] + [_PD] * 17
     lib.fimhip_ocn_momtum.argtypes = [vp, ct.c_int, ct.c_int]
     lib.fimhip_ocn_set_momtum.argtypes = [vp, ct.c_double, ct.c_double, ct.c_double]
+    lib.fimhip_ocn_set_transport.argtypes = [vp, ct.c_int, ct.c_double]
+    lib.fimhip_ocn_transp0_r8.argtypes = [vp, ct.c_int, ct.c_int, _PD, _PD, _PD]
+    lib.fimhip_ocn_transp1_r8.argtypes = [vp, ct.c_int, _PD, _PD]
+    lib.fimhip_ocn_transp2_r8.argtypes = [vp, ct.c_int] + [_PD] * 6
+    lib.fimhip_ocn_transp0.argtypes = [vp, ct.c_int]
+    lib.fimhip_ocn_transp1.argtypes = [vp]
+    lib.fimhip_ocn_transp2.argtypes = [vp, ct.c_int]
     lib.fimhip_ocn_profile.argtypes = [vp, ct.c_int]
     lib.fimhip_ocn_profile_report.argtypes = [vp, ct.c_char_p, ct.c_int]
     _LIB = lib
@@ -437,47 +446,6 @@ class Model(Instance):
                     massfx=(3,) + e, u_edg=e, v_edg=e, dp_edg=e, lp_edg=e, mont_edg=e, uvsq_edg=e,
                     trc_edg=(ntra,) + e, trb=(self.ntrb,) + c, cumufx=e, dpinit=c)[name]
 
-    def edgvar(self, nstep, leapm, leapn, uclin, vclin, utrop, vtrop, dp, pres, geop, montg, spcifv, um_edg,
-               vm_edg, un_edg, vn_edg, dp_edg, pr_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg):
-        arrs = (uclin, vclin, utrop, vtrop, dp, pres, geop, montg, spcifv, um_edg, vm_edg, un_edg, vn_edg, dp_edg,
-                pr_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg)
-        names = OCN_EDG_IN[:4] + ["dp", "pres"] + OCN_EDG_IN[4:] + ["um_edg", "vm_edg", "un_edg", "vn_edg",
-                                                                     "dpm_edg"] + OCN_EDG_OUT
-        for n, a in zip(names, arrs):
-            if a.shape != self._shape(n):
-                raise ValueError("%s: shape %s, expected %s" % (n, a.shape, self._shape(n)))
-        _chk(self.lib.fimhip_ocn_edgvar_r8(self.h, nstep, leapm, leapn, *map(_p, arrs)), "ocn_edgvar_r8")
-
-    def step_edgvar(self, leapm: int, leapn: int):
-        """bclin_edgvar on the device-resident arrays: its edge values are the
-        ones ``step`` (cnuity) then reads."""
-        _chk(self.lib.fimhip_ocn_edgvar(self.h, leapm, leapn), "ocn_edgvar")
-
-    def momtum(self, nstep, leapm, leapn, utrop, vtrop, uclin, vclin, um_edg, vm_edg, geop_edg, mont_edg,
-               uvsq_edg, spcv_edg, taux, tauy, ubforc, vbforc, dp, pres, ustarb):
-        """momtum (hycom_momtum.F90:16); ubforc and vbforc are not written
-        (modesplit is off in the reference)."""
-        arrs = (utrop, vtrop, uclin, vclin, um_edg, vm_edg, geop_edg, mont_edg, uvsq_edg, spcv_edg, taux, tauy,
-                ubforc, vbforc, dp, pres, ustarb)
-        names = ["utrop", "vtrop", "uclin", "vclin", "um_edg", "vm_edg", "geop_edg", "mont_edg", "uvsq_edg",
-                 "spcv_edg", "taux", "tauy", "taux", "tauy", "dp", "pres", "ustarb"]
-        for n, a in zip(names, arrs):
-            if a.shape != self._shape(n):
-                raise ValueError("%s: shape %s, expected %s" % (n, a.shape, self._shape(n)))
-        _chk(self.lib.fimhip_ocn_momtum_r8(self.h, nstep, leapm, leapn, *map(_p, arrs)), "ocn_momtum_r8")
-
-    def step_momtum(self, leapm: int, leapn: int):
-        """momtum on the device-resident arrays."""
-        _chk(self.lib.fimhip_ocn_momtum(self.h, leapm, leapn), "ocn_momtum")
-
-    def set_momtum(self, batrop: float, veldff: float = 0.1, smagcf: float = 0.3):
-        """batrop (hycom_init.F90:46) and the namelist's veldff, smagcf."""
-        _chk(self.lib.fimhip_ocn_set_momtum(self.h, float(batrop), float(veldff), float(smagcf)), "ocn_set_momtum")
-
-    def set_janjic(self, on: bool):
-        """janjic_ocn (fimnamelist.F90:246, default off)."""
-        _chk(self.lib.fimhip_ocn_set_janjic(self.h, int(on)), "ocn_set_janjic")
-
     def upload(self, name: str, a):
         a = np.ascontiguousarray(a, np.float64)
         if a.shape != self.shape(name):
@@ -675,9 +643,12 @@ class Ensemble:
 
 OCN_FIELDS = ["dp", "pres", "um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "massflx", "mssfx_avg",
               "uclin", "vclin", "utrop", "vtrop", "geop", "montg", "spcifv", "pr_edg", "geop_edg", "mont_edg",
-              "uvsq_edg", "spcv_edg", "taux", "tauy", "ustarb"]
+              "uvsq_edg", "spcv_edg", "taux", "tauy", "ustarb", "cumuflx", "dpinit", "dpfinl", "temp", "saln",
+              "passv"]
 OCN_CNUITY_FIELDS = OCN_FIELDS[:9]       # the arrays cnuity reads or writes
 OCN_EDGVAR_FIELDS = OCN_FIELDS[:21]      # with those of bclin_edgvar
+OCN_MOMTUM_FIELDS = OCN_FIELDS[:24]      # with those of momtum
+OCN_TRANSP_FIELDS = OCN_FIELDS[24:]      # the tracer transport's (after set_transport)
 # bclin_edgvar's inputs and its outputs besides um/vm/un/vn_edg and dp_edg (dpm_edg here)
 OCN_EDG_IN = ["uclin", "vclin", "utrop", "vtrop", "geop", "montg", "spcifv"]
 OCN_EDG_OUT = ["pr_edg", "geop_edg", "mont_edg", "uvsq_edg", "spcv_edg"]
@@ -685,14 +656,17 @@ FIMHIP_EDUP = -4
 
 
 class Ocean:
-    """The HYCOM ocean's edge interpolation and continuity equation on one GPU
-    (fimhip_ocn_*).  ``cnuity`` keeps the argument list of hycom_cnuity.F90:21
+    """The HYCOM ocean's dynamics on one GPU (fimhip_ocn_*): the edge
+    interpolation, continuity equation, momentum equation and tracer
+    transport.  ``cnuity`` keeps the argument list of hycom_cnuity.F90:21
     (nstep, leapm, leapn, um_edg, vm_edg, un_edg, vn_edg, dpm_edg, dp, pres,
-    massflx, mssfx_avg) and ``edgvar`` that of bclin_edgvar
-    (hycom_edgvar.F90:128), with numpy float64 arrays in the reference layout
-    (C order: dp, uclin, vclin (2,nip,kdm); pres, geop (nip,kdm+1); utrop,
-    vtrop (nip,3); montg, spcifv (nip,kdm); edges (nip,6,kdm), pr_edg
-    (nip,6,kdm+1)), the written arrays updated in place.  ``mass`` is
+    massflx, mssfx_avg), ``edgvar`` that of bclin_edgvar
+    (hycom_edgvar.F90:128), ``momtum`` that of hycom_momtum.F90:16 and
+    ``transp0/1/2`` those of hycom_transp3d.F90:18, 54, 88, with numpy float64
+    arrays in the reference layout (C order: dp, uclin, vclin (2,nip,kdm);
+    pres, geop (nip,kdm+1); utrop, vtrop (nip,3); montg, spcifv, dpinit,
+    dpfinl, temp, saln (nip,kdm); passv (numtr,nip,kdm); edges (nip,6,kdm),
+    pr_edg (nip,6,kdm+1)), the written arrays updated in place.  ``mass`` is
     hycom_variables' (totmass, massglb0, masscor).  As the reference stops on
     differing duplicate edge fluxes (chkdup), so this raises FimhipError."""
 
@@ -700,6 +674,7 @@ class Ocean:
                  biharm: float = 0.75, device: int = 0):
         self.lib = load_library()
         self.grid, self.kdm, self.nip = grid, kdm, grid.nip
+        self.numtr = 0
         gd, self._keep = _grid_desc(grid, 1, 1, dtleap, np.zeros(1), np.zeros(1))
         self._keep["rprox_ln"] = np.ascontiguousarray(grid.rprox_ln, np.float64)
         self._keep["wet"] = np.ascontiguousarray(wet, np.int32)
@@ -725,7 +700,9 @@ class Ocean:
         n, k = self.nip, self.kdm
         return {"dp": (2, n, k), "pres": (n, k + 1), "uclin": (2, n, k), "vclin": (2, n, k), "utrop": (n, 3),
                 "vtrop": (n, 3), "geop": (n, k + 1), "montg": (n, k), "spcifv": (n, k),
-                "pr_edg": (n, 6, k + 1), "taux": (n,), "tauy": (n,), "ustarb": (n,)}.get(name, (n, 6, k))
+                "pr_edg": (n, 6, k + 1), "taux": (n,), "tauy": (n,), "ustarb": (n,), "dpinit": (n, k),
+                "dpfinl": (n, k), "temp": (n, k), "saln": (n, k),
+                "passv": (self.numtr, n, k)}.get(name, (n, 6, k))
 
     @staticmethod
     def _step_rc(rc: int, what: str):
@@ -780,6 +757,50 @@ class Ocean:
     def set_janjic(self, on: bool):
         """janjic_ocn (fimnamelist.F90:246, default off)."""
         _chk(self.lib.fimhip_ocn_set_janjic(self.h, int(on)), "ocn_set_janjic")
+
+    def set_transport(self, numtr: int = 1, temdff: float = 0.02):
+        """numtr passive tracers (hycom_control.F90:3) and temdff
+        (fimnamelist.F90:255); batrop comes from ``set_momtum``."""
+        _chk(self.lib.fimhip_ocn_set_transport(self.h, int(numtr), float(temdff)), "ocn_set_transport")
+        self.numtr = int(numtr)
+
+    def _check(self, names, arrs):
+        for n, a in zip(names, arrs):
+            if a.shape != self._shape(n) or a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError("%s: shape %s %s, expected %s float64" % (n, a.shape, a.dtype, self._shape(n)))
+
+    def transp0(self, nstep, leapn, cumuflx, dp, dpinit):
+        """transp0 (hycom_transp3d.F90:18): cumuflx restarts, dpinit =
+        dp[leapn-1] on the ocean."""
+        self._check(["cumuflx", "dp", "dpinit"], (cumuflx, dp, dpinit))
+        _chk(self.lib.fimhip_ocn_transp0_r8(self.h, nstep, leapn, _p(cumuflx), _p(dp), _p(dpinit)), "ocn_transp0_r8")
+
+    def transp1(self, nstep, cumuflx, massflx):
+        """transp1 (hycom_transp3d.F90:54): cumuflx += massflx*2*batrop."""
+        self._check(["cumuflx", "massflx"], (cumuflx, massflx))
+        _chk(self.lib.fimhip_ocn_transp1_r8(self.h, nstep, _p(cumuflx), _p(massflx)), "ocn_transp1_r8")
+
+    def transp2(self, nstep, temp, saln, passv_tr, cumuflx, dpinit, dpfinl):
+        """transp2 (hycom_transp3d.F90:88) with fct3d and dffusn_lyr: temp,
+        saln and passv_tr advected in place (passv_tr None when numtr = 0)."""
+        names = ["temp", "saln", "cumuflx", "dpinit", "dpfinl"] + (["passv"] if self.numtr else [])
+        arrs = (temp, saln, cumuflx, dpinit, dpfinl) + ((passv_tr,) if self.numtr else ())
+        self._check(names, arrs)
+        pv = _p(passv_tr) if self.numtr else None
+        _chk(self.lib.fimhip_ocn_transp2_r8(self.h, nstep, _p(temp), _p(saln), pv, _p(cumuflx), _p(dpinit),
+                                            _p(dpfinl)), "ocn_transp2_r8")
+
+    def step_transp0(self, leapn: int):
+        """transp0 on the device-resident arrays, from dp's level leapn."""
+        _chk(self.lib.fimhip_ocn_transp0(self.h, leapn), "ocn_transp0")
+
+    def step_transp1(self):
+        """transp1 with the device-resident massflx."""
+        _chk(self.lib.fimhip_ocn_transp1(self.h), "ocn_transp1")
+
+    def step_transp2(self, leapn: int):
+        """transp2 with dpfinl = dp's level leapn (hycom_run.F90:398-400)."""
+        _chk(self.lib.fimhip_ocn_transp2(self.h, leapn), "ocn_transp2")
 
     def upload(self, name: str, a):
         a = np.ascontiguousarray(a, np.float64)

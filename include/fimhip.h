@@ -386,6 +386,15 @@ typedef struct fimhip_ocn fimhip_ocn;
 #define FIMHIP_OCN_TAUX 21
 #define FIMHIP_OCN_TAUY 22
 #define FIMHIP_OCN_USTARB 23
+/* the tracer transport's arrays (fimhip_ocn_set_transport allocates them):
+ * cumuflx (kdm,6,nip), dpinit, dpfinl, temp, saln (kdm,nip), passv_tr
+ * (kdm,nip,numtr) */
+#define FIMHIP_OCN_CUMUFLX 24
+#define FIMHIP_OCN_DPINIT 25
+#define FIMHIP_OCN_DPFINL 26
+#define FIMHIP_OCN_TEMP 27
+#define FIMHIP_OCN_SALN 28
+#define FIMHIP_OCN_PASSV 29
 /* hycom_init's constants (hycom_constants.F90 wet, dtleap; fimnamelist kdm,
  * thkdff, biharm) on the grid of the descriptor (nprox, prox, proxs, cs, sn,
  * sidevec_e, sidevec_c, sideln, area, lat; nvl, ntra, dt, hfcoef, lon,
@@ -438,6 +447,32 @@ int fimhip_ocn_set_momtum(fimhip_ocn *ocn, double batrop, double veldff, double 
 /* janjic_ocn (fimnamelist.F90:246, default off): the geopotential on the
  * edges from hydrostatic integration in the neighbouring columns */
 int fimhip_ocn_set_janjic(fimhip_ocn *ocn, int on);
+/* the tracer transport's numtr passive tracers (0..8; hycom_control.F90:3,
+ * 1 there) and temdff (fimnamelist.F90:255, the tracers' mixing velocity in
+ * m/s, 0.02 there; dffusn_lyr runs where > 0): allocates the transport's arrays, once per
+ * instance (a later call may change temdff only).  batrop comes from
+ * fimhip_ocn_set_momtum. */
+int fimhip_ocn_set_transport(fimhip_ocn *ocn, int numtr, double temdff);
+/* hycom_transp3d.F90:18 transp0(nstep, leapn, cumuflx, dp, dpinit) on host
+ * arrays: cumuflx restarts (0 on the ocean, -1e33 on land), dpinit (ocean
+ * columns) = dp(:,:,leapn) */
+int fimhip_ocn_transp0_r8(fimhip_ocn *ocn, int nstep, int leapn, double *cumuflx, const double *dp,
+                          double *dpinit);
+/* hycom_transp3d.F90:54 transp1(nstep, cumuflx, massflx): cumuflx +=
+ * massflx*2*batrop on the ocean's edges */
+int fimhip_ocn_transp1_r8(fimhip_ocn *ocn, int nstep, double *cumuflx, const double *massflx);
+/* hycom_transp3d.F90:88 transp2(nstep, temp, saln, passv_tr, cumuflx, dpinit,
+ * dpfinl) with hycom_fct3d.F90 fct3d and hycom_dffusn.F90 dffusn_lyr:
+ * temp, saln and passv_tr (NULL when numtr = 0) are advected in place */
+int fimhip_ocn_transp2_r8(fimhip_ocn *ocn, int nstep, double *temp, double *saln, double *passv_tr,
+                          const double *cumuflx, const double *dpinit, const double *dpfinl);
+/* the same on the device-resident arrays: transp0 from FIMHIP_OCN_DP's level
+ * leapn, transp1 with FIMHIP_OCN_MASSFLX (both asynchronous), transp2 with
+ * dpfinl = FIMHIP_OCN_DP's level leapn (hycom_run.F90:398-400; returns when
+ * done: fct3d's conservation shift is summed on the host) */
+int fimhip_ocn_transp0(fimhip_ocn *ocn, int leapn);
+int fimhip_ocn_transp1(fimhip_ocn *ocn);
+int fimhip_ocn_transp2(fimhip_ocn *ocn, int leapn);
 /* hycom_constants' dtleap for the calls that follow (hycom_run.F90:136 sets
  * it every step: batrop at the first, 2*batrop after) */
 int fimhip_ocn_set_dtleap(fimhip_ocn *ocn, double dtleap);

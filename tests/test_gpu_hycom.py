@@ -1,13 +1,15 @@
-"""The HYCOM ocean's edge interpolation and continuity equation on the GPU
-(fimhip_ocn_*, the FP64 kernels of hycom_edgvar.hip and hycom_cnuity.hip)
+"""The HYCOM ocean's dynamics on the GPU (fimhip_ocn_*, the FP64 kernels of
+hycom_edgvar.hip, hycom_cnuity.hip, hycom_momtum.hip and hycom_transp.hip)
 against the checker's restatement (oracle/hycom_oracle.c, pinned to the
-reference's hycom_edgvar.F90 and hycom_cnuity.F90 bit for bit): every output
-array and the mass state must hold the checker's bytes, through the
-host-array drop-ins and the device-resident path, for every
-interface-smoothing branch (biharmonic, Laplacian, none), across the 10-day
-mass correction, with and without janjic_ocn, with 32 layers (two columns
-per wavefront) and with 48 (one column per wavefront), at G3 to G7 (163842
-cells)."""
+reference's hycom_edgvar.F90, hycom_cnuity.F90, hycom_momtum.F90 with
+hycom_dissip.F90, and hycom_transp3d.F90 with hycom_fct3d.F90 and
+hycom_dffusn.F90, bit for bit): every output array and the mass state must
+hold the checker's bytes, through the host-array drop-ins and the
+device-resident path, for every interface-smoothing branch (biharmonic,
+Laplacian, none), across the 10-day mass correction, with and without
+janjic_ocn, with 0 to 3 passive tracers and with and without the tracers'
+diffusion, with 32 layers (two columns per wavefront) and with 48 (one
+column per wavefront), at G3 to G7 (163842 cells)."""
 import numpy as np
 import pytest
 
@@ -273,7 +275,7 @@ def test_ocean_step_resident_matches_checker(oracle_lib, level, kdm, janjic):
     try:
         o.set_janjic(janjic)
         o.set_momtum(case["batrop"], case["veldff"], case["smagcf"])
-        for n in fimhip.OCN_FIELDS:
+        for n in fimhip.OCN_MOMTUM_FIELDS:
             o.upload(n, case[n])
         for st in (1, 2, 3):
             leapm = 1 + st % 2
@@ -301,7 +303,7 @@ def test_ocean_step_g7_full_size(oracle_lib):
     o = fimhip.Ocean(g, 32, case["wet"], case["dtleap"])
     try:
         o.set_momtum(case["batrop"], case["veldff"], case["smagcf"])
-        for n in fimhip.OCN_FIELDS:
+        for n in fimhip.OCN_MOMTUM_FIELDS:
             o.upload(n, case[n])
         for st in (1, 2):
             leapm = 1 + st % 2
@@ -315,5 +317,136 @@ def test_ocean_step_g7_full_size(oracle_lib):
             assert _same(o.download(n), h.a[n]), n
         for n in ("uclin", "vclin", "ustarb"):
             assert _same(o.download(n), h.e[n]), n
+    finally:
+        o.close()
+
+
+TR_OUT = ("temp", "saln", "passv")
+
+
+def _tr_case(g, kdm, numtr, seed, temdff=None):
+    from fimocn_subx_realtime_hera_apr2020_amd import ocean
+    case = ocean.hycom_transport_case(g, kdm=kdm, numtr=numtr, seed=seed)
+    if temdff is not None:
+        case["temdff"] = temdff
+    return case
+
+
+def _tr_ocean(g, case):
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    o = fimhip.Ocean(g, case["kdm"], case["wet"], case["dtleap"])
+    o.set_momtum(case["batrop"])
+    o.set_transport(case["numtr"], case["temdff"])
+    return o
+
+
+@pytest.mark.parametrize("numtr,temdff", [(1, None), (0, 0.0), (3, None)])
+def test_transport_dropin_matches_checker_bitwise_g3(oracle_lib, numtr, temdff):
+    """transp0, transp1 and transp2 through the host-array drop-ins."""
+    print("case: G3 x 32, transport drop-in, numtr %d, temdff %s" % (numtr, temdff), flush=True)
+    g = _grid(3)
+    case = _tr_case(g, 32, numtr, 29 + numtr, temdff)
+    o = _tr_ocean(g, case)
+    try:
+        h0 = oracle_lib.HycomCnuity(g, case)
+        h0.transport(leapn=2, transp0=True)
+        cu = np.ascontiguousarray(case["cumuflx"], np.float64).copy()
+        di = np.ascontiguousarray(case["dpinit"], np.float64).copy()
+        o.transp0(1, 2, cu, np.ascontiguousarray(case["dp"], np.float64), di)
+        assert _same(cu, h0.t["cumuflx"]) and _same(di, h0.t["dpinit"])
+        h = oracle_lib.HycomCnuity(g, case)
+        h.transport()
+        a = {n: np.ascontiguousarray(case[n], np.float64).copy()
+             for n in ("cumuflx", "massflx_tr", "dpinit", "dpfinl", "temp", "saln", "passv")}
+        o.transp1(2, a["cumuflx"], a["massflx_tr"])
+        assert _same(a["cumuflx"], h.t["cumuflx"])
+        o.transp2(2, a["temp"], a["saln"], a["passv"] if numtr else None, a["cumuflx"], a["dpinit"], a["dpfinl"])
+        for n in TR_OUT:
+            assert _same(a[n], h.t[n]), n
+    finally:
+        o.close()
+
+
+@pytest.mark.parametrize("level,kdm,numtr", [(4, 48, 2), (5, 32, 1)])
+def test_transport_resident_matches_checker(oracle_lib, level, kdm, numtr):
+    """The device-resident entries as hycom_run calls them: transp0 from dp's
+    new level, transp1 on the step's mass flux, transp2 with dpfinl = dp's
+    level leapn; two transport intervals, the second starting from the
+    first's tracers."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    print("case: G%d x %d, transport resident, numtr %d" % (level, kdm, numtr), flush=True)
+    g = _grid(level)
+    case = _tr_case(g, kdm, numtr, 53 + level, None)
+    o = _tr_ocean(g, case)
+    try:
+        dp = np.stack([case["dpinit"], case["dpfinl"]])
+        c = dict(case, dp=dp)
+        for n in ("temp", "saln", "passv", "cumuflx", "dpinit"):
+            o.upload(n, case[n])
+        o.upload("dp", dp)
+        o.upload("massflx", case["massflx_tr"])
+        h = oracle_lib.HycomCnuity(g, c)
+        h.transport()
+        o.step_transp1()
+        o.step_transp2(2)
+        for n in TR_OUT + ("cumuflx",):
+            assert _same(o.download(n), h.t[n]), n
+        # the next interval: dpinit from dp's level 2, a fresh flux integral
+        h.transport(leapn=2, transp0=True)
+        o.step_transp0(2)
+        for n in ("cumuflx", "dpinit"):
+            assert _same(o.download(n), h.t[n]), n
+        h.t["cumuflx"][...] = case["cumuflx"]
+        o.upload("cumuflx", case["cumuflx"])
+        h.t["dpfinl"][...] = case["dpinit"]
+        o.upload("dp", np.stack([case["dpfinl"], case["dpinit"]]))
+        h.transport()
+        o.step_transp1()
+        o.step_transp2(2)
+        for n in TR_OUT + ("cumuflx",):
+            assert _same(o.download(n), h.t[n]), ("second", n)
+    finally:
+        o.close()
+
+
+def test_transport_g7_full_size(oracle_lib):
+    print("case: G7 x 32, transport resident, numtr 1", flush=True)
+    g = _grid(7)
+    case = _tr_case(g, 32, 1, 71, None)
+    o = _tr_ocean(g, case)
+    try:
+        for n in ("temp", "saln", "passv", "cumuflx", "dpinit"):
+            o.upload(n, case[n])
+        o.upload("dp", np.stack([case["dpinit"], case["dpfinl"]]))
+        o.upload("massflx", case["massflx_tr"])
+        h = oracle_lib.HycomCnuity(g, case)
+        h.transport()
+        o.step_transp1()
+        o.step_transp2(2)
+        for n in TR_OUT:
+            assert _same(o.download(n), h.t[n]), n
+    finally:
+        o.close()
+
+
+def test_transport_bad_arguments_are_rejected():
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    g = _grid(2)
+    case = _tr_case(g, 32, 1, 5, None)
+    o = fimhip.Ocean(g, 32, case["wet"], case["dtleap"])
+    try:
+        with pytest.raises(fimhip.FimhipError):
+            o.step_transp1()                 # before set_transport
+        o.set_momtum(case["batrop"])
+        with pytest.raises(fimhip.FimhipError):
+            o.set_transport(9, 0.02)
+        o.set_transport(1, 0.02)
+        with pytest.raises(fimhip.FimhipError):
+            o.set_transport(2, 0.02)         # numtr is fixed once set
+        o.set_transport(1, 0.0)
+        with pytest.raises(fimhip.FimhipError):
+            o.step_transp2(3)
+        with pytest.raises(fimhip.FimhipError):
+            o.step_transp0(0)
     finally:
         o.close()
