@@ -584,6 +584,32 @@ HYC_OCEAN_STEP_BYTES = (32 + 16 + 8 + 16 + 48) + (32 + 16 + 8 + 48 + 48)
 # edges x kdm+1 interfaces), massflx, mssfx_avg, dp (2 levels) and pres
 HYC_CHAIN_BYTES = (32 + 16 + 8 + 16 + 48) + (8 * 48 + 50) + (2 * 48 + 16 + 8)
 
+# the tracer transport's kernels with nfld = 2 + numtr tracers: the bytes one
+# launch must move per ocean cell-level (the fct3d kernels run all tracers in
+# one launch; the arrays every tracer shares, cumuflx, w, dpinit, dpfinl,
+# counted once)
+def hyc_tr_kernel_bytes(nfld):
+    return {
+        "hyc_tr_init": 48 + 8 + 8,                     # -> cumuflx; dp -> dpinit
+        "hyc_tr_accum": 96 + 48,                       # cumuflx += massflx
+        "hyc_tr_w": 48 + 16 + 8 + 16 * nfld,           # cumuflx dpinit dpfinl -> w; tracers -> working copies
+        "hyc_fct_vert": 16 + 24 * nfld,                # dpinit w; fld -> vertical flux, its divergence
+        "hyc_fct_lo": 48 + 24 + 96 * nfld,             # cumuflx w dpinit dpfinl; fld vfx vdv -> uan flo fmx fmn
+        "hyc_fct_ratio": 8 + 104 * nfld,               # fmx fmn flo uan -> flp fln fmx fmn
+        "hyc_fct_clip": 8 + 96 * nfld,                 # flp fln uan flo fmx fmn -> fld
+        "hyc_fct_fix": 16 * nfld,                      # fld -> the tracers
+        "hyc_dff_flux": 8 + 16 * 2,                    # dpfinl; temp saln -> their flux divergence
+        "hyc_dff_apply": 8 + 24 * 2,
+    }
+
+
+# one transport interval (hycom_run.F90:386-400, 553): transp1 on every
+# second of its 30 steps, transp2, transp0; what it must move at least per
+# ocean cell-level: massflx in 15 times, cumuflx in and out, dpinit and dpfinl
+# in, each tracer in and out
+def hyc_tr_interval_bytes(nfld):
+    return 15 * 48 + 96 + 16 + 16 * nfld
+
 
 def _ocean_leg(a, o, wcl, step, byte_model):
     """Time 'step(nstep)' over a.steps steps after a.warmup, then the same
@@ -630,8 +656,12 @@ def run_hycom(a, fimhip, cpu=True):
     calls them (hycom_edgvar.F90 bclin_edgvar, then cnuity on its edge
     values); and those two followed by the momentum equation
     (hycom_momtum.F90 momtum with hycom_dissip), the velocities it writes
-    feeding the next step; the checker's C restatement of each timed on one
-    host core over 2 steps of the same input."""
+    feeding the next step; and the tracer transport over one 30-step
+    interval (hycom_transp3d.F90 transp1 every second step, transp2 with
+    hycom_fct3d and hycom_dffusn, transp0).  The checker's C restatement of
+    each is timed on one host core over 2 steps (one interval) of the same
+    input."""
+    import numpy as np
     from fimocn_subx_realtime_hera_apr2020_amd import ocean
     g = fimhip.icos_grid(8)
     kdm, thkdff, biharm = 32, 0.05, 0.75
@@ -683,6 +713,39 @@ def run_hycom(a, fimhip, cpu=True):
         out["ocean_step"] = leg
     finally:
         o.close()
+    # the tracer transport, one interval of 30 steps per timed step (transp1
+    # 15 times, transp2, transp0), temperature, salinity and one passive
+    # tracer; transp1's mass flux scaled so that the interval integrates it
+    # to the case's cumuflx size
+    tcase = ocean.hycom_transport_case(g, kdm=kdm, numtr=1)
+    tcase["massflx_tr"] = np.where(tcase["massflx_tr"] == -1.0e33, -1.0e33,
+                                   tcase["massflx_tr"] * (0.2 / 2.0e-4) / (2.0 * tcase["batrop"] * 15))
+    tcase["dp"] = np.stack([tcase["dpinit"], tcase["dpfinl"]])
+    nfld = 2 + tcase["numtr"]
+    o = fimhip.Ocean(g, kdm, tcase["wet"], tcase["dtleap"], thkdff=thkdff, biharm=biharm)
+    try:
+        o.set_momtum(tcase["batrop"])
+        o.set_transport(tcase["numtr"], tcase["temdff"])
+        for n in ("temp", "saln", "passv", "dpinit", "dp"):
+            o.upload(n, tcase[n])
+        o.upload("massflx", tcase["massflx_tr"])
+
+        def interval(st):
+            o.step_transp0(1)
+            for _ in range(15):
+                o.step_transp1()
+            o.step_transp2(2)
+        leg = _ocean_leg(a, o, wcl, interval, hyc_tr_kernel_bytes(nfld))
+        ms = leg["ms_per_step"]
+        ib = hyc_tr_interval_bytes(nfld)
+        leg["unit"] = "ocean cell-level-intervals/s"
+        leg["tracers"] = nfld
+        leg["step_roofline"] = {"compulsory_bytes_per_cell_level": ib,
+                                "achieved": ib * wcl / (ms * 1e-3) / 1e9, "unit": "GB/s",
+                                "frac": ib * wcl / (ms * 1e-3) / HBM_PEAK}
+        out["transport"] = leg
+    finally:
+        o.close()
     if cpu:
         from oracle import oracle as O
         for name, with_edgvar, with_momtum in (("cnuity", False, False), ("edgvar_cnuity", True, False),
@@ -701,6 +764,16 @@ def run_hycom(a, fimhip, cpu=True):
                                   "kind": "port", "sample": "2 steps of the same G8 x 32 input",
                                   "ms_per_step": tc / 2 * 1e3}
             leg["speedup_vs_cpu_checker"] = leg["value"] / leg["cpu_checker"]["value"]
+        h = O.HycomCnuity(g, tcase)
+        t0 = time.perf_counter()
+        h.transport(leapn=1, transp0=True)
+        h.transport(transp1_calls=15)
+        tc = time.perf_counter() - t0
+        leg = out["transport"]
+        leg["cpu_checker"] = {"value": wcl / tc, "unit": "ocean cell-level-intervals/s", "cores": 1, "kind": "port",
+                              "sample": "1 interval (transp0, 15 x transp1, transp2) of the same G8 x 32 input",
+                              "ms_per_step": tc * 1e3}
+        leg["speedup_vs_cpu_checker"] = leg["value"] / leg["cpu_checker"]["value"]
     return out
 
 

@@ -120,12 +120,13 @@ struct HycMom {
 // the tracer transport's arrays (hycom_transp3d.F90, hycom_fct3d.F90): the
 // time-integrated mass fluxes, the thicknesses at both ends of the interval,
 // the tracers, and fct3d's work arrays over the nfld = 2 + numtr fields
-// ((nfld,nip,kdm); uan (nfld,nip,6,kdm); colsum (nfld,4,nip) the column parts
-// of the conservation sums)
+// ((nfld,nip,kdm); uan (nfld,nip,6,kdm); colsum (nfld,4,nwp) the column
+// parts of the conservation sums by ocean rank)
 struct HycTr {
   double *cumuflx, *dpinit, *dpfinl, *temp, *saln, *passv;
   double *w, *fld, *flo, *vfx, *vdv, *fmx, *fmn, *flp, *fln, *uan, *colsum, *flxdv, *q;
-  int nfld, numtr;
+  const int *rank;                // (nip) an ocean cell's rank among the nwet ocean cells
+  int nfld, numtr, nwet, nwp;    // nwp: the padded row length of colsum
   double temdff;
 };
 

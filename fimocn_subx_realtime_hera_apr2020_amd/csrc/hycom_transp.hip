@@ -16,12 +16,11 @@
 //                 ratios
 //   hyc_fct_clip  the limited antidiffusive update; the conservation sums'
 //                 column parts after it
-//   hyc_fct_fix   the shift that restores each tracer's ocean mean (summed on
-//                 the host over the columns in cell order, as the reference
-//                 sums serially), the tracers written back
+//   hyc_fct_sums  the conservation sums over the cells, one wave per sum,
+//                 in cell order as the reference's serial loops add them
+//   hyc_fct_fix   the shift that restores each tracer's ocean mean, the
+//                 tracers written back
 //   hyc_dff_flux, hyc_dff_apply  dffusn_lyr for temperature and salinity
-#include <string.h>
-
 #include <vector>
 
 #include "hycom_internal.h"
@@ -227,8 +226,9 @@ __global__ void __launch_bounds__(256) hyc_fct_vert(HycGrid g, HycTr tr) {
   const double col1 = col_sum0(sm[cc], live ? th * area : 0., k, kdm);
   const double col2 = col_sum0(sm[cc], live ? f * th * area : 0., k, kdm);
   if (k == 0) {
-    tr.colsum[((size_t)t * 4 + 0) * nip + i] = col1;
-    tr.colsum[((size_t)t * 4 + 1) * nip + i] = col2;
+    const int r = tr.rank[i];
+    tr.colsum[((size_t)t * 4 + 0) * tr.nwp + r] = col1;
+    tr.colsum[((size_t)t * 4 + 1) * tr.nwp + r] = col2;
   }
 }
 
@@ -358,19 +358,66 @@ __global__ void __launch_bounds__(256) hyc_fct_clip(HycGrid g, HycTr tr, const d
   const double col1 = col_sum0(sh[c.cc], live ? tn * area : 0., k, kdm);
   const double col2 = col_sum0(sh[c.cc], live ? fnew * tn * area : 0., k, kdm);
   if (k == 0) {
-    tr.colsum[((size_t)t * 4 + 2) * nip + i] = col1;
-    tr.colsum[((size_t)t * 4 + 3) * nip + i] = col2;
+    const int r = tr.rank[i];
+    tr.colsum[((size_t)t * 4 + 2) * tr.nwp + r] = col1;
+    tr.colsum[((size_t)t * 4 + 3) * tr.nwp + r] = col2;
   }
 }
 
+// sum c of the conservation step: 0 wtold (the thickness before, part 0 of
+// the first tracer's column parts), 1 wtnew (after, part 2), then each
+// tracer's bfore (part 1) and after (part 3), over the ocean columns in cell
+// order (the column parts are stored by ocean rank, each part's row padded
+// with +0 to nwp = a multiple of 256 past nwet + 256).  One wave per sum: it
+// loads blocks of 256 parts with coalesced loads, the next block in flight
+// while the current one, staged in LDS, is added in order into the chain
+// every lane carries (a dependent FP64 add takes ~7 ns here, so a sum over
+// G8's 458753 ocean columns takes ~3.4 ms).  The padding leaves a sum
+// unchanged: no partial sum is -0 (it starts at +0, and a sum is -0 only when
+// both terms are).
+constexpr int SUMS_BLK = 256;
+__global__ void __launch_bounds__(64) hyc_fct_sums(const double *__restrict__ colsum, int nwet, int nwp,
+                                                   double *__restrict__ sums) {
+  __shared__ double buf[SUMS_BLK];
+  const int c = blockIdx.x, l = threadIdx.x;
+  const int t = c < 2 ? 0 : (c - 2) >> 1;
+  const int part = c == 0 ? 0 : c == 1 ? 2 : (c & 1) ? 3 : 1;
+  const double *__restrict__ x = colsum + ((size_t)t * 4 + part) * nwp;
+  double r[SUMS_BLK / 64];
+#pragma unroll
+  for (int j = 0; j < SUMS_BLK / 64; j++) r[j] = x[l + 64 * j];
+  double acc = 0.;
+  for (int base = 0; base < nwet; base += SUMS_BLK) {
+#pragma unroll
+    for (int j = 0; j < SUMS_BLK / 64; j++) buf[l + 64 * j] = r[j];
+    const int nx = base + SUMS_BLK;
+#pragma unroll
+    for (int j = 0; j < SUMS_BLK / 64; j++) r[j] = x[nx + l + 64 * j];     // nx + 256 <= nwp
+    wave_sync();
+    // 32 parts in registers ahead of their adds, so that the LDS reads
+    // overlap the chain
+#pragma unroll
+    for (int h = 0; h < SUMS_BLK; h += 32) {
+      double v[32];
+#pragma unroll
+      for (int j = 0; j < 32; j++) v[j] = buf[h + j];
+#pragma unroll
+      for (int j = 0; j < 32; j++) acc = acc + v[j];
+    }
+    wave_sync();
+  }
+  if (l == 0) sums[c] = acc;
+}
+
 template <int LPC>
-__global__ void __launch_bounds__(256) hyc_fct_fix(HycGrid g, HycTr tr, const double *q) {
+__global__ void __launch_bounds__(256) hyc_fct_fix(HycGrid g, HycTr tr, const double *sums) {
   const int kdm = g.kdm, nip = g.nip;
   const Col<LPC> c = hcol<LPC>(nip);
   if (!c.ok || c.k >= kdm) return;
   const int i = c.i, k = c.k, t = blockIdx.y;
   if (!ci_wet(g.cinfo[i])) return;
-  const double f = TF(tr.fld, t, i, k) + q[t];
+  const double bfore = sums[2 + 2 * t] / sums[0], after = sums[3 + 2 * t] / sums[1];
+  const double f = TF(tr.fld, t, i, k) + (bfore - after);
   double *dst = t == 0 ? tr.temp : t == 1 ? tr.saln : tr.passv + (size_t)(t - 2) * nip * kdm;
   HC(dst, i, k) = t == 0 ? f - H_TOFFSET : f;
 }
@@ -435,29 +482,10 @@ static int tr2_kernels(fimhip_ocn *o, const double *dpfinl) {
   KT_LAUNCH("hyc_fct_lo", st, hyc_fct_lo<LPC><<<gf, blk, 0, st>>>(g, tr, dpfinl));
   KT_LAUNCH("hyc_fct_ratio", st, hyc_fct_ratio<LPC><<<gf, blk, 0, st>>>(g, tr, dpfinl));
   KT_LAUNCH("hyc_fct_clip", st, hyc_fct_clip<LPC><<<gf, blk, 0, st>>>(g, tr, dpfinl));
-  HIPCHK(hipGetLastError());
-  // the conservation shift of each tracer: the sums over the ocean cells in
-  // cell order (hycom_fct3d.F90:290-309, 482-509; serial, as written)
-  const int nip = o->nip, nf = tr.nfld;
-  std::vector<double> cs((size_t)nf * 4 * nip), q(nf);
-  HIPCHK(hipMemcpyAsync(cs.data(), tr.colsum, cs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  for (int t = 0; t < nf; t++) {
-    double wtold = 0., bfore = 0., wtnew = 0., after = 0.;
-    const double *c0 = &cs[((size_t)t * 4 + 0) * nip], *c1 = &cs[((size_t)t * 4 + 1) * nip];
-    const double *c2 = &cs[((size_t)t * 4 + 2) * nip], *c3 = &cs[((size_t)t * 4 + 3) * nip];
-    for (int i = 0; i < nip; i++) {
-      if (!o->wet[i]) continue;
-      wtold = wtold + c0[i];
-      bfore = bfore + c1[i];
-      wtnew = wtnew + c2[i];
-      after = after + c3[i];
-    }
-    bfore = bfore / wtold;
-    after = after / wtnew;
-    q[t] = bfore - after;
-  }
-  HIPCHK(hipMemcpyAsync(tr.q, q.data(), nf * sizeof(double), hipMemcpyHostToDevice, st));
+  // the conservation shift of each tracer (hycom_fct3d.F90:290-309, 482-509:
+  // serial sums over the ocean cells, as written; the same thickness sums
+  // serve every tracer)
+  KT_LAUNCH("hyc_fct_sums", st, hyc_fct_sums<<<dim3(2 + 2 * tr.nfld), dim3(64), 0, st>>>(tr.colsum, tr.nwet, tr.nwp, tr.q));
   KT_LAUNCH("hyc_fct_fix", st, hyc_fct_fix<LPC><<<gf, blk, 0, st>>>(g, tr, tr.q));
   if (tr.temdff > 0.) {
     const dim3 g2(nb, 2);
@@ -466,7 +494,6 @@ static int tr2_kernels(fimhip_ocn *o, const double *dpfinl) {
     KT_LAUNCH("hyc_dff_apply", st, hyc_dff_apply<LPC><<<g2, blk, 0, st>>>(g, tr, dpfinl, dffveldt));
   }
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(st));      // q is a host vector
   return FIMHIP_OK;
 }
 
@@ -490,14 +517,29 @@ extern "C" int fimhip_ocn_set_transport(fimhip_ocn *o, int numtr, double temdff)
   const int nf = 2 + numtr, nip = o->nip, kdm = o->kdm;
   const size_t cz = (size_t)nip * kdm, ez = cz * NPP;
   HycTr &tr = o->tr;
+  // each ocean cell's rank among the ocean cells in cell order; the rows of
+  // the conservation sums' column parts padded for their summing kernel
+  std::vector<int> rank(nip, 0);
+  int nwet = 0;
+  for (int i = 0; i < nip; i++)
+    if (o->wet[i]) rank[i] = nwet++;
+  const int nwp = (nwet + 255) / 256 * 256 + 256;
   int r = 0;
   struct { double **p; size_t n; } arrs[] = {
-      {&tr.cumuflx, ez}, {&tr.dpinit, cz}, {&tr.dpfinl, cz}, {&tr.temp, cz}, {&tr.saln, cz}, {&tr.passv, numtr > 0 ? numtr * cz : 1},
-      {&tr.w, cz}, {&tr.fld, nf * cz}, {&tr.flo, nf * cz}, {&tr.vfx, nf * cz}, {&tr.vdv, nf * cz},
-      {&tr.fmx, nf * cz}, {&tr.fmn, nf * cz}, {&tr.flp, nf * cz}, {&tr.fln, nf * cz}, {&tr.uan, nf * ez},
-      {&tr.colsum, (size_t)nf * 4 * nip}, {&tr.flxdv, 2 * cz}, {&tr.q, (size_t)nf}};
+      {&tr.cumuflx, ez}, {&tr.dpinit, cz}, {&tr.dpfinl, cz}, {&tr.temp, cz}, {&tr.saln, cz},
+      {&tr.passv, numtr > 0 ? numtr * cz : 1}, {&tr.w, cz}, {&tr.fld, nf * cz}, {&tr.flo, nf * cz},
+      {&tr.vfx, nf * cz}, {&tr.vdv, nf * cz}, {&tr.fmx, nf * cz}, {&tr.fmn, nf * cz}, {&tr.flp, nf * cz},
+      {&tr.fln, nf * cz}, {&tr.uan, nf * ez}, {&tr.colsum, (size_t)nf * 4 * nwp}, {&tr.flxdv, 2 * cz},
+      {&tr.q, (size_t)(2 + 2 * nf)}};
   for (auto &a : arrs)
     if ((r = o->arena.alloc(a.p, a.n))) return r;
+  int *drank = nullptr;
+  if ((r = o->arena.alloc(&drank, (size_t)nip))) return r;
+  HIPCHK(hipMemcpyAsync(drank, rank.data(), nip * sizeof(int), hipMemcpyHostToDevice, o->stream));
+  HIPCHK(hipMemsetAsync(tr.colsum, 0, (size_t)nf * 4 * nwp * sizeof(double), o->stream));
+  tr.rank = drank;
+  tr.nwet = nwet;
+  tr.nwp = nwp;
   HIPCHK(hipStreamSynchronize(o->stream));
   tr.nfld = nf;
   tr.numtr = numtr;

@@ -180,6 +180,28 @@ module fimhip_bind
     integer(c_int) function fimhip_ocn_set_momtum(ocn, batrop, veldff, smagcf) bind(C, name="fimhip_ocn_set_momtum")
       import; type(c_ptr), value :: ocn; real(c_double), value :: batrop, veldff, smagcf
     end function
+    integer(c_int) function fimhip_ocn_set_transport(ocn, numtr, temdff) bind(C, name="fimhip_ocn_set_transport")
+      import; type(c_ptr), value :: ocn; integer(c_int), value :: numtr; real(c_double), value :: temdff
+    end function
+    ! hycom_transp3d.F90:18 transp0, :54 transp1, :88 transp2 (passv_tr is
+    ! not read when numtr = 0)
+    integer(c_int) function fimhip_ocn_transp0_r8(ocn, nstep, leapn, cumuflx, dp, dpinit) &
+        bind(C, name="fimhip_ocn_transp0_r8")
+      import; type(c_ptr), value :: ocn; integer(c_int), value :: nstep, leapn
+      real(c_double), intent(in) :: dp(*)
+      real(c_double) :: cumuflx(*), dpinit(*)
+    end function
+    integer(c_int) function fimhip_ocn_transp1_r8(ocn, nstep, cumuflx, massflx) bind(C, name="fimhip_ocn_transp1_r8")
+      import; type(c_ptr), value :: ocn; integer(c_int), value :: nstep
+      real(c_double) :: cumuflx(*)
+      real(c_double), intent(in) :: massflx(*)
+    end function
+    integer(c_int) function fimhip_ocn_transp2_r8(ocn, nstep, temp, saln, passv_tr, cumuflx, dpinit, dpfinl) &
+        bind(C, name="fimhip_ocn_transp2_r8")
+      import; type(c_ptr), value :: ocn; integer(c_int), value :: nstep
+      real(c_double) :: temp(*), saln(*), passv_tr(*)
+      real(c_double), intent(in) :: cumuflx(*), dpinit(*), dpfinl(*)
+    end function
     integer(c_int) function fimhip_ocn_set_janjic(ocn, on) bind(C, name="fimhip_ocn_set_janjic")
       import; type(c_ptr), value :: ocn; integer(c_int), value :: on
     end function

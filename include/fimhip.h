@@ -466,10 +466,9 @@ int fimhip_ocn_transp1_r8(fimhip_ocn *ocn, int nstep, double *cumuflx, const dou
  * temp, saln and passv_tr (NULL when numtr = 0) are advected in place */
 int fimhip_ocn_transp2_r8(fimhip_ocn *ocn, int nstep, double *temp, double *saln, double *passv_tr,
                           const double *cumuflx, const double *dpinit, const double *dpfinl);
-/* the same on the device-resident arrays: transp0 from FIMHIP_OCN_DP's level
- * leapn, transp1 with FIMHIP_OCN_MASSFLX (both asynchronous), transp2 with
- * dpfinl = FIMHIP_OCN_DP's level leapn (hycom_run.F90:398-400; returns when
- * done: fct3d's conservation shift is summed on the host) */
+/* the same on the device-resident arrays, asynchronous: transp0 from
+ * FIMHIP_OCN_DP's level leapn, transp1 with FIMHIP_OCN_MASSFLX, transp2 with
+ * dpfinl = FIMHIP_OCN_DP's level leapn (hycom_run.F90:398-400) */
 int fimhip_ocn_transp0(fimhip_ocn *ocn, int leapn);
 int fimhip_ocn_transp1(fimhip_ocn *ocn);
 int fimhip_ocn_transp2(fimhip_ocn *ocn, int leapn);

@@ -284,10 +284,11 @@ class HycomCnuity:
                               a["dp"].ctypes.data_as(_PD), a["pres"].ctypes.data_as(_PD),
                               e["ustarb"].ctypes.data_as(_PD))
 
-    def transport(self, leapn: int = None, transp0: bool = False):
+    def transport(self, leapn: int = None, transp0: bool = False, transp1_calls: int = 1):
         """HYCOM's tracer transport on copies of the case's arrays in self.t:
         transp0 (with leapn) when asked, otherwise transp1 with massflx_tr on
-        cumuflx and then transp2 on temp, saln and passv."""
+        cumuflx (transp1_calls times) and then transp2 on temp, saln and
+        passv."""
         case = self.case
         if not hasattr(self, "t"):
             self.t = {n: np.ascontiguousarray(case[n], np.float64).copy()
@@ -298,7 +299,8 @@ class HycomCnuity:
             lib().fo_hycom_transp0(ct.byref(self.g), leapn, self.a["dp"].ctypes.data_as(_PD), P("dpinit"),
                                    P("cumuflx"))
             return
-        lib().fo_hycom_transp1(ct.byref(self.g), ct.c_double(case["batrop"]), P("massflx_tr"), P("cumuflx"))
+        for _ in range(transp1_calls):
+            lib().fo_hycom_transp1(ct.byref(self.g), ct.c_double(case["batrop"]), P("massflx_tr"), P("cumuflx"))
         lib().fo_hycom_transp2(ct.byref(self.g), case["numtr"], ct.c_double(case["batrop"]),
                                ct.c_double(case["temdff"]), P("temp"), P("saln"), P("passv"), P("cumuflx"),
                                P("dpinit"), P("dpfinl"))
