@@ -5,22 +5,25 @@
 // tracers) go through fct3d together, one per grid row (blockIdx.y):
 //   hyc_tr_w      the vertical mass flux the time-integrated horizontal
 //                 fluxes imply (a column sum) and the tracers' working copies
+//                 (hycom_transp3d.F90:149-172, 196-230)
 //   hyc_fct_vert  parabolic fits in each layer and the vertical fluxes: each
 //                 interface's lane walks the layers its displacement spans,
 //                 over fits the column's lanes put in LDS; the
 //                 conservation sums' column parts before the update
+//                 (hycom_fct3d.F90:135-309)
 //   hyc_fct_lo    low-order and antidiffusive edge fluxes, local bounds, the
 //                 low-order update (into its own array: the neighbours still
-//                 read the old values)
+//                 read the old values) (311-388)
 //   hyc_fct_ratio the bounds widened by the low-order neighbours, limiter
-//                 ratios
+//                 ratios (389-436)
 //   hyc_fct_clip  the limited antidiffusive update; the conservation sums'
-//                 column parts after it
+//                 column parts after it (437-498)
 //   hyc_fct_sums  the conservation sums over the cells, one wave per sum,
 //                 in cell order as the reference's serial loops add them
 //   hyc_fct_fix   the shift that restores each tracer's ocean mean, the
-//                 tracers written back
+//                 tracers written back (512-544, hycom_transp3d.F90:250-264)
 //   hyc_dff_flux, hyc_dff_apply  dffusn_lyr for temperature and salinity
+//                 (hycom_dffusn.F90:11-116)
 #include <vector>
 
 #include "hycom_internal.h"
@@ -482,7 +485,7 @@ static int tr2_kernels(fimhip_ocn *o, const double *dpfinl) {
   KT_LAUNCH("hyc_fct_lo", st, hyc_fct_lo<LPC><<<gf, blk, 0, st>>>(g, tr, dpfinl));
   KT_LAUNCH("hyc_fct_ratio", st, hyc_fct_ratio<LPC><<<gf, blk, 0, st>>>(g, tr, dpfinl));
   KT_LAUNCH("hyc_fct_clip", st, hyc_fct_clip<LPC><<<gf, blk, 0, st>>>(g, tr, dpfinl));
-  // the conservation shift of each tracer (hycom_fct3d.F90:290-309, 482-509:
+  // the conservation shift of each tracer (hycom_fct3d.F90:290-309, 482-511:
   // serial sums over the ocean cells, as written; the same thickness sums
   // serve every tracer)
   KT_LAUNCH("hyc_fct_sums", st, hyc_fct_sums<<<dim3(2 + 2 * tr.nfld), dim3(64), 0, st>>>(tr.colsum, tr.nwet, tr.nwp, tr.q));

@@ -813,7 +813,7 @@ static void hyc_fct3d(const fo_hyc_grid *g, double *fld3d, int numfld, const dou
     for (int i = 0; i < nip; i++)
       for (int k = 0; k < kdm; k++) C(fld, i, k) = WET(i) ? C(f3, i, k) : LAND_SPVAL;
     /* piecewise-parabolic fits in each layer and the vertical fluxes they give
-     * (hycom_fct3d.F90:131-228) */
+     * (hycom_fct3d.F90:135-288) */
     for (int i = 0; i < nip; i++) {
       if (!WET(i)) continue;
       a[0] = C(fld, i, 0);
@@ -899,7 +899,7 @@ static void hyc_fct3d(const fo_hyc_grid *g, double *fld3d, int numfld, const dou
       wtold = wtold + col1;
       bfore = bfore + col2;
     }
-    /* low-order and antidiffusive fluxes, local bounds (hycom_fct3d.F90:249-290) */
+    /* low-order and antidiffusive fluxes, local bounds (hycom_fct3d.F90:311-379) */
     for (int i = 0; i < nip; i++) {
       for (int e = 0; e < NPP; e++)
         for (int k = 0; k < kdm; k++) E(flx, i, e, k) = E(uan, i, e, k) = LAND_SPVAL;
@@ -928,7 +928,7 @@ static void hyc_fct3d(const fo_hyc_grid *g, double *fld3d, int numfld, const dou
         }
       }
     }
-    /* the low-order update (292-311) */
+    /* the low-order update (380-388) */
     for (int i = 0; i < nip; i++) {
       if (!WET(i)) continue;
       for (int k = 0; k < kdm; k++) {
@@ -941,7 +941,7 @@ static void hyc_fct3d(const fo_hyc_grid *g, double *fld3d, int numfld, const dou
         C(fld, i, k) = (C(fld, i, k) * onemu + bounded) / (onemu + C(thkn, i, k));
       }
     }
-    /* limiter ratios with the bounds widened by the low-order neighbours (313-334) */
+    /* limiter ratios with the bounds widened by the low-order neighbours (389-436) */
     for (int i = 0; i < nip; i++) {
       if (!WET(i)) continue;
       for (int k = 0; k < kdm; k++) {
@@ -959,7 +959,7 @@ static void hyc_fct3d(const fo_hyc_grid *g, double *fld3d, int numfld, const dou
         C(fln, i, k) = (C(fld, i, k) - C(fmn, i, k)) * C(thkn, i, k) / (totou + epsil) * g->rarea[i];
       }
     }
-    /* the limited antidiffusive update (336-371) */
+    /* the limited antidiffusive update (437-481) */
     for (int i = 0; i < nip; i++) {
       if (!WET(i)) continue;
       for (int k = 0; k < kdm; k++) {
@@ -979,7 +979,7 @@ static void hyc_fct3d(const fo_hyc_grid *g, double *fld3d, int numfld, const dou
         C(fld, i, k) = (C(fld, i, k) * onemu + bounded) / (onemu + C(thkn, i, k));
       }
     }
-    /* global conservation (recovr, 373-405): the shift of the mean */
+    /* global conservation (recovr, 482-524): the shift of the mean */
     double after = 0., wtnew = 0.;
     for (int i = 0; i < nip; i++) {
       if (!WET(i)) continue;
