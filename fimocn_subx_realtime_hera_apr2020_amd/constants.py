@@ -52,3 +52,11 @@ DPSIG64 = [
 OCN_DPMIN32 = [2.0, 3.0, 4.0, 5.0, 6.0, 8.0, 10.0, 12.0, 15.0, 18.0, 21.0, 25.0, 29.0, 33.0, 38.0, 43.0,
                48.0, 54.0, 60.0, 66.0, 73.0, 81.0, 90.0, 101.0, 114.0, 129.0, 146.0, 166.0, 189.0, 215.0,
                244.0, 276.0]
+
+# target densities (sigma_1, kg/m^3 - 1000), surface layer first: the first
+# 32 values of FIMrun/ocn_target.txt, which hycom_init.F90:558-559 reads into
+# hycom_constants' theta for kdm = 32
+OCN_TARGET32 = [20.820, 21.640, 22.460, 23.280, 24.100, 24.920, 25.740, 26.550, 27.340, 28.100,
+                28.820, 29.490, 30.100, 30.640, 31.100, 31.480, 31.780, 32.010, 32.180, 32.300,
+                32.380, 32.430, 32.460, 32.480, 32.500, 32.520, 32.540, 32.560, 32.580, 32.600,
+                32.620, 32.640]

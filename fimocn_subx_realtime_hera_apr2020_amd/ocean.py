@@ -306,3 +306,92 @@ def hycom_transport_case(grid, kdm: int = 32, numtr: int = 1, seed: int = EDGVAR
                 temp=np.ascontiguousarray(temp), saln=saln, passv=np.ascontiguousarray(passv), numtr=numtr,
                 batrop=batrop, temdff=0.02)
     return case
+
+
+def sigma1(t, s):
+    """Potential density sigma_1 (kg/m^3 - 1000) of temperature t (deg C) and
+    salinity s (psu): hycom_sigetc.F90's sigocn polynomial (a rational fit at
+    1000 dbar), evaluated in numpy to build inputs."""
+    t = np.asarray(t, np.float64)
+    s = np.asarray(s, np.float64)
+    p = 1.0e7 * 1.0e-4
+    pn = (9.9984085444849347e2 + t * (7.3471625860981584 + t * (-5.3211231792841769e-2 + 3.6492439109814549e-4 * t))
+          + s * (2.588057102399139 - 6.7168282786692355e-3 * t + 1.9203202055760151e-3 * s)
+          + p * (1.1798263740430364e-2 + 9.8920219266399117e-8 * t * t + 4.699664277175473e-6 * s
+                 + p * (-2.5862187075154352e-8 - 3.2921414007960662e-12 * t * t)))
+    pd = (1.0 + t * (7.2815210113327091e-3 - 4.4787265461983921e-5 * t
+                     + t * t * (3.3851002965802430e-7 + 1.3651202389758572e-10 * t))
+          + s * (1.7632126669040377e-3 - 8.8066583251206474e-6 * t - 1.8832689434804897e-10 * t * t * t
+                 + np.sqrt(np.maximum(0.0, s)) * (5.7463776745432097e-6 + 1.4716275472242334e-9 * t * t))
+          + p * (6.7103246285651894e-6 + p * t * (-2.4461698007024582e-17 * t * t - 9.1534417604289062e-18 * p)))
+    return pn / pd - 1000.0
+
+
+def salmin_of(targt, temmin: float = -2.0):
+    """hycom_init.F90:562-566: the salinity at which water of temperature
+    temmin has density targt (hycom_sigetc.F90 sofsig, the quadratic in s of
+    the sigma_1 fit's c1..c9)."""
+    c1, c2, c3, c4, c5, c6, c7, c8, c9 = (5.019935E+00, 1.470290E-02, 7.904674E-01, -6.861022E-03, -3.051459E-03,
+                                          3.099320E-05, 3.050299E-05, 1.046036E-04, 5.287567E-06)
+    t = temmin
+    aa = c8 + c9 * t
+    bb = c3 + c5 * t + c7 * t * t
+    cc = c1 + c2 * t + c4 * t * t + c6 * t * t * t - np.asarray(targt, np.float64)
+    return (-bb + np.sqrt(bb * bb - 4.0 * aa * cc)) / (2.0 * aa)
+
+
+def hycom_hybgen_case(grid, kdm: int = 32, seed: int = 20240617, numtr: int = 1, batrop: float = 900.0) -> dict:
+    """The inputs of HYCOM's hybrid grid generator (hycom_hybgn1.F90:9) on
+    top of hycom_cnuity_case's mask and thickness: in a quarter of the
+    columns the three bottom layers thinned to 0.2-5 m and in a tenth the
+    bottom two to 0-5 cm (massless layers for the sea-floor absorption), the
+    thickness taken from the layer above; temperature a smooth surface field
+    of 10-28 deg C decaying to 2 with depth (e-folding 800 m) plus
+    N(0, 0.3), salinity U[34, 36], density their sigma_1 (so layers are
+    lighter and denser than their targets in turn); velocities (both
+    leapfrog levels) N(0, 0.3) m/s; numtr passive tracers U[0, 1]; the
+    target densities of FIMrun/ocn_target.txt, salmin from them at -2 deg C,
+    the minimum thicknesses of ocn_dpmin.txt (m); latitude in degrees;
+    batrop.  Arrays in the reference layout (C order): dens, temp, saln
+    (nip, kdm); dp, uvel, vvel (2, nip, kdm); pres (nip, kdm+1); passv
+    (numtr, nip, kdm)."""
+    case = hycom_cnuity_case(grid, kdm=kdm, seed=seed)
+    rng = np.random.default_rng(seed + 11)
+    nip = case["wet"].shape[0]
+    lat = np.asarray(grid.lat, np.float64)
+    lon = np.asarray(grid.lon, np.float64)
+    dp = case["dp"].copy()
+    thin = np.flatnonzero(rng.uniform(size=nip) < 0.25)
+    nothing = np.flatnonzero(rng.uniform(size=nip) < 0.10)
+    for lv in range(2):
+        new = rng.uniform(0.2, 5.0, (thin.size, 3)) * ONEM
+        moved = dp[lv][thin][:, kdm - 3:].sum(axis=1) - new.sum(axis=1)
+        dp[lv][thin, kdm - 3:] = new
+        dp[lv][thin, kdm - 4] += moved
+        new = rng.uniform(0.0, 0.05, (nothing.size, 2)) * ONEM
+        moved = dp[lv][nothing][:, kdm - 2:].sum(axis=1) - new.sum(axis=1)
+        dp[lv][nothing, kdm - 2:] = new
+        dp[lv][nothing, kdm - 3] += moved
+    pres = np.zeros((nip, kdm + 1))
+    for k in range(kdm):
+        pres[:, k + 1] = pres[:, k] + dp[0][:, k]
+    depth = pres / ONEM
+    zmid = 0.5 * (depth[:, :-1] + depth[:, 1:])
+    sst = np.clip(10.0 + 18.0 * smooth_field(lat, lon, seed + 7), 10.0, 28.0)
+    temp = 2.0 + (sst[:, None] - 2.0) * np.exp(-zmid / 800.0) + 0.3 * rng.normal(size=(nip, kdm))
+    saln = rng.uniform(34.0, 36.0, (nip, kdm))
+    dens = sigma1(temp, saln)
+    uvel = 0.3 * rng.normal(size=(2, nip, kdm))
+    vvel = 0.3 * rng.normal(size=(2, nip, kdm))
+    passv = rng.uniform(0.0, 1.0, (numtr, nip, kdm))
+    targt = np.array(C.OCN_TARGET32, np.float64)
+    dpmin = np.array(C.OCN_DPMIN32, np.float64)
+    if kdm != 32:
+        x = np.linspace(0.0, 31.0, kdm)
+        targt = np.interp(x, np.arange(32), targt)
+        dpmin = np.interp(x, np.arange(32), dpmin)
+    return dict(kdm=kdm, wet=case["wet"], dp=np.ascontiguousarray(dp), pres=pres,
+                temp=np.ascontiguousarray(temp), saln=saln, dens=np.ascontiguousarray(dens),
+                uvel=np.ascontiguousarray(uvel), vvel=np.ascontiguousarray(vvel), passv=np.ascontiguousarray(passv),
+                targt=targt, salmin=salmin_of(targt), dpmin=dpmin, latij=np.degrees(lat), batrop=batrop,
+                numtr=numtr)

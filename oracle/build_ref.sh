@@ -15,6 +15,7 @@
 #   oracle/_ref/bin/hycomref              reference HYCOM ocean continuity equation
 #                                          (hycom_cnuity.F90), -fdefault-real-8
 #   oracle/_ref/bin/hyctrref              reference HYCOM tracer transport
+#   oracle/_ref/bin/hychybref             reference HYCOM hybrid grid generator (hybgn1)
 #                                          (hycom_transp3d.F90 with hycom_fct3d.F90
 #                                          and hycom_dffusn.F90), -fdefault-real-8
 #   oracle/_ref/bin/hycmomref             reference HYCOM momentum equation
@@ -111,13 +112,13 @@ done
 HYC_SRC="$CNTL_SRC $S/cntl/module_constants.F90 $H/global_bounds.F90 $S/dummygptl/gptl.F90 $H/findmxmn.F90
   $H/stencilprint.F90 $H/stenedgprint.F90 $H/hycom_control.F90 $H/hycom_constants.F90 $H/hycom_variables.F90
   $H/hycom_sverbal.F90 $H/hycom_diag.F90 $H/hycom_cnuity.F90 $H/hycom_edgvar.F90 $H/hycom_dissip.F90
-  $H/hycom_momtum.F90 $H/hycom_fct3d.F90 $H/hycom_dffusn.F90 $H/hycom_transp3d.F90 $H/GetIpnGlobal.F90
-  $H/its2time.F90"
+  $H/hycom_momtum.F90 $H/hycom_fct3d.F90 $H/hycom_dffusn.F90 $H/hycom_transp3d.F90 $H/hycom_sigetc.F90
+  $H/hycom_hybgn1.F90 $H/GetIpnGlobal.F90 $H/its2time.F90"
 OD=$OUT/obj_hycom
 FL="-cpp -DSERIAL -O2 -ffp-contract=off -fdefault-real-8 -fdefault-double-8 -I$H -I$S/dummygptl"
 compile "$OD" "$FL" $HYC_SRC "$HERE/ref/hycomref.F90" "$HERE/ref/hycedgref.F90" "$HERE/ref/hycmomref.F90" \
-  "$HERE/ref/hyctrref.F90"
-for drv in hycomref hycedgref hycmomref hyctrref; do
+  "$HERE/ref/hyctrref.F90" "$HERE/ref/hychybref.F90"
+for drv in hycomref hycedgref hycmomref hyctrref hychybref; do
   objs=""
   for f in $HYC_SRC "$HERE/ref/$drv.F90"; do b=$(basename "${f%.*}"); objs="$objs $OD/$b.o"; done
   $FC -o "$OUT/bin/$drv" $objs

@@ -2,10 +2,12 @@
  * continuity equation (FIMsrc/fim/horizontal/hycom_cnuity.F90, subroutine
  * cnuity), of its edge interpolation (hycom_edgvar.F90, bclin_edgvar with
  * uvedge and blokf), of its momentum equation (hycom_momtum.F90 momtum
- * with hycom_dissip.F90's del4prep and dissip) and of its tracer transport
+ * with hycom_dissip.F90's del4prep and dissip), of its tracer transport
  * (hycom_transp3d.F90 transp0/1/2 with hycom_fct3d.F90's fct3d and
- * hycom_dffusn.F90's dffusn_lyr).  TEST INFRASTRUCTURE ONLY: the checker the HIP kernels
- * are compared with; the product never links it.
+ * hycom_dffusn.F90's dffusn_lyr) and of its hybrid grid generator
+ * (hycom_hybgn1.F90 hybgn1 with hycom_sigetc.F90's equation of state).
+ * TEST INFRASTRUCTURE ONLY: the checker the HIP kernels are compared with;
+ * the product never links it.
  *
  * Every loop follows the reference operation for operation, in the
  * reference loop order, so that with fp contraction off it reproduces the
@@ -1076,4 +1078,541 @@ void fo_hycom_transp2(const fo_hyc_grid *g, int numtr, double batrop, double tem
     hyc_dffusn_lyr(g, saln, dpfinl, temdff * batrop * BCLIN_FRQ);
   }
   free(vertfx); free(all);
+}
+
+/* ------------------------------------------------------------------------
+ * hycom_hybgn1.F90 (hybgn1, hybgn1_1d, ppmad3, pcwise, equilb) with
+ * hycom_sigetc.F90's sigocn, dsigdt and dsigds.  The column routines use
+ * 1-based arrays, as the reference does (element 0 unused). */
+static const double HYB_PREF = 1.e7;          /* hycom_constants.F90:56 (sigma_1) */
+static const double HYB_C2 = 1.470290E-02, HYB_C3 = 7.904674E-01, HYB_C4 = -6.861022E-03;
+static const double HYB_C5 = -3.051459E-03, HYB_C6 = 3.099320E-05, HYB_C7 = 3.050299E-05;
+static const double HYB_C8 = 1.046036E-04, HYB_C9 = 5.287567E-06;
+
+/* hycom_sigetc.F90 sigocn: potential density (sigma_1) of t, s */
+static double hyb_sigocn(double t, double s) {
+  const double k01 = 9.9984085444849347e2, a1 = 7.3471625860981584, a2 = -5.3211231792841769e-2,
+               a3 = 3.6492439109814549e-4, b1 = 2.588057102399139, b2 = -6.7168282786692355e-3,
+               b3 = 1.9203202055760151e-3, cc1 = 1.1798263740430364e-2, cc2 = 9.8920219266399117e-8,
+               cc3 = 4.699664277175473e-6, cc4 = -2.5862187075154352e-8, cc5 = -3.2921414007960662e-12;
+  const double k02 = 1.0, d1 = 7.2815210113327091e-3, d2 = -4.4787265461983921e-5, d3 = 3.3851002965802430e-7,
+               d4 = 1.3651202389758572e-10, e1 = 1.7632126669040377e-3, e2 = -8.8066583251206474e-6,
+               e3 = -1.8832689434804897e-10, e4 = 5.7463776745432097e-6, e5 = 1.4716275472242334e-9,
+               f1 = 6.7103246285651894e-6, f2 = -2.4461698007024582e-17, f3 = -9.1534417604289062e-18;
+  const double p = HYB_PREF * 1.e-4;
+  const double pn = k01 + t * (a1 + t * (a2 + a3 * t)) + s * (b1 + b2 * t + b3 * s) +
+                    p * (cc1 + cc2 * t * t + cc3 * s + p * (cc4 + cc5 * t * t));
+  const double pd = k02 + t * (d1 + d2 * t + t * t * (d3 + d4 * t)) +
+                    s * (e1 + e2 * t + e3 * t * t * t + sqrt(dmax2(0., s)) * (e4 + e5 * t * t)) +
+                    p * (f1 + p * t * (f2 * t * t + f3 * p));
+  return pn / pd - 1000.;
+}
+static double hyb_dsigdt(double t, double s) {
+  return HYB_C2 + s * (HYB_C5 + HYB_C9 * s) + 2. * t * (HYB_C4 + HYB_C7 * s + 1.5 * HYB_C6 * t);
+}
+static double hyb_dsigds(double t, double s) {
+  return HYB_C3 + 2. * s * (HYB_C8 + HYB_C9 * t) + t * (HYB_C5 + t * HYB_C7);
+}
+
+/* the parabola of a cell (value y, neighbours yl/yr at its faces), with the
+ * over/undershoot fix the reference applies in ppmad3, pcwise and fct3d */
+static void hyb_parab(double y, double yl, double yr, double *a, double *b, double *c) {
+  *a = 1.5 * y - .25 * (yl + yr);
+  *b = yr - yl;
+  *c = 6. * (.5 * (yl + yr) - y);
+  if (fabs(yr - yl) < 6. * fabs(.5 * (yl + yr) - y)) {
+    if (fabs(yr - yl) > 2. * fabs(.5 * (yl + yr) - y)) {
+      if ((yr - yl) * (.5 * (yl + yr) - y) > 0.) {
+        *a = .25 * (3. * y + yl);
+        *c = 3. * (y - yl);
+        *b = *c;
+      } else {
+        *a = .25 * (3. * y + yr);
+        *c = 3. * (y - yr);
+        *b = -*c;
+      }
+    } else {
+      *a = y;
+      *b = 0.;
+      *c = 0.;
+    }
+  }
+}
+
+/* hycom_hybgn1.F90 ppmad3(x, dx, y): three cells x(1..4), displacements
+ * dx(1..4), values y(1..3) updated */
+static void hyb_ppmad3(const double *x, const double *dx, double *y) {
+  const double onemu = .098, athird = 1. / 3.;
+  double yold[4], ytmp[4];
+  for (int n = 1; n <= 3; n++) {
+    yold[n] = y[n];
+    ytmp[n] = y[n] * (x[n + 1] - x[n]);
+  }
+  const double dxlft = dmax2(onemu, x[2] - x[1]), dxmid = dmax2(onemu, x[3] - x[2]);
+  const double dxrgt = dmax2(onemu, x[4] - x[3]);
+  const double yl = (dxlft * yold[2] + dxmid * yold[1]) / (dxlft + dxmid);
+  const double yr = (dxrgt * yold[2] + dxmid * yold[3]) / (dxrgt + dxmid);
+  double a, b, c;
+  hyb_parab(yold[2], yl, yr, &a, &b, &c);
+  if (dx[2] > 0.) {
+    const double wdth = dx[2] / (x[3] - x[2]);
+    const double slab = (x[3] - x[2]) * wdth * (a + b * .5 * (wdth - 1.) + c * (.25 - wdth * (.5 - wdth * athird)));
+    ytmp[2] = ytmp[2] - slab;
+    ytmp[1] = ytmp[1] + slab;
+  }
+  if (dx[3] < 0.) {
+    const double wdth = -dx[3] / (x[3] - x[2]);
+    const double slab = (x[3] - x[2]) * wdth * (a + b * .5 * (1. - wdth) + c * (.25 - wdth * (.5 - wdth * athird)));
+    ytmp[2] = ytmp[2] - slab;
+    ytmp[3] = ytmp[3] + slab;
+  }
+  for (int n = 1; n <= 3; n++) {
+    const double dxnew = x[n + 1] - x[n] + dx[n + 1] - dx[n];
+    if (dxnew > 0.) y[n] = ytmp[n] / dxnew;
+  }
+}
+
+/* hycom_hybgn1.F90 pcwise(remap_optn, kk, xold, fldold, xnew, fldnew,
+ * miglim) with miglim = -1 (no migration limit: hybgn1_1d's only use),
+ * remap_optn 1 (PCM) or 3 (PPM); arrays 1-based, w a work area of
+ * 11*(kk+2) */
+static void hyb_pcwise(int optn, int kk, const double *xold, const double *fldold, const double *xnew,
+                       double *fldnew, double *w) {
+  const double athird = 1. / 3., small = 1.e-11;
+  const int n = kk + 2;
+  double *zold = w, *znew = w + n, *delz = w + 2 * n, *fco = w + 3 * n, *fcn = w + 4 * n;
+  double *vertfx = w + 5 * n, *vertdv = w + 6 * n, *a = w + 7 * n, *b = w + 8 * n, *c = w + 9 * n;
+  if (xold[1] < xold[kk + 1]) {
+    for (int k = 1; k <= kk + 1; k++) {
+      zold[k] = xold[k];
+      znew[k] = xnew[k];
+    }
+  } else {
+    for (int k = 1; k <= kk + 1; k++) {
+      zold[k] = -xold[k];
+      znew[k] = -xnew[k];
+    }
+  }
+  for (int k = 1; k <= kk + 1; k++) delz[k] = znew[k] - zold[k];
+  for (int k = 1; k <= kk; k++) {
+    fco[k] = dmax2(0., zold[k + 1] - zold[k]);
+    fcn[k] = dmax2(0., znew[k + 1] - znew[k]);
+  }
+  for (int k = 1; k <= kk; k++) fldnew[k] = fldold[k];
+  /* zero-width cells take their neighbours' values */
+  for (int k = kk - 1; k >= 1; k--) fldnew[k] = (fldnew[k] * fco[k] + fldnew[k + 1] * small) / (fco[k] + small);
+  for (int k = 2; k <= kk; k++) fldnew[k] = (fldnew[k] * fco[k] + fldnew[k - 1] * small) / (fco[k] + small);
+  a[1] = fldnew[1];
+  b[1] = c[1] = 0.;
+  a[kk] = fldnew[kk];
+  b[kk] = c[kk] = 0.;
+  for (int k = 2; k <= kk - 1; k++) {
+    if (optn == 1) {
+      a[k] = fldnew[k];
+      b[k] = c[k] = 0.;
+    } else {
+      const double yl = (dmax2(small, fco[k - 1]) * fldnew[k] + dmax2(small, fco[k]) * fldnew[k - 1]) /
+                        (dmax2(small, fco[k - 1]) + dmax2(small, fco[k]));
+      const double yr = (dmax2(small, fco[k + 1]) * fldnew[k] + dmax2(small, fco[k]) * fldnew[k + 1]) /
+                        (dmax2(small, fco[k + 1]) + dmax2(small, fco[k]));
+      hyb_parab(fldnew[k], yl, yr, &a[k], &b[k], &c[k]);
+    }
+  }
+  /* the flux through each interface from the upstream slab it sweeps */
+  for (int k = 2; k <= kk; k++) {
+    double slab = 0., amount = 0.;
+    vertfx[k] = 0.;
+    if (delz[k] > 0.) {
+      int lyr = k - 1;
+      for (;;) {
+        lyr = lyr + 1;
+        if (slab >= delz[k]) break;
+        if (fco[lyr] > 0.) {
+          const double dslab = dmin2(slab + fco[lyr], delz[k]) - dmin2(slab, delz[k]);
+          const double dx = dslab / fco[lyr];
+          const double fcdx = a[lyr] + b[lyr] * .5 * (dx - 1.) + c[lyr] * (.25 - dx * (.5 - dx * athird));
+          amount = amount + fcdx * dslab;
+          slab = slab + dslab;
+        }
+        if (!(lyr < kk)) break;
+      }
+    } else if (delz[k] < 0.) {
+      /* the reference's walk stops after layer 2 (goto 25 while lyr > 2):
+       * layer 1 is swept only from interface 2 */
+      int lyr = k;
+      for (;;) {
+        lyr = lyr - 1;
+        if (slab >= -delz[k]) break;
+        if (fco[lyr] > 0.) {
+          const double dslab = dmin2(slab + fco[lyr], -delz[k]) - dmin2(slab, -delz[k]);
+          const double dx = dslab / fco[lyr];
+          const double fcdx = a[lyr] + b[lyr] * .5 * (1. - dx) + c[lyr] * (.25 - dx * (.5 - dx * athird));
+          amount = amount + fcdx * dslab;
+          slab = slab + dslab;
+        }
+        if (!(lyr > 2)) break;
+      }
+    }
+    if (slab != 0.) vertfx[k] = -delz[k] * amount / slab;
+  }
+  vertfx[1] = 0.;
+  vertfx[kk + 1] = 0.;
+  for (int k = 1; k <= kk; k++) vertdv[k] = vertfx[k + 1] - vertfx[k];
+  for (int k = 1; k <= kk; k++) {
+    const double amount = fldnew[k] * fco[k] - vertdv[k];
+    fldnew[k] = (fldnew[k] * small + amount) / (small + fcn[k]);
+  }
+}
+
+/* hycom_hybgn1.F90 equilb(nstep, pres, thet, temp, saln, kdm): heat
+ * diffusion (3 passes) to inflate thin layers, then scenario 4 (the others
+ * are commented out in the reference); thet is read only; w a work area of
+ * 7*(kdm+2) */
+static void hyb_equilb(int kdm, double *pres, const double *thet, double *temp, double *saln, double *w) {
+  const double tenm = 10. * ONEM, tencm = 0.1 * ONEM, onemu = 1.e-6 * ONEM;
+  const double slak = .25, dffudt = .1;
+  const int n = kdm + 2;
+  double *pnew = w, *pwrk = w + n, *heatfx = w + 2 * n, *tdz = w + 3 * n, *sdz = w + 4 * n;
+  double *twrk = w + 5 * n, *swrk = w + 6 * n;
+  for (int k = 1; k <= kdm + 1; k++) pnew[k] = pwrk[k] = pres[k];
+  for (int k = 1; k <= kdm; k++) {
+    twrk[k] = temp[k];
+    swrk[k] = saln[k];
+    tdz[k] = dmax2(EPSIL, pres[k + 1] - pres[k]) * temp[k];
+    sdz[k] = dmax2(EPSIL, pres[k + 1] - pres[k]) * saln[k];
+  }
+  heatfx[kdm] = 0.;
+  heatfx[1] = 0.;
+  for (int iter = 1; iter <= 3; iter++) {
+    for (int k = 2; k <= kdm - 1; k++) {
+      heatfx[k] = 0.;
+      if (pwrk[k] > pwrk[1] + ONEM && pwrk[k + 1] < pwrk[kdm + 1] - ONEM) {
+        const double cnv = ONEM;
+        heatfx[k] = dffudt * cnv * cnv * dmin2(thet[k + 1] - thet[k], thet[k] - thet[k - 1]) /
+                    dmax2(.01 * cnv, pwrk[k + 1] - pwrk[k]);
+        const double plimit1 = dmax2(pwrk[k] - tenm, .5 * (pwrk[k] + pwrk[k - 1]));
+        const double plimit2 = dmin2(pwrk[k + 1] + tenm, .5 * (pwrk[k + 1] + pwrk[k + 2]));
+        heatfx[k] = dmin2(dmin2(dmax2(0., heatfx[k]), dmax2(0., (plimit2 - pwrk[k + 1]) * (thet[k + 1] - thet[k]))),
+                          dmax2(0., (pwrk[k] - plimit1) * (thet[k] - thet[k - 1])));
+      }
+    }
+    for (int k = 1; k <= kdm - 1; k++) {
+      if (thet[k + 1] > thet[k]) {
+        pnew[k + 1] = pwrk[k + 1] + (heatfx[k] - heatfx[k + 1]) / (thet[k + 1] - thet[k]);
+        const double delp = pnew[k + 1] - pwrk[k + 1];
+        double dtdz, dsdz;
+        if (delp > 0.) {
+          dtdz = twrk[k + 1] * delp;
+          dsdz = swrk[k + 1] * delp;
+        } else {
+          dtdz = twrk[k] * delp;
+          dsdz = swrk[k] * delp;
+        }
+        tdz[k] = tdz[k] + dtdz;
+        tdz[k + 1] = tdz[k + 1] - dtdz;
+        sdz[k] = sdz[k] + dsdz;
+        sdz[k + 1] = sdz[k + 1] - dsdz;
+      }
+    }
+    for (int k = 1; k <= kdm; k++) {
+      if (pnew[k + 1] > pnew[k]) {
+        twrk[k] = tdz[k] / (pnew[k + 1] - pnew[k]);
+        swrk[k] = sdz[k] / (pnew[k + 1] - pnew[k]);
+      }
+    }
+    for (int k = 1; k <= kdm + 1; k++) pwrk[k] = pnew[k];
+  }
+  /* scenario 4: thick-(thin-thin)-thick */
+  for (int k = 4; k <= kdm - 2; k++) {
+    if (pnew[k - 2] < pnew[1] + tencm || pnew[k + 2] > pnew[kdm + 1] - tencm) continue;
+    const double dp1 = pnew[k - 1] - pnew[k - 2], dp2 = pnew[k] - pnew[k - 1];
+    const double dp3 = pnew[k + 1] - pnew[k], dp4 = pnew[k + 2] - pnew[k + 1];
+    const double th1 = thet[k - 2], th2 = thet[k - 1], th3 = thet[k], th4 = thet[k + 1];
+    if (dp1 > dp2 + dp3 && dp4 > dp2 + dp3) {
+      const double goal = .5 * (dp2 + dp3 + dmin2(dp1, dp4));
+      double dis2 = (goal - dp2 - dp3) * slak, dis4 = dis2;
+      if (th2 > th1 && th3 > th2 && th4 > th3) {
+        const double ratio = (th4 - th3) / (th2 - th1);
+        if (ratio > 1.) dis4 = dis2 / ratio;
+        else dis2 = dis4 * ratio;
+      }
+      pnew[k - 1] = pnew[k - 1] - dis2;
+      pnew[k + 1] = pnew[k + 1] + dis4;
+      tdz[k - 2] = tdz[k - 2] - dis2 * twrk[k - 2];
+      tdz[k - 1] = tdz[k - 1] + dis2 * twrk[k - 2];
+      tdz[k] = tdz[k] + dis4 * twrk[k + 1];
+      tdz[k + 1] = tdz[k + 1] - dis4 * twrk[k + 1];
+      sdz[k - 2] = sdz[k - 2] - dis2 * swrk[k - 2];
+      sdz[k - 1] = sdz[k - 1] + dis2 * swrk[k - 2];
+      sdz[k] = sdz[k] + dis4 * swrk[k + 1];
+      sdz[k + 1] = sdz[k + 1] - dis4 * swrk[k + 1];
+    }
+  }
+  for (int k = 1; k <= kdm; k++) {
+    if (pnew[k + 1] - pnew[k] > onemu) {
+      temp[k] = tdz[k] / (pnew[k + 1] - pnew[k]);
+      saln[k] = sdz[k] / (pnew[k + 1] - pnew[k]);
+    }
+    pres[k] = pnew[k];
+  }
+}
+
+/* hycom_hybgn1.F90 hybgn1_1d: one column (1-based arrays); trc (numtr
+ * columns of kdm+2) moved only with dotrcr; w a work area of
+ * (20 + 2*numtr)*(kdm+2) */
+static void hyb_1d(int kdm, const double *targt, const double *salmin, const double *dpmin, double batrop,
+                   double lati, double *pr, double *tem, double *sal, double *dns, double *u, double *v,
+                   double *trc, int numtr, int dotrcr, double *w) {
+  const double tencm = 0.1 * ONEM, onemm = 0.001 * ONEM, r_onem = 1. / ONEM, huuge = 1.e33;
+  const double tfreez = -2.2, sigjmp = .01, scalt = -30., scals = 10.;
+  const double slak = 1. / 86400.;
+  const double dlt = batrop * 30;                   /* batrop*bclin_frq (hycom_control.F90:9) */
+  const int n = kdm + 2;
+  double *uold = w, *vold = w + n, *pold = w + 2 * n, *unew = w + 3 * n, *vnew = w + 4 * n;
+  double *trold = w + 5 * n, *trnew = w + (5 + numtr) * n, *wk = w + (5 + 2 * numtr) * n;
+  double p_hat = 0., q, displ[5];
+#define SLAKF() ((1. + 9. * dmax2(0., 1. - .01 * r_onem * p_hat)) * slak * dlt)
+  for (int k = 1; k <= kdm; k++) {
+    uold[k] = u[k];
+    vold[k] = v[k];
+    if (dotrcr)
+      for (int nt = 0; nt < numtr; nt++) trold[nt * n + k] = trc[nt * n + k];
+  }
+  /* absorb near-massless layers on the sea floor in the layer above */
+  int kp = 1;
+  for (int k = 2; k <= kdm; k++) {
+    if (pr[k] < pr[kdm + 1] - tencm) {
+      kp = k;
+    } else {
+      const double q1 = dmax2(EPSIL, pr[k] - pr[kp]), q2 = dmax2(0., pr[k + 1] - pr[k]);
+      q = q1 / (q1 + q2);
+      tem[kp] = tem[kp] * q + tem[k] * (1. - q);
+      sal[kp] = sal[kp] * q + sal[k] * (1. - q);
+      tem[k] = tem[kp];
+      sal[k] = dmax2(sal[kp], salmin[k]);
+      if (dotrcr)
+        for (int nt = 0; nt < numtr; nt++) {
+          double *t = trold + nt * n;
+          t[kp] = t[kp] * q + t[k] * (1. - q);
+          t[k] = t[kp];
+        }
+    }
+  }
+  dns[kp] = hyb_sigocn(tem[kp], sal[kp]);
+  for (int k = kp + 1; k <= kdm; k++) {
+    dns[k] = dns[kp];
+    pr[k] = pr[kdm + 1];
+  }
+  /* a layer touching the sea floor too light: split it, the lower part at
+   * its target density, the upper part joined to the layer above */
+  if (kp != 1) {
+    const int k = kp;
+    if (!(dns[k] <= dmax2(dns[k - 1], targt[k - 1]) || dns[k] >= targt[k])) {
+      const double tem_ = tem[k], sal_ = sal[k];
+      const double dsgdt = hyb_dsigdt(tem_, sal_) * scalt, dsgds = hyb_dsigds(tem_, sal_) * scals;
+      q = 1. / (dsgdt + dsgds);
+      const double sal_lo = dmin2(sal_ + (targt[k] - dns[k]) * q * scals, dmax2(sal[k - 1], sal_ + .01 * scals));
+      const double tem_lo = dmax2(tem_ + (targt[k] - dns[k]) * q * scalt, tfreez);
+      const double rho_lo = hyb_sigocn(tem_lo, sal_lo);
+      const double rho_up = dmax2(dns[k - 1], dmin2(targt[k - 1], dns[k]));
+      q = (rho_lo - dns[k]) / dmax2(EPSIL, rho_lo - rho_up);
+      if (!(q < 0. || q > 1.)) {
+        p_hat = pr[k] * (1. - q) + pr[k + 1] * q;
+        q = (pr[k + 1] - p_hat) / dmax2(EPSIL, p_hat - pr[k]);
+        const double sal_up = sal_ * (1. + q) - sal_lo * q;
+        const double tem_up = tem_ * (1. + q) - tem_lo * q;
+        q = (p_hat - pr[k]) / dmax2(p_hat - pr[k - 1], EPSIL);
+        sal[k - 1] = sal_up * q + sal[k - 1] * (1. - q);
+        tem[k - 1] = tem_up * q + tem[k - 1] * (1. - q);
+        if (dotrcr)
+          for (int nt = 0; nt < numtr; nt++) {
+            double *t = trold + nt * n;
+            t[k - 1] = t[k] * q + t[k - 1] * (1. - q);
+          }
+        dns[k - 1] = hyb_sigocn(tem[k - 1], sal[k - 1]);
+        tem[k] = tem_lo;
+        sal[k] = sal_lo;
+        dns[k] = rho_lo;
+        pr[k] = p_hat;
+      }
+    }
+  }
+  for (int k = 1; k <= kdm + 1; k++) pold[k] = pr[k];
+  /* restore isopycnic conditions by moving interfaces */
+  double dpsum = 0., dp0 = huuge;
+  for (int k = 1; k <= kdm; k++) {
+    const double dp0abv = dp0;
+    dp0 = dpmin[k] * ONEM;
+    if (k > 1) dp0 = dp0 * dmax2(.6, dmin2(1., (fabs(lati) + 5.) * .04));
+    if (4 * k < kdm) dp0 = dp0 * dmin2(1., pr[kdm + 1] / (200. * ONEM) + .4);
+    dpsum = dpsum + dp0;
+    int lower = 0;                   /* 1: go to the lower-interface move (label 5) */
+    if (k == 1) {
+      p_hat = dp0;
+      if (p_hat > pr[2]) {           /* layer 1 too thin */
+        p_hat = dmin2(p_hat, pr[2] + dmax2(ONECM, SLAKF() * (p_hat - pr[2])));
+        lower = 1;
+      } else if (p_hat < pr[2]) {    /* too thick: layer 1 water into layer 2 */
+        p_hat = dmax2(p_hat, pr[2] + dmin2(-ONECM, SLAKF() * (p_hat - pr[2])));
+        q = (pr[2] - p_hat) / dmax2(pr[3] - p_hat, EPSIL);
+        sal[2] = sal[2] * (1. - q) + sal[1] * q;
+        tem[2] = tem[2] * (1. - q) + tem[1] * q;
+        dns[2] = hyb_sigocn(tem[2], sal[2]);
+        pr[2] = p_hat;
+      }
+      if (!lower) continue;
+    } else {
+      if (pr[k] == pr[kdm + 1]) dns[k] = dmax2(targt[k], dns[k]);
+      if (pr[k] > pr[kdm + 1] - ONECM) continue;
+      p_hat = dpsum;
+      if (k < kdm && p_hat > pr[k + 1]) {          /* lower interface too close to the surface */
+        p_hat = dmin2(p_hat, pr[k + 1] + dmax2(ONECM, SLAKF() * (p_hat - pr[k + 1])));
+        lower = 1;
+      } else {
+        if (fabs(dns[k] - targt[k]) < .1 * sigjmp) continue;
+        if (dns[k] <= targt[k]) {                  /* too light (label 7) */
+          if (k >= kdm || pr[k + 1] > pr[kdm + 1] - onemm) continue;
+          q = (dns[k] - targt[k]) / dmax2(dns[k + 1] - targt[k], sigjmp * 10.);
+          p_hat = pr[k + 1] * (1. - q) + pr[k] * q;
+          p_hat = dmin2(p_hat, dmax2(.5 * (pr[k] + pr[k + 2]), 2. * pr[k + 1] - pr[k]));
+          p_hat = dmin2(p_hat, pr[k + 1] + dmax2(ONECM, SLAKF() * (p_hat - pr[k + 1])));
+          lower = 1;
+        } else {                                   /* too dense */
+          if (k != 2) {
+            q = (targt[k] - dns[k]) / dmax2(targt[k] - dns[k - 1], sigjmp * 10.);
+            p_hat = pr[k] * (1. - q) + pr[k + 1] * q;
+            p_hat = dmax2(p_hat, pr[k - 1] + dp0abv);
+            p_hat = dmin2(p_hat, .5 * (pr[k - 1] + pr[k + 1]));
+            if (p_hat < pr[k]) {                   /* upper interface up: entrain layer k-1 */
+              p_hat = dmax2(dmax2(p_hat, pr[k - 1]), pr[k] + dmin2(-ONECM, SLAKF() * (p_hat - pr[k])));
+              if (fabs(dns[k - 1] - targt[k - 1]) > .1 * sigjmp) {
+                displ[1] = 0.;
+                displ[2] = 0.;
+                displ[3] = p_hat - pr[k];
+                displ[4] = 0.;
+                hyb_ppmad3(pr + k - 3, displ, sal + k - 3);
+                hyb_ppmad3(pr + k - 3, displ, tem + k - 3);
+                dns[k - 1] = hyb_sigocn(tem[k - 1], sal[k - 1]);
+                dns[k] = hyb_sigocn(tem[k], sal[k]);
+              } else {
+                q = (pr[k] - p_hat) / dmax2(pr[k + 1] - p_hat, EPSIL);
+                sal[k] = sal[k] * (1. - q) + sal[k - 1] * q;
+                tem[k] = tem[k] * (1. - q) + tem[k - 1] * q;
+                dns[k] = hyb_sigocn(tem[k], sal[k]);
+              }
+              pr[k] = p_hat;
+            } else if (p_hat > pr[k]) {            /* upper interface down: layer k into k-1 */
+              p_hat = dmin2(dmin2(p_hat, pr[k + 1]), pr[k] + dmax2(ONECM, SLAKF() * (p_hat - pr[k])));
+              if (k < kdm) {
+                displ[1] = 0.;
+                displ[2] = p_hat - pr[k];
+                displ[3] = 0.;
+                displ[4] = 0.;
+                hyb_ppmad3(pr + k - 2, displ, sal + k - 2);
+                hyb_ppmad3(pr + k - 2, displ, tem + k - 2);
+                dns[k - 1] = hyb_sigocn(tem[k - 1], sal[k - 1]);
+                dns[k] = hyb_sigocn(tem[k], sal[k]);
+              } else {
+                q = (p_hat - pr[k]) / dmax2(p_hat - pr[k - 1], EPSIL);
+                sal[k - 1] = sal[k - 1] * (1. - q) + sal[k] * q;
+                tem[k - 1] = tem[k - 1] * (1. - q) + tem[k] * q;
+                dns[k - 1] = hyb_sigocn(tem[k - 1], sal[k - 1]);
+              }
+              pr[k] = p_hat;
+            }
+          }
+          /* label 6: inflate layer k by lowering its lower interface? */
+          p_hat = pr[k] + dpmin[2] * ONEM;
+          if (k < kdm && pr[k + 1] < pr[kdm + 1] - onemm && pr[k + 1] < p_hat) {
+            p_hat = dmin2(p_hat, pr[k + 1] + dmax2(ONECM, SLAKF() * (p_hat - pr[k + 1])));
+            lower = 1;
+          }
+          if (!lower) continue;
+        }
+      }
+    }
+    /* label 5: move the lower interface down to p_hat */
+    p_hat = dmin2(p_hat, pr[k + 2]);
+    if (p_hat > pr[k + 1] + onemm) {
+      if (k < kdm - 1 && fabs(dns[k + 1] - targt[k + 1]) > .1 * sigjmp) {
+        displ[1] = 0.;
+        displ[2] = p_hat - pr[k + 1];
+        displ[3] = 0.;
+        displ[4] = 0.;
+        hyb_ppmad3(pr + k - 1, displ, sal + k - 1);
+        hyb_ppmad3(pr + k - 1, displ, tem + k - 1);
+        dns[k] = hyb_sigocn(tem[k], sal[k]);
+        dns[k + 1] = hyb_sigocn(tem[k + 1], sal[k + 1]);
+      } else {
+        q = (p_hat - pr[k + 1]) / dmax2(p_hat - pr[k], EPSIL);
+        sal[k] = sal[k] * (1. - q) + sal[k + 1] * q;
+        tem[k] = tem[k] * (1. - q) + tem[k + 1] * q;
+        dns[k] = hyb_sigocn(tem[k], sal[k]);
+      }
+      pr[k + 1] = p_hat;
+    }
+  }
+#undef SLAKF
+  hyb_equilb(kdm, pr, dns, tem, sal, wk);
+  for (int k = 1; k <= kdm - 2; k++)
+    if (pr[k + 1] < pr[k] || pr[k + 1] > pr[k + 2]) pr[k + 1] = dmax2(pr[k], dmin2(pr[k + 1], pold[k + 2]));
+  hyb_pcwise(1, kdm, pold, uold, pr, unew, wk);
+  hyb_pcwise(1, kdm, pold, vold, pr, vnew, wk);
+  if (dotrcr)
+    for (int nt = 0; nt < numtr; nt++) hyb_pcwise(3, kdm, pold, trold + nt * n, pr, trnew + nt * n, wk);
+  for (int k = 1; k <= kdm; k++) {
+    u[k] = unew[k];
+    v[k] = vnew[k];
+    if (dotrcr)
+      for (int nt = 0; nt < numtr; nt++) trc[nt * n + k] = trnew[nt * n + k];
+  }
+}
+
+void fo_hycom_hybgn1(int nip, int kdm, const int *wet, int leap, const double *targt, const double *salmin,
+                     const double *dpmin, double batrop, const double *latij, double *dens, double *dp, double *pres,
+                     double *temp, double *saln, double *uvel, double *vvel, int numtr, double *passv) {
+  const int n = kdm + 2, dotrcr = passv != NULL && numtr > 0;
+  const size_t cz = (size_t)nip * kdm;
+  double *col = malloc((size_t)(7 + numtr) * n * sizeof(double));
+  double *w = malloc((size_t)(20 + 2 * numtr) * n * sizeof(double));
+  double *tg = malloc((size_t)3 * n * sizeof(double));
+  double *pr = col, *tem = col + n, *sal = col + 2 * n, *dns = col + 3 * n, *u = col + 4 * n, *v = col + 5 * n;
+  double *trc = col + 7 * n;
+  for (int k = 1; k <= kdm; k++) {
+    tg[k] = targt[k - 1];
+    tg[n + k] = salmin[k - 1];
+    tg[2 * n + k] = dpmin[k - 1];
+  }
+  double *dpl = dp + (size_t)(leap - 1) * cz, *ul = uvel + (size_t)(leap - 1) * cz, *vl = vvel + (size_t)(leap - 1) * cz;
+  for (int i = 0; i < nip; i++) {
+    if (wet[i] <= 0) continue;
+    pr[1] = pres[(size_t)i * (kdm + 1)];
+    for (int k = 1; k <= kdm; k++) {
+      const size_t ik = (size_t)i * kdm + k - 1;
+      dns[k] = dens[ik];
+      tem[k] = temp[ik];
+      sal[k] = saln[ik];
+      u[k] = ul[ik];
+      v[k] = vl[ik];
+      if (dotrcr)
+        for (int nt = 0; nt < numtr; nt++) trc[nt * n + k] = passv[nt * cz + ik];
+      pr[k + 1] = pr[k] + dpl[ik];
+    }
+    hyb_1d(kdm, tg, tg + n, tg + 2 * n, batrop, latij[i], pr, tem, sal, dns, u, v, trc, numtr, dotrcr, w);
+    for (int k = 1; k <= kdm; k++) {
+      const size_t ik = (size_t)i * kdm + k - 1;
+      dens[ik] = dns[k];
+      temp[ik] = tem[k];
+      saln[ik] = sal[k];
+      ul[ik] = u[k];
+      vl[ik] = v[k];
+      dpl[ik] = pr[k + 1] - pr[k];
+      if (dotrcr)
+        for (int nt = 0; nt < numtr; nt++) passv[nt * cz + ik] = trc[nt * n + k];
+      pres[(size_t)i * (kdm + 1) + k] = pr[k + 1];
+    }
+  }
+  free(col);
+  free(w);
+  free(tg);
 }

@@ -70,3 +70,16 @@ void fo_hycom_transp0(const fo_hyc_grid *g, int leapn, const double *dp, double 
 void fo_hycom_transp1(const fo_hyc_grid *g, double batrop, const double *massflx, double *cumuflx);
 void fo_hycom_transp2(const fo_hyc_grid *g, int numtr, double batrop, double temdff, double *temp, double *saln,
                       double *passv, const double *cumuflx, const double *dpinit, const double *dpfinl);
+
+/* hycom_hybgn1.F90 hybgn1(nstep, leap, targt, dens, dp, pres, temp, saln,
+ * uvel, vvel, latij[, passv_tr]) with hybgn1_1d, ppmad3, pcwise and
+ * equilb, and hycom_sigetc's sigocn, dsigdt, dsigds: the hybrid vertical
+ * coordinate restored column by column on the ocean cells.  targt, salmin,
+ * dpmin (kdm) are hycom_constants' theta (target densities), salmin and
+ * dpmin (m); batrop the barotropic step; latij (nip) latitude in degrees;
+ * dens, temp, saln (kdm,nip); dp, uvel, vvel (kdm,nip,2), level leap
+ * updated; pres (kdm+1,nip); passv (kdm,nip,numtr) moved when not NULL
+ * (hycom_run.F90:516 passes none). */
+void fo_hycom_hybgn1(int nip, int kdm, const int *wet, int leap, const double *targt, const double *salmin,
+                     const double *dpmin, double batrop, const double *latij, double *dens, double *dp, double *pres,
+                     double *temp, double *saln, double *uvel, double *vvel, int numtr, double *passv);

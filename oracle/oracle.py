@@ -326,3 +326,25 @@ class HycomCnuity:
                               *[e[n].ctypes.data_as(_PD) for n in ("geop", "montg", "spcifv")],
                               *[a[n].ctypes.data_as(_PD) for n in ("um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg")],
                               *[e[n].ctypes.data_as(_PD) for n in self.EDG_OUT])
+
+
+# ---- HYCOM hybrid vertical coordinate (hycom_oracle.c fo_hycom_hybgn1) ------
+HYB_FIELDS = ("dens", "dp", "pres", "temp", "saln", "uvel", "vvel", "passv")
+
+
+def hycom_hybgn1(case: dict, leap: int, with_tracers: bool = False) -> dict:
+    """The checker's hybgn1 (hycom_hybgn1.F90:9) on copies of the case's
+    arrays (ocean.hycom_hybgen_case): returns them updated.  with_tracers
+    moves passv as well (hycom_run.F90:516 passes no tracers)."""
+    build()
+    a = {n: np.ascontiguousarray(case[n], np.float64).copy() for n in HYB_FIELDS}
+    kdm, nip = case["kdm"], case["wet"].shape[0]
+    wet = np.ascontiguousarray(case["wet"], np.int32)
+    tab = {n: np.ascontiguousarray(case[n], np.float64) for n in ("targt", "salmin", "dpmin", "latij")}
+    P = lambda x: x.ctypes.data_as(_PD)
+    numtr = case["passv"].shape[0] if with_tracers else 0
+    lib().fo_hycom_hybgn1(ct.c_int(nip), ct.c_int(kdm), wet.ctypes.data_as(_PI), ct.c_int(leap), P(tab["targt"]),
+                          P(tab["salmin"]), P(tab["dpmin"]), ct.c_double(case["batrop"]), P(tab["latij"]),
+                          P(a["dens"]), P(a["dp"]), P(a["pres"]), P(a["temp"]), P(a["saln"]), P(a["uvel"]),
+                          P(a["vvel"]), ct.c_int(numtr), P(a["passv"]) if numtr else None)
+    return a

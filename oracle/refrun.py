@@ -298,3 +298,34 @@ def run_hycomref(glvl_path, case: dict, *, nsteps: int, thkdff: float = 0.05, bi
     os.remove(fin)
     os.remove(fout)
     return rec
+
+
+HYCH_MAGIC = 1213809480
+
+
+def run_hychybref(case: dict, *, leap: int = 1, with_tracers: bool = False, workdir=None) -> dict:
+    """Run the reference hybrid grid generator (oracle/ref/hychybref.F90,
+    hycom_hybgn1.F90 hybgn1) once on the inputs of ocean.hycom_hybgen_case;
+    returns the records (dens, dp, pres, temp, saln, uvel, vvel, passv)."""
+    workdir = workdir or os.path.join(REF, "runs")
+    os.makedirs(workdir, exist_ok=True)
+    kdm = case["kdm"]
+    nip = case["wet"].shape[0]
+    numtr = case["passv"].shape[0]
+    fin = os.path.join(workdir, "hych_state_%d.bin" % os.getpid())
+    fout = os.path.join(workdir, "hych_out_%d.bin" % os.getpid())
+    with open(fin, "wb") as f:
+        f.write(struct.pack("<8i", HYCH_MAGIC, kdm, nip, leap, numtr, int(with_tracers), 0, 0))
+        f.write(struct.pack("<d", case["batrop"]))
+        f.write(np.ascontiguousarray(case["wet"], dtype="<i4").tobytes())
+        for n in ("targt", "salmin", "dpmin", "latij", "dens", "dp", "pres", "temp", "saln", "uvel", "vvel", "passv"):
+            f.write(np.ascontiguousarray(case[n], dtype="<f8").tobytes())
+    r = subprocess.run([os.path.join(BIN, "hychybref"), fin, fout], cwd=workdir, capture_output=True, text=True,
+                       preexec_fn=_big_stack)
+    if r.returncode != 0:
+        raise RuntimeError("hychybref failed (%d): %s" % (r.returncode, (r.stdout + r.stderr)[-3000:]))
+    rec = read_records(fout)
+    rec["_stdout"] = r.stdout
+    os.remove(fin)
+    os.remove(fout)
+    return rec

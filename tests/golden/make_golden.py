@@ -287,6 +287,23 @@ def hycom_transport_reference():
         json.dump(out, f, indent=1, sort_keys=True)
 
 
+def hycom_hybgen_reference():
+    """Digests of the reference hybrid grid generator (hybgn1) at G2 and G3 x
+    32 layers on ocean.hycom_hybgen_case: level 1 without tracers (as
+    hycom_run calls it) and level 2 with one passive tracer."""
+    from fimocn_subx_realtime_hera_apr2020_amd import ocean
+    out = {"kdm": 32, "runs": {}}
+    for lv in (2, 3):
+        g = glvl.read_glvl(os.path.join(HERE, "glvl_G%d.dat" % lv))
+        case = ocean.hycom_hybgen_case(g, kdm=32)
+        for leap, wt in ((1, False), (2, True)):
+            r = refrun.run_hychybref(case, leap=leap, with_tracers=wt)
+            out["runs"]["G%d_leap%d%s" % (lv, leap, "_tr" if wt else "")] = {
+                k: sha(v) for k, v in r.items() if not k.startswith("_")}
+    with open(os.path.join(HERE, "hycom_hybgen_ref.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 if __name__ == "__main__":
     main()
     schemes()
@@ -296,3 +313,4 @@ if __name__ == "__main__":
     hycom_edgvar_reference()
     hycom_momtum_reference()
     hycom_transport_reference()
+    hycom_hybgen_reference()

@@ -290,3 +290,67 @@ def test_hycom_transport_properties(oracle_lib):
     h2 = oracle_lib.HycomCnuity(g, c)
     h2.transport()
     assert np.allclose(h2.t["passv"][0][wet], 0.25, rtol=0, atol=1e-13)
+
+
+HYB_RUNS = ((1, False), (2, True))
+
+
+@pytest.mark.parametrize("level", [2, 3])
+def test_hycom_hybgen_checker_equals_reference_digests(oracle_lib, level):
+    with open(os.path.join(GOLDEN, "hycom_hybgen_ref.json")) as f:
+        ref = json.load(f)
+    g = glvl.read_glvl(os.path.join(GOLDEN, "glvl_G%d.dat" % level))
+    case = ocean.hycom_hybgen_case(g, kdm=32)
+    for leap, wt in HYB_RUNS:
+        out = oracle_lib.hycom_hybgn1(case, leap, with_tracers=wt)
+        want = ref["runs"]["G%d_leap%d%s" % (level, leap, "_tr" if wt else "")]
+        assert set(out) == set(want)
+        for n, v in out.items():
+            assert _sha(v) == want[n], (leap, wt, n)
+
+
+def test_hycom_hybgen_checker_equals_live_reference_g4(oracle_lib):
+    from oracle import refrun
+    if not os.path.exists(os.path.join(refrun.BIN, "hychybref")):
+        pytest.skip("reference build (oracle/_ref) not present")
+    g = glvl.read_glvl(refrun.ref_glvl(4))
+    case = ocean.hycom_hybgen_case(g, kdm=32, seed=23, numtr=2)
+    for leap, wt in HYB_RUNS:
+        out = oracle_lib.hycom_hybgn1(case, leap, with_tracers=wt)
+        r = refrun.run_hychybref(case, leap=leap, with_tracers=wt)
+        for n, v in out.items():
+            assert np.array_equal(np.ravel(v).view(np.uint64), np.ravel(r[n]).view(np.uint64)), (leap, n)
+
+
+def test_hycom_hybgen_properties(oracle_lib):
+    """What the grid generator fixes: each column's heat and salt content
+    (the thickness-weighted sums) and its bottom pressure are kept to
+    rounding, interfaces stay ordered, land and the other leapfrog level are
+    untouched; and the case moves interfaces in most ocean columns (the
+    checker's branches are exercised: a coverage run of it on this case
+    leaves only pcwise's decreasing-coordinate branch and one ratio branch of
+    equilb's scenario 4 unexecuted)."""
+    g = glvl.read_glvl(os.path.join(GOLDEN, "glvl_G3.dat"))
+    case = ocean.hycom_hybgen_case(g, kdm=32)
+    wet = case["wet"] > 0
+    out = oracle_lib.hycom_hybgn1(case, 2, with_tracers=True)
+    for n in ("temp", "saln"):
+        c0 = (case[n] * case["dp"][1]).sum(axis=1)
+        c1 = (out[n] * out["dp"][1]).sum(axis=1)
+        assert np.max(np.abs(c1 - c0)[wet] / np.abs(c0[wet])) < 1e-13, n
+    tr0 = (case["passv"][0] * case["dp"][1]).sum(axis=1)
+    tr1 = (out["passv"][0] * out["dp"][1]).sum(axis=1)
+    assert np.max(np.abs(tr1 - tr0)[wet] / np.abs(tr0[wet])) < 1e-12
+    bottom = case["pres"][:, 0].copy()          # the column's bottom from level 2's thicknesses
+    for k in range(case["kdm"]):
+        bottom = bottom + case["dp"][1][:, k]
+    assert np.array_equal(out["pres"][wet, -1], bottom[wet])
+    assert np.array_equal(out["pres"][~wet], case["pres"][~wet])
+    assert (np.diff(out["pres"][wet], axis=1) >= 0).all()
+    assert (out["dp"][1][wet] >= 0).all()
+    for n in ("dens", "temp", "saln"):
+        assert np.array_equal(out[n][~wet], case[n][~wet]), n
+    for n in ("dp", "uvel", "vvel"):
+        assert np.array_equal(out[n][0], case[n][0]), n
+    moved = (out["dp"][1] != case["dp"][1]).any(axis=1)
+    assert moved[wet].mean() > 0.5
