@@ -18,7 +18,7 @@ LIB = os.path.join(PKG, "libfimhip.so")
 
 SOURCES = ["fimhip_api.hip", "kernels_plain.hip", "transp3d.hip", "hybgen.hip", "ens_api.hip", "kernels_ens.hip",
            "transp3d_ens.hip", "fct_tiles.hip", "hycom_cnuity.hip",
-           "hycom_edgvar.hip", "hycom_momtum.hip", "hycom_transp.hip"]
+           "hycom_edgvar.hip", "hycom_momtum.hip", "hycom_transp.hip", "hycom_hybgen.hip"]
 HOST_SOURCES = ["grid.cpp", "lsq_svd.cpp"]      # host-only C++ (g++)
 HEADERS = ["fimhip_internal.h", "fim_pow.h", "fim_pow_tables.h", "synth.h", "host_common.h", "ens_internal.h",
            "hycom_internal.h"]

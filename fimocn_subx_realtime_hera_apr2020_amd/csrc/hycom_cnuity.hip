@@ -611,7 +611,9 @@ extern "C" int fimhip_ocn_create(const fimhip_grid_desc *gd, const double *rprox
       // the momentum equation's forcing and work arrays
       {&o->mo.taux, (size_t)nip}, {&o->mo.tauy, (size_t)nip}, {&o->mo.ustarb, (size_t)nip},
       {&o->mo.drag, (size_t)nip}, {&o->mo.dpbot, (size_t)nip}, {&o->mo.gradx, cz}, {&o->mo.grady, cz},
-      {&o->mo.d4u, cz}, {&o->mo.d4v, cz}, {&o->mo.du, cz}, {&o->mo.dv, cz}};
+      {&o->mo.d4u, cz}, {&o->mo.d4v, cz}, {&o->mo.du, cz}, {&o->mo.dv, cz},
+      // the tracers and density the transport and the grid generator share
+      {&o->tr.temp, cz}, {&o->tr.saln, cz}, {&o->hy.dens, cz}};
   for (auto &a : arrs)
     if ((r = o->arena.alloc(a.p, a.n))) { fimhip_ocn_destroy(o); return r; }
   s.um = um; s.vm = vm; s.un = un; s.vn = vn; s.dpm_edg = dpe;
@@ -666,6 +668,7 @@ double *ocn_field(fimhip_ocn *o, int f, size_t *n) {
     case FIMHIP_OCN_TEMP: *n = cz; return o->tr.temp;
     case FIMHIP_OCN_SALN: *n = cz; return o->tr.saln;
     case FIMHIP_OCN_PASSV: *n = o->tr.numtr * cz; return o->tr.numtr > 0 ? o->tr.passv : nullptr;
+    case FIMHIP_OCN_DENS: *n = cz; return o->hy.dens;
   }
   return nullptr;
 }

@@ -1,8 +1,8 @@
 // Definitions shared by the HYCOM ocean kernels (hycom_cnuity.hip,
-// hycom_edgvar.hip, hycom_momtum.hip, hycom_transp.hip): the constants of
-// hycom_constants.F90, the grid and state the kernels read, the lane-to-column
-// mapping and the wavefront-local column sums, and the ocean instance behind
-// the fimhip_ocn_* entries.
+// hycom_edgvar.hip, hycom_momtum.hip, hycom_transp.hip, hycom_hybgen.hip):
+// the constants of hycom_constants.F90, the grid and state the kernels read,
+// the lane-to-column mapping and the wavefront-local column sums, and the
+// ocean instance behind the fimhip_ocn_* entries.
 #pragma once
 #include <vector>
 
@@ -130,6 +130,14 @@ struct HycTr {
   double temdff;
 };
 
+// the hybrid grid generator's tables (hycom_constants' theta, salmin, dpmin;
+// the latitude in degrees) and the ocean columns it runs on; dens (kdm,nip)
+struct HycHybState {
+  int *cols;
+  int ncol;
+  double *targt, *salmin, *dpmin, *latij, *dens;
+};
+
 }  // namespace hyc
 }  // namespace fimhip
 
@@ -149,6 +157,7 @@ struct fimhip_ocn {
   fimhip::hyc::HycEdg ed{};
   fimhip::hyc::HycMom mo{};
   fimhip::hyc::HycTr tr{};
+  fimhip::hyc::HycHybState hy{};
   bool janjic = false;               // janjic_ocn (fimnamelist.F90:246)
   std::vector<int> wet;
   std::vector<double> area;

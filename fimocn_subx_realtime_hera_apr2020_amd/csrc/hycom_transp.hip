@@ -529,8 +529,7 @@ extern "C" int fimhip_ocn_set_transport(fimhip_ocn *o, int numtr, double temdff)
   const int nwp = (nwet + 255) / 256 * 256 + 256;
   int r = 0;
   struct { double **p; size_t n; } arrs[] = {
-      {&tr.cumuflx, ez}, {&tr.dpinit, cz}, {&tr.dpfinl, cz}, {&tr.temp, cz}, {&tr.saln, cz},
-      {&tr.passv, numtr > 0 ? numtr * cz : 1}, {&tr.w, cz}, {&tr.fld, nf * cz}, {&tr.flo, nf * cz},
+      {&tr.cumuflx, ez}, {&tr.dpinit, cz}, {&tr.dpfinl, cz}, {&tr.passv, numtr > 0 ? numtr * cz : 1}, {&tr.w, cz}, {&tr.fld, nf * cz}, {&tr.flo, nf * cz},
       {&tr.vfx, nf * cz}, {&tr.vdv, nf * cz}, {&tr.fmx, nf * cz}, {&tr.fmn, nf * cz}, {&tr.flp, nf * cz},
       {&tr.fln, nf * cz}, {&tr.uan, nf * ez}, {&tr.colsum, (size_t)nf * 4 * nwp}, {&tr.flxdv, 2 * cz},
       {&tr.q, (size_t)(2 + 2 * nf)}};

@@ -53,7 +53,7 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ocn_edgvar_r8", "fimhip_ocn_edgvar", "fimhip_ocn_set_janjic", "fimhip_ocn_momtum_r8",
            "fimhip_ocn_momtum", "fimhip_ocn_set_momtum", "fimhip_ocn_set_transport", "fimhip_ocn_transp0_r8",
            "fimhip_ocn_transp1_r8", "fimhip_ocn_transp2_r8", "fimhip_ocn_transp0", "fimhip_ocn_transp1",
-           "fimhip_ocn_transp2"]
+           "fimhip_ocn_transp2", "fimhip_ocn_set_hybgen", "fimhip_ocn_hybgn1_r8", "fimhip_ocn_hybgn1"]
 
 # real*4 drop-ins: name -> (number of float arrays, trailing int argument)
 R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, True), "mass_adv": (12, True),
@@ -190,6 +190,9 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ocn_transp0.argtypes = [vp, ct.c_int]
     lib.fimhip_ocn_transp1.argtypes = [vp]
     lib.fimhip_ocn_transp2.argtypes = [vp, ct.c_int]
+    lib.fimhip_ocn_set_hybgen.argtypes = [vp, _PD, _PD, _PD, _PD]
+    lib.fimhip_ocn_hybgn1_r8.argtypes = [vp, ct.c_int, ct.c_int] + [_PD] * 10
+    lib.fimhip_ocn_hybgn1.argtypes = [vp, ct.c_int, ct.c_int]
     lib.fimhip_ocn_profile.argtypes = [vp, ct.c_int]
     lib.fimhip_ocn_profile_report.argtypes = [vp, ct.c_char_p, ct.c_int]
     _LIB = lib
@@ -644,11 +647,11 @@ class Ensemble:
 OCN_FIELDS = ["dp", "pres", "um_edg", "vm_edg", "un_edg", "vn_edg", "dpm_edg", "massflx", "mssfx_avg",
               "uclin", "vclin", "utrop", "vtrop", "geop", "montg", "spcifv", "pr_edg", "geop_edg", "mont_edg",
               "uvsq_edg", "spcv_edg", "taux", "tauy", "ustarb", "cumuflx", "dpinit", "dpfinl", "temp", "saln",
-              "passv"]
+              "passv", "dens"]
 OCN_CNUITY_FIELDS = OCN_FIELDS[:9]       # the arrays cnuity reads or writes
 OCN_EDGVAR_FIELDS = OCN_FIELDS[:21]      # with those of bclin_edgvar
 OCN_MOMTUM_FIELDS = OCN_FIELDS[:24]      # with those of momtum
-OCN_TRANSP_FIELDS = OCN_FIELDS[24:]      # the tracer transport's (after set_transport)
+OCN_TRANSP_FIELDS = OCN_FIELDS[24:30]    # the tracer transport's (passv after set_transport)
 # bclin_edgvar's inputs and its outputs besides um/vm/un/vn_edg and dp_edg (dpm_edg here)
 OCN_EDG_IN = ["uclin", "vclin", "utrop", "vtrop", "geop", "montg", "spcifv"]
 OCN_EDG_OUT = ["pr_edg", "geop_edg", "mont_edg", "uvsq_edg", "spcv_edg"]
@@ -701,7 +704,7 @@ class Ocean:
         return {"dp": (2, n, k), "pres": (n, k + 1), "uclin": (2, n, k), "vclin": (2, n, k), "utrop": (n, 3),
                 "vtrop": (n, 3), "geop": (n, k + 1), "montg": (n, k), "spcifv": (n, k),
                 "pr_edg": (n, 6, k + 1), "taux": (n,), "tauy": (n,), "ustarb": (n,), "dpinit": (n, k),
-                "dpfinl": (n, k), "temp": (n, k), "saln": (n, k),
+                "dpfinl": (n, k), "temp": (n, k), "saln": (n, k), "dens": (n, k),
                 "passv": (self.numtr, n, k)}.get(name, (n, 6, k))
 
     @staticmethod
@@ -789,6 +792,31 @@ class Ocean:
         pv = _p(passv_tr) if self.numtr else None
         _chk(self.lib.fimhip_ocn_transp2_r8(self.h, nstep, _p(temp), _p(saln), pv, _p(cumuflx), _p(dpinit),
                                             _p(dpfinl)), "ocn_transp2_r8")
+
+    def set_hybgen(self, targt, salmin, dpmin, latij):
+        """hycom_constants' theta (targt), salmin, dpmin (kdm) and the latitude
+        in degrees (nip) for the grid generator; batrop from set_momtum."""
+        t = [np.ascontiguousarray(x, np.float64) for x in (targt, salmin, dpmin, latij)]
+        if any(x.shape != (self.kdm,) for x in t[:3]) or t[3].shape != (self.nip,):
+            raise ValueError("set_hybgen: tables of kdm values and nip latitudes expected")
+        _chk(self.lib.fimhip_ocn_set_hybgen(self.h, *map(_p, t)), "ocn_set_hybgen")
+
+    def hybgn1(self, nstep, leap, targt, dens, dp, pres, temp, saln, uvel, vvel, latij, passv_tr=None):
+        """hybgn1 (hycom_hybgn1.F90:9) on host arrays, updated in place
+        (passv_tr remapped only when given, as hycom_run passes none)."""
+        names = ["dens", "dp", "pres", "temp", "saln", "uclin", "vclin"] + (["passv"] if passv_tr is not None else [])
+        arrs = (dens, dp, pres, temp, saln, uvel, vvel) + ((passv_tr,) if passv_tr is not None else ())
+        self._check(names, arrs)
+        t = np.ascontiguousarray(targt, np.float64)
+        la = np.ascontiguousarray(latij, np.float64)
+        _chk(self.lib.fimhip_ocn_hybgn1_r8(self.h, nstep, leap, _p(t), _p(dens), _p(dp), _p(pres), _p(temp),
+                                           _p(saln), _p(uvel), _p(vvel), _p(la),
+                                           _p(passv_tr) if passv_tr is not None else None), "ocn_hybgn1_r8")
+
+    def step_hybgn1(self, leap: int, with_tracers: bool = False):
+        """hybgn1 on the device-resident arrays (dens, dp, pres, temp, saln,
+        uclin, vclin; passv with_tracers)."""
+        _chk(self.lib.fimhip_ocn_hybgn1(self.h, leap, int(with_tracers)), "ocn_hybgn1")
 
     def step_transp0(self, leapn: int):
         """transp0 on the device-resident arrays, from dp's level leapn."""

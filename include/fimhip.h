@@ -386,15 +386,18 @@ typedef struct fimhip_ocn fimhip_ocn;
 #define FIMHIP_OCN_TAUX 21
 #define FIMHIP_OCN_TAUY 22
 #define FIMHIP_OCN_USTARB 23
-/* the tracer transport's arrays (fimhip_ocn_set_transport allocates them):
- * cumuflx (kdm,6,nip), dpinit, dpfinl, temp, saln (kdm,nip), passv_tr
- * (kdm,nip,numtr) */
+/* the tracer transport's arrays: cumuflx (kdm,6,nip), dpinit, dpfinl
+ * (kdm,nip) and passv_tr (kdm,nip,numtr), which fimhip_ocn_set_transport
+ * allocates, and temp, saln (kdm,nip) */
 #define FIMHIP_OCN_CUMUFLX 24
 #define FIMHIP_OCN_DPINIT 25
 #define FIMHIP_OCN_DPFINL 26
 #define FIMHIP_OCN_TEMP 27
 #define FIMHIP_OCN_SALN 28
 #define FIMHIP_OCN_PASSV 29
+/* the potential density (sigma_1) the grid generator reads and writes
+ * (kdm,nip) */
+#define FIMHIP_OCN_DENS 30
 /* hycom_init's constants (hycom_constants.F90 wet, dtleap; fimnamelist kdm,
  * thkdff, biharm) on the grid of the descriptor (nprox, prox, proxs, cs, sn,
  * sidevec_e, sidevec_c, sideln, area, lat; nvl, ntra, dt, hfcoef, lon,
@@ -472,6 +475,26 @@ int fimhip_ocn_transp2_r8(fimhip_ocn *ocn, int nstep, double *temp, double *saln
 int fimhip_ocn_transp0(fimhip_ocn *ocn, int leapn);
 int fimhip_ocn_transp1(fimhip_ocn *ocn);
 int fimhip_ocn_transp2(fimhip_ocn *ocn, int leapn);
+/* the hybrid grid generator's tables: hycom_constants' theta (the target
+ * densities, which hycom_run passes as targt), salmin and dpmin (m), kdm
+ * each (hycom_init.F90:558-566, ocn_target.txt, ocn_dpmin.txt), and the
+ * latitude in degrees (nip; dyn_init.F90:754 deg_lat).  Needs
+ * fimhip_ocn_set_momtum's batrop. */
+int fimhip_ocn_set_hybgen(fimhip_ocn *ocn, const double *targt, const double *salmin, const double *dpmin,
+                          const double *latij);
+/* hycom_hybgn1.F90:9 hybgn1(nstep, leap, targt, dens, dp, pres, temp, saln,
+ * uvel, vvel, latij, passv_tr) with hybgn1_1d, ppmad3, pcwise, equilb and
+ * hycom_sigetc's equation of state, on host arrays: dens, temp, saln,
+ * pres, and level leap of dp, uvel, vvel (kdm,nip,2) are updated; passv_tr
+ * (kdm,nip,numtr of fimhip_ocn_set_transport) is remapped too when not NULL
+ * (hycom_run.F90:516 passes none).  salmin and dpmin from
+ * fimhip_ocn_set_hybgen. */
+int fimhip_ocn_hybgn1_r8(fimhip_ocn *ocn, int nstep, int leap, const double *targt, double *dens, double *dp,
+                         double *pres, double *temp, double *saln, double *uvel, double *vvel, const double *latij,
+                         double *passv_tr);
+/* the same on the device-resident arrays (FIMHIP_OCN_DENS, _DP, _PRES,
+ * _TEMP, _SALN, _UCLIN, _VCLIN, with_tracers: _PASSV), asynchronous */
+int fimhip_ocn_hybgn1(fimhip_ocn *ocn, int leap, int with_tracers);
 /* hycom_constants' dtleap for the calls that follow (hycom_run.F90:136 sets
  * it every step: batrop at the first, 2*batrop after) */
 int fimhip_ocn_set_dtleap(fimhip_ocn *ocn, double dtleap);

@@ -450,3 +450,111 @@ def test_transport_bad_arguments_are_rejected():
             o.step_transp0(0)
     finally:
         o.close()
+
+
+HYB_OUT = ("dens", "dp", "pres", "temp", "saln", "uvel", "vvel")
+
+
+def _hyb_ocean(g, case, numtr=0):
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip
+    o = fimhip.Ocean(g, case["kdm"], case["wet"], 1800.0)
+    o.set_momtum(case["batrop"])
+    if numtr:
+        o.set_transport(numtr, 0.02)
+    o.set_hybgen(case["targt"], case["salmin"], case["dpmin"], case["latij"])
+    return o
+
+
+@pytest.mark.parametrize("leap,with_tracers", [(1, False), (2, True)])
+def test_hybgen_dropin_matches_checker_bitwise_g3(oracle_lib, leap, with_tracers):
+    """hybgn1 through the host-array drop-in, as hycom_run calls it (no
+    tracers) and with two passive tracers."""
+    from fimocn_subx_realtime_hera_apr2020_amd import ocean
+    print("case: G3 x 32, hybgn1 drop-in, leap %d, tracers %s" % (leap, with_tracers), flush=True)
+    g = _grid(3)
+    case = ocean.hycom_hybgen_case(g, kdm=32, numtr=2)
+    want = oracle_lib.hycom_hybgn1(case, leap, with_tracers=with_tracers)
+    o = _hyb_ocean(g, case, numtr=2 if with_tracers else 0)
+    try:
+        a = {n: np.ascontiguousarray(case[n], np.float64).copy() for n in oracle_lib.HYB_FIELDS}
+        o.hybgn1(1, leap, case["targt"], a["dens"], a["dp"], a["pres"], a["temp"], a["saln"], a["uvel"], a["vvel"],
+                 case["latij"], a["passv"] if with_tracers else None)
+        for n in oracle_lib.HYB_FIELDS:
+            assert _same(a[n], want[n]), n
+    finally:
+        o.close()
+
+
+@pytest.mark.parametrize("level,kdm,numtr", [(4, 48, 1), (5, 32, 0)])
+def test_hybgen_resident_matches_checker(oracle_lib, level, kdm, numtr):
+    """hybgn1 on the device-resident arrays at 32 layers and at 48 (the
+    64-layer build), twice in a row on alternating levels."""
+    from fimocn_subx_realtime_hera_apr2020_amd import ocean
+    print("case: G%d x %d, hybgn1 resident, numtr %d" % (level, kdm, numtr), flush=True)
+    g = _grid(level)
+    case = ocean.hycom_hybgen_case(g, kdm=kdm, seed=61 + level, numtr=max(numtr, 1))
+    o = _hyb_ocean(g, case, numtr=numtr)
+    try:
+        o.upload("dens", case["dens"])
+        o.upload("dp", case["dp"])
+        o.upload("pres", case["pres"])
+        o.upload("temp", case["temp"])
+        o.upload("saln", case["saln"])
+        o.upload("uclin", case["uvel"])
+        o.upload("vclin", case["vvel"])
+        if numtr:
+            o.upload("passv", case["passv"])
+        c = dict(case)
+        for leap in (2, 1):
+            want = oracle_lib.hycom_hybgn1(c, leap, with_tracers=bool(numtr))
+            o.step_hybgn1(leap, with_tracers=bool(numtr))
+            got = {n: o.download(m) for n, m in (("dens", "dens"), ("dp", "dp"), ("pres", "pres"), ("temp", "temp"),
+                                                  ("saln", "saln"), ("uvel", "uclin"), ("vvel", "vclin"))}
+            if numtr:
+                got["passv"] = o.download("passv")
+            for n, v in got.items():
+                assert _same(v, want[n]), (leap, n)
+            c = dict(c, **want)
+    finally:
+        o.close()
+
+
+def test_hybgen_g7_full_size(oracle_lib):
+    from fimocn_subx_realtime_hera_apr2020_amd import ocean
+    print("case: G7 x 32, hybgn1 resident", flush=True)
+    g = _grid(7)
+    case = ocean.hycom_hybgen_case(g, kdm=32)
+    want = oracle_lib.hycom_hybgn1(case, 1)
+    o = _hyb_ocean(g, case)
+    try:
+        for n, m in (("dens", "dens"), ("dp", "dp"), ("pres", "pres"), ("temp", "temp"), ("saln", "saln"),
+                     ("uvel", "uclin"), ("vvel", "vclin")):
+            o.upload(m, case[n])
+        o.step_hybgn1(1)
+        for n, m in (("dens", "dens"), ("dp", "dp"), ("pres", "pres"), ("temp", "temp"), ("saln", "saln"),
+                     ("uvel", "uclin"), ("vvel", "vclin")):
+            assert _same(o.download(m), want[n]), n
+    finally:
+        o.close()
+
+
+def test_hybgen_bad_arguments_are_rejected():
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, ocean
+    g = _grid(2)
+    case = ocean.hycom_hybgen_case(g, kdm=32)
+    o = fimhip.Ocean(g, 32, case["wet"], 1800.0)
+    try:
+        with pytest.raises(fimhip.FimhipError):
+            o.step_hybgn1(1)                  # before set_hybgen
+        o.set_hybgen(case["targt"], case["salmin"], case["dpmin"], case["latij"])
+        with pytest.raises(fimhip.FimhipError):
+            o.step_hybgn1(1)                  # no batrop yet
+        o.set_momtum(case["batrop"])
+        with pytest.raises(fimhip.FimhipError):
+            o.step_hybgn1(3)
+        with pytest.raises(fimhip.FimhipError):
+            o.step_hybgn1(1, with_tracers=True)   # no tracers set
+        with pytest.raises(ValueError):
+            o.set_hybgen(case["targt"][:5], case["salmin"], case["dpmin"], case["latij"])
+    finally:
+        o.close()
