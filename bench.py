@@ -611,6 +611,11 @@ def hyc_tr_interval_bytes(nfld):
     return 15 * 48 + 96 + 16 + 16 * nfld
 
 
+# the grid generator (hycom_hybgn1.F90): dens, temp, saln, dp, u, v in and
+# out, and pres, per ocean cell-level
+HYC_HYB_KERNEL_BYTES = {"hyc_hybgn1": 2 * 6 * 8 + 8 + 8}
+
+
 def _ocean_leg(a, o, wcl, step, byte_model):
     """Time 'step(nstep)' over a.steps steps after a.warmup, then the same
     number under per-kernel HIP-event timing; throughput, kernel table and
@@ -658,8 +663,9 @@ def run_hycom(a, fimhip, cpu=True):
     (hycom_momtum.F90 momtum with hycom_dissip), the velocities it writes
     feeding the next step; and the tracer transport over one 30-step
     interval (hycom_transp3d.F90 transp1 every second step, transp2 with
-    hycom_fct3d and hycom_dffusn, transp0).  The checker's C restatement of
-    each is timed on one host core over 2 steps (one interval) of the same
+    hycom_fct3d and hycom_dffusn, transp0); and the hybrid grid generator
+    (hycom_hybgn1.F90 hybgn1).  The checker's C restatement of each is timed
+    on one host core over 2 steps (one interval, one call) of the same
     input."""
     import numpy as np
     from fimocn_subx_realtime_hera_apr2020_amd import ocean
@@ -746,6 +752,25 @@ def run_hycom(a, fimhip, cpu=True):
         out["transport"] = leg
     finally:
         o.close()
+    # the hybrid grid generator (hybgn1, as hycom_run calls it: no tracers),
+    # one call per timed step on alternating levels
+    hcase = ocean.hycom_hybgen_case(g, kdm=kdm)
+    o = fimhip.Ocean(g, kdm, hcase["wet"], case["dtleap"], thkdff=thkdff, biharm=biharm)
+    try:
+        o.set_momtum(hcase["batrop"])
+        o.set_hybgen(hcase["targt"], hcase["salmin"], hcase["dpmin"], hcase["latij"])
+        for n, m in (("dens", "dens"), ("dp", "dp"), ("pres", "pres"), ("temp", "temp"), ("saln", "saln"),
+                     ("uvel", "uclin"), ("vvel", "vclin")):
+            o.upload(m, hcase[n])
+        leg = _ocean_leg(a, o, wcl, lambda st: o.step_hybgn1(1 + st % 2), HYC_HYB_KERNEL_BYTES)
+        ms = leg["ms_per_step"]
+        hb = HYC_HYB_KERNEL_BYTES["hyc_hybgn1"]
+        leg["step_roofline"] = {"compulsory_bytes_per_cell_level": hb,
+                                "achieved": hb * wcl / (ms * 1e-3) / 1e9, "unit": "GB/s",
+                                "frac": hb * wcl / (ms * 1e-3) / HBM_PEAK}
+        out["hybgen"] = leg
+    finally:
+        o.close()
     if cpu:
         from oracle import oracle as O
         for name, with_edgvar, with_momtum in (("cnuity", False, False), ("edgvar_cnuity", True, False),
@@ -773,6 +798,13 @@ def run_hycom(a, fimhip, cpu=True):
         leg["cpu_checker"] = {"value": wcl / tc, "unit": "ocean cell-level-intervals/s", "cores": 1, "kind": "port",
                               "sample": "1 interval (transp0, 15 x transp1, transp2) of the same G8 x 32 input",
                               "ms_per_step": tc * 1e3}
+        leg["speedup_vs_cpu_checker"] = leg["value"] / leg["cpu_checker"]["value"]
+        t0 = time.perf_counter()
+        O.hycom_hybgn1(hcase, 1)
+        tc = time.perf_counter() - t0
+        leg = out["hybgen"]
+        leg["cpu_checker"] = {"value": wcl / tc, "unit": "ocean cell-level-steps/s", "cores": 1, "kind": "port",
+                              "sample": "1 call of the same G8 x 32 input", "ms_per_step": tc * 1e3}
         leg["speedup_vs_cpu_checker"] = leg["value"] / leg["cpu_checker"]["value"]
     return out
 

@@ -202,6 +202,20 @@ module fimhip_bind
       real(c_double) :: temp(*), saln(*), passv_tr(*)
       real(c_double), intent(in) :: cumuflx(*), dpinit(*), dpfinl(*)
     end function
+    ! hycom_hybgn1.F90:9 hybgn1 (passv_tr: pass c_null_ptr for none, as
+    ! hycom_run.F90:516 calls it)
+    integer(c_int) function fimhip_ocn_set_hybgen(ocn, targt, salmin, dpmin, latij) &
+        bind(C, name="fimhip_ocn_set_hybgen")
+      import; type(c_ptr), value :: ocn
+      real(c_double), intent(in) :: targt(*), salmin(*), dpmin(*), latij(*)
+    end function
+    integer(c_int) function fimhip_ocn_hybgn1_r8(ocn, nstep, leap, targt, dens, dp, pres, temp, saln, uvel, &
+        vvel, latij, passv_tr) bind(C, name="fimhip_ocn_hybgn1_r8")
+      import; type(c_ptr), value :: ocn; integer(c_int), value :: nstep, leap
+      real(c_double), intent(in) :: targt(*), latij(*)
+      real(c_double) :: dens(*), dp(*), pres(*), temp(*), saln(*), uvel(*), vvel(*)
+      type(c_ptr), value :: passv_tr
+    end function
     integer(c_int) function fimhip_ocn_set_janjic(ocn, on) bind(C, name="fimhip_ocn_set_janjic")
       import; type(c_ptr), value :: ocn; integer(c_int), value :: on
     end function
