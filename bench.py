@@ -611,9 +611,13 @@ def hyc_tr_interval_bytes(nfld):
     return 15 * 48 + 96 + 16 + 16 * nfld
 
 
-# the grid generator (hycom_hybgn1.F90): dens, temp, saln, dp, u, v in and
-# out, and pres, per ocean cell-level
-HYC_HYB_KERNEL_BYTES = {"hyc_hybgn1": 2 * 6 * 8 + 8 + 8}
+# the grid generator (hycom_hybgn1.F90), bytes per ocean cell-level: the
+# interface moves read dens, temp, saln, dp and write pres, the old
+# interfaces, dens, temp, saln; the remap reads pres, the old interfaces,
+# dens, temp, saln, u, v and writes temp, saln, dp, pres, u, v.  The whole
+# call must move dens, temp, saln, dp, u, v in and out and pres out.
+HYC_HYB_KERNEL_BYTES = {"hyc_hyb_move": 4 * 8 + 5 * 8, "hyc_hyb_remap": 7 * 8 + 6 * 8}
+HYC_HYB_STEP_BYTES = 2 * 6 * 8 + 8 + 8
 
 
 def _ocean_leg(a, o, wcl, step, byte_model):
@@ -764,7 +768,7 @@ def run_hycom(a, fimhip, cpu=True):
             o.upload(m, hcase[n])
         leg = _ocean_leg(a, o, wcl, lambda st: o.step_hybgn1(1 + st % 2), HYC_HYB_KERNEL_BYTES)
         ms = leg["ms_per_step"]
-        hb = HYC_HYB_KERNEL_BYTES["hyc_hybgn1"]
+        hb = HYC_HYB_STEP_BYTES
         leg["step_roofline"] = {"compulsory_bytes_per_cell_level": hb,
                                 "achieved": hb * wcl / (ms * 1e-3) / 1e9, "unit": "GB/s",
                                 "frac": hb * wcl / (ms * 1e-3) / HBM_PEAK}

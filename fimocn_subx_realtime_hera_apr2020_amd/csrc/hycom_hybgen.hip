@@ -1,11 +1,22 @@
 // HYCOM's hybrid grid generator (FIMsrc/fim/horizontal/hycom_hybgn1.F90:
 // hybgn1, hybgn1_1d, ppmad3, pcwise, equilb; hycom_sigetc.F90: sigocn,
 // dsigdt, dsigds) on the GPU in FP64, with the reference's operations in its
-// order.  The routine is column-local and its column chains are serial (each
-// interface move changes what the next one sees), so one thread takes one
-// ocean column (a compacted list of the ocean cells: land leaves no idle
-// lanes); its working columns live in private memory.  The velocities and
-// the passive tracers are remapped in place: pcwise copies its input column
+// order.  The routine is column-local; two kernels split it by the shape of
+// its work:
+//   hyc_hyb_move   one thread per ocean column (a compacted list of the ocean
+//                  cells: land leaves no idle lanes): the sea-floor
+//                  absorption, the split of a too-light bottom layer and the
+//                  interface moves (hycom_hybgn1.F90:240-758), whose chains
+//                  are serial down the column (each move changes what the
+//                  next one tests)
+//   hyc_hyb_remap  the column's lanes over its layers, its work columns in
+//                  LDS: equilb's heat diffusion (per layer and per interface)
+//                  and scenario 4, the monotonicity guard, and pcwise's
+//                  remapping of the velocities and tracers (fits, sweeps and
+//                  updates per layer or interface; its two filling
+//                  recurrences and equilb's scenario 4 run on one lane)
+// The passive tracers are moved in place: the sea-floor steps change them
+// where the reference changes its copy (trold), and pcwise reads its column
 // before it writes the result over it.
 #include <vector>
 
@@ -28,6 +39,7 @@ struct HycHyb {
   const double *targt, *salmin, *dpmin, *latij;
   double batrop;
   double *dens, *dp, *pres, *temp, *saln, *u, *v, *passv;   // dp, u, v: the level hybgn1 works on
+  double *pold;                         // (kdm+1,nip): the interfaces before the moves
 };
 
 // hycom_sigetc.F90 sigocn: sigma_1 of t, s (a rational fit at 1000 dbar)
@@ -71,7 +83,8 @@ __device__ __forceinline__ void y_parab(double y, double yl, double yr, double &
 
 // ppmad3 on three cells x0[1..4], values y0[1..3]; one interface moves:
 // interface 2 (j == 2) or 3 (j == 3) by d, the others stay
-__device__ void y_ppmad3(const double *x0, double *y0, int j, double d) {
+template <typename A>
+__device__ void y_ppmad3(const A x0, const A y0, int j, double d) {
   const double onemu = .098, athird = 1. / 3.;
   const double x1 = x0[1], x2 = x0[2], x3 = x0[3], x4 = x0[4];
   const double y1 = y0[1], y2 = y0[2], y3 = y0[3];
@@ -103,194 +116,33 @@ __device__ void y_ppmad3(const double *x0, double *y0, int j, double d) {
   if (dxnew > 0.) y0[3] = t3 / dxnew;
 }
 
-template <int KM>
-struct YCol {
-  double pr[KM + 2], tem[KM + 1], sal[KM + 1], dns[KM + 1], pold[KM + 2];
+// a column of the thread-per-column kernel in LDS, the block's T threads
+// interleaved (adjacent threads, adjacent banks)
+template <int T>
+struct YS {
+  double *p;
+  __device__ __forceinline__ double &operator[](int k) const { return p[k * T]; }
+  __device__ __forceinline__ YS operator+(int d) const { return YS{p + d * T}; }
+  __device__ __forceinline__ YS operator-(int d) const { return YS{p - d * T}; }
 };
-template <int KM>
-struct YWork {                          // equilb's and pcwise's columns (not live together)
-  double w0[KM + 2], w1[KM + 2], w2[KM + 2], w3[KM + 2], w4[KM + 2], w5[KM + 2], w6[KM + 2];
-};
-
-// equilb: heat diffusion to inflate thin layers (3 passes), then scenario 4
-template <int KM>
-__device__ void y_equilb(int kdm, YCol<KM> &c, YWork<KM> &w) {
-  const double slak = .25, dffudt = .1;
-  double *pnew = w.w0, *pwrk = w.w1, *heatfx = w.w2, *tdz = w.w3, *sdz = w.w4, *twrk = w.w5, *swrk = w.w6;
-  const double *thet = c.dns;
-  double *pres = c.pr, *temp = c.tem, *saln = c.sal;
-  for (int k = 1; k <= kdm + 1; k++) pnew[k] = pwrk[k] = pres[k];
-  for (int k = 1; k <= kdm; k++) {
-    twrk[k] = temp[k];
-    swrk[k] = saln[k];
-    tdz[k] = hmax(H_EPSIL, pres[k + 1] - pres[k]) * temp[k];
-    sdz[k] = hmax(H_EPSIL, pres[k + 1] - pres[k]) * saln[k];
-  }
-  heatfx[kdm] = 0.;
-  heatfx[1] = 0.;
-  for (int iter = 1; iter <= 3; iter++) {
-    for (int k = 2; k <= kdm - 1; k++) {
-      double hf = 0.;
-      if (pwrk[k] > pwrk[1] + H_ONEM && pwrk[k + 1] < pwrk[kdm + 1] - H_ONEM) {
-        const double cnv = H_ONEM;
-        hf = dffudt * cnv * cnv * hmin(thet[k + 1] - thet[k], thet[k] - thet[k - 1]) /
-             hmax(.01 * cnv, pwrk[k + 1] - pwrk[k]);
-        const double plimit1 = hmax(pwrk[k] - Y_TENM, .5 * (pwrk[k] + pwrk[k - 1]));
-        const double plimit2 = hmin(pwrk[k + 1] + Y_TENM, .5 * (pwrk[k + 1] + pwrk[k + 2]));
-        hf = hmin(hmin(hmax(0., hf), hmax(0., (plimit2 - pwrk[k + 1]) * (thet[k + 1] - thet[k]))),
-                  hmax(0., (pwrk[k] - plimit1) * (thet[k] - thet[k - 1])));
-      }
-      heatfx[k] = hf;
-    }
-    for (int k = 1; k <= kdm - 1; k++) {
-      if (thet[k + 1] > thet[k]) {
-        pnew[k + 1] = pwrk[k + 1] + (heatfx[k] - heatfx[k + 1]) / (thet[k + 1] - thet[k]);
-        const double delp = pnew[k + 1] - pwrk[k + 1];
-        const double dtdz = (delp > 0. ? twrk[k + 1] : twrk[k]) * delp;
-        const double dsdz = (delp > 0. ? swrk[k + 1] : swrk[k]) * delp;
-        tdz[k] = tdz[k] + dtdz;
-        tdz[k + 1] = tdz[k + 1] - dtdz;
-        sdz[k] = sdz[k] + dsdz;
-        sdz[k + 1] = sdz[k + 1] - dsdz;
-      }
-    }
-    for (int k = 1; k <= kdm; k++) {
-      if (pnew[k + 1] > pnew[k]) {
-        twrk[k] = tdz[k] / (pnew[k + 1] - pnew[k]);
-        swrk[k] = sdz[k] / (pnew[k + 1] - pnew[k]);
-      }
-    }
-    for (int k = 1; k <= kdm + 1; k++) pwrk[k] = pnew[k];
-  }
-  for (int k = 4; k <= kdm - 2; k++) {
-    if (pnew[k - 2] < pnew[1] + Y_TENCM || pnew[k + 2] > pnew[kdm + 1] - Y_TENCM) continue;
-    const double dp1 = pnew[k - 1] - pnew[k - 2], dp2 = pnew[k] - pnew[k - 1];
-    const double dp3 = pnew[k + 1] - pnew[k], dp4 = pnew[k + 2] - pnew[k + 1];
-    const double th1 = thet[k - 2], th2 = thet[k - 1], th3 = thet[k], th4 = thet[k + 1];
-    if (dp1 > dp2 + dp3 && dp4 > dp2 + dp3) {
-      const double goal = .5 * (dp2 + dp3 + hmin(dp1, dp4));
-      double dis2 = (goal - dp2 - dp3) * slak, dis4 = dis2;
-      if (th2 > th1 && th3 > th2 && th4 > th3) {
-        const double ratio = (th4 - th3) / (th2 - th1);
-        if (ratio > 1.) dis4 = dis2 / ratio;
-        else dis2 = dis4 * ratio;
-      }
-      pnew[k - 1] = pnew[k - 1] - dis2;
-      pnew[k + 1] = pnew[k + 1] + dis4;
-      tdz[k - 2] = tdz[k - 2] - dis2 * twrk[k - 2];
-      tdz[k - 1] = tdz[k - 1] + dis2 * twrk[k - 2];
-      tdz[k] = tdz[k] + dis4 * twrk[k + 1];
-      tdz[k + 1] = tdz[k + 1] - dis4 * twrk[k + 1];
-      sdz[k - 2] = sdz[k - 2] - dis2 * swrk[k - 2];
-      sdz[k - 1] = sdz[k - 1] + dis2 * swrk[k - 2];
-      sdz[k] = sdz[k] + dis4 * swrk[k + 1];
-      sdz[k + 1] = sdz[k + 1] - dis4 * swrk[k + 1];
-    }
-  }
-  for (int k = 1; k <= kdm; k++) {
-    if (pnew[k + 1] - pnew[k] > Y_ONEMU) {
-      temp[k] = tdz[k] / (pnew[k + 1] - pnew[k]);
-      saln[k] = sdz[k] / (pnew[k + 1] - pnew[k]);
-    }
-    pres[k] = pnew[k];
-  }
-}
-
-// pcwise(optn, kdm, pold, fld, pr, fld) without a migration limit: the
-// column f (stride st in global memory) remapped from pold to pr in place
-template <int KM>
-__device__ void y_pcwise(int optn, int kdm, const YCol<KM> &c, YWork<KM> &w, double *f, bool flip) {
-  const double athird = 1. / 3., small = 1.e-11;
-  double *fco = w.w0, *fcn = w.w1, *delz = w.w2, *fn = w.w3, *a = w.w4, *b = w.w5, *cc = w.w6;
-  for (int k = 1; k <= kdm + 1; k++) {
-    const double zo = flip ? -c.pold[k] : c.pold[k], zn = flip ? -c.pr[k] : c.pr[k];
-    delz[k] = zn - zo;
-  }
-  for (int k = 1; k <= kdm; k++) {
-    const double zo0 = flip ? -c.pold[k] : c.pold[k], zo1 = flip ? -c.pold[k + 1] : c.pold[k + 1];
-    const double zn0 = flip ? -c.pr[k] : c.pr[k], zn1 = flip ? -c.pr[k + 1] : c.pr[k + 1];
-    fco[k] = hmax(0., zo1 - zo0);
-    fcn[k] = hmax(0., zn1 - zn0);
-    fn[k] = f[k - 1];
-  }
-  for (int k = kdm - 1; k >= 1; k--) fn[k] = (fn[k] * fco[k] + fn[k + 1] * small) / (fco[k] + small);
-  for (int k = 2; k <= kdm; k++) fn[k] = (fn[k] * fco[k] + fn[k - 1] * small) / (fco[k] + small);
-  a[1] = fn[1];
-  b[1] = cc[1] = 0.;
-  a[kdm] = fn[kdm];
-  b[kdm] = cc[kdm] = 0.;
-  for (int k = 2; k <= kdm - 1; k++) {
-    if (optn == 1) {
-      a[k] = fn[k];
-      b[k] = cc[k] = 0.;
-    } else {
-      const double fl = hmax(small, fco[k - 1]), fm = hmax(small, fco[k]), fr = hmax(small, fco[k + 1]);
-      const double yl = (fl * fn[k] + fm * fn[k - 1]) / (fl + fm);
-      const double yr = (fr * fn[k] + fm * fn[k + 1]) / (fr + fm);
-      y_parab(fn[k], yl, yr, a[k], b[k], cc[k]);
-    }
-  }
-  // the flux through interface k from the slab it sweeps; vertdv on the fly
-  double vlo = 0.;                      // vertfx(1) = 0
-  for (int k = 1; k <= kdm; k++) {
-    double vhi = 0.;                    // vertfx(k+1); vertfx(kdm+1) = 0
-    const int ki = k + 1;
-    if (ki <= kdm) {
-      double slab = 0., amount = 0.;
-      const double dz = delz[ki];
-      if (dz > 0.) {
-        for (int lyr = ki;; lyr++) {
-          if (slab >= dz) break;
-          if (fco[lyr] > 0.) {
-            const double dslab = hmin(slab + fco[lyr], dz) - hmin(slab, dz);
-            const double dx = dslab / fco[lyr];
-            const double fcdx = a[lyr] + b[lyr] * .5 * (dx - 1.) + cc[lyr] * (.25 - dx * (.5 - dx * athird));
-            amount = amount + fcdx * dslab;
-            slab = slab + dslab;
-          }
-          if (!(lyr < kdm)) break;
-        }
-      } else if (dz < 0.) {
-        for (int lyr = ki - 1;; lyr--) {  // down to layer 2 only (layer 1 from interface 2)
-          if (slab >= -dz) break;
-          if (fco[lyr] > 0.) {
-            const double dslab = hmin(slab + fco[lyr], -dz) - hmin(slab, -dz);
-            const double dx = dslab / fco[lyr];
-            const double fcdx = a[lyr] + b[lyr] * .5 * (1. - dx) + cc[lyr] * (.25 - dx * (.5 - dx * athird));
-            amount = amount + fcdx * dslab;
-            slab = slab + dslab;
-          }
-          if (!(lyr > 2)) break;
-        }
-      }
-      if (slab != 0.) vhi = -dz * amount / slab;
-    }
-    const double vertdv = vhi - vlo;
-    const double amount = fn[k] * fco[k] - vertdv;
-    f[k - 1] = (fn[k] * small + amount) / (small + fcn[k]);
-    vlo = vhi;
-  }
-}
-
-template <int KM>
-__global__ void __launch_bounds__(64) hyc_hybgn1(HycHyb h) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+template <int KM, int T>
+__global__ void __launch_bounds__(T) hyc_hyb_move(HycHyb h) {
+  __shared__ double sbuf[(KM + 2) + 3 * (KM + 1)][T];
+  const int t = threadIdx.x, j = blockIdx.x * T + t;
   if (j >= h.ncol) return;
   const int i = h.cols[j], kdm = h.kdm;
-  YCol<KM> c;
-  YWork<KM> w;
+  const YS<T> pr{&sbuf[0][t]}, tem{&sbuf[KM + 2][t]}, sal{&sbuf[2 * KM + 3][t]}, dns{&sbuf[3 * KM + 4][t]};
   const double tfreez = -2.2, sigjmp = .01, scalt = -30., scals = 10., slak = 1. / 86400.;
   const double dlt = h.batrop * 30;                 // batrop*bclin_frq
   const double lati = h.latij[i];
   const size_t base = (size_t)i * kdm;
-  c.pr[1] = h.pres[(size_t)i * (kdm + 1)];
+  pr[1] = h.pres[(size_t)i * (kdm + 1)];
   for (int k = 1; k <= kdm; k++) {
-    c.dns[k] = h.dens[base + k - 1];
-    c.tem[k] = h.temp[base + k - 1];
-    c.sal[k] = h.saln[base + k - 1];
-    c.pr[k + 1] = c.pr[k] + h.dp[base + k - 1];
+    dns[k] = h.dens[base + k - 1];
+    tem[k] = h.temp[base + k - 1];
+    sal[k] = h.saln[base + k - 1];
+    pr[k + 1] = pr[k] + h.dp[base + k - 1];
   }
-  double *pr = c.pr, *tem = c.tem, *sal = c.sal, *dns = c.dns;
   const double *targt = h.targt - 1, *salmin = h.salmin - 1, *dpmin = h.dpmin - 1;   // 1-based
   double q, p_hat = 0.;
   auto slakf = [&]() { return (1. + 9. * hmax(0., 1. - .01 * Y_RONEM * p_hat)) * slak * dlt; };
@@ -350,7 +202,8 @@ __global__ void __launch_bounds__(64) hyc_hybgn1(HycHyb h) {
       }
     }
   }
-  for (int k = 1; k <= kdm + 1; k++) c.pold[k] = pr[k];
+  const size_t pb = (size_t)i * (kdm + 1);
+  for (int k = 1; k <= kdm + 1; k++) h.pold[pb + k - 1] = pr[k];     // pcwise's old interfaces
 
   // restore isopycnic conditions by moving the interfaces
   double dpsum = 0., dp0 = Y_HUUGE;
@@ -451,22 +304,221 @@ __global__ void __launch_bounds__(64) hyc_hybgn1(HycHyb h) {
     }
   }
 
-  y_equilb<KM>(kdm, c, w);
-  for (int k = 1; k <= kdm - 2; k++)
-    if (pr[k + 1] < pr[k] || pr[k + 1] > pr[k + 2]) pr[k + 1] = hmax(pr[k], hmin(pr[k + 1], c.pold[k + 2]));
-
-  // the velocities (piecewise constant) and the tracers (parabolic) remapped
-  const bool flip = !(c.pold[1] < c.pold[kdm + 1]);
-  y_pcwise<KM>(1, kdm, c, w, h.u + base, flip);
-  y_pcwise<KM>(1, kdm, c, w, h.v + base, flip);
-  for (int nt = 0; nt < h.numtr; nt++) y_pcwise<KM>(3, kdm, c, w, &trc(nt, 1), flip);
-
+  // handed to hyc_hyb_remap: the moved interfaces and the layers
+  for (int k = 1; k <= kdm + 1; k++) h.pres[pb + k - 1] = pr[k];
   for (int k = 1; k <= kdm; k++) {
     h.dens[base + k - 1] = dns[k];
     h.temp[base + k - 1] = tem[k];
     h.saln[base + k - 1] = sal[k];
-    h.dp[base + k - 1] = pr[k + 1] - pr[k];
-    h.pres[(size_t)i * (kdm + 1) + k] = pr[k + 1];
+  }
+}
+
+// equilb, the monotonicity guard and the pcwise remapping of one column on
+// LPC lanes: lane l holds layer k = l + 1 and the interface below it (k + 1)
+template <int LPC>
+__global__ void __launch_bounds__(256) hyc_hyb_remap(HycHyb h) {
+  constexpr int CPB = 256 / LPC, N = LPC + 2;
+  __shared__ double sP[CPB][N], sW[CPB][N], sO[CPB][N], sT[CPB][N], sTW[CPB][N], sSW[CPB][N], sTD[CPB][N],
+      sSD[CPB][N], sH[CPB][N];
+  const int cc = threadIdx.x / LPC, l = threadIdx.x % LPC;
+  const int j = blockIdx.x * CPB + cc;
+  if (j >= h.ncol) return;
+  const int i = h.cols[j], kdm = h.kdm, k = l + 1;
+  const bool lay = k <= kdm;
+  double *P = sP[cc], *W = sW[cc], *PO = sO[cc], *TH = sT[cc], *TW = sTW[cc], *SW = sSW[cc], *TD = sTD[cc];
+  double *SD = sSD[cc], *HF = sH[cc];
+  const size_t base = (size_t)i * kdm, pb = (size_t)i * (kdm + 1);
+  for (int m = k; m <= kdm + 1; m += LPC) {
+    P[m] = W[m] = h.pres[pb + m - 1];
+    PO[m] = h.pold[pb + m - 1];
+  }
+  double tem = 0., sal = 0.;
+  if (lay) {
+    TH[k] = h.dens[base + k - 1];
+    tem = h.temp[base + k - 1];
+    sal = h.saln[base + k - 1];
+    TW[k] = tem;
+    SW[k] = sal;
+  }
+  wave_sync();
+  // equilb (hycom_hybgn1.F90:1289-1752): heat diffusion, three passes
+  const double slak = .25, dffudt = .1;
+  double td = 0., sd = 0.;
+  if (lay) {
+    td = hmax(H_EPSIL, P[k + 1] - P[k]) * tem;
+    sd = hmax(H_EPSIL, P[k + 1] - P[k]) * sal;
+  }
+  for (int iter = 1; iter <= 3; iter++) {
+    double hf = 0.;
+    if (k >= 2 && k <= kdm - 1 && W[k] > W[1] + H_ONEM && W[k + 1] < W[kdm + 1] - H_ONEM) {
+      const double cnv = H_ONEM;
+      hf = dffudt * cnv * cnv * hmin(TH[k + 1] - TH[k], TH[k] - TH[k - 1]) / hmax(.01 * cnv, W[k + 1] - W[k]);
+      const double plimit1 = hmax(W[k] - Y_TENM, .5 * (W[k] + W[k - 1]));
+      const double plimit2 = hmin(W[k + 1] + Y_TENM, .5 * (W[k + 1] + W[k + 2]));
+      hf = hmin(hmin(hmax(0., hf), hmax(0., (plimit2 - W[k + 1]) * (TH[k + 1] - TH[k]))),
+                hmax(0., (W[k] - plimit1) * (TH[k] - TH[k - 1])));
+    }
+    if (lay) HF[k] = hf;
+    wave_sync();
+    // the interface below layer k moves by the flux difference; the heat
+    // and salt it sweeps leave one layer for the other
+    double dt = 0., ds = 0.;
+    int moved = 0;
+    if (k <= kdm - 1 && TH[k + 1] > TH[k]) {
+      moved = 1;
+      const double pn = W[k + 1] + (HF[k] - HF[k + 1]) / (TH[k + 1] - TH[k]);
+      const double delp = pn - W[k + 1];
+      dt = (delp > 0. ? TW[k + 1] : TW[k]) * delp;
+      ds = (delp > 0. ? SW[k + 1] : SW[k]) * delp;
+      P[k + 1] = pn;
+    }
+    const double dtu = __shfl_up(dt, 1, LPC), dsu = __shfl_up(ds, 1, LPC);
+    const int movu = __shfl_up(moved, 1, LPC);
+    if (lay) {                          // the interface above layer k is handled first
+      if (k > 1 && movu) {
+        td = td - dtu;
+        sd = sd - dsu;
+      }
+      if (moved) {
+        td = td + dt;
+        sd = sd + ds;
+      }
+    }
+    wave_sync();
+    if (lay && P[k + 1] > P[k]) {
+      TW[k] = td / (P[k + 1] - P[k]);
+      SW[k] = sd / (P[k + 1] - P[k]);
+    }
+    wave_sync();
+    for (int m = k; m <= kdm + 1; m += LPC) W[m] = P[m];
+    wave_sync();
+  }
+  if (lay) {
+    TD[k] = td;
+    SD[k] = sd;
+  }
+  wave_sync();
+  if (l == 0) {                         // scenario 4: thick-(thin-thin)-thick, down the column
+    for (int kk = 4; kk <= kdm - 2; kk++) {
+      if (P[kk - 2] < P[1] + Y_TENCM || P[kk + 2] > P[kdm + 1] - Y_TENCM) continue;
+      const double dp1 = P[kk - 1] - P[kk - 2], dp2 = P[kk] - P[kk - 1];
+      const double dp3 = P[kk + 1] - P[kk], dp4 = P[kk + 2] - P[kk + 1];
+      const double th1 = TH[kk - 2], th2 = TH[kk - 1], th3 = TH[kk], th4 = TH[kk + 1];
+      if (dp1 > dp2 + dp3 && dp4 > dp2 + dp3) {
+        const double goal = .5 * (dp2 + dp3 + hmin(dp1, dp4));
+        double dis2 = (goal - dp2 - dp3) * slak, dis4 = dis2;
+        if (th2 > th1 && th3 > th2 && th4 > th3) {
+          const double ratio = (th4 - th3) / (th2 - th1);
+          if (ratio > 1.) dis4 = dis2 / ratio;
+          else dis2 = dis4 * ratio;
+        }
+        P[kk - 1] = P[kk - 1] - dis2;
+        P[kk + 1] = P[kk + 1] + dis4;
+        TD[kk - 2] = TD[kk - 2] - dis2 * TW[kk - 2];
+        TD[kk - 1] = TD[kk - 1] + dis2 * TW[kk - 2];
+        TD[kk] = TD[kk] + dis4 * TW[kk + 1];
+        TD[kk + 1] = TD[kk + 1] - dis4 * TW[kk + 1];
+        SD[kk - 2] = SD[kk - 2] - dis2 * SW[kk - 2];
+        SD[kk - 1] = SD[kk - 1] + dis2 * SW[kk - 2];
+        SD[kk] = SD[kk] + dis4 * SW[kk + 1];
+        SD[kk + 1] = SD[kk + 1] - dis4 * SW[kk + 1];
+      }
+    }
+  }
+  wave_sync();
+  if (lay && P[k + 1] - P[k] > Y_ONEMU) {
+    tem = TD[k] / (P[k + 1] - P[k]);
+    sal = SD[k] / (P[k + 1] - P[k]);
+  }
+  wave_sync();
+  // the monotonicity guard of hybgn1_1d (hycom_hybgn1.F90:768-776)
+  if (l == 0)
+    for (int kk = 1; kk <= kdm - 2; kk++)
+      if (P[kk + 1] < P[kk] || P[kk + 1] > P[kk + 2]) P[kk + 1] = hmax(P[kk], hmin(P[kk + 1], PO[kk + 2]));
+  wave_sync();
+  // pcwise (hycom_hybgn1.F90:981-1244) with no migration limit: the
+  // velocities piecewise constant, the tracers piecewise parabolic
+  const double athird = 1. / 3., small = 1.e-11;
+  const bool flip = !(PO[1] < PO[kdm + 1]);
+  auto z = [&](double x) { return flip ? -x : x; };
+  double fco = 0., fcn = 0., dz = 0.;
+  if (lay) {
+    fco = hmax(0., z(PO[k + 1]) - z(PO[k]));
+    fcn = hmax(0., z(P[k + 1]) - z(P[k]));
+  }
+  if (k <= kdm - 1) dz = z(P[k + 1]) - z(PO[k + 1]);       // delz of the interface below layer k
+  double *FC = TW, *FN = W, *A = SW, *B = TD, *CQ = SD;    // equilb's columns are free now
+  if (lay) FC[k] = fco;
+  const int nfield = 2 + h.numtr;
+  for (int f = 0; f < nfield; f++) {
+    double *g = f == 0 ? h.u + base : f == 1 ? h.v + base : h.passv + ((size_t)(f - 2) * h.nip + i) * kdm;
+    wave_sync();
+    if (lay) FN[k] = g[k - 1];
+    wave_sync();
+    if (l == 0) {                       // zero-width cells take their neighbours' values
+      for (int kk = kdm - 1; kk >= 1; kk--) FN[kk] = (FN[kk] * FC[kk] + FN[kk + 1] * small) / (FC[kk] + small);
+      for (int kk = 2; kk <= kdm; kk++) FN[kk] = (FN[kk] * FC[kk] + FN[kk - 1] * small) / (FC[kk] + small);
+    }
+    wave_sync();
+    const double fn = lay ? FN[k] : 0.;
+    if (lay) {
+      double a = fn, b = 0., c = 0.;
+      if (f >= 2 && k > 1 && k < kdm) {
+        const double fl = hmax(small, FC[k - 1]), fm = hmax(small, fco), fr = hmax(small, FC[k + 1]);
+        const double yl = (fl * fn + fm * FN[k - 1]) / (fl + fm);
+        const double yr = (fr * fn + fm * FN[k + 1]) / (fr + fm);
+        y_parab(fn, yl, yr, a, b, c);
+      }
+      A[k] = a;
+      B[k] = b;
+      CQ[k] = c;
+    }
+    wave_sync();
+    double vhi = 0.;                    // vertfx of the interface below layer k
+    if (k <= kdm - 1) {
+      const int ki = k + 1;
+      double slab = 0., amount = 0.;
+      if (dz > 0.) {
+        for (int lyr = ki;; lyr++) {
+          if (slab >= dz) break;
+          const double fc = FC[lyr];
+          if (fc > 0.) {
+            const double dslab = hmin(slab + fc, dz) - hmin(slab, dz);
+            const double dx = dslab / fc;
+            const double fcdx = A[lyr] + B[lyr] * .5 * (dx - 1.) + CQ[lyr] * (.25 - dx * (.5 - dx * athird));
+            amount = amount + fcdx * dslab;
+            slab = slab + dslab;
+          }
+          if (!(lyr < kdm)) break;
+        }
+      } else if (dz < 0.) {
+        for (int lyr = ki - 1;; lyr--) {  // down to layer 2 (layer 1 from interface 2 only)
+          if (slab >= -dz) break;
+          const double fc = FC[lyr];
+          if (fc > 0.) {
+            const double dslab = hmin(slab + fc, -dz) - hmin(slab, -dz);
+            const double dx = dslab / fc;
+            const double fcdx = A[lyr] + B[lyr] * .5 * (1. - dx) + CQ[lyr] * (.25 - dx * (.5 - dx * athird));
+            amount = amount + fcdx * dslab;
+            slab = slab + dslab;
+          }
+          if (!(lyr > 2)) break;
+        }
+      }
+      if (slab != 0.) vhi = -dz * amount / slab;
+    }
+    const double vup = __shfl_up(vhi, 1, LPC);
+    if (lay) {
+      const double vertdv = vhi - (k == 1 ? 0. : vup);
+      const double amount = fn * fco - vertdv;
+      g[k - 1] = (fn * small + amount) / (small + fcn);
+    }
+  }
+  if (lay) {
+    h.temp[base + k - 1] = tem;
+    h.saln[base + k - 1] = sal;
+    h.dp[base + k - 1] = P[k + 1] - P[k];
+    h.pres[pb + k] = P[k + 1];
   }
 }
 
@@ -502,10 +554,16 @@ static int hyb_launch(fimhip_ocn *o, int leap, int with_tracers) {
   h.v = o->ed.vclin + (leap - 1) * cz;
   h.passv = o->tr.passv;
   if (h.ncol == 0) return FIMHIP_OK;
+  h.pold = o->hy.pold;
   g_ktimer = o->kt;
-  const dim3 grid((h.ncol + 63) / 64), blk(64);
-  if (o->kdm <= 32) KT_LAUNCH("hyc_hybgn1", o->stream, hyc_hybgn1<32><<<grid, blk, 0, o->stream>>>(h));
-  else KT_LAUNCH("hyc_hybgn1", o->stream, hyc_hybgn1<64><<<grid, blk, 0, o->stream>>>(h));
+  const dim3 b2(256);
+  if (o->kdm <= 32) {
+    KT_LAUNCH("hyc_hyb_move", o->stream, hyc_hyb_move<32, 32><<<dim3((h.ncol + 31) / 32), dim3(32), 0, o->stream>>>(h));
+    KT_LAUNCH("hyc_hyb_remap", o->stream, hyc_hyb_remap<32><<<dim3((h.ncol + 7) / 8), b2, 0, o->stream>>>(h));
+  } else {
+    KT_LAUNCH("hyc_hyb_move", o->stream, hyc_hyb_move<64, 16><<<dim3((h.ncol + 15) / 16), dim3(16), 0, o->stream>>>(h));
+    KT_LAUNCH("hyc_hyb_remap", o->stream, hyc_hyb_remap<64><<<dim3((h.ncol + 3) / 4), b2, 0, o->stream>>>(h));
+  }
   g_ktimer = nullptr;
   HIPCHK(hipGetLastError());
   return FIMHIP_OK;
@@ -525,7 +583,8 @@ extern "C" int fimhip_ocn_set_hybgen(fimhip_ocn *o, const double *targt, const d
     int *dc = nullptr;
     if ((r = o->arena.alloc(&dc, cols.size() + 1))) return r;
     if ((r = o->arena.alloc(&y.targt, (size_t)kdm)) || (r = o->arena.alloc(&y.salmin, (size_t)kdm)) ||
-        (r = o->arena.alloc(&y.dpmin, (size_t)kdm)) || (r = o->arena.alloc(&y.latij, (size_t)nip)))
+        (r = o->arena.alloc(&y.dpmin, (size_t)kdm)) || (r = o->arena.alloc(&y.latij, (size_t)nip)) ||
+        (r = o->arena.alloc(&y.pold, (size_t)nip * (kdm + 1))))
       return r;
     HIPCHK(hipMemcpyAsync(dc, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice, o->stream));
     HIPCHK(hipStreamSynchronize(o->stream));

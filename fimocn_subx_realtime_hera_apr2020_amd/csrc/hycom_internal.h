@@ -136,6 +136,7 @@ struct HycHybState {
   int *cols;
   int ncol;
   double *targt, *salmin, *dpmin, *latij, *dens;
+  double *pold;                   // (kdm+1,nip) work: the interfaces before the moves
 };
 
 }  // namespace hyc
