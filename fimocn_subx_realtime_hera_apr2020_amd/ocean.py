@@ -395,3 +395,35 @@ def hycom_hybgen_case(grid, kdm: int = 32, seed: int = 20240617, numtr: int = 1,
                 uvel=np.ascontiguousarray(uvel), vvel=np.ascontiguousarray(vvel), passv=np.ascontiguousarray(passv),
                 targt=targt, salmin=salmin_of(targt), dpmin=dpmin, latij=np.degrees(lat), batrop=batrop,
                 numtr=numtr)
+
+
+def hycom_ocean_case(grid, kdm: int = 32, seed: int = EDGVAR_CASE_SEED, numtr: int = 1) -> dict:
+    """The inputs of a whole HYCOM dynamics interval (hycom_run.F90:364-556:
+    bclin_edgvar, cnuity and momtum every step, transp1 every second step,
+    transp2, hybgn1 and transp0 at the end of the interval): hycom_momtum_case
+    plus temperature (a smooth surface field of 10-28 deg C decaying to 2 with
+    depth, e-folding 800 m, plus N(0, 0.3)), salinity U[34, 36], their
+    sigma_1 as dens, numtr passive tracers U[0, 1], the tracers' time
+    integral and start thickness (set by transp0), temdff = 0.02 and the grid
+    generator's tables (the targets of FIMrun/ocn_target.txt, salmin from
+    them, ocn_dpmin.txt, latitude in degrees)."""
+    case = hycom_momtum_case(grid, kdm=kdm, seed=seed)
+    rng = np.random.default_rng(seed + 19)
+    nip = case["wet"].shape[0]
+    lat = np.asarray(grid.lat, np.float64)
+    lon = np.asarray(grid.lon, np.float64)
+    zmid = 0.5 * (case["pres"][:, :-1] + case["pres"][:, 1:]) / ONEM
+    sst = np.clip(10.0 + 18.0 * smooth_field(lat, lon, seed + 7), 10.0, 28.0)
+    temp = 2.0 + (sst[:, None] - 2.0) * np.exp(-zmid / 800.0) + 0.3 * rng.normal(size=(nip, kdm))
+    saln = rng.uniform(34.0, 36.0, (nip, kdm))
+    targt = np.array(C.OCN_TARGET32, np.float64)
+    dpmin = np.array(C.OCN_DPMIN32, np.float64)
+    if kdm != 32:
+        x = np.linspace(0.0, 31.0, kdm)
+        targt = np.interp(x, np.arange(32), targt)
+        dpmin = np.interp(x, np.arange(32), dpmin)
+    case.update(temp=np.ascontiguousarray(temp), saln=saln, dens=np.ascontiguousarray(sigma1(temp, saln)),
+                passv=rng.uniform(0.0, 1.0, (numtr, nip, kdm)), numtr=numtr, temdff=0.02,
+                cumuflx=np.zeros((nip, 6, kdm)), dpinit=np.zeros((nip, kdm)), targt=targt, salmin=salmin_of(targt),
+                dpmin=dpmin, latij=np.degrees(lat))
+    return case

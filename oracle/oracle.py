@@ -348,3 +348,40 @@ def hycom_hybgn1(case: dict, leap: int, with_tracers: bool = False) -> dict:
                           P(a["dens"]), P(a["dp"]), P(a["pres"]), P(a["temp"]), P(a["saln"]), P(a["uvel"]),
                           P(a["vvel"]), ct.c_int(numtr), P(a["passv"]) if numtr else None)
     return a
+
+
+def _hyc_ocean_arrays(h):
+    if not hasattr(h, "o"):
+        h.o = {n: np.ascontiguousarray(h.case[n], np.float64).copy()
+               for n in ("temp", "saln", "dens", "passv", "cumuflx", "dpinit")}
+    return h.o
+
+
+def hycom_interval_op(h, op: str, leapn: int):
+    """One of hycom_run's interval operations on a HycomCnuity instance whose
+    case is ocean.hycom_ocean_case, sharing its arrays the way hycom_run does:
+    'transp1' (the step's massflx into cumuflx), 'transp2' (dpfinl = dp's
+    level leapn), 'hybgn1' (dens, dp, pres, temp, saln, uclin, vclin; no
+    tracers, hycom_run.F90:516) or 'transp0'.  The tracers, dens, cumuflx and
+    dpinit live in h.o."""
+    o, a, case = _hyc_ocean_arrays(h), h.a, h.case
+    P = lambda x: x.ctypes.data_as(_PD)
+    g = ct.byref(h.g)
+    if op == "transp1":
+        lib().fo_hycom_transp1(g, ct.c_double(case["batrop"]), P(a["massflx"]), P(o["cumuflx"]))
+    elif op == "transp2":
+        cz = case["kdm"] * case["wet"].shape[0]
+        dpfinl = np.ascontiguousarray(a["dp"].reshape(2, cz)[leapn - 1])
+        lib().fo_hycom_transp2(g, case["numtr"], ct.c_double(case["batrop"]), ct.c_double(case["temdff"]),
+                               P(o["temp"]), P(o["saln"]), P(o["passv"]), P(o["cumuflx"]), P(o["dpinit"]), P(dpfinl))
+    elif op == "transp0":
+        lib().fo_hycom_transp0(g, leapn, P(a["dp"]), P(o["dpinit"]), P(o["cumuflx"]))
+    elif op == "hybgn1":
+        tab = {n: np.ascontiguousarray(case[n], np.float64) for n in ("targt", "salmin", "dpmin", "latij")}
+        wet = np.ascontiguousarray(case["wet"], np.int32)
+        lib().fo_hycom_hybgn1(ct.c_int(case["wet"].shape[0]), ct.c_int(case["kdm"]), wet.ctypes.data_as(_PI),
+                              ct.c_int(leapn), P(tab["targt"]), P(tab["salmin"]), P(tab["dpmin"]),
+                              ct.c_double(case["batrop"]), P(tab["latij"]), P(o["dens"]), P(a["dp"]), P(a["pres"]),
+                              P(o["temp"]), P(o["saln"]), P(h.e["uclin"]), P(h.e["vclin"]), ct.c_int(0), None)
+    else:
+        raise ValueError(op)

@@ -558,3 +558,58 @@ def test_hybgen_bad_arguments_are_rejected():
             o.set_hybgen(case["targt"][:5], case["salmin"], case["dpmin"], case["latij"])
     finally:
         o.close()
+
+
+def _interval(oracle_lib, level, kdm, nsteps, numtr=1):
+    """hycom_run's dynamics over nsteps steps (hycom_run.F90:364-556):
+    bclin_edgvar, cnuity and momtum every step, transp1 every second step,
+    transp2 (dpfinl = dp's new level), hybgn1 and transp0 every 30th; the
+    checker on shared arrays beside the device-resident chain."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, ocean
+    g = _grid(level)
+    case = ocean.hycom_ocean_case(g, kdm=kdm, seed=73 + level, numtr=numtr)
+    h = oracle_lib.HycomCnuity(g, case)
+    o = fimhip.Ocean(g, kdm, case["wet"], case["dtleap"])
+    o.set_momtum(case["batrop"], case["veldff"], case["smagcf"])
+    o.set_transport(numtr, case["temdff"])
+    o.set_hybgen(case["targt"], case["salmin"], case["dpmin"], case["latij"])
+    for n in fimhip.OCN_MOMTUM_FIELDS + ["temp", "saln", "dens", "passv", "cumuflx", "dpinit"]:
+        o.upload(n, case[n])
+    oracle_lib.hycom_interval_op(h, "transp0", 1)
+    o.step_transp0(1)
+    for st in range(1, nsteps + 1):
+        leapm = 1 + st % 2
+        leapn = 3 - leapm
+        h.edgvar(g, leapm, leapn, False)
+        assert h.step(st, leapm, leapn) == 0
+        h.momtum(g, leapm, leapn, False)
+        o.step_edgvar(leapm, leapn)
+        o.step(st, leapm, leapn)
+        o.step_momtum(leapm, leapn)
+        if st % 2 == 0:
+            oracle_lib.hycom_interval_op(h, "transp1", leapn)
+            o.step_transp1()
+        if st % 30 == 0:
+            for op in ("transp2", "hybgn1", "transp0"):
+                oracle_lib.hycom_interval_op(h, op, leapn)
+            o.step_transp2(leapn)
+            o.step_hybgn1(leapn)
+            o.step_transp0(leapn)
+    return h, o
+
+
+@pytest.mark.parametrize("level,kdm", [(4, 32), (3, 48)])
+def test_ocean_interval_resident_matches_checker(oracle_lib, level, kdm):
+    print("case: G%d x %d, 31 ocean steps (edgvar, cnuity, momtum, transport, hybgn1) resident" % (level, kdm),
+          flush=True)
+    h, o = _interval(oracle_lib, level, kdm, 31)
+    try:
+        for n in ARRAYS:
+            assert _same(o.download(n), h.a[n]), n
+        for n in ("uclin", "vclin", "ustarb"):
+            assert _same(o.download(n), h.e[n]), n
+        for n in ("temp", "saln", "dens", "passv", "cumuflx", "dpinit"):
+            assert _same(o.download(n), h.o[n]), n
+        assert o.mass == h.mass
+    finally:
+        o.close()
