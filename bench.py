@@ -620,24 +620,38 @@ HYC_HYB_KERNEL_BYTES = {"hyc_hyb_move": 4 * 8 + 5 * 8, "hyc_hyb_remap": 7 * 8 + 
 HYC_HYB_STEP_BYTES = 2 * 6 * 8 + 8 + 8
 
 
-def _ocean_leg(a, o, wcl, step, byte_model):
+def _ocean_leg(a, o, wcl, step, byte_model, reset=None):
     """Time 'step(nstep)' over a.steps steps after a.warmup, then the same
     number under per-kernel HIP-event timing; throughput, kernel table and
-    the dominant kernel against its algorithmic bytes."""
+    the dominant kernel against its algorithmic bytes.  With reset, every
+    step starts from the state reset() restores, outside the timed region."""
     import torch
     nstep = 1
     for _ in range(a.warmup):
+        if reset:
+            reset()
         step(nstep)
         nstep += 1
     torch.cuda.synchronize()
+    wall = 0.
     t0 = time.perf_counter()
     for _ in range(a.steps):
+        if reset:
+            reset()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
         step(nstep)
         nstep += 1
+        if reset:
+            torch.cuda.synchronize()
+            wall += time.perf_counter() - t0
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    if not reset:
+        wall = time.perf_counter() - t0
     o.profile(True)
     for _ in range(a.steps):
+        if reset:
+            reset()
         step(nstep)
         nstep += 1
     prof = o.profile_report()
@@ -668,9 +682,11 @@ def run_hycom(a, fimhip, cpu=True):
     feeding the next step; and the tracer transport over one 30-step
     interval (hycom_transp3d.F90 transp1 every second step, transp2 with
     hycom_fct3d and hycom_dffusn, transp0); and the hybrid grid generator
-    (hycom_hybgn1.F90 hybgn1).  The checker's C restatement of each is timed
-    on one host core over 2 steps (one interval, one call) of the same
-    input."""
+    (hycom_hybgn1.F90 hybgn1); and all of them together over a whole 30-step
+    interval in hycom_run's order.  The checker's C restatement of each
+    routine is timed on one host core over 2 steps (one interval, one call)
+    of the same input (not for the whole interval: 30 steps of it take
+    minutes on one core)."""
     import numpy as np
     from fimocn_subx_realtime_hera_apr2020_amd import ocean
     g = fimhip.icos_grid(8)
@@ -773,6 +789,53 @@ def run_hycom(a, fimhip, cpu=True):
                                 "achieved": hb * wcl / (ms * 1e-3) / 1e9, "unit": "GB/s",
                                 "frac": hb * wcl / (ms * 1e-3) / HBM_PEAK}
         out["hybgen"] = leg
+    finally:
+        o.close()
+    # a whole 30-step interval of the ocean's dynamics in hycom_run's order:
+    # bclin_edgvar, cnuity and momtum every step, transp1 every second step,
+    # transp2, hybgn1 and transp0 at the end (temperature, salinity and one
+    # passive tracer); a timed step is one interval
+    import types
+    ocase = ocean.hycom_ocean_case(g, kdm=kdm, numtr=1)
+    o = fimhip.Ocean(g, kdm, ocase["wet"], ocase["dtleap"], thkdff=thkdff, biharm=biharm)
+    try:
+        o.set_momtum(ocase["batrop"], ocase["veldff"], ocase["smagcf"])
+        o.set_transport(ocase["numtr"], ocase["temdff"])
+        o.set_hybgen(ocase["targt"], ocase["salmin"], ocase["dpmin"], ocase["latij"])
+        state = {"st": 0}
+
+        def reset():
+            # the synthetic state drifts without the routines outside this
+            # path (hystat, barotp, the mixing): each interval starts afresh
+            for n in fimhip.OCN_MOMTUM_FIELDS + ["temp", "saln", "dens", "passv", "cumuflx", "dpinit"]:
+                o.upload(n, ocase[n])
+            o.mass = (0.0, 0.0, 0.0)
+            o.step_transp0(1)
+            state["st"] = 0
+
+        def interval(_):
+            for _j in range(30):
+                state["st"] += 1
+                st = state["st"]
+                leapm = 1 + st % 2
+                leapn = 3 - leapm
+                o.step_edgvar(leapm, leapn)
+                o.step(st, leapm, leapn)
+                o.step_momtum(leapm, leapn)
+                if st % 2 == 0:
+                    o.step_transp1()
+            o.step_transp2(leapn)
+            o.step_hybgn1(leapn)
+            o.step_transp0(leapn)
+        ia = types.SimpleNamespace(steps=max(1, a.steps // 10), warmup=1 if a.steps > 2 else 0)
+        bytes_all = dict(HYC_KERNEL_BYTES, **HYC_EDG_KERNEL_BYTES, **HYC_MOM_KERNEL_BYTES,
+                         **hyc_tr_kernel_bytes(2 + ocase["numtr"]), **HYC_HYB_KERNEL_BYTES)
+        leg = _ocean_leg(ia, o, wcl, interval, bytes_all, reset=reset)
+        leg["unit"] = "ocean cell-level-intervals/s"
+        leg["steps_per_interval"] = 30
+        leg["ocean_cell_level_steps_per_s"] = leg["value"] * 30
+        leg["intervals_timed"] = ia.steps
+        out["ocean_interval"] = leg
     finally:
         o.close()
     if cpu:
