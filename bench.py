@@ -473,7 +473,9 @@ def run_uneven(a, th, ds, fimhip):
     32 layers, temperature and salinity, ~30 % land, dt = 45 s), both in the
     mixed-precision form, stepped side by side (each step both instances'
     captured steps enqueued on their own streams before any wait);
-    beside it each instance alone, and the atmosphere part in FP64."""
+    beside it each instance alone, and the atmosphere part in FP64; and the
+    atmosphere beside HYCOM's own ocean dynamics (bclin_edgvar, cnuity,
+    momtum in FP64) on the G8 ocean."""
     import torch
     from fimocn_subx_realtime_hera_apr2020_amd import glvl, ocean
     gu = glvl.concat([fimhip.icos_grid(6), fimhip.icos_grid(7)])
@@ -550,6 +552,66 @@ def run_uneven(a, th, ds, fimhip):
                        "cell_levels_per_step": cells + ocells,
                        "note": "atmosphere and ocean instances, each step every instance's captured step "
                                "enqueued on its own stream before any wait"}
+    # the same atmosphere beside HYCOM's own dynamics on the G8 ocean
+    # (hycom_edgvar/cnuity/momtum, FP64, 32 layers, ~30 % land): each step
+    # the atmosphere's captured step is enqueued, then the ocean's three
+    # routines on the ocean instance's stream (cnuity waits for its own
+    # kernels only: it reads the duplicate-flux check and the mass sums)
+    hcase = ocean.hycom_momtum_case(go, kdm=32)
+    nwet = int((hcase["wet"] > 0).sum())
+    hcells = nwet * 32
+    atm = fimhip.Ensemble(gu, 1, a.nvl, a.ntra, dt, th, ds, device=0)
+    hyo = fimhip.Ocean(go, 32, hcase["wet"], hcase["dtleap"])
+    try:
+        atm.synth(0, seed=SEED, synth_member=0, flags=fimhip.SYNTH_WIND_NOISE)
+        atm.init_state(True)
+        hyo.set_momtum(hcase["batrop"], hcase["veldff"], hcase["smagcf"])
+        for n in fimhip.OCN_MOMTUM_FIELDS:
+            hyo.upload(n, hcase[n])
+
+        def hycom_step(st):
+            leapm = 1 + st % 2
+            hyo.step_edgvar(leapm, 3 - leapm)
+            hyo.step(st, leapm, 3 - leapm)
+            hyo.step_momtum(leapm, 3 - leapm)
+        st = 0
+        for _ in range(a.warmup):
+            st += 1
+            hycom_step(st)
+        hyo.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            st += 1
+            hycom_step(st)
+        hyo.sync()
+        torch.cuda.synchronize()
+        wh = time.perf_counter() - t0
+        for _ in range(a.warmup):
+            st += 1
+            atm.step(1, a.mode)
+            hycom_step(st)
+        atm.sync()
+        hyo.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            st += 1
+            atm.step(1, a.mode)
+            hycom_step(st)
+        atm.sync()
+        hyo.sync()
+        torch.cuda.synchronize()
+        wth = time.perf_counter() - t0
+    finally:
+        atm.close()
+        hyo.close()
+    out["hycom_ocean"] = {"grid": "G8 x 32 layers, HYCOM dynamics (bclin_edgvar, cnuity, momtum), FP64, %d ocean "
+                                  "cells" % nwet, "value": hcells * a.steps / wh, "ms_per_step": wh / a.steps * 1e3}
+    out["together_hycom"] = {"value": (cells + hcells) * a.steps / wth, "ms_per_step": wth / a.steps * 1e3,
+                             "cell_levels_per_step": cells + hcells,
+                             "note": "the atmosphere instance's captured step and HYCOM's ocean routines on their "
+                                     "own streams, side by side"}
     return out
 
 

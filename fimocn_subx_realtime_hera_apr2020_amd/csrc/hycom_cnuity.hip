@@ -673,6 +673,14 @@ double *ocn_field(fimhip_ocn *o, int f, size_t *n) {
   return nullptr;
 }
 
+// waits for every asynchronous call queued on the instance
+extern "C" int fimhip_ocn_sync(fimhip_ocn *o) {
+  if (!o) return FIMHIP_EINVAL;
+  HIPCHK(hipSetDevice(o->device));
+  HIPCHK(hipStreamSynchronize(o->stream));
+  return FIMHIP_OK;
+}
+
 extern "C" long long fimhip_ocn_field_size(const fimhip_ocn *o, int f) {
   if (!o) return FIMHIP_EINVAL;
   size_t n = 0;

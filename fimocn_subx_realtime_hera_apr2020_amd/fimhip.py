@@ -53,7 +53,8 @@ EXPORTS = ["fimhip_create", "fimhip_destroy", "fimhip_abstart", "fimhip_set_ab",
            "fimhip_ocn_edgvar_r8", "fimhip_ocn_edgvar", "fimhip_ocn_set_janjic", "fimhip_ocn_momtum_r8",
            "fimhip_ocn_momtum", "fimhip_ocn_set_momtum", "fimhip_ocn_set_transport", "fimhip_ocn_transp0_r8",
            "fimhip_ocn_transp1_r8", "fimhip_ocn_transp2_r8", "fimhip_ocn_transp0", "fimhip_ocn_transp1",
-           "fimhip_ocn_transp2", "fimhip_ocn_set_hybgen", "fimhip_ocn_hybgn1_r8", "fimhip_ocn_hybgn1"]
+           "fimhip_ocn_transp2", "fimhip_ocn_set_hybgen", "fimhip_ocn_hybgn1_r8", "fimhip_ocn_hybgn1",
+           "fimhip_ocn_sync"]
 
 # real*4 drop-ins: name -> (number of float arrays, trailing int argument)
 R4_ENTRIES = {"edgvar1": (16, False), "edgvar3": (16, False), "cnuity": (13, True), "mass_adv": (12, True),
@@ -193,6 +194,7 @@ def load_library() -> ct.CDLL:
     lib.fimhip_ocn_set_hybgen.argtypes = [vp, _PD, _PD, _PD, _PD]
     lib.fimhip_ocn_hybgn1_r8.argtypes = [vp, ct.c_int, ct.c_int] + [_PD] * 10
     lib.fimhip_ocn_hybgn1.argtypes = [vp, ct.c_int, ct.c_int]
+    lib.fimhip_ocn_sync.argtypes = [vp]
     lib.fimhip_ocn_profile.argtypes = [vp, ct.c_int]
     lib.fimhip_ocn_profile_report.argtypes = [vp, ct.c_char_p, ct.c_int]
     _LIB = lib
@@ -817,6 +819,10 @@ class Ocean:
         """hybgn1 on the device-resident arrays (dens, dp, pres, temp, saln,
         uclin, vclin; passv with_tracers)."""
         _chk(self.lib.fimhip_ocn_hybgn1(self.h, leap, int(with_tracers)), "ocn_hybgn1")
+
+    def sync(self):
+        """Wait for the instance's asynchronous calls."""
+        _chk(self.lib.fimhip_ocn_sync(self.h), "ocn_sync")
 
     def step_transp0(self, leapn: int):
         """transp0 on the device-resident arrays, from dp's level leapn."""

@@ -216,6 +216,9 @@ module fimhip_bind
       real(c_double) :: dens(*), dp(*), pres(*), temp(*), saln(*), uvel(*), vvel(*)
       type(c_ptr), value :: passv_tr
     end function
+    integer(c_int) function fimhip_ocn_sync(ocn) bind(C, name="fimhip_ocn_sync")
+      import; type(c_ptr), value :: ocn
+    end function
     integer(c_int) function fimhip_ocn_set_janjic(ocn, on) bind(C, name="fimhip_ocn_set_janjic")
       import; type(c_ptr), value :: ocn; integer(c_int), value :: on
     end function

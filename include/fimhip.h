@@ -414,6 +414,10 @@ int fimhip_ocn_cnuity_r8(fimhip_ocn *ocn, int nstep, int leapm, int leapn, const
 /* the same step on the device-resident arrays (fimhip_ocn_upload first) */
 int fimhip_ocn_cnuity(fimhip_ocn *ocn, int nstep, int leapm, int leapn);
 long long fimhip_ocn_field_size(const fimhip_ocn *ocn, int field);
+/* waits for the instance's asynchronous calls (edgvar, momtum, transp0/1/2,
+ * hybgn1 on the device-resident arrays); no counterpart in the reference,
+ * whose routines return when done */
+int fimhip_ocn_sync(fimhip_ocn *ocn);
 int fimhip_ocn_upload(fimhip_ocn *ocn, int field, const double *host);
 int fimhip_ocn_download(fimhip_ocn *ocn, int field, double *host);
 /* hycom_edgvar.F90:128 bclin_edgvar(nstep, leapm, leapn, uclin, vclin, utrop,
