@@ -598,8 +598,11 @@ def _interval(oracle_lib, level, kdm, nsteps, numtr=1):
     return h, o
 
 
-@pytest.mark.parametrize("level,kdm", [(4, 32), (3, 48)])
+@pytest.mark.parametrize("level,kdm", [(4, 32), (3, 48), (3, 64), (3, 16)])
 def test_ocean_interval_resident_matches_checker(oracle_lib, level, kdm):
+    """The whole interval at 32 layers (two columns per wavefront), 48 and
+    the maximum 64 (one column per wavefront; at 64 the bottom interface has
+    no lane of its own), and 16 (half of each column's lanes idle)."""
     print("case: G%d x %d, 31 ocean steps (edgvar, cnuity, momtum, transport, hybgn1) resident" % (level, kdm),
           flush=True)
     h, o = _interval(oracle_lib, level, kdm, 31)
