@@ -471,7 +471,9 @@ __global__ void __launch_bounds__(256) hyc_dff_apply(HycGrid g, HycTr tr, const 
 using namespace fimhip;
 using namespace fimhip::hyc;
 
-static int tr_ready(const fimhip_ocn *o) { return o->tr.nfld > 0 && o->mo.batrop > 0.; }
+// the transport's arrays exist; transp1 and transp2 also need batrop
+static int tr_alloc(const fimhip_ocn *o) { return o->tr.nfld > 0; }
+static int tr_ready(const fimhip_ocn *o) { return tr_alloc(o) && o->mo.batrop > 0.; }
 
 template <int LPC>
 static int tr2_kernels(fimhip_ocn *o, const double *dpfinl) {
@@ -551,7 +553,7 @@ extern "C" int fimhip_ocn_set_transport(fimhip_ocn *o, int numtr, double temdff)
 
 // transp0 on the device-resident arrays (hycom_transp3d.F90:18-51)
 extern "C" int fimhip_ocn_transp0(fimhip_ocn *o, int leapn) {
-  if (!o || !tr_ready(o) || leapn < 1 || leapn > 2) return FIMHIP_EINVAL;
+  if (!o || !tr_alloc(o) || leapn < 1 || leapn > 2) return FIMHIP_EINVAL;
   HIPCHK(hipSetDevice(o->device));
   g_ktimer = o->kt;
   const dim3 blk(256);
@@ -600,7 +602,7 @@ static int ocn_get(fimhip_ocn *o, int f, double *h) {
 extern "C" int fimhip_ocn_transp0_r8(fimhip_ocn *o, int nstep, int leapn, double *cumuflx, const double *dp,
                                     double *dpinit) {
   (void)nstep;
-  if (!o || !cumuflx || !dp || !dpinit || !tr_ready(o)) return FIMHIP_EINVAL;
+  if (!o || !cumuflx || !dp || !dpinit || !tr_alloc(o)) return FIMHIP_EINVAL;
   HIPCHK(hipSetDevice(o->device));
   int r;
   if ((r = ocn_put(o, FIMHIP_OCN_DP, dp)) || (r = ocn_put(o, FIMHIP_OCN_DPINIT, dpinit))) return r;

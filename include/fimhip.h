@@ -456,9 +456,9 @@ int fimhip_ocn_set_momtum(fimhip_ocn *ocn, double batrop, double veldff, double 
 int fimhip_ocn_set_janjic(fimhip_ocn *ocn, int on);
 /* the tracer transport's numtr passive tracers (0..8; hycom_control.F90:3,
  * 1 there) and temdff (fimnamelist.F90:255, the tracers' mixing velocity in
- * m/s, 0.02 there; dffusn_lyr runs where > 0): allocates the transport's arrays, once per
- * instance (a later call may change temdff only).  batrop comes from
- * fimhip_ocn_set_momtum. */
+ * m/s, 0.02 there; dffusn_lyr runs where > 0): allocates the transport's
+ * arrays, once per instance (a later call may change temdff only).  transp1
+ * and transp2 use fimhip_ocn_set_momtum's batrop. */
 int fimhip_ocn_set_transport(fimhip_ocn *ocn, int numtr, double temdff);
 /* hycom_transp3d.F90:18 transp0(nstep, leapn, cumuflx, dp, dpinit) on host
  * arrays: cumuflx restarts (0 on the ocean, -1e33 on land), dpinit (ocean

@@ -437,10 +437,13 @@ def test_transport_bad_arguments_are_rejected():
     try:
         with pytest.raises(fimhip.FimhipError):
             o.step_transp1()                 # before set_transport
+        o.set_transport(1, 0.02)
+        o.step_transp0(1)                    # transp0 needs no batrop
+        with pytest.raises(fimhip.FimhipError):
+            o.step_transp1()                 # transp1 does
         o.set_momtum(case["batrop"])
         with pytest.raises(fimhip.FimhipError):
             o.set_transport(9, 0.02)
-        o.set_transport(1, 0.02)
         with pytest.raises(fimhip.FimhipError):
             o.set_transport(2, 0.02)         # numtr is fixed once set
         o.set_transport(1, 0.0)
@@ -450,9 +453,6 @@ def test_transport_bad_arguments_are_rejected():
             o.step_transp0(0)
     finally:
         o.close()
-
-
-HYB_OUT = ("dens", "dp", "pres", "temp", "saln", "uvel", "vvel")
 
 
 def _hyb_ocean(g, case, numtr=0):
