@@ -76,12 +76,15 @@ FC = os.environ.get("FC_AMD", "/opt/rocm/lib/llvm/bin/amdflang")
 FORTRAN_BIND = os.path.join(PKG, "fortran", "fimhip_bind.F90")
 CALLER_SRC = os.path.join(ROOT, "tests", "fortran", "fimhip_caller.F90")
 CALLER = os.path.join(ROOT, "tests", "fortran", "fimhip_caller")
+OCN_CALLER_SRC = os.path.join(ROOT, "tests", "fortran", "ocn_caller.F90")
+OCN_CALLER = os.path.join(ROOT, "tests", "fortran", "ocn_caller")
 
 
 def build_fortran(verbose: bool = False):
     """The Fortran binding module (fortran/fimhip_bind.F90) and the test
-    driver that calls the library through it (tests/fortran/fimhip_caller),
-    with amdflang; None when no Fortran compiler is installed."""
+    drivers that call the library through it (tests/fortran/fimhip_caller for
+    the atmosphere, tests/fortran/ocn_caller for the HYCOM ocean), with
+    amdflang; None when no Fortran compiler is installed."""
     if not os.path.exists(FC):
         return None
     od = os.path.join(_objs_dir(), "fortran")
@@ -89,12 +92,16 @@ def build_fortran(verbose: bool = False):
     cmds = [[FC, "-c", "-O2", "-J" + od, FORTRAN_BIND, "-o", os.path.join(od, "fimhip_bind.o")],
             [FC, "-c", "-O2", "-I" + od, "-J" + od, CALLER_SRC, "-o", os.path.join(od, "fimhip_caller.o")],
             [FC, os.path.join(od, "fimhip_caller.o"), os.path.join(od, "fimhip_bind.o"), "-L" + PKG, "-lfimhip",
-             "-Wl,-rpath,$ORIGIN/../../" + os.path.basename(PKG), "-o", CALLER + ".tmp"]]
+             "-Wl,-rpath,$ORIGIN/../../" + os.path.basename(PKG), "-o", CALLER + ".tmp"],
+            [FC, "-c", "-O2", "-I" + od, "-J" + od, OCN_CALLER_SRC, "-o", os.path.join(od, "ocn_caller.o")],
+            [FC, os.path.join(od, "ocn_caller.o"), os.path.join(od, "fimhip_bind.o"), "-L" + PKG, "-lfimhip",
+             "-Wl,-rpath,$ORIGIN/../../" + os.path.basename(PKG), "-o", OCN_CALLER + ".tmp"]]
     for cmd in cmds:
         if verbose:
             print(" ".join(cmd), file=sys.stderr)
         subprocess.check_call(cmd, cwd=od)
     os.replace(CALLER + ".tmp", CALLER)
+    os.replace(OCN_CALLER + ".tmp", OCN_CALLER)
     return CALLER
 
 

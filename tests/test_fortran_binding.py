@@ -75,3 +75,63 @@ def test_fortran_caller_steps_equal_checker(tmp_path, oracle_lib):
         off += a.size
         assert np.array_equal(got, a), (n, float(np.abs(got - a).max()))
     assert off == buf.size
+
+
+OCN_CALLER = os.path.join(ROOT, "tests", "fortran", "ocn_caller")
+
+
+def write_ocean_case(path, g, case):
+    """The case file ocn_caller reads (layout in its header comment)."""
+    nip, kdm = case["wet"].shape[0], case["kdm"]
+    with open(path, "wb") as f:
+        f.write(struct.pack("<6i", 1329808716, nip, kdm, case["numtr"], 0, 0))
+        f.write(struct.pack("<7d", case["dtleap"], 0.05, 0.75, case["batrop"], case["veldff"], case["smagcf"],
+                            case["temdff"]))
+        for a in (g.nprox, g.prox, g.proxs, case["wet"]):
+            f.write(np.ascontiguousarray(a, "<i4").tobytes())
+        for a in (g.cs, g.sn, g.sidevec_c, g.sidevec_e, g.sideln, g.area, g.lat, g.rprox_ln):
+            f.write(np.ascontiguousarray(a, "<f8").tobytes())
+        for n in ("uclin", "vclin", "utrop", "vtrop", "dp", "pres", "geop", "montg", "spcifv", "um_edg", "vm_edg",
+                  "un_edg", "vn_edg", "dpm_edg", "pr_edg", "geop_edg", "mont_edg", "uvsq_edg", "spcv_edg",
+                  "massflx", "mssfx_avg", "taux", "tauy", "ustarb", "temp", "saln", "dens", "passv", "targt",
+                  "salmin", "dpmin", "latij"):
+            f.write(np.ascontiguousarray(case[n], "<f8").tobytes())
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(OCN_CALLER), reason="Fortran driver not built (no amdflang)")
+@pytest.mark.skipif(not gpu_available(), reason="needs a GPU")
+def test_fortran_ocean_caller_equals_checker(tmp_path, oracle_lib):
+    """The HYCOM ocean from Fortran through the _r8 wrappers in hycom_run's
+    order (transp0; bclin_edgvar, cnuity, momtum twice; transp1; transp2,
+    hybgn1, transp0): every array has the checker's bytes."""
+    from fimocn_subx_realtime_hera_apr2020_amd import fimhip, ocean
+    O = oracle_lib
+    g = fimhip.icos_grid(3)
+    case = ocean.hycom_ocean_case(g, kdm=32, seed=79, numtr=1)
+    path, out = str(tmp_path / "ocn.bin"), str(tmp_path / "ocn_out.bin")
+    write_ocean_case(path, g, case)
+    print("case: Fortran ocean caller, G3 x 32, 2 ocean steps and an interval end", flush=True)
+    r = subprocess.run([OCN_CALLER, path, out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    h = O.HycomCnuity(g, case)
+    O.hycom_interval_op(h, "transp0", 1)
+    for st in (1, 2):
+        leapm = 1 + st % 2
+        leapn = 3 - leapm
+        h.edgvar(g, leapm, leapn, False)
+        assert h.step(st, leapm, leapn) == 0
+        h.momtum(g, leapm, leapn, False)
+        if st % 2 == 0:
+            O.hycom_interval_op(h, "transp1", leapn)
+    for op in ("transp2", "hybgn1", "transp0"):
+        O.hycom_interval_op(h, op, leapn)
+    want = [h.a["dp"], h.a["pres"], h.a["massflx"], h.a["mssfx_avg"], h.e["uclin"], h.e["vclin"], h.e["ustarb"],
+            h.o["temp"], h.o["saln"], h.o["dens"], h.o["cumuflx"], h.o["dpinit"], h.o["passv"]]
+    buf = np.fromfile(out, "<f8")
+    off = 0
+    for j, a in enumerate(want):
+        got = buf[off:off + a.size]
+        off += a.size
+        assert np.array_equal(got.view(np.uint64), np.ravel(a).view(np.uint64)), j
+    assert off == buf.size
