@@ -626,8 +626,10 @@ HYC_KERNEL_BYTES = {"hyc_flux": 352, "hyc_lowsum": 144, "hyc_limits": 48, "hyc_c
 HYC_STEP_BYTES = 8 * 48 + 2 * 24
 # the same for the edge interpolation's kernels (neighbour values counted once)
 HYC_EDG_KERNEL_BYTES = {
-    "hyc_uvedge": 8 * 2 + 2 * 48 + 8,          # one launch per level: uclin vclin -> u, v edges; dp -> pres (first, half counted)
-    "hyc_edgint": 8 + 8 + 16 + 96 + 50 + 4 * 48,   # dp pres montg spcifv um vm -> pr_edg dp_edg mont uvsq spcv
+    # one launch per level: uclin vclin -> u, v edges; the first also dp ->
+    # pres and u, v edges -> uvsq (both counted half, the mean per launch)
+    "hyc_uvedge": 8 * 2 + 2 * 48 + 8 + 24,
+    "hyc_edgint": 8 + 8 + 16 + 50 + 3 * 48,    # dp pres montg spcifv -> pr_edg dp_edg mont spcv
     "hyc_janjic": 8 + 8 + 8 + 48,              # pres geop spcifv -> geop_edg
 }
 HYC_MOM_KERNEL_BYTES = {

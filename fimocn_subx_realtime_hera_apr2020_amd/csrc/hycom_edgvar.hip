@@ -12,13 +12,13 @@
 //                cell, its neighbour and the two joint neighbours, one launch
 //                per leapfrog level (6-124, 273-296); with the first, the
 //                interface pressure of the mid-time thickness (the column sum
-//                through LDS).
+//                through LDS) and the kinetic energy on the edges (411-412).
 //   hyc_edgint   interface pressure and thickness on the edges: each edge's
 //                two sequential chains (down from the shallower bottom, then
 //                up, limited by the interpolated thickness) run on one lane
 //                over increments the column's lanes put in LDS; Montgomery
-//                potential, specific volume with the sidewall blend, and
-//                kinetic energy on the edges  (300-418).
+//                potential and specific volume with the sidewall blend on the
+//                edges  (300-418).
 // A neighbour's column values are loaded once per lane and shared by the
 // three edges that use it (as the edge's cell or as a joint neighbour).
 #include "hycom_internal.h"
@@ -138,7 +138,7 @@ __device__ __forceinline__ UV uv_edge(const Rot &rt, bool wi, bool wpx, bool wm1
 
 template <int LPC>
 __global__ void __launch_bounds__(256) hyc_uvedge(HycGrid g, HycState s, HycEdg ed, int leapm, int slot,
-                                                  double *ue, double *ve, int with_pres) {
+                                                  double *ue, double *ve, int with_pres, double *uvsq) {
   __shared__ double sh[256 / LPC][LPC + 1];
   __shared__ double rot[256 / LPC][2 * NPP * 4];     // the cell's cs then sn, (4,6) each
   const int kdm = g.kdm, nip = g.nip;
@@ -196,6 +196,9 @@ __global__ void __launch_bounds__(256) hyc_uvedge(HycGrid g, HycState s, HycEdg 
       if (live) {
         HE(ue, i, e, k) = r.u;
         HE(ve, i, e, k) = r.v;
+        // the kinetic energy on the edges, from the values just stored (0 on
+        // land and in a pentagon's sixth slot)
+        if (uvsq) HE(uvsq, i, e, k) = wi && e < np ? .5 * (r.u * r.u + r.v * r.v) : 0.;
       }
     }
   }
@@ -218,8 +221,7 @@ __global__ void __launch_bounds__(256) hyc_edgint(HycGrid g, HycState s, HycEdg 
   if (!ci_wet(ci)) {
 #pragma unroll
     for (int e = 0; e < NPP; e++) {
-      if (live) HE(s.dpm_edg, i, e, k) = HE(ed.mont_edg, i, e, k) = HE(ed.uvsq_edg, i, e, k) =
-                    HE(ed.spcv_edg, i, e, k) = 0.;
+      if (live) HE(s.dpm_edg, i, e, k) = HE(ed.mont_edg, i, e, k) = HE(ed.spcv_edg, i, e, k) = 0.;
       for (int j = k; j <= kdm; j += LPC) HPR(i, e, j) = 0.;
     }
     return;
@@ -278,8 +280,7 @@ __global__ void __launch_bounds__(256) hyc_edgint(HycGrid g, HycState s, HycEdg 
 #pragma unroll
   for (int e = 0; e < NPP; e++) {
     if (e >= np) {
-      if (live) HE(s.dpm_edg, i, e, k) = HE(ed.mont_edg, i, e, k) = HE(ed.uvsq_edg, i, e, k) =
-                    HE(ed.spcv_edg, i, e, k) = 0.;
+      if (live) HE(s.dpm_edg, i, e, k) = HE(ed.mont_edg, i, e, k) = HE(ed.spcv_edg, i, e, k) = 0.;
       for (int j = k; j <= kdm; j += LPC) HPR(i, e, j) = 0.;
       continue;
     }
@@ -295,8 +296,6 @@ __global__ void __launch_bounds__(256) hyc_edgint(HycGrid g, HycState s, HycEdg 
     if (live) {
       HE(s.dpm_edg, i, e, k) = hmax(0., hrow[k + 1] - hrow[k]);
       HE(ed.mont_edg, i, e, k) = (8. * (mi + nm1) + nm2 + nm3) * H_DIVBY18;
-      const double um = HE(s.um, i, e, k), vm = HE(s.vm, i, e, k);
-      HE(ed.uvsq_edg, i, e, k) = .5 * (um * um + vm * vm);
       HE(ed.spcv_edg, i, e, k) = (8. * (svi + nv1) + nv2 + nv3) * H_DIVBY18;
     }
   }
@@ -319,8 +318,9 @@ static void edg_kernels(fimhip_ocn *o, int leapm, int leapn) {
   if (o->janjic) KT_LAUNCH("hyc_janjic", st, hyc_janjic<LPC><<<grid, blk, 0, st>>>(g, s, ed));
   // one launch per leapfrog level (both levels in one kernel would hold every
   // edge's rotation coefficients in registers across them)
-  KT_LAUNCH("hyc_uvedge", st, hyc_uvedge<LPC><<<grid, blk, 0, st>>>(g, s, ed, leapm, leapm, s.um, s.vm, 1));
-  KT_LAUNCH("hyc_uvedge", st, hyc_uvedge<LPC><<<grid, blk, 0, st>>>(g, s, ed, leapm, leapn, s.un, s.vn, 0));
+  KT_LAUNCH("hyc_uvedge", st,
+            hyc_uvedge<LPC><<<grid, blk, 0, st>>>(g, s, ed, leapm, leapm, s.um, s.vm, 1, ed.uvsq_edg));
+  KT_LAUNCH("hyc_uvedge", st, hyc_uvedge<LPC><<<grid, blk, 0, st>>>(g, s, ed, leapm, leapn, s.un, s.vn, 0, nullptr));
   KT_LAUNCH("hyc_edgint", st, hyc_edgint<LPC><<<grid, blk, 0, st>>>(g, s, ed, leapm));
 }
 

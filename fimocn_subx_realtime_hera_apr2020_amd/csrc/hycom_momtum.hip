@@ -138,24 +138,35 @@ __global__ void __launch_bounds__(256) hyc_del4(HycGrid g, HycEdg ed, HycMom m, 
   const size_t cz = (size_t)nip * kdm;
   const double *un = ed.uclin + (size_t)(leapn - 1) * cz, *vn = ed.vclin + (size_t)(leapn - 1) * cz;
   const double ui = HC(un, i, k), vi = HC(vn, i, k);
+  // every slot's operands are loaded before any is used (a pentagon's sixth
+  // slot and a land neighbour are valid cells), so the loads overlap
+  double ux[NPP], vx[NPP], c0[NPP], s0[NPP], c1[NPP], s1[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    ux[e] = HC(un, nb[e], k);
+    vx[e] = HC(vn, nb[e], k);
+    c0[e] = HCS(i, e, 0);
+    s0[e] = HSN(i, e, 0);
+    c1[e] = HCS(i, e, 1);
+    s1[e] = HSN(i, e, 1);
+  }
   double avgu = 0., avgv = 0.;
 #pragma unroll
   for (int e = 0; e < NPP; e++) {
-    if (e >= np) break;
-    const int ix = nb[e];
-    const double ux = HC(un, ix, k), vx = HC(vn, ix, k);
     double ungbor, vngbor;
     if (ci_open(ci, e)) {
-      const double uedge = HCS(i, e, 1) * ux + HSN(i, e, 1) * vx;
-      const double vedge = -HSN(i, e, 1) * ux + HCS(i, e, 1) * vx;
-      ungbor = HCS(i, e, 0) * uedge - HSN(i, e, 0) * vedge;
-      vngbor = HSN(i, e, 0) * uedge + HCS(i, e, 0) * vedge;
+      const double uedge = c1[e] * ux[e] + s1[e] * vx[e];
+      const double vedge = -s1[e] * ux[e] + c1[e] * vx[e];
+      ungbor = c0[e] * uedge - s0[e] * vedge;
+      vngbor = s0[e] * uedge + c0[e] * vedge;
     } else {
       ungbor = H_SLIP * ui;
       vngbor = H_SLIP * vi;
     }
-    avgu = avgu + ungbor;
-    avgv = avgv + vngbor;
+    if (e < np) {
+      avgu = avgu + ungbor;
+      avgv = avgv + vngbor;
+    }
   }
   const double factor = 1. / np;
   HC(m.d4u, i, k) = ui - avgu * factor;

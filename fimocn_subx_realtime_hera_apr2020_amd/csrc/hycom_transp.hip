@@ -249,11 +249,19 @@ __global__ void __launch_bounds__(256) hyc_fct_lo(HycGrid g, HycTr tr, const dou
   const int np = ci_np(ci);
   const double f = TF(tr.fld, t, i, k);
   double fmx = f, fmn = f, hordiv = 0.;
+  // every slot's operands are loaded before any is used (a pentagon's sixth
+  // slot repeats its fifth), so the loads overlap
+  double uea[NPP], uxa[NPP], fxa[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    uea[e] = HE(tr.cumuflx, i, e, k);
+    uxa[e] = HE(tr.cumuflx, nb[e], g.proxs[i * NPP + e], k);
+    fxa[e] = TF(tr.fld, t, nb[e], k);
+  }
 #pragma unroll
   for (int e = 0; e < NPP; e++) {
     if (e >= np) break;
-    const int ix = nb[e], edx = g.proxs[i * NPP + e];
-    const double ue = HE(tr.cumuflx, i, e, k), ux = HE(tr.cumuflx, ix, edx, k), fx = TF(tr.fld, t, ix, k);
+    const double ue = uea[e], ux = uxa[e], fx = fxa[e];
     double flx = 0., uan = 0.;
     if (ci_open(ci, e)) {
       flx = 0.5 * ((ue + fabs(ue)) * f - (ux + fabs(ux)) * fx);
@@ -339,12 +347,17 @@ __global__ void __launch_bounds__(256) hyc_fct_clip(HycGrid g, HycTr tr, const d
   const int np = ci_np(ci);
   const double fln = TF(tr.fln, t, i, kk), flp = TF(tr.flp, t, i, kk);
   double hordiv = 0.;
+  double ua[NPP], flpa[NPP], flna[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    ua[e] = tr.uan[((size_t)t * nip * NPP + (size_t)i * NPP + e) * kdm + kk];
+    flpa[e] = TF(tr.flp, t, nb[e], kk);
+    flna[e] = TF(tr.fln, t, nb[e], kk);
+  }
 #pragma unroll
   for (int e = 0; e < NPP; e++) {
     if (e >= np) break;
-    const int ix = nb[e];
-    const double u = tr.uan[((size_t)t * nip * NPP + (size_t)i * NPP + e) * kdm + kk];
-    const double flpx = TF(tr.flp, t, ix, kk), flnx = TF(tr.fln, t, ix, kk);
+    const double u = ua[e], flpx = flpa[e], flnx = flna[e];
     if (ci_open(ci, e)) {
       const double clip = u >= 0. ? hmin(hmin(1., fln), flpx) : hmin(hmin(1., flp), flnx);
       hordiv = hordiv + u * clip;
