@@ -155,30 +155,44 @@ __global__ void __launch_bounds__(256) hyc_uvedge(HycGrid g, HycState s, HycEdg 
   const bool wi = ci_wet(ci);
   const size_t cz = (size_t)nip * kdm;
   const double *dpm = s.dp + (size_t)(leapm - 1) * cz;
+  const double *uc = ed.uclin + (size_t)(slot - 1) * cz, *vc = ed.vclin + (size_t)(slot - 1) * cz;
+  // every operand is loaded before the column sum and the LDS staging, so
+  // that the loads' latencies overlap each other and the sum: the cell's
+  // rotation coefficients (one or two per lane), and the total velocities
+  // (baroclinic + barotropic; land: -1e33) of the cell and its neighbours at
+  // leapfrog level 'slot' (a land cell's and a pentagon's sixth slot's
+  // values are valid memory, replaced or unused)
+  constexpr int NROT = 2 * NPP * 4, NRL = (NROT + LPC - 1) / LPC;
+  double rv[NRL];
+#pragma unroll
+  for (int q = 0; q < NRL; q++) {
+    const int j = k + q * LPC;
+    rv[q] = j < NPP * 4 ? g.cs[(size_t)i * NPP * 4 + j] : j < NROT ? g.sn[(size_t)i * NPP * 4 + j - NPP * 4] : 0.;
+  }
+  const double uo = HC(uc, i, kk) + ed.utrop[i * 3 + slot - 1];
+  const double vo = HC(vc, i, kk) + ed.vtrop[i * 3 + slot - 1];
+  double ux[NPP], vx[NPP];
+#pragma unroll
+  for (int e = 0; e < NPP; e++) {
+    const int x = nb[e];
+    const bool wx = ci_open(ci, e);
+    const double a = HC(uc, x, kk) + ed.utrop[x * 3 + slot - 1], b = HC(vc, x, kk) + ed.vtrop[x * 3 + slot - 1];
+    ux[e] = wx ? a : H_LAND;
+    vx[e] = wx ? b : H_LAND;
+  }
   // the interface pressure of the mid-time thickness (273-291)
   if (with_pres && wi) {
     const double p = col_pres(sh[c.cc], live ? HC(dpm, i, k) : 0., HP(i, 0), k, kdm);
     if (live) HP(i, k + 1) = p;
   }
   // the column's lanes stage the cell's rotation coefficients in LDS
-  for (int j = k; j < 2 * NPP * 4; j += LPC)
-    rot[c.cc][j] = j < NPP * 4 ? g.cs[(size_t)i * NPP * 4 + j] : g.sn[(size_t)i * NPP * 4 + j - NPP * 4];
-  wave_sync();
-  // total velocities (baroclinic + barotropic; land: -1e33) of the cell and
-  // its neighbours at leapfrog level 'slot'
-  {
-    const double *uc = ed.uclin + (size_t)(slot - 1) * cz, *vc = ed.vclin + (size_t)(slot - 1) * cz;
-    const double ui = wi ? HC(uc, i, kk) + ed.utrop[i * 3 + slot - 1] : H_LAND;
-    const double vi = wi ? HC(vc, i, kk) + ed.vtrop[i * 3 + slot - 1] : H_LAND;
-    double ux[NPP], vx[NPP];
 #pragma unroll
-    for (int e = 0; e < NPP; e++) {
-      const int x = nb[e];
-      const bool wx = ci_open(ci, e);
-      const double a = HC(uc, x, kk) + ed.utrop[x * 3 + slot - 1], b = HC(vc, x, kk) + ed.vtrop[x * 3 + slot - 1];
-      ux[e] = wx ? a : H_LAND;
-      vx[e] = wx ? b : H_LAND;
-    }
+  for (int q = 0; q < NRL; q++)
+    if (k + q * LPC < NROT) rot[c.cc][k + q * LPC] = rv[q];
+  wave_sync();
+  {
+    const double ui = wi ? uo : H_LAND;
+    const double vi = wi ? vo : H_LAND;
 #pragma unroll
     for (int e = 0; e < NPP; e++) {
       UV r{0., 0.};
