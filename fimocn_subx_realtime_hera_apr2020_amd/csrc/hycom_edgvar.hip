@@ -208,11 +208,11 @@ __global__ void __launch_bounds__(256) hyc_uvedge(HycGrid g, HycState s, HycEdg 
                     at_prev(ux, e, np), at_prev(vx, e, np), at_next(ux, e, np), at_next(vx, e, np));
       }
       if (live) {
-        HE(ue, i, e, k) = r.u;
-        HE(ve, i, e, k) = r.v;
+        __builtin_nontemporal_store(r.u, &HE(ue, i, e, k));
+        __builtin_nontemporal_store(r.v, &HE(ve, i, e, k));
         // the kinetic energy on the edges, from the values just stored (0 on
         // land and in a pentagon's sixth slot)
-        if (uvsq) HE(uvsq, i, e, k) = wi && e < np ? .5 * (r.u * r.u + r.v * r.v) : 0.;
+        if (uvsq) __builtin_nontemporal_store(wi && e < np ? .5 * (r.u * r.u + r.v * r.v) : 0., &HE(uvsq, i, e, k));
       }
     }
   }
@@ -235,8 +235,12 @@ __global__ void __launch_bounds__(256) hyc_edgint(HycGrid g, HycState s, HycEdg 
   if (!ci_wet(ci)) {
 #pragma unroll
     for (int e = 0; e < NPP; e++) {
-      if (live) HE(s.dpm_edg, i, e, k) = HE(ed.mont_edg, i, e, k) = HE(ed.spcv_edg, i, e, k) = 0.;
-      for (int j = k; j <= kdm; j += LPC) HPR(i, e, j) = 0.;
+      if (live) {
+        __builtin_nontemporal_store(0., &HE(s.dpm_edg, i, e, k));
+        __builtin_nontemporal_store(0., &HE(ed.mont_edg, i, e, k));
+        __builtin_nontemporal_store(0., &HE(ed.spcv_edg, i, e, k));
+      }
+      for (int j = k; j <= kdm; j += LPC) __builtin_nontemporal_store(0., &HPR(i, e, j));
     }
     return;
   }
@@ -294,12 +298,16 @@ __global__ void __launch_bounds__(256) hyc_edgint(HycGrid g, HycState s, HycEdg 
 #pragma unroll
   for (int e = 0; e < NPP; e++) {
     if (e >= np) {
-      if (live) HE(s.dpm_edg, i, e, k) = HE(ed.mont_edg, i, e, k) = HE(ed.spcv_edg, i, e, k) = 0.;
-      for (int j = k; j <= kdm; j += LPC) HPR(i, e, j) = 0.;
+      if (live) {
+        __builtin_nontemporal_store(0., &HE(s.dpm_edg, i, e, k));
+        __builtin_nontemporal_store(0., &HE(ed.mont_edg, i, e, k));
+        __builtin_nontemporal_store(0., &HE(ed.spcv_edg, i, e, k));
+      }
+      for (int j = k; j <= kdm; j += LPC) __builtin_nontemporal_store(0., &HPR(i, e, j));
       continue;
     }
     const double *hrow = sh_h[cc][e];
-    for (int j = k; j <= kdm; j += LPC) HPR(i, e, j) = hrow[j];
+    for (int j = k; j <= kdm; j += LPC) __builtin_nontemporal_store(hrow[j], &HPR(i, e, j));
     const int em1 = e_prev(e, np), ep1 = e_next(e, np);
     double blok = ci_open(ci, e) ? blokf(di, pik1, PB[e]) : 1.;
     const double nm1 = M[e] * (1. - blok) + mi * blok, nv1 = SV[e] * (1. - blok) + svi * blok;
@@ -308,9 +316,9 @@ __global__ void __launch_bounds__(256) hyc_edgint(HycGrid g, HycState s, HycEdg 
     blok = ci_open(ci, ep1) ? blokf(di, pik1, at_next(PB, e, np)) : 1.;
     const double nm3 = at_next(M, e, np) * (1. - blok) + mi * blok, nv3 = at_next(SV, e, np) * (1. - blok) + svi * blok;
     if (live) {
-      HE(s.dpm_edg, i, e, k) = hmax(0., hrow[k + 1] - hrow[k]);
-      HE(ed.mont_edg, i, e, k) = (8. * (mi + nm1) + nm2 + nm3) * H_DIVBY18;
-      HE(ed.spcv_edg, i, e, k) = (8. * (svi + nv1) + nv2 + nv3) * H_DIVBY18;
+      __builtin_nontemporal_store(hmax(0., hrow[k + 1] - hrow[k]), &HE(s.dpm_edg, i, e, k));
+      __builtin_nontemporal_store((8. * (mi + nm1) + nm2 + nm3) * H_DIVBY18, &HE(ed.mont_edg, i, e, k));
+      __builtin_nontemporal_store((8. * (svi + nv1) + nv2 + nv3) * H_DIVBY18, &HE(ed.spcv_edg, i, e, k));
     }
   }
 }

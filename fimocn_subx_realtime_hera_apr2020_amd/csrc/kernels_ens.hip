@@ -241,8 +241,8 @@ __device__ __forceinline__ void e_edgvar3_col(const EnsGrid &g, const EnsState &
       out[i] = b0 + acc;
     }
 #pragma unroll
-    for (int i = 0; i < NP; i++) ETE(s.trc_edg, f - 1, col, i, k) = out[i];
-    if (NP < NPP) ETE(s.trc_edg, f - 1, col, NPP - 1, k) = 0.f;
+    for (int i = 0; i < NP; i++) __builtin_nontemporal_store(out[i], &ETE(s.trc_edg, f - 1, col, i, k));
+    if (NP < NPP) __builtin_nontemporal_store(0.f, &ETE(s.trc_edg, f - 1, col, NPP - 1, k));
   }
 }
 
@@ -626,20 +626,20 @@ __device__ __forceinline__ void e_trcadv1_col(const EnsGrid &g, const EnsState &
     for (int e = 0; e < NP; e++) {
       const float fl = .5f * ((m[e] + fabsf(m[e])) * ti - (fabsf(m[e]) - m[e]) * v[e]);
       const float a = m[e] * te[e] - fl;
-      ETE(s.antiflx, t, col, e, k) = a;
+      __builtin_nontemporal_store(a, &ETE(s.antiflx, t, col, e, k));
       sp = sp - fminf(0.f, a);
       sm = sm + fmaxf(0.f, a);
       acc = acc + fl;
     }
-    if (NP < NPP) ETE(s.antiflx, t, col, NPP - 1, k) = 0.f;
+    if (NP < NPP) __builtin_nontemporal_store(0.f, &ETE(s.antiflx, t, col, NPP - 1, k));
     const float tlo = -acc * rar;
-    ETAB(s.trl_tdcy, t, nf, col, k) = tlo;
+    __builtin_nontemporal_store(tlo, &ETAB(s.trl_tdcy, t, nf, col, k));
     const float lo = tdp0 + a1 * tlo + a2 * tlo_of + a3 * tlo_vof;
     mx = fmaxf(0.f, mx);
     mn = fmaxf(0.f, mn);
-    ETR(s.trcr_lo, t, col, k) = fmaxf(mn, fminf(mx, lo * rdp));
-    ETR(s.tr_rp, t, col, k) = sp;
-    ETR(s.tr_rm, t, col, k) = sm;
+    __builtin_nontemporal_store(fmaxf(mn, fminf(mx, lo * rdp)), &ETR(s.trcr_lo, t, col, k));
+    __builtin_nontemporal_store(sp, &ETR(s.tr_rp, t, col, k));
+    __builtin_nontemporal_store(sm, &ETR(s.tr_rm, t, col, k));
   }
 }
 
