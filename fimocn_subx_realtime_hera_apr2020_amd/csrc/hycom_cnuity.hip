@@ -100,10 +100,10 @@ __global__ void __launch_bounds__(256) hyc_flux(HycGrid g, HycState s, int leapm
     if (open && i < ix[e]) {
       const EdgeFlux fa = edge_flux(g, s, dpm, dpn, kdm, i, e, ix[e], ex[e], k, dtleap);
       const EdgeFlux fb = edge_flux(g, s, dpm, dpn, kdm, ix[e], ex[e], i, e, k, dtleap);
-      HE(s.antiflx, i, e, k) = fa.anti;
-      HE(s.massflx, i, e, k) = fa.low;
-      HE(s.antiflx, ix[e], ex[e], k) = fb.anti;
-      HE(s.massflx, ix[e], ex[e], k) = fb.low;
+      __builtin_nontemporal_store(fa.anti, &HE(s.antiflx, i, e, k));
+      __builtin_nontemporal_store(fa.low, &HE(s.massflx, i, e, k));
+      __builtin_nontemporal_store(fb.anti, &HE(s.antiflx, ix[e], ex[e], k));
+      __builtin_nontemporal_store(fb.low, &HE(s.massflx, ix[e], ex[e], k));
     } else if (!open) {      // an edge to land carries no flux; a pentagon's sixth slot is land-valued
       HE(s.antiflx, i, e, k) = HE(s.massflx, i, e, k) = e < np ? 0. : H_LAND;
     }
@@ -124,9 +124,9 @@ __global__ void __launch_bounds__(256) hyc_lowsum(HycGrid g, HycState s, int lea
   const int ci = g.cinfo[i];
   if (!ci_wet(ci)) {
     if (live) {
-      HC(s.dp_lo, i, k) = 0.;
-      HC(s.flxplus, i, k) = H_EPSIL;
-      HC(s.flxmnus, i, k) = -H_EPSIL;
+      __builtin_nontemporal_store(0., &HC(s.dp_lo, i, k));
+      __builtin_nontemporal_store(H_EPSIL, &HC(s.flxplus, i, k));
+      __builtin_nontemporal_store(-H_EPSIL, &HC(s.flxmnus, i, k));
     }
     return;
   }
@@ -143,13 +143,13 @@ __global__ void __launch_bounds__(256) hyc_lowsum(HycGrid g, HycState s, int lea
 #pragma unroll
   for (int e = 0; e < NPP; e++)
     if (e < np) flx = flx + mf[e];
-  if (live) HC(s.dpold, i, k) = d;
+  if (live) __builtin_nontemporal_store(d, &HC(s.dpold, i, k));
   const double p = col_pres(sh[c.cc], live ? d : 0., HP(i, 0), k, kdm);
   if (!live) return;
-  HP(i, k + 1) = p;
+  __builtin_nontemporal_store(p, &HP(i, k + 1));
   double lo = d - flx * g.rarea[i] * dtleap;
   if (lo < 0. && lo + H_ONECM > 0.) lo = 0.;
-  HC(s.dp_lo, i, k) = lo;
+  __builtin_nontemporal_store(lo, &HC(s.dp_lo, i, k));
   double fp = H_EPSIL, fm = -H_EPSIL;
 #pragma unroll
   for (int e = 0; e < NPP; e++) {
@@ -157,8 +157,8 @@ __global__ void __launch_bounds__(256) hyc_lowsum(HycGrid g, HycState s, int lea
     if (af[e] >= 0.) fp = fp + af[e];
     else fm = fm + af[e];
   }
-  HC(s.flxplus, i, k) = fp;
-  HC(s.flxmnus, i, k) = fm;
+  __builtin_nontemporal_store(fp, &HC(s.flxplus, i, k));
+  __builtin_nontemporal_store(fm, &HC(s.flxmnus, i, k));
 }
 
 template <int LPC>
@@ -188,8 +188,8 @@ __global__ void __launch_bounds__(256) hyc_limits(HycGrid g, HycState s, int lea
     dpmin = open ? nmin : dpmin;
   }
   dpmin = hmax(0., dpmin);
-  HC(s.rmnus, i, k) = (lo - dpmin) * g.area[i] / (dtleap * HC(s.flxplus, i, k));
-  HC(s.rplus, i, k) = (lo - dpmax) * g.area[i] / (dtleap * HC(s.flxmnus, i, k));
+  __builtin_nontemporal_store((lo - dpmin) * g.area[i] / (dtleap * HC(s.flxplus, i, k)), &HC(s.rmnus, i, k));
+  __builtin_nontemporal_store((lo - dpmax) * g.area[i] / (dtleap * HC(s.flxmnus, i, k)), &HC(s.rplus, i, k));
 }
 
 template <int LPC>
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(256) hyc_clip(HycGrid g, HycState s, double dt
     const bool open = ci_open(ci, e);
     const double ac = a * clip;
     part[e] = open && live ? a * (1. - clip) : 0.;
-    if (open && live) HE(s.massflx, i, e, k) = mfv + ac;
+    if (open && live) __builtin_nontemporal_store(mfv + ac, &HE(s.massflx, i, e, k));
     if (e < np) flx = flx + (open ? ac : a);    // a closed edge's antidiffusive flux is 0
   }
   const double cl = col_sums6<LPC>(sh[c.cc], part, k, kdm);
@@ -230,10 +230,10 @@ __global__ void __launch_bounds__(256) hyc_clip(HycGrid g, HycState s, double dt
   if (live) {
     d = HC(s.dp_lo, i, k) - flx * g.rarea[i] * dtleap;
     if (d < 0. && d + H_ONECM > 0.) d = 0.;
-    HC(s.dpc, i, k) = d;
+    __builtin_nontemporal_store(d, &HC(s.dpc, i, k));
   }
   const double p = col_pres(sh[c.cc][0], d, HP(i, 0), k, kdm);
-  if (live) HP(i, k + 1) = p;
+  if (live) __builtin_nontemporal_store(p, &HP(i, k + 1));
   if (k == kdm - 1) s.rpbot[i] = 1. / p;
 }
 
@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(256) hyc_recover(HycGrid g, HycState s, int le
 #pragma unroll
   for (int e = 0; e < NPP; e++) nb[e] = g.prox[i * NPP + e];
   if (!ci_wet(ci)) {
-    if (smooth && live) HC(s.del2p, i, k) = biharm == 0. && k >= 1 ? HP(i, k) : H_LAND;
+    if (smooth && live) __builtin_nontemporal_store(biharm == 0. && k >= 1 ? HP(i, k) : H_LAND, &HC(s.del2p, i, k));
     return;
   }
   double *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
@@ -269,7 +269,7 @@ __global__ void __launch_bounds__(256) hyc_recover(HycGrid g, HycState s, int le
     const double q = cl > 0. ? cl * (dci * rpi) : (cl < 0. ? cl * (dcx * rpx) : 0.);
     if (ci_open(ci, e)) {
       flx = flx + q;
-      if (live) HE(s.massflx, i, e, k) = mfv + q;
+      if (live) __builtin_nontemporal_store(mfv + q, &HE(s.massflx, i, e, k));
     }
   }
   double d = 0.;
@@ -279,16 +279,16 @@ __global__ void __launch_bounds__(256) hyc_recover(HycGrid g, HycState s, int le
   }
   const double p0 = HP(i, 0);
   const double p = col_pres(sh[c.cc], d, p0, k, kdm);
-  if (live) HP(i, k + 1) = p;
+  if (live) __builtin_nontemporal_store(p, &HP(i, k + 1));
   if (smooth) {
     const double pa = __shfl_up(p, 1, LPC);
     const double pk = k > 0 ? pa : p0;         // the interface above layer k
     if (live) {
-      HC(dpn, i, k) = p - pk;
-      HC(s.del2p, i, k) = biharm == 0. && k >= 1 ? pk : H_LAND;
+      __builtin_nontemporal_store(p - pk, &HC(dpn, i, k));
+      __builtin_nontemporal_store(biharm == 0. && k >= 1 ? pk : H_LAND, &HC(s.del2p, i, k));
     }
   } else if (live) {
-    HC(dpn, i, k) = d;
+    __builtin_nontemporal_store(d, &HC(dpn, i, k));
   }
 }
 
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(256) hyc_del2p(HycGrid g, HycState s, double b
     sum = ci_open(ci, e) ? sum + deltap : sum;
   }
   const int ncount = __builtin_popcount((ci >> 4) & 63);
-  HC(s.del2p, i, k) = biharm * sum / (double)ncount + (1. - biharm) * pi;
+  __builtin_nontemporal_store(biharm * sum / (double)ncount + (1. - biharm) * pi, &HC(s.del2p, i, k));
 }
 
 // bolus fluxes (605-646; the antidiffusive-flux array holds them) and, for
@@ -357,12 +357,12 @@ __global__ void __launch_bounds__(256) hyc_bolus(HycGrid g, HycState s, int leap
     const double bn = __shfl_down(b, 1, LPC);
     double f = k < kdm - 1 ? bn - b : -b;
     f = ci_open(ci, e) ? f : 0.;
-    if (live) HE(s.antiflx, i, e, k) = f;
+    if (live) __builtin_nontemporal_store(f, &HE(s.antiflx, i, e, k));
     if (biharm > 0. && e < np) fp = fp + hmax(0., f);
   }
   if (biharm > 0. && live) {
     const double *dpn = s.dp + (size_t)(leapn - 1) * nip * kdm;
-    HC(s.rmnus, i, k) = HC(dpn, i, k) * ai / fp;
+    __builtin_nontemporal_store(HC(dpn, i, k) * ai / fp, &HC(s.rmnus, i, k));
   }
 }
 
@@ -411,11 +411,11 @@ __global__ void __launch_bounds__(256) hyc_bclip(HycGrid g, HycState s, int leap
     d = HC(dpn, i, k) - flx * g.rarea[i];
     if (d < 0. && d + H_ONECM > 0.) d = 0.;
     d = d * (1. + masscor);
-    HC(dpn, i, k) = d;
-    HC(s.dptmp, i, k) = d;
+    __builtin_nontemporal_store(d, &HC(dpn, i, k));
+    __builtin_nontemporal_store(d, &HC(s.dptmp, i, k));
   }
   const double p = col_pres(sh[c.cc][0], d, HP(i, 0), k, kdm);
-  if (live) HP(i, k + 1) = p;
+  if (live) __builtin_nontemporal_store(p, &HP(i, k + 1));
   if (k == kdm - 1) s.rpbot[i] = 1. / p;
 }
 
@@ -455,20 +455,20 @@ __global__ void __launch_bounds__(256) hyc_final(HycGrid g, HycState s, int leap
       if (e < np) {
         flx = flx + q;
         mf[e] = mf[e] + q * dtinv;
-        if (live) HE(s.massflx, i, e, k) = mf[e];
+        if (live) __builtin_nontemporal_store(mf[e], &HE(s.massflx, i, e, k));
       }
     }
     dn = dn - flx * g.rarea[i];
-    if (live) HC(dpn, i, k) = dn;
+    if (live) __builtin_nontemporal_store(dn, &HC(dpn, i, k));
     const double p = col_pres(sh[c.cc], live ? dn : 0., HP(i, 0), k, kdm);
-    if (live) HP(i, k + 1) = p;
+    if (live) __builtin_nontemporal_store(p, &HP(i, k + 1));
   }
   if (!live) return;
   if (smooth)
 #pragma unroll
     for (int e = 0; e < NPP; e++)
-      if (e < np) HE(s.mssfx_avg, i, e, k) = HE(s.mssfx_avg, i, e, k) + mf[e];
-  HC(dpm, i, k) = H_ASSLDP * (HC(s.dpold, i, k) + dn) + (1. - H_ASSLDP - H_ASSLDP) * HC(dpm, i, k);
+      if (e < np) __builtin_nontemporal_store(HE(s.mssfx_avg, i, e, k) + mf[e], &HE(s.mssfx_avg, i, e, k));
+  __builtin_nontemporal_store(H_ASSLDP * (HC(s.dpold, i, k) + dn) + (1. - H_ASSLDP - H_ASSLDP) * HC(dpm, i, k), &HC(dpm, i, k));
   if (k == kdm - 1) s.aux[i] = HP(i, kdm);
 }
 

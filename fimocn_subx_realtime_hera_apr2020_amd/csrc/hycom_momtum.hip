@@ -118,9 +118,9 @@ __global__ void __launch_bounds__(256) hyc_mom_pgf(HycGrid g, HycState s, HycEdg
     }
   }
   if (!live) return;
-  HP(i, k + 1) = pk1;
-  HC(m.gradx, i, k) = pgfx * g.rarea[i];
-  HC(m.grady, i, k) = pgfy * g.rarea[i];
+  __builtin_nontemporal_store(pk1, &HP(i, k + 1));
+  __builtin_nontemporal_store(pgfx * g.rarea[i], &HC(m.gradx, i, k));
+  __builtin_nontemporal_store(pgfy * g.rarea[i], &HC(m.grady, i, k));
 }
 
 template <int LPC>
@@ -169,8 +169,8 @@ __global__ void __launch_bounds__(256) hyc_del4(HycGrid g, HycEdg ed, HycMom m, 
     }
   }
   const double factor = 1. / np;
-  HC(m.d4u, i, k) = ui - avgu * factor;
-  HC(m.d4v, i, k) = vi - avgv * factor;
+  __builtin_nontemporal_store(ui - avgu * factor, &HC(m.d4u, i, k));
+  __builtin_nontemporal_store(vi - avgv * factor, &HC(m.d4v, i, k));
 }
 
 template <int LPC>
@@ -224,8 +224,8 @@ __global__ void __launch_bounds__(256) hyc_dissip(HycGrid g, HycState s, HycMom 
     du = du + uflux * factor;
     dv = dv + vflux * factor;
   }
-  HC(m.du, i, k) = du;
-  HC(m.dv, i, k) = dv;
+  __builtin_nontemporal_store(du, &HC(m.du, i, k));
+  __builtin_nontemporal_store(dv, &HC(m.dv, i, k));
 }
 
 template <int LPC>
@@ -306,10 +306,10 @@ __global__ void __launch_bounds__(256) hyc_mom_upd(HycGrid g, HycState s, HycEdg
   wave_sync();
   if (!live) return;
   const double uf = sx[cc][0][k], vf = sx[cc][1][k];
-  HC(ucn, i, k) = uf;
-  HC(vcn, i, k) = vf;
-  HC(ucm, i, k) = H_ASSLUV * (uold + uf) + (1. - H_ASSLUV - H_ASSLUV) * HC(ucm, i, k);
-  HC(vcm, i, k) = H_ASSLUV * (vold + vf) + (1. - H_ASSLUV - H_ASSLUV) * HC(vcm, i, k);
+  __builtin_nontemporal_store(uf, &HC(ucn, i, k));
+  __builtin_nontemporal_store(vf, &HC(vcn, i, k));
+  __builtin_nontemporal_store(H_ASSLUV * (uold + uf) + (1. - H_ASSLUV - H_ASSLUV) * HC(ucm, i, k), &HC(ucm, i, k));
+  __builtin_nontemporal_store(H_ASSLUV * (vold + vf) + (1. - H_ASSLUV - H_ASSLUV) * HC(vcm, i, k), &HC(vcm, i, k));
 }
 
 }  // namespace

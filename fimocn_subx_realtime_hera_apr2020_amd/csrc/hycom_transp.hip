@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) hyc_tr_init(HycGrid g, HycState s, HycTr 
   const bool wi = ci_wet(g.cinfo[i]);
 #pragma unroll
   for (int e = 0; e < NPP; e++) HE(tr.cumuflx, i, e, k) = wi ? 0. : H_LAND;
-  if (wi) HC(tr.dpinit, i, k) = HC(s.dp + (size_t)(leapn - 1) * nip * kdm, i, k);
+  if (wi) __builtin_nontemporal_store(HC(s.dp + (size_t)(leapn - 1) * nip * kdm, i, k), &HC(tr.dpinit, i, k));
 }
 
 // transp1: cumuflx += massflx * 2 * batrop
@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(256) hyc_tr_accum(HycGrid g, HycTr tr, const d
   const int np = ci_np(ci);
 #pragma unroll
   for (int e = 0; e < NPP; e++)
-    if (e < np) HE(tr.cumuflx, i, e, k) = HE(tr.cumuflx, i, e, k) + HE(massflx, i, e, k) * 2. * batrop;
+    if (e < np) __builtin_nontemporal_store(HE(tr.cumuflx, i, e, k) + HE(massflx, i, e, k) * 2. * batrop, &HE(tr.cumuflx, i, e, k));
 }
 
 template <int LPC>
@@ -104,10 +104,10 @@ __global__ void __launch_bounds__(256) hyc_tr_w(HycGrid g, HycTr tr, const doubl
   if (live)
     for (int t = 0; t < tr.nfld; t++) {
       const double *src = t == 0 ? tr.temp : t == 1 ? tr.saln : tr.passv + (size_t)(t - 2) * nip * kdm;
-      TF(tr.fld, t, i, k) = wi ? (t == 0 ? HC(src, i, k) + H_TOFFSET : HC(src, i, k)) : H_LAND;
+      __builtin_nontemporal_store(wi ? (t == 0 ? HC(src, i, k) + H_TOFFSET : HC(src, i, k)) : H_LAND, &TF(tr.fld, t, i, k));
     }
   if (!wi) {
-    if (live) HC(tr.w, i, k) = H_LAND;
+    if (live) __builtin_nontemporal_store(H_LAND, &HC(tr.w, i, k));
     return;
   }
   const int np = ci_np(ci);
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) hyc_tr_w(HycGrid g, HycTr tr, const doubl
     if (e < np) hordiv = hordiv + HE(tr.cumuflx, i, e, kk);
   const double vertdv = (HC(tr.dpinit, i, kk) - HC(dpfinl, i, kk)) - hordiv * g.rarea[i];
   const double w = col_scan0(sh[c.cc], live ? vertdv : 0., k, kdm);
-  if (live) HC(tr.w, i, k) = w;
+  if (live) __builtin_nontemporal_store(w, &HC(tr.w, i, k));
 }
 
 template <int LPC>
@@ -221,8 +221,8 @@ __global__ void __launch_bounds__(256) hyc_fct_vert(HycGrid g, HycTr tr) {
   const double vfx = k < kdm - 1 ? wk * amount / slab : 0.;
   const double vfu = __shfl_up(vfx, 1, LPC);
   if (live) {
-    TF(tr.vfx, t, i, k) = vfx;
-    TF(tr.vdv, t, i, k) = k == 0 ? vfx : vfx - vfu;
+    __builtin_nontemporal_store(vfx, &TF(tr.vfx, t, i, k));
+    __builtin_nontemporal_store(k == 0 ? vfx : vfx - vfu, &TF(tr.vdv, t, i, k));
   }
   // the conservation sums before the update: this column's parts
   const double area = g.area[i];
@@ -291,9 +291,9 @@ __global__ void __launch_bounds__(256) hyc_fct_lo(HycGrid g, HycTr tr, const dou
   const double tn = HC(thkn, i, k);
   const double unbound = f * HC(tr.dpinit, i, k) - hordiv - TF(tr.vdv, t, i, k);
   const double bounded = hmax(hmax(0., fmn * tn), hmin(unbound, fmx * tn));
-  TF(tr.flo, t, i, k) = (f * H_ONEMU_F + bounded) / (H_ONEMU_F + tn);
-  TF(tr.fmx, t, i, k) = fmx;
-  TF(tr.fmn, t, i, k) = fmn;
+  __builtin_nontemporal_store((f * H_ONEMU_F + bounded) / (H_ONEMU_F + tn), &TF(tr.flo, t, i, k));
+  __builtin_nontemporal_store(fmx, &TF(tr.fmx, t, i, k));
+  __builtin_nontemporal_store(fmn, &TF(tr.fmn, t, i, k));
 }
 
 template <int LPC>
@@ -324,10 +324,10 @@ __global__ void __launch_bounds__(256) hyc_fct_ratio(HycGrid g, HycTr tr, const 
     }
   }
   const double tn = HC(thkn, i, k), ra = g.rarea[i];
-  TF(tr.flp, t, i, k) = (fmx - f) * tn / (totin + 1.e-11) * ra;
-  TF(tr.fln, t, i, k) = (f - fmn) * tn / (totou + 1.e-11) * ra;
-  TF(tr.fmx, t, i, k) = fmx;
-  TF(tr.fmn, t, i, k) = fmn;
+  __builtin_nontemporal_store((fmx - f) * tn / (totin + 1.e-11) * ra, &TF(tr.flp, t, i, k));
+  __builtin_nontemporal_store((f - fmn) * tn / (totou + 1.e-11) * ra, &TF(tr.fln, t, i, k));
+  __builtin_nontemporal_store(fmx, &TF(tr.fmx, t, i, k));
+  __builtin_nontemporal_store(fmn, &TF(tr.fmn, t, i, k));
 }
 
 template <int LPC>
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(256) hyc_fct_clip(HycGrid g, HycTr tr, const d
   const double unbound = f * tn - hordiv;
   const double bounded = hmax(hmax(0., TF(tr.fmn, t, i, kk) * tn), hmin(unbound, TF(tr.fmx, t, i, kk) * tn));
   const double fnew = (f * H_ONEMU_F + bounded) / (H_ONEMU_F + tn);
-  if (live) TF(tr.fld, t, i, k) = fnew;
+  if (live) __builtin_nontemporal_store(fnew, &TF(tr.fld, t, i, k));
   // the conservation sums after the update: this column's parts
   const double area = g.area[i];
   const double col1 = col_sum0(sh[c.cc], live ? tn * area : 0., k, kdm);
@@ -435,7 +435,7 @@ __global__ void __launch_bounds__(256) hyc_fct_fix(HycGrid g, HycTr tr, const do
   const double bfore = sums[2 + 2 * t] / sums[0], after = sums[3 + 2 * t] / sums[1];
   const double f = TF(tr.fld, t, i, k) + (bfore - after);
   double *dst = t == 0 ? tr.temp : t == 1 ? tr.saln : tr.passv + (size_t)(t - 2) * nip * kdm;
-  HC(dst, i, k) = t == 0 ? f - H_TOFFSET : f;
+  __builtin_nontemporal_store(t == 0 ? f - H_TOFFSET : f, &HC(dst, i, k));
 }
 
 // dffusn_lyr: the fluxes through the ocean edges, then the update (two
@@ -462,7 +462,7 @@ __global__ void __launch_bounds__(256) hyc_dff_flux(HycGrid g, HycTr tr, const d
     const double fx = HC(fld, ix, k), b = hmax(HC(delp, ix, k), 1.e-11);
     if (ci_open(ci, e)) flxdv = flxdv + (fi - fx) * g.sideln[i * NPP + e] * 2. * (a * b / (a + b));
   }
-  TF(tr.flxdv, t, i, k) = flxdv;
+  __builtin_nontemporal_store(flxdv, &TF(tr.flxdv, t, i, k));
 }
 
 template <int LPC>
@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(256) hyc_dff_apply(HycGrid g, HycTr tr, const 
   if (!ci_wet(g.cinfo[i])) return;
   double *fld = t == 0 ? tr.temp : tr.saln;
   const double factor = -dffveldt * g.rarea[i] / hmax(HC(delp, i, k), 1.e-11);
-  HC(fld, i, k) = HC(fld, i, k) + TF(tr.flxdv, t, i, k) * factor;
+  __builtin_nontemporal_store(HC(fld, i, k) + TF(tr.flxdv, t, i, k) * factor, &HC(fld, i, k));
 }
 
 }  // namespace

@@ -175,11 +175,11 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
     lp[e] = (float)((pi0 + pi1 + np0[e] + np1[e] + .5 * (np0[et] + np1[et] + np0[el] + np1[el])) * (1. / 6.));
   }
   const float2 pg = e_pres_grad<NP>(g, ipn, lp);
-  EC(s.pgx, col, k) = pg.x;
-  EC(s.pgy, col, k) = pg.y;
+  __builtin_nontemporal_store(pg.x, &EC(s.pgx, col, k));
+  __builtin_nontemporal_store(pg.y, &EC(s.pgy, col, k));
   // theta at the start of the step, for momtum's edge values (the tracer
   // itself is updated by trcadv before momtum runs)
-  EC(s.th0, col, k) = thi;
+  __builtin_nontemporal_store(thi, &EC(s.th0, col, k));
 }
 
 template <int COLS, bool SWZ>
@@ -227,7 +227,7 @@ __device__ __forceinline__ void e_edgvar3_col(const EnsGrid &g, const EnsState &
     }
 #pragma unroll
     for (int i = 0; i < NP; i++) EE(s.dp_edg, col, i, k) = out[i];
-    if (NP < NPP) EE(s.dp_edg, col, NPP - 1, k) = 0.f;
+    if (NP < NPP) __builtin_nontemporal_store(0.f, &EE(s.dp_edg, col, NPP - 1, k));
   }
   for (int f = 1; f <= s.edg3_ntr; f++) {
     const float b0 = ETR(s.tr, f, col, k);
@@ -293,20 +293,20 @@ __device__ __forceinline__ void e_cnuity12_col(const EnsGrid &g, const EnsState 
     const float hi = ipn < x[e] ? vn[e] * dpe : -(vx[e] * dpe);
     const float m = .5f * ((vn[e] + fabsf(vn[e])) * dpf - (vx[e] + fabsf(vx[e])) * dpx[e]);
     const float a = hi - m;
-    EE(s.massfx, col, e, k) = m;
-    EE(s.antifx, col, e, k) = a;
+    __builtin_nontemporal_store(m, &EE(s.massfx, col, e, k));
+    __builtin_nontemporal_store(a, &EE(s.antifx, col, e, k));
     if (a <= 0.f) pp = pp - a; else pm = pm + a;
     acc = acc + (double)m;
   }
-  EC(s.r_plus, col, k) = pp;
-  EC(s.r_mnus, col, k) = pm;
+  __builtin_nontemporal_store(pp, &EC(s.r_plus, col, k));
+  __builtin_nontemporal_store(pm, &EC(s.r_mnus, col, k));
   if (NP < NPP) {
-    EE(s.massfx, col, NPP - 1, k) = 0.f;
-    EE(s.antifx, col, NPP - 1, k) = 0.f;
+    __builtin_nontemporal_store(0.f, &EE(s.massfx, col, NPP - 1, k));
+    __builtin_nontemporal_store(0.f, &EE(s.antifx, col, NPP - 1, k));
   }
   const double dplo = -acc * GD(rarea, ipn);
   EAB(s.dpl_tdcy, nf, col, k) = (float)dplo;
-  EC(s.delp_lo, col, k) = (float)(dpi + c.ab1 * dplo + c.ab2 * lo_of + c.ab3 * lo_vof);
+  __builtin_nontemporal_store((float)(dpi + c.ab1 * dplo + c.ab2 * lo_of + c.ab3 * lo_vof), &EC(s.delp_lo, col, k));
 }
 
 template <int COLS, bool SWZ>
@@ -347,8 +347,8 @@ __global__ void __launch_bounds__(64 * COLS) k_e_cnuity3(EnsGrid g, EnsState s) 
   const double rab1 = 1. / c.ab1;
   const float qp = (float)(((double)mx - lo - (c.ab2 * fmin(0., to) + c.ab3 * fmax(0., tv))) * rab1) * GF(area, ipn);
   const float qm = (float)(((double)lo - mn + (c.ab2 * fmax(0., to) + c.ab3 * fmin(0., tv))) * rab1) * GF(area, ipn);
-  EC(s.r_plus, col, k) = fmaxf(0.f, fminf(1.f, qp * frcp(fmaxf(pp, thshld))));
-  EC(s.r_mnus, col, k) = fmaxf(0.f, fminf(1.f, qm * frcp(fmaxf(pm, thshld))));
+  __builtin_nontemporal_store(fmaxf(0.f, fminf(1.f, qp * frcp(fmaxf(pp, thshld)))), &EC(s.r_plus, col, k));
+  __builtin_nontemporal_store(fmaxf(0.f, fminf(1.f, qm * frcp(fmaxf(pm, thshld)))), &EC(s.r_mnus, col, k));
 }
 
 // cnuity5 (cnuity_compute.F90:376-451), also update_pres of mass_adv
@@ -366,10 +366,10 @@ __device__ __forceinline__ void e_update_pres_col(const EnsGrid &g, const EnsSta
   const float dpdx = EC(s.pgx, col, k), dpdy = EC(s.pgy, col, k);   // formed by edgvar1
   const float ui = EC(s.us, col, k), vi = EC(s.vs, col, k);
   const double p = EL(s.pr, col, nvl) + sp;
-  EL(s.pr, col, k) = p;
-  EL(s.ex, col, k) = CP * fim_pow_fast(p / P1000, RD / CP);
+  __builtin_nontemporal_store(p, &EL(s.pr, col, k));
+  __builtin_nontemporal_store(CP * fim_pow_fast(p / P1000, RD / CP), &EL(s.ex, col, k));
   const float om = (float)(sl - .5 * lt);
-  EC(s.ws, col, k) = om + (ui * dpdx + vi * dpdy) * GF(rarea, ipn);
+  __builtin_nontemporal_store(om + (ui * dpdx + vi * dpdy) * GF(rarea, ipn), &EC(s.ws, col, k));
 }
 
 // cnuity4 + cnuity5 (cnuity_compute.F90:261-451).  The thickness tendency is
@@ -402,14 +402,14 @@ __device__ __forceinline__ void e_cnuity45_col(const EnsGrid &g, const EnsState 
 #pragma unroll
     for (int e = 0; e < NP; e++) {
       const float mf = m[e] + a[e] * fminf(a[e] >= 0.f ? rm : rp, r[e]);
-      EE(s.massfx, col, e, k) = mf;
+      __builtin_nontemporal_store(mf, &EE(s.massfx, col, e, k));
       at = at + (double)mf;
     }
     const double dtn = -at * GD(rarea, ipn);
     EAB(s.dp_tdcy, nf, col, k) = dtn;
     const double inc = c.ab1 * dtn + c.ab2 * t_of + c.ab3 * t_vof;
     dpn = dp0 + inc;
-    EC(s.dp, col, k) = dpn;
+    __builtin_nontemporal_store(dpn, &EC(s.dp, col, k));
     lt = inc / g.d.dt;
   }
   e_update_pres_col<NP>(g, s, ipn, col, k, live, dpn, lt);
@@ -472,7 +472,7 @@ __device__ __forceinline__ void e_massadv1_col(const EnsGrid &g, const EnsState 
     const float nm = dpx[(e + NP - 1) % NP], np1 = dpx[(e + 1) % NP];
     const float lo4 = fminf(fminf(gvf, dpx[e]), fminf(nm, np1)), hi4 = fmaxf(fmaxf(gvf, dpx[e]), fmaxf(nm, np1));
     const float qs = fmaxf(lo4, fminf(hi4, .5f * (de[e] + dx[e])));
-    EE(s.antifx, col, e, k) = qs;
+    __builtin_nontemporal_store(qs, &EE(s.antifx, col, e, k));
     if (vnorm > 0.f) {
       c_out = c_out + (double)cn;
     } else {
@@ -492,8 +492,8 @@ __device__ __forceinline__ void e_massadv1_col(const EnsGrid &g, const EnsState 
     qmo = (float)((rest + a * cq_max - q_max_next * keep) * w);
     qxo = (float)((rest + a * cq_min - q_min_next * keep) * w);
   }
-  EC(s.r_plus, col, k) = qmo;
-  EC(s.r_mnus, col, k) = qxo;
+  __builtin_nontemporal_store(qmo, &EC(s.r_plus, col, k));
+  __builtin_nontemporal_store(qxo, &EC(s.r_mnus, col, k));
 }
 
 template <int COLS, bool SWZ>
@@ -535,15 +535,15 @@ __device__ __forceinline__ void e_massadv2_col(const EnsGrid &g, const EnsState 
 #pragma unroll
     for (int e = 0; e < NP; e++) {
       const float mf = vnorm[e] * fmaxf(lo[e], fminf(hi[e], qs[e]));
-      EE(s.massfx, col, e, k) = mf;
+      __builtin_nontemporal_store(mf, &EE(s.massfx, col, e, k));
       at = at + (double)mf;
     }
-    if (NP < NPP) EE(s.massfx, col, NPP - 1, k) = 0.f;
+    if (NP < NPP) __builtin_nontemporal_store(0.f, &EE(s.massfx, col, NPP - 1, k));
     const double dtn = -at * GD(rarea, ipn);
     EAB(s.dp_tdcy, nf, col, k) = dtn;
     const double inc = c.ab1 * dtn + c.ab2 * t_of + c.ab3 * t_vof;
     dpn = fmax(dp0 + inc, 0.);
-    EC(s.dp, col, k) = dpn;
+    __builtin_nontemporal_store(dpn, &EC(s.dp, col, k));
     lt = inc / g.d.dt;
   }
   e_update_pres_col<NP>(g, s, ipn, col, k, live, dpn, lt);
@@ -688,13 +688,13 @@ __global__ void __launch_bounds__(64 * COLS) k_e_trcadv2(EnsGrid g, EnsState s) 
     }
     mx = fmaxf(0.f, mx);
     mn = fmaxf(0.f, mn);
-    ETR(s.trmax, t, col, k) = mx;
-    ETR(s.trmin, t, col, k) = mn;
+    __builtin_nontemporal_store(mx, &ETR(s.trmax, t, col, k));
+    __builtin_nontemporal_store(mn, &ETR(s.trmin, t, col, k));
     const float to = ETAB(s.trc_tdcy, t, of, col, k), tv = ETAB(s.trc_tdcy, t, vof, col, k);
     const float qp = ((mx - lo) * dp - (a2 * fminf(0.f, to) + a3 * fmaxf(0.f, tv))) * ra1;
     const float qm = ((lo - mn) * dp + (a2 * fmaxf(0.f, to) + a3 * fminf(0.f, tv))) * ra1;
-    ETR(s.tr_rp, t, col, k) = fmaxf(0.f, fminf(1.f, qp * frcp(fmaxf(sp, thshld))));
-    ETR(s.tr_rm, t, col, k) = fmaxf(0.f, fminf(1.f, qm * frcp(fmaxf(sm, thshld))));
+    __builtin_nontemporal_store(fmaxf(0.f, fminf(1.f, qp * frcp(fmaxf(sp, thshld)))), &ETR(s.tr_rp, t, col, k));
+    __builtin_nontemporal_store(fmaxf(0.f, fminf(1.f, qm * frcp(fmaxf(sm, thshld)))), &ETR(s.tr_rm, t, col, k));
   }
 }
 
@@ -733,8 +733,8 @@ __device__ __forceinline__ void e_trcadv3_col(const EnsGrid &g, const EnsState &
     at = -at * rar;
     const float tt = trl + at;
     const float tdp = tdp0 + a1 * tt + a2 * tco + a3 * tcv;
-    ETAB(s.trc_tdcy, t, nf, col, k) = tt;
-    ETR(s.trdp, t, col, k) = tdp;
+    __builtin_nontemporal_store(tt, &ETAB(s.trc_tdcy, t, nf, col, k));
+    __builtin_nontemporal_store(tdp, &ETR(s.trdp, t, col, k));
     if (col_wet(s.wet, col)) ETR(s.tr, t, col, k) = fmaxf(tmn, fminf(tmx, tdp * rdp));   // land keeps its tracers
   }
 }
@@ -830,7 +830,7 @@ __device__ __forceinline__ void e_momtum_col(const EnsGrid &g, const EnsState &s
   const float ut = pgx + f * v1, vt = pgy - f * u1;
   un = u1 + (float)c.ab1 * ut + (float)c.ab2 * uo + (float)c.ab3 * uvo;
   vn = v1 + (float)c.ab1 * vt + (float)c.ab2 * vo + (float)c.ab3 * vvo;
-  EC(s.relvor, col, k) = rv;
+  __builtin_nontemporal_store(rv, &EC(s.relvor, col, k));
   EAB(s.u_tdcy, nf, col, k) = ut;
   EAB(s.v_tdcy, nf, col, k) = vt;
 }
@@ -848,11 +848,11 @@ __global__ void __launch_bounds__(64 * COLS) __attribute__((amdgpu_waves_per_eu(
   if (!col_wet(s.wet, col)) {         // a land column keeps its winds
     if (live) {
       const int nf = c.nf - 1;
-      EC(s.u2, col, k) = EC(s.us, col, k);
-      EC(s.v2, col, k) = EC(s.vs, col, k);
+      __builtin_nontemporal_store(EC(s.us, col, k), &EC(s.u2, col, k));
+      __builtin_nontemporal_store(EC(s.vs, col, k), &EC(s.v2, col, k));
       EAB(s.u_tdcy, nf, col, k) = 0.f;
       EAB(s.v_tdcy, nf, col, k) = 0.f;
-      EC(s.relvor, col, k) = 0.f;
+      __builtin_nontemporal_store(0.f, &EC(s.relvor, col, k));
     }
     return;
   }
@@ -876,8 +876,8 @@ __global__ void __launch_bounds__(64 * COLS) __attribute__((amdgpu_waves_per_eu(
     un = un * fac;
     vn = vn * fac;
   }
-  EC(s.u2, col, k) = un;
-  EC(s.v2, col, k) = vn;
+  __builtin_nontemporal_store(un, &EC(s.u2, col, k));
+  __builtin_nontemporal_store(vn, &EC(s.v2, col, k));
 }
 
 // ---------------------------------------------------------------------------
@@ -902,8 +902,8 @@ __device__ __forceinline__ void e_hystat_col(const EnsGrid &g, const EnsState &s
     if (its > 0) s.ptdcy[(size_t)(its % 2) * ncol + col] = ps - old;
   }
   if (!live) return;
-  EC(s.mp, col, k) = mp;
-  EL(s.ph, col, k + 1) = mp - th * exk1;
+  __builtin_nontemporal_store(mp, &EC(s.mp, col, k));
+  __builtin_nontemporal_store(mp - th * exk1, &EL(s.ph, col, k + 1));
   // tracers 2 and 3 clamped (qv, qw), then trdp = tr * dp; four tracers'
   // loads are issued before their stores
   const float dpf = (float)dp;
@@ -918,9 +918,9 @@ __device__ __forceinline__ void e_hystat_col(const EnsGrid &g, const EnsState &s
       float x = v[j];
       if (t == 1 || t == 2) {
         x = fmaxf(t == 1 ? (float)QVMIN : (float)QWMIN, x);
-        ETR(s.tr, t, col, k) = x;
+        __builtin_nontemporal_store(x, &ETR(s.tr, t, col, k));
       }
-      ETR(s.trdp, t, col, k) = x * dpf;
+      __builtin_nontemporal_store(x * dpf, &ETR(s.trdp, t, col, k));
     }
   }
   (void)nip;
