@@ -123,8 +123,16 @@ __global__ void __launch_bounds__(256) k_dffusn_lev(HybCols<V> s) {
       double factor = -s.smoo[kk] * ldc(s.rarea, ipn);
       out = f + flx * factor;
     }
-    L2(s.exsmo, col, kk) = out;
-    if (top) L2(s.exsmo, col, nvlp1 - 1) = ftop;
+    // nontemporal for the batched FP32 state (measured: 1.15 -> 1.07 ms at
+    // 16 x G7; the same in the column kernels, whose threads store strided
+    // rows, made them up to 3.7x slower)
+    if constexpr (sizeof(V) < sizeof(double)) {
+      __builtin_nontemporal_store(out, &L2(s.exsmo, col, kk));
+      if (top) __builtin_nontemporal_store(ftop, &L2(s.exsmo, col, nvlp1 - 1));
+    } else {
+      L2(s.exsmo, col, kk) = out;
+      if (top) L2(s.exsmo, col, nvlp1 - 1) = ftop;
+    }
     if (kk + (int)blockDim.x == nvlp1 - 1) break;   // the top interface is done
   }
 }
