@@ -171,7 +171,7 @@ __device__ __forceinline__ void e_edgvar1_col(const EnsGrid &g, const EnsState &
     const float2 w = e_edge_wind(g, ipn, e, ui, vi, nu[e], nv[e], nu[et], nv[et], nu[el], nv[el], mi, mn[e], mn[et],
                                  mn[el]);
     const float vnrm = FSVE(ipn, e, 1) * w.x - FSVE(ipn, e, 0) * w.y;
-    EE(s.vn_edg, col, e, k) = (wc && col_wet(s.wet, nb[e])) ? vnrm : 0.f;   // no flux through a land edge
+    __builtin_nontemporal_store((wc && col_wet(s.wet, nb[e])) ? vnrm : 0.f, &EE(s.vn_edg, col, e, k));   // no flux through a land edge
     lp[e] = (float)((pi0 + pi1 + np0[e] + np1[e] + .5 * (np0[et] + np1[et] + np0[el] + np1[el])) * (1. / 6.));
   }
   const float2 pg = e_pres_grad<NP>(g, ipn, lp);
@@ -226,7 +226,7 @@ __device__ __forceinline__ void e_edgvar3_col(const EnsGrid &g, const EnsState &
       out[i] = (float)(b0 + (double)acc);
     }
 #pragma unroll
-    for (int i = 0; i < NP; i++) EE(s.dp_edg, col, i, k) = out[i];
+    for (int i = 0; i < NP; i++) __builtin_nontemporal_store(out[i], &EE(s.dp_edg, col, i, k));
     if (NP < NPP) __builtin_nontemporal_store(0.f, &EE(s.dp_edg, col, NPP - 1, k));
   }
   for (int f = 1; f <= s.edg3_ntr; f++) {
@@ -735,7 +735,7 @@ __device__ __forceinline__ void e_trcadv3_col(const EnsGrid &g, const EnsState &
     const float tdp = tdp0 + a1 * tt + a2 * tco + a3 * tcv;
     __builtin_nontemporal_store(tt, &ETAB(s.trc_tdcy, t, nf, col, k));
     __builtin_nontemporal_store(tdp, &ETR(s.trdp, t, col, k));
-    if (col_wet(s.wet, col)) ETR(s.tr, t, col, k) = fmaxf(tmn, fminf(tmx, tdp * rdp));   // land keeps its tracers
+    if (col_wet(s.wet, col)) __builtin_nontemporal_store(fmaxf(tmn, fminf(tmx, tdp * rdp)), &ETR(s.tr, t, col, k));   // land keeps its tracers
   }
 }
 
