@@ -84,15 +84,15 @@ __device__ __forceinline__ void edgvar1_col(const DevGrid &g, const DevState &s,
     double m_2 = (2. * (m_1ldg + m_2ldg) + m_3trl + m_3ldg) * divby6;
     ue = ue * m_2 / dmax(eps, m_1);
     ve = ve * m_2 / dmax(eps, m_1);
-    E3(s.u_edg, ipn, edg, k) = ue;
-    E3(s.v_edg, ipn, edg, k) = ve;
+    __builtin_nontemporal_store(ue, &E3(s.u_edg, ipn, edg, k));
+    __builtin_nontemporal_store(ve, &E3(s.v_edg, ipn, edg, k));
     E3(s.lp_edg, ipn, edg, k) =
         (cp0[0] + cp1[0] + cp0[i2] + cp1[i2] + .5 * (cp0[i3t] + cp1[i3t] + cp0[i3l] + cp1[i3l])) * divby6;
-    E3(s.mont_edg, ipn, edg, k) = (2. * (cm[0] + cm[i2]) + cm[i3t] + cm[i3l]) * divby6;
-    E3(s.uvsq_edg, ipn, edg, k) = .5 * (ue * ue + ve * ve);
-    E3(s.trc_edg, ipn, edg, k) = (2. * (ct[0] + ct[i2]) + ct[i3t] + ct[i3l]) * divby6;
+    __builtin_nontemporal_store((2. * (cm[0] + cm[i2]) + cm[i3t] + cm[i3l]) * divby6, &E3(s.mont_edg, ipn, edg, k));
+    __builtin_nontemporal_store(.5 * (ue * ue + ve * ve), &E3(s.uvsq_edg, ipn, edg, k));
+    __builtin_nontemporal_store((2. * (ct[0] + ct[i2]) + ct[i3t] + ct[i3l]) * divby6, &E3(s.trc_edg, ipn, edg, k));
   }
-  if (NP < NPP) E3(s.trc_edg, ipn, NPP - 1, k) = 0.;
+  if (NP < NPP) __builtin_nontemporal_store(0., &E3(s.trc_edg, ipn, NPP - 1, k));
 }
 
 __global__ void __launch_bounds__(256) k_edgvar1(DevGrid g, DevState s) {
@@ -198,16 +198,16 @@ __global__ void __launch_bounds__(256) k_cnuity12(DevGrid g, DevState s, AB ab) 
     double vx = SVE(ipx, edx, 1) * E3(s.u_edg, ipx, edx, k) - SVE(ipx, edx, 0) * E3(s.v_edg, ipx, edx, k);
     double flxhi = vn * edge_val<TWO_SIDED>(s.dp_edg, ipn, edg, ipx, edx, k, nvl);
     double m = 0.5 * ((vn + fabs(vn)) * dpi - (vx + fabs(vx)) * C2(s.dp, ipx, k));
-    MF(s.massfx, nf, ipn, edg, k) = m;
-    E3(s.antifx, ipn, edg, k) = flxhi - m;
+    __builtin_nontemporal_store(m, &MF(s.massfx, nf, ipn, edg, k));
+    __builtin_nontemporal_store(flxhi - m, &E3(s.antifx, ipn, edg, k));
     acc = acc + m;
   }
   if (np < NPP) {
-    MF(s.massfx, nf, ipn, NPP - 1, k) = 0.;
-    E3(s.antifx, ipn, NPP - 1, k) = 0.;
+    __builtin_nontemporal_store(0., &MF(s.massfx, nf, ipn, NPP - 1, k));
+    __builtin_nontemporal_store(0., &E3(s.antifx, ipn, NPP - 1, k));
   }
   double dplo = -acc * GV(rarea, ipn);
-  AB3(s.dpl_tdcy, nf, ipn, k) = dplo;
+  __builtin_nontemporal_store(dplo, &AB3(s.dpl_tdcy, nf, ipn, k));
   C2(s.delp_lo, ipn, k) = dpi + ab.ab1 * dplo + ab.ab2 * AB3(s.dpl_tdcy, of, ipn, k) +
                           ab.ab3 * AB3(s.dpl_tdcy, vof, ipn, k);
 }
@@ -240,8 +240,8 @@ __global__ void __launch_bounds__(256) k_cnuity3(DevGrid g, DevState s, AB ab) {
   const double to = AB3(s.dp_tdcy, of, ipn, k), tv = AB3(s.dp_tdcy, vof, ipn, k);
   double q_plus = (dpmax - lo - (ab.ab2 * dmin(0., to) + ab.ab3 * dmax(0., tv))) / ab.ab1 * GV(area, ipn);
   double q_mnus = (lo - dpmin + (ab.ab2 * dmax(0., to) + ab.ab3 * dmin(0., tv))) / ab.ab1 * GV(area, ipn);
-  C2(s.r_plus, ipn, k) = dmax(0., dmin(1., q_plus / dmax(pp, thshld)));
-  C2(s.r_mnus, ipn, k) = dmax(0., dmin(1., q_mnus / dmax(pm, thshld)));
+  __builtin_nontemporal_store(dmax(0., dmin(1., q_plus / dmax(pp, thshld))), &C2(s.r_plus, ipn, k));
+  __builtin_nontemporal_store(dmax(0., dmin(1., q_mnus / dmax(pm, thshld))), &C2(s.r_mnus, ipn, k));
 }
 
 // cnuity5 (cnuity_compute.F90:376-451), which is also update_pres of mass_adv
@@ -268,15 +268,15 @@ __device__ __forceinline__ void update_pres_col(const DevGrid &g, const DevState
   __syncthreads();
   if (live) {
     const double p = pr[k];
-    L2(s.pr, ipn, k) = p;
-    L2(s.ex, ipn, k) = CP * fim_pow(p / P1000, RD / CP);
+    __builtin_nontemporal_store(p, &L2(s.pr, ipn, k));
+    __builtin_nontemporal_store(CP * fim_pow(p / P1000, RD / CP), &L2(s.ex, ipn, k));
     const int np = GNP(ipn);
     double dpdx = 0., dpdy = 0.;
     for (int edg = 0; edg < np; edg++) {
       dpdx = dpdx + E3(s.lp_edg, ipn, edg, k) * SVC(ipn, edg, 1);
       dpdy = dpdy - E3(s.lp_edg, ipn, edg, k) * SVC(ipn, edg, 0);
     }
-    C2(s.ws, ipn, k) = om[k] + (C2(s.us, ipn, k) * dpdx + C2(s.vs, ipn, k) * dpdy) * GV(rarea, ipn);
+    __builtin_nontemporal_store(om[k] + (C2(s.us, ipn, k) * dpdx + C2(s.vs, ipn, k) * dpdy) * GV(rarea, ipn), &C2(s.ws, ipn, k));
   }
 }
 
@@ -303,15 +303,15 @@ __global__ void __launch_bounds__(256) k_cnuity45(DevGrid g, DevState s, AB ab) 
       if (a >= 0.) clip = dmin(C2(s.r_mnus, ipn, k), C2(s.r_plus, x, k));
       else clip = dmin(C2(s.r_plus, ipn, k), C2(s.r_mnus, x, k));
       a = a * clip;
-      MF(s.massfx, nf, ipn, edg, k) = MF(s.massfx, nf, ipn, edg, k) + a;
+      __builtin_nontemporal_store(MF(s.massfx, nf, ipn, edg, k) + a, &MF(s.massfx, nf, ipn, edg, k));
       at = at + a;
     }
     at = -at * GV(rarea, ipn);
     dtn = AB3(s.dpl_tdcy, nf, ipn, k) + at;
-    AB3(s.dp_tdcy, nf, ipn, k) = dtn;
+    __builtin_nontemporal_store(dtn, &AB3(s.dp_tdcy, nf, ipn, k));
     const double to = AB3(s.dp_tdcy, of, ipn, k), tv = AB3(s.dp_tdcy, vof, ipn, k);
     double dpn = C2(s.dp, ipn, k) + ab.ab1 * dtn + ab.ab2 * to + ab.ab3 * tv;
-    C2(s.dp, ipn, k) = dpn;
+    __builtin_nontemporal_store(dpn, &C2(s.dp, ipn, k));
     sh_dp[c][k] = dpn;
     sh_lt[c][k] = (ab.ab1 * dtn + ab.ab2 * to + ab.ab3 * tv) / g.dt;
   }
@@ -376,8 +376,8 @@ __global__ void __launch_bounds__(256) k_massadv1(DevGrid g, DevState s, AB ab) 
     qmo = -max_out_bound;
     qxo = max_out_bound;
   }
-  C2(s.r_plus, ipn, k) = qmo;
-  C2(s.r_mnus, ipn, k) = qxo;
+  __builtin_nontemporal_store(qmo, &C2(s.r_plus, ipn, k));
+  __builtin_nontemporal_store(qxo, &C2(s.r_mnus, ipn, k));
 }
 
 // k_massadv2: the last limiter clamp of each edge value by the upwind
@@ -406,18 +406,18 @@ __global__ void __launch_bounds__(256) k_massadv2(DevGrid g, DevState s, AB ab) 
       const int up = vn > 0. ? ipn : ipx;
       const double qs =
           dmax(C2(s.r_plus, up, k), dmin(C2(s.r_mnus, up, k), E3(TWO_SIDED ? s.dp_edg : s.antifx, ipn, edg, k)));
-      E3(s.dp_edg, ipn, edg, k) = qs;
+      __builtin_nontemporal_store(qs, &E3(s.dp_edg, ipn, edg, k));
       const double m = vn * qs;
-      MF(s.massfx, nf, ipn, edg, k) = m;
+      __builtin_nontemporal_store(m, &MF(s.massfx, nf, ipn, edg, k));
       acc = acc + m;
     }
-    if (np < NPP) MF(s.massfx, nf, ipn, NPP - 1, k) = 0.;
+    if (np < NPP) __builtin_nontemporal_store(0., &MF(s.massfx, nf, ipn, NPP - 1, k));
     const double dtn = -acc * GV(rarea, ipn);
-    AB3(s.dp_tdcy, nf, ipn, k) = dtn;
+    __builtin_nontemporal_store(dtn, &AB3(s.dp_tdcy, nf, ipn, k));
     const double to = AB3(s.dp_tdcy, of, ipn, k), tv = AB3(s.dp_tdcy, vof, ipn, k);
     const double d = C2(s.dp, ipn, k) + ab.ab1 * dtn + ab.ab2 * to + ab.ab3 * tv;
     const double dpn = dmax(d, 0.0);
-    C2(s.dp, ipn, k) = dpn;
+    __builtin_nontemporal_store(dpn, &C2(s.dp, ipn, k));
     sh_dp[c][k] = dpn;
     sh_lt[c][k] = (ab.ab1 * dtn + ab.ab2 * to + ab.ab3 * tv) / g.dt;
   }
@@ -454,7 +454,7 @@ __global__ void __launch_bounds__(256) k_trcadv1(DevGrid g, DevState s, AB ab) {
   }
   if (np < NPP) E3(antiflx, ipn, NPP - 1, k) = 0.;
   double tlo = -acc * GV(rarea, ipn);
-  TAB(s.trl_tdcy, t, nf, ipn, k) = tlo;
+  __builtin_nontemporal_store(tlo, &TAB(s.trl_tdcy, t, nf, ipn, k));
   double trcdp_lo = C2(s.trdp + t * csz, ipn, k) + ab.ab1 * tlo + ab.ab2 * TAB(s.trl_tdcy, t, of, ipn, k) +
                     ab.ab3 * TAB(s.trl_tdcy, t, vof, ipn, k);
   double mx = ti, mn = ti;
@@ -465,9 +465,9 @@ __global__ void __launch_bounds__(256) k_trcadv1(DevGrid g, DevState s, AB ab) {
   }
   mx = dmax(0., mx);
   mn = dmax(0., mn);
-  C2(s.trmax + t * csz, ipn, k) = mx;
-  C2(s.trmin + t * csz, ipn, k) = mn;
-  C2(s.trcr_lo + t * csz, ipn, k) = dmax(mn, dmin(mx, trcdp_lo / dmax(thshld, C2(s.dp, ipn, k))));
+  __builtin_nontemporal_store(mx, &C2(s.trmax + t * csz, ipn, k));
+  __builtin_nontemporal_store(mn, &C2(s.trmin + t * csz, ipn, k));
+  __builtin_nontemporal_store(dmax(mn, dmin(mx, trcdp_lo / dmax(thshld, C2(s.dp, ipn, k)))), &C2(s.trcr_lo + t * csz, ipn, k));
 }
 
 __global__ void __launch_bounds__(256) k_trcadv2(DevGrid g, DevState s, AB ab) {
@@ -498,14 +498,14 @@ __global__ void __launch_bounds__(256) k_trcadv2(DevGrid g, DevState s, AB ab) {
   }
   mx = dmax(0., mx);
   mn = dmax(0., mn);
-  C2(s.trmax + t * csz, ipn, k) = mx;
-  C2(s.trmin + t * csz, ipn, k) = mn;
+  __builtin_nontemporal_store(mx, &C2(s.trmax + t * csz, ipn, k));
+  __builtin_nontemporal_store(mn, &C2(s.trmin + t * csz, ipn, k));
   const double dp = C2(s.dp, ipn, k);
   const double to = TAB(s.trc_tdcy, t, of, ipn, k), tv = TAB(s.trc_tdcy, t, vof, ipn, k);
   double q_plus = ((mx - lo) * dp - (ab.ab2 * dmin(0., to) + ab.ab3 * dmax(0., tv))) / ab.ab1 * GV(area, ipn);
   double q_mnus = ((lo - mn) * dp + (ab.ab2 * dmax(0., to) + ab.ab3 * dmin(0., tv))) / ab.ab1 * GV(area, ipn);
-  C2(s.tr_rp + t * csz, ipn, k) = dmax(0., dmin(1., q_plus / dmax(sp, thshld)));
-  C2(s.tr_rm + t * csz, ipn, k) = dmax(0., dmin(1., q_mnus / dmax(sm, thshld)));
+  __builtin_nontemporal_store(dmax(0., dmin(1., q_plus / dmax(sp, thshld))), &C2(s.tr_rp + t * csz, ipn, k));
+  __builtin_nontemporal_store(dmax(0., dmin(1., q_mnus / dmax(sm, thshld))), &C2(s.tr_rm + t * csz, ipn, k));
 }
 
 __global__ void __launch_bounds__(256) k_trcadv3(DevGrid g, DevState s, AB ab) {
@@ -530,7 +530,7 @@ __global__ void __launch_bounds__(256) k_trcadv3(DevGrid g, DevState s, AB ab) {
   }
   at = -at * GV(rarea, ipn);
   double tt = TAB(s.trl_tdcy, t, nf, ipn, k) + at;
-  TAB(s.trc_tdcy, t, nf, ipn, k) = tt;
+  __builtin_nontemporal_store(tt, &TAB(s.trc_tdcy, t, nf, ipn, k));
   double *trcdp = s.trdp + t * csz;
   double tdp = C2(trcdp, ipn, k) + ab.ab1 * tt + ab.ab2 * TAB(s.trc_tdcy, t, of, ipn, k) +
                ab.ab3 * TAB(s.trc_tdcy, t, vof, ipn, k);
@@ -587,12 +587,12 @@ __global__ void __launch_bounds__(256) k_momtum(DevGrid g, DevState s, AB ab) {
     px = px * GV(rarea, ipn);
     py = py * GV(rarea, ipn);
     rv = rv * GV(rarea, ipn);
-    C2(s.relvor, ipn, k) = rv;
+    __builtin_nontemporal_store(rv, &C2(s.relvor, ipn, k));
     double uzeta = (GV(corio, ipn) + rv) * u1;
     double vzeta = (GV(corio, ipn) + rv) * v1;
     double ut = px + vzeta, vt = py - uzeta;
-    AB3(s.u_tdcy, nf, ipn, k) = ut;
-    AB3(s.v_tdcy, nf, ipn, k) = vt;
+    __builtin_nontemporal_store(ut, &AB3(s.u_tdcy, nf, ipn, k));
+    __builtin_nontemporal_store(vt, &AB3(s.v_tdcy, nf, ipn, k));
     un = u1 + ab.ab1 * ut + ab.ab2 * AB3(s.u_tdcy, of, ipn, k) + ab.ab3 * AB3(s.u_tdcy, vof, ipn, k);
     vn = v1 + ab.ab1 * vt + ab.ab2 * AB3(s.v_tdcy, of, ipn, k) + ab.ab3 * AB3(s.v_tdcy, vof, ipn, k);
     if (k >= nvl - 3) sh_q[c][nvl - 1 - k] = un * un + vn * vn;
@@ -609,8 +609,8 @@ __global__ void __launch_bounds__(256) k_momtum(DevGrid g, DevState s, AB ab) {
       un = un * factor;
       vn = vn * factor;
     }
-    C2(s.u2, ipn, k) = un;
-    C2(s.v2, ipn, k) = vn;
+    __builtin_nontemporal_store(un, &C2(s.u2, ipn, k));
+    __builtin_nontemporal_store(vn, &C2(s.v2, ipn, k));
   }
 }
 
@@ -652,8 +652,8 @@ __global__ void __launch_bounds__(256) k_hystat(DevGrid g, DevState s, int its) 
   __syncthreads();
   if (live && k < nvl) {
     const double mp = sh_mp[c][k];
-    C2(s.mp, ipn, k) = mp;
-    L2(s.ph, ipn, k + 1) = mp - sh_th[c][k] * sh_ex[c][k + 1];
+    __builtin_nontemporal_store(mp, &C2(s.mp, ipn, k));
+    __builtin_nontemporal_store(mp - sh_th[c][k] * sh_ex[c][k + 1], &L2(s.ph, ipn, k + 1));
   }
 }
 
@@ -675,7 +675,7 @@ __global__ void __launch_bounds__(256) k_pres_from_exner(DevGrid g, DevState s) 
   const int ipn = wave_col();
   if (ipn >= nip) return;
   for (int k = threadIdx.x; k < nvlp1; k += blockDim.x)
-    L2(s.pr, ipn, k) = P1000 * fim_pow(L2(s.ex, ipn, k) / CP, CP / RD);
+    __builtin_nontemporal_store(P1000 * fim_pow(L2(s.ex, ipn, k) / CP, CP / RD), &L2(s.pr, ipn, k));
 }
 
 __global__ void __launch_bounds__(256) k_trdp(DevGrid g, DevState s) {
